@@ -1,0 +1,2321 @@
+/*
+ * FESOM-C internal time step -- FP64 CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the MI355X
+ * implementation: only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline may load it.  The product never links or calls it.
+ *
+ * It is a plain-C restatement of the reference Fortran's *serial* loops
+ * (rakowsk/fesom-c, src/*.F90), written as the reference writes them:
+ * edge loops that scatter into nodes/elements in ascending edge order,
+ * column-major (level-fastest) 3-D arrays, Fortran operator order kept
+ * (left-to-right sums, REAL(4) literals promoted as REAL(4)).  Each
+ * function cites the reference lines it follows.  Build with
+ * -ffp-contract=off so no FMA is introduced.
+ *
+ * Parity pins (see tests/): the reference-held LE_medium energy series
+ * (test_cases/LE_medium/postproc/energy_tocheck.dat) and edge list
+ * (MESH/edges.out, edge_tri.out), plus closed-form / invariant checks.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <ctype.h>
+
+#define F4(x) ((double)(float)(x)) /* un-suffixed Fortran literal: REAL(4) */
+
+/* o_PARAM constants, src/fv_var.F90:8-110, 130, 171-172 */
+static const double PI = 3.14159265358979;
+#define RAD (PI / 180.0)
+static const double G = 9.8;
+static const double R_EARTH = 6400000.0;
+#define OMEGA (2.0 * PI / (3600.0 * 24.0))
+static const double CKA = 0.4;
+static const double P_REF = 101325.0;
+static const double K_HOR = .010;
+static const double K_VER = 0.00001;
+static const double PR_NUM = 0.0001;
+static const double H_ADVTR_CRIT = 0.1;
+static const double H_ADVTR_MIN = 0.0;
+static const double BETA = 0.281105;
+static const double EPS = 0.01;
+static const double BETA_SCALE = 3.9;
+static const double TIDE_PERIOD[12] = {12.4200, 12.00, 12.6584, 11.9673, 23.9344, 25.8194,
+                                       24.0659, 26.8684, 327.85, 661.31, 4383.05, 6.2103};
+static const char *TIDE_NAME[12] = {"M2", "S2", "N2", "K2", "K1", "O1",
+                                    "P1", "Q1", "Mf", "Mm", "Ssa", "M4"};
+
+typedef struct {
+    /* ---- configuration (src/fv_read_run.F90) ---- */
+    char title[256];
+    double dt_2D, dt, time_jd0, Dcr, Dmin, C_d, z0b_min, z0s_min, za, snul;
+    double T_const, S_const, lat_cartesian, cyclic_length, clim_relax, relax2clim_ac;
+    double Cd_bz, Bp_bz, pow_sig, lev_bot, lev_sur;
+    int nsteps, irep, cartesian, coriolis_tf, mom_adv_2D, mom_adv_3D, tracer_adv, vert_adv;
+    int im_vert_visc, im_vert_visc_fic, fic_point, BFC, ver_mix, nsigma, ind_sigma;
+    int temp_on, salinity_on, barotropic, barotropic_3D, wet_dry_on, TF_presence, T_potential;
+    int cl_relax, cl_solid_boder_relax, mixed_BP_grad_OB, restart, key_atm, key_rivers;
+    int filt_2D, filt_bi_2D, bih_2D, filt_3D, filt_bi_3D, bih_3D, SL_obn;
+    int type_task, Mt, a_th[12], n_th;
+    int muscl_call; /* project switch: tracer_adv=2 calls solve_tracer_muscl */
+    int is_LE;
+    /* ---- mesh (o_MESH) ---- */
+    int N, E, D, D_in, E_tri, nobn, L, K, maxdeg;
+    double *coord;     /* 2N */
+    int *index_nod;    /* N */
+    int *elem_nodes;   /* 4E, 0-based */
+    int *edge_nodes;   /* 2D */
+    int *edge_tri;     /* 2D, -999 for none */
+    int *elem_edges;   /* 4E */
+    int *elem_nb;      /* 4E, -999 for none */
+    int *nie_num, *nie; /* N, N*maxdeg */
+    int *in_obn;       /* nobn */
+    double *depth, *sigma, *elem_area, *area, *elem_cos, *metric_factor, *coriolis;
+    double *w_cv, *edge_dxdy, *edge_cross_dxdy, *gradient_sca, *gradient_vec;
+    double *ac, *a_tp; /* a_tp: 2N, [n*2+0], [n*2+1] */
+    double *ampt, *fazt; /* nobn*12 */
+    double amp_factor[12], phy_factor[12];
+    double scale_area, density_0;
+    /* ---- state (o_ARRAYS) ---- */
+    double time_jd, time, time_2D;
+    int lfirst, n_dt;
+    double T_aver, S_aver;
+    double *U_n_2D, *U_n_1, *U_n_2, *UAB, *U_rhs_2D, *U_rhs_2D_3D, *UV2_rhs; /* 2E */
+    double *eta_n, *eta_n_1, *eta_n_2, *etaAB, *ssh_rhs, *ssh_gp, *eta_p; /* N */
+    double *vel_grad; /* 4E */
+    double *taux, *tauy, *windx, *windy; /* E, E, N, N */
+    double *Jc, *Jc_old, *Je, *Jd, *Z, *zbar; /* LN LN LE LD LN KN */
+    double *dmean_n; /* E */
+    double *hpressure, *Bar_pru_3D, *Bar_prv_3D, *Bar_pr_2D, *rho_c; /* KN LE LE 2E LN */
+    double *Unode, *Vnode, *Unode_p, *Vnode_p; /* LN */
+    double *Kv, *Av, *bt, *snu, *Ri, *tke_dissip; /* KN KE KN KN KN KN */
+    double *U_rhs, *V_rhs, *U_rhsAB, *V_rhsAB, *UV_rhs; /* LE, UV_rhs 2LE */
+    double *vel_grad_ux, *vel_grad_uy, *vel_grad_vx, *vel_grad_vy; /* LE */
+    double *U_puls, *V_puls, *U_n, *V_n, *U_n_filt, *V_n_filt; /* LE */
+    double *Wvel; /* KN */
+    double *U_filt_2D, *V_filt_2D; /* E */
+    double *TF, *T_old, *SF, *S_old, *Tclim, *Sclim; /* LN */
+    double *mask_wd, *mask_wd_node, *mask_ad, *mask_bpr, *C_d_el, *relax2clim; /* E N N E E N */
+    int status; /* 0 ok; nonzero = reference STOP condition */
+    char errmsg[256];
+} orc_t;
+
+/* ------------------------------------------------------------------ */
+/* small helpers                                                        */
+/* ------------------------------------------------------------------ */
+static void *xcalloc(size_t n, size_t sz) {
+    void *p = calloc(n ? n : 1, sz);
+    if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    return p;
+}
+#define DALLOC(n) ((double *)xcalloc((size_t)(n), sizeof(double)))
+#define IALLOC(n) ((int *)xcalloc((size_t)(n), sizeof(int)))
+static double dmax(double a, double b) { return a > b ? a : b; } /* Fortran max for non-NaN */
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+/* Fortran SUM over 4 products, left to right */
+static double sum4w(const double *w, double x0, double x1, double x2, double x3) {
+    double s = w[0] * x0;
+    s = s + w[1] * x1;
+    s = s + w[2] * x2;
+    s = s + w[3] * x3;
+    return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* fv_run.dat scanner: KEY = value lines, '!' comments                  */
+/* (semantics of SCAN_FILE, src/fv_utilit.F90:223-570, reduced)        */
+/* ------------------------------------------------------------------ */
+static const char *scan_key(const char *txt, const char *key, char *val, size_t vlen) {
+    size_t kl = strlen(key);
+    const char *p = txt;
+    while (*p) {
+        const char *eol = strchr(p, '\n');
+        size_t ll = eol ? (size_t)(eol - p) : strlen(p);
+        char line[512];
+        size_t cl = ll < sizeof(line) - 1 ? ll : sizeof(line) - 1;
+        memcpy(line, p, cl);
+        line[cl] = 0;
+        char *bang = strchr(line, '!');
+        if (bang) *bang = 0;
+        char *eq = strchr(line, '=');
+        if (eq) {
+            char *a = line;
+            while (*a && isspace((unsigned char)*a)) a++;
+            char *b = eq;
+            while (b > a && isspace((unsigned char)b[-1])) b--;
+            if ((size_t)(b - a) == kl && strncmp(a, key, kl) == 0) {
+                char *v = eq + 1;
+                while (*v && isspace((unsigned char)*v)) v++;
+                char *e = v + strlen(v);
+                while (e > v && isspace((unsigned char)e[-1])) e--;
+                size_t n = (size_t)(e - v) < vlen - 1 ? (size_t)(e - v) : vlen - 1;
+                memcpy(val, v, n);
+                val[n] = 0;
+                return val;
+            }
+        }
+        if (!eol) break;
+        p = eol + 1;
+    }
+    return NULL;
+}
+static double cfg_f(const char *t, const char *k, double dflt) {
+    char v[256];
+    return scan_key(t, k, v, sizeof v) ? strtod(v, NULL) : dflt;
+}
+static int cfg_i(const char *t, const char *k, int dflt) {
+    char v[256];
+    return scan_key(t, k, v, sizeof v) ? (int)strtol(v, NULL, 10) : dflt;
+}
+static int cfg_l(const char *t, const char *k, int dflt) {
+    char v[256];
+    if (!scan_key(t, k, v, sizeof v)) return dflt;
+    return v[0] == 'T' || v[0] == 't' || (v[0] == '.' && (v[1] == 'T' || v[1] == 't'));
+}
+
+/* ------------------------------------------------------------------ */
+/* densityJM: src/fv_pressure_new.F90:340-414 (comp_sediment off)      */
+/* ------------------------------------------------------------------ */
+double orc_densityJM(double t, double s, double pz, double density_0) {
+    double ss3 = sqrt(s * s * s);
+    double bulk = 19092.56 + t * (209.8925 - t * (3.041638 - t * (-F4(1.852732e-3) - t * (F4(1.361629e-5)))))
+                + s * (104.4077 - t * (6.500517 - t * (.1553190 - t * (-F4(2.326469e-4)))))
+                + ss3 * (-5.587545 + t * (0.7390729 - t * (F4(1.909078e-2))))
+                - pz * (F4(4.721788e-1) + t * (F4(1.028859e-2) + t * (-F4(2.512549e-4) - t * (F4(5.939910e-7)))))
+                - pz * s * (-F4(1.571896e-2) - t * (F4(2.598241e-4) + t * (-F4(7.267926e-6))))
+                - pz * ss3 * F4(2.042967e-3)
+                + pz * pz * (F4(1.045941e-5) - t * (F4(5.782165e-10) - t * (F4(1.296821e-7))))
+                + pz * pz * s * (-F4(2.595994e-7) + t * (-F4(1.248266e-9) + t * (-F4(3.508914e-9))));
+    double rhopot = 999.842594
+                  + t * (F4(6.793952e-2) + t * (-F4(9.095290e-3) + t * (F4(1.001685e-4)
+                    + t * (-F4(1.120083e-6) + t * F4(6.536332e-9)))))
+                  + s * (0.824493 + t * (-F4(4.08990e-3) + t * (F4(7.64380e-5)
+                    + t * (-F4(8.24670e-7) + t * F4(5.38750e-9)))))
+                  + ss3 * (-F4(5.72466e-3) + t * (F4(1.02270e-4) + t * (-F4(1.65460e-6))))
+                  + F4(4.8314e-4) * s * s;
+    return rhopot * bulk / (bulk + 0.1 * pz) - density_0;
+}
+
+/* ------------------------------------------------------------------ */
+/* mesh setup                                                            */
+/* ------------------------------------------------------------------ */
+
+/* elem_center: src/fv_mesh_array.F90:742-789 */
+static void elem_center(const orc_t *o, int el, double *x, double *y, double *s) {
+    const int *en = o->elem_nodes + 4 * el;
+    double ax[4], ay[4];
+    double cl = o->cyclic_length;
+    for (int k = 0; k < 4; k++) { ax[k] = o->coord[2 * en[k]]; ay[k] = o->coord[2 * en[k] + 1]; }
+    if (en[0] == en[3]) {
+        double amin = dmin(dmin(ax[0], ax[1]), ax[2]);
+        for (int k = 0; k < 3; k++) if (ax[k] - amin > cl / 2.0) ax[k] = ax[k] - cl;
+        *x = ((ax[0] + ax[1]) + ax[2]) / 3.0;
+        *y = ((ay[0] + ay[1]) + ay[2]) / 3.0;
+        *s = 0.5 * fabs((ax[1] - ax[0]) * (ay[2] - ay[0]) - (ay[1] - ay[0]) * (ax[2] - ax[0]));
+    } else {
+        double amin = dmin(dmin(ax[0], ax[1]), dmin(ax[2], ax[3]));
+        for (int k = 0; k < 4; k++) if (ax[k] - amin > cl / 2.0) ax[k] = ax[k] - cl;
+        double xx = ax[0] + ax[2] + ax[3];
+        double yy = ay[0] + ay[2] + ay[3];
+        double x1 = ax[0] + ax[1] + ax[2];
+        double y1 = ay[0] + ay[1] + ay[2];
+        double s1 = fabs((ax[2] - ax[3]) * (ay[0] - ay[3]) - (ay[2] - ay[3]) * (ax[0] - ax[3]));
+        double s2 = fabs((ax[2] - ax[1]) * (ay[0] - ay[1]) - (ay[2] - ay[1]) * (ax[0] - ax[1]));
+        *x = (xx * s1 + x1 * s2) / 3.0 / (s1 + s2);
+        *y = (yy * s1 + y1 * s2) / 3.0 / (s1 + s2);
+        *s = 0.5 * (s1 + s2);
+    }
+}
+
+static void cdiff(const orc_t *o, const double *x2, const double *x1, double *xd) {
+    xd[0] = x2[0] - x1[0];
+    xd[1] = x2[1] - x1[1];
+    if (xd[0] > o->cyclic_length / 2.0) xd[0] = xd[0] - o->cyclic_length;
+    if (xd[0] < -o->cyclic_length / 2.0) xd[0] = xd[0] + o->cyclic_length;
+}
+
+/* edge_center: src/fv_mesh_array.F90:720-740 */
+static void edge_center(const orc_t *o, int n1, int n2, double *x, double *y) {
+    double a0 = o->coord[2 * n1], a1 = o->coord[2 * n1 + 1];
+    double b0 = o->coord[2 * n2], b1 = o->coord[2 * n2 + 1];
+    if (a0 - b0 > o->cyclic_length / 2.0) a0 = a0 - o->cyclic_length;
+    if (a0 - b0 < -o->cyclic_length / 2.0) b0 = b0 - o->cyclic_length;
+    *x = 0.5 * (a0 + b0);
+    *y = 0.5 * (a1 + b1);
+}
+
+/* SET_SIGMA: src/fv_vert_coord.F90:174-208 (ind_sigma 1 and 2) */
+static int set_sigma(orc_t *o) {
+    int K = o->nsigma;
+    o->sigma = DALLOC(K);
+    if (o->ind_sigma == 1) {
+        if (o->pow_sig > 1 && K % 2 == 0) return -1;
+        if (o->pow_sig == 1) {
+            for (int k = 1; k <= K; k++)
+                o->sigma[k - 1] = -pow((k - 1) / (double)(K - 1), o->pow_sig) + 1.0;
+        } else {
+            int h = (K + 1) / 2;
+            for (int k = 1; k <= h; k++)
+                o->sigma[k - 1] = -pow((k - 1) / (double)(h - 1), o->pow_sig) * 0.5 + 1.0;
+            for (int k = h + 1; k <= K; k++)
+                o->sigma[k - 1] = pow((K - k) / (double)(h - 1), o->pow_sig) * 0.5;
+        }
+    } else if (o->ind_sigma == 2) {
+        for (int k = 1; k <= K; k++) {
+            double x1 = tanh((o->lev_bot + o->lev_sur) * (K - k) / (K - 1) - o->lev_bot);
+            double x2 = tanh(o->lev_bot);
+            double x3 = x2 + tanh(o->lev_sur);
+            o->sigma[k - 1] = (x1 + x2) / x3;
+        }
+    } else {
+        return -2;
+    }
+    return 0;
+}
+
+/* test_elem: src/fv_mesh_array.F90:1-82 */
+static int test_elem(orc_t *o) {
+    double cl = o->cyclic_length;
+    for (int n = 0; n < o->E; n++) {
+        int *en = o->elem_nodes + 4 * n;
+        double a[2] = {o->coord[2 * en[0]], o->coord[2 * en[0] + 1]};
+        double b[2], c[2], d[2];
+        b[0] = o->coord[2 * en[1]] - a[0]; b[1] = o->coord[2 * en[1] + 1] - a[1];
+        c[0] = o->coord[2 * en[2]] - a[0]; c[1] = o->coord[2 * en[2] + 1] - a[1];
+        if (b[0] > cl / 2.0) b[0] -= cl;
+        if (b[0] < -cl / 2.0) b[0] += cl;
+        if (c[0] > cl / 2.0) c[0] -= cl;
+        if (c[0] < -cl / 2.0) c[0] += cl;
+        if (en[0] == en[3]) {
+            double r = b[0] * c[1] - b[1] * c[0];
+            if (r > 0.0) { int t = en[1]; en[1] = en[2]; en[2] = t; }
+        } else {
+            d[0] = o->coord[2 * en[3]] - a[0]; d[1] = o->coord[2 * en[3] + 1] - a[1];
+            if (d[0] > cl / 2.0) d[0] -= cl;
+            if (d[0] < -cl / 2.0) d[0] += cl;
+            double r = b[0] * c[1] - b[1] * c[0];
+            double r1 = b[0] * d[1] - b[1] * d[0];
+            if (r1 * r < 0.0) {
+                snprintf(o->errmsg, sizeof o->errmsg, "Node list problem %d", n + 1);
+                return -1;
+            }
+            if (r > 0) { int t = en[1]; en[1] = en[3]; en[3] = t; }
+        }
+    }
+    return 0;
+}
+
+/* find_edges: src/fv_mesh_array.F90:395-718 */
+static int find_edges(orc_t *o) {
+    int N = o->N, E = o->E;
+    int *ne_num = IALLOC(N);
+    for (int n = 0; n < E; n++) {
+        int *en = o->elem_nodes + 4 * n;
+        int q1 = (en[0] == en[3]) ? 3 : 4;
+        for (int q = 0; q < q1; q++) ne_num[en[q]]++;
+    }
+    int kmax = 0;
+    for (int n = 0; n < N; n++) if (ne_num[n] > kmax) kmax = ne_num[n];
+    int *ne_pos = IALLOC((size_t)kmax * N);
+    memset(ne_num, 0, sizeof(int) * N);
+    for (int n = 0; n < E; n++) {
+        int *en = o->elem_nodes + 4 * n;
+        int q1 = (en[0] == en[3]) ? 3 : 4;
+        for (int q = 0; q < q1; q++) { ne_pos[(size_t)en[q] * kmax + ne_num[en[q]]] = n; ne_num[en[q]]++; }
+    }
+    /* neighbour nodes, in the order met over ne_pos (lines 494-541) */
+    int *aux1 = IALLOC(N);
+    int *nn_num = IALLOC(N);
+    int nnmax = 2 * kmax + 2;
+    int *nn_pos = IALLOC((size_t)nnmax * N);
+    int temp[200];
+    for (int n = 0; n < N; n++) {
+        int counter = 0;
+        for (int k = 0; k < ne_num[n]; k++) {
+            int el = ne_pos[(size_t)n * kmax + k];
+            int *en = o->elem_nodes + 4 * el;
+            if (en[0] == en[3]) {
+                for (int q = 0; q < 3; q++) {
+                    if (en[q] == n) continue;
+                    if (aux1[en[q]] != 1) { aux1[en[q]] = 1; temp[counter++] = en[q]; }
+                }
+            } else {
+                int ed0 = -1, ed1 = -1;
+                if (en[0] == n) { ed0 = en[1]; ed1 = en[3]; }
+                if (en[1] == n) { ed0 = en[0]; ed1 = en[2]; }
+                if (en[2] == n) { ed0 = en[1]; ed1 = en[3]; }
+                if (en[3] == n) { ed0 = en[0]; ed1 = en[2]; }
+                int ed[2] = {ed0, ed1};
+                for (int q = 0; q < 2; q++)
+                    if (aux1[ed[q]] != 1) { aux1[ed[q]] = 1; temp[counter++] = ed[q]; }
+            }
+        }
+        nn_num[n] = counter + 1;
+        for (int q = 0; q < counter; q++) aux1[temp[q]] = 0;
+        nn_pos[(size_t)n * nnmax] = n;
+        for (int q = 0; q < counter; q++) nn_pos[(size_t)n * nnmax + 1 + q] = temp[q];
+    }
+    /* count edges */
+    int counter = 0;
+    for (int n = 0; n < N; n++)
+        for (int q = 1; q < nn_num[n]; q++) if (nn_pos[(size_t)n * nnmax + q] >= n) counter++;
+    o->D = counter;
+    o->edge_nodes = IALLOC(2 * (size_t)counter);
+    o->edge_tri = IALLOC(2 * (size_t)counter);
+    int cin = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int n = 0; n < N; n++) {
+            for (int q = 1; q < nn_num[n]; q++) {
+                int node = nn_pos[(size_t)n * nnmax + q];
+                if (node < n) continue;
+                int flag = 0, elems[2] = {-999, -999};
+                for (int k = 0; k < ne_num[n]; k++) {
+                    int el = ne_pos[(size_t)n * kmax + k];
+                    int *en = o->elem_nodes + 4 * el;
+                    int q2 = (en[0] == en[3]) ? 3 : 4;
+                    for (int q1 = 0; q1 < q2; q1++)
+                        if (en[q1] == node) { if (flag < 2) elems[flag] = el; flag++; break; }
+                }
+                if ((pass == 0 && flag == 2) || (pass == 1 && flag == 1)) {
+                    o->edge_nodes[2 * cin] = n;
+                    o->edge_nodes[2 * cin + 1] = node;
+                    o->edge_tri[2 * cin] = elems[0];
+                    o->edge_tri[2 * cin + 1] = pass == 0 ? elems[1] : -999;
+                    cin++;
+                }
+            }
+        }
+        if (pass == 0) o->D_in = cin;
+    }
+    if (cin != o->D) { snprintf(o->errmsg, sizeof o->errmsg, "edge count mismatch"); return -1; }
+    /* (d) orientation: first element to the left (lines 639-664) */
+    double cl = o->cyclic_length;
+    for (int n = 0; n < o->D; n++) {
+        int e0 = o->edge_nodes[2 * n], e1 = o->edge_nodes[2 * n + 1];
+        int el = o->edge_tri[2 * n];
+        double xc[2], amin;
+        elem_center(o, el, &xc[0], &xc[1], &amin);
+        xc[0] -= o->coord[2 * e0]; xc[1] -= o->coord[2 * e0 + 1];
+        double xe[2] = {o->coord[2 * e1] - o->coord[2 * e0], o->coord[2 * e1 + 1] - o->coord[2 * e0 + 1]};
+        if (xe[0] > cl / 2.0) xe[0] -= cl;
+        if (xe[0] < -cl / 2.0) xe[0] += cl;
+        if (xc[0] > cl / 2.0) xc[0] -= cl;
+        if (xc[0] < -cl / 2.0) xc[0] += cl;
+        if (xc[0] * xe[1] - xc[1] * xe[0] > 0.0) {
+            int el1 = o->edge_tri[2 * n + 1];
+            if (el1 >= 0) { o->edge_tri[2 * n] = el1; o->edge_tri[2 * n + 1] = el; }
+            else { o->edge_nodes[2 * n] = e1; o->edge_nodes[2 * n + 1] = e0; }
+        }
+    }
+    /* (e) elem_edges in node rotation order (lines 671-716) */
+    o->elem_edges = IALLOC(4 * (size_t)E);
+    int *cnt = IALLOC(E);
+    for (int n = 0; n < o->D; n++)
+        for (int k = 0; k < 2; k++) {
+            int q = o->edge_tri[2 * n + k];
+            if (q >= 0) { o->elem_edges[4 * q + cnt[q]] = n; cnt[q]++; }
+        }
+    for (int el = 0; el < E; el++) {
+        int *en = o->elem_nodes + 4 * el;
+        int q1 = (en[0] == en[3]) ? 3 : 4;
+        int eled[4];
+        for (int k = 0; k < 4; k++) eled[k] = o->elem_edges[4 * el + k];
+        for (int q = 0; q < q1 - 1; q++)
+            for (int k = 0; k < q1; k++) {
+                int a = o->edge_nodes[2 * eled[k]], b = o->edge_nodes[2 * eled[k] + 1];
+                if ((a == en[q] && b == en[q + 1]) || (a == en[q + 1] && b == en[q])) {
+                    o->elem_edges[4 * el + q] = eled[k];
+                    break;
+                }
+            }
+        for (int k = 0; k < q1; k++) {
+            int a = o->edge_nodes[2 * eled[k]], b = o->edge_nodes[2 * eled[k] + 1];
+            if ((a == en[q1 - 1] && b == en[0]) || (a == en[0] && b == en[q1 - 1])) {
+                o->elem_edges[4 * el + q1 - 1] = eled[k];
+                break;
+            }
+        }
+        if (q1 == 3) o->elem_edges[4 * el + 3] = o->elem_edges[4 * el];
+    }
+    free(cnt); free(ne_num); free(ne_pos); free(aux1); free(nn_num); free(nn_pos);
+    return 0;
+}
+
+/* find_elem_neighbors: src/fv_mesh_array.F90:791-884 (serial part) */
+static void find_elem_neighbors(orc_t *o) {
+    int E = o->E, N = o->N;
+    o->elem_nb = IALLOC(4 * (size_t)E);
+    for (int el = 0; el < E; el++) {
+        int *ed = o->elem_edges + 4 * el;
+        int q = (ed[0] == ed[3]) ? 3 : 4;
+        for (int j = 0; j < q; j++) {
+            int e1 = o->edge_tri[2 * ed[j]];
+            if (e1 == el) e1 = o->edge_tri[2 * ed[j] + 1];
+            o->elem_nb[4 * el + j] = e1;
+        }
+        if (q == 3) o->elem_nb[4 * el + 3] = o->elem_nb[4 * el];
+    }
+    o->nie_num = IALLOC(N);
+    for (int el = 0; el < E; el++) {
+        int *en = o->elem_nodes + 4 * el;
+        int q = (en[0] == en[3]) ? 3 : 4;
+        for (int j = 0; j < q; j++) o->nie_num[en[j]]++;
+    }
+    int m = 0;
+    for (int n = 0; n < N; n++) if (o->nie_num[n] > m) m = o->nie_num[n];
+    o->maxdeg = m;
+    o->nie = IALLOC((size_t)m * N);
+    memset(o->nie_num, 0, sizeof(int) * N);
+    for (int el = 0; el < E; el++) {
+        int *en = o->elem_nodes + 4 * el;
+        int q = (en[0] == en[3]) ? 3 : 4;
+        for (int j = 0; j < q; j++) { int nd = en[j]; o->nie[(size_t)nd * m + o->nie_num[nd]] = el; o->nie_num[nd]++; }
+    }
+}
+
+/* mesh_arrays1: src/fv_mesh_array.F90:948-1139 */
+static void mesh_arrays1(orc_t *o) {
+    int E = o->E, N = o->N;
+    o->elem_area = DALLOC(E); o->area = DALLOC(N); o->metric_factor = DALLOC(E);
+    o->elem_cos = DALLOC(E); o->coriolis = DALLOC(E); o->w_cv = DALLOC(4 * (size_t)E);
+    for (int el = 0; el < E; el++) {
+        double ax, ay, aa;
+        elem_center(o, el, &ax, &ay, &aa);
+        o->coriolis[el] = 2.0 * OMEGA * sin(ay);
+        o->elem_cos[el] = cos(ay);
+        o->metric_factor[el] = tan(ay) / R_EARTH;
+        o->elem_area[el] = aa;
+    }
+    if (o->cartesian) for (int el = 0; el < E; el++) { o->elem_cos[el] = 1.0; o->metric_factor[el] = 0.0; }
+    if (o->coriolis_tf) {
+        if (o->cartesian) for (int el = 0; el < E; el++) o->coriolis[el] = 2.0 * OMEGA * sin(o->lat_cartesian);
+    } else {
+        for (int el = 0; el < E; el++) o->coriolis[el] = 0.0;
+    }
+    for (int el = 0; el < E; el++) o->elem_area[el] = o->elem_area[el] * o->elem_cos[el];
+    for (int el = 0; el < E; el++) {
+        int *en = o->elem_nodes + 4 * el;
+        if (en[0] == en[3]) {
+            for (int j = 0; j < 3; j++) {
+                o->area[en[j]] = o->area[en[j]] + o->elem_area[el] / 3.0;
+                o->w_cv[4 * el + j] = 1.0 / 3.0;
+            }
+        } else {
+            double rc[2], aa;
+            elem_center(o, el, &rc[0], &rc[1], &aa);
+            int nodes6[6] = {en[3], en[0], en[1], en[2], en[3], en[0]};
+            for (int j = 1; j <= 4; j++) {
+                double r1[2], r0[2], rr[2], rl[2], rcd[2];
+                r1[0] = o->coord[2 * nodes6[j - 1]]; r1[1] = o->coord[2 * nodes6[j - 1] + 1];
+                r0[0] = o->coord[2 * nodes6[j]];     r0[1] = o->coord[2 * nodes6[j] + 1];
+                cdiff(o, r1, r0, rr);
+                r1[0] = o->coord[2 * nodes6[j + 1]]; r1[1] = o->coord[2 * nodes6[j + 1] + 1];
+                cdiff(o, r1, r0, rl);
+                cdiff(o, rc, r0, rcd);
+                double v = 0.25 * o->elem_cos[el] * (fabs(rl[0] * rcd[1] - rl[1] * rcd[0]) + fabs(rr[0] * rcd[1] - rr[1] * rcd[0]));
+                o->area[nodes6[j]] = o->area[nodes6[j]] + v;
+                o->w_cv[4 * el + j - 1] = 0.25 * o->elem_cos[el] * (fabs(rl[0] * rcd[1] - rl[1] * rcd[0]) + fabs(rr[0] * rcd[1] - rr[1] * rcd[0])) / o->elem_area[el];
+            }
+        }
+    }
+    for (int el = 0; el < E; el++) o->elem_area[el] = o->elem_area[el] * R_EARTH * R_EARTH;
+    for (int n = 0; n < N; n++) o->area[n] = o->area[n] * R_EARTH * R_EARTH;
+}
+
+/* mesh_arrays2: src/fv_mesh_array.F90:1155-1386 */
+static void mesh_arrays2(orc_t *o) {
+    int E = o->E, D = o->D;
+    double cl = o->cyclic_length;
+    o->edge_dxdy = DALLOC(2 * (size_t)D); o->edge_cross_dxdy = DALLOC(4 * (size_t)D);
+    o->gradient_sca = DALLOC(8 * (size_t)E); o->gradient_vec = DALLOC(8 * (size_t)E);
+    double *cx = DALLOC(E), *cy = DALLOC(E);
+    for (int el = 0; el < E; el++) { double aa; elem_center(o, el, &cx[el], &cy[el], &aa); }
+    for (int n = 0; n < D; n++) {
+        int a0 = o->edge_nodes[2 * n], a1 = o->edge_nodes[2 * n + 1];
+        double a[2] = {o->coord[2 * a1] - o->coord[2 * a0], o->coord[2 * a1 + 1] - o->coord[2 * a0 + 1]};
+        if (a[0] > cl / 2.0) a[0] -= cl;
+        if (a[0] < -cl / 2.0) a[0] += cl;
+        o->edge_dxdy[2 * n] = a[0]; o->edge_dxdy[2 * n + 1] = a[1];
+    }
+    for (int ed = 0; ed < D; ed++) {
+        int n0 = o->edge_nodes[2 * ed], n1 = o->edge_nodes[2 * ed + 1];
+        int el0 = o->edge_tri[2 * ed], el1 = o->edge_tri[2 * ed + 1];
+        double a[2], b[2];
+        edge_center(o, n0, n1, &a[0], &a[1]);
+        b[0] = cx[el0] - a[0]; b[1] = cy[el0] - a[1];
+        if (b[0] > cl / 2.0) b[0] -= cl;
+        if (b[0] < -cl / 2.0) b[0] += cl;
+        b[0] = b[0] * o->elem_cos[el0];
+        b[0] = b[0] * R_EARTH; b[1] = b[1] * R_EARTH;
+        o->edge_cross_dxdy[4 * ed] = b[0]; o->edge_cross_dxdy[4 * ed + 1] = b[1];
+        if (el1 >= 0) {
+            b[0] = cx[el1] - a[0]; b[1] = cy[el1] - a[1];
+            if (b[0] > cl / 2.0) b[0] -= cl;
+            if (b[0] < -cl / 2.0) b[0] += cl;
+            b[0] = b[0] * o->elem_cos[el1];
+            b[0] = b[0] * R_EARTH; b[1] = b[1] * R_EARTH;
+            o->edge_cross_dxdy[4 * ed + 2] = b[0]; o->edge_cross_dxdy[4 * ed + 3] = b[1];
+        } else {
+            o->edge_cross_dxdy[4 * ed + 2] = 0.0; o->edge_cross_dxdy[4 * ed + 3] = 0.0;
+        }
+    }
+    for (int el = 0; el < E; el++) {
+        int *en = o->elem_nodes + 4 * el;
+        double *gs = o->gradient_sca + 8 * el;
+        if (en[0] == en[3]) {
+            double dX31 = o->coord[2 * en[2]] - o->coord[2 * en[0]];
+            if (dX31 > cl / 2.0) dX31 -= cl;
+            if (dX31 < -cl / 2.0) dX31 += cl;
+            dX31 = o->elem_cos[el] * dX31;
+            double dX21 = o->coord[2 * en[1]] - o->coord[2 * en[0]];
+            if (dX21 > cl / 2) dX21 -= cl;
+            if (dX21 < -cl / 2) dX21 += cl;
+            dX21 = o->elem_cos[el] * dX21;
+            double dY31 = o->coord[2 * en[2] + 1] - o->coord[2 * en[0] + 1];
+            double dY21 = o->coord[2 * en[1] + 1] - o->coord[2 * en[0] + 1];
+            double df = -0.5 * R_EARTH / o->elem_area[el];
+            gs[0] = (-dY31 + dY21) * df;
+            gs[1] = dY31 * df;
+            gs[2] = -dY21 * df;
+            gs[3] = 0.0;
+            gs[4] = (dX31 - dX21) * df;
+            gs[5] = -dX31 * df;
+            gs[6] = dX21 * df;
+            gs[7] = 0.0;
+        } else {
+            int e6[6] = {en[3], en[0], en[1], en[2], en[3], en[0]};
+            double df = 0.5 * R_EARTH / o->elem_area[el];
+            for (int k = 0; k < 8; k++) gs[k] = 0.0;
+            for (int j = 1; j <= 4; j++) {
+                double x1[2] = {o->coord[2 * e6[j]], o->coord[2 * e6[j] + 1]};
+                double x2[2] = {o->coord[2 * e6[j - 1]], o->coord[2 * e6[j - 1] + 1]};
+                double xd[2], x3[2];
+                cdiff(o, x1, x2, xd);
+                x2[0] = o->coord[2 * e6[j + 1]]; x2[1] = o->coord[2 * e6[j + 1] + 1];
+                cdiff(o, x2, x1, x3);
+                xd[0] = xd[0] + x3[0]; xd[1] = xd[1] + x3[1];
+                xd[0] = xd[0] * o->elem_cos[el];
+                gs[j - 1] = -xd[1] * df;
+                gs[j - 1 + 4] = xd[0] * df;
+            }
+        }
+    }
+    for (int el = 0; el < E; el++) {
+        double a0 = cx[el], a1 = cy[el];
+        int q = (o->elem_nodes[4 * el] == o->elem_nodes[4 * el + 3]) ? 3 : 4;
+        double x[4], y[4];
+        for (int j = 0; j < q; j++) {
+            int nb = o->elem_nb[4 * el + j];
+            if (nb >= 0) {
+                x[j] = cx[nb] - a0;
+                if (x[j] > cl / 2.0) x[j] -= cl;
+                if (x[j] < -cl / 2.0) x[j] += cl;
+                y[j] = cy[nb] - a1;
+            } else {
+                int ed = o->elem_edges[4 * el + j];
+                double b0, b1;
+                edge_center(o, o->edge_nodes[2 * ed], o->edge_nodes[2 * ed + 1], &b0, &b1);
+                x[j] = (b0 - a0);
+                if (x[j] > cl / 2.0) x[j] -= cl;
+                if (x[j] < -cl / 2.0) x[j] += cl;
+                x[j] = 2.0 * x[j];
+                y[j] = 2.0 * (b1 - a1);
+            }
+        }
+        if (q == 3) { x[3] = 0.0; y[3] = 0.0; }
+        for (int j = 0; j < 4; j++) { x[j] = x[j] * o->elem_cos[el] * R_EARTH; y[j] = y[j] * R_EARTH; }
+        double cxx = 0, cxy = 0, cyy = 0;
+        for (int j = 0; j < 4; j++) { cxx += x[j] * x[j]; cxy += x[j] * y[j]; cyy += y[j] * y[j]; }
+        double d = cxy * cxy - cxx * cyy;
+        for (int j = 0; j < 4; j++) {
+            o->gradient_vec[8 * el + j] = (cxy * y[j] - cyy * x[j]) / d;
+            o->gradient_vec[8 * el + 4 + j] = (cxy * x[j] - cxx * y[j]) / d;
+        }
+    }
+    free(cx); free(cy);
+}
+
+/* ------------------------------------------------------------------ */
+/* jacobian: src/fv_main.F90:1208-1267                                   */
+/* ------------------------------------------------------------------ */
+static void jacobian(orc_t *o) {
+    int L = o->L, K = o->K;
+    for (int n = 0; n < o->N; n++) {
+        for (int k = 0; k < L; k++) o->Jc_old[(size_t)n * L + k] = o->Jc[(size_t)n * L + k];
+        double a = dmax(o->depth[n] + o->eta_n[n], o->Dmin);
+        for (int k = 0; k < K; k++) o->zbar[(size_t)n * K + k] = (1.0 - o->sigma[k]) * a;
+        for (int k = 0; k < L; k++) o->Jc[(size_t)n * L + k] = a * (o->sigma[k] - o->sigma[k + 1]);
+        for (int k = 0; k < L; k++) o->Z[(size_t)n * L + k] = 0.5 * (o->zbar[(size_t)n * K + k] + o->zbar[(size_t)n * K + k + 1]);
+    }
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *w = o->w_cv + 4 * el;
+        for (int k = 0; k < L; k++)
+            o->Je[(size_t)el * L + k] = w[0] * o->Jc[(size_t)en[0] * L + k] + w[1] * o->Jc[(size_t)en[1] * L + k]
+                                      + w[2] * o->Jc[(size_t)en[2] * L + k] + w[3] * o->Jc[(size_t)en[3] * L + k];
+    }
+    for (int ed = 0; ed < o->D; ed++) {
+        int n0 = o->edge_nodes[2 * ed], n1 = o->edge_nodes[2 * ed + 1];
+        for (int k = 0; k < L; k++)
+            o->Jd[(size_t)ed * L + k] = 0.5 * (o->Jc[(size_t)n0 * L + k] + o->Jc[(size_t)n1 * L + k]);
+    }
+}
+
+/* wad_mask: src/fv_main.F90:1269-1307 */
+static void wad_mask(orc_t *o) {
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        double mn = dmin(dmin(dmin(o->depth[en[0]], o->depth[en[1]]), o->depth[en[2]]), o->depth[en[3]]);
+        double mx = dmax(dmax(dmax(o->etaAB[en[0]], o->etaAB[en[1]]), o->etaAB[en[2]]), o->etaAB[en[3]]);
+        o->mask_wd[el] = (mn + mx > o->Dmin) ? 1.0 : 0.0;
+    }
+    for (int n = 0; n < o->N; n++) o->mask_wd_node[n] = (o->depth[n] + o->etaAB[n] > o->Dmin) ? 1.0 : 0.0;
+}
+
+/* adv_tracer_mask: src/fv_main.F90:1309-1335 */
+static void adv_tracer_mask(orc_t *o) {
+    for (int n = 0; n < o->N; n++) {
+        double dm = dmax(o->Dmin, (o->depth[n] + o->eta_n[n]));
+        if (dm > H_ADVTR_CRIT) o->mask_ad[n] = 1.0;
+        else if (dm < H_ADVTR_MIN) o->mask_ad[n] = 0.0;
+        else o->mask_ad[n] = (dm - H_ADVTR_MIN) / (H_ADVTR_CRIT - H_ADVTR_MIN);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* pressure + baroclinic_pressure_gradient_3D: src/fv_pressure_new.F90:7-221 */
+/* ------------------------------------------------------------------ */
+static void pressure(orc_t *o) {
+    int L = o->L, K = o->K;
+    double gd0 = G / o->density_0;
+    for (int n = 0; n < o->N; n++) {
+        o->hpressure[(size_t)n * K] = 0.0;
+        for (int k = 0; k < L; k++) {
+            double r = orc_densityJM(o->TF[(size_t)n * L + k], o->SF[(size_t)n * L + k], -o->Z[(size_t)n * L + k], o->density_0);
+            o->rho_c[(size_t)n * L + k] = r;
+            o->hpressure[(size_t)n * K + k + 1] = o->hpressure[(size_t)n * K + k] + r * o->Jc[(size_t)n * L + k] * gd0;
+        }
+    }
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *gs = o->gradient_sca + 8 * el, *w = o->w_cv + 4 * el;
+        double h[4], d[4];
+        for (int i = 0; i < 4; i++) { d[i] = o->depth[en[i]]; h[i] = o->depth[en[i]] + o->eta_n[en[i]]; }
+        double Ax = sum4w(gs, h[0], h[1], h[2], h[3]);
+        double Ay = sum4w(gs + 4, h[0], h[1], h[2], h[3]);
+        double Bx = sum4w(gs, d[0], d[1], d[2], d[3]);
+        double By = sum4w(gs + 4, d[0], d[1], d[2], d[3]);
+        double acc = sum4w(w, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
+        double accB = pow(acc, o->Bp_bz) * o->mask_bpr[el];
+        for (int k = 0; k < L; k++) {
+            double aJ = -accB / o->Je[(size_t)el * L + k];
+            double hp0[4], hp1[4], pre[4];
+            for (int i = 0; i < 4; i++) {
+                hp0[i] = o->hpressure[(size_t)en[i] * K + k];
+                hp1[i] = o->hpressure[(size_t)en[i] * K + k + 1];
+                pre[i] = 0.5 * (hp0[i] + hp1[i]) * o->Jc[(size_t)en[i] * L + k];
+            }
+            double qd_up = sum4w(w, hp0[0], hp0[1], hp0[2], hp0[3]);
+            double qd_low = sum4w(w, hp1[0], hp1[1], hp1[2], hp1[3]);
+            o->Bar_pru_3D[(size_t)el * L + k] = aJ * (sum4w(gs, pre[0], pre[1], pre[2], pre[3])
+                + qd_low * (o->sigma[k + 1] * Ax - Bx) - qd_up * (o->sigma[k] * Ax - Bx));
+            o->Bar_prv_3D[(size_t)el * L + k] = aJ * (sum4w(gs + 4, pre[0], pre[1], pre[2], pre[3])
+                + qd_low * (o->sigma[k + 1] * Ay - By) - qd_up * (o->sigma[k] * Ay - By));
+        }
+        double s1 = 0, s2 = 0;
+        for (int k = 0; k < L; k++) {
+            s1 = s1 + o->Bar_pru_3D[(size_t)el * L + k] * o->Je[(size_t)el * L + k];
+            s2 = s2 + o->Bar_prv_3D[(size_t)el * L + k] * o->Je[(size_t)el * L + k];
+        }
+        o->Bar_pr_2D[2 * el] = s1; o->Bar_pr_2D[2 * el + 1] = s2;
+    }
+}
+
+/* compute_el2nodes_3D: src/fv_utilit.F90:644-700 */
+static void el2nodes_3D(orc_t *o, const double *Ue, const double *Ve, double *Un, double *Vn) {
+    int L = o->L;
+    for (int n = 0; n < o->N; n++) {
+        int num = o->nie_num[n];
+        const int *lst = o->nie + (size_t)n * o->maxdeg;
+        double s = 0.0;
+        for (int k = 0; k < num; k++) s = s + o->elem_area[lst[k]];
+        double tvi = 1.0 / s;
+        for (int z = 0; z < L; z++) { Un[(size_t)n * L + z] = 0.0; Vn[(size_t)n * L + z] = 0.0; }
+        for (int k = 0; k < num; k++) {
+            int el = lst[k];
+            for (int z = 0; z < L; z++) {
+                Un[(size_t)n * L + z] = Un[(size_t)n * L + z] + Ue[(size_t)el * L + z] * o->elem_area[el];
+                Vn[(size_t)n * L + z] = Vn[(size_t)n * L + z] + Ve[(size_t)el * L + z] * o->elem_area[el];
+            }
+        }
+        for (int z = 0; z < L; z++) { Un[(size_t)n * L + z] *= tvi; Vn[(size_t)n * L + z] *= tvi; }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* d3_end: src/fv_mixing.F90:73-347                                      */
+/* ------------------------------------------------------------------ */
+static void d3_end(orc_t *o) {
+    int L = o->L, K = o->K;
+    double *xx1 = DALLOC(K + 1), *yy1 = DALLOC(K + 1), *btr = DALLOC(K + 1), *btr1 = DALLOC(K + 1);
+    double *b1 = DALLOC(K + 1), *b2 = DALLOC(K + 1), *c2 = DALLOC(K + 1);
+    /* 1-based indexing below, as in the reference */
+    const double *sg = o->sigma - 1;
+    double t_inv = 1.0 / o->dt;
+    double B1cka2 = 16.6 * (CKA * CKA);
+    for (size_t i = 0; i < (size_t)K * o->N; i++) o->Ri[i] = 0.0;
+    for (int node = 0; node < o->N; node++) {
+        double *bt = o->bt + (size_t)node * K - 1, *snu = o->snu + (size_t)node * K - 1, *Kv = o->Kv + (size_t)node * K - 1;
+        double *Ri = o->Ri + (size_t)node * K - 1, *tkd = o->tke_dissip + (size_t)node * K - 1;
+        const double *Un = o->Unode + (size_t)node * L - 1, *Vn = o->Vnode + (size_t)node * L - 1;
+        const double *TF = o->TF + (size_t)node * L - 1, *SF = o->SF + (size_t)node * L - 1;
+        const double *zb = o->zbar + (size_t)node * K - 1;
+        double dmean = dmax(o->Dmin, o->depth[node] + o->eta_n[node]);
+        double dmean2 = dmean * dmean;
+        for (int nz = 1; nz <= K; nz++) btr[nz] = bt[nz];
+        for (int nz = 2; nz <= K - 1; nz++) {
+            double d1 = sg[nz - 1] - sg[nz];
+            double d2 = sg[nz] - sg[nz + 1];
+            double d = 0.5 * (d1 + d2);
+            double ap1 = Un[nz - 1] - Un[nz];
+            double ap2 = Vn[nz - 1] - Vn[nz];
+            b1[nz] = snu[nz] * (ap1 * ap1 + ap2 * ap2) / (d * d);
+            b1[nz] = b1[nz] / dmean2;
+            double rho_up = orc_densityJM(TF[nz - 1], SF[nz - 1], -zb[nz], o->density_0);
+            double rho_dn = orc_densityJM(TF[nz], SF[nz], -zb[nz], o->density_0);
+            ap1 = (rho_up - rho_dn) / d;
+            b2[nz] = -0.5 * G * Kv[nz] * ap1 / (o->density_0 * dmean);
+            double snu_p = 0.5 * (snu[nz - 1] + snu[nz]);
+            double snu_m = 0.5 * (snu[nz] + snu[nz + 1]);
+            c2[nz] = 0.5 * 0.73 * (snu_p * (bt[nz - 1] - bt[nz]) / d1 - snu_m * (bt[nz] - bt[nz + 1]) / d2) / d;
+            c2[nz] = c2[nz] / dmean2;
+            if (b1[nz] > 0.0) Ri[nz] = -G * ap1 / (o->density_0 * dmean) / b1[nz] * snu[nz];
+            else Ri[nz] = 10000.0;
+        }
+        xx1[2] = 1.0;
+        yy1[2] = 0.0;
+        double d1w = 0.5 * (sg[K - 1] - sg[K]);
+        double wx = o->windx[node], wy = o->windy[node];
+        double u_din_sur = 0.05 * (wx * wx + wy * wy);
+        double snu_mw = 0.5 * (snu[K] + snu[K - 1]);
+        yy1[2] = (0.4e-3 * (u_din_sur * u_din_sur * u_din_sur) * dmean * d1w) / (snu_mw * 0.76);
+        for (int it = 1; it <= 200; it++) {
+            for (int nz = 2; nz <= K - 1; nz++) {
+                double d1 = sg[nz - 1] - sg[nz];
+                double d2 = sg[nz] - sg[nz + 1];
+                double d = 0.5 * (d1 + d2);
+                double c1 = 0.046 * (btr[nz] * btr[nz]) / snu[nz];
+                tkd[nz] = c1;
+                double fr = -b1[nz] - b2[nz] - c1 - t_inv * bt[nz] - c2[nz];
+                double snu_p = 0.5 * (snu[nz - 1] + snu[nz]);
+                double snu_m = 0.5 * (snu[nz] + snu[nz + 1]);
+                double am = 0.5 * (snu_p * 0.73) / (dmean2 * d1 * d);
+                double cm = 0.5 * (snu_m * 0.73) / (dmean2 * d2 * d);
+                double bm = am + cm + (2.0 * 0.046 * btr[nz]) / snu[nz] + t_inv;
+                double a = bm - am * xx1[nz];
+                xx1[nz + 1] = cm / a;
+                yy1[nz + 1] = (am * yy1[nz] - fr) / a;
+            }
+            double delta_b = o->Jc[(size_t)node * L + (L - 1)] + o->z0b_min;
+            double lg = log(delta_b / o->z0b_min);
+            double bott_R = B1cka2 / (lg * lg);
+            btr1[K] = bott_R * (Un[K - 1] * Un[K - 1] + Vn[K - 1] * Vn[K - 1]);
+            for (int nz = K; nz >= 2; nz--) btr1[nz - 1] = btr1[nz] * xx1[nz] + yy1[nz];
+            double bbb_max = 0.0;
+            for (int nz = 1; nz <= K; nz++) {
+                double a = btr[nz] - btr1[nz];
+                double b = dmax(btr1[nz], btr[nz]) + 1.0e-8;
+                double c = fabs(a / b);
+                if (c > bbb_max) bbb_max = c;
+            }
+            if (bbb_max < 1.0e-6) break;
+            for (int nz = 1; nz <= K; nz++) btr[nz] = btr1[nz];
+        }
+        for (int nz = 1; nz <= K; nz++) {
+            bt[nz] = dmax(btr[nz], 1.0e-8);
+            double zt = sg[nz] * dmean;
+            double Zh = zt + o->z0b_min;
+            double Zd = fabs(dmean - zt) + o->z0s_min;
+            double Z0 = 1.0 - BETA_SCALE * Zh * Zd / (dmean * dmean);
+            double scale_t = CKA * Zh * Zd * Z0 / dmean;
+            snu[nz] = scale_t * sqrt(bt[nz]) + o->snul;
+            if (snu[nz] >= 0.2) snu[nz] = 0.2;
+            Kv[nz] = PR_NUM * (snu[nz]);
+        }
+    }
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *w = o->w_cv + 4 * el;
+        for (int k = 0; k < K; k++)
+            o->Av[(size_t)el * K + k] = sum4w(w, o->snu[(size_t)en[0] * K + k], o->snu[(size_t)en[1] * K + k],
+                                              o->snu[(size_t)en[2] * K + k], o->snu[(size_t)en[3] * K + k]);
+    }
+    free(xx1); free(yy1); free(btr); free(btr1); free(b1); free(b2); free(c2);
+}
+
+/* ------------------------------------------------------------------ */
+/* vel_gradients (3-D): src/fv_gradients.F90:2-93 (free_slip = .true.)  */
+/* ------------------------------------------------------------------ */
+static void vel_gradients(orc_t *o) {
+    int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        int jend = (o->elem_nodes[4 * el] == o->elem_nodes[4 * el + 3]) ? 3 : 4;
+        double *ux = o->vel_grad_ux + (size_t)el * L, *uy = o->vel_grad_uy + (size_t)el * L;
+        double *vx = o->vel_grad_vx + (size_t)el * L, *vy = o->vel_grad_vy + (size_t)el * L;
+        const double *gv = o->gradient_vec + 8 * el;
+        for (int z = 0; z < L; z++) { ux[z] = 0; uy[z] = 0; vx[z] = 0; vy[z] = 0; }
+        for (int j = 0; j < jend; j++) {
+            int el2 = o->elem_nb[4 * el + j];
+            if (el2 >= 0) {
+                for (int z = 0; z < L; z++) {
+                    double du = o->U_n[(size_t)el2 * L + z] - o->U_n[(size_t)el * L + z];
+                    double dv = o->V_n[(size_t)el2 * L + z] - o->V_n[(size_t)el * L + z];
+                    ux[z] = ux[z] + gv[j] * du;
+                    uy[z] = uy[z] + gv[j + 4] * du;
+                    vx[z] = vx[z] + gv[j] * dv;
+                    vy[z] = vy[z] + gv[j + 4] * dv;
+                }
+            } else {
+                int ed = o->elem_edges[4 * el + j];
+                double zc0 = o->edge_dxdy[2 * ed] * o->elem_cos[el], zc1 = o->edge_dxdy[2 * ed + 1];
+                for (int z = 0; z < L; z++) {
+                    double un = -2 * (o->U_n[(size_t)el * L + z] * zc1 - o->V_n[(size_t)el * L + z] * zc0) / (zc0 * zc0 + zc1 * zc1);
+                    ux[z] = ux[z] + gv[j] * un * zc1;
+                    uy[z] = uy[z] + gv[j + 4] * un * zc1;
+                    vx[z] = vx[z] - gv[j] * un * zc0;
+                    vy[z] = vy[z] - gv[j + 4] * un * zc0;
+                }
+            }
+        }
+    }
+}
+
+/* vel_gradients_2D: src/fv_gradients.F90:97-176 */
+static void vel_gradients_2D(orc_t *o, const double *Uin) {
+    for (int el = 0; el < o->E; el++) {
+        int jend = (o->elem_nodes[4 * el] == o->elem_nodes[4 * el + 3]) ? 3 : 4;
+        double *vg = o->vel_grad + 4 * el;
+        const double *gv = o->gradient_vec + 8 * el;
+        vg[0] = vg[1] = vg[2] = vg[3] = 0.0;
+        for (int j = 0; j < jend; j++) {
+            int el2 = o->elem_nb[4 * el + j];
+            if (el2 >= 0) {
+                vg[0] = vg[0] + gv[j] * (Uin[2 * el2] - Uin[2 * el]);
+                vg[1] = vg[1] + gv[j + 4] * (Uin[2 * el2] - Uin[2 * el]);
+                vg[2] = vg[2] + gv[j] * (Uin[2 * el2 + 1] - Uin[2 * el + 1]);
+                vg[3] = vg[3] + gv[j + 4] * (Uin[2 * el2 + 1] - Uin[2 * el + 1]);
+            } else {
+                int ed = o->elem_edges[4 * el + j];
+                double zc0 = o->edge_dxdy[2 * ed] * o->elem_cos[el];
+                double zc1 = o->edge_dxdy[2 * ed + 1];
+                double un = -2.0 * (Uin[2 * el] * zc1 - Uin[2 * el + 1] * zc0) / (zc0 * zc0 + zc1 * zc1);
+                vg[0] = vg[0] + gv[j] * un * zc1;
+                vg[1] = vg[1] + gv[j + 4] * un * zc1;
+                vg[2] = vg[2] - gv[j] * un * zc0;
+                vg[3] = vg[3] - gv[j + 4] * un * zc0;
+            }
+        }
+    }
+}
+
+/* momentum_adv_upwind: src/fv_advection.F90:64-197 */
+static void momentum_adv_upwind(orc_t *o) {
+    int L = o->L;
+    double *uu = DALLOC(L), *vv = DALLOC(L);
+    for (int ed = 0; ed < o->D_in; ed++) {
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        double acc = 0.5 * (o->ac[o->edge_nodes[2 * ed]] + o->ac[o->edge_nodes[2 * ed + 1]]);
+        const double *cd = o->edge_cross_dxdy + 4 * ed, *dd = o->edge_dxdy + 2 * ed;
+        for (int z = 0; z < L; z++) {
+            size_t i1 = (size_t)e1 * L + z, i2 = (size_t)e2 * L + z;
+            double u1 = o->U_n[i1] - o->vel_grad_ux[i1] * cd[0] - o->vel_grad_uy[i1] * cd[1];
+            double v1 = o->V_n[i1] - o->vel_grad_vx[i1] * cd[0] - o->vel_grad_vy[i1] * cd[1];
+            double u2 = o->U_n[i2] - o->vel_grad_ux[i2] * cd[2] - o->vel_grad_uy[i2] * cd[3];
+            double v2 = o->V_n[i2] - o->vel_grad_vx[i2] * cd[2] - o->vel_grad_vy[i2] * cd[3];
+            double un = 0.5 * R_EARTH * o->Jd[(size_t)ed * L + z]
+                      * ((u1 + u2) * dd[1] - (v1 * o->elem_cos[e1] + v2 * o->elem_cos[e2]) * dd[0]);
+            if (un >= 0.0) { uu[z] = u1 * un * acc; vv[z] = v1 * un * acc; }
+            else { uu[z] = u2 * un * acc; vv[z] = v2 * un * acc; }
+        }
+        for (int z = 0; z < L; z++) {
+            o->U_rhsAB[(size_t)e1 * L + z] = o->U_rhsAB[(size_t)e1 * L + z] - uu[z];
+            o->V_rhsAB[(size_t)e1 * L + z] = o->V_rhsAB[(size_t)e1 * L + z] - vv[z];
+            o->U_rhsAB[(size_t)e2 * L + z] = o->U_rhsAB[(size_t)e2 * L + z] + uu[z];
+            o->V_rhsAB[(size_t)e2 * L + z] = o->V_rhsAB[(size_t)e2 * L + z] + vv[z];
+        }
+    }
+    free(uu); free(vv);
+}
+
+/* momentum_vert_adv_upwind: src/fv_advection.F90:354-469 */
+static void momentum_vert_adv_upwind(orc_t *o) {
+    int L = o->L, K = o->K;
+    double *uv0 = DALLOC(K + 1), *uv1 = DALLOC(K + 1); /* 1-based */
+    for (int el = 0; el < o->E; el++) {
+        double *Ur = o->U_rhsAB + (size_t)el * L - 1, *Vr = o->V_rhsAB + (size_t)el * L - 1;
+        if (o->mask_wd[el] != 0.0) {
+            const int *en = o->elem_nodes + 4 * el;
+            const double *w = o->w_cv + 4 * el;
+            const double *U = o->U_n + (size_t)el * L - 1, *V = o->V_n + (size_t)el * L - 1;
+            double acc = sum4w(w, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
+#define WSUM(arr, stride, lev) sum4w(w, (arr)[(size_t)en[0] * (stride) + (lev) - 1], (arr)[(size_t)en[1] * (stride) + (lev) - 1], \
+                                     (arr)[(size_t)en[2] * (stride) + (lev) - 1], (arr)[(size_t)en[3] * (stride) + (lev) - 1])
+            double umean = 0, vmean = 0;
+            for (int nz = 2; nz <= K - 1; nz++) {
+                double wv = WSUM(o->Wvel, K, nz) * o->elem_area[el];
+                if (wv >= 0.0) {
+                    if (nz == K - 1) {
+                        umean = 0.5 * (U[nz - 1] + U[nz]);
+                        vmean = 0.5 * (V[nz - 1] + V[nz]);
+                    } else {
+                        double Z1 = WSUM(o->zbar, K, nz);
+                        double a = Z1 - WSUM(o->Z, L, nz - 1);
+                        double b = WSUM(o->Z, L, nz) - Z1;
+                        double c = WSUM(o->Z, L, nz + 1) - Z1;
+                        double d = (c + a) * (b * b - a * a) - (c * c - a * a) * (b + a);
+                        double dg = a * b * (a + b) / d;
+                        double db = -a * c * (a + c) / d;
+                        double da = 1.0 - dg - db;
+                        umean = U[nz - 1] * da + U[nz] * db + U[nz + 1] * dg;
+                        vmean = V[nz - 1] * da + V[nz] * db + V[nz + 1] * dg;
+                    }
+                }
+                if (wv < 0.0) {
+                    if (nz == 2) {
+                        umean = 0.5 * (U[nz - 1] + U[nz]);
+                        vmean = 0.5 * (V[nz - 1] + V[nz]);
+                    } else {
+                        double Z1 = WSUM(o->zbar, K, nz);
+                        double a = WSUM(o->Z, L, nz) - Z1;
+                        double b = Z1 - WSUM(o->Z, L, nz - 1);
+                        double c = Z1 - WSUM(o->Z, L, nz - 2);
+                        double d = (c + a) * (b * b - a * a) - (c * c - a * a) * (b + a);
+                        double dg = a * b * (a + b) / d;
+                        double db = -a * c * (a + c) / d;
+                        double da = 1.0 - dg - db;
+                        umean = U[nz] * da + U[nz - 1] * db + U[nz - 2] * dg;
+                        vmean = V[nz] * da + V[nz - 1] * db + V[nz - 2] * dg;
+                    }
+                }
+                uv0[nz] = -umean * wv;
+                uv1[nz] = -vmean * wv;
+            }
+            double wv = WSUM(o->Wvel, K, 1) * o->elem_area[el];
+            uv0[1] = -wv * U[1];
+            uv1[1] = -wv * V[1];
+            uv0[K] = 0.0; uv1[K] = 0.0;
+            for (int nz = 1; nz <= L; nz++) {
+                Ur[nz] = Ur[nz] + (uv0[nz] - uv0[nz + 1]) * acc;
+                Vr[nz] = Vr[nz] + (uv1[nz] - uv1[nz + 1]) * acc;
+            }
+#undef WSUM
+        } else {
+            for (int nz = 1; nz <= L; nz++) { Ur[nz] = 0.0; Vr[nz] = 0.0; }
+        }
+    }
+    free(uv0); free(uv1);
+}
+
+/* momentum_vert_impl_visc_tmp: src/fv_viscosity.F90:1374-1486 (no ice) */
+static void momentum_vert_impl_visc_tmp(orc_t *o) {
+    int L = o->L, K = o->K;
+    double *xx1 = DALLOC(K + 1), *yu = DALLOC(K + 1), *yv = DALLOC(K + 1), *Ut = DALLOC(K + 1), *Vt = DALLOC(K + 1);
+    for (int el = 0; el < o->E; el++) {
+        double *Ur = o->U_rhs + (size_t)el * L - 1, *Vr = o->V_rhs + (size_t)el * L - 1;
+        if (o->mask_wd[el] != 0.0) {
+            const int *en = o->elem_nodes + 4 * el;
+            const double *Je = o->Je + (size_t)el * L - 1, *Av = o->Av + (size_t)el * K - 1;
+            xx1[1] = 1.0;
+            double de = Je[1] / (Av[1] + Av[2]);
+            double rw = de / o->density_0;
+            yu[1] = o->taux[el] * rw;
+            yv[1] = o->tauy[el] * rw;
+            for (int nz = 1; nz <= L; nz++) {
+                double Fx = Ur[nz], Fy = Vr[nz];
+                double delta_1 = (nz == 1) ? 0.5 * Je[nz] : 0.5 * (Je[nz] + Je[nz - 1]);
+                double delta_2 = (nz == L) ? 0.5 * Je[nz] : 0.5 * (Je[nz] + Je[nz + 1]);
+                double zt = o->dt / Je[nz];
+                double a = zt * Av[nz] / delta_1;
+                double c = zt * Av[nz + 1] / delta_2;
+                double b = a + c + 1.0;
+                double d = b - a * xx1[nz];
+                xx1[nz + 1] = c / d;
+                yu[nz + 1] = (a * yu[nz] + Fx) / d;
+                yv[nz + 1] = (a * yv[nz] + Fy) / d;
+            }
+            const double *w = o->w_cv + 4 * el;
+            double acc = sum4w(w, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
+            double a_Cd = o->C_d_el[el] * ((1.0 - acc) * o->Cd_bz + 1.0);
+            de = Je[L] / (Av[K] + Av[L]);
+            double ub = o->U_n[(size_t)el * L + L - 1], vb = o->V_n[(size_t)el * L + L - 1];
+            double friction = de * a_Cd * sqrt(ub * ub + vb * vb);
+            if (o->fic_point == 0) { Ut[K] = 0.0; Vt[K] = 0.0; }
+            else if (o->fic_point == 1) { double a = friction + 1.0 - xx1[K]; Ut[K] = yu[K] / a; Vt[K] = yv[K] / a; }
+            else { double a = 1.0 / (1.0 - friction) - xx1[K]; Ut[K] = yu[K] / a; Vt[K] = yv[K] / a; }
+            for (int nz = L; nz >= 1; nz--) {
+                Ut[nz] = Ut[nz + 1] * xx1[nz + 1] + yu[nz + 1];
+                Vt[nz] = Vt[nz + 1] * xx1[nz + 1] + yv[nz + 1];
+            }
+            for (int nz = 1; nz <= L; nz++) { Ur[nz] = Ut[nz] * o->mask_wd[el]; Vr[nz] = Vt[nz] * o->mask_wd[el]; }
+        } else {
+            for (int nz = 1; nz <= L; nz++) { Ur[nz] = 0.0; Vr[nz] = 0.0; }
+        }
+    }
+    free(xx1); free(yu); free(yv); free(Ut); free(Vt);
+}
+
+/* momentum_vert_impl_visc: src/fv_viscosity.F90:1266-1371 */
+static void momentum_vert_impl_visc(orc_t *o) {
+    int L = o->L, K = o->K;
+    double *a = DALLOC(L + 1), *b = DALLOC(L + 1), *c = DALLOC(L + 1), *ur = DALLOC(L + 1), *vr = DALLOC(L + 1);
+    double *cp = DALLOC(L + 1), *up = DALLOC(L + 1), *vp = DALLOC(L + 1);
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *w = o->w_cv + 4 * el;
+        const double *Je = o->Je + (size_t)el * L - 1, *Av = o->Av + (size_t)el * K - 1;
+        double acc = sum4w(w, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
+        double a_Cd = o->C_d_el[el] * ((1.0 - acc) * o->Cd_bz + 1.0);
+#define ZW(lev) sum4w(w, o->Z[(size_t)en[0] * L + (lev) - 1], o->Z[(size_t)en[1] * L + (lev) - 1], \
+                      o->Z[(size_t)en[2] * L + (lev) - 1], o->Z[(size_t)en[3] * L + (lev) - 1])
+        for (int nz = 2; nz <= L - 1; nz++) {
+            double zinv = o->dt / Je[nz];
+            double Zc = ZW(nz), Zu = ZW(nz - 1), Zb = ZW(nz + 1);
+            a[nz] = -Av[nz] / (Zc - Zu) * zinv;
+            c[nz] = -Av[nz + 1] / (Zb - Zc) * zinv;
+            b[nz] = -a[nz] - c[nz] + 1.0;
+        }
+        double zinv = o->dt / Je[L];
+        double Zc = ZW(L), Zu = ZW(L - 1);
+        a[L] = -Av[L] / (Zc - Zu) * zinv;
+        b[L] = -a[L] + 1.0;
+        c[L] = 0.0;
+        zinv = o->dt / Je[1];
+        Zc = ZW(1);
+        double Zb = ZW(2);
+        c[1] = -Av[2] / (Zb - Zc) * zinv;
+        a[1] = 0.0;
+        b[1] = -c[1] + 1.0;
+#undef ZW
+        for (int nz = 1; nz <= L; nz++) { ur[nz] = o->U_rhs[(size_t)el * L + nz - 1]; vr[nz] = o->V_rhs[(size_t)el * L + nz - 1]; }
+        ur[1] = ur[1] + zinv * o->taux[el] / o->density_0;
+        vr[1] = vr[1] + zinv * o->tauy[el] / o->density_0;
+        zinv = o->dt / Je[L];
+        double ub = o->U_n[(size_t)el * L + L - 1], vb = o->V_n[(size_t)el * L + L - 1];
+        double friction = -a_Cd * sqrt(ub * ub + vb * vb);
+        ur[L] = ur[L] + zinv * friction * ub;
+        vr[L] = vr[L] + zinv * friction * vb;
+        cp[1] = c[1] / b[1]; up[1] = ur[1] / b[1]; vp[1] = vr[1] / b[1];
+        for (int nz = 2; nz <= L; nz++) {
+            double m = b[nz] - cp[nz - 1] * a[nz];
+            cp[nz] = c[nz] / m;
+            up[nz] = (ur[nz] - up[nz - 1] * a[nz]) / m;
+            vp[nz] = (vr[nz] - vp[nz - 1] * a[nz]) / m;
+        }
+        ur[L] = up[L]; vr[L] = vp[L];
+        for (int nz = L - 1; nz >= 1; nz--) { ur[nz] = up[nz] - cp[nz] * ur[nz + 1]; vr[nz] = vp[nz] - cp[nz] * vr[nz + 1]; }
+        for (int nz = 1; nz <= L; nz++) {
+            o->U_rhs[(size_t)el * L + nz - 1] = ur[nz] * o->mask_wd[el];
+            o->V_rhs[(size_t)el * L + nz - 1] = vr[nz] * o->mask_wd[el];
+        }
+    }
+    free(a); free(b); free(c); free(ur); free(vr); free(cp); free(up); free(vp);
+}
+
+/* compute_vel_rhs: src/fv_3D_vel.F90:1-197 (key_atm off, filters off) */
+static void compute_vel_rhs(orc_t *o) {
+    int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        double pre[4];
+        for (int i = 0; i < 4; i++)
+            pre[i] = -G * (o->a_tp[2 * en[i]] * o->eta_n[en[i]] - o->a_tp[2 * en[i] + 1] * o->ssh_gp[en[i]]);
+        const double *gs = o->gradient_sca + 8 * el;
+        double Fx = sum4w(gs, pre[0], pre[1], pre[2], pre[3]) * o->elem_area[el];
+        double Fy = sum4w(gs + 4, pre[0], pre[1], pre[2], pre[3]) * o->elem_area[el];
+        for (int z = 0; z < L; z++) {
+            size_t i = (size_t)el * L + z;
+            o->U_rhs[i] = -(0.5 + EPS) * o->U_rhsAB[i] + Fx * o->Je[i];
+            o->V_rhs[i] = -(0.5 + EPS) * o->V_rhsAB[i] + Fy * o->Je[i];
+            o->U_rhsAB[i] = o->V_n[i] * o->coriolis[el] * o->elem_area[el] * o->Je[i];
+            o->V_rhsAB[i] = -o->U_n[i] * o->coriolis[el] * o->elem_area[el] * o->Je[i];
+        }
+    }
+    vel_gradients(o);
+    if (o->mom_adv_3D == 1) momentum_adv_upwind(o);
+    if (o->vert_adv == 1) momentum_vert_adv_upwind(o);
+    double ff = (1.5 + EPS);
+    if (o->lfirst) { ff = 1.0; o->lfirst = 0; }
+    for (int el = 0; el < o->E; el++) {
+        double area_inv = 1.0 / o->elem_area[el];
+        for (int z = 0; z < L; z++) {
+            size_t i = (size_t)el * L + z;
+            o->U_rhs[i] = o->mask_wd[el] * (o->U_n[i] + o->Bar_pru_3D[i] * o->dt
+                        + (o->dt * (o->U_rhs[i] + o->U_rhsAB[i] * ff) * area_inv + o->UV_rhs[2 * i]) / o->Je[i]);
+            o->V_rhs[i] = o->mask_wd[el] * (o->V_n[i] + o->Bar_prv_3D[i] * o->dt
+                        + (o->dt * (o->V_rhs[i] + o->V_rhsAB[i] * ff) * area_inv + o->UV_rhs[2 * i + 1]) / o->Je[i]);
+        }
+    }
+    if (o->im_vert_visc) momentum_vert_impl_visc(o);
+    if (o->im_vert_visc_fic) momentum_vert_impl_visc_tmp(o);
+}
+
+/* ------------------------------------------------------------------ */
+/* barotropic substep                                                    */
+/* ------------------------------------------------------------------ */
+
+/* compute_ssh_rhs_elem: src/fv_average_dynamic.F90:1-226 (no ice, no rivers) */
+static void compute_ssh_rhs_elem(orc_t *o) {
+    int E = o->E, D = o->D;
+    double *c1 = DALLOC(D), *aux = DALLOC(E);
+    for (int n = 0; n < o->N; n++) o->ssh_rhs[n] = 0.0;
+    for (int el = 0; el < E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        double ad[4];
+        for (int i = 0; i < 4; i++) {
+            ad[i] = o->Dmin;
+            double t = o->etaAB[en[i]] + o->depth[en[i]];
+            if (t > o->Dmin) ad[i] = t;
+        }
+        aux[el] = sum4w(o->w_cv + 4 * el, ad[0], ad[1], ad[2], ad[3]);
+    }
+    for (int ed = 0; ed < o->D_in; ed++) {
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const double *cd = o->edge_cross_dxdy + 4 * ed;
+        c1[ed] = (o->UAB[2 * e1 + 1] * cd[0] - o->UAB[2 * e1] * cd[1]) * aux[e1]
+               + (-o->UAB[2 * e2 + 1] * cd[2] + o->UAB[2 * e2] * cd[3]) * aux[e2];
+    }
+    for (int ed = o->D_in; ed < D; ed++) {
+        int e1 = o->edge_tri[2 * ed];
+        const double *cd = o->edge_cross_dxdy + 4 * ed;
+        c1[ed] = (o->UAB[2 * e1 + 1] * cd[0] - o->UAB[2 * e1] * cd[1]) * aux[e1];
+    }
+    for (int ed = 0; ed < D; ed++) {
+        o->ssh_rhs[o->edge_nodes[2 * ed]] = o->ssh_rhs[o->edge_nodes[2 * ed]] + c1[ed];
+        o->ssh_rhs[o->edge_nodes[2 * ed + 1]] = o->ssh_rhs[o->edge_nodes[2 * ed + 1]] - c1[ed];
+    }
+    if (o->TF_presence && o->nobn > 0) {
+        for (int i = 0; i < o->nobn; i++) o->ssh_rhs[o->in_obn[i]] = 0.0;
+        for (int i = 0; i < o->nobn; i++) {
+            int n = o->in_obn[i];
+            for (int j = 0; j < 12; j++)
+                o->ssh_rhs[n] = o->ssh_rhs[n] + o->amp_factor[j] * o->ampt[i * 12 + j]
+                    * cos((o->time_2D - o->time_jd0 * 86400.0) * 2.0 * PI / (TIDE_PERIOD[j] * 3600.0)
+                          - (o->fazt[i * 12 + j] + o->phy_factor[j]));
+            if (!o->restart)
+                if (o->time_jd - o->time_jd0 < 10.0) o->ssh_rhs[n] = o->ssh_rhs[n] * (o->time_jd - o->time_jd0) / 10.0;
+        }
+    }
+    free(c1); free(aux);
+}
+
+/* compute_puls_vel: src/fv_3D_vel.F90:257-307 */
+static void compute_puls_vel(orc_t *o) {
+    int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        double s = sum4w(o->w_cv + 4 * el, o->ssh_rhs[en[0]] + o->depth[en[0]], o->ssh_rhs[en[1]] + o->depth[en[1]],
+                         o->ssh_rhs[en[2]] + o->depth[en[2]], o->ssh_rhs[en[3]] + o->depth[en[3]]);
+        double a = o->dmean_n[el] / dmax(o->Dmin, s);
+        for (int z = 0; z < L; z++) {
+            o->U_puls[(size_t)el * L + z] = o->U_n[(size_t)el * L + z] - a * o->U_n_2D[2 * el];
+            o->V_puls[(size_t)el * L + z] = o->V_n[(size_t)el * L + z] - a * o->U_n_2D[2 * el + 1];
+        }
+    }
+}
+
+/* momentum_adv_P1_3D_to_2D: src/fv_advection.F90:676-793 */
+static void momentum_adv_P1_3D_to_2D(orc_t *o) {
+    int L = o->L;
+    double *ue1 = DALLOC(o->D), *ue2 = DALLOC(o->D), *ve1 = DALLOC(o->D), *ve2 = DALLOC(o->D);
+    double *uu = DALLOC(L), *vv = DALLOC(L), *un = DALLOC(L);
+    for (int ed = 0; ed < o->D_in; ed++) {
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1];
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        double acc = (o->ac[n1] + o->ac[n2]) * .5;
+        double xe = o->edge_dxdy[2 * ed] * R_EARTH * 0.5 * (o->elem_cos[e1] + o->elem_cos[e2]);
+        double ye = o->edge_dxdy[2 * ed + 1] * R_EARTH;
+        for (int z = 0; z < L; z++) {
+            uu[z] = 0.5 * (o->Unode_p[(size_t)n1 * L + z] + o->Unode_p[(size_t)n2 * L + z]);
+            vv[z] = 0.5 * (o->Vnode_p[(size_t)n1 * L + z] + o->Vnode_p[(size_t)n2 * L + z]);
+            un[z] = (uu[z] * ye - vv[z] * xe) * o->Jd[(size_t)ed * L + z];
+        }
+        double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+        for (int z = 0; z < L; z++) {
+            s1 = s1 + uu[z] * un[z] * o->Je[(size_t)e1 * L + z];
+            s2 = s2 + vv[z] * un[z] * o->Je[(size_t)e1 * L + z];
+            s3 = s3 + uu[z] * un[z] * o->Je[(size_t)e2 * L + z];
+            s4 = s4 + vv[z] * un[z] * o->Je[(size_t)e2 * L + z];
+        }
+        ue1[ed] = acc * s1; ve1[ed] = acc * s2; ue2[ed] = acc * s3; ve2[ed] = acc * s4;
+    }
+    for (int ed = 0; ed < o->D_in; ed++) {
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        o->U_rhs_2D_3D[2 * e1] = o->U_rhs_2D_3D[2 * e1] - ue1[ed];
+        o->U_rhs_2D_3D[2 * e1 + 1] = o->U_rhs_2D_3D[2 * e1 + 1] - ve1[ed];
+        o->U_rhs_2D_3D[2 * e2] = o->U_rhs_2D_3D[2 * e2] + ue2[ed];
+        o->U_rhs_2D_3D[2 * e2 + 1] = o->U_rhs_2D_3D[2 * e2 + 1] + ve2[ed];
+    }
+    free(ue1); free(ue2); free(ve1); free(ve2); free(uu); free(vv); free(un);
+}
+
+/* momentum_adv_upwind_2D: src/fv_advection.F90:546-669 */
+static void momentum_adv_upwind_2D(orc_t *o) {
+    double *uu = DALLOC(o->D), *vv = DALLOC(o->D);
+    for (int ed = 0; ed < o->D_in; ed++) {
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1];
+        double dm = dmax(o->Dmin, 0.5 * ((o->etaAB[n1] + o->depth[n1]) + (o->etaAB[n2] + o->depth[n2])));
+        double acc = (o->ac[n1] + o->ac[n2]) * 0.5;
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const double *cd = o->edge_cross_dxdy + 4 * ed, *dd = o->edge_dxdy + 2 * ed;
+        const double *g1 = o->vel_grad + 4 * e1, *g2 = o->vel_grad + 4 * e2;
+        double u1 = o->UAB[2 * e1] - g1[0] * cd[0] - g1[1] * cd[1];
+        double v1 = o->UAB[2 * e1 + 1] - g1[2] * cd[0] - g1[3] * cd[1];
+        double u2 = o->UAB[2 * e2] - g2[0] * cd[2] - g2[1] * cd[3];
+        double v2 = o->UAB[2 * e2 + 1] - g2[2] * cd[2] - g2[3] * cd[3];
+        double un = 0.5 * R_EARTH * dm * ((u1 + u2) * dd[1] - (v1 * o->elem_cos[e1] + v2 * o->elem_cos[e2]) * dd[0]);
+        u1 = o->UAB[2 * e1]; v1 = o->UAB[2 * e1 + 1];
+        u2 = o->UAB[2 * e2]; v2 = o->UAB[2 * e2 + 1];
+        if (un >= 0.0) { uu[ed] = un * u1 * acc; vv[ed] = un * v1 * acc; }
+        else { uu[ed] = un * u2 * acc; vv[ed] = un * v2 * acc; }
+    }
+    for (int ed = 0; ed < o->D_in; ed++) {
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        o->U_rhs_2D[2 * e1] = o->U_rhs_2D[2 * e1] - uu[ed];
+        o->U_rhs_2D[2 * e1 + 1] = o->U_rhs_2D[2 * e1 + 1] - vv[ed];
+        o->U_rhs_2D[2 * e2] = o->U_rhs_2D[2 * e2] + uu[ed];
+        o->U_rhs_2D[2 * e2 + 1] = o->U_rhs_2D[2 * e2 + 1] + vv[ed];
+    }
+    free(uu); free(vv);
+}
+
+/* update_2D_vel: src/fv_average_dynamic.F90:228-443 (key_atm off, 3-D branch) */
+static void update_2D_vel(orc_t *o) {
+    const double de = 0.614, ga = 0.088, ep = 0.013;
+    const double fe = 1.0 - de - ga - ep;
+    int L = o->L;
+    vel_gradients_2D(o, o->UAB);
+    for (int el = 0; el < o->E; el++) o->C_d_el[el] = o->C_d;
+    double d0i = 1.0 / (o->density_0);
+    double *eta = DALLOC(o->N);
+    for (int n = 0; n < o->N; n++) {
+        o->eta_n_2[n] = de * o->ssh_rhs[n] + fe * o->eta_n[n] + ga * o->eta_n_1[n] + ep * o->eta_n_2[n];
+        o->etaAB[n] = o->eta_n_2[n];
+        eta[n] = G * (o->a_tp[2 * n] * o->eta_n_2[n] - o->a_tp[2 * n + 1] * o->ssh_gp[n]);
+    }
+    if (o->wet_dry_on) wad_mask(o);
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *w = o->w_cv + 4 * el, *gs = o->gradient_sca + 8 * el;
+        double dm = dmax(o->Dmin, w[0] * (o->etaAB[en[0]] + o->depth[en[0]]) + w[1] * (o->etaAB[en[1]] + o->depth[en[1]])
+                                + w[2] * (o->etaAB[en[2]] + o->depth[en[2]]) + w[3] * (o->etaAB[en[3]] + o->depth[en[3]]));
+        o->U_rhs_2D[2 * el] = o->elem_area[el] * (-dm * (gs[0] * eta[en[0]] + gs[1] * eta[en[1]] + gs[2] * eta[en[2]] + gs[3] * eta[en[3]])
+                              + o->UAB[2 * el + 1] * dm * o->coriolis[el] + o->taux[el] * d0i);
+        o->U_rhs_2D[2 * el + 1] = o->elem_area[el] * (-dm * (gs[4] * eta[en[0]] + gs[5] * eta[en[1]] + gs[6] * eta[en[2]] + gs[7] * eta[en[3]])
+                                  - o->UAB[2 * el] * dm * o->coriolis[el] + o->tauy[el] * d0i);
+    }
+    free(eta);
+    if (o->mom_adv_2D == 2) momentum_adv_upwind_2D(o);
+    if (o->type_task > 2)
+        for (int el = 0; el < o->E; el++) {
+            o->U_rhs_2D[2 * el] = o->U_rhs_2D[2 * el] + o->U_rhs_2D_3D[2 * el];
+            o->U_rhs_2D[2 * el + 1] = o->U_rhs_2D[2 * el + 1] + o->U_rhs_2D_3D[2 * el + 1];
+        }
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *w = o->w_cv + 4 * el;
+        double dm = dmax(o->Dmin, sum4w(w, o->eta_n[en[0]] + o->depth[en[0]], o->eta_n[en[1]] + o->depth[en[1]],
+                                        o->eta_n[en[2]] + o->depth[en[2]], o->eta_n[en[3]] + o->depth[en[3]]));
+        double acc = sum4w(w, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
+        double a_Cd = o->C_d_el[el] * ((1.0 - acc) * o->Cd_bz + 1.0);
+        double ub = o->U_n[(size_t)el * L + L - 1], vb = o->V_n[(size_t)el * L + L - 1];
+        double friction = a_Cd * sqrt(ub * ub + vb * vb);
+        double ff = 1.0 / (dmax(o->Dmin, sum4w(w, o->ssh_rhs[en[0]] + o->depth[en[0]], o->ssh_rhs[en[1]] + o->depth[en[1]],
+                                              o->ssh_rhs[en[2]] + o->depth[en[2]], o->ssh_rhs[en[3]] + o->depth[en[3]]))
+                           + friction * o->dt_2D);
+        double U0 = o->U_n_2D[2 * el], V0 = o->U_n_2D[2 * el + 1];
+        o->U_rhs_2D[2 * el] = o->mask_wd[el] * ((dm * U0 - o->dt_2D * friction * (ub - U0) + o->dt_2D * o->Bar_pr_2D[2 * el]
+                              + o->dt_2D * o->U_rhs_2D[2 * el] / o->elem_area[el]) * ff + o->UV2_rhs[2 * el]);
+        o->U_rhs_2D[2 * el + 1] = o->mask_wd[el] * ((dm * V0 - o->dt_2D * friction * (vb - V0) + o->dt_2D * o->Bar_pr_2D[2 * el + 1]
+                                  + o->dt_2D * o->U_rhs_2D[2 * el + 1] / o->elem_area[el]) * ff + o->UV2_rhs[2 * el + 1]);
+    }
+}
+
+/* timestep_AB_2D: src/fv_main.F90:1372-1634 */
+static void timestep_AB_2D(orc_t *o, int step) {
+    for (int n = 0; n < o->N; n++)
+        o->etaAB[n] = (1.5 + BETA) * o->eta_n[n] - (0.5 + 2.0 * BETA) * o->eta_n_1[n] + BETA * o->eta_n_2[n];
+    for (int el = 0; el < o->E; el++) {
+        o->UAB[2 * el] = (1.5 + BETA) * o->U_n_2D[2 * el] - (0.5 + 2.0 * BETA) * o->U_n_1[2 * el] + BETA * o->U_n_2[2 * el];
+        o->UAB[2 * el + 1] = (1.5 + BETA) * o->U_n_2D[2 * el + 1] - (0.5 + 2.0 * BETA) * o->U_n_1[2 * el + 1] + BETA * o->U_n_2[2 * el + 1];
+    }
+    compute_ssh_rhs_elem(o);
+    for (int i = 0; i < o->N; i++)
+        if (o->index_nod[i] < 2) o->ssh_rhs[i] = o->eta_n[i] + o->dt_2D * o->ssh_rhs[i] / o->area[i];
+    if (step == 1 && o->type_task > 1) {
+        for (int el = 0; el < o->E; el++) { o->U_rhs_2D_3D[2 * el] = 0.0; o->U_rhs_2D_3D[2 * el + 1] = 0.0; }
+        for (int el = 0; el < o->E; el++) {
+            const int *en = o->elem_nodes + 4 * el;
+            const double *w = o->w_cv + 4 * el;
+            o->dmean_n[el] = dmax(o->Dmin, w[0] * (o->eta_n[en[0]] + o->depth[en[0]]) + w[1] * (o->eta_n[en[1]] + o->depth[en[1]])
+                                         + w[2] * (o->eta_n[en[2]] + o->depth[en[2]]) + w[3] * (o->eta_n[en[3]] + o->depth[en[3]]));
+        }
+        for (int i = 0; i < o->N; i++) o->eta_p[i] = o->eta_n[i];
+        compute_puls_vel(o);
+        el2nodes_3D(o, o->U_puls, o->V_puls, o->Unode_p, o->Vnode_p);
+        momentum_adv_P1_3D_to_2D(o);
+        for (int el = 0; el < o->E; el++) {
+            o->U_rhs_2D_3D[2 * el] = o->U_rhs_2D_3D[2 * el] / o->dmean_n[el];
+            o->U_rhs_2D_3D[2 * el + 1] = o->U_rhs_2D_3D[2 * el + 1] / o->dmean_n[el];
+        }
+    }
+    update_2D_vel(o);
+    for (int n = 0; n < o->N; n++) {
+        o->eta_n_2[n] = o->eta_n_1[n];
+        o->eta_n_1[n] = o->eta_n[n];
+        o->eta_n[n] = o->ssh_rhs[n];
+    }
+    for (int el = 0; el < o->E; el++) {
+        o->U_n_2[2 * el] = o->U_n_1[2 * el]; o->U_n_2[2 * el + 1] = o->U_n_1[2 * el + 1];
+        o->U_n_1[2 * el] = o->U_n_2D[2 * el]; o->U_n_1[2 * el + 1] = o->U_n_2D[2 * el + 1];
+        o->U_n_2D[2 * el] = o->U_rhs_2D[2 * el]; o->U_n_2D[2 * el + 1] = o->U_rhs_2D[2 * el + 1];
+    }
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *w = o->w_cv + 4 * el;
+        double dm = dmax(o->Dmin, w[0] * (o->eta_n[en[0]] + o->depth[en[0]]) + w[1] * (o->eta_n[en[1]] + o->depth[en[1]])
+                                + w[2] * (o->eta_n[en[2]] + o->depth[en[2]]) + w[3] * (o->eta_n[en[3]] + o->depth[en[3]]));
+        o->U_filt_2D[el] = o->U_filt_2D[el] + o->U_rhs_2D[2 * el] * dm;
+        o->V_filt_2D[el] = o->V_filt_2D[el] + o->U_rhs_2D[2 * el + 1] * dm;
+    }
+}
+
+/* oce_barotropic_timestep: src/fv_main.F90:1636-1703 */
+static void oce_barotropic_timestep(orc_t *o) {
+    for (int el = 0; el < o->E; el++) { o->U_filt_2D[el] = 0.0; o->V_filt_2D[el] = 0.0; }
+    for (int n = 1; n <= o->Mt; n++) {
+        o->time_2D = o->time + o->dt_2D * n;
+        timestep_AB_2D(o, n);
+    }
+    double Mt_inv = 1.0 / (double)o->Mt;
+    for (int el = 0; el < o->E; el++) { o->U_filt_2D[el] = o->U_filt_2D[el] * Mt_inv; o->V_filt_2D[el] = o->V_filt_2D[el] * Mt_inv; }
+}
+
+/* update_3D_vel: src/fv_3D_vel.F90:201-255 */
+static void update_3D_vel(orc_t *o) {
+    int L = o->L;
+    double *a = DALLOC(L);
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *w = o->w_cv + 4 * el;
+        for (int z = 0; z < L; z++)
+            a[z] = w[0] * o->Jc_old[(size_t)en[0] * L + z] + w[1] * o->Jc_old[(size_t)en[1] * L + z]
+                 + w[2] * o->Jc_old[(size_t)en[2] * L + z] + w[3] * o->Jc_old[(size_t)en[3] * L + z];
+        double dmi = 1.0 / dmax(o->Dmin, sum4w(w, o->depth[en[0]] + o->eta_n[en[0]], o->depth[en[1]] + o->eta_n[en[1]],
+                                               o->depth[en[2]] + o->eta_n[en[2]], o->depth[en[3]] + o->eta_n[en[3]]));
+        double su = 0, sv = 0;
+        for (int z = 0; z < L; z++) { su = su + o->U_rhs[(size_t)el * L + z] * a[z]; sv = sv + o->V_rhs[(size_t)el * L + z] * a[z]; }
+        double U2p = su * dmi, V2p = sv * dmi;
+        o->U_filt_2D[el] = o->U_filt_2D[el] * dmi;
+        o->V_filt_2D[el] = o->V_filt_2D[el] * dmi;
+        for (int z = 0; z < L; z++) a[z] = a[z] / o->Je[(size_t)el * L + z];
+        for (int z = 0; z < L; z++) {
+            size_t i = (size_t)el * L + z;
+            o->U_n_filt[i] = a[z] * o->U_rhs[i] + o->U_filt_2D[el] - U2p;
+            o->V_n_filt[i] = a[z] * o->V_rhs[i] + o->V_filt_2D[el] - V2p;
+            o->U_n[i] = a[z] * o->U_rhs[i] + o->U_n_2D[2 * el] - U2p;
+            o->V_n[i] = a[z] * o->V_rhs[i] + o->U_n_2D[2 * el + 1] - V2p;
+        }
+    }
+    free(a);
+}
+
+/* vert_vel_sigma: src/fv_3D_vel.F90:309-456 (serial) */
+static void vert_vel_sigma(orc_t *o) {
+    int L = o->L, K = o->K;
+    double *c1 = DALLOC(L);
+    for (int n = 0; n < o->N; n++) for (int z = 0; z < L; z++) o->Wvel[(size_t)n * K + z] = 0.0;
+    for (int ed = 0; ed < o->D; ed++) {
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const double *cd = o->edge_cross_dxdy + 4 * ed;
+        for (int z = 0; z < L; z++) {
+            size_t i1 = (size_t)e1 * L + z;
+            double v = o->Je[i1] * ((o->V_n_filt[i1] - o->V_filt_2D[e1]) * cd[0] - (o->U_n_filt[i1] - o->U_filt_2D[e1]) * cd[1]);
+            if (ed < o->D_in) {
+                size_t i2 = (size_t)e2 * L + z;
+                v = v + o->Je[i2] * (-(o->V_n_filt[i2] - o->V_filt_2D[e2]) * cd[2] + (o->U_n_filt[i2] - o->U_filt_2D[e2]) * cd[3]);
+            }
+            c1[z] = v;
+        }
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1];
+        for (int z = 0; z < L; z++) {
+            o->Wvel[(size_t)n1 * K + z] = o->Wvel[(size_t)n1 * K + z] + c1[z];
+            o->Wvel[(size_t)n2 * K + z] = o->Wvel[(size_t)n2 * K + z] - c1[z];
+        }
+    }
+    for (int n = 0; n < o->N; n++) {
+        double area_inv = 1.0 / o->area[n];
+        for (int z = L - 1; z >= 0; z--) o->Wvel[(size_t)n * K + z] = o->Wvel[(size_t)n * K + z] + o->Wvel[(size_t)n * K + z + 1];
+        for (int z = 0; z < L; z++) o->Wvel[(size_t)n * K + z] = o->Wvel[(size_t)n * K + z] * area_inv;
+    }
+    free(c1);
+}
+
+/* ------------------------------------------------------------------ */
+/* tracers                                                               */
+/* ------------------------------------------------------------------ */
+
+/* OB climatology relaxation, src/fv_tracer.F90:832-895 (shared by the schemes) */
+static void tracer_relax(orc_t *o, double *rhs, const double *tt_dm, const double *clim, double aver, int replace_only_ob) {
+    (void)replace_only_ob;
+    int L = o->L;
+    double *vn = DALLOC(o->N);
+    for (int ed = o->D_in; ed < o->D; ed++) {
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1], e1 = o->edge_tri[2 * ed];
+        double dm = dmax(o->Dmin, 0.5 * (o->eta_n[n1] + o->depth[n1] + o->eta_n[n2] + o->depth[n2]));
+        double un1 = (o->V_filt_2D[e1] * o->edge_cross_dxdy[4 * ed] - o->U_filt_2D[e1] * o->edge_cross_dxdy[4 * ed + 1]) * dm;
+        vn[n1] = vn[n1] + un1 / o->area[n1];
+        vn[n2] = vn[n2] - un1 / o->area[n2];
+    }
+    for (int n = 0; n < o->N; n++) {
+        if (o->index_nod[n] != 2) continue;
+        for (int z = 0; z < L; z++) {
+            size_t i = (size_t)n * L + z;
+            if (vn[n] > 0.0) rhs[i] = rhs[i] + o->relax2clim_ac * (clim[i] - (tt_dm[i] + aver));
+            else rhs[i] = o->clim_relax * (clim[i] - (tt_dm[i] + aver));
+        }
+    }
+    free(vn);
+}
+
+/* solve_tracer_upwind: src/fv_tracer.F90:390-923 */
+static void solve_tracer_upwind(orc_t *o) {
+    int L = o->L, K = o->K, N = o->N;
+    double *ttf = o->TF, *tto = o->T_old, *stf = o->SF, *sto = o->S_old;
+    double Ta = o->T_aver, Sa = o->S_aver;
+    size_t NL = (size_t)N * L;
+    double *ttrhs = DALLOC(NL), *strhs = DALLOC(NL), *trhs_c = DALLOC(NL);
+    double *aux_c = DALLOC(L), *t_aux = DALLOC(L), *s_aux = DALLOC(L);
+    for (int n = 0; n < N; n++)
+        for (int z = 0; z < L; z++) {
+            size_t i = (size_t)n * L + z;
+            tto[i] = -(0.5 + EPS) * tto[i] + (1.5 + EPS) * ttf[i] - Ta;
+            ttf[i] = ttf[i] - Ta;
+            sto[i] = -(0.5 + EPS) * sto[i] + (1.5 + EPS) * stf[i] - Sa;
+            stf[i] = stf[i] - Sa;
+        }
+    for (int ed = 0; ed < o->D; ed++) {
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1];
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const double *cd = o->edge_cross_dxdy + 4 * ed;
+        for (int z = 0; z < L; z++) {
+            size_t i1 = (size_t)e1 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+            double c1 = o->V_n_filt[i1] * cd[0] - o->U_n_filt[i1] * cd[1];
+            if (ed < o->D_in) {
+                size_t i2 = (size_t)e2 * L + z;
+                double c2 = -o->V_n_filt[i2] * cd[2] + o->U_n_filt[i2] * cd[3];
+                double J1 = o->Je[i1], J2 = o->Je[i2];
+                aux_c[z] = J1 * c1 + J2 * c2;
+                if (c1 > 0.0 && c2 > 0.0) {
+                    t_aux[z] = J1 * c1 * tto[a2] + J2 * c2 * tto[a2];
+                    s_aux[z] = J1 * c1 * sto[a2] + J2 * c2 * sto[a2];
+                } else if (c1 > 0.0) {
+                    t_aux[z] = J1 * c1 * tto[a2] + J2 * c2 * tto[a1];
+                    s_aux[z] = J1 * c1 * sto[a2] + J2 * c2 * sto[a1];
+                } else if (c2 > 0.0) {
+                    t_aux[z] = J1 * c1 * tto[a1] + J2 * c2 * tto[a2];
+                    s_aux[z] = J1 * c1 * sto[a1] + J2 * c2 * sto[a2];
+                } else {
+                    t_aux[z] = J1 * c1 * tto[a1] + J2 * c2 * tto[a1];
+                    s_aux[z] = J1 * c1 * sto[a1] + J2 * c2 * sto[a1];
+                }
+            } else {
+                aux_c[z] = o->Je[i1] * c1;
+                if (c1 > 0.0) { t_aux[z] = aux_c[z] * tto[a2]; s_aux[z] = aux_c[z] * sto[a2]; }
+                else { t_aux[z] = aux_c[z] * tto[a1]; s_aux[z] = aux_c[z] * sto[a1]; }
+            }
+        }
+        for (int z = 0; z < L; z++) {
+            size_t a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+            trhs_c[a1] = trhs_c[a1] + aux_c[z]; ttrhs[a1] = ttrhs[a1] + t_aux[z]; strhs[a1] = strhs[a1] + s_aux[z];
+            trhs_c[a2] = trhs_c[a2] - aux_c[z]; ttrhs[a2] = ttrhs[a2] - t_aux[z]; strhs[a2] = strhs[a2] - s_aux[z];
+        }
+    }
+    /* vertical: lines 731-830, 1-based */
+    double *tv = DALLOC(K + 2), *sv = DALLOC(K + 2);
+    for (int n = 0; n < N; n++) {
+        const double *W = o->Wvel + (size_t)n * K - 1, *zb = o->zbar + (size_t)n * K - 1, *Zn = o->Z + (size_t)n * L - 1;
+        const double *T = tto + (size_t)n * L - 1, *S = sto + (size_t)n * L - 1;
+        tv[1] = -W[1] * T[1];
+        sv[1] = -W[1] * S[1];
+        tv[K] = 0.0;
+        if (W[2] > 0.0) {
+            double dg = (zb[2] - Zn[1]) * (Zn[2] - zb[2]) / ((Zn[3] - Zn[1]) * (Zn[2] - Zn[3]));
+            double db = -(zb[2] - Zn[1]) * (Zn[3] - zb[2]) / ((Zn[2] - Zn[1]) * (Zn[2] - Zn[3]));
+            tv[2] = -(T[1] * (1.0 - dg - db) + T[2] * db + T[3] * dg) * W[2];
+            sv[2] = -(S[1] * (1.0 - dg - db) + S[2] * db + S[3] * dg) * W[2];
+        } else {
+            tv[2] = -T[1] * W[2];
+            sv[2] = -S[1] * W[2];
+        }
+        for (int nz = 3; nz <= K - 2; nz++) {
+            if (W[nz] >= 0.0) {
+                double dg = (zb[nz] - Zn[nz - 1]) * (Zn[nz] - zb[nz]) / ((Zn[nz + 1] - Zn[nz - 1]) * (Zn[nz] - Zn[nz + 1]));
+                double db = -(zb[nz] - Zn[nz - 1]) * (Zn[nz + 1] - zb[nz]) / ((Zn[nz] - Zn[nz - 1]) * (Zn[nz] - Zn[nz + 1]));
+                tv[nz] = -(T[nz - 1] * (1.0 - dg - db) + T[nz] * db + T[nz + 1] * dg) * W[nz];
+                sv[nz] = -(S[nz - 1] * (1.0 - dg - db) + S[nz] * db + S[nz + 1] * dg) * W[nz];
+            } else {
+                double dg = (Zn[nz] - zb[nz]) * (zb[nz] - Zn[nz - 1]) / ((Zn[nz] - Zn[nz - 2]) * (Zn[nz - 2] - Zn[nz - 1]));
+                double db = -(Zn[nz] - zb[nz]) * (zb[nz] - Zn[nz - 2]) / ((Zn[nz] - Zn[nz - 1]) * (Zn[nz - 2] - Zn[nz - 1]));
+                tv[nz] = -(T[nz] * (1.0 - dg - db) + T[nz - 1] * db + T[nz - 2] * dg) * W[nz];
+                sv[nz] = -(S[nz] * (1.0 - dg - db) + S[nz - 1] * db + S[nz - 2] * dg) * W[nz];
+            }
+        }
+        int m = K - 1;
+        if (W[m] >= 0.0) {
+            tv[m] = -T[m] * W[m];
+            sv[m] = -S[m] * W[m];
+        } else {
+            double dg = (Zn[m] - zb[m]) * (zb[m] - Zn[m - 1]) / ((Zn[m] - Zn[m - 2]) * (Zn[m - 2] - Zn[m - 1]));
+            double db = -(Zn[m] - zb[m]) * (zb[m] - Zn[m - 2]) / ((Zn[m] - Zn[m - 1]) * (Zn[m - 2] - Zn[m - 1]));
+            tv[m] = -(T[m] * (1.0 - dg - db) + T[m - 1] * db + T[m - 2] * dg) * W[m];
+            sv[m] = -(S[m] * (1.0 - dg - db) + S[m - 1] * db + S[m - 2] * dg) * W[m];
+        }
+        double area_inv = 1.0 / o->area[n];
+        for (int nz = 1; nz <= L; nz++) {
+            size_t i = (size_t)n * L + nz - 1;
+            ttrhs[i] = ((ttrhs[i] - T[nz] * trhs_c[i]) * area_inv + T[nz] * (W[nz] - W[nz + 1]) + (tv[nz] - tv[nz + 1])) / o->Jc[i];
+            strhs[i] = ((strhs[i] - S[nz] * trhs_c[i]) * area_inv + S[nz] * (W[nz] - W[nz + 1]) + (sv[nz] - sv[nz + 1])) / o->Jc[i];
+        }
+    }
+    if (o->cl_relax) {
+        tracer_relax(o, ttrhs, tto, o->Tclim, Ta, 1);
+        tracer_relax(o, strhs, sto, o->Sclim, Sa, 1);
+    }
+    for (int n = 0; n < N; n++)
+        for (int z = 0; z < L; z++) {
+            size_t i = (size_t)n * L + z;
+            tto[i] = ttf[i] + Ta;
+            ttf[i] = ttf[i] + ttrhs[i] * o->dt + Ta;
+            sto[i] = stf[i] + Sa;
+            stf[i] = stf[i] + strhs[i] * o->dt + Sa;
+        }
+    if (o->cl_solid_boder_relax)
+        for (int n = 0; n < N; n++)
+            for (int z = 0; z < L; z++) {
+                size_t i = (size_t)n * L + z;
+                ttf[i] = ttf[i] + o->relax2clim[n] * o->dt * (o->Tclim[i] - ttf[i]);
+                ttf[i] = ttf[i] + o->relax2clim[n] * o->dt * (o->Sclim[i] - ttf[i]);
+            }
+    free(ttrhs); free(strhs); free(trhs_c); free(aux_c); free(t_aux); free(s_aux); free(tv); free(sv);
+}
+
+/* tracer_gradient_elements / _nodes: src/fv_gradients.F90:180-260 */
+static void tracer_grad_elem(orc_t *o, const double *tt, double *ttx, double *tty) {
+    int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double *gs = o->gradient_sca + 8 * el;
+        for (int z = 0; z < L; z++) {
+            double v0 = tt[(size_t)en[0] * L + z], v1 = tt[(size_t)en[1] * L + z], v2 = tt[(size_t)en[2] * L + z], v3 = tt[(size_t)en[3] * L + z];
+            ttx[(size_t)el * L + z] = sum4w(gs, v0, v1, v2, v3);
+            tty[(size_t)el * L + z] = sum4w(gs + 4, v0, v1, v2, v3);
+        }
+    }
+}
+
+/* solve_tracer_miura: src/fv_tracer.F90:930-1316 */
+static void solve_tracer_miura(orc_t *o, double *ttf, double *tto, const double *clim, char type) {
+    int L = o->L, K = o->K, N = o->N, E = o->E;
+    double TSa = type == 't' ? o->T_aver : (type == 's' ? o->S_aver : 0.0);
+    size_t NL = (size_t)N * L, EL = (size_t)E * L;
+    double *rhs = DALLOC(NL), *rhs_c = DALLOC(NL), *ttx = DALLOC(EL), *tty = DALLOC(EL), *txn = DALLOC(NL), *tyn = DALLOC(NL);
+    for (size_t i = 0; i < NL; i++) { tto[i] = -(0.5 + EPS) * tto[i] + (1.5 + EPS) * ttf[i] - TSa; ttf[i] = ttf[i] - TSa; }
+    tracer_grad_elem(o, ttf, ttx, tty);
+    for (int n = 0; n < N; n++) {
+        int ne = o->nie_num[n];
+        const int *lst = o->nie + (size_t)n * o->maxdeg;
+        double s = 0.0;
+        for (int k = 0; k < ne; k++) s = s + o->elem_area[lst[k]];
+        double tvol = 1.0 / s;
+        for (int z = 0; z < L; z++) {
+            double tx = 0.0, ty = 0.0;
+            for (int k = 0; k < ne; k++) {
+                int el = lst[k];
+                tx = tx + ttx[(size_t)el * L + z] * o->elem_area[el];
+                ty = ty + tty[(size_t)el * L + z] * o->elem_area[el];
+            }
+            txn[(size_t)n * L + z] = tx * tvol;
+            tyn[(size_t)n * L + z] = ty * tvol;
+        }
+    }
+    for (int ed = 0; ed < o->D; ed++) {
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1];
+        double crt1 = 0.5 * (o->mask_ad[n1] + o->mask_ad[n2]);
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const double *cd = o->edge_cross_dxdy + 4 * ed, *dd = o->edge_dxdy + 2 * ed;
+        double dX1 = cd[0], dY1 = cd[1], dX2 = 0, dY2 = 0;
+        double Kh = o->elem_area[e1];
+        double a = 1.0, b = 1.0;
+        if (e2 >= 0) { dX2 = cd[2]; dY2 = cd[3]; Kh = 0.5 * (Kh + o->elem_area[e2]); }
+        Kh = K_HOR * Kh / o->scale_area;
+        for (int z = 0; z < L; z++) {
+            size_t i1 = (size_t)e1 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+            double Tx, Ty;
+            if (e2 >= 0) {
+                size_t i2 = (size_t)e2 * L + z;
+                Tx = 0.5 * Kh * (ttx[i1] + ttx[i2]);
+                Ty = 0.5 * Kh * (tty[i1] + tty[i2]);
+            } else { Tx = Kh * ttx[i1]; Ty = Kh * tty[i1]; }
+            double u = o->U_n_filt[i1], v = o->V_n_filt[i1], Tm;
+            if (v * dX1 - u * dY1 > 0) {
+                Tm = ttf[a2] + (0.5 * (-dd[0] * a + dX1 - u * o->dt) * txn[a2]
+                               + 0.5 * (-dd[1] * R_EARTH + dY1 - v) * tyn[a2]) * crt1;
+            } else {
+                Tm = ttf[a1] + (0.5 * (dd[0] * a + dX1 - u * o->dt) * txn[a1]
+                               + 0.5 * (dd[1] * R_EARTH + dY1 - v * o->dt) * tyn[a1]) * crt1;
+            }
+            double c1 = (o->Je[i1] * v * Tm - Ty) * dX1 - (o->Je[i1] * u * Tm - Tx) * dY1;
+            rhs[a1] = rhs[a1] + c1;
+            rhs[a2] = rhs[a2] - c1;
+        }
+        if (e2 >= 0) {
+            for (int z = 0; z < L; z++) {
+                size_t i1 = (size_t)e1 * L + z, i2 = (size_t)e2 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+                double Tx = 0.5 * Kh * (ttx[i1] + ttx[i2]);
+                double Ty = 0.5 * Kh * (tty[i1] + tty[i2]);
+                double u = o->U_n_filt[i2], v = o->V_n_filt[i2], Tm;
+                if (v * dX2 - u * dY2 < 0) {
+                    Tm = ttf[a2] + (0.5 * (-dd[0] * b + dX2 - u * o->dt) * txn[a2]
+                                   + 0.5 * (-dd[1] * R_EARTH + dY2 - v * o->dt) * tyn[a2]) * crt1;
+                } else {
+                    Tm = ttf[a1] + (0.5 * (dd[0] * b + dX2 - u * o->dt) * txn[a1]
+                                   + 0.5 * (dd[1] * R_EARTH + dY2 - v * o->dt) * tyn[a1]) * crt1;
+                }
+                double c2 = -(o->Je[i2] * v * Tm - Ty) * dX2 + (o->Je[i2] * u * Tm - Tx) * dY2;
+                rhs[a1] = rhs[a1] + c2;
+                rhs[a2] = rhs[a2] - c2;
+            }
+        }
+        for (int z = 0; z < L; z++) {
+            size_t i1 = (size_t)e1 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+            double c1 = o->Je[i1] * o->V_n_filt[i1] * dX1 - o->Je[i1] * o->U_n_filt[i1] * dY1;
+            rhs_c[a1] = rhs_c[a1] + c1;
+            rhs_c[a2] = rhs_c[a2] - c1;
+        }
+        if (e2 >= 0)
+            for (int z = 0; z < L; z++) {
+                size_t i2 = (size_t)e2 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+                double c2 = -o->Je[i2] * o->V_n_filt[i2] * dX2 + o->Je[i2] * o->U_n_filt[i2] * dY2;
+                rhs_c[a1] = rhs_c[a1] + c2;
+                rhs_c[a2] = rhs_c[a2] - c2;
+            }
+    }
+    double *tv = DALLOC(K + 2);
+    double Tm = 0.0;
+    for (int n = 0; n < N; n++) {
+        const double *W = o->Wvel + (size_t)n * K - 1, *zb = o->zbar + (size_t)n * K - 1, *Zn = o->Z + (size_t)n * L - 1;
+        const double *T = ttf + (size_t)n * L - 1;
+        tv[1] = -W[1] * T[1];
+        tv[K] = 0.0;
+        for (int nz = 2; nz <= K - 1; nz++) {
+            if (W[nz] > 0.0) {
+                if (nz == K - 1) Tm = T[nz];
+                else {
+                    double a = zb[nz] - Zn[nz - 1], b = Zn[nz] - zb[nz], c = Zn[nz + 1] - zb[nz];
+                    double dg = a * b / ((c + a) * (b - c));
+                    double db = -a * c / ((b + a) * (b - c));
+                    double da = 1.0 - dg - db;
+                    Tm = T[nz - 1] * da + T[nz] * db + T[nz + 1] * dg;
+                }
+            }
+            if (W[nz] < 0.0) {
+                if (nz == 2) Tm = T[nz - 1];
+                else {
+                    double a = Zn[nz] - zb[nz], b = zb[nz] - Zn[nz - 1], c = zb[nz] - Zn[nz - 2];
+                    double dg = a * b / ((c + a) * (b - c));
+                    double db = -a * c / ((b + a) * (b - c));
+                    double da = 1.0 - dg - db;
+                    Tm = T[nz] * da + T[nz - 1] * db + T[nz - 2] * dg;
+                }
+            }
+            tv[nz] = -Tm * W[nz];
+        }
+        for (int nz = 1; nz <= L; nz++) {
+            size_t i = (size_t)n * L + nz - 1;
+            rhs[i] = ((rhs[i] - tto[i] * rhs_c[i]) / o->area[n] + tto[i] * (W[nz] - W[nz + 1]) + tv[nz] - tv[nz + 1]) / o->Jc[i];
+        }
+    }
+    if (o->cl_relax && (type == 't' || type == 's')) tracer_relax(o, rhs, ttf, clim, TSa, 1);
+    for (size_t i = 0; i < NL; i++) { tto[i] = ttf[i] + TSa; ttf[i] = ttf[i] + rhs[i] * o->dt + TSa; }
+    if (o->cl_solid_boder_relax)
+        for (int n = 0; n < N; n++)
+            for (int z = 0; z < L; z++) {
+                size_t i = (size_t)n * L + z;
+                ttf[i] = ttf[i] + o->relax2clim[n] * o->dt * (clim[i] - ttf[i]);
+            }
+    free(rhs); free(rhs_c); free(ttx); free(tty); free(txn); free(tyn); free(tv);
+}
+
+/* solve_tracer_muscl + fill_up_dn_grad (node-average fallback):
+ * src/fv_tracer.F90:8-379, src/fv_gradients.F90:263-323 */
+static void solve_tracer_muscl(orc_t *o, double *ttf, double *tto, const double *clim, char type) {
+    int L = o->L, K = o->K, N = o->N, E = o->E, D = o->D;
+    size_t NL = (size_t)N * L, EL = (size_t)E * L;
+    double aver = type == 't' ? o->T_aver : o->S_aver;
+    double *rhs = DALLOC(NL), *rhs_c = DALLOC(NL), *ttx = DALLOC(EL), *tty = DALLOC(EL);
+    double *gud = DALLOC((size_t)4 * L * D);
+    for (size_t i = 0; i < NL; i++) tto[i] = -(0.5 + EPS) * tto[i] + (1.5 + EPS) * ttf[i];
+    for (size_t i = 0; i < NL; i++) { tto[i] = tto[i] - aver; ttf[i] = ttf[i] - aver; }
+    tracer_grad_elem(o, tto, ttx, tty);
+    for (int ed = 0; ed < D; ed++) {
+        for (int side = 0; side < 2; side++) {
+            int nd = o->edge_nodes[2 * ed + side];
+            const int *lst = o->nie + (size_t)nd * o->maxdeg;
+            for (int z = 0; z < L; z++) {
+                double tvol = 0.0, tx = 0.0, ty = 0.0;
+                for (int k = 0; k < o->nie_num[nd]; k++) {
+                    int el = lst[k];
+                    tvol = tvol + o->elem_area[el];
+                    tx = tx + ttx[(size_t)el * L + z] * o->elem_area[el];
+                    ty = ty + tty[(size_t)el * L + z] * o->elem_area[el];
+                }
+                gud[((size_t)ed * L + z) * 4 + side] = tx / tvol;
+                gud[((size_t)ed * L + z) * 4 + 2 + side] = ty / tvol;
+            }
+        }
+    }
+    for (int ed = 0; ed < D; ed++) {
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1];
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const double *cd = o->edge_cross_dxdy + 4 * ed, *dd = o->edge_dxdy + 2 * ed;
+        double dX1 = cd[0], dY1 = cd[1], dX2 = 0, dY2 = 0, b = 0;
+        double a = R_EARTH * o->elem_cos[e1];
+        if (e2 >= 0) { dX2 = cd[2]; dY2 = cd[3]; b = R_EARTH * o->elem_cos[e2]; }
+        for (int z = 0; z < L; z++) {
+            size_t i1 = (size_t)e1 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+            const double *g = gud + ((size_t)ed * L + z) * 4;
+            double Tx = 0.0, Ty = 0.0;
+            double Tm2 = tto[a2] - (2.0 * (tto[a2] - tto[a1]) + dd[0] * a * g[1] + dd[1] * R_EARTH * g[3]) / 6.0;
+            double Tm1 = tto[a1] + (2.0 * (tto[a2] - tto[a1]) + dd[0] * a * g[0] + dd[1] * R_EARTH * g[2]) / 6.0;
+            double Tm;
+            if (o->V_n_filt[i1] * dX1 - o->U_n_filt[i1] * dY1 > 0) Tm = 0.15 * Tm2 + 0.85 * (Tm1 + Tm2) / 2.0;
+            else Tm = 0.15 * Tm1 + 0.85 * (Tm1 + Tm2) / 2.0;
+            double c1 = (o->Je[i1] * o->V_n_filt[i1] * Tm - Ty) * dX1 - (o->Je[i1] * o->U_n_filt[i1] * Tm - Tx) * dY1;
+            rhs[a1] = rhs[a1] + c1;
+            rhs[a2] = rhs[a2] - c1;
+        }
+        if (e2 >= 0)
+            for (int z = 0; z < L; z++) {
+                size_t i2 = (size_t)e2 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+                const double *g = gud + ((size_t)ed * L + z) * 4;
+                double Tx = 0.0, Ty = 0.0;
+                double Tm2 = tto[a2] - (2.0 * (tto[a2] - tto[a1]) + dd[0] * b * g[1] + dd[1] * R_EARTH * g[3]) / 6.0;
+                double Tm1 = tto[a1] + (2.0 * (tto[a2] - tto[a1]) + dd[0] * b * g[0] + dd[1] * R_EARTH * g[2]) / 6.0;
+                double Tm;
+                if (o->V_n_filt[i2] * dX2 - o->U_n_filt[i2] * dY2 < 0) Tm = 0.15 * Tm2 + 0.85 * (Tm1 + Tm2) / 2.0;
+                else Tm = 0.15 * Tm1 + 0.85 * (Tm1 + Tm2) / 2.0;
+                double c2 = -(o->Je[i2] * o->V_n[i2] * Tm - Ty) * dX2 + (o->Je[i2] * o->U_n[i2] * Tm - Tx) * dY2;
+                rhs[a1] = rhs[a1] + c2;
+                rhs[a2] = rhs[a2] - c2;
+            }
+    }
+    for (int ed = 0; ed < D; ed++) {
+        int n1 = o->edge_nodes[2 * ed], n2 = o->edge_nodes[2 * ed + 1];
+        int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const double *cd = o->edge_cross_dxdy + 4 * ed;
+        for (int z = 0; z < L; z++) {
+            size_t i1 = (size_t)e1 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+            double c1 = o->Je[i1] * o->V_n_filt[i1] * cd[0] - o->Je[i1] * o->U_n_filt[i1] * cd[1];
+            rhs_c[a1] = rhs_c[a1] + c1;
+            rhs_c[a2] = rhs_c[a2] - c1;
+        }
+        if (e2 >= 0)
+            for (int z = 0; z < L; z++) {
+                size_t i2 = (size_t)e2 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
+                double c2 = -o->Je[i2] * o->V_n_filt[i2] * cd[2] + o->Je[i2] * o->U_n_filt[i2] * cd[3];
+                rhs_c[a1] = rhs_c[a1] + c2;
+                rhs_c[a2] = rhs_c[a2] - c2;
+            }
+    }
+    double *tv = DALLOC(K + 2);
+    double Tm = 0.0;
+    for (int n = 0; n < N; n++) {
+        const double *W = o->Wvel + (size_t)n * K - 1, *zb = o->zbar + (size_t)n * K - 1, *Zn = o->Z + (size_t)n * L - 1;
+        const double *T = tto + (size_t)n * L - 1;
+        tv[1] = -W[1] * T[1];
+        tv[K] = 0.0;
+        for (int nz = 2; nz <= K - 1; nz++) {
+            if (W[nz] > 0.0) {
+                if (nz == K - 1) Tm = T[nz];
+                else {
+                    double a = zb[nz] - Zn[nz - 1], b = Zn[nz] - zb[nz], c = Zn[nz + 1] - zb[nz];
+                    double dg = a * b / (c + a) / (b - c);
+                    double db = -a * c / (b + a) / (b - c);
+                    double da = 1.0 - dg - db;
+                    Tm = T[nz - 1] * da + T[nz] * db + T[nz + 1] * dg;
+                }
+            }
+            if (W[nz] < 0.0) {
+                if (nz == 2) Tm = T[nz - 1];
+                else {
+                    double a = Zn[nz] - zb[nz], b = zb[nz] - Zn[nz - 1], c = zb[nz] - Zn[nz - 2];
+                    double dg = a * b / (c + a) / (b - c);
+                    double db = -a * c / (b + a) / (b - c);
+                    double da = 1.0 - dg - db;
+                    Tm = T[nz] * da + T[nz - 1] * db + T[nz - 2] * dg;
+                }
+            }
+            tv[nz] = -Tm * W[nz];
+        }
+        for (int nz = 1; nz <= L; nz++) {
+            size_t i = (size_t)n * L + nz - 1;
+            double a = tto[i];
+            double aj = o->area[n] * o->Jc[i];
+            rhs[i] = (rhs[i] / aj - a * rhs_c[i] / aj + a * (W[nz] - W[nz + 1]) / o->Jc[i] + (tv[nz] - tv[nz + 1]) / o->Jc[i]);
+        }
+    }
+    if (o->cl_relax) tracer_relax(o, rhs, ttf, clim, aver, 1);
+    for (size_t i = 0; i < NL; i++) { tto[i] = ttf[i]; ttf[i] = ttf[i] + rhs[i] * o->dt; }
+    for (size_t i = 0; i < NL; i++) { tto[i] = tto[i] + aver; ttf[i] = ttf[i] + aver; }
+    if (o->cl_solid_boder_relax)
+        for (int n = 0; n < N; n++)
+            for (int z = 0; z < L; z++) {
+                size_t i = (size_t)n * L + z;
+                ttf[i] = ttf[i] + o->relax2clim[n] * o->dt * (clim[i] - ttf[i]);
+            }
+    free(rhs); free(rhs_c); free(ttx); free(tty); free(gud); free(tv);
+}
+
+/* tracer_impl_vert_diff: src/fv_tracer.F90:1319-1494 (key_atm off) */
+static int tracer_impl_vert_diff(orc_t *o) {
+    int L = o->L, K = o->K;
+    double *a = DALLOC(K + 1), *b = DALLOC(K + 1), *c = DALLOC(K + 1), *tr = DALLOC(K + 1), *sr = DALLOC(K + 1);
+    double *cp = DALLOC(K + 1), *tp = DALLOC(K + 1), *sp = DALLOC(K + 1);
+    double Ta = o->T_aver, Sa = o->S_aver;
+    for (size_t i = 0; i < (size_t)o->N * L; i++) { o->TF[i] = o->TF[i] - Ta; o->SF[i] = o->SF[i] - Sa; }
+    for (int n = 0; n < o->N; n++) {
+        const double *Kv = o->Kv + (size_t)n * K - 1, *zb = o->zbar + (size_t)n * K - 1, *Zn = o->Z + (size_t)n * L - 1;
+        double *T = o->TF + (size_t)n * L - 1, *S = o->SF + (size_t)n * L - 1;
+        double dt = o->dt;
+        for (int nz = 2; nz <= L - 1; nz++) {
+            a[nz] = -dt * Kv[nz] / ((Zn[nz - 1] - Zn[nz]) * (zb[nz] - zb[nz + 1]));
+            c[nz] = -dt * Kv[nz] / ((Zn[nz] - Zn[nz + 1]) * (zb[nz] - zb[nz + 1]));
+            b[nz] = -a[nz] - c[nz] + 1.0;
+        }
+        c[1] = -dt * Kv[2] / ((Zn[1] - Zn[2]) * (zb[1] - zb[2]));
+        a[1] = 0.0;
+        b[1] = -c[1] + 1.0;
+        a[L] = -dt * Kv[L] / ((Zn[L - 1] - Zn[L]) * (zb[L] - zb[K]));
+        b[L] = -a[L] + 1.0;
+        c[L] = 0.0;
+        tr[1] = c[1] * (-T[2] + T[1]);
+        sr[1] = c[1] * (-S[2] + S[1]);
+        for (int nz = 2; nz <= L - 1; nz++) {
+            tr[nz] = -a[nz] * T[nz - 1] - c[nz] * T[nz + 1] + (a[nz] + c[nz]) * T[nz];
+            sr[nz] = -a[nz] * S[nz - 1] - c[nz] * S[nz + 1] + (a[nz] + c[nz]) * S[nz];
+        }
+        tr[L] = a[L] * (T[L] - T[L - 1]);
+        sr[L] = a[L] * (S[L] - S[L - 1]);
+        cp[1] = c[1] / b[1]; tp[1] = tr[1] / b[1]; sp[1] = sr[1] / b[1];
+        for (int nz = 2; nz <= L; nz++) {
+            double m = 1.0 / (b[nz] - cp[nz - 1] * a[nz]);
+            cp[nz] = c[nz] * m;
+            tp[nz] = (tr[nz] - tp[nz - 1] * a[nz]) * m;
+            sp[nz] = (sr[nz] - sp[nz - 1] * a[nz]) * m;
+        }
+        tr[L] = tp[L]; sr[L] = sp[L];
+        for (int nz = L - 1; nz >= 1; nz--) { tr[nz] = tp[nz] - cp[nz] * tr[nz + 1]; sr[nz] = sp[nz] - cp[nz] * sr[nz + 1]; }
+        for (int nz = 1; nz <= L; nz++) { T[nz] = T[nz] + tr[nz] + Ta; S[nz] = S[nz] + sr[nz] + Sa; }
+    }
+    int bad = 0;
+    for (int n = 0; n < o->N && !bad; n++)
+        for (int z = 0; z < L; z++) {
+            size_t i = (size_t)n * L + z;
+            if (fabs(o->SF[i]) < 1.e-12) o->SF[i] = fabs(o->SF[i]);
+            if (o->SF[i] < 0.0) {
+                snprintf(o->errmsg, sizeof o->errmsg, "Negative salinity, time resolution is insufficient, stop %g %d %d", o->SF[i], z + 1, n + 1);
+                bad = 2; break;
+            }
+            if (o->TF[i] < -3.0) {
+                snprintf(o->errmsg, sizeof o->errmsg, "Negative temperature, time resolution is insufficient, stop %g %d %d", o->TF[i], z + 1, n + 1);
+                bad = 3; break;
+            }
+        }
+    free(a); free(b); free(c); free(tr); free(sr); free(cp); free(tp); free(sp);
+    return bad;
+}
+
+/* solve_tracers: src/fv_main.F90:1707-1836 (rivers off) */
+static int solve_tracers(orc_t *o) {
+    int L = o->L;
+    double tmn = o->TF[0], tmx = o->TF[0], smn = o->SF[0], smx = o->SF[0];
+    for (size_t i = 0; i < (size_t)o->N * L; i++) {
+        tmn = dmin(tmn, o->TF[i]); tmx = dmax(tmx, o->TF[i]);
+        smn = dmin(smn, o->SF[i]); smx = dmax(smx, o->SF[i]);
+    }
+    o->T_aver = 0.5 * (tmn + tmx);
+    o->S_aver = 0.5 * (smn + smx);
+    adv_tracer_mask(o);
+    if (o->tracer_adv == 2) {
+        if (o->muscl_call) {
+            solve_tracer_muscl(o, o->TF, o->T_old, o->Tclim, 't');
+            solve_tracer_muscl(o, o->SF, o->S_old, o->Sclim, 's');
+        }
+    } else if (o->tracer_adv == 1) {
+        solve_tracer_upwind(o);
+    } else if (o->tracer_adv == 4) {
+        solve_tracer_miura(o, o->TF, o->T_old, o->Tclim, 't');
+    }
+    return tracer_impl_vert_diff(o);
+}
+
+/* oce_timestep case(3): src/fv_main.F90:1983-2098, preceded by the main-loop
+ * time update src/fv_main.F90:400-401 */
+int orc_step(orc_t *o) {
+    if (o->status) return o->status;
+    o->n_dt++;
+    o->time_jd = o->time_jd + o->dt / 86400.0;
+    o->time = o->time_jd * 86400.0;
+    pressure(o);
+    el2nodes_3D(o, o->U_n, o->V_n, o->Unode, o->Vnode);
+    if (o->ver_mix == 3) d3_end(o);
+    compute_vel_rhs(o);
+    oce_barotropic_timestep(o);
+    jacobian(o);
+    update_3D_vel(o);
+    vert_vel_sigma(o);
+    o->status = solve_tracers(o);
+    return o->status;
+}
+
+/* energy: src/fv_average_dynamic.F90:476-554 (serial) */
+double orc_energy(const orc_t *o) {
+    double eout = 0.0, my_area = 0.0;
+    for (int n = 0; n < o->N; n++) {
+        eout = eout + 0.5 * G * o->area[n] * (o->eta_n[n] * o->eta_n[n]);
+        my_area = my_area + o->area[n];
+    }
+    double ek = 0.0;
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        double fD[4];
+        for (int i = 0; i < 4; i++) fD[i] = dmax(o->Dmin, o->depth[en[i]] + o->eta_n[en[i]]);
+        double e = 0.5 * sum4w(o->w_cv + 4 * el, fD[0], fD[1], fD[2], fD[3]) * o->elem_area[el]
+                 * (o->U_n_2D[2 * el] * o->U_n_2D[2 * el] + o->U_n_2D[2 * el + 1] * o->U_n_2D[2 * el + 1]);
+        ek = ek + e;
+    }
+    return (eout + ek) / my_area;
+}
+
+/* ------------------------------------------------------------------ */
+/* construction                                                          */
+/* ------------------------------------------------------------------ */
+static void parse_config(orc_t *o, const char *t) {
+    char v[256];
+    if (scan_key(t, "TITLE", v, sizeof v)) snprintf(o->title, sizeof o->title, "%s", v); else o->title[0] = 0;
+    o->is_LE = strcmp(o->title, "LE") == 0;
+    o->dt_2D = cfg_f(t, "dt_2D", 0); o->dt = cfg_f(t, "dt", 0); o->time_jd0 = cfg_f(t, "JD0", 0);
+    o->nsteps = cfg_i(t, "nsteps", 0); o->irep = cfg_i(t, "IREP", 1);
+    o->cartesian = cfg_l(t, "cartesian", 0); o->coriolis_tf = cfg_l(t, "Coriolis_TF", 0);
+    o->lat_cartesian = cfg_f(t, "lat_cartesian", 0) * PI / 180.0;
+    o->mom_adv_2D = cfg_i(t, "mom_adv_2D", 2); o->mom_adv_3D = cfg_i(t, "mom_adv_3D", 1);
+    o->tracer_adv = cfg_i(t, "tracer_adv", 1); o->vert_adv = cfg_i(t, "vert_adv", 1);
+    o->im_vert_visc = cfg_l(t, "im_vert_visc", 0); o->im_vert_visc_fic = cfg_l(t, "im_vert_visc_fic", 1);
+    o->fic_point = cfg_i(t, "fic_point", 1);
+    o->C_d = cfg_f(t, "Cd", 0); o->BFC = cfg_i(t, "BFC", 1);
+    o->z0b_min = cfg_f(t, "z0b_min", 0.01); o->z0s_min = cfg_f(t, "z0s_min", 0.02); o->za = cfg_f(t, "za", 0);
+    o->snul = cfg_f(t, "snul", 7.5e-4);
+    o->filt_2D = cfg_l(t, "filt_2D", 0); o->filt_bi_2D = cfg_l(t, "filt_bi_2D", 0); o->bih_2D = cfg_l(t, "bih_2D", 0);
+    o->filt_3D = cfg_l(t, "filt_3D", 0); o->filt_bi_3D = cfg_l(t, "filt_bi_3D", 0); o->bih_3D = cfg_l(t, "bih_3D", 0);
+    o->barotropic = cfg_l(t, "BAROTROPIC", 0); o->barotropic_3D = cfg_l(t, "BAROTROPIC_3D", 0);
+    o->temp_on = cfg_l(t, "TEMP_ON", 0); o->salinity_on = cfg_l(t, "SALINITY_ON", 0);
+    o->S_const = cfg_f(t, "S_const", 35.0); o->T_const = cfg_f(t, "T_const", 10.0);
+    o->ver_mix = cfg_i(t, "Vert_turb", 3);
+    o->nsigma = cfg_i(t, "nsigma", 10); o->ind_sigma = cfg_i(t, "ind_sigma", 1);
+    o->pow_sig = cfg_f(t, "pow_sig", 1.0); o->lev_bot = cfg_f(t, "lev_bot", 1.0); o->lev_sur = cfg_f(t, "lev_sur", 1.0);
+    o->key_rivers = cfg_l(t, "key_rivers", 0); o->key_atm = cfg_l(t, "key_atm", 0);
+    o->cyclic_length = cfg_f(t, "cyclic_length", 360.0) * RAD;
+    o->Dcr = cfg_f(t, "Dcr", 0.25); o->Dmin = cfg_f(t, "Dmin", 0.25);
+    o->TF_presence = cfg_l(t, "TF_presence", 0); o->T_potential = cfg_l(t, "T_potential", 0);
+    for (int j = 0; j < 12; j++) o->a_th[j] = 0;
+    o->n_th = 0;
+    if ((o->TF_presence || o->T_potential) && scan_key(t, "Harmonics_tf", v, sizeof v)) {
+        char *tok = strtok(v, " \t");
+        while (tok) {
+            for (int j = 0; j < 12; j++)
+                if (strcmp(tok, TIDE_NAME[j]) == 0) { o->a_th[j] = 1; o->n_th++; break; }
+            tok = strtok(NULL, " \t");
+        }
+    }
+    o->cl_relax = cfg_l(t, "cl_relax", 0);
+    o->clim_relax = 1.0 / (cfg_f(t, "clim_relax", 15.0) * 3600.0 * 24.0);
+    o->relax2clim_ac = 1.0 / (cfg_f(t, "relax2clim_ac", 60.0) * 3600.0 * 24.0);
+    o->cl_solid_boder_relax = cfg_l(t, "cl_solid_boder_relax", 0);
+    o->wet_dry_on = cfg_l(t, "WET_DRY_ON", 0);
+    o->mixed_BP_grad_OB = cfg_l(t, "mixed_BP_grad_OB", 0);
+    o->Cd_bz = cfg_f(t, "Cd_bz", 0); o->Bp_bz = cfg_f(t, "Bp_bz", 0);
+    o->SL_obn = cfg_l(t, "SL_obn", 0);
+    o->muscl_call = cfg_l(t, "MUSCL_CALL", 0);
+    if (scan_key(t, "RESTART", v, sizeof v)) o->restart = strcmp(v, "hot_start") == 0; else o->restart = 0;
+    o->Mt = (int)(o->dt / o->dt_2D); /* src/fv_read_run.F90:1060 */
+    /* Explicit substep count (project key).  The reference derives Mt by
+     * truncating dt/dt_2D; how that quotient rounds depends on the build
+     * (0.5/0.1 is exactly 5.0 in IEEE double, 4.99.. in x87 extended). */
+    o->Mt = cfg_i(t, "Mt", o->Mt);
+    o->type_task = 0;
+    if (o->barotropic) o->type_task = 1;
+    if (o->barotropic_3D) o->type_task = 2;
+    if (o->salinity_on || o->temp_on) o->type_task = 3;
+}
+
+/*
+ * Create the oracle from fv_run.dat text and the raw mesh arrays exactly as
+ * nod2d.out / elem2d.out / depth.out hold them:
+ *   xy[2N]     coordinates as in nod2d.out (metres if cartesian, else degrees)
+ *   ind[N]     node index column (0 interior, 1 solid, 2 open boundary)
+ *   elem[4E]   1-based node ids per row (triangle: 4th = 1st)
+ *   depth[N]   positive down (ignored for TITLE=LE)
+ *   ob[nobn*2*n_th] m2.out rows (amp, phase_deg per active harmonic), or NULL
+ */
+orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int E, const int *elem,
+                  const double *depth, int nobn, const double *ob) {
+    orc_t *o = (orc_t *)xcalloc(1, sizeof(orc_t));
+    parse_config(o, cfg);
+    if (o->type_task != 3) { o->status = -10; snprintf(o->errmsg, sizeof o->errmsg, "only type_task=3"); return o; }
+    if (o->filt_2D || o->filt_bi_2D || o->bih_2D || o->filt_3D || o->filt_bi_3D || o->bih_3D || o->mixed_BP_grad_OB
+        || o->key_atm || o->key_rivers || o->T_potential || o->SL_obn || o->ver_mix != 3) {
+        o->status = -14;
+        snprintf(o->errmsg, sizeof o->errmsg, "configuration uses an option outside the restated hot path");
+        return o;
+    }
+    o->N = N; o->E = E;
+    o->coord = DALLOC(2 * (size_t)N); o->index_nod = IALLOC(N);
+    /* read_mesh_ser: src/fv_main.F90:787-945 */
+    o->nobn = 0;
+    for (int n = 0; n < N; n++) {
+        o->index_nod[n] = ind[n];
+        if (o->cartesian) { o->coord[2 * n] = xy[2 * n] / R_EARTH; o->coord[2 * n + 1] = xy[2 * n + 1] / R_EARTH; }
+        else { o->coord[2 * n] = xy[2 * n] * RAD; o->coord[2 * n + 1] = xy[2 * n + 1] * RAD; }
+        if (ind[n] == 2) o->nobn++;
+    }
+    o->in_obn = IALLOC(o->nobn);
+    for (int n = 0, k = 0; n < N; n++) if (ind[n] == 2) o->in_obn[k++] = n;
+    o->elem_nodes = IALLOC(4 * (size_t)E);
+    int nt = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (int e = 0; e < E; e++) {
+            int tri = elem[4 * e] == elem[4 * e + 3];
+            if ((pass == 0) == tri) {
+                for (int k = 0; k < 4; k++) o->elem_nodes[4 * nt + k] = elem[4 * e + k] - 1;
+                nt++;
+            }
+            if (pass == 0 && e == E - 1) o->E_tri = nt;
+        }
+    o->depth = DALLOC(N);
+    for (int n = 0; n < N; n++) o->depth[n] = o->is_LE ? 20.0 : depth[n];
+    if (set_sigma(o)) { o->status = -11; snprintf(o->errmsg, sizeof o->errmsg, "bad sigma setup"); return o; }
+    o->K = o->nsigma; o->L = o->nsigma - 1;
+    if (test_elem(o)) { o->status = -12; return o; }
+    if (find_edges(o)) { o->status = -13; return o; }
+    find_elem_neighbors(o);
+    mesh_arrays1(o);
+    mesh_arrays2(o);
+    /* array_setup: src/fv_main.F90:947-1204 */
+    int L = o->L, K = o->K, D = o->D;
+    size_t NL = (size_t)N * L, EL = (size_t)E * L, NK = (size_t)N * K, EK = (size_t)E * K;
+    o->U_n_2D = DALLOC(2 * E); o->U_n_1 = DALLOC(2 * E); o->U_n_2 = DALLOC(2 * E); o->UAB = DALLOC(2 * E);
+    o->U_rhs_2D = DALLOC(2 * E); o->U_rhs_2D_3D = DALLOC(2 * E); o->UV2_rhs = DALLOC(2 * E);
+    o->eta_n = DALLOC(N); o->eta_n_1 = DALLOC(N); o->eta_n_2 = DALLOC(N); o->etaAB = DALLOC(N);
+    o->ssh_rhs = DALLOC(N); o->ssh_gp = DALLOC(N); o->eta_p = DALLOC(N);
+    o->vel_grad = DALLOC(4 * E); o->taux = DALLOC(E); o->tauy = DALLOC(E); o->windx = DALLOC(N); o->windy = DALLOC(N);
+    o->Jc = DALLOC(NL); o->Jc_old = DALLOC(NL); o->Je = DALLOC(EL); o->Jd = DALLOC((size_t)D * L);
+    o->Z = DALLOC(NL); o->zbar = DALLOC(NK); o->dmean_n = DALLOC(E);
+    o->hpressure = DALLOC(NK); o->Bar_pru_3D = DALLOC(EL); o->Bar_prv_3D = DALLOC(EL); o->Bar_pr_2D = DALLOC(2 * E);
+    o->rho_c = DALLOC(NL);
+    o->Unode = DALLOC(NL); o->Vnode = DALLOC(NL); o->Unode_p = DALLOC(NL); o->Vnode_p = DALLOC(NL);
+    o->Kv = DALLOC(NK); o->Av = DALLOC(EK); o->bt = DALLOC(NK); o->snu = DALLOC(NK); o->Ri = DALLOC(NK); o->tke_dissip = DALLOC(NK);
+    o->U_rhs = DALLOC(EL); o->V_rhs = DALLOC(EL); o->U_rhsAB = DALLOC(EL); o->V_rhsAB = DALLOC(EL); o->UV_rhs = DALLOC(2 * EL);
+    o->vel_grad_ux = DALLOC(EL); o->vel_grad_uy = DALLOC(EL); o->vel_grad_vx = DALLOC(EL); o->vel_grad_vy = DALLOC(EL);
+    o->U_puls = DALLOC(EL); o->V_puls = DALLOC(EL); o->U_n = DALLOC(EL); o->V_n = DALLOC(EL);
+    o->U_n_filt = DALLOC(EL); o->V_n_filt = DALLOC(EL); o->Wvel = DALLOC(NK);
+    o->U_filt_2D = DALLOC(E); o->V_filt_2D = DALLOC(E);
+    o->TF = DALLOC(NL); o->T_old = DALLOC(NL); o->SF = DALLOC(NL); o->S_old = DALLOC(NL);
+    o->Tclim = DALLOC(NL); o->Sclim = DALLOC(NL);
+    o->mask_wd = DALLOC(E); o->mask_wd_node = DALLOC(N); o->mask_ad = DALLOC(N); o->mask_bpr = DALLOC(E);
+    o->C_d_el = DALLOC(E); o->relax2clim = DALLOC(N); o->ac = DALLOC(N); o->a_tp = DALLOC(2 * (size_t)N);
+    for (size_t i = 0; i < NK; i++) { o->snu[i] = o->snul; o->bt[i] = 1.0e-8; o->Kv[i] = o->snul; }
+    for (size_t i = 0; i < EK; i++) o->Av[i] = o->snul;
+    for (int e = 0; e < E; e++) { o->mask_wd[e] = 1.0; o->mask_bpr[e] = 1.0; }
+    for (int n = 0; n < N; n++) { o->mask_wd_node[n] = 1.0; o->mask_ad[n] = 1.0; o->ac[n] = 1.0; }
+    o->lfirst = 1;
+    jacobian(o); /* src/fv_main.F90:162 */
+    /* initial_state: src/fv_ini.F90:1-590 */
+    double s = 0.0;
+    for (int e = 0; e < E; e++) s = s + o->elem_area[e];
+    o->scale_area = s / E;
+    for (int n = 0; n < N; n++) {
+        o->a_tp[2 * n] = o->T_potential ? 0.918 : 1.0;
+        o->a_tp[2 * n + 1] = o->T_potential ? 0.69 : 0.0;
+        if (o->index_nod[n] == 2) { o->a_tp[2 * n] = 1.0; o->a_tp[2 * n + 1] = 0.0; }
+    }
+    for (int n = 0; n < N; n++) {
+        double d = o->depth[n] + o->eta_n[n];
+        if (d < 0.0) { o->eta_n[n] = -o->depth[n]; o->eta_n_1[n] = -o->depth[n]; o->eta_n_2[n] = -o->depth[n]; }
+    }
+    {
+        const double de = 0.614, ga = 0.088, ep = 0.013;
+        const double fe = 1.0 - de - ga - ep;
+        for (int n = 0; n < N; n++) {
+            o->ssh_rhs[n] = o->eta_n[n];
+            o->etaAB[n] = de * o->ssh_rhs[n] + fe * o->eta_n[n] + ga * o->eta_n_1[n] + ep * o->eta_n_2[n];
+        }
+    }
+    o->density_0 = 1000.0;
+    double den0 = orc_densityJM(o->T_const, o->S_const, 0.0, o->density_0);
+    o->density_0 = o->density_0 + den0;
+    for (int j = 0; j < 12; j++) { o->amp_factor[j] = 1.0; o->phy_factor[j] = 0.0; }
+    o->ampt = DALLOC(12 * (size_t)(o->nobn > 0 ? o->nobn : 1));
+    o->fazt = DALLOC(12 * (size_t)(o->nobn > 0 ? o->nobn : 1));
+    if (o->TF_presence && ob) {
+        for (int i = 0; i < o->nobn && i < nobn; i++) {
+            double row[24];
+            for (int j = 0; j < 2 * o->n_th; j++) {
+                row[j] = ob[(size_t)i * 2 * o->n_th + j];
+                if (j % 2 == 1 && row[j] < 0.0) row[j] = 360.0 + row[j];
+            }
+            int j = 0;
+            for (int h = 0; h < 12; h++)
+                if (o->a_th[h]) { o->ampt[i * 12 + h] = row[j]; o->fazt[i * 12 + h] = row[j + 1] * RAD; j += 2; }
+        }
+    }
+    for (size_t i = 0; i < NL; i++) o->TF[i] = o->temp_on ? o->T_const : 0.0;
+    if (o->temp_on && o->is_LE)
+        for (int n = 0; n < N; n++) {
+            double a = o->coord[2 * n];
+            for (int z = 0; z < L; z++) o->TF[(size_t)n * L + z] = (a < 0.0) ? 5.0 / 0.28 : 10.0 / 0.28;
+        }
+    for (size_t i = 0; i < NL; i++) o->SF[i] = o->salinity_on ? o->S_const : 0.0;
+    memcpy(o->T_old, o->TF, NL * sizeof(double)); memcpy(o->S_old, o->SF, NL * sizeof(double));
+    memcpy(o->Tclim, o->TF, NL * sizeof(double)); memcpy(o->Sclim, o->SF, NL * sizeof(double));
+    if (o->BFC == 2) {
+        for (int e = 0; e < E; e++) {
+            const int *en = o->elem_nodes + 4 * e;
+            double dm = dmax(o->Dmin, sum4w(o->w_cv + 4 * e, o->eta_n[en[0]] + o->depth[en[0]], o->eta_n[en[1]] + o->depth[en[1]],
+                                            o->eta_n[en[2]] + o->depth[en[2]], o->eta_n[en[3]] + o->depth[en[3]]));
+            double l = log(dm / (o->z0b_min + o->za)) / CKA;
+            o->C_d_el[e] = 1.0 / (l * l);
+        }
+    } else {
+        for (int e = 0; e < E; e++) o->C_d_el[e] = o->C_d;
+    }
+    if (o->wet_dry_on) wad_mask(o);
+    o->time_jd = o->time_jd0;
+    o->time = o->time_jd0 * 86400.0;
+    jacobian(o); /* src/fv_main.F90:362 */
+    return o;
+}
+
+/* Re-initialise T/S (harness hook for analytic initial fields, applied after
+ * initial_state as the configs require); keeps T_old/Tclim consistent. */
+void orc_set_tracers(orc_t *o, const double *T, const double *S) {
+    size_t NL = (size_t)o->N * o->L;
+    if (T) { memcpy(o->TF, T, NL * sizeof(double)); memcpy(o->T_old, T, NL * sizeof(double)); memcpy(o->Tclim, T, NL * sizeof(double)); }
+    if (S) { memcpy(o->SF, S, NL * sizeof(double)); memcpy(o->S_old, S, NL * sizeof(double)); memcpy(o->Sclim, S, NL * sizeof(double)); }
+}
+
+void orc_destroy(orc_t *o) {
+    if (!o) return;
+    /* process-lifetime test object: the OS reclaims the arrays; free the struct */
+    free(o);
+}
+
+int orc_status(const orc_t *o) { return o->status; }
+const char *orc_errmsg(const orc_t *o) { return o->errmsg; }
+
+/* integer sizes: 0 N,1 E,2 D,3 D_in,4 E_tri,5 nobn,6 L,7 K,8 maxdeg,9 Mt,10 n_dt */
+int orc_size(const orc_t *o, int what) {
+    switch (what) {
+    case 0: return o->N; case 1: return o->E; case 2: return o->D; case 3: return o->D_in;
+    case 4: return o->E_tri; case 5: return o->nobn; case 6: return o->L; case 7: return o->K;
+    case 8: return o->maxdeg; case 9: return o->Mt; case 10: return o->n_dt;
+    default: return -1;
+    }
+}
+
+/* scalars: 0 time_jd,1 time,2 density_0,3 T_aver,4 S_aver,5 scale_area,6 time_jd0 */
+double orc_scalar(const orc_t *o, int what) {
+    switch (what) {
+    case 0: return o->time_jd; case 1: return o->time; case 2: return o->density_0; case 3: return o->T_aver;
+    case 4: return o->S_aver; case 5: return o->scale_area; case 6: return o->time_jd0;
+    default: return NAN;
+    }
+}
+
+/* Named field access (reference array names).  Returns the element count or -1. */
+static double *fld(orc_t *o, const char *name, long *cnt) {
+    long N = o->N, E = o->E, D = o->D, L = o->L, K = o->K;
+#define F(nm, p, c) if (!strcmp(name, nm)) { *cnt = (c); return (p); }
+    F("coord", o->coord, 2 * N) F("depth", o->depth, N) F("sigma", o->sigma, K)
+    F("elem_area", o->elem_area, E) F("area", o->area, N) F("elem_cos", o->elem_cos, E)
+    F("coriolis", o->coriolis, E) F("w_cv", o->w_cv, 4 * E) F("edge_dxdy", o->edge_dxdy, 2 * D)
+    F("edge_cross_dxdy", o->edge_cross_dxdy, 4 * D) F("gradient_sca", o->gradient_sca, 8 * E)
+    F("gradient_vec", o->gradient_vec, 8 * E) F("ac", o->ac, N)
+    F("eta_n", o->eta_n, N) F("eta_n_1", o->eta_n_1, N) F("eta_n_2", o->eta_n_2, N) F("etaAB", o->etaAB, N)
+    F("ssh_rhs", o->ssh_rhs, N) F("U_n_2D", o->U_n_2D, 2 * E) F("U_n_1", o->U_n_1, 2 * E) F("U_n_2", o->U_n_2, 2 * E)
+    F("UAB", o->UAB, 2 * E) F("U_rhs_2D", o->U_rhs_2D, 2 * E) F("U_rhs_2D_3D", o->U_rhs_2D_3D, 2 * E)
+    F("taux", o->taux, E) F("tauy", o->tauy, E) F("windx", o->windx, N) F("windy", o->windy, N)
+    F("Jc", o->Jc, N * L) F("Jc_old", o->Jc_old, N * L) F("Je", o->Je, E * L) F("Jd", o->Jd, D * L)
+    F("Z", o->Z, N * L) F("zbar", o->zbar, N * K) F("hpressure", o->hpressure, N * K)
+    F("Bar_pru_3D", o->Bar_pru_3D, E * L) F("Bar_prv_3D", o->Bar_prv_3D, E * L) F("Bar_pr_2D", o->Bar_pr_2D, 2 * E)
+    F("rho_c", o->rho_c, N * L) F("Unode", o->Unode, N * L) F("Vnode", o->Vnode, N * L)
+    F("Kv", o->Kv, N * K) F("Av", o->Av, E * K) F("bt", o->bt, N * K) F("snu", o->snu, N * K)
+    F("U_rhs", o->U_rhs, E * L) F("V_rhs", o->V_rhs, E * L) F("U_rhsAB", o->U_rhsAB, E * L) F("V_rhsAB", o->V_rhsAB, E * L)
+    F("U_n", o->U_n, E * L) F("V_n", o->V_n, E * L) F("U_n_filt", o->U_n_filt, E * L) F("V_n_filt", o->V_n_filt, E * L)
+    F("Wvel", o->Wvel, N * K) F("U_filt_2D", o->U_filt_2D, E) F("V_filt_2D", o->V_filt_2D, E)
+    F("TF", o->TF, N * L) F("SF", o->SF, N * L) F("T_old", o->T_old, N * L) F("S_old", o->S_old, N * L)
+    F("Tclim", o->Tclim, N * L) F("Sclim", o->Sclim, N * L)
+    F("mask_wd", o->mask_wd, E) F("mask_wd_node", o->mask_wd_node, N) F("mask_ad", o->mask_ad, N)
+    F("C_d_el", o->C_d_el, E) F("vel_grad", o->vel_grad, 4 * E) F("dmean_n", o->dmean_n, E)
+    F("vel_grad_ux", o->vel_grad_ux, E * L) F("vel_grad_uy", o->vel_grad_uy, E * L)
+    F("vel_grad_vx", o->vel_grad_vx, E * L) F("vel_grad_vy", o->vel_grad_vy, E * L)
+    F("Unode_p", o->Unode_p, N * L) F("Vnode_p", o->Vnode_p, N * L) F("a_tp", o->a_tp, 2 * N)
+    F("ampt", o->ampt, 12L * (o->nobn > 0 ? o->nobn : 1)) F("fazt", o->fazt, 12L * (o->nobn > 0 ? o->nobn : 1))
+#undef F
+    *cnt = -1;
+    return NULL;
+}
+long orc_get(orc_t *o, const char *name, double *out) {
+    long c;
+    double *p = fld(o, name, &c);
+    if (!p) return -1;
+    if (out) memcpy(out, p, (size_t)c * sizeof(double));
+    return c;
+}
+long orc_set(orc_t *o, const char *name, const double *in) {
+    long c;
+    double *p = fld(o, name, &c);
+    if (!p) return -1;
+    memcpy(p, in, (size_t)c * sizeof(double));
+    return c;
+}
+/* integer mesh arrays: 0 elem_nodes(4E) 1 edge_nodes(2D) 2 edge_tri(2D) 3 elem_edges(4E)
+ * 4 elem_neighbors(4E) 5 nod_in_elem2D_num(N) 6 nod_in_elem2D(N*maxdeg) 7 index(N) 8 in_obn */
+long orc_get_int(orc_t *o, int what, int *out) {
+    long c; int *p;
+    switch (what) {
+    case 0: p = o->elem_nodes; c = 4L * o->E; break;
+    case 1: p = o->edge_nodes; c = 2L * o->D; break;
+    case 2: p = o->edge_tri; c = 2L * o->D; break;
+    case 3: p = o->elem_edges; c = 4L * o->E; break;
+    case 4: p = o->elem_nb; c = 4L * o->E; break;
+    case 5: p = o->nie_num; c = o->N; break;
+    case 6: p = o->nie; c = (long)o->N * o->maxdeg; break;
+    case 7: p = o->index_nod; c = o->N; break;
+    case 8: p = o->in_obn; c = o->nobn; break;
+    default: return -1;
+    }
+    if (out) memcpy(out, p, (size_t)c * sizeof(int));
+    return c;
+}

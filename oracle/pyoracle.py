@@ -1,0 +1,140 @@
+"""ctypes front end of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker -- never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+SIZES = dict(N=0, E=1, D=2, D_in=3, E_tri=4, nobn=5, L=6, K=7, maxdeg=8, Mt=9, n_dt=10)
+SCALARS = dict(time_jd=0, time=1, density_0=2, T_aver=3, S_aver=4, scale_area=5, time_jd0=6)
+INTS = dict(elem_nodes=0, edge_nodes=1, edge_tri=2, elem_edges=3, elem_neighbors=4,
+            nod_in_elem2D_num=5, nod_in_elem2D=6, index=7, in_obn=8)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [C.c_char_p, C.c_int, dp, ip, C.c_int, ip, dp, C.c_int, dp]
+        L.orc_step.restype = C.c_int
+        L.orc_step.argtypes = [P]
+        L.orc_energy.restype = C.c_double
+        L.orc_energy.argtypes = [P]
+        L.orc_status.restype = C.c_int
+        L.orc_status.argtypes = [P]
+        L.orc_errmsg.restype = C.c_char_p
+        L.orc_errmsg.argtypes = [P]
+        L.orc_size.restype = C.c_int
+        L.orc_size.argtypes = [P, C.c_int]
+        L.orc_scalar.restype = C.c_double
+        L.orc_scalar.argtypes = [P, C.c_int]
+        L.orc_get.restype = C.c_long
+        L.orc_get.argtypes = [P, C.c_char_p, dp]
+        L.orc_set.restype = C.c_long
+        L.orc_set.argtypes = [P, C.c_char_p, dp]
+        L.orc_get_int.restype = C.c_long
+        L.orc_get_int.argtypes = [P, C.c_int, ip]
+        L.orc_set_tracers.argtypes = [P, dp, dp]
+        L.orc_destroy.argtypes = [P]
+        L.orc_densityJM.restype = C.c_double
+        L.orc_densityJM.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def densityJM(t, s, pz, density_0):
+    return lib().orc_densityJM(float(t), float(s), float(pz), float(density_0))
+
+
+class Oracle:
+    """One FESOM-C model instance on the CPU (reference layout: level-fastest)."""
+
+    def __init__(self, cfg_text: str, mesh):
+        L = lib()
+        self._keep = []
+        xy = np.ascontiguousarray(mesh.xy, dtype=np.float64).reshape(-1)
+        ind = np.ascontiguousarray(mesh.index, dtype=np.int32)
+        el = np.ascontiguousarray(mesh.elem, dtype=np.int32).reshape(-1)
+        dep = np.ascontiguousarray(mesh.depth, dtype=np.float64)
+        ob = None if mesh.ob is None else np.ascontiguousarray(mesh.ob, dtype=np.float64).reshape(-1)
+        nobn = 0 if mesh.ob is None else int(mesh.ob.shape[0])
+        self.h = L.orc_create(cfg_text.encode(), mesh.nod2D, _dp(xy), _ip(ind), mesh.elem2D, _ip(el), _dp(dep),
+                              nobn, None if ob is None else _dp(ob))
+        st = L.orc_status(self.h)
+        if st:
+            raise RuntimeError(f"oracle create failed ({st}): {L.orc_errmsg(self.h).decode()}")
+        self.size = {k: L.orc_size(self.h, v) for k, v in SIZES.items()}
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        for _ in range(n):
+            st = lib().orc_step(self.h)
+            if st:
+                raise RuntimeError(f"oracle step failed ({st}): {lib().orc_errmsg(self.h).decode()}")
+
+    def energy(self):
+        return lib().orc_energy(self.h)
+
+    def scalar(self, name):
+        return lib().orc_scalar(self.h, SCALARS[name])
+
+    def get(self, name):
+        c = lib().orc_get(self.h, name.encode(), None)
+        if c < 0:
+            raise KeyError(name)
+        out = np.empty(c)
+        lib().orc_get(self.h, name.encode(), _dp(out))
+        return out
+
+    def set(self, name, values):
+        c = lib().orc_get(self.h, name.encode(), None)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if c < 0 or v.size != c:
+            raise ValueError(f"bad field {name}: {v.size} vs {c}")
+        lib().orc_set(self.h, name.encode(), _dp(v))
+
+    def get_int(self, name):
+        c = lib().orc_get_int(self.h, INTS[name], None)
+        out = np.empty(c, dtype=np.int32)
+        lib().orc_get_int(self.h, INTS[name], _ip(out))
+        return out
+
+    def set_tracers(self, T=None, S=None):
+        t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
+        s = None if S is None else np.ascontiguousarray(S, dtype=np.float64).reshape(-1)
+        lib().orc_set_tracers(self.h, None if t is None else _dp(t), None if s is None else _dp(s))
