@@ -1,0 +1,592 @@
+// C-ABI of libfesomc_gpu: context set-up, state transfer and the per-call-site
+// entry points of oce_timestep (src/fv_main.F90:1877-2106).  See fesomc_gpu.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fcg_internal.h"
+
+namespace fcgk {
+__global__ void k_pressure(DevMesh, DevState, double, double);
+__global__ void k_bpg(DevMesh, DevState);
+__global__ void k_d3end(DevMesh, DevState, double, double, double, double, double, double);
+__global__ void k_velgrad3d(DevMesh, DevState);
+__global__ void k_velrhs(DevMesh, DevState, double, double, double, double, int, int, int, int);
+__global__ void k_bt_elem1(DevMesh, DevState, double);
+__global__ void k_bt_node(DevMesh, DevState, double, const double *);
+__global__ void k_bt_velgrad(DevMesh, DevState);
+__global__ void k_bt_puls(DevMesh, DevState, double);
+__global__ void k_el2nodes(DevMesh, const double *, const double *, double *, double *);
+__global__ void k_bt_p1adv(DevMesh, DevState);
+__global__ void k_bt_elem2(DevMesh, DevState, double, double, double, double, double, int, int, int);
+__global__ void k_jac_node(DevMesh, DevState, double);
+__global__ void k_je(DevMesh, DevState);
+__global__ void k_update3d(DevMesh, DevState, double, double);
+__global__ void k_vertvel(DevMesh, DevState);
+__global__ void k_minmax_partial(const double *, const double *, size_t, double *);
+__global__ void k_minmax_final(const double *, int, double *);
+__global__ void k_tr_ab(DevState, size_t);
+__global__ void k_tr_upwind(DevMesh, DevState, double, double, int, double, double);
+__global__ void k_diag_partial(DevMesh, DevState, double, double *);
+__global__ void k_diag_final(const double *, int, double *);
+__global__ void k_mask_count(const double *, int, int *);
+}  // namespace fcgk
+
+static const double kPiHost = 3.14159265358979;
+static const double kTidePeriod[12] = {12.4200, 12.00, 12.6584, 11.9673, 23.9344, 25.8194,
+                                       24.0659, 26.8684, 327.85, 661.31, 4383.05, 6.2103};
+
+int fcg_fail(fcg_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int fcg_fail_hip(fcg_ctx *ctx, hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    return fcg_fail(ctx, FCG_E_HIP, buf);
+}
+
+namespace {
+
+constexpr int kBlk = 256;
+inline unsigned nblk(size_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
+
+template <typename T>
+int dalloc(fcg_ctx *ctx, T **p, size_t n) {
+    void *q = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    FCG_CHECK(hipMalloc(&q, bytes));
+    FCG_CHECK(hipMemset(q, 0, bytes));
+    ctx->allocs.push_back(q);
+    *p = (T *)q;
+    return FCG_OK;
+}
+
+template <typename T>
+int upload(fcg_ctx *ctx, T **p, const std::vector<T> &h) {
+    int rc = dalloc(ctx, p, h.size());
+    if (rc) return rc;
+    if (!h.empty()) FCG_CHECK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FCG_OK;
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int rc__ = (x);        \
+        if (rc__) return rc__; \
+    } while (0)
+
+// 3-D field descriptor for Fortran-layout transfer
+struct Field {
+    double *dev;
+    long levels;   // levels per column (L or K), or components for 2-D vectors
+    long ents;     // entities
+    bool interleaved;  // device stores (comp, ent) interleaved like Fortran (2,E)
+};
+
+bool find_field(fcg_ctx *c, const char *name, Field &f) {
+    const long N = c->m.N, E = c->m.E, L = c->m.L, K = c->m.K;
+    DevState &s = c->s;
+    struct Ent { const char *n; double *p; long lev; long ents; bool il; };
+    const Ent tab[] = {
+        {"eta_n", s.eta_n, 1, N, false},     {"eta_n_1", s.eta_n_1, 1, N, false}, {"eta_n_2", s.eta_n_2, 1, N, false},
+        {"etaAB", s.etaAB, 1, N, false},     {"ssh_rhs", s.ssh_rhs, 1, N, false}, {"windx", s.windx, 1, N, false},
+        {"windy", s.windy, 1, N, false},     {"U_n_2D", s.U2, 2, E, true},        {"U_n_1", s.U2_1, 2, E, true},
+        {"U_n_2", s.U2_2, 2, E, true},       {"UAB", s.UAB, 2, E, true},          {"U_rhs_2D_3D", s.U_rhs_2D_3D, 2, E, true},
+        {"Bar_pr_2D", s.Bar_pr_2D, 2, E, true}, {"vel_grad", s.vel_grad, 4, E, true},
+        {"U_filt_2D", s.U_filt, 1, E, false}, {"V_filt_2D", s.V_filt, 1, E, false}, {"mask_wd", s.mask_wd, 1, E, false},
+        {"taux", s.taux, 1, E, false},       {"tauy", s.tauy, 1, E, false},       {"C_d_el", s.C_d_el, 1, E, false},
+        {"dmean_n", s.dmean_n, 1, E, false},
+        {"U_n", s.U_n, L, E, false},         {"V_n", s.V_n, L, E, false},         {"U_rhs", s.U_rhs, L, E, false},
+        {"V_rhs", s.V_rhs, L, E, false},     {"U_rhsAB", s.U_rhsAB, L, E, false}, {"V_rhsAB", s.V_rhsAB, L, E, false},
+        {"U_n_filt", s.U_nf, L, E, false},   {"V_n_filt", s.V_nf, L, E, false},   {"Bar_pru_3D", s.Bpu, L, E, false},
+        {"Bar_prv_3D", s.Bpv, L, E, false},  {"Je", s.Je, L, E, false},
+        {"vel_grad_ux", s.gux, L, E, false}, {"vel_grad_uy", s.guy, L, E, false},
+        {"vel_grad_vx", s.gvx, L, E, false}, {"vel_grad_vy", s.gvy, L, E, false},
+        {"TF", s.TF, L, N, false},           {"SF", s.SF, L, N, false},           {"T_old", s.T_old, L, N, false},
+        {"S_old", s.S_old, L, N, false},     {"Tclim", s.Tclim, L, N, false},     {"Sclim", s.Sclim, L, N, false},
+        {"Unode_p", s.Unode_p, L, N, false}, {"Vnode_p", s.Vnode_p, L, N, false},
+        {"hpressure", s.hp, K, N, false},    {"Kv", s.Kv, K, N, false},           {"snu", s.snu, K, N, false},
+        {"bt", s.bt, K, N, false},           {"Wvel", s.W, K, N, false},
+    };
+    for (const Ent &t : tab)
+        if (!std::strcmp(t.n, name)) {
+            f = Field{t.p, t.lev, t.ents, t.il};
+            return true;
+        }
+    return false;
+}
+
+// Tidal elevation at the open boundary for each substep of the coming
+// barotropic loop: compute_ssh_rhs_elem, src/fv_average_dynamic.F90:190-216,
+// evaluated on the host with the same libm cos as a CPU build.
+void fill_tides(fcg_ctx *c) {
+    const int nb = c->m.nobn, Mt = c->p.Mt;
+    if (!nb || !c->p.tf_presence) return;
+    for (int sub = 1; sub <= Mt; ++sub) {
+        double time_2D = c->time + c->p.dt_2D * sub;
+        for (int i = 0; i < nb; ++i) {
+            double v = 0.0;
+            for (int j = 0; j < 12; ++j)
+                v = v + c->amp_factor[j] * c->ampt[(size_t)j * nb + i]
+                        * std::cos((time_2D - c->p.time_jd0 * 86400.0) * 2.0 * kPiHost / (kTidePeriod[j] * 3600.0)
+                                   - (c->fazt[(size_t)j * nb + i] + c->phy_factor[j]));
+            if (!c->p.restart)
+                if (c->time_jd - c->p.time_jd0 < 10.0) v = v * (c->time_jd - c->p.time_jd0) / 10.0;
+            c->tide_pinned[(size_t)(sub - 1) * nb + i] = v;
+        }
+    }
+}
+
+int enqueue_substeps(fcg_ctx *ctx) {
+    const DevMesh &m = ctx->m;
+    const DevState &s = ctx->s;
+    const fcg_params &p = ctx->p;
+    hipStream_t st = ctx->stream;
+    for (int sub = 1; sub <= p.Mt; ++sub) {
+        const double *tide = (p.tf_presence && m.nobn) ? s.tide + (size_t)(sub - 1) * m.nobn : nullptr;
+        hipLaunchKernelGGL(fcgk::k_bt_elem1, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
+        hipLaunchKernelGGL(fcgk::k_bt_node, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide);
+        hipLaunchKernelGGL(fcgk::k_bt_velgrad, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+        if (sub == 1) {
+            hipLaunchKernelGGL(fcgk::k_bt_puls, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
+            hipLaunchKernelGGL(fcgk::k_el2nodes, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, (const double *)s.U_puls,
+                               (const double *)s.V_puls, s.Unode_p, s.Vnode_p);
+            hipLaunchKernelGGL(fcgk::k_bt_p1adv, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+        }
+        hipLaunchKernelGGL(fcgk::k_bt_elem2, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
+                           ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0);
+    }
+    return FCG_OK;
+}
+
+}  // namespace
+
+extern "C" const char *fcg_version(void) { return "fesomc_gpu 0.1 gfx950 fp64+graph"; }
+
+extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx **out) {
+    if (!mesh || !par || !out) return FCG_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FCG_E_NOGPU;
+    auto *ctx = new fcg_ctx();
+    ctx->p = *par;
+    const fcg_params &p = ctx->p;
+    if (mesh->nsigma < 4 || mesh->nsigma != p.nsigma || p.Mt < 1 || p.dt <= 0 || p.dt_2D <= 0) {
+        fcg_fail(ctx, FCG_E_ARG, "inconsistent nsigma/Mt/dt (nsigma >= 4 required)");
+        *out = ctx;
+        return FCG_E_ARG;
+    }
+    if (!(p.tracer_adv == 1 || (p.tracer_adv == 2 && !p.muscl_call)) || p.precision != FCG_FP64 ||
+        p.closure != FCG_CLOSURE_D3END) {
+        fcg_fail(ctx, FCG_E_ARG, "option not available in this build (tracer_adv/precision/closure)");
+        *out = ctx;
+        return FCG_E_ARG;
+    }
+    FCG_CHECK(hipGetDevice(&ctx->device));
+    FCG_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    DevMesh &m = ctx->m;
+    const int N = mesh->nod2D, E = mesh->elem2D, D = mesh->edge2D, K = mesh->nsigma, L = K - 1;
+    m.N = N; m.E = E; m.D = D; m.Din = mesh->edge2D_in; m.K = K; m.L = L; m.nobn = mesh->nobn;
+    ctx->density_0 = mesh->density_0;
+    ctx->scale_area = mesh->scale_area;
+    // ---- element constants
+    std::vector<int4> en(E), nb(E), eled(E), ied(E);
+    std::vector<double4> wcv(E), gsx(E), gsy(E), gvx(E), gvy(E);
+    std::vector<double> accB(E), acc_el(E);
+    auto z = [](int v) { return v > 0 ? v - 1 : -1; };
+    for (int e = 0; e < E; ++e) {
+        const int *q = mesh->elem2D_nodes + 4 * e;
+        en[e] = make_int4(q[0] - 1, q[1] - 1, q[2] - 1, q[3] - 1);
+        const int *b = mesh->elem_neighbors + 4 * e;
+        nb[e] = make_int4(z(b[0]), z(b[1]), z(b[2]), z(b[3]));
+        const int *d = mesh->elem_edges + 4 * e;
+        eled[e] = make_int4(d[0] - 1, d[1] - 1, d[2] - 1, d[3] - 1);
+        const double *w = mesh->w_cv + 4 * e;
+        wcv[e] = make_double4(w[0], w[1], w[2], w[3]);
+        const double *g = mesh->gradient_sca + 8 * e;
+        gsx[e] = make_double4(g[0], g[1], g[2], g[3]);
+        gsy[e] = make_double4(g[4], g[5], g[6], g[7]);
+        const double *v = mesh->gradient_vec + 8 * e;
+        gvx[e] = make_double4(v[0], v[1], v[2], v[3]);
+        gvy[e] = make_double4(v[4], v[5], v[6], v[7]);
+        double acc = w[0] * mesh->ac[q[0] - 1];
+        acc = acc + w[1] * mesh->ac[q[1] - 1];
+        acc = acc + w[2] * mesh->ac[q[2] - 1];
+        acc = acc + w[3] * mesh->ac[q[3] - 1];
+        acc_el[e] = acc;
+        accB[e] = std::pow(acc, p.Bp_bz) * 1.0;  // mask_bpr = 1 (Mask_Bar_pr = .false., src/fv_var.F90:57)
+        ied[e] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+    }
+    // ---- edge constants + gathers
+    std::vector<int2> ednodes(D), edtri(D);
+    std::vector<double4> ecross(D);
+    std::vector<double2> edxdy(D);
+    std::vector<std::vector<int>> n2e(N);
+    std::vector<int> icnt(E, 0);
+    for (int d = 0; d < D; ++d) {
+        int a = mesh->edge_nodes[2 * d] - 1, b = mesh->edge_nodes[2 * d + 1] - 1;
+        int t1 = z(mesh->edge_tri[2 * d]), t2 = z(mesh->edge_tri[2 * d + 1]);
+        ednodes[d] = make_int2(a, b);
+        edtri[d] = make_int2(t1, t2);
+        const double *c = mesh->edge_cross_dxdy + 4 * d;
+        ecross[d] = make_double4(c[0], c[1], c[2], c[3]);
+        edxdy[d] = make_double2(mesh->edge_dxdy[2 * d], mesh->edge_dxdy[2 * d + 1]);
+        n2e[a].push_back(d);
+        n2e[b].push_back(~d);
+        if (d < m.Din) {
+            int *s1 = &ied[t1].x, *s2 = &ied[t2].x;
+            if (icnt[t1] >= 4 || icnt[t2] >= 4) {
+                fcg_fail(ctx, FCG_E_MESH, "element with more than 4 internal edges");
+                *out = ctx;
+                return FCG_E_MESH;
+            }
+            s1[icnt[t1]++] = ~d;  // element is edge_tri(1): receives -flux
+            s2[icnt[t2]++] = d;   // element is edge_tri(2): receives +flux
+        }
+    }
+    std::vector<int> n2e_ptr(N + 1, 0), n2e_idx;
+    for (int n = 0; n < N; ++n) {
+        n2e_ptr[n + 1] = n2e_ptr[n] + (int)n2e[n].size();
+        n2e_idx.insert(n2e_idx.end(), n2e[n].begin(), n2e[n].end());
+    }
+    std::vector<std::vector<int>> n2el(N);
+    for (int e = 0; e < E; ++e) {
+        const int *q = mesh->elem2D_nodes + 4 * e;
+        int nq = (q[0] == q[3]) ? 3 : 4;
+        for (int j = 0; j < nq; ++j) n2el[q[j] - 1].push_back(e);
+    }
+    std::vector<int> n2el_ptr(N + 1, 0), n2el_idx;
+    for (int n = 0; n < N; ++n) {
+        n2el_ptr[n + 1] = n2el_ptr[n] + (int)n2el[n].size();
+        n2el_idx.insert(n2el_idx.end(), n2el[n].begin(), n2el[n].end());
+    }
+    // ---- node constants
+    std::vector<double> depth(mesh->depth, mesh->depth + N), area(mesh->area, mesh->area + N), ac(mesh->ac, mesh->ac + N);
+    std::vector<double> atp1(mesh->a_tp, mesh->a_tp + N), atp2(mesh->a_tp + N, mesh->a_tp + 2 * N);
+    std::vector<int> index(mesh->index_nod2D, mesh->index_nod2D + N), obidx(N, -1);
+    for (int i = 0; i < mesh->nobn; ++i) {
+        obidx[mesh->in_obn[i] - 1] = i;
+        ctx->in_obn0.push_back(mesh->in_obn[i] - 1);
+    }
+    std::vector<double> sigma(mesh->sigma, mesh->sigma + K), dsig(L), oms(K);
+    for (int k = 0; k < L; ++k) dsig[k] = sigma[k] - sigma[k + 1];
+    for (int k = 0; k < K; ++k) oms[k] = 1.0 - sigma[k];
+    if (mesh->nobn > 0) {
+        ctx->ampt.assign(mesh->ampt, mesh->ampt + 12 * (size_t)mesh->nobn);
+        ctx->fazt.assign(mesh->fazt, mesh->fazt + 12 * (size_t)mesh->nobn);
+    }
+    ctx->amp_factor.assign(12, 1.0);
+    ctx->phy_factor.assign(12, 0.0);
+    if (mesh->amp_factor) ctx->amp_factor.assign(mesh->amp_factor, mesh->amp_factor + 12);
+    if (mesh->phy_factor) ctx->phy_factor.assign(mesh->phy_factor, mesh->phy_factor + 12);
+    int rc = 0;
+#define UP(dst, src) if ((rc = upload(ctx, &(dst), src))) { *out = ctx; return rc; }
+    UP(m.en, en) UP(m.elnb, nb) UP(m.eledges, eled) UP(m.el_ied, ied) UP(m.wcv, wcv) UP(m.gsx, gsx) UP(m.gsy, gsy)
+    UP(m.gvx, gvx) UP(m.gvy, gvy) UP(m.accB, accB) UP(m.acc_el, acc_el)
+    std::vector<double> v_ea(mesh->elem_area, mesh->elem_area + E), v_ec(mesh->elem_cos, mesh->elem_cos + E),
+        v_co(mesh->coriolis, mesh->coriolis + E);
+    UP(m.elem_area, v_ea) UP(m.elem_cos, v_ec) UP(m.coriolis, v_co)
+    UP(m.ednodes, ednodes) UP(m.edtri, edtri) UP(m.ecross, ecross) UP(m.edxdy, edxdy)
+    UP(m.depth, depth) UP(m.area, area) UP(m.ac, ac) UP(m.atp1, atp1) UP(m.atp2, atp2) UP(m.index, index)
+    UP(m.obidx, obidx) UP(m.n2e_ptr, n2e_ptr) UP(m.n2e, n2e_idx) UP(m.n2el_ptr, n2el_ptr) UP(m.n2el, n2el_idx)
+    UP(m.sigma, sigma) UP(m.dsig, dsig) UP(m.oms, oms)
+    // ---- state
+    DevState &s = ctx->s;
+    const size_t NL = (size_t)N * L, EL = (size_t)E * L, NK = (size_t)N * K, EK = (size_t)E * K;
+#define DA(ptr, n) if ((rc = dalloc(ctx, &(ptr), (n)))) { *out = ctx; return rc; }
+    DA(s.eta_n, N) DA(s.eta_n_1, N) DA(s.eta_n_2, N) DA(s.etaAB, N) DA(s.eta_g, N) DA(s.ssh_rhs, N)
+    DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
+    DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
+    DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
+    DA(s.a, N) DA(s.a_old, N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    DA(s.U_n, EL) DA(s.V_n, EL) DA(s.U_rhs, EL) DA(s.V_rhs, EL) DA(s.U_rhsAB, EL) DA(s.V_rhsAB, EL)
+    DA(s.U_nf, EL) DA(s.V_nf, EL) DA(s.gux, EL) DA(s.guy, EL) DA(s.gvx, EL) DA(s.gvy, EL)
+    DA(s.Bpu, EL) DA(s.Bpv, EL) DA(s.Je, EL) DA(s.U_puls, EL) DA(s.V_puls, EL)
+    DA(s.vxx, EK) DA(s.vyu, EK) DA(s.vyv, EK)
+    DA(s.TF, NL) DA(s.SF, NL) DA(s.T_old, NL) DA(s.S_old, NL) DA(s.Tclim, NL) DA(s.Sclim, NL)
+    DA(s.abT, NL) DA(s.abS, NL) DA(s.Unode_p, NL) DA(s.Vnode_p, NL) DA(s.tcp, NL) DA(s.ttp, NL) DA(s.tsp, NL)
+    DA(s.hp, NK) DA(s.Kv, NK) DA(s.snu, NK) DA(s.bt, NK) DA(s.W, NK)
+    DA(s.mx_p, NK) DA(s.mx_q, NK) DA(s.mx_c2, NK) DA(s.mx_xx, NK) DA(s.mx_yy, NK) DA(s.mx_bA, NK) DA(s.mx_bB, NK)
+    ctx->red_blocks = 512;
+    DA(s.red, 16 * (size_t)ctx->red_blocks) DA(s.scal, 32) DA(s.err, 4)
+    FCG_CHECK(hipHostMalloc((void **)&ctx->tide_pinned, sizeof(double) * std::max(1, p.Mt * mesh->nobn)));
+    // ---- cold start: array_setup + jacobian + initial_state (src/fv_main.F90:157-168, 320-362)
+    std::vector<double> eta(N, 0.0), ea(N), a1(N), a2(N);
+    for (int n = 0; n < N; ++n) {
+        a1[n] = std::max(depth[n] + 0.0, p.Dmin);   // first jacobian, eta = 0 (fv_main.F90:162)
+        double d = depth[n] + eta[n];
+        if (d < 0.0) eta[n] = -depth[n];           // dry start, fv_ini.F90:127-136
+    }
+    {
+        const double de = 0.614, ga = 0.088, ep = 0.013;
+        const double fe = 1.0 - de - ga - ep;
+        for (int n = 0; n < N; ++n) {
+            ea[n] = de * eta[n] + fe * eta[n] + ga * eta[n] + ep * eta[n];
+            a2[n] = std::max(depth[n] + eta[n], p.Dmin);  // second jacobian (fv_main.F90:362)
+        }
+    }
+    std::vector<double> cst(NK, p.snul), bt0(NK, 1.0e-8), ones(E, 1.0), cd(E, p.C_d);
+    FCG_CHECK(hipMemcpy(s.eta_n, eta.data(), N * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.eta_n_1, eta.data(), N * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.eta_n_2, eta.data(), N * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.etaAB, ea.data(), N * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.a_old, a1.data(), N * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.a, a2.data(), N * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.snu, cst.data(), NK * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.Kv, cst.data(), NK * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.bt, bt0.data(), NK * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.mask_wd, ones.data(), E * 8, hipMemcpyHostToDevice));
+    FCG_CHECK(hipMemcpy(s.C_d_el, cd.data(), E * 8, hipMemcpyHostToDevice));
+    if (p.wet_dry_on) {  // wad_mask at set-up (fv_main.F90:320-324)
+        std::vector<double> mk(E);
+        for (int e = 0; e < E; ++e) {
+            const int4 q = en[e];
+            double mn = std::min(std::min(std::min(depth[q.x], depth[q.y]), depth[q.z]), depth[q.w]);
+            double mx = std::max(std::max(std::max(ea[q.x], ea[q.y]), ea[q.z]), ea[q.w]);
+            mk[e] = (mn + mx > p.Dmin) ? 1.0 : 0.0;
+        }
+        FCG_CHECK(hipMemcpy(s.mask_wd, mk.data(), E * 8, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(fcgk::k_je, dim3(nblk(E)), dim3(kBlk), 0, ctx->stream, m, s);
+    FCG_CHECK(hipGetLastError());
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->time_jd = p.time_jd0;
+    ctx->time = p.time_jd0 * 86400.0;
+    ctx->n_dt = 0;
+    ctx->lfirst = 1;
+    *out = ctx;
+    return FCG_OK;
+#undef UP
+#undef DA
+}
+
+extern "C" void fcg_destroy(fcg_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->bt_graph) (void)hipGraphExecDestroy(ctx->bt_graph);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    if (ctx->tide_pinned) (void)hipHostFree(ctx->tide_pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" int fcg_last_error(const fcg_ctx *ctx, char *buf, int len) {
+    if (!buf || len <= 0) return FCG_E_ARG;
+    const char *t = ctx ? ctx->err.c_str() : "no context";
+    std::snprintf(buf, (size_t)len, "%s", t);
+    return FCG_OK;
+}
+
+extern "C" int fcg_sync(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    return FCG_OK;
+}
+
+static int transfer(fcg_ctx *ctx, const char *name, double *host, long count, bool to_dev) {
+    if (!ctx || !name || !host) return FCG_E_ARG;
+    Field f;
+    if (!find_field(ctx, name, f)) return fcg_fail(ctx, FCG_E_ARG, std::string("unknown field ") + name);
+    if (count != f.levels * f.ents)
+        return fcg_fail(ctx, FCG_E_ARG, std::string("size mismatch for ") + name);
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    if (f.interleaved || f.levels == 1) {
+        FCG_CHECK(hipMemcpy(to_dev ? (void *)f.dev : (void *)host, to_dev ? (void *)host : (void *)f.dev,
+                            sizeof(double) * count, to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+        return FCG_OK;
+    }
+    // Fortran (levels, ents) column-major  <->  device planes [level][ent]
+    std::vector<double> tmp((size_t)count);
+    if (to_dev) {
+        for (long e = 0; e < f.ents; ++e)
+            for (long k = 0; k < f.levels; ++k) tmp[(size_t)k * f.ents + e] = host[(size_t)e * f.levels + k];
+        FCG_CHECK(hipMemcpy(f.dev, tmp.data(), sizeof(double) * count, hipMemcpyHostToDevice));
+    } else {
+        FCG_CHECK(hipMemcpy(tmp.data(), f.dev, sizeof(double) * count, hipMemcpyDeviceToHost));
+        for (long e = 0; e < f.ents; ++e)
+            for (long k = 0; k < f.levels; ++k) host[(size_t)e * f.levels + k] = tmp[(size_t)k * f.ents + e];
+    }
+    return FCG_OK;
+}
+
+extern "C" int fcg_set_state(fcg_ctx *ctx, const char *name, const double *host, long count) {
+    return transfer(ctx, name, const_cast<double *>(host), count, true);
+}
+extern "C" int fcg_get_state(fcg_ctx *ctx, const char *name, double *host, long count) {
+    return transfer(ctx, name, host, count, false);
+}
+
+extern "C" int fcg_set_clock(fcg_ctx *ctx, double time_jd, int lfirst, int n_dt) {
+    if (!ctx) return FCG_E_ARG;
+    ctx->time_jd = time_jd;
+    ctx->time = time_jd * 86400.0;
+    ctx->lfirst = lfirst;
+    ctx->n_dt = n_dt;
+    return FCG_OK;
+}
+
+extern "C" int fcg_advance_clock(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    ctx->time_jd = ctx->time_jd + ctx->p.dt / 86400.0;  // src/fv_main.F90:400-401
+    ctx->time = ctx->time_jd * 86400.0;
+    ctx->n_dt++;
+    return FCG_OK;
+}
+
+extern "C" int fcg_pressure(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    const double gd0 = kG / ctx->density_0;
+    hipLaunchKernelGGL(fcgk::k_pressure, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, gd0,
+                       ctx->density_0);
+    hipLaunchKernelGGL(fcgk::k_bpg, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_vertical_mixing(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    const fcg_params &p = ctx->p;
+    // z0b_min/z0s_min/snul as fv_run.dat gives them
+    hipLaunchKernelGGL(fcgk::k_d3end, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, p.dt, p.Dmin,
+                       p.z0b_min, p.z0s_min, p.snul, ctx->density_0);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_compute_vel_rhs(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    const fcg_params &p = ctx->p;
+    double ff = 1.5 + kEps;
+    if (ctx->lfirst) {  // src/fv_3D_vel.F90:124-128
+        ff = 1.0;
+        ctx->lfirst = 0;
+    }
+    hipLaunchKernelGGL(fcgk::k_velgrad3d, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
+    hipLaunchKernelGGL(fcgk::k_velrhs, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, p.dt, ff,
+                       ctx->density_0, p.Cd_bz, p.fic_point, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0,
+                       p.im_vert_visc_fic ? 1 : 0);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    const fcg_params &p = ctx->p;
+    if (p.tf_presence && ctx->m.nobn) {
+        FCG_CHECK(hipStreamSynchronize(ctx->stream));  // tide_pinned is reused each step
+        fill_tides(ctx);
+        FCG_CHECK(hipMemcpyAsync(ctx->s.tide, ctx->tide_pinned, sizeof(double) * p.Mt * ctx->m.nobn,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (!ctx->bt_graph_ok) {
+        hipGraph_t g = nullptr;
+        FCG_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_substeps(ctx);
+        FCG_CHECK(hipStreamEndCapture(ctx->stream, &g));
+        FCG_CHECK(hipGraphInstantiate(&ctx->bt_graph, g, nullptr, nullptr, 0));
+        FCG_CHECK(hipGraphDestroy(g));
+        ctx->bt_graph_ok = true;
+    }
+    FCG_CHECK(hipGraphLaunch(ctx->bt_graph, ctx->stream));
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_jacobian(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    hipLaunchKernelGGL(fcgk::k_jac_node, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, ctx->p.Dmin);
+    hipLaunchKernelGGL(fcgk::k_je, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    hipLaunchKernelGGL(fcgk::k_update3d, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s,
+                       ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_vert_vel_sigma(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    hipLaunchKernelGGL(fcgk::k_vertvel, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_solve_tracers(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    const fcg_params &p = ctx->p;
+    const DevMesh &m = ctx->m;
+    const DevState &s = ctx->s;
+    const size_t NL = (size_t)m.N * m.L;
+    int nb = ctx->red_blocks;
+    hipLaunchKernelGGL(fcgk::k_minmax_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, (const double *)s.TF,
+                       (const double *)s.SF, NL, s.red);
+    hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal);
+    hipLaunchKernelGGL(fcgk::k_tr_ab, dim3(nblk(NL)), dim3(kBlk), 0, ctx->stream, s, NL);
+    hipLaunchKernelGGL(fcgk::k_tr_upwind, dim3(nblk(m.N)), dim3(kBlk), 0, ctx->stream, m, s, p.dt, p.Dmin,
+                       p.cl_relax ? 1 : 0, p.clim_relax, p.relax2clim_ac);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_oce_timestep(fcg_ctx *ctx) {
+    TRY(fcg_advance_clock(ctx));
+    TRY(fcg_pressure(ctx));
+    TRY(fcg_vertical_mixing(ctx));
+    TRY(fcg_compute_vel_rhs(ctx));
+    TRY(fcg_barotropic_timestep(ctx));
+    TRY(fcg_jacobian(ctx));
+    TRY(fcg_update_3d_vel(ctx));
+    TRY(fcg_vert_vel_sigma(ctx));
+    TRY(fcg_solve_tracers(ctx));
+    return FCG_OK;
+}
+
+extern "C" int fcg_run(fcg_ctx *ctx, int nsteps) {
+    if (!ctx || nsteps < 0) return FCG_E_ARG;
+    for (int i = 0; i < nsteps; ++i) TRY(fcg_oce_timestep(ctx));
+    return FCG_OK;
+}
+
+extern "C" int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out) {
+    if (!ctx || !out) return FCG_E_ARG;
+    const DevMesh &m = ctx->m;
+    const DevState &s = ctx->s;
+    int nb = ctx->red_blocks;
+    hipLaunchKernelGGL(fcgk::k_diag_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, m, s, ctx->p.Dmin, s.red);
+    hipLaunchKernelGGL(fcgk::k_diag_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal + 4);
+    FCG_CHECK(hipGetLastError());
+    double r[12];
+    int wet = 0, err = 0;
+    FCG_CHECK(hipMemsetAsync(s.err + 1, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(fcgk::k_mask_count, dim3(64), dim3(kBlk), 0, ctx->stream, (const double *)s.mask_wd, m.E,
+                       s.err + 1);
+    FCG_CHECK(hipMemcpyAsync(r, s.scal + 4, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+    FCG_CHECK(hipMemcpyAsync(&err, s.err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCG_CHECK(hipMemcpyAsync(&wet, s.err + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memset(out, 0, sizeof *out);
+    out->energy = (r[0] + r[2]) / r[1];
+    out->volume = r[3];
+    out->heat = r[4];
+    out->salt = r[5];
+    out->eta_min = r[6];
+    out->eta_max = r[7];
+    out->T_min = r[8];
+    out->T_max = r[9];
+    out->S_min = r[10];
+    out->S_max = r[11];
+    out->time_jd = ctx->time_jd;
+    out->n_dt = ctx->n_dt;
+    out->n_wet_elem = wet;
+    out->status = err;
+    return FCG_OK;
+}

@@ -1,0 +1,116 @@
+// Internal declarations shared by the HIP translation units of libfesomc_gpu.
+// Device layout: per-entity constants as small AoS vectors (int4/double4 per
+// element, int2/double4 per edge) so a wavefront reads them coalesced; 3-D
+// fields level-major ("planes"): X[k*n_entities + i].
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fesomc_gpu.h"
+
+#define FCG_CHECK(expr)                                                            \
+    do {                                                                           \
+        hipError_t e__ = (expr);                                                   \
+        if (e__ != hipSuccess) return fcg_fail_hip(ctx, e__, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+constexpr double kG = 9.8;
+constexpr double kREarthD = 6400000.0;
+constexpr double kBeta = 0.281105;    // AB3 (src/fv_var.F90:108)
+constexpr double kEps = 0.01;         // AB2 offset (src/fv_var.F90:110)
+constexpr double kCka = 0.4;
+constexpr double kPrNum = 0.0001;
+constexpr double kBetaScale = 3.9;
+constexpr int kEmpty = INT32_MIN;     // unused slot in a fixed-width edge list
+
+// Device mirror of the mesh and state of one model instance.
+struct DevMesh {
+    int N = 0, E = 0, D = 0, Din = 0, L = 0, K = 0, nobn = 0;
+    // element constants
+    int4 *en = nullptr;        // nodes (0-based); triangle: .w == .x
+    int4 *elnb = nullptr;      // neighbour elements, -1 none (free-slip mirror)
+    int4 *eledges = nullptr;   // edges in node rotation order (for free-slip)
+    int4 *el_ied = nullptr;    // internal edges, ascending id; e>=0: element is edge_tri(2) (+), ~e: edge_tri(1) (-)
+    double4 *wcv = nullptr;    // w_cv
+    double4 *gsx = nullptr, *gsy = nullptr;   // gradient_sca(1:4), (5:8)
+    double4 *gvx = nullptr, *gvy = nullptr;   // gradient_vec(1:4), (5:8)
+    double *elem_area = nullptr, *elem_cos = nullptr, *coriolis = nullptr;
+    double *accB = nullptr;    // (sum w_cv*ac)**Bp_bz * mask_bpr, host-evaluated with libm pow
+    double *acc_el = nullptr;  // sum w_cv*ac
+    // edge constants
+    int2 *ednodes = nullptr;   // 0-based
+    int2 *edtri = nullptr;     // 0-based, -1 none
+    double4 *ecross = nullptr; // edge_cross_dxdy
+    double2 *edxdy = nullptr;  // edge_dxdy
+    // node constants
+    double *depth = nullptr, *area = nullptr, *ac = nullptr, *atp1 = nullptr, *atp2 = nullptr;
+    int *index = nullptr, *obidx = nullptr;   // obidx: position in the OB list or -1
+    // CSR gathers
+    int *n2e_ptr = nullptr, *n2e = nullptr;   // node -> edges, ascending; e>=0: node is edge_nodes(1) (+), ~e: (2) (-)
+    int *n2el_ptr = nullptr, *n2el = nullptr; // node -> elements, ascending (nod_in_elem2D)
+    // vertical grid
+    double *sigma = nullptr;   // K
+    double *dsig = nullptr;    // L: sigma(k)-sigma(k+1)
+    double *oms = nullptr;     // K: 1-sigma(k)
+};
+
+struct DevState {
+    // 2-D barotropic (FP64)
+    double *eta_n = nullptr, *eta_n_1 = nullptr, *eta_n_2 = nullptr, *etaAB = nullptr, *eta_g = nullptr;
+    double *ssh_rhs = nullptr;            // flux sum of the last substep (diagnostic)
+    double *U2 = nullptr, *U2_1 = nullptr, *U2_2 = nullptr;   // (2,E) interleaved [2e],[2e+1]
+    double *UAB = nullptr, *aux_el = nullptr, *vel_grad = nullptr; // 2E, E, 4E
+    double *U_rhs_2D_3D = nullptr, *Bar_pr_2D = nullptr;      // 2E
+    double *U_filt = nullptr, *V_filt = nullptr, *dmean_n = nullptr; // E
+    double *mask_wd = nullptr, *taux = nullptr, *tauy = nullptr, *C_d_el = nullptr; // E
+    double *windx = nullptr, *windy = nullptr;                 // N
+    double *a = nullptr, *a_old = nullptr;                     // N: max(depth+eta_n, Dmin) now / previous jacobian
+    double *tide = nullptr;                                    // Mt*nobn prescribed OB elevation per substep
+    // 3-D, level-major planes
+    double *U_n = nullptr, *V_n = nullptr, *U_rhs = nullptr, *V_rhs = nullptr;   // L*E
+    double *U_rhsAB = nullptr, *V_rhsAB = nullptr, *U_nf = nullptr, *V_nf = nullptr; // L*E
+    double *gux = nullptr, *guy = nullptr, *gvx = nullptr, *gvy = nullptr;         // L*E
+    double *Bpu = nullptr, *Bpv = nullptr, *Je = nullptr;                          // L*E
+    double *U_puls = nullptr, *V_puls = nullptr;                                   // L*E
+    double *vxx = nullptr, *vyu = nullptr, *vyv = nullptr;                         // K*E Thomas scratch
+    double *TF = nullptr, *SF = nullptr, *T_old = nullptr, *S_old = nullptr;       // L*N
+    double *Tclim = nullptr, *Sclim = nullptr, *abT = nullptr, *abS = nullptr;     // L*N
+    double *Unode_p = nullptr, *Vnode_p = nullptr;                                 // L*N
+    double *tcp = nullptr, *ttp = nullptr, *tsp = nullptr;                         // L*N Thomas scratch
+    double *hp = nullptr, *Kv = nullptr, *snu = nullptr, *bt = nullptr, *W = nullptr; // K*N
+    double *mx_p = nullptr, *mx_q = nullptr, *mx_c2 = nullptr, *mx_xx = nullptr, *mx_yy = nullptr; // K*N d3_end scratch
+    double *mx_bA = nullptr, *mx_bB = nullptr;                                     // K*N
+    // reductions
+    double *red = nullptr;       // partial results
+    double *scal = nullptr;      // [0] T_aver [1] S_aver, diagnostics after
+    int *err = nullptr;          // device STOP flag
+};
+
+struct fcg_ctx {
+    fcg_params p{};
+    double density_0 = 0, scale_area = 0;
+    DevMesh m;
+    DevState s;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // host clock (src/fv_main.F90:400-401, 1663)
+    double time_jd = 0, time = 0;
+    int n_dt = 0, lfirst = 1;
+    // host copies needed for tides and the Fortran-layout transfer
+    std::vector<int> in_obn0;            // 0-based OB nodes
+    std::vector<double> ampt, fazt, amp_factor, phy_factor; // (nobn,12), (12)
+    std::vector<double> tide_host;       // Mt*nobn staging (pinned below)
+    double *tide_pinned = nullptr;
+    // barotropic graph
+    hipGraphExec_t bt_graph = nullptr;
+    bool bt_graph_ok = false;
+    int red_blocks = 0;
+    std::vector<void *> allocs;
+    std::string err;
+};
+
+int fcg_fail_hip(fcg_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
+int fcg_fail(fcg_ctx *ctx, int code, const std::string &msg);

@@ -1,0 +1,1236 @@
+// HIP kernels of the FESOM-C internal time step for gfx950 (FP64 drop-in).
+//
+// Every reference edge loop that scatters into nodes or elements
+// (ssh_rhs, Wvel, tracer rhs, U_rhsAB, U_rhs_2D, U_rhs_2D_3D) is a
+// deterministic gather here: one thread per receiving node/element walks its
+// incident edges in ascending edge id, recomputing each edge flux from the
+// same operands, so the summation order -- and in FP64 every bit -- equals
+// the reference's serial scatter (no atomics).  The translation unit is
+// compiled with -ffp-contract=off to keep the reference's rounding.
+//
+// Vertical Jacobians are not stored: Jc = a*(sigma_k - sigma_k+1),
+// zbar = (1-sigma_k)*a, Z = mid(zbar), Jd = (Jc1+Jc2)/2 are recomputed from
+// the node scalar a = max(depth+eta_n, Dmin) (src/fv_main.F90:1228-1263),
+// which gives the same bits as the reference's stored arrays.
+#include "fcg_internal.h"
+
+namespace fcgk {
+
+__device__ __forceinline__ double sum4(const double4 w, double x0, double x1, double x2, double x3) {
+    double s = w.x * x0;
+    s = s + w.y * x1;
+    s = s + w.z * x2;
+    s = s + w.w * x3;
+    return s;
+}
+
+__device__ __forceinline__ double dmaxr(double a, double b) { return a > b ? a : b; }
+
+// densityJM: src/fv_pressure_new.F90:340-414 (un-suffixed literals are REAL(4))
+__device__ double densityJM(double t, double s, double pz, double rho0) {
+    double ss3 = sqrt(s * s * s);
+    double bulk = 19092.56 + t * (209.8925 - t * (3.041638 - t * (-(double)1.852732e-3f - t * ((double)1.361629e-5f))))
+                + s * (104.4077 - t * (6.500517 - t * (.1553190 - t * (-(double)2.326469e-4f))))
+                + ss3 * (-5.587545 + t * (0.7390729 - t * ((double)1.909078e-2f)))
+                - pz * ((double)4.721788e-1f + t * ((double)1.028859e-2f + t * (-(double)2.512549e-4f - t * ((double)5.939910e-7f))))
+                - pz * s * (-(double)1.571896e-2f - t * ((double)2.598241e-4f + t * (-(double)7.267926e-6f)))
+                - pz * ss3 * (double)2.042967e-3f
+                + pz * pz * ((double)1.045941e-5f - t * ((double)5.782165e-10f - t * ((double)1.296821e-7f)))
+                + pz * pz * s * (-(double)2.595994e-7f + t * (-(double)1.248266e-9f + t * (-(double)3.508914e-9f)));
+    double rhopot = 999.842594
+                  + t * ((double)6.793952e-2f + t * (-(double)9.095290e-3f + t * ((double)1.001685e-4f
+                    + t * (-(double)1.120083e-6f + t * (double)6.536332e-9f))))
+                  + s * (0.824493 + t * (-(double)4.08990e-3f + t * ((double)7.64380e-5f
+                    + t * (-(double)8.24670e-7f + t * (double)5.38750e-9f))))
+                  + ss3 * (-(double)5.72466e-3f + t * ((double)1.02270e-4f + t * (-(double)1.65460e-6f)))
+                  + (double)4.8314e-4f * s * s;
+    return rhopot * bulk / (bulk + 0.1 * pz) - rho0;
+}
+
+#define TID() (blockIdx.x * blockDim.x + threadIdx.x)
+#define PL(arr, k, n, stride) (arr)[(size_t)(k) * (stride) + (n)]
+
+// ---------------------------------------------------------------------------
+// pressure: hydrostatic column scan, src/fv_pressure_new.F90:91-107
+// ---------------------------------------------------------------------------
+__global__ void k_pressure(DevMesh m, DevState s, double gd0, double rho0) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N;
+    double a = s.a[n];
+    double hp = 0.0;
+    PL(s.hp, 0, n, N) = 0.0;
+    double zb0 = m.oms[0] * a;
+    for (int k = 0; k < m.L; ++k) {
+        double zb1 = m.oms[k + 1] * a;
+        double Z = 0.5 * (zb0 + zb1);
+        double r = densityJM(PL(s.TF, k, n, N), PL(s.SF, k, n, N), -Z, rho0);
+        hp = hp + r * (a * m.dsig[k]) * gd0;
+        PL(s.hp, k + 1, n, N) = hp;
+        zb0 = zb1;
+    }
+}
+
+// baroclinic_pressure_gradient_3D: src/fv_pressure_new.F90:146-221 (FP64)
+__global__ void k_bpg(DevMesh m, DevState s) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int N = m.N, E = m.E;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
+    const double d0 = m.depth[q.x], d1 = m.depth[q.y], d2 = m.depth[q.z], d3 = m.depth[q.w];
+    const double h0 = d0 + s.eta_n[q.x], h1 = d1 + s.eta_n[q.y], h2 = d2 + s.eta_n[q.z], h3 = d3 + s.eta_n[q.w];
+    const double Ax = sum4(gx, h0, h1, h2, h3), Ay = sum4(gy, h0, h1, h2, h3);
+    const double Bx = sum4(gx, d0, d1, d2, d3), By = sum4(gy, d0, d1, d2, d3);
+    const double accB = m.accB[e];
+    const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
+    double p0 = PL(s.hp, 0, q.x, N), p1 = PL(s.hp, 0, q.y, N), p2 = PL(s.hp, 0, q.z, N), p3 = PL(s.hp, 0, q.w, N);
+    double qd_up = sum4(w, p0, p1, p2, p3);
+    double su = 0.0, sv = 0.0;
+    for (int k = 0; k < m.L; ++k) {
+        const double ds = m.dsig[k];
+        const double je = sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
+        const double aJ = -accB / je;
+        double r0 = PL(s.hp, k + 1, q.x, N), r1 = PL(s.hp, k + 1, q.y, N), r2 = PL(s.hp, k + 1, q.z, N), r3 = PL(s.hp, k + 1, q.w, N);
+        double qd_low = sum4(w, r0, r1, r2, r3);
+        double pr0 = 0.5 * (p0 + r0) * (a0 * ds), pr1 = 0.5 * (p1 + r1) * (a1 * ds);
+        double pr2 = 0.5 * (p2 + r2) * (a2 * ds), pr3 = 0.5 * (p3 + r3) * (a3 * ds);
+        double bu = aJ * (sum4(gx, pr0, pr1, pr2, pr3) + qd_low * (m.sigma[k + 1] * Ax - Bx) - qd_up * (m.sigma[k] * Ax - Bx));
+        double bv = aJ * (sum4(gy, pr0, pr1, pr2, pr3) + qd_low * (m.sigma[k + 1] * Ay - By) - qd_up * (m.sigma[k] * Ay - By));
+        PL(s.Bpu, k, e, E) = bu;
+        PL(s.Bpv, k, e, E) = bv;
+        su = su + bu * je;
+        sv = sv + bv * je;
+        p0 = r0; p1 = r1; p2 = r2; p3 = r3;
+        qd_up = qd_low;
+    }
+    s.Bar_pr_2D[2 * e] = su;
+    s.Bar_pr_2D[2 * e + 1] = sv;
+}
+
+// ---------------------------------------------------------------------------
+// compute_el2nodes_3D (src/fv_utilit.F90:644-700) fused into d3_end
+// (src/fv_mixing.F90:73-347): one thread per node column.
+// Scratch planes (K*N): p=-b1-b2, q=b_t/dt, c2, xx1, yy1, btr ping-pong.
+// ---------------------------------------------------------------------------
+__global__ void k_d3end(DevMesh m, DevState s, double dt, double Dmin, double z0b_min, double z0s_min,
+                        double snul, double rho0) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, E = m.E, K = m.K, L = m.L;
+    const double t_inv = 1.0 / dt;
+    const double B1cka2 = 16.6 * (kCka * kCka);
+    const double a = s.a[n];
+    const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]) ;
+    const double dmean2 = dmean * dmean;
+    // node average of the element velocities (area weighted)
+    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    double sa = 0.0;
+    for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
+    const double tvi = 1.0 / sa;
+    auto unode = [&](int k, double &u, double &v) {
+        double su = 0.0, sv = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            int el = m.n2el[j];
+            double ar = m.elem_area[el];
+            su = su + PL(s.U_n, k, el, E) * ar;
+            sv = sv + PL(s.V_n, k, el, E) * ar;
+        }
+        u = su * tvi;
+        v = sv * tvi;
+    };
+    // 1-based level helpers
+#define SNU(nz) PL(s.snu, (nz) - 1, n, N)
+#define BT(nz) PL(s.bt, (nz) - 1, n, N)
+#define KV(nz) PL(s.Kv, (nz) - 1, n, N)
+#define SG(nz) m.sigma[(nz) - 1]
+    double uprev, vprev;
+    unode(0, uprev, vprev);
+    for (int nz = 2; nz <= K - 1; ++nz) {
+        double d1 = SG(nz - 1) - SG(nz);
+        double d2 = SG(nz) - SG(nz + 1);
+        double d = 0.5 * (d1 + d2);
+        double ucur, vcur;
+        unode(nz - 1, ucur, vcur);
+        double ap1 = uprev - ucur;
+        double ap2 = vprev - vcur;
+        double b1 = SNU(nz) * (ap1 * ap1 + ap2 * ap2) / (d * d);
+        b1 = b1 / dmean2;
+        double zb = m.oms[nz - 1] * a;
+        double rho_up = densityJM(PL(s.TF, nz - 2, n, N), PL(s.SF, nz - 2, n, N), -zb, rho0);
+        double rho_dn = densityJM(PL(s.TF, nz - 1, n, N), PL(s.SF, nz - 1, n, N), -zb, rho0);
+        ap1 = (rho_up - rho_dn) / d;
+        double b2 = -0.5 * kG * KV(nz) * ap1 / (rho0 * dmean);
+        double snu_p = 0.5 * (SNU(nz - 1) + SNU(nz));
+        double snu_m = 0.5 * (SNU(nz) + SNU(nz + 1));
+        double c2 = 0.5 * 0.73 * (snu_p * (BT(nz - 1) - BT(nz)) / d1 - snu_m * (BT(nz) - BT(nz + 1)) / d2) / d;
+        c2 = c2 / dmean2;
+        PL(s.mx_p, nz - 1, n, N) = -b1 - b2;
+        PL(s.mx_q, nz - 1, n, N) = t_inv * BT(nz);
+        PL(s.mx_c2, nz - 1, n, N) = c2;
+        uprev = ucur;
+        vprev = vcur;
+    }
+    // bottom: uprev/vprev now hold the deepest layer (nsigma-1)
+    const double ub = uprev, vb = vprev;
+    const double d1w = 0.5 * (SG(K - 1) - SG(K));
+    const double wx = s.windx[n], wy = s.windy[n];
+    const double uds = 0.05 * (wx * wx + wy * wy);
+    const double snu_mw = 0.5 * (SNU(K) + SNU(K - 1));
+    const double yy2 = (0.4e-3 * (uds * uds * uds) * dmean * d1w) / (snu_mw * 0.76);
+    const double delta_b = a * m.dsig[L - 1] + z0b_min;
+    const double lg = log(delta_b / z0b_min);
+    const double bott_R = B1cka2 / (lg * lg);
+    double *bc = s.mx_bA, *bn = s.mx_bB;
+    for (int nz = 1; nz <= K; ++nz) PL(bc, nz - 1, n, N) = BT(nz);
+    for (int it = 1; it <= 200; ++it) {
+        double xxp = 1.0, yyp = yy2;
+        for (int nz = 2; nz <= K - 1; ++nz) {
+            double d1 = SG(nz - 1) - SG(nz);
+            double d2 = SG(nz) - SG(nz + 1);
+            double d = 0.5 * (d1 + d2);
+            double btr = PL(bc, nz - 1, n, N);
+            double sn = SNU(nz);
+            double c1 = 0.046 * (btr * btr) / sn;
+            double fr = PL(s.mx_p, nz - 1, n, N) - c1 - PL(s.mx_q, nz - 1, n, N) - PL(s.mx_c2, nz - 1, n, N);
+            double snu_p = 0.5 * (SNU(nz - 1) + sn);
+            double snu_m = 0.5 * (sn + SNU(nz + 1));
+            double am = 0.5 * (snu_p * 0.73) / (dmean2 * d1 * d);
+            double cm = 0.5 * (snu_m * 0.73) / (dmean2 * d2 * d);
+            double bm = am + cm + (2.0 * 0.046 * btr) / sn + t_inv;
+            double aa = bm - am * xxp;
+            double xxn = cm / aa;
+            double yyn = (am * yyp - fr) / aa;
+            PL(s.mx_xx, nz, n, N) = xxn;   // xx1(nz+1)
+            PL(s.mx_yy, nz, n, N) = yyn;
+            xxp = xxn;
+            yyp = yyn;
+        }
+        // back substitution from the bottom boundary value, with convergence measure
+        double b1v = bott_R * (ub * ub + vb * vb);
+        PL(bn, K - 1, n, N) = b1v;
+        double bbb = 0.0;
+        {
+            double bo = PL(bc, K - 1, n, N);
+            double aa = bo - b1v;
+            double bb = dmaxr(b1v, bo) + 1.0e-8;
+            double cc = fabs(aa / bb);
+            if (cc > bbb) bbb = cc;
+        }
+        for (int nz = K; nz >= 2; --nz) {
+            double xx = (nz == 2) ? 1.0 : PL(s.mx_xx, nz - 1, n, N);
+            double yy = (nz == 2) ? yy2 : PL(s.mx_yy, nz - 1, n, N);
+            if (nz == K) { xx = xxp; yy = yyp; }
+            b1v = b1v * xx + yy;
+            PL(bn, nz - 2, n, N) = b1v;
+            double bo = PL(bc, nz - 2, n, N);
+            double aa = bo - b1v;
+            double bb = dmaxr(b1v, bo) + 1.0e-8;
+            double cc = fabs(aa / bb);
+            if (cc > bbb) bbb = cc;
+        }
+        if (bbb < 1.0e-6) break;
+        double *t = bc; bc = bn; bn = t;
+    }
+    for (int nz = 1; nz <= K; ++nz) {
+        double btr = PL(bc, nz - 1, n, N);
+        double btv = dmaxr(btr, 1.0e-8);
+        BT(nz) = btv;
+        double zt = SG(nz) * dmean;
+        double Zh = zt + z0b_min;
+        double Zd = fabs(dmean - zt) + z0s_min;
+        double Z0 = 1.0 - kBetaScale * Zh * Zd / (dmean * dmean);
+        double scale_t = kCka * Zh * Zd * Z0 / dmean;
+        double sv = scale_t * sqrt(btv) + snul;
+        if (sv >= 0.2) sv = 0.2;
+        SNU(nz) = sv;
+        KV(nz) = kPrNum * (sv);
+    }
+#undef SNU
+#undef BT
+#undef KV
+#undef SG
+}
+
+// ---------------------------------------------------------------------------
+// vel_gradients (3-D, free slip): src/fv_gradients.F90:2-93
+// ---------------------------------------------------------------------------
+__global__ void k_velgrad3d(DevMesh m, DevState s) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int E = m.E;
+    const int4 q = m.en[e];
+    const int jend = (q.x == q.w) ? 3 : 4;
+    const int4 nb = m.elnb[e];
+    const int4 ed = m.eledges[e];
+    const double4 gx = m.gvx[e], gy = m.gvy[e];
+    const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
+    const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
+    const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
+    const double ec = m.elem_cos[e];
+    for (int k = 0; k < m.L; ++k) {
+        const double u = PL(s.U_n, k, e, E), v = PL(s.V_n, k, e, E);
+        double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
+        for (int j = 0; j < jend; ++j) {
+            int e2 = nbs[j];
+            if (e2 >= 0) {
+                double du = PL(s.U_n, k, e2, E) - u;
+                double dv = PL(s.V_n, k, e2, E) - v;
+                ux = ux + gxs[j] * du;
+                uy = uy + gys[j] * du;
+                vx = vx + gxs[j] * dv;
+                vy = vy + gys[j] * dv;
+            } else {
+                double2 dd = m.edxdy[eds[j]];
+                double zc0 = dd.x * ec, zc1 = dd.y;
+                double un = -2 * (u * zc1 - v * zc0) / (zc0 * zc0 + zc1 * zc1);
+                ux = ux + gxs[j] * un * zc1;
+                uy = uy + gys[j] * un * zc1;
+                vx = vx - gxs[j] * un * zc0;
+                vy = vy - gys[j] * un * zc0;
+            }
+        }
+        PL(s.gux, k, e, E) = ux;
+        PL(s.guy, k, e, E) = uy;
+        PL(s.gvx, k, e, E) = vx;
+        PL(s.gvy, k, e, E) = vy;
+    }
+}
+
+// momentum_adv_upwind edge flux at one level: src/fv_advection.F90:120-165
+__device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevState &s, int ed, int k, double &uu, double &vv) {
+    const int E = m.E;
+    const int2 nd = m.ednodes[ed];
+    const int2 et = m.edtri[ed];
+    const double acc = 0.5 * (m.ac[nd.x] + m.ac[nd.y]);
+    const double4 cd = m.ecross[ed];
+    const double2 dd = m.edxdy[ed];
+    const int e1 = et.x, e2 = et.y;
+    double u1 = PL(s.U_n, k, e1, E) - PL(s.gux, k, e1, E) * cd.x - PL(s.guy, k, e1, E) * cd.y;
+    double v1 = PL(s.V_n, k, e1, E) - PL(s.gvx, k, e1, E) * cd.x - PL(s.gvy, k, e1, E) * cd.y;
+    double u2 = PL(s.U_n, k, e2, E) - PL(s.gux, k, e2, E) * cd.z - PL(s.guy, k, e2, E) * cd.w;
+    double v2 = PL(s.V_n, k, e2, E) - PL(s.gvx, k, e2, E) * cd.z - PL(s.gvy, k, e2, E) * cd.w;
+    const double ds = m.dsig[k];
+    const double jd = 0.5 * (s.a[nd.x] * ds + s.a[nd.y] * ds);
+    double un = 0.5 * kREarthD * jd * ((u1 + u2) * dd.y - (v1 * m.elem_cos[e1] + v2 * m.elem_cos[e2]) * dd.x);
+    if (un >= 0.0) {
+        uu = u1 * un * acc;
+        vv = v1 * un * acc;
+    } else {
+        uu = u2 * un * acc;
+        vv = v2 * un * acc;
+    }
+}
+
+// compute_vel_rhs (src/fv_3D_vel.F90:1-197) for one element column:
+// elemental pressure/Coriolis/AB2 (35-78), momentum_adv_upwind gather
+// (fv_advection.F90:64-197), momentum_vert_adv_upwind (354-469), the
+// explicit update (124-143) and momentum_vert_impl_visc_tmp
+// (fv_viscosity.F90:1374-1486) with Av formed from snu (fv_mixing.F90:308-313).
+__global__ void k_velrhs(DevMesh m, DevState s, double dt, double ff, double rho0, double Cd_bz, int fic_point,
+                         int hadv, int vadv, int visc) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int N = m.N, E = m.E, L = m.L, K = m.K;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
+    const double area = m.elem_area[e], cor = m.coriolis[e];
+    const double area_inv = 1.0 / area;
+    const double mask = s.mask_wd[e];
+    const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
+    double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
+    double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
+    double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
+    double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
+    const double Fx = sum4(gx, pre0, pre1, pre2, pre3) * area;
+    const double Fy = sum4(gy, pre0, pre1, pre2, pre3) * area;
+    const double acc = sum4(w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+    const int4 ied = m.el_ied[e];
+    const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+    auto Je = [&](int k) {
+        double ds = m.dsig[k];
+        return sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
+    };
+    auto Av = [&](int k) {
+        return sum4(w, PL(s.snu, k, q.x, N), PL(s.snu, k, q.y, N), PL(s.snu, k, q.z, N), PL(s.snu, k, q.w, N));
+    };
+    auto zbw = [&](int lev) {  // sum w_cv*zbar at 0-based level
+        double o = m.oms[lev];
+        return sum4(w, o * a0, o * a1, o * a2, o * a3);
+    };
+    auto Zw = [&](int lay) {  // sum w_cv*Z at 0-based layer
+        double o0 = m.oms[lay], o1 = m.oms[lay + 1];
+        return sum4(w, 0.5 * (o0 * a0 + o1 * a0), 0.5 * (o0 * a1 + o1 * a1), 0.5 * (o0 * a2 + o1 * a2),
+                    0.5 * (o0 * a3 + o1 * a3));
+    };
+    auto U = [&](int k) { return PL(s.U_n, k, e, E); };
+    auto V = [&](int k) { return PL(s.V_n, k, e, E); };
+    // vertical advective flux through 1-based interface nz (2..K-1): fv_advection.F90:389-433
+    auto uvert = [&](int nz, double &fu, double &fv) {
+        double wv = sum4(w, PL(s.W, nz - 1, q.x, N), PL(s.W, nz - 1, q.y, N), PL(s.W, nz - 1, q.z, N),
+                         PL(s.W, nz - 1, q.w, N)) * area;
+        double um = 0.0, vm = 0.0;
+        if (wv >= 0.0) {
+            if (nz == K - 1) {
+                um = 0.5 * (U(nz - 2) + U(nz - 1));
+                vm = 0.5 * (V(nz - 2) + V(nz - 1));
+            } else {
+                double Z1 = zbw(nz - 1);
+                double aa = Z1 - Zw(nz - 2);
+                double bb = Zw(nz - 1) - Z1;
+                double cc = Zw(nz) - Z1;
+                double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+                double dg = aa * bb * (aa + bb) / d;
+                double db = -aa * cc * (aa + cc) / d;
+                double da = 1.0 - dg - db;
+                um = U(nz - 2) * da + U(nz - 1) * db + U(nz) * dg;
+                vm = V(nz - 2) * da + V(nz - 1) * db + V(nz) * dg;
+            }
+        } else {
+            if (nz == 2) {
+                um = 0.5 * (U(nz - 2) + U(nz - 1));
+                vm = 0.5 * (V(nz - 2) + V(nz - 1));
+            } else {
+                double Z1 = zbw(nz - 1);
+                double aa = Zw(nz - 1) - Z1;
+                double bb = Z1 - Zw(nz - 2);
+                double cc = Z1 - Zw(nz - 3);
+                double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+                double dg = aa * bb * (aa + bb) / d;
+                double db = -aa * cc * (aa + cc) / d;
+                double da = 1.0 - dg - db;
+                um = U(nz - 1) * da + U(nz - 2) * db + U(nz - 3) * dg;
+                vm = V(nz - 1) * da + V(nz - 2) * db + V(nz - 3) * dg;
+            }
+        }
+        fu = -um * wv;
+        fv = -vm * wv;
+    };
+    double uvt_u = 0.0, uvt_v = 0.0;  // flux through the top of the current layer
+    if (vadv && mask != 0.0) {
+        double w1 = sum4(w, PL(s.W, 0, q.x, N), PL(s.W, 0, q.y, N), PL(s.W, 0, q.z, N), PL(s.W, 0, q.w, N)) * area;
+        uvt_u = -w1 * U(0);
+        uvt_v = -w1 * V(0);
+    }
+    // implicit vertical viscosity: forward-sweep state
+    double xxp = 1.0, yup = 0.0, yvp = 0.0;
+    const bool solve = visc && mask != 0.0;
+    if (solve) {
+        double de = Je(0) / (Av(0) + Av(1));
+        double rw = de / rho0;
+        yup = s.taux[e] * rw;
+        yvp = s.tauy[e] * rw;
+    }
+    double je_prev = 0.0, je_cur = Je(0);
+    double av_cur = Av(0);
+    for (int k = 0; k < L; ++k) {
+        const double je = je_cur;
+        const double je_next = (k + 1 < L) ? Je(k + 1) : 0.0;
+        const double u = U(k), v = V(k);
+        double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, E) + Fx * je;
+        double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, E) + Fy * je;
+        double abu = v * cor * area * je;
+        double abv = -u * cor * area * je;
+        if (hadv) {
+            for (int j = 0; j < 4; ++j) {
+                int c = slots[j];
+                if (c == kEmpty) break;
+                double uu, vv;
+                if (c >= 0) {
+                    edge_uu3d(m, s, c, k, uu, vv);
+                    abu = abu + uu;
+                    abv = abv + vv;
+                } else {
+                    edge_uu3d(m, s, ~c, k, uu, vv);
+                    abu = abu - uu;
+                    abv = abv - vv;
+                }
+            }
+        }
+        if (vadv) {
+            if (mask != 0.0) {
+                double fbu = 0.0, fbv = 0.0;
+                if (k + 2 <= K - 1) uvert(k + 2, fbu, fbv);
+                abu = abu + (uvt_u - fbu) * acc;
+                abv = abv + (uvt_v - fbv) * acc;
+                uvt_u = fbu;
+                uvt_v = fbv;
+            } else {
+                abu = 0.0;
+                abv = 0.0;
+            }
+        }
+        PL(s.U_rhsAB, k, e, E) = abu;
+        PL(s.V_rhsAB, k, e, E) = abv;
+        ur = mask * (u + PL(s.Bpu, k, e, E) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
+        vr = mask * (v + PL(s.Bpv, k, e, E) * dt + (dt * (vr + abv * ff) * area_inv + 0.0) / je);
+        if (solve) {
+            double av_next = Av(k + 1);
+            double delta_1 = (k == 0) ? 0.5 * je : 0.5 * (je + je_prev);
+            double delta_2 = (k == L - 1) ? 0.5 * je : 0.5 * (je + je_next);
+            double zt = dt / je;
+            double aa = zt * av_cur / delta_1;
+            double cc = zt * av_next / delta_2;
+            double bb = aa + cc + 1.0;
+            double d = bb - aa * xxp;
+            xxp = cc / d;
+            yup = (aa * yup + ur) / d;
+            yvp = (aa * yvp + vr) / d;
+            PL(s.vxx, k + 1, e, E) = xxp;
+            PL(s.vyu, k + 1, e, E) = yup;
+            PL(s.vyv, k + 1, e, E) = yvp;
+            av_cur = av_next;
+        } else {
+            PL(s.U_rhs, k, e, E) = ur;
+            PL(s.V_rhs, k, e, E) = vr;
+        }
+        je_prev = je;
+        je_cur = je_next;
+    }
+    if (visc) {
+        if (mask != 0.0) {
+            double a_Cd = s.C_d_el[e] * ((1.0 - acc) * Cd_bz + 1.0);
+            double de = Je(L - 1) / (Av(K - 1) + Av(L - 1));
+            double ub = U(L - 1), vb = V(L - 1);
+            double friction = de * a_Cd * sqrt(ub * ub + vb * vb);
+            double ut, vt;
+            if (fic_point == 0) {
+                ut = 0.0;
+                vt = 0.0;
+            } else if (fic_point == 1) {
+                double aa = friction + 1.0 - xxp;
+                ut = yup / aa;
+                vt = yvp / aa;
+            } else {
+                double aa = 1.0 / (1.0 - friction) - xxp;
+                ut = yup / aa;
+                vt = yvp / aa;
+            }
+            for (int k = L - 1; k >= 0; --k) {
+                double xx = PL(s.vxx, k + 1, e, E);
+                ut = ut * xx + PL(s.vyu, k + 1, e, E);
+                vt = vt * xx + PL(s.vyv, k + 1, e, E);
+                PL(s.U_rhs, k, e, E) = ut * mask;
+                PL(s.V_rhs, k, e, E) = vt * mask;
+            }
+        } else {
+            for (int k = 0; k < L; ++k) {
+                PL(s.U_rhs, k, e, E) = 0.0;
+                PL(s.V_rhs, k, e, E) = 0.0;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// barotropic substep, timestep_AB_2D: src/fv_main.F90:1372-1634
+// ---------------------------------------------------------------------------
+constexpr double kAB0 = 1.5 + kBeta, kAB1 = 0.5 + 2.0 * kBeta;
+
+__device__ __forceinline__ double etaAB3(const DevState &s, int n) {
+    return kAB0 * s.eta_n[n] - kAB1 * s.eta_n_1[n] + kBeta * s.eta_n_2[n];
+}
+
+// AB3 velocity + element depth of compute_ssh_rhs_elem (fv_average_dynamic.F90:66-86)
+__global__ void k_bt_elem1(DevMesh m, DevState s, double Dmin) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e];
+    s.UAB[2 * e] = kAB0 * s.U2[2 * e] - kAB1 * s.U2_1[2 * e] + kBeta * s.U2_2[2 * e];
+    s.UAB[2 * e + 1] = kAB0 * s.U2[2 * e + 1] - kAB1 * s.U2_1[2 * e + 1] + kBeta * s.U2_2[2 * e + 1];
+    const int ns[4] = {q.x, q.y, q.z, q.w};
+    double ad[4];
+    for (int i = 0; i < 4; ++i) {
+        ad[i] = Dmin;
+        double t = etaAB3(s, ns[i]) + m.depth[ns[i]];
+        if (t > Dmin) ad[i] = t;
+    }
+    s.aux_el[e] = sum4(w, ad[0], ad[1], ad[2], ad[3]);
+}
+
+// ssh_rhs gather (fv_average_dynamic.F90:106-169), eta update (fv_main.F90:1432-1435),
+// tidal OB value (fv_average_dynamic.F90:190-216, host-evaluated), the AM4
+// elevation of update_2D_vel (fv_average_dynamic.F90:288-296) and the time-level
+// shift (fv_main.F90:1547-1551).
+__global__ void k_bt_node(DevMesh m, DevState s, double dt_2D, const double *__restrict__ tide) {
+    int n = TID();
+    if (n >= m.N) return;
+    double ssh = 0.0;
+    for (int j = m.n2e_ptr[n]; j < m.n2e_ptr[n + 1]; ++j) {
+        int c = m.n2e[j];
+        int ed = c >= 0 ? c : ~c;
+        const int2 et = m.edtri[ed];
+        const double4 cd = m.ecross[ed];
+        double c1 = (s.UAB[2 * et.x + 1] * cd.x - s.UAB[2 * et.x] * cd.y) * s.aux_el[et.x];
+        if (et.y >= 0) c1 = c1 + (-s.UAB[2 * et.y + 1] * cd.z + s.UAB[2 * et.y] * cd.w) * s.aux_el[et.y];
+        ssh = (c >= 0) ? ssh + c1 : ssh - c1;
+    }
+    s.ssh_rhs[n] = ssh;
+    const double e0 = s.eta_n[n], e1 = s.eta_n_1[n], e2 = s.eta_n_2[n];
+    double en;
+    if (m.index[n] < 2) en = e0 + dt_2D * ssh / m.area[n];
+    else en = tide ? tide[m.obidx[n]] : ssh;
+    const double de = 0.614, ga = 0.088, ep = 0.013;
+    const double fe = 1.0 - de - ga - ep;
+    double ea = de * en + fe * e0 + ga * e1 + ep * e2;
+    s.etaAB[n] = ea;
+    s.eta_g[n] = kG * (m.atp1[n] * ea - m.atp2[n] * 0.0);
+    s.eta_n_2[n] = e1;
+    s.eta_n_1[n] = e0;
+    s.eta_n[n] = en;
+}
+
+// vel_gradients_2D(UAB): src/fv_gradients.F90:97-176
+__global__ void k_bt_velgrad(DevMesh m, DevState s) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int4 q = m.en[e];
+    const int jend = (q.x == q.w) ? 3 : 4;
+    const int4 nb = m.elnb[e];
+    const int4 ed = m.eledges[e];
+    const double4 gx = m.gvx[e], gy = m.gvy[e];
+    const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
+    const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
+    const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
+    const double u = s.UAB[2 * e], v = s.UAB[2 * e + 1];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    for (int j = 0; j < jend; ++j) {
+        int e2 = nbs[j];
+        if (e2 >= 0) {
+            g0 = g0 + gxs[j] * (s.UAB[2 * e2] - u);
+            g1 = g1 + gys[j] * (s.UAB[2 * e2] - u);
+            g2 = g2 + gxs[j] * (s.UAB[2 * e2 + 1] - v);
+            g3 = g3 + gys[j] * (s.UAB[2 * e2 + 1] - v);
+        } else {
+            double2 dd = m.edxdy[eds[j]];
+            double zc0 = dd.x * m.elem_cos[e], zc1 = dd.y;
+            double un = -2.0 * (u * zc1 - v * zc0) / (zc0 * zc0 + zc1 * zc1);
+            g0 = g0 + gxs[j] * un * zc1;
+            g1 = g1 + gys[j] * un * zc1;
+            g2 = g2 - gxs[j] * un * zc0;
+            g3 = g3 - gys[j] * un * zc0;
+        }
+    }
+    s.vel_grad[4 * e] = g0;
+    s.vel_grad[4 * e + 1] = g1;
+    s.vel_grad[4 * e + 2] = g2;
+    s.vel_grad[4 * e + 3] = g3;
+}
+
+// first substep only: dmean_n (fv_main.F90:1467-1473) and compute_puls_vel
+// (fv_3D_vel.F90:257-307).  After k_bt_node's shift eta_n holds eta^{n+1}
+// (= ssh_rhs of the reference) and eta_n_1 holds eta^n.
+__global__ void k_bt_puls(DevMesh m, DevState s, double Dmin) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int E = m.E;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e];
+    const double dm = dmaxr(Dmin, w.x * (s.eta_n_1[q.x] + m.depth[q.x]) + w.y * (s.eta_n_1[q.y] + m.depth[q.y])
+                                  + w.z * (s.eta_n_1[q.z] + m.depth[q.z]) + w.w * (s.eta_n_1[q.w] + m.depth[q.w]));
+    s.dmean_n[e] = dm;
+    const double snew = sum4(w, s.eta_n[q.x] + m.depth[q.x], s.eta_n[q.y] + m.depth[q.y], s.eta_n[q.z] + m.depth[q.z],
+                             s.eta_n[q.w] + m.depth[q.w]);
+    const double ap = dm / dmaxr(Dmin, snew);
+    const double u2 = s.U2[2 * e], v2 = s.U2[2 * e + 1];
+    for (int k = 0; k < m.L; ++k) {
+        PL(s.U_puls, k, e, E) = PL(s.U_n, k, e, E) - ap * u2;
+        PL(s.V_puls, k, e, E) = PL(s.V_n, k, e, E) - ap * v2;
+    }
+}
+
+// compute_el2nodes_3D(U_puls, V_puls): src/fv_utilit.F90:669-689
+__global__ void k_el2nodes(DevMesh m, const double *__restrict__ Ue, const double *__restrict__ Ve,
+                           double *__restrict__ Un, double *__restrict__ Vn) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, E = m.E;
+    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    double sa = 0.0;
+    for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
+    const double tvi = 1.0 / sa;
+    for (int k = 0; k < m.L; ++k) {
+        double su = 0.0, sv = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            int el = m.n2el[j];
+            double ar = m.elem_area[el];
+            su = su + PL(Ue, k, el, E) * ar;
+            sv = sv + PL(Ve, k, el, E) * ar;
+        }
+        PL(Un, k, n, N) = su * tvi;
+        PL(Vn, k, n, N) = sv * tvi;
+    }
+}
+
+// momentum_adv_P1_3D_to_2D (fv_advection.F90:676-793) as an element gather,
+// then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513)
+__global__ void k_bt_p1adv(DevMesh m, DevState s) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int N = m.N, E = m.E;
+    const int4 ied = m.el_ied[e];
+    const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+    double ru = 0.0, rv = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        int c = slots[j];
+        if (c == kEmpty) break;
+        int ed = c >= 0 ? c : ~c;
+        const int2 nd = m.ednodes[ed];
+        const int2 et = m.edtri[ed];
+        const double acc = (m.ac[nd.x] + m.ac[nd.y]) * .5;
+        const double2 dd = m.edxdy[ed];
+        const double xe = dd.x * kREarthD * 0.5 * (m.elem_cos[et.x] + m.elem_cos[et.y]);
+        const double ye = dd.y * kREarthD;
+        const double an1 = s.a[nd.x], an2 = s.a[nd.y];
+        double su = 0.0, sv = 0.0;
+        for (int k = 0; k < m.L; ++k) {
+            double uu = 0.5 * (PL(s.Unode_p, k, nd.x, N) + PL(s.Unode_p, k, nd.y, N));
+            double vv = 0.5 * (PL(s.Vnode_p, k, nd.x, N) + PL(s.Vnode_p, k, nd.y, N));
+            double ds = m.dsig[k];
+            double jd = 0.5 * (an1 * ds + an2 * ds);
+            double un = (uu * ye - vv * xe) * jd;
+            double je = PL(s.Je, k, e, E);
+            su = su + uu * un * je;
+            sv = sv + vv * un * je;
+        }
+        if (c >= 0) {  // element is edge_tri(2)
+            ru = ru + acc * su;
+            rv = rv + acc * sv;
+        } else {
+            ru = ru - acc * su;
+            rv = rv - acc * sv;
+        }
+    }
+    s.U_rhs_2D_3D[2 * e] = ru / s.dmean_n[e];
+    s.U_rhs_2D_3D[2 * e + 1] = rv / s.dmean_n[e];
+}
+
+// momentum_adv_upwind_2D edge flux: fv_advection.F90:589-643
+__device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevState &s, int ed, double Dmin, double &uu,
+                                          double &vv) {
+    const int2 nd = m.ednodes[ed];
+    const int2 et = m.edtri[ed];
+    const double dm = dmaxr(Dmin, 0.5 * ((s.etaAB[nd.x] + m.depth[nd.x]) + (s.etaAB[nd.y] + m.depth[nd.y])));
+    const double acc = (m.ac[nd.x] + m.ac[nd.y]) * 0.5;
+    const double4 cd = m.ecross[ed];
+    const double2 dd = m.edxdy[ed];
+    const int e1 = et.x, e2 = et.y;
+    const double *g1 = s.vel_grad + 4 * e1, *g2 = s.vel_grad + 4 * e2;
+    double u1 = s.UAB[2 * e1] - g1[0] * cd.x - g1[1] * cd.y;
+    double v1 = s.UAB[2 * e1 + 1] - g1[2] * cd.x - g1[3] * cd.y;
+    double u2 = s.UAB[2 * e2] - g2[0] * cd.z - g2[1] * cd.w;
+    double v2 = s.UAB[2 * e2 + 1] - g2[2] * cd.z - g2[3] * cd.w;
+    double un = 0.5 * kREarthD * dm * ((u1 + u2) * dd.y - (v1 * m.elem_cos[e1] + v2 * m.elem_cos[e2]) * dd.x);
+    if (un >= 0.0) {
+        uu = un * s.UAB[2 * e1] * acc;
+        vv = un * s.UAB[2 * e1 + 1] * acc;
+    } else {
+        uu = un * s.UAB[2 * e2] * acc;
+        vv = un * s.UAB[2 * e2 + 1] * acc;
+    }
+}
+
+// update_2D_vel (fv_average_dynamic.F90:228-443, 3-D branch) for one element,
+// followed by the velocity shift and U_filt accumulation (fv_main.F90:1569-1606).
+__global__ void k_bt_elem2(DevMesh m, DevState s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
+                           int wet_dry, int adv2d, int first) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int E = m.E, L = m.L;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
+    const double area = m.elem_area[e], cor = m.coriolis[e];
+    const double d0 = m.depth[q.x], d1 = m.depth[q.y], d2 = m.depth[q.z], d3 = m.depth[q.w];
+    const double ea0 = s.etaAB[q.x], ea1 = s.etaAB[q.y], ea2 = s.etaAB[q.z], ea3 = s.etaAB[q.w];
+    double mask = s.mask_wd[e];
+    if (wet_dry) {  // wad_mask: fv_main.F90:1284-1294
+        double mn = fmin(fmin(fmin(d0, d1), d2), d3);
+        double mx = fmax(fmax(fmax(ea0, ea1), ea2), ea3);
+        mask = (mn + mx > Dmin) ? 1.0 : 0.0;
+        s.mask_wd[e] = mask;
+    }
+    s.C_d_el[e] = C_d;
+    const double rho_inv = 1.0 / (rho0);
+    const double dm = dmaxr(Dmin, w.x * (ea0 + d0) + w.y * (ea1 + d1) + w.z * (ea2 + d2) + w.w * (ea3 + d3));
+    const double eg0 = s.eta_g[q.x], eg1 = s.eta_g[q.y], eg2 = s.eta_g[q.z], eg3 = s.eta_g[q.w];
+    const double uab = s.UAB[2 * e], vab = s.UAB[2 * e + 1];
+    double ru = area * (-dm * (gx.x * eg0 + gx.y * eg1 + gx.z * eg2 + gx.w * eg3) + vab * dm * cor + s.taux[e] * rho_inv);
+    double rv = area * (-dm * (gy.x * eg0 + gy.y * eg1 + gy.z * eg2 + gy.w * eg3) - uab * dm * cor + s.tauy[e] * rho_inv);
+    if (adv2d) {
+        const int4 ied = m.el_ied[e];
+        const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+        for (int j = 0; j < 4; ++j) {
+            int c = slots[j];
+            if (c == kEmpty) break;
+            double uu, vv;
+            if (c >= 0) {
+                edge_uu2d(m, s, c, Dmin, uu, vv);
+                ru = ru + uu;
+                rv = rv + vv;
+            } else {
+                edge_uu2d(m, s, ~c, Dmin, uu, vv);
+                ru = ru - uu;
+                rv = rv - vv;
+            }
+        }
+    }
+    ru = ru + s.U_rhs_2D_3D[2 * e];
+    rv = rv + s.U_rhs_2D_3D[2 * e + 1];
+    // eta_n_1 holds eta^n, eta_n holds eta^{n+1} (the reference's ssh_rhs)
+    const double dmo = dmaxr(Dmin, sum4(w, s.eta_n_1[q.x] + d0, s.eta_n_1[q.y] + d1, s.eta_n_1[q.z] + d2,
+                                        s.eta_n_1[q.w] + d3));
+    const double acc = sum4(w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+    const double a_Cd = C_d * ((1.0 - acc) * Cd_bz + 1.0);
+    const double ub = PL(s.U_n, L - 1, e, E), vb = PL(s.V_n, L - 1, e, E);
+    const double friction = a_Cd * sqrt(ub * ub + vb * vb);
+    const double snew = sum4(w, s.eta_n[q.x] + d0, s.eta_n[q.y] + d1, s.eta_n[q.z] + d2, s.eta_n[q.w] + d3);
+    const double ffc = 1.0 / (dmaxr(Dmin, snew) + friction * dt_2D);
+    const double U0 = s.U2[2 * e], V0 = s.U2[2 * e + 1];
+    const double un = mask * ((dmo * U0 - dt_2D * friction * (ub - U0) + dt_2D * s.Bar_pr_2D[2 * e]
+                               + dt_2D * ru / area) * ffc + 0.0);
+    const double vn = mask * ((dmo * V0 - dt_2D * friction * (vb - V0) + dt_2D * s.Bar_pr_2D[2 * e + 1]
+                               + dt_2D * rv / area) * ffc + 0.0);
+    s.U2_2[2 * e] = s.U2_1[2 * e];
+    s.U2_2[2 * e + 1] = s.U2_1[2 * e + 1];
+    s.U2_1[2 * e] = U0;
+    s.U2_1[2 * e + 1] = V0;
+    s.U2[2 * e] = un;
+    s.U2[2 * e + 1] = vn;
+    const double dmf = dmaxr(Dmin, w.x * (s.eta_n[q.x] + d0) + w.y * (s.eta_n[q.y] + d1) + w.z * (s.eta_n[q.z] + d2)
+                                   + w.w * (s.eta_n[q.w] + d3));
+    const double uf0 = first ? 0.0 : s.U_filt[e];
+    const double vf0 = first ? 0.0 : s.V_filt[e];
+    s.U_filt[e] = uf0 + un * dmf;
+    s.V_filt[e] = vf0 + vn * dmf;
+}
+
+// ---------------------------------------------------------------------------
+// jacobian: src/fv_main.F90:1228-1252 (node scalar + stored Je)
+// ---------------------------------------------------------------------------
+__global__ void k_jac_node(DevMesh m, DevState s, double Dmin) {
+    int n = TID();
+    if (n >= m.N) return;
+    s.a_old[n] = s.a[n];
+    s.a[n] = fmax(m.depth[n] + s.eta_n[n], Dmin);
+}
+
+__global__ void k_je(DevMesh m, DevState s) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int E = m.E;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e];
+    const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
+    for (int k = 0; k < m.L; ++k) {
+        double ds = m.dsig[k];
+        PL(s.Je, k, e, E) = sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
+    }
+}
+
+// update_3D_vel: src/fv_3D_vel.F90:201-255, with U_filt_2D *= 1/Mt from
+// oce_barotropic_timestep (fv_main.F90:1670-1677)
+__global__ void k_update3d(DevMesh m, DevState s, double Dmin, double Mt_inv) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int E = m.E, L = m.L;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e];
+    const double o0 = s.a_old[q.x], o1 = s.a_old[q.y], o2 = s.a_old[q.z], o3 = s.a_old[q.w];
+    const double dmi = 1.0 / dmaxr(Dmin, sum4(w, m.depth[q.x] + s.eta_n[q.x], m.depth[q.y] + s.eta_n[q.y],
+                                              m.depth[q.z] + s.eta_n[q.z], m.depth[q.w] + s.eta_n[q.w]));
+    double su = 0.0, sv = 0.0;
+    for (int k = 0; k < L; ++k) {
+        double ds = m.dsig[k];
+        double ak = sum4(w, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
+        su = su + PL(s.U_rhs, k, e, E) * ak;
+        sv = sv + PL(s.V_rhs, k, e, E) * ak;
+    }
+    const double U2p = su * dmi, V2p = sv * dmi;
+    const double uf = (s.U_filt[e] * Mt_inv) * dmi;
+    const double vf = (s.V_filt[e] * Mt_inv) * dmi;
+    s.U_filt[e] = uf;
+    s.V_filt[e] = vf;
+    const double u2 = s.U2[2 * e], v2 = s.U2[2 * e + 1];
+    for (int k = 0; k < L; ++k) {
+        double ds = m.dsig[k];
+        double ak = sum4(w, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
+        ak = ak / PL(s.Je, k, e, E);
+        double ur = PL(s.U_rhs, k, e, E), vr = PL(s.V_rhs, k, e, E);
+        PL(s.U_nf, k, e, E) = ak * ur + uf - U2p;
+        PL(s.V_nf, k, e, E) = ak * vr + vf - V2p;
+        PL(s.U_n, k, e, E) = ak * ur + u2 - U2p;
+        PL(s.V_n, k, e, E) = ak * vr + v2 - V2p;
+    }
+}
+
+// vert_vel_sigma: src/fv_3D_vel.F90:309-456 as a node gather + bottom-up scan
+__global__ void k_vertvel(DevMesh m, DevState s) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, E = m.E, L = m.L;
+    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const double area_inv = 1.0 / m.area[n];
+    double run = 0.0;  // Wvel(nsigma) = 0
+    for (int k = L - 1; k >= 0; --k) {
+        double c = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            int cc = m.n2e[j];
+            int ed = cc >= 0 ? cc : ~cc;
+            const int2 et = m.edtri[ed];
+            const double4 cd = m.ecross[ed];
+            const int e1 = et.x;
+            double f = PL(s.Je, k, e1, E) * ((PL(s.V_nf, k, e1, E) - s.V_filt[e1]) * cd.x
+                                             - (PL(s.U_nf, k, e1, E) - s.U_filt[e1]) * cd.y);
+            if (et.y >= 0) {
+                const int e2 = et.y;
+                f = f + PL(s.Je, k, e2, E) * (-(PL(s.V_nf, k, e2, E) - s.V_filt[e2]) * cd.z
+                                              + (PL(s.U_nf, k, e2, E) - s.U_filt[e2]) * cd.w);
+            }
+            c = (cc >= 0) ? c + f : c - f;
+        }
+        run = c + run;
+        PL(s.W, k, n, N) = run * area_inv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tracers: solve_tracers (fv_main.F90:1707-1836)
+// ---------------------------------------------------------------------------
+// global min/max of TF and SF (fv_main.F90:1737-1777): exact, order-free
+__global__ void k_minmax_partial(const double *__restrict__ T, const double *__restrict__ S, size_t n, double *out) {
+    __shared__ double sm[4][256];
+    double tmn = INFINITY, tmx = -INFINITY, smn = INFINITY, smx = -INFINITY;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double t = T[i], v = S[i];
+        tmn = fmin(tmn, t);
+        tmx = fmax(tmx, t);
+        smn = fmin(smn, v);
+        smx = fmax(smx, v);
+    }
+    sm[0][threadIdx.x] = tmn;
+    sm[1][threadIdx.x] = tmx;
+    sm[2][threadIdx.x] = smn;
+    sm[3][threadIdx.x] = smx;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sm[0][threadIdx.x] = fmin(sm[0][threadIdx.x], sm[0][threadIdx.x + st]);
+            sm[1][threadIdx.x] = fmax(sm[1][threadIdx.x], sm[1][threadIdx.x + st]);
+            sm[2][threadIdx.x] = fmin(sm[2][threadIdx.x], sm[2][threadIdx.x + st]);
+            sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 4; ++c) out[4 * blockIdx.x + c] = sm[c][0];
+}
+
+__global__ void k_minmax_final(const double *part, int nb, double *scal) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double tmn = part[0], tmx = part[1], smn = part[2], smx = part[3];
+    for (int b = 1; b < nb; ++b) {
+        tmn = fmin(tmn, part[4 * b]);
+        tmx = fmax(tmx, part[4 * b + 1]);
+        smn = fmin(smn, part[4 * b + 2]);
+        smx = fmax(smx, part[4 * b + 3]);
+    }
+    scal[0] = 0.5 * (tmn + tmx);
+    scal[1] = 0.5 * (smn + smx);
+}
+
+// AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444)
+__global__ void k_tr_ab(DevState s, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double Ta = s.scal[0], Sa = s.scal[1];
+    s.abT[i] = -(0.5 + kEps) * s.T_old[i] + (1.5 + kEps) * s.TF[i] - Ta;
+    s.abS[i] = -(0.5 + kEps) * s.S_old[i] + (1.5 + kEps) * s.SF[i] - Sa;
+}
+
+// QUICK vertical flux through 1-based interface nz of one node column
+// (fv_tracer.F90:740-815), for tracer column X (de-meaned AB values)
+__device__ __forceinline__ double upw_vert(const DevMesh &m, const DevState &s, const double *X, int n, int nz,
+                                           double a) {
+    const int N = m.N, K = m.K;
+    auto Wv = [&](int z) { return PL(s.W, z - 1, n, N); };
+    auto T = [&](int z) { return PL(X, z - 1, n, N); };
+    auto zb = [&](int z) { return m.oms[z - 1] * a; };
+    auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
+    if (nz == 1) return -Wv(1) * T(1);
+    if (nz == K) return 0.0;
+    if (nz == 2) {
+        if (Wv(2) > 0.0) {
+            double dg = (zb(2) - Zc(1)) * (Zc(2) - zb(2)) / ((Zc(3) - Zc(1)) * (Zc(2) - Zc(3)));
+            double db = -(zb(2) - Zc(1)) * (Zc(3) - zb(2)) / ((Zc(2) - Zc(1)) * (Zc(2) - Zc(3)));
+            return -(T(1) * (1.0 - dg - db) + T(2) * db + T(3) * dg) * Wv(2);
+        }
+        return -T(1) * Wv(2);
+    }
+    if (nz == K - 1) {
+        if (Wv(nz) >= 0.0) return -T(nz) * Wv(nz);
+        double dg = (Zc(nz) - zb(nz)) * (zb(nz) - Zc(nz - 1)) / ((Zc(nz) - Zc(nz - 2)) * (Zc(nz - 2) - Zc(nz - 1)));
+        double db = -(Zc(nz) - zb(nz)) * (zb(nz) - Zc(nz - 2)) / ((Zc(nz) - Zc(nz - 1)) * (Zc(nz - 2) - Zc(nz - 1)));
+        return -(T(nz) * (1.0 - dg - db) + T(nz - 1) * db + T(nz - 2) * dg) * Wv(nz);
+    }
+    if (Wv(nz) >= 0.0) {
+        double dg = (zb(nz) - Zc(nz - 1)) * (Zc(nz) - zb(nz)) / ((Zc(nz + 1) - Zc(nz - 1)) * (Zc(nz) - Zc(nz + 1)));
+        double db = -(zb(nz) - Zc(nz - 1)) * (Zc(nz + 1) - zb(nz)) / ((Zc(nz) - Zc(nz - 1)) * (Zc(nz) - Zc(nz + 1)));
+        return -(T(nz - 1) * (1.0 - dg - db) + T(nz) * db + T(nz + 1) * dg) * Wv(nz);
+    }
+    double dg = (Zc(nz) - zb(nz)) * (zb(nz) - Zc(nz - 1)) / ((Zc(nz) - Zc(nz - 2)) * (Zc(nz - 2) - Zc(nz - 1)));
+    double db = -(Zc(nz) - zb(nz)) * (zb(nz) - Zc(nz - 2)) / ((Zc(nz) - Zc(nz - 1)) * (Zc(nz - 2) - Zc(nz - 1)));
+    return -(T(nz) * (1.0 - dg - db) + T(nz - 1) * db + T(nz - 2) * dg) * Wv(nz);
+}
+
+// solve_tracer_upwind (fv_tracer.F90:390-923) followed by
+// tracer_impl_vert_diff (fv_tracer.F90:1319-1494), one thread per node column.
+__global__ void k_tr_upwind(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
+                            double relax2clim_ac) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, E = m.E, L = m.L, K = m.K;
+    const double Ta = s.scal[0], Sa = s.scal[1];
+    const double a = s.a[n];
+    const double area_inv = 1.0 / m.area[n];
+    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const bool ob = cl_relax && m.index[n] == 2;
+    double vel_nor = 0.0;
+    if (ob) {  // fv_tracer.F90:862-873: boundary-edge normal transport
+        for (int j = p0; j < p1; ++j) {
+            int cc = m.n2e[j];
+            int ed = cc >= 0 ? cc : ~cc;
+            if (ed < m.Din) continue;
+            const int2 nd = m.ednodes[ed];
+            const int e1 = m.edtri[ed].x;
+            const double4 cd = m.ecross[ed];
+            double dmn = dmaxr(Dmin, 0.5 * (s.eta_n[nd.x] + m.depth[nd.x] + s.eta_n[nd.y] + m.depth[nd.y]));
+            double un1 = (s.V_filt[e1] * cd.x - s.U_filt[e1] * cd.y) * dmn;
+            vel_nor = (cc >= 0) ? vel_nor + un1 / m.area[nd.x] : vel_nor - un1 / m.area[nd.y];
+        }
+    }
+    double tvt = upw_vert(m, s, s.abT, n, 1, a);
+    double svt = upw_vert(m, s, s.abS, n, 1, a);
+    for (int k = 0; k < L; ++k) {
+        double tc = 0.0, tt = 0.0, ts = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            int cc = m.n2e[j];
+            int ed = cc >= 0 ? cc : ~cc;
+            const int2 nd = m.ednodes[ed];
+            const int2 et = m.edtri[ed];
+            const double4 cd = m.ecross[ed];
+            const int e1 = et.x;
+            double c1 = PL(s.V_nf, k, e1, E) * cd.x - PL(s.U_nf, k, e1, E) * cd.y;
+            double auxc, taux_, saux_;
+            const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
+            const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
+            if (et.y >= 0) {
+                const int e2 = et.y;
+                double c2 = -PL(s.V_nf, k, e2, E) * cd.z + PL(s.U_nf, k, e2, E) * cd.w;
+                double J1 = PL(s.Je, k, e1, E), J2 = PL(s.Je, k, e2, E);
+                auxc = J1 * c1 + J2 * c2;
+                if (c1 > 0.0 && c2 > 0.0) {
+                    taux_ = J1 * c1 * T2 + J2 * c2 * T2;
+                    saux_ = J1 * c1 * S2 + J2 * c2 * S2;
+                } else if (c1 > 0.0) {
+                    taux_ = J1 * c1 * T2 + J2 * c2 * T1;
+                    saux_ = J1 * c1 * S2 + J2 * c2 * S1;
+                } else if (c2 > 0.0) {
+                    taux_ = J1 * c1 * T1 + J2 * c2 * T2;
+                    saux_ = J1 * c1 * S1 + J2 * c2 * S2;
+                } else {
+                    taux_ = J1 * c1 * T1 + J2 * c2 * T1;
+                    saux_ = J1 * c1 * S1 + J2 * c2 * S1;
+                }
+            } else {
+                auxc = PL(s.Je, k, e1, E) * c1;
+                if (c1 > 0.0) {
+                    taux_ = auxc * T2;
+                    saux_ = auxc * S2;
+                } else {
+                    taux_ = auxc * T1;
+                    saux_ = auxc * S1;
+                }
+            }
+            if (cc >= 0) {
+                tc = tc + auxc;
+                tt = tt + taux_;
+                ts = ts + saux_;
+            } else {
+                tc = tc - auxc;
+                tt = tt - taux_;
+                ts = ts - saux_;
+            }
+        }
+        const double tvb = upw_vert(m, s, s.abT, n, k + 2, a);
+        const double svb = upw_vert(m, s, s.abS, n, k + 2, a);
+        const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
+        const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+        const double jc = a * m.dsig[k];
+        double rt = ((tt - tab * tc) * area_inv + tab * wdiff + (tvt - tvb)) / jc;
+        double rs = ((ts - sab * tc) * area_inv + sab * wdiff + (svt - svb)) / jc;
+        if (ob) {
+            if (vel_nor > 0.0) {
+                rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tab + Ta));
+                rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sab + Sa));
+            } else {
+                rt = clim_relax * (PL(s.Tclim, k, n, N) - (tab + Ta));
+                rs = clim_relax * (PL(s.Sclim, k, n, N) - (sab + Sa));
+            }
+        }
+        const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
+        PL(s.T_old, k, n, N) = tdm + Ta;
+        PL(s.S_old, k, n, N) = sdm + Sa;
+        PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
+        PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
+        tvt = tvb;
+        svt = svb;
+    }
+    // ---- tracer_impl_vert_diff (key_atm off) ----
+    auto zb = [&](int z) { return m.oms[z - 1] * a; };
+    auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
+    auto KV = [&](int z) { return PL(s.Kv, z - 1, n, N); };
+    auto Td = [&](int z) { return PL(s.TF, z - 1, n, N) - Ta; };
+    auto Sd = [&](int z) { return PL(s.SF, z - 1, n, N) - Sa; };
+    double cp = 0, tp = 0, sp = 0;
+    for (int nz = 1; nz <= L; ++nz) {
+        double aa, bb, cc;
+        if (nz == 1) {
+            cc = -dt * KV(2) / ((Zc(1) - Zc(2)) * (zb(1) - zb(2)));
+            aa = 0.0;
+            bb = -cc + 1.0;
+        } else if (nz == L) {
+            aa = -dt * KV(L) / ((Zc(L - 1) - Zc(L)) * (zb(L) - zb(K)));
+            bb = -aa + 1.0;
+            cc = 0.0;
+        } else {
+            aa = -dt * KV(nz) / ((Zc(nz - 1) - Zc(nz)) * (zb(nz) - zb(nz + 1)));
+            cc = -dt * KV(nz) / ((Zc(nz) - Zc(nz + 1)) * (zb(nz) - zb(nz + 1)));
+            bb = -aa - cc + 1.0;
+        }
+        double tr, sr;
+        if (nz == 1) {
+            tr = cc * (-Td(2) + Td(1));
+            sr = cc * (-Sd(2) + Sd(1));
+        } else if (nz == L) {
+            tr = aa * (Td(L) - Td(L - 1));
+            sr = aa * (Sd(L) - Sd(L - 1));
+        } else {
+            tr = -aa * Td(nz - 1) - cc * Td(nz + 1) + (aa + cc) * Td(nz);
+            sr = -aa * Sd(nz - 1) - cc * Sd(nz + 1) + (aa + cc) * Sd(nz);
+        }
+        if (nz == 1) {
+            cp = cc / bb;
+            tp = tr / bb;
+            sp = sr / bb;
+        } else {
+            double mm = 1.0 / (bb - cp * aa);
+            cp = cc * mm;
+            tp = (tr - tp * aa) * mm;
+            sp = (sr - sp * aa) * mm;
+        }
+        PL(s.tcp, nz - 1, n, N) = cp;
+        PL(s.ttp, nz - 1, n, N) = tp;
+        PL(s.tsp, nz - 1, n, N) = sp;
+    }
+    double trn = tp, srn = sp;
+    int bad = 0;
+    for (int nz = L; nz >= 1; --nz) {
+        if (nz < L) {
+            double c = PL(s.tcp, nz - 1, n, N);
+            trn = PL(s.ttp, nz - 1, n, N) - c * trn;
+            srn = PL(s.tsp, nz - 1, n, N) - c * srn;
+        }
+        double tnew = Td(nz) + trn + Ta;
+        double snew = Sd(nz) + srn + Sa;
+        if (fabs(snew) < 1.e-12) snew = fabs(snew);
+        if (snew < 0.0) bad = bad ? bad : FCG_E_SALINITY;
+        if (tnew < -3.0) bad = bad ? bad : FCG_E_TEMP;
+        PL(s.TF, nz - 1, n, N) = tnew;
+        PL(s.SF, nz - 1, n, N) = snew;
+    }
+    if (bad) atomicCAS(s.err, 0, bad);
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: energy (fv_average_dynamic.F90:476-554), volume, tracer
+// content, extrema -- fixed-order block tree, deterministic run to run
+// ---------------------------------------------------------------------------
+constexpr int kDiag = 12;
+__global__ void k_diag_partial(DevMesh m, DevState s, double Dmin, double *out) {
+    __shared__ double sm[kDiag][256];
+    double v[kDiag] = {0, 0, 0, 0, 0, 0, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    // v: 0 PE, 1 area, 2 KE, 3 volume, 4 heat, 5 salt, 6/7 eta, 8/9 T, 10/11 S
+    const int N = m.N, E = m.E, L = m.L;
+    const int stride = gridDim.x * blockDim.x;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
+        double ar = m.area[n], et = s.eta_n[n];
+        v[0] += 0.5 * kG * ar * (et * et);
+        v[1] += ar;
+        v[3] += ar * (m.depth[n] + et);
+        v[6] = fmin(v[6], et);
+        v[7] = fmax(v[7], et);
+        double a = s.a[n];
+        for (int k = 0; k < L; ++k) {
+            double jc = a * m.dsig[k];
+            double t = PL(s.TF, k, n, N), sv = PL(s.SF, k, n, N);
+            v[4] += ar * jc * t;
+            v[5] += ar * jc * sv;
+            v[8] = fmin(v[8], t);
+            v[9] = fmax(v[9], t);
+            v[10] = fmin(v[10], sv);
+            v[11] = fmax(v[11], sv);
+        }
+    }
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        double f0 = dmaxr(Dmin, m.depth[q.x] + s.eta_n[q.x]), f1 = dmaxr(Dmin, m.depth[q.y] + s.eta_n[q.y]);
+        double f2 = dmaxr(Dmin, m.depth[q.z] + s.eta_n[q.z]), f3 = dmaxr(Dmin, m.depth[q.w] + s.eta_n[q.w]);
+        double u = s.U2[2 * e], vv = s.U2[2 * e + 1];
+        v[2] += 0.5 * sum4(w, f0, f1, f2, f3) * m.elem_area[e] * (u * u + vv * vv);
+    }
+    for (int c = 0; c < kDiag; ++c) sm[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            for (int c = 0; c < 6; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + st];
+            for (int c = 6; c < kDiag; c += 2) {
+                sm[c][threadIdx.x] = fmin(sm[c][threadIdx.x], sm[c][threadIdx.x + st]);
+                sm[c + 1][threadIdx.x] = fmax(sm[c + 1][threadIdx.x], sm[c + 1][threadIdx.x + st]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < kDiag; ++c) out[kDiag * blockIdx.x + c] = sm[c][0];
+}
+
+__global__ void k_diag_final(const double *part, int nb, double *res) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double v[kDiag];
+    for (int c = 0; c < kDiag; ++c) v[c] = part[c];
+    for (int b = 1; b < nb; ++b) {
+        for (int c = 0; c < 6; ++c) v[c] += part[kDiag * b + c];
+        for (int c = 6; c < kDiag; c += 2) {
+            v[c] = fmin(v[c], part[kDiag * b + c]);
+            v[c + 1] = fmax(v[c + 1], part[kDiag * b + c + 1]);
+        }
+    }
+    for (int c = 0; c < kDiag; ++c) res[c] = v[c];
+}
+
+__global__ void k_mask_count(const double *mask, int E, int *out) {
+    __shared__ int sm[256];
+    int c = 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) c += mask[e] != 0.0;
+    sm[threadIdx.x] = c;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sm[threadIdx.x] += sm[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, sm[0]);
+}
+
+}  // namespace fcgk

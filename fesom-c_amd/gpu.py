@@ -1,0 +1,341 @@
+"""ctypes binding of libfesomc_gpu.so -- the C-ABI declared in include/fesomc_gpu.h.
+
+This is the Python host mirror of the reference call sequence: it reads the
+fv_run.dat values (runconfig), builds the mesh through ``fcg_mesh_build``,
+initialises tracers as ``initial_state`` does (src/fv_ini.F90:476-546) and
+drives the per-call-site entry points of ``oce_timestep``.  There is no CPU
+fallback: without the library or without a HIP device every call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import meshio
+from .runconfig import RunConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfesomc_gpu.so")
+
+FCG_FP64, FCG_MIXED = 0, 1
+FCG_CLOSURE_D3END, FCG_CLOSURE_MLP = 0, 1
+ERRORS = {0: "ok", -1: "bad argument", -2: "mesh check failed", -3: "HIP error", -4: "non-finite state",
+          -5: "Negative salinity, time resolution is insufficient, stop",
+          -6: "Negative temperature, time resolution is insufficient, stop", -7: "no HIP device"}
+
+EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fcg_destroy", "fcg_set_state",
+           "fcg_get_state", "fcg_set_clock", "fcg_advance_clock", "fcg_pressure", "fcg_vertical_mixing",
+           "fcg_compute_vel_rhs", "fcg_barotropic_timestep", "fcg_jacobian", "fcg_update_3d_vel",
+           "fcg_vert_vel_sigma", "fcg_solve_tracers", "fcg_oce_timestep", "fcg_run", "fcg_diagnostics",
+           "fcg_sync", "fcg_last_error", "fcg_version"]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dt", "dt_2D", "time_jd0", "Dmin", "C_d", "z0b_min", "z0s_min", "snul",
+                                          "Cd_bz", "Bp_bz", "clim_relax", "relax2clim_ac")] + \
+               [(n, C.c_int) for n in ("nsigma", "Mt", "tracer_adv", "muscl_call", "mom_adv_2D", "mom_adv_3D",
+                                       "vert_adv", "im_vert_visc_fic", "fic_point", "wet_dry_on", "tf_presence",
+                                       "restart", "cl_relax", "precision", "closure")]
+
+
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+
+class Mesh(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nod2D", "elem2D", "edge2D", "edge2D_in", "nsigma", "nobn")] + \
+               [(n, _IP) for n in ("elem2D_nodes", "edge_nodes", "edge_tri", "elem_edges", "elem_neighbors",
+                                   "index_nod2D", "in_obn")] + \
+               [(n, _DP) for n in ("depth", "sigma", "area", "elem_area", "elem_cos", "coriolis", "w_cv",
+                                   "gradient_sca", "gradient_vec", "edge_dxdy", "edge_cross_dxdy", "ac", "a_tp",
+                                   "ampt", "fazt", "amp_factor", "phy_factor")] + \
+               [("density_0", C.c_double), ("scale_area", C.c_double)]
+
+
+class MeshCfg(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("cartesian", "coriolis_tf", "nsigma", "ind_sigma", "is_LE", "T_potential",
+                                       "tf_presence")] + \
+               [(n, C.c_double) for n in ("lat_cartesian_deg", "cyclic_length_deg", "pow_sig", "lev_bot",
+                                          "lev_sur", "T_const", "S_const")] + \
+               [("harmonics", C.c_int * 12)]
+
+
+class Diag(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("energy", "volume", "heat", "salt", "eta_min", "eta_max", "T_min",
+                                          "T_max", "S_min", "S_max", "time_jd")] + \
+               [(n, C.c_int) for n in ("n_dt", "n_wet_elem", "mlp_fallback", "status")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libfesomc_gpu.so; raises if it was not built (no silent fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.fcg_mesh_build.restype = C.c_int
+        L.fcg_mesh_build.argtypes = [C.POINTER(MeshCfg), C.c_int, _DP, _IP, C.c_int, _IP, _DP, C.c_int, _DP,
+                                     C.POINTER(P)]
+        L.fcg_mesh_view.restype = C.c_int
+        L.fcg_mesh_view.argtypes = [P, C.POINTER(Mesh)]
+        L.fcg_mesh_free.argtypes = [P]
+        L.fcg_create.restype = C.c_int
+        L.fcg_create.argtypes = [C.POINTER(Mesh), C.POINTER(Params), C.POINTER(P)]
+        L.fcg_destroy.argtypes = [P]
+        for n in ("fcg_set_state", "fcg_get_state"):
+            getattr(L, n).restype = C.c_int
+            getattr(L, n).argtypes = [P, C.c_char_p, _DP, C.c_long]
+        L.fcg_set_clock.restype = C.c_int
+        L.fcg_set_clock.argtypes = [P, C.c_double, C.c_int, C.c_int]
+        for n in ("fcg_advance_clock", "fcg_pressure", "fcg_vertical_mixing", "fcg_compute_vel_rhs",
+                  "fcg_barotropic_timestep", "fcg_jacobian", "fcg_update_3d_vel", "fcg_vert_vel_sigma",
+                  "fcg_solve_tracers", "fcg_oce_timestep", "fcg_sync"):
+            getattr(L, n).restype = C.c_int
+            getattr(L, n).argtypes = [P]
+        L.fcg_run.restype = C.c_int
+        L.fcg_run.argtypes = [P, C.c_int]
+        L.fcg_diagnostics.restype = C.c_int
+        L.fcg_diagnostics.argtypes = [P, C.POINTER(Diag)]
+        L.fcg_last_error.restype = C.c_int
+        L.fcg_last_error.argtypes = [P, C.c_char_p, C.c_int]
+        L.fcg_version.restype = C.c_char_p
+        L.fcg_version.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def mesh_cfg(cfg: RunConfig) -> MeshCfg:
+    c = MeshCfg()
+    c.cartesian = int(cfg.l("cartesian"))
+    c.coriolis_tf = int(cfg.l("Coriolis_TF"))
+    c.nsigma = cfg.nsigma
+    c.ind_sigma = cfg.i("ind_sigma", 1)
+    c.is_LE = int(cfg.is_LE)
+    c.T_potential = int(cfg.l("T_potential"))
+    c.tf_presence = int(cfg.l("TF_presence"))
+    c.lat_cartesian_deg = cfg.f("lat_cartesian", 0.0)
+    c.cyclic_length_deg = cfg.f("cyclic_length", 360.0)
+    c.pow_sig = cfg.f("pow_sig", 1.0)
+    c.lev_bot = cfg.f("lev_bot", 1.0)
+    c.lev_sur = cfg.f("lev_sur", 1.0)
+    c.T_const = cfg.f("T_const", 10.0)
+    c.S_const = cfg.f("S_const", 35.0)
+    for j, v in enumerate(cfg.harmonics):
+        c.harmonics[j] = v
+    return c
+
+
+def params(cfg: RunConfig, precision=FCG_FP64, closure=FCG_CLOSURE_D3END) -> Params:
+    p = Params()
+    p.dt, p.dt_2D, p.time_jd0 = cfg.f("dt"), cfg.f("dt_2D"), cfg.f("JD0", 0.0)
+    p.Dmin, p.C_d = cfg.f("Dmin", 0.25), cfg.f("Cd", 0.0)
+    p.z0b_min, p.z0s_min, p.snul = cfg.f("z0b_min", 0.01), cfg.f("z0s_min", 0.02), cfg.f("snul", 7.5e-4)
+    p.Cd_bz, p.Bp_bz = cfg.f("Cd_bz", 0.0), cfg.f("Bp_bz", 0.0)
+    p.clim_relax, p.relax2clim_ac = cfg.clim_relax, cfg.relax2clim_ac
+    p.nsigma, p.Mt = cfg.nsigma, cfg.Mt
+    p.tracer_adv, p.muscl_call = cfg.i("tracer_adv", 1), int(cfg.l("MUSCL_CALL"))
+    p.mom_adv_2D, p.mom_adv_3D, p.vert_adv = cfg.i("mom_adv_2D", 2), cfg.i("mom_adv_3D", 1), cfg.i("vert_adv", 1)
+    p.im_vert_visc_fic, p.fic_point = int(cfg.l("im_vert_visc_fic", True)), cfg.i("fic_point", 1)
+    p.wet_dry_on, p.tf_presence = int(cfg.l("WET_DRY_ON")), int(cfg.l("TF_presence"))
+    p.restart, p.cl_relax = int(cfg.restart), int(cfg.l("cl_relax"))
+    p.precision, p.closure = precision, closure
+    return p
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+class Model:
+    """One FESOM-C model instance resident on the GPU."""
+
+    def __init__(self, cfg: RunConfig, mesh: meshio.RawMesh, precision=FCG_FP64, closure=FCG_CLOSURE_D3END):
+        L = lib()
+        bad = cfg.unsupported()
+        if bad:
+            raise GpuError(f"options outside this build's hot path: {bad}")
+        self.cfg = cfg
+        self._mc = mesh_cfg(cfg)
+        xy = np.ascontiguousarray(mesh.xy, dtype=np.float64).reshape(-1)
+        ind = np.ascontiguousarray(mesh.index, dtype=np.int32)
+        el = np.ascontiguousarray(mesh.elem, dtype=np.int32).reshape(-1)
+        dep = np.ascontiguousarray(mesh.depth, dtype=np.float64)
+        ob = None if mesh.ob is None else np.ascontiguousarray(mesh.ob, dtype=np.float64).reshape(-1)
+        nobn = 0 if mesh.ob is None else int(mesh.ob.shape[0])
+        self._mesh = C.c_void_p()
+        rc = L.fcg_mesh_build(C.byref(self._mc), mesh.nod2D, _dp(xy), _ip(ind), mesh.elem2D, _ip(el), _dp(dep), nobn,
+                              None if ob is None else _dp(ob), C.byref(self._mesh))
+        if rc:
+            raise GpuError(f"fcg_mesh_build failed: {ERRORS.get(rc, rc)}")
+        self.view = Mesh()
+        L.fcg_mesh_view(self._mesh, C.byref(self.view))
+        v = self.view
+        self.N, self.E, self.D, self.D_in = v.nod2D, v.elem2D, v.edge2D, v.edge2D_in
+        self.K = v.nsigma
+        self.L = self.K - 1
+        self.par = params(cfg, precision, closure)
+        self.ctx = C.c_void_p()
+        rc = L.fcg_create(C.byref(self.view), C.byref(self.par), C.byref(self.ctx))
+        if rc:
+            msg = self.last_error() if self.ctx else ""
+            raise GpuError(f"fcg_create failed: {ERRORS.get(rc, rc)} {msg}")
+        self.init_tracers(mesh)
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        L = lib()
+        if getattr(self, "ctx", None):
+            L.fcg_destroy(self.ctx)
+            self.ctx = None
+        if getattr(self, "_mesh", None):
+            L.fcg_mesh_free(self._mesh)
+            self._mesh = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self):
+        buf = C.create_string_buffer(512)
+        lib().fcg_last_error(self.ctx, buf, 512)
+        return buf.value.decode()
+
+    def _ck(self, rc, what):
+        if rc:
+            raise GpuError(f"{what}: {ERRORS.get(rc, rc)} {self.last_error()}")
+
+    # -- initial state (initial_state, src/fv_ini.F90:476-546) ------------
+    def init_tracers(self, mesh):
+        cfg = self.cfg
+        T = np.zeros((self.N, self.L))
+        S = np.zeros((self.N, self.L))
+        if cfg.l("TEMP_ON"):
+            T[:] = cfg.f("T_const", 10.0)
+            if cfg.is_LE:
+                T[mesh.xy[:, 0] * (np.pi / 180.0) < 0.0] = 5.0 / 0.28
+                T[~(mesh.xy[:, 0] * (np.pi / 180.0) < 0.0)] = 10.0 / 0.28
+        if cfg.l("SALINITY_ON"):
+            S[:] = cfg.f("S_const", 35.0)
+        self.set_tracers(T, S)
+
+    def set_tracers(self, T=None, S=None):
+        """Set TF (and T_old, Tclim) / SF (and S_old, Sclim); arrays (nod2D, nsigma-1)."""
+        if T is not None:
+            for n in ("TF", "T_old", "Tclim"):
+                self.set(n, T)
+        if S is not None:
+            for n in ("SF", "S_old", "Sclim"):
+                self.set(n, S)
+
+    # -- state transfer (Fortran layout) ----------------------------------
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        self._ck(lib().fcg_set_state(self.ctx, name.encode(), _dp(a), a.size), f"set {name}")
+
+    def get(self, name, count=None):
+        sizes = {"N": self.N, "E": self.E}
+        if count is None:
+            count = self._count(name)
+        out = np.empty(count)
+        self._ck(lib().fcg_get_state(self.ctx, name.encode(), _dp(out), count), f"get {name}")
+        return out
+
+    def _count(self, name):
+        N, E, L, K = self.N, self.E, self.L, self.K
+        node1 = {"eta_n", "eta_n_1", "eta_n_2", "etaAB", "ssh_rhs", "windx", "windy"}
+        el1 = {"U_filt_2D", "V_filt_2D", "mask_wd", "taux", "tauy", "C_d_el", "dmean_n"}
+        el2 = {"U_n_2D", "U_n_1", "U_n_2", "UAB", "U_rhs_2D_3D", "Bar_pr_2D"}
+        nodeL = {"TF", "SF", "T_old", "S_old", "Tclim", "Sclim", "Unode_p", "Vnode_p"}
+        nodeK = {"hpressure", "Kv", "snu", "bt", "Wvel"}
+        if name in node1:
+            return N
+        if name in el1:
+            return E
+        if name in el2:
+            return 2 * E
+        if name == "vel_grad":
+            return 4 * E
+        if name in nodeL:
+            return N * L
+        if name in nodeK:
+            return N * K
+        return E * L
+
+    # -- stepping ---------------------------------------------------------
+    PHASES = ("fcg_advance_clock", "fcg_pressure", "fcg_vertical_mixing", "fcg_compute_vel_rhs",
+              "fcg_barotropic_timestep", "fcg_jacobian", "fcg_update_3d_vel", "fcg_vert_vel_sigma",
+              "fcg_solve_tracers")
+
+    def phase(self, name):
+        self._ck(getattr(lib(), name)(self.ctx), name)
+
+    def step(self, n=1):
+        self._ck(lib().fcg_run(self.ctx, int(n)), "fcg_run")
+
+    def sync(self):
+        self._ck(lib().fcg_sync(self.ctx), "fcg_sync")
+
+    def diagnostics(self):
+        d = Diag()
+        self._ck(lib().fcg_diagnostics(self.ctx, C.byref(d)), "fcg_diagnostics")
+        return d.as_dict()
+
+    def energy(self):
+        return self.diagnostics()["energy"]
+
+
+def mesh_arrays(cfg: RunConfig, mesh: meshio.RawMesh) -> dict:
+    """Run only the host mesh preprocessing (no GPU needed) and return its arrays."""
+    L = lib()
+    mc = mesh_cfg(cfg)
+    xy = np.ascontiguousarray(mesh.xy, dtype=np.float64).reshape(-1)
+    ind = np.ascontiguousarray(mesh.index, dtype=np.int32)
+    el = np.ascontiguousarray(mesh.elem, dtype=np.int32).reshape(-1)
+    dep = np.ascontiguousarray(mesh.depth, dtype=np.float64)
+    ob = None if mesh.ob is None else np.ascontiguousarray(mesh.ob, dtype=np.float64).reshape(-1)
+    nobn = 0 if mesh.ob is None else int(mesh.ob.shape[0])
+    h = C.c_void_p()
+    rc = L.fcg_mesh_build(C.byref(mc), mesh.nod2D, _dp(xy), _ip(ind), mesh.elem2D, _ip(el), _dp(dep), nobn,
+                          None if ob is None else _dp(ob), C.byref(h))
+    if rc:
+        raise GpuError(f"fcg_mesh_build failed: {ERRORS.get(rc, rc)}")
+    v = Mesh()
+    L.fcg_mesh_view(h, C.byref(v))
+    N, E, D, K = v.nod2D, v.elem2D, v.edge2D, v.nsigma
+
+    def I(p, n):
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    out = dict(nod2D=N, elem2D=E, edge2D=D, edge2D_in=v.edge2D_in, nobn=v.nobn,
+               elem_nodes=I(v.elem2D_nodes, 4 * E), edge_nodes=I(v.edge_nodes, 2 * D), edge_tri=I(v.edge_tri, 2 * D),
+               elem_edges=I(v.elem_edges, 4 * E), elem_neighbors=I(v.elem_neighbors, 4 * E),
+               index=I(v.index_nod2D, N), depth=I(v.depth, N), sigma=I(v.sigma, K), area=I(v.area, N),
+               elem_area=I(v.elem_area, E), elem_cos=I(v.elem_cos, E), coriolis=I(v.coriolis, E),
+               w_cv=I(v.w_cv, 4 * E), gradient_sca=I(v.gradient_sca, 8 * E), gradient_vec=I(v.gradient_vec, 8 * E),
+               edge_dxdy=I(v.edge_dxdy, 2 * D), edge_cross_dxdy=I(v.edge_cross_dxdy, 4 * D), a_tp=I(v.a_tp, 2 * N),
+               density_0=v.density_0, scale_area=v.scale_area)
+    if v.nobn:
+        out["in_obn"] = I(v.in_obn, v.nobn)
+        out["ampt"] = I(v.ampt, 12 * v.nobn)
+        out["fazt"] = I(v.fazt, 12 * v.nobn)
+    L.fcg_mesh_free(h)
+    return out

@@ -1,0 +1,205 @@
+/*
+ * fesomc_gpu.h -- C-ABI of the MI355X-native FESOM-C internal time step.
+ *
+ * The reference (rakowsk/fesom-c) has no operator API: its hot path is a set
+ * of argument-less Fortran subroutines that mutate module globals
+ * (o_MESH / o_ARRAYS / o_PARAM, src/fv_var.F90:303-438).  This header is
+ * the boundary a Fortran host binds with ISO_C_BINDING: one entry point per
+ * call site of oce_timestep (src/fv_main.F90:1996-2076) plus set-up,
+ * state transfer, diagnostics and teardown.  All arguments are plain
+ * pointers and sizes; arrays use the reference's Fortran layout
+ * (column-major, e.g. X(nsigma-1, nod2D) has the level index fastest) and
+ * 1-based node/element/edge ids exactly as the Fortran arrays hold them.
+ *
+ * Every function returns 0 on success or a negative FCG_E* code; the text
+ * of the last error is available from fcg_last_error().  Calls on one
+ * context are stream-ordered and asynchronous unless stated otherwise; a
+ * context is not re-entrant (one host thread per context), matching the
+ * reference's single-threaded call sites.
+ */
+#ifndef FESOMC_GPU_H
+#define FESOMC_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FCG_OK 0
+#define FCG_E_ARG (-1)        /* invalid argument / unsupported option      */
+#define FCG_E_MESH (-2)       /* mesh check failed (test_elem, edges)       */
+#define FCG_E_HIP (-3)        /* HIP runtime error                          */
+#define FCG_E_NAN (-4)        /* non-finite state detected                  */
+#define FCG_E_SALINITY (-5)   /* "Negative salinity ..." STOP (fv_tracer.F90:1469-1473) */
+#define FCG_E_TEMP (-6)       /* "Negative temperature ..." STOP (fv_tracer.F90:1479-1482) */
+#define FCG_E_NOGPU (-7)      /* no HIP device: the library does not fall back */
+
+/* ---- precision / closure selectors ------------------------------------ */
+#define FCG_FP64 0            /* plain FP64 drop-in                          */
+#define FCG_MIXED 1           /* FP32 fluxes/tendencies, FP64 elevation, wet/dry, BPG */
+#define FCG_CLOSURE_D3END 0   /* reference closure d3_end (fv_mixing.F90:73-347) */
+#define FCG_CLOSURE_MLP 1     /* learned per-column closure with d3_end fallback */
+
+/*
+ * Run parameters: the o_PARAM values READ_DATA_RUN derives from fv_run.dat
+ * (src/fv_read_run.F90).  Units and meaning follow the reference; time
+ * scalars in days (time_jd0) and seconds (dt, dt_2D).
+ */
+typedef struct fcg_params {
+    double dt, dt_2D;          /* baroclinic / barotropic steps [s]          */
+    double time_jd0;           /* JD0                                        */
+    double Dmin;               /* minimal depth (wet/dry threshold)          */
+    double C_d;                /* bottom drag Cd                             */
+    double z0b_min, z0s_min;   /* roughness lengths (d3_end)                 */
+    double snul;               /* background viscosity                       */
+    double Cd_bz, Bp_bz;       /* buffer-zone drag / BPG factors             */
+    double clim_relax;         /* [1/s], already inverted as fv_read_run.F90:884 */
+    double relax2clim_ac;      /* [1/s], fv_read_run.F90:894                 */
+    int nsigma;                /* number of sigma levels (layers = nsigma-1) */
+    int Mt;                    /* barotropic substeps per internal step      */
+    int tracer_adv;            /* 1 upwind, 2 MUSCL, 4 Miura                 */
+    int muscl_call;            /* 1: tracer_adv=2 runs solve_tracer_muscl    */
+    int mom_adv_2D, mom_adv_3D, vert_adv;
+    int im_vert_visc_fic;      /* 1: momentum_vert_impl_visc_tmp             */
+    int fic_point;             /* 0,1,2 (fv_viscosity.F90:1449-1460)         */
+    int wet_dry_on;
+    int tf_presence;           /* tidal open-boundary elevation              */
+    int restart;               /* hot start: no 10-day tidal ramp            */
+    int cl_relax;              /* OB climatology relaxation                  */
+    int precision;             /* FCG_FP64 | FCG_MIXED                       */
+    int closure;               /* FCG_CLOSURE_D3END | FCG_CLOSURE_MLP        */
+} fcg_params;
+
+/*
+ * Mesh after the reference's own preprocessing (read_mesh_ser, test_elem,
+ * find_edges, find_elem_neighbors, mesh_arrays1/2, SET_SIGMA,
+ * initial_state constants).  Arrays are the reference module arrays with
+ * their Fortran shapes; ids 1-based; -999 marks a missing neighbour.
+ */
+typedef struct fcg_mesh {
+    int nod2D, elem2D, edge2D, edge2D_in, nsigma, nobn;
+    const int *elem2D_nodes;     /* (4, elem2D)                              */
+    const int *edge_nodes;       /* (2, edge2D)                              */
+    const int *edge_tri;         /* (2, edge2D)                              */
+    const int *elem_edges;       /* (4, elem2D)                              */
+    const int *elem_neighbors;   /* (4, elem2D)                              */
+    const int *index_nod2D;      /* (nod2D)                                  */
+    const int *in_obn;           /* (nobn) open-boundary nodes, ascending    */
+    const double *depth;         /* (nod2D)                                  */
+    const double *sigma;         /* (nsigma)                                 */
+    const double *area;          /* (nod2D)                                  */
+    const double *elem_area;     /* (elem2D)                                 */
+    const double *elem_cos;      /* (elem2D)                                 */
+    const double *coriolis;      /* (elem2D)                                 */
+    const double *w_cv;          /* (4, elem2D)                              */
+    const double *gradient_sca;  /* (8, elem2D)                              */
+    const double *gradient_vec;  /* (8, elem2D)                              */
+    const double *edge_dxdy;     /* (2, edge2D)                              */
+    const double *edge_cross_dxdy; /* (4, edge2D)                            */
+    const double *ac;            /* (nod2D) sponge coefficient               */
+    const double *a_tp;          /* (nod2D, 2)                               */
+    const double *ampt, *fazt;   /* (nobn, 12) tidal amplitude / phase [rad] */
+    const double *amp_factor, *phy_factor; /* (12)                           */
+    double density_0;            /* fv_ini.F90:187-199                        */
+    double scale_area;           /* fv_ini.F90:81                             */
+} fcg_mesh;
+
+typedef struct fcg_ctx fcg_ctx;
+
+/* Diagnostics gathered on the device (deterministic reductions). */
+typedef struct fcg_diag {
+    double energy;       /* energy(eout): fv_average_dynamic.F90:476-554     */
+    double volume;       /* sum area*(depth+eta_n)                            */
+    double heat;         /* sum area*Jc*TF  (tracer content)                  */
+    double salt;         /* sum area*Jc*SF                                    */
+    double eta_min, eta_max, T_min, T_max, S_min, S_max;
+    double time_jd;      /* model clock after the last step                   */
+    int n_dt;            /* internal steps taken                              */
+    int n_wet_elem;      /* sum of mask_wd                                    */
+    int mlp_fallback;    /* columns sent back to d3_end by the OOD guard      */
+    int status;          /* first device-side STOP condition (FCG_E_* or 0)   */
+} fcg_diag;
+
+/* ---- set-up -------------------------------------------------------------
+ * fcg_mesh_build: the project's own C++ restatement of the reference mesh
+ * preprocessing (read_mesh_ser ordering src/fv_main.F90:853-893, test_elem
+ * src/fv_mesh_array.F90:1-82, find_edges :395-718, find_elem_neighbors
+ * :791-946, mesh_arrays1/2 :948-1386, SET_SIGMA src/fv_vert_coord.F90:174-195,
+ * initial_state constants src/fv_ini.F90:81-107,187-240), for hosts that do
+ * not run the reference's Fortran set-up.  Inputs are the raw file contents:
+ *   xy(2,nod2D) as nod2d.out holds them, ind(nod2D), elem(4,elem2D) 1-based
+ *   as elem2d.out holds them, depth(nod2D), ob(2*n_harm, nobn) m2.out rows.
+ * cfg carries the scalar switches below (cartesian, Coriolis, sigma law ...).
+ * The mesh object owns its arrays; fcg_mesh_view exposes them as fcg_mesh.
+ */
+typedef struct fcg_mesh_cfg {
+    int cartesian, coriolis_tf, nsigma, ind_sigma, is_LE, T_potential, tf_presence;
+    double lat_cartesian_deg, cyclic_length_deg, pow_sig, lev_bot, lev_sur, T_const, S_const;
+    int harmonics[12];           /* 1 if harmonic j (M2,S2,...,M4) is active  */
+} fcg_mesh_cfg;
+typedef struct fcg_mesh_obj fcg_mesh_obj;
+
+int fcg_mesh_build(const fcg_mesh_cfg *cfg, int nod2D, const double *xy, const int *ind, int elem2D,
+                   const int *elem, const double *depth, int nobn, const double *ob, fcg_mesh_obj **out);
+int fcg_mesh_view(const fcg_mesh_obj *m, fcg_mesh *view);
+void fcg_mesh_free(fcg_mesh_obj *m);
+
+/* Replaces the device part of array_setup + jacobian + initial_state
+ * (src/fv_main.F90:157-168, 320-362): uploads topology/geometry, builds the
+ * CSR gather lists, allocates the device state and initialises it exactly as
+ * the reference does on a cold start.  Fails with FCG_E_NOGPU when no HIP
+ * device is present. */
+int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx **out);
+void fcg_destroy(fcg_ctx *ctx);
+
+/* ---- state transfer (host mirrors <-> device), synchronous -------------
+ * name is the reference array name: eta_n, eta_n_1, eta_n_2, U_n_2D, U_n_1,
+ * U_n_2, U_n, V_n, U_rhsAB, V_rhsAB, Wvel, bt, snu, Kv, TF, SF, T_old,
+ * S_old, Tclim, Sclim, taux, tauy, windx, windy, mask_wd, Jc, Je,
+ * Bar_pru_3D, Bar_prv_3D, Bar_pr_2D, hpressure, U_n_filt, V_n_filt,
+ * U_filt_2D, V_filt_2D, U_rhs, V_rhs, ssh_rhs, etaAB, Av, Z, zbar.
+ * count is the number of doubles the Fortran array holds; the library
+ * transposes between Fortran layout and its device layout. */
+int fcg_set_state(fcg_ctx *ctx, const char *name, const double *host, long count);
+int fcg_get_state(fcg_ctx *ctx, const char *name, double *host, long count);
+/* Scalar state: time_jd (days), lfirst, n_dt. */
+int fcg_set_clock(fcg_ctx *ctx, double time_jd, int lfirst, int n_dt);
+
+/* ---- the hot path: one entry per call site of oce_timestep ------------
+ * fcg_advance_clock:        time_jd += dt/86400; time = time_jd*86400 (src/fv_main.F90:400-401)
+ * fcg_pressure:             call pressure                  (src/fv_main.F90:1996)
+ * fcg_vertical_mixing:      call compute_el2nodes_3D + d3_end (src/fv_main.F90:2006,2025)
+ * fcg_compute_vel_rhs:      call compute_vel_rhs           (src/fv_main.F90:2034)
+ * fcg_barotropic_timestep:  call oce_barotropic_timestep   (src/fv_main.F90:2041), the
+ *                           Mt-substep loop, replayed as one captured HIP graph
+ * fcg_jacobian:             call jacobian                  (src/fv_main.F90:2050)
+ * fcg_update_3d_vel:        call update_3D_vel             (src/fv_main.F90:2057)
+ * fcg_vert_vel_sigma:       call vert_vel_sigma            (src/fv_main.F90:2064)
+ * fcg_solve_tracers:        call solve_tracers             (src/fv_main.F90:2076)
+ * fcg_oce_timestep:         the clock update followed by all of the above, in order
+ */
+int fcg_advance_clock(fcg_ctx *ctx);
+int fcg_pressure(fcg_ctx *ctx);
+int fcg_vertical_mixing(fcg_ctx *ctx);
+int fcg_compute_vel_rhs(fcg_ctx *ctx);
+int fcg_barotropic_timestep(fcg_ctx *ctx);
+int fcg_jacobian(fcg_ctx *ctx);
+int fcg_update_3d_vel(fcg_ctx *ctx);
+int fcg_vert_vel_sigma(fcg_ctx *ctx);
+int fcg_solve_tracers(fcg_ctx *ctx);
+int fcg_oce_timestep(fcg_ctx *ctx);
+/* nsteps internal steps back to back (no host round trips). */
+int fcg_run(fcg_ctx *ctx, int nsteps);
+
+/* ---- diagnostics / control --------------------------------------------- */
+int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out);  /* synchronous */
+int fcg_sync(fcg_ctx *ctx);
+/* Copy of the last error text, NUL-terminated, truncated to len. */
+int fcg_last_error(const fcg_ctx *ctx, char *buf, int len);
+/* Library build identification ("fesomc_gpu <ver> gfx950 ..."). */
+const char *fcg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FESOMC_GPU_H */

@@ -1990,6 +1990,27 @@ int orc_step(orc_t *o) {
     return o->status;
 }
 
+/* One phase of orc_step, in call order (for phase-by-phase comparisons):
+ * 0 clock, 1 pressure, 2 el2nodes+d3_end, 3 compute_vel_rhs,
+ * 4 oce_barotropic_timestep, 5 jacobian, 6 update_3D_vel, 7 vert_vel_sigma,
+ * 8 solve_tracers */
+int orc_phase(orc_t *o, int ph) {
+    if (o->status) return o->status;
+    switch (ph) {
+    case 0: o->n_dt++; o->time_jd = o->time_jd + o->dt / 86400.0; o->time = o->time_jd * 86400.0; break;
+    case 1: pressure(o); break;
+    case 2: el2nodes_3D(o, o->U_n, o->V_n, o->Unode, o->Vnode); if (o->ver_mix == 3) d3_end(o); break;
+    case 3: compute_vel_rhs(o); break;
+    case 4: oce_barotropic_timestep(o); break;
+    case 5: jacobian(o); break;
+    case 6: update_3D_vel(o); break;
+    case 7: vert_vel_sigma(o); break;
+    case 8: o->status = solve_tracers(o); break;
+    default: return -1;
+    }
+    return o->status;
+}
+
 /* energy: src/fv_average_dynamic.F90:476-554 (serial) */
 double orc_energy(const orc_t *o) {
     double eout = 0.0, my_area = 0.0;

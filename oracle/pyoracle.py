@@ -38,6 +38,8 @@ def lib():
         L.orc_create.argtypes = [C.c_char_p, C.c_int, dp, ip, C.c_int, ip, dp, C.c_int, dp]
         L.orc_step.restype = C.c_int
         L.orc_step.argtypes = [P]
+        L.orc_phase.restype = C.c_int
+        L.orc_phase.argtypes = [P, C.c_int]
         L.orc_energy.restype = C.c_double
         L.orc_energy.argtypes = [P]
         L.orc_status.restype = C.c_int
@@ -106,6 +108,11 @@ class Oracle:
             st = lib().orc_step(self.h)
             if st:
                 raise RuntimeError(f"oracle step failed ({st}): {lib().orc_errmsg(self.h).decode()}")
+
+    def phase(self, i):
+        st = lib().orc_phase(self.h, int(i))
+        if st:
+            raise RuntimeError(f"oracle phase {i} failed ({st}): {lib().orc_errmsg(self.h).decode()}")
 
     def energy(self):
         return lib().orc_energy(self.h)
