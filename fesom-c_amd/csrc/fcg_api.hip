@@ -86,36 +86,37 @@ struct Field {
     long levels;   // levels per column (L or K), or components for 2-D vectors
     long ents;     // entities
     bool interleaved;  // device stores (comp, ent) interleaved like Fortran (2,E)
+    bool nodal;        // node field (else element field)
 };
 
 bool find_field(fcg_ctx *c, const char *name, Field &f) {
     const long N = c->m.N, E = c->m.E, L = c->m.L, K = c->m.K;
     DevState &s = c->s;
-    struct Ent { const char *n; double *p; long lev; long ents; bool il; };
+    struct Ent { const char *n; double *p; long lev; bool nodal; bool il; };
     const Ent tab[] = {
-        {"eta_n", s.eta_n, 1, N, false},     {"eta_n_1", s.eta_n_1, 1, N, false}, {"eta_n_2", s.eta_n_2, 1, N, false},
-        {"etaAB", s.etaAB, 1, N, false},     {"ssh_rhs", s.ssh_rhs, 1, N, false}, {"windx", s.windx, 1, N, false},
-        {"windy", s.windy, 1, N, false},     {"U_n_2D", s.U2, 2, E, true},        {"U_n_1", s.U2_1, 2, E, true},
-        {"U_n_2", s.U2_2, 2, E, true},       {"UAB", s.UAB, 2, E, true},          {"U_rhs_2D_3D", s.U_rhs_2D_3D, 2, E, true},
-        {"Bar_pr_2D", s.Bar_pr_2D, 2, E, true}, {"vel_grad", s.vel_grad, 4, E, true},
-        {"U_filt_2D", s.U_filt, 1, E, false}, {"V_filt_2D", s.V_filt, 1, E, false}, {"mask_wd", s.mask_wd, 1, E, false},
-        {"taux", s.taux, 1, E, false},       {"tauy", s.tauy, 1, E, false},       {"C_d_el", s.C_d_el, 1, E, false},
-        {"dmean_n", s.dmean_n, 1, E, false},
-        {"U_n", s.U_n, L, E, false},         {"V_n", s.V_n, L, E, false},         {"U_rhs", s.U_rhs, L, E, false},
-        {"V_rhs", s.V_rhs, L, E, false},     {"U_rhsAB", s.U_rhsAB, L, E, false}, {"V_rhsAB", s.V_rhsAB, L, E, false},
-        {"U_n_filt", s.U_nf, L, E, false},   {"V_n_filt", s.V_nf, L, E, false},   {"Bar_pru_3D", s.Bpu, L, E, false},
-        {"Bar_prv_3D", s.Bpv, L, E, false},  {"Je", s.Je, L, E, false},
-        {"vel_grad_ux", s.gux, L, E, false}, {"vel_grad_uy", s.guy, L, E, false},
-        {"vel_grad_vx", s.gvx, L, E, false}, {"vel_grad_vy", s.gvy, L, E, false},
-        {"TF", s.TF, L, N, false},           {"SF", s.SF, L, N, false},           {"T_old", s.T_old, L, N, false},
-        {"S_old", s.S_old, L, N, false},     {"Tclim", s.Tclim, L, N, false},     {"Sclim", s.Sclim, L, N, false},
-        {"Unode_p", s.Unode_p, L, N, false}, {"Vnode_p", s.Vnode_p, L, N, false},
-        {"hpressure", s.hp, K, N, false},    {"Kv", s.Kv, K, N, false},           {"snu", s.snu, K, N, false},
-        {"bt", s.bt, K, N, false},           {"Wvel", s.W, K, N, false},
+        {"eta_n", s.eta_n, 1, true, false},    {"eta_n_1", s.eta_n_1, 1, true, false}, {"eta_n_2", s.eta_n_2, 1, true, false},
+        {"etaAB", s.etaAB, 1, true, false},    {"ssh_rhs", s.ssh_rhs, 1, true, false}, {"windx", s.windx, 1, true, false},
+        {"windy", s.windy, 1, true, false},    {"U_n_2D", s.U2, 2, false, true},       {"U_n_1", s.U2_1, 2, false, true},
+        {"U_n_2", s.U2_2, 2, false, true},     {"UAB", s.UAB, 2, false, true},         {"U_rhs_2D_3D", s.U_rhs_2D_3D, 2, false, true},
+        {"Bar_pr_2D", s.Bar_pr_2D, 2, false, true}, {"vel_grad", s.vel_grad, 4, false, true},
+        {"U_filt_2D", s.U_filt, 1, false, false}, {"V_filt_2D", s.V_filt, 1, false, false},
+        {"mask_wd", s.mask_wd, 1, false, false}, {"taux", s.taux, 1, false, false},   {"tauy", s.tauy, 1, false, false},
+        {"C_d_el", s.C_d_el, 1, false, false}, {"dmean_n", s.dmean_n, 1, false, false},
+        {"U_n", s.U_n, L, false, false},       {"V_n", s.V_n, L, false, false},       {"U_rhs", s.U_rhs, L, false, false},
+        {"V_rhs", s.V_rhs, L, false, false},   {"U_rhsAB", s.U_rhsAB, L, false, false}, {"V_rhsAB", s.V_rhsAB, L, false, false},
+        {"U_n_filt", s.U_nf, L, false, false}, {"V_n_filt", s.V_nf, L, false, false}, {"Bar_pru_3D", s.Bpu, L, false, false},
+        {"Bar_prv_3D", s.Bpv, L, false, false}, {"Je", s.Je, L, false, false},
+        {"vel_grad_ux", s.gux, L, false, false}, {"vel_grad_uy", s.guy, L, false, false},
+        {"vel_grad_vx", s.gvx, L, false, false}, {"vel_grad_vy", s.gvy, L, false, false},
+        {"TF", s.TF, L, true, false},          {"SF", s.SF, L, true, false},          {"T_old", s.T_old, L, true, false},
+        {"S_old", s.S_old, L, true, false},    {"Tclim", s.Tclim, L, true, false},    {"Sclim", s.Sclim, L, true, false},
+        {"Unode_p", s.Unode_p, L, true, false}, {"Vnode_p", s.Vnode_p, L, true, false},
+        {"hpressure", s.hp, K, true, false},   {"Kv", s.Kv, K, true, false},          {"snu", s.snu, K, true, false},
+        {"bt", s.bt, K, true, false},          {"Wvel", s.W, K, true, false},
     };
     for (const Ent &t : tab)
         if (!std::strcmp(t.n, name)) {
-            f = Field{t.p, t.lev, t.ents, t.il};
+            f = Field{t.p, t.lev, t.nodal ? N : E, t.il, t.nodal};
             return true;
         }
     return false;
@@ -166,6 +167,47 @@ int enqueue_substeps(fcg_ctx *ctx) {
 
 }  // namespace
 
+// Launch a kernel on the context stream, bracketed by probe events when the
+// kernel's name is the probed one.
+#define LAUNCH(ctx, kern, grid, ...)                                                        \
+    do {                                                                                   \
+        bool pr__ = (ctx)->probe_name == #kern;                                            \
+        hipEvent_t a__ = nullptr, b__ = nullptr;                                           \
+        if (pr__) {                                                                        \
+            (void)hipEventCreate(&a__);                                                    \
+            (void)hipEventCreate(&b__);                                                    \
+            (void)hipEventRecord(a__, (ctx)->stream);                                      \
+        }                                                                                  \
+        hipLaunchKernelGGL(fcgk::kern, dim3(grid), dim3(kBlk), 0, (ctx)->stream, __VA_ARGS__); \
+        if (pr__) {                                                                        \
+            (void)hipEventRecord(b__, (ctx)->stream);                                      \
+            (ctx)->probe_ev.emplace_back(a__, b__);                                        \
+        }                                                                                  \
+    } while (0)
+
+extern "C" int fcg_probe(fcg_ctx *ctx, const char *kernel) {
+    if (!ctx) return FCG_E_ARG;
+    ctx->probe_name = kernel ? kernel : "";
+    return FCG_OK;
+}
+
+extern "C" int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches) {
+    if (!ctx || !total_ms || !launches) return FCG_E_ARG;
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    double t = 0.0;
+    for (auto &pr : ctx->probe_ev) {
+        float ms = 0.0f;
+        FCG_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        t += ms;
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    *total_ms = t;
+    *launches = (int)ctx->probe_ev.size();
+    ctx->probe_ev.clear();
+    return FCG_OK;
+}
+
 extern "C" const char *fcg_version(void) { return "fesomc_gpu 0.1 gfx950 fp64+graph"; }
 
 extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx **out) {
@@ -194,59 +236,120 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     m.N = N; m.E = E; m.D = D; m.Din = mesh->edge2D_in; m.K = K; m.L = L; m.nobn = mesh->nobn;
     ctx->density_0 = mesh->density_0;
     ctx->scale_area = mesh->scale_area;
+    // ---- locality order.  Device arrays hold nodes, elements and edges in
+    // Morton (Z-curve) order of their positions, so the neighbour gathers of a
+    // wavefront touch a compact patch of memory.  Every gather list stays sorted
+    // by the reference's ids, so summation order -- and FP64 bits -- are unchanged.
+    auto z = [](int v) { return v > 0 ? v - 1 : -1; };
+    std::vector<int> pn(N), pe(E), pd(D);  // device index -> reference index (0-based)
+    for (int i = 0; i < N; ++i) pn[i] = i;
+    for (int i = 0; i < E; ++i) pe[i] = i;
+    for (int i = 0; i < D; ++i) pd[i] = i;
+    if (mesh->coord_nod2D) {
+        const double *xy = mesh->coord_nod2D;
+        double x0 = xy[0], x1 = xy[0], y0 = xy[1], y1 = xy[1];
+        for (int n = 0; n < N; ++n) {
+            x0 = std::min(x0, xy[2 * n]); x1 = std::max(x1, xy[2 * n]);
+            y0 = std::min(y0, xy[2 * n + 1]); y1 = std::max(y1, xy[2 * n + 1]);
+        }
+        const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+        auto spread = [](uint64_t v) {
+            v &= 0xffffull;
+            v = (v | (v << 8)) & 0x00ff00ffull;
+            v = (v | (v << 4)) & 0x0f0f0f0full;
+            v = (v | (v << 2)) & 0x33333333ull;
+            v = (v | (v << 1)) & 0x55555555ull;
+            return v;
+        };
+        auto key = [&](double x, double y) {
+            uint64_t qx = (uint64_t)std::lround((x - x0) * sx), qy = (uint64_t)std::lround((y - y0) * sy);
+            return spread(qx) | (spread(qy) << 1);
+        };
+        std::vector<uint64_t> kn(N), ke(E), kd(D);
+        for (int n = 0; n < N; ++n) kn[n] = key(xy[2 * n], xy[2 * n + 1]);
+        for (int e = 0; e < E; ++e) {
+            const int *q = mesh->elem2D_nodes + 4 * e;
+            int nq = (q[0] == q[3]) ? 3 : 4;
+            double cx = 0, cy = 0;
+            for (int j = 0; j < nq; ++j) { cx += xy[2 * (q[j] - 1)]; cy += xy[2 * (q[j] - 1) + 1]; }
+            ke[e] = key(cx / nq, cy / nq);
+        }
+        for (int d = 0; d < D; ++d) {
+            int a0 = mesh->edge_nodes[2 * d] - 1, a1 = mesh->edge_nodes[2 * d + 1] - 1;
+            kd[d] = key(0.5 * (xy[2 * a0] + xy[2 * a1]), 0.5 * (xy[2 * a0 + 1] + xy[2 * a1 + 1]));
+        }
+        std::stable_sort(pn.begin(), pn.end(), [&](int i, int j) { return kn[i] < kn[j]; });
+        std::stable_sort(pe.begin(), pe.end(), [&](int i, int j) { return ke[i] < ke[j]; });
+        std::stable_sort(pd.begin(), pd.end(), [&](int i, int j) { return kd[i] < kd[j]; });
+    }
+    std::vector<int> qn(N), qe(E), qd(D);  // reference index -> device index
+    for (int i = 0; i < N; ++i) qn[pn[i]] = i;
+    for (int i = 0; i < E; ++i) qe[pe[i]] = i;
+    for (int i = 0; i < D; ++i) qd[pd[i]] = i;
+    ctx->pn = pn;
+    ctx->pe = pe;
+    auto QN = [&](int v1) { return v1 > 0 ? qn[v1 - 1] : -1; };  // 1-based reference id -> device index
+    auto QE = [&](int v1) { return v1 > 0 ? qe[v1 - 1] : -1; };
+    auto QD = [&](int v1) { return v1 > 0 ? qd[v1 - 1] : -1; };
     // ---- element constants
     std::vector<int4> en(E), nb(E), eled(E), ied(E);
     std::vector<double4> wcv(E), gsx(E), gsy(E), gvx(E), gvy(E);
-    std::vector<double> accB(E), acc_el(E);
-    auto z = [](int v) { return v > 0 ? v - 1 : -1; };
-    for (int e = 0; e < E; ++e) {
+    std::vector<double> accB(E), acc_el(E), v_ea(E), v_ec(E), v_co(E);
+    for (int i = 0; i < E; ++i) {
+        const int e = pe[i];
         const int *q = mesh->elem2D_nodes + 4 * e;
-        en[e] = make_int4(q[0] - 1, q[1] - 1, q[2] - 1, q[3] - 1);
+        en[i] = make_int4(QN(q[0]), QN(q[1]), QN(q[2]), QN(q[3]));
         const int *b = mesh->elem_neighbors + 4 * e;
-        nb[e] = make_int4(z(b[0]), z(b[1]), z(b[2]), z(b[3]));
+        nb[i] = make_int4(QE(b[0]), QE(b[1]), QE(b[2]), QE(b[3]));
         const int *d = mesh->elem_edges + 4 * e;
-        eled[e] = make_int4(d[0] - 1, d[1] - 1, d[2] - 1, d[3] - 1);
+        eled[i] = make_int4(QD(d[0]), QD(d[1]), QD(d[2]), QD(d[3]));
         const double *w = mesh->w_cv + 4 * e;
-        wcv[e] = make_double4(w[0], w[1], w[2], w[3]);
+        wcv[i] = make_double4(w[0], w[1], w[2], w[3]);
         const double *g = mesh->gradient_sca + 8 * e;
-        gsx[e] = make_double4(g[0], g[1], g[2], g[3]);
-        gsy[e] = make_double4(g[4], g[5], g[6], g[7]);
+        gsx[i] = make_double4(g[0], g[1], g[2], g[3]);
+        gsy[i] = make_double4(g[4], g[5], g[6], g[7]);
         const double *v = mesh->gradient_vec + 8 * e;
-        gvx[e] = make_double4(v[0], v[1], v[2], v[3]);
-        gvy[e] = make_double4(v[4], v[5], v[6], v[7]);
+        gvx[i] = make_double4(v[0], v[1], v[2], v[3]);
+        gvy[i] = make_double4(v[4], v[5], v[6], v[7]);
         double acc = w[0] * mesh->ac[q[0] - 1];
         acc = acc + w[1] * mesh->ac[q[1] - 1];
         acc = acc + w[2] * mesh->ac[q[2] - 1];
         acc = acc + w[3] * mesh->ac[q[3] - 1];
-        acc_el[e] = acc;
-        accB[e] = std::pow(acc, p.Bp_bz) * 1.0;  // mask_bpr = 1 (Mask_Bar_pr = .false., src/fv_var.F90:57)
-        ied[e] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+        acc_el[i] = acc;
+        accB[i] = std::pow(acc, p.Bp_bz) * 1.0;  // mask_bpr = 1 (Mask_Bar_pr = .false., src/fv_var.F90:57)
+        ied[i] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+        v_ea[i] = mesh->elem_area[e];
+        v_ec[i] = mesh->elem_cos[e];
+        v_co[i] = mesh->coriolis[e];
     }
-    // ---- edge constants + gathers
+    // ---- edge constants + gathers (lists in ascending reference edge id)
     std::vector<int2> ednodes(D), edtri(D);
     std::vector<double4> ecross(D);
     std::vector<double2> edxdy(D);
+    for (int i = 0; i < D; ++i) {
+        const int d = pd[i];
+        ednodes[i] = make_int2(QN(mesh->edge_nodes[2 * d]), QN(mesh->edge_nodes[2 * d + 1]));
+        edtri[i] = make_int2(QE(mesh->edge_tri[2 * d]), QE(mesh->edge_tri[2 * d + 1]));
+        const double *c = mesh->edge_cross_dxdy + 4 * d;
+        ecross[i] = make_double4(c[0], c[1], c[2], c[3]);
+        edxdy[i] = make_double2(mesh->edge_dxdy[2 * d], mesh->edge_dxdy[2 * d + 1]);
+    }
     std::vector<std::vector<int>> n2e(N);
     std::vector<int> icnt(E, 0);
-    for (int d = 0; d < D; ++d) {
-        int a = mesh->edge_nodes[2 * d] - 1, b = mesh->edge_nodes[2 * d + 1] - 1;
-        int t1 = z(mesh->edge_tri[2 * d]), t2 = z(mesh->edge_tri[2 * d + 1]);
-        ednodes[d] = make_int2(a, b);
-        edtri[d] = make_int2(t1, t2);
-        const double *c = mesh->edge_cross_dxdy + 4 * d;
-        ecross[d] = make_double4(c[0], c[1], c[2], c[3]);
-        edxdy[d] = make_double2(mesh->edge_dxdy[2 * d], mesh->edge_dxdy[2 * d + 1]);
-        n2e[a].push_back(d);
-        n2e[b].push_back(~d);
+    for (int d = 0; d < D; ++d) {  // reference order
+        const int dv = qd[d];
+        const int a = QN(mesh->edge_nodes[2 * d]), b = QN(mesh->edge_nodes[2 * d + 1]);
+        n2e[a].push_back(dv);
+        n2e[b].push_back(~dv);
         if (d < m.Din) {
-            int *s1 = &ied[t1].x, *s2 = &ied[t2].x;
-            if (icnt[t1] >= 4 || icnt[t2] >= 4) {
-                fcg_fail(ctx, FCG_E_MESH, "element with more than 4 internal edges");
+            const int t1 = QE(mesh->edge_tri[2 * d]), t2 = QE(mesh->edge_tri[2 * d + 1]);
+            if (t1 < 0 || t2 < 0 || icnt[t1] >= 4 || icnt[t2] >= 4) {
+                fcg_fail(ctx, FCG_E_MESH, "internal edge without two elements, or element with more than 4 edges");
                 *out = ctx;
                 return FCG_E_MESH;
             }
-            s1[icnt[t1]++] = ~d;  // element is edge_tri(1): receives -flux
-            s2[icnt[t2]++] = d;   // element is edge_tri(2): receives +flux
+            (&ied[t1].x)[icnt[t1]++] = ~dv;  // element is edge_tri(1): receives -flux
+            (&ied[t2].x)[icnt[t2]++] = dv;   // element is edge_tri(2): receives +flux
         }
     }
     std::vector<int> n2e_ptr(N + 1, 0), n2e_idx;
@@ -255,10 +358,10 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
         n2e_idx.insert(n2e_idx.end(), n2e[n].begin(), n2e[n].end());
     }
     std::vector<std::vector<int>> n2el(N);
-    for (int e = 0; e < E; ++e) {
+    for (int e = 0; e < E; ++e) {  // reference order: nod_in_elem2D ascending element id
         const int *q = mesh->elem2D_nodes + 4 * e;
         int nq = (q[0] == q[3]) ? 3 : 4;
-        for (int j = 0; j < nq; ++j) n2el[q[j] - 1].push_back(e);
+        for (int j = 0; j < nq; ++j) n2el[QN(q[j])].push_back(qe[e]);
     }
     std::vector<int> n2el_ptr(N + 1, 0), n2el_idx;
     for (int n = 0; n < N; ++n) {
@@ -266,11 +369,19 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
         n2el_idx.insert(n2el_idx.end(), n2el[n].begin(), n2el[n].end());
     }
     // ---- node constants
-    std::vector<double> depth(mesh->depth, mesh->depth + N), area(mesh->area, mesh->area + N), ac(mesh->ac, mesh->ac + N);
-    std::vector<double> atp1(mesh->a_tp, mesh->a_tp + N), atp2(mesh->a_tp + N, mesh->a_tp + 2 * N);
-    std::vector<int> index(mesh->index_nod2D, mesh->index_nod2D + N), obidx(N, -1);
+    std::vector<double> depth(N), area(N), ac(N), atp1(N), atp2(N);
+    std::vector<int> index(N), obidx(N, -1);
+    for (int i = 0; i < N; ++i) {
+        const int n = pn[i];
+        depth[i] = mesh->depth[n];
+        area[i] = mesh->area[n];
+        ac[i] = mesh->ac[n];
+        atp1[i] = mesh->a_tp[n];
+        atp2[i] = mesh->a_tp[N + n];
+        index[i] = mesh->index_nod2D[n];
+    }
     for (int i = 0; i < mesh->nobn; ++i) {
-        obidx[mesh->in_obn[i] - 1] = i;
+        obidx[QN(mesh->in_obn[i])] = i;
         ctx->in_obn0.push_back(mesh->in_obn[i] - 1);
     }
     std::vector<double> sigma(mesh->sigma, mesh->sigma + K), dsig(L), oms(K);
@@ -288,8 +399,6 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
 #define UP(dst, src) if ((rc = upload(ctx, &(dst), src))) { *out = ctx; return rc; }
     UP(m.en, en) UP(m.elnb, nb) UP(m.eledges, eled) UP(m.el_ied, ied) UP(m.wcv, wcv) UP(m.gsx, gsx) UP(m.gsy, gsy)
     UP(m.gvx, gvx) UP(m.gvy, gvy) UP(m.accB, accB) UP(m.acc_el, acc_el)
-    std::vector<double> v_ea(mesh->elem_area, mesh->elem_area + E), v_ec(mesh->elem_cos, mesh->elem_cos + E),
-        v_co(mesh->coriolis, mesh->coriolis + E);
     UP(m.elem_area, v_ea) UP(m.elem_cos, v_ec) UP(m.coriolis, v_co)
     UP(m.ednodes, ednodes) UP(m.edtri, edtri) UP(m.ecross, ecross) UP(m.edxdy, edxdy)
     UP(m.depth, depth) UP(m.area, area) UP(m.ac, ac) UP(m.atp1, atp1) UP(m.atp2, atp2) UP(m.index, index)
@@ -395,21 +504,22 @@ static int transfer(fcg_ctx *ctx, const char *name, double *host, long count, bo
     if (count != f.levels * f.ents)
         return fcg_fail(ctx, FCG_E_ARG, std::string("size mismatch for ") + name);
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
-    if (f.interleaved || f.levels == 1) {
-        FCG_CHECK(hipMemcpy(to_dev ? (void *)f.dev : (void *)host, to_dev ? (void *)host : (void *)f.dev,
-                            sizeof(double) * count, to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
-        return FCG_OK;
-    }
-    // Fortran (levels, ents) column-major  <->  device planes [level][ent]
+    // device entity i holds reference entity perm[i]
+    const std::vector<int> &perm = f.nodal ? ctx->pn : ctx->pe;
     std::vector<double> tmp((size_t)count);
+    // Fortran (levels, ents) column-major  <->  device: planes [level][ent], or
+    // interleaved [ent][comp] for the (2,E)/(4,E) 2-D vectors
+    auto dev_at = [&](long lev, long i) -> size_t {
+        return f.interleaved ? (size_t)i * f.levels + lev : (size_t)lev * f.ents + i;
+    };
     if (to_dev) {
-        for (long e = 0; e < f.ents; ++e)
-            for (long k = 0; k < f.levels; ++k) tmp[(size_t)k * f.ents + e] = host[(size_t)e * f.levels + k];
+        for (long i = 0; i < f.ents; ++i)
+            for (long k = 0; k < f.levels; ++k) tmp[dev_at(k, i)] = host[(size_t)perm[i] * f.levels + k];
         FCG_CHECK(hipMemcpy(f.dev, tmp.data(), sizeof(double) * count, hipMemcpyHostToDevice));
     } else {
         FCG_CHECK(hipMemcpy(tmp.data(), f.dev, sizeof(double) * count, hipMemcpyDeviceToHost));
-        for (long e = 0; e < f.ents; ++e)
-            for (long k = 0; k < f.levels; ++k) host[(size_t)e * f.levels + k] = tmp[(size_t)k * f.ents + e];
+        for (long i = 0; i < f.ents; ++i)
+            for (long k = 0; k < f.levels; ++k) host[(size_t)perm[i] * f.levels + k] = tmp[dev_at(k, i)];
     }
     return FCG_OK;
 }
@@ -441,9 +551,8 @@ extern "C" int fcg_advance_clock(fcg_ctx *ctx) {
 extern "C" int fcg_pressure(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
     const double gd0 = kG / ctx->density_0;
-    hipLaunchKernelGGL(fcgk::k_pressure, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, gd0,
-                       ctx->density_0);
-    hipLaunchKernelGGL(fcgk::k_bpg, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
+    LAUNCH(ctx, k_pressure, nblk(ctx->m.N), ctx->m, ctx->s, gd0, ctx->density_0);
+    LAUNCH(ctx, k_bpg, nblk(ctx->m.E), ctx->m, ctx->s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -452,8 +561,7 @@ extern "C" int fcg_vertical_mixing(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
-    hipLaunchKernelGGL(fcgk::k_d3end, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, p.dt, p.Dmin,
-                       p.z0b_min, p.z0s_min, p.snul, ctx->density_0);
+    LAUNCH(ctx, k_d3end, nblk(ctx->m.N), ctx->m, ctx->s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -466,10 +574,9 @@ extern "C" int fcg_compute_vel_rhs(fcg_ctx *ctx) {
         ff = 1.0;
         ctx->lfirst = 0;
     }
-    hipLaunchKernelGGL(fcgk::k_velgrad3d, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
-    hipLaunchKernelGGL(fcgk::k_velrhs, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, p.dt, ff,
-                       ctx->density_0, p.Cd_bz, p.fic_point, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0,
-                       p.im_vert_visc_fic ? 1 : 0);
+    LAUNCH(ctx, k_velgrad3d, nblk(ctx->m.E), ctx->m, ctx->s);
+    LAUNCH(ctx, k_velrhs, nblk(ctx->m.E), ctx->m, ctx->s, p.dt, ff, ctx->density_0, p.Cd_bz, p.fic_point,
+           p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, p.im_vert_visc_fic ? 1 : 0);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -499,23 +606,22 @@ extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
 
 extern "C" int fcg_jacobian(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    hipLaunchKernelGGL(fcgk::k_jac_node, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s, ctx->p.Dmin);
-    hipLaunchKernelGGL(fcgk::k_je, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
+    LAUNCH(ctx, k_jac_node, nblk(ctx->m.N), ctx->m, ctx->s, ctx->p.Dmin);
+    LAUNCH(ctx, k_je, nblk(ctx->m.E), ctx->m, ctx->s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
 
 extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    hipLaunchKernelGGL(fcgk::k_update3d, dim3(nblk(ctx->m.E)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s,
-                       ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
+    LAUNCH(ctx, k_update3d, nblk(ctx->m.E), ctx->m, ctx->s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
 
 extern "C" int fcg_vert_vel_sigma(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    hipLaunchKernelGGL(fcgk::k_vertvel, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, ctx->s);
+    LAUNCH(ctx, k_vertvel, nblk(ctx->m.N), ctx->m, ctx->s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -530,9 +636,8 @@ extern "C" int fcg_solve_tracers(fcg_ctx *ctx) {
     hipLaunchKernelGGL(fcgk::k_minmax_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, (const double *)s.TF,
                        (const double *)s.SF, NL, s.red);
     hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal);
-    hipLaunchKernelGGL(fcgk::k_tr_ab, dim3(nblk(NL)), dim3(kBlk), 0, ctx->stream, s, NL);
-    hipLaunchKernelGGL(fcgk::k_tr_upwind, dim3(nblk(m.N)), dim3(kBlk), 0, ctx->stream, m, s, p.dt, p.Dmin,
-                       p.cl_relax ? 1 : 0, p.clim_relax, p.relax2clim_ac);
+    LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
+    LAUNCH(ctx, k_tr_upwind, nblk(m.N), m, s, p.dt, p.Dmin, p.cl_relax ? 1 : 0, p.clim_relax, p.relax2clim_ac);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }

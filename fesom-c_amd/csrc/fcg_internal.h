@@ -100,7 +100,8 @@ struct fcg_ctx {
     double time_jd = 0, time = 0;
     int n_dt = 0, lfirst = 1;
     // host copies needed for tides and the Fortran-layout transfer
-    std::vector<int> in_obn0;            // 0-based OB nodes
+    std::vector<int> in_obn0;            // 0-based OB nodes (reference ids)
+    std::vector<int> pn, pe;             // device index -> reference index (nodes, elements)
     std::vector<double> ampt, fazt, amp_factor, phy_factor; // (nobn,12), (12)
     std::vector<double> tide_host;       // Mt*nobn staging (pinned below)
     double *tide_pinned = nullptr;
@@ -108,6 +109,8 @@ struct fcg_ctx {
     hipGraphExec_t bt_graph = nullptr;
     bool bt_graph_ok = false;
     int red_blocks = 0;
+    std::string probe_name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev;
     std::vector<void *> allocs;
     std::string err;
 };

@@ -47,7 +47,15 @@ __device__ double densityJM(double t, double s, double pz, double rho0) {
     return rhopot * bulk / (bulk + 0.1 * pz) - rho0;
 }
 
-#define TID() (blockIdx.x * blockDim.x + threadIdx.x)
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin
+// to the 8 XCDs; remap so each XCD (own L2) walks one contiguous range of the
+// locality-ordered entities instead of every 8th block.
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    const unsigned q = G / 8u, r = G % 8u, x = b % 8u;
+    return x * q + (x < r ? x : r) + b / 8u;
+}
+#define TID() (xcd_block() * blockDim.x + threadIdx.x)
 #define PL(arr, k, n, stride) (arr)[(size_t)(k) * (stride) + (n)]
 
 // ---------------------------------------------------------------------------
@@ -1000,7 +1008,7 @@ __global__ void k_tr_upwind(DevMesh m, DevState s, double dt, double Dmin, int c
         for (int j = p0; j < p1; ++j) {
             int cc = m.n2e[j];
             int ed = cc >= 0 ? cc : ~cc;
-            if (ed < m.Din) continue;
+            if (m.edtri[ed].y >= 0) continue;  // internal edge
             const int2 nd = m.ednodes[ed];
             const int e1 = m.edtri[ed].x;
             const double4 cd = m.ecross[ed];

@@ -550,6 +550,7 @@ extern "C" int fcg_mesh_view(const fcg_mesh_obj *m, fcg_mesh *v) {
     v->fazt = m->fazt.data();
     v->amp_factor = m->amp_factor.data();
     v->phy_factor = m->phy_factor.data();
+    v->coord_nod2D = m->xy.data();
     v->density_0 = m->density_0;
     v->scale_area = m->scale_area;
     return FCG_OK;

@@ -29,7 +29,7 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_get_state", "fcg_set_clock", "fcg_advance_clock", "fcg_pressure", "fcg_vertical_mixing",
            "fcg_compute_vel_rhs", "fcg_barotropic_timestep", "fcg_jacobian", "fcg_update_3d_vel",
            "fcg_vert_vel_sigma", "fcg_solve_tracers", "fcg_oce_timestep", "fcg_run", "fcg_diagnostics",
-           "fcg_sync", "fcg_last_error", "fcg_version"]
+           "fcg_sync", "fcg_last_error", "fcg_version", "fcg_probe", "fcg_probe_read"]
 
 
 class Params(C.Structure):
@@ -50,7 +50,7 @@ class Mesh(C.Structure):
                                    "index_nod2D", "in_obn")] + \
                [(n, _DP) for n in ("depth", "sigma", "area", "elem_area", "elem_cos", "coriolis", "w_cv",
                                    "gradient_sca", "gradient_vec", "edge_dxdy", "edge_cross_dxdy", "ac", "a_tp",
-                                   "ampt", "fazt", "amp_factor", "phy_factor")] + \
+                                   "ampt", "fazt", "amp_factor", "phy_factor", "coord_nod2D")] + \
                [("density_0", C.c_double), ("scale_area", C.c_double)]
 
 
@@ -107,6 +107,10 @@ def lib():
         L.fcg_diagnostics.argtypes = [P, C.POINTER(Diag)]
         L.fcg_last_error.restype = C.c_int
         L.fcg_last_error.argtypes = [P, C.c_char_p, C.c_int]
+        L.fcg_probe.restype = C.c_int
+        L.fcg_probe.argtypes = [P, C.c_char_p]
+        L.fcg_probe_read.restype = C.c_int
+        L.fcg_probe_read.argtypes = [P, _DP, _IP]
         L.fcg_version.restype = C.c_char_p
         L.fcg_version.argtypes = []
         _lib = L
@@ -301,6 +305,15 @@ class Model:
 
     def energy(self):
         return self.diagnostics()["energy"]
+
+    def probe(self, kernel):
+        self._ck(lib().fcg_probe(self.ctx, (kernel or "").encode()), "fcg_probe")
+
+    def probe_read(self):
+        t = C.c_double()
+        n = C.c_int()
+        self._ck(lib().fcg_probe_read(self.ctx, C.byref(t), C.byref(n)), "fcg_probe_read")
+        return t.value, n.value
 
 
 def mesh_arrays(cfg: RunConfig, mesh: meshio.RawMesh) -> dict:

@@ -228,7 +228,7 @@ def config(name, seed=0, scale=None):
         return fv_run("c1", 2.0, 20.0, 10, tracer_adv=1, wetdry=False), m, {}
     if name in ("c2", "c2_small"):
         if name == "c2":
-            nx, ny = 1415, 708
+            nx, ny = scale or (1415, 708)
         else:
             nx, ny = (scale or (61, 31))
         m = channel(nx, ny, split=0.9, seed=seed, intertidal=0.15, depth_west=30.0, depth_dry=-2.0)
@@ -236,7 +236,7 @@ def config(name, seed=0, scale=None):
         dt_2D = min(2.0, round(0.45 * dx / math.sqrt(9.8 * 34.0), 2))
         dt = 10 * dt_2D
         nsig = 24 if name == "c2" else 8
-        txt = fv_run(name, dt_2D, dt, nsig, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.05)
+        txt = fv_run(name, dt_2D, dt, nsig, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.25)
         T, S = stratified_ts(m, nsig, seed=seed)
         rng = np.random.default_rng(seed + 7)
         spd, ang = rng.uniform(0.0, 10.0), rng.uniform(0.0, 2 * np.pi)

@@ -100,6 +100,8 @@ typedef struct fcg_mesh {
     const double *a_tp;          /* (nod2D, 2)                               */
     const double *ampt, *fazt;   /* (nobn, 12) tidal amplitude / phase [rad] */
     const double *amp_factor, *phy_factor; /* (12)                           */
+    const double *coord_nod2D;   /* (2, nod2D) radians; optional: used only to
+                                    order the device arrays for locality     */
     double density_0;            /* fv_ini.F90:187-199                        */
     double scale_area;           /* fv_ini.F90:81                             */
 } fcg_mesh;
@@ -196,6 +198,13 @@ int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out);  /* synchronous */
 int fcg_sync(fcg_ctx *ctx);
 /* Copy of the last error text, NUL-terminated, truncated to len. */
 int fcg_last_error(const fcg_ctx *ctx, char *buf, int len);
+/* Kernel timing probe (measurement hook; no counterpart in the reference):
+ * bracket every launch of the named device kernel (e.g. "k_d3end") with HIP
+ * events on the context's stream.  Kernels inside the captured barotropic
+ * graph are not probed.  name NULL or "" disables.  fcg_probe_read is
+ * synchronous, returns the summed event time and launch count, and resets. */
+int fcg_probe(fcg_ctx *ctx, const char *kernel);
+int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches);
 /* Library build identification ("fesomc_gpu <ver> gfx950 ..."). */
 const char *fcg_version(void);
 

@@ -105,16 +105,21 @@ def test_runs_are_bitwise_deterministic():
         assert np.array_equal(a, b)
 
 
-def test_negative_salinity_stop_is_reported():
-    """The reference STOPs on S<0 (src/fv_tracer.F90:1469-1473); the device
-    raises the same condition as a status code (forced from the host)."""
+def test_negative_temperature_stop_is_reported():
+    """The reference STOPs when T<-3 after the tracer step
+    (src/fv_tracer.F90:1478-1482); the device raises the same condition as a
+    status code.  Forced from the host with one cold cell; the oracle stops too."""
     from fesomc_amd import meshgen
     txt, mesh, ex = meshgen.config("c1")
-    g, _o = _model(txt, mesh, ex)
-    S = np.full((g.N, g.L), 30.0)
-    S[100, 3] = -5.0
-    g.set("SF", S)
+    g, o = _model(txt, mesh, ex)
+    T = np.full((g.N, g.L), 10.0)
+    T[100, 3] = -3.5
+    for m in (g, o):
+        m.set("TF", T)
+        m.set("T_old", T)
     g.step(1)
     d = g.diagnostics()
-    assert d["status"] == -5
+    assert d["status"] == -6, d
+    with pytest.raises(RuntimeError, match="Negative temperature"):
+        o.step(1)
     g.close()
