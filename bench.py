@@ -32,20 +32,23 @@ def kernel_bytes(k, N, E, D, L, K):
     d8, i4 = 8, 4
     LN, LE, KN, KE = L * N, L * E, K * N, K * E
     mesh_e = E * (4 * i4 + 4 * d8)          # elem nodes + w_cv
+    edge_geo = i4 * (N + 1) + i4 * 2 * D + D * (8 + 8 + 32 + 16)   # n2e CSR, ednodes, edtri, ecross, edxdy
+    if k == "k_tr_muscl":
+        rd = edge_geo + d8 * E + N * (8 + 8 + 4) + d8 * (2 * LN + 4 * LN + 5 * LE + KN + 2 * LN + KN)
+        return rd + d8 * 4 * LN      # reads: ab T/S, node gradients, U/V(filt), Je, W, T/S, Kv; writes T/S, T/S_old
+    if k == "k_tr_gradn_muscl":
+        return i4 * (N + 1) + i4 * 4 * E + E * (8 + 16 + 64) + d8 * 2 * LN + d8 * 4 * LN
     if k == "k_tr_upwind":
         rd = (i4 * (N + 1) + i4 * 2 * D + 2 * i4 * D * 2 + 4 * d8 * D + 3 * d8 * N + i4 * N
               + d8 * (2 * LN) + d8 * (3 * LE) + d8 * KN + d8 * (2 * LN) + d8 * KN)
-        wr = d8 * 4 * LN + d8 * 3 * LN  # TF,SF,T_old,S_old + Thomas scratch (written, read back)
-        return rd + wr + d8 * 3 * LN
+        return rd + d8 * 4 * LN       # writes TF,SF,T_old,S_old
     if k == "k_velrhs":
         rd = (mesh_e + E * 8 * d8 + 3 * d8 * E + 4 * i4 * E + d8 * N * 4 + d8 * (2 * LE + 2 * LE + 2 * LE + 4 * LE)
               + d8 * KN * 2 + i4 * D * 4 + d8 * D * 6 + d8 * E * 4)
-        wr = d8 * 4 * LE + d8 * 3 * KE
-        return rd + wr + d8 * 3 * KE
+        return rd + d8 * 4 * LE       # writes U_n, V_n, U_rhsAB, V_rhsAB
     if k == "k_d3end":
         rd = i4 * (N + 1) + i4 * 4 * E + d8 * E + d8 * 2 * LE + d8 * 2 * LN + d8 * 3 * KN + d8 * 5 * N
-        wr = d8 * 3 * KN + d8 * 7 * KN   # bt, snu, Kv + column scratch
-        return rd + wr
+        return rd + d8 * 3 * KN       # writes bt, snu, Kv
     if k == "k_vertvel":
         return i4 * (N + 1) + i4 * 2 * D + i4 * 2 * D + d8 * 4 * D + d8 * N + d8 * 3 * LE + d8 * 2 * E + d8 * KN
     if k == "k_bpg":
@@ -66,7 +69,6 @@ def build_model(name, scale=None, precision=0, closure=0):
     from fesomc_amd import gpu, meshgen
     from fesomc_amd.runconfig import RunConfig
     txt, mesh, ex = meshgen.config(name, scale=scale)
-    txt = txt.replace("tracer_adv = 2", "tracer_adv = 1").replace("MUSCL_CALL = T", "MUSCL_CALL = F")
     cfg = RunConfig(txt)
     model = gpu.Model(cfg, mesh, precision=precision, closure=closure)
     if ex.get("T") is not None:
@@ -90,7 +92,6 @@ def cpu_baseline(budget_s=15.0):
     from oracle.pyoracle import Oracle
     scale = (354, 177)
     txt, mesh, ex = meshgen.config("c2", scale=scale)
-    txt = txt.replace("tracer_adv = 2", "tracer_adv = 1").replace("MUSCL_CALL = T", "MUSCL_CALL = F")
     o = Oracle(txt, mesh)
     o.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
     wx, wy = ex["wind"]
@@ -120,7 +121,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--probe", default="k_tr_upwind")
+    ap.add_argument("--probe", default="k_tr_muscl")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -167,7 +168,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded channel mesh generator, analytic forcing)",
         "config": {"workload": f"{args.config}: channel {N} nodes / {E} elements / {D} edges, nsigma {K}, "
-                               f"Mt {model.par.Mt}, wet/dry, upwind tracers, plain FP64 drop-in",
+                               f"Mt {model.par.Mt}, wet/dry, MUSCL tracers, plain FP64 drop-in",
                    "nodes": N, "elements": E, "layers": L, "Mt": model.par.Mt, "path": "fp64"},
         "roofline": {"kernel": args.probe, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -30,6 +30,11 @@ __global__ void k_minmax_partial(const double *, const double *, size_t, double 
 __global__ void k_minmax_final(const double *, int, double *);
 __global__ void k_tr_ab(DevState, size_t);
 __global__ void k_tr_upwind(DevMesh, DevState, double, double, int, double, double);
+__global__ void k_tr_gradn_muscl(DevMesh, DevState);
+__global__ void k_tr_muscl(DevMesh, DevState, double, double, int, double, double);
+__global__ void k_tr_gradn_miura(DevMesh, DevState);
+__global__ void k_tr_miura(DevMesh, DevState, double, double, double, int, double, double);
+__global__ void k_tr_vdiff(DevMesh, DevState, double);
 __global__ void k_diag_partial(DevMesh, DevState, double, double *);
 __global__ void k_diag_final(const double *, int, double *);
 __global__ void k_mask_count(const double *, int, int *);
@@ -223,7 +228,7 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
         *out = ctx;
         return FCG_E_ARG;
     }
-    if (!(p.tracer_adv == 1 || (p.tracer_adv == 2 && !p.muscl_call)) || p.precision != FCG_FP64 ||
+    if (!(p.tracer_adv == 1 || p.tracer_adv == 2 || p.tracer_adv == 4) || p.precision != FCG_FP64 ||
         p.closure != FCG_CLOSURE_D3END) {
         fcg_fail(ctx, FCG_E_ARG, "option not available in this build (tracer_adv/precision/closure)");
         *out = ctx;
@@ -419,6 +424,15 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     DA(s.vxx, EK) DA(s.vyu, EK) DA(s.vyv, EK)
     DA(s.TF, NL) DA(s.SF, NL) DA(s.T_old, NL) DA(s.S_old, NL) DA(s.Tclim, NL) DA(s.Sclim, NL)
     DA(s.abT, NL) DA(s.abS, NL) DA(s.Unode_p, NL) DA(s.Vnode_p, NL) DA(s.tcp, NL) DA(s.ttp, NL) DA(s.tsp, NL)
+    if (p.tracer_adv == 4 || (p.tracer_adv == 2 && p.muscl_call)) {
+        DA(s.gTx, NL) DA(s.gTy, NL)
+    }
+    if (p.tracer_adv == 4) {
+        DA(s.tcur, NL)
+    }
+    if (p.tracer_adv == 2 && p.muscl_call) {
+        DA(s.gSx, NL) DA(s.gSy, NL)
+    }
     DA(s.hp, NK) DA(s.Kv, NK) DA(s.snu, NK) DA(s.bt, NK) DA(s.W, NK)
     DA(s.mx_p, NK) DA(s.mx_q, NK) DA(s.mx_c2, NK) DA(s.mx_xx, NK) DA(s.mx_yy, NK) DA(s.mx_bA, NK) DA(s.mx_bB, NK)
     ctx->red_blocks = 512;
@@ -636,8 +650,23 @@ extern "C" int fcg_solve_tracers(fcg_ctx *ctx) {
     hipLaunchKernelGGL(fcgk::k_minmax_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, (const double *)s.TF,
                        (const double *)s.SF, NL, s.red);
     hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal);
-    LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
-    LAUNCH(ctx, k_tr_upwind, nblk(m.N), m, s, p.dt, p.Dmin, p.cl_relax ? 1 : 0, p.clim_relax, p.relax2clim_ac);
+    const int relax = p.cl_relax ? 1 : 0;
+    // scheme dispatch of src/fv_main.F90:1788-1800
+    if (p.tracer_adv == 1) {
+        LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_upwind, nblk(m.N), m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+    } else if (p.tracer_adv == 2 && p.muscl_call) {
+        LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_gradn_muscl, nblk(m.N), m, s);
+        LAUNCH(ctx, k_tr_muscl, nblk(m.N), m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+    } else if (p.tracer_adv == 4) {
+        LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_gradn_miura, nblk(m.N), m, s);
+        LAUNCH(ctx, k_tr_miura, nblk(m.N), m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax,
+               p.relax2clim_ac);
+    } else {
+        LAUNCH(ctx, k_tr_vdiff, nblk(m.N), m, s, p.dt);
+    }
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }

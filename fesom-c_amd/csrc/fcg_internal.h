@@ -80,6 +80,8 @@ struct DevState {
     double *Tclim = nullptr, *Sclim = nullptr, *abT = nullptr, *abS = nullptr;     // L*N
     double *Unode_p = nullptr, *Vnode_p = nullptr;                                 // L*N
     double *tcp = nullptr, *ttp = nullptr, *tsp = nullptr;                         // L*N Thomas scratch
+    double *gTx = nullptr, *gTy = nullptr, *gSx = nullptr, *gSy = nullptr;         // L*N node-averaged tracer gradients (MUSCL/Miura)
+    double *tcur = nullptr;                                                        // L*N de-meaned T before the Miura update
     double *hp = nullptr, *Kv = nullptr, *snu = nullptr, *bt = nullptr, *W = nullptr; // K*N
     double *mx_p = nullptr, *mx_q = nullptr, *mx_c2 = nullptr, *mx_xx = nullptr, *mx_yy = nullptr; // K*N d3_end scratch
     double *mx_bA = nullptr, *mx_bB = nullptr;                                     // K*N

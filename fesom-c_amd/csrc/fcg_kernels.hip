@@ -991,6 +991,77 @@ __device__ __forceinline__ double upw_vert(const DevMesh &m, const DevState &s, 
     return -(T(nz) * (1.0 - dg - db) + T(nz - 1) * db + T(nz - 2) * dg) * Wv(nz);
 }
 
+// tracer_impl_vert_diff (fv_tracer.F90:1319-1494, key_atm off) of one node
+// column, applied to TF/SF in place, followed by the STOP checks of
+// fv_tracer.F90:1466-1485 (reported through the device error flag).
+__device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevState &s, int n, double a, double Ta,
+                                             double Sa, double dt) {
+    const int N = m.N, L = m.L, K = m.K;
+    auto zb = [&](int z) { return m.oms[z - 1] * a; };
+    auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
+    auto KV = [&](int z) { return PL(s.Kv, z - 1, n, N); };
+    auto Td = [&](int z) { return PL(s.TF, z - 1, n, N) - Ta; };
+    auto Sd = [&](int z) { return PL(s.SF, z - 1, n, N) - Sa; };
+    double cp = 0, tp = 0, sp = 0;
+    for (int nz = 1; nz <= L; ++nz) {
+        double aa, bb, cc;
+        if (nz == 1) {
+            cc = -dt * KV(2) / ((Zc(1) - Zc(2)) * (zb(1) - zb(2)));
+            aa = 0.0;
+            bb = -cc + 1.0;
+        } else if (nz == L) {
+            aa = -dt * KV(L) / ((Zc(L - 1) - Zc(L)) * (zb(L) - zb(K)));
+            bb = -aa + 1.0;
+            cc = 0.0;
+        } else {
+            aa = -dt * KV(nz) / ((Zc(nz - 1) - Zc(nz)) * (zb(nz) - zb(nz + 1)));
+            cc = -dt * KV(nz) / ((Zc(nz) - Zc(nz + 1)) * (zb(nz) - zb(nz + 1)));
+            bb = -aa - cc + 1.0;
+        }
+        double tr, sr;
+        if (nz == 1) {
+            tr = cc * (-Td(2) + Td(1));
+            sr = cc * (-Sd(2) + Sd(1));
+        } else if (nz == L) {
+            tr = aa * (Td(L) - Td(L - 1));
+            sr = aa * (Sd(L) - Sd(L - 1));
+        } else {
+            tr = -aa * Td(nz - 1) - cc * Td(nz + 1) + (aa + cc) * Td(nz);
+            sr = -aa * Sd(nz - 1) - cc * Sd(nz + 1) + (aa + cc) * Sd(nz);
+        }
+        if (nz == 1) {
+            cp = cc / bb;
+            tp = tr / bb;
+            sp = sr / bb;
+        } else {
+            double mm = 1.0 / (bb - cp * aa);
+            cp = cc * mm;
+            tp = (tr - tp * aa) * mm;
+            sp = (sr - sp * aa) * mm;
+        }
+        PL(s.tcp, nz - 1, n, N) = cp;
+        PL(s.ttp, nz - 1, n, N) = tp;
+        PL(s.tsp, nz - 1, n, N) = sp;
+    }
+    double trn = tp, srn = sp;
+    int bad = 0;
+    for (int nz = L; nz >= 1; --nz) {
+        if (nz < L) {
+            double c = PL(s.tcp, nz - 1, n, N);
+            trn = PL(s.ttp, nz - 1, n, N) - c * trn;
+            srn = PL(s.tsp, nz - 1, n, N) - c * srn;
+        }
+        double tnew = Td(nz) + trn + Ta;
+        double snew = Sd(nz) + srn + Sa;
+        if (fabs(snew) < 1.e-12) snew = fabs(snew);
+        if (snew < 0.0) bad = bad ? bad : FCG_E_SALINITY;
+        if (tnew < -3.0) bad = bad ? bad : FCG_E_TEMP;
+        PL(s.TF, nz - 1, n, N) = tnew;
+        PL(s.SF, nz - 1, n, N) = snew;
+    }
+    if (bad) atomicCAS(s.err, 0, bad);
+}
+
 // solve_tracer_upwind (fv_tracer.F90:390-923) followed by
 // tracer_impl_vert_diff (fv_tracer.F90:1319-1494), one thread per node column.
 __global__ void k_tr_upwind(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
@@ -1094,70 +1165,380 @@ __global__ void k_tr_upwind(DevMesh m, DevState s, double dt, double Dmin, int c
         tvt = tvb;
         svt = svb;
     }
-    // ---- tracer_impl_vert_diff (key_atm off) ----
+    vdiff_column(m, s, n, a, Ta, Sa, dt);
+}
+
+// Vertical flux through 1-based interface nz for the MUSCL and Miura schemes
+// (fv_tracer.F90:236-272, 1172-1208): QUICK from the upwind side, first
+// order next to the surface/bottom.  The column value is PL(X) - sub (sub =
+// 0 for the AB-interpolated column, the tracer mean for the current one).
+// Tm keeps its previous value when W is exactly zero, as in the reference;
+// the product -Tm*W is then a zero either way.  The two schemes spell the
+// QUICK weights with one or two divisions, hence kChained.
+template <bool kChained>
+__device__ __forceinline__ double hi_vert(const DevMesh &m, const DevState &s, const double *X, double sub, int n,
+                                          int nz, double a, double &Tm) {
+    const int N = m.N, K = m.K;
+    auto Wv = [&](int z) { return PL(s.W, z - 1, n, N); };
+    auto T = [&](int z) { return PL(X, z - 1, n, N) - sub; };
     auto zb = [&](int z) { return m.oms[z - 1] * a; };
     auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
-    auto KV = [&](int z) { return PL(s.Kv, z - 1, n, N); };
-    auto Td = [&](int z) { return PL(s.TF, z - 1, n, N) - Ta; };
-    auto Sd = [&](int z) { return PL(s.SF, z - 1, n, N) - Sa; };
-    double cp = 0, tp = 0, sp = 0;
-    for (int nz = 1; nz <= L; ++nz) {
-        double aa, bb, cc;
-        if (nz == 1) {
-            cc = -dt * KV(2) / ((Zc(1) - Zc(2)) * (zb(1) - zb(2)));
-            aa = 0.0;
-            bb = -cc + 1.0;
-        } else if (nz == L) {
-            aa = -dt * KV(L) / ((Zc(L - 1) - Zc(L)) * (zb(L) - zb(K)));
-            bb = -aa + 1.0;
-            cc = 0.0;
+    if (nz == 1) return -Wv(1) * T(1);
+    if (nz == K) return 0.0;
+    const double w = Wv(nz);
+    if (w > 0.0) {
+        if (nz == K - 1) {
+            Tm = T(nz);
         } else {
-            aa = -dt * KV(nz) / ((Zc(nz - 1) - Zc(nz)) * (zb(nz) - zb(nz + 1)));
-            cc = -dt * KV(nz) / ((Zc(nz) - Zc(nz + 1)) * (zb(nz) - zb(nz + 1)));
-            bb = -aa - cc + 1.0;
+            const double aa = zb(nz) - Zc(nz - 1), bb = Zc(nz) - zb(nz), cc = Zc(nz + 1) - zb(nz);
+            double dg, db;
+            if (kChained) {
+                dg = aa * bb / (cc + aa) / (bb - cc);
+                db = -aa * cc / (bb + aa) / (bb - cc);
+            } else {
+                dg = aa * bb / ((cc + aa) * (bb - cc));
+                db = -aa * cc / ((bb + aa) * (bb - cc));
+            }
+            const double da = 1.0 - dg - db;
+            Tm = T(nz - 1) * da + T(nz) * db + T(nz + 1) * dg;
         }
-        double tr, sr;
-        if (nz == 1) {
-            tr = cc * (-Td(2) + Td(1));
-            sr = cc * (-Sd(2) + Sd(1));
-        } else if (nz == L) {
-            tr = aa * (Td(L) - Td(L - 1));
-            sr = aa * (Sd(L) - Sd(L - 1));
-        } else {
-            tr = -aa * Td(nz - 1) - cc * Td(nz + 1) + (aa + cc) * Td(nz);
-            sr = -aa * Sd(nz - 1) - cc * Sd(nz + 1) + (aa + cc) * Sd(nz);
-        }
-        if (nz == 1) {
-            cp = cc / bb;
-            tp = tr / bb;
-            sp = sr / bb;
-        } else {
-            double mm = 1.0 / (bb - cp * aa);
-            cp = cc * mm;
-            tp = (tr - tp * aa) * mm;
-            sp = (sr - sp * aa) * mm;
-        }
-        PL(s.tcp, nz - 1, n, N) = cp;
-        PL(s.ttp, nz - 1, n, N) = tp;
-        PL(s.tsp, nz - 1, n, N) = sp;
     }
-    double trn = tp, srn = sp;
-    int bad = 0;
-    for (int nz = L; nz >= 1; --nz) {
-        if (nz < L) {
-            double c = PL(s.tcp, nz - 1, n, N);
-            trn = PL(s.ttp, nz - 1, n, N) - c * trn;
-            srn = PL(s.tsp, nz - 1, n, N) - c * srn;
+    if (w < 0.0) {
+        if (nz == 2) {
+            Tm = T(nz - 1);
+        } else {
+            const double aa = Zc(nz) - zb(nz), bb = zb(nz) - Zc(nz - 1), cc = zb(nz) - Zc(nz - 2);
+            double dg, db;
+            if (kChained) {
+                dg = aa * bb / (cc + aa) / (bb - cc);
+                db = -aa * cc / (bb + aa) / (bb - cc);
+            } else {
+                dg = aa * bb / ((cc + aa) * (bb - cc));
+                db = -aa * cc / ((bb + aa) * (bb - cc));
+            }
+            const double da = 1.0 - dg - db;
+            Tm = T(nz) * da + T(nz - 1) * db + T(nz - 2) * dg;
         }
-        double tnew = Td(nz) + trn + Ta;
-        double snew = Sd(nz) + srn + Sa;
-        if (fabs(snew) < 1.e-12) snew = fabs(snew);
-        if (snew < 0.0) bad = bad ? bad : FCG_E_SALINITY;
-        if (tnew < -3.0) bad = bad ? bad : FCG_E_TEMP;
-        PL(s.TF, nz - 1, n, N) = tnew;
-        PL(s.SF, nz - 1, n, N) = snew;
     }
-    if (bad) atomicCAS(s.err, 0, bad);
+    return -Tm * w;
+}
+
+// Net normal barotropic transport into an open-boundary node through its
+// boundary edges (fv_tracer.F90:862-873 and the same block of the other
+// schemes), summed in ascending edge id; decides inflow relaxation vs.
+// replacement by climatology.
+__device__ __forceinline__ double ob_inflow(const DevMesh &m, const DevState &s, int p0, int p1, double Dmin) {
+    double vel_nor = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const int cc = m.n2e[j];
+        const int ed = cc >= 0 ? cc : ~cc;
+        if (m.edtri[ed].y >= 0) continue;  // internal edge
+        const int2 nd = m.ednodes[ed];
+        const int e1 = m.edtri[ed].x;
+        const double4 cd = m.ecross[ed];
+        double dmn = dmaxr(Dmin, 0.5 * (s.eta_n[nd.x] + m.depth[nd.x] + s.eta_n[nd.y] + m.depth[nd.y]));
+        double un1 = (s.V_filt[e1] * cd.x - s.U_filt[e1] * cd.y) * dmn;
+        vel_nor = (cc >= 0) ? vel_nor + un1 / m.area[nd.x] : vel_nor - un1 / m.area[nd.y];
+    }
+    return vel_nor;
+}
+
+// Element gradient of a node column at level k (tracer_gradient_elements,
+// fv_gradients.F90:180-205), the column value being PL(X) - sub.
+__device__ __forceinline__ double2 elem_grad(const DevMesh &m, const double *X, double sub, int el, int k) {
+    const int N = m.N;
+    const int4 q = m.en[el];
+    const double v0 = PL(X, k, q.x, N) - sub, v1 = PL(X, k, q.y, N) - sub;
+    const double v2 = PL(X, k, q.z, N) - sub, v3 = PL(X, k, q.w, N) - sub;
+    return make_double2(sum4(m.gsx[el], v0, v1, v2, v3), sum4(m.gsy[el], v0, v1, v2, v3));
+}
+
+// Node-averaged gradients of the AB-interpolated T and S columns used by the
+// MUSCL reconstruction (fill_up_dn_grad node-average branch,
+// fv_gradients.F90:263-323): area-weighted mean over the node's elements in
+// ascending element id, divided by the summed area at every level.
+__global__ void k_tr_gradn_muscl(DevMesh m, DevState s) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, L = m.L;
+    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    for (int k = 0; k < L; ++k) {
+        double tvol = 0.0, tx = 0.0, ty = 0.0, sx = 0.0, sy = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            const int el = m.n2el[j];
+            const double ea = m.elem_area[el];
+            const double2 gt = elem_grad(m, s.abT, 0.0, el, k);
+            const double2 gs = elem_grad(m, s.abS, 0.0, el, k);
+            tvol = tvol + ea;
+            tx = tx + gt.x * ea;
+            ty = ty + gt.y * ea;
+            sx = sx + gs.x * ea;
+            sy = sy + gs.y * ea;
+        }
+        PL(s.gTx, k, n, N) = tx / tvol;
+        PL(s.gTy, k, n, N) = ty / tvol;
+        PL(s.gSx, k, n, N) = sx / tvol;
+        PL(s.gSy, k, n, N) = sy / tvol;
+    }
+}
+
+// Node-averaged gradient of the current de-meaned T column for the Miura
+// scheme (fv_tracer.F90:984-998): the same mean, with 1/(summed area)
+// formed once per node.  Also stages the de-meaned column itself in tcur:
+// the Miura update overwrites TF while neighbouring columns still read it.
+__global__ void k_tr_gradn_miura(DevMesh m, DevState s) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, L = m.L;
+    const double Ta = s.scal[0];
+    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    double ssum = 0.0;
+    for (int j = p0; j < p1; ++j) ssum = ssum + m.elem_area[m.n2el[j]];
+    const double tvol = 1.0 / ssum;
+    for (int k = 0; k < L; ++k) {
+        double tx = 0.0, ty = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            const int el = m.n2el[j];
+            const double ea = m.elem_area[el];
+            const double2 g = elem_grad(m, s.TF, Ta, el, k);
+            tx = tx + g.x * ea;
+            ty = ty + g.y * ea;
+        }
+        PL(s.gTx, k, n, N) = tx * tvol;
+        PL(s.gTy, k, n, N) = ty * tvol;
+        PL(s.tcur, k, n, N) = PL(s.TF, k, n, N) - Ta;
+    }
+}
+
+// MUSCL edge value: linear reconstructions from both end nodes, 15 %
+// upwind-biased blend (fv_tracer.F90:104-122).
+__device__ __forceinline__ double muscl_mean(double T1, double T2, double dT, double dx, double dy, double g1x,
+                                             double g1y, double g2x, double g2y, bool from2) {
+    const double Tm2 = T2 - (dT + dx * g2x + dy * kREarthD * g2y) / 6.0;
+    const double Tm1 = T1 + (dT + dx * g1x + dy * kREarthD * g1y) / 6.0;
+    return from2 ? 0.15 * Tm2 + 0.85 * (Tm1 + Tm2) / 2.0 : 0.15 * Tm1 + 0.85 * (Tm1 + Tm2) / 2.0;
+}
+
+// solve_tracer_muscl for T and S (fv_tracer.F90:8-379) followed by
+// tracer_impl_vert_diff, one thread per node column.  Per edge the first
+// segment uses the filtered velocity of element 1; the second segment's flux
+// uses the unfiltered U_n/V_n of element 2 (its upwind test and the volume
+// flux use the filtered one), as the reference does.
+__global__ void k_tr_muscl(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
+                           double relax2clim_ac) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, E = m.E, L = m.L;
+    const double Ta = s.scal[0], Sa = s.scal[1];
+    const double a = s.a[n];
+    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const bool ob = cl_relax && m.index[n] == 2;
+    const double vel_nor = ob ? ob_inflow(m, s, p0, p1, Dmin) : 0.0;
+    double TmT = 0.0, TmS = 0.0;
+    double tvt = hi_vert<true>(m, s, s.abT, 0.0, n, 1, a, TmT);
+    double svt = hi_vert<true>(m, s, s.abS, 0.0, n, 1, a, TmS);
+    for (int k = 0; k < L; ++k) {
+        double rt = 0.0, rs = 0.0, rc = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            const int cc = m.n2e[j];
+            const int ed = cc >= 0 ? cc : ~cc;
+            const int2 nd = m.ednodes[ed];
+            const int2 et = m.edtri[ed];
+            const double4 cd = m.ecross[ed];
+            const double2 dd = m.edxdy[ed];
+            const int e1 = et.x;
+            const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
+            const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
+            const double gT1x = PL(s.gTx, k, nd.x, N), gT2x = PL(s.gTx, k, nd.y, N);
+            const double gT1y = PL(s.gTy, k, nd.x, N), gT2y = PL(s.gTy, k, nd.y, N);
+            const double gS1x = PL(s.gSx, k, nd.x, N), gS2x = PL(s.gSx, k, nd.y, N);
+            const double gS1y = PL(s.gSy, k, nd.x, N), gS2y = PL(s.gSy, k, nd.y, N);
+            const double dTt = 2.0 * (T2 - T1), dTs = 2.0 * (S2 - S1);
+            // first segment
+            const double dxa = dd.x * (kREarthD * m.elem_cos[e1]);
+            const double u1 = PL(s.U_nf, k, e1, E), v1 = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
+            const bool f2 = v1 * cd.x - u1 * cd.y > 0;
+            const double mT = muscl_mean(T1, T2, dTt, dxa, dd.y, gT1x, gT1y, gT2x, gT2y, f2);
+            const double mS = muscl_mean(S1, S2, dTs, dxa, dd.y, gS1x, gS1y, gS2x, gS2y, f2);
+            double cT = (J1 * v1 * mT) * cd.x - (J1 * u1 * mT) * cd.y;
+            double cS = (J1 * v1 * mS) * cd.x - (J1 * u1 * mS) * cd.y;
+            double cV = J1 * v1 * cd.x - J1 * u1 * cd.y;
+            if (cc >= 0) {
+                rt = rt + cT;
+                rs = rs + cS;
+            } else {
+                rt = rt - cT;
+                rs = rs - cS;
+            }
+            if (et.y >= 0) {  // second segment
+                const int e2 = et.y;
+                const double dxb = dd.x * (kREarthD * m.elem_cos[e2]);
+                const double u2f = PL(s.U_nf, k, e2, E), v2f = PL(s.V_nf, k, e2, E);
+                const double u2 = PL(s.U_n, k, e2, E), v2 = PL(s.V_n, k, e2, E), J2 = PL(s.Je, k, e2, E);
+                const bool g2 = v2f * cd.z - u2f * cd.w < 0;
+                const double nT = muscl_mean(T1, T2, dTt, dxb, dd.y, gT1x, gT1y, gT2x, gT2y, g2);
+                const double nS = muscl_mean(S1, S2, dTs, dxb, dd.y, gS1x, gS1y, gS2x, gS2y, g2);
+                const double c2T = -(J2 * v2 * nT) * cd.z + (J2 * u2 * nT) * cd.w;
+                const double c2S = -(J2 * v2 * nS) * cd.z + (J2 * u2 * nS) * cd.w;
+                const double c2V = -J2 * v2f * cd.z + J2 * u2f * cd.w;
+                if (cc >= 0) {
+                    rt = rt + c2T;
+                    rs = rs + c2S;
+                    rc = rc + cV;
+                    rc = rc + c2V;
+                } else {
+                    rt = rt - c2T;
+                    rs = rs - c2S;
+                    rc = rc - cV;
+                    rc = rc - c2V;
+                }
+            } else {
+                rc = (cc >= 0) ? rc + cV : rc - cV;
+            }
+        }
+        const double tvb = hi_vert<true>(m, s, s.abT, 0.0, n, k + 2, a, TmT);
+        const double svb = hi_vert<true>(m, s, s.abS, 0.0, n, k + 2, a, TmS);
+        const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
+        const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+        const double jc = a * m.dsig[k];
+        const double aj = m.area[n] * jc;
+        rt = rt / aj - tab * rc / aj + tab * wdiff / jc + (tvt - tvb) / jc;
+        rs = rs / aj - sab * rc / aj + sab * wdiff / jc + (svt - svb) / jc;
+        const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
+        if (ob) {  // climatology relaxation on the current de-meaned tracer
+            if (vel_nor > 0.0) {
+                rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+                rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sdm + Sa));
+            } else {
+                rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+                rs = clim_relax * (PL(s.Sclim, k, n, N) - (sdm + Sa));
+            }
+        }
+        PL(s.T_old, k, n, N) = tdm + Ta;
+        PL(s.S_old, k, n, N) = sdm + Sa;
+        PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
+        PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
+        tvt = tvb;
+        svt = svb;
+    }
+    vdiff_column(m, s, n, a, Ta, Sa, dt);
+}
+
+// mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
+// H_advtr_min = 0 (fv_var.F90:49-51)
+__device__ __forceinline__ double mask_ad(const DevMesh &m, const DevState &s, int n, double Dmin) {
+    const double dm = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+    if (dm > 0.1) return 1.0;
+    if (dm < 0.0) return 0.0;
+    return (dm - 0.0) / (0.1 - 0.0);
+}
+
+// solve_tracer_miura for T (fv_tracer.F90:930-1316) followed by
+// tracer_impl_vert_diff of T and S (S is not advected under this scheme),
+// one thread per node column.  The reconstruction keeps the reference's
+// unit metric factors a = b = 1 and its first-segment y term without *dt.
+__global__ void k_tr_miura(DevMesh m, DevState s, double dt, double Dmin, double scale_area, int cl_relax,
+                           double clim_relax, double relax2clim_ac) {
+    int n = TID();
+    if (n >= m.N) return;
+    const int N = m.N, E = m.E, L = m.L;
+    const double Ta = s.scal[0], Sa = s.scal[1];
+    const double a = s.a[n];
+    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const bool ob = cl_relax && m.index[n] == 2;
+    const double vel_nor = ob ? ob_inflow(m, s, p0, p1, Dmin) : 0.0;
+    const double ab = 1.0;
+    double Tmv = 0.0;
+    double tvt = hi_vert<false>(m, s, s.tcur, 0.0, n, 1, a, Tmv);
+    for (int k = 0; k < L; ++k) {
+        double rt = 0.0, rc = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            const int cc = m.n2e[j];
+            const int ed = cc >= 0 ? cc : ~cc;
+            const int2 nd = m.ednodes[ed];
+            const int2 et = m.edtri[ed];
+            const double4 cd = m.ecross[ed];
+            const double2 dd = m.edxdy[ed];
+            const int e1 = et.x, e2 = et.y;
+            const double crt1 = 0.5 * (mask_ad(m, s, nd.x, Dmin) + mask_ad(m, s, nd.y, Dmin));
+            double Kh = m.elem_area[e1];
+            if (e2 >= 0) Kh = 0.5 * (Kh + m.elem_area[e2]);
+            Kh = 0.010 * Kh / scale_area;
+            const double T1 = PL(s.tcur, k, nd.x, N), T2 = PL(s.tcur, k, nd.y, N);
+            const double tx1 = PL(s.gTx, k, nd.x, N), tx2 = PL(s.gTx, k, nd.y, N);
+            const double ty1 = PL(s.gTy, k, nd.x, N), ty2 = PL(s.gTy, k, nd.y, N);
+            const double2 g1 = elem_grad(m, s.tcur, 0.0, e1, k);
+            double Tx, Ty;
+            double2 g2 = make_double2(0.0, 0.0);
+            if (e2 >= 0) {
+                g2 = elem_grad(m, s.tcur, 0.0, e2, k);
+                Tx = 0.5 * Kh * (g1.x + g2.x);
+                Ty = 0.5 * Kh * (g1.y + g2.y);
+            } else {
+                Tx = Kh * g1.x;
+                Ty = Kh * g1.y;
+            }
+            const double dX1 = cd.x, dY1 = cd.y;
+            const double u = PL(s.U_nf, k, e1, E), v = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
+            double Tm;
+            if (v * dX1 - u * dY1 > 0) {
+                Tm = T2 + (0.5 * (-dd.x * ab + dX1 - u * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY1 - v) * ty2) * crt1;
+            } else {
+                Tm = T1 + (0.5 * (dd.x * ab + dX1 - u * dt) * tx1 + 0.5 * (dd.y * kREarthD + dY1 - v * dt) * ty1) * crt1;
+            }
+            const double c1 = (J1 * v * Tm - Ty) * dX1 - (J1 * u * Tm - Tx) * dY1;
+            const double cV = J1 * v * dX1 - J1 * u * dY1;
+            rt = (cc >= 0) ? rt + c1 : rt - c1;
+            if (e2 >= 0) {
+                const double dX2 = cd.z, dY2 = cd.w;
+                const double Tx2 = 0.5 * Kh * (g1.x + g2.x), Ty2 = 0.5 * Kh * (g1.y + g2.y);
+                const double u2 = PL(s.U_nf, k, e2, E), v2 = PL(s.V_nf, k, e2, E), J2 = PL(s.Je, k, e2, E);
+                double Tn;
+                if (v2 * dX2 - u2 * dY2 < 0) {
+                    Tn = T2 + (0.5 * (-dd.x * ab + dX2 - u2 * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY2 - v2 * dt) * ty2) *
+                                  crt1;
+                } else {
+                    Tn = T1 + (0.5 * (dd.x * ab + dX2 - u2 * dt) * tx1 + 0.5 * (dd.y * kREarthD + dY2 - v2 * dt) * ty1) *
+                                  crt1;
+                }
+                const double c2 = -(J2 * v2 * Tn - Ty2) * dX2 + (J2 * u2 * Tn - Tx2) * dY2;
+                const double c2V = -J2 * v2 * dX2 + J2 * u2 * dY2;
+                if (cc >= 0) {
+                    rt = rt + c2;
+                    rc = rc + cV;
+                    rc = rc + c2V;
+                } else {
+                    rt = rt - c2;
+                    rc = rc - cV;
+                    rc = rc - c2V;
+                }
+            } else {
+                rc = (cc >= 0) ? rc + cV : rc - cV;
+            }
+        }
+        const double tvb = hi_vert<false>(m, s, s.tcur, 0.0, n, k + 2, a, Tmv);
+        const double tab = PL(s.abT, k, n, N);
+        const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+        const double jc = a * m.dsig[k];
+        rt = ((rt - tab * rc) / m.area[n] + tab * wdiff + tvt - tvb) / jc;
+        const double tdm = PL(s.tcur, k, n, N);
+        if (ob) {
+            if (vel_nor > 0.0) rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+            else rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+        }
+        PL(s.T_old, k, n, N) = tdm + Ta;
+        PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
+        tvt = tvb;
+    }
+    vdiff_column(m, s, n, a, Ta, Sa, dt);
+}
+
+// Tracer step without horizontal advection (tracer_adv = 2 built without
+// MUSCL_CALL, fv_main.F90:1790-1800): implicit vertical diffusion only.
+__global__ void k_tr_vdiff(DevMesh m, DevState s, double dt) {
+    int n = TID();
+    if (n >= m.N) return;
+    vdiff_column(m, s, n, s.a[n], s.scal[0], s.scal[1], dt);
 }
 
 // ---------------------------------------------------------------------------
