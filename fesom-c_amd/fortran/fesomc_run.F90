@@ -1,0 +1,370 @@
+! fesomc_run: Fortran host driver of the GPU internal step.
+!
+! Follows the reference program's order (src/fv_main.F90): READ_DATA_RUN
+! reads fv_run.dat, read_mesh_ser reads the MESHPATH files, the mesh
+! preprocessing and array set-up build the context, initial_state sets T/S,
+! then the main loop (src/fv_main.F90:389-746) advances the clock and calls
+! oce_timestep's call sites one by one, writing energy.dat every IREP steps
+! in the reference's record format (src/fv_main.F90:614-619).
+!
+! usage: fesomc_run RUN_DIR [NSTEPS] [--mesh-only]
+!   RUN_DIR holds fv_run.dat; MESHPATH inside it is relative to RUN_DIR.
+!   NSTEPS overrides fv_run.dat's nsteps.  --mesh-only stops after the mesh
+!   preprocessing and prints its sizes (needs no GPU).
+program fesomc_run
+  use iso_c_binding
+  use fesomc_gpu
+  implicit none
+
+  integer, parameter :: maxkeys = 400
+  character(len=64) :: keys(maxkeys)
+  character(len=256) :: vals(maxkeys)
+  integer :: nkeys = 0
+
+  character(len=512) :: rundir, arg, meshdir
+  integer :: nsteps, irep, n_dt, i, k, n, nargs, L
+  logical :: mesh_only
+  type(fcg_mesh_cfg) :: mc
+  type(fcg_params) :: par
+  type(fcg_mesh) :: view
+  type(fcg_diag) :: dg
+  type(c_ptr) :: mobj, ctx
+  integer(c_int) :: rc
+  integer :: nod2D, elem2D, nobn, n_harm
+  real(c_double), allocatable, target :: xy(:, :), depth(:), ob(:, :), T(:, :), S(:, :)
+  integer(c_int), allocatable :: ind(:), elem(:, :)
+  real(c_double) :: time
+  character(len=3), parameter :: harm(12) = [character(len=3) :: 'M2', 'S2', 'N2', 'K2', 'K1', 'O1', &
+       'P1', 'Q1', 'Mf', 'Mm', 'Ssa', 'M4']
+
+  nargs = command_argument_count()
+  if (nargs < 1) then
+     write(*,*) 'usage: fesomc_run RUN_DIR [NSTEPS] [--mesh-only]'
+     stop 2
+  end if
+  call get_command_argument(1, rundir)
+  nsteps = -1
+  mesh_only = .false.
+  do i = 2, nargs
+     call get_command_argument(i, arg)
+     if (trim(arg) == '--mesh-only') then
+        mesh_only = .true.
+     else
+        read(arg, *) nsteps
+     end if
+  end do
+
+  ! ---- READ_DATA_RUN (src/fv_read_run.F90) -------------------------------
+  call read_run(trim(rundir) // '/fv_run.dat')
+  if (nsteps < 0) nsteps = geti('nsteps', 0)
+  irep = geti('IREP', 100)
+
+  mc%cartesian = merge(1, 0, getl('cartesian'))
+  mc%coriolis_tf = merge(1, 0, getl('Coriolis_TF'))
+  mc%nsigma = geti('nsigma', 10)
+  mc%ind_sigma = geti('ind_sigma', 1)
+  mc%is_LE = merge(1, 0, trim(gets('TITLE', '')) == 'LE')
+  mc%T_potential = merge(1, 0, getl('T_potential'))
+  mc%tf_presence = merge(1, 0, getl('TF_presence'))
+  mc%lat_cartesian_deg = getf('lat_cartesian', 0.0d0)
+  mc%cyclic_length_deg = getf('cyclic_length', 360.0d0)
+  mc%pow_sig = getf('pow_sig', 1.0d0)
+  mc%lev_bot = getf('lev_bot', 1.0d0)
+  mc%lev_sur = getf('lev_sur', 1.0d0)
+  mc%T_const = getf('T_const', 10.0d0)
+  mc%S_const = getf('S_const', 35.0d0)
+  mc%harmonics = 0
+  if (getl('TF_presence') .or. getl('T_potential')) call parse_harmonics(gets_line('Harmonics_tf'), mc%harmonics)
+  n_harm = sum(mc%harmonics)
+
+  par%dt = getf('dt', 0.0d0)
+  par%dt_2D = getf('dt_2D', 0.0d0)
+  par%time_jd0 = getf('JD0', 0.0d0)
+  par%Dmin = getf('Dmin', 0.25d0)
+  par%C_d = getf('Cd', 0.0d0)
+  par%z0b_min = getf('z0b_min', 0.01d0)
+  par%z0s_min = getf('z0s_min', 0.02d0)
+  par%snul = getf('snul', 7.5d-4)
+  par%Cd_bz = getf('Cd_bz', 0.0d0)
+  par%Bp_bz = getf('Bp_bz', 0.0d0)
+  par%clim_relax = 1.0d0 / (getf('clim_relax', 15.0d0) * 3600.0d0 * 24.0d0)       ! fv_read_run.F90:884
+  par%relax2clim_ac = 1.0d0 / (getf('relax2clim_ac', 60.0d0) * 3600.0d0 * 24.0d0)
+  par%nsigma = mc%nsigma
+  if (has('Mt')) then
+     par%Mt = geti('Mt', 1)
+  else
+     par%Mt = int(par%dt / par%dt_2D)                                                ! fv_read_run.F90:1060
+  end if
+  par%tracer_adv = geti('tracer_adv', 1)
+  par%muscl_call = merge(1, 0, getl('MUSCL_CALL'))
+  par%mom_adv_2D = geti('mom_adv_2D', 2)
+  par%mom_adv_3D = geti('mom_adv_3D', 1)
+  par%vert_adv = geti('vert_adv', 1)
+  par%im_vert_visc_fic = merge(1, 0, getl('im_vert_visc_fic', .true.))
+  par%fic_point = geti('fic_point', 1)
+  par%wet_dry_on = merge(1, 0, getl('WET_DRY_ON'))
+  par%tf_presence = mc%tf_presence
+  par%restart = merge(1, 0, trim(gets('RESTART', 'cold_start')) == 'hot_start')
+  par%cl_relax = merge(1, 0, getl('cl_relax'))
+  par%precision = FCG_FP64
+  par%closure = FCG_CLOSURE_D3END
+
+  ! ---- read_mesh_ser (src/fv_main.F90:787-945) -----------------------------
+  meshdir = trim(rundir) // '/' // trim(gets('MESHPATH', './MESH/'))
+  call read_nod2d(trim(meshdir) // '/nod2d.out')
+  call read_elem2d(trim(meshdir) // '/elem2d.out')
+  allocate(depth(nod2D))
+  open(21, file=trim(meshdir) // '/depth.out', status='old', action='read')
+  read(21, *) depth
+  close(21)
+  nobn = 0
+  if (mc%tf_presence == 1 .and. n_harm > 0) call read_m2(trim(meshdir) // '/m2.out')
+  if (nobn == 0) allocate(ob(1, 1))
+
+  rc = fcg_mesh_build(mc, nod2D, xy, ind, elem2D, elem, depth, nobn, &
+       merge(c_loc(ob), c_null_ptr, nobn > 0), mobj)
+  call fcg_check(rc, c_null_ptr, 'fcg_mesh_build')
+  rc = fcg_mesh_view(mobj, view)
+  call fcg_check(rc, c_null_ptr, 'fcg_mesh_view')
+  write(*,'(a,5i10)') ' mesh: nod2D elem2D edge2D edge2D_in nobn', view%nod2D, view%elem2D, view%edge2D, &
+       view%edge2D_in, view%nobn
+  if (mesh_only) then
+     call fcg_mesh_free(mobj)
+     stop
+  end if
+
+  rc = fcg_create(view, par, ctx)
+  call fcg_check(rc, ctx, 'fcg_create')
+  write(*,*) fcg_version_string()
+
+  ! ---- initial_state T/S (src/fv_ini.F90:476-546) --------------------------
+  L = mc%nsigma - 1
+  allocate(T(L, nod2D), S(L, nod2D))
+  T = 0.0d0
+  S = 0.0d0
+  if (getl('TEMP_ON')) then
+     T = mc%T_const
+     if (mc%is_LE == 1) then          ! lock exchange: two water masses split at x = 0
+        do n = 1, nod2D
+           if (xy(1, n) < 0.0d0) then
+              T(:, n) = 5.0d0 / 0.28d0
+           else
+              T(:, n) = 10.0d0 / 0.28d0
+           end if
+        end do
+     end if
+  end if
+  if (getl('SALINITY_ON')) S = mc%S_const
+  call put('TF', T)
+  call put('T_old', T)
+  call put('Tclim', T)
+  call put('SF', S)
+  call put('S_old', S)
+  call put('Sclim', S)
+
+  ! ---- main loop (src/fv_main.F90:389-746) ---------------------------------
+  open(53, file='energy.dat')
+  do n_dt = 1, nsteps
+     call fcg_check(fcg_advance_clock(ctx), ctx, 'clock')            ! fv_main.F90:400-401
+     call fcg_check(fcg_pressure(ctx), ctx, 'pressure')              ! fv_main.F90:1996
+     call fcg_check(fcg_vertical_mixing(ctx), ctx, 'd3_end')         ! fv_main.F90:2006, 2025
+     call fcg_check(fcg_compute_vel_rhs(ctx), ctx, 'compute_vel_rhs') ! fv_main.F90:2034
+     call fcg_check(fcg_barotropic_timestep(ctx), ctx, 'oce_barotropic_timestep') ! fv_main.F90:2041
+     call fcg_check(fcg_jacobian(ctx), ctx, 'jacobian')              ! fv_main.F90:2050
+     call fcg_check(fcg_update_3d_vel(ctx), ctx, 'update_3D_vel')    ! fv_main.F90:2057
+     call fcg_check(fcg_vert_vel_sigma(ctx), ctx, 'vert_vel_sigma')  ! fv_main.F90:2064
+     call fcg_check(fcg_solve_tracers(ctx), ctx, 'solve_tracers')    ! fv_main.F90:2076
+     if (mod(n_dt, irep) == 0 .or. n_dt == nsteps) then
+        call fcg_check(fcg_diagnostics(ctx, dg), ctx, 'fcg_diagnostics')
+        call fcg_check(dg%status, ctx, 'solve_tracers')
+        if (mod(n_dt, irep) == 0) then
+           time = dg%time_jd * 86400.0d0
+           write(53, '(4e16.7)') time / 86400.0d0 - par%time_jd0, dg%energy
+           write(*,*) 'step ', n_dt, ' energy= ', dg%energy, ' max_min_ssh:', dg%eta_max, dg%eta_min
+        end if
+     end if
+  end do
+  close(53, status='KEEP')
+  write(*,*) 'volume ', dg%volume, ' heat ', dg%heat, ' salt ', dg%salt
+  call fcg_destroy(ctx)
+  call fcg_mesh_free(mobj)
+
+contains
+
+  subroutine put(name, a)
+    character(len=*), intent(in) :: name
+    real(c_double), intent(in) :: a(:, :)
+    call fcg_check(fcg_set_state(ctx, cstr(name), a, int(size(a), c_long)), ctx, 'fcg_set_state ' // name)
+  end subroutine put
+
+  ! fv_run.dat scanner: KEY = value lines, '!' comments, first occurrence wins
+  ! (SCAN_FILE semantics, src/fv_utilit.F90:223-570)
+  subroutine read_run(path)
+    character(len=*), intent(in) :: path
+    character(len=512) :: line
+    integer :: ios, p, q
+    open(20, file=path, status='old', action='read')
+    do
+       read(20, '(a)', iostat=ios) line
+       if (ios /= 0) exit
+       p = index(line, '!')
+       if (p > 0) line = line(1:p - 1)
+       q = index(line, '=')
+       if (q == 0) cycle
+       if (len_trim(adjustl(line(1:q - 1))) == 0) cycle
+       if (has(trim(adjustl(line(1:q - 1))))) cycle
+       if (nkeys == maxkeys) exit
+       nkeys = nkeys + 1
+       keys(nkeys) = trim(adjustl(line(1:q - 1)))
+       vals(nkeys) = trim(adjustl(line(q + 1:)))
+    end do
+    close(20)
+  end subroutine read_run
+
+  integer function find(key)
+    character(len=*), intent(in) :: key
+    integer :: j
+    find = 0
+    do j = 1, nkeys
+       if (trim(keys(j)) == key) then
+          find = j
+          return
+       end if
+    end do
+  end function find
+
+  logical function has(key)
+    character(len=*), intent(in) :: key
+    has = find(key) > 0
+  end function has
+
+  function gets_line(key) result(v)
+    character(len=*), intent(in) :: key
+    character(len=256) :: v
+    integer :: j
+    v = ''
+    j = find(key)
+    if (j > 0) v = vals(j)
+  end function gets_line
+
+  function gets(key, def) result(v)
+    character(len=*), intent(in) :: key, def
+    character(len=256) :: v
+    integer :: j, sp
+    v = def
+    j = find(key)
+    if (j > 0) then
+       sp = index(trim(vals(j)), ' ')
+       if (sp > 0) then
+          v = vals(j)(1:sp - 1)
+       else
+          v = vals(j)
+       end if
+    end if
+  end function gets
+
+  real(c_double) function getf(key, def)
+    character(len=*), intent(in) :: key
+    real(c_double), intent(in) :: def
+    getf = def
+    if (has(key)) read(gets(key, ''), *) getf
+  end function getf
+
+  integer function geti(key, def)
+    character(len=*), intent(in) :: key
+    integer, intent(in) :: def
+    geti = def
+    if (has(key)) read(gets(key, ''), *) geti
+  end function geti
+
+  logical function getl(key, def)
+    character(len=*), intent(in) :: key
+    logical, intent(in), optional :: def
+    character(len=256) :: v
+    getl = .false.
+    if (present(def)) getl = def
+    if (.not. has(key)) return
+    v = adjustl(gets(key, ''))
+    getl = v(1:1) == 'T' .or. v(1:1) == 't' .or. v(1:2) == '.T' .or. v(1:2) == '.t'
+  end function getl
+
+  subroutine parse_harmonics(line, on)
+    character(len=*), intent(in) :: line
+    integer(c_int), intent(inout) :: on(12)
+    character(len=256) :: rest
+    character(len=16) :: tok
+    integer :: sp, j
+    rest = adjustl(line)
+    do while (len_trim(rest) > 0)
+       sp = index(trim(rest), ' ')
+       if (sp == 0) then
+          tok = trim(rest)
+          rest = ''
+       else
+          tok = rest(1:sp - 1)
+          rest = adjustl(rest(sp + 1:))
+       end if
+       do j = 1, 12
+          if (trim(tok) == trim(harm(j))) on(j) = 1
+       end do
+    end do
+  end subroutine parse_harmonics
+
+  ! nod2d.out: nod2D, then "n x y index" (nodes stored by their id)
+  subroutine read_nod2d(path)
+    character(len=*), intent(in) :: path
+    integer :: j, id, ix
+    real(c_double) :: x, y
+    open(21, file=path, status='old', action='read')
+    read(21, *) nod2D
+    allocate(xy(2, nod2D), ind(nod2D))
+    do j = 1, nod2D
+       read(21, *) id, x, y, ix
+       xy(1, id) = x
+       xy(2, id) = y
+       ind(id) = ix
+    end do
+    close(21)
+  end subroutine read_nod2d
+
+  ! elem2d.out: elem2D, then 4 node ids per element (3 for triangle-only files)
+  subroutine read_elem2d(path)
+    character(len=*), intent(in) :: path
+    character(len=256) :: line
+    integer :: j, ncol, p
+    logical :: intok
+    open(21, file=path, status='old', action='read')
+    read(21, *) elem2D
+    read(21, '(a)') line
+    ncol = 0
+    intok = .false.
+    do p = 1, len_trim(line)
+       if (line(p:p) /= ' ' .and. .not. intok) ncol = ncol + 1
+       intok = line(p:p) /= ' '
+    end do
+    rewind(21)
+    read(21, *) elem2D
+    allocate(elem(4, elem2D))
+    if (ncol >= 4) then
+       read(21, *) ((elem(p, j), p = 1, 4), j = 1, elem2D)
+    else
+       read(21, *) ((elem(p, j), p = 1, 3), j = 1, elem2D)
+       elem(4, :) = elem(1, :)
+    end if
+    close(21)
+  end subroutine read_elem2d
+
+  ! m2.out: nobn, then "node amp phase ..." per open-boundary node
+  subroutine read_m2(path)
+    character(len=*), intent(in) :: path
+    integer :: j, id
+    open(21, file=path, status='old', action='read')
+    read(21, *) nobn
+    allocate(ob(2 * n_harm, nobn))
+    do j = 1, nobn
+       read(21, *) id, ob(:, j)
+    end do
+    close(21)
+  end subroutine read_m2
+
+end program fesomc_run

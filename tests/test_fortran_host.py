@@ -1,0 +1,90 @@
+"""Fortran host over the C-ABI: the ISO_C_BINDING module and the driver that
+replaces fv_main's call sites (fesom-c_amd/fortran).  The driver reads the
+reference's own input files (fv_run.dat, nod2d.out, elem2d.out, depth.out,
+m2.out) and writes energy.dat in the reference's record format."""
+import gzip
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, have_gpu
+
+EXE = os.path.join(ROOT, "fesom-c_amd", "fortran", "fesomc_run")
+LE = os.path.join(GOLDEN, "le_medium")
+
+
+def _le_rundir(tmp_path):
+    d = tmp_path / "le"
+    (d / "MESH").mkdir(parents=True)
+    for f in ("nod2d", "elem2d", "depth"):
+        with gzip.open(os.path.join(LE, f + ".out.gz"), "rb") as src, open(d / "MESH" / (f + ".out"), "wb") as dst:
+            shutil.copyfileobj(src, dst)
+    # the reference's energy series was produced with Mt = 4 (see test_oracle_pins)
+    (d / "fv_run.dat").write_text(open(os.path.join(LE, "fv_run.dat")).read() + "\n Mt = 4\n")
+    return d
+
+
+def _run(args, cwd, timeout=120):
+    assert os.path.exists(EXE), "Fortran driver not built (run __graft_entry__.build())"
+    return subprocess.run([EXE] + [str(a) for a in args], cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+def _mesh_line(out):
+    line = [ln for ln in out.splitlines() if "mesh:" in ln][0]
+    return [int(v) for v in line.split()[-5:]]
+
+
+def test_driver_le_mesh_matches_reference_edge_files(tmp_path):
+    """nod2D/elem2D and the edge counts equal the reference's MESH/edgenum.out."""
+    d = _le_rundir(tmp_path)
+    r = _run([d, 1, "--mesh-only"], d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with gzip.open(os.path.join(LE, "edgenum.out.gz"), "rt") as f:
+        edgenum = [int(v) for v in f.read().split()]
+    nod, elem, edges, edges_in, nobn = _mesh_line(r.stdout)
+    assert (nod, elem, nobn) == (4515, 4352, 0)
+    assert [edges, edges_in] == edgenum[:2]
+
+
+def test_driver_channel_mesh_with_tides(tmp_path):
+    """Triangles + quads + m2.out open boundary: the driver's preprocessing
+    sizes equal the Python host mirror's on the same files."""
+    from fesomc_amd import gpu, meshgen, meshio
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, _ = meshgen.config("c1")
+    d = tmp_path / "c1"
+    meshio.write_mesh(str(d / "MESH"), mesh)
+    (d / "fv_run.dat").write_text(txt)
+    r = _run([d, 1, "--mesh-only"], d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    back = meshio.read_mesh(str(d / "MESH"), n_harm=1)
+    ref = gpu.mesh_arrays(RunConfig(txt), back)
+    assert _mesh_line(r.stdout) == [ref["nod2D"], ref["elem2D"], ref["edge2D"], ref["edge2D_in"], ref["nobn"]]
+    assert ref["nobn"] > 0
+
+
+@pytest.mark.skipif(have_gpu(), reason="checks the behaviour without a GPU")
+def test_driver_without_gpu_stops_with_status(tmp_path):
+    d = _le_rundir(tmp_path)
+    r = _run([d, 5], d)
+    assert r.returncode != 0
+    assert "fcg_create failed with status" in r.stdout and "-7" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_reproduces_reference_energy_rows(tmp_path):
+    """300 internal steps of LE_medium through the Fortran call sites: the
+    three energy.dat rows match energy_tocheck.dat at its printed digits."""
+    from test_oracle_pins import digits7_ok
+    d = _le_rundir(tmp_path)
+    r = _run([d, 300], d, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = np.loadtxt(d / "energy.dat", ndmin=2)
+    ref = np.loadtxt(os.path.join(LE, "energy_tocheck_head200.dat"), ndmin=2)
+    assert rows.shape == (3, 2)
+    for k in range(3):
+        assert abs(rows[k, 0] - ref[k, 0]) <= 1e-6 * ref[k, 0]
+        assert digits7_ok(rows[k, 1], ref[k, 1]), (k, rows[k, 1], ref[k, 1])
