@@ -49,7 +49,7 @@ def kernel_bytes(k, N, E, D, L, K):
     if k == "k_d3end":
         rd = i4 * (N + 1) + i4 * 4 * E + d8 * E + d8 * 2 * LE + d8 * 2 * LN + d8 * 3 * KN + d8 * 5 * N
         return rd + d8 * 3 * KN       # writes bt, snu, Kv
-    if k == "k_vertvel":
+    if k in ("k_vertvel", "k_vertvel_div"):
         return i4 * (N + 1) + i4 * 2 * D + i4 * 2 * D + d8 * 4 * D + d8 * N + d8 * 3 * LE + d8 * 2 * E + d8 * KN
     if k == "k_bpg":
         return mesh_e + d8 * E * 8 + d8 * 3 * N + d8 * KN + d8 * 2 * LE + d8 * 2 * E + d8 * E

@@ -14,7 +14,8 @@ __global__ void k_pressure(DevMesh, DevState, double, double);
 __global__ void k_bpg(DevMesh, DevState);
 __global__ void k_d3end(DevMesh, DevState, double, double, double, double, double, double);
 __global__ void k_velgrad3d(DevMesh, DevState);
-__global__ void k_velrhs(DevMesh, DevState, double, double, double, double, int, int, int, int);
+__global__ void k_velrhs(DevMesh, DevState, double, double, int, int);
+__global__ void k_velvisc(DevMesh, DevState, double, double, double, int);
 __global__ void k_bt_elem1(DevMesh, DevState, double);
 __global__ void k_bt_node(DevMesh, DevState, double, const double *);
 __global__ void k_bt_velgrad(DevMesh, DevState);
@@ -25,7 +26,8 @@ __global__ void k_bt_elem2(DevMesh, DevState, double, double, double, double, do
 __global__ void k_jac_node(DevMesh, DevState, double);
 __global__ void k_je(DevMesh, DevState);
 __global__ void k_update3d(DevMesh, DevState, double, double);
-__global__ void k_vertvel(DevMesh, DevState);
+__global__ void k_vertvel_div(DevMesh, DevState);
+__global__ void k_vertvel_scan(DevMesh, DevState);
 __global__ void k_minmax_partial(const double *, const double *, size_t, double *);
 __global__ void k_minmax_final(const double *, int, double *);
 __global__ void k_tr_ab(DevState, size_t);
@@ -59,6 +61,8 @@ namespace {
 
 constexpr int kBlk = 256;
 inline unsigned nblk(size_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
+// blocks for node x layer kernels: 64 nodes x 4 layers per block of kBlk
+inline unsigned nblk_lv(int N, int L) { return (unsigned)(((N + 63) / 64) * ((L + 3) / 4)); }
 
 template <typename T>
 int dalloc(fcg_ctx *ctx, T **p, size_t n) {
@@ -589,8 +593,10 @@ extern "C" int fcg_compute_vel_rhs(fcg_ctx *ctx) {
         ctx->lfirst = 0;
     }
     LAUNCH(ctx, k_velgrad3d, nblk(ctx->m.E), ctx->m, ctx->s);
-    LAUNCH(ctx, k_velrhs, nblk(ctx->m.E), ctx->m, ctx->s, p.dt, ff, ctx->density_0, p.Cd_bz, p.fic_point,
-           p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, p.im_vert_visc_fic ? 1 : 0);
+    LAUNCH(ctx, k_velrhs, nblk_lv(ctx->m.E, ctx->m.L), ctx->m, ctx->s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0,
+           p.vert_adv == 1 ? 1 : 0);
+    if (p.im_vert_visc_fic)
+        LAUNCH(ctx, k_velvisc, nblk(ctx->m.E), ctx->m, ctx->s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -635,7 +641,8 @@ extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
 
 extern "C" int fcg_vert_vel_sigma(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    LAUNCH(ctx, k_vertvel, nblk(ctx->m.N), ctx->m, ctx->s);
+    LAUNCH(ctx, k_vertvel_div, nblk_lv(ctx->m.N, ctx->m.L), ctx->m, ctx->s);
+    LAUNCH(ctx, k_vertvel_scan, nblk(ctx->m.N), ctx->m, ctx->s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -651,22 +658,22 @@ extern "C" int fcg_solve_tracers(fcg_ctx *ctx) {
                        (const double *)s.SF, NL, s.red);
     hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal);
     const int relax = p.cl_relax ? 1 : 0;
-    // scheme dispatch of src/fv_main.F90:1788-1800
+    // scheme dispatch of src/fv_main.F90:1788-1800; node x layer kernels, then
+    // the column-sequential implicit vertical diffusion
+    const unsigned glv = nblk_lv(m.N, m.L);
     if (p.tracer_adv == 1) {
         LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
-        LAUNCH(ctx, k_tr_upwind, nblk(m.N), m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH(ctx, k_tr_upwind, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 2 && p.muscl_call) {
         LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
-        LAUNCH(ctx, k_tr_gradn_muscl, nblk(m.N), m, s);
-        LAUNCH(ctx, k_tr_muscl, nblk(m.N), m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH(ctx, k_tr_gradn_muscl, glv, m, s);
+        LAUNCH(ctx, k_tr_muscl, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 4) {
         LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
-        LAUNCH(ctx, k_tr_gradn_miura, nblk(m.N), m, s);
-        LAUNCH(ctx, k_tr_miura, nblk(m.N), m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax,
-               p.relax2clim_ac);
-    } else {
-        LAUNCH(ctx, k_tr_vdiff, nblk(m.N), m, s, p.dt);
+        LAUNCH(ctx, k_tr_gradn_miura, glv, m, s);
+        LAUNCH(ctx, k_tr_miura, glv, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
     }
+    LAUNCH(ctx, k_tr_vdiff, nblk(m.N), m, s, p.dt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }

@@ -58,10 +58,24 @@ __device__ __forceinline__ unsigned xcd_block() {
 #define TID() (xcd_block() * blockDim.x + threadIdx.x)
 #define PL(arr, k, n, stride) (arr)[(size_t)(k) * (stride) + (n)]
 
+// (node, level) of this thread.  Layer-parallel node kernels run one thread
+// per node and layer: a block of 256 covers 64 consecutive (locality-
+// ordered) nodes x 4 consecutive layers, and the layer groups of one node
+// tile are adjacent logical blocks, so they land on the same XCD and share
+// its L2 for the level-invariant edge geometry.  Launch with nblk_lv(N, L)
+// blocks.
+__device__ __forceinline__ bool node_level(int N, int L, int &n, int &k) {
+    const unsigned b = xcd_block();
+    const unsigned nlg = (unsigned)(L + 3) / 4u;
+    n = (int)((b / nlg) * 64u + (threadIdx.x & 63u));
+    k = (int)((b % nlg) * 4u + (threadIdx.x >> 6));
+    return n < N && k < L;
+}
+
 // ---------------------------------------------------------------------------
 // pressure: hydrostatic column scan, src/fv_pressure_new.F90:91-107
 // ---------------------------------------------------------------------------
-__global__ void k_pressure(DevMesh m, DevState s, double gd0, double rho0) {
+__global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevState s, double gd0, double rho0) {
     int n = TID();
     if (n >= m.N) return;
     const int N = m.N;
@@ -80,7 +94,7 @@ __global__ void k_pressure(DevMesh m, DevState s, double gd0, double rho0) {
 }
 
 // baroclinic_pressure_gradient_3D: src/fv_pressure_new.F90:146-221 (FP64)
-__global__ void k_bpg(DevMesh m, DevState s) {
+__global__ __launch_bounds__(256) void k_bpg(DevMesh m, DevState s) {
     int e = TID();
     if (e >= m.E) return;
     const int N = m.N, E = m.E;
@@ -121,7 +135,7 @@ __global__ void k_bpg(DevMesh m, DevState s) {
 // (src/fv_mixing.F90:73-347): one thread per node column.
 // Scratch planes (K*N): p=-b1-b2, q=b_t/dt, c2, xx1, yy1, btr ping-pong.
 // ---------------------------------------------------------------------------
-__global__ void k_d3end(DevMesh m, DevState s, double dt, double Dmin, double z0b_min, double z0s_min,
+__global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevState s, double dt, double Dmin, double z0b_min, double z0s_min,
                         double snul, double rho0) {
     int n = TID();
     if (n >= m.N) return;
@@ -263,7 +277,7 @@ __global__ void k_d3end(DevMesh m, DevState s, double dt, double Dmin, double z0
 // ---------------------------------------------------------------------------
 // vel_gradients (3-D, free slip): src/fv_gradients.F90:2-93
 // ---------------------------------------------------------------------------
-__global__ void k_velgrad3d(DevMesh m, DevState s) {
+__global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevState s) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E;
@@ -330,22 +344,116 @@ __device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevState &s, i
     }
 }
 
-// compute_vel_rhs (src/fv_3D_vel.F90:1-197) for one element column:
-// elemental pressure/Coriolis/AB2 (35-78), momentum_adv_upwind gather
-// (fv_advection.F90:64-197), momentum_vert_adv_upwind (354-469), the
-// explicit update (124-143) and momentum_vert_impl_visc_tmp
-// (fv_viscosity.F90:1374-1486) with Av formed from snu (fv_mixing.F90:308-313).
-__global__ void k_velrhs(DevMesh m, DevState s, double dt, double ff, double rho0, double Cd_bz, int fic_point,
-                         int hadv, int vadv, int visc) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int N = m.N, E = m.E, L = m.L, K = m.K;
-    const int4 q = m.en[e];
-    const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
+// (element, level) of this thread: blocks of 64 consecutive elements x 4
+// layers, layer groups of one element tile adjacent (see node_level).
+__device__ __forceinline__ bool elem_level(int E, int L, int &e, int &k) {
+    const unsigned b = xcd_block();
+    const unsigned nlg = (unsigned)(L + 3) / 4u;
+    e = (int)((b / nlg) * 64u + (threadIdx.x & 63u));
+    k = (int)((b % nlg) * 4u + (threadIdx.x >> 6));
+    return e < E && k < L;
+}
+
+// Element quantities shared by the two compute_vel_rhs kernels.
+struct ElemCol {
+    int4 q;
+    double4 w;
+    double a0, a1, a2, a3;
+    __device__ double Je(const DevMesh &m, int k) const {
+        const double ds = m.dsig[k];
+        return sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
+    }
+    __device__ double Av(const DevMesh &m, const DevState &s, int k) const {
+        const int N = m.N;
+        return sum4(w, PL(s.snu, k, q.x, N), PL(s.snu, k, q.y, N), PL(s.snu, k, q.z, N), PL(s.snu, k, q.w, N));
+    }
+    __device__ double zbw(const DevMesh &m, int lev) const {  // sum w_cv*zbar at 0-based level
+        const double o = m.oms[lev];
+        return sum4(w, o * a0, o * a1, o * a2, o * a3);
+    }
+    __device__ double Zw(const DevMesh &m, int lay) const {  // sum w_cv*Z at 0-based layer
+        const double o0 = m.oms[lay], o1 = m.oms[lay + 1];
+        return sum4(w, 0.5 * (o0 * a0 + o1 * a0), 0.5 * (o0 * a1 + o1 * a1), 0.5 * (o0 * a2 + o1 * a2),
+                    0.5 * (o0 * a3 + o1 * a3));
+    }
+};
+
+__device__ __forceinline__ ElemCol elem_col(const DevMesh &m, const DevState &s, int e) {
+    ElemCol c;
+    c.q = m.en[e];
+    c.w = m.wcv[e];
+    c.a0 = s.a[c.q.x];
+    c.a1 = s.a[c.q.y];
+    c.a2 = s.a[c.q.z];
+    c.a3 = s.a[c.q.w];
+    return c;
+}
+
+// momentum_vert_adv_upwind flux through 1-based interface nz (2..K-1) of an
+// element column (fv_advection.F90:389-433)
+__device__ __forceinline__ void uvert(const DevMesh &m, const DevState &s, const ElemCol &c, int e, double area,
+                                      int nz, double &fu, double &fv) {
+    const int N = m.N, E = m.E, K = m.K;
+    const int4 q = c.q;
+    auto U = [&](int k) { return PL(s.U_n, k, e, E); };
+    auto V = [&](int k) { return PL(s.V_n, k, e, E); };
+    const double wv = sum4(c.w, PL(s.W, nz - 1, q.x, N), PL(s.W, nz - 1, q.y, N), PL(s.W, nz - 1, q.z, N),
+                           PL(s.W, nz - 1, q.w, N)) * area;
+    double um = 0.0, vm = 0.0;
+    if (wv >= 0.0) {
+        if (nz == K - 1) {
+            um = 0.5 * (U(nz - 2) + U(nz - 1));
+            vm = 0.5 * (V(nz - 2) + V(nz - 1));
+        } else {
+            double Z1 = c.zbw(m, nz - 1);
+            double aa = Z1 - c.Zw(m, nz - 2);
+            double bb = c.Zw(m, nz - 1) - Z1;
+            double cc = c.Zw(m, nz) - Z1;
+            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+            double dg = aa * bb * (aa + bb) / d;
+            double db = -aa * cc * (aa + cc) / d;
+            double da = 1.0 - dg - db;
+            um = U(nz - 2) * da + U(nz - 1) * db + U(nz) * dg;
+            vm = V(nz - 2) * da + V(nz - 1) * db + V(nz) * dg;
+        }
+    } else {
+        if (nz == 2) {
+            um = 0.5 * (U(nz - 2) + U(nz - 1));
+            vm = 0.5 * (V(nz - 2) + V(nz - 1));
+        } else {
+            double Z1 = c.zbw(m, nz - 1);
+            double aa = c.Zw(m, nz - 1) - Z1;
+            double bb = Z1 - c.Zw(m, nz - 2);
+            double cc = Z1 - c.Zw(m, nz - 3);
+            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+            double dg = aa * bb * (aa + bb) / d;
+            double db = -aa * cc * (aa + cc) / d;
+            double da = 1.0 - dg - db;
+            um = U(nz - 1) * da + U(nz - 2) * db + U(nz - 3) * dg;
+            vm = V(nz - 1) * da + V(nz - 2) * db + V(nz - 3) * dg;
+        }
+    }
+    fu = -um * wv;
+    fv = -vm * wv;
+}
+
+// compute_vel_rhs (src/fv_3D_vel.F90:1-197), explicit part, one thread per
+// element and layer: elemental pressure/Coriolis/AB2 (35-78), the
+// momentum_adv_upwind gather over the element's internal edges in ascending
+// id (fv_advection.F90:64-197), momentum_vert_adv_upwind at the layer's two
+// interfaces (354-469) and the explicit update (124-143).  Writes the new AB
+// history and the provisional velocity into U_rhs/V_rhs; k_velvisc then
+// applies the implicit vertical viscosity column by column.
+__global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevState s, double dt, double ff, int hadv, int vadv) {
+    int e, k;
+    if (!elem_level(m.E, m.L, e, k)) return;
+    const int N = m.N, E = m.E, K = m.K;
+    const ElemCol c = elem_col(m, s, e);
+    const int4 q = c.q;
+    const double4 w = c.w, gx = m.gsx[e], gy = m.gsy[e];
     const double area = m.elem_area[e], cor = m.coriolis[e];
     const double area_inv = 1.0 / area;
     const double mask = s.mask_wd[e];
-    const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
     double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
     double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
     double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
@@ -353,180 +461,139 @@ __global__ void k_velrhs(DevMesh m, DevState s, double dt, double ff, double rho
     const double Fx = sum4(gx, pre0, pre1, pre2, pre3) * area;
     const double Fy = sum4(gy, pre0, pre1, pre2, pre3) * area;
     const double acc = sum4(w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
-    const int4 ied = m.el_ied[e];
-    const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
-    auto Je = [&](int k) {
-        double ds = m.dsig[k];
-        return sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
-    };
-    auto Av = [&](int k) {
-        return sum4(w, PL(s.snu, k, q.x, N), PL(s.snu, k, q.y, N), PL(s.snu, k, q.z, N), PL(s.snu, k, q.w, N));
-    };
-    auto zbw = [&](int lev) {  // sum w_cv*zbar at 0-based level
-        double o = m.oms[lev];
-        return sum4(w, o * a0, o * a1, o * a2, o * a3);
-    };
-    auto Zw = [&](int lay) {  // sum w_cv*Z at 0-based layer
-        double o0 = m.oms[lay], o1 = m.oms[lay + 1];
-        return sum4(w, 0.5 * (o0 * a0 + o1 * a0), 0.5 * (o0 * a1 + o1 * a1), 0.5 * (o0 * a2 + o1 * a2),
-                    0.5 * (o0 * a3 + o1 * a3));
-    };
-    auto U = [&](int k) { return PL(s.U_n, k, e, E); };
-    auto V = [&](int k) { return PL(s.V_n, k, e, E); };
-    // vertical advective flux through 1-based interface nz (2..K-1): fv_advection.F90:389-433
-    auto uvert = [&](int nz, double &fu, double &fv) {
-        double wv = sum4(w, PL(s.W, nz - 1, q.x, N), PL(s.W, nz - 1, q.y, N), PL(s.W, nz - 1, q.z, N),
-                         PL(s.W, nz - 1, q.w, N)) * area;
-        double um = 0.0, vm = 0.0;
-        if (wv >= 0.0) {
-            if (nz == K - 1) {
-                um = 0.5 * (U(nz - 2) + U(nz - 1));
-                vm = 0.5 * (V(nz - 2) + V(nz - 1));
+    const double je = c.Je(m, k);
+    const double u = PL(s.U_n, k, e, E), v = PL(s.V_n, k, e, E);
+    double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, E) + Fx * je;
+    double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, E) + Fy * je;
+    double abu = v * cor * area * je;
+    double abv = -u * cor * area * je;
+    if (hadv) {
+        const int4 ied = m.el_ied[e];
+#pragma nounroll
+        for (int j = 0; j < 4; ++j) {
+            const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
+            if (cc == kEmpty) break;
+            double uu, vv;
+            edge_uu3d(m, s, cc >= 0 ? cc : ~cc, k, uu, vv);
+            if (cc >= 0) {
+                abu = abu + uu;
+                abv = abv + vv;
             } else {
-                double Z1 = zbw(nz - 1);
-                double aa = Z1 - Zw(nz - 2);
-                double bb = Zw(nz - 1) - Z1;
-                double cc = Zw(nz) - Z1;
-                double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
-                double dg = aa * bb * (aa + bb) / d;
-                double db = -aa * cc * (aa + cc) / d;
-                double da = 1.0 - dg - db;
-                um = U(nz - 2) * da + U(nz - 1) * db + U(nz) * dg;
-                vm = V(nz - 2) * da + V(nz - 1) * db + V(nz) * dg;
-            }
-        } else {
-            if (nz == 2) {
-                um = 0.5 * (U(nz - 2) + U(nz - 1));
-                vm = 0.5 * (V(nz - 2) + V(nz - 1));
-            } else {
-                double Z1 = zbw(nz - 1);
-                double aa = Zw(nz - 1) - Z1;
-                double bb = Z1 - Zw(nz - 2);
-                double cc = Z1 - Zw(nz - 3);
-                double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
-                double dg = aa * bb * (aa + bb) / d;
-                double db = -aa * cc * (aa + cc) / d;
-                double da = 1.0 - dg - db;
-                um = U(nz - 1) * da + U(nz - 2) * db + U(nz - 3) * dg;
-                vm = V(nz - 1) * da + V(nz - 2) * db + V(nz - 3) * dg;
+                abu = abu - uu;
+                abv = abv - vv;
             }
         }
-        fu = -um * wv;
-        fv = -vm * wv;
-    };
-    double uvt_u = 0.0, uvt_v = 0.0;  // flux through the top of the current layer
-    if (vadv && mask != 0.0) {
-        double w1 = sum4(w, PL(s.W, 0, q.x, N), PL(s.W, 0, q.y, N), PL(s.W, 0, q.z, N), PL(s.W, 0, q.w, N)) * area;
-        uvt_u = -w1 * U(0);
-        uvt_v = -w1 * V(0);
     }
-    // implicit vertical viscosity: forward-sweep state
-    double xxp = 1.0, yup = 0.0, yvp = 0.0;
-    const bool solve = visc && mask != 0.0;
-    if (solve) {
-        double de = Je(0) / (Av(0) + Av(1));
-        double rw = de / rho0;
+    if (vadv) {
+        if (mask != 0.0) {
+            // fluxes through the layer's top (t = 0) and bottom (t = 1) interface
+            double ftu = 0.0, ftv = 0.0, fbu = 0.0, fbv = 0.0;
+#pragma nounroll
+            for (int t = 0; t < 2; ++t) {
+                const int nz = k + 1 + t;
+                double fu = 0.0, fv = 0.0;
+                if (nz == 1) {
+                    const double w1 = sum4(w, PL(s.W, 0, q.x, N), PL(s.W, 0, q.y, N), PL(s.W, 0, q.z, N),
+                                           PL(s.W, 0, q.w, N)) * area;
+                    fu = -w1 * PL(s.U_n, 0, e, E);
+                    fv = -w1 * PL(s.V_n, 0, e, E);
+                } else if (nz <= K - 1) {
+                    uvert(m, s, c, e, area, nz, fu, fv);
+                }
+                if (t == 0) {
+                    ftu = fu;
+                    ftv = fv;
+                } else {
+                    fbu = fu;
+                    fbv = fv;
+                }
+            }
+            abu = abu + (ftu - fbu) * acc;
+            abv = abv + (ftv - fbv) * acc;
+        } else {
+            abu = 0.0;
+            abv = 0.0;
+        }
+    }
+    PL(s.U_rhsAB, k, e, E) = abu;
+    PL(s.V_rhsAB, k, e, E) = abv;
+    PL(s.U_rhs, k, e, E) = mask * (u + PL(s.Bpu, k, e, E) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
+    PL(s.V_rhs, k, e, E) = mask * (v + PL(s.Bpv, k, e, E) * dt + (dt * (vr + abv * ff) * area_inv + 0.0) / je);
+}
+
+// momentum_vert_impl_visc_tmp (fv_viscosity.F90:1374-1486), one thread per
+// element column: Av formed from snu (fv_mixing.F90:308-313), surface
+// stress, forward sweep over the provisional velocity, bottom condition by
+// fic_point, back substitution; dry elements get zero velocity.
+__global__ __launch_bounds__(256) void k_velvisc(DevMesh m, DevState s, double dt, double rho0, double Cd_bz, int fic_point) {
+    int e = TID();
+    if (e >= m.E) return;
+    const int E = m.E, L = m.L, K = m.K;
+    const double mask = s.mask_wd[e];
+    if (mask == 0.0) {
+        for (int k = 0; k < L; ++k) {
+            PL(s.U_rhs, k, e, E) = 0.0;
+            PL(s.V_rhs, k, e, E) = 0.0;
+        }
+        return;
+    }
+    const ElemCol c = elem_col(m, s, e);
+    const int4 q = c.q;
+    const double acc = sum4(c.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+    double xxp = 1.0, yup, yvp;
+    {
+        const double de = c.Je(m, 0) / (c.Av(m, s, 0) + c.Av(m, s, 1));
+        const double rw = de / rho0;
         yup = s.taux[e] * rw;
         yvp = s.tauy[e] * rw;
     }
-    double je_prev = 0.0, je_cur = Je(0);
-    double av_cur = Av(0);
+    double je_prev = 0.0, je_cur = c.Je(m, 0);
+    double av_cur = c.Av(m, s, 0);
     for (int k = 0; k < L; ++k) {
         const double je = je_cur;
-        const double je_next = (k + 1 < L) ? Je(k + 1) : 0.0;
-        const double u = U(k), v = V(k);
-        double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, E) + Fx * je;
-        double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, E) + Fy * je;
-        double abu = v * cor * area * je;
-        double abv = -u * cor * area * je;
-        if (hadv) {
-            for (int j = 0; j < 4; ++j) {
-                int c = slots[j];
-                if (c == kEmpty) break;
-                double uu, vv;
-                if (c >= 0) {
-                    edge_uu3d(m, s, c, k, uu, vv);
-                    abu = abu + uu;
-                    abv = abv + vv;
-                } else {
-                    edge_uu3d(m, s, ~c, k, uu, vv);
-                    abu = abu - uu;
-                    abv = abv - vv;
-                }
-            }
-        }
-        if (vadv) {
-            if (mask != 0.0) {
-                double fbu = 0.0, fbv = 0.0;
-                if (k + 2 <= K - 1) uvert(k + 2, fbu, fbv);
-                abu = abu + (uvt_u - fbu) * acc;
-                abv = abv + (uvt_v - fbv) * acc;
-                uvt_u = fbu;
-                uvt_v = fbv;
-            } else {
-                abu = 0.0;
-                abv = 0.0;
-            }
-        }
-        PL(s.U_rhsAB, k, e, E) = abu;
-        PL(s.V_rhsAB, k, e, E) = abv;
-        ur = mask * (u + PL(s.Bpu, k, e, E) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
-        vr = mask * (v + PL(s.Bpv, k, e, E) * dt + (dt * (vr + abv * ff) * area_inv + 0.0) / je);
-        if (solve) {
-            double av_next = Av(k + 1);
-            double delta_1 = (k == 0) ? 0.5 * je : 0.5 * (je + je_prev);
-            double delta_2 = (k == L - 1) ? 0.5 * je : 0.5 * (je + je_next);
-            double zt = dt / je;
-            double aa = zt * av_cur / delta_1;
-            double cc = zt * av_next / delta_2;
-            double bb = aa + cc + 1.0;
-            double d = bb - aa * xxp;
-            xxp = cc / d;
-            yup = (aa * yup + ur) / d;
-            yvp = (aa * yvp + vr) / d;
-            PL(s.vxx, k + 1, e, E) = xxp;
-            PL(s.vyu, k + 1, e, E) = yup;
-            PL(s.vyv, k + 1, e, E) = yvp;
-            av_cur = av_next;
-        } else {
-            PL(s.U_rhs, k, e, E) = ur;
-            PL(s.V_rhs, k, e, E) = vr;
-        }
+        const double je_next = (k + 1 < L) ? c.Je(m, k + 1) : 0.0;
+        const double ur = PL(s.U_rhs, k, e, E), vr = PL(s.V_rhs, k, e, E);
+        const double av_next = c.Av(m, s, k + 1);
+        const double delta_1 = (k == 0) ? 0.5 * je : 0.5 * (je + je_prev);
+        const double delta_2 = (k == L - 1) ? 0.5 * je : 0.5 * (je + je_next);
+        const double zt = dt / je;
+        const double aa = zt * av_cur / delta_1;
+        const double cc = zt * av_next / delta_2;
+        const double bb = aa + cc + 1.0;
+        const double d = bb - aa * xxp;
+        xxp = cc / d;
+        yup = (aa * yup + ur) / d;
+        yvp = (aa * yvp + vr) / d;
+        PL(s.vxx, k + 1, e, E) = xxp;
+        PL(s.vyu, k + 1, e, E) = yup;
+        PL(s.vyv, k + 1, e, E) = yvp;
+        av_cur = av_next;
         je_prev = je;
         je_cur = je_next;
     }
-    if (visc) {
-        if (mask != 0.0) {
-            double a_Cd = s.C_d_el[e] * ((1.0 - acc) * Cd_bz + 1.0);
-            double de = Je(L - 1) / (Av(K - 1) + Av(L - 1));
-            double ub = U(L - 1), vb = V(L - 1);
-            double friction = de * a_Cd * sqrt(ub * ub + vb * vb);
-            double ut, vt;
-            if (fic_point == 0) {
-                ut = 0.0;
-                vt = 0.0;
-            } else if (fic_point == 1) {
-                double aa = friction + 1.0 - xxp;
-                ut = yup / aa;
-                vt = yvp / aa;
-            } else {
-                double aa = 1.0 / (1.0 - friction) - xxp;
-                ut = yup / aa;
-                vt = yvp / aa;
-            }
-            for (int k = L - 1; k >= 0; --k) {
-                double xx = PL(s.vxx, k + 1, e, E);
-                ut = ut * xx + PL(s.vyu, k + 1, e, E);
-                vt = vt * xx + PL(s.vyv, k + 1, e, E);
-                PL(s.U_rhs, k, e, E) = ut * mask;
-                PL(s.V_rhs, k, e, E) = vt * mask;
-            }
-        } else {
-            for (int k = 0; k < L; ++k) {
-                PL(s.U_rhs, k, e, E) = 0.0;
-                PL(s.V_rhs, k, e, E) = 0.0;
-            }
-        }
+    const double a_Cd = s.C_d_el[e] * ((1.0 - acc) * Cd_bz + 1.0);
+    const double de = c.Je(m, L - 1) / (c.Av(m, s, K - 1) + c.Av(m, s, L - 1));
+    const double ub = PL(s.U_n, L - 1, e, E), vb = PL(s.V_n, L - 1, e, E);
+    const double friction = de * a_Cd * sqrt(ub * ub + vb * vb);
+    double ut, vt;
+    if (fic_point == 0) {
+        ut = 0.0;
+        vt = 0.0;
+    } else if (fic_point == 1) {
+        const double aa = friction + 1.0 - xxp;
+        ut = yup / aa;
+        vt = yvp / aa;
+    } else {
+        const double aa = 1.0 / (1.0 - friction) - xxp;
+        ut = yup / aa;
+        vt = yvp / aa;
+    }
+    for (int k = L - 1; k >= 0; --k) {
+        const double xx = PL(s.vxx, k + 1, e, E);
+        ut = ut * xx + PL(s.vyu, k + 1, e, E);
+        vt = vt * xx + PL(s.vyv, k + 1, e, E);
+        PL(s.U_rhs, k, e, E) = ut * mask;
+        PL(s.V_rhs, k, e, E) = vt * mask;
     }
 }
 
@@ -540,7 +607,7 @@ __device__ __forceinline__ double etaAB3(const DevState &s, int n) {
 }
 
 // AB3 velocity + element depth of compute_ssh_rhs_elem (fv_average_dynamic.F90:66-86)
-__global__ void k_bt_elem1(DevMesh m, DevState s, double Dmin) {
+__global__ __launch_bounds__(256) void k_bt_elem1(DevMesh m, DevState s, double Dmin) {
     int e = TID();
     if (e >= m.E) return;
     const int4 q = m.en[e];
@@ -561,7 +628,7 @@ __global__ void k_bt_elem1(DevMesh m, DevState s, double Dmin) {
 // tidal OB value (fv_average_dynamic.F90:190-216, host-evaluated), the AM4
 // elevation of update_2D_vel (fv_average_dynamic.F90:288-296) and the time-level
 // shift (fv_main.F90:1547-1551).
-__global__ void k_bt_node(DevMesh m, DevState s, double dt_2D, const double *__restrict__ tide) {
+__global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevState s, double dt_2D, const double *__restrict__ tide) {
     int n = TID();
     if (n >= m.N) return;
     double ssh = 0.0;
@@ -590,7 +657,7 @@ __global__ void k_bt_node(DevMesh m, DevState s, double dt_2D, const double *__r
 }
 
 // vel_gradients_2D(UAB): src/fv_gradients.F90:97-176
-__global__ void k_bt_velgrad(DevMesh m, DevState s) {
+__global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevState s) {
     int e = TID();
     if (e >= m.E) return;
     const int4 q = m.en[e];
@@ -629,7 +696,7 @@ __global__ void k_bt_velgrad(DevMesh m, DevState s) {
 // first substep only: dmean_n (fv_main.F90:1467-1473) and compute_puls_vel
 // (fv_3D_vel.F90:257-307).  After k_bt_node's shift eta_n holds eta^{n+1}
 // (= ssh_rhs of the reference) and eta_n_1 holds eta^n.
-__global__ void k_bt_puls(DevMesh m, DevState s, double Dmin) {
+__global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevState s, double Dmin) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E;
@@ -649,7 +716,7 @@ __global__ void k_bt_puls(DevMesh m, DevState s, double Dmin) {
 }
 
 // compute_el2nodes_3D(U_puls, V_puls): src/fv_utilit.F90:669-689
-__global__ void k_el2nodes(DevMesh m, const double *__restrict__ Ue, const double *__restrict__ Ve,
+__global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const double *__restrict__ Ue, const double *__restrict__ Ve,
                            double *__restrict__ Un, double *__restrict__ Vn) {
     int n = TID();
     if (n >= m.N) return;
@@ -673,7 +740,7 @@ __global__ void k_el2nodes(DevMesh m, const double *__restrict__ Ue, const doubl
 
 // momentum_adv_P1_3D_to_2D (fv_advection.F90:676-793) as an element gather,
 // then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513)
-__global__ void k_bt_p1adv(DevMesh m, DevState s) {
+__global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevState s) {
     int e = TID();
     if (e >= m.E) return;
     const int N = m.N, E = m.E;
@@ -741,7 +808,7 @@ __device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevState &s, i
 
 // update_2D_vel (fv_average_dynamic.F90:228-443, 3-D branch) for one element,
 // followed by the velocity shift and U_filt accumulation (fv_main.F90:1569-1606).
-__global__ void k_bt_elem2(DevMesh m, DevState s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
+__global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevState s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
                            int wet_dry, int adv2d, int first) {
     int e = TID();
     if (e >= m.E) return;
@@ -816,14 +883,14 @@ __global__ void k_bt_elem2(DevMesh m, DevState s, double dt_2D, double Dmin, dou
 // ---------------------------------------------------------------------------
 // jacobian: src/fv_main.F90:1228-1252 (node scalar + stored Je)
 // ---------------------------------------------------------------------------
-__global__ void k_jac_node(DevMesh m, DevState s, double Dmin) {
+__global__ __launch_bounds__(256) void k_jac_node(DevMesh m, DevState s, double Dmin) {
     int n = TID();
     if (n >= m.N) return;
     s.a_old[n] = s.a[n];
     s.a[n] = fmax(m.depth[n] + s.eta_n[n], Dmin);
 }
 
-__global__ void k_je(DevMesh m, DevState s) {
+__global__ __launch_bounds__(256) void k_je(DevMesh m, DevState s) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E;
@@ -838,7 +905,7 @@ __global__ void k_je(DevMesh m, DevState s) {
 
 // update_3D_vel: src/fv_3D_vel.F90:201-255, with U_filt_2D *= 1/Mt from
 // oce_barotropic_timestep (fv_main.F90:1670-1677)
-__global__ void k_update3d(DevMesh m, DevState s, double Dmin, double Mt_inv) {
+__global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevState s, double Dmin, double Mt_inv) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E, L = m.L;
@@ -872,32 +939,43 @@ __global__ void k_update3d(DevMesh m, DevState s, double Dmin, double Mt_inv) {
     }
 }
 
-// vert_vel_sigma: src/fv_3D_vel.F90:309-456 as a node gather + bottom-up scan
-__global__ void k_vertvel(DevMesh m, DevState s) {
+// vert_vel_sigma (src/fv_3D_vel.F90:309-456), first part: per node and layer
+// the divergence of the baroclinic transport (U_n_filt - U_filt_2D)*Je,
+// gathered over the node's edges in ascending id (354-437), stored in W.
+__global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevState s) {
+    int n, k;
+    if (!node_level(m.N, m.L, n, k)) return;
+    const int N = m.N, E = m.E;
+    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    double c = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const int cc = m.n2e[j];
+        const int ed = cc >= 0 ? cc : ~cc;
+        const int2 et = m.edtri[ed];
+        const double4 cd = m.ecross[ed];
+        const int e1 = et.x;
+        double f = PL(s.Je, k, e1, E) * ((PL(s.V_nf, k, e1, E) - s.V_filt[e1]) * cd.x
+                                         - (PL(s.U_nf, k, e1, E) - s.U_filt[e1]) * cd.y);
+        if (et.y >= 0) {
+            const int e2 = et.y;
+            f = f + PL(s.Je, k, e2, E) * (-(PL(s.V_nf, k, e2, E) - s.V_filt[e2]) * cd.z
+                                          + (PL(s.U_nf, k, e2, E) - s.U_filt[e2]) * cd.w);
+        }
+        c = (cc >= 0) ? c + f : c - f;
+    }
+    PL(s.W, k, n, N) = c;
+}
+
+// second part: bottom-up cumulative sum and /area (444-450); Wvel(nsigma)
+// stays zero.
+__global__ __launch_bounds__(256) void k_vertvel_scan(DevMesh m, DevState s) {
     int n = TID();
     if (n >= m.N) return;
-    const int N = m.N, E = m.E, L = m.L;
-    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const int N = m.N;
     const double area_inv = 1.0 / m.area[n];
-    double run = 0.0;  // Wvel(nsigma) = 0
-    for (int k = L - 1; k >= 0; --k) {
-        double c = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            int cc = m.n2e[j];
-            int ed = cc >= 0 ? cc : ~cc;
-            const int2 et = m.edtri[ed];
-            const double4 cd = m.ecross[ed];
-            const int e1 = et.x;
-            double f = PL(s.Je, k, e1, E) * ((PL(s.V_nf, k, e1, E) - s.V_filt[e1]) * cd.x
-                                             - (PL(s.U_nf, k, e1, E) - s.U_filt[e1]) * cd.y);
-            if (et.y >= 0) {
-                const int e2 = et.y;
-                f = f + PL(s.Je, k, e2, E) * (-(PL(s.V_nf, k, e2, E) - s.V_filt[e2]) * cd.z
-                                              + (PL(s.U_nf, k, e2, E) - s.U_filt[e2]) * cd.w);
-            }
-            c = (cc >= 0) ? c + f : c - f;
-        }
-        run = c + run;
+    double run = 0.0;
+    for (int k = m.L - 1; k >= 0; --k) {
+        run = PL(s.W, k, n, N) + run;
         PL(s.W, k, n, N) = run * area_inv;
     }
 }
@@ -906,7 +984,7 @@ __global__ void k_vertvel(DevMesh m, DevState s) {
 // tracers: solve_tracers (fv_main.F90:1707-1836)
 // ---------------------------------------------------------------------------
 // global min/max of TF and SF (fv_main.F90:1737-1777): exact, order-free
-__global__ void k_minmax_partial(const double *__restrict__ T, const double *__restrict__ S, size_t n, double *out) {
+__global__ __launch_bounds__(256) void k_minmax_partial(const double *__restrict__ T, const double *__restrict__ S, size_t n, double *out) {
     __shared__ double sm[4][256];
     double tmn = INFINITY, tmx = -INFINITY, smn = INFINITY, smx = -INFINITY;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -934,7 +1012,7 @@ __global__ void k_minmax_partial(const double *__restrict__ T, const double *__r
         for (int c = 0; c < 4; ++c) out[4 * blockIdx.x + c] = sm[c][0];
 }
 
-__global__ void k_minmax_final(const double *part, int nb, double *scal) {
+__global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb, double *scal) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double tmn = part[0], tmx = part[1], smn = part[2], smx = part[3];
     for (int b = 1; b < nb; ++b) {
@@ -948,7 +1026,7 @@ __global__ void k_minmax_final(const double *part, int nb, double *scal) {
 }
 
 // AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444)
-__global__ void k_tr_ab(DevState s, size_t n) {
+__global__ __launch_bounds__(256) void k_tr_ab(DevState s, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double Ta = s.scal[0], Sa = s.scal[1];
@@ -1062,110 +1140,25 @@ __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevState &s
     if (bad) atomicCAS(s.err, 0, bad);
 }
 
-// solve_tracer_upwind (fv_tracer.F90:390-923) followed by
-// tracer_impl_vert_diff (fv_tracer.F90:1319-1494), one thread per node column.
-__global__ void k_tr_upwind(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
-                            double relax2clim_ac) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N, E = m.E, L = m.L, K = m.K;
-    const double Ta = s.scal[0], Sa = s.scal[1];
-    const double a = s.a[n];
-    const double area_inv = 1.0 / m.area[n];
-    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
-    const bool ob = cl_relax && m.index[n] == 2;
+
+// Net normal barotropic transport into an open-boundary node through its
+// boundary edges (fv_tracer.F90:862-873 and the same block of the other
+// schemes), summed in ascending edge id; decides inflow relaxation vs.
+// replacement by climatology.
+__device__ __forceinline__ double ob_inflow(const DevMesh &m, const DevState &s, int p0, int p1, double Dmin) {
     double vel_nor = 0.0;
-    if (ob) {  // fv_tracer.F90:862-873: boundary-edge normal transport
-        for (int j = p0; j < p1; ++j) {
-            int cc = m.n2e[j];
-            int ed = cc >= 0 ? cc : ~cc;
-            if (m.edtri[ed].y >= 0) continue;  // internal edge
-            const int2 nd = m.ednodes[ed];
-            const int e1 = m.edtri[ed].x;
-            const double4 cd = m.ecross[ed];
-            double dmn = dmaxr(Dmin, 0.5 * (s.eta_n[nd.x] + m.depth[nd.x] + s.eta_n[nd.y] + m.depth[nd.y]));
-            double un1 = (s.V_filt[e1] * cd.x - s.U_filt[e1] * cd.y) * dmn;
-            vel_nor = (cc >= 0) ? vel_nor + un1 / m.area[nd.x] : vel_nor - un1 / m.area[nd.y];
-        }
+    for (int j = p0; j < p1; ++j) {
+        const int cc = m.n2e[j];
+        const int ed = cc >= 0 ? cc : ~cc;
+        if (m.edtri[ed].y >= 0) continue;  // internal edge
+        const int2 nd = m.ednodes[ed];
+        const int e1 = m.edtri[ed].x;
+        const double4 cd = m.ecross[ed];
+        double dmn = dmaxr(Dmin, 0.5 * (s.eta_n[nd.x] + m.depth[nd.x] + s.eta_n[nd.y] + m.depth[nd.y]));
+        double un1 = (s.V_filt[e1] * cd.x - s.U_filt[e1] * cd.y) * dmn;
+        vel_nor = (cc >= 0) ? vel_nor + un1 / m.area[nd.x] : vel_nor - un1 / m.area[nd.y];
     }
-    double tvt = upw_vert(m, s, s.abT, n, 1, a);
-    double svt = upw_vert(m, s, s.abS, n, 1, a);
-    for (int k = 0; k < L; ++k) {
-        double tc = 0.0, tt = 0.0, ts = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            int cc = m.n2e[j];
-            int ed = cc >= 0 ? cc : ~cc;
-            const int2 nd = m.ednodes[ed];
-            const int2 et = m.edtri[ed];
-            const double4 cd = m.ecross[ed];
-            const int e1 = et.x;
-            double c1 = PL(s.V_nf, k, e1, E) * cd.x - PL(s.U_nf, k, e1, E) * cd.y;
-            double auxc, taux_, saux_;
-            const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
-            const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
-            if (et.y >= 0) {
-                const int e2 = et.y;
-                double c2 = -PL(s.V_nf, k, e2, E) * cd.z + PL(s.U_nf, k, e2, E) * cd.w;
-                double J1 = PL(s.Je, k, e1, E), J2 = PL(s.Je, k, e2, E);
-                auxc = J1 * c1 + J2 * c2;
-                if (c1 > 0.0 && c2 > 0.0) {
-                    taux_ = J1 * c1 * T2 + J2 * c2 * T2;
-                    saux_ = J1 * c1 * S2 + J2 * c2 * S2;
-                } else if (c1 > 0.0) {
-                    taux_ = J1 * c1 * T2 + J2 * c2 * T1;
-                    saux_ = J1 * c1 * S2 + J2 * c2 * S1;
-                } else if (c2 > 0.0) {
-                    taux_ = J1 * c1 * T1 + J2 * c2 * T2;
-                    saux_ = J1 * c1 * S1 + J2 * c2 * S2;
-                } else {
-                    taux_ = J1 * c1 * T1 + J2 * c2 * T1;
-                    saux_ = J1 * c1 * S1 + J2 * c2 * S1;
-                }
-            } else {
-                auxc = PL(s.Je, k, e1, E) * c1;
-                if (c1 > 0.0) {
-                    taux_ = auxc * T2;
-                    saux_ = auxc * S2;
-                } else {
-                    taux_ = auxc * T1;
-                    saux_ = auxc * S1;
-                }
-            }
-            if (cc >= 0) {
-                tc = tc + auxc;
-                tt = tt + taux_;
-                ts = ts + saux_;
-            } else {
-                tc = tc - auxc;
-                tt = tt - taux_;
-                ts = ts - saux_;
-            }
-        }
-        const double tvb = upw_vert(m, s, s.abT, n, k + 2, a);
-        const double svb = upw_vert(m, s, s.abS, n, k + 2, a);
-        const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
-        const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
-        const double jc = a * m.dsig[k];
-        double rt = ((tt - tab * tc) * area_inv + tab * wdiff + (tvt - tvb)) / jc;
-        double rs = ((ts - sab * tc) * area_inv + sab * wdiff + (svt - svb)) / jc;
-        if (ob) {
-            if (vel_nor > 0.0) {
-                rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tab + Ta));
-                rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sab + Sa));
-            } else {
-                rt = clim_relax * (PL(s.Tclim, k, n, N) - (tab + Ta));
-                rs = clim_relax * (PL(s.Sclim, k, n, N) - (sab + Sa));
-            }
-        }
-        const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
-        PL(s.T_old, k, n, N) = tdm + Ta;
-        PL(s.S_old, k, n, N) = sdm + Sa;
-        PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
-        PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
-        tvt = tvb;
-        svt = svb;
-    }
-    vdiff_column(m, s, n, a, Ta, Sa, dt);
+    return vel_nor;
 }
 
 // Vertical flux through 1-based interface nz for the MUSCL and Miura schemes
@@ -1223,26 +1216,6 @@ __device__ __forceinline__ double hi_vert(const DevMesh &m, const DevState &s, c
     return -Tm * w;
 }
 
-// Net normal barotropic transport into an open-boundary node through its
-// boundary edges (fv_tracer.F90:862-873 and the same block of the other
-// schemes), summed in ascending edge id; decides inflow relaxation vs.
-// replacement by climatology.
-__device__ __forceinline__ double ob_inflow(const DevMesh &m, const DevState &s, int p0, int p1, double Dmin) {
-    double vel_nor = 0.0;
-    for (int j = p0; j < p1; ++j) {
-        const int cc = m.n2e[j];
-        const int ed = cc >= 0 ? cc : ~cc;
-        if (m.edtri[ed].y >= 0) continue;  // internal edge
-        const int2 nd = m.ednodes[ed];
-        const int e1 = m.edtri[ed].x;
-        const double4 cd = m.ecross[ed];
-        double dmn = dmaxr(Dmin, 0.5 * (s.eta_n[nd.x] + m.depth[nd.x] + s.eta_n[nd.y] + m.depth[nd.y]));
-        double un1 = (s.V_filt[e1] * cd.x - s.U_filt[e1] * cd.y) * dmn;
-        vel_nor = (cc >= 0) ? vel_nor + un1 / m.area[nd.x] : vel_nor - un1 / m.area[nd.y];
-    }
-    return vel_nor;
-}
-
 // Element gradient of a node column at level k (tracer_gradient_elements,
 // fv_gradients.F90:180-205), the column value being PL(X) - sub.
 __device__ __forceinline__ double2 elem_grad(const DevMesh &m, const double *X, double sub, int el, int k) {
@@ -1253,61 +1226,144 @@ __device__ __forceinline__ double2 elem_grad(const DevMesh &m, const double *X, 
     return make_double2(sum4(m.gsx[el], v0, v1, v2, v3), sum4(m.gsy[el], v0, v1, v2, v3));
 }
 
-// Node-averaged gradients of the AB-interpolated T and S columns used by the
-// MUSCL reconstruction (fill_up_dn_grad node-average branch,
-// fv_gradients.F90:263-323): area-weighted mean over the node's elements in
-// ascending element id, divided by the summed area at every level.
-__global__ void k_tr_gradn_muscl(DevMesh m, DevState s) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N, L = m.L;
-    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
-    for (int k = 0; k < L; ++k) {
-        double tvol = 0.0, tx = 0.0, ty = 0.0, sx = 0.0, sy = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            const int el = m.n2el[j];
-            const double ea = m.elem_area[el];
-            const double2 gt = elem_grad(m, s.abT, 0.0, el, k);
-            const double2 gs = elem_grad(m, s.abS, 0.0, el, k);
-            tvol = tvol + ea;
-            tx = tx + gt.x * ea;
-            ty = ty + gt.y * ea;
-            sx = sx + gs.x * ea;
-            sy = sy + gs.y * ea;
+// solve_tracer_upwind (fv_tracer.F90:390-923), one thread per node and layer:
+// edge fluxes gathered in ascending edge id, QUICK vertical fluxes at the
+// layer's two interfaces, OB relaxation, AB update.  The implicit vertical
+// diffusion follows in k_tr_vdiff.
+__global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
+                            double relax2clim_ac) {
+    int n, k;
+    if (!node_level(m.N, m.L, n, k)) return;
+    const int N = m.N, E = m.E;
+    const double Ta = s.scal[0], Sa = s.scal[1];
+    const double a = s.a[n];
+    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const bool ob = cl_relax && m.index[n] == 2;
+    double tc = 0.0, tt = 0.0, ts = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const int cc = m.n2e[j];
+        const int ed = cc >= 0 ? cc : ~cc;
+        const int2 nd = m.ednodes[ed];
+        const int2 et = m.edtri[ed];
+        const double4 cd = m.ecross[ed];
+        const int e1 = et.x;
+        const double c1 = PL(s.V_nf, k, e1, E) * cd.x - PL(s.U_nf, k, e1, E) * cd.y;
+        const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
+        const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
+        double auxc, taux_, saux_;
+        if (et.y >= 0) {
+            const int e2 = et.y;
+            const double c2 = -PL(s.V_nf, k, e2, E) * cd.z + PL(s.U_nf, k, e2, E) * cd.w;
+            const double J1 = PL(s.Je, k, e1, E), J2 = PL(s.Je, k, e2, E);
+            auxc = J1 * c1 + J2 * c2;
+            if (c1 > 0.0 && c2 > 0.0) {
+                taux_ = J1 * c1 * T2 + J2 * c2 * T2;
+                saux_ = J1 * c1 * S2 + J2 * c2 * S2;
+            } else if (c1 > 0.0) {
+                taux_ = J1 * c1 * T2 + J2 * c2 * T1;
+                saux_ = J1 * c1 * S2 + J2 * c2 * S1;
+            } else if (c2 > 0.0) {
+                taux_ = J1 * c1 * T1 + J2 * c2 * T2;
+                saux_ = J1 * c1 * S1 + J2 * c2 * S2;
+            } else {
+                taux_ = J1 * c1 * T1 + J2 * c2 * T1;
+                saux_ = J1 * c1 * S1 + J2 * c2 * S1;
+            }
+        } else {
+            auxc = PL(s.Je, k, e1, E) * c1;
+            if (c1 > 0.0) {
+                taux_ = auxc * T2;
+                saux_ = auxc * S2;
+            } else {
+                taux_ = auxc * T1;
+                saux_ = auxc * S1;
+            }
         }
-        PL(s.gTx, k, n, N) = tx / tvol;
-        PL(s.gTy, k, n, N) = ty / tvol;
-        PL(s.gSx, k, n, N) = sx / tvol;
-        PL(s.gSy, k, n, N) = sy / tvol;
+        if (cc >= 0) {
+            tc = tc + auxc;
+            tt = tt + taux_;
+            ts = ts + saux_;
+        } else {
+            tc = tc - auxc;
+            tt = tt - taux_;
+            ts = ts - saux_;
+        }
     }
+    const double tvt = upw_vert(m, s, s.abT, n, k + 1, a), tvb = upw_vert(m, s, s.abT, n, k + 2, a);
+    const double svt = upw_vert(m, s, s.abS, n, k + 1, a), svb = upw_vert(m, s, s.abS, n, k + 2, a);
+    const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
+    const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+    const double jc = a * m.dsig[k];
+    const double area_inv = 1.0 / m.area[n];
+    double rt = ((tt - tab * tc) * area_inv + tab * wdiff + (tvt - tvb)) / jc;
+    double rs = ((ts - sab * tc) * area_inv + sab * wdiff + (svt - svb)) / jc;
+    if (ob) {  // fv_tracer.F90:832-895, on the AB-interpolated tracer
+        if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) {
+            rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tab + Ta));
+            rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sab + Sa));
+        } else {
+            rt = clim_relax * (PL(s.Tclim, k, n, N) - (tab + Ta));
+            rs = clim_relax * (PL(s.Sclim, k, n, N) - (sab + Sa));
+        }
+    }
+    const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
+    PL(s.T_old, k, n, N) = tdm + Ta;
+    PL(s.S_old, k, n, N) = sdm + Sa;
+    PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
+    PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
 }
 
-// Node-averaged gradient of the current de-meaned T column for the Miura
-// scheme (fv_tracer.F90:984-998): the same mean, with 1/(summed area)
-// formed once per node.  Also stages the de-meaned column itself in tcur:
-// the Miura update overwrites TF while neighbouring columns still read it.
-__global__ void k_tr_gradn_miura(DevMesh m, DevState s) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N, L = m.L;
+// Node-averaged gradients of the AB-interpolated T and S used by the MUSCL
+// reconstruction (fill_up_dn_grad node-average branch,
+// fv_gradients.F90:263-323): area-weighted mean over the node's elements in
+// ascending element id, divided by the summed area.
+__global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevState s) {
+    int n, k;
+    if (!node_level(m.N, m.L, n, k)) return;
+    const int N = m.N;
+    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    double tvol = 0.0, tx = 0.0, ty = 0.0, sx = 0.0, sy = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const int el = m.n2el[j];
+        const double ea = m.elem_area[el];
+        const double2 gt = elem_grad(m, s.abT, 0.0, el, k);
+        const double2 gs = elem_grad(m, s.abS, 0.0, el, k);
+        tvol = tvol + ea;
+        tx = tx + gt.x * ea;
+        ty = ty + gt.y * ea;
+        sx = sx + gs.x * ea;
+        sy = sy + gs.y * ea;
+    }
+    PL(s.gTx, k, n, N) = tx / tvol;
+    PL(s.gTy, k, n, N) = ty / tvol;
+    PL(s.gSx, k, n, N) = sx / tvol;
+    PL(s.gSy, k, n, N) = sy / tvol;
+}
+
+// Node-averaged gradient of the current de-meaned T for the Miura scheme
+// (fv_tracer.F90:984-998): the same mean, with 1/(summed area) formed first.
+// Also stages the de-meaned value itself in tcur: the Miura update
+// overwrites TF while neighbouring nodes still read it.
+__global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevState s) {
+    int n, k;
+    if (!node_level(m.N, m.L, n, k)) return;
+    const int N = m.N;
     const double Ta = s.scal[0];
     const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
     double ssum = 0.0;
     for (int j = p0; j < p1; ++j) ssum = ssum + m.elem_area[m.n2el[j]];
     const double tvol = 1.0 / ssum;
-    for (int k = 0; k < L; ++k) {
-        double tx = 0.0, ty = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            const int el = m.n2el[j];
-            const double ea = m.elem_area[el];
-            const double2 g = elem_grad(m, s.TF, Ta, el, k);
-            tx = tx + g.x * ea;
-            ty = ty + g.y * ea;
-        }
-        PL(s.gTx, k, n, N) = tx * tvol;
-        PL(s.gTy, k, n, N) = ty * tvol;
-        PL(s.tcur, k, n, N) = PL(s.TF, k, n, N) - Ta;
+    double tx = 0.0, ty = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const int el = m.n2el[j];
+        const double ea = m.elem_area[el];
+        const double2 g = elem_grad(m, s.TF, Ta, el, k);
+        tx = tx + g.x * ea;
+        ty = ty + g.y * ea;
     }
+    PL(s.gTx, k, n, N) = tx * tvol;
+    PL(s.gTy, k, n, N) = ty * tvol;
+    PL(s.tcur, k, n, N) = PL(s.TF, k, n, N) - Ta;
 }
 
 // MUSCL edge value: linear reconstructions from both end nodes, 15 %
@@ -1319,109 +1375,105 @@ __device__ __forceinline__ double muscl_mean(double T1, double T2, double dT, do
     return from2 ? 0.15 * Tm2 + 0.85 * (Tm1 + Tm2) / 2.0 : 0.15 * Tm1 + 0.85 * (Tm1 + Tm2) / 2.0;
 }
 
-// solve_tracer_muscl for T and S (fv_tracer.F90:8-379) followed by
-// tracer_impl_vert_diff, one thread per node column.  Per edge the first
-// segment uses the filtered velocity of element 1; the second segment's flux
-// uses the unfiltered U_n/V_n of element 2 (its upwind test and the volume
-// flux use the filtered one), as the reference does.
-__global__ void k_tr_muscl(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
+// solve_tracer_muscl for T and S (fv_tracer.F90:8-379), one thread per node
+// and layer.  Per edge the first segment uses the filtered velocity of
+// element 1; the second segment's flux uses the unfiltered U_n/V_n of
+// element 2 (its upwind test and the volume flux use the filtered one), as
+// the reference does.  The implicit vertical diffusion follows in k_tr_vdiff.
+__global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
                            double relax2clim_ac) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N, E = m.E, L = m.L;
+    int n, k;
+    if (!node_level(m.N, m.L, n, k)) return;
+    const int N = m.N, E = m.E;
     const double Ta = s.scal[0], Sa = s.scal[1];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
-    const bool ob = cl_relax && m.index[n] == 2;
-    const double vel_nor = ob ? ob_inflow(m, s, p0, p1, Dmin) : 0.0;
-    double TmT = 0.0, TmS = 0.0;
-    double tvt = hi_vert<true>(m, s, s.abT, 0.0, n, 1, a, TmT);
-    double svt = hi_vert<true>(m, s, s.abS, 0.0, n, 1, a, TmS);
-    for (int k = 0; k < L; ++k) {
-        double rt = 0.0, rs = 0.0, rc = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            const int cc = m.n2e[j];
-            const int ed = cc >= 0 ? cc : ~cc;
-            const int2 nd = m.ednodes[ed];
-            const int2 et = m.edtri[ed];
-            const double4 cd = m.ecross[ed];
-            const double2 dd = m.edxdy[ed];
-            const int e1 = et.x;
-            const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
-            const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
-            const double gT1x = PL(s.gTx, k, nd.x, N), gT2x = PL(s.gTx, k, nd.y, N);
-            const double gT1y = PL(s.gTy, k, nd.x, N), gT2y = PL(s.gTy, k, nd.y, N);
-            const double gS1x = PL(s.gSx, k, nd.x, N), gS2x = PL(s.gSx, k, nd.y, N);
-            const double gS1y = PL(s.gSy, k, nd.x, N), gS2y = PL(s.gSy, k, nd.y, N);
-            const double dTt = 2.0 * (T2 - T1), dTs = 2.0 * (S2 - S1);
-            // first segment
-            const double dxa = dd.x * (kREarthD * m.elem_cos[e1]);
-            const double u1 = PL(s.U_nf, k, e1, E), v1 = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
-            const bool f2 = v1 * cd.x - u1 * cd.y > 0;
-            const double mT = muscl_mean(T1, T2, dTt, dxa, dd.y, gT1x, gT1y, gT2x, gT2y, f2);
-            const double mS = muscl_mean(S1, S2, dTs, dxa, dd.y, gS1x, gS1y, gS2x, gS2y, f2);
-            double cT = (J1 * v1 * mT) * cd.x - (J1 * u1 * mT) * cd.y;
-            double cS = (J1 * v1 * mS) * cd.x - (J1 * u1 * mS) * cd.y;
-            double cV = J1 * v1 * cd.x - J1 * u1 * cd.y;
+    double rt = 0.0, rs = 0.0, rc = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const int cc = m.n2e[j];
+        const int ed = cc >= 0 ? cc : ~cc;
+        const int2 nd = m.ednodes[ed];
+        const int2 et = m.edtri[ed];
+        const double4 cd = m.ecross[ed];
+        const double2 dd = m.edxdy[ed];
+        const int e1 = et.x;
+        const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
+        const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
+        const double gT1x = PL(s.gTx, k, nd.x, N), gT2x = PL(s.gTx, k, nd.y, N);
+        const double gT1y = PL(s.gTy, k, nd.x, N), gT2y = PL(s.gTy, k, nd.y, N);
+        const double gS1x = PL(s.gSx, k, nd.x, N), gS2x = PL(s.gSx, k, nd.y, N);
+        const double gS1y = PL(s.gSy, k, nd.x, N), gS2y = PL(s.gSy, k, nd.y, N);
+        const double dTt = 2.0 * (T2 - T1), dTs = 2.0 * (S2 - S1);
+        // first segment
+        const double dxa = dd.x * (kREarthD * m.elem_cos[e1]);
+        const double u1 = PL(s.U_nf, k, e1, E), v1 = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
+        const bool f2 = v1 * cd.x - u1 * cd.y > 0;
+        const double mT = muscl_mean(T1, T2, dTt, dxa, dd.y, gT1x, gT1y, gT2x, gT2y, f2);
+        const double mS = muscl_mean(S1, S2, dTs, dxa, dd.y, gS1x, gS1y, gS2x, gS2y, f2);
+        const double cT = (J1 * v1 * mT) * cd.x - (J1 * u1 * mT) * cd.y;
+        const double cS = (J1 * v1 * mS) * cd.x - (J1 * u1 * mS) * cd.y;
+        const double cV = J1 * v1 * cd.x - J1 * u1 * cd.y;
+        if (cc >= 0) {
+            rt = rt + cT;
+            rs = rs + cS;
+        } else {
+            rt = rt - cT;
+            rs = rs - cS;
+        }
+        if (et.y >= 0) {  // second segment
+            const int e2 = et.y;
+            const double dxb = dd.x * (kREarthD * m.elem_cos[e2]);
+            const double u2f = PL(s.U_nf, k, e2, E), v2f = PL(s.V_nf, k, e2, E);
+            const double u2 = PL(s.U_n, k, e2, E), v2 = PL(s.V_n, k, e2, E), J2 = PL(s.Je, k, e2, E);
+            const bool g2 = v2f * cd.z - u2f * cd.w < 0;
+            const double nT = muscl_mean(T1, T2, dTt, dxb, dd.y, gT1x, gT1y, gT2x, gT2y, g2);
+            const double nS = muscl_mean(S1, S2, dTs, dxb, dd.y, gS1x, gS1y, gS2x, gS2y, g2);
+            const double c2T = -(J2 * v2 * nT) * cd.z + (J2 * u2 * nT) * cd.w;
+            const double c2S = -(J2 * v2 * nS) * cd.z + (J2 * u2 * nS) * cd.w;
+            const double c2V = -J2 * v2f * cd.z + J2 * u2f * cd.w;
             if (cc >= 0) {
-                rt = rt + cT;
-                rs = rs + cS;
+                rt = rt + c2T;
+                rs = rs + c2S;
+                rc = rc + cV;
+                rc = rc + c2V;
             } else {
-                rt = rt - cT;
-                rs = rs - cS;
+                rt = rt - c2T;
+                rs = rs - c2S;
+                rc = rc - cV;
+                rc = rc - c2V;
             }
-            if (et.y >= 0) {  // second segment
-                const int e2 = et.y;
-                const double dxb = dd.x * (kREarthD * m.elem_cos[e2]);
-                const double u2f = PL(s.U_nf, k, e2, E), v2f = PL(s.V_nf, k, e2, E);
-                const double u2 = PL(s.U_n, k, e2, E), v2 = PL(s.V_n, k, e2, E), J2 = PL(s.Je, k, e2, E);
-                const bool g2 = v2f * cd.z - u2f * cd.w < 0;
-                const double nT = muscl_mean(T1, T2, dTt, dxb, dd.y, gT1x, gT1y, gT2x, gT2y, g2);
-                const double nS = muscl_mean(S1, S2, dTs, dxb, dd.y, gS1x, gS1y, gS2x, gS2y, g2);
-                const double c2T = -(J2 * v2 * nT) * cd.z + (J2 * u2 * nT) * cd.w;
-                const double c2S = -(J2 * v2 * nS) * cd.z + (J2 * u2 * nS) * cd.w;
-                const double c2V = -J2 * v2f * cd.z + J2 * u2f * cd.w;
-                if (cc >= 0) {
-                    rt = rt + c2T;
-                    rs = rs + c2S;
-                    rc = rc + cV;
-                    rc = rc + c2V;
-                } else {
-                    rt = rt - c2T;
-                    rs = rs - c2S;
-                    rc = rc - cV;
-                    rc = rc - c2V;
-                }
-            } else {
-                rc = (cc >= 0) ? rc + cV : rc - cV;
-            }
+        } else {
+            rc = (cc >= 0) ? rc + cV : rc - cV;
         }
-        const double tvb = hi_vert<true>(m, s, s.abT, 0.0, n, k + 2, a, TmT);
-        const double svb = hi_vert<true>(m, s, s.abS, 0.0, n, k + 2, a, TmS);
-        const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
-        const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
-        const double jc = a * m.dsig[k];
-        const double aj = m.area[n] * jc;
-        rt = rt / aj - tab * rc / aj + tab * wdiff / jc + (tvt - tvb) / jc;
-        rs = rs / aj - sab * rc / aj + sab * wdiff / jc + (svt - svb) / jc;
-        const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
-        if (ob) {  // climatology relaxation on the current de-meaned tracer
-            if (vel_nor > 0.0) {
-                rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
-                rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sdm + Sa));
-            } else {
-                rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
-                rs = clim_relax * (PL(s.Sclim, k, n, N) - (sdm + Sa));
-            }
-        }
-        PL(s.T_old, k, n, N) = tdm + Ta;
-        PL(s.S_old, k, n, N) = sdm + Sa;
-        PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
-        PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
-        tvt = tvb;
-        svt = svb;
     }
-    vdiff_column(m, s, n, a, Ta, Sa, dt);
+    // QUICK vertical fluxes at the layer's interfaces; when W is exactly zero
+    // the reference's carried Tm multiplies a zero, so any finite start value
+    // gives the same bits.
+    double TmT = 0.0, TmS = 0.0;
+    const double tvt = hi_vert<true>(m, s, s.abT, 0.0, n, k + 1, a, TmT);
+    const double tvb = hi_vert<true>(m, s, s.abT, 0.0, n, k + 2, a, TmT);
+    const double svt = hi_vert<true>(m, s, s.abS, 0.0, n, k + 1, a, TmS);
+    const double svb = hi_vert<true>(m, s, s.abS, 0.0, n, k + 2, a, TmS);
+    const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
+    const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+    const double jc = a * m.dsig[k];
+    const double aj = m.area[n] * jc;
+    rt = rt / aj - tab * rc / aj + tab * wdiff / jc + (tvt - tvb) / jc;
+    rs = rs / aj - sab * rc / aj + sab * wdiff / jc + (svt - svb) / jc;
+    const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
+    if (cl_relax && m.index[n] == 2) {  // climatology relaxation on the current de-meaned tracer
+        if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) {
+            rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+            rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sdm + Sa));
+        } else {
+            rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+            rs = clim_relax * (PL(s.Sclim, k, n, N) - (sdm + Sa));
+        }
+    }
+    PL(s.T_old, k, n, N) = tdm + Ta;
+    PL(s.S_old, k, n, N) = sdm + Sa;
+    PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
+    PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
 }
 
 // mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
@@ -1433,109 +1485,105 @@ __device__ __forceinline__ double mask_ad(const DevMesh &m, const DevState &s, i
     return (dm - 0.0) / (0.1 - 0.0);
 }
 
-// solve_tracer_miura for T (fv_tracer.F90:930-1316) followed by
-// tracer_impl_vert_diff of T and S (S is not advected under this scheme),
-// one thread per node column.  The reconstruction keeps the reference's
-// unit metric factors a = b = 1 and its first-segment y term without *dt.
-__global__ void k_tr_miura(DevMesh m, DevState s, double dt, double Dmin, double scale_area, int cl_relax,
+// solve_tracer_miura for T (fv_tracer.F90:930-1316), one thread per node and
+// layer; S is not advected under this scheme.  The reconstruction keeps the
+// reference's unit metric factors a = b = 1 and its first-segment y term
+// without *dt.  The implicit vertical diffusion of T and S follows in
+// k_tr_vdiff.
+__global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevState s, double dt, double Dmin, double scale_area, int cl_relax,
                            double clim_relax, double relax2clim_ac) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N, E = m.E, L = m.L;
-    const double Ta = s.scal[0], Sa = s.scal[1];
+    int n, k;
+    if (!node_level(m.N, m.L, n, k)) return;
+    const int N = m.N, E = m.E;
+    const double Ta = s.scal[0];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
-    const bool ob = cl_relax && m.index[n] == 2;
-    const double vel_nor = ob ? ob_inflow(m, s, p0, p1, Dmin) : 0.0;
     const double ab = 1.0;
-    double Tmv = 0.0;
-    double tvt = hi_vert<false>(m, s, s.tcur, 0.0, n, 1, a, Tmv);
-    for (int k = 0; k < L; ++k) {
-        double rt = 0.0, rc = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            const int cc = m.n2e[j];
-            const int ed = cc >= 0 ? cc : ~cc;
-            const int2 nd = m.ednodes[ed];
-            const int2 et = m.edtri[ed];
-            const double4 cd = m.ecross[ed];
-            const double2 dd = m.edxdy[ed];
-            const int e1 = et.x, e2 = et.y;
-            const double crt1 = 0.5 * (mask_ad(m, s, nd.x, Dmin) + mask_ad(m, s, nd.y, Dmin));
-            double Kh = m.elem_area[e1];
-            if (e2 >= 0) Kh = 0.5 * (Kh + m.elem_area[e2]);
-            Kh = 0.010 * Kh / scale_area;
-            const double T1 = PL(s.tcur, k, nd.x, N), T2 = PL(s.tcur, k, nd.y, N);
-            const double tx1 = PL(s.gTx, k, nd.x, N), tx2 = PL(s.gTx, k, nd.y, N);
-            const double ty1 = PL(s.gTy, k, nd.x, N), ty2 = PL(s.gTy, k, nd.y, N);
-            const double2 g1 = elem_grad(m, s.tcur, 0.0, e1, k);
-            double Tx, Ty;
-            double2 g2 = make_double2(0.0, 0.0);
-            if (e2 >= 0) {
-                g2 = elem_grad(m, s.tcur, 0.0, e2, k);
-                Tx = 0.5 * Kh * (g1.x + g2.x);
-                Ty = 0.5 * Kh * (g1.y + g2.y);
-            } else {
-                Tx = Kh * g1.x;
-                Ty = Kh * g1.y;
-            }
-            const double dX1 = cd.x, dY1 = cd.y;
-            const double u = PL(s.U_nf, k, e1, E), v = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
-            double Tm;
-            if (v * dX1 - u * dY1 > 0) {
-                Tm = T2 + (0.5 * (-dd.x * ab + dX1 - u * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY1 - v) * ty2) * crt1;
-            } else {
-                Tm = T1 + (0.5 * (dd.x * ab + dX1 - u * dt) * tx1 + 0.5 * (dd.y * kREarthD + dY1 - v * dt) * ty1) * crt1;
-            }
-            const double c1 = (J1 * v * Tm - Ty) * dX1 - (J1 * u * Tm - Tx) * dY1;
-            const double cV = J1 * v * dX1 - J1 * u * dY1;
-            rt = (cc >= 0) ? rt + c1 : rt - c1;
-            if (e2 >= 0) {
-                const double dX2 = cd.z, dY2 = cd.w;
-                const double Tx2 = 0.5 * Kh * (g1.x + g2.x), Ty2 = 0.5 * Kh * (g1.y + g2.y);
-                const double u2 = PL(s.U_nf, k, e2, E), v2 = PL(s.V_nf, k, e2, E), J2 = PL(s.Je, k, e2, E);
-                double Tn;
-                if (v2 * dX2 - u2 * dY2 < 0) {
-                    Tn = T2 + (0.5 * (-dd.x * ab + dX2 - u2 * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY2 - v2 * dt) * ty2) *
-                                  crt1;
-                } else {
-                    Tn = T1 + (0.5 * (dd.x * ab + dX2 - u2 * dt) * tx1 + 0.5 * (dd.y * kREarthD + dY2 - v2 * dt) * ty1) *
-                                  crt1;
-                }
-                const double c2 = -(J2 * v2 * Tn - Ty2) * dX2 + (J2 * u2 * Tn - Tx2) * dY2;
-                const double c2V = -J2 * v2 * dX2 + J2 * u2 * dY2;
-                if (cc >= 0) {
-                    rt = rt + c2;
-                    rc = rc + cV;
-                    rc = rc + c2V;
-                } else {
-                    rt = rt - c2;
-                    rc = rc - cV;
-                    rc = rc - c2V;
-                }
-            } else {
-                rc = (cc >= 0) ? rc + cV : rc - cV;
-            }
+    double rt = 0.0, rc = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const int cc = m.n2e[j];
+        const int ed = cc >= 0 ? cc : ~cc;
+        const int2 nd = m.ednodes[ed];
+        const int2 et = m.edtri[ed];
+        const double4 cd = m.ecross[ed];
+        const double2 dd = m.edxdy[ed];
+        const int e1 = et.x, e2 = et.y;
+        const double crt1 = 0.5 * (mask_ad(m, s, nd.x, Dmin) + mask_ad(m, s, nd.y, Dmin));
+        double Kh = m.elem_area[e1];
+        if (e2 >= 0) Kh = 0.5 * (Kh + m.elem_area[e2]);
+        Kh = 0.010 * Kh / scale_area;
+        const double T1 = PL(s.tcur, k, nd.x, N), T2 = PL(s.tcur, k, nd.y, N);
+        const double tx1 = PL(s.gTx, k, nd.x, N), tx2 = PL(s.gTx, k, nd.y, N);
+        const double ty1 = PL(s.gTy, k, nd.x, N), ty2 = PL(s.gTy, k, nd.y, N);
+        const double2 g1 = elem_grad(m, s.tcur, 0.0, e1, k);
+        double Tx, Ty;
+        double2 g2 = make_double2(0.0, 0.0);
+        if (e2 >= 0) {
+            g2 = elem_grad(m, s.tcur, 0.0, e2, k);
+            Tx = 0.5 * Kh * (g1.x + g2.x);
+            Ty = 0.5 * Kh * (g1.y + g2.y);
+        } else {
+            Tx = Kh * g1.x;
+            Ty = Kh * g1.y;
         }
-        const double tvb = hi_vert<false>(m, s, s.tcur, 0.0, n, k + 2, a, Tmv);
-        const double tab = PL(s.abT, k, n, N);
-        const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
-        const double jc = a * m.dsig[k];
-        rt = ((rt - tab * rc) / m.area[n] + tab * wdiff + tvt - tvb) / jc;
-        const double tdm = PL(s.tcur, k, n, N);
-        if (ob) {
-            if (vel_nor > 0.0) rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
-            else rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+        const double dX1 = cd.x, dY1 = cd.y;
+        const double u = PL(s.U_nf, k, e1, E), v = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
+        double Tm;
+        if (v * dX1 - u * dY1 > 0) {
+            Tm = T2 + (0.5 * (-dd.x * ab + dX1 - u * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY1 - v) * ty2) * crt1;
+        } else {
+            Tm = T1 + (0.5 * (dd.x * ab + dX1 - u * dt) * tx1 + 0.5 * (dd.y * kREarthD + dY1 - v * dt) * ty1) * crt1;
         }
-        PL(s.T_old, k, n, N) = tdm + Ta;
-        PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
-        tvt = tvb;
+        const double c1 = (J1 * v * Tm - Ty) * dX1 - (J1 * u * Tm - Tx) * dY1;
+        const double cV = J1 * v * dX1 - J1 * u * dY1;
+        rt = (cc >= 0) ? rt + c1 : rt - c1;
+        if (e2 >= 0) {
+            const double dX2 = cd.z, dY2 = cd.w;
+            const double Tx2 = 0.5 * Kh * (g1.x + g2.x), Ty2 = 0.5 * Kh * (g1.y + g2.y);
+            const double u2 = PL(s.U_nf, k, e2, E), v2 = PL(s.V_nf, k, e2, E), J2 = PL(s.Je, k, e2, E);
+            double Tn;
+            if (v2 * dX2 - u2 * dY2 < 0) {
+                Tn = T2 + (0.5 * (-dd.x * ab + dX2 - u2 * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY2 - v2 * dt) * ty2) *
+                              crt1;
+            } else {
+                Tn = T1 + (0.5 * (dd.x * ab + dX2 - u2 * dt) * tx1 + 0.5 * (dd.y * kREarthD + dY2 - v2 * dt) * ty1) *
+                              crt1;
+            }
+            const double c2 = -(J2 * v2 * Tn - Ty2) * dX2 + (J2 * u2 * Tn - Tx2) * dY2;
+            const double c2V = -J2 * v2 * dX2 + J2 * u2 * dY2;
+            if (cc >= 0) {
+                rt = rt + c2;
+                rc = rc + cV;
+                rc = rc + c2V;
+            } else {
+                rt = rt - c2;
+                rc = rc - cV;
+                rc = rc - c2V;
+            }
+        } else {
+            rc = (cc >= 0) ? rc + cV : rc - cV;
+        }
     }
-    vdiff_column(m, s, n, a, Ta, Sa, dt);
+    double Tmv = 0.0;
+    const double tvt = hi_vert<false>(m, s, s.tcur, 0.0, n, k + 1, a, Tmv);
+    const double tvb = hi_vert<false>(m, s, s.tcur, 0.0, n, k + 2, a, Tmv);
+    const double tab = PL(s.abT, k, n, N);
+    const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+    const double jc = a * m.dsig[k];
+    rt = ((rt - tab * rc) / m.area[n] + tab * wdiff + tvt - tvb) / jc;
+    const double tdm = PL(s.tcur, k, n, N);
+    if (cl_relax && m.index[n] == 2) {
+        if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+        else rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+    }
+    PL(s.T_old, k, n, N) = tdm + Ta;
+    PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
 }
 
-// Tracer step without horizontal advection (tracer_adv = 2 built without
-// MUSCL_CALL, fv_main.F90:1790-1800): implicit vertical diffusion only.
-__global__ void k_tr_vdiff(DevMesh m, DevState s, double dt) {
+// tracer_impl_vert_diff of T and S (fv_tracer.F90:1319-1494), one thread per
+// node column; after every advection scheme, and alone for tracer_adv = 2
+// without MUSCL_CALL (fv_main.F90:1788-1800).
+__global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevState s, double dt) {
     int n = TID();
     if (n >= m.N) return;
     vdiff_column(m, s, n, s.a[n], s.scal[0], s.scal[1], dt);
@@ -1546,7 +1594,7 @@ __global__ void k_tr_vdiff(DevMesh m, DevState s, double dt) {
 // content, extrema -- fixed-order block tree, deterministic run to run
 // ---------------------------------------------------------------------------
 constexpr int kDiag = 12;
-__global__ void k_diag_partial(DevMesh m, DevState s, double Dmin, double *out) {
+__global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevState s, double Dmin, double *out) {
     __shared__ double sm[kDiag][256];
     double v[kDiag] = {0, 0, 0, 0, 0, 0, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
     // v: 0 PE, 1 area, 2 KE, 3 volume, 4 heat, 5 salt, 6/7 eta, 8/9 T, 10/11 S
@@ -1595,7 +1643,7 @@ __global__ void k_diag_partial(DevMesh m, DevState s, double Dmin, double *out) 
         for (int c = 0; c < kDiag; ++c) out[kDiag * blockIdx.x + c] = sm[c][0];
 }
 
-__global__ void k_diag_final(const double *part, int nb, double *res) {
+__global__ __launch_bounds__(256) void k_diag_final(const double *part, int nb, double *res) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double v[kDiag];
     for (int c = 0; c < kDiag; ++c) v[c] = part[c];
@@ -1609,7 +1657,7 @@ __global__ void k_diag_final(const double *part, int nb, double *res) {
     for (int c = 0; c < kDiag; ++c) res[c] = v[c];
 }
 
-__global__ void k_mask_count(const double *mask, int E, int *out) {
+__global__ __launch_bounds__(256) void k_mask_count(const double *mask, int E, int *out) {
     __shared__ int sm[256];
     int c = 0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) c += mask[e] != 0.0;
