@@ -62,6 +62,19 @@ def kernel_bytes(k, N, E, D, L, K):
     raise KeyError(k)
 
 
+def pmc_traffic(kernel, path=os.path.join(ROOT, "profiles", "pmc_traffic_fp64_step_c2.md")):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes over this workload (copy-calibrated), or None."""
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 5 and cells[0] == kernel:
+                return float(cells[4])
+    except OSError:
+        pass
+    return None
+
+
 def build_model(name, scale=None, precision=0, closure=0):
     import numpy as np
     import __graft_entry__ as ge
@@ -123,7 +136,20 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--probe", default="k_tr_muscl")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        x = torch.rand(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs one FESOM-C mesh partition on one GPU; --gpus {args.gpus} is not "
               "supported (the multi-GPU halo-exchange design is documented, not built)", file=sys.stderr)
@@ -171,7 +197,8 @@ def main():
                                f"Mt {model.par.Mt}, wet/dry, MUSCL tracers, plain FP64 drop-in",
                    "nodes": N, "elements": E, "layers": L, "Mt": model.par.Mt, "path": "fp64"},
         "roofline": {"kernel": args.probe, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic(args.probe) if args.config == "c2" else None,
                      "avg_ms": avg_ms, "launches": k_n, "algorithmic_bytes": kb},
         "setup_s": setup_s,
         "energy": diag["energy"],

@@ -90,6 +90,7 @@ typedef struct {
     double *hpressure, *Bar_pru_3D, *Bar_prv_3D, *Bar_pr_2D, *rho_c; /* KN LE LE 2E LN */
     double *Unode, *Vnode, *Unode_p, *Vnode_p; /* LN */
     double *Kv, *Av, *bt, *snu, *Ri, *tke_dissip; /* KN KE KN KN KN KN */
+    double *d3_iter; /* N: d3_end iterations of the last call (diagnostic) */
     double *U_rhs, *V_rhs, *U_rhsAB, *V_rhsAB, *UV_rhs; /* LE, UV_rhs 2LE */
     double *vel_grad_ux, *vel_grad_uy, *vel_grad_vx, *vel_grad_vy; /* LE */
     double *U_puls, *V_puls, *U_n, *V_n, *U_n_filt, *V_n_filt; /* LE */
@@ -830,6 +831,7 @@ static void d3_end(orc_t *o) {
                 double c = fabs(a / b);
                 if (c > bbb_max) bbb_max = c;
             }
+            o->d3_iter[node] = it;
             if (bbb_max < 1.0e-6) break;
             for (int nz = 1; nz <= K; nz++) btr[nz] = btr1[nz];
         }
@@ -2156,7 +2158,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     o->hpressure = DALLOC(NK); o->Bar_pru_3D = DALLOC(EL); o->Bar_prv_3D = DALLOC(EL); o->Bar_pr_2D = DALLOC(2 * E);
     o->rho_c = DALLOC(NL);
     o->Unode = DALLOC(NL); o->Vnode = DALLOC(NL); o->Unode_p = DALLOC(NL); o->Vnode_p = DALLOC(NL);
-    o->Kv = DALLOC(NK); o->Av = DALLOC(EK); o->bt = DALLOC(NK); o->snu = DALLOC(NK); o->Ri = DALLOC(NK); o->tke_dissip = DALLOC(NK);
+    o->Kv = DALLOC(NK); o->Av = DALLOC(EK); o->bt = DALLOC(NK); o->snu = DALLOC(NK); o->Ri = DALLOC(NK); o->tke_dissip = DALLOC(NK); o->d3_iter = DALLOC(N);
     o->U_rhs = DALLOC(EL); o->V_rhs = DALLOC(EL); o->U_rhsAB = DALLOC(EL); o->V_rhsAB = DALLOC(EL); o->UV_rhs = DALLOC(2 * EL);
     o->vel_grad_ux = DALLOC(EL); o->vel_grad_uy = DALLOC(EL); o->vel_grad_vx = DALLOC(EL); o->vel_grad_vy = DALLOC(EL);
     o->U_puls = DALLOC(EL); o->V_puls = DALLOC(EL); o->U_n = DALLOC(EL); o->V_n = DALLOC(EL);
@@ -2297,7 +2299,7 @@ static double *fld(orc_t *o, const char *name, long *cnt) {
     F("Wvel", o->Wvel, N * K) F("U_filt_2D", o->U_filt_2D, E) F("V_filt_2D", o->V_filt_2D, E)
     F("TF", o->TF, N * L) F("SF", o->SF, N * L) F("T_old", o->T_old, N * L) F("S_old", o->S_old, N * L)
     F("Tclim", o->Tclim, N * L) F("Sclim", o->Sclim, N * L)
-    F("mask_wd", o->mask_wd, E) F("mask_wd_node", o->mask_wd_node, N) F("mask_ad", o->mask_ad, N)
+    F("d3_iter", o->d3_iter, N) F("mask_wd", o->mask_wd, E) F("mask_wd_node", o->mask_wd_node, N) F("mask_ad", o->mask_ad, N)
     F("C_d_el", o->C_d_el, E) F("vel_grad", o->vel_grad, 4 * E) F("dmean_n", o->dmean_n, E)
     F("vel_grad_ux", o->vel_grad_ux, E * L) F("vel_grad_uy", o->vel_grad_uy, E * L)
     F("vel_grad_vx", o->vel_grad_vx, E * L) F("vel_grad_vy", o->vel_grad_vy, E * L)
