@@ -135,6 +135,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--probe", default="k_tr_muscl")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
@@ -157,7 +158,8 @@ def main():
     import torch
 
     t_setup = time.perf_counter()
-    txt, mesh, ex, model = build_model(args.config)
+    precision = 1 if args.precision == "mixed" else 0
+    txt, mesh, ex, model = build_model(args.config, precision=precision)
     N, E, D, L, K = model.N, model.E, model.D, model.L, model.K
     setup_s = time.perf_counter() - t_setup
     model.step(args.warmup)

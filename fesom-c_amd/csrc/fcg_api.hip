@@ -10,34 +10,34 @@
 #include "fcg_internal.h"
 
 namespace fcgk {
-__global__ void k_pressure(DevMesh, DevState, double, double);
-__global__ void k_bpg(DevMesh, DevState);
-__global__ void k_d3end(DevMesh, DevState, double, double, double, double, double, double);
-__global__ void k_velgrad3d(DevMesh, DevState);
-__global__ void k_velrhs(DevMesh, DevState, double, double, int, int);
-__global__ void k_velvisc(DevMesh, DevState, double, double, double, int);
-__global__ void k_bt_elem1(DevMesh, DevState, double);
-__global__ void k_bt_node(DevMesh, DevState, double, const double *);
-__global__ void k_bt_velgrad(DevMesh, DevState);
-__global__ void k_bt_puls(DevMesh, DevState, double);
-__global__ void k_el2nodes(DevMesh, const double *, const double *, double *, double *);
-__global__ void k_bt_p1adv(DevMesh, DevState);
-__global__ void k_bt_elem2(DevMesh, DevState, double, double, double, double, double, int, int, int);
-__global__ void k_jac_node(DevMesh, DevState, double);
-__global__ void k_je(DevMesh, DevState);
-__global__ void k_update3d(DevMesh, DevState, double, double);
-__global__ void k_vertvel_div(DevMesh, DevState);
-__global__ void k_vertvel_scan(DevMesh, DevState);
-__global__ void k_minmax_partial(const double *, const double *, size_t, double *);
+template <typename R> __global__ void k_pressure(DevMesh, DevStateT<R>, double, double);
+template <typename R> __global__ void k_bpg(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, double, double, double);
+template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int);
+template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
+template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
+template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *);
+template <typename R> __global__ void k_bt_velgrad(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_bt_puls(DevMesh, DevStateT<R>, double);
+template <typename R> __global__ void k_el2nodes(DevMesh, const R *, const R *, R *, R *);
+template <typename R> __global__ void k_bt_p1adv(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int);
+template <typename R> __global__ void k_jac_node(DevMesh, DevStateT<R>, double);
+template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_update3d(DevMesh, DevStateT<R>, double, double);
+template <typename R> __global__ void k_vertvel_div(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_vertvel_scan(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_minmax_partial(const R *, const R *, size_t, double *);
 __global__ void k_minmax_final(const double *, int, double *);
-__global__ void k_tr_ab(DevState, size_t);
-__global__ void k_tr_upwind(DevMesh, DevState, double, double, int, double, double);
-__global__ void k_tr_gradn_muscl(DevMesh, DevState);
-__global__ void k_tr_muscl(DevMesh, DevState, double, double, int, double, double);
-__global__ void k_tr_gradn_miura(DevMesh, DevState);
-__global__ void k_tr_miura(DevMesh, DevState, double, double, double, int, double, double);
-__global__ void k_tr_vdiff(DevMesh, DevState, double);
-__global__ void k_diag_partial(DevMesh, DevState, double, double *);
+template <typename R> __global__ void k_tr_ab(DevStateT<R>, size_t);
+template <typename R> __global__ void k_tr_upwind(DevMesh, DevStateT<R>, double, double, int, double, double);
+template <typename R> __global__ void k_tr_gradn_muscl(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_tr_muscl(DevMesh, DevStateT<R>, double, double, int, double, double);
+template <typename R> __global__ void k_tr_gradn_miura(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_tr_miura(DevMesh, DevStateT<R>, double, double, double, int, double, double);
+template <typename R> __global__ void k_tr_vdiff(DevMesh, DevStateT<R>, double);
+template <typename R> __global__ void k_diag_partial(DevMesh, DevStateT<R>, double, double *);
 __global__ void k_diag_final(const double *, int, double *);
 __global__ void k_mask_count(const double *, int, int *);
 }  // namespace fcgk
@@ -75,6 +75,27 @@ int dalloc(fcg_ctx *ctx, T **p, size_t n) {
     return FCG_OK;
 }
 
+int dalloc_bytes(fcg_ctx *ctx, void **p, size_t bytes) {
+    void *q = nullptr;
+    bytes = std::max<size_t>(bytes, 8);
+    FCG_CHECK(hipMalloc(&q, bytes));
+    FCG_CHECK(hipMemset(q, 0, bytes));
+    ctx->allocs.push_back(q);
+    *p = q;
+    return FCG_OK;
+}
+
+// host doubles -> a 3-D device field in the context's storage type
+int upload3(fcg_ctx *ctx, void *dev, const std::vector<double> &h) {
+    if (ctx->mixed) {
+        std::vector<float> f(h.begin(), h.end());
+        FCG_CHECK(hipMemcpy(dev, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        FCG_CHECK(hipMemcpy(dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return FCG_OK;
+}
+
 template <typename T>
 int upload(fcg_ctx *ctx, T **p, const std::vector<T> &h) {
     int rc = dalloc(ctx, p, h.size());
@@ -91,11 +112,12 @@ int upload(fcg_ctx *ctx, T **p, const std::vector<T> &h) {
 
 // 3-D field descriptor for Fortran-layout transfer
 struct Field {
-    double *dev;
+    double *dev;       // float storage when r3 and the context is mixed precision
     long levels;   // levels per column (L or K), or components for 2-D vectors
     long ents;     // entities
     bool interleaved;  // device stores (comp, ent) interleaved like Fortran (2,E)
     bool nodal;        // node field (else element field)
+    bool r3;           // 3-D field held in the context's storage type
 };
 
 bool find_field(fcg_ctx *c, const char *name, Field &f) {
@@ -125,7 +147,8 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
     };
     for (const Ent &t : tab)
         if (!std::strcmp(t.n, name)) {
-            f = Field{t.p, t.lev, t.nodal ? N : E, t.il, t.nodal};
+            const bool r3 = !t.il && (t.lev == L || t.lev == K) && std::strcmp(t.n, "hpressure") != 0;
+            f = Field{t.p, t.lev, t.nodal ? N : E, t.il, t.nodal, r3};
             return true;
         }
     return false;
@@ -152,23 +175,23 @@ void fill_tides(fcg_ctx *c) {
     }
 }
 
-int enqueue_substeps(fcg_ctx *ctx) {
+template <typename R>
+int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
     const DevMesh &m = ctx->m;
-    const DevState &s = ctx->s;
     const fcg_params &p = ctx->p;
     hipStream_t st = ctx->stream;
     for (int sub = 1; sub <= p.Mt; ++sub) {
         const double *tide = (p.tf_presence && m.nobn) ? s.tide + (size_t)(sub - 1) * m.nobn : nullptr;
-        hipLaunchKernelGGL(fcgk::k_bt_elem1, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
-        hipLaunchKernelGGL(fcgk::k_bt_node, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide);
-        hipLaunchKernelGGL(fcgk::k_bt_velgrad, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+        hipLaunchKernelGGL(fcgk::k_bt_elem1<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
+        hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide);
+        hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         if (sub == 1) {
-            hipLaunchKernelGGL(fcgk::k_bt_puls, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
-            hipLaunchKernelGGL(fcgk::k_el2nodes, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, (const double *)s.U_puls,
-                               (const double *)s.V_puls, s.Unode_p, s.Vnode_p);
-            hipLaunchKernelGGL(fcgk::k_bt_p1adv, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+            hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
+            hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, (const R *)s.U_puls,
+                               (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
+            hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         }
-        hipLaunchKernelGGL(fcgk::k_bt_elem2, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
+        hipLaunchKernelGGL(fcgk::k_bt_elem2<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
                            ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0);
     }
     return FCG_OK;
@@ -177,10 +200,15 @@ int enqueue_substeps(fcg_ctx *ctx) {
 }  // namespace
 
 // Launch a kernel on the context stream, bracketed by probe events when the
-// kernel's name is the probed one.
+// kernel's name (without its storage-type argument) is the probed one.
+static bool probe_is(const std::string &probe, const char *kern) {
+    if (probe.empty()) return false;
+    size_t n = std::strcspn(kern, "<");
+    return probe.size() == n && probe.compare(0, n, kern, n) == 0;
+}
 #define LAUNCH(ctx, kern, grid, ...)                                                        \
     do {                                                                                   \
-        bool pr__ = (ctx)->probe_name == #kern;                                            \
+        bool pr__ = probe_is((ctx)->probe_name, #kern);                                    \
         hipEvent_t a__ = nullptr, b__ = nullptr;                                           \
         if (pr__) {                                                                        \
             (void)hipEventCreate(&a__);                                                    \
@@ -232,7 +260,8 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
         *out = ctx;
         return FCG_E_ARG;
     }
-    if (!(p.tracer_adv == 1 || p.tracer_adv == 2 || p.tracer_adv == 4) || p.precision != FCG_FP64 ||
+    if (!(p.tracer_adv == 1 || p.tracer_adv == 2 || p.tracer_adv == 4) ||
+        !(p.precision == FCG_FP64 || p.precision == FCG_MIXED) ||
         p.closure != FCG_CLOSURE_D3END) {
         fcg_fail(ctx, FCG_E_ARG, "option not available in this build (tracer_adv/precision/closure)");
         *out = ctx;
@@ -417,28 +446,32 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     DevState &s = ctx->s;
     const size_t NL = (size_t)N * L, EL = (size_t)E * L, NK = (size_t)N * K, EK = (size_t)E * K;
 #define DA(ptr, n) if ((rc = dalloc(ctx, &(ptr), (n)))) { *out = ctx; return rc; }
+    // 3-D fields: element size of the precision mode (the pointer is retyped on use)
+#define DA3(ptr, n) if ((rc = dalloc_bytes(ctx, (void **)&(ptr), (n) * ctx->esize))) { *out = ctx; return rc; }
+    ctx->mixed = p.precision == FCG_MIXED;
+    ctx->esize = ctx->mixed ? sizeof(float) : sizeof(double);
     DA(s.eta_n, N) DA(s.eta_n_1, N) DA(s.eta_n_2, N) DA(s.etaAB, N) DA(s.eta_g, N) DA(s.ssh_rhs, N)
     DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
     DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
     DA(s.a, N) DA(s.a_old, N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
-    DA(s.U_n, EL) DA(s.V_n, EL) DA(s.U_rhs, EL) DA(s.V_rhs, EL) DA(s.U_rhsAB, EL) DA(s.V_rhsAB, EL)
-    DA(s.U_nf, EL) DA(s.V_nf, EL) DA(s.gux, EL) DA(s.guy, EL) DA(s.gvx, EL) DA(s.gvy, EL)
-    DA(s.Bpu, EL) DA(s.Bpv, EL) DA(s.Je, EL) DA(s.U_puls, EL) DA(s.V_puls, EL)
-    DA(s.vxx, EK) DA(s.vyu, EK) DA(s.vyv, EK)
-    DA(s.TF, NL) DA(s.SF, NL) DA(s.T_old, NL) DA(s.S_old, NL) DA(s.Tclim, NL) DA(s.Sclim, NL)
-    DA(s.abT, NL) DA(s.abS, NL) DA(s.Unode_p, NL) DA(s.Vnode_p, NL) DA(s.tcp, NL) DA(s.ttp, NL) DA(s.tsp, NL)
+    DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
+    DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
+    DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA3(s.U_puls, EL) DA3(s.V_puls, EL)
+    DA3(s.vxx, EK) DA3(s.vyu, EK) DA3(s.vyv, EK)
+    DA3(s.TF, NL) DA3(s.SF, NL) DA3(s.T_old, NL) DA3(s.S_old, NL) DA3(s.Tclim, NL) DA3(s.Sclim, NL)
+    DA3(s.abT, NL) DA3(s.abS, NL) DA3(s.Unode_p, NL) DA3(s.Vnode_p, NL) DA3(s.tcp, NL) DA3(s.ttp, NL) DA3(s.tsp, NL)
     if (p.tracer_adv == 4 || (p.tracer_adv == 2 && p.muscl_call)) {
-        DA(s.gTx, NL) DA(s.gTy, NL)
+        DA3(s.gTx, NL) DA3(s.gTy, NL)
     }
     if (p.tracer_adv == 4) {
-        DA(s.tcur, NL)
+        DA3(s.tcur, NL)
     }
     if (p.tracer_adv == 2 && p.muscl_call) {
-        DA(s.gSx, NL) DA(s.gSy, NL)
+        DA3(s.gSx, NL) DA3(s.gSy, NL)
     }
-    DA(s.hp, NK) DA(s.Kv, NK) DA(s.snu, NK) DA(s.bt, NK) DA(s.W, NK)
-    DA(s.mx_p, NK) DA(s.mx_q, NK) DA(s.mx_c2, NK) DA(s.mx_xx, NK) DA(s.mx_yy, NK) DA(s.mx_bA, NK) DA(s.mx_bB, NK)
+    DA(s.hp, NK) DA3(s.Kv, NK) DA3(s.snu, NK) DA3(s.bt, NK) DA3(s.W, NK)
+    DA3(s.mx_p, NK) DA3(s.mx_q, NK) DA3(s.mx_c2, NK) DA3(s.mx_xx, NK) DA3(s.mx_yy, NK) DA3(s.mx_bA, NK) DA3(s.mx_bB, NK)
     ctx->red_blocks = 512;
     DA(s.red, 16 * (size_t)ctx->red_blocks) DA(s.scal, 32) DA(s.err, 4)
     FCG_CHECK(hipHostMalloc((void **)&ctx->tide_pinned, sizeof(double) * std::max(1, p.Mt * mesh->nobn)));
@@ -464,9 +497,10 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     FCG_CHECK(hipMemcpy(s.etaAB, ea.data(), N * 8, hipMemcpyHostToDevice));
     FCG_CHECK(hipMemcpy(s.a_old, a1.data(), N * 8, hipMemcpyHostToDevice));
     FCG_CHECK(hipMemcpy(s.a, a2.data(), N * 8, hipMemcpyHostToDevice));
-    FCG_CHECK(hipMemcpy(s.snu, cst.data(), NK * 8, hipMemcpyHostToDevice));
-    FCG_CHECK(hipMemcpy(s.Kv, cst.data(), NK * 8, hipMemcpyHostToDevice));
-    FCG_CHECK(hipMemcpy(s.bt, bt0.data(), NK * 8, hipMemcpyHostToDevice));
+    if ((rc = upload3(ctx, s.snu, cst)) || (rc = upload3(ctx, s.Kv, cst)) || (rc = upload3(ctx, s.bt, bt0))) {
+        *out = ctx;
+        return rc;
+    }
     FCG_CHECK(hipMemcpy(s.mask_wd, ones.data(), E * 8, hipMemcpyHostToDevice));
     FCG_CHECK(hipMemcpy(s.C_d_el, cd.data(), E * 8, hipMemcpyHostToDevice));
     if (p.wet_dry_on) {  // wad_mask at set-up (fv_main.F90:320-324)
@@ -479,7 +513,10 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
         }
         FCG_CHECK(hipMemcpy(s.mask_wd, mk.data(), E * 8, hipMemcpyHostToDevice));
     }
-    hipLaunchKernelGGL(fcgk::k_je, dim3(nblk(E)), dim3(kBlk), 0, ctx->stream, m, s);
+    if (ctx->mixed)
+        hipLaunchKernelGGL(fcgk::k_je<float>, dim3(nblk(E)), dim3(kBlk), 0, ctx->stream, m, retype<float>(s));
+    else
+        hipLaunchKernelGGL(fcgk::k_je<double>, dim3(nblk(E)), dim3(kBlk), 0, ctx->stream, m, s);
     FCG_CHECK(hipGetLastError());
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->time_jd = p.time_jd0;
@@ -490,6 +527,7 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     return FCG_OK;
 #undef UP
 #undef DA
+#undef DA3
 }
 
 extern "C" void fcg_destroy(fcg_ctx *ctx) {
@@ -530,12 +568,24 @@ static int transfer(fcg_ctx *ctx, const char *name, double *host, long count, bo
     auto dev_at = [&](long lev, long i) -> size_t {
         return f.interleaved ? (size_t)i * f.levels + lev : (size_t)lev * f.ents + i;
     };
+    const bool f32 = f.r3 && ctx->mixed;  // device holds float: round on upload, widen on download
+    std::vector<float> tmpf(f32 ? (size_t)count : 0);
     if (to_dev) {
         for (long i = 0; i < f.ents; ++i)
             for (long k = 0; k < f.levels; ++k) tmp[dev_at(k, i)] = host[(size_t)perm[i] * f.levels + k];
-        FCG_CHECK(hipMemcpy(f.dev, tmp.data(), sizeof(double) * count, hipMemcpyHostToDevice));
+        if (f32) {
+            for (long j = 0; j < count; ++j) tmpf[j] = (float)tmp[j];
+            FCG_CHECK(hipMemcpy(f.dev, tmpf.data(), sizeof(float) * count, hipMemcpyHostToDevice));
+        } else {
+            FCG_CHECK(hipMemcpy(f.dev, tmp.data(), sizeof(double) * count, hipMemcpyHostToDevice));
+        }
     } else {
-        FCG_CHECK(hipMemcpy(tmp.data(), f.dev, sizeof(double) * count, hipMemcpyDeviceToHost));
+        if (f32) {
+            FCG_CHECK(hipMemcpy(tmpf.data(), f.dev, sizeof(float) * count, hipMemcpyDeviceToHost));
+            for (long j = 0; j < count; ++j) tmp[j] = (double)tmpf[j];
+        } else {
+            FCG_CHECK(hipMemcpy(tmp.data(), f.dev, sizeof(double) * count, hipMemcpyDeviceToHost));
+        }
         for (long i = 0; i < f.ents; ++i)
             for (long k = 0; k < f.levels; ++k) host[(size_t)perm[i] * f.levels + k] = tmp[dev_at(k, i)];
     }
@@ -566,54 +616,69 @@ extern "C" int fcg_advance_clock(fcg_ctx *ctx) {
     return FCG_OK;
 }
 
+template <typename R>
+static int pressure_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
+    const double gd0 = kG / ctx->density_0;
+    LAUNCH(ctx, k_pressure<R>, nblk(ctx->m.N), ctx->m, s, gd0, ctx->density_0);
+    LAUNCH(ctx, k_bpg<R>, nblk(ctx->m.E), ctx->m, s);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
 extern "C" int fcg_pressure(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    const double gd0 = kG / ctx->density_0;
-    LAUNCH(ctx, k_pressure, nblk(ctx->m.N), ctx->m, ctx->s, gd0, ctx->density_0);
-    LAUNCH(ctx, k_bpg, nblk(ctx->m.E), ctx->m, ctx->s);
+    return ctx->mixed ? pressure_impl(ctx, retype<float>(ctx->s)) : pressure_impl(ctx, ctx->s);
+}
+
+template <typename R>
+static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
+    const fcg_params &p = ctx->p;
+    // z0b_min/z0s_min/snul as fv_run.dat gives them
+    LAUNCH(ctx, k_d3end<R>, nblk(ctx->m.N), ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
 
 extern "C" int fcg_vertical_mixing(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    const fcg_params &p = ctx->p;
-    // z0b_min/z0s_min/snul as fv_run.dat gives them
-    LAUNCH(ctx, k_d3end, nblk(ctx->m.N), ctx->m, ctx->s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0);
-    FCG_CHECK(hipGetLastError());
-    return FCG_OK;
+    return ctx->mixed ? vertical_mixing_impl(ctx, retype<float>(ctx->s)) : vertical_mixing_impl(ctx, ctx->s);
 }
 
-extern "C" int fcg_compute_vel_rhs(fcg_ctx *ctx) {
-    if (!ctx) return FCG_E_ARG;
+template <typename R>
+static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     double ff = 1.5 + kEps;
     if (ctx->lfirst) {  // src/fv_3D_vel.F90:124-128
         ff = 1.0;
         ctx->lfirst = 0;
     }
-    LAUNCH(ctx, k_velgrad3d, nblk(ctx->m.E), ctx->m, ctx->s);
-    LAUNCH(ctx, k_velrhs, nblk_lv(ctx->m.E, ctx->m.L), ctx->m, ctx->s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0,
+    LAUNCH(ctx, k_velgrad3d<R>, nblk(ctx->m.E), ctx->m, s);
+    LAUNCH(ctx, k_velrhs<R>, nblk_lv(ctx->m.E, ctx->m.L), ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0,
            p.vert_adv == 1 ? 1 : 0);
     if (p.im_vert_visc_fic)
-        LAUNCH(ctx, k_velvisc, nblk(ctx->m.E), ctx->m, ctx->s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
+        LAUNCH(ctx, k_velvisc<R>, nblk(ctx->m.E), ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
 
-extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
+extern "C" int fcg_compute_vel_rhs(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
+    return ctx->mixed ? compute_vel_rhs_impl(ctx, retype<float>(ctx->s)) : compute_vel_rhs_impl(ctx, ctx->s);
+}
+
+template <typename R>
+static int barotropic_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     if (p.tf_presence && ctx->m.nobn) {
         FCG_CHECK(hipStreamSynchronize(ctx->stream));  // tide_pinned is reused each step
         fill_tides(ctx);
-        FCG_CHECK(hipMemcpyAsync(ctx->s.tide, ctx->tide_pinned, sizeof(double) * p.Mt * ctx->m.nobn,
+        FCG_CHECK(hipMemcpyAsync(s.tide, ctx->tide_pinned, sizeof(double) * p.Mt * ctx->m.nobn,
                                  hipMemcpyHostToDevice, ctx->stream));
     }
     if (!ctx->bt_graph_ok) {
         hipGraph_t g = nullptr;
         FCG_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_substeps(ctx);
+        enqueue_substeps(ctx, s);
         FCG_CHECK(hipStreamEndCapture(ctx->stream, &g));
         FCG_CHECK(hipGraphInstantiate(&ctx->bt_graph, g, nullptr, nullptr, 0));
         FCG_CHECK(hipGraphDestroy(g));
@@ -624,58 +689,82 @@ extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
     return FCG_OK;
 }
 
+extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    return ctx->mixed ? barotropic_impl(ctx, retype<float>(ctx->s)) : barotropic_impl(ctx, ctx->s);
+}
+
+template <typename R>
+static int jacobian_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
+    LAUNCH(ctx, k_jac_node<R>, nblk(ctx->m.N), ctx->m, s, ctx->p.Dmin);
+    LAUNCH(ctx, k_je<R>, nblk(ctx->m.E), ctx->m, s);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
 extern "C" int fcg_jacobian(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    LAUNCH(ctx, k_jac_node, nblk(ctx->m.N), ctx->m, ctx->s, ctx->p.Dmin);
-    LAUNCH(ctx, k_je, nblk(ctx->m.E), ctx->m, ctx->s);
+    return ctx->mixed ? jacobian_impl(ctx, retype<float>(ctx->s)) : jacobian_impl(ctx, ctx->s);
+}
+
+template <typename R>
+static int update_3d_vel_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
+    LAUNCH(ctx, k_update3d<R>, nblk(ctx->m.E), ctx->m, s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
 
 extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    LAUNCH(ctx, k_update3d, nblk(ctx->m.E), ctx->m, ctx->s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
+    return ctx->mixed ? update_3d_vel_impl(ctx, retype<float>(ctx->s)) : update_3d_vel_impl(ctx, ctx->s);
+}
+
+template <typename R>
+static int vert_vel_sigma_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
+    LAUNCH(ctx, k_vertvel_div<R>, nblk_lv(ctx->m.N, ctx->m.L), ctx->m, s);
+    LAUNCH(ctx, k_vertvel_scan<R>, nblk(ctx->m.N), ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
 
 extern "C" int fcg_vert_vel_sigma(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
-    LAUNCH(ctx, k_vertvel_div, nblk_lv(ctx->m.N, ctx->m.L), ctx->m, ctx->s);
-    LAUNCH(ctx, k_vertvel_scan, nblk(ctx->m.N), ctx->m, ctx->s);
-    FCG_CHECK(hipGetLastError());
-    return FCG_OK;
+    return ctx->mixed ? vert_vel_sigma_impl(ctx, retype<float>(ctx->s)) : vert_vel_sigma_impl(ctx, ctx->s);
 }
 
-extern "C" int fcg_solve_tracers(fcg_ctx *ctx) {
-    if (!ctx) return FCG_E_ARG;
+template <typename R>
+static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     const DevMesh &m = ctx->m;
-    const DevState &s = ctx->s;
     const size_t NL = (size_t)m.N * m.L;
     int nb = ctx->red_blocks;
-    hipLaunchKernelGGL(fcgk::k_minmax_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, (const double *)s.TF,
-                       (const double *)s.SF, NL, s.red);
+    hipLaunchKernelGGL(fcgk::k_minmax_partial<R>, dim3(nb), dim3(kBlk), 0, ctx->stream, (const R *)s.TF,
+                       (const R *)s.SF, NL, s.red);
     hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal);
     const int relax = p.cl_relax ? 1 : 0;
     // scheme dispatch of src/fv_main.F90:1788-1800; node x layer kernels, then
     // the column-sequential implicit vertical diffusion
     const unsigned glv = nblk_lv(m.N, m.L);
     if (p.tracer_adv == 1) {
-        LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
-        LAUNCH(ctx, k_tr_upwind, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_upwind<R>, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 2 && p.muscl_call) {
-        LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
-        LAUNCH(ctx, k_tr_gradn_muscl, glv, m, s);
-        LAUNCH(ctx, k_tr_muscl, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_gradn_muscl<R>, glv, m, s);
+        LAUNCH(ctx, k_tr_muscl<R>, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 4) {
-        LAUNCH(ctx, k_tr_ab, nblk(NL), s, NL);
-        LAUNCH(ctx, k_tr_gradn_miura, glv, m, s);
-        LAUNCH(ctx, k_tr_miura, glv, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_gradn_miura<R>, glv, m, s);
+        LAUNCH(ctx, k_tr_miura<R>, glv, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
     }
-    LAUNCH(ctx, k_tr_vdiff, nblk(m.N), m, s, p.dt);
+    LAUNCH(ctx, k_tr_vdiff<R>, nblk(m.N), m, s, p.dt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
+}
+
+extern "C" int fcg_solve_tracers(fcg_ctx *ctx) {
+    if (!ctx) return FCG_E_ARG;
+    return ctx->mixed ? solve_tracers_impl(ctx, retype<float>(ctx->s)) : solve_tracers_impl(ctx, ctx->s);
 }
 
 extern "C" int fcg_oce_timestep(fcg_ctx *ctx) {
@@ -697,12 +786,11 @@ extern "C" int fcg_run(fcg_ctx *ctx, int nsteps) {
     return FCG_OK;
 }
 
-extern "C" int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out) {
-    if (!ctx || !out) return FCG_E_ARG;
+template <typename R>
+static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out) {
     const DevMesh &m = ctx->m;
-    const DevState &s = ctx->s;
     int nb = ctx->red_blocks;
-    hipLaunchKernelGGL(fcgk::k_diag_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, m, s, ctx->p.Dmin, s.red);
+    hipLaunchKernelGGL(fcgk::k_diag_partial<R>, dim3(nb), dim3(kBlk), 0, ctx->stream, m, s, ctx->p.Dmin, s.red);
     hipLaunchKernelGGL(fcgk::k_diag_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal + 4);
     FCG_CHECK(hipGetLastError());
     double r[12];
@@ -730,4 +818,9 @@ extern "C" int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out) {
     out->n_wet_elem = wet;
     out->status = err;
     return FCG_OK;
+}
+
+extern "C" int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out) {
+    if (!ctx || !out) return FCG_E_ARG;
+    return ctx->mixed ? diagnostics_impl(ctx, retype<float>(ctx->s), out) : diagnostics_impl(ctx, ctx->s, out);
 }

@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -57,7 +59,11 @@ struct DevMesh {
     double *oms = nullptr;     // K: 1-sigma(k)
 };
 
-struct DevState {
+// Device state.  2-D barotropic fields, the hydrostatic pressure and the
+// reductions are FP64 in every precision mode; the 3-D fields use the storage
+// type R (double for the FP64 drop-in, float for the mixed-precision path).
+template <typename R>
+struct DevStateT {
     // 2-D barotropic (FP64)
     double *eta_n = nullptr, *eta_n_1 = nullptr, *eta_n_2 = nullptr, *etaAB = nullptr, *eta_g = nullptr;
     double *ssh_rhs = nullptr;            // flux sum of the last substep (diagnostic)
@@ -70,29 +76,43 @@ struct DevState {
     double *a = nullptr, *a_old = nullptr;                     // N: max(depth+eta_n, Dmin) now / previous jacobian
     double *tide = nullptr;                                    // Mt*nobn prescribed OB elevation per substep
     // 3-D, level-major planes
-    double *U_n = nullptr, *V_n = nullptr, *U_rhs = nullptr, *V_rhs = nullptr;   // L*E
-    double *U_rhsAB = nullptr, *V_rhsAB = nullptr, *U_nf = nullptr, *V_nf = nullptr; // L*E
-    double *gux = nullptr, *guy = nullptr, *gvx = nullptr, *gvy = nullptr;         // L*E
-    double *Bpu = nullptr, *Bpv = nullptr, *Je = nullptr;                          // L*E
-    double *U_puls = nullptr, *V_puls = nullptr;                                   // L*E
-    double *vxx = nullptr, *vyu = nullptr, *vyv = nullptr;                         // K*E Thomas scratch
-    double *TF = nullptr, *SF = nullptr, *T_old = nullptr, *S_old = nullptr;       // L*N
-    double *Tclim = nullptr, *Sclim = nullptr, *abT = nullptr, *abS = nullptr;     // L*N
-    double *Unode_p = nullptr, *Vnode_p = nullptr;                                 // L*N
-    double *tcp = nullptr, *ttp = nullptr, *tsp = nullptr;                         // L*N Thomas scratch
-    double *gTx = nullptr, *gTy = nullptr, *gSx = nullptr, *gSy = nullptr;         // L*N node-averaged tracer gradients (MUSCL/Miura)
-    double *tcur = nullptr;                                                        // L*N de-meaned T before the Miura update
-    double *hp = nullptr, *Kv = nullptr, *snu = nullptr, *bt = nullptr, *W = nullptr; // K*N
-    double *mx_p = nullptr, *mx_q = nullptr, *mx_c2 = nullptr, *mx_xx = nullptr, *mx_yy = nullptr; // K*N d3_end scratch
-    double *mx_bA = nullptr, *mx_bB = nullptr;                                     // K*N
+    R *U_n = nullptr, *V_n = nullptr, *U_rhs = nullptr, *V_rhs = nullptr;   // L*E
+    R *U_rhsAB = nullptr, *V_rhsAB = nullptr, *U_nf = nullptr, *V_nf = nullptr; // L*E
+    R *gux = nullptr, *guy = nullptr, *gvx = nullptr, *gvy = nullptr;         // L*E
+    R *Bpu = nullptr, *Bpv = nullptr, *Je = nullptr;                          // L*E
+    R *U_puls = nullptr, *V_puls = nullptr;                                   // L*E
+    R *vxx = nullptr, *vyu = nullptr, *vyv = nullptr;                         // K*E Thomas scratch
+    R *TF = nullptr, *SF = nullptr, *T_old = nullptr, *S_old = nullptr;       // L*N
+    R *Tclim = nullptr, *Sclim = nullptr, *abT = nullptr, *abS = nullptr;     // L*N
+    R *Unode_p = nullptr, *Vnode_p = nullptr;                                 // L*N
+    R *tcp = nullptr, *ttp = nullptr, *tsp = nullptr;                         // L*N Thomas scratch
+    R *gTx = nullptr, *gTy = nullptr, *gSx = nullptr, *gSy = nullptr;         // L*N node-averaged tracer gradients (MUSCL/Miura)
+    R *tcur = nullptr;                                                        // L*N de-meaned T before the Miura update
+    double *hp = nullptr;                                                          // K*N hydrostatic pressure (FP64)
+    R *Kv = nullptr, *snu = nullptr, *bt = nullptr, *W = nullptr; // K*N
+    R *mx_p = nullptr, *mx_q = nullptr, *mx_c2 = nullptr, *mx_xx = nullptr, *mx_yy = nullptr; // K*N d3_end scratch
+    R *mx_bA = nullptr, *mx_bB = nullptr;                                     // K*N
     // reductions
     double *red = nullptr;       // partial results
     double *scal = nullptr;      // [0] T_aver [1] S_aver, diagnostics after
     int *err = nullptr;          // device STOP flag
 };
+using DevState = DevStateT<double>;
+
+// The same allocations seen with the mixed-precision storage type (the
+// pointer layout of DevStateT does not depend on R).
+template <typename R>
+inline DevStateT<R> retype(const DevState &s) {
+    static_assert(sizeof(DevStateT<R>) == sizeof(DevState), "state layout");
+    DevStateT<R> t;
+    std::memcpy(static_cast<void *>(&t), static_cast<const void *>(&s), sizeof s);
+    return t;
+}
 
 struct fcg_ctx {
     fcg_params p{};
+    bool mixed = false;    // FCG_MIXED: 3-D fields stored as float
+    size_t esize = 8;      // bytes per 3-D field element
     double density_0 = 0, scale_area = 0;
     DevMesh m;
     DevState s;

@@ -75,7 +75,8 @@ __device__ __forceinline__ bool node_level(int N, int L, int &n, int &k) {
 // ---------------------------------------------------------------------------
 // pressure: hydrostatic column scan, src/fv_pressure_new.F90:91-107
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevState s, double gd0, double rho0) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
     int n = TID();
     if (n >= m.N) return;
     const int N = m.N;
@@ -94,7 +95,8 @@ __global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevState s, double 
 }
 
 // baroclinic_pressure_gradient_3D: src/fv_pressure_new.F90:146-221 (FP64)
-__global__ __launch_bounds__(256) void k_bpg(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_bpg(DevMesh m, DevStateT<R> s) {
     int e = TID();
     if (e >= m.E) return;
     const int N = m.N, E = m.E;
@@ -135,7 +137,8 @@ __global__ __launch_bounds__(256) void k_bpg(DevMesh m, DevState s) {
 // (src/fv_mixing.F90:73-347): one thread per node column.
 // Scratch planes (K*N): p=-b1-b2, q=b_t/dt, c2, xx1, yy1, btr ping-pong.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevState s, double dt, double Dmin, double z0b_min, double z0s_min,
+template <typename R>
+__global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double dt, double Dmin, double z0b_min, double z0s_min,
                         double snul, double rho0) {
     int n = TID();
     if (n >= m.N) return;
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevState s, double dt,
     const double delta_b = a * m.dsig[L - 1] + z0b_min;
     const double lg = log(delta_b / z0b_min);
     const double bott_R = B1cka2 / (lg * lg);
-    double *bc = s.mx_bA, *bn = s.mx_bB;
+    R *bc = s.mx_bA, *bn = s.mx_bB;
     for (int nz = 1; nz <= K; ++nz) PL(bc, nz - 1, n, N) = BT(nz);
     for (int it = 1; it <= 200; ++it) {
         double xxp = 1.0, yyp = yy2;
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevState s, double dt,
             if (cc > bbb) bbb = cc;
         }
         if (bbb < 1.0e-6) break;
-        double *t = bc; bc = bn; bn = t;
+        R *t = bc; bc = bn; bn = t;
     }
     for (int nz = 1; nz <= K; ++nz) {
         double btr = PL(bc, nz - 1, n, N);
@@ -277,7 +280,8 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevState s, double dt,
 // ---------------------------------------------------------------------------
 // vel_gradients (3-D, free slip): src/fv_gradients.F90:2-93
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevStateT<R> s) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E;
@@ -320,7 +324,8 @@ __global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevState s) {
 }
 
 // momentum_adv_upwind edge flux at one level: src/fv_advection.F90:120-165
-__device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevState &s, int ed, int k, double &uu, double &vv) {
+template <typename R>
+__device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevStateT<R> &s, int ed, int k, double &uu, double &vv) {
     const int E = m.E;
     const int2 nd = m.ednodes[ed];
     const int2 et = m.edtri[ed];
@@ -363,7 +368,8 @@ struct ElemCol {
         const double ds = m.dsig[k];
         return sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
     }
-    __device__ double Av(const DevMesh &m, const DevState &s, int k) const {
+    template <typename R>
+    __device__ double Av(const DevMesh &m, const DevStateT<R> &s, int k) const {
         const int N = m.N;
         return sum4(w, PL(s.snu, k, q.x, N), PL(s.snu, k, q.y, N), PL(s.snu, k, q.z, N), PL(s.snu, k, q.w, N));
     }
@@ -378,7 +384,8 @@ struct ElemCol {
     }
 };
 
-__device__ __forceinline__ ElemCol elem_col(const DevMesh &m, const DevState &s, int e) {
+template <typename R>
+__device__ __forceinline__ ElemCol elem_col(const DevMesh &m, const DevStateT<R> &s, int e) {
     ElemCol c;
     c.q = m.en[e];
     c.w = m.wcv[e];
@@ -391,7 +398,8 @@ __device__ __forceinline__ ElemCol elem_col(const DevMesh &m, const DevState &s,
 
 // momentum_vert_adv_upwind flux through 1-based interface nz (2..K-1) of an
 // element column (fv_advection.F90:389-433)
-__device__ __forceinline__ void uvert(const DevMesh &m, const DevState &s, const ElemCol &c, int e, double area,
+template <typename R>
+__device__ __forceinline__ void uvert(const DevMesh &m, const DevStateT<R> &s, const ElemCol &c, int e, double area,
                                       int nz, double &fu, double &fv) {
     const int N = m.N, E = m.E, K = m.K;
     const int4 q = c.q;
@@ -444,7 +452,8 @@ __device__ __forceinline__ void uvert(const DevMesh &m, const DevState &s, const
 // interfaces (354-469) and the explicit update (124-143).  Writes the new AB
 // history and the provisional velocity into U_rhs/V_rhs; k_velvisc then
 // applies the implicit vertical viscosity column by column.
-__global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevState s, double dt, double ff, int hadv, int vadv) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv) {
     int e, k;
     if (!elem_level(m.E, m.L, e, k)) return;
     const int N = m.N, E = m.E, K = m.K;
@@ -525,7 +534,8 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevState s, double dt
 // element column: Av formed from snu (fv_mixing.F90:308-313), surface
 // stress, forward sweep over the provisional velocity, bottom condition by
 // fic_point, back substitution; dry elements get zero velocity.
-__global__ __launch_bounds__(256) void k_velvisc(DevMesh m, DevState s, double dt, double rho0, double Cd_bz, int fic_point) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_velvisc(DevMesh m, DevStateT<R> s, double dt, double rho0, double Cd_bz, int fic_point) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E, L = m.L, K = m.K;
@@ -602,12 +612,14 @@ __global__ __launch_bounds__(256) void k_velvisc(DevMesh m, DevState s, double d
 // ---------------------------------------------------------------------------
 constexpr double kAB0 = 1.5 + kBeta, kAB1 = 0.5 + 2.0 * kBeta;
 
-__device__ __forceinline__ double etaAB3(const DevState &s, int n) {
+template <typename R>
+__device__ __forceinline__ double etaAB3(const DevStateT<R> &s, int n) {
     return kAB0 * s.eta_n[n] - kAB1 * s.eta_n_1[n] + kBeta * s.eta_n_2[n];
 }
 
 // AB3 velocity + element depth of compute_ssh_rhs_elem (fv_average_dynamic.F90:66-86)
-__global__ __launch_bounds__(256) void k_bt_elem1(DevMesh m, DevState s, double Dmin) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_elem1(DevMesh m, DevStateT<R> s, double Dmin) {
     int e = TID();
     if (e >= m.E) return;
     const int4 q = m.en[e];
@@ -628,7 +640,8 @@ __global__ __launch_bounds__(256) void k_bt_elem1(DevMesh m, DevState s, double 
 // tidal OB value (fv_average_dynamic.F90:190-216, host-evaluated), the AM4
 // elevation of update_2D_vel (fv_average_dynamic.F90:288-296) and the time-level
 // shift (fv_main.F90:1547-1551).
-__global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevState s, double dt_2D, const double *__restrict__ tide) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, double dt_2D, const double *__restrict__ tide) {
     int n = TID();
     if (n >= m.N) return;
     double ssh = 0.0;
@@ -657,7 +670,8 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevState s, double d
 }
 
 // vel_gradients_2D(UAB): src/fv_gradients.F90:97-176
-__global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
     int e = TID();
     if (e >= m.E) return;
     const int4 q = m.en[e];
@@ -696,7 +710,8 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevState s) {
 // first substep only: dmean_n (fv_main.F90:1467-1473) and compute_puls_vel
 // (fv_3D_vel.F90:257-307).  After k_bt_node's shift eta_n holds eta^{n+1}
 // (= ssh_rhs of the reference) and eta_n_1 holds eta^n.
-__global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevState s, double Dmin) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevStateT<R> s, double Dmin) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E;
@@ -716,8 +731,9 @@ __global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevState s, double D
 }
 
 // compute_el2nodes_3D(U_puls, V_puls): src/fv_utilit.F90:669-689
-__global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const double *__restrict__ Ue, const double *__restrict__ Ve,
-                           double *__restrict__ Un, double *__restrict__ Vn) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict__ Ue, const R *__restrict__ Ve,
+                           R *__restrict__ Un, R *__restrict__ Vn) {
     int n = TID();
     if (n >= m.N) return;
     const int N = m.N, E = m.E;
@@ -740,7 +756,8 @@ __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const double *__res
 
 // momentum_adv_P1_3D_to_2D (fv_advection.F90:676-793) as an element gather,
 // then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513)
-__global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
     int e = TID();
     if (e >= m.E) return;
     const int N = m.N, E = m.E;
@@ -782,7 +799,8 @@ __global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevState s) {
 }
 
 // momentum_adv_upwind_2D edge flux: fv_advection.F90:589-643
-__device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevState &s, int ed, double Dmin, double &uu,
+template <typename R>
+__device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevStateT<R> &s, int ed, double Dmin, double &uu,
                                           double &vv) {
     const int2 nd = m.ednodes[ed];
     const int2 et = m.edtri[ed];
@@ -808,7 +826,8 @@ __device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevState &s, i
 
 // update_2D_vel (fv_average_dynamic.F90:228-443, 3-D branch) for one element,
 // followed by the velocity shift and U_filt accumulation (fv_main.F90:1569-1606).
-__global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevState s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
                            int wet_dry, int adv2d, int first) {
     int e = TID();
     if (e >= m.E) return;
@@ -883,14 +902,16 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevState s, double 
 // ---------------------------------------------------------------------------
 // jacobian: src/fv_main.F90:1228-1252 (node scalar + stored Je)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_jac_node(DevMesh m, DevState s, double Dmin) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_jac_node(DevMesh m, DevStateT<R> s, double Dmin) {
     int n = TID();
     if (n >= m.N) return;
     s.a_old[n] = s.a[n];
     s.a[n] = fmax(m.depth[n] + s.eta_n[n], Dmin);
 }
 
-__global__ __launch_bounds__(256) void k_je(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_je(DevMesh m, DevStateT<R> s) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E;
@@ -905,7 +926,8 @@ __global__ __launch_bounds__(256) void k_je(DevMesh m, DevState s) {
 
 // update_3D_vel: src/fv_3D_vel.F90:201-255, with U_filt_2D *= 1/Mt from
 // oce_barotropic_timestep (fv_main.F90:1670-1677)
-__global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevState s, double Dmin, double Mt_inv) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, double Dmin, double Mt_inv) {
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E, L = m.L;
@@ -942,7 +964,8 @@ __global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevState s, double 
 // vert_vel_sigma (src/fv_3D_vel.F90:309-456), first part: per node and layer
 // the divergence of the baroclinic transport (U_n_filt - U_filt_2D)*Je,
 // gathered over the node's edges in ascending id (354-437), stored in W.
-__global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevStateT<R> s) {
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
@@ -968,7 +991,8 @@ __global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevState s) {
 
 // second part: bottom-up cumulative sum and /area (444-450); Wvel(nsigma)
 // stays zero.
-__global__ __launch_bounds__(256) void k_vertvel_scan(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_vertvel_scan(DevMesh m, DevStateT<R> s) {
     int n = TID();
     if (n >= m.N) return;
     const int N = m.N;
@@ -984,7 +1008,8 @@ __global__ __launch_bounds__(256) void k_vertvel_scan(DevMesh m, DevState s) {
 // tracers: solve_tracers (fv_main.F90:1707-1836)
 // ---------------------------------------------------------------------------
 // global min/max of TF and SF (fv_main.F90:1737-1777): exact, order-free
-__global__ __launch_bounds__(256) void k_minmax_partial(const double *__restrict__ T, const double *__restrict__ S, size_t n, double *out) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_minmax_partial(const R *__restrict__ T, const R *__restrict__ S, size_t n, double *out) {
     __shared__ double sm[4][256];
     double tmn = INFINITY, tmx = -INFINITY, smn = INFINITY, smx = -INFINITY;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -1026,7 +1051,8 @@ __global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb
 }
 
 // AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444)
-__global__ __launch_bounds__(256) void k_tr_ab(DevState s, size_t n) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_ab(DevStateT<R> s, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double Ta = s.scal[0], Sa = s.scal[1];
@@ -1036,7 +1062,8 @@ __global__ __launch_bounds__(256) void k_tr_ab(DevState s, size_t n) {
 
 // QUICK vertical flux through 1-based interface nz of one node column
 // (fv_tracer.F90:740-815), for tracer column X (de-meaned AB values)
-__device__ __forceinline__ double upw_vert(const DevMesh &m, const DevState &s, const double *X, int n, int nz,
+template <typename R>
+__device__ __forceinline__ double upw_vert(const DevMesh &m, const DevStateT<R> &s, const R *X, int n, int nz,
                                            double a) {
     const int N = m.N, K = m.K;
     auto Wv = [&](int z) { return PL(s.W, z - 1, n, N); };
@@ -1072,7 +1099,8 @@ __device__ __forceinline__ double upw_vert(const DevMesh &m, const DevState &s, 
 // tracer_impl_vert_diff (fv_tracer.F90:1319-1494, key_atm off) of one node
 // column, applied to TF/SF in place, followed by the STOP checks of
 // fv_tracer.F90:1466-1485 (reported through the device error flag).
-__device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevState &s, int n, double a, double Ta,
+template <typename R>
+__device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R> &s, int n, double a, double Ta,
                                              double Sa, double dt) {
     const int N = m.N, L = m.L, K = m.K;
     auto zb = [&](int z) { return m.oms[z - 1] * a; };
@@ -1145,7 +1173,8 @@ __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevState &s
 // boundary edges (fv_tracer.F90:862-873 and the same block of the other
 // schemes), summed in ascending edge id; decides inflow relaxation vs.
 // replacement by climatology.
-__device__ __forceinline__ double ob_inflow(const DevMesh &m, const DevState &s, int p0, int p1, double Dmin) {
+template <typename R>
+__device__ __forceinline__ double ob_inflow(const DevMesh &m, const DevStateT<R> &s, int p0, int p1, double Dmin) {
     double vel_nor = 0.0;
     for (int j = p0; j < p1; ++j) {
         const int cc = m.n2e[j];
@@ -1168,8 +1197,8 @@ __device__ __forceinline__ double ob_inflow(const DevMesh &m, const DevState &s,
 // Tm keeps its previous value when W is exactly zero, as in the reference;
 // the product -Tm*W is then a zero either way.  The two schemes spell the
 // QUICK weights with one or two divisions, hence kChained.
-template <bool kChained>
-__device__ __forceinline__ double hi_vert(const DevMesh &m, const DevState &s, const double *X, double sub, int n,
+template <bool kChained, typename R>
+__device__ __forceinline__ double hi_vert(const DevMesh &m, const DevStateT<R> &s, const R *X, double sub, int n,
                                           int nz, double a, double &Tm) {
     const int N = m.N, K = m.K;
     auto Wv = [&](int z) { return PL(s.W, z - 1, n, N); };
@@ -1218,7 +1247,8 @@ __device__ __forceinline__ double hi_vert(const DevMesh &m, const DevState &s, c
 
 // Element gradient of a node column at level k (tracer_gradient_elements,
 // fv_gradients.F90:180-205), the column value being PL(X) - sub.
-__device__ __forceinline__ double2 elem_grad(const DevMesh &m, const double *X, double sub, int el, int k) {
+template <typename R>
+__device__ __forceinline__ double2 elem_grad(const DevMesh &m, const R *X, double sub, int el, int k) {
     const int N = m.N;
     const int4 q = m.en[el];
     const double v0 = PL(X, k, q.x, N) - sub, v1 = PL(X, k, q.y, N) - sub;
@@ -1230,7 +1260,8 @@ __device__ __forceinline__ double2 elem_grad(const DevMesh &m, const double *X, 
 // edge fluxes gathered in ascending edge id, QUICK vertical fluxes at the
 // layer's two interfaces, OB relaxation, AB update.  The implicit vertical
 // diffusion follows in k_tr_vdiff.
-__global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax, double clim_relax,
                             double relax2clim_ac) {
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
@@ -1317,7 +1348,8 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevState s, double
 // reconstruction (fill_up_dn_grad node-average branch,
 // fv_gradients.F90:263-323): area-weighted mean over the node's elements in
 // ascending element id, divided by the summed area.
-__global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> s) {
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N;
@@ -1344,7 +1376,8 @@ __global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevState s) {
 // (fv_tracer.F90:984-998): the same mean, with 1/(summed area) formed first.
 // Also stages the de-meaned value itself in tcur: the Miura update
 // overwrites TF while neighbouring nodes still read it.
-__global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevState s) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevStateT<R> s) {
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N;
@@ -1380,7 +1413,8 @@ __device__ __forceinline__ double muscl_mean(double T1, double T2, double dT, do
 // element 1; the second segment's flux uses the unfiltered U_n/V_n of
 // element 2 (its upwind test and the volume flux use the filtered one), as
 // the reference does.  The implicit vertical diffusion follows in k_tr_vdiff.
-__global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevState s, double dt, double Dmin, int cl_relax, double clim_relax,
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax, double clim_relax,
                            double relax2clim_ac) {
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
@@ -1478,7 +1512,8 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevState s, double 
 
 // mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
 // H_advtr_min = 0 (fv_var.F90:49-51)
-__device__ __forceinline__ double mask_ad(const DevMesh &m, const DevState &s, int n, double Dmin) {
+template <typename R>
+__device__ __forceinline__ double mask_ad(const DevMesh &m, const DevStateT<R> &s, int n, double Dmin) {
     const double dm = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
     if (dm > 0.1) return 1.0;
     if (dm < 0.0) return 0.0;
@@ -1490,7 +1525,8 @@ __device__ __forceinline__ double mask_ad(const DevMesh &m, const DevState &s, i
 // reference's unit metric factors a = b = 1 and its first-segment y term
 // without *dt.  The implicit vertical diffusion of T and S follows in
 // k_tr_vdiff.
-__global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevState s, double dt, double Dmin, double scale_area, int cl_relax,
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, double dt, double Dmin, double scale_area, int cl_relax,
                            double clim_relax, double relax2clim_ac) {
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
@@ -1583,7 +1619,8 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevState s, double 
 // tracer_impl_vert_diff of T and S (fv_tracer.F90:1319-1494), one thread per
 // node column; after every advection scheme, and alone for tracer_adv = 2
 // without MUSCL_CALL (fv_main.F90:1788-1800).
-__global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevState s, double dt) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt) {
     int n = TID();
     if (n >= m.N) return;
     vdiff_column(m, s, n, s.a[n], s.scal[0], s.scal[1], dt);
@@ -1594,7 +1631,8 @@ __global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevState s, double 
 // content, extrema -- fixed-order block tree, deterministic run to run
 // ---------------------------------------------------------------------------
 constexpr int kDiag = 12;
-__global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevState s, double Dmin, double *out) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevStateT<R> s, double Dmin, double *out) {
     __shared__ double sm[kDiag][256];
     double v[kDiag] = {0, 0, 0, 0, 0, 0, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
     // v: 0 PE, 1 area, 2 KE, 3 volume, 4 heat, 5 salt, 6/7 eta, 8/9 T, 10/11 S
@@ -1669,5 +1707,41 @@ __global__ __launch_bounds__(256) void k_mask_count(const double *mask, int E, i
     }
     if (threadIdx.x == 0) atomicAdd(out, sm[0]);
 }
+
+
+// Both storage types of every state kernel are built in this translation
+// unit; the C-ABI unit launches them through their declarations.
+template <typename R>
+void instantiate_kernels() {
+    (void)&k_pressure<R>;
+    (void)&k_bpg<R>;
+    (void)&k_d3end<R>;
+    (void)&k_velgrad3d<R>;
+    (void)&k_velrhs<R>;
+    (void)&k_velvisc<R>;
+    (void)&k_bt_elem1<R>;
+    (void)&k_bt_node<R>;
+    (void)&k_bt_velgrad<R>;
+    (void)&k_bt_puls<R>;
+    (void)&k_el2nodes<R>;
+    (void)&k_bt_p1adv<R>;
+    (void)&k_bt_elem2<R>;
+    (void)&k_jac_node<R>;
+    (void)&k_je<R>;
+    (void)&k_update3d<R>;
+    (void)&k_vertvel_div<R>;
+    (void)&k_vertvel_scan<R>;
+    (void)&k_minmax_partial<R>;
+    (void)&k_tr_ab<R>;
+    (void)&k_tr_upwind<R>;
+    (void)&k_tr_gradn_muscl<R>;
+    (void)&k_tr_gradn_miura<R>;
+    (void)&k_tr_muscl<R>;
+    (void)&k_tr_miura<R>;
+    (void)&k_tr_vdiff<R>;
+    (void)&k_diag_partial<R>;
+}
+template void instantiate_kernels<double>();
+template void instantiate_kernels<float>();
 
 }  // namespace fcgk
