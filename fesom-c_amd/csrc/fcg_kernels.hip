@@ -325,22 +325,24 @@ __global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevStateT<R> s) {
 
 // momentum_adv_upwind edge flux at one level: src/fv_advection.F90:120-165
 template <typename R>
-__device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevStateT<R> &s, int ed, int k, double &uu, double &vv) {
+__device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevStateT<R> &s, int ed, int k, R &uu, R &vv) {
+    // in the storage type: FP32 advective fluxes in the mixed-precision path
     const int E = m.E;
     const int2 nd = m.ednodes[ed];
     const int2 et = m.edtri[ed];
-    const double acc = 0.5 * (m.ac[nd.x] + m.ac[nd.y]);
+    const R acc = (R)(0.5 * (m.ac[nd.x] + m.ac[nd.y]));
     const double4 cd = m.ecross[ed];
     const double2 dd = m.edxdy[ed];
+    const R cx = (R)cd.x, cy = (R)cd.y, cz = (R)cd.z, cw = (R)cd.w, dx = (R)dd.x, dy = (R)dd.y;
     const int e1 = et.x, e2 = et.y;
-    double u1 = PL(s.U_n, k, e1, E) - PL(s.gux, k, e1, E) * cd.x - PL(s.guy, k, e1, E) * cd.y;
-    double v1 = PL(s.V_n, k, e1, E) - PL(s.gvx, k, e1, E) * cd.x - PL(s.gvy, k, e1, E) * cd.y;
-    double u2 = PL(s.U_n, k, e2, E) - PL(s.gux, k, e2, E) * cd.z - PL(s.guy, k, e2, E) * cd.w;
-    double v2 = PL(s.V_n, k, e2, E) - PL(s.gvx, k, e2, E) * cd.z - PL(s.gvy, k, e2, E) * cd.w;
+    const R u1 = PL(s.U_n, k, e1, E) - PL(s.gux, k, e1, E) * cx - PL(s.guy, k, e1, E) * cy;
+    const R v1 = PL(s.V_n, k, e1, E) - PL(s.gvx, k, e1, E) * cx - PL(s.gvy, k, e1, E) * cy;
+    const R u2 = PL(s.U_n, k, e2, E) - PL(s.gux, k, e2, E) * cz - PL(s.guy, k, e2, E) * cw;
+    const R v2 = PL(s.V_n, k, e2, E) - PL(s.gvx, k, e2, E) * cz - PL(s.gvy, k, e2, E) * cw;
     const double ds = m.dsig[k];
-    const double jd = 0.5 * (s.a[nd.x] * ds + s.a[nd.y] * ds);
-    double un = 0.5 * kREarthD * jd * ((u1 + u2) * dd.y - (v1 * m.elem_cos[e1] + v2 * m.elem_cos[e2]) * dd.x);
-    if (un >= 0.0) {
+    const R jd = (R)(0.5 * (s.a[nd.x] * ds + s.a[nd.y] * ds));
+    const R un = R(0.5) * R(kREarthD) * jd * ((u1 + u2) * dy - (v1 * (R)m.elem_cos[e1] + v2 * (R)m.elem_cos[e2]) * dx);
+    if (un >= R(0)) {
         uu = u1 * un * acc;
         vv = v1 * un * acc;
     } else {
@@ -474,15 +476,15 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
     const double u = PL(s.U_n, k, e, E), v = PL(s.V_n, k, e, E);
     double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, E) + Fx * je;
     double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, E) + Fy * je;
-    double abu = v * cor * area * je;
-    double abv = -u * cor * area * je;
+    R abu = (R)(v * cor * area * je);
+    R abv = (R)(-u * cor * area * je);
     if (hadv) {
         const int4 ied = m.el_ied[e];
 #pragma nounroll
         for (int j = 0; j < 4; ++j) {
             const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
             if (cc == kEmpty) break;
-            double uu, vv;
+            R uu, vv;
             edge_uu3d(m, s, cc >= 0 ? cc : ~cc, k, uu, vv);
             if (cc >= 0) {
                 abu = abu + uu;
@@ -517,11 +519,11 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
                     fbv = fv;
                 }
             }
-            abu = abu + (ftu - fbu) * acc;
-            abv = abv + (ftv - fbv) * acc;
+            abu = abu + (R)((ftu - fbu) * acc);
+            abv = abv + (R)((ftv - fbv) * acc);
         } else {
-            abu = 0.0;
-            abv = 0.0;
+            abu = R(0);
+            abv = R(0);
         }
     }
     PL(s.U_rhsAB, k, e, E) = abu;
@@ -970,19 +972,19 @@ __global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevStateT<R> s) 
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
-    double c = 0.0;
+    R c = R(0);  // storage type: FP32 transport fluxes in the mixed-precision path
     for (int j = p0; j < p1; ++j) {
         const int cc = m.n2e[j];
         const int ed = cc >= 0 ? cc : ~cc;
         const int2 et = m.edtri[ed];
         const double4 cd = m.ecross[ed];
         const int e1 = et.x;
-        double f = PL(s.Je, k, e1, E) * ((PL(s.V_nf, k, e1, E) - s.V_filt[e1]) * cd.x
-                                         - (PL(s.U_nf, k, e1, E) - s.U_filt[e1]) * cd.y);
+        R f = PL(s.Je, k, e1, E) * ((PL(s.V_nf, k, e1, E) - (R)s.V_filt[e1]) * (R)cd.x
+                                    - (PL(s.U_nf, k, e1, E) - (R)s.U_filt[e1]) * (R)cd.y);
         if (et.y >= 0) {
             const int e2 = et.y;
-            f = f + PL(s.Je, k, e2, E) * (-(PL(s.V_nf, k, e2, E) - s.V_filt[e2]) * cd.z
-                                          + (PL(s.U_nf, k, e2, E) - s.U_filt[e2]) * cd.w);
+            f = f + PL(s.Je, k, e2, E) * (-(PL(s.V_nf, k, e2, E) - (R)s.V_filt[e2]) * (R)cd.z
+                                          + (PL(s.U_nf, k, e2, E) - (R)s.U_filt[e2]) * (R)cd.w);
         }
         c = (cc >= 0) ? c + f : c - f;
     }
@@ -1401,11 +1403,11 @@ __global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevStateT<R> 
 
 // MUSCL edge value: linear reconstructions from both end nodes, 15 %
 // upwind-biased blend (fv_tracer.F90:104-122).
-__device__ __forceinline__ double muscl_mean(double T1, double T2, double dT, double dx, double dy, double g1x,
-                                             double g1y, double g2x, double g2y, bool from2) {
-    const double Tm2 = T2 - (dT + dx * g2x + dy * kREarthD * g2y) / 6.0;
-    const double Tm1 = T1 + (dT + dx * g1x + dy * kREarthD * g1y) / 6.0;
-    return from2 ? 0.15 * Tm2 + 0.85 * (Tm1 + Tm2) / 2.0 : 0.15 * Tm1 + 0.85 * (Tm1 + Tm2) / 2.0;
+template <typename R>
+__device__ __forceinline__ R muscl_mean(R T1, R T2, R dT, R dx, R dy, R g1x, R g1y, R g2x, R g2y, bool from2) {
+    const R Tm2 = T2 - (dT + dx * g2x + dy * R(kREarthD) * g2y) / R(6.0);
+    const R Tm1 = T1 + (dT + dx * g1x + dy * R(kREarthD) * g1y) / R(6.0);
+    return from2 ? R(0.15) * Tm2 + R(0.85) * (Tm1 + Tm2) / R(2.0) : R(0.15) * Tm1 + R(0.85) * (Tm1 + Tm2) / R(2.0);
 }
 
 // solve_tracer_muscl for T and S (fv_tracer.F90:8-379), one thread per node
@@ -1422,31 +1424,33 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
     const double Ta = s.scal[0], Sa = s.scal[1];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
-    double rt = 0.0, rs = 0.0, rc = 0.0;
+    // horizontal fluxes in the storage type (FP32 in the mixed-precision path)
+    R rt = R(0), rs = R(0), rc = R(0);
     for (int j = p0; j < p1; ++j) {
         const int cc = m.n2e[j];
         const int ed = cc >= 0 ? cc : ~cc;
         const int2 nd = m.ednodes[ed];
         const int2 et = m.edtri[ed];
-        const double4 cd = m.ecross[ed];
-        const double2 dd = m.edxdy[ed];
+        const double4 cdd = m.ecross[ed];
+        const double2 ddd = m.edxdy[ed];
+        const R cdx = (R)cdd.x, cdy = (R)cdd.y, cdz = (R)cdd.z, cdw = (R)cdd.w, ddy = (R)ddd.y;
         const int e1 = et.x;
-        const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
-        const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
-        const double gT1x = PL(s.gTx, k, nd.x, N), gT2x = PL(s.gTx, k, nd.y, N);
-        const double gT1y = PL(s.gTy, k, nd.x, N), gT2y = PL(s.gTy, k, nd.y, N);
-        const double gS1x = PL(s.gSx, k, nd.x, N), gS2x = PL(s.gSx, k, nd.y, N);
-        const double gS1y = PL(s.gSy, k, nd.x, N), gS2y = PL(s.gSy, k, nd.y, N);
-        const double dTt = 2.0 * (T2 - T1), dTs = 2.0 * (S2 - S1);
+        const R T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
+        const R S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
+        const R gT1x = PL(s.gTx, k, nd.x, N), gT2x = PL(s.gTx, k, nd.y, N);
+        const R gT1y = PL(s.gTy, k, nd.x, N), gT2y = PL(s.gTy, k, nd.y, N);
+        const R gS1x = PL(s.gSx, k, nd.x, N), gS2x = PL(s.gSx, k, nd.y, N);
+        const R gS1y = PL(s.gSy, k, nd.x, N), gS2y = PL(s.gSy, k, nd.y, N);
+        const R dTt = R(2.0) * (T2 - T1), dTs = R(2.0) * (S2 - S1);
         // first segment
-        const double dxa = dd.x * (kREarthD * m.elem_cos[e1]);
-        const double u1 = PL(s.U_nf, k, e1, E), v1 = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
-        const bool f2 = v1 * cd.x - u1 * cd.y > 0;
-        const double mT = muscl_mean(T1, T2, dTt, dxa, dd.y, gT1x, gT1y, gT2x, gT2y, f2);
-        const double mS = muscl_mean(S1, S2, dTs, dxa, dd.y, gS1x, gS1y, gS2x, gS2y, f2);
-        const double cT = (J1 * v1 * mT) * cd.x - (J1 * u1 * mT) * cd.y;
-        const double cS = (J1 * v1 * mS) * cd.x - (J1 * u1 * mS) * cd.y;
-        const double cV = J1 * v1 * cd.x - J1 * u1 * cd.y;
+        const R dxa = (R)(ddd.x * (kREarthD * m.elem_cos[e1]));
+        const R u1 = PL(s.U_nf, k, e1, E), v1 = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
+        const bool f2 = v1 * cdx - u1 * cdy > R(0);
+        const R mT = muscl_mean(T1, T2, dTt, dxa, ddy, gT1x, gT1y, gT2x, gT2y, f2);
+        const R mS = muscl_mean(S1, S2, dTs, dxa, ddy, gS1x, gS1y, gS2x, gS2y, f2);
+        const R cT = (J1 * v1 * mT) * cdx - (J1 * u1 * mT) * cdy;
+        const R cS = (J1 * v1 * mS) * cdx - (J1 * u1 * mS) * cdy;
+        const R cV = J1 * v1 * cdx - J1 * u1 * cdy;
         if (cc >= 0) {
             rt = rt + cT;
             rs = rs + cS;
@@ -1456,15 +1460,15 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
         }
         if (et.y >= 0) {  // second segment
             const int e2 = et.y;
-            const double dxb = dd.x * (kREarthD * m.elem_cos[e2]);
-            const double u2f = PL(s.U_nf, k, e2, E), v2f = PL(s.V_nf, k, e2, E);
-            const double u2 = PL(s.U_n, k, e2, E), v2 = PL(s.V_n, k, e2, E), J2 = PL(s.Je, k, e2, E);
-            const bool g2 = v2f * cd.z - u2f * cd.w < 0;
-            const double nT = muscl_mean(T1, T2, dTt, dxb, dd.y, gT1x, gT1y, gT2x, gT2y, g2);
-            const double nS = muscl_mean(S1, S2, dTs, dxb, dd.y, gS1x, gS1y, gS2x, gS2y, g2);
-            const double c2T = -(J2 * v2 * nT) * cd.z + (J2 * u2 * nT) * cd.w;
-            const double c2S = -(J2 * v2 * nS) * cd.z + (J2 * u2 * nS) * cd.w;
-            const double c2V = -J2 * v2f * cd.z + J2 * u2f * cd.w;
+            const R dxb = (R)(ddd.x * (kREarthD * m.elem_cos[e2]));
+            const R u2f = PL(s.U_nf, k, e2, E), v2f = PL(s.V_nf, k, e2, E);
+            const R u2 = PL(s.U_n, k, e2, E), v2 = PL(s.V_n, k, e2, E), J2 = PL(s.Je, k, e2, E);
+            const bool g2 = v2f * cdz - u2f * cdw < R(0);
+            const R nT = muscl_mean(T1, T2, dTt, dxb, ddy, gT1x, gT1y, gT2x, gT2y, g2);
+            const R nS = muscl_mean(S1, S2, dTs, dxb, ddy, gS1x, gS1y, gS2x, gS2y, g2);
+            const R c2T = -(J2 * v2 * nT) * cdz + (J2 * u2 * nT) * cdw;
+            const R c2S = -(J2 * v2 * nS) * cdz + (J2 * u2 * nS) * cdw;
+            const R c2V = -J2 * v2f * cdz + J2 * u2f * cdw;
             if (cc >= 0) {
                 rt = rt + c2T;
                 rs = rs + c2S;
