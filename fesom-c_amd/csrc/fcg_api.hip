@@ -8,11 +8,12 @@
 #include <vector>
 
 #include "fcg_internal.h"
+#include "fcg_mlp.h"
 
 namespace fcgk {
 template <typename R> __global__ void k_pressure(DevMesh, DevStateT<R>, double, double);
 template <typename R> __global__ void k_bpg(DevMesh, DevStateT<R>);
-template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, double, double, double);
+template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, double, double, double, const int *);
 template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
@@ -40,6 +41,14 @@ template <typename R> __global__ void k_tr_vdiff(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_diag_partial(DevMesh, DevStateT<R>, double, double *);
 __global__ void k_diag_final(const double *, int, double *);
 __global__ void k_mask_count(const double *, int, int *);
+template <typename R>
+__global__ void k_mlp_collect(DevMesh, DevStateT<R>, double, double, double, int, long, float *, float *);
+template <typename R>
+__global__ void k_mlp_closure(DevMesh, DevStateT<R>, const float *, const fcgm::bf16x4 *, MlpNorm, double, double, double,
+                              double, int *);
+__global__ void k_count_flags(const int *, int, int *);
+template <typename R>
+__global__ void k_mlp_apply(DevMesh, DevStateT<R>, const int *);
 }  // namespace fcgk
 
 static const double kPiHost = 3.14159265358979;
@@ -262,7 +271,7 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     }
     if (!(p.tracer_adv == 1 || p.tracer_adv == 2 || p.tracer_adv == 4) ||
         !(p.precision == FCG_FP64 || p.precision == FCG_MIXED) ||
-        p.closure != FCG_CLOSURE_D3END) {
+        !(p.closure == FCG_CLOSURE_D3END || p.closure == FCG_CLOSURE_MLP)) {
         fcg_fail(ctx, FCG_E_ARG, "option not available in this build (tracer_adv/precision/closure)");
         *out = ctx;
         return FCG_E_ARG;
@@ -474,6 +483,9 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     DA3(s.mx_p, NK) DA3(s.mx_q, NK) DA3(s.mx_c2, NK) DA3(s.mx_xx, NK) DA3(s.mx_yy, NK) DA3(s.mx_bA, NK) DA3(s.mx_bB, NK)
     ctx->red_blocks = 512;
     DA(s.red, 16 * (size_t)ctx->red_blocks) DA(s.scal, 32) DA(s.err, 4)
+    if (p.closure == FCG_CLOSURE_MLP) {
+        DA(ctx->mlp_flag, N) DA(ctx->mlp_count, 1)
+    }
     FCG_CHECK(hipHostMalloc((void **)&ctx->tide_pinned, sizeof(double) * std::max(1, p.Mt * mesh->nobn)));
     // ---- cold start: array_setup + jacobian + initial_state (src/fv_main.F90:157-168, 320-362)
     std::vector<double> eta(N, 0.0), ea(N), a1(N), a2(N);
@@ -634,7 +646,22 @@ template <typename R>
 static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
-    LAUNCH(ctx, k_d3end<R>, nblk(ctx->m.N), ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0);
+    if (p.closure == FCG_CLOSURE_MLP) {
+        // learned closure; columns outside its training box go back to d3_end
+        if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
+        const fcg_mlp *net = ctx->mlp;
+        FCG_CHECK(hipMemsetAsync(ctx->mlp_flag, 0, sizeof(int) * ctx->m.N, ctx->stream));
+        LAUNCH(ctx, k_mlp_closure<R>, nblk_lv(ctx->m.N, ctx->m.K), ctx->m, s, (const float *)net->w,
+               (const fcgm::bf16x4 *)net->frag, net->norm, p.Dmin, ctx->density_0, p.z0b_min, p.snul, ctx->mlp_flag);
+        LAUNCH(ctx, k_d3end<R>, nblk(ctx->m.N), ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
+               (const int *)ctx->mlp_flag);
+        LAUNCH(ctx, k_mlp_apply<R>, nblk_lv(ctx->m.N, ctx->m.K), ctx->m, s, (const int *)ctx->mlp_flag);
+        hipLaunchKernelGGL(fcgk::k_count_flags, dim3(64), dim3(kBlk), 0, ctx->stream, (const int *)ctx->mlp_flag,
+                           ctx->m.N, ctx->mlp_count);
+    } else {
+        LAUNCH(ctx, k_d3end<R>, nblk(ctx->m.N), ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
+               (const int *)nullptr);
+    }
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -817,10 +844,42 @@ static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out) 
     out->n_dt = ctx->n_dt;
     out->n_wet_elem = wet;
     out->status = err;
+    if (ctx->mlp_count) {
+        int fb = 0;
+        FCG_CHECK(hipMemcpy(&fb, ctx->mlp_count, sizeof(int), hipMemcpyDeviceToHost));
+        out->mlp_fallback = fb;
+    }
     return FCG_OK;
 }
 
 extern "C" int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out) {
     if (!ctx || !out) return FCG_E_ARG;
     return ctx->mixed ? diagnostics_impl(ctx, retype<float>(ctx->s), out) : diagnostics_impl(ctx, ctx->s, out);
+}
+
+extern "C" int fcg_set_mlp(fcg_ctx *ctx, fcg_mlp *net) {
+    if (!ctx) return FCG_E_ARG;
+    if (ctx->p.closure != FCG_CLOSURE_MLP) return fcg_fail(ctx, FCG_E_ARG, "context was created with closure = d3_end");
+    ctx->mlp = net;
+    return FCG_OK;
+}
+
+template <typename R>
+static int mlp_collect_impl(fcg_ctx *ctx, const DevStateT<R> &s, float *X, float *T, long cap, int stride,
+                            long *count) {
+    const long cols = (ctx->m.N + stride - 1) / stride;
+    const long n = std::min<long>(cap, cols * ctx->m.K);
+    if (n > 0)
+        hipLaunchKernelGGL(fcgk::k_mlp_collect<R>, dim3((unsigned)((n + kBlk - 1) / kBlk)), dim3(kBlk), 0,
+                           ctx->stream, ctx->m, s, ctx->p.Dmin, ctx->density_0, ctx->p.z0b_min, stride, n, X, T);
+    FCG_CHECK(hipGetLastError());
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    *count = n;
+    return FCG_OK;
+}
+
+extern "C" int fcg_mlp_collect(fcg_ctx *ctx, float *X, float *T, long cap, int stride, long *count) {
+    if (!ctx || !X || !T || !count || cap < 0 || stride < 1) return FCG_E_ARG;
+    return ctx->mixed ? mlp_collect_impl(ctx, retype<float>(ctx->s), X, T, cap, stride, count)
+                      : mlp_collect_impl(ctx, ctx->s, X, T, cap, stride, count);
 }

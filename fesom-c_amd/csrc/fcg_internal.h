@@ -131,6 +131,10 @@ struct fcg_ctx {
     hipGraphExec_t bt_graph = nullptr;
     bool bt_graph_ok = false;
     int red_blocks = 0;
+    // learned closure
+    fcg_mlp *mlp = nullptr;      // attached network (not owned)
+    int *mlp_flag = nullptr;     // N: column sent back to d3_end this step
+    int *mlp_count = nullptr;    // cumulative count of such columns
     std::string probe_name;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev;
     std::vector<void *> allocs;

@@ -13,6 +13,7 @@
 // the node scalar a = max(depth+eta_n, Dmin) (src/fv_main.F90:1228-1263),
 // which gives the same bits as the reference's stored arrays.
 #include "fcg_internal.h"
+#include "fcg_mlp.h"
 
 namespace fcgk {
 
@@ -139,9 +140,10 @@ __global__ __launch_bounds__(256) void k_bpg(DevMesh m, DevStateT<R> s) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double dt, double Dmin, double z0b_min, double z0s_min,
-                        double snul, double rho0) {
+                        double snul, double rho0, const int *__restrict__ only) {
     int n = TID();
     if (n >= m.N) return;
+    if (only && only[n] == 0) return;  // learned closure active: only the columns it sent back
     const int N = m.N, E = m.E, K = m.K, L = m.L;
     const double t_inv = 1.0 / dt;
     const double B1cka2 = 16.6 * (kCka * kCka);
@@ -1630,6 +1632,143 @@ __global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevStateT<R> s, dou
     vdiff_column(m, s, n, s.a[n], s.scal[0], s.scal[1], dt);
 }
 
+
+// ---------------------------------------------------------------------------
+// learned closure: features, training samples, fused inference
+// ---------------------------------------------------------------------------
+// Features of one (node, level 1..K) sample, from the quantities d3_end uses
+// (src/fv_mixing.F90:111-194): log10 shear^2 and the buoyancy frequency N^2
+// (asinh-compressed) at the level's interface (the nearest interior one at
+// the surface and bottom), log10 of the column depth, log10 of d3_end's
+// bottom boundary value bott_R*|u_b|^2, and the sigma level.
+template <typename R>
+__device__ void column_features(const DevMesh &m, const DevStateT<R> &s, int n, int nz, double Dmin, double rho0,
+                                double z0b_min, float *f) {
+    const int N = m.N, E = m.E, K = m.K, L = m.L;
+    const double a = s.a[n];
+    const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    double sa = 0.0;
+    for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
+    const double tvi = 1.0 / sa;
+    auto uv = [&](int k, double &u, double &v) {
+        double su = 0.0, sv = 0.0;
+        for (int j = p0; j < p1; ++j) {
+            const int el = m.n2el[j];
+            const double ar = m.elem_area[el];
+            su = su + PL(s.U_n, k, el, E) * ar;
+            sv = sv + PL(s.V_n, k, el, E) * ar;
+        }
+        u = su * tvi;
+        v = sv * tvi;
+    };
+    const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
+    double u1, v1, u2, v2, ub, vb;
+    uv(iz - 2, u1, v1);
+    uv(iz - 1, u2, v2);
+    uv(L - 1, ub, vb);
+    const double d1 = m.sigma[iz - 2] - m.sigma[iz - 1], d2 = m.sigma[iz - 1] - m.sigma[iz];
+    const double d = 0.5 * (d1 + d2);
+    const double sh2 = ((u1 - u2) * (u1 - u2) + (v1 - v2) * (v1 - v2)) / (d * d) / (dmean * dmean);
+    const double zb = m.oms[iz - 1] * a;
+    const double rho_up = densityJM(PL(s.TF, iz - 2, n, N), PL(s.SF, iz - 2, n, N), -zb, rho0);
+    const double rho_dn = densityJM(PL(s.TF, iz - 1, n, N), PL(s.SF, iz - 1, n, N), -zb, rho0);
+    const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
+    const double delta_b = a * m.dsig[L - 1] + z0b_min;
+    const double lg = log(delta_b / z0b_min);
+    const double bott = 16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb);
+    f[0] = (float)log10(sh2 + 1e-12);
+    f[1] = (float)asinh(n2 * 1e5);
+    f[2] = (float)log10(dmean);
+    f[3] = (float)log10(bott + 1e-12);
+    f[4] = (float)m.sigma[nz - 1];
+}
+
+// Training samples from the current state, taken right after d3_end: raw
+// features X[i][5] and targets T[i][2] = log10(snu), log10(Kv) for every
+// level of the columns n = 0, stride, 2*stride, ... (i = (n/stride)*K + nz-1).
+template <typename R>
+__global__ __launch_bounds__(256) void k_mlp_collect(DevMesh m, DevStateT<R> s, double Dmin, double rho0,
+                                                     double z0b_min, int stride, long cap, float *X, float *T) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = m.K;
+    const long col = i / K;
+    const int nz = (int)(i % K) + 1;
+    const long n = col * stride;
+    if (n >= m.N || i >= cap) return;
+    float f[kMlpF];
+    column_features(m, s, (int)n, nz, Dmin, rho0, z0b_min, f);
+    for (int j = 0; j < kMlpF; ++j) X[i * kMlpF + j] = f[j];
+    T[i * 2] = (float)log10((double)PL(s.snu, nz - 1, n, m.N));
+    T[i * 2 + 1] = (float)log10((double)PL(s.Kv, nz - 1, n, m.N));
+}
+
+// The closure step: one thread per (node, level 1..K) computes and normalises
+// its features; each wave runs its 64 samples through the network as four
+// 16-sample MFMA tiles; snu and Kv = 10^(de-normalised outputs), snu held in
+// [snul, 0.2] as d3_end holds it (fv_mixing.F90:282-285), staged in the
+// d3_end scratch planes.  A column with any feature outside the training box
+// is flagged and left to d3_end.
+template <typename R>
+__global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, const float *__restrict__ w,
+                                                     const fcgm::bf16x4 *__restrict__ frag, MlpNorm nm, double Dmin,
+                                                     double rho0, double z0b_min, double snul, int *flag) {
+    __shared__ float xs[4][64 * kMlpF];
+    __shared__ float ys[4][64 * 2];
+    int n, k;
+    const bool valid = node_level(m.N, m.K, n, k);  // k: 0-based level
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    float f[kMlpF] = {0, 0, 0, 0, 0};
+    bool ood = false;
+    if (valid) {
+        column_features(m, s, n, k + 1, Dmin, rho0, z0b_min, f);
+        for (int j = 0; j < kMlpF; ++j) {
+            f[j] = (f[j] - nm.x_mean[j]) / nm.x_std[j];
+            ood = ood || !(f[j] >= nm.x_lo[j] && f[j] <= nm.x_hi[j]);
+        }
+    }
+    for (int j = 0; j < kMlpF; ++j) xs[wv][lane * kMlpF + j] = f[j];
+    __syncthreads();
+    for (int t = 0; t < 4; ++t) {
+        fcgm::Fwd o;
+        fcgm::fwd_tile(frag, w, fcgm::load_bx(&xs[wv][t * 16 * kMlpF], lane), lane, o);
+        if (lane < 16) {
+            ys[wv][(t * 16 + lane) * 2] = o.y[0];
+            ys[wv][(t * 16 + lane) * 2 + 1] = o.y[1];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    if (ood) {
+        atomicOr(&flag[n], 1);
+        return;
+    }
+    const double ls = (double)ys[wv][lane * 2] * nm.y_std[0] + nm.y_mean[0];
+    const double lk = (double)ys[wv][lane * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
+    double sv = pow(10.0, ls);
+    sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
+    // staged until the column's fallback decision is complete (d3_end reads snu)
+    PL(s.mx_bA, k, n, m.N) = (R)sv;
+    PL(s.mx_bB, k, n, m.N) = (R)pow(10.0, lk);
+}
+
+// Columns the network kept: its staged snu/Kv become the state.
+template <typename R>
+__global__ __launch_bounds__(256) void k_mlp_apply(DevMesh m, DevStateT<R> s, const int *__restrict__ flag) {
+    int n, k;
+    if (!node_level(m.N, m.K, n, k)) return;
+    if (flag[n]) return;
+    PL(s.snu, k, n, m.N) = PL(s.mx_bA, k, n, m.N);
+    PL(s.Kv, k, n, m.N) = PL(s.mx_bB, k, n, m.N);
+}
+
+__global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, int *count) {
+    int c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += flag[i] != 0;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
 // ---------------------------------------------------------------------------
 // diagnostics: energy (fv_average_dynamic.F90:476-554), volume, tracer
 // content, extrema -- fixed-order block tree, deterministic run to run
@@ -1744,6 +1883,9 @@ void instantiate_kernels() {
     (void)&k_tr_miura<R>;
     (void)&k_tr_vdiff<R>;
     (void)&k_diag_partial<R>;
+    (void)&k_mlp_collect<R>;
+    (void)&k_mlp_closure<R>;
+    (void)&k_mlp_apply<R>;
 }
 template void instantiate_kernels<double>();
 template void instantiate_kernels<float>();

@@ -29,7 +29,11 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_get_state", "fcg_set_clock", "fcg_advance_clock", "fcg_pressure", "fcg_vertical_mixing",
            "fcg_compute_vel_rhs", "fcg_barotropic_timestep", "fcg_jacobian", "fcg_update_3d_vel",
            "fcg_vert_vel_sigma", "fcg_solve_tracers", "fcg_oce_timestep", "fcg_run", "fcg_diagnostics",
-           "fcg_sync", "fcg_last_error", "fcg_version", "fcg_probe", "fcg_probe_read"]
+           "fcg_sync", "fcg_last_error", "fcg_version", "fcg_probe", "fcg_probe_read", "fcg_mlp_create",
+           "fcg_mlp_destroy", "fcg_mlp_get_params", "fcg_mlp_set_params", "fcg_mlp_set_norm", "fcg_mlp_forward",
+           "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect"]
+
+MLP_FEATURES, MLP_PARAMS = 5, 6288
 
 
 class Params(C.Structure):
@@ -113,6 +117,25 @@ def lib():
         L.fcg_probe_read.argtypes = [P, _DP, _IP]
         L.fcg_version.restype = C.c_char_p
         L.fcg_version.argtypes = []
+        _FP = C.POINTER(C.c_float)
+        L.fcg_mlp_create.restype = C.c_int
+        L.fcg_mlp_create.argtypes = [C.c_uint, C.POINTER(P)]
+        L.fcg_mlp_destroy.argtypes = [P]
+        for n in ("fcg_mlp_get_params", "fcg_mlp_set_params"):
+            getattr(L, n).restype = C.c_int
+            getattr(L, n).argtypes = [P, _FP, C.c_long]
+        L.fcg_mlp_set_norm.restype = C.c_int
+        L.fcg_mlp_set_norm.argtypes = [P] + [_FP] * 6
+        L.fcg_mlp_forward.restype = C.c_int
+        L.fcg_mlp_forward.argtypes = [P, P, P, C.c_long]
+        L.fcg_mlp_gradient.restype = C.c_int
+        L.fcg_mlp_gradient.argtypes = [P, P, P, C.c_long, _FP]
+        L.fcg_mlp_train_step.restype = C.c_int
+        L.fcg_mlp_train_step.argtypes = [P, P, P, C.c_long, C.c_float]
+        L.fcg_set_mlp.restype = C.c_int
+        L.fcg_set_mlp.argtypes = [P, P]
+        L.fcg_mlp_collect.restype = C.c_int
+        L.fcg_mlp_collect.argtypes = [P, P, P, C.c_long, C.c_int, C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -352,3 +375,78 @@ def mesh_arrays(cfg: RunConfig, mesh: meshio.RawMesh) -> dict:
         out["fazt"] = I(v.fazt, 12 * v.nobn)
     L.fcg_mesh_free(h)
     return out
+
+
+def _fp(a):
+    return np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """Device network of the learned closure (fcg_mlp_*).  Device buffers are
+    passed as raw pointers (e.g. ``tensor.data_ptr()`` of a float32 CUDA
+    tensor); there is no host fallback."""
+
+    def __init__(self, seed=0):
+        self.h = C.c_void_p()
+        rc = lib().fcg_mlp_create(seed, C.byref(self.h))
+        if rc:
+            raise GpuError(f"fcg_mlp_create failed: {ERRORS.get(rc, rc)}")
+        self.norm = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fcg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc:
+            raise GpuError(f"{what}: {ERRORS.get(rc, rc)}")
+
+    def params(self):
+        a = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._ck(lib().fcg_mlp_get_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "get_params")
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self._ck(lib().fcg_mlp_set_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "set_params")
+
+    def set_norm(self, x_mean, x_std, y_mean, y_std, x_lo, x_hi):
+        arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in (x_mean, x_std, y_mean, y_std, x_lo, x_hi)]
+        self.norm = arrs
+        self._ck(lib().fcg_mlp_set_norm(self.h, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in arrs]),
+                 "set_norm")
+
+    def forward(self, X_ptr, Y_ptr, n):
+        self._ck(lib().fcg_mlp_forward(self.h, C.c_void_p(X_ptr), C.c_void_p(Y_ptr), n), "forward")
+
+    def gradient(self, X_ptr, T_ptr, n):
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._ck(lib().fcg_mlp_gradient(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), n,
+                                        g.ctypes.data_as(C.POINTER(C.c_float))), "gradient")
+        return g
+
+    def train_step(self, X_ptr, T_ptr, n, lr):
+        self._ck(lib().fcg_mlp_train_step(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), n, lr), "train_step")
+
+
+def model_attach_mlp(model, net: Mlp):
+    model._ck(lib().fcg_set_mlp(model.ctx, net.h), "fcg_set_mlp")
+    model._mlp = net  # keep alive
+
+
+def model_collect(model, X_ptr, T_ptr, cap, stride):
+    cnt = C.c_long()
+    model._ck(lib().fcg_mlp_collect(model.ctx, C.c_void_p(X_ptr), C.c_void_p(T_ptr), cap, stride, C.byref(cnt)),
+              "fcg_mlp_collect")
+    return cnt.value
+
+
+Model.attach_mlp = model_attach_mlp
+Model.collect = model_collect

@@ -107,6 +107,7 @@ typedef struct fcg_mesh {
 } fcg_mesh;
 
 typedef struct fcg_ctx fcg_ctx;
+typedef struct fcg_mlp fcg_mlp;
 
 /* Diagnostics gathered on the device (deterministic reductions). */
 typedef struct fcg_diag {
@@ -205,6 +206,43 @@ int fcg_last_error(const fcg_ctx *ctx, char *buf, int len);
  * synchronous, returns the summed event time and launch count, and resets. */
 int fcg_probe(fcg_ctx *ctx, const char *kernel);
 int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches);
+/* ---- learned vertical closure ------------------------------------------
+ * No counterpart in the reference: these stand in for d3_end
+ * (src/fv_mixing.F90:73-347, called at src/fv_main.F90:2025) when a context
+ * is created with closure = FCG_CLOSURE_MLP.  The network maps five features
+ * of each (column, level) -- log10 shear^2, asinh(1e5 N^2), log10 depth,
+ * log10 bottom stress, sigma -- to log10 snu and log10 Kv (5 -> 64 -> 64 -> 2,
+ * tanh, BF16 matrix-core products with FP32 accumulation).  Columns whose
+ * normalised features leave the box [x_lo, x_hi] fall back to d3_end; their
+ * cumulative count is fcg_diag.mlp_fallback.
+ *
+ * Training uses device buffers (X: n x 5 normalised features, T: n x 2
+ * normalised targets, float, row-major) owned by the caller. */
+int fcg_mlp_create(unsigned seed, fcg_mlp **out);        /* Glorot init */
+void fcg_mlp_destroy(fcg_mlp *net);
+/* Flat FP32 parameters W1[64][16] b1[64] W2[64][64] b2[64] W3[16][64] b3[16]
+ * (count 6288); padding entries stay zero. */
+int fcg_mlp_get_params(fcg_mlp *net, float *host, long count);
+int fcg_mlp_set_params(fcg_mlp *net, const float *host, long count);
+/* Feature/target normalisation x' = (x - mean)/std, y = y'*std + mean, and
+ * the accepted feature box in normalised units (5, 5, 2, 2, 5, 5 floats). */
+int fcg_mlp_set_norm(fcg_mlp *net, const float *x_mean, const float *x_std, const float *y_mean,
+                     const float *y_std, const float *x_lo, const float *x_hi);
+int fcg_mlp_forward(fcg_mlp *net, const float *X_dev, float *Y_dev, long n);   /* synchronous */
+/* Gradient of 0.5*mean_i |f(X_i) - T_i|^2 with respect to the parameters,
+ * copied to grad_host (count 6288); synchronous. */
+int fcg_mlp_gradient(fcg_mlp *net, const float *X_dev, const float *T_dev, long n, float *grad_host);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on that loss; synchronous. */
+int fcg_mlp_train_step(fcg_mlp *net, const float *X_dev, const float *T_dev, long n, float lr);
+/* Attach a trained network to a context created with FCG_CLOSURE_MLP (the
+ * network must outlive the context's steps). */
+int fcg_set_mlp(fcg_ctx *ctx, fcg_mlp *net);
+/* Training samples from the current state, to be called right after
+ * fcg_vertical_mixing: raw features X_dev[i][5] and targets T_dev[i][2] =
+ * log10 snu, log10 Kv for all nsigma levels of every stride-th column
+ * (column-major sample order), at most cap samples; *count = samples written. */
+int fcg_mlp_collect(fcg_ctx *ctx, float *X_dev, float *T_dev, long cap, int stride, long *count);
+
 /* Library build identification ("fesomc_gpu <ver> gfx950 ..."). */
 const char *fcg_version(void);
 

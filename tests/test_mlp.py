@@ -1,0 +1,217 @@
+"""Learned column closure: the BF16 matrix-core network against the CPU
+reference (forward and gradients), training, and the closure in the model
+(gain over the method without it, out-of-distribution fallback)."""
+import math
+
+import numpy as np
+import pytest
+
+import mlp_ref
+
+
+def test_reference_backprop_matches_finite_differences():
+    """Pins the CPU reference: its float64 gradient equals central differences."""
+    p = mlp_ref.init(1).astype(np.float64)
+    rng = np.random.default_rng(0)
+    X, T = rng.normal(size=(37, 5)), rng.normal(size=(37, 2))
+    g = mlp_ref.gradient(p, X, T, bf=False)
+    idx = np.flatnonzero(mlp_ref.used_mask())[::53]
+    fd = mlp_ref.finite_difference(p, X, T, idx)
+    assert np.max(np.abs(fd - g[idx])) <= 1e-6 * max(1.0, np.abs(fd).max())
+
+
+def _torch():
+    import torch
+    return torch
+
+
+GROUPS = {"W1": (mlp_ref.W1, mlp_ref.B1), "b1": (mlp_ref.B1, mlp_ref.W2), "W2": (mlp_ref.W2, mlp_ref.B2),
+          "b2": (mlp_ref.B2, mlp_ref.W3), "W3": (mlp_ref.W3, mlp_ref.B3), "b3": (mlp_ref.B3, mlp_ref.P)}
+
+
+@pytest.mark.gpu
+def test_forward_matches_cpu_reference():
+    torch = _torch()
+    from fesomc_amd import gpu
+    net = gpu.Mlp(seed=3)
+    p = mlp_ref.init(5)
+    net.set_params(p)
+    rng = np.random.default_rng(1)
+    n = 1000  # not a multiple of the 16-sample tile
+    X = rng.normal(size=(n, 5)).astype(np.float32)
+    Xd = torch.tensor(X, device="cuda")
+    Yd = torch.zeros((n, 2), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    net.forward(Xd.data_ptr(), Yd.data_ptr(), n)
+    y = Yd.cpu().numpy()
+    yb, _ = mlp_ref.forward(p, X, bf=True)
+    y64, _ = mlp_ref.forward(p, X, bf=False)
+    # same BF16 roundings: differences only from FP32 accumulation order
+    assert np.abs(y - yb).max() <= 2e-3, np.abs(y - yb).max()
+    # and the BF16 network is close to the float64 one
+    assert np.abs(y - y64).max() <= 5e-2, np.abs(y - y64).max()
+    net.close()
+
+
+@pytest.mark.gpu
+def test_gradient_matches_cpu_reference():
+    """fcg_mlp_gradient (MFMA forward + backward, per-wave partials, ordered
+    reduction) against the CPU reference with the same BF16 roundings."""
+    torch = _torch()
+    from fesomc_amd import gpu
+    net = gpu.Mlp(seed=4)
+    p = mlp_ref.init(7)
+    net.set_params(p)
+    rng = np.random.default_rng(2)
+    n = 5000
+    X = rng.normal(size=(n, 5)).astype(np.float32)
+    T = rng.normal(size=(n, 2)).astype(np.float32)
+    Xd, Td = torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda")
+    torch.cuda.synchronize()
+    g = net.gradient(Xd.data_ptr(), Td.data_ptr(), n).astype(np.float64)
+    gb = mlp_ref.gradient(p, X, T, bf=True)
+    g64 = mlp_ref.gradient(p, X, T, bf=False)
+    for name, (a, b) in GROUPS.items():
+        ref = np.linalg.norm(gb[a:b])
+        assert np.linalg.norm(g[a:b] - gb[a:b]) <= 1e-2 * ref, (name, np.linalg.norm(g[a:b] - gb[a:b]) / ref)
+        assert np.linalg.norm(g[a:b] - g64[a:b]) <= 5e-2 * np.linalg.norm(g64[a:b]), name
+    assert np.all(g[~mlp_ref.used_mask()] == 0.0)
+    net.close()
+
+
+@pytest.mark.gpu
+def test_training_fits_a_smooth_function():
+    torch = _torch()
+    from fesomc_amd import gpu
+    net = gpu.Mlp(seed=5)
+    rng = np.random.default_rng(3)
+    n = 20000
+    X = rng.uniform(-2, 2, size=(n, 5)).astype(np.float32)
+    T = np.stack([np.sin(X[:, 0]) * X[:, 4] + 0.3 * X[:, 1], np.tanh(X[:, 2] - X[:, 3])], axis=1).astype(np.float32)
+    Xd, Td = torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda")
+    torch.cuda.synchronize()
+    l0 = mlp_ref.loss(net.params(), X, T)
+    for it in range(600):
+        lr = 3e-3 if it < 400 else 1e-3
+        net.train_step(Xd.data_ptr(), Td.data_ptr(), n, lr)
+    l1 = mlp_ref.loss(net.params(), X, T)
+    assert l1 < 0.05 * l0, (l0, l1)
+    net.close()
+
+
+# ---- the closure inside the model --------------------------------------------------------
+
+def _model(closure=0, precision=0):
+    from fesomc_amd import gpu, meshgen
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, ex = meshgen.config("c2_small")
+    m = gpu.Model(RunConfig(txt), mesh, precision=precision, closure=closure)
+    m.set_tracers(ex["T"], ex["S"])
+    wx, wy = ex["wind"]
+    tx, ty = meshgen.wind_stress(wx, wy)
+    m.set("windx", np.full(mesh.nod2D, wx))
+    m.set("windy", np.full(mesh.nod2D, wy))
+    m.set("taux", np.full(mesh.elem2D, tx))
+    m.set("tauy", np.full(mesh.elem2D, ty))
+    return m
+
+
+def train_closure(steps=40, iters=1500, seed=0):
+    """Columns from the plain model (d3_end) -> trained network with its
+    normalisation and out-of-distribution box."""
+    torch = _torch()
+    from fesomc_amd import gpu
+    m = _model()
+    cap = m.N * m.K
+    Xs, Ts = [], []
+    for _ in range(steps):
+        for ph in gpu.Model.PHASES:
+            m.phase(ph)
+            if ph == "fcg_vertical_mixing":
+                X = torch.empty((cap, 5), dtype=torch.float32, device="cuda")
+                T = torch.empty((cap, 2), dtype=torch.float32, device="cuda")
+                cnt = m.collect(X.data_ptr(), T.data_ptr(), cap, 1)
+                Xs.append(X[:cnt])
+                Ts.append(T[:cnt])
+    m.close()
+    X, T = torch.cat(Xs), torch.cat(Ts)
+    xm, xs = X.mean(0), X.std(0).clamp_min(1e-6)
+    ym, ys = T.mean(0), T.std(0).clamp_min(1e-6)
+    Xn, Tn = ((X - xm) / xs).contiguous(), ((T - ym) / ys).contiguous()
+    lo, hi = Xn.min(0).values - 0.5, Xn.max(0).values + 0.5
+    net = gpu.Mlp(seed=seed)
+    net.set_norm(xm.cpu().numpy(), xs.cpu().numpy(), ym.cpu().numpy(), ys.cpu().numpy(), lo.cpu().numpy(),
+                 hi.cpu().numpy())
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    n = Xn.shape[0]
+    batch = 8192
+    for it in range(iters):
+        idx = torch.randint(0, n, (batch,), device="cuda", generator=g)
+        xb, tb = Xn[idx].contiguous(), Tn[idx].contiguous()
+        torch.cuda.synchronize()
+        lr = 3e-3 * (0.5 * (1 + math.cos(math.pi * it / iters))) + 1e-4
+        net.train_step(xb.data_ptr(), tb.data_ptr(), batch, lr)
+    torch.cuda.synchronize()
+    return net
+
+
+@pytest.fixture(scope="module")
+def trained_net():
+    net = train_closure()
+    yield net
+    net.close()
+
+
+def _errors(a, b):
+    return {f: float(np.sqrt(np.mean((a.get(f) - b.get(f)) ** 2))) for f in ("eta_n", "U_n", "TF", "SF")}
+
+
+@pytest.mark.gpu
+def test_learned_closure_gains_over_no_closure(trained_net):
+    """The plain closure (d3_end) judges: after 60 steps the learned closure's
+    SSH/current/T/S errors are below those of the method without it (no
+    turbulence closure: background viscosity and diffusivity only)."""
+    from fesomc_amd import gpu
+    ref = _model()
+    mlp = _model(closure=gpu.FCG_CLOSURE_MLP)
+    mlp.attach_mlp(trained_net)
+    none_net = gpu.Mlp(seed=9)
+    none_net.set_params(np.zeros(gpu.MLP_PARAMS, dtype=np.float32))
+    snul = 7.5e-4
+    none_net.set_norm(np.zeros(5), np.ones(5), np.array([math.log10(snul), math.log10(1e-4 * snul)]), np.ones(2),
+                      np.full(5, -1e30), np.full(5, 1e30))
+    bg = _model(closure=gpu.FCG_CLOSURE_MLP)
+    bg.attach_mlp(none_net)
+    for m in (ref, mlp, bg):
+        m.step(60)
+    e_mlp, e_bg = _errors(mlp, ref), _errors(bg, ref)
+    print("learned closure vs d3_end:", e_mlp, "\nno closure vs d3_end:", e_bg)
+    assert mlp.diagnostics()["status"] == 0
+    assert e_mlp["U_n"] < 0.5 * e_bg["U_n"], (e_mlp, e_bg)
+    assert e_mlp["TF"] < e_bg["TF"] and e_mlp["SF"] < e_bg["SF"], (e_mlp, e_bg)
+    for m in (ref, mlp, bg):
+        m.close()
+    none_net.close()
+
+
+@pytest.mark.gpu
+def test_out_of_distribution_columns_fall_back_to_d3end(trained_net):
+    """An empty accepted box sends every column back to d3_end: the run is then
+    bit-identical to the plain one, and every column is counted."""
+    from fesomc_amd import gpu
+    params = trained_net.params()
+    net = gpu.Mlp(seed=1)
+    net.set_params(params)
+    xm, xs, ym, ys, lo, hi = trained_net.norm
+    net.set_norm(xm, xs, ym, ys, np.full(5, 1e30), np.full(5, -1e30))
+    ref = _model()
+    fb = _model(closure=gpu.FCG_CLOSURE_MLP)
+    fb.attach_mlp(net)
+    ref.step(5)
+    fb.step(5)
+    for f in ("eta_n", "U_n", "TF", "Kv", "snu"):
+        assert np.array_equal(ref.get(f), fb.get(f)), f
+    assert fb.diagnostics()["mlp_fallback"] == 5 * fb.N
+    for m in (ref, fb):
+        m.close()
+    net.close()
