@@ -1,17 +1,25 @@
 """Benchmark: node-level-steps/s of the FESOM-C internal step on one MI355X.
 
-Workload (config 2 of the benchmark definition): seeded 20x10 km channel,
-1415x708 = 1,001,820 nodes, 90 % of quads split (~1.9M elements), 24 sigma
-levels (23 layers), wetting/drying on (depth -2..30 m, 15 % intertidal),
-stratified T/S with seeded smooth noise, seeded uniform wind, M2 tide at the
-open west boundary.  One node-level-step = one 2-D node x one layer advanced
-by one internal step (including its Mt barotropic substeps).
+Workload (the 1M-node channel): seeded 20x10 km channel, 1415x708 = 1,001,820
+nodes, 90 % of quads split (~1.9M elements), 24 sigma levels (23 layers),
+wetting/drying on (depth -2..30 m, 15 % intertidal), stratified T/S with
+seeded smooth noise, seeded uniform wind, M2 tide at the open west boundary,
+MUSCL tracers.  One node-level-step = one 2-D node x one layer advanced by one
+internal step (including its Mt barotropic substeps).
 
-The timed region holds exactly --steps internal steps with all inputs resident
-in HBM, bracketed by synchronisation on both sides.  The dominant kernel is
-timed with HIP events on the library's stream (fcg_probe) and reported
-against its algorithmic HBM bytes.  The CPU baseline is the FP64 oracle
-(scalar C, one core) on a bounded sample of the same physics.
+Default (--path headline): the mixed-precision step with the learned closure
+(weights fesom-c_amd/closure_c2.npz, trained on columns of the plain model by
+fesomc_amd.closure) is timed, then the plain FP64 drop-in on the same
+workload; the line reports both rates, the speed-up and the SSH/velocity/T/S
+differences between the two after the same steps.  --path fp64 / mixed time
+one path alone.
+
+Each timed region holds exactly --steps internal steps with all inputs
+resident in HBM, bracketed by synchronisation on both sides.  The probed
+kernel is timed with HIP events on the library's stream (fcg_probe) and
+reported against its algorithmic HBM bytes; traffic comes from the committed
+copy-calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes.  The CPU baseline is
+the FP64 oracle (scalar C, one core) on a bounded sample of the same physics.
 """
 import argparse
 import json
@@ -20,49 +28,54 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(k, N, E, D, L, K):
+def kernel_bytes(k, N, E, D, L, K, e3=8):
     """Algorithmic bytes of one launch: sum over distinct arrays read or
-    written of entities x components x width (index arrays at 4 B)."""
-    d8, i4 = 8, 4
+    written of entities x components x width (index arrays at 4 B; 3-D
+    fields at e3 bytes: 8 in the FP64 path, 4 in the mixed-precision path;
+    mesh geometry and 2-D fields stay FP64)."""
+    i4 = 4
     LN, LE, KN, KE = L * N, L * E, K * N, K * E
+    d8 = 8
     mesh_e = E * (4 * i4 + 4 * d8)          # elem nodes + w_cv
     edge_geo = i4 * (N + 1) + i4 * 2 * D + D * (8 + 8 + 32 + 16)   # n2e CSR, ednodes, edtri, ecross, edxdy
     if k == "k_tr_muscl":
-        rd = edge_geo + d8 * E + N * (8 + 8 + 4) + d8 * (2 * LN + 4 * LN + 5 * LE + KN + 2 * LN + KN)
-        return rd + d8 * 4 * LN      # reads: ab T/S, node gradients, U/V(filt), Je, W, T/S, Kv; writes T/S, T/S_old
+        rd = edge_geo + d8 * E + N * (8 + 8 + 4) + e3 * (2 * LN + 4 * LN + 5 * LE + KN + 2 * LN + KN)
+        return rd + e3 * 4 * LN      # reads: ab T/S, node gradients, U/V(filt), Je, W, T/S, Kv; writes T/S, T/S_old
     if k == "k_tr_gradn_muscl":
-        return i4 * (N + 1) + i4 * 4 * E + E * (8 + 16 + 64) + d8 * 2 * LN + d8 * 4 * LN
+        return i4 * (N + 1) + i4 * 4 * E + E * (8 + 16 + 64) + e3 * 2 * LN + e3 * 4 * LN
     if k == "k_tr_upwind":
         rd = (i4 * (N + 1) + i4 * 2 * D + 2 * i4 * D * 2 + 4 * d8 * D + 3 * d8 * N + i4 * N
-              + d8 * (2 * LN) + d8 * (3 * LE) + d8 * KN + d8 * (2 * LN) + d8 * KN)
-        return rd + d8 * 4 * LN       # writes TF,SF,T_old,S_old
+              + e3 * (2 * LN) + e3 * (3 * LE) + e3 * KN + e3 * (2 * LN) + e3 * KN)
+        return rd + e3 * 4 * LN       # writes TF,SF,T_old,S_old
     if k == "k_velrhs":
-        rd = (mesh_e + E * 8 * d8 + 3 * d8 * E + 4 * i4 * E + d8 * N * 4 + d8 * (2 * LE + 2 * LE + 2 * LE + 4 * LE)
-              + d8 * KN * 2 + i4 * D * 4 + d8 * D * 6 + d8 * E * 4)
-        return rd + d8 * 4 * LE       # writes U_n, V_n, U_rhsAB, V_rhsAB
+        rd = (mesh_e + E * 8 * d8 + 3 * d8 * E + 4 * i4 * E + d8 * N * 4 + e3 * (2 * LE + 2 * LE + 2 * LE + 4 * LE)
+              + e3 * KN * 2 + i4 * D * 4 + d8 * D * 6 + d8 * E * 4)
+        return rd + e3 * 4 * LE       # writes U_n, V_n, U_rhsAB, V_rhsAB
     if k == "k_d3end":
-        rd = i4 * (N + 1) + i4 * 4 * E + d8 * E + d8 * 2 * LE + d8 * 2 * LN + d8 * 3 * KN + d8 * 5 * N
-        return rd + d8 * 3 * KN       # writes bt, snu, Kv
+        rd = i4 * (N + 1) + i4 * 4 * E + d8 * E + e3 * 2 * LE + e3 * 2 * LN + e3 * 3 * KN + d8 * 5 * N
+        return rd + e3 * 3 * KN       # writes bt, snu, Kv
     if k in ("k_vertvel", "k_vertvel_div"):
-        return i4 * (N + 1) + i4 * 2 * D + i4 * 2 * D + d8 * 4 * D + d8 * N + d8 * 3 * LE + d8 * 2 * E + d8 * KN
+        return i4 * (N + 1) + i4 * 2 * D + i4 * 2 * D + d8 * 4 * D + d8 * N + e3 * 3 * LE + d8 * 2 * E + e3 * KN
     if k == "k_bpg":
-        return mesh_e + d8 * E * 8 + d8 * 3 * N + d8 * KN + d8 * 2 * LE + d8 * 2 * E + d8 * E
+        return mesh_e + d8 * E * 8 + d8 * 3 * N + d8 * KN + e3 * 2 * LE + d8 * 2 * E + d8 * E
     if k == "k_pressure":
-        return d8 * N + d8 * 2 * LN + d8 * KN
+        return d8 * N + e3 * 2 * LN + d8 * KN
     if k == "k_velgrad3d":
-        return E * (4 * i4 * 3 + 8 * d8) + d8 * D * 2 + d8 * E + d8 * 2 * LE + d8 * 4 * LE
+        return E * (4 * i4 * 3 + 8 * d8) + d8 * D * 2 + d8 * E + e3 * 2 * LE + e3 * 4 * LE
     if k == "k_update3d":
-        return mesh_e + d8 * 3 * N + d8 * 2 * LE + d8 * LE + d8 * 4 * E + d8 * 4 * LE
+        return mesh_e + d8 * 3 * N + e3 * 2 * LE + e3 * LE + d8 * 4 * E + e3 * 4 * LE
     raise KeyError(k)
 
 
-def pmc_traffic(kernel, path=os.path.join(ROOT, "profiles", "pmc_traffic_fp64_step_c2.md")):
+def pmc_traffic(kernel, path):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes over this workload (copy-calibrated), or None."""
     try:
@@ -73,27 +86,6 @@ def pmc_traffic(kernel, path=os.path.join(ROOT, "profiles", "pmc_traffic_fp64_st
     except OSError:
         pass
     return None
-
-
-def build_model(name, scale=None, precision=0, closure=0):
-    import numpy as np
-    import __graft_entry__ as ge
-    ge.load()
-    from fesomc_amd import gpu, meshgen
-    from fesomc_amd.runconfig import RunConfig
-    txt, mesh, ex = meshgen.config(name, scale=scale)
-    cfg = RunConfig(txt)
-    model = gpu.Model(cfg, mesh, precision=precision, closure=closure)
-    if ex.get("T") is not None:
-        model.set_tracers(ex["T"], ex["S"])
-    if ex.get("wind") is not None:
-        wx, wy = ex["wind"]
-        tx, ty = meshgen.wind_stress(wx, wy)
-        model.set("windx", np.full(mesh.nod2D, wx))
-        model.set("windy", np.full(mesh.nod2D, wy))
-        model.set("taux", np.full(mesh.elem2D, tx))
-        model.set("tauy", np.full(mesh.elem2D, ty))
-    return txt, mesh, ex, model
 
 
 def cpu_baseline(budget_s=15.0):
@@ -135,7 +127,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--probe", default="k_tr_muscl")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--path", choices=["headline", "mixed-mlp", "fp64", "mixed"], default="headline",
+                    help="headline: mixed precision + learned closure, timed beside the plain FP64 drop-in")
+    ap.add_argument("--closure-weights", default=os.path.join(ROOT, "fesom-c_amd", "closure_c2.npz"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
@@ -156,32 +150,48 @@ def main():
               "supported (the multi-GPU halo-exchange design is documented, not built)", file=sys.stderr)
         sys.exit(2)
     import torch
+    import __graft_entry__ as ge
+    ge.load()
+    from fesomc_amd import closure, gpu
+    from fesomc_amd.workload import build_model
 
-    t_setup = time.perf_counter()
-    precision = 1 if args.precision == "mixed" else 0
-    txt, mesh, ex, model = build_model(args.config, precision=precision)
+    def run(precision, clos, probe):
+        t_setup = time.perf_counter()
+        txt, mesh, ex, model = build_model(args.config, precision=precision, closure=clos)
+        if clos == gpu.FCG_CLOSURE_MLP:
+            model.attach_mlp(closure.load(args.closure_weights))
+        setup = time.perf_counter() - t_setup
+        model.step(args.warmup)
+        model.sync()
+        torch.cuda.synchronize()
+        model.probe(probe)
+        t0 = time.perf_counter()
+        model.step(args.steps)
+        model.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        model.probe(None)
+        k_ms, k_n = model.probe_read()
+        diag = model.diagnostics()
+        if diag["status"] != 0 or not math.isfinite(diag["energy"]):
+            print(json.dumps({"error": "model state invalid", "diag": diag}), file=sys.stderr)
+            sys.exit(1)
+        return model, t1 - t0, k_ms / max(k_n, 1), k_n, setup, diag
+
+    headline = args.path == "headline"
+    prec = 0 if args.path == "fp64" else 1
+    clos = gpu.FCG_CLOSURE_MLP if args.path in ("headline", "mixed-mlp") else gpu.FCG_CLOSURE_D3END
+    model, sec, avg_ms, k_n, setup_s, diag = run(prec, clos, args.probe)
     N, E, D, L, K = model.N, model.E, model.D, model.L, model.K
-    setup_s = time.perf_counter() - t_setup
-    model.step(args.warmup)
-    model.sync()
-    torch.cuda.synchronize()
-    model.probe(args.probe)
-    t0 = time.perf_counter()
-    model.step(args.steps)
-    model.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    model.probe(None)
-    k_ms, k_n = model.probe_read()
-    diag = model.diagnostics()
-    if diag["status"] != 0 or not math.isfinite(diag["energy"]):
-        print(json.dumps({"error": "model state invalid", "diag": diag}), file=sys.stderr)
-        sys.exit(1)
-    sec = t1 - t0
     nls = N * L * args.steps / sec
-    avg_ms = k_ms / max(k_n, 1)
-    kb = kernel_bytes(args.probe, N, E, D, L, K)
+    e3 = 4 if prec else 8
+    kb = kernel_bytes(args.probe, N, E, D, L, K, e3)
     achieved = kb / (avg_ms * 1e-3) / 1e9
+    tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{'mixed' if prec else 'fp64'}_step_c2.md")
+    desc = {"headline": "mixed precision (FP32 3-D fluxes/tendencies; FP64 elevation/volume, wet/dry, BPG) "
+                        "+ learned closure (BF16 MFMA MLP, d3_end fallback)",
+            "fp64": "plain FP64 drop-in", "mixed": "mixed precision, d3_end closure",
+            "mixed-mlp": "mixed precision + learned closure (not compared)"}[args.path]
     out = {
         "metric": "node-level-steps/s, mixed+learned vs plain FP64 GPU at <=1cm SSH RMS",
         "value": nls,
@@ -193,18 +203,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if not prec else "f32/f64 (3-D fluxes f32, 2-D and BPG f64), mlp bf16",
         "data": "synthetic (seeded channel mesh generator, analytic forcing)",
         "config": {"workload": f"{args.config}: channel {N} nodes / {E} elements / {D} edges, nsigma {K}, "
-                               f"Mt {model.par.Mt}, wet/dry, MUSCL tracers, plain FP64 drop-in",
-                   "nodes": N, "elements": E, "layers": L, "Mt": model.par.Mt, "path": "fp64"},
+                               f"Mt {model.par.Mt}, wet/dry, MUSCL tracers; {desc}",
+                   "nodes": N, "elements": E, "layers": L, "Mt": model.par.Mt, "path": args.path},
         "roofline": {"kernel": args.probe, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": pmc_traffic(args.probe) if args.config == "c2" else None,
+                     "traffic": pmc_traffic(args.probe, tfile) if args.config == "c2" else None,
                      "avg_ms": avg_ms, "launches": k_n, "algorithmic_bytes": kb},
         "setup_s": setup_s,
         "energy": diag["energy"],
+        "mlp_fallback_columns": diag["mlp_fallback"],
     }
+    if headline:
+        # the plain FP64 drop-in on the same workload: throughput and the error
+        # of the headline path against it after the same warmup + timed steps
+        ref, sec64, _, _, _, _ = run(0, gpu.FCG_CLOSURE_D3END, "")
+        out["fp64_drop_in"] = {"value": N * L * args.steps / sec64, "ms_per_step": 1e3 * sec64 / args.steps}
+        out["speedup_vs_fp64"] = sec64 / sec
+        errs = {}
+        for f, key in (("eta_n", "ssh_m"), ("U_n", "u_m_s"), ("V_n", "v_m_s"), ("TF", "t_degC"), ("SF", "s_psu")):
+            a, b = model.get(f), ref.get(f)
+            errs[key + "_rms"] = float(np.sqrt(np.mean((a - b) ** 2)))
+            errs[key + "_max"] = float(np.max(np.abs(a - b)))
+        out["error_vs_fp64_after_steps"] = errs
+        ref.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     model.close()

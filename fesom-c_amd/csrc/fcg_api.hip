@@ -47,8 +47,7 @@ template <typename R>
 __global__ void k_mlp_closure(DevMesh, DevStateT<R>, const float *, const fcgm::bf16x4 *, MlpNorm, double, double, double,
                               double, int *);
 __global__ void k_count_flags(const int *, int, int *);
-template <typename R>
-__global__ void k_mlp_apply(DevMesh, DevStateT<R>, const int *);
+
 }  // namespace fcgk
 
 static const double kPiHost = 3.14159265358979;
@@ -151,6 +150,7 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
         {"TF", s.TF, L, true, false},          {"SF", s.SF, L, true, false},          {"T_old", s.T_old, L, true, false},
         {"S_old", s.S_old, L, true, false},    {"Tclim", s.Tclim, L, true, false},    {"Sclim", s.Sclim, L, true, false},
         {"Unode_p", s.Unode_p, L, true, false}, {"Vnode_p", s.Vnode_p, L, true, false},
+        {"Unode", s.Unode, L, true, false},    {"Vnode", s.Vnode, L, true, false},
         {"hpressure", s.hp, K, true, false},   {"Kv", s.Kv, K, true, false},          {"snu", s.snu, K, true, false},
         {"bt", s.bt, K, true, false},          {"Wvel", s.W, K, true, false},
     };
@@ -215,7 +215,8 @@ static bool probe_is(const std::string &probe, const char *kern) {
     size_t n = std::strcspn(kern, "<");
     return probe.size() == n && probe.compare(0, n, kern, n) == 0;
 }
-#define LAUNCH(ctx, kern, grid, ...)                                                        \
+#define LAUNCH(ctx, kern, grid, ...) LAUNCH_EX(ctx, kern, grid, kBlk, 0, __VA_ARGS__)
+#define LAUNCH_EX(ctx, kern, grid, block, lds, ...)                                         \
     do {                                                                                   \
         bool pr__ = probe_is((ctx)->probe_name, #kern);                                    \
         hipEvent_t a__ = nullptr, b__ = nullptr;                                           \
@@ -224,7 +225,7 @@ static bool probe_is(const std::string &probe, const char *kern) {
             (void)hipEventCreate(&b__);                                                    \
             (void)hipEventRecord(a__, (ctx)->stream);                                      \
         }                                                                                  \
-        hipLaunchKernelGGL(fcgk::kern, dim3(grid), dim3(kBlk), 0, (ctx)->stream, __VA_ARGS__); \
+        hipLaunchKernelGGL(fcgk::kern, dim3(grid), dim3(block), (lds), (ctx)->stream, __VA_ARGS__); \
         if (pr__) {                                                                        \
             (void)hipEventRecord(b__, (ctx)->stream);                                      \
             (ctx)->probe_ev.emplace_back(a__, b__);                                        \
@@ -469,7 +470,7 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA3(s.U_puls, EL) DA3(s.V_puls, EL)
     DA3(s.vxx, EK) DA3(s.vyu, EK) DA3(s.vyv, EK)
     DA3(s.TF, NL) DA3(s.SF, NL) DA3(s.T_old, NL) DA3(s.S_old, NL) DA3(s.Tclim, NL) DA3(s.Sclim, NL)
-    DA3(s.abT, NL) DA3(s.abS, NL) DA3(s.Unode_p, NL) DA3(s.Vnode_p, NL) DA3(s.tcp, NL) DA3(s.ttp, NL) DA3(s.tsp, NL)
+    DA3(s.abT, NL) DA3(s.abS, NL) DA3(s.Unode_p, NL) DA3(s.Vnode_p, NL) DA3(s.Unode, NL) DA3(s.Vnode, NL) DA3(s.tcp, NL) DA3(s.ttp, NL) DA3(s.tsp, NL)
     if (p.tracer_adv == 4 || (p.tracer_adv == 2 && p.muscl_call)) {
         DA3(s.gTx, NL) DA3(s.gTy, NL)
     }
@@ -646,16 +647,17 @@ template <typename R>
 static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
+    // compute_el2nodes_3D(U_n, V_n, Unode, Vnode), src/fv_main.F90:2006
+    LAUNCH(ctx, k_el2nodes<R>, nblk(ctx->m.N), ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode);
     if (p.closure == FCG_CLOSURE_MLP) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
         const fcg_mlp *net = ctx->mlp;
-        FCG_CHECK(hipMemsetAsync(ctx->mlp_flag, 0, sizeof(int) * ctx->m.N, ctx->stream));
-        LAUNCH(ctx, k_mlp_closure<R>, nblk_lv(ctx->m.N, ctx->m.K), ctx->m, s, (const float *)net->w,
-               (const fcgm::bf16x4 *)net->frag, net->norm, p.Dmin, ctx->density_0, p.z0b_min, p.snul, ctx->mlp_flag);
+        LAUNCH_EX(ctx, k_mlp_closure<R>, (ctx->m.N + 63) / 64, 64, 0, ctx->m, s, (const float *)net->w,
+                  (const fcgm::bf16x4 *)net->frag, net->norm, p.Dmin, ctx->density_0, p.z0b_min, p.snul,
+                  ctx->mlp_flag);
         LAUNCH(ctx, k_d3end<R>, nblk(ctx->m.N), ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
                (const int *)ctx->mlp_flag);
-        LAUNCH(ctx, k_mlp_apply<R>, nblk_lv(ctx->m.N, ctx->m.K), ctx->m, s, (const int *)ctx->mlp_flag);
         hipLaunchKernelGGL(fcgk::k_count_flags, dim3(64), dim3(kBlk), 0, ctx->stream, (const int *)ctx->mlp_flag,
                            ctx->m.N, ctx->mlp_count);
     } else {

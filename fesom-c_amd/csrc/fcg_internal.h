@@ -85,6 +85,7 @@ struct DevStateT {
     R *TF = nullptr, *SF = nullptr, *T_old = nullptr, *S_old = nullptr;       // L*N
     R *Tclim = nullptr, *Sclim = nullptr, *abT = nullptr, *abS = nullptr;     // L*N
     R *Unode_p = nullptr, *Vnode_p = nullptr;                                 // L*N
+    R *Unode = nullptr, *Vnode = nullptr;                                     // L*N node-averaged U_n, V_n
     R *tcp = nullptr, *ttp = nullptr, *tsp = nullptr;                         // L*N Thomas scratch
     R *gTx = nullptr, *gTy = nullptr, *gSx = nullptr, *gSy = nullptr;         // L*N node-averaged tracer gradients (MUSCL/Miura)
     R *tcur = nullptr;                                                        // L*N de-meaned T before the Miura update

@@ -150,21 +150,10 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
     const double a = s.a[n];
     const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]) ;
     const double dmean2 = dmean * dmean;
-    // node average of the element velocities (area weighted)
-    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
-    double sa = 0.0;
-    for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
-    const double tvi = 1.0 / sa;
+    // node-averaged velocities of compute_el2nodes_3D (src/fv_main.F90:2006)
     auto unode = [&](int k, double &u, double &v) {
-        double su = 0.0, sv = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            int el = m.n2el[j];
-            double ar = m.elem_area[el];
-            su = su + PL(s.U_n, k, el, E) * ar;
-            sv = sv + PL(s.V_n, k, el, E) * ar;
-        }
-        u = su * tvi;
-        v = sv * tvi;
+        u = PL(s.Unode, k, n, N);
+        v = PL(s.Vnode, k, n, N);
     };
     // 1-based level helpers
 #define SNU(nz) PL(s.snu, (nz) - 1, n, N)
@@ -1644,23 +1633,12 @@ __global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevStateT<R> s, dou
 template <typename R>
 __device__ void column_features(const DevMesh &m, const DevStateT<R> &s, int n, int nz, double Dmin, double rho0,
                                 double z0b_min, float *f) {
-    const int N = m.N, E = m.E, K = m.K, L = m.L;
+    const int N = m.N, K = m.K, L = m.L;
     const double a = s.a[n];
     const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
-    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
-    double sa = 0.0;
-    for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
-    const double tvi = 1.0 / sa;
-    auto uv = [&](int k, double &u, double &v) {
-        double su = 0.0, sv = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            const int el = m.n2el[j];
-            const double ar = m.elem_area[el];
-            su = su + PL(s.U_n, k, el, E) * ar;
-            sv = sv + PL(s.V_n, k, el, E) * ar;
-        }
-        u = su * tvi;
-        v = sv * tvi;
+    auto uv = [&](int k, double &u, double &v) {  // node averages of compute_el2nodes_3D
+        u = PL(s.Unode, k, n, N);
+        v = PL(s.Vnode, k, n, N);
     };
     const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
     double u1, v1, u2, v2, ub, vb;
@@ -1703,63 +1681,101 @@ __global__ __launch_bounds__(256) void k_mlp_collect(DevMesh m, DevStateT<R> s, 
     T[i * 2 + 1] = (float)log10((double)PL(s.Kv, nz - 1, n, m.N));
 }
 
-// The closure step: one thread per (node, level 1..K) computes and normalises
-// its features; each wave runs its 64 samples through the network as four
-// 16-sample MFMA tiles; snu and Kv = 10^(de-normalised outputs), snu held in
-// [snul, 0.2] as d3_end holds it (fv_mixing.F90:282-285), staged in the
-// d3_end scratch planes.  A column with any feature outside the training box
-// is flagged and left to d3_end.
+// The closure step, one lane per node column (64-thread blocks, one wave):
+// the lane walks its levels 1..K carrying the node-averaged velocities of
+// the two layers around the current interface, so each layer is averaged
+// once; at every level the wave's 64 samples (one per lane) are normalised,
+// staged in LDS and run through the network as four 16-sample MFMA tiles.
+// snu and Kv = 10^(de-normalised outputs), snu held in [snul, 0.2] as d3_end
+// holds it (fv_mixing.F90:282-285), are staged in the d3_end scratch planes;
+// a column whose features all stay inside the training box takes them, any
+// other column is flagged and left to d3_end.
 template <typename R>
-__global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, const float *__restrict__ w,
-                                                     const fcgm::bf16x4 *__restrict__ frag, MlpNorm nm, double Dmin,
-                                                     double rho0, double z0b_min, double snul, int *flag) {
-    __shared__ float xs[4][64 * kMlpF];
-    __shared__ float ys[4][64 * 2];
-    int n, k;
-    const bool valid = node_level(m.N, m.K, n, k);  // k: 0-based level
-    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-    float f[kMlpF] = {0, 0, 0, 0, 0};
-    bool ood = false;
+__global__ __launch_bounds__(64) void k_mlp_closure(DevMesh m, DevStateT<R> s, const float *__restrict__ w,
+                                                    const fcgm::bf16x4 *__restrict__ frag, MlpNorm nm, double Dmin,
+                                                    double rho0, double z0b_min, double snul, int *flag) {
+    __shared__ float xs[64 * kMlpF];
+    __shared__ float ys[64 * 2];
+    const int lane = threadIdx.x;
+    const int n = (int)(xcd_block() * 64u + lane);
+    const bool valid = n < m.N;
+    const int N = m.N, K = m.K, L = m.L;
+    double a = 0.0, dmean = 1.0, f3 = 0.0;
+    auto uv = [&](int k, double &u, double &v) {  // node averages of compute_el2nodes_3D
+        u = PL(s.Unode, k, n, N);
+        v = PL(s.Vnode, k, n, N);
+    };
+    double u_up = 0.0, v_up = 0.0, u_dn = 0.0, v_dn = 0.0;
     if (valid) {
-        column_features(m, s, n, k + 1, Dmin, rho0, z0b_min, f);
-        for (int j = 0; j < kMlpF; ++j) {
-            f[j] = (f[j] - nm.x_mean[j]) / nm.x_std[j];
-            ood = ood || !(f[j] >= nm.x_lo[j] && f[j] <= nm.x_hi[j]);
-        }
+        a = s.a[n];
+        dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+        double ub, vb;
+        uv(L - 1, ub, vb);
+        const double delta_b = a * m.dsig[L - 1] + z0b_min;
+        const double lg = log(delta_b / z0b_min);
+        f3 = log10(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12);
+        uv(0, u_up, v_up);
+        uv(1, u_dn, v_dn);
     }
-    for (int j = 0; j < kMlpF; ++j) xs[wv][lane * kMlpF + j] = f[j];
-    __syncthreads();
-    for (int t = 0; t < 4; ++t) {
-        fcgm::Fwd o;
-        fcgm::fwd_tile(frag, w, fcgm::load_bx(&xs[wv][t * 16 * kMlpF], lane), lane, o);
-        if (lane < 16) {
-            ys[wv][(t * 16 + lane) * 2] = o.y[0];
-            ys[wv][(t * 16 + lane) * 2 + 1] = o.y[1];
+    bool ood = false;
+    int iz_have = 2;  // interface whose two layers u_up/u_dn hold
+    float f[kMlpF];
+    for (int nz = 1; nz <= K; ++nz) {
+        const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);
+        if (valid) {
+            if (iz != iz_have) {  // one layer further down
+                u_up = u_dn;
+                v_up = v_dn;
+                uv(iz - 1, u_dn, v_dn);
+                iz_have = iz;
+            }
+            const double d1 = m.sigma[iz - 2] - m.sigma[iz - 1], d2 = m.sigma[iz - 1] - m.sigma[iz];
+            const double d = 0.5 * (d1 + d2);
+            const double sh2 = ((u_up - u_dn) * (u_up - u_dn) + (v_up - v_dn) * (v_up - v_dn)) / (d * d) /
+                               (dmean * dmean);
+            const double zb = m.oms[iz - 1] * a;
+            const double rho_up = densityJM(PL(s.TF, iz - 2, n, N), PL(s.SF, iz - 2, n, N), -zb, rho0);
+            const double rho_dn = densityJM(PL(s.TF, iz - 1, n, N), PL(s.SF, iz - 1, n, N), -zb, rho0);
+            const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
+            f[0] = (float)log10(sh2 + 1e-12);
+            f[1] = (float)asinh(n2 * 1e5);
+            f[2] = (float)log10(dmean);
+            f[3] = (float)f3;
+            f[4] = (float)m.sigma[nz - 1];
+            for (int j = 0; j < kMlpF; ++j) {
+                f[j] = (f[j] - nm.x_mean[j]) / nm.x_std[j];
+                ood = ood || !(f[j] >= nm.x_lo[j] && f[j] <= nm.x_hi[j]);
+            }
+        } else {
+            for (int j = 0; j < kMlpF; ++j) f[j] = 0.0f;
         }
+        for (int j = 0; j < kMlpF; ++j) xs[lane * kMlpF + j] = f[j];
+        __syncthreads();
+        for (int t = 0; t < 4; ++t) {
+            const fcgm::f32x4 y = fcgm::fwd_infer(frag, w, fcgm::load_bx(&xs[t * 16 * kMlpF], lane), lane);
+            if (lane < 16) {
+                ys[(t * 16 + lane) * 2] = y[0];
+                ys[(t * 16 + lane) * 2 + 1] = y[1];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            const double ls = (double)ys[lane * 2] * nm.y_std[0] + nm.y_mean[0];
+            const double lk = (double)ys[lane * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
+            double sv = pow(10.0, ls);
+            sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
+            PL(s.mx_bA, nz - 1, n, N) = (R)sv;
+            PL(s.mx_bB, nz - 1, n, N) = (R)pow(10.0, lk);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (!valid) return;
-    if (ood) {
-        atomicOr(&flag[n], 1);
-        return;
+    flag[n] = ood ? 1 : 0;
+    if (ood) return;
+    for (int k = 0; k < K; ++k) {
+        PL(s.snu, k, n, N) = PL(s.mx_bA, k, n, N);
+        PL(s.Kv, k, n, N) = PL(s.mx_bB, k, n, N);
     }
-    const double ls = (double)ys[wv][lane * 2] * nm.y_std[0] + nm.y_mean[0];
-    const double lk = (double)ys[wv][lane * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
-    double sv = pow(10.0, ls);
-    sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
-    // staged until the column's fallback decision is complete (d3_end reads snu)
-    PL(s.mx_bA, k, n, m.N) = (R)sv;
-    PL(s.mx_bB, k, n, m.N) = (R)pow(10.0, lk);
-}
-
-// Columns the network kept: its staged snu/Kv become the state.
-template <typename R>
-__global__ __launch_bounds__(256) void k_mlp_apply(DevMesh m, DevStateT<R> s, const int *__restrict__ flag) {
-    int n, k;
-    if (!node_level(m.N, m.K, n, k)) return;
-    if (flag[n]) return;
-    PL(s.snu, k, n, m.N) = PL(s.mx_bA, k, n, m.N);
-    PL(s.Kv, k, n, m.N) = PL(s.mx_bB, k, n, m.N);
 }
 
 __global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, int *count) {
@@ -1885,7 +1901,6 @@ void instantiate_kernels() {
     (void)&k_diag_partial<R>;
     (void)&k_mlp_collect<R>;
     (void)&k_mlp_closure<R>;
-    (void)&k_mlp_apply<R>;
 }
 template void instantiate_kernels<double>();
 template void instantiate_kernels<float>();

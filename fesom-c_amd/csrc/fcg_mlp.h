@@ -57,13 +57,11 @@ __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
+// Four floats -> BF16 (round to nearest even); the compiler emits the gfx950
+// packed conversion v_cvt_pk_bf16_f32 for the vector convert.
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ bf16x4 pack4(f32x4 v) {
-    bf16x4 r;
-    r[0] = (short)f2bf(v[0]);
-    r[1] = (short)f2bf(v[1]);
-    r[2] = (short)f2bf(v[2]);
-    r[3] = (short)f2bf(v[3]);
-    return r;
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16x4v));
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -77,9 +75,16 @@ __device__ __forceinline__ f32x4 bias4(const float *b, int mt, int lane) {
     return c;
 }
 
+// tanh(x) = 1 - 2/(exp(2x) + 1) on the hardware exp2 and reciprocal: a few
+// instructions instead of a library call, ~1e-7 absolute error, far below the
+// BF16 rounding of the activations that follows.
+__device__ __forceinline__ float fast_tanh(float x) {
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * x) + 1.0f);
+}
+
 __device__ __forceinline__ f32x4 tanh4(f32x4 v) {
     f32x4 r;
-    for (int i = 0; i < 4; ++i) r[i] = tanhf(v[i]);
+    for (int i = 0; i < 4; ++i) r[i] = fast_tanh(v[i]);
     return r;
 }
 
@@ -106,6 +111,23 @@ __device__ __forceinline__ void fwd_tile(const bf16x4 *__restrict__ frag, const 
     f32x4 c = bias4(w + kMlpB3, 0, lane);
     for (int kt = 0; kt < 4; ++kt) c = mfma(F3[kt * 64 + lane], o.b3[kt], c);
     o.y = c;  // lanes 0-15: outputs 0..3 (0 and 1 used) of sample lane%16
+}
+
+// The same forward without keeping activations for a backward pass (fewer
+// live registers); its outputs equal fwd_tile's bit for bit.
+__device__ __forceinline__ f32x4 fwd_infer(const bf16x4 *__restrict__ frag, const float *__restrict__ w, bf16x4 bx,
+                                           int lane) {
+    const bf16x4 *F1 = frag, *F2 = frag + 4 * 64, *F3 = frag + 20 * 64;
+    bf16x4 b2[4], b3[4];
+    for (int mt = 0; mt < 4; ++mt) b2[mt] = pack4(tanh4(mfma(F1[mt * 64 + lane], bx, bias4(w + kMlpB1, mt, lane))));
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4 c = bias4(w + kMlpB2, mt, lane);
+        for (int kt = 0; kt < 4; ++kt) c = mfma(F2[(mt * 4 + kt) * 64 + lane], b2[kt], c);
+        b3[mt] = pack4(tanh4(c));
+    }
+    f32x4 c = bias4(w + kMlpB3, 0, lane);
+    for (int kt = 0; kt < 4; ++kt) c = mfma(F3[kt * 64 + lane], b3[kt], c);
+    return c;
 }
 
 // Features of a sample tile -> B fragment, from an LDS tile xs[16][kMlpF]

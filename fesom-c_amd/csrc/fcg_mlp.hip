@@ -70,11 +70,10 @@ __global__ __launch_bounds__(64) void k_mlp_forward(const float *__restrict__ w,
             xs[j] = s < n ? X[s * kMlpF + j % kMlpF] : 0.0f;
         }
         __syncthreads();
-        Fwd o;
-        fwd_tile(frag, w, load_bx(xs, lane), lane, o);
+        const f32x4 y = fwd_infer(frag, w, load_bx(xs, lane), lane);
         if (lane < 16 && s0 + lane < n) {
-            Y[(s0 + lane) * 2] = o.y[0];
-            Y[(s0 + lane) * 2 + 1] = o.y[1];
+            Y[(s0 + lane) * 2] = y[0];
+            Y[(s0 + lane) * 2 + 1] = y[1];
         }
         __syncthreads();
     }

@@ -84,6 +84,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        # PyTorch (used here for device buffers) bundles its own HIP runtime;
+        # load it first so that the process holds one runtime, shared by both.
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         L.fcg_mesh_build.restype = C.c_int

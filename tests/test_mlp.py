@@ -118,41 +118,12 @@ def _model(closure=0, precision=0):
 
 def train_closure(steps=40, iters=1500, seed=0):
     """Columns from the plain model (d3_end) -> trained network with its
-    normalisation and out-of-distribution box."""
-    torch = _torch()
-    from fesomc_amd import gpu
+    normalisation and out-of-distribution box (fesomc_amd.closure)."""
+    from fesomc_amd import closure
     m = _model()
-    cap = m.N * m.K
-    Xs, Ts = [], []
-    for _ in range(steps):
-        for ph in gpu.Model.PHASES:
-            m.phase(ph)
-            if ph == "fcg_vertical_mixing":
-                X = torch.empty((cap, 5), dtype=torch.float32, device="cuda")
-                T = torch.empty((cap, 2), dtype=torch.float32, device="cuda")
-                cnt = m.collect(X.data_ptr(), T.data_ptr(), cap, 1)
-                Xs.append(X[:cnt])
-                Ts.append(T[:cnt])
+    X, T = closure.collect(m, steps, stride=1)
     m.close()
-    X, T = torch.cat(Xs), torch.cat(Ts)
-    xm, xs = X.mean(0), X.std(0).clamp_min(1e-6)
-    ym, ys = T.mean(0), T.std(0).clamp_min(1e-6)
-    Xn, Tn = ((X - xm) / xs).contiguous(), ((T - ym) / ys).contiguous()
-    lo, hi = Xn.min(0).values - 0.5, Xn.max(0).values + 0.5
-    net = gpu.Mlp(seed=seed)
-    net.set_norm(xm.cpu().numpy(), xs.cpu().numpy(), ym.cpu().numpy(), ys.cpu().numpy(), lo.cpu().numpy(),
-                 hi.cpu().numpy())
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    n = Xn.shape[0]
-    batch = 8192
-    for it in range(iters):
-        idx = torch.randint(0, n, (batch,), device="cuda", generator=g)
-        xb, tb = Xn[idx].contiguous(), Tn[idx].contiguous()
-        torch.cuda.synchronize()
-        lr = 3e-3 * (0.5 * (1 + math.cos(math.pi * it / iters))) + 1e-4
-        net.train_step(xb.data_ptr(), tb.data_ptr(), batch, lr)
-    torch.cuda.synchronize()
-    return net
+    return closure.fit(X, T, iters=iters, batch=8192, seed=seed)
 
 
 @pytest.fixture(scope="module")
