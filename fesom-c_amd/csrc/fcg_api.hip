@@ -29,16 +29,16 @@ template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_update3d(DevMesh, DevStateT<R>, double, double);
 template <typename R> __global__ void k_vertvel_div(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_vertvel_scan(DevMesh, DevStateT<R>);
-template <typename R> __global__ void k_minmax_partial(const R *, const R *, size_t, double *);
-__global__ void k_minmax_final(const double *, int, double *);
-template <typename R> __global__ void k_tr_ab(DevStateT<R>, size_t);
+template <typename R> __global__ void k_minmax_partial(DevMesh, const R *, const R *, double *);
+__global__ void k_minmax_final(const double *, int, int, double *);
+template <typename R> __global__ void k_tr_ab(DevMesh, DevStateT<R>, size_t);
 template <typename R> __global__ void k_tr_upwind(DevMesh, DevStateT<R>, double, double, int, double, double);
 template <typename R> __global__ void k_tr_gradn_muscl(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_muscl(DevMesh, DevStateT<R>, double, double, int, double, double);
 template <typename R> __global__ void k_tr_gradn_miura(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_miura(DevMesh, DevStateT<R>, double, double, double, int, double, double);
 template <typename R> __global__ void k_tr_vdiff(DevMesh, DevStateT<R>, double);
-template <typename R> __global__ void k_diag_partial(DevMesh, DevStateT<R>, double, double *);
+template <typename R> __global__ void k_diag_partial(DevMesh, DevStateT<R>, double, int, int, int, int, double *);
 __global__ void k_diag_final(const double *, int, double *);
 __global__ void k_mask_count(const double *, int, int *);
 template <typename R>
@@ -69,6 +69,7 @@ namespace {
 
 constexpr int kBlk = 256;
 inline unsigned nblk(size_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
+constexpr int kMinmaxChunks = 64;  // blocks per ensemble member in the T/S extrema reduction
 // blocks for node x layer kernels: 64 nodes x 4 layers per block of kBlk
 inline unsigned nblk_lv(int N, int L) { return (unsigned)(((N + 63) / 64) * ((L + 3) / 4)); }
 
@@ -257,8 +258,11 @@ extern "C" int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches) {
 
 extern "C" const char *fcg_version(void) { return "fesomc_gpu 0.1 gfx950 fp64+graph"; }
 
-extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx **out) {
-    if (!mesh || !par || !out) return FCG_E_ARG;
+// node_member: ensemble member of each (union) node, members numbered 0..nmemb-1
+// and contiguous in the union numbering; nullptr for a single model.
+static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *node_member, int nmemb,
+                       fcg_ctx **out) {
+    if (!mesh || !par || !out || nmemb < 1) return FCG_E_ARG;
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FCG_E_NOGPU;
@@ -314,17 +318,19 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
             return spread(qx) | (spread(qy) << 1);
         };
         std::vector<uint64_t> kn(N), ke(E), kd(D);
-        for (int n = 0; n < N; ++n) kn[n] = key(xy[2 * n], xy[2 * n + 1]);
+        // ensemble members stay contiguous: member id above the Morton key
+        auto mem = [&](int n0) { return node_member ? (uint64_t)node_member[n0] << 32 : 0ull; };
+        for (int n = 0; n < N; ++n) kn[n] = key(xy[2 * n], xy[2 * n + 1]) | mem(n);
         for (int e = 0; e < E; ++e) {
             const int *q = mesh->elem2D_nodes + 4 * e;
             int nq = (q[0] == q[3]) ? 3 : 4;
             double cx = 0, cy = 0;
             for (int j = 0; j < nq; ++j) { cx += xy[2 * (q[j] - 1)]; cy += xy[2 * (q[j] - 1) + 1]; }
-            ke[e] = key(cx / nq, cy / nq);
+            ke[e] = key(cx / nq, cy / nq) | mem(q[0] - 1);
         }
         for (int d = 0; d < D; ++d) {
             int a0 = mesh->edge_nodes[2 * d] - 1, a1 = mesh->edge_nodes[2 * d + 1] - 1;
-            kd[d] = key(0.5 * (xy[2 * a0] + xy[2 * a1]), 0.5 * (xy[2 * a0 + 1] + xy[2 * a1 + 1]));
+            kd[d] = key(0.5 * (xy[2 * a0] + xy[2 * a1]), 0.5 * (xy[2 * a0 + 1] + xy[2 * a1 + 1])) | mem(a0);
         }
         std::stable_sort(pn.begin(), pn.end(), [&](int i, int j) { return kn[i] < kn[j]; });
         std::stable_sort(pe.begin(), pe.end(), [&](int i, int j) { return ke[i] < ke[j]; });
@@ -336,6 +342,41 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     for (int i = 0; i < D; ++i) qd[pd[i]] = i;
     ctx->pn = pn;
     ctx->pe = pe;
+    // member layout in device order: offsets of each member's nodes / elements
+    m.nmemb = nmemb;
+    std::vector<int> memb(N, 0);
+    ctx->noff.assign(nmemb + 1, 0);
+    ctx->eoff.assign(nmemb + 1, 0);
+    {
+        std::vector<int> ncount(nmemb, 0), ecount(nmemb, 0);
+        int prev = 0;
+        for (int i = 0; i < N; ++i) {
+            const int mb = node_member ? node_member[pn[i]] : 0;
+            if (mb < prev || mb >= nmemb) {
+                fcg_fail(ctx, FCG_E_MESH, "ensemble members are not contiguous");
+                *out = ctx;
+                return FCG_E_MESH;
+            }
+            prev = mb;
+            memb[i] = mb;
+            ncount[mb]++;
+        }
+        prev = 0;
+        for (int i = 0; i < E; ++i) {
+            const int mb = node_member ? node_member[mesh->elem2D_nodes[4 * pe[i]] - 1] : 0;
+            if (mb < prev) {
+                fcg_fail(ctx, FCG_E_MESH, "ensemble members are not contiguous");
+                *out = ctx;
+                return FCG_E_MESH;
+            }
+            prev = mb;
+            ecount[mb]++;
+        }
+        for (int b = 0; b < nmemb; ++b) {
+            ctx->noff[b + 1] = ctx->noff[b] + ncount[b];
+            ctx->eoff[b + 1] = ctx->eoff[b] + ecount[b];
+        }
+    }
     auto QN = [&](int v1) { return v1 > 0 ? qn[v1 - 1] : -1; };  // 1-based reference id -> device index
     auto QE = [&](int v1) { return v1 > 0 ? qe[v1 - 1] : -1; };
     auto QD = [&](int v1) { return v1 > 0 ? qd[v1 - 1] : -1; };
@@ -451,7 +492,7 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     UP(m.ednodes, ednodes) UP(m.edtri, edtri) UP(m.ecross, ecross) UP(m.edxdy, edxdy)
     UP(m.depth, depth) UP(m.area, area) UP(m.ac, ac) UP(m.atp1, atp1) UP(m.atp2, atp2) UP(m.index, index)
     UP(m.obidx, obidx) UP(m.n2e_ptr, n2e_ptr) UP(m.n2e, n2e_idx) UP(m.n2el_ptr, n2el_ptr) UP(m.n2el, n2el_idx)
-    UP(m.sigma, sigma) UP(m.dsig, dsig) UP(m.oms, oms)
+    UP(m.sigma, sigma) UP(m.dsig, dsig) UP(m.oms, oms) UP(m.memb, memb) UP(m.memb_noff, ctx->noff)
     // ---- state
     DevState &s = ctx->s;
     const size_t NL = (size_t)N * L, EL = (size_t)E * L, NK = (size_t)N * K, EK = (size_t)E * K;
@@ -483,7 +524,8 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
     DA(s.hp, NK) DA3(s.Kv, NK) DA3(s.snu, NK) DA3(s.bt, NK) DA3(s.W, NK)
     DA3(s.mx_p, NK) DA3(s.mx_q, NK) DA3(s.mx_c2, NK) DA3(s.mx_xx, NK) DA3(s.mx_yy, NK) DA3(s.mx_bA, NK) DA3(s.mx_bB, NK)
     ctx->red_blocks = 512;
-    DA(s.red, 16 * (size_t)ctx->red_blocks) DA(s.scal, 32) DA(s.err, 4)
+    DA(s.red, std::max<size_t>(16 * (size_t)ctx->red_blocks, 4 * (size_t)kMinmaxChunks * m.nmemb))
+    DA(s.scal, 32) DA(s.err, 4) DA(s.tsa, 2 * (size_t)m.nmemb)
     if (p.closure == FCG_CLOSURE_MLP) {
         DA(ctx->mlp_flag, N) DA(ctx->mlp_count, 1)
     }
@@ -542,6 +584,125 @@ extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx *
 #undef DA
 #undef DA3
 }
+
+extern "C" int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx **out) {
+    return create_impl(mesh, par, nullptr, 1, out);
+}
+
+// Ensemble: the members' meshes joined into one disjoint mesh (nodes and
+// elements concatenated in member order; internal edges of all members
+// first, then their boundary edges, each member keeping its own order), so
+// every stage is one launch over all members and each member's gathers run
+// in its own reference order.
+extern "C" int fcg_create_ensemble(const fcg_mesh *members, int nmemb, const fcg_params *par, fcg_ctx **out) {
+    if (!members || nmemb < 1 || !par || !out) return FCG_E_ARG;
+    *out = nullptr;
+    const fcg_mesh &m0 = members[0];
+    long N = 0, E = 0, D = 0, Din = 0, nob = 0;
+    for (int b = 0; b < nmemb; ++b) {
+        const fcg_mesh &mb = members[b];
+        if (mb.nsigma != m0.nsigma || mb.density_0 != m0.density_0) return FCG_E_ARG;
+        for (int k = 0; k < m0.nsigma; ++k)
+            if (mb.sigma[k] != m0.sigma[k]) return FCG_E_ARG;
+        if ((mb.coord_nod2D == nullptr) != (m0.coord_nod2D == nullptr)) return FCG_E_ARG;
+        N += mb.nod2D;
+        E += mb.elem2D;
+        D += mb.edge2D;
+        Din += mb.edge2D_in;
+        nob += mb.nobn;
+    }
+    std::vector<int> en(4 * E), edn(2 * D), edt(2 * D), eled(4 * E), nbr(4 * E), idx(N), obn(nob), node_member(N);
+    std::vector<double> depth(N), area(N), ac(N), atp(2 * N), coord(2 * N), ea(E), ec(E), cor(E), wcv(4 * E),
+        gsca(8 * E), gvec(8 * E), dxdy(2 * D), cross(4 * D), ampt(12 * std::max(nob, 1L)), fazt(12 * std::max(nob, 1L));
+    long no = 0, eo = 0, dio = 0, dbo = Din, oo = 0;
+    for (int b = 0; b < nmemb; ++b) {
+        const fcg_mesh &mb = members[b];
+        const int n = mb.nod2D, e = mb.elem2D, d = mb.edge2D, di = mb.edge2D_in, ob = mb.nobn;
+        auto uedge = [&](int d0) { return d0 < di ? dio + d0 : dbo + (d0 - di); };  // 0-based member edge -> union
+        auto off = [](int v, long o) { return v > 0 ? (int)(v + o) : v; };          // 1-based id, <=0 kept
+        for (int i = 0; i < n; ++i) {
+            node_member[no + i] = b;
+            idx[no + i] = mb.index_nod2D[i];
+            depth[no + i] = mb.depth[i];
+            area[no + i] = mb.area[i];
+            ac[no + i] = mb.ac[i];
+            atp[no + i] = mb.a_tp[i];
+            atp[N + no + i] = mb.a_tp[n + i];
+            if (mb.coord_nod2D) {
+                coord[2 * (no + i)] = mb.coord_nod2D[2 * i];
+                coord[2 * (no + i) + 1] = mb.coord_nod2D[2 * i + 1];
+            }
+        }
+        for (int i = 0; i < e; ++i) {
+            for (int j = 0; j < 4; ++j) {
+                en[4 * (eo + i) + j] = off(mb.elem2D_nodes[4 * i + j], no);
+                nbr[4 * (eo + i) + j] = off(mb.elem_neighbors[4 * i + j], eo);
+                const int ed = mb.elem_edges[4 * i + j];
+                eled[4 * (eo + i) + j] = ed > 0 ? (int)(uedge(ed - 1) + 1) : ed;
+                wcv[4 * (eo + i) + j] = mb.w_cv[4 * i + j];
+            }
+            for (int j = 0; j < 8; ++j) {
+                gsca[8 * (eo + i) + j] = mb.gradient_sca[8 * i + j];
+                gvec[8 * (eo + i) + j] = mb.gradient_vec[8 * i + j];
+            }
+            ea[eo + i] = mb.elem_area[i];
+            ec[eo + i] = mb.elem_cos[i];
+            cor[eo + i] = mb.coriolis[i];
+        }
+        for (int i = 0; i < d; ++i) {
+            const long u = uedge(i);
+            for (int j = 0; j < 2; ++j) {
+                edn[2 * u + j] = off(mb.edge_nodes[2 * i + j], no);
+                edt[2 * u + j] = off(mb.edge_tri[2 * i + j], eo);
+                dxdy[2 * u + j] = mb.edge_dxdy[2 * i + j];
+            }
+            for (int j = 0; j < 4; ++j) cross[4 * u + j] = mb.edge_cross_dxdy[4 * i + j];
+        }
+        for (int i = 0; i < ob; ++i) {
+            obn[oo + i] = off(mb.in_obn[i], no);
+            for (int j = 0; j < 12; ++j) {
+                ampt[(size_t)j * nob + oo + i] = mb.ampt[(size_t)j * ob + i];
+                fazt[(size_t)j * nob + oo + i] = mb.fazt[(size_t)j * ob + i];
+            }
+        }
+        no += n;
+        eo += e;
+        dio += di;
+        dbo += d - di;
+        oo += ob;
+    }
+    fcg_mesh u = m0;
+    u.nod2D = (int)N;
+    u.elem2D = (int)E;
+    u.edge2D = (int)D;
+    u.edge2D_in = (int)Din;
+    u.nobn = (int)nob;
+    u.elem2D_nodes = en.data();
+    u.edge_nodes = edn.data();
+    u.edge_tri = edt.data();
+    u.elem_edges = eled.data();
+    u.elem_neighbors = nbr.data();
+    u.index_nod2D = idx.data();
+    u.in_obn = obn.data();
+    u.depth = depth.data();
+    u.area = area.data();
+    u.elem_area = ea.data();
+    u.elem_cos = ec.data();
+    u.coriolis = cor.data();
+    u.w_cv = wcv.data();
+    u.gradient_sca = gsca.data();
+    u.gradient_vec = gvec.data();
+    u.edge_dxdy = dxdy.data();
+    u.edge_cross_dxdy = cross.data();
+    u.ac = ac.data();
+    u.a_tp = atp.data();
+    u.ampt = ampt.data();
+    u.fazt = fazt.data();
+    u.coord_nod2D = m0.coord_nod2D ? coord.data() : nullptr;
+    return create_impl(&u, par, node_member.data(), nmemb, out);
+}
+
+extern "C" int fcg_ensemble_size(const fcg_ctx *ctx) { return ctx ? ctx->m.nmemb : 0; }
 
 extern "C" void fcg_destroy(fcg_ctx *ctx) {
     if (!ctx) return;
@@ -766,23 +927,25 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     const DevMesh &m = ctx->m;
     const size_t NL = (size_t)m.N * m.L;
-    int nb = ctx->red_blocks;
-    hipLaunchKernelGGL(fcgk::k_minmax_partial<R>, dim3(nb), dim3(kBlk), 0, ctx->stream, (const R *)s.TF,
-                       (const R *)s.SF, NL, s.red);
-    hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal);
+    // T_aver/S_aver per ensemble member (one member outside an ensemble)
+    const int nb = kMinmaxChunks;
+    hipLaunchKernelGGL(fcgk::k_minmax_partial<R>, dim3(nb, m.nmemb), dim3(kBlk), 0, ctx->stream, m, (const R *)s.TF,
+                       (const R *)s.SF, s.red);
+    hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(nblk(m.nmemb)), dim3(kBlk), 0, ctx->stream, (const double *)s.red,
+                       nb, m.nmemb, s.tsa);
     const int relax = p.cl_relax ? 1 : 0;
     // scheme dispatch of src/fv_main.F90:1788-1800; node x layer kernels, then
     // the column-sequential implicit vertical diffusion
     const unsigned glv = nblk_lv(m.N, m.L);
     if (p.tracer_adv == 1) {
-        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), m, s, NL);
         LAUNCH(ctx, k_tr_upwind<R>, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 2 && p.muscl_call) {
-        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), m, s, NL);
         LAUNCH(ctx, k_tr_gradn_muscl<R>, glv, m, s);
         LAUNCH(ctx, k_tr_muscl<R>, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 4) {
-        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), s, NL);
+        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), m, s, NL);
         LAUNCH(ctx, k_tr_gradn_miura<R>, glv, m, s);
         LAUNCH(ctx, k_tr_miura<R>, glv, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
     }
@@ -816,17 +979,21 @@ extern "C" int fcg_run(fcg_ctx *ctx, int nsteps) {
 }
 
 template <typename R>
-static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out) {
+static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out, int member) {
     const DevMesh &m = ctx->m;
     int nb = ctx->red_blocks;
-    hipLaunchKernelGGL(fcgk::k_diag_partial<R>, dim3(nb), dim3(kBlk), 0, ctx->stream, m, s, ctx->p.Dmin, s.red);
+    // nodes/elements of one member, or of all
+    const int n0 = member < 0 ? 0 : ctx->noff[member], n1 = member < 0 ? m.N : ctx->noff[member + 1];
+    const int e0 = member < 0 ? 0 : ctx->eoff[member], e1 = member < 0 ? m.E : ctx->eoff[member + 1];
+    hipLaunchKernelGGL(fcgk::k_diag_partial<R>, dim3(nb), dim3(kBlk), 0, ctx->stream, m, s, ctx->p.Dmin, n0, n1, e0,
+                       e1, s.red);
     hipLaunchKernelGGL(fcgk::k_diag_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal + 4);
     FCG_CHECK(hipGetLastError());
     double r[12];
     int wet = 0, err = 0;
     FCG_CHECK(hipMemsetAsync(s.err + 1, 0, sizeof(int), ctx->stream));
-    hipLaunchKernelGGL(fcgk::k_mask_count, dim3(64), dim3(kBlk), 0, ctx->stream, (const double *)s.mask_wd, m.E,
-                       s.err + 1);
+    hipLaunchKernelGGL(fcgk::k_mask_count, dim3(64), dim3(kBlk), 0, ctx->stream, (const double *)s.mask_wd + e0,
+                       e1 - e0, s.err + 1);
     FCG_CHECK(hipMemcpyAsync(r, s.scal + 4, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipMemcpyAsync(&err, s.err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipMemcpyAsync(&wet, s.err + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -856,7 +1023,13 @@ static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out) 
 
 extern "C" int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out) {
     if (!ctx || !out) return FCG_E_ARG;
-    return ctx->mixed ? diagnostics_impl(ctx, retype<float>(ctx->s), out) : diagnostics_impl(ctx, ctx->s, out);
+    return ctx->mixed ? diagnostics_impl(ctx, retype<float>(ctx->s), out, -1) : diagnostics_impl(ctx, ctx->s, out, -1);
+}
+
+extern "C" int fcg_diagnostics_member(fcg_ctx *ctx, int member, fcg_diag *out) {
+    if (!ctx || !out || member < 0 || member >= ctx->m.nmemb) return FCG_E_ARG;
+    return ctx->mixed ? diagnostics_impl(ctx, retype<float>(ctx->s), out, member)
+                      : diagnostics_impl(ctx, ctx->s, out, member);
 }
 
 extern "C" int fcg_set_mlp(fcg_ctx *ctx, fcg_mlp *net) {

@@ -53,6 +53,10 @@ struct DevMesh {
     // CSR gathers
     int *n2e_ptr = nullptr, *n2e = nullptr;   // node -> edges, ascending; e>=0: node is edge_nodes(1) (+), ~e: (2) (-)
     int *n2el_ptr = nullptr, *n2el = nullptr; // node -> elements, ascending (nod_in_elem2D)
+    // ensemble members: nodes and elements of one member are contiguous
+    int nmemb = 1;
+    int *memb = nullptr;       // N: member of each node
+    int *memb_noff = nullptr;  // nmemb+1: first node of each member
     // vertical grid
     double *sigma = nullptr;   // K
     double *dsig = nullptr;    // L: sigma(k)-sigma(k+1)
@@ -95,7 +99,8 @@ struct DevStateT {
     R *mx_bA = nullptr, *mx_bB = nullptr;                                     // K*N
     // reductions
     double *red = nullptr;       // partial results
-    double *scal = nullptr;      // [0] T_aver [1] S_aver, diagnostics after
+    double *scal = nullptr;      // diagnostics (from [4])
+    double *tsa = nullptr;       // per member: [2m] T_aver, [2m+1] S_aver
     int *err = nullptr;          // device STOP flag
 };
 using DevState = DevStateT<double>;
@@ -125,6 +130,7 @@ struct fcg_ctx {
     // host copies needed for tides and the Fortran-layout transfer
     std::vector<int> in_obn0;            // 0-based OB nodes (reference ids)
     std::vector<int> pn, pe;             // device index -> reference index (nodes, elements)
+    std::vector<int> noff, eoff;         // per member: first device node / element (nmemb+1 each)
     std::vector<double> ampt, fazt, amp_factor, phy_factor; // (nobn,12), (12)
     std::vector<double> tide_host;       // Mt*nobn staging (pinned below)
     double *tide_pinned = nullptr;

@@ -1000,13 +1000,20 @@ __global__ __launch_bounds__(256) void k_vertvel_scan(DevMesh m, DevStateT<R> s)
 // ---------------------------------------------------------------------------
 // tracers: solve_tracers (fv_main.F90:1707-1836)
 // ---------------------------------------------------------------------------
-// global min/max of TF and SF (fv_main.F90:1737-1777): exact, order-free
+// global min/max of TF and SF (fv_main.F90:1737-1777), per ensemble member:
+// blockIdx.y = member, blockIdx.x = chunk of the member's (level, node)
+// values; exact and order-free.  part[(m*gridDim.x + chunk)*4 + c]
 template <typename R>
-__global__ __launch_bounds__(256) void k_minmax_partial(const R *__restrict__ T, const R *__restrict__ S, size_t n, double *out) {
+__global__ __launch_bounds__(256) void k_minmax_partial(DevMesh m, const R *__restrict__ T, const R *__restrict__ S,
+                                                        double *part) {
     __shared__ double sm[4][256];
+    const int mb = blockIdx.y;
+    const int n0 = m.memb_noff[mb], cnt = m.memb_noff[mb + 1] - n0;
+    const size_t tot = (size_t)cnt * m.L;
     double tmn = INFINITY, tmx = -INFINITY, smn = INFINITY, smx = -INFINITY;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        double t = T[i], v = S[i];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = i / cnt, n = n0 + i % cnt;
+        const double t = T[k * m.N + n], v = S[k * m.N + n];
         tmn = fmin(tmn, t);
         tmx = fmax(tmx, t);
         smn = fmin(smn, v);
@@ -1027,28 +1034,32 @@ __global__ __launch_bounds__(256) void k_minmax_partial(const R *__restrict__ T,
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int c = 0; c < 4; ++c) out[4 * blockIdx.x + c] = sm[c][0];
+        for (int c = 0; c < 4; ++c) part[((size_t)mb * gridDim.x + blockIdx.x) * 4 + c] = sm[c][0];
 }
 
-__global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb, double *scal) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double tmn = part[0], tmx = part[1], smn = part[2], smx = part[3];
+// T_aver, S_aver of each member: midpoints of its extrema
+__global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb, int nmemb, double *tsa) {
+    const int mb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mb >= nmemb) return;
+    const double *p = part + (size_t)mb * nb * 4;
+    double tmn = p[0], tmx = p[1], smn = p[2], smx = p[3];
     for (int b = 1; b < nb; ++b) {
-        tmn = fmin(tmn, part[4 * b]);
-        tmx = fmax(tmx, part[4 * b + 1]);
-        smn = fmin(smn, part[4 * b + 2]);
-        smx = fmax(smx, part[4 * b + 3]);
+        tmn = fmin(tmn, p[4 * b]);
+        tmx = fmax(tmx, p[4 * b + 1]);
+        smn = fmin(smn, p[4 * b + 2]);
+        smx = fmax(smx, p[4 * b + 3]);
     }
-    scal[0] = 0.5 * (tmn + tmx);
-    scal[1] = 0.5 * (smn + smx);
+    tsa[2 * mb] = 0.5 * (tmn + tmx);
+    tsa[2 * mb + 1] = 0.5 * (smn + smx);
 }
 
 // AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444)
 template <typename R>
-__global__ __launch_bounds__(256) void k_tr_ab(DevStateT<R> s, size_t n) {
+__global__ __launch_bounds__(256) void k_tr_ab(DevMesh m, DevStateT<R> s, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double Ta = s.scal[0], Sa = s.scal[1];
+    const int mb = m.memb[i % m.N];
+    const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
     s.abT[i] = -(0.5 + kEps) * s.T_old[i] + (1.5 + kEps) * s.TF[i] - Ta;
     s.abS[i] = -(0.5 + kEps) * s.S_old[i] + (1.5 + kEps) * s.SF[i] - Sa;
 }
@@ -1259,7 +1270,8 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
-    const double Ta = s.scal[0], Sa = s.scal[1];
+    const int mb = m.memb[n];
+    const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
     const bool ob = cl_relax && m.index[n] == 2;
@@ -1374,7 +1386,7 @@ __global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevStateT<R> 
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N;
-    const double Ta = s.scal[0];
+    const double Ta = s.tsa[2 * m.memb[n]];
     const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
     double ssum = 0.0;
     for (int j = p0; j < p1; ++j) ssum = ssum + m.elem_area[m.n2el[j]];
@@ -1412,7 +1424,8 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
-    const double Ta = s.scal[0], Sa = s.scal[1];
+    const int mb = m.memb[n];
+    const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
     // horizontal fluxes in the storage type (FP32 in the mixed-precision path)
@@ -1526,7 +1539,7 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
     int n, k;
     if (!node_level(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
-    const double Ta = s.scal[0];
+    const double Ta = s.tsa[2 * m.memb[n]];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
     const double ab = 1.0;
@@ -1618,7 +1631,8 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt) {
     int n = TID();
     if (n >= m.N) return;
-    vdiff_column(m, s, n, s.a[n], s.scal[0], s.scal[1], dt);
+    const int mb = m.memb[n];
+    vdiff_column(m, s, n, s.a[n], s.tsa[2 * mb], s.tsa[2 * mb + 1], dt);
 }
 
 
@@ -1791,13 +1805,15 @@ __global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, int
 // ---------------------------------------------------------------------------
 constexpr int kDiag = 12;
 template <typename R>
-__global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevStateT<R> s, double Dmin, double *out) {
+__global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevStateT<R> s, double Dmin, int n0, int n1, int e0,
+                                                      int e1, double *out) {
     __shared__ double sm[kDiag][256];
     double v[kDiag] = {0, 0, 0, 0, 0, 0, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
-    // v: 0 PE, 1 area, 2 KE, 3 volume, 4 heat, 5 salt, 6/7 eta, 8/9 T, 10/11 S
-    const int N = m.N, E = m.E, L = m.L;
+    // v: 0 PE, 1 area, 2 KE, 3 volume, 4 heat, 5 salt, 6/7 eta, 8/9 T, 10/11 S over nodes [n0, n1),
+    // elements [e0, e1) (one ensemble member, or all)
+    const int N = m.N, L = m.L;
     const int stride = gridDim.x * blockDim.x;
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
+    for (int n = n0 + blockIdx.x * blockDim.x + threadIdx.x; n < n1; n += stride) {
         double ar = m.area[n], et = s.eta_n[n];
         v[0] += 0.5 * kG * ar * (et * et);
         v[1] += ar;
@@ -1816,7 +1832,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevStateT<R> s,
             v[11] = fmax(v[11], sv);
         }
     }
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+    for (int e = e0 + blockIdx.x * blockDim.x + threadIdx.x; e < e1; e += stride) {
         const int4 q = m.en[e];
         const double4 w = m.wcv[e];
         double f0 = dmaxr(Dmin, m.depth[q.x] + s.eta_n[q.x]), f1 = dmaxr(Dmin, m.depth[q.y] + s.eta_n[q.y]);

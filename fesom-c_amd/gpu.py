@@ -31,7 +31,8 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_vert_vel_sigma", "fcg_solve_tracers", "fcg_oce_timestep", "fcg_run", "fcg_diagnostics",
            "fcg_sync", "fcg_last_error", "fcg_version", "fcg_probe", "fcg_probe_read", "fcg_mlp_create",
            "fcg_mlp_destroy", "fcg_mlp_get_params", "fcg_mlp_set_params", "fcg_mlp_set_norm", "fcg_mlp_forward",
-           "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect"]
+           "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect",
+           "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member"]
 
 MLP_FEATURES, MLP_PARAMS = 5, 6288
 
@@ -141,6 +142,12 @@ def lib():
         L.fcg_mlp_train_step.argtypes = [P, P, P, C.c_long, C.c_float]
         L.fcg_set_mlp.restype = C.c_int
         L.fcg_set_mlp.argtypes = [P, P]
+        L.fcg_create_ensemble.restype = C.c_int
+        L.fcg_create_ensemble.argtypes = [C.POINTER(Mesh), C.c_int, C.POINTER(Params), C.POINTER(P)]
+        L.fcg_ensemble_size.restype = C.c_int
+        L.fcg_ensemble_size.argtypes = [P]
+        L.fcg_diagnostics_member.restype = C.c_int
+        L.fcg_diagnostics_member.argtypes = [P, C.c_int, C.POINTER(Diag)]
         L.fcg_mlp_collect.restype = C.c_int
         L.fcg_mlp_collect.argtypes = [P, P, P, C.c_long, C.c_int, C.POINTER(C.c_long)]
         _lib = L
@@ -457,3 +464,76 @@ def model_collect(model, X_ptr, T_ptr, cap, stride):
 
 Model.attach_mlp = model_attach_mlp
 Model.collect = model_collect
+
+
+class Ensemble(Model):
+    """Several models with their own meshes (uneven sizes) and forcing,
+    advanced together by one context (fcg_create_ensemble).  ``get``/``set``
+    address the joined arrays; ``split`` cuts them per member."""
+
+    def __init__(self, cfg: RunConfig, meshes, precision=FCG_FP64, closure=FCG_CLOSURE_D3END):
+        L = lib()
+        bad = cfg.unsupported()
+        if bad:
+            raise GpuError(f"options outside this build's hot path: {bad}")
+        self.cfg = cfg
+        self._mc = mesh_cfg(cfg)
+        self._meshes, self.views = [], []
+        for mesh in meshes:
+            xy = np.ascontiguousarray(mesh.xy, dtype=np.float64).reshape(-1)
+            ind = np.ascontiguousarray(mesh.index, dtype=np.int32)
+            el = np.ascontiguousarray(mesh.elem, dtype=np.int32).reshape(-1)
+            dep = np.ascontiguousarray(mesh.depth, dtype=np.float64)
+            ob = None if mesh.ob is None else np.ascontiguousarray(mesh.ob, dtype=np.float64).reshape(-1)
+            nobn = 0 if mesh.ob is None else int(mesh.ob.shape[0])
+            h = C.c_void_p()
+            rc = L.fcg_mesh_build(C.byref(self._mc), mesh.nod2D, _dp(xy), _ip(ind), mesh.elem2D, _ip(el), _dp(dep),
+                                  nobn, None if ob is None else _dp(ob), C.byref(h))
+            if rc:
+                raise GpuError(f"fcg_mesh_build failed: {ERRORS.get(rc, rc)}")
+            v = Mesh()
+            L.fcg_mesh_view(h, C.byref(v))
+            self._meshes.append(h)
+            self.views.append(v)
+        arr = (Mesh * len(self.views))(*self.views)
+        self.nodes = [v.nod2D for v in self.views]
+        self.elems = [v.elem2D for v in self.views]
+        self.N, self.E = sum(self.nodes), sum(self.elems)
+        self.D = sum(v.edge2D for v in self.views)
+        self.K = self.views[0].nsigma
+        self.L = self.K - 1
+        self.par = params(cfg, precision, closure)
+        self.ctx = C.c_void_p()
+        rc = L.fcg_create_ensemble(arr, len(self.views), C.byref(self.par), C.byref(self.ctx))
+        if rc:
+            msg = self.last_error() if self.ctx else ""
+            raise GpuError(f"fcg_create_ensemble failed: {ERRORS.get(rc, rc)} {msg}")
+        self.members = len(self.views)
+        T = np.concatenate([np.full((v.nod2D, self.L), cfg.f("T_const", 10.0)) for v in self.views])
+        S = np.concatenate([np.full((v.nod2D, self.L), cfg.f("S_const", 35.0)) for v in self.views])
+        self.set_tracers(T, S)
+
+    def close(self):
+        L = lib()
+        if getattr(self, "ctx", None):
+            L.fcg_destroy(self.ctx)
+            self.ctx = None
+        for h in getattr(self, "_meshes", []):
+            L.fcg_mesh_free(h)
+        self._meshes = []
+
+    def split(self, arr, nodal=True):
+        """Cut a joined (entities, ...) array into the members' arrays."""
+        sizes = self.nodes if nodal else self.elems
+        a = np.asarray(arr)
+        a = a.reshape(sum(sizes), -1) if a.ndim == 1 else a
+        out, o = [], 0
+        for n in sizes:
+            out.append(a[o:o + n])
+            o += n
+        return out
+
+    def diagnostics_member(self, b):
+        d = Diag()
+        self._ck(lib().fcg_diagnostics_member(self.ctx, b, C.byref(d)), "fcg_diagnostics_member")
+        return d.as_dict()

@@ -242,3 +242,38 @@ def config(name, seed=0, scale=None):
         spd, ang = rng.uniform(0.0, 10.0), rng.uniform(0.0, 2 * np.pi)
         return txt, m, dict(T=T, S=S, wind=(spd * math.cos(ang), spd * math.sin(ang)))
     raise KeyError(name)
+
+
+def ensemble(name="c4", members=None, seed=0):
+    """Ensemble of channels with uneven mesh sizes and perturbed forcing, for
+    fcg_create_ensemble.  Returns (fv_run text, [RawMesh], [extras]) where
+    every member shares nsigma, dt and the physics switches (dt fits the
+    finest member), and differs in resolution, bathymetry seed, tidal
+    amplitude (+-10 %), wind (+-3 m/s per component) and T/S noise.
+
+    ``c4``: 16 members, 150..400 nodes along the channel (~11k..80k nodes
+    each), nsigma 24.  ``c4_small``: 3 members of 30..60 nodes along, nsigma 8.
+    """
+    rng = np.random.default_rng(seed + 4242)
+    if name == "c4":
+        nm, lo, hi, nsig = members or 16, 150, 400, 24
+    elif name == "c4_small":
+        nm, lo, hi, nsig = members or 3, 30, 60, 8
+    else:
+        raise KeyError(name)
+    nxs = rng.integers(lo, hi + 1, nm)
+    dx_min = 20000.0 / (nxs.max() - 1)
+    dt_2D = min(2.0, round(0.45 * dx_min / math.sqrt(9.8 * 34.0), 2))
+    txt = fv_run(name, dt_2D, 10 * dt_2D, nsig, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.25)
+    base = rng.uniform(0.0, 10.0, 2) - 5.0
+    meshes, extras = [], []
+    for b, nx in enumerate(nxs):
+        ny = max(8, int(nx) // 2)
+        amp = 1.0 + rng.uniform(-0.1, 0.1)
+        m = channel(int(nx), ny, split=0.9, seed=seed + 100 + b, intertidal=0.15, depth_west=30.0, depth_dry=-2.0,
+                    tide_amp=amp)
+        T, S = stratified_ts(m, nsig, seed=seed + 200 + b)
+        wx, wy = base + rng.uniform(-3.0, 3.0, 2)
+        meshes.append(m)
+        extras.append(dict(T=T, S=S, wind=(float(wx), float(wy)), tide_amp=amp))
+    return txt, meshes, extras

@@ -155,6 +155,19 @@ void fcg_mesh_free(fcg_mesh_obj *m);
 int fcg_create(const fcg_mesh *mesh, const fcg_params *par, fcg_ctx **out);
 void fcg_destroy(fcg_ctx *ctx);
 
+/* Ensemble (no counterpart in the reference, which runs one model per
+ * executable): nmemb models with their own meshes (uneven sizes) and forcing
+ * advanced together, every stage one launch over all members.  The meshes
+ * are joined into one disjoint mesh: nodes and elements concatenated in
+ * member order, internal edges of all members first, then their boundary
+ * edges, each member keeping its own order, so each member evolves exactly
+ * as it would alone (T_aver/S_aver and diagnostics are per member).  The
+ * members must share nsigma, sigma, density_0 and the run parameters; state
+ * transfers address the joined arrays (member b's nodes follow those of
+ * members 0..b-1).  scale_area (Miura only) is taken from member 0. */
+int fcg_create_ensemble(const fcg_mesh *members, int nmemb, const fcg_params *par, fcg_ctx **out);
+int fcg_ensemble_size(const fcg_ctx *ctx);
+
 /* ---- state transfer (host mirrors <-> device), synchronous -------------
  * name is the reference array name: eta_n, eta_n_1, eta_n_2, U_n_2D, U_n_1,
  * U_n_2, U_n, V_n, U_rhsAB, V_rhsAB, Wvel, bt, snu, Kv, TF, SF, T_old,
@@ -196,6 +209,8 @@ int fcg_run(fcg_ctx *ctx, int nsteps);
 
 /* ---- diagnostics / control --------------------------------------------- */
 int fcg_diagnostics(fcg_ctx *ctx, fcg_diag *out);  /* synchronous */
+/* The same for one ensemble member (0-based). */
+int fcg_diagnostics_member(fcg_ctx *ctx, int member, fcg_diag *out);
 int fcg_sync(fcg_ctx *ctx);
 /* Copy of the last error text, NUL-terminated, truncated to len. */
 int fcg_last_error(const fcg_ctx *ctx, char *buf, int len);
