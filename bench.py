@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--path", choices=["headline", "mixed-mlp", "fp64", "mixed"], default="headline",
                     help="headline: mixed precision + learned closure, timed beside the plain FP64 drop-in")
     ap.add_argument("--closure-weights", default=os.path.join(ROOT, "fesom-c_amd", "closure_c2.npz"))
+    ap.add_argument("--launch-table", default="auto",
+                    help="block-size table from fesomc_amd.autotune (JSON); 'auto' = the stored table of this "
+                         "config and precision if there is one, 'none' = library defaults")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
@@ -152,7 +155,7 @@ def main():
     import torch
     import __graft_entry__ as ge
     ge.load()
-    from fesomc_amd import closure, gpu
+    from fesomc_amd import autotune, closure, gpu
     from fesomc_amd.workload import build_model
 
     def run(precision, clos, probe):
@@ -160,6 +163,12 @@ def main():
         txt, mesh, ex, model = build_model(args.config, precision=precision, closure=clos)
         if clos == gpu.FCG_CLOSURE_MLP:
             model.attach_mlp(closure.load(args.closure_weights))
+        table = args.launch_table
+        if table == "auto":
+            table = os.path.join(ROOT, "fesom-c_amd", f"launch_{'mixed' if precision else 'fp64'}_{args.config}.json")
+            table = table if os.path.exists(table) else "none"
+        if table != "none":
+            autotune.apply(model, autotune.load(table))
         setup = time.perf_counter() - t_setup
         model.step(args.warmup)
         model.sync()

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "fcg_internal.h"
@@ -70,8 +71,6 @@ namespace {
 constexpr int kBlk = 256;
 inline unsigned nblk(size_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
 constexpr int kMinmaxChunks = 64;  // blocks per ensemble member in the T/S extrema reduction
-// blocks for node x layer kernels: 64 nodes x 4 layers per block of kBlk
-inline unsigned nblk_lv(int N, int L) { return (unsigned)(((N + 63) / 64) * ((L + 3) / 4)); }
 
 template <typename T>
 int dalloc(fcg_ctx *ctx, T **p, size_t n) {
@@ -211,12 +210,33 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
 
 // Launch a kernel on the context stream, bracketed by probe events when the
 // kernel's name (without its storage-type argument) is the probed one.
+// Launch block size of a kernel (without its storage-type argument): the
+// autotuned value when one was set, else 256.
+static int block_of(const fcg_ctx *ctx, const char *kern) {
+    const std::string k(kern, std::strcspn(kern, "<"));
+    auto it = ctx->launch_block.find(k);
+    return it == ctx->launch_block.end() ? 256 : it->second;
+}
+
 static bool probe_is(const std::string &probe, const char *kern) {
     if (probe.empty()) return false;
     size_t n = std::strcspn(kern, "<");
     return probe.size() == n && probe.compare(0, n, kern, n) == 0;
 }
 #define LAUNCH(ctx, kern, grid, ...) LAUNCH_EX(ctx, kern, grid, kBlk, 0, __VA_ARGS__)
+// one thread per item, block size from the context's launch table
+#define LAUNCH1(ctx, kern, items, ...)                                                     \
+    do {                                                                                   \
+        const int bs__ = block_of(ctx, #kern);                                             \
+        LAUNCH_EX(ctx, kern, (unsigned)(((size_t)(items) + bs__ - 1) / bs__), bs__, 0, __VA_ARGS__); \
+    } while (0)
+// one thread per (entity, layer): 64 entities x (block/64) layers per block
+#define LAUNCHLV(ctx, kern, ents, levels, ...)                                             \
+    do {                                                                                   \
+        const int bs__ = block_of(ctx, #kern), lpb__ = bs__ / 64;                          \
+        LAUNCH_EX(ctx, kern, (unsigned)(((ents) + 63) / 64) * (unsigned)(((levels) + lpb__ - 1) / lpb__), bs__, 0, \
+                  __VA_ARGS__);                                                            \
+    } while (0)
 #define LAUNCH_EX(ctx, kern, grid, block, lds, ...)                                         \
     do {                                                                                   \
         bool pr__ = probe_is((ctx)->probe_name, #kern);                                    \
@@ -793,8 +813,8 @@ extern "C" int fcg_advance_clock(fcg_ctx *ctx) {
 template <typename R>
 static int pressure_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const double gd0 = kG / ctx->density_0;
-    LAUNCH(ctx, k_pressure<R>, nblk(ctx->m.N), ctx->m, s, gd0, ctx->density_0);
-    LAUNCH(ctx, k_bpg<R>, nblk(ctx->m.E), ctx->m, s);
+    LAUNCH1(ctx, k_pressure<R>, ctx->m.N, ctx->m, s, gd0, ctx->density_0);
+    LAUNCH1(ctx, k_bpg<R>, ctx->m.E, ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -809,7 +829,7 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
     // compute_el2nodes_3D(U_n, V_n, Unode, Vnode), src/fv_main.F90:2006
-    LAUNCH(ctx, k_el2nodes<R>, nblk(ctx->m.N), ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode);
+    LAUNCH1(ctx, k_el2nodes<R>, ctx->m.N, ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode);
     if (p.closure == FCG_CLOSURE_MLP) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
@@ -817,12 +837,12 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCH_EX(ctx, k_mlp_closure<R>, (ctx->m.N + 63) / 64, 64, 0, ctx->m, s, (const float *)net->w,
                   (const fcgm::bf16x4 *)net->frag, net->norm, p.Dmin, ctx->density_0, p.z0b_min, p.snul,
                   ctx->mlp_flag);
-        LAUNCH(ctx, k_d3end<R>, nblk(ctx->m.N), ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
+        LAUNCH1(ctx, k_d3end<R>, ctx->m.N, ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
                (const int *)ctx->mlp_flag);
         hipLaunchKernelGGL(fcgk::k_count_flags, dim3(64), dim3(kBlk), 0, ctx->stream, (const int *)ctx->mlp_flag,
                            ctx->m.N, ctx->mlp_count);
     } else {
-        LAUNCH(ctx, k_d3end<R>, nblk(ctx->m.N), ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
+        LAUNCH1(ctx, k_d3end<R>, ctx->m.N, ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
                (const int *)nullptr);
     }
     FCG_CHECK(hipGetLastError());
@@ -842,11 +862,11 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         ff = 1.0;
         ctx->lfirst = 0;
     }
-    LAUNCH(ctx, k_velgrad3d<R>, nblk(ctx->m.E), ctx->m, s);
-    LAUNCH(ctx, k_velrhs<R>, nblk_lv(ctx->m.E, ctx->m.L), ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0,
+    LAUNCH1(ctx, k_velgrad3d<R>, ctx->m.E, ctx->m, s);
+    LAUNCHLV(ctx, k_velrhs<R>, ctx->m.E, ctx->m.L, ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0,
            p.vert_adv == 1 ? 1 : 0);
     if (p.im_vert_visc_fic)
-        LAUNCH(ctx, k_velvisc<R>, nblk(ctx->m.E), ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
+        LAUNCH1(ctx, k_velvisc<R>, ctx->m.E, ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -886,8 +906,8 @@ extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
 
 template <typename R>
 static int jacobian_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCH(ctx, k_jac_node<R>, nblk(ctx->m.N), ctx->m, s, ctx->p.Dmin);
-    LAUNCH(ctx, k_je<R>, nblk(ctx->m.E), ctx->m, s);
+    LAUNCH1(ctx, k_jac_node<R>, ctx->m.N, ctx->m, s, ctx->p.Dmin);
+    LAUNCH1(ctx, k_je<R>, ctx->m.E, ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -899,7 +919,7 @@ extern "C" int fcg_jacobian(fcg_ctx *ctx) {
 
 template <typename R>
 static int update_3d_vel_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCH(ctx, k_update3d<R>, nblk(ctx->m.E), ctx->m, s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
+    LAUNCH1(ctx, k_update3d<R>, ctx->m.E, ctx->m, s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -911,8 +931,8 @@ extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
 
 template <typename R>
 static int vert_vel_sigma_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCH(ctx, k_vertvel_div<R>, nblk_lv(ctx->m.N, ctx->m.L), ctx->m, s);
-    LAUNCH(ctx, k_vertvel_scan<R>, nblk(ctx->m.N), ctx->m, s);
+    LAUNCHLV(ctx, k_vertvel_div<R>, ctx->m.N, ctx->m.L, ctx->m, s);
+    LAUNCH1(ctx, k_vertvel_scan<R>, ctx->m.N, ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -936,20 +956,19 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const int relax = p.cl_relax ? 1 : 0;
     // scheme dispatch of src/fv_main.F90:1788-1800; node x layer kernels, then
     // the column-sequential implicit vertical diffusion
-    const unsigned glv = nblk_lv(m.N, m.L);
     if (p.tracer_adv == 1) {
-        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), m, s, NL);
-        LAUNCH(ctx, k_tr_upwind<R>, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
+        LAUNCHLV(ctx, k_tr_upwind<R>, m.N, m.L, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 2 && p.muscl_call) {
-        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), m, s, NL);
-        LAUNCH(ctx, k_tr_gradn_muscl<R>, glv, m, s);
-        LAUNCH(ctx, k_tr_muscl<R>, glv, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
+        LAUNCHLV(ctx, k_tr_gradn_muscl<R>, m.N, m.L, m, s);
+        LAUNCHLV(ctx, k_tr_muscl<R>, m.N, m.L, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 4) {
-        LAUNCH(ctx, k_tr_ab<R>, nblk(NL), m, s, NL);
-        LAUNCH(ctx, k_tr_gradn_miura<R>, glv, m, s);
-        LAUNCH(ctx, k_tr_miura<R>, glv, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
+        LAUNCHLV(ctx, k_tr_gradn_miura<R>, m.N, m.L, m, s);
+        LAUNCHLV(ctx, k_tr_miura<R>, m.N, m.L, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
     }
-    LAUNCH(ctx, k_tr_vdiff<R>, nblk(m.N), m, s, p.dt);
+    LAUNCH1(ctx, k_tr_vdiff<R>, m.N, m, s, p.dt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -1057,4 +1076,24 @@ extern "C" int fcg_mlp_collect(fcg_ctx *ctx, float *X, float *T, long cap, int s
     if (!ctx || !X || !T || !count || cap < 0 || stride < 1) return FCG_E_ARG;
     return ctx->mixed ? mlp_collect_impl(ctx, retype<float>(ctx->s), X, T, cap, stride, count)
                       : mlp_collect_impl(ctx, ctx->s, X, T, cap, stride, count);
+}
+
+// Launch shapes for the autotuner: block size 64, 128 or 256 threads (for
+// the layer-parallel kernels: 64 entities x 1, 2 or 4 layers per block).
+extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
+    if (!ctx || !kernel || !(block == 64 || block == 128 || block == 256)) return FCG_E_ARG;
+    static const char *tunable[] = {"k_pressure", "k_bpg", "k_d3end", "k_el2nodes", "k_velgrad3d", "k_velrhs",
+                                    "k_velvisc", "k_jac_node", "k_je", "k_update3d", "k_vertvel_div",
+                                    "k_vertvel_scan", "k_tr_ab", "k_tr_upwind", "k_tr_gradn_muscl", "k_tr_muscl",
+                                    "k_tr_gradn_miura", "k_tr_miura", "k_tr_vdiff"};
+    for (const char *t : tunable)
+        if (!std::strcmp(t, kernel)) {
+            ctx->launch_block[kernel] = block;
+            return FCG_OK;
+        }
+    return fcg_fail(ctx, FCG_E_ARG, std::string("kernel without a tunable launch shape: ") + kernel);
+}
+
+extern "C" int fcg_get_launch(const fcg_ctx *ctx, const char *kernel) {
+    return (ctx && kernel) ? block_of(ctx, kernel) : 0;
 }

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "fesomc_gpu.h"
@@ -143,6 +144,7 @@ struct fcg_ctx {
     int *mlp_flag = nullptr;     // N: column sent back to d3_end this step
     int *mlp_count = nullptr;    // cumulative count of such columns
     std::string probe_name;
+    std::unordered_map<std::string, int> launch_block;  // autotuned block sizes
     std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev;
     std::vector<void *> allocs;
     std::string err;

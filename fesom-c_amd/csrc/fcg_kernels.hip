@@ -60,16 +60,17 @@ __device__ __forceinline__ unsigned xcd_block() {
 #define PL(arr, k, n, stride) (arr)[(size_t)(k) * (stride) + (n)]
 
 // (node, level) of this thread.  Layer-parallel node kernels run one thread
-// per node and layer: a block of 256 covers 64 consecutive (locality-
-// ordered) nodes x 4 consecutive layers, and the layer groups of one node
-// tile are adjacent logical blocks, so they land on the same XCD and share
-// its L2 for the level-invariant edge geometry.  Launch with nblk_lv(N, L)
-// blocks.
+// per node and layer: a block of 64*lpb threads covers 64 consecutive
+// (locality-ordered) nodes x lpb consecutive layers, and the layer groups
+// of one node tile are adjacent logical blocks, so they land on the same XCD
+// and share its L2 for the level-invariant edge geometry.  Launch with
+// ceil(N/64)*ceil(L/lpb) blocks.
 __device__ __forceinline__ bool node_level(int N, int L, int &n, int &k) {
     const unsigned b = xcd_block();
-    const unsigned nlg = (unsigned)(L + 3) / 4u;
+    const unsigned lpb = blockDim.x >> 6;  // layers per block (block = 64 x lpb)
+    const unsigned nlg = ((unsigned)L + lpb - 1u) / lpb;
     n = (int)((b / nlg) * 64u + (threadIdx.x & 63u));
-    k = (int)((b % nlg) * 4u + (threadIdx.x >> 6));
+    k = (int)((b % nlg) * lpb + (threadIdx.x >> 6));
     return n < N && k < L;
 }
 
@@ -346,9 +347,10 @@ __device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevStateT<R> &
 // layers, layer groups of one element tile adjacent (see node_level).
 __device__ __forceinline__ bool elem_level(int E, int L, int &e, int &k) {
     const unsigned b = xcd_block();
-    const unsigned nlg = (unsigned)(L + 3) / 4u;
+    const unsigned lpb = blockDim.x >> 6;
+    const unsigned nlg = ((unsigned)L + lpb - 1u) / lpb;
     e = (int)((b / nlg) * 64u + (threadIdx.x & 63u));
-    k = (int)((b % nlg) * 4u + (threadIdx.x >> 6));
+    k = (int)((b % nlg) * lpb + (threadIdx.x >> 6));
     return e < E && k < L;
 }
 

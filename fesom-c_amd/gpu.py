@@ -32,7 +32,8 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_sync", "fcg_last_error", "fcg_version", "fcg_probe", "fcg_probe_read", "fcg_mlp_create",
            "fcg_mlp_destroy", "fcg_mlp_get_params", "fcg_mlp_set_params", "fcg_mlp_set_norm", "fcg_mlp_forward",
            "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect",
-           "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member"]
+           "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member", "fcg_set_launch",
+           "fcg_get_launch"]
 
 MLP_FEATURES, MLP_PARAMS = 5, 6288
 
@@ -142,6 +143,10 @@ def lib():
         L.fcg_mlp_train_step.argtypes = [P, P, P, C.c_long, C.c_float]
         L.fcg_set_mlp.restype = C.c_int
         L.fcg_set_mlp.argtypes = [P, P]
+        L.fcg_set_launch.restype = C.c_int
+        L.fcg_set_launch.argtypes = [P, C.c_char_p, C.c_int]
+        L.fcg_get_launch.restype = C.c_int
+        L.fcg_get_launch.argtypes = [P, C.c_char_p]
         L.fcg_create_ensemble.restype = C.c_int
         L.fcg_create_ensemble.argtypes = [C.POINTER(Mesh), C.c_int, C.POINTER(Params), C.POINTER(P)]
         L.fcg_ensemble_size.restype = C.c_int
@@ -537,3 +542,15 @@ class Ensemble(Model):
         d = Diag()
         self._ck(lib().fcg_diagnostics_member(self.ctx, b, C.byref(d)), "fcg_diagnostics_member")
         return d.as_dict()
+
+
+def model_set_launch(model, kernel, block):
+    model._ck(lib().fcg_set_launch(model.ctx, kernel.encode(), block), f"fcg_set_launch {kernel}")
+
+
+def model_get_launch(model, kernel):
+    return lib().fcg_get_launch(model.ctx, kernel.encode())
+
+
+Model.set_launch = model_set_launch
+Model.get_launch = model_get_launch

@@ -258,6 +258,13 @@ int fcg_set_mlp(fcg_ctx *ctx, fcg_mlp *net);
  * (column-major sample order), at most cap samples; *count = samples written. */
 int fcg_mlp_collect(fcg_ctx *ctx, float *X_dev, float *T_dev, long cap, int stride, long *count);
 
+/* Launch shapes (no counterpart in the reference; used by the autotuner,
+ * fesomc_amd/autotune.py): block size 64, 128 or 256 threads for the named
+ * device kernel ("k_tr_muscl", "k_velrhs", ...; layer-parallel kernels then
+ * cover 64 entities x block/64 layers per block).  Default 256. */
+int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block);
+int fcg_get_launch(const fcg_ctx *ctx, const char *kernel);
+
 /* Library build identification ("fesomc_gpu <ver> gfx950 ..."). */
 const char *fcg_version(void);
 

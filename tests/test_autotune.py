@@ -1,0 +1,61 @@
+"""Launch shapes: every tunable kernel gives the same results, bit for bit,
+at every block size (each thread computes one (entity, layer) or one column
+whatever the block), and the tuner keeps a candidate for each kernel."""
+import numpy as np
+import pytest
+
+
+def test_set_launch_checks_its_arguments_without_a_gpu():
+    from fesomc_amd import gpu
+    L = gpu.lib()
+    assert L.fcg_set_launch(None, b"k_velrhs", 64) != 0
+    assert L.fcg_get_launch(None, b"k_velrhs") == 0
+
+
+def _model(precision):
+    from fesomc_amd import gpu, meshgen
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, ex = meshgen.config("c2_small")
+    m = gpu.Model(RunConfig(txt), mesh, precision=precision)
+    m.set_tracers(ex["T"], ex["S"])
+    wx, wy = ex["wind"]
+    tx, ty = meshgen.wind_stress(wx, wy)
+    m.set("windx", np.full(mesh.nod2D, wx))
+    m.set("windy", np.full(mesh.nod2D, wy))
+    m.set("taux", np.full(mesh.elem2D, tx))
+    m.set("tauy", np.full(mesh.elem2D, ty))
+    return m
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("block", [64, 128])
+def test_block_size_does_not_change_results(precision, block):
+    from fesomc_amd import autotune
+    a, b = _model(precision), _model(precision)
+    for k in autotune.LAYER_KERNELS + autotune.COLUMN_KERNELS:
+        b.set_launch(k, block)
+        assert b.get_launch(k) == block
+    a.step(4)
+    b.step(4)
+    for f in ("eta_n", "U_n", "V_n", "TF", "SF", "Wvel", "Kv", "snu"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_tuner_picks_a_candidate_for_each_kernel():
+    from fesomc_amd import autotune
+    m = _model(1)
+    with pytest.raises(RuntimeError):
+        m.set_launch("k_velrhs", 96)
+    with pytest.raises(RuntimeError):
+        m.set_launch("k_bt_elem2", 64)
+    kern = ["k_tr_muscl", "k_velrhs", "k_d3end"]
+    table = autotune.tune(m, kernels=kern, steps=1)
+    assert set(table) == set(kern)
+    for k, (bs, ms) in table.items():
+        assert bs in autotune.CANDIDATES and ms > 0 and m.get_launch(k) == bs
+    assert m.diagnostics()["status"] == 0
+    m.close()
