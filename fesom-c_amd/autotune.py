@@ -13,9 +13,8 @@ from __future__ import annotations
 import json
 
 LAYER_KERNELS = ["k_tr_muscl", "k_tr_gradn_muscl", "k_velrhs", "k_vertvel_div", "k_tr_upwind", "k_tr_gradn_miura",
-                 "k_tr_miura"]
-COLUMN_KERNELS = ["k_d3end", "k_el2nodes", "k_jac_node", "k_velgrad3d", "k_velvisc", "k_update3d", "k_tr_vdiff", "k_pressure",
-                  "k_bpg", "k_tr_ab", "k_je", "k_vertvel_scan"]
+                 "k_tr_miura", "k_el2nodes", "k_velgrad3d", "k_je", "k_tr_ab"]
+COLUMN_KERNELS = ["k_d3end", "k_jac_node"]
 CANDIDATES = (64, 128, 256)
 
 

@@ -14,7 +14,9 @@
 namespace fcgk {
 template <typename R> __global__ void k_pressure(DevMesh, DevStateT<R>, double, double);
 template <typename R> __global__ void k_bpg(DevMesh, DevStateT<R>);
-template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, double, double, double, const int *);
+template <typename R> __global__ void k_d3end_prep(DevMesh, DevStateT<R>, double, double, double, const int *);
+template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, const int *);
+template <typename R> __global__ void k_d3end_fin(DevMesh, DevStateT<R>, double, double, double, double, const int *);
 template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
@@ -70,7 +72,21 @@ namespace {
 
 constexpr int kBlk = 256;
 inline unsigned nblk(size_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
-constexpr int kMinmaxChunks = 64;  // blocks per ensemble member in the T/S extrema reduction
+constexpr int kMinmaxChunks = 512;  // blocks per ensemble member in the T/S extrema reduction
+
+// Column-tile kernels: one 64-thread block per 64 columns (16 for the P1
+// advection), LDS tiles of K+1 slots per column.
+constexpr int kColB = 64, kColT = 256, kP1Cols = 16;
+inline unsigned ncolblk(int ents) { return (unsigned)((ents + kColB - 1) / kColB); }
+inline size_t tile_bytes(const DevMesh &m, size_t esz) { return (size_t)kColB * (m.K + 1) * esz; }
+inline size_t lds_p1adv(const DevMesh &m) { return (8 * (size_t)kP1Cols * (m.K + 1) + 8 * kP1Cols) * sizeof(double); }
+// largest LDS request of the tile kernels for this mesh and storage type
+inline size_t lds_max(const DevMesh &m, size_t esz) {
+    size_t b = std::max(2 * tile_bytes(m, 8), tile_bytes(m, 8) + 2 * tile_bytes(m, esz));  // bpg, velvisc
+    b = std::max(b, 4 * kColB * sizeof(double) + 2 * tile_bytes(m, esz));                  // update3d
+    b = std::max(b, 5 * tile_bytes(m, esz));                                               // vdiff, d3_end prep
+    return std::max(b, lds_p1adv(m));
+}
 
 template <typename T>
 int dalloc(fcg_ctx *ctx, T **p, size_t n) {
@@ -195,10 +211,11 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
         hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide);
         hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         if (sub == 1) {
-            hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
-            hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, (const R *)s.U_puls,
-                               (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
-            hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+            hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk((size_t)m.E * m.L)), dim3(kBlk), 0, st, m, s, p.Dmin);
+            hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nblk((size_t)m.N * m.L)), dim3(kBlk), 0, st, m,
+                               (const R *)s.U_puls, (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
+            hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3((m.E + kP1Cols - 1) / kP1Cols), dim3(kColB), lds_p1adv(m), st,
+                               m, s);
         }
         hipLaunchKernelGGL(fcgk::k_bt_elem2<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
                            ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0);
@@ -230,13 +247,8 @@ static bool probe_is(const std::string &probe, const char *kern) {
         const int bs__ = block_of(ctx, #kern);                                             \
         LAUNCH_EX(ctx, kern, (unsigned)(((size_t)(items) + bs__ - 1) / bs__), bs__, 0, __VA_ARGS__); \
     } while (0)
-// one thread per (entity, layer): 64 entities x (block/64) layers per block
-#define LAUNCHLV(ctx, kern, ents, levels, ...)                                             \
-    do {                                                                                   \
-        const int bs__ = block_of(ctx, #kern), lpb__ = bs__ / 64;                          \
-        LAUNCH_EX(ctx, kern, (unsigned)(((ents) + 63) / 64) * (unsigned)(((levels) + lpb__ - 1) / lpb__), bs__, 0, \
-                  __VA_ARGS__);                                                            \
-    } while (0)
+// one thread per (entity, layer), consecutive threads along a column
+#define LAUNCHLV(ctx, kern, ents, levels, ...) LAUNCH1(ctx, kern, (size_t)(ents) * (size_t)(levels), __VA_ARGS__)
 #define LAUNCH_EX(ctx, kern, grid, block, lds, ...)                                         \
     do {                                                                                   \
         bool pr__ = probe_is((ctx)->probe_name, #kern);                                    \
@@ -515,17 +527,27 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     UP(m.sigma, sigma) UP(m.dsig, dsig) UP(m.oms, oms) UP(m.memb, memb) UP(m.memb_noff, ctx->noff)
     // ---- state
     DevState &s = ctx->s;
-    const size_t NL = (size_t)N * L, EL = (size_t)E * L, NK = (size_t)N * K, EK = (size_t)E * K;
+    // 3-D fields by column, K slots per column (layer fields use the first L)
+    const size_t NL = (size_t)N * K, EL = (size_t)E * K, NK = (size_t)N * K, EK = (size_t)E * K;
 #define DA(ptr, n) if ((rc = dalloc(ctx, &(ptr), (n)))) { *out = ctx; return rc; }
     // 3-D fields: element size of the precision mode (the pointer is retyped on use)
 #define DA3(ptr, n) if ((rc = dalloc_bytes(ctx, (void **)&(ptr), (n) * ctx->esize))) { *out = ctx; return rc; }
     ctx->mixed = p.precision == FCG_MIXED;
     ctx->esize = ctx->mixed ? sizeof(float) : sizeof(double);
+    {
+        int lds_limit = 0;
+        FCG_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+        if (lds_max(m, ctx->esize) > (size_t)lds_limit) {
+            fcg_fail(ctx, FCG_E_ARG, "nsigma too large for the LDS column tiles of this device");
+            *out = ctx;
+            return FCG_E_ARG;
+        }
+    }
     DA(s.eta_n, N) DA(s.eta_n_1, N) DA(s.eta_n_2, N) DA(s.etaAB, N) DA(s.eta_g, N) DA(s.ssh_rhs, N)
     DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
     DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
-    DA(s.a, N) DA(s.a_old, N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
     DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
     DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
     DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA3(s.U_puls, EL) DA3(s.V_puls, EL)
@@ -542,7 +564,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         DA3(s.gSx, NL) DA3(s.gSy, NL)
     }
     DA(s.hp, NK) DA3(s.Kv, NK) DA3(s.snu, NK) DA3(s.bt, NK) DA3(s.W, NK)
-    DA3(s.mx_p, NK) DA3(s.mx_q, NK) DA3(s.mx_c2, NK) DA3(s.mx_xx, NK) DA3(s.mx_yy, NK) DA3(s.mx_bA, NK) DA3(s.mx_bB, NK)
+    DA3(s.mx_p, NK) DA3(s.mx_q, NK) DA3(s.mx_c2, NK) DA3(s.mx_xx, NK) DA3(s.mx_yy, NK) DA3(s.mx_bA, NK) DA3(s.mx_bB, NK) DA3(s.mx_s, NK)
     ctx->red_blocks = 512;
     DA(s.red, std::max<size_t>(16 * (size_t)ctx->red_blocks, 4 * (size_t)kMinmaxChunks * m.nmemb))
     DA(s.scal, 32) DA(s.err, 4) DA(s.tsa, 2 * (size_t)m.nmemb)
@@ -589,9 +611,9 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         FCG_CHECK(hipMemcpy(s.mask_wd, mk.data(), E * 8, hipMemcpyHostToDevice));
     }
     if (ctx->mixed)
-        hipLaunchKernelGGL(fcgk::k_je<float>, dim3(nblk(E)), dim3(kBlk), 0, ctx->stream, m, retype<float>(s));
+        hipLaunchKernelGGL(fcgk::k_je<float>, dim3(nblk((size_t)E * L)), dim3(kBlk), 0, ctx->stream, m, retype<float>(s));
     else
-        hipLaunchKernelGGL(fcgk::k_je<double>, dim3(nblk(E)), dim3(kBlk), 0, ctx->stream, m, s);
+        hipLaunchKernelGGL(fcgk::k_je<double>, dim3(nblk((size_t)E * L)), dim3(kBlk), 0, ctx->stream, m, s);
     FCG_CHECK(hipGetLastError());
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->time_jd = p.time_jd0;
@@ -756,29 +778,40 @@ static int transfer(fcg_ctx *ctx, const char *name, double *host, long count, bo
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
     // device entity i holds reference entity perm[i]
     const std::vector<int> &perm = f.nodal ? ctx->pn : ctx->pe;
-    std::vector<double> tmp((size_t)count);
-    // Fortran (levels, ents) column-major  <->  device: planes [level][ent], or
-    // interleaved [ent][comp] for the (2,E)/(4,E) 2-D vectors
-    auto dev_at = [&](long lev, long i) -> size_t {
-        return f.interleaved ? (size_t)i * f.levels + lev : (size_t)lev * f.ents + i;
-    };
+    // Fortran (levels, ents) column-major  <->  device: columns of K slots for
+    // 3-D fields (levels = L or K), interleaved [ent][comp] for the (2,E)/(4,E)
+    // 2-D vectors, plain arrays for 2-D scalars
+    const bool col3 = f.r3 || f.levels == ctx->m.K;
+    const long slots = col3 ? (long)ctx->m.K : f.levels;
+    const size_t dcount = (size_t)slots * f.ents;
+    auto dev_at = [&](long lev, long i) -> size_t { return (size_t)i * slots + lev; };
+    std::vector<double> tmp(dcount, 0.0);
     const bool f32 = f.r3 && ctx->mixed;  // device holds float: round on upload, widen on download
-    std::vector<float> tmpf(f32 ? (size_t)count : 0);
+    std::vector<float> tmpf(f32 ? dcount : 0);
     if (to_dev) {
+        if (col3 && f.levels < slots) {  // keep the unused slot of layer columns as it is on the device
+            if (f32) {
+                FCG_CHECK(hipMemcpy(tmpf.data(), f.dev, sizeof(float) * dcount, hipMemcpyDeviceToHost));
+            } else {
+                FCG_CHECK(hipMemcpy(tmp.data(), f.dev, sizeof(double) * dcount, hipMemcpyDeviceToHost));
+            }
+            if (f32)
+                for (size_t j = 0; j < dcount; ++j) tmp[j] = (double)tmpf[j];
+        }
         for (long i = 0; i < f.ents; ++i)
             for (long k = 0; k < f.levels; ++k) tmp[dev_at(k, i)] = host[(size_t)perm[i] * f.levels + k];
         if (f32) {
-            for (long j = 0; j < count; ++j) tmpf[j] = (float)tmp[j];
-            FCG_CHECK(hipMemcpy(f.dev, tmpf.data(), sizeof(float) * count, hipMemcpyHostToDevice));
+            for (size_t j = 0; j < dcount; ++j) tmpf[j] = (float)tmp[j];
+            FCG_CHECK(hipMemcpy(f.dev, tmpf.data(), sizeof(float) * dcount, hipMemcpyHostToDevice));
         } else {
-            FCG_CHECK(hipMemcpy(f.dev, tmp.data(), sizeof(double) * count, hipMemcpyHostToDevice));
+            FCG_CHECK(hipMemcpy(f.dev, tmp.data(), sizeof(double) * dcount, hipMemcpyHostToDevice));
         }
     } else {
         if (f32) {
-            FCG_CHECK(hipMemcpy(tmpf.data(), f.dev, sizeof(float) * count, hipMemcpyDeviceToHost));
-            for (long j = 0; j < count; ++j) tmp[j] = (double)tmpf[j];
+            FCG_CHECK(hipMemcpy(tmpf.data(), f.dev, sizeof(float) * dcount, hipMemcpyDeviceToHost));
+            for (size_t j = 0; j < dcount; ++j) tmp[j] = (double)tmpf[j];
         } else {
-            FCG_CHECK(hipMemcpy(tmp.data(), f.dev, sizeof(double) * count, hipMemcpyDeviceToHost));
+            FCG_CHECK(hipMemcpy(tmp.data(), f.dev, sizeof(double) * dcount, hipMemcpyDeviceToHost));
         }
         for (long i = 0; i < f.ents; ++i)
             for (long k = 0; k < f.levels; ++k) host[(size_t)perm[i] * f.levels + k] = tmp[dev_at(k, i)];
@@ -813,8 +846,9 @@ extern "C" int fcg_advance_clock(fcg_ctx *ctx) {
 template <typename R>
 static int pressure_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const double gd0 = kG / ctx->density_0;
-    LAUNCH1(ctx, k_pressure<R>, ctx->m.N, ctx->m, s, gd0, ctx->density_0);
-    LAUNCH1(ctx, k_bpg<R>, ctx->m.E, ctx->m, s);
+    const DevMesh &m = ctx->m;
+    LAUNCH_EX(ctx, k_pressure<R>, ncolblk(m.N), kColT, tile_bytes(m, 8), m, s, gd0, ctx->density_0);
+    LAUNCH_EX(ctx, k_bpg<R>, ncolblk(m.E), kColT, 2 * tile_bytes(m, 8), m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -824,12 +858,25 @@ extern "C" int fcg_pressure(fcg_ctx *ctx) {
     return ctx->mixed ? pressure_impl(ctx, retype<float>(ctx->s)) : pressure_impl(ctx, ctx->s);
 }
 
+// d3_end (src/fv_mixing.F90:73-347): right-hand side, iterated column
+// solve, closure values; `only` restricts it to flagged columns.
+template <typename R>
+static void d3end(fcg_ctx *ctx, const DevStateT<R> &s, const int *only) {
+    const fcg_params &p = ctx->p;
+    const DevMesh &m = ctx->m;
+    LAUNCH_EX(ctx, k_d3end_prep<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt, p.Dmin,
+              ctx->density_0, only);
+    LAUNCH1(ctx, k_d3end<R>, m.N, m, s, p.dt, p.Dmin, p.z0b_min, only);
+    LAUNCH_EX(ctx, k_d3end_fin<R>, ncolblk(m.N), kColT, tile_bytes(m, sizeof(R)), m, s, p.Dmin, p.z0b_min, p.z0s_min,
+              p.snul, only);
+}
+
 template <typename R>
 static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
     // compute_el2nodes_3D(U_n, V_n, Unode, Vnode), src/fv_main.F90:2006
-    LAUNCH1(ctx, k_el2nodes<R>, ctx->m.N, ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode);
+    LAUNCHLV(ctx, k_el2nodes<R>, ctx->m.N, ctx->m.L, ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode);
     if (p.closure == FCG_CLOSURE_MLP) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
@@ -837,13 +884,11 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCH_EX(ctx, k_mlp_closure<R>, (ctx->m.N + 63) / 64, 64, 0, ctx->m, s, (const float *)net->w,
                   (const fcgm::bf16x4 *)net->frag, net->norm, p.Dmin, ctx->density_0, p.z0b_min, p.snul,
                   ctx->mlp_flag);
-        LAUNCH1(ctx, k_d3end<R>, ctx->m.N, ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
-               (const int *)ctx->mlp_flag);
+        d3end(ctx, s, (const int *)ctx->mlp_flag);
         hipLaunchKernelGGL(fcgk::k_count_flags, dim3(64), dim3(kBlk), 0, ctx->stream, (const int *)ctx->mlp_flag,
                            ctx->m.N, ctx->mlp_count);
     } else {
-        LAUNCH1(ctx, k_d3end<R>, ctx->m.N, ctx->m, s, p.dt, p.Dmin, p.z0b_min, p.z0s_min, p.snul, ctx->density_0,
-               (const int *)nullptr);
+        d3end(ctx, s, nullptr);
     }
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
@@ -862,11 +907,12 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         ff = 1.0;
         ctx->lfirst = 0;
     }
-    LAUNCH1(ctx, k_velgrad3d<R>, ctx->m.E, ctx->m, s);
+    LAUNCHLV(ctx, k_velgrad3d<R>, ctx->m.E, ctx->m.L, ctx->m, s);
     LAUNCHLV(ctx, k_velrhs<R>, ctx->m.E, ctx->m.L, ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0,
            p.vert_adv == 1 ? 1 : 0);
     if (p.im_vert_visc_fic)
-        LAUNCH1(ctx, k_velvisc<R>, ctx->m.E, ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
+        LAUNCH_EX(ctx, k_velvisc<R>, ncolblk(ctx->m.E), kColT, tile_bytes(ctx->m, 8) + 2 * tile_bytes(ctx->m, sizeof(R)),
+                  ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -907,7 +953,7 @@ extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
 template <typename R>
 static int jacobian_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     LAUNCH1(ctx, k_jac_node<R>, ctx->m.N, ctx->m, s, ctx->p.Dmin);
-    LAUNCH1(ctx, k_je<R>, ctx->m.E, ctx->m, s);
+    LAUNCHLV(ctx, k_je<R>, ctx->m.E, ctx->m.L, ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -919,7 +965,8 @@ extern "C" int fcg_jacobian(fcg_ctx *ctx) {
 
 template <typename R>
 static int update_3d_vel_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCH1(ctx, k_update3d<R>, ctx->m.E, ctx->m, s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
+    LAUNCH_EX(ctx, k_update3d<R>, ncolblk(ctx->m.E), kColT, 4 * kColB * sizeof(double) + 2 * tile_bytes(ctx->m, sizeof(R)),
+              ctx->m, s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -932,7 +979,7 @@ extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
 template <typename R>
 static int vert_vel_sigma_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     LAUNCHLV(ctx, k_vertvel_div<R>, ctx->m.N, ctx->m.L, ctx->m, s);
-    LAUNCH1(ctx, k_vertvel_scan<R>, ctx->m.N, ctx->m, s);
+    LAUNCH_EX(ctx, k_vertvel_scan<R>, ncolblk(ctx->m.N), kColT, tile_bytes(ctx->m, sizeof(R)), ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -946,7 +993,7 @@ template <typename R>
 static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     const DevMesh &m = ctx->m;
-    const size_t NL = (size_t)m.N * m.L;
+    const size_t NL = (size_t)m.N * m.K;  // every slot of the tracer columns
     // T_aver/S_aver per ensemble member (one member outside an ensemble)
     const int nb = kMinmaxChunks;
     hipLaunchKernelGGL(fcgk::k_minmax_partial<R>, dim3(nb, m.nmemb), dim3(kBlk), 0, ctx->stream, m, (const R *)s.TF,
@@ -968,7 +1015,7 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCHLV(ctx, k_tr_gradn_miura<R>, m.N, m.L, m, s);
         LAUNCHLV(ctx, k_tr_miura<R>, m.N, m.L, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
     }
-    LAUNCH1(ctx, k_tr_vdiff<R>, m.N, m, s, p.dt);
+    LAUNCH_EX(ctx, k_tr_vdiff<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -1082,10 +1129,9 @@ extern "C" int fcg_mlp_collect(fcg_ctx *ctx, float *X, float *T, long cap, int s
 // the layer-parallel kernels: 64 entities x 1, 2 or 4 layers per block).
 extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
     if (!ctx || !kernel || !(block == 64 || block == 128 || block == 256)) return FCG_E_ARG;
-    static const char *tunable[] = {"k_pressure", "k_bpg", "k_d3end", "k_el2nodes", "k_velgrad3d", "k_velrhs",
-                                    "k_velvisc", "k_jac_node", "k_je", "k_update3d", "k_vertvel_div",
-                                    "k_vertvel_scan", "k_tr_ab", "k_tr_upwind", "k_tr_gradn_muscl", "k_tr_muscl",
-                                    "k_tr_gradn_miura", "k_tr_miura", "k_tr_vdiff"};
+    static const char *tunable[] = {"k_d3end", "k_el2nodes", "k_velgrad3d",
+                                    "k_velrhs", "k_jac_node", "k_je", "k_vertvel_div", "k_tr_ab", "k_tr_upwind",
+                                    "k_tr_gradn_muscl", "k_tr_muscl", "k_tr_gradn_miura", "k_tr_miura"};
     for (const char *t : tunable)
         if (!std::strcmp(t, kernel)) {
             ctx->launch_block[kernel] = block;

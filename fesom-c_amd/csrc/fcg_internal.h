@@ -1,7 +1,10 @@
 // Internal declarations shared by the HIP translation units of libfesomc_gpu.
 // Device layout: per-entity constants as small AoS vectors (int4/double4 per
 // element, int2/double4 per edge) so a wavefront reads them coalesced; 3-D
-// fields level-major ("planes"): X[k*n_entities + i].
+// fields by column, X[i*K + k] (K = nsigma slots per entity), so that one
+// gather of a neighbour reads a contiguous run of its levels.  The scratch of
+// column-sequential solves (d3_end, the tracer Thomas sweep) is stored by
+// level, X[k*n_entities + i], read across a wavefront of columns.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -79,8 +82,9 @@ struct DevStateT {
     double *mask_wd = nullptr, *taux = nullptr, *tauy = nullptr, *C_d_el = nullptr; // E
     double *windx = nullptr, *windy = nullptr;                 // N
     double *a = nullptr, *a_old = nullptr;                     // N: max(depth+eta_n, Dmin) now / previous jacobian
+    double *ubot = nullptr;                                    // 2E: U_n, V_n of the bottom layer (substep cache)
     double *tide = nullptr;                                    // Mt*nobn prescribed OB elevation per substep
-    // 3-D, level-major planes
+    // 3-D, stored by column: K slots per node/element (layer fields use L)
     R *U_n = nullptr, *V_n = nullptr, *U_rhs = nullptr, *V_rhs = nullptr;   // L*E
     R *U_rhsAB = nullptr, *V_rhsAB = nullptr, *U_nf = nullptr, *V_nf = nullptr; // L*E
     R *gux = nullptr, *guy = nullptr, *gvx = nullptr, *gvy = nullptr;         // L*E
@@ -97,7 +101,7 @@ struct DevStateT {
     double *hp = nullptr;                                                          // K*N hydrostatic pressure (FP64)
     R *Kv = nullptr, *snu = nullptr, *bt = nullptr, *W = nullptr; // K*N
     R *mx_p = nullptr, *mx_q = nullptr, *mx_c2 = nullptr, *mx_xx = nullptr, *mx_yy = nullptr; // K*N d3_end scratch
-    R *mx_bA = nullptr, *mx_bB = nullptr;                                     // K*N
+    R *mx_bA = nullptr, *mx_bB = nullptr, *mx_s = nullptr;                    // K*N
     // reductions
     double *red = nullptr;       // partial results
     double *scal = nullptr;      // diagnostics (from [4])

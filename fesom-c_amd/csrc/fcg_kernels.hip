@@ -57,139 +57,257 @@ __device__ __forceinline__ unsigned xcd_block() {
     return x * q + (x < r ? x : r) + b / 8u;
 }
 #define TID() (xcd_block() * blockDim.x + threadIdx.x)
-#define PL(arr, k, n, stride) (arr)[(size_t)(k) * (stride) + (n)]
+// 3-D fields are stored by column: level k of entity n at n*KC + k, KC = the
+// number of levels (nsigma); layer fields leave their last slot unused.
+#define PL(arr, k, n, kc) (arr)[(size_t)(n) * (kc) + (k)]
 
-// (node, level) of this thread.  Layer-parallel node kernels run one thread
-// per node and layer: a block of 64*lpb threads covers 64 consecutive
-// (locality-ordered) nodes x lpb consecutive layers, and the layer groups
-// of one node tile are adjacent logical blocks, so they land on the same XCD
-// and share its L2 for the level-invariant edge geometry.  Launch with
-// ceil(N/64)*ceil(L/lpb) blocks.
-__device__ __forceinline__ bool node_level(int N, int L, int &n, int &k) {
-    const unsigned b = xcd_block();
-    const unsigned lpb = blockDim.x >> 6;  // layers per block (block = 64 x lpb)
-    const unsigned nlg = ((unsigned)L + lpb - 1u) / lpb;
-    n = (int)((b / nlg) * 64u + (threadIdx.x & 63u));
-    k = (int)((b % nlg) * lpb + (threadIdx.x >> 6));
-    return n < N && k < L;
+// (entity, layer) of this thread in the layer-parallel kernels: consecutive
+// threads take consecutive layers of one column, then the next column, so a
+// wavefront covers about 64/L whole columns and every gather of a neighbour
+// column reads one contiguous run of its levels.  Launch with
+// ceil(ents*L/block) blocks.
+__device__ __forceinline__ bool ent_layer(int ents, int L, int &n, int &k) {
+    const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
+    n = (int)(i / (unsigned)L);
+    k = (int)(i - (unsigned)n * (unsigned)L);
+    return n < ents;
+}
+
+// Column tiles.  Kernels with a recurrence along the column take 64
+// consecutive columns per block (kColB), which are one contiguous range of
+// every column-stored field.  Data move between HBM and LDS tiles
+// t[c*KP + k] (KP = K + 1) in flat loops over the block's (column, slot)
+// items, so global accesses stay contiguous; per-level work runs four
+// threads per column (slots sub, sub+4, ...), which load the column's
+// constants once; the recurrence itself runs one lane per column on the
+// tiles, in the block's first wavefront.
+constexpr int kColB = 64;   // columns per tile block
+constexpr int kColT = 256;  // threads per tile block
+
+// block-strided walk over the (column, slot) items of nc columns x S slots
+struct Items {
+    int c, k, S, dc, dk;
+    __device__ __forceinline__ explicit Items(int S_) : S(S_) {
+        c = (int)threadIdx.x / S;
+        k = (int)threadIdx.x % S;
+        dc = (int)blockDim.x / S;
+        dk = (int)blockDim.x % S;
+    }
+    __device__ __forceinline__ void next() {
+        c += dc;
+        k += dk;
+        if (k >= S) {
+            k -= S;
+            ++c;
+        }
+    }
+};
+
+__device__ __forceinline__ int col_first() { return (int)(xcd_block() * kColB); }
+__device__ __forceinline__ int tcol() { return (int)threadIdx.x >> 2; }  // column of this thread (per-level phases)
+__device__ __forceinline__ int tsub() { return (int)threadIdx.x & 3; }   // its first slot
+
+// the K slots of columns [c0, c0+nc) of g -> tile t
+template <typename T, typename G>
+__device__ __forceinline__ void tile_load(T *t, const G *__restrict__ g, int c0, int nc, int K) {
+    const size_t base = (size_t)c0 * K;
+    for (Items it(K); it.c < nc; it.next()) t[it.c * (K + 1) + it.k] = (T)g[base + (size_t)it.c * K + it.k];
+}
+
+// the first S slots of each tile column -> columns [c0, c0+nc) of g
+template <typename T, typename G>
+__device__ __forceinline__ void tile_store(G *__restrict__ g, const T *t, int c0, int nc, int K, int S) {
+    const size_t base = (size_t)c0 * K;
+    for (Items it(S); it.c < nc; it.next()) g[base + (size_t)it.c * K + it.k] = (G)t[it.c * (K + 1) + it.k];
+}
+
+// slots [k0, k1) of tile t -> level planes x[k*N + n] of columns [c0, c0+nc):
+// each wavefront writes one 64-column row of a plane
+template <typename T, typename G>
+__device__ __forceinline__ void tile_to_planes(G *__restrict__ x, const T *t, int c0, int nc, int K, int N, int k0,
+                                               int k1) {
+    const int j = threadIdx.x & 63;
+    if (j >= nc) return;
+    for (int k = k0 + (int)(threadIdx.x >> 6); k < k1; k += (int)(blockDim.x >> 6))
+        x[(size_t)k * N + c0 + j] = (G)t[j * (K + 1) + k];
+}
+
+template <typename T, typename G>
+__device__ __forceinline__ void planes_to_tile(T *t, const G *__restrict__ x, int c0, int nc, int K, int N) {
+    const int j = threadIdx.x & 63;
+    if (j >= nc) return;
+    for (int k = (int)(threadIdx.x >> 6); k < K; k += (int)(blockDim.x >> 6))
+        t[j * (K + 1) + k] = (T)x[(size_t)k * N + c0 + j];
 }
 
 // ---------------------------------------------------------------------------
 // pressure: hydrostatic column scan, src/fv_pressure_new.F90:91-107
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N;
-    double a = s.a[n];
-    double hp = 0.0;
-    PL(s.hp, 0, n, N) = 0.0;
-    double zb0 = m.oms[0] * a;
-    for (int k = 0; k < m.L; ++k) {
-        double zb1 = m.oms[k + 1] * a;
-        double Z = 0.5 * (zb0 + zb1);
-        double r = densityJM(PL(s.TF, k, n, N), PL(s.SF, k, n, N), -Z, rho0);
-        hp = hp + r * (a * m.dsig[k]) * gd0;
-        PL(s.hp, k + 1, n, N) = hp;
-        zb0 = zb1;
+__global__ __launch_bounds__(kColT) void k_pressure(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    double *th = (double *)lds_;  // per column: 0, then the layer increments, then hp
+    const int K = m.K, L = m.L, KP = K + 1, KC = K;
+    const int c0 = col_first(), nc = min(kColB, m.N - c0);
+    if (nc <= 0) return;
+    if (tcol() < nc) {
+        const int c = tcol(), n = c0 + c;
+        const double a = s.a[n];
+        for (int k = tsub(); k < L; k += 4) {
+            const double Z = 0.5 * (m.oms[k] * a + m.oms[k + 1] * a);
+            const double r = densityJM(PL(s.TF, k, n, KC), PL(s.SF, k, n, KC), -Z, rho0);
+            th[c * KP + k + 1] = r * (a * m.dsig[k]) * gd0;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        double *col = th + threadIdx.x * KP;
+        double hp = 0.0;
+        col[0] = 0.0;
+        for (int k = 0; k < L; ++k) {
+            hp = hp + col[k + 1];
+            col[k + 1] = hp;
+        }
+    }
+    __syncthreads();
+    tile_store(s.hp, th, c0, nc, K, K);
+}
+
+// Per layer (four threads per element column) the gradient and its
+// Bpu/Bpv; the vertical integral Bar_pr_2D then sums the layers in order, one
+// lane per column.
+template <typename R>
+__global__ __launch_bounds__(kColT) void k_bpg(DevMesh m, DevStateT<R> s) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int K = m.K, L = m.L, KP = K + 1, KC = K;
+    double *tu = (double *)lds_, *tv = tu + kColB * KP;  // bu*Je, bv*Je
+    const int c0 = col_first(), nc = min(kColB, m.E - c0);
+    if (nc <= 0) return;
+    if (tcol() < nc) {
+        const int c = tcol(), e = c0 + c;
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
+        const double d0 = m.depth[q.x], d1 = m.depth[q.y], d2 = m.depth[q.z], d3 = m.depth[q.w];
+        const double h0 = d0 + s.eta_n[q.x], h1 = d1 + s.eta_n[q.y], h2 = d2 + s.eta_n[q.z], h3 = d3 + s.eta_n[q.w];
+        const double Ax = sum4(gx, h0, h1, h2, h3), Ay = sum4(gy, h0, h1, h2, h3);
+        const double Bx = sum4(gx, d0, d1, d2, d3), By = sum4(gy, d0, d1, d2, d3);
+        const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
+        const double accB = m.accB[e];
+        for (int k = tsub(); k < L; k += 4) {
+            const double ds = m.dsig[k];
+            const double je = sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
+            const double aJ = -accB / je;
+            const double p0 = PL(s.hp, k, q.x, KC), p1 = PL(s.hp, k, q.y, KC), p2 = PL(s.hp, k, q.z, KC),
+                         p3 = PL(s.hp, k, q.w, KC);
+            const double r0 = PL(s.hp, k + 1, q.x, KC), r1 = PL(s.hp, k + 1, q.y, KC), r2 = PL(s.hp, k + 1, q.z, KC),
+                         r3 = PL(s.hp, k + 1, q.w, KC);
+            const double qd_up = sum4(w, p0, p1, p2, p3), qd_low = sum4(w, r0, r1, r2, r3);
+            const double pr0 = 0.5 * (p0 + r0) * (a0 * ds), pr1 = 0.5 * (p1 + r1) * (a1 * ds);
+            const double pr2 = 0.5 * (p2 + r2) * (a2 * ds), pr3 = 0.5 * (p3 + r3) * (a3 * ds);
+            const double bu = aJ * (sum4(gx, pr0, pr1, pr2, pr3) + qd_low * (m.sigma[k + 1] * Ax - Bx) - qd_up * (m.sigma[k] * Ax - Bx));
+            const double bv = aJ * (sum4(gy, pr0, pr1, pr2, pr3) + qd_low * (m.sigma[k + 1] * Ay - By) - qd_up * (m.sigma[k] * Ay - By));
+            PL(s.Bpu, k, e, KC) = bu;
+            PL(s.Bpv, k, e, KC) = bv;
+            tu[c * KP + k] = bu * je;
+            tv[c * KP + k] = bv * je;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x, e = c0 + c;
+        double su = 0.0, sv = 0.0;
+        for (int k = 0; k < L; ++k) {
+            su = su + tu[c * KP + k];
+            sv = sv + tv[c * KP + k];
+        }
+        s.Bar_pr_2D[2 * e] = su;
+        s.Bar_pr_2D[2 * e + 1] = sv;
     }
 }
 
-// baroclinic_pressure_gradient_3D: src/fv_pressure_new.F90:146-221 (FP64)
+// ---------------------------------------------------------------------------
+// d3_end (src/fv_mixing.F90:73-347) in three kernels: the per-level
+// right-hand side (flat over the column slots), the iterated tridiagonal
+// solve (one thread per node column), and the per-level closure values
+// (flat).  Its scratch is private to these kernels and stored by level,
+// PP(x, k, n) = x[k*N + n], so the column-sequential solve reads it
+// contiguously across the wavefront: p = -b1-b2, q = b_t/dt, c2, the snu
+// column, xx1/yy1, and the btr ping-pong (mx_bA, mx_bB).  With the learned
+// closure, `only` restricts all three to the columns it sent back.
+// ---------------------------------------------------------------------------
+#define PP(arr, k, n, N) (arr)[(size_t)(k) * (N) + (n)]
+#define SG(nz) m.sigma[(nz) - 1]
+
 template <typename R>
-__global__ __launch_bounds__(256) void k_bpg(DevMesh m, DevStateT<R> s) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int N = m.N, E = m.E;
-    const int4 q = m.en[e];
-    const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
-    const double d0 = m.depth[q.x], d1 = m.depth[q.y], d2 = m.depth[q.z], d3 = m.depth[q.w];
-    const double h0 = d0 + s.eta_n[q.x], h1 = d1 + s.eta_n[q.y], h2 = d2 + s.eta_n[q.z], h3 = d3 + s.eta_n[q.w];
-    const double Ax = sum4(gx, h0, h1, h2, h3), Ay = sum4(gy, h0, h1, h2, h3);
-    const double Bx = sum4(gx, d0, d1, d2, d3), By = sum4(gy, d0, d1, d2, d3);
-    const double accB = m.accB[e];
-    const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
-    double p0 = PL(s.hp, 0, q.x, N), p1 = PL(s.hp, 0, q.y, N), p2 = PL(s.hp, 0, q.z, N), p3 = PL(s.hp, 0, q.w, N);
-    double qd_up = sum4(w, p0, p1, p2, p3);
-    double su = 0.0, sv = 0.0;
-    for (int k = 0; k < m.L; ++k) {
-        const double ds = m.dsig[k];
-        const double je = sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
-        const double aJ = -accB / je;
-        double r0 = PL(s.hp, k + 1, q.x, N), r1 = PL(s.hp, k + 1, q.y, N), r2 = PL(s.hp, k + 1, q.z, N), r3 = PL(s.hp, k + 1, q.w, N);
-        double qd_low = sum4(w, r0, r1, r2, r3);
-        double pr0 = 0.5 * (p0 + r0) * (a0 * ds), pr1 = 0.5 * (p1 + r1) * (a1 * ds);
-        double pr2 = 0.5 * (p2 + r2) * (a2 * ds), pr3 = 0.5 * (p3 + r3) * (a3 * ds);
-        double bu = aJ * (sum4(gx, pr0, pr1, pr2, pr3) + qd_low * (m.sigma[k + 1] * Ax - Bx) - qd_up * (m.sigma[k] * Ax - Bx));
-        double bv = aJ * (sum4(gy, pr0, pr1, pr2, pr3) + qd_low * (m.sigma[k + 1] * Ay - By) - qd_up * (m.sigma[k] * Ay - By));
-        PL(s.Bpu, k, e, E) = bu;
-        PL(s.Bpv, k, e, E) = bv;
-        su = su + bu * je;
-        sv = sv + bv * je;
-        p0 = r0; p1 = r1; p2 = r2; p3 = r3;
-        qd_up = qd_low;
+__global__ __launch_bounds__(kColT) void k_d3end_prep(DevMesh m, DevStateT<R> s, double dt, double Dmin, double rho0,
+                                                      const int *__restrict__ only) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int N = m.N, K = m.K, KP = K + 1, KC = K;
+    R *tp = (R *)lds_, *tq = tp + kColB * KP, *tc = tq + kColB * KP, *ts = tc + kColB * KP, *tb = ts + kColB * KP;
+    const int c0 = col_first(), nc = min(kColB, N - c0);
+    if (nc <= 0) return;
+    if (tcol() < nc && !(only && only[c0 + tcol()] == 0)) {
+        const int c = tcol(), n = c0 + c;
+#define SNU(z) PL(s.snu, (z) - 1, n, KC)
+#define BT(z) PL(s.bt, (z) - 1, n, KC)
+        const double t_inv = 1.0 / dt;
+        const double a = s.a[n];
+        const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+        const double dmean2 = dmean * dmean;
+        for (int lev = tsub(); lev < K; lev += 4) {
+            ts[c * KP + lev] = SNU(lev + 1);
+            tb[c * KP + lev] = BT(lev + 1);
+            const int nz = lev + 1;
+            if (nz < 2 || nz > K - 1) continue;
+            const double d1 = SG(nz - 1) - SG(nz);
+            const double d2 = SG(nz) - SG(nz + 1);
+            const double d = 0.5 * (d1 + d2);
+            // node-averaged velocities of compute_el2nodes_3D (src/fv_main.F90:2006)
+            const double uprev = PL(s.Unode, nz - 2, n, KC), vprev = PL(s.Vnode, nz - 2, n, KC);
+            const double ucur = PL(s.Unode, nz - 1, n, KC), vcur = PL(s.Vnode, nz - 1, n, KC);
+            double ap1 = uprev - ucur;
+            const double ap2 = vprev - vcur;
+            double b1 = SNU(nz) * (ap1 * ap1 + ap2 * ap2) / (d * d);
+            b1 = b1 / dmean2;
+            const double zb = m.oms[nz - 1] * a;
+            const double rho_up = densityJM(PL(s.TF, nz - 2, n, KC), PL(s.SF, nz - 2, n, KC), -zb, rho0);
+            const double rho_dn = densityJM(PL(s.TF, nz - 1, n, KC), PL(s.SF, nz - 1, n, KC), -zb, rho0);
+            ap1 = (rho_up - rho_dn) / d;
+            const double b2 = -0.5 * kG * PL(s.Kv, nz - 1, n, KC) * ap1 / (rho0 * dmean);
+            const double snu_p = 0.5 * (SNU(nz - 1) + SNU(nz));
+            const double snu_m = 0.5 * (SNU(nz) + SNU(nz + 1));
+            double c2 = 0.5 * 0.73 * (snu_p * (BT(nz - 1) - BT(nz)) / d1 - snu_m * (BT(nz) - BT(nz + 1)) / d2) / d;
+            c2 = c2 / dmean2;
+            tp[c * KP + lev] = -b1 - b2;
+            tq[c * KP + lev] = t_inv * BT(nz);
+            tc[c * KP + lev] = c2;
+        }
+#undef SNU
+#undef BT
     }
-    s.Bar_pr_2D[2 * e] = su;
-    s.Bar_pr_2D[2 * e + 1] = sv;
+    __syncthreads();
+    tile_to_planes(s.mx_s, ts, c0, nc, K, N, 0, K);
+    tile_to_planes(s.mx_bA, tb, c0, nc, K, N, 0, K);
+    tile_to_planes(s.mx_p, tp, c0, nc, K, N, 1, K - 1);
+    tile_to_planes(s.mx_q, tq, c0, nc, K, N, 1, K - 1);
+    tile_to_planes(s.mx_c2, tc, c0, nc, K, N, 1, K - 1);
 }
 
-// ---------------------------------------------------------------------------
-// compute_el2nodes_3D (src/fv_utilit.F90:644-700) fused into d3_end
-// (src/fv_mixing.F90:73-347): one thread per node column.
-// Scratch planes (K*N): p=-b1-b2, q=b_t/dt, c2, xx1, yy1, btr ping-pong.
-// ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double dt, double Dmin, double z0b_min, double z0s_min,
-                        double snul, double rho0, const int *__restrict__ only) {
+__global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double dt, double Dmin, double z0b_min,
+                                               const int *__restrict__ only) {
     int n = TID();
     if (n >= m.N) return;
     if (only && only[n] == 0) return;  // learned closure active: only the columns it sent back
-    const int N = m.N, E = m.E, K = m.K, L = m.L;
+    const int N = m.N, K = m.K, L = m.L, KC = K;
     const double t_inv = 1.0 / dt;
     const double B1cka2 = 16.6 * (kCka * kCka);
     const double a = s.a[n];
-    const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]) ;
+    const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
     const double dmean2 = dmean * dmean;
-    // node-averaged velocities of compute_el2nodes_3D (src/fv_main.F90:2006)
-    auto unode = [&](int k, double &u, double &v) {
-        u = PL(s.Unode, k, n, N);
-        v = PL(s.Vnode, k, n, N);
-    };
-    // 1-based level helpers
-#define SNU(nz) PL(s.snu, (nz) - 1, n, N)
-#define BT(nz) PL(s.bt, (nz) - 1, n, N)
-#define KV(nz) PL(s.Kv, (nz) - 1, n, N)
-#define SG(nz) m.sigma[(nz) - 1]
-    double uprev, vprev;
-    unode(0, uprev, vprev);
-    for (int nz = 2; nz <= K - 1; ++nz) {
-        double d1 = SG(nz - 1) - SG(nz);
-        double d2 = SG(nz) - SG(nz + 1);
-        double d = 0.5 * (d1 + d2);
-        double ucur, vcur;
-        unode(nz - 1, ucur, vcur);
-        double ap1 = uprev - ucur;
-        double ap2 = vprev - vcur;
-        double b1 = SNU(nz) * (ap1 * ap1 + ap2 * ap2) / (d * d);
-        b1 = b1 / dmean2;
-        double zb = m.oms[nz - 1] * a;
-        double rho_up = densityJM(PL(s.TF, nz - 2, n, N), PL(s.SF, nz - 2, n, N), -zb, rho0);
-        double rho_dn = densityJM(PL(s.TF, nz - 1, n, N), PL(s.SF, nz - 1, n, N), -zb, rho0);
-        ap1 = (rho_up - rho_dn) / d;
-        double b2 = -0.5 * kG * KV(nz) * ap1 / (rho0 * dmean);
-        double snu_p = 0.5 * (SNU(nz - 1) + SNU(nz));
-        double snu_m = 0.5 * (SNU(nz) + SNU(nz + 1));
-        double c2 = 0.5 * 0.73 * (snu_p * (BT(nz - 1) - BT(nz)) / d1 - snu_m * (BT(nz) - BT(nz + 1)) / d2) / d;
-        c2 = c2 / dmean2;
-        PL(s.mx_p, nz - 1, n, N) = -b1 - b2;
-        PL(s.mx_q, nz - 1, n, N) = t_inv * BT(nz);
-        PL(s.mx_c2, nz - 1, n, N) = c2;
-        uprev = ucur;
-        vprev = vcur;
-    }
-    // bottom: uprev/vprev now hold the deepest layer (nsigma-1)
-    const double ub = uprev, vb = vprev;
+#define SNU(z) PP(s.mx_s, (z) - 1, n, N)
+    // bottom: the deepest layer (nsigma-1) of the node-averaged velocity
+    const double ub = PL(s.Unode, L - 1, n, KC), vb = PL(s.Vnode, L - 1, n, KC);
     const double d1w = 0.5 * (SG(K - 1) - SG(K));
     const double wx = s.windx[n], wy = s.windy[n];
     const double uds = 0.05 * (wx * wx + wy * wy);
@@ -199,17 +317,16 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
     const double lg = log(delta_b / z0b_min);
     const double bott_R = B1cka2 / (lg * lg);
     R *bc = s.mx_bA, *bn = s.mx_bB;
-    for (int nz = 1; nz <= K; ++nz) PL(bc, nz - 1, n, N) = BT(nz);
     for (int it = 1; it <= 200; ++it) {
         double xxp = 1.0, yyp = yy2;
         for (int nz = 2; nz <= K - 1; ++nz) {
             double d1 = SG(nz - 1) - SG(nz);
             double d2 = SG(nz) - SG(nz + 1);
             double d = 0.5 * (d1 + d2);
-            double btr = PL(bc, nz - 1, n, N);
+            double btr = PP(bc, nz - 1, n, N);
             double sn = SNU(nz);
             double c1 = 0.046 * (btr * btr) / sn;
-            double fr = PL(s.mx_p, nz - 1, n, N) - c1 - PL(s.mx_q, nz - 1, n, N) - PL(s.mx_c2, nz - 1, n, N);
+            double fr = PP(s.mx_p, nz - 1, n, N) - c1 - PP(s.mx_q, nz - 1, n, N) - PP(s.mx_c2, nz - 1, n, N);
             double snu_p = 0.5 * (SNU(nz - 1) + sn);
             double snu_m = 0.5 * (sn + SNU(nz + 1));
             double am = 0.5 * (snu_p * 0.73) / (dmean2 * d1 * d);
@@ -218,29 +335,29 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
             double aa = bm - am * xxp;
             double xxn = cm / aa;
             double yyn = (am * yyp - fr) / aa;
-            PL(s.mx_xx, nz, n, N) = xxn;   // xx1(nz+1)
-            PL(s.mx_yy, nz, n, N) = yyn;
+            PP(s.mx_xx, nz, n, N) = xxn;   // xx1(nz+1)
+            PP(s.mx_yy, nz, n, N) = yyn;
             xxp = xxn;
             yyp = yyn;
         }
         // back substitution from the bottom boundary value, with convergence measure
         double b1v = bott_R * (ub * ub + vb * vb);
-        PL(bn, K - 1, n, N) = b1v;
+        PP(bn, K - 1, n, N) = b1v;
         double bbb = 0.0;
         {
-            double bo = PL(bc, K - 1, n, N);
+            double bo = PP(bc, K - 1, n, N);
             double aa = bo - b1v;
             double bb = dmaxr(b1v, bo) + 1.0e-8;
             double cc = fabs(aa / bb);
             if (cc > bbb) bbb = cc;
         }
         for (int nz = K; nz >= 2; --nz) {
-            double xx = (nz == 2) ? 1.0 : PL(s.mx_xx, nz - 1, n, N);
-            double yy = (nz == 2) ? yy2 : PL(s.mx_yy, nz - 1, n, N);
+            double xx = (nz == 2) ? 1.0 : PP(s.mx_xx, nz - 1, n, N);
+            double yy = (nz == 2) ? yy2 : PP(s.mx_yy, nz - 1, n, N);
             if (nz == K) { xx = xxp; yy = yyp; }
             b1v = b1v * xx + yy;
-            PL(bn, nz - 2, n, N) = b1v;
-            double bo = PL(bc, nz - 2, n, N);
+            PP(bn, nz - 2, n, N) = b1v;
+            double bo = PP(bc, nz - 2, n, N);
             double aa = bo - b1v;
             double bb = dmaxr(b1v, bo) + 1.0e-8;
             double cc = fabs(aa / bb);
@@ -249,34 +366,51 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
         if (bbb < 1.0e-6) break;
         R *t = bc; bc = bn; bn = t;
     }
-    for (int nz = 1; nz <= K; ++nz) {
-        double btr = PL(bc, nz - 1, n, N);
-        double btv = dmaxr(btr, 1.0e-8);
-        BT(nz) = btv;
-        double zt = SG(nz) * dmean;
-        double Zh = zt + z0b_min;
-        double Zd = fabs(dmean - zt) + z0s_min;
-        double Z0 = 1.0 - kBetaScale * Zh * Zd / (dmean * dmean);
-        double scale_t = kCka * Zh * Zd * Z0 / dmean;
+    // the converged column, for the per-level closure values
+    for (int k = 0; k < K; ++k) PP(s.mx_xx, k, n, N) = PP(bc, k, n, N);
+#undef SNU
+}
+
+template <typename R>
+__global__ __launch_bounds__(kColT) void k_d3end_fin(DevMesh m, DevStateT<R> s, double Dmin, double z0b_min,
+                                                     double z0s_min, double snul, const int *__restrict__ only) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int N = m.N, K = m.K, KP = K + 1, KC = K;
+    R *tx = (R *)lds_;
+    const int c0 = col_first(), nc = min(kColB, N - c0);
+    if (nc <= 0) return;
+    planes_to_tile(tx, s.mx_xx, c0, nc, K, N);
+    __syncthreads();
+    if (tcol() >= nc) return;
+    const int c = tcol(), n = c0 + c;
+    if (only && only[n] == 0) return;
+    const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+    for (int lev = tsub(); lev < K; lev += 4) {
+        const int nz = lev + 1;
+        const double btr = tx[c * KP + lev];
+        const double btv = dmaxr(btr, 1.0e-8);
+        PL(s.bt, lev, n, KC) = btv;
+        const double zt = SG(nz) * dmean;
+        const double Zh = zt + z0b_min;
+        const double Zd = fabs(dmean - zt) + z0s_min;
+        const double Z0 = 1.0 - kBetaScale * Zh * Zd / (dmean * dmean);
+        const double scale_t = kCka * Zh * Zd * Z0 / dmean;
         double sv = scale_t * sqrt(btv) + snul;
         if (sv >= 0.2) sv = 0.2;
-        SNU(nz) = sv;
-        KV(nz) = kPrNum * (sv);
+        PL(s.snu, lev, n, KC) = sv;
+        PL(s.Kv, lev, n, KC) = kPrNum * (sv);
     }
-#undef SNU
-#undef BT
-#undef KV
-#undef SG
 }
+#undef SG
 
 // ---------------------------------------------------------------------------
 // vel_gradients (3-D, free slip): src/fv_gradients.F90:2-93
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevStateT<R> s) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int E = m.E;
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    const int KC = m.K;
     const int4 q = m.en[e];
     const int jend = (q.x == q.w) ? 3 : 4;
     const int4 nb = m.elnb[e];
@@ -286,38 +420,37 @@ __global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevStateT<R> s) {
     const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
     const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
     const double ec = m.elem_cos[e];
-    for (int k = 0; k < m.L; ++k) {
-        const double u = PL(s.U_n, k, e, E), v = PL(s.V_n, k, e, E);
-        double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
-        for (int j = 0; j < jend; ++j) {
-            int e2 = nbs[j];
-            if (e2 >= 0) {
-                double du = PL(s.U_n, k, e2, E) - u;
-                double dv = PL(s.V_n, k, e2, E) - v;
-                ux = ux + gxs[j] * du;
-                uy = uy + gys[j] * du;
-                vx = vx + gxs[j] * dv;
-                vy = vy + gys[j] * dv;
-            } else {
-                double2 dd = m.edxdy[eds[j]];
-                double zc0 = dd.x * ec, zc1 = dd.y;
-                double un = -2 * (u * zc1 - v * zc0) / (zc0 * zc0 + zc1 * zc1);
-                ux = ux + gxs[j] * un * zc1;
-                uy = uy + gys[j] * un * zc1;
-                vx = vx - gxs[j] * un * zc0;
-                vy = vy - gys[j] * un * zc0;
-            }
+    const double u = PL(s.U_n, k, e, KC), v = PL(s.V_n, k, e, KC);
+    double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
+    for (int j = 0; j < jend; ++j) {
+        int e2 = nbs[j];
+        if (e2 >= 0) {
+            double du = PL(s.U_n, k, e2, KC) - u;
+            double dv = PL(s.V_n, k, e2, KC) - v;
+            ux = ux + gxs[j] * du;
+            uy = uy + gys[j] * du;
+            vx = vx + gxs[j] * dv;
+            vy = vy + gys[j] * dv;
+        } else {
+            double2 dd = m.edxdy[eds[j]];
+            double zc0 = dd.x * ec, zc1 = dd.y;
+            double un = -2 * (u * zc1 - v * zc0) / (zc0 * zc0 + zc1 * zc1);
+            ux = ux + gxs[j] * un * zc1;
+            uy = uy + gys[j] * un * zc1;
+            vx = vx - gxs[j] * un * zc0;
+            vy = vy - gys[j] * un * zc0;
         }
-        PL(s.gux, k, e, E) = ux;
-        PL(s.guy, k, e, E) = uy;
-        PL(s.gvx, k, e, E) = vx;
-        PL(s.gvy, k, e, E) = vy;
     }
+    PL(s.gux, k, e, KC) = ux;
+    PL(s.guy, k, e, KC) = uy;
+    PL(s.gvx, k, e, KC) = vx;
+    PL(s.gvy, k, e, KC) = vy;
 }
 
 // momentum_adv_upwind edge flux at one level: src/fv_advection.F90:120-165
 template <typename R>
 __device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevStateT<R> &s, int ed, int k, R &uu, R &vv) {
+    const int KC = m.K;
     // in the storage type: FP32 advective fluxes in the mixed-precision path
     const int E = m.E;
     const int2 nd = m.ednodes[ed];
@@ -327,10 +460,10 @@ __device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevStateT<R> &
     const double2 dd = m.edxdy[ed];
     const R cx = (R)cd.x, cy = (R)cd.y, cz = (R)cd.z, cw = (R)cd.w, dx = (R)dd.x, dy = (R)dd.y;
     const int e1 = et.x, e2 = et.y;
-    const R u1 = PL(s.U_n, k, e1, E) - PL(s.gux, k, e1, E) * cx - PL(s.guy, k, e1, E) * cy;
-    const R v1 = PL(s.V_n, k, e1, E) - PL(s.gvx, k, e1, E) * cx - PL(s.gvy, k, e1, E) * cy;
-    const R u2 = PL(s.U_n, k, e2, E) - PL(s.gux, k, e2, E) * cz - PL(s.guy, k, e2, E) * cw;
-    const R v2 = PL(s.V_n, k, e2, E) - PL(s.gvx, k, e2, E) * cz - PL(s.gvy, k, e2, E) * cw;
+    const R u1 = PL(s.U_n, k, e1, KC) - PL(s.gux, k, e1, KC) * cx - PL(s.guy, k, e1, KC) * cy;
+    const R v1 = PL(s.V_n, k, e1, KC) - PL(s.gvx, k, e1, KC) * cx - PL(s.gvy, k, e1, KC) * cy;
+    const R u2 = PL(s.U_n, k, e2, KC) - PL(s.gux, k, e2, KC) * cz - PL(s.guy, k, e2, KC) * cw;
+    const R v2 = PL(s.V_n, k, e2, KC) - PL(s.gvx, k, e2, KC) * cz - PL(s.gvy, k, e2, KC) * cw;
     const double ds = m.dsig[k];
     const R jd = (R)(0.5 * (s.a[nd.x] * ds + s.a[nd.y] * ds));
     const R un = R(0.5) * R(kREarthD) * jd * ((u1 + u2) * dy - (v1 * (R)m.elem_cos[e1] + v2 * (R)m.elem_cos[e2]) * dx);
@@ -341,17 +474,6 @@ __device__ __forceinline__ void edge_uu3d(const DevMesh &m, const DevStateT<R> &
         uu = u2 * un * acc;
         vv = v2 * un * acc;
     }
-}
-
-// (element, level) of this thread: blocks of 64 consecutive elements x 4
-// layers, layer groups of one element tile adjacent (see node_level).
-__device__ __forceinline__ bool elem_level(int E, int L, int &e, int &k) {
-    const unsigned b = xcd_block();
-    const unsigned lpb = blockDim.x >> 6;
-    const unsigned nlg = ((unsigned)L + lpb - 1u) / lpb;
-    e = (int)((b / nlg) * 64u + (threadIdx.x & 63u));
-    k = (int)((b % nlg) * lpb + (threadIdx.x >> 6));
-    return e < E && k < L;
 }
 
 // Element quantities shared by the two compute_vel_rhs kernels.
@@ -365,8 +487,9 @@ struct ElemCol {
     }
     template <typename R>
     __device__ double Av(const DevMesh &m, const DevStateT<R> &s, int k) const {
+    const int KC = m.K;
         const int N = m.N;
-        return sum4(w, PL(s.snu, k, q.x, N), PL(s.snu, k, q.y, N), PL(s.snu, k, q.z, N), PL(s.snu, k, q.w, N));
+        return sum4(w, PL(s.snu, k, q.x, KC), PL(s.snu, k, q.y, KC), PL(s.snu, k, q.z, KC), PL(s.snu, k, q.w, KC));
     }
     __device__ double zbw(const DevMesh &m, int lev) const {  // sum w_cv*zbar at 0-based level
         const double o = m.oms[lev];
@@ -396,12 +519,13 @@ __device__ __forceinline__ ElemCol elem_col(const DevMesh &m, const DevStateT<R>
 template <typename R>
 __device__ __forceinline__ void uvert(const DevMesh &m, const DevStateT<R> &s, const ElemCol &c, int e, double area,
                                       int nz, double &fu, double &fv) {
+    const int KC = m.K;
     const int N = m.N, E = m.E, K = m.K;
     const int4 q = c.q;
-    auto U = [&](int k) { return PL(s.U_n, k, e, E); };
-    auto V = [&](int k) { return PL(s.V_n, k, e, E); };
-    const double wv = sum4(c.w, PL(s.W, nz - 1, q.x, N), PL(s.W, nz - 1, q.y, N), PL(s.W, nz - 1, q.z, N),
-                           PL(s.W, nz - 1, q.w, N)) * area;
+    auto U = [&](int k) { return PL(s.U_n, k, e, KC); };
+    auto V = [&](int k) { return PL(s.V_n, k, e, KC); };
+    const double wv = sum4(c.w, PL(s.W, nz - 1, q.x, KC), PL(s.W, nz - 1, q.y, KC), PL(s.W, nz - 1, q.z, KC),
+                           PL(s.W, nz - 1, q.w, KC)) * area;
     double um = 0.0, vm = 0.0;
     if (wv >= 0.0) {
         if (nz == K - 1) {
@@ -449,8 +573,9 @@ __device__ __forceinline__ void uvert(const DevMesh &m, const DevStateT<R> &s, c
 // applies the implicit vertical viscosity column by column.
 template <typename R>
 __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv) {
+    const int KC = m.K;
     int e, k;
-    if (!elem_level(m.E, m.L, e, k)) return;
+    if (!ent_layer(m.E, m.L, e, k)) return;
     const int N = m.N, E = m.E, K = m.K;
     const ElemCol c = elem_col(m, s, e);
     const int4 q = c.q;
@@ -466,9 +591,9 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
     const double Fy = sum4(gy, pre0, pre1, pre2, pre3) * area;
     const double acc = sum4(w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
     const double je = c.Je(m, k);
-    const double u = PL(s.U_n, k, e, E), v = PL(s.V_n, k, e, E);
-    double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, E) + Fx * je;
-    double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, E) + Fy * je;
+    const double u = PL(s.U_n, k, e, KC), v = PL(s.V_n, k, e, KC);
+    double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, KC) + Fx * je;
+    double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, KC) + Fy * je;
     R abu = (R)(v * cor * area * je);
     R abv = (R)(-u * cor * area * je);
     if (hadv) {
@@ -497,10 +622,10 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
                 const int nz = k + 1 + t;
                 double fu = 0.0, fv = 0.0;
                 if (nz == 1) {
-                    const double w1 = sum4(w, PL(s.W, 0, q.x, N), PL(s.W, 0, q.y, N), PL(s.W, 0, q.z, N),
-                                           PL(s.W, 0, q.w, N)) * area;
-                    fu = -w1 * PL(s.U_n, 0, e, E);
-                    fv = -w1 * PL(s.V_n, 0, e, E);
+                    const double w1 = sum4(w, PL(s.W, 0, q.x, KC), PL(s.W, 0, q.y, KC), PL(s.W, 0, q.z, KC),
+                                           PL(s.W, 0, q.w, KC)) * area;
+                    fu = -w1 * PL(s.U_n, 0, e, KC);
+                    fv = -w1 * PL(s.V_n, 0, e, KC);
                 } else if (nz <= K - 1) {
                     uvert(m, s, c, e, area, nz, fu, fv);
                 }
@@ -519,87 +644,114 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
             abv = R(0);
         }
     }
-    PL(s.U_rhsAB, k, e, E) = abu;
-    PL(s.V_rhsAB, k, e, E) = abv;
-    PL(s.U_rhs, k, e, E) = mask * (u + PL(s.Bpu, k, e, E) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
-    PL(s.V_rhs, k, e, E) = mask * (v + PL(s.Bpv, k, e, E) * dt + (dt * (vr + abv * ff) * area_inv + 0.0) / je);
+    PL(s.U_rhsAB, k, e, KC) = abu;
+    PL(s.V_rhsAB, k, e, KC) = abv;
+    PL(s.U_rhs, k, e, KC) = mask * (u + PL(s.Bpu, k, e, KC) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
+    PL(s.V_rhs, k, e, KC) = mask * (v + PL(s.Bpv, k, e, KC) * dt + (dt * (vr + abv * ff) * area_inv + 0.0) / je);
 }
 
-// momentum_vert_impl_visc_tmp (fv_viscosity.F90:1374-1486), one thread per
-// element column: Av formed from snu (fv_mixing.F90:308-313), surface
-// stress, forward sweep over the provisional velocity, bottom condition by
-// fic_point, back substitution; dry elements get zero velocity.
+// momentum_vert_impl_visc_tmp (fv_viscosity.F90:1374-1486) on tiles of 64
+// element columns: the provisional velocity and Av (formed from snu,
+// fv_mixing.F90:308-313) are staged in LDS; one lane per column
+// then runs the surface stress, the forward sweep, the bottom condition by
+// fic_point and the back substitution.  The sweep keeps its coefficients in
+// the tile slots it has consumed (xx1(k+1) in Av's slot k, yu/yv in the
+// velocity's slot k), rounded to the storage type as the Thomas scratch is.
+// Dry elements get zero velocity.
 template <typename R>
-__global__ __launch_bounds__(256) void k_velvisc(DevMesh m, DevStateT<R> s, double dt, double rho0, double Cd_bz, int fic_point) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int E = m.E, L = m.L, K = m.K;
-    const double mask = s.mask_wd[e];
-    if (mask == 0.0) {
-        for (int k = 0; k < L; ++k) {
-            PL(s.U_rhs, k, e, E) = 0.0;
-            PL(s.V_rhs, k, e, E) = 0.0;
+__global__ __launch_bounds__(kColT) void k_velvisc(DevMesh m, DevStateT<R> s, double dt, double rho0, double Cd_bz, int fic_point) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int K = m.K, L = m.L, KP = K + 1, KC = K;
+    double *tA = (double *)lds_;
+    R *tU = (R *)(tA + kColB * KP), *tV = tU + kColB * KP;
+    const int c0 = col_first(), nc = min(kColB, m.E - c0);
+    if (nc <= 0) return;
+    tile_load(tU, s.U_rhs, c0, nc, K);
+    tile_load(tV, s.V_rhs, c0, nc, K);
+    if (tcol() < nc) {
+        const int c = tcol(), e = c0 + c;
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        for (int k = tsub(); k < K; k += 4)
+            tA[c * KP + k] = sum4(w, PL(s.snu, k, q.x, KC), PL(s.snu, k, q.y, KC), PL(s.snu, k, q.z, KC),
+                                  PL(s.snu, k, q.w, KC));
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x, e = c0 + c;
+        double *A = tA + c * KP;
+        R *U = tU + c * KP, *V = tV + c * KP;
+        const double mask = s.mask_wd[e];
+        if (mask == 0.0) {
+            for (int k = 0; k < L; ++k) {
+                U[k] = R(0);
+                V[k] = R(0);
+            }
+        } else {
+            const ElemCol col = elem_col(m, s, e);
+            const int4 q = col.q;
+            const double acc = sum4(col.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+            const double avL1 = A[L - 1], avK1 = A[K - 1];
+            double xxp = 1.0, yup, yvp;
+            {
+                const double de = col.Je(m, 0) / (A[0] + A[1]);
+                const double rw = de / rho0;
+                yup = s.taux[e] * rw;
+                yvp = s.tauy[e] * rw;
+            }
+            double je_prev = 0.0, je_cur = col.Je(m, 0);
+            double av_cur = A[0];
+            for (int k = 0; k < L; ++k) {
+                const double je = je_cur;
+                const double je_next = (k + 1 < L) ? col.Je(m, k + 1) : 0.0;
+                const double ur = U[k], vr = V[k];
+                const double av_next = A[k + 1];
+                const double delta_1 = (k == 0) ? 0.5 * je : 0.5 * (je + je_prev);
+                const double delta_2 = (k == L - 1) ? 0.5 * je : 0.5 * (je + je_next);
+                const double zt = dt / je;
+                const double aa = zt * av_cur / delta_1;
+                const double cc = zt * av_next / delta_2;
+                const double bb = aa + cc + 1.0;
+                const double d = bb - aa * xxp;
+                xxp = cc / d;
+                yup = (aa * yup + ur) / d;
+                yvp = (aa * yvp + vr) / d;
+                A[k] = (double)(R)xxp;  // vxx(k+1)
+                U[k] = (R)yup;          // vyu(k+1)
+                V[k] = (R)yvp;          // vyv(k+1)
+                av_cur = av_next;
+                je_prev = je;
+                je_cur = je_next;
+            }
+            const double a_Cd = s.C_d_el[e] * ((1.0 - acc) * Cd_bz + 1.0);
+            const double de = col.Je(m, L - 1) / (avK1 + avL1);
+            const double ub = PL(s.U_n, L - 1, e, KC), vb = PL(s.V_n, L - 1, e, KC);
+            const double friction = de * a_Cd * sqrt(ub * ub + vb * vb);
+            double ut, vt;
+            if (fic_point == 0) {
+                ut = 0.0;
+                vt = 0.0;
+            } else if (fic_point == 1) {
+                const double aa = friction + 1.0 - xxp;
+                ut = yup / aa;
+                vt = yvp / aa;
+            } else {
+                const double aa = 1.0 / (1.0 - friction) - xxp;
+                ut = yup / aa;
+                vt = yvp / aa;
+            }
+            for (int k = L - 1; k >= 0; --k) {
+                const double xx = A[k];
+                ut = ut * xx + (double)U[k];
+                vt = vt * xx + (double)V[k];
+                U[k] = (R)(ut * mask);
+                V[k] = (R)(vt * mask);
+            }
         }
-        return;
     }
-    const ElemCol c = elem_col(m, s, e);
-    const int4 q = c.q;
-    const double acc = sum4(c.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
-    double xxp = 1.0, yup, yvp;
-    {
-        const double de = c.Je(m, 0) / (c.Av(m, s, 0) + c.Av(m, s, 1));
-        const double rw = de / rho0;
-        yup = s.taux[e] * rw;
-        yvp = s.tauy[e] * rw;
-    }
-    double je_prev = 0.0, je_cur = c.Je(m, 0);
-    double av_cur = c.Av(m, s, 0);
-    for (int k = 0; k < L; ++k) {
-        const double je = je_cur;
-        const double je_next = (k + 1 < L) ? c.Je(m, k + 1) : 0.0;
-        const double ur = PL(s.U_rhs, k, e, E), vr = PL(s.V_rhs, k, e, E);
-        const double av_next = c.Av(m, s, k + 1);
-        const double delta_1 = (k == 0) ? 0.5 * je : 0.5 * (je + je_prev);
-        const double delta_2 = (k == L - 1) ? 0.5 * je : 0.5 * (je + je_next);
-        const double zt = dt / je;
-        const double aa = zt * av_cur / delta_1;
-        const double cc = zt * av_next / delta_2;
-        const double bb = aa + cc + 1.0;
-        const double d = bb - aa * xxp;
-        xxp = cc / d;
-        yup = (aa * yup + ur) / d;
-        yvp = (aa * yvp + vr) / d;
-        PL(s.vxx, k + 1, e, E) = xxp;
-        PL(s.vyu, k + 1, e, E) = yup;
-        PL(s.vyv, k + 1, e, E) = yvp;
-        av_cur = av_next;
-        je_prev = je;
-        je_cur = je_next;
-    }
-    const double a_Cd = s.C_d_el[e] * ((1.0 - acc) * Cd_bz + 1.0);
-    const double de = c.Je(m, L - 1) / (c.Av(m, s, K - 1) + c.Av(m, s, L - 1));
-    const double ub = PL(s.U_n, L - 1, e, E), vb = PL(s.V_n, L - 1, e, E);
-    const double friction = de * a_Cd * sqrt(ub * ub + vb * vb);
-    double ut, vt;
-    if (fic_point == 0) {
-        ut = 0.0;
-        vt = 0.0;
-    } else if (fic_point == 1) {
-        const double aa = friction + 1.0 - xxp;
-        ut = yup / aa;
-        vt = yvp / aa;
-    } else {
-        const double aa = 1.0 / (1.0 - friction) - xxp;
-        ut = yup / aa;
-        vt = yvp / aa;
-    }
-    for (int k = L - 1; k >= 0; --k) {
-        const double xx = PL(s.vxx, k + 1, e, E);
-        ut = ut * xx + PL(s.vyu, k + 1, e, E);
-        vt = vt * xx + PL(s.vyv, k + 1, e, E);
-        PL(s.U_rhs, k, e, E) = ut * mask;
-        PL(s.V_rhs, k, e, E) = vt * mask;
-    }
+    __syncthreads();
+    tile_store(s.U_rhs, tU, c0, nc, K, L);
+    tile_store(s.V_rhs, tV, c0, nc, K, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -703,94 +855,139 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
 }
 
 // first substep only: dmean_n (fv_main.F90:1467-1473) and compute_puls_vel
-// (fv_3D_vel.F90:257-307).  After k_bt_node's shift eta_n holds eta^{n+1}
-// (= ssh_rhs of the reference) and eta_n_1 holds eta^n.
+// (fv_3D_vel.F90:257-307), one thread per element and layer.  After
+// k_bt_node's shift eta_n holds eta^{n+1} (= ssh_rhs of the reference) and
+// eta_n_1 holds eta^n.  Also caches the bottom-layer velocity U_n(nsigma-1),
+// which update_2D_vel reads in every substep, as a 2-D field.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevStateT<R> s, double Dmin) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int E = m.E;
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    const int KC = m.K;
     const int4 q = m.en[e];
     const double4 w = m.wcv[e];
     const double dm = dmaxr(Dmin, w.x * (s.eta_n_1[q.x] + m.depth[q.x]) + w.y * (s.eta_n_1[q.y] + m.depth[q.y])
                                   + w.z * (s.eta_n_1[q.z] + m.depth[q.z]) + w.w * (s.eta_n_1[q.w] + m.depth[q.w]));
-    s.dmean_n[e] = dm;
     const double snew = sum4(w, s.eta_n[q.x] + m.depth[q.x], s.eta_n[q.y] + m.depth[q.y], s.eta_n[q.z] + m.depth[q.z],
                              s.eta_n[q.w] + m.depth[q.w]);
     const double ap = dm / dmaxr(Dmin, snew);
     const double u2 = s.U2[2 * e], v2 = s.U2[2 * e + 1];
-    for (int k = 0; k < m.L; ++k) {
-        PL(s.U_puls, k, e, E) = PL(s.U_n, k, e, E) - ap * u2;
-        PL(s.V_puls, k, e, E) = PL(s.V_n, k, e, E) - ap * v2;
+    const double un = PL(s.U_n, k, e, KC), vn = PL(s.V_n, k, e, KC);
+    PL(s.U_puls, k, e, KC) = un - ap * u2;
+    PL(s.V_puls, k, e, KC) = vn - ap * v2;
+    if (k == 0) s.dmean_n[e] = dm;
+    if (k == m.L - 1) {
+        s.ubot[2 * e] = un;
+        s.ubot[2 * e + 1] = vn;
     }
 }
 
-// compute_el2nodes_3D(U_puls, V_puls): src/fv_utilit.F90:669-689
+// compute_el2nodes_3D (src/fv_utilit.F90:669-689), one thread per node and
+// layer: area-weighted mean over the node's elements in ascending id.
 template <typename R>
 __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict__ Ue, const R *__restrict__ Ve,
                            R *__restrict__ Un, R *__restrict__ Vn) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N, E = m.E;
+    int n, k;
+    if (!ent_layer(m.N, m.L, n, k)) return;
+    const int KC = m.K;
     const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
     double sa = 0.0;
     for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
     const double tvi = 1.0 / sa;
-    for (int k = 0; k < m.L; ++k) {
-        double su = 0.0, sv = 0.0;
-        for (int j = p0; j < p1; ++j) {
-            int el = m.n2el[j];
-            double ar = m.elem_area[el];
-            su = su + PL(Ue, k, el, E) * ar;
-            sv = sv + PL(Ve, k, el, E) * ar;
-        }
-        PL(Un, k, n, N) = su * tvi;
-        PL(Vn, k, n, N) = sv * tvi;
+    double su = 0.0, sv = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        int el = m.n2el[j];
+        double ar = m.elem_area[el];
+        su = su + PL(Ue, k, el, KC) * ar;
+        sv = sv + PL(Ve, k, el, KC) * ar;
     }
+    PL(Un, k, n, KC) = su * tvi;
+    PL(Vn, k, n, KC) = sv * tvi;
 }
 
 // momentum_adv_P1_3D_to_2D (fv_advection.F90:676-793) as an element gather,
-// then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513)
+// then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513).  A wavefront takes 16
+// element columns: the per-layer edge terms uu*un*Je of each element's
+// (up to 4) internal edges are formed by four threads per column and staged
+// in LDS; one lane per (column, edge) then sums them over the layers
+// in order, and one lane per column combines the edges in ascending id.
+constexpr int kP1Cols = 16;
 template <typename R>
-__global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int N = m.N, E = m.E;
-    const int4 ied = m.el_ied[e];
-    const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
-    double ru = 0.0, rv = 0.0;
-    for (int j = 0; j < 4; ++j) {
-        int c = slots[j];
-        if (c == kEmpty) break;
-        int ed = c >= 0 ? c : ~c;
-        const int2 nd = m.ednodes[ed];
-        const int2 et = m.edtri[ed];
-        const double acc = (m.ac[nd.x] + m.ac[nd.y]) * .5;
-        const double2 dd = m.edxdy[ed];
-        const double xe = dd.x * kREarthD * 0.5 * (m.elem_cos[et.x] + m.elem_cos[et.y]);
-        const double ye = dd.y * kREarthD;
-        const double an1 = s.a[nd.x], an2 = s.a[nd.y];
-        double su = 0.0, sv = 0.0;
-        for (int k = 0; k < m.L; ++k) {
-            double uu = 0.5 * (PL(s.Unode_p, k, nd.x, N) + PL(s.Unode_p, k, nd.y, N));
-            double vv = 0.5 * (PL(s.Vnode_p, k, nd.x, N) + PL(s.Vnode_p, k, nd.y, N));
-            double ds = m.dsig[k];
-            double jd = 0.5 * (an1 * ds + an2 * ds);
-            double un = (uu * ye - vv * xe) * jd;
-            double je = PL(s.Je, k, e, E);
-            su = su + uu * un * je;
-            sv = sv + vv * un * je;
-        }
-        if (c >= 0) {  // element is edge_tri(2)
-            ru = ru + acc * su;
-            rv = rv + acc * sv;
-        } else {
-            ru = ru - acc * su;
-            rv = rv - acc * sv;
+__global__ __launch_bounds__(kColB) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int K = m.K, L = m.L, KP = K + 1, KC = K;
+    double *tu = (double *)lds_;                      // [edge slot][column][KP]
+    double *tv = tu + 4 * kP1Cols * KP;
+    double *su_ = tv + 4 * kP1Cols * KP, *sv_ = su_ + 4 * kP1Cols;  // [column][edge slot]
+    const int c0 = (int)xcd_block() * kP1Cols, nc = min(kP1Cols, m.E - c0);
+    if (nc <= 0) return;
+    if (tcol() < nc) {
+        const int c = tcol(), e = c0 + c;
+        const int4 ied = m.el_ied[e];
+        const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+        for (int j = 0; j < 4; ++j) {
+            const int cc = slots[j];
+            if (cc == kEmpty) break;
+            const int ed = cc >= 0 ? cc : ~cc;
+            const int2 nd = m.ednodes[ed];
+            const int2 et = m.edtri[ed];
+            const double2 dd = m.edxdy[ed];
+            const double xe = dd.x * kREarthD * 0.5 * (m.elem_cos[et.x] + m.elem_cos[et.y]);
+            const double ye = dd.y * kREarthD;
+            const double an1 = s.a[nd.x], an2 = s.a[nd.y];
+            for (int k = tsub(); k < L; k += 4) {
+                const double ds = m.dsig[k];
+                double uu = 0.5 * (PL(s.Unode_p, k, nd.x, KC) + PL(s.Unode_p, k, nd.y, KC));
+                double vv = 0.5 * (PL(s.Vnode_p, k, nd.x, KC) + PL(s.Vnode_p, k, nd.y, KC));
+                double jd = 0.5 * (an1 * ds + an2 * ds);
+                double un = (uu * ye - vv * xe) * jd;
+                const double je = PL(s.Je, k, e, KC);
+                tu[(j * kP1Cols + c) * KP + k] = uu * un * je;
+                tv[(j * kP1Cols + c) * KP + k] = vv * un * je;
+            }
         }
     }
-    s.U_rhs_2D_3D[2 * e] = ru / s.dmean_n[e];
-    s.U_rhs_2D_3D[2 * e + 1] = rv / s.dmean_n[e];
+    __syncthreads();
+    {
+        const int c = threadIdx.x >> 2, j = threadIdx.x & 3;
+        if (c < nc) {
+            const int4 ied = m.el_ied[c0 + c];
+            const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
+            double su = 0.0, sv = 0.0;
+            if (cc != kEmpty) {
+                const double *pu = tu + (j * kP1Cols + c) * KP, *pv = tv + (j * kP1Cols + c) * KP;
+                for (int k = 0; k < L; ++k) {
+                    su = su + pu[k];
+                    sv = sv + pv[k];
+                }
+            }
+            su_[c * 4 + j] = su;
+            sv_[c * 4 + j] = sv;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x, e = c0 + c;
+        const int4 ied = m.el_ied[e];
+        const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+        double ru = 0.0, rv = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            const int cc = slots[j];
+            if (cc == kEmpty) break;
+            const int2 nd = m.ednodes[cc >= 0 ? cc : ~cc];
+            const double acc = (m.ac[nd.x] + m.ac[nd.y]) * .5;
+            const double su = su_[c * 4 + j], sv = sv_[c * 4 + j];
+            if (cc >= 0) {  // element is edge_tri(2)
+                ru = ru + acc * su;
+                rv = rv + acc * sv;
+            } else {
+                ru = ru - acc * su;
+                rv = rv - acc * sv;
+            }
+        }
+        s.U_rhs_2D_3D[2 * e] = ru / s.dmean_n[e];
+        s.U_rhs_2D_3D[2 * e + 1] = rv / s.dmean_n[e];
+    }
 }
 
 // momentum_adv_upwind_2D edge flux: fv_advection.F90:589-643
@@ -824,6 +1021,7 @@ __device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevStateT<R> &
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
                            int wet_dry, int adv2d, int first) {
+    const int KC = m.K;
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E, L = m.L;
@@ -871,7 +1069,7 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
                                         s.eta_n_1[q.w] + d3));
     const double acc = sum4(w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
     const double a_Cd = C_d * ((1.0 - acc) * Cd_bz + 1.0);
-    const double ub = PL(s.U_n, L - 1, e, E), vb = PL(s.V_n, L - 1, e, E);
+    const double ub = s.ubot[2 * e], vb = s.ubot[2 * e + 1];  // U_n, V_n at the bottom layer
     const double friction = a_Cd * sqrt(ub * ub + vb * vb);
     const double snew = sum4(w, s.eta_n[q.x] + d0, s.eta_n[q.y] + d1, s.eta_n[q.z] + d2, s.eta_n[q.w] + d3);
     const double ffc = 1.0 / (dmaxr(Dmin, snew) + friction * dt_2D);
@@ -907,52 +1105,71 @@ __global__ __launch_bounds__(256) void k_jac_node(DevMesh m, DevStateT<R> s, dou
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_je(DevMesh m, DevStateT<R> s) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int E = m.E;
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    const int KC = m.K;
     const int4 q = m.en[e];
     const double4 w = m.wcv[e];
     const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
-    for (int k = 0; k < m.L; ++k) {
-        double ds = m.dsig[k];
-        PL(s.Je, k, e, E) = sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
-    }
+    const double ds = m.dsig[k];
+    PL(s.Je, k, e, KC) = sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
 }
 
 // update_3D_vel: src/fv_3D_vel.F90:201-255, with U_filt_2D *= 1/Mt from
-// oce_barotropic_timestep (fv_main.F90:1670-1677)
+// oce_barotropic_timestep (fv_main.F90:1670-1677), on tiles of 64 element
+// columns: the vertical integral runs one lane per column over the staged
+// U_rhs/V_rhs, the per-layer update flat over the block's items.
 template <typename R>
-__global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, double Dmin, double Mt_inv) {
-    int e = TID();
-    if (e >= m.E) return;
-    const int E = m.E, L = m.L;
+__global__ __launch_bounds__(kColT) void k_update3d(DevMesh m, DevStateT<R> s, double Dmin, double Mt_inv) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int K = m.K, L = m.L, KP = K + 1, KC = K;
+    double *sc = (double *)lds_;  // per column: U2p, V2p, uf, vf
+    R *tU = (R *)(sc + 4 * kColB), *tV = tU + kColB * KP;
+    const int c0 = col_first(), nc = min(kColB, m.E - c0);
+    if (nc <= 0) return;
+    tile_load(tU, s.U_rhs, c0, nc, K);
+    tile_load(tV, s.V_rhs, c0, nc, K);
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x, e = c0 + c;
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        const double o0 = s.a_old[q.x], o1 = s.a_old[q.y], o2 = s.a_old[q.z], o3 = s.a_old[q.w];
+        const double dmi = 1.0 / dmaxr(Dmin, sum4(w, m.depth[q.x] + s.eta_n[q.x], m.depth[q.y] + s.eta_n[q.y],
+                                                  m.depth[q.z] + s.eta_n[q.z], m.depth[q.w] + s.eta_n[q.w]));
+        double su = 0.0, sv = 0.0;
+        for (int k = 0; k < L; ++k) {
+            double ds = m.dsig[k];
+            double ak = sum4(w, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
+            su = su + tU[c * KP + k] * ak;
+            sv = sv + tV[c * KP + k] * ak;
+        }
+        const double uf = (s.U_filt[e] * Mt_inv) * dmi;
+        const double vf = (s.V_filt[e] * Mt_inv) * dmi;
+        s.U_filt[e] = uf;
+        s.V_filt[e] = vf;
+        sc[0 * kColB + c] = su * dmi;
+        sc[1 * kColB + c] = sv * dmi;
+        sc[2 * kColB + c] = uf;
+        sc[3 * kColB + c] = vf;
+    }
+    __syncthreads();
+    if (tcol() >= nc) return;
+    const int c = tcol(), e = c0 + c;
     const int4 q = m.en[e];
     const double4 w = m.wcv[e];
     const double o0 = s.a_old[q.x], o1 = s.a_old[q.y], o2 = s.a_old[q.z], o3 = s.a_old[q.w];
-    const double dmi = 1.0 / dmaxr(Dmin, sum4(w, m.depth[q.x] + s.eta_n[q.x], m.depth[q.y] + s.eta_n[q.y],
-                                              m.depth[q.z] + s.eta_n[q.z], m.depth[q.w] + s.eta_n[q.w]));
-    double su = 0.0, sv = 0.0;
-    for (int k = 0; k < L; ++k) {
-        double ds = m.dsig[k];
-        double ak = sum4(w, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
-        su = su + PL(s.U_rhs, k, e, E) * ak;
-        sv = sv + PL(s.V_rhs, k, e, E) * ak;
-    }
-    const double U2p = su * dmi, V2p = sv * dmi;
-    const double uf = (s.U_filt[e] * Mt_inv) * dmi;
-    const double vf = (s.V_filt[e] * Mt_inv) * dmi;
-    s.U_filt[e] = uf;
-    s.V_filt[e] = vf;
+    const double U2p = sc[c], V2p = sc[kColB + c], uf = sc[2 * kColB + c], vf = sc[3 * kColB + c];
     const double u2 = s.U2[2 * e], v2 = s.U2[2 * e + 1];
-    for (int k = 0; k < L; ++k) {
-        double ds = m.dsig[k];
+    for (int k = tsub(); k < L; k += 4) {
+        const double ds = m.dsig[k];
         double ak = sum4(w, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
-        ak = ak / PL(s.Je, k, e, E);
-        double ur = PL(s.U_rhs, k, e, E), vr = PL(s.V_rhs, k, e, E);
-        PL(s.U_nf, k, e, E) = ak * ur + uf - U2p;
-        PL(s.V_nf, k, e, E) = ak * vr + vf - V2p;
-        PL(s.U_n, k, e, E) = ak * ur + u2 - U2p;
-        PL(s.V_n, k, e, E) = ak * vr + v2 - V2p;
+        ak = ak / PL(s.Je, k, e, KC);
+        const double ur = tU[c * KP + k], vr = tV[c * KP + k];
+        PL(s.U_nf, k, e, KC) = ak * ur + uf - U2p;
+        PL(s.V_nf, k, e, KC) = ak * vr + vf - V2p;
+        PL(s.U_n, k, e, KC) = ak * ur + u2 - U2p;
+        PL(s.V_n, k, e, KC) = ak * vr + v2 - V2p;
     }
 }
 
@@ -961,8 +1178,9 @@ __global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, dou
 // gathered over the node's edges in ascending id (354-437), stored in W.
 template <typename R>
 __global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevStateT<R> s) {
+    const int KC = m.K;
     int n, k;
-    if (!node_level(m.N, m.L, n, k)) return;
+    if (!ent_layer(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
     R c = R(0);  // storage type: FP32 transport fluxes in the mixed-precision path
@@ -972,31 +1190,41 @@ __global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevStateT<R> s) 
         const int2 et = m.edtri[ed];
         const double4 cd = m.ecross[ed];
         const int e1 = et.x;
-        R f = PL(s.Je, k, e1, E) * ((PL(s.V_nf, k, e1, E) - (R)s.V_filt[e1]) * (R)cd.x
-                                    - (PL(s.U_nf, k, e1, E) - (R)s.U_filt[e1]) * (R)cd.y);
+        R f = PL(s.Je, k, e1, KC) * ((PL(s.V_nf, k, e1, KC) - (R)s.V_filt[e1]) * (R)cd.x
+                                    - (PL(s.U_nf, k, e1, KC) - (R)s.U_filt[e1]) * (R)cd.y);
         if (et.y >= 0) {
             const int e2 = et.y;
-            f = f + PL(s.Je, k, e2, E) * (-(PL(s.V_nf, k, e2, E) - (R)s.V_filt[e2]) * (R)cd.z
-                                          + (PL(s.U_nf, k, e2, E) - (R)s.U_filt[e2]) * (R)cd.w);
+            f = f + PL(s.Je, k, e2, KC) * (-(PL(s.V_nf, k, e2, KC) - (R)s.V_filt[e2]) * (R)cd.z
+                                          + (PL(s.U_nf, k, e2, KC) - (R)s.U_filt[e2]) * (R)cd.w);
         }
         c = (cc >= 0) ? c + f : c - f;
     }
-    PL(s.W, k, n, N) = c;
+    PL(s.W, k, n, KC) = c;
 }
 
-// second part: bottom-up cumulative sum and /area (444-450); Wvel(nsigma)
-// stays zero.
+// second part: bottom-up cumulative sum and /area (444-450), one lane per
+// column of a staged tile; Wvel(nsigma) stays zero.
 template <typename R>
-__global__ __launch_bounds__(256) void k_vertvel_scan(DevMesh m, DevStateT<R> s) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int N = m.N;
-    const double area_inv = 1.0 / m.area[n];
-    double run = 0.0;
-    for (int k = m.L - 1; k >= 0; --k) {
-        run = PL(s.W, k, n, N) + run;
-        PL(s.W, k, n, N) = run * area_inv;
+__global__ __launch_bounds__(kColT) void k_vertvel_scan(DevMesh m, DevStateT<R> s) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int K = m.K, L = m.L, KP = K + 1;
+    R *tW = (R *)lds_;
+    const int c0 = col_first(), nc = min(kColB, m.N - c0);
+    if (nc <= 0) return;
+    tile_load(tW, s.W, c0, nc, K);
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x;
+        R *W = tW + c * KP;
+        const double area_inv = 1.0 / m.area[c0 + c];
+        double run = 0.0;
+        for (int k = L - 1; k >= 0; --k) {
+            run = W[k] + run;
+            W[k] = run * area_inv;
+        }
     }
+    __syncthreads();
+    tile_store(s.W, tW, c0, nc, K, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1014,8 +1242,8 @@ __global__ __launch_bounds__(256) void k_minmax_partial(DevMesh m, const R *__re
     const size_t tot = (size_t)cnt * m.L;
     double tmn = INFINITY, tmx = -INFINITY, smn = INFINITY, smx = -INFINITY;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = i / cnt, n = n0 + i % cnt;
-        const double t = T[k * m.N + n], v = S[k * m.N + n];
+        const size_t n = n0 + i / m.L, k = i % m.L;
+        const double t = T[n * m.K + k], v = S[n * m.K + k];
         tmn = fmin(tmn, t);
         tmx = fmax(tmx, t);
         smn = fmin(smn, v);
@@ -1055,12 +1283,13 @@ __global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb
     tsa[2 * mb + 1] = 0.5 * (smn + smx);
 }
 
-// AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444)
+// AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444), over every
+// slot of the tracer columns (the unused last slot stays finite)
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_ab(DevMesh m, DevStateT<R> s, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int mb = m.memb[i % m.N];
+    const int mb = m.memb[i / (size_t)m.K];
     const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
     s.abT[i] = -(0.5 + kEps) * s.T_old[i] + (1.5 + kEps) * s.TF[i] - Ta;
     s.abS[i] = -(0.5 + kEps) * s.S_old[i] + (1.5 + kEps) * s.SF[i] - Sa;
@@ -1071,9 +1300,10 @@ __global__ __launch_bounds__(256) void k_tr_ab(DevMesh m, DevStateT<R> s, size_t
 template <typename R>
 __device__ __forceinline__ double upw_vert(const DevMesh &m, const DevStateT<R> &s, const R *X, int n, int nz,
                                            double a) {
+    const int KC = m.K;
     const int N = m.N, K = m.K;
-    auto Wv = [&](int z) { return PL(s.W, z - 1, n, N); };
-    auto T = [&](int z) { return PL(X, z - 1, n, N); };
+    auto Wv = [&](int z) { return PL(s.W, z - 1, n, KC); };
+    auto T = [&](int z) { return PL(X, z - 1, n, KC); };
     auto zb = [&](int z) { return m.oms[z - 1] * a; };
     auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
     if (nz == 1) return -Wv(1) * T(1);
@@ -1103,17 +1333,21 @@ __device__ __forceinline__ double upw_vert(const DevMesh &m, const DevStateT<R> 
 }
 
 // tracer_impl_vert_diff (fv_tracer.F90:1319-1494, key_atm off) of one node
-// column, applied to TF/SF in place, followed by the STOP checks of
-// fv_tracer.F90:1466-1485 (reported through the device error flag).
+// column held in LDS (T, S, Kv columns), applied to T/S in place, followed by
+// the STOP checks of fv_tracer.F90:1466-1485 (reported through the device
+// error flag).  The sweep keeps cp in the Kv slots it has consumed (cp(nz) in
+// slot nz-1: the sweep reads Kv(2) at nz = 1 and Kv(nz) at nz >= 2, and never
+// Kv(1)) and tp/sp in the columns Tp/Sp, each rounded to the storage type as
+// the reference-order scratch arrays are.
 template <typename R>
-__device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R> &s, int n, double a, double Ta,
-                                             double Sa, double dt) {
-    const int N = m.N, L = m.L, K = m.K;
+__device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R> &s, R *Tc, R *Sc, R *Kc, R *Tp,
+                                             R *Sp, double a, double Ta, double Sa, double dt) {
+    const int L = m.L, K = m.K;
     auto zb = [&](int z) { return m.oms[z - 1] * a; };
     auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
-    auto KV = [&](int z) { return PL(s.Kv, z - 1, n, N); };
-    auto Td = [&](int z) { return PL(s.TF, z - 1, n, N) - Ta; };
-    auto Sd = [&](int z) { return PL(s.SF, z - 1, n, N) - Sa; };
+    auto KV = [&](int z) { return Kc[z - 1]; };
+    auto Td = [&](int z) { return Tc[z - 1] - Ta; };
+    auto Sd = [&](int z) { return Sc[z - 1] - Sa; };
     double cp = 0, tp = 0, sp = 0;
     for (int nz = 1; nz <= L; ++nz) {
         double aa, bb, cc;
@@ -1151,25 +1385,25 @@ __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R
             tp = (tr - tp * aa) * mm;
             sp = (sr - sp * aa) * mm;
         }
-        PL(s.tcp, nz - 1, n, N) = cp;
-        PL(s.ttp, nz - 1, n, N) = tp;
-        PL(s.tsp, nz - 1, n, N) = sp;
+        Kc[nz - 1] = (R)cp;
+        Tp[nz - 1] = (R)tp;
+        Sp[nz - 1] = (R)sp;
     }
     double trn = tp, srn = sp;
     int bad = 0;
     for (int nz = L; nz >= 1; --nz) {
         if (nz < L) {
-            double c = PL(s.tcp, nz - 1, n, N);
-            trn = PL(s.ttp, nz - 1, n, N) - c * trn;
-            srn = PL(s.tsp, nz - 1, n, N) - c * srn;
+            double c = Kc[nz - 1];
+            trn = Tp[nz - 1] - c * trn;
+            srn = Sp[nz - 1] - c * srn;
         }
         double tnew = Td(nz) + trn + Ta;
         double snew = Sd(nz) + srn + Sa;
         if (fabs(snew) < 1.e-12) snew = fabs(snew);
         if (snew < 0.0) bad = bad ? bad : FCG_E_SALINITY;
         if (tnew < -3.0) bad = bad ? bad : FCG_E_TEMP;
-        PL(s.TF, nz - 1, n, N) = tnew;
-        PL(s.SF, nz - 1, n, N) = snew;
+        Tc[nz - 1] = tnew;
+        Sc[nz - 1] = snew;
     }
     if (bad) atomicCAS(s.err, 0, bad);
 }
@@ -1206,9 +1440,10 @@ __device__ __forceinline__ double ob_inflow(const DevMesh &m, const DevStateT<R>
 template <bool kChained, typename R>
 __device__ __forceinline__ double hi_vert(const DevMesh &m, const DevStateT<R> &s, const R *X, double sub, int n,
                                           int nz, double a, double &Tm) {
+    const int KC = m.K;
     const int N = m.N, K = m.K;
-    auto Wv = [&](int z) { return PL(s.W, z - 1, n, N); };
-    auto T = [&](int z) { return PL(X, z - 1, n, N) - sub; };
+    auto Wv = [&](int z) { return PL(s.W, z - 1, n, KC); };
+    auto T = [&](int z) { return PL(X, z - 1, n, KC) - sub; };
     auto zb = [&](int z) { return m.oms[z - 1] * a; };
     auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
     if (nz == 1) return -Wv(1) * T(1);
@@ -1255,10 +1490,11 @@ __device__ __forceinline__ double hi_vert(const DevMesh &m, const DevStateT<R> &
 // fv_gradients.F90:180-205), the column value being PL(X) - sub.
 template <typename R>
 __device__ __forceinline__ double2 elem_grad(const DevMesh &m, const R *X, double sub, int el, int k) {
+    const int KC = m.K;
     const int N = m.N;
     const int4 q = m.en[el];
-    const double v0 = PL(X, k, q.x, N) - sub, v1 = PL(X, k, q.y, N) - sub;
-    const double v2 = PL(X, k, q.z, N) - sub, v3 = PL(X, k, q.w, N) - sub;
+    const double v0 = PL(X, k, q.x, KC) - sub, v1 = PL(X, k, q.y, KC) - sub;
+    const double v2 = PL(X, k, q.z, KC) - sub, v3 = PL(X, k, q.w, KC) - sub;
     return make_double2(sum4(m.gsx[el], v0, v1, v2, v3), sum4(m.gsy[el], v0, v1, v2, v3));
 }
 
@@ -1269,8 +1505,9 @@ __device__ __forceinline__ double2 elem_grad(const DevMesh &m, const R *X, doubl
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax, double clim_relax,
                             double relax2clim_ac) {
+    const int KC = m.K;
     int n, k;
-    if (!node_level(m.N, m.L, n, k)) return;
+    if (!ent_layer(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
     const int mb = m.memb[n];
     const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
@@ -1285,14 +1522,14 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
         const int2 et = m.edtri[ed];
         const double4 cd = m.ecross[ed];
         const int e1 = et.x;
-        const double c1 = PL(s.V_nf, k, e1, E) * cd.x - PL(s.U_nf, k, e1, E) * cd.y;
-        const double T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
-        const double S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
+        const double c1 = PL(s.V_nf, k, e1, KC) * cd.x - PL(s.U_nf, k, e1, KC) * cd.y;
+        const double T1 = PL(s.abT, k, nd.x, KC), T2 = PL(s.abT, k, nd.y, KC);
+        const double S1 = PL(s.abS, k, nd.x, KC), S2 = PL(s.abS, k, nd.y, KC);
         double auxc, taux_, saux_;
         if (et.y >= 0) {
             const int e2 = et.y;
-            const double c2 = -PL(s.V_nf, k, e2, E) * cd.z + PL(s.U_nf, k, e2, E) * cd.w;
-            const double J1 = PL(s.Je, k, e1, E), J2 = PL(s.Je, k, e2, E);
+            const double c2 = -PL(s.V_nf, k, e2, KC) * cd.z + PL(s.U_nf, k, e2, KC) * cd.w;
+            const double J1 = PL(s.Je, k, e1, KC), J2 = PL(s.Je, k, e2, KC);
             auxc = J1 * c1 + J2 * c2;
             if (c1 > 0.0 && c2 > 0.0) {
                 taux_ = J1 * c1 * T2 + J2 * c2 * T2;
@@ -1308,7 +1545,7 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
                 saux_ = J1 * c1 * S1 + J2 * c2 * S1;
             }
         } else {
-            auxc = PL(s.Je, k, e1, E) * c1;
+            auxc = PL(s.Je, k, e1, KC) * c1;
             if (c1 > 0.0) {
                 taux_ = auxc * T2;
                 saux_ = auxc * S2;
@@ -1329,26 +1566,26 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
     }
     const double tvt = upw_vert(m, s, s.abT, n, k + 1, a), tvb = upw_vert(m, s, s.abT, n, k + 2, a);
     const double svt = upw_vert(m, s, s.abS, n, k + 1, a), svb = upw_vert(m, s, s.abS, n, k + 2, a);
-    const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
-    const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+    const double tab = PL(s.abT, k, n, KC), sab = PL(s.abS, k, n, KC);
+    const double wdiff = PL(s.W, k, n, KC) - PL(s.W, k + 1, n, KC);
     const double jc = a * m.dsig[k];
     const double area_inv = 1.0 / m.area[n];
     double rt = ((tt - tab * tc) * area_inv + tab * wdiff + (tvt - tvb)) / jc;
     double rs = ((ts - sab * tc) * area_inv + sab * wdiff + (svt - svb)) / jc;
     if (ob) {  // fv_tracer.F90:832-895, on the AB-interpolated tracer
         if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) {
-            rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tab + Ta));
-            rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sab + Sa));
+            rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, KC) - (tab + Ta));
+            rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, KC) - (sab + Sa));
         } else {
-            rt = clim_relax * (PL(s.Tclim, k, n, N) - (tab + Ta));
-            rs = clim_relax * (PL(s.Sclim, k, n, N) - (sab + Sa));
+            rt = clim_relax * (PL(s.Tclim, k, n, KC) - (tab + Ta));
+            rs = clim_relax * (PL(s.Sclim, k, n, KC) - (sab + Sa));
         }
     }
-    const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
-    PL(s.T_old, k, n, N) = tdm + Ta;
-    PL(s.S_old, k, n, N) = sdm + Sa;
-    PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
-    PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
+    const double tdm = PL(s.TF, k, n, KC) - Ta, sdm = PL(s.SF, k, n, KC) - Sa;
+    PL(s.T_old, k, n, KC) = tdm + Ta;
+    PL(s.S_old, k, n, KC) = sdm + Sa;
+    PL(s.TF, k, n, KC) = tdm + rt * dt + Ta;
+    PL(s.SF, k, n, KC) = sdm + rs * dt + Sa;
 }
 
 // Node-averaged gradients of the AB-interpolated T and S used by the MUSCL
@@ -1357,8 +1594,9 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
 // ascending element id, divided by the summed area.
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> s) {
+    const int KC = m.K;
     int n, k;
-    if (!node_level(m.N, m.L, n, k)) return;
+    if (!ent_layer(m.N, m.L, n, k)) return;
     const int N = m.N;
     const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
     double tvol = 0.0, tx = 0.0, ty = 0.0, sx = 0.0, sy = 0.0;
@@ -1373,10 +1611,10 @@ __global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> 
         sx = sx + gs.x * ea;
         sy = sy + gs.y * ea;
     }
-    PL(s.gTx, k, n, N) = tx / tvol;
-    PL(s.gTy, k, n, N) = ty / tvol;
-    PL(s.gSx, k, n, N) = sx / tvol;
-    PL(s.gSy, k, n, N) = sy / tvol;
+    PL(s.gTx, k, n, KC) = tx / tvol;
+    PL(s.gTy, k, n, KC) = ty / tvol;
+    PL(s.gSx, k, n, KC) = sx / tvol;
+    PL(s.gSy, k, n, KC) = sy / tvol;
 }
 
 // Node-averaged gradient of the current de-meaned T for the Miura scheme
@@ -1385,8 +1623,9 @@ __global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> 
 // overwrites TF while neighbouring nodes still read it.
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevStateT<R> s) {
+    const int KC = m.K;
     int n, k;
-    if (!node_level(m.N, m.L, n, k)) return;
+    if (!ent_layer(m.N, m.L, n, k)) return;
     const int N = m.N;
     const double Ta = s.tsa[2 * m.memb[n]];
     const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
@@ -1401,9 +1640,9 @@ __global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevStateT<R> 
         tx = tx + g.x * ea;
         ty = ty + g.y * ea;
     }
-    PL(s.gTx, k, n, N) = tx * tvol;
-    PL(s.gTy, k, n, N) = ty * tvol;
-    PL(s.tcur, k, n, N) = PL(s.TF, k, n, N) - Ta;
+    PL(s.gTx, k, n, KC) = tx * tvol;
+    PL(s.gTy, k, n, KC) = ty * tvol;
+    PL(s.tcur, k, n, KC) = PL(s.TF, k, n, KC) - Ta;
 }
 
 // MUSCL edge value: linear reconstructions from both end nodes, 15 %
@@ -1423,8 +1662,9 @@ __device__ __forceinline__ R muscl_mean(R T1, R T2, R dT, R dx, R dy, R g1x, R g
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax, double clim_relax,
                            double relax2clim_ac) {
+    const int KC = m.K;
     int n, k;
-    if (!node_level(m.N, m.L, n, k)) return;
+    if (!ent_layer(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
     const int mb = m.memb[n];
     const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
@@ -1441,16 +1681,16 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
         const double2 ddd = m.edxdy[ed];
         const R cdx = (R)cdd.x, cdy = (R)cdd.y, cdz = (R)cdd.z, cdw = (R)cdd.w, ddy = (R)ddd.y;
         const int e1 = et.x;
-        const R T1 = PL(s.abT, k, nd.x, N), T2 = PL(s.abT, k, nd.y, N);
-        const R S1 = PL(s.abS, k, nd.x, N), S2 = PL(s.abS, k, nd.y, N);
-        const R gT1x = PL(s.gTx, k, nd.x, N), gT2x = PL(s.gTx, k, nd.y, N);
-        const R gT1y = PL(s.gTy, k, nd.x, N), gT2y = PL(s.gTy, k, nd.y, N);
-        const R gS1x = PL(s.gSx, k, nd.x, N), gS2x = PL(s.gSx, k, nd.y, N);
-        const R gS1y = PL(s.gSy, k, nd.x, N), gS2y = PL(s.gSy, k, nd.y, N);
+        const R T1 = PL(s.abT, k, nd.x, KC), T2 = PL(s.abT, k, nd.y, KC);
+        const R S1 = PL(s.abS, k, nd.x, KC), S2 = PL(s.abS, k, nd.y, KC);
+        const R gT1x = PL(s.gTx, k, nd.x, KC), gT2x = PL(s.gTx, k, nd.y, KC);
+        const R gT1y = PL(s.gTy, k, nd.x, KC), gT2y = PL(s.gTy, k, nd.y, KC);
+        const R gS1x = PL(s.gSx, k, nd.x, KC), gS2x = PL(s.gSx, k, nd.y, KC);
+        const R gS1y = PL(s.gSy, k, nd.x, KC), gS2y = PL(s.gSy, k, nd.y, KC);
         const R dTt = R(2.0) * (T2 - T1), dTs = R(2.0) * (S2 - S1);
         // first segment
         const R dxa = (R)(ddd.x * (kREarthD * m.elem_cos[e1]));
-        const R u1 = PL(s.U_nf, k, e1, E), v1 = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
+        const R u1 = PL(s.U_nf, k, e1, KC), v1 = PL(s.V_nf, k, e1, KC), J1 = PL(s.Je, k, e1, KC);
         const bool f2 = v1 * cdx - u1 * cdy > R(0);
         const R mT = muscl_mean(T1, T2, dTt, dxa, ddy, gT1x, gT1y, gT2x, gT2y, f2);
         const R mS = muscl_mean(S1, S2, dTs, dxa, ddy, gS1x, gS1y, gS2x, gS2y, f2);
@@ -1467,8 +1707,8 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
         if (et.y >= 0) {  // second segment
             const int e2 = et.y;
             const R dxb = (R)(ddd.x * (kREarthD * m.elem_cos[e2]));
-            const R u2f = PL(s.U_nf, k, e2, E), v2f = PL(s.V_nf, k, e2, E);
-            const R u2 = PL(s.U_n, k, e2, E), v2 = PL(s.V_n, k, e2, E), J2 = PL(s.Je, k, e2, E);
+            const R u2f = PL(s.U_nf, k, e2, KC), v2f = PL(s.V_nf, k, e2, KC);
+            const R u2 = PL(s.U_n, k, e2, KC), v2 = PL(s.V_n, k, e2, KC), J2 = PL(s.Je, k, e2, KC);
             const bool g2 = v2f * cdz - u2f * cdw < R(0);
             const R nT = muscl_mean(T1, T2, dTt, dxb, ddy, gT1x, gT1y, gT2x, gT2y, g2);
             const R nS = muscl_mean(S1, S2, dTs, dxb, ddy, gS1x, gS1y, gS2x, gS2y, g2);
@@ -1498,26 +1738,26 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
     const double tvb = hi_vert<true>(m, s, s.abT, 0.0, n, k + 2, a, TmT);
     const double svt = hi_vert<true>(m, s, s.abS, 0.0, n, k + 1, a, TmS);
     const double svb = hi_vert<true>(m, s, s.abS, 0.0, n, k + 2, a, TmS);
-    const double tab = PL(s.abT, k, n, N), sab = PL(s.abS, k, n, N);
-    const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+    const double tab = PL(s.abT, k, n, KC), sab = PL(s.abS, k, n, KC);
+    const double wdiff = PL(s.W, k, n, KC) - PL(s.W, k + 1, n, KC);
     const double jc = a * m.dsig[k];
     const double aj = m.area[n] * jc;
     rt = rt / aj - tab * rc / aj + tab * wdiff / jc + (tvt - tvb) / jc;
     rs = rs / aj - sab * rc / aj + sab * wdiff / jc + (svt - svb) / jc;
-    const double tdm = PL(s.TF, k, n, N) - Ta, sdm = PL(s.SF, k, n, N) - Sa;
+    const double tdm = PL(s.TF, k, n, KC) - Ta, sdm = PL(s.SF, k, n, KC) - Sa;
     if (cl_relax && m.index[n] == 2) {  // climatology relaxation on the current de-meaned tracer
         if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) {
-            rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
-            rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, N) - (sdm + Sa));
+            rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, KC) - (tdm + Ta));
+            rs = rs + relax2clim_ac * (PL(s.Sclim, k, n, KC) - (sdm + Sa));
         } else {
-            rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
-            rs = clim_relax * (PL(s.Sclim, k, n, N) - (sdm + Sa));
+            rt = clim_relax * (PL(s.Tclim, k, n, KC) - (tdm + Ta));
+            rs = clim_relax * (PL(s.Sclim, k, n, KC) - (sdm + Sa));
         }
     }
-    PL(s.T_old, k, n, N) = tdm + Ta;
-    PL(s.S_old, k, n, N) = sdm + Sa;
-    PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
-    PL(s.SF, k, n, N) = sdm + rs * dt + Sa;
+    PL(s.T_old, k, n, KC) = tdm + Ta;
+    PL(s.S_old, k, n, KC) = sdm + Sa;
+    PL(s.TF, k, n, KC) = tdm + rt * dt + Ta;
+    PL(s.SF, k, n, KC) = sdm + rs * dt + Sa;
 }
 
 // mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
@@ -1538,8 +1778,9 @@ __device__ __forceinline__ double mask_ad(const DevMesh &m, const DevStateT<R> &
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, double dt, double Dmin, double scale_area, int cl_relax,
                            double clim_relax, double relax2clim_ac) {
+    const int KC = m.K;
     int n, k;
-    if (!node_level(m.N, m.L, n, k)) return;
+    if (!ent_layer(m.N, m.L, n, k)) return;
     const int N = m.N, E = m.E;
     const double Ta = s.tsa[2 * m.memb[n]];
     const double a = s.a[n];
@@ -1558,9 +1799,9 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
         double Kh = m.elem_area[e1];
         if (e2 >= 0) Kh = 0.5 * (Kh + m.elem_area[e2]);
         Kh = 0.010 * Kh / scale_area;
-        const double T1 = PL(s.tcur, k, nd.x, N), T2 = PL(s.tcur, k, nd.y, N);
-        const double tx1 = PL(s.gTx, k, nd.x, N), tx2 = PL(s.gTx, k, nd.y, N);
-        const double ty1 = PL(s.gTy, k, nd.x, N), ty2 = PL(s.gTy, k, nd.y, N);
+        const double T1 = PL(s.tcur, k, nd.x, KC), T2 = PL(s.tcur, k, nd.y, KC);
+        const double tx1 = PL(s.gTx, k, nd.x, KC), tx2 = PL(s.gTx, k, nd.y, KC);
+        const double ty1 = PL(s.gTy, k, nd.x, KC), ty2 = PL(s.gTy, k, nd.y, KC);
         const double2 g1 = elem_grad(m, s.tcur, 0.0, e1, k);
         double Tx, Ty;
         double2 g2 = make_double2(0.0, 0.0);
@@ -1573,7 +1814,7 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
             Ty = Kh * g1.y;
         }
         const double dX1 = cd.x, dY1 = cd.y;
-        const double u = PL(s.U_nf, k, e1, E), v = PL(s.V_nf, k, e1, E), J1 = PL(s.Je, k, e1, E);
+        const double u = PL(s.U_nf, k, e1, KC), v = PL(s.V_nf, k, e1, KC), J1 = PL(s.Je, k, e1, KC);
         double Tm;
         if (v * dX1 - u * dY1 > 0) {
             Tm = T2 + (0.5 * (-dd.x * ab + dX1 - u * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY1 - v) * ty2) * crt1;
@@ -1586,7 +1827,7 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
         if (e2 >= 0) {
             const double dX2 = cd.z, dY2 = cd.w;
             const double Tx2 = 0.5 * Kh * (g1.x + g2.x), Ty2 = 0.5 * Kh * (g1.y + g2.y);
-            const double u2 = PL(s.U_nf, k, e2, E), v2 = PL(s.V_nf, k, e2, E), J2 = PL(s.Je, k, e2, E);
+            const double u2 = PL(s.U_nf, k, e2, KC), v2 = PL(s.V_nf, k, e2, KC), J2 = PL(s.Je, k, e2, KC);
             double Tn;
             if (v2 * dX2 - u2 * dY2 < 0) {
                 Tn = T2 + (0.5 * (-dd.x * ab + dX2 - u2 * dt) * tx2 + 0.5 * (-dd.y * kREarthD + dY2 - v2 * dt) * ty2) *
@@ -1613,30 +1854,43 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
     double Tmv = 0.0;
     const double tvt = hi_vert<false>(m, s, s.tcur, 0.0, n, k + 1, a, Tmv);
     const double tvb = hi_vert<false>(m, s, s.tcur, 0.0, n, k + 2, a, Tmv);
-    const double tab = PL(s.abT, k, n, N);
-    const double wdiff = PL(s.W, k, n, N) - PL(s.W, k + 1, n, N);
+    const double tab = PL(s.abT, k, n, KC);
+    const double wdiff = PL(s.W, k, n, KC) - PL(s.W, k + 1, n, KC);
     const double jc = a * m.dsig[k];
     rt = ((rt - tab * rc) / m.area[n] + tab * wdiff + tvt - tvb) / jc;
-    const double tdm = PL(s.tcur, k, n, N);
+    const double tdm = PL(s.tcur, k, n, KC);
     if (cl_relax && m.index[n] == 2) {
-        if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, N) - (tdm + Ta));
-        else rt = clim_relax * (PL(s.Tclim, k, n, N) - (tdm + Ta));
+        if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) rt = rt + relax2clim_ac * (PL(s.Tclim, k, n, KC) - (tdm + Ta));
+        else rt = clim_relax * (PL(s.Tclim, k, n, KC) - (tdm + Ta));
     }
-    PL(s.T_old, k, n, N) = tdm + Ta;
-    PL(s.TF, k, n, N) = tdm + rt * dt + Ta;
+    PL(s.T_old, k, n, KC) = tdm + Ta;
+    PL(s.TF, k, n, KC) = tdm + rt * dt + Ta;
 }
 
-// tracer_impl_vert_diff of T and S (fv_tracer.F90:1319-1494), one thread per
-// node column; after every advection scheme, and alone for tracer_adv = 2
+// tracer_impl_vert_diff of T and S (fv_tracer.F90:1319-1494) on tiles of 64
+// node columns, after every advection scheme, and alone for tracer_adv = 2
 // without MUSCL_CALL (fv_main.F90:1788-1800).
 template <typename R>
-__global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt) {
-    int n = TID();
-    if (n >= m.N) return;
-    const int mb = m.memb[n];
-    vdiff_column(m, s, n, s.a[n], s.tsa[2 * mb], s.tsa[2 * mb + 1], dt);
+__global__ __launch_bounds__(kColT) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int K = m.K, L = m.L, KP = K + 1;
+    R *tT = (R *)lds_, *tS = tT + kColB * KP, *tK = tS + kColB * KP, *tTp = tK + kColB * KP, *tSp = tTp + kColB * KP;
+    const int c0 = col_first(), nc = min(kColB, m.N - c0);
+    if (nc <= 0) return;
+    tile_load(tT, s.TF, c0, nc, K);
+    tile_load(tS, s.SF, c0, nc, K);
+    tile_load(tK, s.Kv, c0, nc, K);
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x, n = c0 + c;
+        const int mb = m.memb[n];
+        vdiff_column(m, s, tT + c * KP, tS + c * KP, tK + c * KP, tTp + c * KP, tSp + c * KP, s.a[n], s.tsa[2 * mb],
+                     s.tsa[2 * mb + 1], dt);
+    }
+    __syncthreads();
+    tile_store(s.TF, tT, c0, nc, K, L);
+    tile_store(s.SF, tS, c0, nc, K, L);
 }
-
 
 // ---------------------------------------------------------------------------
 // learned closure: features, training samples, fused inference
@@ -1649,12 +1903,13 @@ __global__ __launch_bounds__(256) void k_tr_vdiff(DevMesh m, DevStateT<R> s, dou
 template <typename R>
 __device__ void column_features(const DevMesh &m, const DevStateT<R> &s, int n, int nz, double Dmin, double rho0,
                                 double z0b_min, float *f) {
+    const int KC = m.K;
     const int N = m.N, K = m.K, L = m.L;
     const double a = s.a[n];
     const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
     auto uv = [&](int k, double &u, double &v) {  // node averages of compute_el2nodes_3D
-        u = PL(s.Unode, k, n, N);
-        v = PL(s.Vnode, k, n, N);
+        u = PL(s.Unode, k, n, KC);
+        v = PL(s.Vnode, k, n, KC);
     };
     const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
     double u1, v1, u2, v2, ub, vb;
@@ -1665,8 +1920,8 @@ __device__ void column_features(const DevMesh &m, const DevStateT<R> &s, int n, 
     const double d = 0.5 * (d1 + d2);
     const double sh2 = ((u1 - u2) * (u1 - u2) + (v1 - v2) * (v1 - v2)) / (d * d) / (dmean * dmean);
     const double zb = m.oms[iz - 1] * a;
-    const double rho_up = densityJM(PL(s.TF, iz - 2, n, N), PL(s.SF, iz - 2, n, N), -zb, rho0);
-    const double rho_dn = densityJM(PL(s.TF, iz - 1, n, N), PL(s.SF, iz - 1, n, N), -zb, rho0);
+    const double rho_up = densityJM(PL(s.TF, iz - 2, n, KC), PL(s.SF, iz - 2, n, KC), -zb, rho0);
+    const double rho_dn = densityJM(PL(s.TF, iz - 1, n, KC), PL(s.SF, iz - 1, n, KC), -zb, rho0);
     const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
     const double delta_b = a * m.dsig[L - 1] + z0b_min;
     const double lg = log(delta_b / z0b_min);
@@ -1684,6 +1939,7 @@ __device__ void column_features(const DevMesh &m, const DevStateT<R> &s, int n, 
 template <typename R>
 __global__ __launch_bounds__(256) void k_mlp_collect(DevMesh m, DevStateT<R> s, double Dmin, double rho0,
                                                      double z0b_min, int stride, long cap, float *X, float *T) {
+    const int KC = m.K;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int K = m.K;
     const long col = i / K;
@@ -1693,8 +1949,8 @@ __global__ __launch_bounds__(256) void k_mlp_collect(DevMesh m, DevStateT<R> s, 
     float f[kMlpF];
     column_features(m, s, (int)n, nz, Dmin, rho0, z0b_min, f);
     for (int j = 0; j < kMlpF; ++j) X[i * kMlpF + j] = f[j];
-    T[i * 2] = (float)log10((double)PL(s.snu, nz - 1, n, m.N));
-    T[i * 2 + 1] = (float)log10((double)PL(s.Kv, nz - 1, n, m.N));
+    T[i * 2] = (float)log10((double)PL(s.snu, nz - 1, n, KC));
+    T[i * 2 + 1] = (float)log10((double)PL(s.Kv, nz - 1, n, KC));
 }
 
 // The closure step, one lane per node column (64-thread blocks, one wave):
@@ -1710,6 +1966,7 @@ template <typename R>
 __global__ __launch_bounds__(64) void k_mlp_closure(DevMesh m, DevStateT<R> s, const float *__restrict__ w,
                                                     const fcgm::bf16x4 *__restrict__ frag, MlpNorm nm, double Dmin,
                                                     double rho0, double z0b_min, double snul, int *flag) {
+    const int KC = m.K;
     __shared__ float xs[64 * kMlpF];
     __shared__ float ys[64 * 2];
     const int lane = threadIdx.x;
@@ -1718,8 +1975,8 @@ __global__ __launch_bounds__(64) void k_mlp_closure(DevMesh m, DevStateT<R> s, c
     const int N = m.N, K = m.K, L = m.L;
     double a = 0.0, dmean = 1.0, f3 = 0.0;
     auto uv = [&](int k, double &u, double &v) {  // node averages of compute_el2nodes_3D
-        u = PL(s.Unode, k, n, N);
-        v = PL(s.Vnode, k, n, N);
+        u = PL(s.Unode, k, n, KC);
+        v = PL(s.Vnode, k, n, KC);
     };
     double u_up = 0.0, v_up = 0.0, u_dn = 0.0, v_dn = 0.0;
     if (valid) {
@@ -1750,8 +2007,8 @@ __global__ __launch_bounds__(64) void k_mlp_closure(DevMesh m, DevStateT<R> s, c
             const double sh2 = ((u_up - u_dn) * (u_up - u_dn) + (v_up - v_dn) * (v_up - v_dn)) / (d * d) /
                                (dmean * dmean);
             const double zb = m.oms[iz - 1] * a;
-            const double rho_up = densityJM(PL(s.TF, iz - 2, n, N), PL(s.SF, iz - 2, n, N), -zb, rho0);
-            const double rho_dn = densityJM(PL(s.TF, iz - 1, n, N), PL(s.SF, iz - 1, n, N), -zb, rho0);
+            const double rho_up = densityJM(PL(s.TF, iz - 2, n, KC), PL(s.SF, iz - 2, n, KC), -zb, rho0);
+            const double rho_dn = densityJM(PL(s.TF, iz - 1, n, KC), PL(s.SF, iz - 1, n, KC), -zb, rho0);
             const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
             f[0] = (float)log10(sh2 + 1e-12);
             f[1] = (float)asinh(n2 * 1e5);
@@ -1780,8 +2037,8 @@ __global__ __launch_bounds__(64) void k_mlp_closure(DevMesh m, DevStateT<R> s, c
             const double lk = (double)ys[lane * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
             double sv = pow(10.0, ls);
             sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
-            PL(s.mx_bA, nz - 1, n, N) = (R)sv;
-            PL(s.mx_bB, nz - 1, n, N) = (R)pow(10.0, lk);
+            PL(s.mx_bA, nz - 1, n, KC) = (R)sv;
+            PL(s.mx_bB, nz - 1, n, KC) = (R)pow(10.0, lk);
         }
         __syncthreads();
     }
@@ -1789,8 +2046,8 @@ __global__ __launch_bounds__(64) void k_mlp_closure(DevMesh m, DevStateT<R> s, c
     flag[n] = ood ? 1 : 0;
     if (ood) return;
     for (int k = 0; k < K; ++k) {
-        PL(s.snu, k, n, N) = PL(s.mx_bA, k, n, N);
-        PL(s.Kv, k, n, N) = PL(s.mx_bB, k, n, N);
+        PL(s.snu, k, n, KC) = PL(s.mx_bA, k, n, KC);
+        PL(s.Kv, k, n, KC) = PL(s.mx_bB, k, n, KC);
     }
 }
 
@@ -1809,6 +2066,7 @@ constexpr int kDiag = 12;
 template <typename R>
 __global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevStateT<R> s, double Dmin, int n0, int n1, int e0,
                                                       int e1, double *out) {
+    const int KC = m.K;
     __shared__ double sm[kDiag][256];
     double v[kDiag] = {0, 0, 0, 0, 0, 0, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
     // v: 0 PE, 1 area, 2 KE, 3 volume, 4 heat, 5 salt, 6/7 eta, 8/9 T, 10/11 S over nodes [n0, n1),
@@ -1825,7 +2083,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevStateT<R> s,
         double a = s.a[n];
         for (int k = 0; k < L; ++k) {
             double jc = a * m.dsig[k];
-            double t = PL(s.TF, k, n, N), sv = PL(s.SF, k, n, N);
+            double t = PL(s.TF, k, n, KC), sv = PL(s.SF, k, n, KC);
             v[4] += ar * jc * t;
             v[5] += ar * jc * sv;
             v[8] = fmin(v[8], t);
@@ -1892,7 +2150,9 @@ template <typename R>
 void instantiate_kernels() {
     (void)&k_pressure<R>;
     (void)&k_bpg<R>;
+    (void)&k_d3end_prep<R>;
     (void)&k_d3end<R>;
+    (void)&k_d3end_fin<R>;
     (void)&k_velgrad3d<R>;
     (void)&k_velrhs<R>;
     (void)&k_velvisc<R>;
