@@ -74,17 +74,26 @@ constexpr int kBlk = 256;
 inline unsigned nblk(size_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
 constexpr int kMinmaxChunks = 512;  // blocks per ensemble member in the T/S extrema reduction
 
-// Column-tile kernels: one 64-thread block per 64 columns (16 for the P1
-// advection), LDS tiles of K+1 slots per column.
-constexpr int kColB = 64, kColT = 256, kP1Cols = 16;
+// Column-tile kernels (BPG, vertical viscosity, tracer vertical diffusion): 64
+// columns per 256-thread block, LDS tiles of K+1 slots per column.
+// Column-aligned flat kernels: 256-thread blocks of cpb = 256/S whole columns
+// of S slots, one thread per (column, slot).
+constexpr int kColB = 64, kColT = 256, kFlatT = 256;
 inline unsigned ncolblk(int ents) { return (unsigned)((ents + kColB - 1) / kColB); }
 inline size_t tile_bytes(const DevMesh &m, size_t esz) { return (size_t)kColB * (m.K + 1) * esz; }
-inline size_t lds_p1adv(const DevMesh &m) { return (8 * (size_t)kP1Cols * (m.K + 1) + 8 * kP1Cols) * sizeof(double); }
+inline int cpb_of(int S) { return kFlatT / S; }
+inline unsigned nflatblk(int ents, int S) { return (unsigned)((ents + cpb_of(S) - 1) / cpb_of(S)); }
+inline size_t flat_bytes(int S, size_t esz) { return (size_t)cpb_of(S) * S * esz; }
+// k_velrhs: per column one element record and four edge records
+inline size_t velrhs_bytes(const DevMesh &m, size_t esz) {
+    return (size_t)cpb_of(m.L) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz));
+}
+inline size_t lds_p1adv(const DevMesh &m) { return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8; }
 // largest LDS request of the tile kernels for this mesh and storage type
 inline size_t lds_max(const DevMesh &m, size_t esz) {
     size_t b = std::max(2 * tile_bytes(m, 8), tile_bytes(m, 8) + 2 * tile_bytes(m, esz));  // bpg, velvisc
-    b = std::max(b, 4 * kColB * sizeof(double) + 2 * tile_bytes(m, esz));                  // update3d
-    b = std::max(b, 5 * tile_bytes(m, esz));                                               // vdiff, d3_end prep
+    b = std::max(b, 5 * tile_bytes(m, esz));                // vdiff
+    b = std::max(b, 5 * flat_bytes(m.K, esz));              // d3_end prep
     return std::max(b, lds_p1adv(m));
 }
 
@@ -214,8 +223,7 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
             hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk((size_t)m.E * m.L)), dim3(kBlk), 0, st, m, s, p.Dmin);
             hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nblk((size_t)m.N * m.L)), dim3(kBlk), 0, st, m,
                                (const R *)s.U_puls, (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
-            hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3((m.E + kP1Cols - 1) / kP1Cols), dim3(kColB), lds_p1adv(m), st,
-                               m, s);
+            hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3(nflatblk(m.E, m.L)), dim3(kFlatT), lds_p1adv(m), st, m, s);
         }
         hipLaunchKernelGGL(fcgk::k_bt_elem2<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
                            ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0);
@@ -537,7 +545,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     {
         int lds_limit = 0;
         FCG_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-        if (lds_max(m, ctx->esize) > (size_t)lds_limit) {
+        if (m.K > kFlatT || lds_max(m, ctx->esize) > (size_t)lds_limit) {
             fcg_fail(ctx, FCG_E_ARG, "nsigma too large for the LDS column tiles of this device");
             *out = ctx;
             return FCG_E_ARG;
@@ -847,7 +855,7 @@ template <typename R>
 static int pressure_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const double gd0 = kG / ctx->density_0;
     const DevMesh &m = ctx->m;
-    LAUNCH_EX(ctx, k_pressure<R>, ncolblk(m.N), kColT, tile_bytes(m, 8), m, s, gd0, ctx->density_0);
+    LAUNCH_EX(ctx, k_pressure<R>, nflatblk(m.N, m.L), kFlatT, flat_bytes(m.L, 8), m, s, gd0, ctx->density_0);
     LAUNCH_EX(ctx, k_bpg<R>, ncolblk(m.E), kColT, 2 * tile_bytes(m, 8), m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
@@ -864,11 +872,11 @@ template <typename R>
 static void d3end(fcg_ctx *ctx, const DevStateT<R> &s, const int *only) {
     const fcg_params &p = ctx->p;
     const DevMesh &m = ctx->m;
-    LAUNCH_EX(ctx, k_d3end_prep<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt, p.Dmin,
+    LAUNCH_EX(ctx, k_d3end_prep<R>, nflatblk(m.N, m.K), kFlatT, 5 * flat_bytes(m.K, sizeof(R)), m, s, p.dt, p.Dmin,
               ctx->density_0, only);
     LAUNCH1(ctx, k_d3end<R>, m.N, m, s, p.dt, p.Dmin, p.z0b_min, only);
-    LAUNCH_EX(ctx, k_d3end_fin<R>, ncolblk(m.N), kColT, tile_bytes(m, sizeof(R)), m, s, p.Dmin, p.z0b_min, p.z0s_min,
-              p.snul, only);
+    LAUNCH_EX(ctx, k_d3end_fin<R>, nflatblk(m.N, m.K), kFlatT, flat_bytes(m.K, sizeof(R)), m, s, p.Dmin, p.z0b_min,
+              p.z0s_min, p.snul, only);
 }
 
 template <typename R>
@@ -881,7 +889,7 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
         const fcg_mlp *net = ctx->mlp;
-        LAUNCH_EX(ctx, k_mlp_closure<R>, (ctx->m.N + 63) / 64, 64, 0, ctx->m, s, (const float *)net->w,
+        LAUNCH_EX(ctx, k_mlp_closure<R>, nflatblk(ctx->m.N, ctx->m.K), kFlatT, 0, ctx->m, s, (const float *)net->w,
                   (const fcgm::bf16x4 *)net->frag, net->norm, p.Dmin, ctx->density_0, p.z0b_min, p.snul,
                   ctx->mlp_flag);
         d3end(ctx, s, (const int *)ctx->mlp_flag);
@@ -908,8 +916,8 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         ctx->lfirst = 0;
     }
     LAUNCHLV(ctx, k_velgrad3d<R>, ctx->m.E, ctx->m.L, ctx->m, s);
-    LAUNCHLV(ctx, k_velrhs<R>, ctx->m.E, ctx->m.L, ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0,
-           p.vert_adv == 1 ? 1 : 0);
+    LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, velrhs_bytes(ctx->m, sizeof(R)), ctx->m, s, p.dt,
+              ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0);
     if (p.im_vert_visc_fic)
         LAUNCH_EX(ctx, k_velvisc<R>, ncolblk(ctx->m.E), kColT, tile_bytes(ctx->m, 8) + 2 * tile_bytes(ctx->m, sizeof(R)),
                   ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
@@ -965,8 +973,9 @@ extern "C" int fcg_jacobian(fcg_ctx *ctx) {
 
 template <typename R>
 static int update_3d_vel_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCH_EX(ctx, k_update3d<R>, ncolblk(ctx->m.E), kColT, 4 * kColB * sizeof(double) + 2 * tile_bytes(ctx->m, sizeof(R)),
-              ctx->m, s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
+    LAUNCH_EX(ctx, k_update3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT,
+              2 * flat_bytes(ctx->m.L, 8) + 4 * (size_t)cpb_of(ctx->m.L) * 8, ctx->m, s, ctx->p.Dmin,
+              1.0 / (double)ctx->p.Mt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -979,7 +988,7 @@ extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
 template <typename R>
 static int vert_vel_sigma_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     LAUNCHLV(ctx, k_vertvel_div<R>, ctx->m.N, ctx->m.L, ctx->m, s);
-    LAUNCH_EX(ctx, k_vertvel_scan<R>, ncolblk(ctx->m.N), kColT, tile_bytes(ctx->m, sizeof(R)), ctx->m, s);
+    LAUNCH_EX(ctx, k_vertvel_scan<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, flat_bytes(ctx->m.L, sizeof(R)), ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }

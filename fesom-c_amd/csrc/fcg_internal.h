@@ -120,6 +120,11 @@ inline DevStateT<R> retype(const DevState &s) {
     return t;
 }
 
+// LDS bytes per column of k_velrhs's element and edge records (checked
+// against the record types where they are defined)
+constexpr size_t kVelrhsElemBytes = 160;
+constexpr size_t velrhs_edge_bytes(size_t esz) { return esz == 8 ? 104 : 64; }
+
 struct fcg_ctx {
     fcg_params p{};
     bool mixed = false;    // FCG_MIXED: 3-D fields stored as float

@@ -107,11 +107,29 @@ __device__ __forceinline__ int col_first() { return (int)(xcd_block() * kColB); 
 __device__ __forceinline__ int tcol() { return (int)threadIdx.x >> 2; }  // column of this thread (per-level phases)
 __device__ __forceinline__ int tsub() { return (int)threadIdx.x & 3; }   // its first slot
 
-// the K slots of columns [c0, c0+nc) of g -> tile t
+// the K slots of columns [c0, c0+nc) of g -> tile t; loads are issued in
+// batches of eight per thread before their LDS stores, so their latencies
+// overlap
 template <typename T, typename G>
 __device__ __forceinline__ void tile_load(T *t, const G *__restrict__ g, int c0, int nc, int K) {
     const size_t base = (size_t)c0 * K;
-    for (Items it(K); it.c < nc; it.next()) t[it.c * (K + 1) + it.k] = (T)g[base + (size_t)it.c * K + it.k];
+    Items it(K);
+    while (it.c < nc) {
+        G v[8];
+        int d[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            d[j] = -1;
+            if (it.c < nc) {
+                v[j] = g[base + (size_t)it.c * K + it.k];
+                d[j] = it.c * (K + 1) + it.k;
+            }
+            it.next();
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (d[j] >= 0) t[d[j]] = (T)v[j];
+    }
 }
 
 // the first S slots of each tile column -> columns [c0, c0+nc) of g
@@ -140,37 +158,55 @@ __device__ __forceinline__ void planes_to_tile(T *t, const G *__restrict__ x, in
         t[j * (K + 1) + k] = (T)x[(size_t)k * N + c0 + j];
 }
 
+// Column-aligned flat blocks: cpb = blockDim.x / S whole columns per block,
+// one thread per (column, slot); a column recurrence then runs on threads
+// 0..nc-1 over LDS arrays x[c*S + k].  Every thread has a single item, so the
+// block's loads are all in flight together.
+struct ColFlat {
+    int cpb, c0, nc, c, k;
+    __device__ __forceinline__ bool on() const { return c < nc; }
+};
+__device__ __forceinline__ ColFlat col_flat(int ents, int S) {
+    ColFlat f;
+    f.cpb = (int)blockDim.x / S;
+    f.c0 = (int)xcd_block() * f.cpb;
+    f.nc = min(f.cpb, ents - f.c0);
+    f.c = (int)threadIdx.x / S;
+    f.k = (int)threadIdx.x - f.c * S;
+    return f;
+}
+
 // ---------------------------------------------------------------------------
 // pressure: hydrostatic column scan, src/fv_pressure_new.F90:91-107
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(kColT) void k_pressure(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
+__global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
     extern __shared__ __align__(16) unsigned char lds_[];
-    double *th = (double *)lds_;  // per column: 0, then the layer increments, then hp
-    const int K = m.K, L = m.L, KP = K + 1, KC = K;
-    const int c0 = col_first(), nc = min(kColB, m.N - c0);
-    if (nc <= 0) return;
-    if (tcol() < nc) {
-        const int c = tcol(), n = c0 + c;
+    double *th = (double *)lds_;  // per column: the layer increments, then their running sums
+    const int L = m.L, KC = m.K;
+    const ColFlat f = col_flat(m.N, L);
+    if (f.nc <= 0) return;
+    const int n = f.c0 + f.c, k = f.k;
+    if (f.on()) {
         const double a = s.a[n];
-        for (int k = tsub(); k < L; k += 4) {
-            const double Z = 0.5 * (m.oms[k] * a + m.oms[k + 1] * a);
-            const double r = densityJM(PL(s.TF, k, n, KC), PL(s.SF, k, n, KC), -Z, rho0);
-            th[c * KP + k + 1] = r * (a * m.dsig[k]) * gd0;
-        }
+        const double Z = 0.5 * (m.oms[k] * a + m.oms[k + 1] * a);
+        const double r = densityJM(PL(s.TF, k, n, KC), PL(s.SF, k, n, KC), -Z, rho0);
+        th[f.c * L + k] = r * (a * m.dsig[k]) * gd0;
     }
     __syncthreads();
-    if ((int)threadIdx.x < nc) {
-        double *col = th + threadIdx.x * KP;
+    if ((int)threadIdx.x < f.nc) {
+        double *col = th + threadIdx.x * L;
         double hp = 0.0;
-        col[0] = 0.0;
-        for (int k = 0; k < L; ++k) {
-            hp = hp + col[k + 1];
-            col[k + 1] = hp;
+        for (int j = 0; j < L; ++j) {
+            hp = hp + col[j];
+            col[j] = hp;
         }
     }
     __syncthreads();
-    tile_store(s.hp, th, c0, nc, K, K);
+    if (f.on()) {
+        PL(s.hp, k + 1, n, KC) = th[f.c * L + k];
+        if (k == 0) PL(s.hp, 0, n, KC) = 0.0;
+    }
 }
 
 // Per layer (four threads per element column) the gradient and its
@@ -238,27 +274,38 @@ __global__ __launch_bounds__(kColT) void k_bpg(DevMesh m, DevStateT<R> s) {
 #define PP(arr, k, n, N) (arr)[(size_t)(k) * (N) + (n)]
 #define SG(nz) m.sigma[(nz) - 1]
 
+// level-plane rows of the block's nc columns: x[k*N + c0 + j] = t[j*S + k]
+// for k in [k0, k1), each row written by consecutive threads
+template <typename T, typename G>
+__device__ __forceinline__ void rows_store(G *__restrict__ x, const T *t, int c0, int nc, int S, int N, int k0, int k1) {
+    for (int r = (int)threadIdx.x; r < (k1 - k0) * nc; r += (int)blockDim.x) {
+        const int k = k0 + r / nc, j = r % nc;
+        x[(size_t)k * N + c0 + j] = (G)t[j * S + k];
+    }
+}
+
 template <typename R>
-__global__ __launch_bounds__(kColT) void k_d3end_prep(DevMesh m, DevStateT<R> s, double dt, double Dmin, double rho0,
-                                                      const int *__restrict__ only) {
+__global__ __launch_bounds__(256) void k_d3end_prep(DevMesh m, DevStateT<R> s, double dt, double Dmin, double rho0,
+                                                    const int *__restrict__ only) {
     extern __shared__ __align__(16) unsigned char lds_[];
-    const int N = m.N, K = m.K, KP = K + 1, KC = K;
-    R *tp = (R *)lds_, *tq = tp + kColB * KP, *tc = tq + kColB * KP, *ts = tc + kColB * KP, *tb = ts + kColB * KP;
-    const int c0 = col_first(), nc = min(kColB, N - c0);
-    if (nc <= 0) return;
-    if (tcol() < nc && !(only && only[c0 + tcol()] == 0)) {
-        const int c = tcol(), n = c0 + c;
+    const int N = m.N, K = m.K, KC = K;
+    const ColFlat f = col_flat(N, K);
+    const int T = f.cpb * K;
+    R *tp = (R *)lds_, *tq = tp + T, *tc = tq + T, *ts = tc + T, *tb = ts + T;
+    if (f.nc <= 0) return;
+    const int n = f.c0 + f.c, lev = f.k;
+    if (f.on() && !(only && only[n] == 0)) {
 #define SNU(z) PL(s.snu, (z) - 1, n, KC)
 #define BT(z) PL(s.bt, (z) - 1, n, KC)
-        const double t_inv = 1.0 / dt;
-        const double a = s.a[n];
-        const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
-        const double dmean2 = dmean * dmean;
-        for (int lev = tsub(); lev < K; lev += 4) {
-            ts[c * KP + lev] = SNU(lev + 1);
-            tb[c * KP + lev] = BT(lev + 1);
-            const int nz = lev + 1;
-            if (nz < 2 || nz > K - 1) continue;
+        const int i = f.c * K + lev;
+        ts[i] = SNU(lev + 1);
+        tb[i] = BT(lev + 1);
+        const int nz = lev + 1;
+        if (nz >= 2 && nz <= K - 1) {
+            const double t_inv = 1.0 / dt;
+            const double a = s.a[n];
+            const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+            const double dmean2 = dmean * dmean;
             const double d1 = SG(nz - 1) - SG(nz);
             const double d2 = SG(nz) - SG(nz + 1);
             const double d = 0.5 * (d1 + d2);
@@ -278,19 +325,19 @@ __global__ __launch_bounds__(kColT) void k_d3end_prep(DevMesh m, DevStateT<R> s,
             const double snu_m = 0.5 * (SNU(nz) + SNU(nz + 1));
             double c2 = 0.5 * 0.73 * (snu_p * (BT(nz - 1) - BT(nz)) / d1 - snu_m * (BT(nz) - BT(nz + 1)) / d2) / d;
             c2 = c2 / dmean2;
-            tp[c * KP + lev] = -b1 - b2;
-            tq[c * KP + lev] = t_inv * BT(nz);
-            tc[c * KP + lev] = c2;
+            tp[i] = -b1 - b2;
+            tq[i] = t_inv * BT(nz);
+            tc[i] = c2;
         }
 #undef SNU
 #undef BT
     }
     __syncthreads();
-    tile_to_planes(s.mx_s, ts, c0, nc, K, N, 0, K);
-    tile_to_planes(s.mx_bA, tb, c0, nc, K, N, 0, K);
-    tile_to_planes(s.mx_p, tp, c0, nc, K, N, 1, K - 1);
-    tile_to_planes(s.mx_q, tq, c0, nc, K, N, 1, K - 1);
-    tile_to_planes(s.mx_c2, tc, c0, nc, K, N, 1, K - 1);
+    rows_store(s.mx_s, ts, f.c0, f.nc, K, N, 0, K);
+    rows_store(s.mx_bA, tb, f.c0, f.nc, K, N, 0, K);
+    rows_store(s.mx_p, tp, f.c0, f.nc, K, N, 1, K - 1);
+    rows_store(s.mx_q, tq, f.c0, f.nc, K, N, 1, K - 1);
+    rows_store(s.mx_c2, tc, f.c0, f.nc, K, N, 1, K - 1);
 }
 
 template <typename R>
@@ -372,34 +419,35 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
 }
 
 template <typename R>
-__global__ __launch_bounds__(kColT) void k_d3end_fin(DevMesh m, DevStateT<R> s, double Dmin, double z0b_min,
-                                                     double z0s_min, double snul, const int *__restrict__ only) {
+__global__ __launch_bounds__(256) void k_d3end_fin(DevMesh m, DevStateT<R> s, double Dmin, double z0b_min,
+                                                   double z0s_min, double snul, const int *__restrict__ only) {
     extern __shared__ __align__(16) unsigned char lds_[];
-    const int N = m.N, K = m.K, KP = K + 1, KC = K;
+    const int N = m.N, K = m.K, KC = K;
+    const ColFlat f = col_flat(N, K);
     R *tx = (R *)lds_;
-    const int c0 = col_first(), nc = min(kColB, N - c0);
-    if (nc <= 0) return;
-    planes_to_tile(tx, s.mx_xx, c0, nc, K, N);
-    __syncthreads();
-    if (tcol() >= nc) return;
-    const int c = tcol(), n = c0 + c;
-    if (only && only[n] == 0) return;
-    const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
-    for (int lev = tsub(); lev < K; lev += 4) {
-        const int nz = lev + 1;
-        const double btr = tx[c * KP + lev];
-        const double btv = dmaxr(btr, 1.0e-8);
-        PL(s.bt, lev, n, KC) = btv;
-        const double zt = SG(nz) * dmean;
-        const double Zh = zt + z0b_min;
-        const double Zd = fabs(dmean - zt) + z0s_min;
-        const double Z0 = 1.0 - kBetaScale * Zh * Zd / (dmean * dmean);
-        const double scale_t = kCka * Zh * Zd * Z0 / dmean;
-        double sv = scale_t * sqrt(btv) + snul;
-        if (sv >= 0.2) sv = 0.2;
-        PL(s.snu, lev, n, KC) = sv;
-        PL(s.Kv, lev, n, KC) = kPrNum * (sv);
+    if (f.nc <= 0) return;
+    for (int r = (int)threadIdx.x; r < K * f.nc; r += (int)blockDim.x) {
+        const int k = r / f.nc, j = r % f.nc;
+        tx[j * K + k] = PP(s.mx_xx, k, f.c0 + j, N);
     }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, lev = f.k;
+    if (only && only[n] == 0) return;
+    const int nz = lev + 1;
+    const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+    const double btr = tx[f.c * K + lev];
+    const double btv = dmaxr(btr, 1.0e-8);
+    PL(s.bt, lev, n, KC) = btv;
+    const double zt = SG(nz) * dmean;
+    const double Zh = zt + z0b_min;
+    const double Zd = fabs(dmean - zt) + z0s_min;
+    const double Z0 = 1.0 - kBetaScale * Zh * Zd / (dmean * dmean);
+    const double scale_t = kCka * Zh * Zd * Z0 / dmean;
+    double sv = scale_t * sqrt(btv) + snul;
+    if (sv >= 0.2) sv = 0.2;
+    PL(s.snu, lev, n, KC) = sv;
+    PL(s.Kv, lev, n, KC) = kPrNum * (sv);
 }
 #undef SG
 
@@ -564,47 +612,123 @@ __device__ __forceinline__ void uvert(const DevMesh &m, const DevStateT<R> &s, c
     fv = -vm * wv;
 }
 
-// compute_vel_rhs (src/fv_3D_vel.F90:1-197), explicit part, one thread per
-// element and layer: elemental pressure/Coriolis/AB2 (35-78), the
-// momentum_adv_upwind gather over the element's internal edges in ascending
-// id (fv_advection.F90:64-197), momentum_vert_adv_upwind at the layer's two
-// interfaces (354-469) and the explicit update (124-143).  Writes the new AB
-// history and the provisional velocity into U_rhs/V_rhs; k_velvisc then
-// applies the implicit vertical viscosity column by column.
+// Per-edge constants of momentum_adv_upwind (fv_advection.F90:120-165), as
+// the flux uses them (storage type R for the geometry, FP64 node depths).
+template <typename R>
+struct EdgeRec {
+    int cc, e1, e2;      // signed edge id (kEmpty: none), its two elements
+    R acc, cx, cy, cz, cw, dx, dy, cos1, cos2;
+    double an1, an2;     // a at the edge's nodes
+};
+
+// Element constants of compute_vel_rhs
+struct ElemRec {
+    ElemCol col;
+    double Fx, Fy, acc, area, cor, mask;
+};
+
+static_assert(sizeof(ElemRec) <= kVelrhsElemBytes && alignof(ElemRec) <= 16 + 16, "element record size");
+static_assert(sizeof(EdgeRec<double>) == velrhs_edge_bytes(8) && sizeof(EdgeRec<float>) == velrhs_edge_bytes(4),
+              "edge record size");
+
+// compute_vel_rhs (src/fv_3D_vel.F90:1-197), explicit part, on column-aligned
+// blocks (cpb = 256/L element columns, one thread per element and layer):
+// the element's and its internal edges' constants are read once per column
+// into LDS; each thread then forms the elemental pressure/Coriolis/AB2 terms
+// (35-78), the momentum_adv_upwind gather over the element's internal edges
+// in ascending id (fv_advection.F90:64-197), momentum_vert_adv_upwind at the
+// layer's two interfaces (354-469) and the explicit update (124-143).  Writes
+// the new AB history and the provisional velocity into U_rhs/V_rhs;
+// k_velvisc then applies the implicit vertical viscosity column by column.
 template <typename R>
 __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv) {
-    const int KC = m.K;
-    int e, k;
-    if (!ent_layer(m.E, m.L, e, k)) return;
-    const int N = m.N, E = m.E, K = m.K;
-    const ElemCol c = elem_col(m, s, e);
-    const int4 q = c.q;
-    const double4 w = c.w, gx = m.gsx[e], gy = m.gsy[e];
-    const double area = m.elem_area[e], cor = m.coriolis[e];
-    const double area_inv = 1.0 / area;
-    const double mask = s.mask_wd[e];
-    double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
-    double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
-    double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
-    double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
-    const double Fx = sum4(gx, pre0, pre1, pre2, pre3) * area;
-    const double Fy = sum4(gy, pre0, pre1, pre2, pre3) * area;
-    const double acc = sum4(w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
-    const double je = c.Je(m, k);
-    const double u = PL(s.U_n, k, e, KC), v = PL(s.V_n, k, e, KC);
-    double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, KC) + Fx * je;
-    double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, KC) + Fy * je;
-    R abu = (R)(v * cor * area * je);
-    R abv = (R)(-u * cor * area * je);
-    if (hadv) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int L = m.L, K = m.K, KC = K;
+    const ColFlat f = col_flat(m.E, L);
+    ElemRec *er = (ElemRec *)lds_;                                        // [cpb]
+    EdgeRec<R> *edr = (EdgeRec<R> *)(lds_ + f.cpb * kVelrhsElemBytes);   // [cpb][4]
+    if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {
+        const int c = threadIdx.x, e = f.c0 + c;
+        ElemRec r;
+        r.col = elem_col(m, s, e);
+        const int4 q = r.col.q;
+        const double4 gx = m.gsx[e], gy = m.gsy[e];
+        r.area = m.elem_area[e];
+        r.cor = m.coriolis[e];
+        r.mask = s.mask_wd[e];
+        const double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
+        const double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
+        const double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
+        const double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
+        r.Fx = sum4(gx, pre0, pre1, pre2, pre3) * r.area;
+        r.Fy = sum4(gy, pre0, pre1, pre2, pre3) * r.area;
+        r.acc = sum4(r.col.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+        er[c] = r;
         const int4 ied = m.el_ied[e];
-#pragma nounroll
         for (int j = 0; j < 4; ++j) {
             const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
-            if (cc == kEmpty) break;
+            EdgeRec<R> g;
+            g.cc = cc;
+            if (cc != kEmpty && hadv) {
+                const int ed = cc >= 0 ? cc : ~cc;
+                const int2 nd = m.ednodes[ed];
+                const int2 et = m.edtri[ed];
+                const double4 cd = m.ecross[ed];
+                const double2 dd = m.edxdy[ed];
+                g.e1 = et.x;
+                g.e2 = et.y;
+                g.acc = (R)(0.5 * (m.ac[nd.x] + m.ac[nd.y]));
+                g.cx = (R)cd.x;
+                g.cy = (R)cd.y;
+                g.cz = (R)cd.z;
+                g.cw = (R)cd.w;
+                g.dx = (R)dd.x;
+                g.dy = (R)dd.y;
+                g.cos1 = (R)m.elem_cos[et.x];
+                g.cos2 = (R)m.elem_cos[et.y];
+                g.an1 = s.a[nd.x];
+                g.an2 = s.a[nd.y];
+            }
+            edr[c * 4 + j] = g;
+        }
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int c = f.c, e = f.c0 + c, k = f.k;
+    const ElemRec &r = er[c];
+    const ElemCol &col = r.col;
+    const int4 q = col.q;
+    const double4 w = col.w;
+    const double area = r.area;
+    const double area_inv = 1.0 / area;
+    const double je = col.Je(m, k);
+    const double u = PL(s.U_n, k, e, KC), v = PL(s.V_n, k, e, KC);
+    double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, KC) + r.Fx * je;
+    double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, KC) + r.Fy * je;
+    R abu = (R)(v * r.cor * area * je);
+    R abv = (R)(-u * r.cor * area * je);
+    if (hadv) {
+        const double ds = m.dsig[k];
+        for (int j = 0; j < 4; ++j) {
+            const EdgeRec<R> &g = edr[c * 4 + j];
+            if (g.cc == kEmpty) break;
+            // in the storage type: FP32 advective fluxes in the mixed-precision path
+            const R u1 = PL(s.U_n, k, g.e1, KC) - PL(s.gux, k, g.e1, KC) * g.cx - PL(s.guy, k, g.e1, KC) * g.cy;
+            const R v1 = PL(s.V_n, k, g.e1, KC) - PL(s.gvx, k, g.e1, KC) * g.cx - PL(s.gvy, k, g.e1, KC) * g.cy;
+            const R u2 = PL(s.U_n, k, g.e2, KC) - PL(s.gux, k, g.e2, KC) * g.cz - PL(s.guy, k, g.e2, KC) * g.cw;
+            const R v2 = PL(s.V_n, k, g.e2, KC) - PL(s.gvx, k, g.e2, KC) * g.cz - PL(s.gvy, k, g.e2, KC) * g.cw;
+            const R jd = (R)(0.5 * (g.an1 * ds + g.an2 * ds));
+            const R un = R(0.5) * R(kREarthD) * jd * ((u1 + u2) * g.dy - (v1 * g.cos1 + v2 * g.cos2) * g.dx);
             R uu, vv;
-            edge_uu3d(m, s, cc >= 0 ? cc : ~cc, k, uu, vv);
-            if (cc >= 0) {
+            if (un >= R(0)) {
+                uu = u1 * un * g.acc;
+                vv = v1 * un * g.acc;
+            } else {
+                uu = u2 * un * g.acc;
+                vv = v2 * un * g.acc;
+            }
+            if (g.cc >= 0) {
                 abu = abu + uu;
                 abv = abv + vv;
             } else {
@@ -614,7 +738,7 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
         }
     }
     if (vadv) {
-        if (mask != 0.0) {
+        if (r.mask != 0.0) {
             // fluxes through the layer's top (t = 0) and bottom (t = 1) interface
             double ftu = 0.0, ftv = 0.0, fbu = 0.0, fbv = 0.0;
 #pragma nounroll
@@ -627,7 +751,7 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
                     fu = -w1 * PL(s.U_n, 0, e, KC);
                     fv = -w1 * PL(s.V_n, 0, e, KC);
                 } else if (nz <= K - 1) {
-                    uvert(m, s, c, e, area, nz, fu, fv);
+                    uvert(m, s, col, e, area, nz, fu, fv);
                 }
                 if (t == 0) {
                     ftu = fu;
@@ -637,13 +761,14 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
                     fbv = fv;
                 }
             }
-            abu = abu + (R)((ftu - fbu) * acc);
-            abv = abv + (R)((ftv - fbv) * acc);
+            abu = abu + (R)((ftu - fbu) * r.acc);
+            abv = abv + (R)((ftv - fbv) * r.acc);
         } else {
             abu = R(0);
             abv = R(0);
         }
     }
+    const double mask = r.mask;
     PL(s.U_rhsAB, k, e, KC) = abu;
     PL(s.V_rhsAB, k, e, KC) = abv;
     PL(s.U_rhs, k, e, KC) = mask * (u + PL(s.Bpu, k, e, KC) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
@@ -672,9 +797,24 @@ __global__ __launch_bounds__(kColT) void k_velvisc(DevMesh m, DevStateT<R> s, do
         const int c = tcol(), e = c0 + c;
         const int4 q = m.en[e];
         const double4 w = m.wcv[e];
-        for (int k = tsub(); k < K; k += 4)
-            tA[c * KP + k] = sum4(w, PL(s.snu, k, q.x, KC), PL(s.snu, k, q.y, KC), PL(s.snu, k, q.z, KC),
-                                  PL(s.snu, k, q.w, KC));
+        for (int k0 = tsub(); k0 < K; k0 += 32) {  // eight levels per batch, loads first
+            R v[8][4];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + 4 * j;
+                if (k < K) {
+                    v[j][0] = PL(s.snu, k, q.x, KC);
+                    v[j][1] = PL(s.snu, k, q.y, KC);
+                    v[j][2] = PL(s.snu, k, q.z, KC);
+                    v[j][3] = PL(s.snu, k, q.w, KC);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + 4 * j;
+                if (k < K) tA[c * KP + k] = sum4(w, v[j][0], v[j][1], v[j][2], v[j][3]);
+            }
+        }
     }
     __syncthreads();
     if ((int)threadIdx.x < nc) {
@@ -906,25 +1046,25 @@ __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict
 }
 
 // momentum_adv_P1_3D_to_2D (fv_advection.F90:676-793) as an element gather,
-// then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513).  A wavefront takes 16
-// element columns: the per-layer edge terms uu*un*Je of each element's
-// (up to 4) internal edges are formed by four threads per column and staged
-// in LDS; one lane per (column, edge) then sums them over the layers
-// in order, and one lane per column combines the edges in ascending id.
-constexpr int kP1Cols = 16;
+// then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513).  One thread per
+// (element, layer) forms the layer terms uu*un*Je of the element's (up to 4)
+// internal edges; one thread per (element, edge) sums them over the layers
+// in order, and one per element combines the edges in ascending id.
 template <typename R>
-__global__ __launch_bounds__(kColB) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
+__global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
     extern __shared__ __align__(16) unsigned char lds_[];
-    const int K = m.K, L = m.L, KP = K + 1, KC = K;
-    double *tu = (double *)lds_;                      // [edge slot][column][KP]
-    double *tv = tu + 4 * kP1Cols * KP;
-    double *su_ = tv + 4 * kP1Cols * KP, *sv_ = su_ + 4 * kP1Cols;  // [column][edge slot]
-    const int c0 = (int)xcd_block() * kP1Cols, nc = min(kP1Cols, m.E - c0);
-    if (nc <= 0) return;
-    if (tcol() < nc) {
-        const int c = tcol(), e = c0 + c;
+    const int L = m.L, KC = m.K;
+    const ColFlat f = col_flat(m.E, L);
+    const int T = f.cpb * L;
+    double *tu = (double *)lds_, *tv = tu + 4 * T;       // [edge slot][column*L + layer]
+    double *su_ = tv + 4 * T, *sv_ = su_ + 4 * f.cpb;    // [column*4 + edge slot]
+    if (f.nc <= 0) return;
+    if (f.on()) {
+        const int c = f.c, e = f.c0 + c, k = f.k;
         const int4 ied = m.el_ied[e];
         const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+        const double je = PL(s.Je, k, e, KC);
+        const double ds = m.dsig[k];
         for (int j = 0; j < 4; ++j) {
             const int cc = slots[j];
             if (cc == kEmpty) break;
@@ -935,39 +1075,33 @@ __global__ __launch_bounds__(kColB) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
             const double xe = dd.x * kREarthD * 0.5 * (m.elem_cos[et.x] + m.elem_cos[et.y]);
             const double ye = dd.y * kREarthD;
             const double an1 = s.a[nd.x], an2 = s.a[nd.y];
-            for (int k = tsub(); k < L; k += 4) {
-                const double ds = m.dsig[k];
-                double uu = 0.5 * (PL(s.Unode_p, k, nd.x, KC) + PL(s.Unode_p, k, nd.y, KC));
-                double vv = 0.5 * (PL(s.Vnode_p, k, nd.x, KC) + PL(s.Vnode_p, k, nd.y, KC));
-                double jd = 0.5 * (an1 * ds + an2 * ds);
-                double un = (uu * ye - vv * xe) * jd;
-                const double je = PL(s.Je, k, e, KC);
-                tu[(j * kP1Cols + c) * KP + k] = uu * un * je;
-                tv[(j * kP1Cols + c) * KP + k] = vv * un * je;
-            }
+            double uu = 0.5 * (PL(s.Unode_p, k, nd.x, KC) + PL(s.Unode_p, k, nd.y, KC));
+            double vv = 0.5 * (PL(s.Vnode_p, k, nd.x, KC) + PL(s.Vnode_p, k, nd.y, KC));
+            double jd = 0.5 * (an1 * ds + an2 * ds);
+            double un = (uu * ye - vv * xe) * jd;
+            tu[j * T + c * L + k] = uu * un * je;
+            tv[j * T + c * L + k] = vv * un * je;
         }
     }
     __syncthreads();
-    {
-        const int c = threadIdx.x >> 2, j = threadIdx.x & 3;
-        if (c < nc) {
-            const int4 ied = m.el_ied[c0 + c];
-            const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
-            double su = 0.0, sv = 0.0;
-            if (cc != kEmpty) {
-                const double *pu = tu + (j * kP1Cols + c) * KP, *pv = tv + (j * kP1Cols + c) * KP;
-                for (int k = 0; k < L; ++k) {
-                    su = su + pu[k];
-                    sv = sv + pv[k];
-                }
+    for (int t = threadIdx.x; t < 4 * f.nc; t += blockDim.x) {
+        const int c = t >> 2, j = t & 3;
+        const int4 ied = m.el_ied[f.c0 + c];
+        const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
+        double su = 0.0, sv = 0.0;
+        if (cc != kEmpty) {
+            const double *pu = tu + j * T + c * L, *pv = tv + j * T + c * L;
+            for (int k = 0; k < L; ++k) {
+                su = su + pu[k];
+                sv = sv + pv[k];
             }
-            su_[c * 4 + j] = su;
-            sv_[c * 4 + j] = sv;
         }
+        su_[c * 4 + j] = su;
+        sv_[c * 4 + j] = sv;
     }
     __syncthreads();
-    if ((int)threadIdx.x < nc) {
-        const int c = threadIdx.x, e = c0 + c;
+    if ((int)threadIdx.x < f.nc) {
+        const int c = threadIdx.x, e = f.c0 + c;
         const int4 ied = m.el_ied[e];
         const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
         double ru = 0.0, rv = 0.0;
@@ -1116,61 +1250,58 @@ __global__ __launch_bounds__(256) void k_je(DevMesh m, DevStateT<R> s) {
 }
 
 // update_3D_vel: src/fv_3D_vel.F90:201-255, with U_filt_2D *= 1/Mt from
-// oce_barotropic_timestep (fv_main.F90:1670-1677), on tiles of 64 element
-// columns: the vertical integral runs one lane per column over the staged
-// U_rhs/V_rhs, the per-layer update flat over the block's items.
+// oce_barotropic_timestep (fv_main.F90:1670-1677); one thread per (element,
+// layer) forms U_rhs*Je_old, the vertical integral runs one thread per
+// column over those products in layer order.
 template <typename R>
-__global__ __launch_bounds__(kColT) void k_update3d(DevMesh m, DevStateT<R> s, double Dmin, double Mt_inv) {
+__global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, double Dmin, double Mt_inv) {
     extern __shared__ __align__(16) unsigned char lds_[];
-    const int K = m.K, L = m.L, KP = K + 1, KC = K;
-    double *sc = (double *)lds_;  // per column: U2p, V2p, uf, vf
-    R *tU = (R *)(sc + 4 * kColB), *tV = tU + kColB * KP;
-    const int c0 = col_first(), nc = min(kColB, m.E - c0);
-    if (nc <= 0) return;
-    tile_load(tU, s.U_rhs, c0, nc, K);
-    tile_load(tV, s.V_rhs, c0, nc, K);
-    __syncthreads();
-    if ((int)threadIdx.x < nc) {
-        const int c = threadIdx.x, e = c0 + c;
+    const int L = m.L, KC = m.K;
+    const ColFlat f = col_flat(m.E, L);
+    double *pu = (double *)lds_, *pv = pu + f.cpb * L, *sc = pv + f.cpb * L;  // sc per column: U2p, V2p, uf, vf
+    if (f.nc <= 0) return;
+    const int e = f.c0 + f.c, k = f.k;
+    double ak = 0.0, ur = 0.0, vr = 0.0, je = 1.0;
+    if (f.on()) {
         const int4 q = m.en[e];
         const double4 w = m.wcv[e];
-        const double o0 = s.a_old[q.x], o1 = s.a_old[q.y], o2 = s.a_old[q.z], o3 = s.a_old[q.w];
+        const double ds = m.dsig[k];
+        ak = sum4(w, s.a_old[q.x] * ds, s.a_old[q.y] * ds, s.a_old[q.z] * ds, s.a_old[q.w] * ds);
+        ur = PL(s.U_rhs, k, e, KC);
+        vr = PL(s.V_rhs, k, e, KC);
+        je = PL(s.Je, k, e, KC);
+        pu[f.c * L + k] = ur * ak;
+        pv[f.c * L + k] = vr * ak;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < f.nc) {
+        const int c = threadIdx.x, ec = f.c0 + c;
+        const int4 q = m.en[ec];
+        const double4 w = m.wcv[ec];
         const double dmi = 1.0 / dmaxr(Dmin, sum4(w, m.depth[q.x] + s.eta_n[q.x], m.depth[q.y] + s.eta_n[q.y],
                                                   m.depth[q.z] + s.eta_n[q.z], m.depth[q.w] + s.eta_n[q.w]));
         double su = 0.0, sv = 0.0;
-        for (int k = 0; k < L; ++k) {
-            double ds = m.dsig[k];
-            double ak = sum4(w, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
-            su = su + tU[c * KP + k] * ak;
-            sv = sv + tV[c * KP + k] * ak;
+        for (int j = 0; j < L; ++j) {
+            su = su + pu[c * L + j];
+            sv = sv + pv[c * L + j];
         }
-        const double uf = (s.U_filt[e] * Mt_inv) * dmi;
-        const double vf = (s.V_filt[e] * Mt_inv) * dmi;
-        s.U_filt[e] = uf;
-        s.V_filt[e] = vf;
-        sc[0 * kColB + c] = su * dmi;
-        sc[1 * kColB + c] = sv * dmi;
-        sc[2 * kColB + c] = uf;
-        sc[3 * kColB + c] = vf;
+        const double uf = (s.U_filt[ec] * Mt_inv) * dmi;
+        const double vf = (s.V_filt[ec] * Mt_inv) * dmi;
+        s.U_filt[ec] = uf;
+        s.V_filt[ec] = vf;
+        sc[4 * c] = su * dmi;
+        sc[4 * c + 1] = sv * dmi;
+        sc[4 * c + 2] = uf;
+        sc[4 * c + 3] = vf;
     }
     __syncthreads();
-    if (tcol() >= nc) return;
-    const int c = tcol(), e = c0 + c;
-    const int4 q = m.en[e];
-    const double4 w = m.wcv[e];
-    const double o0 = s.a_old[q.x], o1 = s.a_old[q.y], o2 = s.a_old[q.z], o3 = s.a_old[q.w];
-    const double U2p = sc[c], V2p = sc[kColB + c], uf = sc[2 * kColB + c], vf = sc[3 * kColB + c];
-    const double u2 = s.U2[2 * e], v2 = s.U2[2 * e + 1];
-    for (int k = tsub(); k < L; k += 4) {
-        const double ds = m.dsig[k];
-        double ak = sum4(w, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
-        ak = ak / PL(s.Je, k, e, KC);
-        const double ur = tU[c * KP + k], vr = tV[c * KP + k];
-        PL(s.U_nf, k, e, KC) = ak * ur + uf - U2p;
-        PL(s.V_nf, k, e, KC) = ak * vr + vf - V2p;
-        PL(s.U_n, k, e, KC) = ak * ur + u2 - U2p;
-        PL(s.V_n, k, e, KC) = ak * vr + v2 - V2p;
-    }
+    if (!f.on()) return;
+    const double U2p = sc[4 * f.c], V2p = sc[4 * f.c + 1], uf = sc[4 * f.c + 2], vf = sc[4 * f.c + 3];
+    ak = ak / je;
+    PL(s.U_nf, k, e, KC) = ak * ur + uf - U2p;
+    PL(s.V_nf, k, e, KC) = ak * vr + vf - V2p;
+    PL(s.U_n, k, e, KC) = ak * ur + s.U2[2 * e] - U2p;
+    PL(s.V_n, k, e, KC) = ak * vr + s.U2[2 * e + 1] - V2p;
 }
 
 // vert_vel_sigma (src/fv_3D_vel.F90:309-456), first part: per node and layer
@@ -1202,29 +1333,29 @@ __global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevStateT<R> s) 
     PL(s.W, k, n, KC) = c;
 }
 
-// second part: bottom-up cumulative sum and /area (444-450), one lane per
-// column of a staged tile; Wvel(nsigma) stays zero.
+// second part: bottom-up cumulative sum and /area (444-450), one thread per
+// column over the block's staged layers; Wvel(nsigma) stays zero.
 template <typename R>
-__global__ __launch_bounds__(kColT) void k_vertvel_scan(DevMesh m, DevStateT<R> s) {
+__global__ __launch_bounds__(256) void k_vertvel_scan(DevMesh m, DevStateT<R> s) {
     extern __shared__ __align__(16) unsigned char lds_[];
-    const int K = m.K, L = m.L, KP = K + 1;
-    R *tW = (R *)lds_;
-    const int c0 = col_first(), nc = min(kColB, m.N - c0);
-    if (nc <= 0) return;
-    tile_load(tW, s.W, c0, nc, K);
+    const int L = m.L, KC = m.K;
+    const ColFlat f = col_flat(m.N, L);
+    R *tw = (R *)lds_;
+    if (f.nc <= 0) return;
+    const int n = f.c0 + f.c, k = f.k;
+    if (f.on()) tw[f.c * L + k] = PL(s.W, k, n, KC);
     __syncthreads();
-    if ((int)threadIdx.x < nc) {
-        const int c = threadIdx.x;
-        R *W = tW + c * KP;
-        const double area_inv = 1.0 / m.area[c0 + c];
+    if ((int)threadIdx.x < f.nc) {
+        R *W = tw + threadIdx.x * L;
+        const double area_inv = 1.0 / m.area[f.c0 + threadIdx.x];
         double run = 0.0;
-        for (int k = L - 1; k >= 0; --k) {
-            run = W[k] + run;
-            W[k] = run * area_inv;
+        for (int j = L - 1; j >= 0; --j) {
+            run = W[j] + run;
+            W[j] = run * area_inv;
         }
     }
     __syncthreads();
-    tile_store(s.W, tW, c0, nc, K, L);
+    if (f.on()) PL(s.W, k, n, KC) = tw[f.c * L + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1953,102 +2084,87 @@ __global__ __launch_bounds__(256) void k_mlp_collect(DevMesh m, DevStateT<R> s, 
     T[i * 2 + 1] = (float)log10((double)PL(s.Kv, nz - 1, n, KC));
 }
 
-// The closure step, one lane per node column (64-thread blocks, one wave):
-// the lane walks its levels 1..K carrying the node-averaged velocities of
-// the two layers around the current interface, so each layer is averaged
-// once; at every level the wave's 64 samples (one per lane) are normalised,
-// staged in LDS and run through the network as four 16-sample MFMA tiles.
-// snu and Kv = 10^(de-normalised outputs), snu held in [snul, 0.2] as d3_end
-// holds it (fv_mixing.F90:282-285), are staged in the d3_end scratch planes;
-// a column whose features all stay inside the training box takes them, any
-// other column is flagged and left to d3_end.
+// The closure step on column-aligned blocks (cpb = 256/K whole node columns,
+// one thread per (column, level 1..K) sample).  Per column, the depth
+// factors and the bottom-stress feature are formed once; per sample the five
+// features (as column_features), normalised and checked against the
+// training box; each wavefront then runs its 64 samples through the network
+// as four 16-sample MFMA tiles.  snu and Kv = 10^(de-normalised outputs),
+// snu held in [snul, 0.2] as d3_end holds it (fv_mixing.F90:282-285), go
+// straight to the column slots, for every column whose samples all lie in
+// the box; any other column is flagged and left to d3_end.
 template <typename R>
-__global__ __launch_bounds__(64) void k_mlp_closure(DevMesh m, DevStateT<R> s, const float *__restrict__ w,
-                                                    const fcgm::bf16x4 *__restrict__ frag, MlpNorm nm, double Dmin,
-                                                    double rho0, double z0b_min, double snul, int *flag) {
-    const int KC = m.K;
-    __shared__ float xs[64 * kMlpF];
-    __shared__ float ys[64 * 2];
-    const int lane = threadIdx.x;
-    const int n = (int)(xcd_block() * 64u + lane);
-    const bool valid = n < m.N;
-    const int N = m.N, K = m.K, L = m.L;
-    double a = 0.0, dmean = 1.0, f3 = 0.0;
-    auto uv = [&](int k, double &u, double &v) {  // node averages of compute_el2nodes_3D
-        u = PL(s.Unode, k, n, KC);
-        v = PL(s.Vnode, k, n, KC);
-    };
-    double u_up = 0.0, v_up = 0.0, u_dn = 0.0, v_dn = 0.0;
-    if (valid) {
-        a = s.a[n];
-        dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
-        double ub, vb;
-        uv(L - 1, ub, vb);
+__global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, const float *__restrict__ w,
+                                                     const fcgm::bf16x4 *__restrict__ frag, MlpNorm nm, double Dmin,
+                                                     double rho0, double z0b_min, double snul, int *flag) {
+    __shared__ float xs[256 * kMlpF];
+    __shared__ float ys[256 * 2];
+    __shared__ double col_a[256], col_dm[256], col_f3[256];
+    __shared__ int col_ood[256];
+    const int K = m.K, L = m.L, KC = K;
+    const ColFlat f = col_flat(m.N, K);
+    if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {
+        const int n = f.c0 + threadIdx.x;
+        const double a = s.a[n];
+        const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
+        const double ub = PL(s.Unode, L - 1, n, KC), vb = PL(s.Vnode, L - 1, n, KC);
         const double delta_b = a * m.dsig[L - 1] + z0b_min;
         const double lg = log(delta_b / z0b_min);
-        f3 = log10(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12);
-        uv(0, u_up, v_up);
-        uv(1, u_dn, v_dn);
+        col_a[threadIdx.x] = a;
+        col_dm[threadIdx.x] = dmean;
+        col_f3[threadIdx.x] = log10(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12);
+        col_ood[threadIdx.x] = 0;
     }
-    bool ood = false;
-    int iz_have = 2;  // interface whose two layers u_up/u_dn hold
-    float f[kMlpF];
-    for (int nz = 1; nz <= K; ++nz) {
-        const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);
-        if (valid) {
-            if (iz != iz_have) {  // one layer further down
-                u_up = u_dn;
-                v_up = v_dn;
-                uv(iz - 1, u_dn, v_dn);
-                iz_have = iz;
-            }
-            const double d1 = m.sigma[iz - 2] - m.sigma[iz - 1], d2 = m.sigma[iz - 1] - m.sigma[iz];
-            const double d = 0.5 * (d1 + d2);
-            const double sh2 = ((u_up - u_dn) * (u_up - u_dn) + (v_up - v_dn) * (v_up - v_dn)) / (d * d) /
-                               (dmean * dmean);
-            const double zb = m.oms[iz - 1] * a;
-            const double rho_up = densityJM(PL(s.TF, iz - 2, n, KC), PL(s.SF, iz - 2, n, KC), -zb, rho0);
-            const double rho_dn = densityJM(PL(s.TF, iz - 1, n, KC), PL(s.SF, iz - 1, n, KC), -zb, rho0);
-            const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
-            f[0] = (float)log10(sh2 + 1e-12);
-            f[1] = (float)asinh(n2 * 1e5);
-            f[2] = (float)log10(dmean);
-            f[3] = (float)f3;
-            f[4] = (float)m.sigma[nz - 1];
-            for (int j = 0; j < kMlpF; ++j) {
-                f[j] = (f[j] - nm.x_mean[j]) / nm.x_std[j];
-                ood = ood || !(f[j] >= nm.x_lo[j] && f[j] <= nm.x_hi[j]);
-            }
-        } else {
-            for (int j = 0; j < kMlpF; ++j) f[j] = 0.0f;
+    __syncthreads();
+    const int n = f.c0 + f.c, nz = f.k + 1;
+    float fx[kMlpF] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (f.on()) {
+        const double a = col_a[f.c], dmean = col_dm[f.c];
+        const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
+        const double u1 = PL(s.Unode, iz - 2, n, KC), v1 = PL(s.Vnode, iz - 2, n, KC);
+        const double u2 = PL(s.Unode, iz - 1, n, KC), v2 = PL(s.Vnode, iz - 1, n, KC);
+        const double d1 = m.sigma[iz - 2] - m.sigma[iz - 1], d2 = m.sigma[iz - 1] - m.sigma[iz];
+        const double d = 0.5 * (d1 + d2);
+        const double sh2 = ((u1 - u2) * (u1 - u2) + (v1 - v2) * (v1 - v2)) / (d * d) / (dmean * dmean);
+        const double zb = m.oms[iz - 1] * a;
+        const double rho_up = densityJM(PL(s.TF, iz - 2, n, KC), PL(s.SF, iz - 2, n, KC), -zb, rho0);
+        const double rho_dn = densityJM(PL(s.TF, iz - 1, n, KC), PL(s.SF, iz - 1, n, KC), -zb, rho0);
+        const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
+        fx[0] = (float)log10(sh2 + 1e-12);
+        fx[1] = (float)asinh(n2 * 1e5);
+        fx[2] = (float)log10(dmean);
+        fx[3] = (float)col_f3[f.c];
+        fx[4] = (float)m.sigma[nz - 1];
+        bool ood = false;
+        for (int j = 0; j < kMlpF; ++j) {
+            fx[j] = (fx[j] - nm.x_mean[j]) / nm.x_std[j];
+            ood = ood || !(fx[j] >= nm.x_lo[j] && fx[j] <= nm.x_hi[j]);
         }
-        for (int j = 0; j < kMlpF; ++j) xs[lane * kMlpF + j] = f[j];
-        __syncthreads();
-        for (int t = 0; t < 4; ++t) {
-            const fcgm::f32x4 y = fcgm::fwd_infer(frag, w, fcgm::load_bx(&xs[t * 16 * kMlpF], lane), lane);
-            if (lane < 16) {
-                ys[(t * 16 + lane) * 2] = y[0];
-                ys[(t * 16 + lane) * 2 + 1] = y[1];
-            }
-        }
-        __syncthreads();
-        if (valid) {
-            const double ls = (double)ys[lane * 2] * nm.y_std[0] + nm.y_mean[0];
-            const double lk = (double)ys[lane * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
-            double sv = pow(10.0, ls);
-            sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
-            PL(s.mx_bA, nz - 1, n, KC) = (R)sv;
-            PL(s.mx_bB, nz - 1, n, KC) = (R)pow(10.0, lk);
-        }
-        __syncthreads();
+        if (ood) atomicOr(&col_ood[f.c], 1);
     }
-    if (!valid) return;
-    flag[n] = ood ? 1 : 0;
+    for (int j = 0; j < kMlpF; ++j) xs[threadIdx.x * kMlpF + j] = fx[j];
+    __syncthreads();
+    // the wavefront's 64 samples as four tiles
+    const int lane = threadIdx.x & 63, wbase = threadIdx.x & ~63;
+    for (int t = 0; t < 4; ++t) {
+        const fcgm::f32x4 y = fcgm::fwd_infer(frag, w, fcgm::load_bx(&xs[(wbase + t * 16) * kMlpF], lane), lane);
+        if (lane < 16) {
+            ys[(wbase + t * 16 + lane) * 2] = y[0];
+            ys[(wbase + t * 16 + lane) * 2 + 1] = y[1];
+        }
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const bool ood = col_ood[f.c] != 0;
+    if (f.k == 0) flag[n] = ood ? 1 : 0;
     if (ood) return;
-    for (int k = 0; k < K; ++k) {
-        PL(s.snu, k, n, KC) = PL(s.mx_bA, k, n, KC);
-        PL(s.Kv, k, n, KC) = PL(s.mx_bB, k, n, KC);
-    }
+    const float ls = ys[threadIdx.x * 2] * nm.y_std[0] + nm.y_mean[0];
+    const float lk = ys[threadIdx.x * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
+    double sv = (double)exp10f(ls);
+    sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
+    PL(s.snu, nz - 1, n, KC) = (R)sv;
+    PL(s.Kv, nz - 1, n, KC) = (R)exp10f(lk);
 }
 
 __global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, int *count) {
