@@ -932,6 +932,7 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, doub
     int n = TID();
     if (n >= m.N) return;
     double ssh = 0.0;
+    #pragma unroll 4
     for (int j = m.n2e_ptr[n]; j < m.n2e_ptr[n + 1]; ++j) {
         int c = m.n2e[j];
         int ed = c >= 0 ? c : ~c;
@@ -1035,6 +1036,7 @@ __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict
     for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
     const double tvi = 1.0 / sa;
     double su = 0.0, sv = 0.0;
+    #pragma unroll 4
     for (int j = p0; j < p1; ++j) {
         int el = m.n2el[j];
         double ar = m.elem_area[el];
@@ -1617,6 +1619,72 @@ __device__ __forceinline__ double hi_vert(const DevMesh &m, const DevStateT<R> &
     return -Tm * w;
 }
 
+// hi_vert for two tracer columns X and Y at once: the QUICK weights depend
+// only on the geometry and on W, so they are formed once for both (the same
+// operations on the same operands as two hi_vert calls, hence the same bits).
+template <bool kChained, typename R>
+__device__ __forceinline__ void hi_vert2(const DevMesh &m, const DevStateT<R> &s, const R *X, const R *Y, int n,
+                                         int nz, double a, double &TmX, double &TmY, double &fx, double &fy) {
+    const int K = m.K, KC = K;
+    auto Wv = [&](int z) { return PL(s.W, z - 1, n, KC); };
+    auto Xv = [&](int z) { return PL(X, z - 1, n, KC) - 0.0; };
+    auto Yv = [&](int z) { return PL(Y, z - 1, n, KC) - 0.0; };
+    auto zb = [&](int z) { return m.oms[z - 1] * a; };
+    auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
+    if (nz == 1) {
+        const double w1 = Wv(1);
+        fx = -w1 * Xv(1);
+        fy = -w1 * Yv(1);
+        return;
+    }
+    if (nz == K) {
+        fx = 0.0;
+        fy = 0.0;
+        return;
+    }
+    const double w = Wv(nz);
+    if (w > 0.0) {
+        if (nz == K - 1) {
+            TmX = Xv(nz);
+            TmY = Yv(nz);
+        } else {
+            const double aa = zb(nz) - Zc(nz - 1), bb = Zc(nz) - zb(nz), cc = Zc(nz + 1) - zb(nz);
+            double dg, db;
+            if (kChained) {
+                dg = aa * bb / (cc + aa) / (bb - cc);
+                db = -aa * cc / (bb + aa) / (bb - cc);
+            } else {
+                dg = aa * bb / ((cc + aa) * (bb - cc));
+                db = -aa * cc / ((bb + aa) * (bb - cc));
+            }
+            const double da = 1.0 - dg - db;
+            TmX = Xv(nz - 1) * da + Xv(nz) * db + Xv(nz + 1) * dg;
+            TmY = Yv(nz - 1) * da + Yv(nz) * db + Yv(nz + 1) * dg;
+        }
+    }
+    if (w < 0.0) {
+        if (nz == 2) {
+            TmX = Xv(nz - 1);
+            TmY = Yv(nz - 1);
+        } else {
+            const double aa = Zc(nz) - zb(nz), bb = zb(nz) - Zc(nz - 1), cc = zb(nz) - Zc(nz - 2);
+            double dg, db;
+            if (kChained) {
+                dg = aa * bb / (cc + aa) / (bb - cc);
+                db = -aa * cc / (bb + aa) / (bb - cc);
+            } else {
+                dg = aa * bb / ((cc + aa) * (bb - cc));
+                db = -aa * cc / ((bb + aa) * (bb - cc));
+            }
+            const double da = 1.0 - dg - db;
+            TmX = Xv(nz) * da + Xv(nz - 1) * db + Xv(nz - 2) * dg;
+            TmY = Yv(nz) * da + Yv(nz - 1) * db + Yv(nz - 2) * dg;
+        }
+    }
+    fx = -TmX * w;
+    fy = -TmY * w;
+}
+
 // Element gradient of a node column at level k (tracer_gradient_elements,
 // fv_gradients.F90:180-205), the column value being PL(X) - sub.
 template <typename R>
@@ -1778,10 +1846,16 @@ __global__ __launch_bounds__(256) void k_tr_gradn_miura(DevMesh m, DevStateT<R> 
 
 // MUSCL edge value: linear reconstructions from both end nodes, 15 %
 // upwind-biased blend (fv_tracer.F90:104-122).
+// x/6 in the storage type: the reference's division in FP64; in the FP32
+// path a multiplication by the rounded reciprocal (within an ulp of the
+// quotient, at a fraction of the cost of an IEEE division)
+__device__ __forceinline__ double div6(double x) { return x / 6.0; }
+__device__ __forceinline__ float div6(float x) { return x * (1.0f / 6.0f); }
+
 template <typename R>
 __device__ __forceinline__ R muscl_mean(R T1, R T2, R dT, R dx, R dy, R g1x, R g1y, R g2x, R g2y, bool from2) {
-    const R Tm2 = T2 - (dT + dx * g2x + dy * R(kREarthD) * g2y) / R(6.0);
-    const R Tm1 = T1 + (dT + dx * g1x + dy * R(kREarthD) * g1y) / R(6.0);
+    const R Tm2 = T2 - div6(dT + dx * g2x + dy * R(kREarthD) * g2y);
+    const R Tm1 = T1 + div6(dT + dx * g1x + dy * R(kREarthD) * g1y);
     return from2 ? R(0.15) * Tm2 + R(0.85) * (Tm1 + Tm2) / R(2.0) : R(0.15) * Tm1 + R(0.85) * (Tm1 + Tm2) / R(2.0);
 }
 
@@ -1864,11 +1938,9 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
     // QUICK vertical fluxes at the layer's interfaces; when W is exactly zero
     // the reference's carried Tm multiplies a zero, so any finite start value
     // gives the same bits.
-    double TmT = 0.0, TmS = 0.0;
-    const double tvt = hi_vert<true>(m, s, s.abT, 0.0, n, k + 1, a, TmT);
-    const double tvb = hi_vert<true>(m, s, s.abT, 0.0, n, k + 2, a, TmT);
-    const double svt = hi_vert<true>(m, s, s.abS, 0.0, n, k + 1, a, TmS);
-    const double svb = hi_vert<true>(m, s, s.abS, 0.0, n, k + 2, a, TmS);
+    double TmT = 0.0, TmS = 0.0, tvt, tvb, svt, svb;
+    hi_vert2<true>(m, s, s.abT, s.abS, n, k + 1, a, TmT, TmS, tvt, svt);
+    hi_vert2<true>(m, s, s.abT, s.abS, n, k + 2, a, TmT, TmS, tvb, svb);
     const double tab = PL(s.abT, k, n, KC), sab = PL(s.abS, k, n, KC);
     const double wdiff = PL(s.W, k, n, KC) - PL(s.W, k + 1, n, KC);
     const double jc = a * m.dsig[k];
