@@ -89,6 +89,14 @@ inline size_t velrhs_bytes(const DevMesh &m, size_t esz) {
     return (size_t)cpb_of(m.L) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz));
 }
 inline size_t lds_p1adv(const DevMesh &m) { return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8; }
+// node-gather blocks: LDS records for every CSR slot of the block's nodes
+template <typename R>
+inline size_t lds_muscl(const fcg_ctx *c) { return (size_t)c->cap_n2e * (sizeof(EdgeGeo<R>) + 1); }
+template <typename R>
+inline size_t lds_vdiv(const fcg_ctx *c) { return (size_t)c->cap_n2e * sizeof(DivRec<R>); }
+inline size_t lds_gradn(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(GradRec); }
+inline size_t lds_el2n(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(AreaRec); }
+
 // largest LDS request of the tile kernels for this mesh and storage type
 inline size_t lds_max(const DevMesh &m, size_t esz) {
     size_t b = std::max(2 * tile_bytes(m, 8), tile_bytes(m, 8) + 2 * tile_bytes(m, esz));  // bpg, velvisc
@@ -134,6 +142,40 @@ int upload(fcg_ctx *ctx, T **p, const std::vector<T> &h) {
     int rc = dalloc(ctx, p, h.size());
     if (rc) return rc;
     if (!h.empty()) FCG_CHECK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FCG_OK;
+}
+
+// EdgeGeo<R> of every edge (see fcg_internal.h), uploaded to m.egeo
+template <typename R>
+int upload_egeo(fcg_ctx *ctx, const std::vector<int2> &ednodes, const std::vector<int2> &edtri,
+                const std::vector<double4> &ecross, const std::vector<double2> &edxdy,
+                const std::vector<double> &elem_cos) {
+    const int K = ctx->m.K;
+    std::vector<EdgeGeo<R>> g(ednodes.size());
+    for (size_t d = 0; d < g.size(); ++d) {
+        const int2 nd = ednodes[d], et = edtri[d];
+        const double4 c = ecross[d];
+        const double2 dd = edxdy[d];
+        EdgeGeo<R> &r = g[d];
+        r.n1K = nd.x * K;
+        r.n2K = nd.y * K;
+        r.e1 = et.x;
+        r.e2 = et.y;
+        r.e1K = et.x * K;
+        r.e2K = et.y >= 0 ? et.y * K : -1;
+        r.cx = (R)c.x;
+        r.cy = (R)c.y;
+        r.cz = (R)c.z;
+        r.cw = (R)c.w;
+        r.dy = (R)dd.y;
+        r.dxa = (R)(dd.x * (kREarthD * elem_cos[et.x]));
+        r.dxb = et.y >= 0 ? (R)(dd.x * (kREarthD * elem_cos[et.y])) : R(0);
+    }
+    void *p = nullptr;
+    int rc = dalloc_bytes(ctx, &p, g.size() * sizeof(EdgeGeo<R>));
+    if (rc) return rc;
+    FCG_CHECK(hipMemcpy(p, g.data(), g.size() * sizeof(EdgeGeo<R>), hipMemcpyHostToDevice));
+    ctx->m.egeo = p;
     return FCG_OK;
 }
 
@@ -221,7 +263,7 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
         hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         if (sub == 1) {
             hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk((size_t)m.E * m.L)), dim3(kBlk), 0, st, m, s, p.Dmin);
-            hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nblk((size_t)m.N * m.L)), dim3(kBlk), 0, st, m,
+            hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nflatblk(m.N, m.L)), dim3(kFlatT), lds_el2n(ctx), st, m,
                                (const R *)s.U_puls, (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
             hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3(nflatblk(m.E, m.L)), dim3(kFlatT), lds_p1adv(m), st, m, s);
         }
@@ -497,6 +539,14 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         n2el_ptr[n + 1] = n2el_ptr[n] + (int)n2el[n].size();
         n2el_idx.insert(n2el_idx.end(), n2el[n].begin(), n2el[n].end());
     }
+    {  // most CSR slots in one node-gather block of cpb = 256/L consecutive nodes
+        const int cpb = std::max(1, 256 / L);
+        for (int c0 = 0; c0 < N; c0 += cpb) {
+            const int c1 = std::min(N, c0 + cpb);
+            ctx->cap_n2e = std::max(ctx->cap_n2e, n2e_ptr[c1] - n2e_ptr[c0]);
+            ctx->cap_n2el = std::max(ctx->cap_n2el, n2el_ptr[c1] - n2el_ptr[c0]);
+        }
+    }
     // ---- node constants
     std::vector<double> depth(N), area(N), ac(N), atp1(N), atp2(N);
     std::vector<int> index(N), obidx(N, -1);
@@ -542,10 +592,25 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
 #define DA3(ptr, n) if ((rc = dalloc_bytes(ctx, (void **)&(ptr), (n) * ctx->esize))) { *out = ctx; return rc; }
     ctx->mixed = p.precision == FCG_MIXED;
     ctx->esize = ctx->mixed ? sizeof(float) : sizeof(double);
+    {  // column offsets of element nodes and node elements; per-edge gather operands
+        std::vector<int4> enK(E);
+        for (int e = 0; e < E; ++e) enK[e] = make_int4(en[e].x * K, en[e].y * K, en[e].z * K, en[e].w * K);
+        std::vector<int> n2elK(n2el_idx.size());
+        for (size_t j = 0; j < n2el_idx.size(); ++j) n2elK[j] = n2el_idx[j] * K;
+        UP(m.enK, enK) UP(m.n2elK, n2elK)
+        rc = ctx->mixed ? upload_egeo<float>(ctx, ednodes, edtri, ecross, edxdy, v_ec)
+                        : upload_egeo<double>(ctx, ednodes, edtri, ecross, edxdy, v_ec);
+        if (rc) {
+            *out = ctx;
+            return rc;
+        }
+    }
     {
         int lds_limit = 0;
         FCG_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-        if (m.K > kFlatT || lds_max(m, ctx->esize) > (size_t)lds_limit) {
+        const size_t gather = std::max(std::max(lds_muscl<double>(ctx), lds_vdiv<double>(ctx)),
+                                       std::max(lds_gradn(ctx), lds_el2n(ctx)));
+        if (m.K > kFlatT || std::max(lds_max(m, ctx->esize), gather) > (size_t)lds_limit) {
             fcg_fail(ctx, FCG_E_ARG, "nsigma too large for the LDS column tiles of this device");
             *out = ctx;
             return FCG_E_ARG;
@@ -884,7 +949,8 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
     // compute_el2nodes_3D(U_n, V_n, Unode, Vnode), src/fv_main.F90:2006
-    LAUNCHLV(ctx, k_el2nodes<R>, ctx->m.N, ctx->m.L, ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode);
+    LAUNCH_EX(ctx, k_el2nodes<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, lds_el2n(ctx), ctx->m, (const R *)s.U_n,
+              (const R *)s.V_n, s.Unode, s.Vnode);
     if (p.closure == FCG_CLOSURE_MLP) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
@@ -987,7 +1053,7 @@ extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
 
 template <typename R>
 static int vert_vel_sigma_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCHLV(ctx, k_vertvel_div<R>, ctx->m.N, ctx->m.L, ctx->m, s);
+    LAUNCH_EX(ctx, k_vertvel_div<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, lds_vdiv<R>(ctx), ctx->m, s);
     LAUNCH_EX(ctx, k_vertvel_scan<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, flat_bytes(ctx->m.L, sizeof(R)), ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
@@ -1017,8 +1083,9 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCHLV(ctx, k_tr_upwind<R>, m.N, m.L, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 2 && p.muscl_call) {
         LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
-        LAUNCHLV(ctx, k_tr_gradn_muscl<R>, m.N, m.L, m, s);
-        LAUNCHLV(ctx, k_tr_muscl<R>, m.N, m.L, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
+        LAUNCH_EX(ctx, k_tr_gradn_muscl<R>, nflatblk(m.N, m.L), kFlatT, lds_gradn(ctx), m, s);
+        LAUNCH_EX(ctx, k_tr_muscl<R>, nflatblk(m.N, m.L), kFlatT, lds_muscl<R>(ctx), m, s, p.dt, p.Dmin, relax,
+                  p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 4) {
         LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
         LAUNCHLV(ctx, k_tr_gradn_miura<R>, m.N, m.L, m, s);
@@ -1138,9 +1205,8 @@ extern "C" int fcg_mlp_collect(fcg_ctx *ctx, float *X, float *T, long cap, int s
 // the layer-parallel kernels: 64 entities x 1, 2 or 4 layers per block).
 extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
     if (!ctx || !kernel || !(block == 64 || block == 128 || block == 256)) return FCG_E_ARG;
-    static const char *tunable[] = {"k_d3end", "k_el2nodes", "k_velgrad3d",
-                                    "k_velrhs", "k_jac_node", "k_je", "k_vertvel_div", "k_tr_ab", "k_tr_upwind",
-                                    "k_tr_gradn_muscl", "k_tr_muscl", "k_tr_gradn_miura", "k_tr_miura"};
+    static const char *tunable[] = {"k_d3end", "k_velgrad3d", "k_jac_node", "k_je", "k_tr_ab", "k_tr_upwind",
+                                    "k_tr_gradn_miura", "k_tr_miura"};
     for (const char *t : tunable)
         if (!std::strcmp(t, kernel)) {
             ctx->launch_block[kernel] = block;

@@ -32,6 +32,37 @@ constexpr double kPrNum = 0.0001;
 constexpr double kBetaScale = 3.9;
 constexpr int kEmpty = INT32_MIN;     // unused slot in a fixed-width edge list
 
+// Per-edge operands of the node gathers over edges (tracer fluxes, transport
+// divergence) in the storage type R: column offsets (entity id * K) and ids of
+// the edge's nodes and elements, the cross distances edge_cross_dxdy, the
+// edge's dy and the metric x-length dx*R_earth*cos of each segment, formed
+// at set-up with the same operations the flux formulas apply to them.
+template <typename R>
+struct EdgeGeo {
+    int n1K, n2K, e1K, e2K;   // e2K < 0: boundary edge (no second element)
+    int e1, e2;
+    R cx, cy, cz, cw, dy, dxa, dxb;
+};
+
+// LDS records of the node-gather blocks (one per CSR slot of the block's
+// nodes): element operands of the node-averaged tracer gradient, element
+// column offset and area of compute_el2nodes_3D, edge operands of the
+// transport divergence (with the element's filtered 2-D velocity).
+struct GradRec {
+    int qK[4];            // column offsets of the element's nodes
+    double ea;            // elem_area
+    double gx[4], gy[4];  // gradient_sca
+};
+struct AreaRec {
+    int elK;
+    double ar;
+};
+template <typename R>
+struct DivRec {
+    int e1K, e2K, pos;    // pos: node is edge_nodes(1)
+    R cx, cy, cz, cw, vf1, uf1, vf2, uf2;
+};
+
 // Device mirror of the mesh and state of one model instance.
 struct DevMesh {
     int N = 0, E = 0, D = 0, Din = 0, L = 0, K = 0, nobn = 0;
@@ -57,6 +88,9 @@ struct DevMesh {
     // CSR gathers
     int *n2e_ptr = nullptr, *n2e = nullptr;   // node -> edges, ascending; e>=0: node is edge_nodes(1) (+), ~e: (2) (-)
     int *n2el_ptr = nullptr, *n2el = nullptr; // node -> elements, ascending (nod_in_elem2D)
+    int *n2elK = nullptr;      // n2el * K: column offsets of those elements
+    int4 *enK = nullptr;       // en * K: column offsets of the element's nodes
+    void *egeo = nullptr;      // EdgeGeo<R>[D] of the context's storage type
     // ensemble members: nodes and elements of one member are contiguous
     int nmemb = 1;
     int *memb = nullptr;       // N: member of each node
@@ -148,6 +182,8 @@ struct fcg_ctx {
     hipGraphExec_t bt_graph = nullptr;
     bool bt_graph_ok = false;
     int red_blocks = 0;
+    // node-gather blocks (cpb = 256/L nodes): most CSR slots of any block
+    int cap_n2e = 0, cap_n2el = 0;
     // learned closure
     fcg_mlp *mlp = nullptr;      // attached network (not owned)
     int *mlp_flag = nullptr;     // N: column sent back to d3_end this step

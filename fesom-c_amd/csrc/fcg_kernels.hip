@@ -176,6 +176,15 @@ __device__ __forceinline__ ColFlat col_flat(int ents, int S) {
     return f;
 }
 
+// Node-gather blocks: the column-aligned flat blocks of the node kernels
+// first copy the per-edge (or per-element) operands of all their nodes' CSR
+// slots -- one contiguous range of the gather lists -- into LDS, one load per
+// slot, so the per-level threads of a node read them from LDS instead of each
+// loading the same records through the vector memory path.
+__device__ __forceinline__ int2 csr_block(const int *ptr, const ColFlat &f) {
+    return make_int2(ptr[f.c0], ptr[f.c0 + f.nc]);
+}
+
 // ---------------------------------------------------------------------------
 // pressure: hydrostatic column scan, src/fv_pressure_new.F90:91-107
 // ---------------------------------------------------------------------------
@@ -1024,24 +1033,35 @@ __global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevStateT<R> s, doub
 }
 
 // compute_el2nodes_3D (src/fv_utilit.F90:669-689), one thread per node and
-// layer: area-weighted mean over the node's elements in ascending id.
+// layer on node-gather blocks: area-weighted mean over the node's elements
+// in ascending id.
 template <typename R>
 __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict__ Ue, const R *__restrict__ Ve,
                            R *__restrict__ Un, R *__restrict__ Vn) {
-    int n, k;
-    if (!ent_layer(m.N, m.L, n, k)) return;
+    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    const ColFlat f = col_flat(m.N, m.L);
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2el_ptr, f);
+    AreaRec *rec = (AreaRec *)lds_;
+    for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
+        AreaRec r;
+        r.elK = m.n2elK[jr.x + t];
+        r.ar = m.elem_area[m.n2el[jr.x + t]];
+        rec[t] = r;
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, k = f.k;
+    const int p0 = m.n2el_ptr[n] - jr.x, p1 = m.n2el_ptr[n + 1] - jr.x;
     double sa = 0.0;
-    for (int j = p0; j < p1; ++j) sa = sa + m.elem_area[m.n2el[j]];
+    for (int j = p0; j < p1; ++j) sa = sa + rec[j].ar;
     const double tvi = 1.0 / sa;
     double su = 0.0, sv = 0.0;
-    #pragma unroll 4
     for (int j = p0; j < p1; ++j) {
-        int el = m.n2el[j];
-        double ar = m.elem_area[el];
-        su = su + PL(Ue, k, el, KC) * ar;
-        sv = sv + PL(Ve, k, el, KC) * ar;
+        const double ar = rec[j].ar;
+        su = su + Ue[rec[j].elK + k] * ar;
+        sv = sv + Ve[rec[j].elK + k] * ar;
     }
     PL(Un, k, n, KC) = su * tvi;
     PL(Vn, k, n, KC) = sv * tvi;
@@ -1311,26 +1331,46 @@ __global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, dou
 // gathered over the node's edges in ascending id (354-437), stored in W.
 template <typename R>
 __global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevStateT<R> s) {
+    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    int n, k;
-    if (!ent_layer(m.N, m.L, n, k)) return;
-    const int N = m.N, E = m.E;
-    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const ColFlat f = col_flat(m.N, m.L);
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2e_ptr, f);
+    DivRec<R> *rec = (DivRec<R> *)lds_;
+    {
+        const EdgeGeo<R> *eg = (const EdgeGeo<R> *)m.egeo;
+        for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
+            const int cc = m.n2e[jr.x + t];
+            const EdgeGeo<R> g = eg[cc >= 0 ? cc : ~cc];
+            DivRec<R> r;
+            r.e1K = g.e1K;
+            r.e2K = g.e2K;
+            r.pos = cc >= 0;
+            r.cx = g.cx;
+            r.cy = g.cy;
+            r.cz = g.cz;
+            r.cw = g.cw;
+            r.vf1 = (R)s.V_filt[g.e1];
+            r.uf1 = (R)s.U_filt[g.e1];
+            r.vf2 = g.e2K >= 0 ? (R)s.V_filt[g.e2] : R(0);
+            r.uf2 = g.e2K >= 0 ? (R)s.U_filt[g.e2] : R(0);
+            rec[t] = r;
+        }
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, k = f.k;
+    const int p0 = m.n2e_ptr[n] - jr.x, p1 = m.n2e_ptr[n + 1] - jr.x;
     R c = R(0);  // storage type: FP32 transport fluxes in the mixed-precision path
     for (int j = p0; j < p1; ++j) {
-        const int cc = m.n2e[j];
-        const int ed = cc >= 0 ? cc : ~cc;
-        const int2 et = m.edtri[ed];
-        const double4 cd = m.ecross[ed];
-        const int e1 = et.x;
-        R f = PL(s.Je, k, e1, KC) * ((PL(s.V_nf, k, e1, KC) - (R)s.V_filt[e1]) * (R)cd.x
-                                    - (PL(s.U_nf, k, e1, KC) - (R)s.U_filt[e1]) * (R)cd.y);
-        if (et.y >= 0) {
-            const int e2 = et.y;
-            f = f + PL(s.Je, k, e2, KC) * (-(PL(s.V_nf, k, e2, KC) - (R)s.V_filt[e2]) * (R)cd.z
-                                          + (PL(s.U_nf, k, e2, KC) - (R)s.U_filt[e2]) * (R)cd.w);
+        const DivRec<R> &g = rec[j];
+        const int i1 = g.e1K + k;
+        R fl = s.Je[i1] * ((s.V_nf[i1] - g.vf1) * g.cx - (s.U_nf[i1] - g.uf1) * g.cy);
+        if (g.e2K >= 0) {
+            const int i2 = g.e2K + k;
+            fl = fl + s.Je[i2] * (-(s.V_nf[i2] - g.vf2) * g.cz + (s.U_nf[i2] - g.uf2) * g.cw);
         }
-        c = (cc >= 0) ? c + f : c - f;
+        c = g.pos ? c + fl : c - fl;
     }
     PL(s.W, k, n, KC) = c;
 }
@@ -1691,9 +1731,9 @@ template <typename R>
 __device__ __forceinline__ double2 elem_grad(const DevMesh &m, const R *X, double sub, int el, int k) {
     const int KC = m.K;
     const int N = m.N;
-    const int4 q = m.en[el];
-    const double v0 = PL(X, k, q.x, KC) - sub, v1 = PL(X, k, q.y, KC) - sub;
-    const double v2 = PL(X, k, q.z, KC) - sub, v3 = PL(X, k, q.w, KC) - sub;
+    const int4 q = m.enK[el];  // column offsets of the element's nodes
+    const double v0 = X[q.x + k] - sub, v1 = X[q.y + k] - sub;
+    const double v2 = X[q.z + k] - sub, v3 = X[q.w + k] - sub;
     return make_double2(sum4(m.gsx[el], v0, v1, v2, v3), sum4(m.gsy[el], v0, v1, v2, v3));
 }
 
@@ -1791,24 +1831,59 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
 // reconstruction (fill_up_dn_grad node-average branch,
 // fv_gradients.F90:263-323): area-weighted mean over the node's elements in
 // ascending element id, divided by the summed area.
+__device__ __forceinline__ double sum4a(const double *w, double x0, double x1, double x2, double x3) {
+    double r = w[0] * x0;
+    r = r + w[1] * x1;
+    r = r + w[2] * x2;
+    r = r + w[3] * x3;
+    return r;
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> s) {
+    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    int n, k;
-    if (!ent_layer(m.N, m.L, n, k)) return;
-    const int N = m.N;
-    const int p0 = m.n2el_ptr[n], p1 = m.n2el_ptr[n + 1];
+    const ColFlat f = col_flat(m.N, m.L);
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2el_ptr, f);
+    GradRec *rec = (GradRec *)lds_;
+    for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
+        const int el = m.n2el[jr.x + t];
+        const int4 q = m.enK[el];
+        const double4 gx = m.gsx[el], gy = m.gsy[el];
+        GradRec r;
+        r.qK[0] = q.x;
+        r.qK[1] = q.y;
+        r.qK[2] = q.z;
+        r.qK[3] = q.w;
+        r.ea = m.elem_area[el];
+        r.gx[0] = gx.x;
+        r.gx[1] = gx.y;
+        r.gx[2] = gx.z;
+        r.gx[3] = gx.w;
+        r.gy[0] = gy.x;
+        r.gy[1] = gy.y;
+        r.gy[2] = gy.z;
+        r.gy[3] = gy.w;
+        rec[t] = r;
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, k = f.k;
+    const int p0 = m.n2el_ptr[n] - jr.x, p1 = m.n2el_ptr[n + 1] - jr.x;
     double tvol = 0.0, tx = 0.0, ty = 0.0, sx = 0.0, sy = 0.0;
     for (int j = p0; j < p1; ++j) {
-        const int el = m.n2el[j];
-        const double ea = m.elem_area[el];
-        const double2 gt = elem_grad(m, s.abT, 0.0, el, k);
-        const double2 gs = elem_grad(m, s.abS, 0.0, el, k);
+        const GradRec &r = rec[j];
+        const double ea = r.ea;
+        const double t0 = s.abT[r.qK[0] + k] - 0.0, t1 = s.abT[r.qK[1] + k] - 0.0;
+        const double t2 = s.abT[r.qK[2] + k] - 0.0, t3 = s.abT[r.qK[3] + k] - 0.0;
+        const double s0 = s.abS[r.qK[0] + k] - 0.0, s1 = s.abS[r.qK[1] + k] - 0.0;
+        const double s2 = s.abS[r.qK[2] + k] - 0.0, s3 = s.abS[r.qK[3] + k] - 0.0;
         tvol = tvol + ea;
-        tx = tx + gt.x * ea;
-        ty = ty + gt.y * ea;
-        sx = sx + gs.x * ea;
-        sy = sy + gs.y * ea;
+        tx = tx + sum4a(r.gx, t0, t1, t2, t3) * ea;
+        ty = ty + sum4a(r.gy, t0, t1, t2, t3) * ea;
+        sx = sx + sum4a(r.gx, s0, s1, s2, s3) * ea;
+        sy = sy + sum4a(r.gy, s0, s1, s2, s3) * ea;
     }
     PL(s.gTx, k, n, KC) = tx / tvol;
     PL(s.gTy, k, n, KC) = ty / tvol;
@@ -1860,67 +1935,82 @@ __device__ __forceinline__ R muscl_mean(R T1, R T2, R dT, R dx, R dy, R g1x, R g
 }
 
 // solve_tracer_muscl for T and S (fv_tracer.F90:8-379), one thread per node
-// and layer.  Per edge the first segment uses the filtered velocity of
-// element 1; the second segment's flux uses the unfiltered U_n/V_n of
-// element 2 (its upwind test and the volume flux use the filtered one), as
-// the reference does.  The implicit vertical diffusion follows in k_tr_vdiff.
+// and layer on node-gather blocks.  Per edge the first segment uses the
+// filtered velocity of element 1; the second segment's flux uses the
+// unfiltered U_n/V_n of element 2 (its upwind test and the volume flux use
+// the filtered one), as the reference does.  The node's own T, S and
+// gradients are loaded once; per edge only the other end's.  The implicit
+// vertical diffusion follows in k_tr_vdiff.
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax, double clim_relax,
                            double relax2clim_ac) {
+    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    int n, k;
-    if (!ent_layer(m.N, m.L, n, k)) return;
-    const int N = m.N, E = m.E;
+    const ColFlat f = col_flat(m.N, m.L);
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2e_ptr, f);
+    EdgeGeo<R> *rec = (EdgeGeo<R> *)lds_;
+    unsigned char *first = (unsigned char *)(rec + (jr.y - jr.x));  // node is edge_nodes(1)
+    {
+        const EdgeGeo<R> *eg = (const EdgeGeo<R> *)m.egeo;
+        for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
+            const int cc = m.n2e[jr.x + t];
+            rec[t] = eg[cc >= 0 ? cc : ~cc];
+            first[t] = cc >= 0;
+        }
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, k = f.k, nk = n * KC + k;
     const int mb = m.memb[n];
     const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const R To = s.abT[nk], So = s.abS[nk], gTxo = s.gTx[nk], gTyo = s.gTy[nk], gSxo = s.gSx[nk], gSyo = s.gSy[nk];
     // horizontal fluxes in the storage type (FP32 in the mixed-precision path)
     R rt = R(0), rs = R(0), rc = R(0);
-    for (int j = p0; j < p1; ++j) {
-        const int cc = m.n2e[j];
-        const int ed = cc >= 0 ? cc : ~cc;
-        const int2 nd = m.ednodes[ed];
-        const int2 et = m.edtri[ed];
-        const double4 cdd = m.ecross[ed];
-        const double2 ddd = m.edxdy[ed];
-        const R cdx = (R)cdd.x, cdy = (R)cdd.y, cdz = (R)cdd.z, cdw = (R)cdd.w, ddy = (R)ddd.y;
-        const int e1 = et.x;
-        const R T1 = PL(s.abT, k, nd.x, KC), T2 = PL(s.abT, k, nd.y, KC);
-        const R S1 = PL(s.abS, k, nd.x, KC), S2 = PL(s.abS, k, nd.y, KC);
-        const R gT1x = PL(s.gTx, k, nd.x, KC), gT2x = PL(s.gTx, k, nd.y, KC);
-        const R gT1y = PL(s.gTy, k, nd.x, KC), gT2y = PL(s.gTy, k, nd.y, KC);
-        const R gS1x = PL(s.gSx, k, nd.x, KC), gS2x = PL(s.gSx, k, nd.y, KC);
-        const R gS1y = PL(s.gSy, k, nd.x, KC), gS2y = PL(s.gSy, k, nd.y, KC);
+    for (int j = p0 - jr.x; j < p1 - jr.x; ++j) {
+        const EdgeGeo<R> &g = rec[j];
+        const bool own1 = first[j];
+        const R cdx = g.cx, cdy = g.cy, cdz = g.cz, cdw = g.cw, ddy = g.dy;
+        const int ao = (own1 ? g.n2K : g.n1K) + k, i1 = g.e1K + k;
+        const R Tx = s.abT[ao], Sx = s.abS[ao];
+        const R gTxx = s.gTx[ao], gTyx = s.gTy[ao], gSxx = s.gSx[ao], gSyx = s.gSy[ao];
+        const R T1 = own1 ? To : Tx, T2 = own1 ? Tx : To;
+        const R S1 = own1 ? So : Sx, S2 = own1 ? Sx : So;
+        const R gT1x = own1 ? gTxo : gTxx, gT2x = own1 ? gTxx : gTxo;
+        const R gT1y = own1 ? gTyo : gTyx, gT2y = own1 ? gTyx : gTyo;
+        const R gS1x = own1 ? gSxo : gSxx, gS2x = own1 ? gSxx : gSxo;
+        const R gS1y = own1 ? gSyo : gSyx, gS2y = own1 ? gSyx : gSyo;
         const R dTt = R(2.0) * (T2 - T1), dTs = R(2.0) * (S2 - S1);
         // first segment
-        const R dxa = (R)(ddd.x * (kREarthD * m.elem_cos[e1]));
-        const R u1 = PL(s.U_nf, k, e1, KC), v1 = PL(s.V_nf, k, e1, KC), J1 = PL(s.Je, k, e1, KC);
+        const R dxa = g.dxa;
+        const R u1 = s.U_nf[i1], v1 = s.V_nf[i1], J1 = s.Je[i1];
         const bool f2 = v1 * cdx - u1 * cdy > R(0);
         const R mT = muscl_mean(T1, T2, dTt, dxa, ddy, gT1x, gT1y, gT2x, gT2y, f2);
         const R mS = muscl_mean(S1, S2, dTs, dxa, ddy, gS1x, gS1y, gS2x, gS2y, f2);
         const R cT = (J1 * v1 * mT) * cdx - (J1 * u1 * mT) * cdy;
         const R cS = (J1 * v1 * mS) * cdx - (J1 * u1 * mS) * cdy;
         const R cV = J1 * v1 * cdx - J1 * u1 * cdy;
-        if (cc >= 0) {
+        if (own1) {
             rt = rt + cT;
             rs = rs + cS;
         } else {
             rt = rt - cT;
             rs = rs - cS;
         }
-        if (et.y >= 0) {  // second segment
-            const int e2 = et.y;
-            const R dxb = (R)(ddd.x * (kREarthD * m.elem_cos[e2]));
-            const R u2f = PL(s.U_nf, k, e2, KC), v2f = PL(s.V_nf, k, e2, KC);
-            const R u2 = PL(s.U_n, k, e2, KC), v2 = PL(s.V_n, k, e2, KC), J2 = PL(s.Je, k, e2, KC);
+        if (g.e2K >= 0) {  // second segment
+            const int i2 = g.e2K + k;
+            const R dxb = g.dxb;
+            const R u2f = s.U_nf[i2], v2f = s.V_nf[i2];
+            const R u2 = s.U_n[i2], v2 = s.V_n[i2], J2 = s.Je[i2];
             const bool g2 = v2f * cdz - u2f * cdw < R(0);
             const R nT = muscl_mean(T1, T2, dTt, dxb, ddy, gT1x, gT1y, gT2x, gT2y, g2);
             const R nS = muscl_mean(S1, S2, dTs, dxb, ddy, gS1x, gS1y, gS2x, gS2y, g2);
             const R c2T = -(J2 * v2 * nT) * cdz + (J2 * u2 * nT) * cdw;
             const R c2S = -(J2 * v2 * nS) * cdz + (J2 * u2 * nS) * cdw;
             const R c2V = -J2 * v2f * cdz + J2 * u2f * cdw;
-            if (cc >= 0) {
+            if (own1) {
                 rt = rt + c2T;
                 rs = rs + c2S;
                 rc = rc + cV;
@@ -1932,7 +2022,7 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
                 rc = rc - c2V;
             }
         } else {
-            rc = (cc >= 0) ? rc + cV : rc - cV;
+            rc = own1 ? rc + cV : rc - cV;
         }
     }
     // QUICK vertical fluxes at the layer's interfaces; when W is exactly zero
@@ -1941,7 +2031,7 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
     double TmT = 0.0, TmS = 0.0, tvt, tvb, svt, svb;
     hi_vert2<true>(m, s, s.abT, s.abS, n, k + 1, a, TmT, TmS, tvt, svt);
     hi_vert2<true>(m, s, s.abT, s.abS, n, k + 2, a, TmT, TmS, tvb, svb);
-    const double tab = PL(s.abT, k, n, KC), sab = PL(s.abS, k, n, KC);
+    const double tab = To, sab = So;
     const double wdiff = PL(s.W, k, n, KC) - PL(s.W, k + 1, n, KC);
     const double jc = a * m.dsig[k];
     const double aj = m.area[n] * jc;
