@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import json
 
-LAYER_KERNELS = ["k_tr_upwind", "k_tr_gradn_miura", "k_tr_miura", "k_velgrad3d", "k_je", "k_tr_ab"]
+LAYER_KERNELS = ["k_tr_upwind", "k_tr_gradn_miura", "k_tr_miura", "k_tr_ab"]
 COLUMN_KERNELS = ["k_d3end", "k_jac_node"]
 CANDIDATES = (64, 128, 256)
 

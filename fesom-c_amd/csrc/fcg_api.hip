@@ -88,7 +88,9 @@ inline size_t flat_bytes(int S, size_t esz) { return (size_t)cpb_of(S) * S * esz
 inline size_t velrhs_bytes(const DevMesh &m, size_t esz) {
     return (size_t)cpb_of(m.L) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz));
 }
-inline size_t lds_p1adv(const DevMesh &m) { return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8; }
+inline size_t lds_p1adv(const DevMesh &m) {
+    return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8 + 4 * (size_t)cpb_of(m.L) * sizeof(P1Rec);
+}
 // node-gather blocks: LDS records for every CSR slot of the block's nodes
 template <typename R>
 inline size_t lds_muscl(const fcg_ctx *c) { return (size_t)c->cap_n2e * (sizeof(EdgeGeo<R>) + 1); }
@@ -262,7 +264,8 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
         hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide);
         hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         if (sub == 1) {
-            hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk((size_t)m.E * m.L)), dim3(kBlk), 0, st, m, s, p.Dmin);
+            hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nflatblk(m.E, m.L)), dim3(kFlatT), 3 * cpb_of(m.L) * sizeof(double),
+                               st, m, s, p.Dmin);
             hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nflatblk(m.N, m.L)), dim3(kFlatT), lds_el2n(ctx), st, m,
                                (const R *)s.U_puls, (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
             hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3(nflatblk(m.E, m.L)), dim3(kFlatT), lds_p1adv(m), st, m, s);
@@ -684,9 +687,11 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         FCG_CHECK(hipMemcpy(s.mask_wd, mk.data(), E * 8, hipMemcpyHostToDevice));
     }
     if (ctx->mixed)
-        hipLaunchKernelGGL(fcgk::k_je<float>, dim3(nblk((size_t)E * L)), dim3(kBlk), 0, ctx->stream, m, retype<float>(s));
+        hipLaunchKernelGGL(fcgk::k_je<float>, dim3(nflatblk(E, L)), dim3(kFlatT), 8 * cpb_of(L) * sizeof(double),
+                           ctx->stream, m, retype<float>(s));
     else
-        hipLaunchKernelGGL(fcgk::k_je<double>, dim3(nblk((size_t)E * L)), dim3(kBlk), 0, ctx->stream, m, s);
+        hipLaunchKernelGGL(fcgk::k_je<double>, dim3(nflatblk(E, L)), dim3(kFlatT), 8 * cpb_of(L) * sizeof(double),
+                           ctx->stream, m, s);
     FCG_CHECK(hipGetLastError());
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->time_jd = p.time_jd0;
@@ -981,7 +986,7 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         ff = 1.0;
         ctx->lfirst = 0;
     }
-    LAUNCHLV(ctx, k_velgrad3d<R>, ctx->m.E, ctx->m.L, ctx->m, s);
+    LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec), ctx->m, s);
     LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, velrhs_bytes(ctx->m, sizeof(R)), ctx->m, s, p.dt,
               ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0);
     if (p.im_vert_visc_fic)
@@ -1027,7 +1032,7 @@ extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
 template <typename R>
 static int jacobian_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     LAUNCH1(ctx, k_jac_node<R>, ctx->m.N, ctx->m, s, ctx->p.Dmin);
-    LAUNCHLV(ctx, k_je<R>, ctx->m.E, ctx->m.L, ctx->m, s);
+    LAUNCH_EX(ctx, k_je<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, 8 * cpb_of(ctx->m.L) * sizeof(double), ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -1040,7 +1045,7 @@ extern "C" int fcg_jacobian(fcg_ctx *ctx) {
 template <typename R>
 static int update_3d_vel_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     LAUNCH_EX(ctx, k_update3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT,
-              2 * flat_bytes(ctx->m.L, 8) + 4 * (size_t)cpb_of(ctx->m.L) * 8, ctx->m, s, ctx->p.Dmin,
+              2 * flat_bytes(ctx->m.L, 8) + 14 * (size_t)cpb_of(ctx->m.L) * 8, ctx->m, s, ctx->p.Dmin,
               1.0 / (double)ctx->p.Mt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
@@ -1205,8 +1210,8 @@ extern "C" int fcg_mlp_collect(fcg_ctx *ctx, float *X, float *T, long cap, int s
 // the layer-parallel kernels: 64 entities x 1, 2 or 4 layers per block).
 extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
     if (!ctx || !kernel || !(block == 64 || block == 128 || block == 256)) return FCG_E_ARG;
-    static const char *tunable[] = {"k_d3end", "k_velgrad3d", "k_jac_node", "k_je", "k_tr_ab", "k_tr_upwind",
-                                    "k_tr_gradn_miura", "k_tr_miura"};
+    static const char *tunable[] = {"k_d3end", "k_jac_node", "k_tr_ab", "k_tr_upwind", "k_tr_gradn_miura",
+                                    "k_tr_miura"};
     for (const char *t : tunable)
         if (!std::strcmp(t, kernel)) {
             ctx->launch_block[kernel] = block;

@@ -63,6 +63,17 @@ struct DivRec {
     R cx, cy, cz, cw, vf1, uf1, vf2, uf2;
 };
 
+// per-column LDS records of vel_gradients (3-D) and of the P1 3D-to-2D
+// advection (per internal edge slot)
+struct VelGradRec {
+    int jend, nbK[4];            // neighbour column offsets, < 0: free-slip boundary side
+    double gx[4], gy[4], zc0[4], zc1[4];
+};
+struct P1Rec {
+    int n1K, n2K;                // column offsets of the edge's nodes; n1K < 0: no edge in this slot
+    double xe, ye, an1, an2;
+};
+
 // Device mirror of the mesh and state of one model instance.
 struct DevMesh {
     int N = 0, E = 0, D = 0, Din = 0, L = 0, K = 0, nobn = 0;

@@ -25,6 +25,14 @@ __device__ __forceinline__ double sum4(const double4 w, double x0, double x1, do
     return s;
 }
 
+__device__ __forceinline__ double sum4a(const double *w, double x0, double x1, double x2, double x3) {
+    double r = w[0] * x0;
+    r = r + w[1] * x1;
+    r = r + w[2] * x2;
+    r = r + w[3] * x3;
+    return r;
+}
+
 __device__ __forceinline__ double dmaxr(double a, double b) { return a > b ? a : b; }
 
 // densityJM: src/fv_pressure_new.F90:340-414 (un-suffixed literals are REAL(4))
@@ -463,39 +471,63 @@ __global__ __launch_bounds__(256) void k_d3end_fin(DevMesh m, DevStateT<R> s, do
 // ---------------------------------------------------------------------------
 // vel_gradients (3-D, free slip): src/fv_gradients.F90:2-93
 // ---------------------------------------------------------------------------
+// vel_gradients (3-D, free slip) on column-aligned blocks: the element's
+// gradient_vec, neighbours and boundary-side metric are read once per column
+// into LDS.
 template <typename R>
 __global__ __launch_bounds__(256) void k_velgrad3d(DevMesh m, DevStateT<R> s) {
-    int e, k;
-    if (!ent_layer(m.E, m.L, e, k)) return;
+    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    const int4 q = m.en[e];
-    const int jend = (q.x == q.w) ? 3 : 4;
-    const int4 nb = m.elnb[e];
-    const int4 ed = m.eledges[e];
-    const double4 gx = m.gvx[e], gy = m.gvy[e];
-    const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
-    const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
-    const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
-    const double ec = m.elem_cos[e];
+    const ColFlat f = col_flat(m.E, m.L);
+    VelGradRec *rec = (VelGradRec *)lds_;
+    if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {
+        const int e = f.c0 + threadIdx.x;
+        const int4 q = m.en[e];
+        const int4 nb = m.elnb[e];
+        const int4 ed = m.eledges[e];
+        const double4 gx = m.gvx[e], gy = m.gvy[e];
+        const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
+        const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
+        const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
+        const double ec = m.elem_cos[e];
+        VelGradRec r;
+        r.jend = (q.x == q.w) ? 3 : 4;
+        for (int j = 0; j < 4; ++j) {
+            r.gx[j] = gxs[j];
+            r.gy[j] = gys[j];
+            r.nbK[j] = nbs[j] >= 0 ? nbs[j] * KC : -1;
+            r.zc0[j] = 0.0;
+            r.zc1[j] = 0.0;
+            if (j < r.jend && nbs[j] < 0) {
+                const double2 dd = m.edxdy[eds[j]];
+                r.zc0[j] = dd.x * ec;
+                r.zc1[j] = dd.y;
+            }
+        }
+        rec[threadIdx.x] = r;
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int e = f.c0 + f.c, k = f.k;
+    const VelGradRec &r = rec[f.c];
     const double u = PL(s.U_n, k, e, KC), v = PL(s.V_n, k, e, KC);
     double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
-    for (int j = 0; j < jend; ++j) {
-        int e2 = nbs[j];
-        if (e2 >= 0) {
-            double du = PL(s.U_n, k, e2, KC) - u;
-            double dv = PL(s.V_n, k, e2, KC) - v;
-            ux = ux + gxs[j] * du;
-            uy = uy + gys[j] * du;
-            vx = vx + gxs[j] * dv;
-            vy = vy + gys[j] * dv;
+    for (int j = 0; j < r.jend; ++j) {
+        if (r.nbK[j] >= 0) {
+            double du = s.U_n[r.nbK[j] + k] - u;
+            double dv = s.V_n[r.nbK[j] + k] - v;
+            ux = ux + r.gx[j] * du;
+            uy = uy + r.gy[j] * du;
+            vx = vx + r.gx[j] * dv;
+            vy = vy + r.gy[j] * dv;
         } else {
-            double2 dd = m.edxdy[eds[j]];
-            double zc0 = dd.x * ec, zc1 = dd.y;
+            const double zc0 = r.zc0[j], zc1 = r.zc1[j];
             double un = -2 * (u * zc1 - v * zc0) / (zc0 * zc0 + zc1 * zc1);
-            ux = ux + gxs[j] * un * zc1;
-            uy = uy + gys[j] * un * zc1;
-            vx = vx - gxs[j] * un * zc0;
-            vy = vy - gys[j] * un * zc0;
+            ux = ux + r.gx[j] * un * zc1;
+            uy = uy + r.gy[j] * un * zc1;
+            vx = vx - r.gx[j] * un * zc0;
+            vy = vy - r.gy[j] * un * zc0;
         }
     }
     PL(s.gux, k, e, KC) = ux;
@@ -625,7 +657,7 @@ __device__ __forceinline__ void uvert(const DevMesh &m, const DevStateT<R> &s, c
 // the flux uses them (storage type R for the geometry, FP64 node depths).
 template <typename R>
 struct EdgeRec {
-    int cc, e1, e2;      // signed edge id (kEmpty: none), its two elements
+    int cc, oK, pad;     // signed edge id (kEmpty: none; < 0: the element is edge_tri(1)), the other element's column offset
     R acc, cx, cy, cz, cw, dx, dy, cos1, cos2;
     double an1, an2;     // a at the edge's nodes
 };
@@ -685,8 +717,7 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
                 const int2 et = m.edtri[ed];
                 const double4 cd = m.ecross[ed];
                 const double2 dd = m.edxdy[ed];
-                g.e1 = et.x;
-                g.e2 = et.y;
+                g.oK = (cc >= 0 ? et.x : et.y) * K;
                 g.acc = (R)(0.5 * (m.ac[nd.x] + m.ac[nd.y]));
                 g.cx = (R)cd.x;
                 g.cy = (R)cd.y;
@@ -719,14 +750,23 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
     R abv = (R)(-u * r.cor * area * je);
     if (hadv) {
         const double ds = m.dsig[k];
+        const int ek = e * KC + k;
+        // the element's own velocity and gradients at this layer (one end of every edge)
+        const R uo = s.U_n[ek], vo = s.V_n[ek], guxo = s.gux[ek], guyo = s.guy[ek], gvxo = s.gvx[ek], gvyo = s.gvy[ek];
         for (int j = 0; j < 4; ++j) {
             const EdgeRec<R> &g = edr[c * 4 + j];
             if (g.cc == kEmpty) break;
+            const bool own1 = g.cc < 0;  // the element is edge_tri(1)
+            const int io = g.oK + k;
+            const R ux = s.U_n[io], vx = s.V_n[io], guxx = s.gux[io], guyx = s.guy[io], gvxx = s.gvx[io], gvyx = s.gvy[io];
+            const R U1 = own1 ? uo : ux, V1 = own1 ? vo : vx, U2 = own1 ? ux : uo, V2 = own1 ? vx : vo;
+            const R gux1 = own1 ? guxo : guxx, guy1 = own1 ? guyo : guyx, gvx1 = own1 ? gvxo : gvxx, gvy1 = own1 ? gvyo : gvyx;
+            const R gux2 = own1 ? guxx : guxo, guy2 = own1 ? guyx : guyo, gvx2 = own1 ? gvxx : gvxo, gvy2 = own1 ? gvyx : gvyo;
             // in the storage type: FP32 advective fluxes in the mixed-precision path
-            const R u1 = PL(s.U_n, k, g.e1, KC) - PL(s.gux, k, g.e1, KC) * g.cx - PL(s.guy, k, g.e1, KC) * g.cy;
-            const R v1 = PL(s.V_n, k, g.e1, KC) - PL(s.gvx, k, g.e1, KC) * g.cx - PL(s.gvy, k, g.e1, KC) * g.cy;
-            const R u2 = PL(s.U_n, k, g.e2, KC) - PL(s.gux, k, g.e2, KC) * g.cz - PL(s.guy, k, g.e2, KC) * g.cw;
-            const R v2 = PL(s.V_n, k, g.e2, KC) - PL(s.gvx, k, g.e2, KC) * g.cz - PL(s.gvy, k, g.e2, KC) * g.cw;
+            const R u1 = U1 - gux1 * g.cx - guy1 * g.cy;
+            const R v1 = V1 - gvx1 * g.cx - gvy1 * g.cy;
+            const R u2 = U2 - gux2 * g.cz - guy2 * g.cw;
+            const R v2 = V2 - gvx2 * g.cz - gvy2 * g.cw;
             const R jd = (R)(0.5 * (g.an1 * ds + g.an2 * ds));
             const R un = R(0.5) * R(kREarthD) * jd * ((u1 + u2) * g.dy - (v1 * g.cos1 + v2 * g.cos2) * g.dx);
             R uu, vv;
@@ -1011,21 +1051,31 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
 // which update_2D_vel reads in every substep, as a 2-D field.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevStateT<R> s, double Dmin) {
-    int e, k;
-    if (!ent_layer(m.E, m.L, e, k)) return;
+    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    const int4 q = m.en[e];
-    const double4 w = m.wcv[e];
-    const double dm = dmaxr(Dmin, w.x * (s.eta_n_1[q.x] + m.depth[q.x]) + w.y * (s.eta_n_1[q.y] + m.depth[q.y])
-                                  + w.z * (s.eta_n_1[q.z] + m.depth[q.z]) + w.w * (s.eta_n_1[q.w] + m.depth[q.w]));
-    const double snew = sum4(w, s.eta_n[q.x] + m.depth[q.x], s.eta_n[q.y] + m.depth[q.y], s.eta_n[q.z] + m.depth[q.z],
-                             s.eta_n[q.w] + m.depth[q.w]);
-    const double ap = dm / dmaxr(Dmin, snew);
-    const double u2 = s.U2[2 * e], v2 = s.U2[2 * e + 1];
+    const ColFlat f = col_flat(m.E, m.L);
+    double *cs = (double *)lds_;  // per column: ap, U2, V2
+    if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {
+        const int e = f.c0 + threadIdx.x;
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        const double dm = dmaxr(Dmin, w.x * (s.eta_n_1[q.x] + m.depth[q.x]) + w.y * (s.eta_n_1[q.y] + m.depth[q.y])
+                                      + w.z * (s.eta_n_1[q.z] + m.depth[q.z]) + w.w * (s.eta_n_1[q.w] + m.depth[q.w]));
+        s.dmean_n[e] = dm;
+        const double snew = sum4(w, s.eta_n[q.x] + m.depth[q.x], s.eta_n[q.y] + m.depth[q.y],
+                                 s.eta_n[q.z] + m.depth[q.z], s.eta_n[q.w] + m.depth[q.w]);
+        cs[3 * threadIdx.x] = dm / dmaxr(Dmin, snew);
+        cs[3 * threadIdx.x + 1] = s.U2[2 * e];
+        cs[3 * threadIdx.x + 2] = s.U2[2 * e + 1];
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int e = f.c0 + f.c, k = f.k;
+    const double ap = cs[3 * f.c], u2 = cs[3 * f.c + 1], v2 = cs[3 * f.c + 2];
     const double un = PL(s.U_n, k, e, KC), vn = PL(s.V_n, k, e, KC);
     PL(s.U_puls, k, e, KC) = un - ap * u2;
     PL(s.V_puls, k, e, KC) = vn - ap * v2;
-    if (k == 0) s.dmean_n[e] = dm;
     if (k == m.L - 1) {
         s.ubot[2 * e] = un;
         s.ubot[2 * e + 1] = vn;
@@ -1081,26 +1131,39 @@ __global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
     double *tu = (double *)lds_, *tv = tu + 4 * T;       // [edge slot][column*L + layer]
     double *su_ = tv + 4 * T, *sv_ = su_ + 4 * f.cpb;    // [column*4 + edge slot]
     if (f.nc <= 0) return;
-    if (f.on()) {
-        const int c = f.c, e = f.c0 + c, k = f.k;
-        const int4 ied = m.el_ied[e];
-        const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
-        const double je = PL(s.Je, k, e, KC);
-        const double ds = m.dsig[k];
-        for (int j = 0; j < 4; ++j) {
-            const int cc = slots[j];
-            if (cc == kEmpty) break;
+    P1Rec *rec = (P1Rec *)(sv_ + 4 * f.cpb);             // [column*4 + edge slot]
+    for (int t = threadIdx.x; t < 4 * f.nc; t += blockDim.x) {
+        const int c = t >> 2, j = t & 3;
+        const int4 ied = m.el_ied[f.c0 + c];
+        const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
+        P1Rec r;
+        r.n1K = -1;
+        if (cc != kEmpty) {
             const int ed = cc >= 0 ? cc : ~cc;
             const int2 nd = m.ednodes[ed];
             const int2 et = m.edtri[ed];
             const double2 dd = m.edxdy[ed];
-            const double xe = dd.x * kREarthD * 0.5 * (m.elem_cos[et.x] + m.elem_cos[et.y]);
-            const double ye = dd.y * kREarthD;
-            const double an1 = s.a[nd.x], an2 = s.a[nd.y];
-            double uu = 0.5 * (PL(s.Unode_p, k, nd.x, KC) + PL(s.Unode_p, k, nd.y, KC));
-            double vv = 0.5 * (PL(s.Vnode_p, k, nd.x, KC) + PL(s.Vnode_p, k, nd.y, KC));
-            double jd = 0.5 * (an1 * ds + an2 * ds);
-            double un = (uu * ye - vv * xe) * jd;
+            r.n1K = nd.x * KC;
+            r.n2K = nd.y * KC;
+            r.xe = dd.x * kREarthD * 0.5 * (m.elem_cos[et.x] + m.elem_cos[et.y]);
+            r.ye = dd.y * kREarthD;
+            r.an1 = s.a[nd.x];
+            r.an2 = s.a[nd.y];
+        }
+        rec[t] = r;
+    }
+    __syncthreads();
+    if (f.on()) {
+        const int c = f.c, e = f.c0 + c, k = f.k;
+        const double je = PL(s.Je, k, e, KC);
+        const double ds = m.dsig[k];
+        for (int j = 0; j < 4; ++j) {
+            const P1Rec &r = rec[c * 4 + j];
+            if (r.n1K < 0) break;
+            double uu = 0.5 * (s.Unode_p[r.n1K + k] + s.Unode_p[r.n2K + k]);
+            double vv = 0.5 * (s.Vnode_p[r.n1K + k] + s.Vnode_p[r.n2K + k]);
+            double jd = 0.5 * (r.an1 * ds + r.an2 * ds);
+            double un = (uu * r.ye - vv * r.xe) * jd;
             tu[j * T + c * L + k] = uu * un * je;
             tv[j * T + c * L + k] = vv * un * je;
         }
@@ -1261,14 +1324,31 @@ __global__ __launch_bounds__(256) void k_jac_node(DevMesh m, DevStateT<R> s, dou
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_je(DevMesh m, DevStateT<R> s) {
-    int e, k;
-    if (!ent_layer(m.E, m.L, e, k)) return;
+    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    const int4 q = m.en[e];
-    const double4 w = m.wcv[e];
-    const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
+    const ColFlat f = col_flat(m.E, m.L);
+    double *cs = (double *)lds_;  // per column: w_cv(4), a at the nodes(4)
+    if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {
+        const int e = f.c0 + threadIdx.x;
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        double *c = cs + 8 * threadIdx.x;
+        c[0] = w.x;
+        c[1] = w.y;
+        c[2] = w.z;
+        c[3] = w.w;
+        c[4] = s.a[q.x];
+        c[5] = s.a[q.y];
+        c[6] = s.a[q.z];
+        c[7] = s.a[q.w];
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int e = f.c0 + f.c, k = f.k;
+    const double *c = cs + 8 * f.c;
     const double ds = m.dsig[k];
-    PL(s.Je, k, e, KC) = sum4(w, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
+    PL(s.Je, k, e, KC) = sum4a(c, c[4] * ds, c[5] * ds, c[6] * ds, c[7] * ds);
 }
 
 // update_3D_vel: src/fv_3D_vel.F90:201-255, with U_filt_2D *= 1/Mt from
@@ -1280,15 +1360,32 @@ __global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, dou
     extern __shared__ __align__(16) unsigned char lds_[];
     const int L = m.L, KC = m.K;
     const ColFlat f = col_flat(m.E, L);
-    double *pu = (double *)lds_, *pv = pu + f.cpb * L, *sc = pv + f.cpb * L;  // sc per column: U2p, V2p, uf, vf
+    double *pu = (double *)lds_, *pv = pu + f.cpb * L;
+    double *sc = pv + f.cpb * L;  // per column: U2p, V2p, uf, vf, U2, V2, w_cv(4), a_old at the nodes(4)
     if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {
+        const int e = f.c0 + threadIdx.x;
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        double *c = sc + 14 * threadIdx.x;
+        c[4] = s.U2[2 * e];
+        c[5] = s.U2[2 * e + 1];
+        c[6] = w.x;
+        c[7] = w.y;
+        c[8] = w.z;
+        c[9] = w.w;
+        c[10] = s.a_old[q.x];
+        c[11] = s.a_old[q.y];
+        c[12] = s.a_old[q.z];
+        c[13] = s.a_old[q.w];
+    }
+    __syncthreads();
     const int e = f.c0 + f.c, k = f.k;
     double ak = 0.0, ur = 0.0, vr = 0.0, je = 1.0;
     if (f.on()) {
-        const int4 q = m.en[e];
-        const double4 w = m.wcv[e];
+        const double *c = sc + 14 * f.c;
         const double ds = m.dsig[k];
-        ak = sum4(w, s.a_old[q.x] * ds, s.a_old[q.y] * ds, s.a_old[q.z] * ds, s.a_old[q.w] * ds);
+        ak = sum4a(c + 6, c[10] * ds, c[11] * ds, c[12] * ds, c[13] * ds);
         ur = PL(s.U_rhs, k, e, KC);
         vr = PL(s.V_rhs, k, e, KC);
         je = PL(s.Je, k, e, KC);
@@ -1297,33 +1394,35 @@ __global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, dou
     }
     __syncthreads();
     if ((int)threadIdx.x < f.nc) {
-        const int c = threadIdx.x, ec = f.c0 + c;
+        const int ci = threadIdx.x, ec = f.c0 + ci;
         const int4 q = m.en[ec];
         const double4 w = m.wcv[ec];
         const double dmi = 1.0 / dmaxr(Dmin, sum4(w, m.depth[q.x] + s.eta_n[q.x], m.depth[q.y] + s.eta_n[q.y],
                                                   m.depth[q.z] + s.eta_n[q.z], m.depth[q.w] + s.eta_n[q.w]));
         double su = 0.0, sv = 0.0;
         for (int j = 0; j < L; ++j) {
-            su = su + pu[c * L + j];
-            sv = sv + pv[c * L + j];
+            su = su + pu[ci * L + j];
+            sv = sv + pv[ci * L + j];
         }
         const double uf = (s.U_filt[ec] * Mt_inv) * dmi;
         const double vf = (s.V_filt[ec] * Mt_inv) * dmi;
         s.U_filt[ec] = uf;
         s.V_filt[ec] = vf;
-        sc[4 * c] = su * dmi;
-        sc[4 * c + 1] = sv * dmi;
-        sc[4 * c + 2] = uf;
-        sc[4 * c + 3] = vf;
+        double *c = sc + 14 * ci;
+        c[0] = su * dmi;
+        c[1] = sv * dmi;
+        c[2] = uf;
+        c[3] = vf;
     }
     __syncthreads();
     if (!f.on()) return;
-    const double U2p = sc[4 * f.c], V2p = sc[4 * f.c + 1], uf = sc[4 * f.c + 2], vf = sc[4 * f.c + 3];
+    const double *c = sc + 14 * f.c;
+    const double U2p = c[0], V2p = c[1], uf = c[2], vf = c[3];
     ak = ak / je;
     PL(s.U_nf, k, e, KC) = ak * ur + uf - U2p;
     PL(s.V_nf, k, e, KC) = ak * vr + vf - V2p;
-    PL(s.U_n, k, e, KC) = ak * ur + s.U2[2 * e] - U2p;
-    PL(s.V_n, k, e, KC) = ak * vr + s.U2[2 * e + 1] - V2p;
+    PL(s.U_n, k, e, KC) = ak * ur + c[4] - U2p;
+    PL(s.V_n, k, e, KC) = ak * vr + c[5] - V2p;
 }
 
 // vert_vel_sigma (src/fv_3D_vel.F90:309-456), first part: per node and layer
@@ -1831,14 +1930,6 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
 // reconstruction (fill_up_dn_grad node-average branch,
 // fv_gradients.F90:263-323): area-weighted mean over the node's elements in
 // ascending element id, divided by the summed area.
-__device__ __forceinline__ double sum4a(const double *w, double x0, double x1, double x2, double x3) {
-    double r = w[0] * x0;
-    r = r + w[1] * x1;
-    r = r + w[2] * x2;
-    r = r + w[3] * x3;
-    return r;
-}
-
 template <typename R>
 __global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> s) {
     extern __shared__ __align__(16) unsigned char lds_[];

@@ -52,7 +52,7 @@ def test_tuner_picks_a_candidate_for_each_kernel():
         m.set_launch("k_velrhs", 96)
     with pytest.raises(RuntimeError):
         m.set_launch("k_bt_elem2", 64)
-    kern = ["k_velgrad3d", "k_je", "k_d3end"]
+    kern = ["k_tr_ab", "k_jac_node", "k_d3end"]
     table = autotune.tune(m, kernels=kern, steps=1)
     assert set(table) == set(kern)
     for k, (bs, ms) in table.items():
