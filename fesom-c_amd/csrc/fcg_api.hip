@@ -22,6 +22,8 @@ template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, do
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *);
+template <typename R> __global__ void k_bt_edge_ssh(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_bt_edge_mom(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_velgrad(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_puls(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_el2nodes(DevMesh, const R *, const R *, R *, R *);
@@ -261,8 +263,11 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
     for (int sub = 1; sub <= p.Mt; ++sub) {
         const double *tide = (p.tf_presence && m.nobn) ? s.tide + (size_t)(sub - 1) * m.nobn : nullptr;
         hipLaunchKernelGGL(fcgk::k_bt_elem1<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
+        hipLaunchKernelGGL(fcgk::k_bt_edge_ssh<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s);
         hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide);
         hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+        if (p.mom_adv_2D == 2)
+            hipLaunchKernelGGL(fcgk::k_bt_edge_mom<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, p.Dmin);
         if (sub == 1) {
             hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nflatblk(m.E, m.L)), dim3(kFlatT), 3 * cpb_of(m.L) * sizeof(double),
                                st, m, s, p.Dmin);
@@ -623,7 +628,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
     DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
-    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
     DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
     DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
     DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA3(s.U_puls, EL) DA3(s.V_puls, EL)

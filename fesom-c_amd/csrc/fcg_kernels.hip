@@ -972,6 +972,20 @@ __global__ __launch_bounds__(256) void k_bt_elem1(DevMesh m, DevStateT<R> s, dou
     s.aux_el[e] = sum4(w, ad[0], ad[1], ad[2], ad[3]);
 }
 
+// Per edge, the transport through it of compute_ssh_rhs_elem
+// (fv_average_dynamic.F90:106-169): both elements' AB3 velocity times their
+// depth, computed once per edge for the gather of its two nodes.
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_edge_ssh(DevMesh m, DevStateT<R> s) {
+    const int ed = TID();
+    if (ed >= m.D) return;
+    const int2 et = m.edtri[ed];
+    const double4 cd = m.ecross[ed];
+    double c1 = (s.UAB[2 * et.x + 1] * cd.x - s.UAB[2 * et.x] * cd.y) * s.aux_el[et.x];
+    if (et.y >= 0) c1 = c1 + (-s.UAB[2 * et.y + 1] * cd.z + s.UAB[2 * et.y] * cd.w) * s.aux_el[et.y];
+    s.fl_ssh[ed] = c1;
+}
+
 // ssh_rhs gather (fv_average_dynamic.F90:106-169), eta update (fv_main.F90:1432-1435),
 // tidal OB value (fv_average_dynamic.F90:190-216, host-evaluated), the AM4
 // elevation of update_2D_vel (fv_average_dynamic.F90:288-296) and the time-level
@@ -983,12 +997,8 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, doub
     double ssh = 0.0;
     #pragma unroll 4
     for (int j = m.n2e_ptr[n]; j < m.n2e_ptr[n + 1]; ++j) {
-        int c = m.n2e[j];
-        int ed = c >= 0 ? c : ~c;
-        const int2 et = m.edtri[ed];
-        const double4 cd = m.ecross[ed];
-        double c1 = (s.UAB[2 * et.x + 1] * cd.x - s.UAB[2 * et.x] * cd.y) * s.aux_el[et.x];
-        if (et.y >= 0) c1 = c1 + (-s.UAB[2 * et.y + 1] * cd.z + s.UAB[2 * et.y] * cd.w) * s.aux_el[et.y];
+        const int c = m.n2e[j];
+        const double c1 = s.fl_ssh[c >= 0 ? c : ~c];
         ssh = (c >= 0) ? ssh + c1 : ssh - c1;
     }
     s.ssh_rhs[n] = ssh;
@@ -1235,6 +1245,17 @@ __device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevStateT<R> &
     }
 }
 
+// momentum_adv_upwind_2D flux of every internal edge, once per edge (each is
+// gathered by its two elements in k_bt_elem2)
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_edge_mom(DevMesh m, DevStateT<R> s, double Dmin) {
+    const int ed = TID();
+    if (ed >= m.D || m.edtri[ed].y < 0) return;
+    double uu, vv;
+    edge_uu2d(m, s, ed, Dmin, uu, vv);
+    ((double2 *)s.fl_uv)[ed] = make_double2(uu, vv);
+}
+
 // update_2D_vel (fv_average_dynamic.F90:228-443, 3-D branch) for one element,
 // followed by the velocity shift and U_filt accumulation (fv_main.F90:1569-1606).
 template <typename R>
@@ -1263,21 +1284,19 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
     const double uab = s.UAB[2 * e], vab = s.UAB[2 * e + 1];
     double ru = area * (-dm * (gx.x * eg0 + gx.y * eg1 + gx.z * eg2 + gx.w * eg3) + vab * dm * cor + s.taux[e] * rho_inv);
     double rv = area * (-dm * (gy.x * eg0 + gy.y * eg1 + gy.z * eg2 + gy.w * eg3) - uab * dm * cor + s.tauy[e] * rho_inv);
-    if (adv2d) {
+    if (adv2d) {  // edge fluxes of k_bt_edge_mom, in ascending edge id
         const int4 ied = m.el_ied[e];
         const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
         for (int j = 0; j < 4; ++j) {
             int c = slots[j];
             if (c == kEmpty) break;
-            double uu, vv;
+            const double2 f = ((const double2 *)s.fl_uv)[c >= 0 ? c : ~c];
             if (c >= 0) {
-                edge_uu2d(m, s, c, Dmin, uu, vv);
-                ru = ru + uu;
-                rv = rv + vv;
+                ru = ru + f.x;
+                rv = rv + f.y;
             } else {
-                edge_uu2d(m, s, ~c, Dmin, uu, vv);
-                ru = ru - uu;
-                rv = rv - vv;
+                ru = ru - f.x;
+                rv = rv - f.y;
             }
         }
     }
@@ -2527,6 +2546,8 @@ void instantiate_kernels() {
     (void)&k_velvisc<R>;
     (void)&k_bt_elem1<R>;
     (void)&k_bt_node<R>;
+    (void)&k_bt_edge_ssh<R>;
+    (void)&k_bt_edge_mom<R>;
     (void)&k_bt_velgrad<R>;
     (void)&k_bt_puls<R>;
     (void)&k_el2nodes<R>;
