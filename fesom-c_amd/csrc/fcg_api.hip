@@ -88,7 +88,8 @@ inline unsigned nflatblk(int ents, int S) { return (unsigned)((ents + cpb_of(S) 
 inline size_t flat_bytes(int S, size_t esz) { return (size_t)cpb_of(S) * S * esz; }
 // k_velrhs: per column one element record and four edge records
 inline size_t velrhs_bytes(const DevMesh &m, size_t esz) {
-    return (size_t)cpb_of(m.L) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz));
+    return (size_t)cpb_of(m.L) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) +
+           (size_t)cpb_of(m.L) * m.L * (3 * sizeof(double) + 2 * esz);
 }
 inline size_t lds_p1adv(const DevMesh &m) {
     return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8 + 4 * (size_t)cpb_of(m.L) * sizeof(P1Rec);

@@ -653,6 +653,51 @@ __device__ __forceinline__ void uvert(const DevMesh &m, const DevStateT<R> &s, c
     fv = -vm * wv;
 }
 
+// uvert on the column arrays a block of compute_vel_rhs shares in LDS: ws =
+// sum w_cv*W at each level, u/v the element's velocity, Zw/zbw the
+// element-averaged layer and level depths (the same operands uvert forms).
+template <typename R>
+__device__ __forceinline__ void uvert_lds(const double *ws, const R *u, const R *v, const double *Zw, const double *zbw,
+                                          int K, double area, int nz, double &fu, double &fv) {
+    const double wv = ws[nz - 1] * area;
+    double um = 0.0, vm = 0.0;
+    if (wv >= 0.0) {
+        if (nz == K - 1) {
+            um = 0.5 * (u[nz - 2] + u[nz - 1]);
+            vm = 0.5 * (v[nz - 2] + v[nz - 1]);
+        } else {
+            double Z1 = zbw[nz - 1];
+            double aa = Z1 - Zw[nz - 2];
+            double bb = Zw[nz - 1] - Z1;
+            double cc = Zw[nz] - Z1;
+            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+            double dg = aa * bb * (aa + bb) / d;
+            double db = -aa * cc * (aa + cc) / d;
+            double da = 1.0 - dg - db;
+            um = u[nz - 2] * da + u[nz - 1] * db + u[nz] * dg;
+            vm = v[nz - 2] * da + v[nz - 1] * db + v[nz] * dg;
+        }
+    } else {
+        if (nz == 2) {
+            um = 0.5 * (u[nz - 2] + u[nz - 1]);
+            vm = 0.5 * (v[nz - 2] + v[nz - 1]);
+        } else {
+            double Z1 = zbw[nz - 1];
+            double aa = Zw[nz - 1] - Z1;
+            double bb = Z1 - Zw[nz - 2];
+            double cc = Z1 - Zw[nz - 3];
+            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+            double dg = aa * bb * (aa + bb) / d;
+            double db = -aa * cc * (aa + cc) / d;
+            double da = 1.0 - dg - db;
+            um = u[nz - 1] * da + u[nz - 2] * db + u[nz - 3] * dg;
+            vm = v[nz - 1] * da + v[nz - 2] * db + v[nz - 3] * dg;
+        }
+    }
+    fu = -um * wv;
+    fv = -vm * wv;
+}
+
 // Per-edge constants of momentum_adv_upwind (fv_advection.F90:120-165), as
 // the flux uses them (storage type R for the geometry, FP64 node depths).
 template <typename R>
@@ -688,6 +733,10 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
     const ColFlat f = col_flat(m.E, L);
     ElemRec *er = (ElemRec *)lds_;                                        // [cpb]
     EdgeRec<R> *edr = (EdgeRec<R> *)(lds_ + f.cpb * kVelrhsElemBytes);   // [cpb][4]
+    // per column and layer: sum w_cv*W, Zw, zbw, U_n, V_n
+    const int T = f.cpb * L;
+    double *cws = (double *)(edr + 4 * f.cpb), *cZw = cws + T, *czb = cZw + T;
+    R *cu = (R *)(czb + T), *cv = cu + T;
     if (f.nc <= 0) return;
     if ((int)threadIdx.x < f.nc) {
         const int c = threadIdx.x, e = f.c0 + c;
@@ -734,16 +783,24 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
         }
     }
     __syncthreads();
-    if (!f.on()) return;
     const int c = f.c, e = f.c0 + c, k = f.k;
+    if (f.on()) {  // the column arrays of the vertical advection
+        const ElemCol &col = er[c].col;
+        const int4 q = col.q;
+        cws[c * L + k] = sum4(col.w, PL(s.W, k, q.x, KC), PL(s.W, k, q.y, KC), PL(s.W, k, q.z, KC), PL(s.W, k, q.w, KC));
+        cZw[c * L + k] = col.Zw(m, k);
+        czb[c * L + k] = col.zbw(m, k);
+        cu[c * L + k] = PL(s.U_n, k, e, KC);
+        cv[c * L + k] = PL(s.V_n, k, e, KC);
+    }
+    __syncthreads();
+    if (!f.on()) return;
     const ElemRec &r = er[c];
     const ElemCol &col = r.col;
-    const int4 q = col.q;
-    const double4 w = col.w;
     const double area = r.area;
     const double area_inv = 1.0 / area;
     const double je = col.Je(m, k);
-    const double u = PL(s.U_n, k, e, KC), v = PL(s.V_n, k, e, KC);
+    const double u = cu[c * L + k], v = cv[c * L + k];
     double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, KC) + r.Fx * je;
     double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, KC) + r.Fy * je;
     R abu = (R)(v * r.cor * area * je);
@@ -795,12 +852,11 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
                 const int nz = k + 1 + t;
                 double fu = 0.0, fv = 0.0;
                 if (nz == 1) {
-                    const double w1 = sum4(w, PL(s.W, 0, q.x, KC), PL(s.W, 0, q.y, KC), PL(s.W, 0, q.z, KC),
-                                           PL(s.W, 0, q.w, KC)) * area;
-                    fu = -w1 * PL(s.U_n, 0, e, KC);
-                    fv = -w1 * PL(s.V_n, 0, e, KC);
+                    const double w1 = cws[c * L] * area;
+                    fu = -w1 * cu[c * L];
+                    fv = -w1 * cv[c * L];
                 } else if (nz <= K - 1) {
-                    uvert(m, s, col, e, area, nz, fu, fv);
+                    uvert_lds(cws + c * L, cu + c * L, cv + c * L, cZw + c * L, czb + c * L, K, area, nz, fu, fv);
                 }
                 if (t == 0) {
                     ftu = fu;
@@ -2329,10 +2385,11 @@ __device__ void column_features(const DevMesh &m, const DevStateT<R> &s, int n, 
     const double delta_b = a * m.dsig[L - 1] + z0b_min;
     const double lg = log(delta_b / z0b_min);
     const double bott = 16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb);
-    f[0] = (float)log10(sh2 + 1e-12);
-    f[1] = (float)asinh(n2 * 1e5);
-    f[2] = (float)log10(dmean);
-    f[3] = (float)log10(bott + 1e-12);
+    // the transcendental feature maps in FP32: the network sees BF16 inputs
+    f[0] = log10f((float)(sh2 + 1e-12));
+    f[1] = asinhf((float)(n2 * 1e5));
+    f[2] = log10f((float)dmean);
+    f[3] = log10f((float)(bott + 1e-12));
     f[4] = (float)m.sigma[nz - 1];
 }
 
@@ -2385,7 +2442,7 @@ __global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, 
         const double lg = log(delta_b / z0b_min);
         col_a[threadIdx.x] = a;
         col_dm[threadIdx.x] = dmean;
-        col_f3[threadIdx.x] = log10(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12);
+        col_f3[threadIdx.x] = log10f((float)(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12));
         col_ood[threadIdx.x] = 0;
     }
     __syncthreads();
@@ -2403,9 +2460,9 @@ __global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, 
         const double rho_up = densityJM(PL(s.TF, iz - 2, n, KC), PL(s.SF, iz - 2, n, KC), -zb, rho0);
         const double rho_dn = densityJM(PL(s.TF, iz - 1, n, KC), PL(s.SF, iz - 1, n, KC), -zb, rho0);
         const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
-        fx[0] = (float)log10(sh2 + 1e-12);
-        fx[1] = (float)asinh(n2 * 1e5);
-        fx[2] = (float)log10(dmean);
+        fx[0] = log10f((float)(sh2 + 1e-12));
+        fx[1] = asinhf((float)(n2 * 1e5));
+        fx[2] = log10f((float)dmean);
         fx[3] = (float)col_f3[f.c];
         fx[4] = (float)m.sigma[nz - 1];
         bool ood = false;
