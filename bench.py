@@ -126,7 +126,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--probe", default="k_tr_muscl")
+    ap.add_argument("--probe", default="k_velrhs", help="kernel timed live for the roofline (the dominant one)")
+    ap.add_argument("--error-steps", default="0", metavar="N|m2",
+                    help="after the timed run, advance both the headline and the FP64 model (untimed) to N internal "
+                         "steps in total ('m2': one M2 tidal period, 44712 s) and report their differences then")
     ap.add_argument("--path", choices=["headline", "mixed-mlp", "fp64", "mixed"], default="headline",
                     help="headline: mixed precision + learned closure, timed beside the plain FP64 drop-in")
     ap.add_argument("--closure-weights", default=os.path.join(ROOT, "fesom-c_amd", "closure_c2.npz"))
@@ -237,6 +240,29 @@ def main():
             errs[key + "_rms"] = float(np.sqrt(np.mean((a - b) ** 2)))
             errs[key + "_max"] = float(np.max(np.abs(a - b)))
         out["error_vs_fp64_after_steps"] = errs
+        total = args.warmup + args.steps
+        target = (int(math.ceil(44712.0 / model.par.dt)) if args.error_steps == "m2" else int(args.error_steps))
+        if target > total:
+            # both runs advance in lock-step chunks, each chunk reported on stderr
+            t_err = time.perf_counter()
+            while total < target:
+                n = min(500, target - total)
+                model.step(n)
+                ref.step(n)
+                model.sync()
+                ref.sync()
+                total += n
+                print(f"error leg: {total}/{target} steps, {time.perf_counter() - t_err:.0f} s", file=sys.stderr,
+                      flush=True)
+            errs = {}
+            for f, key in (("eta_n", "ssh_m"), ("U_n", "u_m_s"), ("V_n", "v_m_s"), ("TF", "t_degC"), ("SF", "s_psu")):
+                a, b = model.get(f), ref.get(f)
+                errs[key + "_rms"] = float(np.sqrt(np.mean((a - b) ** 2)))
+                errs[key + "_max"] = float(np.max(np.abs(a - b)))
+            d_m, d_r = model.diagnostics(), ref.diagnostics()
+            out["error_vs_fp64_long"] = {"steps": total, "model_time_s": total * model.par.dt, **errs,
+                                         "status": [d_m["status"], d_r["status"]],
+                                         "mlp_fallback_columns": d_m["mlp_fallback"]}
         ref.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

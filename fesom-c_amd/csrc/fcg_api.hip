@@ -1084,7 +1084,7 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const int nb = kMinmaxChunks;
     hipLaunchKernelGGL(fcgk::k_minmax_partial<R>, dim3(nb, m.nmemb), dim3(kBlk), 0, ctx->stream, m, (const R *)s.TF,
                        (const R *)s.SF, s.red);
-    hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(nblk(m.nmemb)), dim3(kBlk), 0, ctx->stream, (const double *)s.red,
+    hipLaunchKernelGGL(fcgk::k_minmax_final, dim3(m.nmemb), dim3(kBlk), 0, ctx->stream, (const double *)s.red,
                        nb, m.nmemb, s.tsa);
     const int relax = p.cl_relax ? 1 : 0;
     // scheme dispatch of src/fv_main.F90:1788-1800; node x layer kernels, then

@@ -294,9 +294,11 @@ __global__ __launch_bounds__(kColT) void k_bpg(DevMesh m, DevStateT<R> s) {
 // level-plane rows of the block's nc columns: x[k*N + c0 + j] = t[j*S + k]
 // for k in [k0, k1), each row written by consecutive threads
 template <typename T, typename G>
-__device__ __forceinline__ void rows_store(G *__restrict__ x, const T *t, int c0, int nc, int S, int N, int k0, int k1) {
+__device__ __forceinline__ void rows_store(G *__restrict__ x, const T *t, int c0, int nc, int S, int N, int k0, int k1,
+                                           const int *__restrict__ only) {
     for (int r = (int)threadIdx.x; r < (k1 - k0) * nc; r += (int)blockDim.x) {
         const int k = k0 + r / nc, j = r % nc;
+        if (only && only[c0 + j] == 0) continue;
         x[(size_t)k * N + c0 + j] = (G)t[j * S + k];
     }
 }
@@ -350,11 +352,11 @@ __global__ __launch_bounds__(256) void k_d3end_prep(DevMesh m, DevStateT<R> s, d
 #undef BT
     }
     __syncthreads();
-    rows_store(s.mx_s, ts, f.c0, f.nc, K, N, 0, K);
-    rows_store(s.mx_bA, tb, f.c0, f.nc, K, N, 0, K);
-    rows_store(s.mx_p, tp, f.c0, f.nc, K, N, 1, K - 1);
-    rows_store(s.mx_q, tq, f.c0, f.nc, K, N, 1, K - 1);
-    rows_store(s.mx_c2, tc, f.c0, f.nc, K, N, 1, K - 1);
+    rows_store(s.mx_s, ts, f.c0, f.nc, K, N, 0, K, only);
+    rows_store(s.mx_bA, tb, f.c0, f.nc, K, N, 0, K, only);
+    rows_store(s.mx_p, tp, f.c0, f.nc, K, N, 1, K - 1, only);
+    rows_store(s.mx_q, tq, f.c0, f.nc, K, N, 1, K - 1, only);
+    rows_store(s.mx_c2, tc, f.c0, f.nc, K, N, 1, K - 1, only);
 }
 
 template <typename R>
@@ -445,6 +447,7 @@ __global__ __launch_bounds__(256) void k_d3end_fin(DevMesh m, DevStateT<R> s, do
     if (f.nc <= 0) return;
     for (int r = (int)threadIdx.x; r < K * f.nc; r += (int)blockDim.x) {
         const int k = r / f.nc, j = r % f.nc;
+        if (only && only[f.c0 + j] == 0) continue;
         tx[j * K + k] = PP(s.mx_xx, k, f.c0 + j, N);
     }
     __syncthreads();
@@ -1614,20 +1617,37 @@ __global__ __launch_bounds__(256) void k_minmax_partial(DevMesh m, const R *__re
         for (int c = 0; c < 4; ++c) part[((size_t)mb * gridDim.x + blockIdx.x) * 4 + c] = sm[c][0];
 }
 
-// T_aver, S_aver of each member: midpoints of its extrema
+// T_aver, S_aver of each member: midpoints of its extrema (one block per
+// member, order-free min/max over its chunks)
 __global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb, int nmemb, double *tsa) {
-    const int mb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (mb >= nmemb) return;
+    __shared__ double sm[4][256];
+    const int mb = blockIdx.x;
     const double *p = part + (size_t)mb * nb * 4;
-    double tmn = p[0], tmx = p[1], smn = p[2], smx = p[3];
-    for (int b = 1; b < nb; ++b) {
+    double tmn = INFINITY, tmx = -INFINITY, smn = INFINITY, smx = -INFINITY;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         tmn = fmin(tmn, p[4 * b]);
         tmx = fmax(tmx, p[4 * b + 1]);
         smn = fmin(smn, p[4 * b + 2]);
         smx = fmax(smx, p[4 * b + 3]);
     }
-    tsa[2 * mb] = 0.5 * (tmn + tmx);
-    tsa[2 * mb + 1] = 0.5 * (smn + smx);
+    sm[0][threadIdx.x] = tmn;
+    sm[1][threadIdx.x] = tmx;
+    sm[2][threadIdx.x] = smn;
+    sm[3][threadIdx.x] = smx;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sm[0][threadIdx.x] = fmin(sm[0][threadIdx.x], sm[0][threadIdx.x + st]);
+            sm[1][threadIdx.x] = fmax(sm[1][threadIdx.x], sm[1][threadIdx.x + st]);
+            sm[2][threadIdx.x] = fmin(sm[2][threadIdx.x], sm[2][threadIdx.x + st]);
+            sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tsa[2 * mb] = 0.5 * (sm[0][0] + sm[1][0]);
+        tsa[2 * mb + 1] = 0.5 * (sm[2][0] + sm[3][0]);
+    }
 }
 
 // AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444), over every

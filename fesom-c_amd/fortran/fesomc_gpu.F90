@@ -175,6 +175,52 @@ module fesomc_gpu
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
      end function c_strlen
+
+     ! ---- ensembles, learned closure, launch shapes (no reference counterpart)
+     integer(c_int) function fcg_create_ensemble(members, nmemb, par, ctx) bind(C, name="fcg_create_ensemble")
+       import :: c_int, c_ptr, fcg_mesh, fcg_params
+       type(fcg_mesh), intent(in) :: members(*)
+       integer(c_int), value :: nmemb
+       type(fcg_params), intent(in) :: par
+       type(c_ptr), intent(out) :: ctx
+     end function fcg_create_ensemble
+     integer(c_int) function fcg_diagnostics_member(ctx, member, d) bind(C, name="fcg_diagnostics_member")
+       import :: c_int, c_ptr, fcg_diag
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: member
+       type(fcg_diag), intent(out) :: d
+     end function fcg_diagnostics_member
+     integer(c_int) function fcg_mlp_create(seed, net) bind(C, name="fcg_mlp_create")
+       import :: c_int, c_ptr
+       integer(c_int), value :: seed
+       type(c_ptr), intent(out) :: net
+     end function fcg_mlp_create
+     subroutine fcg_mlp_destroy(net) bind(C, name="fcg_mlp_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: net
+     end subroutine fcg_mlp_destroy
+     integer(c_int) function fcg_mlp_set_params(net, host, count) bind(C, name="fcg_mlp_set_params")
+       import :: c_int, c_ptr, c_float, c_long
+       type(c_ptr), value :: net
+       real(c_float), intent(in) :: host(*)
+       integer(c_long), value :: count
+     end function fcg_mlp_set_params
+     integer(c_int) function fcg_mlp_set_norm(net, x_mean, x_std, y_mean, y_std, x_lo, x_hi) &
+         bind(C, name="fcg_mlp_set_norm")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: net
+       real(c_float), intent(in) :: x_mean(5), x_std(5), y_mean(2), y_std(2), x_lo(5), x_hi(5)
+     end function fcg_mlp_set_norm
+     integer(c_int) function fcg_set_mlp(ctx, net) bind(C, name="fcg_set_mlp")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, net
+     end function fcg_set_mlp
+     integer(c_int) function fcg_set_launch(ctx, kernel, block) bind(C, name="fcg_set_launch")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: kernel(*)
+       integer(c_int), value :: block
+     end function fcg_set_launch
   end interface
 
 contains
