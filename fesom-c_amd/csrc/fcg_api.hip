@@ -31,6 +31,7 @@ template <typename R> __global__ void k_bt_p1adv(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int);
 template <typename R> __global__ void k_jac_node(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_jac_restart(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_update3d(DevMesh, DevStateT<R>, double, double);
 template <typename R> __global__ void k_vertvel_div(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_vertvel_scan(DevMesh, DevStateT<R>);
@@ -1046,6 +1047,22 @@ static int jacobian_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
 extern "C" int fcg_jacobian(fcg_ctx *ctx) {
     if (!ctx) return FCG_E_ARG;
     return ctx->mixed ? jacobian_impl(ctx, retype<float>(ctx->s)) : jacobian_impl(ctx, ctx->s);
+}
+
+template <typename R>
+static int hot_start_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
+    hipLaunchKernelGGL(fcgk::k_jac_restart<R>, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, s, ctx->p.Dmin);
+    LAUNCH_EX(ctx, k_je<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, 8 * cpb_of(ctx->m.L) * sizeof(double), ctx->m, s);
+    FCG_CHECK(hipGetLastError());
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    return FCG_OK;
+}
+
+extern "C" int fcg_hot_start(fcg_ctx *ctx, double time_jd) {
+    if (!ctx) return FCG_E_ARG;
+    const int rc = ctx->mixed ? hot_start_impl(ctx, retype<float>(ctx->s)) : hot_start_impl(ctx, ctx->s);
+    if (rc) return rc;
+    return fcg_set_clock(ctx, time_jd, 0, 0);  // read_restart_separate: lfirst = .false.
 }
 
 template <typename R>

@@ -313,6 +313,7 @@ __global__ __launch_bounds__(256) void k_d3end_prep(DevMesh m, DevStateT<R> s, d
     R *tp = (R *)lds_, *tq = tp + T, *tc = tq + T, *ts = tc + T, *tb = ts + T;
     if (f.nc <= 0) return;
     const int n = f.c0 + f.c, lev = f.k;
+    if (only && !__syncthreads_or(f.on() && only[n] != 0)) return;  // no flagged column in this block
     if (f.on() && !(only && only[n] == 0)) {
 #define SNU(z) PL(s.snu, (z) - 1, n, KC)
 #define BT(z) PL(s.bt, (z) - 1, n, KC)
@@ -445,6 +446,7 @@ __global__ __launch_bounds__(256) void k_d3end_fin(DevMesh m, DevStateT<R> s, do
     const ColFlat f = col_flat(N, K);
     R *tx = (R *)lds_;
     if (f.nc <= 0) return;
+    if (only && !__syncthreads_or(f.on() && only[f.c0 + f.c] != 0)) return;  // no flagged column in this block
     for (int r = (int)threadIdx.x; r < K * f.nc; r += (int)blockDim.x) {
         const int k = r / f.nc, j = r % f.nc;
         if (only && only[f.c0 + j] == 0) continue;
@@ -1397,6 +1399,16 @@ __global__ __launch_bounds__(256) void k_jac_node(DevMesh m, DevStateT<R> s, dou
     int n = TID();
     if (n >= m.N) return;
     s.a_old[n] = s.a[n];
+    s.a[n] = fmax(m.depth[n] + s.eta_n[n], Dmin);
+}
+
+// the set-up jacobian after a restart has replaced eta_n (src/fv_main.F90:362
+// after read_restart_separate at 336): Jc from the restored elevation; Jc_old
+// keeps the value of the array set-up jacobian (src/fv_main.F90:162)
+template <typename R>
+__global__ __launch_bounds__(256) void k_jac_restart(DevMesh m, DevStateT<R> s, double Dmin) {
+    int n = TID();
+    if (n >= m.N) return;
     s.a[n] = fmax(m.depth[n] + s.eta_n[n], Dmin);
 }
 
@@ -2631,6 +2643,7 @@ void instantiate_kernels() {
     (void)&k_bt_p1adv<R>;
     (void)&k_bt_elem2<R>;
     (void)&k_jac_node<R>;
+    (void)&k_jac_restart<R>;
     (void)&k_je<R>;
     (void)&k_update3d<R>;
     (void)&k_vertvel_div<R>;

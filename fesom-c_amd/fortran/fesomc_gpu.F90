@@ -215,6 +215,11 @@ module fesomc_gpu
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx, net
      end function fcg_set_mlp
+     integer(c_int) function fcg_hot_start(ctx, time_jd) bind(C, name="fcg_hot_start")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: time_jd
+     end function fcg_hot_start
      integer(c_int) function fcg_set_launch(ctx, kernel, block) bind(C, name="fcg_set_launch")
        import :: c_int, c_ptr, c_char
        type(c_ptr), value :: ctx

@@ -22,7 +22,7 @@ program fesomc_run
   integer :: nkeys = 0
 
   character(len=512) :: rundir, arg, meshdir
-  integer :: nsteps, irep, n_dt, i, k, n, nargs, L
+  integer :: nsteps, irep, irestart, ini_time, n_dt, i, k, n, nargs, L
   logical :: mesh_only
   type(fcg_mesh_cfg) :: mc
   type(fcg_params) :: par
@@ -58,6 +58,7 @@ program fesomc_run
   call read_run(trim(rundir) // '/fv_run.dat')
   if (nsteps < 0) nsteps = geti('nsteps', 0)
   irep = geti('IREP', 100)
+  irestart = geti('IRESTART', huge(0))
 
   mc%cartesian = merge(1, 0, getl('cartesian'))
   mc%coriolis_tf = merge(1, 0, getl('Coriolis_TF'))
@@ -161,6 +162,10 @@ program fesomc_run
   call put('SF', S)
   call put('S_old', S)
   call put('Sclim', S)
+  if (par%restart == 1) then          ! read_restart_separate (src/fv_restart.F90:507-613), fv_main.F90:336
+     call read_restart('restarts/restart_0.out')
+     nsteps = nsteps - ini_time
+  end if
 
   ! ---- main loop (src/fv_main.F90:389-746) ---------------------------------
   open(53, file='energy.dat')
@@ -183,6 +188,7 @@ program fesomc_run
            write(*,*) 'step ', n_dt, ' energy= ', dg%energy, ' max_min_ssh:', dg%eta_max, dg%eta_min
         end if
      end if
+     if (mod(n_dt, irestart) == 0) call write_restart(n_dt)           ! fv_main.F90:697
   end do
   close(53, status='KEEP')
   write(*,*) 'volume ', dg%volume, ' heat ', dg%heat, ' salt ', dg%salt
@@ -190,6 +196,95 @@ program fesomc_run
   call fcg_mesh_free(mobj)
 
 contains
+
+  ! ---- restart records (src/fv_restart.F90:474-505): Fortran unformatted
+  ! sequential, as write_restart_separate writes them for ver_mix = 3 and
+  ! type_task = 3, one record per write statement
+  subroutine write_restart(nstep)
+    integer, intent(in) :: nstep
+    character(len=10) :: dirname
+    real(c_double), allocatable :: e0(:), e1(:), e2(:), u2(:, :), u21(:, :), u22(:, :)
+    real(c_double), allocatable :: un(:, :), vn(:, :), ua(:, :), va(:, :), w(:, :), bt(:, :), sn(:, :), kv(:, :)
+    real(c_double), allocatable :: tf(:, :), sf(:, :), to(:, :), so(:, :), tc(:, :), sc(:, :)
+    integer :: ne, nk
+    ne = view%elem2D
+    nk = mc%nsigma
+    allocate(e0(nod2D), e1(nod2D), e2(nod2D), u2(2, ne), u21(2, ne), u22(2, ne))
+    allocate(un(nk - 1, ne), vn(nk - 1, ne), ua(nk - 1, ne), va(nk - 1, ne))
+    allocate(w(nk, nod2D), bt(nk, nod2D), sn(nk, nod2D), kv(nk, nod2D))
+    allocate(tf(nk - 1, nod2D), sf(nk - 1, nod2D), to(nk - 1, nod2D), so(nk - 1, nod2D), tc(nk - 1, nod2D), &
+             sc(nk - 1, nod2D))
+    call get1('eta_n', e0); call get1('eta_n_1', e1); call get1('eta_n_2', e2)
+    call get2('U_n_2D', u2); call get2('U_n_1', u21); call get2('U_n_2', u22)
+    call get2('U_n', un); call get2('V_n', vn); call get2('U_rhsAB', ua); call get2('V_rhsAB', va)
+    call get2('Wvel', w); call get2('bt', bt); call get2('snu', sn); call get2('Kv', kv)
+    call get2('TF', tf); call get2('SF', sf); call get2('T_old', to); call get2('S_old', so)
+    call get2('Tclim', tc); call get2('Sclim', sc)
+    call fcg_check(fcg_diagnostics(ctx, dg), ctx, 'fcg_diagnostics')
+    write(*,*) ' Write Restart at step: ', nstep
+    write(dirname, '(I10.10)') nstep
+    open(25, file='restarts/' // dirname // '_restart_0.out', form='unformatted')
+    write(25) nstep
+    write(25) dg%time_jd, par%time_jd0, par%dt, par%dt_2D
+    write(25) e0, e1, e2, u2, u21, u22
+    write(25) un, vn, ua, va, w
+    write(25) bt, sn, kv
+    write(25) tf, sf, to, so
+    write(25) tc, sc
+    close(25)
+  end subroutine write_restart
+
+  subroutine read_restart(path)
+    character(len=*), intent(in) :: path
+    real(c_double) :: tjd, tjd0, rdt, rdt2
+    real(c_double), allocatable :: e0(:), e1(:), e2(:), u2(:, :), u21(:, :), u22(:, :)
+    real(c_double), allocatable :: un(:, :), vn(:, :), ua(:, :), va(:, :), w(:, :), bt(:, :), sn(:, :), kv(:, :)
+    real(c_double), allocatable :: tf(:, :), sf(:, :), to(:, :), so(:, :), tc(:, :), sc(:, :)
+    integer :: ne, nk
+    ne = view%elem2D
+    nk = mc%nsigma
+    allocate(e0(nod2D), e1(nod2D), e2(nod2D), u2(2, ne), u21(2, ne), u22(2, ne))
+    allocate(un(nk - 1, ne), vn(nk - 1, ne), ua(nk - 1, ne), va(nk - 1, ne))
+    allocate(w(nk, nod2D), bt(nk, nod2D), sn(nk, nod2D), kv(nk, nod2D))
+    allocate(tf(nk - 1, nod2D), sf(nk - 1, nod2D), to(nk - 1, nod2D), so(nk - 1, nod2D), tc(nk - 1, nod2D), &
+             sc(nk - 1, nod2D))
+    write(*,*) 'Reading restart'
+    open(25, file=path, form='unformatted', status='old', action='read')
+    read(25) ini_time
+    read(25) tjd, tjd0, rdt, rdt2                                   ! time_jd0 of the file is not used
+    read(25) e0, e1, e2, u2, u21, u22
+    read(25) un, vn, ua, va, w
+    read(25) bt, sn, kv
+    read(25) tf, sf, to, so
+    read(25) tc, sc
+    close(25)
+    call put1('eta_n', e0); call put1('eta_n_1', e1); call put1('eta_n_2', e2)
+    call put('U_n_2D', u2); call put('U_n_1', u21); call put('U_n_2', u22)
+    call put('U_n', un); call put('V_n', vn); call put('U_rhsAB', ua); call put('V_rhsAB', va)
+    call put('Wvel', w); call put('bt', bt); call put('snu', sn); call put('Kv', kv)
+    call put('TF', tf); call put('SF', sf); call put('T_old', to); call put('S_old', so)
+    call put('Tclim', tc); call put('Sclim', sc)
+    call fcg_check(fcg_hot_start(ctx, tjd), ctx, 'fcg_hot_start')
+    write(*,*) 'Restart reading finished, time_jd= ', tjd, ',time_jd0=', par%time_jd0
+  end subroutine read_restart
+
+  subroutine get1(name, a)
+    character(len=*), intent(in) :: name
+    real(c_double), intent(inout) :: a(:)
+    call fcg_check(fcg_get_state(ctx, cstr(name), a, int(size(a), c_long)), ctx, 'fcg_get_state ' // name)
+  end subroutine get1
+
+  subroutine get2(name, a)
+    character(len=*), intent(in) :: name
+    real(c_double), intent(inout) :: a(:, :)
+    call fcg_check(fcg_get_state(ctx, cstr(name), a, int(size(a), c_long)), ctx, 'fcg_get_state ' // name)
+  end subroutine get2
+
+  subroutine put1(name, a)
+    character(len=*), intent(in) :: name
+    real(c_double), intent(in) :: a(:)
+    call fcg_check(fcg_set_state(ctx, cstr(name), a, int(size(a), c_long)), ctx, 'fcg_set_state ' // name)
+  end subroutine put1
 
   subroutine put(name, a)
     character(len=*), intent(in) :: name

@@ -33,7 +33,7 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_mlp_destroy", "fcg_mlp_get_params", "fcg_mlp_set_params", "fcg_mlp_set_norm", "fcg_mlp_forward",
            "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect",
            "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member", "fcg_set_launch",
-           "fcg_get_launch"]
+           "fcg_get_launch", "fcg_hot_start"]
 
 MLP_FEATURES, MLP_PARAMS = 5, 6288
 
@@ -143,6 +143,8 @@ def lib():
         L.fcg_mlp_train_step.argtypes = [P, P, P, C.c_long, C.c_float]
         L.fcg_set_mlp.restype = C.c_int
         L.fcg_set_mlp.argtypes = [P, P]
+        L.fcg_hot_start.restype = C.c_int
+        L.fcg_hot_start.argtypes = [P, C.c_double]
         L.fcg_set_launch.restype = C.c_int
         L.fcg_set_launch.argtypes = [P, C.c_char_p, C.c_int]
         L.fcg_get_launch.restype = C.c_int

@@ -180,6 +180,13 @@ int fcg_set_state(fcg_ctx *ctx, const char *name, const double *host, long count
 int fcg_get_state(fcg_ctx *ctx, const char *name, double *host, long count);
 /* Scalar state: time_jd (days), lfirst, n_dt. */
 int fcg_set_clock(fcg_ctx *ctx, double time_jd, int lfirst, int n_dt);
+/* Hot start, after the restart records (eta_n ... Sclim, src/fv_restart.F90:
+ * 518-600) have been set with fcg_set_state: the set-up jacobian from the
+ * restored elevation (the call at src/fv_main.F90:362 that follows
+ * read_restart_separate at 336; Jc_old keeps the array set-up value), the
+ * clock at the restart's time_jd and lfirst = .false. (fv_restart.F90:611).
+ * The context must have been created with restart = 1 (no tidal ramp). */
+int fcg_hot_start(fcg_ctx *ctx, double time_jd);
 
 /* ---- the hot path: one entry per call site of oce_timestep ------------
  * fcg_advance_clock:        time_jd += dt/86400; time = time_jd*86400 (src/fv_main.F90:400-401)
