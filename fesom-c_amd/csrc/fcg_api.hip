@@ -90,14 +90,17 @@ inline size_t flat_bytes(int S, size_t esz) { return (size_t)cpb_of(S) * S * esz
 // k_velrhs: per column one element record and four edge records
 inline size_t velrhs_bytes(const DevMesh &m, size_t esz) {
     return (size_t)cpb_of(m.L) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) +
-           (size_t)cpb_of(m.L) * m.L * (3 * sizeof(double) + 2 * esz);
+           (size_t)cpb_of(m.L) * m.L * (3 * sizeof(double) + 6 * esz);
 }
 inline size_t lds_p1adv(const DevMesh &m) {
     return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8 + 4 * (size_t)cpb_of(m.L) * sizeof(P1Rec);
 }
 // node-gather blocks: LDS records for every CSR slot of the block's nodes
 template <typename R>
-inline size_t lds_muscl(const fcg_ctx *c) { return (size_t)c->cap_n2e * (sizeof(EdgeGeo<R>) + 1); }
+inline size_t lds_muscl(const fcg_ctx *c) {
+    return (size_t)c->cap_n2e * sizeof(EdgeGeo<R>) + (size_t)((c->cap_n2e + 1) & ~1) * sizeof(int) +
+           6 * flat_bytes(c->m.L, sizeof(R));
+}
 template <typename R>
 inline size_t lds_vdiv(const fcg_ctx *c) { return (size_t)c->cap_n2e * sizeof(DivRec<R>); }
 inline size_t lds_gradn(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(GradRec); }

@@ -707,7 +707,8 @@ __device__ __forceinline__ void uvert_lds(const double *ws, const R *u, const R 
 // the flux uses them (storage type R for the geometry, FP64 node depths).
 template <typename R>
 struct EdgeRec {
-    int cc, oK, pad;     // signed edge id (kEmpty: none; < 0: the element is edge_tri(1)), the other element's column offset
+    int cc, oK, oc;      // signed edge id (kEmpty: none; < 0: the element is edge_tri(1)), the other element's
+                         // column offset, and its column in this block (-1: outside)
     R acc, cx, cy, cz, cw, dx, dy, cos1, cos2;
     double an1, an2;     // a at the edge's nodes
 };
@@ -738,10 +739,11 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
     const ColFlat f = col_flat(m.E, L);
     ElemRec *er = (ElemRec *)lds_;                                        // [cpb]
     EdgeRec<R> *edr = (EdgeRec<R> *)(lds_ + f.cpb * kVelrhsElemBytes);   // [cpb][4]
-    // per column and layer: sum w_cv*W, Zw, zbw, U_n, V_n
+    // per column and layer: sum w_cv*W, Zw, zbw, U_n, V_n, and the velocity
+    // gradients (read by the block's neighbours across internal edges)
     const int T = f.cpb * L;
     double *cws = (double *)(edr + 4 * f.cpb), *cZw = cws + T, *czb = cZw + T;
-    R *cu = (R *)(czb + T), *cv = cu + T;
+    R *cu = (R *)(czb + T), *cv = cu + T, *cgux = cv + T, *cguy = cgux + T, *cgvx = cguy + T, *cgvy = cgvx + T;
     if (f.nc <= 0) return;
     if ((int)threadIdx.x < f.nc) {
         const int c = threadIdx.x, e = f.c0 + c;
@@ -771,7 +773,9 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
                 const int2 et = m.edtri[ed];
                 const double4 cd = m.ecross[ed];
                 const double2 dd = m.edxdy[ed];
-                g.oK = (cc >= 0 ? et.x : et.y) * K;
+                const int oe = cc >= 0 ? et.x : et.y;
+                g.oK = oe * K;
+                g.oc = (oe >= f.c0 && oe < f.c0 + f.nc) ? oe - f.c0 : -1;
                 g.acc = (R)(0.5 * (m.ac[nd.x] + m.ac[nd.y]));
                 g.cx = (R)cd.x;
                 g.cy = (R)cd.y;
@@ -797,6 +801,12 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
         czb[c * L + k] = col.zbw(m, k);
         cu[c * L + k] = PL(s.U_n, k, e, KC);
         cv[c * L + k] = PL(s.V_n, k, e, KC);
+        if (hadv) {
+            cgux[c * L + k] = PL(s.gux, k, e, KC);
+            cguy[c * L + k] = PL(s.guy, k, e, KC);
+            cgvx[c * L + k] = PL(s.gvx, k, e, KC);
+            cgvy[c * L + k] = PL(s.gvy, k, e, KC);
+        }
     }
     __syncthreads();
     if (!f.on()) return;
@@ -812,15 +822,31 @@ __global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, doubl
     R abv = (R)(-u * r.cor * area * je);
     if (hadv) {
         const double ds = m.dsig[k];
-        const int ek = e * KC + k;
+        const int ck = c * L + k;
         // the element's own velocity and gradients at this layer (one end of every edge)
-        const R uo = s.U_n[ek], vo = s.V_n[ek], guxo = s.gux[ek], guyo = s.guy[ek], gvxo = s.gvx[ek], gvyo = s.gvy[ek];
+        const R uo = cu[ck], vo = cv[ck], guxo = cgux[ck], guyo = cguy[ck], gvxo = cgvx[ck], gvyo = cgvy[ck];
         for (int j = 0; j < 4; ++j) {
             const EdgeRec<R> &g = edr[c * 4 + j];
             if (g.cc == kEmpty) break;
             const bool own1 = g.cc < 0;  // the element is edge_tri(1)
-            const int io = g.oK + k;
-            const R ux = s.U_n[io], vx = s.V_n[io], guxx = s.gux[io], guyx = s.guy[io], gvxx = s.gvx[io], gvyx = s.gvy[io];
+            R ux, vx, guxx, guyx, gvxx, gvyx;
+            if (g.oc >= 0) {  // the other element is a column of this block
+                const int i = g.oc * L + k;
+                ux = cu[i];
+                vx = cv[i];
+                guxx = cgux[i];
+                guyx = cguy[i];
+                gvxx = cgvx[i];
+                gvyx = cgvy[i];
+            } else {
+                const int io = g.oK + k;
+                ux = s.U_n[io];
+                vx = s.V_n[io];
+                guxx = s.gux[io];
+                guyx = s.guy[io];
+                gvxx = s.gvx[io];
+                gvyx = s.gvy[io];
+            }
             const R U1 = own1 ? uo : ux, V1 = own1 ? vo : vx, U2 = own1 ? ux : uo, V2 = own1 ? vx : vo;
             const R gux1 = own1 ? guxo : guxx, guy1 = own1 ? guyo : guyx, gvx1 = own1 ? gvxo : gvxx, gvy1 = own1 ? gvyo : gvyx;
             const R gux2 = own1 ? guxx : guxo, guy2 = own1 ? guyx : guyo, gvx2 = own1 ? gvxx : gvxo, gvy2 = own1 ? gvyx : gvyo;
@@ -2147,33 +2173,64 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
     const ColFlat f = col_flat(m.N, m.L);
     if (f.nc <= 0) return;
     const int2 jr = csr_block(m.n2e_ptr, f);
+    const int L = m.L, T = f.cpb * L;
     EdgeGeo<R> *rec = (EdgeGeo<R> *)lds_;
-    unsigned char *first = (unsigned char *)(rec + (jr.y - jr.x));  // node is edge_nodes(1)
+    int *oc = (int *)(rec + (jr.y - jr.x));  // the other end's column in this block (-1: outside), sign: node is edge_nodes(1)
+    R *cT = (R *)(oc + ((jr.y - jr.x + 1) & ~1)), *cS = cT + T, *cTx = cS + T, *cTy = cTx + T, *cSx = cTy + T, *cSy = cSx + T;
     {
         const EdgeGeo<R> *eg = (const EdgeGeo<R> *)m.egeo;
         for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
             const int cc = m.n2e[jr.x + t];
-            rec[t] = eg[cc >= 0 ? cc : ~cc];
-            first[t] = cc >= 0;
+            const EdgeGeo<R> g = eg[cc >= 0 ? cc : ~cc];
+            rec[t] = g;
+            const int on = (cc >= 0 ? g.n2K : g.n1K) / KC - f.c0;
+            const int col = (on >= 0 && on < f.nc) ? on : -1;
+            oc[t] = cc >= 0 ? col + 2 : -(col + 2);  // |oc| - 2 = column, sign = orientation
         }
+    }
+    const int n = f.c0 + f.c, k = f.k, nk = n * KC + k, ck = f.c * L + k;
+    if (f.on()) {
+        cT[ck] = s.abT[nk];
+        cS[ck] = s.abS[nk];
+        cTx[ck] = s.gTx[nk];
+        cTy[ck] = s.gTy[nk];
+        cSx[ck] = s.gSx[nk];
+        cSy[ck] = s.gSy[nk];
     }
     __syncthreads();
     if (!f.on()) return;
-    const int n = f.c0 + f.c, k = f.k, nk = n * KC + k;
     const int mb = m.memb[n];
     const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
     const double a = s.a[n];
     const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
-    const R To = s.abT[nk], So = s.abS[nk], gTxo = s.gTx[nk], gTyo = s.gTy[nk], gSxo = s.gSx[nk], gSyo = s.gSy[nk];
+    const R To = cT[ck], So = cS[ck], gTxo = cTx[ck], gTyo = cTy[ck], gSxo = cSx[ck], gSyo = cSy[ck];
     // horizontal fluxes in the storage type (FP32 in the mixed-precision path)
     R rt = R(0), rs = R(0), rc = R(0);
     for (int j = p0 - jr.x; j < p1 - jr.x; ++j) {
         const EdgeGeo<R> &g = rec[j];
-        const bool own1 = first[j];
+        const int o = oc[j];
+        const bool own1 = o > 0;
+        const int col = (own1 ? o : -o) - 2;
         const R cdx = g.cx, cdy = g.cy, cdz = g.cz, cdw = g.cw, ddy = g.dy;
-        const int ao = (own1 ? g.n2K : g.n1K) + k, i1 = g.e1K + k;
-        const R Tx = s.abT[ao], Sx = s.abS[ao];
-        const R gTxx = s.gTx[ao], gTyx = s.gTy[ao], gSxx = s.gSx[ao], gSyx = s.gSy[ao];
+        const int i1 = g.e1K + k;
+        R Tx, Sx, gTxx, gTyx, gSxx, gSyx;
+        if (col >= 0) {  // the other end is a column of this block
+            const int i = col * L + k;
+            Tx = cT[i];
+            Sx = cS[i];
+            gTxx = cTx[i];
+            gTyx = cTy[i];
+            gSxx = cSx[i];
+            gSyx = cSy[i];
+        } else {
+            const int ao = (own1 ? g.n2K : g.n1K) + k;
+            Tx = s.abT[ao];
+            Sx = s.abS[ao];
+            gTxx = s.gTx[ao];
+            gTyx = s.gTy[ao];
+            gSxx = s.gSx[ao];
+            gSyx = s.gSy[ao];
+        }
         const R T1 = own1 ? To : Tx, T2 = own1 ? Tx : To;
         const R S1 = own1 ? So : Sx, S2 = own1 ? Sx : So;
         const R gT1x = own1 ? gTxo : gTxx, gT2x = own1 ? gTxx : gTxo;
