@@ -56,9 +56,11 @@ def kernel_bytes(k, N, E, D, L, K, e3=8):
               + e3 * (2 * LN) + e3 * (3 * LE) + e3 * KN + e3 * (2 * LN) + e3 * KN)
         return rd + e3 * 4 * LN       # writes TF,SF,T_old,S_old
     if k == "k_velrhs":
-        rd = (mesh_e + E * 8 * d8 + 3 * d8 * E + 4 * i4 * E + d8 * N * 4 + e3 * (2 * LE + 2 * LE + 2 * LE + 4 * LE)
-              + e3 * KN * 2 + i4 * D * 4 + d8 * D * 6 + d8 * E * 4)
-        return rd + e3 * 4 * LE       # writes U_n, V_n, U_rhsAB, V_rhsAB
+        # element geometry (en, w_cv, gradient_sca, area, coriolis, elem_cos, mask, el_ied), node ac/atp/eta/a,
+        # edge nodes/elements/cross/dxdy; U, V, AB history, Bpu/Bpv, 4 velocity gradients (E x L), W (N x K)
+        rd = (mesh_e + E * 8 * d8 + 4 * d8 * E + 4 * i4 * E + d8 * N * 5 + e3 * (2 * LE + 2 * LE + 2 * LE + 4 * LE)
+              + e3 * KN + i4 * D * 4 + d8 * D * 6)
+        return rd + e3 * 4 * LE       # writes U_rhsAB, V_rhsAB, U_rhs, V_rhs
     if k == "k_d3end":
         rd = i4 * (N + 1) + i4 * 4 * E + d8 * E + e3 * 2 * LE + e3 * 2 * LN + e3 * 3 * KN + d8 * 5 * N
         return rd + e3 * 3 * KN       # writes bt, snu, Kv
@@ -126,6 +128,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2")
+    ap.add_argument("--scale", default=None, metavar="NX,NY",
+                    help="mesh nodes along and across the channel instead of the config's (same generator)")
     ap.add_argument("--probe", default="k_velrhs", help="kernel timed live for the roofline (the dominant one)")
     ap.add_argument("--error-steps", default="0", metavar="N|m2",
                     help="after the timed run, advance both the headline and the FP64 model (untimed) to N internal "
@@ -163,7 +167,8 @@ def main():
 
     def run(precision, clos, probe):
         t_setup = time.perf_counter()
-        txt, mesh, ex, model = build_model(args.config, precision=precision, closure=clos)
+        scale = tuple(int(v) for v in args.scale.split(",")) if args.scale else None
+        txt, mesh, ex, model = build_model(args.config, scale=scale, precision=precision, closure=clos)
         if clos == gpu.FCG_CLOSURE_MLP:
             model.attach_mlp(closure.load(args.closure_weights))
         table = args.launch_table
@@ -222,7 +227,7 @@ def main():
                    "nodes": N, "elements": E, "layers": L, "Mt": model.par.Mt, "path": args.path},
         "roofline": {"kernel": args.probe, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": pmc_traffic(args.probe, tfile) if args.config == "c2" else None,
+                     "traffic": pmc_traffic(args.probe, tfile) if (args.config == "c2" and not args.scale) else None,
                      "avg_ms": avg_ms, "launches": k_n, "algorithmic_bytes": kb},
         "setup_s": setup_s,
         "energy": diag["energy"],

@@ -633,7 +633,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
     DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
-    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
     DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
     DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
     DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA3(s.U_puls, EL) DA3(s.V_puls, EL)

@@ -129,6 +129,7 @@ struct DevStateT {
     double *a = nullptr, *a_old = nullptr;                     // N: max(depth+eta_n, Dmin) now / previous jacobian
     double *ubot = nullptr;                                    // 2E: U_n, V_n of the bottom layer (substep cache)
     double *fl_ssh = nullptr, *fl_uv = nullptr;                // D, 2D: per-edge transport and 2-D momentum flux
+    double *node2d = nullptr;                                  // 4N: etaAB, eta_g, eta_n_1, eta_n per node (substep)
     double *tide = nullptr;                                    // Mt*nobn prescribed OB elevation per substep
     // 3-D, stored by column: K slots per node/element (layer fields use L)
     R *U_n = nullptr, *V_n = nullptr, *U_rhs = nullptr, *V_rhs = nullptr;   // L*E

@@ -1096,11 +1096,14 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, doub
     const double de = 0.614, ga = 0.088, ep = 0.013;
     const double fe = 1.0 - de - ga - ep;
     double ea = de * en + fe * e0 + ga * e1 + ep * e2;
+    const double eg = kG * (m.atp1[n] * ea - m.atp2[n] * 0.0);
     s.etaAB[n] = ea;
-    s.eta_g[n] = kG * (m.atp1[n] * ea - m.atp2[n] * 0.0);
+    s.eta_g[n] = eg;
     s.eta_n_2[n] = e1;
     s.eta_n_1[n] = e0;
     s.eta_n[n] = en;
+    // the four node values update_2D_vel gathers, as one record per node
+    ((double4 *)s.node2d)[n] = make_double4(ea, eg, e0, en);
 }
 
 // vel_gradients_2D(UAB): src/fv_gradients.F90:97-176
@@ -1356,7 +1359,10 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
     const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
     const double area = m.elem_area[e], cor = m.coriolis[e];
     const double d0 = m.depth[q.x], d1 = m.depth[q.y], d2 = m.depth[q.z], d3 = m.depth[q.w];
-    const double ea0 = s.etaAB[q.x], ea1 = s.etaAB[q.y], ea2 = s.etaAB[q.z], ea3 = s.etaAB[q.w];
+    // per node: etaAB, eta_g, eta^n (eta_n_1), eta^{n+1} (eta_n), written by k_bt_node this substep
+    const double4 r0 = ((const double4 *)s.node2d)[q.x], r1 = ((const double4 *)s.node2d)[q.y];
+    const double4 r2 = ((const double4 *)s.node2d)[q.z], r3 = ((const double4 *)s.node2d)[q.w];
+    const double ea0 = r0.x, ea1 = r1.x, ea2 = r2.x, ea3 = r3.x;
     double mask = s.mask_wd[e];
     if (wet_dry) {  // wad_mask: fv_main.F90:1284-1294
         double mn = fmin(fmin(fmin(d0, d1), d2), d3);
@@ -1367,7 +1373,7 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
     s.C_d_el[e] = C_d;
     const double rho_inv = 1.0 / (rho0);
     const double dm = dmaxr(Dmin, w.x * (ea0 + d0) + w.y * (ea1 + d1) + w.z * (ea2 + d2) + w.w * (ea3 + d3));
-    const double eg0 = s.eta_g[q.x], eg1 = s.eta_g[q.y], eg2 = s.eta_g[q.z], eg3 = s.eta_g[q.w];
+    const double eg0 = r0.y, eg1 = r1.y, eg2 = r2.y, eg3 = r3.y;
     const double uab = s.UAB[2 * e], vab = s.UAB[2 * e + 1];
     double ru = area * (-dm * (gx.x * eg0 + gx.y * eg1 + gx.z * eg2 + gx.w * eg3) + vab * dm * cor + s.taux[e] * rho_inv);
     double rv = area * (-dm * (gy.x * eg0 + gy.y * eg1 + gy.z * eg2 + gy.w * eg3) - uab * dm * cor + s.tauy[e] * rho_inv);
@@ -1390,13 +1396,12 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
     ru = ru + s.U_rhs_2D_3D[2 * e];
     rv = rv + s.U_rhs_2D_3D[2 * e + 1];
     // eta_n_1 holds eta^n, eta_n holds eta^{n+1} (the reference's ssh_rhs)
-    const double dmo = dmaxr(Dmin, sum4(w, s.eta_n_1[q.x] + d0, s.eta_n_1[q.y] + d1, s.eta_n_1[q.z] + d2,
-                                        s.eta_n_1[q.w] + d3));
-    const double acc = sum4(w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+    const double dmo = dmaxr(Dmin, sum4(w, r0.z + d0, r1.z + d1, r2.z + d2, r3.z + d3));
+    const double acc = m.acc_el[e];  // sum w_cv*ac, formed at set-up in the same order
     const double a_Cd = C_d * ((1.0 - acc) * Cd_bz + 1.0);
     const double ub = s.ubot[2 * e], vb = s.ubot[2 * e + 1];  // U_n, V_n at the bottom layer
     const double friction = a_Cd * sqrt(ub * ub + vb * vb);
-    const double snew = sum4(w, s.eta_n[q.x] + d0, s.eta_n[q.y] + d1, s.eta_n[q.z] + d2, s.eta_n[q.w] + d3);
+    const double snew = sum4(w, r0.w + d0, r1.w + d1, r2.w + d2, r3.w + d3);
     const double ffc = 1.0 / (dmaxr(Dmin, snew) + friction * dt_2D);
     const double U0 = s.U2[2 * e], V0 = s.U2[2 * e + 1];
     const double un = mask * ((dmo * U0 - dt_2D * friction * (ub - U0) + dt_2D * s.Bar_pr_2D[2 * e]
@@ -1409,8 +1414,7 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
     s.U2_1[2 * e + 1] = V0;
     s.U2[2 * e] = un;
     s.U2[2 * e + 1] = vn;
-    const double dmf = dmaxr(Dmin, w.x * (s.eta_n[q.x] + d0) + w.y * (s.eta_n[q.y] + d1) + w.z * (s.eta_n[q.z] + d2)
-                                   + w.w * (s.eta_n[q.w] + d3));
+    const double dmf = dmaxr(Dmin, w.x * (r0.w + d0) + w.y * (r1.w + d1) + w.z * (r2.w + d2) + w.w * (r3.w + d3));
     const double uf0 = first ? 0.0 : s.U_filt[e];
     const double vf0 = first ? 0.0 : s.V_filt[e];
     s.U_filt[e] = uf0 + un * dmf;
