@@ -52,7 +52,7 @@ __global__ void k_mlp_collect(DevMesh, DevStateT<R>, double, double, double, int
 template <typename R>
 __global__ void k_mlp_closure(DevMesh, DevStateT<R>, const float *, const fcgm::bf16x4 *, MlpNorm, double, double, double,
                               double, int *);
-__global__ void k_count_flags(const int *, int, int *);
+__global__ void k_count_flags(const int *, int, unsigned long long *);
 
 }  // namespace fcgk
 
@@ -1187,8 +1187,8 @@ static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out, 
     out->n_wet_elem = wet;
     out->status = err;
     if (ctx->mlp_count) {
-        int fb = 0;
-        FCG_CHECK(hipMemcpy(&fb, ctx->mlp_count, sizeof(int), hipMemcpyDeviceToHost));
+        unsigned long long fb = 0;
+        FCG_CHECK(hipMemcpy(&fb, ctx->mlp_count, sizeof fb, hipMemcpyDeviceToHost));
         out->mlp_fallback = fb;
     }
     return FCG_OK;

@@ -200,7 +200,7 @@ struct fcg_ctx {
     // learned closure
     fcg_mlp *mlp = nullptr;      // attached network (not owned)
     int *mlp_flag = nullptr;     // N: column sent back to d3_end this step
-    int *mlp_count = nullptr;    // cumulative count of such columns
+    unsigned long long *mlp_count = nullptr;  // such columns summed over steps
     std::string probe_name;
     std::unordered_map<std::string, int> launch_block;  // autotuned block sizes
     std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev;

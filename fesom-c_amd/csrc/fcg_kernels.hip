@@ -2589,8 +2589,8 @@ __global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, 
     PL(s.Kv, nz - 1, n, KC) = (R)exp10f(lk);
 }
 
-__global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, int *count) {
-    int c = 0;
+__global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, unsigned long long *count) {
+    unsigned long long c = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += flag[i] != 0;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);

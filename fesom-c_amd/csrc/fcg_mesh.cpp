@@ -10,6 +10,7 @@
 //   mesh_arrays1, mesh_arrays2          src/fv_mesh_array.F90:948-1386
 //   initial_state constants             src/fv_ini.F90:81,94-107,187-240
 // Built with -ffp-contract=off.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -480,6 +481,41 @@ extern "C" int fcg_mesh_build(const fcg_mesh_cfg *cfg, int N, const double *xy, 
         if (m->index[n] == 2) { m->a_tp[n] = 1.0; m->a_tp[N + n] = 0.0; }
     }
     m->ac.assign(N, 1.0);
+    // Sponge layer (ac_create_Xobn + ac_create, src/fv_main.F90:297-300,
+    // src/fv_utilit.F90:190-212): ac = d/SL_radius where the distance d to the
+    // nearest open-boundary node (Dist2in1, src/fv_utilit.F90:53-88) is below
+    // SL_radius.  Cartesian coordinates are held divided by the earth radius.
+    if (cfg->SL_obn && m->nobn > 0) {
+        if (!(cfg->SL_radius > 0.0)) { delete m; return FCG_E_ARG; }
+        std::vector<double> ox(m->nobn), oy(m->nobn), coy(m->nobn);
+        for (int j = 0; j < m->nobn; ++j) {
+            ox[j] = m->xy[2 * (size_t)(m->in_obn[j] - 1)];
+            oy[j] = m->xy[2 * (size_t)(m->in_obn[j] - 1) + 1];
+            coy[j] = std::cos(oy[j]);
+        }
+        for (int n = 0; n < N; ++n) {
+            const double x1 = m->xy[2 * (size_t)n], x2 = m->xy[2 * (size_t)n + 1];
+            double d2obn;
+            if (cfg->cartesian) {
+                double d = (ox[0] - x1) * (ox[0] - x1) + (oy[0] - x2) * (oy[0] - x2);
+                for (int j = 1; j < m->nobn; ++j) {
+                    const double v = (ox[j] - x1) * (ox[j] - x1) + (oy[j] - x2) * (oy[j] - x2);
+                    if (v < d) d = v;
+                }
+                d2obn = std::sqrt(d) * kREarth;
+            } else {
+                const double cx2 = std::cos(x2);
+                double a = 0.0;
+                for (int j = 0; j < m->nobn; ++j) {
+                    const double s2 = std::sin((oy[j] - x2) * 0.5), s1 = std::sin((ox[j] - x1) * 0.5);
+                    const double v = s2 * s2 + cx2 * coy[j] * s1 * s1;
+                    if (j == 0 || v < a) a = v;
+                }
+                d2obn = 2.0 * kREarth * std::asin(std::min(1.0, std::sqrt(a)));
+            }
+            if (d2obn < cfg->SL_radius) m->ac[n] = d2obn / cfg->SL_radius;
+        }
+    }
     m->density_0 = 1000.0;
     m->density_0 = m->density_0 + density_jm(cfg->T_const, cfg->S_const, 0.0, m->density_0);
     m->amp_factor.assign(12, 1.0);

@@ -37,13 +37,17 @@ module fesomc_gpu
 
   type, bind(C) :: fcg_diag
      real(c_double) :: energy, volume, heat, salt, eta_min, eta_max, T_min, T_max, S_min, S_max, time_jd
-     integer(c_int) :: n_dt, n_wet_elem, mlp_fallback, status
+     integer(c_int) :: n_dt, n_wet_elem
+     integer(c_long_long) :: mlp_fallback
+     integer(c_int) :: status
   end type fcg_diag
 
   type, bind(C) :: fcg_mesh_cfg
      integer(c_int) :: cartesian, coriolis_tf, nsigma, ind_sigma, is_LE, T_potential, tf_presence
      real(c_double) :: lat_cartesian_deg, cyclic_length_deg, pow_sig, lev_bot, lev_sur, T_const, S_const
      integer(c_int) :: harmonics(12)
+     integer(c_int) :: SL_obn
+     real(c_double) :: SL_radius
   end type fcg_mesh_cfg
 
   interface

@@ -74,6 +74,8 @@ program fesomc_run
   mc%lev_sur = getf('lev_sur', 1.0d0)
   mc%T_const = getf('T_const', 10.0d0)
   mc%S_const = getf('S_const', 35.0d0)
+  mc%SL_obn = merge(1, 0, getl('SL_obn') .and. .not. getl('SL_obn_user_def'))
+  mc%SL_radius = getf('SL_radius', 0.0d0)
   mc%harmonics = 0
   if (getl('TF_presence') .or. getl('T_potential')) call parse_harmonics(gets_line('Harmonics_tf'), mc%harmonics)
   n_harm = sum(mc%harmonics)

@@ -65,13 +65,13 @@ class MeshCfg(C.Structure):
                                        "tf_presence")] + \
                [(n, C.c_double) for n in ("lat_cartesian_deg", "cyclic_length_deg", "pow_sig", "lev_bot",
                                           "lev_sur", "T_const", "S_const")] + \
-               [("harmonics", C.c_int * 12)]
+               [("harmonics", C.c_int * 12), ("SL_obn", C.c_int), ("SL_radius", C.c_double)]
 
 
 class Diag(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("energy", "volume", "heat", "salt", "eta_min", "eta_max", "T_min",
                                           "T_max", "S_min", "S_max", "time_jd")] + \
-               [(n, C.c_int) for n in ("n_dt", "n_wet_elem", "mlp_fallback", "status")]
+               [("n_dt", C.c_int), ("n_wet_elem", C.c_int), ("mlp_fallback", C.c_longlong), ("status", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -187,6 +187,9 @@ def mesh_cfg(cfg: RunConfig) -> MeshCfg:
     c.S_const = cfg.f("S_const", 35.0)
     for j, v in enumerate(cfg.harmonics):
         c.harmonics[j] = v
+    # src/fv_main.F90:297-300: ac_create runs for SL_obn unless SL_obn_user_def
+    c.SL_obn = int(cfg.l("SL_obn") and not cfg.l("SL_obn_user_def"))
+    c.SL_radius = cfg.f("SL_radius", 0.0)
     return c
 
 
@@ -388,7 +391,7 @@ def mesh_arrays(cfg: RunConfig, mesh: meshio.RawMesh) -> dict:
                index=I(v.index_nod2D, N), depth=I(v.depth, N), sigma=I(v.sigma, K), area=I(v.area, N),
                elem_area=I(v.elem_area, E), elem_cos=I(v.elem_cos, E), coriolis=I(v.coriolis, E),
                w_cv=I(v.w_cv, 4 * E), gradient_sca=I(v.gradient_sca, 8 * E), gradient_vec=I(v.gradient_vec, 8 * E),
-               edge_dxdy=I(v.edge_dxdy, 2 * D), edge_cross_dxdy=I(v.edge_cross_dxdy, 4 * D), a_tp=I(v.a_tp, 2 * N),
+               edge_dxdy=I(v.edge_dxdy, 2 * D), edge_cross_dxdy=I(v.edge_cross_dxdy, 4 * D), a_tp=I(v.a_tp, 2 * N), ac=I(v.ac, N),
                density_0=v.density_0, scale_area=v.scale_area)
     if v.nobn:
         out["in_obn"] = I(v.in_obn, v.nobn)

@@ -119,7 +119,7 @@ typedef struct fcg_diag {
     double time_jd;      /* model clock after the last step                   */
     int n_dt;            /* internal steps taken                              */
     int n_wet_elem;      /* sum of mask_wd                                    */
-    int mlp_fallback;    /* columns sent back to d3_end by the OOD guard      */
+    long long mlp_fallback; /* column-steps sent back to d3_end by the OOD guard */
     int status;          /* first device-side STOP condition (FCG_E_* or 0)   */
 } fcg_diag;
 
@@ -139,6 +139,8 @@ typedef struct fcg_mesh_cfg {
     int cartesian, coriolis_tf, nsigma, ind_sigma, is_LE, T_potential, tf_presence;
     double lat_cartesian_deg, cyclic_length_deg, pow_sig, lev_bot, lev_sur, T_const, S_const;
     int harmonics[12];           /* 1 if harmonic j (M2,S2,...,M4) is active  */
+    int SL_obn;                  /* sponge layer at the open boundary: ac < 1 */
+    double SL_radius;            /* its width in metres (fv_var.F90:207-209)  */
 } fcg_mesh_cfg;
 typedef struct fcg_mesh_obj fcg_mesh_obj;
 
@@ -235,8 +237,8 @@ int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches);
  * of each (column, level) -- log10 shear^2, asinh(1e5 N^2), log10 depth,
  * log10 bottom stress, sigma -- to log10 snu and log10 Kv (5 -> 64 -> 64 -> 2,
  * tanh, BF16 matrix-core products with FP32 accumulation).  Columns whose
- * normalised features leave the box [x_lo, x_hi] fall back to d3_end; their
- * cumulative count is fcg_diag.mlp_fallback.
+ * normalised features leave the box [x_lo, x_hi] fall back to d3_end; the
+ * number of such columns summed over all steps is fcg_diag.mlp_fallback.
  *
  * Training uses device buffers (X: n x 5 normalised features, T: n x 2
  * normalised targets, float, row-major) owned by the caller. */

@@ -51,7 +51,7 @@ typedef struct {
     char title[256];
     double dt_2D, dt, time_jd0, Dcr, Dmin, C_d, z0b_min, z0s_min, za, snul;
     double T_const, S_const, lat_cartesian, cyclic_length, clim_relax, relax2clim_ac;
-    double Cd_bz, Bp_bz, pow_sig, lev_bot, lev_sur;
+    double Cd_bz, Bp_bz, pow_sig, lev_bot, lev_sur, SL_radius;
     int nsteps, irep, cartesian, coriolis_tf, mom_adv_2D, mom_adv_3D, tracer_adv, vert_adv;
     int im_vert_visc, im_vert_visc_fic, fic_point, BFC, ver_mix, nsigma, ind_sigma;
     int temp_on, salinity_on, barotropic, barotropic_3D, wet_dry_on, TF_presence, T_potential;
@@ -2079,7 +2079,8 @@ static void parse_config(orc_t *o, const char *t) {
     o->wet_dry_on = cfg_l(t, "WET_DRY_ON", 0);
     o->mixed_BP_grad_OB = cfg_l(t, "mixed_BP_grad_OB", 0);
     o->Cd_bz = cfg_f(t, "Cd_bz", 0); o->Bp_bz = cfg_f(t, "Bp_bz", 0);
-    o->SL_obn = cfg_l(t, "SL_obn", 0);
+    o->SL_obn = cfg_l(t, "SL_obn", 0) && !cfg_l(t, "SL_obn_user_def", 0); /* src/fv_main.F90:297 */
+    o->SL_radius = cfg_f(t, "SL_radius", 0.0);
     o->muscl_call = cfg_l(t, "MUSCL_CALL", 0);
     if (scan_key(t, "RESTART", v, sizeof v)) o->restart = strcmp(v, "hot_start") == 0; else o->restart = 0;
     o->Mt = (int)(o->dt / o->dt_2D); /* src/fv_read_run.F90:1060 */
@@ -2108,7 +2109,8 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     parse_config(o, cfg);
     if (o->type_task != 3) { o->status = -10; snprintf(o->errmsg, sizeof o->errmsg, "only type_task=3"); return o; }
     if (o->filt_2D || o->filt_bi_2D || o->bih_2D || o->filt_3D || o->filt_bi_3D || o->bih_3D || o->mixed_BP_grad_OB
-        || o->key_atm || o->key_rivers || o->T_potential || o->SL_obn || o->ver_mix != 3) {
+        || o->key_atm || o->key_rivers || o->T_potential || o->ver_mix != 3
+        || (o->SL_obn && !(o->SL_radius > 0.0))) {
         o->status = -14;
         snprintf(o->errmsg, sizeof o->errmsg, "configuration uses an option outside the restated hot path");
         return o;
@@ -2172,6 +2174,32 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     for (size_t i = 0; i < EK; i++) o->Av[i] = o->snul;
     for (int e = 0; e < E; e++) { o->mask_wd[e] = 1.0; o->mask_bpr[e] = 1.0; }
     for (int n = 0; n < N; n++) { o->mask_wd_node[n] = 1.0; o->mask_ad[n] = 1.0; o->ac[n] = 1.0; }
+    /* sponge layer: ac_create (src/fv_utilit.F90:190-212) with Dist2in1
+     * (src/fv_utilit.F90:53-88) over the open-boundary nodes, in the
+     * order ac_create_Xobn lists them (src/fv_utilit.F90:170-185) */
+    if (o->SL_obn && o->nobn > 0)
+        for (int n = 0; n < N; n++) {
+            double x1 = o->coord[2 * n], x2 = o->coord[2 * n + 1], d2obn;
+            if (o->cartesian) {
+                double d = 0.0;
+                for (int j = 0; j < o->nobn; j++) {
+                    double a = o->coord[2 * o->in_obn[j]] - x1, b = o->coord[2 * o->in_obn[j] + 1] - x2;
+                    double v = a * a + b * b;
+                    if (j == 0 || v < d) d = v;
+                }
+                d2obn = sqrt(d) * R_EARTH;
+            } else {
+                double a = 0.0, cx2 = cos(x2);
+                for (int j = 0; j < o->nobn; j++) {
+                    double y = o->coord[2 * o->in_obn[j] + 1], x = o->coord[2 * o->in_obn[j]];
+                    double s2 = sin((y - x2) * 0.5), s1 = sin((x - x1) * 0.5);
+                    double v = s2 * s2 + cx2 * cos(y) * s1 * s1;
+                    if (j == 0 || v < a) a = v;
+                }
+                d2obn = 2.0 * R_EARTH * asin(fmin(1.0, sqrt(a)));
+            }
+            if (d2obn < o->SL_radius) o->ac[n] = d2obn / o->SL_radius;
+        }
     o->lfirst = 1;
     jacobian(o); /* src/fv_main.F90:162 */
     /* initial_state: src/fv_ini.F90:1-590 */

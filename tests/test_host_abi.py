@@ -52,7 +52,7 @@ def _compare_mesh(txt, mesh):
     for k in ("elem_nodes", "edge_nodes", "edge_tri", "elem_edges", "elem_neighbors"):
         assert np.array_equal(p[k], oi(k)), k
     for k in ("depth", "sigma", "area", "elem_area", "elem_cos", "coriolis", "w_cv", "gradient_sca", "gradient_vec",
-              "edge_dxdy", "edge_cross_dxdy"):
+              "edge_dxdy", "edge_cross_dxdy", "ac"):
         assert np.array_equal(p[k], o.get(k)), k
     atp = o.get("a_tp").reshape(-1, 2)
     assert np.array_equal(p["a_tp"], np.concatenate([atp[:, 0], atp[:, 1]]))
@@ -70,6 +70,30 @@ def test_mesh_build_bitexact_channel_tri_quad_ob():
     from fesomc_amd import meshgen
     txt, mesh, _ = meshgen.config("c1")
     _compare_mesh(txt, mesh)
+
+
+def test_mesh_build_sponge_layer():
+    """SL_obn: ac_create (src/fv_utilit.F90:190-212) gives every node
+    d/SL_radius, d its distance to the nearest open-boundary node, where d is
+    below SL_radius, and 1 elsewhere.  Pinned by brute force in metres, by
+    ac = 0 on the open boundary, and bit for bit between the mesh builder and
+    the oracle."""
+    from fesomc_amd import meshgen
+    txt, mesh, _ = meshgen.config("c2_small")
+    assert "SL_obn = T" in txt
+    _compare_mesh(txt, mesh)
+    from fesomc_amd import gpu
+    from fesomc_amd.runconfig import RunConfig
+    ac = gpu.mesh_arrays(RunConfig(txt), mesh)["ac"]
+    ob = mesh.xy[mesh.index == 2]
+    d = np.sqrt(((mesh.xy[:, None, :] - ob[None, :, :]) ** 2).sum(-1)).min(1)
+    want = np.where(d < meshgen.SPONGE_M, d / meshgen.SPONGE_M, 1.0)
+    np.testing.assert_allclose(ac, want, rtol=1e-12, atol=1e-12)
+    assert np.all(ac[mesh.index == 2] == 0.0)
+    assert 0 < (ac < 1.0).sum() < mesh.nod2D // 2
+    # switched off: ac stays 1
+    off = gpu.mesh_arrays(RunConfig(txt.replace("SL_obn = T", "SL_obn = F")), mesh)["ac"]
+    assert np.all(off == 1.0)
 
 
 def test_mesh_build_rejects_bad_quad():
