@@ -36,6 +36,38 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def field_errors(model, ref, masks=None):
+    """RMS and max differences of SSH, velocity and T/S between two runs, over
+    all nodes/elements, or over the (node, element) masks given."""
+    errs = {}
+    for f, key, where in (("eta_n", "ssh_m", 0), ("U_n", "u_m_s", 1), ("V_n", "v_m_s", 1), ("TF", "t_degC", 0),
+                          ("SF", "s_psu", 0)):
+        a, b = model.get(f), ref.get(f)
+        d = a - b
+        if masks is not None:
+            m = masks[where]
+            d = d.reshape(m.size, -1)[m]
+        errs[key + "_rms"] = float(np.sqrt(np.mean(d ** 2)))
+        errs[key + "_max"] = float(np.max(np.abs(d)))
+    return errs
+
+
+def interior_masks(mesh, txt):
+    """(node, element) masks of the part of the mesh outside the open-boundary
+    sponge layer (ac == 1), elements in the reference's order (triangles
+    first); None when the run has no sponge layer."""
+    from fesomc_amd import gpu
+    from fesomc_amd.runconfig import RunConfig
+    ac = gpu.mesh_arrays(RunConfig(txt), mesh)["ac"]
+    if np.all(ac == 1.0):
+        return None
+    nm = ac == 1.0
+    tri = mesh.elem[:, 0] == mesh.elem[:, 3]
+    order = np.concatenate([np.where(tri)[0], np.where(~tri)[0]])
+    em = nm[mesh.elem[order] - 1].all(axis=1)
+    return nm, em
+
+
 def kernel_bytes(k, N, E, D, L, K, e3=8):
     """Algorithmic bytes of one launch: sum over distinct arrays read or
     written of entities x components x width (index arrays at 4 B; 3-D
@@ -165,10 +197,13 @@ def main():
     from fesomc_amd import autotune, closure, gpu
     from fesomc_amd.workload import build_model
 
+    held = {}
+
     def run(precision, clos, probe):
         t_setup = time.perf_counter()
         scale = tuple(int(v) for v in args.scale.split(",")) if args.scale else None
         txt, mesh, ex, model = build_model(args.config, scale=scale, precision=precision, closure=clos)
+        held["mesh"], held["txt"] = mesh, txt
         if clos == gpu.FCG_CLOSURE_MLP:
             model.attach_mlp(closure.load(args.closure_weights))
         table = args.launch_table
@@ -195,7 +230,7 @@ def main():
             sys.exit(1)
         return model, t1 - t0, k_ms / max(k_n, 1), k_n, setup, diag
 
-    headline = args.path == "headline"
+    headline = args.path in ("headline", "mixed")   # paths compared against the FP64 drop-in
     prec = 0 if args.path == "fp64" else 1
     clos = gpu.FCG_CLOSURE_MLP if args.path in ("headline", "mixed-mlp") else gpu.FCG_CLOSURE_D3END
     model, sec, avg_ms, k_n, setup_s, diag = run(prec, clos, args.probe)
@@ -231,7 +266,7 @@ def main():
                      "avg_ms": avg_ms, "launches": k_n, "algorithmic_bytes": kb},
         "setup_s": setup_s,
         "energy": diag["energy"],
-        "mlp_fallback_columns": diag["mlp_fallback"],
+        "mlp_fallback_column_steps": diag["mlp_fallback"],
     }
     if headline:
         # the plain FP64 drop-in on the same workload: throughput and the error
@@ -239,12 +274,8 @@ def main():
         ref, sec64, _, _, _, _ = run(0, gpu.FCG_CLOSURE_D3END, "")
         out["fp64_drop_in"] = {"value": N * L * args.steps / sec64, "ms_per_step": 1e3 * sec64 / args.steps}
         out["speedup_vs_fp64"] = sec64 / sec
-        errs = {}
-        for f, key in (("eta_n", "ssh_m"), ("U_n", "u_m_s"), ("V_n", "v_m_s"), ("TF", "t_degC"), ("SF", "s_psu")):
-            a, b = model.get(f), ref.get(f)
-            errs[key + "_rms"] = float(np.sqrt(np.mean((a - b) ** 2)))
-            errs[key + "_max"] = float(np.max(np.abs(a - b)))
-        out["error_vs_fp64_after_steps"] = errs
+        interior = interior_masks(held["mesh"], held["txt"])
+        out["error_vs_fp64_after_steps"] = field_errors(model, ref)
         total = args.warmup + args.steps
         target = (int(math.ceil(44712.0 / model.par.dt)) if args.error_steps == "m2" else int(args.error_steps))
         if target > total:
@@ -259,15 +290,12 @@ def main():
                 total += n
                 print(f"error leg: {total}/{target} steps, {time.perf_counter() - t_err:.0f} s", file=sys.stderr,
                       flush=True)
-            errs = {}
-            for f, key in (("eta_n", "ssh_m"), ("U_n", "u_m_s"), ("V_n", "v_m_s"), ("TF", "t_degC"), ("SF", "s_psu")):
-                a, b = model.get(f), ref.get(f)
-                errs[key + "_rms"] = float(np.sqrt(np.mean((a - b) ** 2)))
-                errs[key + "_max"] = float(np.max(np.abs(a - b)))
             d_m, d_r = model.diagnostics(), ref.diagnostics()
-            out["error_vs_fp64_long"] = {"steps": total, "model_time_s": total * model.par.dt, **errs,
+            out["error_vs_fp64_long"] = {"steps": total, "model_time_s": total * model.par.dt,
+                                         **field_errors(model, ref),
+                                         "outside_sponge": field_errors(model, ref, interior) if interior else None,
                                          "status": [d_m["status"], d_r["status"]],
-                                         "mlp_fallback_columns": d_m["mlp_fallback"]}
+                                         "mlp_fallback_column_steps": d_m["mlp_fallback"]}
         ref.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

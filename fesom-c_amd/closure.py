@@ -9,7 +9,10 @@ backward, FP32 accumulation, Adam).  PyTorch only provides device buffers and
 random minibatch indices.
 
     python -c "import __graft_entry__ as g; g.load(); from fesomc_amd import closure; closure.main()" \
-        --config c2 --steps 20 --stride 10 --out closure_c2.npz
+        --config c2 --steps 24 --every 532 --stride 20 --out closure_c2.npz
+
+collects 24 steps spread over one M2 cycle of the 1M-node channel (every
+532nd internal step), every 20th column.
 """
 from __future__ import annotations
 
@@ -22,15 +25,19 @@ import numpy as np
 from . import gpu
 
 
-def collect(model, steps, stride=1, torch=None):
+def collect(model, steps, stride=1, torch=None, every=1, log=None):
     """Advance `model` (plain closure) `steps` internal steps, collecting
-    (features, targets) after every vertical-mixing call.  Returns device tensors."""
+    (features, targets) after every vertical-mixing call; with every > 1 the
+    model advances every-1 further steps before each collected one.  Returns
+    device tensors."""
     import torch as _t
     torch = torch or _t
     cols = (model.N + stride - 1) // stride
     cap = cols * model.K
     Xs, Ts = [], []
-    for _ in range(steps):
+    for i in range(steps):
+        if every > 1:
+            model.step(every - 1)
         for ph in gpu.Model.PHASES:
             model.phase(ph)
             if ph == "fcg_vertical_mixing":
@@ -39,6 +46,8 @@ def collect(model, steps, stride=1, torch=None):
                 n = model.collect(X.data_ptr(), T.data_ptr(), cap, stride)
                 Xs.append(X[:n])
                 Ts.append(T[:n])
+        if log:
+            log(f"collected step {i + 1}/{steps}")
     return torch.cat(Xs), torch.cat(Ts)
 
 
@@ -98,13 +107,17 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--stride", type=int, default=10)
+    ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--iters", type=int, default=3000)
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     from .workload import build_model
     t0 = time.time()
     _, _, _, model = build_model(args.config)
-    X, T = collect(model, args.steps, args.stride)
+    X, T = collect(model, args.steps, args.stride, every=args.every, log=lambda m: print(m, flush=True))
+    d = model.diagnostics()
+    if d["status"] != 0 or not math.isfinite(d["energy"]):
+        raise SystemExit(f"the plain model failed while collecting: {d}")
     model.close()
     print(f"collected {X.shape[0]} samples in {time.time() - t0:.1f} s", flush=True)
     n = X.shape[0]

@@ -2536,7 +2536,9 @@ __global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, 
         col_a[threadIdx.x] = a;
         col_dm[threadIdx.x] = dmean;
         col_f3[threadIdx.x] = log10f((float)(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12));
-        col_ood[threadIdx.x] = 0;
+        // columns of the open-boundary sponge layer (ac < 1), whose damped
+        // dynamics the network is not trained for, keep d3_end
+        col_ood[threadIdx.x] = m.ac[n] < 1.0 ? 1 : 0;
     }
     __syncthreads();
     const int n = f.c0 + f.c, nz = f.k + 1;

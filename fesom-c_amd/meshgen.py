@@ -161,8 +161,8 @@ FV_RUN_TEMPLATE = """! generated by fesomc_amd.meshgen ({name})
  Cd_bz = 10.0
  Bp_bz = 5.0
  cl_relax = T
- clim_relax = 15.0
- relax2clim_ac = 60.0
+ clim_relax = {clim_relax}
+ relax2clim_ac = {relax2clim_ac}
  cl_solid_boder_relax = F
  OB_nudg = F
  comp_sediment = F
@@ -170,14 +170,16 @@ FV_RUN_TEMPLATE = """! generated by fesomc_amd.meshgen ({name})
 
 
 def fv_run(name, dt_2D, dt, nsigma, tracer_adv=1, muscl=False, wetdry=False, Dmin=0.25, Cd=0.0025, nsteps=1000,
-           irep=100, coriolis=True, sponge=0.0, extra=""):
+           irep=100, coriolis=True, sponge=0.0, ob_relax_days=(15.0, 60.0), extra=""):
     """fv_run.dat text; ``sponge`` > 0 turns on the open-boundary sponge layer
-    (SL_obn) of that width in metres."""
+    (SL_obn) of that width in metres; ``ob_relax_days`` are the time scales
+    (clim_relax, relax2clim_ac) of the T/S relaxation at open-boundary nodes."""
     txt = FV_RUN_TEMPLATE.format(name=name, title=name.upper(), dt_2D=dt_2D, dt=dt, nsteps=nsteps, irep=irep,
                                  tracer_adv=tracer_adv, muscl="T" if muscl else "F", nsigma=nsigma,
                                  wetdry="T" if wetdry else "F", Dmin=Dmin, Cd=Cd,
                                  coriolis="T" if coriolis else "F", sl_obn="T" if sponge > 0 else "F",
-                                 sl_radius=sponge if sponge > 0 else 100000.0)
+                                 sl_radius=sponge if sponge > 0 else 100000.0, clim_relax=ob_relax_days[0],
+                                 relax2clim_ac=ob_relax_days[1])
     return txt + extra
 
 
@@ -220,14 +222,19 @@ def stratified_ts(mesh: RawMesh, nsigma, seed=0, T_surf=12.0, T_bot=8.0, S_surf=
     return T.reshape(N, L), S.reshape(N, L)
 
 
-# Width of the open-boundary sponge layer (SL_obn) of the stratified channels.
-# Without it the 3-D velocity of the triangles that touch the open boundary
-# grows without bound within a few hundred internal steps (in the oracle and on
-# the GPU alike): the reference applies no other open-boundary condition to
-# the 3-D momentum, and its answer is this buffer zone, in which ac < 1 raises
-# the bottom drag (Cd_bz) and damps advection, diffusion and the baroclinic
-# pressure gradient (Bp_bz).
-SPONGE_M = 1000.0
+# Open-boundary treatment of the stratified channels.  The reference applies
+# no open-boundary condition to the 3-D momentum and relaxes T/S at
+# open-boundary nodes only; with its defaults (no sponge, 15/60-day T/S
+# relaxation) the 3-D velocity of the elements at the open boundary grows
+# without bound, and T at the nodes next to the open-boundary corners drifts
+# below the -3 degC stop, within a few hundred to ~1500 internal steps, in the
+# oracle and on the GPU alike (1M-node channel: ~350 steps).  A 3 km sponge
+# layer (SL_obn: ac < 1 raises the bottom drag by Cd_bz and damps momentum
+# advection and the baroclinic pressure gradient by ac**Bp_bz) and T/S
+# nudging at open-boundary nodes on a 0.001-day (86 s) time scale carry the
+# channels from 7k to 1M nodes through one M2 cycle.
+SPONGE_M = 3000.0
+OB_RELAX_DAYS = (0.001, 0.001)
 
 
 def config(name, seed=0, scale=None):
@@ -249,7 +256,8 @@ def config(name, seed=0, scale=None):
         dt_2D = min(2.0, round(0.45 * dx / math.sqrt(9.8 * 34.0), 2))
         dt = 10 * dt_2D
         nsig = 24 if name == "c2" else 8
-        txt = fv_run(name, dt_2D, dt, nsig, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.25, sponge=SPONGE_M)
+        txt = fv_run(name, dt_2D, dt, nsig, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.25, sponge=SPONGE_M,
+                     ob_relax_days=OB_RELAX_DAYS)
         T, S = stratified_ts(m, nsig, seed=seed)
         rng = np.random.default_rng(seed + 7)
         spd, ang = rng.uniform(0.0, 10.0), rng.uniform(0.0, 2 * np.pi)
@@ -277,7 +285,8 @@ def ensemble(name="c4", members=None, seed=0):
     nxs = rng.integers(lo, hi + 1, nm)
     dx_min = 20000.0 / (nxs.max() - 1)
     dt_2D = min(2.0, round(0.45 * dx_min / math.sqrt(9.8 * 34.0), 2))
-    txt = fv_run(name, dt_2D, 10 * dt_2D, nsig, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.25, sponge=SPONGE_M)
+    txt = fv_run(name, dt_2D, 10 * dt_2D, nsig, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.25, sponge=SPONGE_M,
+                     ob_relax_days=OB_RELAX_DAYS)
     base = rng.uniform(0.0, 10.0, 2) - 5.0
     meshes, extras = [], []
     for b, nx in enumerate(nxs):

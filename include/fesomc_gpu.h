@@ -237,8 +237,9 @@ int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches);
  * of each (column, level) -- log10 shear^2, asinh(1e5 N^2), log10 depth,
  * log10 bottom stress, sigma -- to log10 snu and log10 Kv (5 -> 64 -> 64 -> 2,
  * tanh, BF16 matrix-core products with FP32 accumulation).  Columns whose
- * normalised features leave the box [x_lo, x_hi] fall back to d3_end; the
- * number of such columns summed over all steps is fcg_diag.mlp_fallback.
+ * normalised features leave the box [x_lo, x_hi], and columns of the
+ * open-boundary sponge layer (ac < 1), fall back to d3_end; the number of
+ * such columns summed over all steps is fcg_diag.mlp_fallback.
  *
  * Training uses device buffers (X: n x 5 normalised features, T: n x 2
  * normalised targets, float, row-major) owned by the caller. */
