@@ -2542,6 +2542,11 @@ __global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, 
     }
     __syncthreads();
     const int n = f.c0 + f.c, nz = f.k + 1;
+    // a block whose columns all lie in the sponge layer skips the network
+    if (__syncthreads_and(!f.on() || col_ood[f.c] != 0)) {
+        if (f.on() && f.k == 0) flag[n] = 1;
+        return;
+    }
     float fx[kMlpF] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (f.on()) {
         const double a = col_a[f.c], dmean = col_dm[f.c];

@@ -22,7 +22,7 @@ program fesomc_run
   integer :: nkeys = 0
 
   character(len=512) :: rundir, arg, meshdir
-  integer :: nsteps, irep, irestart, ini_time, n_dt, i, k, n, nargs, L
+  integer :: nsteps, irep, irestart, ini_time, n_dt, i, k, n, nargs, L, iverbosity
   logical :: mesh_only
   type(fcg_mesh_cfg) :: mc
   type(fcg_params) :: par
@@ -59,6 +59,7 @@ program fesomc_run
   if (nsteps < 0) nsteps = geti('nsteps', 0)
   irep = geti('IREP', 100)
   irestart = geti('IRESTART', huge(0))
+  iverbosity = geti('VERBOSITY', 1)   ! src/fv_read_run.F90:149-155
 
   mc%cartesian = merge(1, 0, getl('cartesian'))
   mc%coriolis_tf = merge(1, 0, getl('Coriolis_TF'))
@@ -186,8 +187,7 @@ program fesomc_run
         call fcg_check(dg%status, ctx, 'solve_tracers')
         if (mod(n_dt, irep) == 0) then
            time = dg%time_jd * 86400.0d0
-           write(53, '(4e16.7)') time / 86400.0d0 - par%time_jd0, dg%energy
-           write(*,*) 'step ', n_dt, ' energy= ', dg%energy, ' max_min_ssh:', dg%eta_max, dg%eta_min
+           call report(time)
         end if
      end if
      if (mod(n_dt, irestart) == 0) call write_restart(n_dt)           ! fv_main.F90:697
@@ -198,6 +198,34 @@ program fesomc_run
   call fcg_mesh_free(mobj)
 
 contains
+
+  ! The control output of the main loop every IREP steps (src/fv_main.F90:
+  ! 607-653, serial branch, type_task 3): the energy.dat record and the stdout
+  ! block.  Extrema of eta_n, TF and SF come from the device reduction of
+  ! fcg_diagnostics, those of U_n_2D from a host copy.  The reference never
+  ! computes the mass-imbalance triple (ib, im_index, ib_mean stay zero) and
+  ! prints icedyn_tmask, an ice variable that stays zero without ice.
+  subroutine report(time)
+    real(c_double), intent(in) :: time
+    real(c_double), allocatable :: u2(:, :)
+    write(*,*) 'Energy ENERGY ENERGY : ', dg%energy
+    write(*,*) 'icedyn_tmask = ', 0.0d0
+    write(53, '(4e16.7)') time / 86400.0d0 - par%time_jd0, dg%energy
+    if (iverbosity >= 1) then
+       write(*,*) 'time= ', time / 86400.0d0 - par%time_jd0, 'time_all= ', time / 86400.0d0 - par%time_jd0
+       write(*,*) ' energy= ', dg%energy
+       write(*,*) 'Mass Conserv. imbalance: max, index_max, mean', 0.0d0, 0, 0.0d0
+       allocate(u2(2, view%elem2D))
+       call get2('U_n_2D', u2)
+       write(*,*) 'max_min_ssh:', dg%eta_max, dg%eta_min
+       write(*,*) 'max_min_vel 2D:', maxval(u2), minval(u2)
+       write(*,*) 'T_maxmin= ', dg%T_max, dg%T_min
+       write(*,*) 'S_maxmin= ', dg%S_max, dg%S_min
+       write(*,*) 'dt', par%dt
+       write(*,*) 'dt_2D', par%dt_2D
+       deallocate(u2)
+    end if
+  end subroutine report
 
   ! ---- restart records (src/fv_restart.F90:474-505): Fortran unformatted
   ! sequential, as write_restart_separate writes them for ver_mix = 3 and

@@ -88,3 +88,34 @@ def test_driver_reproduces_reference_energy_rows(tmp_path):
     for k in range(3):
         assert abs(rows[k, 0] - ref[k, 0]) <= 1e-6 * ref[k, 0]
         assert digits7_ok(rows[k, 1], ref[k, 1]), (k, rows[k, 1], ref[k, 1])
+    _check_control_block(r.stdout, rows)
+
+
+def _block_values(out, label):
+    vals = []
+    for ln in out.splitlines():
+        if label in ln:
+            vals.append([float(v) for v in ln.split(label, 1)[1].split()])
+    return vals
+
+
+def _check_control_block(out, rows):
+    """The stdout block of every IREP-th step (src/fv_main.F90:614-653, serial
+    branch, VERBOSITY = 1): one block per energy.dat row, the energy line equal
+    to the record, and the extrema of the last block equal to those of the
+    FP64 oracle after the same 300 steps (eta_n, U_n_2D, TF, SF)."""
+    from oracle.pyoracle import Oracle
+    from fesomc_amd import meshio
+    from test_oracle_pins import digits7_ok
+    energy = _block_values(out, "Energy ENERGY ENERGY :")
+    assert len(energy) == 3 and len(_block_values(out, "dt_2D")) == 3
+    for k in range(3):
+        assert digits7_ok(energy[k][0], rows[k, 1])
+    txt = open(os.path.join(LE, "fv_run.dat")).read() + "\n Mt = 4\n"
+    o = Oracle(txt, meshio.read_mesh(LE))
+    o.step(300)
+    want = {"max_min_ssh:": o.get("eta_n"), "max_min_vel 2D:": o.get("U_n_2D"), "T_maxmin=": o.get("TF"),
+            "S_maxmin=": o.get("SF")}
+    for label, a in want.items():
+        got = _block_values(out, label)[-1]
+        assert got == pytest.approx([a.max(), a.min()], rel=1e-12, abs=1e-15), label
