@@ -353,6 +353,11 @@ class Model:
     def energy(self):
         return self.diagnostics()["energy"]
 
+    def clock(self):
+        """(n_dt, time_jd) after the last internal step."""
+        d = self.diagnostics()
+        return d["n_dt"], d["time_jd"]
+
     def probe(self, kernel):
         self._ck(lib().fcg_probe(self.ctx, (kernel or "").encode()), "fcg_probe")
 
