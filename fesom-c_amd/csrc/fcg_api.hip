@@ -857,8 +857,51 @@ extern "C" int fcg_sync(fcg_ctx *ctx) {
     return FCG_OK;
 }
 
+// Column quantities of the jacobian (src/fv_main.F90:1228-1252) that the device
+// forms where they are consumed instead of storing them: Jc, Jc_old, zbar and
+// Z, from the stored column depth a = max(depth+eta_n, Dmin) (a_old for Jc_old)
+// and sigma, with the reference's expressions.  Read-only.
+static int derived_get(fcg_ctx *ctx, const char *name, double *host, long count, bool &handled) {
+    const bool jc = !std::strcmp(name, "Jc"), jco = !std::strcmp(name, "Jc_old");
+    const bool zb = !std::strcmp(name, "zbar"), zz = !std::strcmp(name, "Z");
+    handled = jc || jco || zb || zz;
+    if (!handled) return FCG_OK;
+    const long N = ctx->m.N, K = ctx->m.K, L = ctx->m.L;
+    const long lev = zb ? K : L;
+    if (count != lev * N) return fcg_fail(ctx, FCG_E_ARG, std::string("size mismatch for ") + name);
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<double> a(N), sig(K);
+    FCG_CHECK(hipMemcpy(a.data(), jco ? ctx->s.a_old : ctx->s.a, sizeof(double) * N, hipMemcpyDeviceToHost));
+    FCG_CHECK(hipMemcpy(sig.data(), ctx->m.sigma, sizeof(double) * K, hipMemcpyDeviceToHost));
+    std::vector<double> zbar(K);
+    for (long i = 0; i < N; ++i) {
+        double *h = host + (size_t)ctx->pn[i] * lev;
+        const double ai = a[i];
+        if (jc || jco) {
+            for (long k = 0; k < L; ++k) h[k] = ai * (sig[k] - sig[k + 1]);
+        } else {
+            for (long k = 0; k < K; ++k) zbar[k] = (1.0 - sig[k]) * ai;
+            if (zb)
+                for (long k = 0; k < K; ++k) h[k] = zbar[k];
+            else
+                for (long k = 0; k < L; ++k) h[k] = 0.5 * (zbar[k] + zbar[k + 1]);
+        }
+    }
+    return FCG_OK;
+}
+
 static int transfer(fcg_ctx *ctx, const char *name, double *host, long count, bool to_dev) {
     if (!ctx || !name || !host) return FCG_E_ARG;
+    {
+        bool handled = false;
+        if (!to_dev) {
+            const int rc = derived_get(ctx, name, host, count, handled);
+            if (handled) return rc;
+        } else if (!std::strcmp(name, "Jc") || !std::strcmp(name, "Jc_old") || !std::strcmp(name, "zbar")
+                   || !std::strcmp(name, "Z")) {
+            return fcg_fail(ctx, FCG_E_ARG, std::string(name) + " is derived from eta_n and read-only");
+        }
+    }
     Field f;
     if (!find_field(ctx, name, f)) return fcg_fail(ctx, FCG_E_ARG, std::string("unknown field ") + name);
     if (count != f.levels * f.ents)

@@ -315,8 +315,9 @@ class Model:
         node1 = {"eta_n", "eta_n_1", "eta_n_2", "etaAB", "ssh_rhs", "windx", "windy"}
         el1 = {"U_filt_2D", "V_filt_2D", "mask_wd", "taux", "tauy", "C_d_el", "dmean_n"}
         el2 = {"U_n_2D", "U_n_1", "U_n_2", "UAB", "U_rhs_2D_3D", "Bar_pr_2D"}
-        nodeL = {"TF", "SF", "T_old", "S_old", "Tclim", "Sclim", "Unode_p", "Vnode_p"}
-        nodeK = {"hpressure", "Kv", "snu", "bt", "Wvel"}
+        nodeL = {"TF", "SF", "T_old", "S_old", "Tclim", "Sclim", "Unode_p", "Vnode_p", "Unode", "Vnode", "Jc", "Jc_old",
+                 "Z"}
+        nodeK = {"hpressure", "Kv", "snu", "bt", "Wvel", "zbar"}
         if name in node1:
             return N
         if name in el1:

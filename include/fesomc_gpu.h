@@ -173,9 +173,11 @@ int fcg_ensemble_size(const fcg_ctx *ctx);
 /* ---- state transfer (host mirrors <-> device), synchronous -------------
  * name is the reference array name: eta_n, eta_n_1, eta_n_2, U_n_2D, U_n_1,
  * U_n_2, U_n, V_n, U_rhsAB, V_rhsAB, Wvel, bt, snu, Kv, TF, SF, T_old,
- * S_old, Tclim, Sclim, taux, tauy, windx, windy, mask_wd, Jc, Je,
+ * S_old, Tclim, Sclim, taux, tauy, windx, windy, mask_wd, C_d_el, Je,
  * Bar_pru_3D, Bar_prv_3D, Bar_pr_2D, hpressure, U_n_filt, V_n_filt,
- * U_filt_2D, V_filt_2D, U_rhs, V_rhs, ssh_rhs, etaAB, Av, Z, zbar.
+ * U_filt_2D, V_filt_2D, U_rhs, V_rhs, ssh_rhs, etaAB, dmean_n; and, for
+ * fcg_get_state only, Jc, Jc_old, Z, zbar (formed from the column depth and
+ * sigma as the jacobian forms them; the device does not store them).
  * count is the number of doubles the Fortran array holds; the library
  * transposes between Fortran layout and its device layout. */
 int fcg_set_state(fcg_ctx *ctx, const char *name, const double *host, long count);

@@ -13,7 +13,8 @@ from oracle.pyoracle import Oracle
 pytestmark = pytest.mark.gpu
 
 FIELDS = ["eta_n", "eta_n_1", "eta_n_2", "U_n_2D", "U_n", "V_n", "U_n_filt", "V_n_filt", "Wvel", "TF", "SF",
-          "T_old", "S_old", "Kv", "snu", "bt", "U_rhsAB", "V_rhsAB", "Bar_pr_2D", "mask_wd"]
+          "T_old", "S_old", "Kv", "snu", "bt", "U_rhsAB", "V_rhsAB", "Bar_pr_2D", "mask_wd", "etaAB", "C_d_el",
+          "Jc", "Jc_old", "Z", "zbar"]
 RTOL = 1e-12
 
 
