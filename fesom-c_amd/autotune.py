@@ -2,7 +2,8 @@
 
 For each tunable device kernel the candidate block sizes (64, 128, 256
 threads; for the layer-parallel kernels 64 entities x 1, 2 or 4 layers per
-block) are timed on the model's own mesh and state with the library's
+block; 256, 512 or 1024 threads of whole columns for the wide-block kernels,
+whose larger blocks take more neighbours from LDS) are timed on the model's own mesh and state with the library's
 HIP-event probe (fcg_probe), over a few internal steps each, and the fastest
 is kept (fcg_set_launch).  The search advances the model: run it during the
 warm-up of a run, or on a throw-away model of the same mesh.  The resulting
@@ -14,18 +15,31 @@ import json
 
 LAYER_KERNELS = ["k_tr_upwind", "k_tr_gradn_miura", "k_tr_miura", "k_tr_ab"]
 COLUMN_KERNELS = ["k_d3end", "k_jac_node"]
+WIDE_KERNELS = ["k_velrhs"]
 CANDIDATES = (64, 128, 256)
+WIDE_CANDIDATES = (256, 512, 1024)
+
+
+def _cands(k, candidates):
+    return WIDE_CANDIDATES if k in WIDE_KERNELS else candidates
 
 
 def tune(model, kernels=None, candidates=CANDIDATES, steps=2, log=None):
     """Returns {kernel: (block, avg_ms)} and leaves the best shapes set.
     Advances the model by exactly tune_steps(kernels, candidates, steps)
-    internal steps, whatever the timings."""
+    internal steps, whatever the timings (a wide-block candidate whose LDS
+    does not fit is stepped with the default shape)."""
+    from .gpu import GpuError
     table = {}
-    for k in kernels or (LAYER_KERNELS + COLUMN_KERNELS):
+    for k in kernels or (LAYER_KERNELS + COLUMN_KERNELS + WIDE_KERNELS):
         best = None
-        for bs in candidates:
-            model.set_launch(k, bs)
+        for bs in _cands(k, candidates):
+            try:
+                model.set_launch(k, bs)
+            except GpuError:
+                model.set_launch(k, 256)
+                model.step(1 + steps)
+                continue
             model.step(1)                      # first launches of the new shape, untimed
             model.probe(k)
             model.step(steps)
@@ -47,7 +61,8 @@ def tune(model, kernels=None, candidates=CANDIDATES, steps=2, log=None):
 
 
 def tune_steps(kernels=None, candidates=CANDIDATES, steps=2):
-    return len(kernels or (LAYER_KERNELS + COLUMN_KERNELS)) * len(candidates) * (1 + steps)
+    return sum(len(_cands(k, candidates)) for k in kernels or (LAYER_KERNELS + COLUMN_KERNELS + WIDE_KERNELS)) \
+        * (1 + steps)
 
 
 def apply(model, table):

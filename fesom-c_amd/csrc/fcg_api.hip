@@ -84,13 +84,16 @@ constexpr int kMinmaxChunks = 512;  // blocks per ensemble member in the T/S ext
 constexpr int kColB = 64, kColT = 256, kFlatT = 256;
 inline unsigned ncolblk(int ents) { return (unsigned)((ents + kColB - 1) / kColB); }
 inline size_t tile_bytes(const DevMesh &m, size_t esz) { return (size_t)kColB * (m.K + 1) * esz; }
-inline int cpb_of(int S) { return kFlatT / S; }
-inline unsigned nflatblk(int ents, int S) { return (unsigned)((ents + cpb_of(S) - 1) / cpb_of(S)); }
+inline int cpb_of(int S, int T = kFlatT) { return T / S; }
+inline unsigned nflatblk(int ents, int S, int T = kFlatT) {
+    return (unsigned)((ents + cpb_of(S, T) - 1) / cpb_of(S, T));
+}
 inline size_t flat_bytes(int S, size_t esz) { return (size_t)cpb_of(S) * S * esz; }
-// k_velrhs: per column one element record and four edge records
-inline size_t velrhs_bytes(const DevMesh &m, size_t esz) {
-    return (size_t)cpb_of(m.L) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) +
-           (size_t)cpb_of(m.L) * m.L * (3 * sizeof(double) + 6 * esz);
+// k_velrhs: per column one element record and four edge records, for T-thread
+// blocks (wider blocks hold more columns, so more neighbours come from LDS)
+inline size_t velrhs_bytes(const DevMesh &m, size_t esz, int T = kFlatT) {
+    return (size_t)cpb_of(m.L, T) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) +
+           (size_t)cpb_of(m.L, T) * m.L * (3 * sizeof(double) + 6 * esz);
 }
 inline size_t lds_p1adv(const DevMesh &m) {
     return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8 + 4 * (size_t)cpb_of(m.L) * sizeof(P1Rec);
@@ -623,6 +626,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         FCG_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
         const size_t gather = std::max(std::max(lds_muscl<double>(ctx), lds_vdiv<double>(ctx)),
                                        std::max(lds_gradn(ctx), lds_el2n(ctx)));
+        ctx->lds_limit = lds_limit;
         if (m.K > kFlatT || std::max(lds_max(m, ctx->esize), gather) > (size_t)lds_limit) {
             fcg_fail(ctx, FCG_E_ARG, "nsigma too large for the LDS column tiles of this device");
             *out = ctx;
@@ -1040,8 +1044,9 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         ctx->lfirst = 0;
     }
     LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec), ctx->m, s);
-    LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, velrhs_bytes(ctx->m, sizeof(R)), ctx->m, s, p.dt,
-              ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0);
+    const int vt = block_of(ctx, "k_velrhs");
+    LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt), ctx->m, s,
+              p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0);
     if (p.im_vert_visc_fic)
         LAUNCH_EX(ctx, k_velvisc<R>, ncolblk(ctx->m.E), kColT, tile_bytes(ctx->m, 8) + 2 * tile_bytes(ctx->m, sizeof(R)),
                   ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
@@ -1276,9 +1281,18 @@ extern "C" int fcg_mlp_collect(fcg_ctx *ctx, float *X, float *T, long cap, int s
 }
 
 // Launch shapes for the autotuner: block size 64, 128 or 256 threads (for
-// the layer-parallel kernels: 64 entities x 1, 2 or 4 layers per block).
+// the layer-parallel kernels: 64 entities x 1, 2 or 4 layers per block); for
+// k_velrhs 256, 512 or 1024 threads of whole columns, when its LDS fits.
 extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
-    if (!ctx || !kernel || !(block == 64 || block == 128 || block == 256)) return FCG_E_ARG;
+    if (!ctx || !kernel) return FCG_E_ARG;
+    if (!std::strcmp(kernel, "k_velrhs")) {
+        if (!(block == 256 || block == 512 || block == 1024)) return FCG_E_ARG;
+        if (velrhs_bytes(ctx->m, ctx->esize, block) > (size_t)ctx->lds_limit)
+            return fcg_fail(ctx, FCG_E_ARG, "k_velrhs: LDS of this block size exceeds the device limit");
+        ctx->launch_block[kernel] = block;
+        return FCG_OK;
+    }
+    if (!(block == 64 || block == 128 || block == 256)) return FCG_E_ARG;
     static const char *tunable[] = {"k_d3end", "k_jac_node", "k_tr_ab", "k_tr_upwind", "k_tr_gradn_miura",
                                     "k_tr_miura"};
     for (const char *t : tunable)

@@ -197,6 +197,7 @@ struct fcg_ctx {
     int red_blocks = 0;
     // node-gather blocks (cpb = 256/L nodes): most CSR slots of any block
     int cap_n2e = 0, cap_n2el = 0;
+    int lds_limit = 0;           // bytes of LDS one block may request on this device
     // learned closure
     fcg_mlp *mlp = nullptr;      // attached network (not owned)
     int *mlp_flag = nullptr;     // N: column sent back to d3_end this step

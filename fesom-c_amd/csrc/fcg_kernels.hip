@@ -733,7 +733,7 @@ static_assert(sizeof(EdgeRec<double>) == velrhs_edge_bytes(8) && sizeof(EdgeRec<
 // the new AB history and the provisional velocity into U_rhs/V_rhs;
 // k_velvisc then applies the implicit vertical viscosity column by column.
 template <typename R>
-__global__ __launch_bounds__(256) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv) {
+__global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int L = m.L, K = m.K, KC = K;
     const ColFlat f = col_flat(m.E, L);

@@ -272,8 +272,10 @@ int fcg_mlp_collect(fcg_ctx *ctx, float *X_dev, float *T_dev, long cap, int stri
 
 /* Launch shapes (no counterpart in the reference; used by the autotuner,
  * fesomc_amd/autotune.py): block size 64, 128 or 256 threads for the named
- * device kernel ("k_tr_muscl", "k_velrhs", ...; layer-parallel kernels then
- * cover 64 entities x block/64 layers per block).  Default 256. */
+ * device kernel ("k_d3end", "k_tr_upwind", ...; layer-parallel kernels then
+ * cover 64 entities x block/64 layers per block); 256, 512 or 1024 threads of
+ * whole columns for "k_velrhs" (FCG_E_ARG when its LDS would not fit).
+ * Default 256.  Results do not depend on the shape. */
 int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block);
 int fcg_get_launch(const fcg_ctx *ctx, const char *kernel);
 
