@@ -9,7 +9,7 @@ backward, FP32 accumulation, Adam).  PyTorch only provides device buffers and
 random minibatch indices.
 
     python -c "import __graft_entry__ as g; g.load(); from fesomc_amd import closure; closure.main()" \
-        --config c2 --steps 24 --every 532 --stride 20 --out closure_c2.npz
+        --config c2 --steps 24 --every 532 --stride 20 --iters 12000 --out closure_c2.npz
 
 collects 24 steps spread over one M2 cycle of the 1M-node channel (every
 532nd internal step), every 20th column.
@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--stride", type=int, default=10)
     ap.add_argument("--every", type=int, default=1)
-    ap.add_argument("--iters", type=int, default=3000)
+    ap.add_argument("--iters", type=int, default=12000)
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     from .workload import build_model
