@@ -27,7 +27,8 @@ template <typename R> __global__ void k_bt_edge_mom(DevMesh, DevStateT<R>, doubl
 template <typename R> __global__ void k_bt_velgrad(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_puls(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_el2nodes(DevMesh, const R *, const R *, R *, R *);
-template <typename R> __global__ void k_bt_p1adv(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_bt_p1edge(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_bt_p1elem(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int);
 template <typename R> __global__ void k_jac_node(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
@@ -95,8 +96,8 @@ inline size_t velrhs_bytes(const DevMesh &m, size_t esz, int T = kFlatT) {
     return (size_t)cpb_of(m.L, T) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) +
            (size_t)cpb_of(m.L, T) * m.L * (3 * sizeof(double) + 6 * esz);
 }
-inline size_t lds_p1adv(const DevMesh &m) {
-    return 8 * flat_bytes(m.L, 8) + 8 * (size_t)cpb_of(m.L) * 8 + 4 * (size_t)cpb_of(m.L) * sizeof(P1Rec);
+inline size_t lds_p1edge(const DevMesh &m) {
+    return 4 * flat_bytes(m.L, 8) + (size_t)cpb_of(m.L) * sizeof(P1Edge);
 }
 // node-gather blocks: LDS records for every CSR slot of the block's nodes
 template <typename R>
@@ -114,7 +115,7 @@ inline size_t lds_max(const DevMesh &m, size_t esz) {
     size_t b = std::max(2 * tile_bytes(m, 8), tile_bytes(m, 8) + 2 * tile_bytes(m, esz));  // bpg, velvisc
     b = std::max(b, 5 * tile_bytes(m, esz));                // vdiff
     b = std::max(b, 5 * flat_bytes(m.K, esz));              // d3_end prep
-    return std::max(b, lds_p1adv(m));
+    return std::max(b, lds_p1edge(m));
 }
 
 template <typename T>
@@ -281,7 +282,8 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
                                st, m, s, p.Dmin);
             hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nflatblk(m.N, m.L)), dim3(kFlatT), lds_el2n(ctx), st, m,
                                (const R *)s.U_puls, (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
-            hipLaunchKernelGGL(fcgk::k_bt_p1adv<R>, dim3(nflatblk(m.E, m.L)), dim3(kFlatT), lds_p1adv(m), st, m, s);
+            hipLaunchKernelGGL(fcgk::k_bt_p1edge<R>, dim3(nflatblk(m.D, m.L)), dim3(kFlatT), lds_p1edge(m), st, m, s);
+            hipLaunchKernelGGL(fcgk::k_bt_p1elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         }
         hipLaunchKernelGGL(fcgk::k_bt_elem2<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
                            ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0);
@@ -637,7 +639,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
     DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
-    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.fl_p1, 4 * (size_t)D) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
     DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
     DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
     DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA3(s.U_puls, EL) DA3(s.V_puls, EL)

@@ -69,8 +69,8 @@ struct VelGradRec {
     int jend, nbK[4];            // neighbour column offsets, < 0: free-slip boundary side
     double gx[4], gy[4], zc0[4], zc1[4];
 };
-struct P1Rec {
-    int n1K, n2K;                // column offsets of the edge's nodes; n1K < 0: no edge in this slot
+struct P1Edge {
+    int n1K, n2K, e1K, e2K;      // column offsets of the edge's nodes and elements
     double xe, ye, an1, an2;
 };
 
@@ -129,6 +129,7 @@ struct DevStateT {
     double *a = nullptr, *a_old = nullptr;                     // N: max(depth+eta_n, Dmin) now / previous jacobian
     double *ubot = nullptr;                                    // 2E: U_n, V_n of the bottom layer (substep cache)
     double *fl_ssh = nullptr, *fl_uv = nullptr;                // D, 2D: per-edge transport and 2-D momentum flux
+    double *fl_p1 = nullptr;                                   // 4D: layer sums of the P1 3-D-to-2-D advection
     double *node2d = nullptr;                                  // 4N: etaAB, eta_g, eta_n_1, eta_n per node (substep)
     double *tide = nullptr;                                    // Mt*nobn prescribed OB elevation per substep
     // 3-D, stored by column: K slots per node/element (layer fields use L)

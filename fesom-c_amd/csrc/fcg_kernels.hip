@@ -1217,96 +1217,101 @@ __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict
     PL(Vn, k, n, KC) = sv * tvi;
 }
 
-// momentum_adv_P1_3D_to_2D (fv_advection.F90:676-793) as an element gather,
-// then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513).  One thread per
-// (element, layer) forms the layer terms uu*un*Je of the element's (up to 4)
-// internal edges; one thread per (element, edge) sums them over the layers
-// in order, and one per element combines the edges in ascending id.
+// momentum_adv_P1_3D_to_2D (fv_advection.F90:676-793), edge pass: one thread
+// per (edge, layer) forms uu*un*Je for both elements of an internal edge at
+// its layer (each element weighs the edge term with its own Je); one thread
+// per (edge, element side, component) then sums them over the layers in order
+// into fl_p1[4*ed + 2*side + comp] (side 0: edge_tri(1), side 1: edge_tri(2)).
+// Device edge ids follow the locality order, so internal and boundary edges
+// are interleaved: a boundary edge (one element) gets zeros and is never read.
 template <typename R>
-__global__ __launch_bounds__(256) void k_bt_p1adv(DevMesh m, DevStateT<R> s) {
+__global__ __launch_bounds__(256) void k_bt_p1edge(DevMesh m, DevStateT<R> s) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int L = m.L, KC = m.K;
-    const ColFlat f = col_flat(m.E, L);
+    const ColFlat f = col_flat(m.D, L);
     const int T = f.cpb * L;
-    double *tu = (double *)lds_, *tv = tu + 4 * T;       // [edge slot][column*L + layer]
-    double *su_ = tv + 4 * T, *sv_ = su_ + 4 * f.cpb;    // [column*4 + edge slot]
+    double *t = (double *)lds_;             // [side*2 + comp][column*L + layer]
+    P1Edge *rec = (P1Edge *)(t + 4 * T);    // [column]
     if (f.nc <= 0) return;
-    P1Rec *rec = (P1Rec *)(sv_ + 4 * f.cpb);             // [column*4 + edge slot]
-    for (int t = threadIdx.x; t < 4 * f.nc; t += blockDim.x) {
-        const int c = t >> 2, j = t & 3;
-        const int4 ied = m.el_ied[f.c0 + c];
-        const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
-        P1Rec r;
-        r.n1K = -1;
-        if (cc != kEmpty) {
-            const int ed = cc >= 0 ? cc : ~cc;
-            const int2 nd = m.ednodes[ed];
-            const int2 et = m.edtri[ed];
-            const double2 dd = m.edxdy[ed];
-            r.n1K = nd.x * KC;
-            r.n2K = nd.y * KC;
+    if ((int)threadIdx.x < f.nc) {
+        const int ed = f.c0 + threadIdx.x;
+        const int2 nd = m.ednodes[ed];
+        const int2 et = m.edtri[ed];
+        const double2 dd = m.edxdy[ed];
+        P1Edge r;
+        r.n1K = nd.x * KC;
+        r.n2K = nd.y * KC;
+        r.e1K = -1;
+        r.e2K = -1;
+        r.xe = r.ye = r.an1 = r.an2 = 0.0;
+        if (et.x >= 0 && et.y >= 0) {  // internal edge
+            r.e1K = et.x * KC;
+            r.e2K = et.y * KC;
             r.xe = dd.x * kREarthD * 0.5 * (m.elem_cos[et.x] + m.elem_cos[et.y]);
             r.ye = dd.y * kREarthD;
             r.an1 = s.a[nd.x];
             r.an2 = s.a[nd.y];
         }
-        rec[t] = r;
+        rec[threadIdx.x] = r;
     }
     __syncthreads();
-    if (f.on()) {
-        const int c = f.c, e = f.c0 + c, k = f.k;
-        const double je = PL(s.Je, k, e, KC);
+    if (f.on() && rec[f.c].e1K < 0) {
+        const int i = f.c * L + f.k;
+        t[i] = t[T + i] = t[2 * T + i] = t[3 * T + i] = 0.0;
+    } else if (f.on()) {
+        const P1Edge &r = rec[f.c];
+        const int k = f.k;
         const double ds = m.dsig[k];
-        for (int j = 0; j < 4; ++j) {
-            const P1Rec &r = rec[c * 4 + j];
-            if (r.n1K < 0) break;
-            double uu = 0.5 * (s.Unode_p[r.n1K + k] + s.Unode_p[r.n2K + k]);
-            double vv = 0.5 * (s.Vnode_p[r.n1K + k] + s.Vnode_p[r.n2K + k]);
-            double jd = 0.5 * (r.an1 * ds + r.an2 * ds);
-            double un = (uu * r.ye - vv * r.xe) * jd;
-            tu[j * T + c * L + k] = uu * un * je;
-            tv[j * T + c * L + k] = vv * un * je;
-        }
+        double uu = 0.5 * (s.Unode_p[r.n1K + k] + s.Unode_p[r.n2K + k]);
+        double vv = 0.5 * (s.Vnode_p[r.n1K + k] + s.Vnode_p[r.n2K + k]);
+        double jd = 0.5 * (r.an1 * ds + r.an2 * ds);
+        double un = (uu * r.ye - vv * r.xe) * jd;
+        const double pu = uu * un, pv = vv * un;
+        const double je1 = s.Je[r.e1K + k], je2 = s.Je[r.e2K + k];
+        const int i = f.c * L + k;
+        t[i] = pu * je1;
+        t[T + i] = pv * je1;
+        t[2 * T + i] = pu * je2;
+        t[3 * T + i] = pv * je2;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < 4 * f.nc; t += blockDim.x) {
-        const int c = t >> 2, j = t & 3;
-        const int4 ied = m.el_ied[f.c0 + c];
-        const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
-        double su = 0.0, sv = 0.0;
-        if (cc != kEmpty) {
-            const double *pu = tu + j * T + c * L, *pv = tv + j * T + c * L;
-            for (int k = 0; k < L; ++k) {
-                su = su + pu[k];
-                sv = sv + pv[k];
-            }
-        }
-        su_[c * 4 + j] = su;
-        sv_[c * 4 + j] = sv;
+    for (int q = threadIdx.x; q < 4 * f.nc; q += blockDim.x) {
+        const int c = q >> 2, j = q & 3;
+        const double *p = t + j * T + c * L;
+        double sum = 0.0;
+        for (int k = 0; k < L; ++k) sum = sum + p[k];
+        s.fl_p1[4 * (size_t)(f.c0 + c) + j] = sum;
     }
-    __syncthreads();
-    if ((int)threadIdx.x < f.nc) {
-        const int c = threadIdx.x, e = f.c0 + c;
-        const int4 ied = m.el_ied[e];
-        const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
-        double ru = 0.0, rv = 0.0;
-        for (int j = 0; j < 4; ++j) {
-            const int cc = slots[j];
-            if (cc == kEmpty) break;
-            const int2 nd = m.ednodes[cc >= 0 ? cc : ~cc];
-            const double acc = (m.ac[nd.x] + m.ac[nd.y]) * .5;
-            const double su = su_[c * 4 + j], sv = sv_[c * 4 + j];
-            if (cc >= 0) {  // element is edge_tri(2)
-                ru = ru + acc * su;
-                rv = rv + acc * sv;
-            } else {
-                ru = ru - acc * su;
-                rv = rv - acc * sv;
-            }
+}
+
+// element pass: the element's internal edges in ascending id, each term with
+// the edge's ac weight, then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513)
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_p1elem(DevMesh m, DevStateT<R> s) {
+    const int e = TID();
+    if (e >= m.E) return;
+    const int4 ied = m.el_ied[e];
+    const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+    const double2 *fl = (const double2 *)s.fl_p1;
+    double ru = 0.0, rv = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        const int cc = slots[j];
+        if (cc == kEmpty) break;
+        const int ed = cc >= 0 ? cc : ~cc;
+        const int2 nd = m.ednodes[ed];
+        const double acc = (m.ac[nd.x] + m.ac[nd.y]) * .5;
+        if (cc >= 0) {  // element is edge_tri(2)
+            const double2 v = fl[2 * (size_t)ed + 1];
+            ru = ru + acc * v.x;
+            rv = rv + acc * v.y;
+        } else {
+            const double2 v = fl[2 * (size_t)ed];
+            ru = ru - acc * v.x;
+            rv = rv - acc * v.y;
         }
-        s.U_rhs_2D_3D[2 * e] = ru / s.dmean_n[e];
-        s.U_rhs_2D_3D[2 * e + 1] = rv / s.dmean_n[e];
     }
+    s.U_rhs_2D_3D[2 * e] = ru / s.dmean_n[e];
+    s.U_rhs_2D_3D[2 * e + 1] = rv / s.dmean_n[e];
 }
 
 // momentum_adv_upwind_2D edge flux: fv_advection.F90:589-643
@@ -2708,7 +2713,8 @@ void instantiate_kernels() {
     (void)&k_bt_velgrad<R>;
     (void)&k_bt_puls<R>;
     (void)&k_el2nodes<R>;
-    (void)&k_bt_p1adv<R>;
+    (void)&k_bt_p1edge<R>;
+    (void)&k_bt_p1elem<R>;
     (void)&k_bt_elem2<R>;
     (void)&k_jac_node<R>;
     (void)&k_jac_restart<R>;
