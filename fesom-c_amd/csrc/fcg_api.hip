@@ -26,7 +26,8 @@ template <typename R> __global__ void k_bt_edge_ssh(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_edge_mom(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_velgrad(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_puls(DevMesh, DevStateT<R>, double);
-template <typename R> __global__ void k_el2nodes(DevMesh, const R *, const R *, R *, R *);
+template <typename R, bool P>
+__global__ void k_el2nodes(DevMesh, const R *, const R *, R *, R *, const double *, const double *);
 template <typename R> __global__ void k_bt_p1edge(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_p1elem(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int);
@@ -108,7 +109,7 @@ inline size_t lds_muscl(const fcg_ctx *c) {
 template <typename R>
 inline size_t lds_vdiv(const fcg_ctx *c) { return (size_t)c->cap_n2e * sizeof(DivRec<R>); }
 inline size_t lds_gradn(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(GradRec); }
-inline size_t lds_el2n(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(AreaRec); }
+inline size_t lds_el2n(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(PulsRec); }
 
 // largest LDS request of the tile kernels for this mesh and storage type
 inline size_t lds_max(const DevMesh &m, size_t esz) {
@@ -278,10 +279,10 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
         if (p.mom_adv_2D == 2)
             hipLaunchKernelGGL(fcgk::k_bt_edge_mom<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, p.Dmin);
         if (sub == 1) {
-            hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nflatblk(m.E, m.L)), dim3(kFlatT), 3 * cpb_of(m.L) * sizeof(double),
-                               st, m, s, p.Dmin);
-            hipLaunchKernelGGL(fcgk::k_el2nodes<R>, dim3(nflatblk(m.N, m.L)), dim3(kFlatT), lds_el2n(ctx), st, m,
-                               (const R *)s.U_puls, (const R *)s.V_puls, s.Unode_p, s.Vnode_p);
+            hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
+            hipLaunchKernelGGL((fcgk::k_el2nodes<R, true>), dim3(nflatblk(m.N, m.L)), dim3(kFlatT), lds_el2n(ctx), st,
+                               m, (const R *)s.U_n, (const R *)s.V_n, s.Unode_p, s.Vnode_p, (const double *)s.puls_ap,
+                               (const double *)s.U2);
             hipLaunchKernelGGL(fcgk::k_bt_p1edge<R>, dim3(nflatblk(m.D, m.L)), dim3(kFlatT), lds_p1edge(m), st, m, s);
             hipLaunchKernelGGL(fcgk::k_bt_p1elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         }
@@ -642,7 +643,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.fl_p1, 4 * (size_t)D) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
     DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
     DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
-    DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA3(s.U_puls, EL) DA3(s.V_puls, EL)
+    DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA(s.puls_ap, E)
     DA3(s.vxx, EK) DA3(s.vyu, EK) DA3(s.vyv, EK)
     DA3(s.TF, NL) DA3(s.SF, NL) DA3(s.T_old, NL) DA3(s.S_old, NL) DA3(s.Tclim, NL) DA3(s.Sclim, NL)
     DA3(s.abT, NL) DA3(s.abS, NL) DA3(s.Unode_p, NL) DA3(s.Vnode_p, NL) DA3(s.Unode, NL) DA3(s.Vnode, NL) DA3(s.tcp, NL) DA3(s.ttp, NL) DA3(s.tsp, NL)
@@ -1013,8 +1014,8 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
     // compute_el2nodes_3D(U_n, V_n, Unode, Vnode), src/fv_main.F90:2006
-    LAUNCH_EX(ctx, k_el2nodes<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, lds_el2n(ctx), ctx->m, (const R *)s.U_n,
-              (const R *)s.V_n, s.Unode, s.Vnode);
+    hipLaunchKernelGGL((fcgk::k_el2nodes<R, false>), dim3(nflatblk(ctx->m.N, ctx->m.L)), dim3(kFlatT), lds_el2n(ctx),
+                       ctx->stream, ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode, nullptr, nullptr);
     if (p.closure == FCG_CLOSURE_MLP) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");

@@ -57,6 +57,10 @@ struct AreaRec {
     int elK;
     double ar;
 };
+struct PulsRec {                 // AreaRec + the element's pulsation-velocity operands
+    int elK;
+    double ar, ap, u2, v2;
+};
 template <typename R>
 struct DivRec {
     int e1K, e2K, pos;    // pos: node is edge_nodes(1)
@@ -130,6 +134,7 @@ struct DevStateT {
     double *ubot = nullptr;                                    // 2E: U_n, V_n of the bottom layer (substep cache)
     double *fl_ssh = nullptr, *fl_uv = nullptr;                // D, 2D: per-edge transport and 2-D momentum flux
     double *fl_p1 = nullptr;                                   // 4D: layer sums of the P1 3-D-to-2-D advection
+    double *puls_ap = nullptr;                                 // E: a of U_puls = U_n - a*U_n_2D (first substep)
     double *node2d = nullptr;                                  // 4N: etaAB, eta_g, eta_n_1, eta_n per node (substep)
     double *tide = nullptr;                                    // Mt*nobn prescribed OB elevation per substep
     // 3-D, stored by column: K slots per node/element (layer fields use L)
@@ -137,7 +142,7 @@ struct DevStateT {
     R *U_rhsAB = nullptr, *V_rhsAB = nullptr, *U_nf = nullptr, *V_nf = nullptr; // L*E
     R *gux = nullptr, *guy = nullptr, *gvx = nullptr, *gvy = nullptr;         // L*E
     R *Bpu = nullptr, *Bpv = nullptr, *Je = nullptr;                          // L*E
-    R *U_puls = nullptr, *V_puls = nullptr;                                   // L*E
+
     R *vxx = nullptr, *vyu = nullptr, *vyv = nullptr;                         // K*E Thomas scratch
     R *TF = nullptr, *SF = nullptr, *T_old = nullptr, *S_old = nullptr;       // L*N
     R *Tclim = nullptr, *Sclim = nullptr, *abT = nullptr, *abS = nullptr;     // L*N

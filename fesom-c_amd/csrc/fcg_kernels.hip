@@ -1145,59 +1145,55 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
 }
 
 // first substep only: dmean_n (fv_main.F90:1467-1473) and compute_puls_vel
-// (fv_3D_vel.F90:257-307), one thread per element and layer.  After
-// k_bt_node's shift eta_n holds eta^{n+1} (= ssh_rhs of the reference) and
-// eta_n_1 holds eta^n.  Also caches the bottom-layer velocity U_n(nsigma-1),
-// which update_2D_vel reads in every substep, as a 2-D field.
+// (fv_3D_vel.F90:257-307), per element: the factor a = dmean_n/D^{n+1} of
+// U_puls = U_n - a*U_n_2D.  The pulsation velocity itself is formed where it
+// is consumed, in compute_el2nodes_3D (k_el2nodes with puls = 1): on the
+// supported path nothing else reads it.  After k_bt_node's shift eta_n holds
+// eta^{n+1} (= ssh_rhs of the reference) and eta_n_1 holds eta^n.  Also caches
+// the bottom-layer velocity U_n(nsigma-1), which update_2D_vel reads in every
+// substep, as a 2-D field.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevStateT<R> s, double Dmin) {
-    extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
-    const ColFlat f = col_flat(m.E, m.L);
-    double *cs = (double *)lds_;  // per column: ap, U2, V2
-    if (f.nc <= 0) return;
-    if ((int)threadIdx.x < f.nc) {
-        const int e = f.c0 + threadIdx.x;
-        const int4 q = m.en[e];
-        const double4 w = m.wcv[e];
-        const double dm = dmaxr(Dmin, w.x * (s.eta_n_1[q.x] + m.depth[q.x]) + w.y * (s.eta_n_1[q.y] + m.depth[q.y])
-                                      + w.z * (s.eta_n_1[q.z] + m.depth[q.z]) + w.w * (s.eta_n_1[q.w] + m.depth[q.w]));
-        s.dmean_n[e] = dm;
-        const double snew = sum4(w, s.eta_n[q.x] + m.depth[q.x], s.eta_n[q.y] + m.depth[q.y],
-                                 s.eta_n[q.z] + m.depth[q.z], s.eta_n[q.w] + m.depth[q.w]);
-        cs[3 * threadIdx.x] = dm / dmaxr(Dmin, snew);
-        cs[3 * threadIdx.x + 1] = s.U2[2 * e];
-        cs[3 * threadIdx.x + 2] = s.U2[2 * e + 1];
-    }
-    __syncthreads();
-    if (!f.on()) return;
-    const int e = f.c0 + f.c, k = f.k;
-    const double ap = cs[3 * f.c], u2 = cs[3 * f.c + 1], v2 = cs[3 * f.c + 2];
-    const double un = PL(s.U_n, k, e, KC), vn = PL(s.V_n, k, e, KC);
-    PL(s.U_puls, k, e, KC) = un - ap * u2;
-    PL(s.V_puls, k, e, KC) = vn - ap * v2;
-    if (k == m.L - 1) {
-        s.ubot[2 * e] = un;
-        s.ubot[2 * e + 1] = vn;
-    }
+    const int e = TID();
+    if (e >= m.E) return;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e];
+    const double dm = dmaxr(Dmin, w.x * (s.eta_n_1[q.x] + m.depth[q.x]) + w.y * (s.eta_n_1[q.y] + m.depth[q.y])
+                                  + w.z * (s.eta_n_1[q.z] + m.depth[q.z]) + w.w * (s.eta_n_1[q.w] + m.depth[q.w]));
+    s.dmean_n[e] = dm;
+    const double snew = sum4(w, s.eta_n[q.x] + m.depth[q.x], s.eta_n[q.y] + m.depth[q.y],
+                             s.eta_n[q.z] + m.depth[q.z], s.eta_n[q.w] + m.depth[q.w]);
+    s.puls_ap[e] = dm / dmaxr(Dmin, snew);
+    s.ubot[2 * e] = PL(s.U_n, m.L - 1, e, KC);
+    s.ubot[2 * e + 1] = PL(s.V_n, m.L - 1, e, KC);
 }
 
 // compute_el2nodes_3D (src/fv_utilit.F90:669-689), one thread per node and
 // layer on node-gather blocks: area-weighted mean over the node's elements
-// in ascending id.
-template <typename R>
+// in ascending id.  P: the element values are the pulsation velocity
+// U_puls = U_n - a*U_n_2D (compute_puls_vel, fv_3D_vel.F90:286-295), formed
+// here in the storage type from U_n, a (k_bt_puls) and U_n_2D.
+template <typename R, bool P>
 __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict__ Ue, const R *__restrict__ Ve,
-                           R *__restrict__ Un, R *__restrict__ Vn) {
+                           R *__restrict__ Un, R *__restrict__ Vn, const double *__restrict__ ap,
+                           const double *__restrict__ U2) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
     const ColFlat f = col_flat(m.N, m.L);
     if (f.nc <= 0) return;
     const int2 jr = csr_block(m.n2el_ptr, f);
-    AreaRec *rec = (AreaRec *)lds_;
+    PulsRec *rec = (PulsRec *)lds_;
     for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
-        AreaRec r;
+        PulsRec r;
+        const int e = m.n2el[jr.x + t];
         r.elK = m.n2elK[jr.x + t];
-        r.ar = m.elem_area[m.n2el[jr.x + t]];
+        r.ar = m.elem_area[e];
+        if (P) {
+            r.ap = ap[e];
+            r.u2 = U2[2 * e];
+            r.v2 = U2[2 * e + 1];
+        }
         rec[t] = r;
     }
     __syncthreads();
@@ -1210,8 +1206,14 @@ __global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict
     double su = 0.0, sv = 0.0;
     for (int j = p0; j < p1; ++j) {
         const double ar = rec[j].ar;
-        su = su + Ue[rec[j].elK + k] * ar;
-        sv = sv + Ve[rec[j].elK + k] * ar;
+        R ue = Ue[rec[j].elK + k], ve = Ve[rec[j].elK + k];
+        if (P) {
+            const double un = ue, vn = ve;
+            ue = (R)(un - rec[j].ap * rec[j].u2);
+            ve = (R)(vn - rec[j].ap * rec[j].v2);
+        }
+        su = su + ue * ar;
+        sv = sv + ve * ar;
     }
     PL(Un, k, n, KC) = su * tvi;
     PL(Vn, k, n, KC) = sv * tvi;
@@ -2712,7 +2714,8 @@ void instantiate_kernels() {
     (void)&k_bt_edge_mom<R>;
     (void)&k_bt_velgrad<R>;
     (void)&k_bt_puls<R>;
-    (void)&k_el2nodes<R>;
+    (void)&k_el2nodes<R, false>;
+    (void)&k_el2nodes<R, true>;
     (void)&k_bt_p1edge<R>;
     (void)&k_bt_p1elem<R>;
     (void)&k_bt_elem2<R>;
