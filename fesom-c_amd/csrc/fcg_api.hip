@@ -18,10 +18,10 @@ template <typename R> __global__ void k_d3end_prep(DevMesh, DevStateT<R>, double
 template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, const int *);
 template <typename R> __global__ void k_d3end_fin(DevMesh, DevStateT<R>, double, double, double, double, const int *);
 template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
-template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int);
+template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int, int, double);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
-template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *);
+template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *, int, double);
 template <typename R> __global__ void k_bt_edge_ssh(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_edge_mom(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_velgrad(DevMesh, DevStateT<R>);
@@ -45,7 +45,7 @@ template <typename R> __global__ void k_tr_gradn_muscl(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_muscl(DevMesh, DevStateT<R>, double, double, int, double, double);
 template <typename R> __global__ void k_tr_gradn_miura(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_miura(DevMesh, DevStateT<R>, double, double, double, int, double, double);
-template <typename R> __global__ void k_tr_vdiff(DevMesh, DevStateT<R>, double);
+template <typename R> __global__ void k_tr_vdiff(DevMesh, DevStateT<R>, double, AtmPar);
 template <typename R> __global__ void k_diag_partial(DevMesh, DevStateT<R>, double, int, int, int, int, double *);
 __global__ void k_diag_final(const double *, int, double *);
 __global__ void k_mask_count(const double *, int, int *);
@@ -234,10 +234,14 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
         {"Unode", s.Unode, L, true, false},    {"Vnode", s.Vnode, L, true, false},
         {"hpressure", s.hp, K, true, false},   {"Kv", s.Kv, K, true, false},          {"snu", s.snu, K, true, false},
         {"bt", s.bt, K, true, false},          {"Wvel", s.W, K, true, false},
+        {"qsr", s.qsr, 1, true, false},        {"qns", s.qns, 1, true, false},        {"emp", s.emp, 1, true, false},
+        {"mslp", s.mslp, 1, true, false},      {"rho_c", s.rho_c, L, true, false},
     };
     for (const Ent &t : tab)
         if (!std::strcmp(t.n, name)) {
-            const bool r3 = !t.il && (t.lev == L || t.lev == K) && std::strcmp(t.n, "hpressure") != 0;
+            if (!t.p) return false;  // not held in this configuration (rho_c without key_atm)
+            const bool r3 = !t.il && (t.lev == L || t.lev == K) && std::strcmp(t.n, "hpressure") != 0
+                            && std::strcmp(t.n, "rho_c") != 0;
             f = Field{t.p, t.lev, t.nodal ? N : E, t.il, t.nodal, r3};
             return true;
         }
@@ -274,7 +278,8 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
         const double *tide = (p.tf_presence && m.nobn) ? s.tide + (size_t)(sub - 1) * m.nobn : nullptr;
         hipLaunchKernelGGL(fcgk::k_bt_elem1<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
         hipLaunchKernelGGL(fcgk::k_bt_edge_ssh<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s);
-        hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide);
+        hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide, p.key_atm,
+                           1.0 / ctx->density_0);
         hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         if (p.mom_adv_2D == 2)
             hipLaunchKernelGGL(fcgk::k_bt_edge_mom<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, p.Dmin);
@@ -379,6 +384,11 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         !(p.precision == FCG_FP64 || p.precision == FCG_MIXED) ||
         !(p.closure == FCG_CLOSURE_D3END || p.closure == FCG_CLOSURE_MLP)) {
         fcg_fail(ctx, FCG_E_ARG, "option not available in this build (tracer_adv/precision/closure)");
+        *out = ctx;
+        return FCG_E_ARG;
+    }
+    if (p.key_atm && !(p.type_swr_body == 1 || p.type_swr_body == 2)) {  // src/fv_read_run.F90:799-802
+        fcg_fail(ctx, FCG_E_ARG, "type_swr_body should be 1 or 2");
         *out = ctx;
         return FCG_E_ARG;
     }
@@ -640,7 +650,10 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
     DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
-    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.fl_p1, 4 * (size_t)D) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.fl_p1, 4 * (size_t)D) DA(s.qsr, N) DA(s.qns, N) DA(s.emp, N) DA(s.mslp, N) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    if (p.key_atm) {
+        DA(s.rho_c, (size_t)N * L)
+    }
     DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
     DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
     DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA(s.puls_ap, E)
@@ -1049,7 +1062,7 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec), ctx->m, s);
     const int vt = block_of(ctx, "k_velrhs");
     LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt), ctx->m, s,
-              p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0);
+              p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, p.key_atm, 1.0 / ctx->density_0);
     if (p.im_vert_visc_fic)
         LAUNCH_EX(ctx, k_velvisc<R>, ncolblk(ctx->m.E), kColT, tile_bytes(ctx->m, 8) + 2 * tile_bytes(ctx->m, sizeof(R)),
                   ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
@@ -1173,7 +1186,16 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCHLV(ctx, k_tr_gradn_miura<R>, m.N, m.L, m, s);
         LAUNCHLV(ctx, k_tr_miura<R>, m.N, m.L, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
     }
-    LAUNCH_EX(ctx, k_tr_vdiff<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt);
+    AtmPar ap;
+    ap.on = p.key_atm;
+    ap.type_swr = p.type_swr_body;
+    ap.wet_dry = p.wet_dry_on;
+    ap.Rjc = p.Rjc;
+    ap.ajc = p.ajc;
+    ap.bjc = p.bjc;
+    ap.rho0 = ctx->density_0;
+    ap.Dmin = p.Dmin;
+    LAUNCH_EX(ctx, k_tr_vdiff<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt, ap);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }

@@ -24,6 +24,7 @@
     } while (0)
 
 constexpr double kG = 9.8;
+constexpr double kPref = 101325.0;    // reference atmospheric pressure P_ref (src/fv_var.F90:17)
 constexpr double kREarthD = 6400000.0;
 constexpr double kBeta = 0.281105;    // AB3 (src/fv_var.F90:108)
 constexpr double kEps = 0.01;         // AB2 offset (src/fv_var.F90:110)
@@ -56,6 +57,11 @@ struct GradRec {
 struct AreaRec {
     int elK;
     double ar;
+};
+// surface forcing of the tracer vertical diffusion (key_atm)
+struct AtmPar {
+    int on, type_swr, wet_dry;
+    double Rjc, ajc, bjc, rho0, Dmin;
 };
 struct PulsRec {                 // AreaRec + the element's pulsation-velocity operands
     int elK;
@@ -135,6 +141,8 @@ struct DevStateT {
     double *fl_ssh = nullptr, *fl_uv = nullptr;                // D, 2D: per-edge transport and 2-D momentum flux
     double *fl_p1 = nullptr;                                   // 4D: layer sums of the P1 3-D-to-2-D advection
     double *puls_ap = nullptr;                                 // E: a of U_puls = U_n - a*U_n_2D (first substep)
+    double *qsr = nullptr, *qns = nullptr, *emp = nullptr, *mslp = nullptr;  // N: surface forcing (key_atm)
+    double *rho_c = nullptr;                                   // L*N (n*L + k): density anomaly of pressure (key_atm)
     double *node2d = nullptr;                                  // 4N: etaAB, eta_g, eta_n_1, eta_n per node (substep)
     double *tide = nullptr;                                    // Mt*nobn prescribed OB elevation per substep
     // 3-D, stored by column: K slots per node/element (layer fields use L)

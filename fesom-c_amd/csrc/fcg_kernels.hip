@@ -208,6 +208,7 @@ __global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, dou
         const double a = s.a[n];
         const double Z = 0.5 * (m.oms[k] * a + m.oms[k + 1] * a);
         const double r = densityJM(PL(s.TF, k, n, KC), PL(s.SF, k, n, KC), -Z, rho0);
+        if (s.rho_c) s.rho_c[(size_t)n * L + k] = r;  // rho_c, read by the surface heat source
         th[f.c * L + k] = r * (a * m.dsig[k]) * gd0;
     }
     __syncthreads();
@@ -733,7 +734,8 @@ static_assert(sizeof(EdgeRec<double>) == velrhs_edge_bytes(8) && sizeof(EdgeRec<
 // the new AB history and the provisional velocity into U_rhs/V_rhs;
 // k_velvisc then applies the implicit vertical viscosity column by column.
 template <typename R>
-__global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv) {
+__global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
+                                                  int atm, double d0i) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int L = m.L, K = m.K, KC = K;
     const ColFlat f = col_flat(m.E, L);
@@ -754,10 +756,19 @@ __global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, doub
         r.area = m.elem_area[e];
         r.cor = m.coriolis[e];
         r.mask = s.mask_wd[e];
-        const double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
-        const double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
-        const double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
-        const double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
+        double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
+        double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
+        double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
+        double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
+        if (atm) {  // sea-level pressure and freshwater load (fv_3D_vel.F90:40-53)
+            const double4 w = r.col.w;
+            const double am = w.x * s.mslp[q.x] + w.y * s.mslp[q.y] + w.z * s.mslp[q.z] + w.w * s.mslp[q.w];
+            const double em = w.x * s.emp[q.x] + w.y * s.emp[q.y] + w.z * s.emp[q.z] + w.w * s.emp[q.w];
+            pre0 = pre0 - (kPref - am) * d0i + kG * em * d0i;
+            pre1 = pre1 - (kPref - am) * d0i + kG * em * d0i;
+            pre2 = pre2 - (kPref - am) * d0i + kG * em * d0i;
+            pre3 = pre3 - (kPref - am) * d0i + kG * em * d0i;
+        }
         r.Fx = sum4(gx, pre0, pre1, pre2, pre3) * r.area;
         r.Fy = sum4(gy, pre0, pre1, pre2, pre3) * r.area;
         r.acc = sum4(r.col.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
@@ -1078,7 +1089,8 @@ __global__ __launch_bounds__(256) void k_bt_edge_ssh(DevMesh m, DevStateT<R> s) 
 // elevation of update_2D_vel (fv_average_dynamic.F90:288-296) and the time-level
 // shift (fv_main.F90:1547-1551).
 template <typename R>
-__global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, double dt_2D, const double *__restrict__ tide) {
+__global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, double dt_2D, const double *__restrict__ tide,
+                                                 int atm, double d0i) {
     int n = TID();
     if (n >= m.N) return;
     double ssh = 0.0;
@@ -1096,7 +1108,10 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, doub
     const double de = 0.614, ga = 0.088, ep = 0.013;
     const double fe = 1.0 - de - ga - ep;
     double ea = de * en + fe * e0 + ga * e1 + ep * e2;
-    const double eg = kG * (m.atp1[n] * ea - m.atp2[n] * 0.0);
+    // with surface forcing: evaporation minus precipitation and sea-level
+    // pressure (fv_average_dynamic.F90:278-285)
+    const double eg = atm ? kG * (m.atp1[n] * ea - s.emp[n] * d0i - m.atp2[n] * 0.0) + (kPref - s.mslp[n]) * d0i
+                          : kG * (m.atp1[n] * ea - m.atp2[n] * 0.0);
     s.etaAB[n] = ea;
     s.eta_g[n] = eg;
     s.eta_n_2[n] = e1;
@@ -1755,15 +1770,119 @@ __device__ __forceinline__ double upw_vert(const DevMesh &m, const DevStateT<R> 
 // slot nz-1: the sweep reads Kv(2) at nz = 1 and Kv(nz) at nz >= 2, and never
 // Kv(1)) and tp/sp in the columns Tp/Sp, each rounded to the storage type as
 // the reference-order scratch arrays are.
+// sw_cp: src/fv_tracer.F90:1775-1910, the UNESCO 1983 heat capacity of sea
+// water (Fofonoff and Millard 1983), P in bar; literals the reference writes
+// without _WP are REAL(4)
+__device__ double sw_cp(double S, double Ti, double P) {
+    const double T = Ti < 0 ? 0.0 : Ti;
+    double S0, sS;
+    if (S < 0) {
+        S0 = 0.0;
+        sS = 0.0;
+    } else {
+        S0 = S;
+        sS = S * sqrt(S);
+    }
+    const double T68 = T * 1.00024;
+    double c0 = 4217.4, c1 = -3.720283, c2 = 0.1412855, c3 = -(double)2.654387e-3f, c4 = (double)2.093236e-5f;
+    double a0 = -7.64357, a1 = 0.1072763, a2 = -(double)1.38385e-3f;
+    double b0 = 0.1770383, b1 = -(double)4.07718e-3f, b2 = (double)5.148e-5f;
+    const double Cpst0 = (((c4 * T68 + c3) * T68 + c2) * T68 + c1) * T68 + c0 + (a0 + a1 * T68 + a2 * (T68 * T68)) * S0
+                       + (b0 + b1 * T68 + b2 * (T68 * T68)) * sS;
+    a0 = -(double)4.9592e-1f;
+    a1 = (double)1.45747e-2f;
+    a2 = -(double)3.13885e-4f;
+    const double a3 = (double)2.0357e-6f, a4 = (double)1.7168e-8f;
+    b0 = (double)2.4931e-4f;
+    b1 = -(double)1.08645e-5f;
+    b2 = (double)2.87533e-7f;
+    const double b3 = -(double)4.0027e-9f, b4 = (double)2.2956e-11f;
+    c0 = -(double)5.422e-8f;
+    c1 = (double)2.6380e-9f;
+    c2 = -(double)6.5637e-11f;
+    c3 = (double)6.136e-13f;
+    const double del_Cp0t0 = (((((c3 * T68 + c2) * T68 + c1) * T68 + c0) * P
+                             + ((((b4 * T68 + b3) * T68 + b2) * T68 + b1) * T68 + b0)) * P
+                             + ((((a4 * T68 + a3) * T68 + a2) * T68 + a1) * T68 + a0)) * P;
+    const double d0 = (double)4.9247e-3f, d1 = -(double)1.28315e-4f, d2 = (double)9.802e-7f, d3 = (double)2.5941e-8f,
+                 d4 = -(double)2.9179e-10f;
+    const double e0 = -(double)1.2331e-4f, e1 = -(double)1.517e-6f, e2 = (double)3.122e-8f;
+    const double f0 = -(double)2.9558e-6f, f1 = (double)1.17054e-7f, f2 = -(double)2.3905e-9f, f3 = (double)1.8448e-11f;
+    const double g0 = (double)9.971e-8f;
+    const double h0 = (double)5.540e-10f, h1 = -(double)1.7682e-11f, h2 = (double)3.513e-13f;
+    const double j1 = -(double)1.4300e-12f;
+    const double S3_2 = sS;
+    const double del_Cpstp =
+        (((((d4 * T68 + d3) * T68 + d2) * T68 + d1) * T68 + d0) * S0 + ((e2 * T68 + e1) * T68 + e0) * S3_2) * P
+        + ((((f3 * T68 + f2) * T68 + f1) * T68 + f0) * S0 + g0 * S3_2) * (P * P)
+        + (((h2 * T68 + h1) * T68 + h0) * S0 + j1 * T68 * S3_2) * (P * P * P);
+    return Cpst0 + del_Cp0t0 + del_Cpstp;
+}
+
+// calc_heat_flux_sw (src/fv_tracer.F90:1635-1700): heat source of layer nz
+// (1-based) of a column of depth a.  As the reference does: To/So (T_old,
+// S_old) already hold full values and the tracer means are added again, and
+// sw_cp(S, T, P) receives (T, S, P).  Shortwave absorption: Paulson and
+// Simpson with constant R, a, b (swr_paul_sim, 1590-1610) or Soloviev
+// (swr_soloviev, 1565-1588: exp(10 z) + 0 + 0).
+__device__ double heat_src(const DevMesh &m, const AtmPar &ap, int nz, double a, double To, double So, double rc,
+                           double Ta, double Sa, double qsr, double qns, double mslp) {
+    const int L = m.L;
+    const double d1 = m.oms[nz - 1] * a, d2 = m.oms[nz] * a;
+    const double T = To + Ta, S = So + Sa;
+    const double aux = (d1 + d2) * 0.5;
+    const double rho = rc + ap.rho0;
+    const double P = mslp == 0.0 ? (rho * kG * aux + kPref) * 10.0e-5 : (rho * kG * aux + mslp) * 10.0e-5;
+    const double c_p = sw_cp(T, S, P);
+    double swr1, swr2;
+    if (ap.type_swr == 1) {
+        swr1 = ap.Rjc * exp(-d1 / ap.ajc) + (1.0 - ap.Rjc) * exp(-d1 / ap.bjc);
+        swr2 = ap.Rjc * exp(-d2 / ap.ajc) + (1.0 - ap.Rjc) * exp(-d2 / ap.bjc);
+    } else {
+        swr1 = exp(10.0 * -d1);
+        swr2 = exp(10.0 * -d2);
+    }
+    const double jc = a * m.dsig[nz - 1];
+    double src = nz == 1 ? (qsr * (swr1 - swr2) + qns) / (jc * c_p * rho) : qsr * (swr1 - swr2) / (jc * c_p * rho);
+    if (nz == L) src = src + (double)0.2f * qsr * (swr2) / (jc * c_p * rho);
+    return src;
+}
+
+// tracer_impl_vert_diff of one column (fv_tracer.F90:1319-1494), the
+// coefficients and right-hand sides formed on the fly, the Thomas sweep
+// keeping its coefficients in the slots it has consumed.  With surface
+// forcing on a wet node (1376-1413) the heat source of every layer and the
+// surface freshwater source (calc_evap_precip, 1702-1752) join the
+// right-hand sides; TF/SF are anomalies there, T_old/S_old full values.
 template <typename R>
 __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R> &s, R *Tc, R *Sc, R *Kc, R *Tp,
-                                             R *Sp, double a, double Ta, double Sa, double dt) {
+                                             R *Sp, double a, double Ta, double Sa, double dt, const AtmPar &ap,
+                                             int n) {
     const int L = m.L, K = m.K;
     auto zb = [&](int z) { return m.oms[z - 1] * a; };
     auto Zc = [&](int z) { return 0.5 * (m.oms[z - 1] * a + m.oms[z] * a); };
     auto KV = [&](int z) { return Kc[z - 1]; };
     auto Td = [&](int z) { return Tc[z - 1] - Ta; };
     auto Sd = [&](int z) { return Sc[z - 1] - Sa; };
+    const int KC = K;
+    const bool atm = ap.on && (ap.wet_dry ? (m.depth[n] + s.etaAB[n] > ap.Dmin ? 1.0 : 0.0) : 1.0) != 0.0;
+    double frw = 0.0, qsr = 0.0, qns = 0.0, mslp = 0.0;
+    if (atm) {
+        qsr = s.qsr[n];
+        qns = s.qns[n];
+        mslp = s.mslp[n];
+        const double T = Td(1) + Ta, S = Sd(1) + Sa;
+        const double rho = densityJM(T, 0.0, -(m.oms[0] * a), ap.rho0);
+        const double jc1 = a * m.dsig[0];
+        if ((rho + ap.rho0) * jc1 != 0) {
+            frw = S * s.emp[n] / ((rho + ap.rho0) * jc1);
+            if (S <= 0.2 && frw < 0) frw = 0.0;
+        }
+    }
+    auto src = [&](int z) {
+        return heat_src(m, ap, z, a, (double)PL(s.T_old, z - 1, n, KC), (double)PL(s.S_old, z - 1, n, KC),
+                        s.rho_c[(size_t)n * L + z - 1], Ta, Sa, qsr, qns, mslp);
+    };
     double cp = 0, tp = 0, sp = 0;
     for (int nz = 1; nz <= L; ++nz) {
         double aa, bb, cc;
@@ -1784,12 +1903,18 @@ __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R
         if (nz == 1) {
             tr = cc * (-Td(2) + Td(1));
             sr = cc * (-Sd(2) + Sd(1));
+            if (atm) {
+                tr = tr + dt * src(1);
+                sr = sr + dt * frw;
+            }
         } else if (nz == L) {
             tr = aa * (Td(L) - Td(L - 1));
             sr = aa * (Sd(L) - Sd(L - 1));
+            if (atm) tr = tr + dt * src(L);
         } else {
             tr = -aa * Td(nz - 1) - cc * Td(nz + 1) + (aa + cc) * Td(nz);
             sr = -aa * Sd(nz - 1) - cc * Sd(nz + 1) + (aa + cc) * Sd(nz);
+            if (atm) tr = tr + dt * src(nz);
         }
         if (nz == 1) {
             cp = cc / bb;
@@ -2430,7 +2555,7 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
 // node columns, after every advection scheme, and alone for tracer_adv = 2
 // without MUSCL_CALL (fv_main.F90:1788-1800).
 template <typename R>
-__global__ __launch_bounds__(kColT) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt) {
+__global__ __launch_bounds__(kColT) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt, AtmPar ap) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int K = m.K, L = m.L, KP = K + 1;
     R *tT = (R *)lds_, *tS = tT + kColB * KP, *tK = tS + kColB * KP, *tTp = tK + kColB * KP, *tSp = tTp + kColB * KP;
@@ -2444,7 +2569,7 @@ __global__ __launch_bounds__(kColT) void k_tr_vdiff(DevMesh m, DevStateT<R> s, d
         const int c = threadIdx.x, n = c0 + c;
         const int mb = m.memb[n];
         vdiff_column(m, s, tT + c * KP, tS + c * KP, tK + c * KP, tTp + c * KP, tSp + c * KP, s.a[n], s.tsa[2 * mb],
-                     s.tsa[2 * mb + 1], dt);
+                     s.tsa[2 * mb + 1], dt, ap, n);
     }
     __syncthreads();
     tile_store(s.TF, tT, c0, nc, K, L);

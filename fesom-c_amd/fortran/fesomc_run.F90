@@ -112,6 +112,11 @@ program fesomc_run
   par%cl_relax = merge(1, 0, getl('cl_relax'))
   par%precision = FCG_FP64
   par%closure = FCG_CLOSURE_D3END
+  par%key_atm = merge(1, 0, getl('key_atm'))
+  par%type_swr_body = geti('type_swr_body', 2)
+  par%Rjc = getf('R', 0.58d0)
+  par%ajc = getf('a', 0.35d0)
+  par%bjc = getf('b', 23.0d0)
 
   ! ---- read_mesh_ser (src/fv_main.F90:787-945) -----------------------------
   meshdir = trim(rundir) // '/' // trim(gets('MESHPATH', './MESH/'))

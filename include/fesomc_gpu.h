@@ -68,6 +68,13 @@ typedef struct fcg_params {
     int cl_relax;              /* OB climatology relaxation                  */
     int precision;             /* FCG_FP64 | FCG_MIXED                       */
     int closure;               /* FCG_CLOSURE_D3END | FCG_CLOSURE_MLP        */
+    /* surface forcing (key_atm): the fields qsr, qns, emp, mslp are set by
+     * the host (fcg_set_state) from its own sbc_do; the step applies them
+     * as the reference does (fv_average_dynamic.F90:278-285, fv_3D_vel.F90:
+     * 40-53, fv_tracer.F90:1376-1413) */
+    int key_atm;
+    int type_swr_body;         /* 1 Paulson-Simpson (constant R, a, b), 2 Soloviev */
+    double Rjc, ajc, bjc;      /* fv_run.dat keys R, a, b (jer_const = T)    */
 } fcg_params;
 
 /*

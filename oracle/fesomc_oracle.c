@@ -52,6 +52,8 @@ typedef struct {
     double dt_2D, dt, time_jd0, Dcr, Dmin, C_d, z0b_min, z0s_min, za, snul;
     double T_const, S_const, lat_cartesian, cyclic_length, clim_relax, relax2clim_ac;
     double Cd_bz, Bp_bz, pow_sig, lev_bot, lev_sur, SL_radius;
+    int type_swr_body, jer_const;   /* shortwave absorption profile (key_atm) */
+    double Rjc, ajc, bjc;
     int nsteps, irep, cartesian, coriolis_tf, mom_adv_2D, mom_adv_3D, tracer_adv, vert_adv;
     int im_vert_visc, im_vert_visc_fic, fic_point, BFC, ver_mix, nsigma, ind_sigma;
     int temp_on, salinity_on, barotropic, barotropic_3D, wet_dry_on, TF_presence, T_potential;
@@ -98,6 +100,7 @@ typedef struct {
     double *U_filt_2D, *V_filt_2D; /* E */
     double *TF, *T_old, *SF, *S_old, *Tclim, *Sclim; /* LN */
     double *mask_wd, *mask_wd_node, *mask_ad, *mask_bpr, *C_d_el, *relax2clim; /* E N N E E N */
+    double *qsr, *qns, *emp, *mslp; /* N: surface fluxes and pressure (key_atm; set by the host) */
     int status; /* 0 ok; nonzero = reference STOP condition */
     char errmsg[256];
 } orc_t;
@@ -1125,7 +1128,7 @@ static void momentum_vert_impl_visc(orc_t *o) {
     free(a); free(b); free(c); free(ur); free(vr); free(cp); free(up); free(vp);
 }
 
-/* compute_vel_rhs: src/fv_3D_vel.F90:1-197 (key_atm off, filters off) */
+/* compute_vel_rhs: src/fv_3D_vel.F90:1-197 (filters off) */
 static void compute_vel_rhs(orc_t *o) {
     int L = o->L;
     for (int el = 0; el < o->E; el++) {
@@ -1133,6 +1136,12 @@ static void compute_vel_rhs(orc_t *o) {
         double pre[4];
         for (int i = 0; i < 4; i++)
             pre[i] = -G * (o->a_tp[2 * en[i]] * o->eta_n[en[i]] - o->a_tp[2 * en[i] + 1] * o->ssh_gp[en[i]]);
+        if (o->key_atm) {  /* src/fv_3D_vel.F90:40-53 */
+            const double *w = o->w_cv + 4 * el, d0i = 1.0 / o->density_0;
+            double atm = w[0] * o->mslp[en[0]] + w[1] * o->mslp[en[1]] + w[2] * o->mslp[en[2]] + w[3] * o->mslp[en[3]];
+            double empm = w[0] * o->emp[en[0]] + w[1] * o->emp[en[1]] + w[2] * o->emp[en[2]] + w[3] * o->emp[en[3]];
+            for (int i = 0; i < 4; i++) pre[i] = pre[i] - (P_REF - atm) * d0i + G * empm * d0i;
+        }
         const double *gs = o->gradient_sca + 8 * el;
         double Fx = sum4w(gs, pre[0], pre[1], pre[2], pre[3]) * o->elem_area[el];
         double Fy = sum4w(gs + 4, pre[0], pre[1], pre[2], pre[3]) * o->elem_area[el];
@@ -1292,7 +1301,7 @@ static void momentum_adv_upwind_2D(orc_t *o) {
     free(uu); free(vv);
 }
 
-/* update_2D_vel: src/fv_average_dynamic.F90:228-443 (key_atm off, 3-D branch) */
+/* update_2D_vel: src/fv_average_dynamic.F90:228-443 (3-D branch) */
 static void update_2D_vel(orc_t *o) {
     const double de = 0.614, ga = 0.088, ep = 0.013;
     const double fe = 1.0 - de - ga - ep;
@@ -1304,7 +1313,11 @@ static void update_2D_vel(orc_t *o) {
     for (int n = 0; n < o->N; n++) {
         o->eta_n_2[n] = de * o->ssh_rhs[n] + fe * o->eta_n[n] + ga * o->eta_n_1[n] + ep * o->eta_n_2[n];
         o->etaAB[n] = o->eta_n_2[n];
-        eta[n] = G * (o->a_tp[2 * n] * o->eta_n_2[n] - o->a_tp[2 * n + 1] * o->ssh_gp[n]);
+        if (o->key_atm)  /* src/fv_average_dynamic.F90:278-285 */
+            eta[n] = G * (o->a_tp[2 * n] * o->eta_n_2[n] - o->emp[n] * d0i - o->a_tp[2 * n + 1] * o->ssh_gp[n])
+                   + (P_REF - o->mslp[n]) * d0i;
+        else
+            eta[n] = G * (o->a_tp[2 * n] * o->eta_n_2[n] - o->a_tp[2 * n + 1] * o->ssh_gp[n]);
     }
     if (o->wet_dry_on) wad_mask(o);
     for (int el = 0; el < o->E; el++) {
@@ -1890,7 +1903,93 @@ static void solve_tracer_muscl(orc_t *o, double *ttf, double *tto, const double 
     free(rhs); free(rhs_c); free(ttx); free(tty); free(gud); free(tv);
 }
 
-/* tracer_impl_vert_diff: src/fv_tracer.F90:1319-1494 (key_atm off) */
+/* swr_paul_sim: src/fv_tracer.F90:1590-1610 (Paulson and Simpson 1977) */
+static double swr_paul_sim(double z, double R, double a, double b) { return R * exp(z / a) + (1.0 - R) * exp(z / b); }
+
+/* swr_soloviev: src/fv_tracer.F90:1565-1588, a = (1,0,0), b = (10,0,0):
+ * sum(a*exp(b*z)) = exp(10 z) + 0 + 0 */
+static double swr_soloviev(double z) {
+    double s = 1.0 * exp(10.0 * z);
+    s = s + 0.0 * exp(0.0 * z);
+    return s + 0.0 * exp(0.0 * z);
+}
+
+/* sw_cp: src/fv_tracer.F90:1775-1910, the UNESCO 1983 heat capacity of sea
+ * water (Fofonoff and Millard 1983), P in bar; literals without _WP are REAL(4) */
+double orc_sw_cp(double S, double Ti, double P) {
+    double T = Ti < 0 ? 0.0 : Ti, S0, sS;
+    if (S < 0) { S0 = 0.0; sS = 0.0; } else { S0 = S; sS = S * sqrt(S); }
+    double T68 = T * 1.00024;
+    double c0 = 4217.4, c1 = -3.720283, c2 = 0.1412855, c3 = -F4(2.654387e-3), c4 = F4(2.093236e-5);
+    double a0 = -7.64357, a1 = 0.1072763, a2 = -F4(1.38385e-3);
+    double b0 = 0.1770383, b1 = -F4(4.07718e-3), b2 = F4(5.148e-5);
+    double Cpst0 = (((c4 * T68 + c3) * T68 + c2) * T68 + c1) * T68 + c0 + (a0 + a1 * T68 + a2 * (T68 * T68)) * S0
+                 + (b0 + b1 * T68 + b2 * (T68 * T68)) * sS;
+    a0 = -F4(4.9592e-1); a1 = F4(1.45747e-2); a2 = -F4(3.13885e-4);
+    double a3 = F4(2.0357e-6), a4 = F4(1.7168e-8);
+    b0 = F4(2.4931e-4); b1 = -F4(1.08645e-5); b2 = F4(2.87533e-7);
+    double b3 = -F4(4.0027e-9), b4 = F4(2.2956e-11);
+    c0 = -F4(5.422e-8); c1 = F4(2.6380e-9); c2 = -F4(6.5637e-11); c3 = F4(6.136e-13);
+    double del_Cp0t0 = (((((c3 * T68 + c2) * T68 + c1) * T68 + c0) * P
+                       + ((((b4 * T68 + b3) * T68 + b2) * T68 + b1) * T68 + b0)) * P
+                       + ((((a4 * T68 + a3) * T68 + a2) * T68 + a1) * T68 + a0)) * P;
+    double d0 = F4(4.9247e-3), d1 = -F4(1.28315e-4), d2 = F4(9.802e-7), d3 = F4(2.5941e-8), d4 = -F4(2.9179e-10);
+    double e0 = -F4(1.2331e-4), e1 = -F4(1.517e-6), e2 = F4(3.122e-8);
+    double f0 = -F4(2.9558e-6), f1 = F4(1.17054e-7), f2 = -F4(2.3905e-9), f3 = F4(1.8448e-11);
+    double g0 = F4(9.971e-8);
+    double h0 = F4(5.540e-10), h1 = -F4(1.7682e-11), h2 = F4(3.513e-13);
+    double j1 = -F4(1.4300e-12);
+    double S3_2 = sS;
+    double del_Cpstp = (((((d4 * T68 + d3) * T68 + d2) * T68 + d1) * T68 + d0) * S0 + ((e2 * T68 + e1) * T68 + e0) * S3_2) * P
+                     + ((((f3 * T68 + f2) * T68 + f1) * T68 + f0) * S0 + g0 * S3_2) * (P * P)
+                     + (((h2 * T68 + h1) * T68 + h0) * S0 + j1 * T68 * S3_2) * (P * P * P);
+    return Cpst0 + del_Cp0t0 + del_Cpstp;
+}
+
+/* calc_heat_flux_sw: src/fv_tracer.F90:1635-1700, the heat source of layer nz
+ * (1-based) of node n.  As in the reference: T_old/S_old already hold full
+ * values and T_aver/S_aver are added again, and sw_cp(S, T, P) is called with
+ * (T, S, P). */
+static double heat_flux_sw(const orc_t *o, int nz, int n) {
+    int L = o->L, K = o->K;
+    double d1 = o->zbar[(size_t)n * K + nz - 1], d2 = o->zbar[(size_t)n * K + nz];
+    double T = o->T_old[(size_t)n * L + nz - 1] + o->T_aver;
+    double S = o->S_old[(size_t)n * L + nz - 1] + o->S_aver;
+    double aux = (d1 + d2) * 0.5;
+    double rho = o->rho_c[(size_t)n * L + nz - 1] + o->density_0;
+    double P = o->mslp[n] == 0.0 ? (rho * G * aux + P_REF) * 10.0e-5 : (rho * G * aux + o->mslp[n]) * 10.0e-5;
+    double c_p = orc_sw_cp(T, S, P);
+    double swr1, swr2;
+    if (o->type_swr_body == 1) {
+        swr1 = swr_paul_sim(-d1, o->Rjc, o->ajc, o->bjc);
+        swr2 = swr_paul_sim(-d2, o->Rjc, o->ajc, o->bjc);
+    } else {
+        swr1 = swr_soloviev(-d1);
+        swr2 = swr_soloviev(-d2);
+    }
+    double jc = o->Jc[(size_t)n * L + nz - 1], src;
+    if (nz == 1) src = (o->qsr[n] * (swr1 - swr2) + o->qns[n]) / (jc * c_p * rho);
+    else src = o->qsr[n] * (swr1 - swr2) / (jc * c_p * rho);
+    if (nz == L) src = src + F4(0.2) * o->qsr[n] * (swr2) / (jc * c_p * rho);
+    return src;
+}
+
+/* calc_evap_precip: src/fv_tracer.F90:1702-1752, the surface salt source of
+ * node n; TF/SF hold anomalies here (tracer_impl_vert_diff removed T_aver) */
+static double evap_precip(const orc_t *o, int n) {
+    int L = o->L, K = o->K;
+    double T = o->TF[(size_t)n * L] + o->T_aver, S = o->SF[(size_t)n * L] + o->S_aver;
+    double rho = orc_densityJM(T, 0.0, -o->zbar[(size_t)n * K], o->density_0);
+    double frw = 0.0;
+    if ((rho + o->density_0) * o->Jc[(size_t)n * L] != 0) {
+        frw = S * o->emp[n] / ((rho + o->density_0) * o->Jc[(size_t)n * L]);
+        if (S <= 0.2 && frw < 0) frw = 0.0;
+    }
+    return frw;
+}
+
+/* tracer_impl_vert_diff: src/fv_tracer.F90:1319-1494; with key_atm the
+ * surface heat/freshwater and shortwave sources of wet nodes (1376-1413) */
 static int tracer_impl_vert_diff(orc_t *o) {
     int L = o->L, K = o->K;
     double *a = DALLOC(K + 1), *b = DALLOC(K + 1), *c = DALLOC(K + 1), *tr = DALLOC(K + 1), *sr = DALLOC(K + 1);
@@ -1901,6 +2000,8 @@ static int tracer_impl_vert_diff(orc_t *o) {
         const double *Kv = o->Kv + (size_t)n * K - 1, *zb = o->zbar + (size_t)n * K - 1, *Zn = o->Z + (size_t)n * L - 1;
         double *T = o->TF + (size_t)n * L - 1, *S = o->SF + (size_t)n * L - 1;
         double dt = o->dt;
+        const int atm = o->key_atm && o->mask_wd_node[n] != 0.0;
+        const double frw = atm ? evap_precip(o, n) : 0.0;
         for (int nz = 2; nz <= L - 1; nz++) {
             a[nz] = -dt * Kv[nz] / ((Zn[nz - 1] - Zn[nz]) * (zb[nz] - zb[nz + 1]));
             c[nz] = -dt * Kv[nz] / ((Zn[nz] - Zn[nz + 1]) * (zb[nz] - zb[nz + 1]));
@@ -1912,14 +2013,25 @@ static int tracer_impl_vert_diff(orc_t *o) {
         a[L] = -dt * Kv[L] / ((Zn[L - 1] - Zn[L]) * (zb[L] - zb[K]));
         b[L] = -a[L] + 1.0;
         c[L] = 0.0;
-        tr[1] = c[1] * (-T[2] + T[1]);
-        sr[1] = c[1] * (-S[2] + S[1]);
-        for (int nz = 2; nz <= L - 1; nz++) {
-            tr[nz] = -a[nz] * T[nz - 1] - c[nz] * T[nz + 1] + (a[nz] + c[nz]) * T[nz];
-            sr[nz] = -a[nz] * S[nz - 1] - c[nz] * S[nz + 1] + (a[nz] + c[nz]) * S[nz];
+        if (atm) {
+            tr[1] = c[1] * (-T[2] + T[1]) + dt * heat_flux_sw(o, 1, n);
+            sr[1] = c[1] * (-S[2] + S[1]) + dt * frw;
+            for (int nz = 2; nz <= L - 1; nz++) {
+                tr[nz] = -a[nz] * T[nz - 1] - c[nz] * T[nz + 1] + (a[nz] + c[nz]) * T[nz] + dt * heat_flux_sw(o, nz, n);
+                sr[nz] = -a[nz] * S[nz - 1] - c[nz] * S[nz + 1] + (a[nz] + c[nz]) * S[nz];
+            }
+            tr[L] = a[L] * (T[L] - T[L - 1]) + dt * heat_flux_sw(o, L, n);
+            sr[L] = a[L] * (S[L] - S[L - 1]);
+        } else {
+            tr[1] = c[1] * (-T[2] + T[1]);
+            sr[1] = c[1] * (-S[2] + S[1]);
+            for (int nz = 2; nz <= L - 1; nz++) {
+                tr[nz] = -a[nz] * T[nz - 1] - c[nz] * T[nz + 1] + (a[nz] + c[nz]) * T[nz];
+                sr[nz] = -a[nz] * S[nz - 1] - c[nz] * S[nz + 1] + (a[nz] + c[nz]) * S[nz];
+            }
+            tr[L] = a[L] * (T[L] - T[L - 1]);
+            sr[L] = a[L] * (S[L] - S[L - 1]);
         }
-        tr[L] = a[L] * (T[L] - T[L - 1]);
-        sr[L] = a[L] * (S[L] - S[L - 1]);
         cp[1] = c[1] / b[1]; tp[1] = tr[1] / b[1]; sp[1] = sr[1] / b[1];
         for (int nz = 2; nz <= L; nz++) {
             double m = 1.0 / (b[nz] - cp[nz - 1] * a[nz]);
@@ -2081,6 +2193,9 @@ static void parse_config(orc_t *o, const char *t) {
     o->Cd_bz = cfg_f(t, "Cd_bz", 0); o->Bp_bz = cfg_f(t, "Bp_bz", 0);
     o->SL_obn = cfg_l(t, "SL_obn", 0) && !cfg_l(t, "SL_obn_user_def", 0); /* src/fv_main.F90:297 */
     o->SL_radius = cfg_f(t, "SL_radius", 0.0);
+    o->type_swr_body = cfg_i(t, "type_swr_body", 2);
+    o->jer_const = cfg_l(t, "jer_const", 1);
+    o->Rjc = cfg_f(t, "R", 0.58); o->ajc = cfg_f(t, "a", 0.35); o->bjc = cfg_f(t, "b", 23.0);
     o->muscl_call = cfg_l(t, "MUSCL_CALL", 0);
     if (scan_key(t, "RESTART", v, sizeof v)) o->restart = strcmp(v, "hot_start") == 0; else o->restart = 0;
     o->Mt = (int)(o->dt / o->dt_2D); /* src/fv_read_run.F90:1060 */
@@ -2109,7 +2224,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     parse_config(o, cfg);
     if (o->type_task != 3) { o->status = -10; snprintf(o->errmsg, sizeof o->errmsg, "only type_task=3"); return o; }
     if (o->filt_2D || o->filt_bi_2D || o->bih_2D || o->filt_3D || o->filt_bi_3D || o->bih_3D || o->mixed_BP_grad_OB
-        || o->key_atm || o->key_rivers || o->T_potential || o->ver_mix != 3
+        || (o->key_atm && o->type_swr_body == 1 && !o->jer_const) || o->key_rivers || o->T_potential || o->ver_mix != 3
         || (o->SL_obn && !(o->SL_radius > 0.0))) {
         o->status = -14;
         snprintf(o->errmsg, sizeof o->errmsg, "configuration uses an option outside the restated hot path");
@@ -2169,6 +2284,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     o->TF = DALLOC(NL); o->T_old = DALLOC(NL); o->SF = DALLOC(NL); o->S_old = DALLOC(NL);
     o->Tclim = DALLOC(NL); o->Sclim = DALLOC(NL);
     o->mask_wd = DALLOC(E); o->mask_wd_node = DALLOC(N); o->mask_ad = DALLOC(N); o->mask_bpr = DALLOC(E);
+    o->qsr = DALLOC(N); o->qns = DALLOC(N); o->emp = DALLOC(N); o->mslp = DALLOC(N);
     o->C_d_el = DALLOC(E); o->relax2clim = DALLOC(N); o->ac = DALLOC(N); o->a_tp = DALLOC(2 * (size_t)N);
     for (size_t i = 0; i < NK; i++) { o->snu[i] = o->snul; o->bt[i] = 1.0e-8; o->Kv[i] = o->snul; }
     for (size_t i = 0; i < EK; i++) o->Av[i] = o->snul;
@@ -2332,6 +2448,7 @@ static double *fld(orc_t *o, const char *name, long *cnt) {
     F("vel_grad_ux", o->vel_grad_ux, E * L) F("vel_grad_uy", o->vel_grad_uy, E * L)
     F("vel_grad_vx", o->vel_grad_vx, E * L) F("vel_grad_vy", o->vel_grad_vy, E * L)
     F("Unode_p", o->Unode_p, N * L) F("Vnode_p", o->Vnode_p, N * L) F("a_tp", o->a_tp, 2 * N)
+    F("qsr", o->qsr, N) F("qns", o->qns, N) F("emp", o->emp, N) F("mslp", o->mslp, N)
     F("ampt", o->ampt, 12L * (o->nobn > 0 ? o->nobn : 1)) F("fazt", o->fazt, 12L * (o->nobn > 0 ? o->nobn : 1))
 #undef F
     *cnt = -1;

@@ -60,6 +60,8 @@ def lib():
         L.orc_destroy.argtypes = [P]
         L.orc_densityJM.restype = C.c_double
         L.orc_densityJM.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_sw_cp.restype = C.c_double
+        L.orc_sw_cp.argtypes = [C.c_double, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -74,6 +76,10 @@ def _ip(a):
 
 def densityJM(t, s, pz, density_0):
     return lib().orc_densityJM(float(t), float(s), float(pz), float(density_0))
+
+
+def sw_cp(s, t, p_bar):
+    return lib().orc_sw_cp(float(s), float(t), float(p_bar))
 
 
 class Oracle:

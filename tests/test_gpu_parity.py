@@ -164,3 +164,31 @@ def test_negative_temperature_stop_is_reported():
     with pytest.raises(RuntimeError, match="Negative temperature"):
         o.step(1)
     g.close()
+
+
+@pytest.mark.parametrize("swr", [1, 2])
+def test_surface_forcing_matches_oracle(swr):
+    """key_atm on the stratified wet/dry channel: spatially varying shortwave
+    and non-solar heat flux, evaporation minus precipitation and sea-level
+    pressure (fv_average_dynamic.F90:278-285, fv_3D_vel.F90:40-53,
+    fv_tracer.F90:1376-1413), with the Paulson-Simpson (1) or Soloviev (2)
+    shortwave profile: bit for bit (1e-12) against the oracle."""
+    import re
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    txt = re.sub(r"^(\s*key_atm\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    txt += f"\n type_swr_body = {swr}\n jer_const = T\n R = 0.58\n a = 0.35\n b = 23.0\n"
+    g, o = _model(txt, mesh, ex)
+    x, y = mesh.xy[:, 0], mesh.xy[:, 1]
+    forcing = {"qsr": 300.0 * (0.5 + 0.5 * np.sin(2 * np.pi * x / 20000.0)),
+               "qns": -80.0 + 40.0 * np.cos(2 * np.pi * y / 10000.0),
+               "emp": 2e-5 * np.sin(2 * np.pi * (x + y) / 15000.0),
+               "mslp": 101325.0 + 500.0 * np.sin(2 * np.pi * y / 10000.0)}
+    for k, v in forcing.items():
+        g.set(k, v)
+        o.set(k, v)
+    g.step(5)
+    o.step(5)
+    _compare(g, o, FIELDS + ["rho_c"])
+    assert g.diagnostics()["status"] == 0
+    g.close()

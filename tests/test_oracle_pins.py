@@ -154,3 +154,96 @@ def test_min_max_principle_upwind():
     o.step(10)
     T = o.get("TF")
     assert T.max() <= T0.max() + 1e-6 and T.min() >= T0.min() - 1e-6
+
+
+# ---- surface forcing (key_atm): src/fv_tracer.F90:1376-1413, 1635-1910,
+# src/fv_average_dynamic.F90:278-285, src/fv_3D_vel.F90:40-53
+
+def test_sw_cp_unesco_check_values():
+    """UNESCO 1983 (Fofonoff and Millard) check value of the heat capacity:
+    3849.500 J/(kg degC) at S = 40, T68 = 40 degC (IPTS-68; the routine takes
+    ITS-90 and converts T68 = 1.00024 T), P = 10000 dbar (1000 bar); at
+    S = 0, T = 0, P = 0 the polynomial reduces to its constant 4217.4."""
+    from oracle.pyoracle import sw_cp
+    assert sw_cp(40.0, 40.0 / 1.00024, 1000.0) == pytest.approx(3849.500, abs=5e-3)
+    assert sw_cp(0.0, 0.0, 0.0) == 4217.4
+
+
+def _le_atm(le_case, extra):
+    import re
+    txt, mesh = le_case
+    txt, n = re.subn(r"^(\s*key_atm\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    assert n == 1
+    return Oracle(txt + extra, mesh), mesh
+
+
+def _content(o, field):
+    """sum over nodes and layers of area * Jc * field (the tracer content)."""
+    L = o.size["L"]
+    return float((o.get("area")[:, None] * o.get("Jc").reshape(-1, L) * o.get(field).reshape(-1, L)).sum())
+
+
+def test_surface_heat_flux_closes_the_heat_budget(le_case):
+    """In the closed LE basin advection and diffusion conserve heat content,
+    so one step with a uniform non-solar heat flux Q (qsr = 0) adds
+    sum(area*dt*Q/(c_p*rho)) of heat content against the same step without
+    it (c_p*rho ~ 3.9e6 J/m3/K: the check catches a lost term, a wrong sign,
+    a missing dt or Jc factor)."""
+    from fesomc_amd.runconfig import RunConfig
+    Q = 500.0
+    dH = []
+    for q in (0.0, Q):
+        o, mesh = _le_atm(le_case, "")
+        N = o.size["N"]
+        o.set("mslp", np.full(N, 101325.0))
+        o.set("qns", np.full(N, q))
+        h0 = _content(o, "TF")
+        o.step(1)
+        dH.append(_content(o, "TF") - h0)
+    dt = RunConfig(le_case[0]).f("dt")
+    want = float((o.get("area") * dt * Q).sum()) / 3.9e6
+    assert dH[1] - dH[0] == pytest.approx(want, rel=0.05)
+
+
+def test_freshwater_flux_closes_the_salt_budget():
+    """One step of the stratified channel with a uniform evaporation minus
+    precipitation E adds sum(area*dt*S_surface*E/rho_fresh) of salt content
+    over the wet nodes against the same step without it (rho_fresh ~ 1000
+    kg/m3; the open boundary passes the same fluxes in both runs)."""
+    import re
+    from fesomc_amd import meshgen
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, ex = meshgen.config("c2_small")
+    txt = re.sub(r"^(\s*key_atm\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    E = 1e-4
+    dS = []
+    for e in (0.0, E):
+        o = Oracle(txt, mesh)
+        o.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
+        N = o.size["N"]
+        o.set("mslp", np.full(N, 101325.0))
+        o.set("emp", np.full(N, e))
+        s0 = _content(o, "SF")
+        o.step(1)
+        dS.append(_content(o, "SF") - s0)
+    dt = RunConfig(txt).f("dt")
+    wet = o.get("mask_wd_node") != 0
+    S_surf = o.get("SF").reshape(-1, o.size["L"])[:, 0]
+    want = float((o.get("area") * dt * S_surf * E)[wet].sum()) / 1.0e3
+    assert want > 0
+    assert dS[1] - dS[0] == pytest.approx(want, rel=0.05)
+
+
+def test_uniform_air_pressure_exerts_no_force(le_case):
+    """A uniform sea-level pressure and freshwater flux add the same value to
+    every node's pressure term, whose gradient vanishes: velocities after a
+    few steps equal those of the run without surface forcing to round-off."""
+    txt, mesh = le_case
+    o0 = Oracle(txt, mesh)
+    o1, _ = _le_atm(le_case, "")
+    N = o1.size["N"]
+    o1.set("mslp", np.full(N, 101325.0 + 2000.0))
+    o0.step(3)
+    o1.step(3)
+    u0, u1 = o0.get("U_n"), o1.get("U_n")
+    assert np.abs(u1 - u0).max() <= 1e-9 * np.abs(u0).max()
