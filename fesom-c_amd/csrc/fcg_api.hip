@@ -13,12 +13,14 @@
 
 namespace fcgk {
 template <typename R> __global__ void k_pressure(DevMesh, DevStateT<R>, double, double);
+template <typename R> __global__ void k_pressure_atm(DevMesh, DevStateT<R>, double, double);
 template <typename R> __global__ void k_bpg(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_d3end_prep(DevMesh, DevStateT<R>, double, double, double, const int *);
 template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, const int *);
 template <typename R> __global__ void k_d3end_fin(DevMesh, DevStateT<R>, double, double, double, double, const int *);
 template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
-template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int, int, double);
+template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int, double);
+template <typename R> __global__ void k_velrhs_atm(DevMesh, DevStateT<R>, double, double, int, int, double);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *, int, double);
@@ -46,6 +48,7 @@ template <typename R> __global__ void k_tr_muscl(DevMesh, DevStateT<R>, double, 
 template <typename R> __global__ void k_tr_gradn_miura(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_miura(DevMesh, DevStateT<R>, double, double, double, int, double, double);
 template <typename R> __global__ void k_tr_vdiff(DevMesh, DevStateT<R>, double, AtmPar);
+template <typename R> __global__ void k_tr_vdiff_atm(DevMesh, DevStateT<R>, double, AtmPar);
 template <typename R> __global__ void k_diag_partial(DevMesh, DevStateT<R>, double, int, int, int, int, double *);
 __global__ void k_diag_final(const double *, int, double *);
 __global__ void k_mask_count(const double *, int, int *);
@@ -998,7 +1001,10 @@ template <typename R>
 static int pressure_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const double gd0 = kG / ctx->density_0;
     const DevMesh &m = ctx->m;
-    LAUNCH_EX(ctx, k_pressure<R>, nflatblk(m.N, m.L), kFlatT, flat_bytes(m.L, 8), m, s, gd0, ctx->density_0);
+    if (ctx->p.key_atm)
+        LAUNCH_EX(ctx, k_pressure_atm<R>, nflatblk(m.N, m.L), kFlatT, flat_bytes(m.L, 8), m, s, gd0, ctx->density_0);
+    else
+        LAUNCH_EX(ctx, k_pressure<R>, nflatblk(m.N, m.L), kFlatT, flat_bytes(m.L, 8), m, s, gd0, ctx->density_0);
     LAUNCH_EX(ctx, k_bpg<R>, ncolblk(m.E), kColT, 2 * tile_bytes(m, 8), m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
@@ -1061,8 +1067,12 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     }
     LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec), ctx->m, s);
     const int vt = block_of(ctx, "k_velrhs");
-    LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt), ctx->m, s,
-              p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, p.key_atm, 1.0 / ctx->density_0);
+    if (p.key_atm)
+        LAUNCH_EX(ctx, k_velrhs_atm<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt),
+                  ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, 1.0 / ctx->density_0);
+    else
+        LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt), ctx->m,
+                  s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, 1.0 / ctx->density_0);
     if (p.im_vert_visc_fic)
         LAUNCH_EX(ctx, k_velvisc<R>, ncolblk(ctx->m.E), kColT, tile_bytes(ctx->m, 8) + 2 * tile_bytes(ctx->m, sizeof(R)),
                   ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
@@ -1195,7 +1205,10 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     ap.bjc = p.bjc;
     ap.rho0 = ctx->density_0;
     ap.Dmin = p.Dmin;
-    LAUNCH_EX(ctx, k_tr_vdiff<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt, ap);
+    if (p.key_atm)
+        LAUNCH_EX(ctx, k_tr_vdiff_atm<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt, ap);
+    else
+        LAUNCH_EX(ctx, k_tr_vdiff<R>, ncolblk(m.N), kColT, 5 * tile_bytes(m, sizeof(R)), m, s, p.dt, ap);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }

@@ -196,8 +196,8 @@ __device__ __forceinline__ int2 csr_block(const int *ptr, const ColFlat &f) {
 // ---------------------------------------------------------------------------
 // pressure: hydrostatic column scan, src/fv_pressure_new.F90:91-107
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
+template <typename R, bool ATM>
+__device__ __forceinline__ void pressure_body(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
     extern __shared__ __align__(16) unsigned char lds_[];
     double *th = (double *)lds_;  // per column: the layer increments, then their running sums
     const int L = m.L, KC = m.K;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, dou
         const double a = s.a[n];
         const double Z = 0.5 * (m.oms[k] * a + m.oms[k + 1] * a);
         const double r = densityJM(PL(s.TF, k, n, KC), PL(s.SF, k, n, KC), -Z, rho0);
-        if (s.rho_c) s.rho_c[(size_t)n * L + k] = r;  // rho_c, read by the surface heat source
+        if (ATM) s.rho_c[(size_t)n * L + k] = r;  // rho_c, read by the surface heat source
         th[f.c * L + k] = r * (a * m.dsig[k]) * gd0;
     }
     __syncthreads();
@@ -225,6 +225,15 @@ __global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, dou
         PL(s.hp, k + 1, n, KC) = th[f.c * L + k];
         if (k == 0) PL(s.hp, 0, n, KC) = 0.0;
     }
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_pressure(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
+    pressure_body<R, false>(m, s, gd0, rho0);
+}
+// the same, also storing rho_c for the surface heat source (key_atm)
+template <typename R>
+__global__ __launch_bounds__(256) void k_pressure_atm(DevMesh m, DevStateT<R> s, double gd0, double rho0) {
+    pressure_body<R, true>(m, s, gd0, rho0);
 }
 
 // Per layer (four threads per element column) the gradient and its
@@ -733,9 +742,9 @@ static_assert(sizeof(EdgeRec<double>) == velrhs_edge_bytes(8) && sizeof(EdgeRec<
 // layer's two interfaces (354-469) and the explicit update (124-143).  Writes
 // the new AB history and the provisional velocity into U_rhs/V_rhs;
 // k_velvisc then applies the implicit vertical viscosity column by column.
-template <typename R>
-__global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
-                                                  int atm, double d0i) {
+template <typename R, bool ATM>
+__device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
+                                            double d0i) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int L = m.L, K = m.K, KC = K;
     const ColFlat f = col_flat(m.E, L);
@@ -760,7 +769,7 @@ __global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, doub
         double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
         double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
         double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
-        if (atm) {  // sea-level pressure and freshwater load (fv_3D_vel.F90:40-53)
+        if (ATM) {  // sea-level pressure and freshwater load (fv_3D_vel.F90:40-53)
             const double4 w = r.col.w;
             const double am = w.x * s.mslp[q.x] + w.y * s.mslp[q.y] + w.z * s.mslp[q.z] + w.w * s.mslp[q.w];
             const double em = w.x * s.emp[q.x] + w.y * s.emp[q.y] + w.z * s.emp[q.z] + w.w * s.emp[q.w];
@@ -920,6 +929,18 @@ __global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, doub
     PL(s.V_rhsAB, k, e, KC) = abv;
     PL(s.U_rhs, k, e, KC) = mask * (u + PL(s.Bpu, k, e, KC) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
     PL(s.V_rhs, k, e, KC) = mask * (v + PL(s.Bpv, k, e, KC) * dt + (dt * (vr + abv * ff) * area_inv + 0.0) / je);
+}
+
+template <typename R>
+__global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
+                                                  double d0i) {
+    velrhs_body<R, false>(m, s, dt, ff, hadv, vadv, d0i);
+}
+// the same with surface forcing (key_atm)
+template <typename R>
+__global__ __launch_bounds__(1024) void k_velrhs_atm(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
+                                                      int vadv, double d0i) {
+    velrhs_body<R, true>(m, s, dt, ff, hadv, vadv, d0i);
 }
 
 // momentum_vert_impl_visc_tmp (fv_viscosity.F90:1374-1486) on tiles of 64
@@ -1854,7 +1875,7 @@ __device__ double heat_src(const DevMesh &m, const AtmPar &ap, int nz, double a,
 // forcing on a wet node (1376-1413) the heat source of every layer and the
 // surface freshwater source (calc_evap_precip, 1702-1752) join the
 // right-hand sides; TF/SF are anomalies there, T_old/S_old full values.
-template <typename R>
+template <typename R, bool ATM>
 __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R> &s, R *Tc, R *Sc, R *Kc, R *Tp,
                                              R *Sp, double a, double Ta, double Sa, double dt, const AtmPar &ap,
                                              int n) {
@@ -1865,7 +1886,7 @@ __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R
     auto Td = [&](int z) { return Tc[z - 1] - Ta; };
     auto Sd = [&](int z) { return Sc[z - 1] - Sa; };
     const int KC = K;
-    const bool atm = ap.on && (ap.wet_dry ? (m.depth[n] + s.etaAB[n] > ap.Dmin ? 1.0 : 0.0) : 1.0) != 0.0;
+    const bool atm = ATM && (ap.wet_dry ? (m.depth[n] + s.etaAB[n] > ap.Dmin ? 1.0 : 0.0) : 1.0) != 0.0;
     double frw = 0.0, qsr = 0.0, qns = 0.0, mslp = 0.0;
     if (atm) {
         qsr = s.qsr[n];
@@ -2554,8 +2575,8 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
 // tracer_impl_vert_diff of T and S (fv_tracer.F90:1319-1494) on tiles of 64
 // node columns, after every advection scheme, and alone for tracer_adv = 2
 // without MUSCL_CALL (fv_main.F90:1788-1800).
-template <typename R>
-__global__ __launch_bounds__(kColT) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt, AtmPar ap) {
+template <typename R, bool ATM>
+__device__ __forceinline__ void tr_vdiff_body(DevMesh m, DevStateT<R> s, double dt, AtmPar ap) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int K = m.K, L = m.L, KP = K + 1;
     R *tT = (R *)lds_, *tS = tT + kColB * KP, *tK = tS + kColB * KP, *tTp = tK + kColB * KP, *tSp = tTp + kColB * KP;
@@ -2568,12 +2589,21 @@ __global__ __launch_bounds__(kColT) void k_tr_vdiff(DevMesh m, DevStateT<R> s, d
     if ((int)threadIdx.x < nc) {
         const int c = threadIdx.x, n = c0 + c;
         const int mb = m.memb[n];
-        vdiff_column(m, s, tT + c * KP, tS + c * KP, tK + c * KP, tTp + c * KP, tSp + c * KP, s.a[n], s.tsa[2 * mb],
-                     s.tsa[2 * mb + 1], dt, ap, n);
+        vdiff_column<R, ATM>(m, s, tT + c * KP, tS + c * KP, tK + c * KP, tTp + c * KP, tSp + c * KP, s.a[n],
+                             s.tsa[2 * mb], s.tsa[2 * mb + 1], dt, ap, n);
     }
     __syncthreads();
     tile_store(s.TF, tT, c0, nc, K, L);
     tile_store(s.SF, tS, c0, nc, K, L);
+}
+template <typename R>
+__global__ __launch_bounds__(kColT) void k_tr_vdiff(DevMesh m, DevStateT<R> s, double dt, AtmPar ap) {
+    tr_vdiff_body<R, false>(m, s, dt, ap);
+}
+// the same with surface forcing (key_atm)
+template <typename R>
+__global__ __launch_bounds__(kColT) void k_tr_vdiff_atm(DevMesh m, DevStateT<R> s, double dt, AtmPar ap) {
+    tr_vdiff_body<R, true>(m, s, dt, ap);
 }
 
 // ---------------------------------------------------------------------------
@@ -2826,12 +2856,14 @@ __global__ __launch_bounds__(256) void k_mask_count(const double *mask, int E, i
 template <typename R>
 void instantiate_kernels() {
     (void)&k_pressure<R>;
+    (void)&k_pressure_atm<R>;
     (void)&k_bpg<R>;
     (void)&k_d3end_prep<R>;
     (void)&k_d3end<R>;
     (void)&k_d3end_fin<R>;
     (void)&k_velgrad3d<R>;
     (void)&k_velrhs<R>;
+    (void)&k_velrhs_atm<R>;
     (void)&k_velvisc<R>;
     (void)&k_bt_elem1<R>;
     (void)&k_bt_node<R>;
@@ -2858,6 +2890,7 @@ void instantiate_kernels() {
     (void)&k_tr_muscl<R>;
     (void)&k_tr_miura<R>;
     (void)&k_tr_vdiff<R>;
+    (void)&k_tr_vdiff_atm<R>;
     (void)&k_diag_partial<R>;
     (void)&k_mlp_collect<R>;
     (void)&k_mlp_closure<R>;
