@@ -23,7 +23,8 @@ template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, do
 template <typename R> __global__ void k_velrhs_atm(DevMesh, DevStateT<R>, double, double, int, int, double);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
-template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *, int, double);
+template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *, int, double,
+                                                const double *);
 template <typename R> __global__ void k_bt_edge_ssh(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_edge_mom(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_velgrad(DevMesh, DevStateT<R>);
@@ -47,6 +48,7 @@ template <typename R> __global__ void k_tr_gradn_muscl(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_muscl(DevMesh, DevStateT<R>, double, double, int, double, double);
 template <typename R> __global__ void k_tr_gradn_miura(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_miura(DevMesh, DevStateT<R>, double, double, double, int, double, double);
+template <typename R> __global__ void k_tr_rivers(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_tr_vdiff(DevMesh, DevStateT<R>, double, AtmPar);
 template <typename R> __global__ void k_tr_vdiff_atm(DevMesh, DevStateT<R>, double, AtmPar);
 template <typename R> __global__ void k_diag_partial(DevMesh, DevStateT<R>, double, int, int, int, int, double *);
@@ -239,6 +241,7 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
         {"bt", s.bt, K, true, false},          {"Wvel", s.W, K, true, false},
         {"qsr", s.qsr, 1, true, false},        {"qns", s.qns, 1, true, false},        {"emp", s.emp, 1, true, false},
         {"mslp", s.mslp, 1, true, false},      {"rho_c", s.rho_c, L, true, false},
+        {"runoff", s.runoff, 1, true, false},
     };
     for (const Ent &t : tab)
         if (!std::strcmp(t.n, name)) {
@@ -282,7 +285,7 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
         hipLaunchKernelGGL(fcgk::k_bt_elem1<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
         hipLaunchKernelGGL(fcgk::k_bt_edge_ssh<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s);
         hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide, p.key_atm,
-                           1.0 / ctx->density_0);
+                           1.0 / ctx->density_0, p.key_rivers ? (const double *)s.runoff : nullptr);
         hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         if (p.mom_adv_2D == 2)
             hipLaunchKernelGGL(fcgk::k_bt_edge_mom<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, p.Dmin);
@@ -653,7 +656,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
     DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
-    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.fl_p1, 4 * (size_t)D) DA(s.qsr, N) DA(s.qns, N) DA(s.emp, N) DA(s.mslp, N) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
+    DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.fl_p1, 4 * (size_t)D) DA(s.qsr, N) DA(s.qns, N) DA(s.emp, N) DA(s.mslp, N) DA(s.runoff, N) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
     if (p.key_atm) {
         DA(s.rho_c, (size_t)N * L)
     }
@@ -1196,6 +1199,7 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCHLV(ctx, k_tr_gradn_miura<R>, m.N, m.L, m, s);
         LAUNCHLV(ctx, k_tr_miura<R>, m.N, m.L, m, s, p.dt, p.Dmin, ctx->scale_area, relax, p.clim_relax, p.relax2clim_ac);
     }
+    if (p.key_rivers) LAUNCH1(ctx, k_tr_rivers<R>, m.N, m, s, p.dt);
     AtmPar ap;
     ap.on = p.key_atm;
     ap.type_swr = p.type_swr_body;

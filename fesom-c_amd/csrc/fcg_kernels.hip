@@ -1105,13 +1105,14 @@ __global__ __launch_bounds__(256) void k_bt_edge_ssh(DevMesh m, DevStateT<R> s) 
     s.fl_ssh[ed] = c1;
 }
 
-// ssh_rhs gather (fv_average_dynamic.F90:106-169), eta update (fv_main.F90:1432-1435),
+// ssh_rhs gather (fv_average_dynamic.F90:106-169) with the river discharge of
+// the mouths (177-183; runoff null without rivers), eta update (fv_main.F90:1432-1435),
 // tidal OB value (fv_average_dynamic.F90:190-216, host-evaluated), the AM4
 // elevation of update_2D_vel (fv_average_dynamic.F90:288-296) and the time-level
 // shift (fv_main.F90:1547-1551).
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, double dt_2D, const double *__restrict__ tide,
-                                                 int atm, double d0i) {
+                                                 int atm, double d0i, const double *__restrict__ runoff) {
     int n = TID();
     if (n >= m.N) return;
     double ssh = 0.0;
@@ -1120,6 +1121,10 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, doub
         const int c = m.n2e[j];
         const double c1 = s.fl_ssh[c >= 0 ? c : ~c];
         ssh = (c >= 0) ? ssh + c1 : ssh - c1;
+    }
+    if (runoff) {
+        const double q = runoff[n];
+        if (q != 0.0) ssh = ssh + q;
     }
     s.ssh_rhs[n] = ssh;
     const double e0 = s.eta_n[n], e1 = s.eta_n_1[n], e2 = s.eta_n_2[n];
@@ -1745,6 +1750,26 @@ __global__ __launch_bounds__(256) void k_tr_ab(DevMesh m, DevStateT<R> s, size_t
     const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
     s.abT[i] = -(0.5 + kEps) * s.T_old[i] + (1.5 + kEps) * s.TF[i] - Ta;
     s.abS[i] = -(0.5 + kEps) * s.S_old[i] + (1.5 + kEps) * s.SF[i] - Sa;
+}
+
+// river water of salinity 0.2 diluting the salinity of its mouth's column
+// (fv_main.F90:1805-1822), after the advection and before the implicit
+// vertical diffusion: SF += (0.2 - S_old) * Q dsig dt / (area Jc), Jc =
+// a dsig with a = max(depth+eta, Dmin) of this step; one thread per node,
+// nodes without discharge return at once
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_rivers(DevMesh m, DevStateT<R> s, double dt) {
+    const int n = TID();
+    if (n >= m.N) return;
+    const double q = s.runoff[n];
+    if (q == 0.0) return;
+    const double a = s.a[n], ar = m.area[n];
+    for (int k = 0; k < m.L; ++k) {
+        const size_t i = (size_t)n * m.K + k;
+        const double coeff = q * m.dsig[k] * dt / (ar * (a * m.dsig[k]));
+        const double delta = (double)0.2f - (double)s.S_old[i];
+        s.SF[i] = (R)((double)s.SF[i] + delta * coeff);
+    }
 }
 
 // QUICK vertical flux through 1-based interface nz of one node column
@@ -2884,6 +2909,7 @@ void instantiate_kernels() {
     (void)&k_vertvel_scan<R>;
     (void)&k_minmax_partial<R>;
     (void)&k_tr_ab<R>;
+    (void)&k_tr_rivers<R>;
     (void)&k_tr_upwind<R>;
     (void)&k_tr_gradn_muscl<R>;
     (void)&k_tr_gradn_miura<R>;

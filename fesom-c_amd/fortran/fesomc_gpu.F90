@@ -26,6 +26,7 @@ module fesomc_gpu
           im_vert_visc_fic, fic_point, wet_dry_on, tf_presence, restart, cl_relax, precision, closure, &
           key_atm, type_swr_body
      real(c_double) :: Rjc, ajc, bjc
+     integer(c_int) :: key_rivers
   end type fcg_params
 
   ! the o_MESH arrays after the reference's preprocessing (pointers into host memory)

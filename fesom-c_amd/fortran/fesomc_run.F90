@@ -117,6 +117,7 @@ program fesomc_run
   par%Rjc = getf('R', 0.58d0)
   par%ajc = getf('a', 0.35d0)
   par%bjc = getf('b', 23.0d0)
+  par%key_rivers = merge(1, 0, getl('key_rivers'))
 
   ! ---- read_mesh_ser (src/fv_main.F90:787-945) -----------------------------
   meshdir = trim(rundir) // '/' // trim(gets('MESHPATH', './MESH/'))

@@ -44,7 +44,7 @@ class Params(C.Structure):
                [(n, C.c_int) for n in ("nsigma", "Mt", "tracer_adv", "muscl_call", "mom_adv_2D", "mom_adv_3D",
                                        "vert_adv", "im_vert_visc_fic", "fic_point", "wet_dry_on", "tf_presence",
                                        "restart", "cl_relax", "precision", "closure", "key_atm", "type_swr_body")] + \
-               [(n, C.c_double) for n in ("Rjc", "ajc", "bjc")]
+               [(n, C.c_double) for n in ("Rjc", "ajc", "bjc")] + [("key_rivers", C.c_int)]
 
 
 _IP = C.POINTER(C.c_int)
@@ -210,6 +210,7 @@ def params(cfg: RunConfig, precision=FCG_FP64, closure=FCG_CLOSURE_D3END) -> Par
     p.precision, p.closure = precision, closure
     p.key_atm, p.type_swr_body = int(cfg.l("key_atm")), cfg.i("type_swr_body", 2)
     p.Rjc, p.ajc, p.bjc = cfg.f("R", 0.58), cfg.f("a", 0.35), cfg.f("b", 23.0)
+    p.key_rivers = int(cfg.l("key_rivers"))
     return p
 
 
@@ -315,7 +316,8 @@ class Model:
 
     def _count(self, name):
         N, E, L, K = self.N, self.E, self.L, self.K
-        node1 = {"eta_n", "eta_n_1", "eta_n_2", "etaAB", "ssh_rhs", "windx", "windy", "qsr", "qns", "emp", "mslp"}
+        node1 = {"eta_n", "eta_n_1", "eta_n_2", "etaAB", "ssh_rhs", "windx", "windy", "qsr", "qns", "emp", "mslp",
+                 "runoff"}
         el1 = {"U_filt_2D", "V_filt_2D", "mask_wd", "taux", "tauy", "C_d_el", "dmean_n"}
         el2 = {"U_n_2D", "U_n_1", "U_n_2", "UAB", "U_rhs_2D_3D", "Bar_pr_2D"}
         nodeL = {"TF", "SF", "T_old", "S_old", "Tclim", "Sclim", "Unode_p", "Vnode_p", "Unode", "Vnode", "Jc", "Jc_old",

@@ -75,6 +75,12 @@ typedef struct fcg_params {
     int key_atm;
     int type_swr_body;         /* 1 Paulson-Simpson (constant R, a, b), 2 Soloviev */
     double Rjc, ajc, bjc;      /* fv_run.dat keys R, a, b (jer_const = T)    */
+    /* rivers (key_rivers): the field "runoff" (m3/s per node, 0 away from
+     * river mouths) is set by the host from its own river table; the step
+     * adds it to the elevation's right-hand side (fv_average_dynamic.F90:
+     * 177-183) and dilutes the mouth's salinity towards 0.2
+     * (fv_main.F90:1805-1822) */
+    int key_rivers;
 } fcg_params;
 
 /*

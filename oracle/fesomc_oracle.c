@@ -101,6 +101,7 @@ typedef struct {
     double *TF, *T_old, *SF, *S_old, *Tclim, *Sclim; /* LN */
     double *mask_wd, *mask_wd_node, *mask_ad, *mask_bpr, *C_d_el, *relax2clim; /* E N N E E N */
     double *qsr, *qns, *emp, *mslp; /* N: surface fluxes and pressure (key_atm; set by the host) */
+    double *runoff; /* N: river discharge [m3/s] at river nodes, 0 elsewhere (key_rivers; set by the host) */
     int status; /* 0 ok; nonzero = reference STOP condition */
     char errmsg[256];
 } orc_t;
@@ -1176,7 +1177,9 @@ static void compute_vel_rhs(orc_t *o) {
 /* barotropic substep                                                    */
 /* ------------------------------------------------------------------ */
 
-/* compute_ssh_rhs_elem: src/fv_average_dynamic.F90:1-226 (no ice, no rivers) */
+/* compute_ssh_rhs_elem: src/fv_average_dynamic.F90:1-226 (no ice); river
+ * runoff (177-183) at the nodes the host gives a nonzero runoff (one river
+ * per node) */
 static void compute_ssh_rhs_elem(orc_t *o) {
     int E = o->E, D = o->D;
     double *c1 = DALLOC(D), *aux = DALLOC(E);
@@ -1206,6 +1209,9 @@ static void compute_ssh_rhs_elem(orc_t *o) {
         o->ssh_rhs[o->edge_nodes[2 * ed]] = o->ssh_rhs[o->edge_nodes[2 * ed]] + c1[ed];
         o->ssh_rhs[o->edge_nodes[2 * ed + 1]] = o->ssh_rhs[o->edge_nodes[2 * ed + 1]] - c1[ed];
     }
+    if (o->key_rivers)
+        for (int n = 0; n < o->N; n++)
+            if (o->runoff[n] != 0.0) o->ssh_rhs[n] = o->ssh_rhs[n] + o->runoff[n];
     if (o->TF_presence && o->nobn > 0) {
         for (int i = 0; i < o->nobn; i++) o->ssh_rhs[o->in_obn[i]] = 0.0;
         for (int i = 0; i < o->nobn; i++) {
@@ -2082,6 +2088,18 @@ static int solve_tracers(orc_t *o) {
     } else if (o->tracer_adv == 4) {
         solve_tracer_miura(o, o->TF, o->T_old, o->Tclim, 't');
     }
+    if (o->key_rivers) {  /* river water (S = 0.2) diluting its node: src/fv_main.F90:1805-1822 */
+        int L = o->L;
+        for (int n = 0; n < o->N; n++) {
+            if (o->runoff[n] == 0.0) continue;
+            for (int nz = 0; nz < L; nz++) {
+                size_t i = (size_t)n * L + nz;
+                double river_coeff = o->runoff[n] * (o->sigma[nz] - o->sigma[nz + 1]) * o->dt / (o->area[n] * o->Jc[i]);
+                double river_delta = (F4(0.2) - o->S_old[i]);
+                o->SF[i] = o->SF[i] + river_delta * river_coeff;
+            }
+        }
+    }
     return tracer_impl_vert_diff(o);
 }
 
@@ -2224,7 +2242,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     parse_config(o, cfg);
     if (o->type_task != 3) { o->status = -10; snprintf(o->errmsg, sizeof o->errmsg, "only type_task=3"); return o; }
     if (o->filt_2D || o->filt_bi_2D || o->bih_2D || o->filt_3D || o->filt_bi_3D || o->bih_3D || o->mixed_BP_grad_OB
-        || (o->key_atm && o->type_swr_body == 1 && !o->jer_const) || o->key_rivers || o->T_potential || o->ver_mix != 3
+        || (o->key_atm && o->type_swr_body == 1 && !o->jer_const) || o->T_potential || o->ver_mix != 3
         || (o->SL_obn && !(o->SL_radius > 0.0))) {
         o->status = -14;
         snprintf(o->errmsg, sizeof o->errmsg, "configuration uses an option outside the restated hot path");
@@ -2284,7 +2302,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     o->TF = DALLOC(NL); o->T_old = DALLOC(NL); o->SF = DALLOC(NL); o->S_old = DALLOC(NL);
     o->Tclim = DALLOC(NL); o->Sclim = DALLOC(NL);
     o->mask_wd = DALLOC(E); o->mask_wd_node = DALLOC(N); o->mask_ad = DALLOC(N); o->mask_bpr = DALLOC(E);
-    o->qsr = DALLOC(N); o->qns = DALLOC(N); o->emp = DALLOC(N); o->mslp = DALLOC(N);
+    o->qsr = DALLOC(N); o->qns = DALLOC(N); o->emp = DALLOC(N); o->mslp = DALLOC(N); o->runoff = DALLOC(N);
     o->C_d_el = DALLOC(E); o->relax2clim = DALLOC(N); o->ac = DALLOC(N); o->a_tp = DALLOC(2 * (size_t)N);
     for (size_t i = 0; i < NK; i++) { o->snu[i] = o->snul; o->bt[i] = 1.0e-8; o->Kv[i] = o->snul; }
     for (size_t i = 0; i < EK; i++) o->Av[i] = o->snul;
@@ -2448,7 +2466,7 @@ static double *fld(orc_t *o, const char *name, long *cnt) {
     F("vel_grad_ux", o->vel_grad_ux, E * L) F("vel_grad_uy", o->vel_grad_uy, E * L)
     F("vel_grad_vx", o->vel_grad_vx, E * L) F("vel_grad_vy", o->vel_grad_vy, E * L)
     F("Unode_p", o->Unode_p, N * L) F("Vnode_p", o->Vnode_p, N * L) F("a_tp", o->a_tp, 2 * N)
-    F("qsr", o->qsr, N) F("qns", o->qns, N) F("emp", o->emp, N) F("mslp", o->mslp, N)
+    F("qsr", o->qsr, N) F("qns", o->qns, N) F("emp", o->emp, N) F("mslp", o->mslp, N) F("runoff", o->runoff, N)
     F("ampt", o->ampt, 12L * (o->nobn > 0 ? o->nobn : 1)) F("fazt", o->fazt, 12L * (o->nobn > 0 ? o->nobn : 1))
 #undef F
     *cnt = -1;

@@ -192,3 +192,29 @@ def test_surface_forcing_matches_oracle(swr):
     _compare(g, o, FIELDS + ["rho_c"])
     assert g.diagnostics()["status"] == 0
     g.close()
+
+
+def test_rivers_match_oracle():
+    """key_rivers on the stratified wet/dry channel: discharge at three
+    interior nodes adds to their elevation's right-hand side
+    (fv_average_dynamic.F90:177-183) and dilutes their salinity towards 0.2
+    (fv_main.F90:1805-1822): bit for bit (1e-12) against the oracle."""
+    import re
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    txt = re.sub(r"^(\s*key_rivers\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    g, o = _model(txt, mesh, ex)
+    depth = o.get("depth")
+    mouths = np.argsort(np.where(o.get("ac") >= 1.0, depth, -np.inf))[-3:]
+    r = np.zeros(mesh.nod2D)
+    r[mouths] = (50.0, 20.0, 5.0)
+    g.set("runoff", r)
+    o.set("runoff", r)
+    S0 = o.get("SF").copy()
+    g.step(5)
+    o.step(5)
+    _compare(g, o, FIELDS)
+    assert g.diagnostics()["status"] == 0
+    L = o.size["L"]
+    assert (o.get("SF").reshape(-1, L)[mouths] < S0.reshape(-1, L)[mouths]).all()
+    g.close()

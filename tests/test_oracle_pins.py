@@ -247,3 +247,69 @@ def test_uniform_air_pressure_exerts_no_force(le_case):
     o1.step(3)
     u0, u1 = o0.get("U_n"), o1.get("U_n")
     assert np.abs(u1 - u0).max() <= 1e-9 * np.abs(u0).max()
+
+
+def test_river_runoff_closes_the_volume_budget(le_case):
+    """A river of Q m3/s at one node of the closed LE basin raises the total
+    volume sum(area*(depth+eta)) by Q per barotropic second: Mt*dt_2D*Q over
+    one internal step (compute_ssh_rhs_elem, src/fv_average_dynamic.F90:
+    177-183; the edge fluxes cancel in the sum)."""
+    import re
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh = le_case
+    txt = re.sub(r"^(\s*key_rivers\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    cfg = RunConfig(txt)
+    Q = 5.0
+    o = Oracle(txt, mesh)
+    N = o.size["N"]
+    r = np.zeros(N)
+    r[N // 3] = Q
+    o.set("runoff", r)
+    area = o.get("area")
+    e0 = o.get("eta_n").copy()
+    o.step(1)
+    dv = float((area * (o.get("eta_n") - e0)).sum())
+    assert dv == pytest.approx(o.size["Mt"] * cfg.f("dt_2D") * Q, rel=1e-9)
+    # Without the river the basin keeps its volume: the term is the river's.
+    o2 = Oracle(txt, mesh)
+    e0 = o2.get("eta_n").copy()
+    o2.step(1)
+    assert abs(float((area * (o2.get("eta_n") - e0)).sum())) < 1e-9 * o.size["Mt"] * cfg.f("dt_2D") * Q
+
+
+def test_river_water_dilutes_its_node():
+    """A river of Q m3/s at one wet interior node of the stratified channel
+    brings Q*dt m3 of water of salinity 0.2 per step: the salt content
+    sum(area*Jc*S) gains Q*dt*0.2 against the same step without it.  The
+    advective-form transport keeps the node's concentration while its volume
+    grows by Q*dt (+Q*dt*S_col, S_col the layer-thickness-weighted column
+    mean), and the dilution term of src/fv_main.F90:1805-1822 takes
+    Q*dt*(S_col - 0.2) back.  A dropped dilution term leaves +Q*dt*S_col
+    (~150 times more), a wrong sign or a lost dt, area or Jc factor misses by
+    far more than the tolerance."""
+    import re
+    from fesomc_amd import meshgen
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, ex = meshgen.config("c2_small")
+    txt = re.sub(r"^(\s*key_rivers\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    Q = 50.0
+    dS, node = [], None
+    for q in (0.0, Q):
+        o = Oracle(txt, mesh)
+        o.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
+        N, L = o.size["N"], o.size["L"]
+        if node is None:
+            depth = o.get("depth")
+            node = int(np.argmax(np.where(o.get("ac") >= 1.0, depth, -np.inf)))
+        r = np.zeros(N)
+        r[node] = q
+        o.set("runoff", r)
+        S0 = o.get("SF").reshape(N, L)[node].copy()
+        s0 = _content(o, "SF")
+        o.step(1)
+        dS.append(_content(o, "SF") - s0)
+    dsig = -np.diff(o.get("sigma"))
+    S_col = float((S0 * dsig).sum() / dsig.sum())
+    Qdt = Q * RunConfig(txt).f("dt")
+    assert S_col > 10.0
+    assert dS[1] - dS[0] == pytest.approx(Qdt * 0.2, rel=0.05)
