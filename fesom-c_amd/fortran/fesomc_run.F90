@@ -33,7 +33,13 @@ program fesomc_run
   integer :: nod2D, elem2D, nobn, n_harm
   real(c_double), allocatable, target :: xy(:, :), depth(:), ob(:, :), T(:, :), S(:, :)
   integer(c_int), allocatable :: ind(:), elem(:, :)
-  real(c_double) :: time
+  real(c_double) :: time, time_jd
+  ! river table (src/fv_rivers.F90): records, mouths, interpolation state
+  integer :: ntime_riv = 0, nriv = 0, t_riv = 0, t_riv_p1 = 0
+  logical :: one_field_riv = .false.
+  real(c_double), allocatable :: time_riv(:), data_riv(:, :), coef_a_riv(:), coef_b_riv(:), runoff_riv(:), &
+       runoff_node(:)
+  integer, allocatable :: nodes_riv(:)
   character(len=3), parameter :: harm(12) = [character(len=3) :: 'M2', 'S2', 'N2', 'K2', 'K1', 'O1', &
        'P1', 'Q1', 'Mf', 'Mm', 'Ssa', 'M4']
 
@@ -171,15 +177,19 @@ program fesomc_run
   call put('SF', S)
   call put('S_old', S)
   call put('Sclim', S)
+  time_jd = par%time_jd0                                            ! fv_main.F90:326
   if (par%restart == 1) then          ! read_restart_separate (src/fv_restart.F90:507-613), fv_main.F90:336
      call read_restart('restarts/restart_0.out')
      nsteps = nsteps - ini_time
   end if
+  if (par%key_rivers == 1) call rivers_ini(trim(meshdir) // '/rivers.dat')   ! fv_main.F90:352
 
   ! ---- main loop (src/fv_main.F90:389-746) ---------------------------------
   open(53, file='energy.dat')
   do n_dt = 1, nsteps
      call fcg_check(fcg_advance_clock(ctx), ctx, 'clock')            ! fv_main.F90:400-401
+     time_jd = time_jd + par%dt / 86400.0d0                            ! the same clock, kept by the host
+     if (par%key_rivers == 1) call rivers_do()                         ! fv_main.F90:412
      call fcg_check(fcg_pressure(ctx), ctx, 'pressure')              ! fv_main.F90:1996
      call fcg_check(fcg_vertical_mixing(ctx), ctx, 'd3_end')         ! fv_main.F90:2006, 2025
      call fcg_check(fcg_compute_vel_rhs(ctx), ctx, 'compute_vel_rhs') ! fv_main.F90:2034
@@ -301,8 +311,109 @@ contains
     call put('TF', tf); call put('SF', sf); call put('T_old', to); call put('S_old', so)
     call put('Tclim', tc); call put('Sclim', sc)
     call fcg_check(fcg_hot_start(ctx, tjd), ctx, 'fcg_hot_start')
+    time_jd = tjd
     write(*,*) 'Restart reading finished, time_jd= ', tjd, ',time_jd0=', par%time_jd0
   end subroutine read_restart
+
+  ! rivers_ini (src/fv_rivers.F90:75-214), serial: the table read
+  ! list-directed -- Ntime, Nnods, the record times (julian days), the mouth
+  ! nodes, then one record of Nnods discharges [m3/s] per time -- and no
+  ! discharge until the first rivers_do.
+  subroutine rivers_ini(path)
+    character(len=*), intent(in) :: path
+    integer :: i, iost
+    open(27, file=path, status='old', action='read', iostat=iost)
+    if (iost /= 0) then
+       write(*,*) 'ERROR: --> bad opening file   : ', trim(path), ' ; iostat=', iost
+       stop 1
+    end if
+    read(27, *) ntime_riv
+    read(27, *) nriv
+    allocate(time_riv(ntime_riv), data_riv(ntime_riv, nriv), nodes_riv(nriv), coef_a_riv(nriv), &
+             coef_b_riv(nriv), runoff_riv(nriv), runoff_node(nod2D))
+    read(27, *) time_riv
+    read(27, *) nodes_riv
+    do i = 1, ntime_riv
+       read(27, *) data_riv(i, :)
+    end do
+    close(27)
+    if (nriv > 0) then
+       if (minval(nodes_riv) < 1 .or. maxval(nodes_riv) > nod2D) then
+          write(*,*) 'ERROR: rivers.dat: mouth node outside 1..nod2D'
+          stop 1
+       end if
+    end if
+    runoff_riv = 0.0d0
+    if (nriv > 0) call rivers_coeff(time_jd)
+    write(*,*) 'rivers: ', nriv, ' mouths, ', ntime_riv, ' records'
+  end subroutine rivers_ini
+
+  ! getcoeffld (src/fv_rivers.F90:216-261): the straight line through the
+  ! records that bracket rdate; past the last record its value, after which
+  ! the discharge is no longer updated (one_field).
+  subroutine rivers_coeff(rdate)
+    real(c_double), intent(in) :: rdate
+    real(c_double) :: delta
+    t_riv = rivers_search(rdate)
+    if (t_riv == 0) then
+       write(*,*) 'ERROR: model time ', rdate, ' before the first river record ', time_riv(1)
+       stop 1
+    end if
+    if (t_riv < ntime_riv) then
+       t_riv_p1 = t_riv + 1
+       delta = time_riv(t_riv_p1) - time_riv(t_riv)
+    else
+       t_riv_p1 = t_riv
+       delta = 1.0d0
+       one_field_riv = .true.
+    end if
+    if (rdate < time_riv(1)) then
+       t_riv_p1 = t_riv
+       delta = 1.0d0
+    end if
+    coef_a_riv = (data_riv(t_riv_p1, :) - data_riv(t_riv, :)) / delta
+    coef_b_riv = data_riv(t_riv, :) - coef_a_riv * time_riv(t_riv)
+  end subroutine rivers_coeff
+
+  ! binarysearch (src/fv_rivers.F90:305-350): the record within 1e-9 (a
+  ! REAL(4) constant) of value, else the last one before it, 0 if none
+  integer function rivers_search(value) result(ind)
+    real(c_double), intent(in) :: value
+    real(c_double) :: tol
+    integer :: lo, hi, mid
+    tol = real(1.0e-9, c_double)
+    lo = 1
+    hi = ntime_riv
+    do while (lo <= hi)
+       mid = (lo + hi + 1) / 2
+       if (abs(time_riv(mid) - value) <= tol) then
+          ind = mid
+          return
+       else if (time_riv(mid) > value) then
+          hi = mid - 1
+       else
+          lo = mid + 1
+       end if
+    end do
+    ind = hi
+  end function rivers_search
+
+  ! rivers_do (src/fv_rivers.F90:263-300) at the advanced clock, ramped over
+  ! the first ten days; the per-node sum goes to the step as "runoff"
+  subroutine rivers_do()
+    integer :: j
+    if (nriv == 0) return
+    if (.not. one_field_riv) then
+       if (time_jd > time_riv(t_riv_p1)) call rivers_coeff(time_jd)
+       runoff_riv = time_jd * coef_a_riv + coef_b_riv
+       if (time_jd - par%time_jd0 < 10.0d0) runoff_riv = runoff_riv * (time_jd - par%time_jd0) / 10.0d0
+    end if
+    runoff_node = 0.0d0
+    do j = 1, nriv
+       runoff_node(nodes_riv(j)) = runoff_node(nodes_riv(j)) + runoff_riv(j)
+    end do
+    call put1('runoff', runoff_node)
+  end subroutine rivers_do
 
   subroutine get1(name, a)
     character(len=*), intent(in) :: name
