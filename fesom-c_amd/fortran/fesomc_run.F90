@@ -66,6 +66,7 @@ program fesomc_run
   irep = geti('IREP', 100)
   irestart = geti('IRESTART', huge(0))
   iverbosity = geti('VERBOSITY', 1)   ! src/fv_read_run.F90:149-155
+  call refuse_unsupported()
 
   mc%cartesian = merge(1, 0, getl('cartesian'))
   mc%coriolis_tf = merge(1, 0, getl('Coriolis_TF'))
@@ -518,6 +519,35 @@ contains
     geti = def
     if (has(key)) read(gets(key, ''), *) geti
   end function geti
+
+  ! Options whose code this build does not hold stop the run before any set-up
+  ! (the same list as the Python host's RunConfig.unsupported).
+  subroutine refuse_unsupported()
+    character(len=20), parameter :: opts(9) = [character(len=20) :: 'filt_2D', 'filt_bi_2D', 'bih_2D', &
+         'filt_3D', 'filt_bi_3D', 'bih_3D', 'mixed_BP_grad_OB', 'T_potential', 'comp_sediment']
+    integer :: j
+    logical :: bad
+    bad = .false.
+    do j = 1, size(opts)
+       if (getl(trim(opts(j)))) then
+          write(*,*) 'ERROR: option not available in this build: ', trim(opts(j))
+          bad = .true.
+       end if
+    end do
+    if (geti('Vert_turb', 3) /= 3) then
+       write(*,*) 'ERROR: option not available in this build: Vert_turb /= 3'
+       bad = .true.
+    end if
+    if (getl('key_atm') .and. geti('type_swr_body', 2) == 1 .and. .not. getl('jer_const', .true.)) then
+       write(*,*) 'ERROR: option not available in this build: jer_const = F (per-node Jerlov coefficients)'
+       bad = .true.
+    end if
+    if (.not. (getl('SALINITY_ON') .or. getl('TEMP_ON'))) then          ! type_task 3 (fv_read_run.F90)
+       write(*,*) 'ERROR: option not available in this build: type_task /= 3 (baroclinic with T or S)'
+       bad = .true.
+    end if
+    if (bad) stop 2
+  end subroutine refuse_unsupported
 
   logical function getl(key, def)
     character(len=*), intent(in) :: key

@@ -157,3 +157,15 @@ def test_driver_reads_and_applies_the_river_table(tmp_path):
     for label, a in want.items():
         got = _block_values(r.stdout, label)[-1]
         assert got == pytest.approx([a.max(), a.min()], rel=1e-12, abs=1e-15), label
+
+
+def test_driver_refuses_options_it_does_not_hold(tmp_path):
+    """A horizontal filter in fv_run.dat stops the driver before any set-up,
+    naming the option (no GPU: the check precedes the mesh)."""
+    d = _le_rundir(tmp_path)
+    import re
+    txt, n = re.subn(r"^(\s*filt_2D\s*=\s*)\S+", r"\g<1>T", (d / "fv_run.dat").read_text(), count=1, flags=re.M)
+    assert n == 1
+    (d / "fv_run.dat").write_text(txt)
+    r = _run([d, 1, "--mesh-only"], d)
+    assert r.returncode == 2 and "filt_2D" in r.stdout
