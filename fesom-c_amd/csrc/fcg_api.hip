@@ -21,6 +21,9 @@ template <typename R> __global__ void k_d3end_fin(DevMesh, DevStateT<R>, double,
 template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int, double);
 template <typename R> __global__ void k_velrhs_atm(DevMesh, DevStateT<R>, double, double, int, int, double);
+template <typename R> __global__ void k_velrhs_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
+template <typename R>
+__global__ void k_velrhs_atm_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *, int, double,
@@ -32,8 +35,9 @@ template <typename R> __global__ void k_bt_puls(DevMesh, DevStateT<R>, double);
 template <typename R, bool P>
 __global__ void k_el2nodes(DevMesh, const R *, const R *, R *, R *, const double *, const double *);
 template <typename R> __global__ void k_bt_p1edge(DevMesh, DevStateT<R>);
-template <typename R> __global__ void k_bt_p1elem(DevMesh, DevStateT<R>);
-template <typename R> __global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int);
+template <typename R> __global__ void k_bt_p1elem(DevMesh, DevStateT<R>, double);
+template <typename R>
+__global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int, double);
 template <typename R> __global__ void k_jac_node(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_jac_restart(DevMesh, DevStateT<R>, double);
@@ -295,10 +299,12 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
                                m, (const R *)s.U_n, (const R *)s.V_n, s.Unode_p, s.Vnode_p, (const double *)s.puls_ap,
                                (const double *)s.U2);
             hipLaunchKernelGGL(fcgk::k_bt_p1edge<R>, dim3(nflatblk(m.D, m.L)), dim3(kFlatT), lds_p1edge(m), st, m, s);
-            hipLaunchKernelGGL(fcgk::k_bt_p1elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+            hipLaunchKernelGGL(fcgk::k_bt_p1elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s,
+                               p.filt_3D ? p.tau_inv_filt : 0.0);
         }
         hipLaunchKernelGGL(fcgk::k_bt_elem2<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
-                           ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0);
+                           ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0,
+                           p.filt_2D ? p.tau_inv_filt : 0.0);
     }
     return FCG_OK;
 }
@@ -1070,7 +1076,15 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     }
     LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec), ctx->m, s);
     const int vt = block_of(ctx, "k_velrhs");
-    if (p.key_atm)
+    if (p.filt_3D && p.key_atm)
+        LAUNCH_EX(ctx, k_velrhs_atm_filt<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt),
+                  ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, 1.0 / ctx->density_0,
+                  p.tau_inv_filt);
+    else if (p.filt_3D)
+        LAUNCH_EX(ctx, k_velrhs_filt<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt),
+                  ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, 1.0 / ctx->density_0,
+                  p.tau_inv_filt);
+    else if (p.key_atm)
         LAUNCH_EX(ctx, k_velrhs_atm<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt),
                   ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, 1.0 / ctx->density_0);
     else

@@ -742,9 +742,9 @@ static_assert(sizeof(EdgeRec<double>) == velrhs_edge_bytes(8) && sizeof(EdgeRec<
 // layer's two interfaces (354-469) and the explicit update (124-143).  Writes
 // the new AB history and the provisional velocity into U_rhs/V_rhs;
 // k_velvisc then applies the implicit vertical viscosity column by column.
-template <typename R, bool ATM>
+template <typename R, bool ATM, bool FILT>
 __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
-                                            double d0i) {
+                                            double d0i, double tau) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int L = m.L, K = m.K, KC = K;
     const ColFlat f = col_flat(m.E, L);
@@ -838,6 +838,21 @@ __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt
     const double u = cu[c * L + k], v = cv[c * L + k];
     double ur = -(0.5 + kEps) * PL(s.U_rhsAB, k, e, KC) + r.Fx * je;
     double vr = -(0.5 + kEps) * PL(s.V_rhsAB, k, e, KC) + r.Fy * je;
+    if (FILT) {  // viscosity_filt_3D (fv_viscosity.F90:822-868): neighbours minus own U_n, elem_neighbors order
+        const int4 q4 = m.en[e];
+        const int4 nb4 = m.elnb[e];
+        const int nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        const int jend = (q4.x == q4.w) ? 3 : 4;
+        double fu = 0.0, fv = 0.0;
+        for (int j = 0; j < jend; ++j) {
+            if (nbs[j] >= 0) {
+                fu = fu + (double)PL(s.U_n, k, nbs[j], KC) - u;
+                fv = fv + (double)PL(s.V_n, k, nbs[j], KC) - v;
+            }
+        }
+        ur = ur + tau * fu * area * je;
+        vr = vr + tau * fv * area * je;
+    }
     R abu = (R)(v * r.cor * area * je);
     R abv = (R)(-u * r.cor * area * je);
     if (hadv) {
@@ -934,13 +949,24 @@ __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt
 template <typename R>
 __global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
                                                   double d0i) {
-    velrhs_body<R, false>(m, s, dt, ff, hadv, vadv, d0i);
+    velrhs_body<R, false, false>(m, s, dt, ff, hadv, vadv, d0i, 0.0);
 }
 // the same with surface forcing (key_atm)
 template <typename R>
 __global__ __launch_bounds__(1024) void k_velrhs_atm(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
                                                       int vadv, double d0i) {
-    velrhs_body<R, true>(m, s, dt, ff, hadv, vadv, d0i);
+    velrhs_body<R, true, false>(m, s, dt, ff, hadv, vadv, d0i, 0.0);
+}
+// the same with the neighbour-difference filter (filt_3D), without and with surface forcing
+template <typename R>
+__global__ __launch_bounds__(1024) void k_velrhs_filt(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
+                                                       int vadv, double d0i, double tau) {
+    velrhs_body<R, false, true>(m, s, dt, ff, hadv, vadv, d0i, tau);
+}
+template <typename R>
+__global__ __launch_bounds__(1024) void k_velrhs_atm_filt(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
+                                                           int vadv, double d0i, double tau) {
+    velrhs_body<R, true, true>(m, s, dt, ff, hadv, vadv, d0i, tau);
 }
 
 // momentum_vert_impl_visc_tmp (fv_viscosity.F90:1374-1486) on tiles of 64
@@ -1330,7 +1356,8 @@ __global__ __launch_bounds__(256) void k_bt_p1edge(DevMesh m, DevStateT<R> s) {
 // element pass: the element's internal edges in ascending id, each term with
 // the edge's ac weight, then U_rhs_2D_3D /= dmean_n (fv_main.F90:1504-1513)
 template <typename R>
-__global__ __launch_bounds__(256) void k_bt_p1elem(DevMesh m, DevStateT<R> s) {
+__global__ __launch_bounds__(256) void k_bt_p1elem(DevMesh m, DevStateT<R> s, double tau) {
+    const int KC = m.K;
     const int e = TID();
     if (e >= m.E) return;
     const int4 ied = m.el_ied[e];
@@ -1351,6 +1378,32 @@ __global__ __launch_bounds__(256) void k_bt_p1elem(DevMesh m, DevStateT<R> s) {
             const double2 v = fl[2 * (size_t)ed];
             ru = ru - acc * v.x;
             rv = rv - acc * v.y;
+        }
+    }
+    if (tau != 0.0) {  // filt_3D: viscosity_filt_3D_to_2D (fv_viscosity.F90:870-939, fv_main.F90:1499)
+        // on the pulsation transports U_puls*Je, U_puls = U_n - puls_ap*U_n_2D
+        // (compute_puls_vel), layer by layer, added to the column sum in order
+        const int4 q4 = m.en[e];
+        const int4 nb4 = m.elnb[e];
+        const int nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        const int jend = (q4.x == q4.w) ? 3 : 4;
+        const double ao = s.puls_ap[e], uo2 = s.U2[2 * e], vo2 = s.U2[2 * e + 1];
+        const double area = m.elem_area[e];
+        for (int k = 0; k < m.L; ++k) {
+            const double jo = (double)PL(s.Je, k, e, KC);
+            const double po = ((double)PL(s.U_n, k, e, KC) - ao * uo2) * jo;
+            const double qo = ((double)PL(s.V_n, k, e, KC) - ao * vo2) * jo;
+            double fu = 0.0, fv = 0.0;
+            for (int j = 0; j < jend; ++j) {
+                const int nb = nbs[j];
+                if (nb >= 0) {
+                    const double an = s.puls_ap[nb], jn = (double)PL(s.Je, k, nb, KC);
+                    fu = fu + ((double)PL(s.U_n, k, nb, KC) - an * s.U2[2 * nb]) * jn - po;
+                    fv = fv + ((double)PL(s.V_n, k, nb, KC) - an * s.U2[2 * nb + 1]) * jn - qo;
+                }
+            }
+            ru = ru + tau * fu * area * jo;
+            rv = rv + tau * fv * area * jo;
         }
     }
     s.U_rhs_2D_3D[2 * e] = ru / s.dmean_n[e];
@@ -1398,7 +1451,7 @@ __global__ __launch_bounds__(256) void k_bt_edge_mom(DevMesh m, DevStateT<R> s, 
 // followed by the velocity shift and U_filt accumulation (fv_main.F90:1569-1606).
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
-                           int wet_dry, int adv2d, int first) {
+                           int wet_dry, int adv2d, int first, double tau2d) {
     const int KC = m.K;
     int e = TID();
     if (e >= m.E) return;
@@ -1440,6 +1493,21 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
                 rv = rv - f.y;
             }
         }
+    }
+    if (tau2d != 0.0) {  // filt_2D: viscosity_filt_2D (fv_viscosity.F90:769-808, fv_average_dynamic.F90:346)
+        const int4 nb4 = m.elnb[e];
+        const int nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        const int jend = (q.x == q.w) ? 3 : 4;
+        double fu = 0.0, fv = 0.0;
+        for (int j = 0; j < jend; ++j) {
+            if (nbs[j] >= 0) {
+                fu = fu + s.UAB[2 * nbs[j]] - uab;
+                fv = fv + s.UAB[2 * nbs[j] + 1] - vab;
+            }
+        }
+        const double dsum = sum4(w, d0, d1, d2, d3);
+        ru = ru + tau2d * fu * area * dsum;
+        rv = rv + tau2d * fv * area * dsum;
     }
     ru = ru + s.U_rhs_2D_3D[2 * e];
     rv = rv + s.U_rhs_2D_3D[2 * e + 1];
@@ -2909,6 +2977,8 @@ void instantiate_kernels() {
     (void)&k_vertvel_scan<R>;
     (void)&k_minmax_partial<R>;
     (void)&k_tr_ab<R>;
+    (void)&k_velrhs_filt<R>;
+    (void)&k_velrhs_atm_filt<R>;
     (void)&k_tr_rivers<R>;
     (void)&k_tr_upwind<R>;
     (void)&k_tr_gradn_muscl<R>;

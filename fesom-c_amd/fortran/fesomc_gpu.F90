@@ -26,7 +26,8 @@ module fesomc_gpu
           im_vert_visc_fic, fic_point, wet_dry_on, tf_presence, restart, cl_relax, precision, closure, &
           key_atm, type_swr_body
      real(c_double) :: Rjc, ajc, bjc
-     integer(c_int) :: key_rivers
+     integer(c_int) :: key_rivers, filt_2D, filt_3D
+     real(c_double) :: tau_inv_filt
   end type fcg_params
 
   ! the o_MESH arrays after the reference's preprocessing (pointers into host memory)

@@ -44,7 +44,8 @@ class Params(C.Structure):
                [(n, C.c_int) for n in ("nsigma", "Mt", "tracer_adv", "muscl_call", "mom_adv_2D", "mom_adv_3D",
                                        "vert_adv", "im_vert_visc_fic", "fic_point", "wet_dry_on", "tf_presence",
                                        "restart", "cl_relax", "precision", "closure", "key_atm", "type_swr_body")] + \
-               [(n, C.c_double) for n in ("Rjc", "ajc", "bjc")] + [("key_rivers", C.c_int)]
+               [(n, C.c_double) for n in ("Rjc", "ajc", "bjc")] + \
+               [(n, C.c_int) for n in ("key_rivers", "filt_2D", "filt_3D")] + [("tau_inv_filt", C.c_double)]
 
 
 _IP = C.POINTER(C.c_int)
@@ -211,6 +212,8 @@ def params(cfg: RunConfig, precision=FCG_FP64, closure=FCG_CLOSURE_D3END) -> Par
     p.key_atm, p.type_swr_body = int(cfg.l("key_atm")), cfg.i("type_swr_body", 2)
     p.Rjc, p.ajc, p.bjc = cfg.f("R", 0.58), cfg.f("a", 0.35), cfg.f("b", 23.0)
     p.key_rivers = int(cfg.l("key_rivers"))
+    p.filt_2D, p.filt_3D = int(cfg.l("filt_2D")), int(cfg.l("filt_3D"))
+    p.tau_inv_filt = cfg.f("tau_inv_filt", 0.0)
     return p
 
 

@@ -81,6 +81,12 @@ typedef struct fcg_params {
      * 177-183) and dilutes the mouth's salinity towards 0.2
      * (fv_main.F90:1805-1822) */
     int key_rivers;
+    /* neighbour-difference filters (fv_viscosity.F90:769-939): filt_2D on
+     * the barotropic velocity, filt_3D on the 3-D velocity and its
+     * pulsation part in the barotropic forcing, both at the rate
+     * tau_inv_filt [1/s]; the biharmonic and filt_bi options are not built */
+    int filt_2D, filt_3D;
+    double tau_inv_filt;
 } fcg_params;
 
 /*

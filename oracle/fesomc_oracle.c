@@ -52,6 +52,7 @@ typedef struct {
     double dt_2D, dt, time_jd0, Dcr, Dmin, C_d, z0b_min, z0s_min, za, snul;
     double T_const, S_const, lat_cartesian, cyclic_length, clim_relax, relax2clim_ac;
     double Cd_bz, Bp_bz, pow_sig, lev_bot, lev_sur, SL_radius;
+    double tau_inv_filt; /* relaxation rate of the neighbour-difference filters [1/s] */
     int type_swr_body, jer_const;   /* shortwave absorption profile (key_atm) */
     double Rjc, ajc, bjc;
     int nsteps, irep, cartesian, coriolis_tf, mom_adv_2D, mom_adv_3D, tracer_adv, vert_adv;
@@ -1129,7 +1130,99 @@ static void momentum_vert_impl_visc(orc_t *o) {
     free(a); free(b); free(c); free(ur); free(vr); free(cp); free(up); free(vp);
 }
 
-/* compute_vel_rhs: src/fv_3D_vel.F90:1-197 (filters off) */
+/* Neighbour-difference filters (src/fv_viscosity.F90:769-939): each element
+ * relaxes towards its neighbours at the rate tau_inv_filt, summing
+ * (neighbour - own) over its 3 or 4 edges in elem_neighbors order
+ * (boundary edges contribute nothing). */
+
+/* viscosity_filt_2D: src/fv_viscosity.F90:769-808, on UAB into U_rhs_2D */
+static void viscosity_filt_2D(orc_t *o) {
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const int jend = (en[0] == en[3]) ? 3 : 4;
+        double u1 = 0.0, v1 = 0.0;
+        for (int j = 0; j < jend; j++) {
+            const int nb = o->elem_nb[4 * el + j];
+            if (nb >= 0) {
+                u1 = u1 + o->UAB[2 * nb] - o->UAB[2 * el];
+                v1 = v1 + o->UAB[2 * nb + 1] - o->UAB[2 * el + 1];
+            }
+        }
+        const double dsum = sum4w(o->w_cv + 4 * el, o->depth[en[0]], o->depth[en[1]], o->depth[en[2]], o->depth[en[3]]);
+        o->U_rhs_2D[2 * el] = o->U_rhs_2D[2 * el] + o->tau_inv_filt * u1 * o->elem_area[el] * dsum;
+        o->U_rhs_2D[2 * el + 1] = o->U_rhs_2D[2 * el + 1] + o->tau_inv_filt * v1 * o->elem_area[el] * dsum;
+    }
+}
+
+/* viscosity_filt_3D: src/fv_viscosity.F90:822-868, on U_n, V_n into U_rhs, V_rhs */
+static void viscosity_filt_3D(orc_t *o) {
+    const int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const int jend = (en[0] == en[3]) ? 3 : 4;
+        for (int z = 0; z < L; z++) {
+            const size_t i = (size_t)el * L + z;
+            double u1 = 0.0, v1 = 0.0;
+            for (int j = 0; j < jend; j++) {
+                const int nb = o->elem_nb[4 * el + j];
+                if (nb >= 0) {
+                    u1 = u1 + o->U_n[(size_t)nb * L + z] - o->U_n[i];
+                    v1 = v1 + o->V_n[(size_t)nb * L + z] - o->V_n[i];
+                }
+            }
+            o->U_rhs[i] = o->U_rhs[i] + o->tau_inv_filt * u1 * o->elem_area[el] * o->Je[i];
+            o->V_rhs[i] = o->V_rhs[i] + o->tau_inv_filt * v1 * o->elem_area[el] * o->Je[i];
+        }
+    }
+}
+
+/* viscosity_filt_3D_to_2D: src/fv_viscosity.F90:870-939, on the pulsation
+ * transports U_puls*Je, summed over the column into U_rhs_2D_3D */
+static void viscosity_filt_3D_to_2D(orc_t *o) {
+    const int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const int jend = (en[0] == en[3]) ? 3 : 4;
+        for (int z = 0; z < L; z++) {
+            const size_t i = (size_t)el * L + z;
+            double u1 = 0.0, v1 = 0.0;
+            for (int j = 0; j < jend; j++) {
+                const int nb = o->elem_nb[4 * el + j];
+                if (nb >= 0) {
+                    const size_t q = (size_t)nb * L + z;
+                    u1 = u1 + o->U_puls[q] * o->Je[q] - o->U_puls[i] * o->Je[i];
+                    v1 = v1 + o->V_puls[q] * o->Je[q] - o->V_puls[i] * o->Je[i];
+                }
+            }
+            const double fu = o->tau_inv_filt * u1 * o->elem_area[el], fv = o->tau_inv_filt * v1 * o->elem_area[el];
+            o->U_rhs_2D_3D[2 * el] = o->U_rhs_2D_3D[2 * el] + fu * o->Je[i];
+            o->U_rhs_2D_3D[2 * el + 1] = o->U_rhs_2D_3D[2 * el + 1] + fv * o->Je[i];
+        }
+    }
+}
+
+/* Test hook: one filter alone on the current state, its target zeroed first
+ * (2: filt_2D into U_rhs_2D, 3: filt_3D into U_rhs/V_rhs, 4: filt_3D_to_2D
+ * into U_rhs_2D_3D). */
+int orc_filter_only(orc_t *o, int which) {
+    if (!o) return -1;
+    if (which == 2) {
+        memset(o->U_rhs_2D, 0, sizeof(double) * 2 * (size_t)o->E);
+        viscosity_filt_2D(o);
+    } else if (which == 3) {
+        memset(o->U_rhs, 0, sizeof(double) * (size_t)o->E * o->L);
+        memset(o->V_rhs, 0, sizeof(double) * (size_t)o->E * o->L);
+        viscosity_filt_3D(o);
+    } else if (which == 4) {
+        memset(o->U_rhs_2D_3D, 0, sizeof(double) * 2 * (size_t)o->E);
+        viscosity_filt_3D_to_2D(o);
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+/* compute_vel_rhs: src/fv_3D_vel.F90:1-197 */
 static void compute_vel_rhs(orc_t *o) {
     int L = o->L;
     for (int el = 0; el < o->E; el++) {
@@ -1156,6 +1249,7 @@ static void compute_vel_rhs(orc_t *o) {
     }
     vel_gradients(o);
     if (o->mom_adv_3D == 1) momentum_adv_upwind(o);
+    if (o->filt_3D) viscosity_filt_3D(o);   /* src/fv_3D_vel.F90:111 */
     if (o->vert_adv == 1) momentum_vert_adv_upwind(o);
     double ff = (1.5 + EPS);
     if (o->lfirst) { ff = 1.0; o->lfirst = 0; }
@@ -1338,6 +1432,7 @@ static void update_2D_vel(orc_t *o) {
     }
     free(eta);
     if (o->mom_adv_2D == 2) momentum_adv_upwind_2D(o);
+    if (o->filt_2D) viscosity_filt_2D(o);   /* src/fv_average_dynamic.F90:346 */
     if (o->type_task > 2)
         for (int el = 0; el < o->E; el++) {
             o->U_rhs_2D[2 * el] = o->U_rhs_2D[2 * el] + o->U_rhs_2D_3D[2 * el];
@@ -1386,6 +1481,7 @@ static void timestep_AB_2D(orc_t *o, int step) {
         compute_puls_vel(o);
         el2nodes_3D(o, o->U_puls, o->V_puls, o->Unode_p, o->Vnode_p);
         momentum_adv_P1_3D_to_2D(o);
+        if (o->filt_3D) viscosity_filt_3D_to_2D(o);   /* src/fv_main.F90:1499 */
         for (int el = 0; el < o->E; el++) {
             o->U_rhs_2D_3D[2 * el] = o->U_rhs_2D_3D[2 * el] / o->dmean_n[el];
             o->U_rhs_2D_3D[2 * el + 1] = o->U_rhs_2D_3D[2 * el + 1] / o->dmean_n[el];
@@ -2211,6 +2307,7 @@ static void parse_config(orc_t *o, const char *t) {
     o->Cd_bz = cfg_f(t, "Cd_bz", 0); o->Bp_bz = cfg_f(t, "Bp_bz", 0);
     o->SL_obn = cfg_l(t, "SL_obn", 0) && !cfg_l(t, "SL_obn_user_def", 0); /* src/fv_main.F90:297 */
     o->SL_radius = cfg_f(t, "SL_radius", 0.0);
+    o->tau_inv_filt = cfg_f(t, "tau_inv_filt", 0.0);
     o->type_swr_body = cfg_i(t, "type_swr_body", 2);
     o->jer_const = cfg_l(t, "jer_const", 1);
     o->Rjc = cfg_f(t, "R", 0.58); o->ajc = cfg_f(t, "a", 0.35); o->bjc = cfg_f(t, "b", 23.0);
@@ -2241,7 +2338,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     orc_t *o = (orc_t *)xcalloc(1, sizeof(orc_t));
     parse_config(o, cfg);
     if (o->type_task != 3) { o->status = -10; snprintf(o->errmsg, sizeof o->errmsg, "only type_task=3"); return o; }
-    if (o->filt_2D || o->filt_bi_2D || o->bih_2D || o->filt_3D || o->filt_bi_3D || o->bih_3D || o->mixed_BP_grad_OB
+    if (o->filt_bi_2D || o->bih_2D || o->filt_bi_3D || o->bih_3D || o->mixed_BP_grad_OB
         || (o->key_atm && o->type_swr_body == 1 && !o->jer_const) || o->T_potential || o->ver_mix != 3
         || (o->SL_obn && !(o->SL_radius > 0.0))) {
         o->status = -14;
@@ -2457,6 +2554,7 @@ static double *fld(orc_t *o, const char *name, long *cnt) {
     F("rho_c", o->rho_c, N * L) F("Unode", o->Unode, N * L) F("Vnode", o->Vnode, N * L)
     F("Kv", o->Kv, N * K) F("Av", o->Av, E * K) F("bt", o->bt, N * K) F("snu", o->snu, N * K)
     F("U_rhs", o->U_rhs, E * L) F("V_rhs", o->V_rhs, E * L) F("U_rhsAB", o->U_rhsAB, E * L) F("V_rhsAB", o->V_rhsAB, E * L)
+    F("U_puls", o->U_puls, E * L) F("V_puls", o->V_puls, E * L)
     F("U_n", o->U_n, E * L) F("V_n", o->V_n, E * L) F("U_n_filt", o->U_n_filt, E * L) F("V_n_filt", o->V_n_filt, E * L)
     F("Wvel", o->Wvel, N * K) F("U_filt_2D", o->U_filt_2D, E) F("V_filt_2D", o->V_filt_2D, E)
     F("TF", o->TF, N * L) F("SF", o->SF, N * L) F("T_old", o->T_old, N * L) F("S_old", o->S_old, N * L)

@@ -61,6 +61,8 @@ def lib():
         L.orc_densityJM.restype = C.c_double
         L.orc_densityJM.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_sw_cp.restype = C.c_double
+        L.orc_filter_only.restype = C.c_int
+        L.orc_filter_only.argtypes = [C.c_void_p, C.c_int]
         L.orc_sw_cp.argtypes = [C.c_double, C.c_double, C.c_double]
         _lib = L
     return _lib
@@ -122,6 +124,12 @@ class Oracle:
 
     def energy(self):
         return lib().orc_energy(self.h)
+
+    def filter_only(self, which):
+        """One neighbour-difference filter alone on the current state into its
+        zeroed target (2: filt_2D, 3: filt_3D, 4: filt_3D_to_2D)."""
+        if lib().orc_filter_only(self.h, int(which)):
+            raise ValueError(f"no filter {which}")
 
     def scalar(self, name):
         return lib().orc_scalar(self.h, SCALARS[name])
