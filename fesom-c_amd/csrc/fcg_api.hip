@@ -25,6 +25,7 @@ template <typename R> __global__ void k_velrhs_filt(DevMesh, DevStateT<R>, doubl
 template <typename R>
 __global__ void k_velrhs_atm_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
+template <typename R> __global__ void k_velvisc_im(DevMesh, DevStateT<R>, double, double, double);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *, int, double,
                                                 const double *);
@@ -1090,7 +1091,11 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     else
         LAUNCH_EX(ctx, k_velrhs<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt), ctx->m,
                   s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, 1.0 / ctx->density_0);
-    if (p.im_vert_visc_fic)
+    if (p.im_vert_visc)  // src/fv_3D_vel.F90:183
+        LAUNCH_EX(ctx, k_velvisc_im<R>, ncolblk(ctx->m.E), kColT,
+                  tile_bytes(ctx->m, 8) + 2 * tile_bytes(ctx->m, sizeof(R)), ctx->m, s, p.dt, ctx->density_0,
+                  p.Cd_bz);
+    if (p.im_vert_visc_fic)  // src/fv_3D_vel.F90:184-186
         LAUNCH_EX(ctx, k_velvisc<R>, ncolblk(ctx->m.E), kColT, tile_bytes(ctx->m, 8) + 2 * tile_bytes(ctx->m, sizeof(R)),
                   ctx->m, s, p.dt, ctx->density_0, p.Cd_bz, p.fic_point);
     FCG_CHECK(hipGetLastError());
@@ -1267,13 +1272,14 @@ static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out, 
     hipLaunchKernelGGL(fcgk::k_diag_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal + 4);
     FCG_CHECK(hipGetLastError());
     double r[12];
-    int wet = 0, err = 0;
+    int wet = 0, err = 0, nonfinite = 0;
     FCG_CHECK(hipMemsetAsync(s.err + 1, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(fcgk::k_mask_count, dim3(64), dim3(kBlk), 0, ctx->stream, (const double *)s.mask_wd + e0,
                        e1 - e0, s.err + 1);
     FCG_CHECK(hipMemcpyAsync(r, s.scal + 4, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipMemcpyAsync(&err, s.err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipMemcpyAsync(&wet, s.err + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCG_CHECK(hipMemcpyAsync(&nonfinite, s.err + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
     std::memset(out, 0, sizeof *out);
     out->energy = (r[0] + r[2]) / r[1];
@@ -1289,7 +1295,9 @@ static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out, 
     out->time_jd = ctx->time_jd;
     out->n_dt = ctx->n_dt;
     out->n_wet_elem = wet;
-    out->status = err;
+    // the reference's STOP conditions first (a NaN passes its tests), then a
+    // non-finite elevation or tracer
+    out->status = err ? err : (nonfinite ? FCG_E_NAN : FCG_OK);
     if (ctx->mlp_count) {
         unsigned long long fb = 0;
         FCG_CHECK(hipMemcpy(&fb, ctx->mlp_count, sizeof fb, hipMemcpyDeviceToHost));

@@ -1088,6 +1088,104 @@ __global__ __launch_bounds__(kColT) void k_velvisc(DevMesh m, DevStateT<R> s, do
     tile_store(s.V_rhs, tV, c0, nc, K, L);
 }
 
+// momentum_vert_impl_visc (fv_viscosity.F90:1266-1371, the im_vert_visc
+// option): a Thomas solve per element column whose off-diagonals come from Av
+// over the spacing of the w_cv-weighted layer depths Z, with the wind stress
+// added to the surface row and the bottom drag -a_Cd*|u_b|*u_b (explicit in
+// the bottom layer's velocity U_n) to the bottom row.  Same tiles as
+// k_velvisc; the forward sweep keeps c' in Av's consumed slot and the primed
+// right-hand sides in the velocity slots.  Every element is solved and the
+// result multiplied by mask_wd, as the reference does.
+template <typename R>
+__global__ __launch_bounds__(kColT) void k_velvisc_im(DevMesh m, DevStateT<R> s, double dt, double rho0, double Cd_bz) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int K = m.K, L = m.L, KP = K + 1, KC = K;
+    double *tA = (double *)lds_;
+    R *tU = (R *)(tA + kColB * KP), *tV = tU + kColB * KP;
+    const int c0 = col_first(), nc = min(kColB, m.E - c0);
+    if (nc <= 0) return;
+    tile_load(tU, s.U_rhs, c0, nc, K);
+    tile_load(tV, s.V_rhs, c0, nc, K);
+    if (tcol() < nc) {
+        const int c = tcol(), e = c0 + c;
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        for (int k = tsub(); k < K; k += 4)
+            tA[c * KP + k] = sum4(w, PL(s.snu, k, q.x, KC), PL(s.snu, k, q.y, KC), PL(s.snu, k, q.z, KC),
+                                  PL(s.snu, k, q.w, KC));
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x, e = c0 + c;
+        double *A = tA + c * KP;
+        R *U = tU + c * KP, *V = tV + c * KP;
+        const double mask = s.mask_wd[e];
+        const ElemCol col = elem_col(m, s, e);
+        const int4 q = col.q;
+        const double acc = sum4(col.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+        const double a_Cd = s.C_d_el[e] * ((1.0 - acc) * Cd_bz + 1.0);
+        const double ub = PL(s.U_n, L - 1, e, KC), vb = PL(s.V_n, L - 1, e, KC);
+        const double friction = -a_Cd * sqrt(ub * ub + vb * vb);
+        const double zinvB = dt / col.Je(m, L - 1);
+        double Zu = 0.0, Zc = col.Zw(m, 0);
+        double cpp = 0.0, upp = 0.0, vpp = 0.0;  // c', u', v' of the row above
+        for (int k = 0; k < L; ++k) {
+            const double Zb = (k + 1 < L) ? col.Zw(m, k + 1) : 0.0;
+            const double zinv = dt / col.Je(m, k);
+            double aa, bb, cc;
+            if (k == 0) {  // first row (fv_viscosity.F90:1315-1320)
+                cc = -A[1] / (Zb - Zc) * zinv;
+                aa = 0.0;
+                bb = -cc + 1.0;
+            } else if (k == L - 1) {  // last row (1308-1313)
+                aa = -A[k] / (Zc - Zu) * zinv;
+                bb = -aa + 1.0;
+                cc = 0.0;
+            } else {  // regular rows (1299-1306)
+                aa = -A[k] / (Zc - Zu) * zinv;
+                cc = -A[k + 1] / (Zb - Zc) * zinv;
+                bb = -aa - cc + 1.0;
+            }
+            double ur = U[k], vr = V[k];
+            if (k == 0) {  // surface stress (1327-1328)
+                ur = ur + zinv * s.taux[e] / rho0;
+                vr = vr + zinv * s.tauy[e] / rho0;
+            }
+            if (k == L - 1) {  // bottom friction (1330-1334)
+                ur = ur + zinvB * friction * ub;
+                vr = vr + zinvB * friction * vb;
+            }
+            if (k == 0) {
+                cpp = cc / bb;
+                upp = ur / bb;
+                vpp = vr / bb;
+            } else {
+                const double mm = bb - cpp * aa;
+                cpp = cc / mm;
+                upp = (ur - upp * aa) / mm;
+                vpp = (vr - vpp * aa) / mm;
+            }
+            A[k] = cpp;  // Av(k) is not read again
+            U[k] = (R)upp;
+            V[k] = (R)vpp;
+            Zu = Zc;
+            Zc = Zb;
+        }
+        double ut = U[L - 1], vt = V[L - 1];
+        U[L - 1] = (R)(ut * mask);
+        V[L - 1] = (R)(vt * mask);
+        for (int k = L - 2; k >= 0; --k) {
+            ut = (double)U[k] - A[k] * ut;
+            vt = (double)V[k] - A[k] * vt;
+            U[k] = (R)(ut * mask);
+            V[k] = (R)(vt * mask);
+        }
+    }
+    __syncthreads();
+    tile_store(s.U_rhs, tU, c0, nc, K, L);
+    tile_store(s.V_rhs, tV, c0, nc, K, L);
+}
+
 // ---------------------------------------------------------------------------
 // barotropic substep, timestep_AB_2D: src/fv_main.F90:1372-1634
 // ---------------------------------------------------------------------------
@@ -1164,6 +1262,7 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, doub
     // pressure (fv_average_dynamic.F90:278-285)
     const double eg = atm ? kG * (m.atp1[n] * ea - s.emp[n] * d0i - m.atp2[n] * 0.0) + (kPref - s.mslp[n]) * d0i
                           : kG * (m.atp1[n] * ea - m.atp2[n] * 0.0);
+    if (!isfinite(en)) atomicOr(s.err + 2, 1);  // non-finite state flag (FCG_E_NAN)
     s.etaAB[n] = ea;
     s.eta_g[n] = eg;
     s.eta_n_2[n] = e1;
@@ -2045,7 +2144,7 @@ __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R
         Sp[nz - 1] = (R)sp;
     }
     double trn = tp, srn = sp;
-    int bad = 0;
+    int bad = 0, nonfinite = 0;
     for (int nz = L; nz >= 1; --nz) {
         if (nz < L) {
             double c = Kc[nz - 1];
@@ -2057,10 +2156,12 @@ __device__ __forceinline__ void vdiff_column(const DevMesh &m, const DevStateT<R
         if (fabs(snew) < 1.e-12) snew = fabs(snew);
         if (snew < 0.0) bad = bad ? bad : FCG_E_SALINITY;
         if (tnew < -3.0) bad = bad ? bad : FCG_E_TEMP;
+        if (!isfinite(tnew) || !isfinite(snew)) nonfinite = 1;  // NaN passes both tests above
         Tc[nz - 1] = tnew;
         Sc[nz - 1] = snew;
     }
     if (bad) atomicCAS(s.err, 0, bad);
+    if (nonfinite) atomicOr(s.err + 2, 1);
 }
 
 
@@ -2958,6 +3059,7 @@ void instantiate_kernels() {
     (void)&k_velrhs<R>;
     (void)&k_velrhs_atm<R>;
     (void)&k_velvisc<R>;
+    (void)&k_velvisc_im<R>;
     (void)&k_bt_elem1<R>;
     (void)&k_bt_node<R>;
     (void)&k_bt_edge_ssh<R>;

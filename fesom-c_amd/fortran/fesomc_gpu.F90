@@ -23,7 +23,7 @@ module fesomc_gpu
      real(c_double) :: dt, dt_2D, time_jd0, Dmin, C_d, z0b_min, z0s_min, snul, Cd_bz, Bp_bz, &
           clim_relax, relax2clim_ac
      integer(c_int) :: nsigma, Mt, tracer_adv, muscl_call, mom_adv_2D, mom_adv_3D, vert_adv, &
-          im_vert_visc_fic, fic_point, wet_dry_on, tf_presence, restart, cl_relax, precision, closure, &
+          im_vert_visc, im_vert_visc_fic, fic_point, wet_dry_on, tf_presence, restart, cl_relax, precision, closure, &
           key_atm, type_swr_body
      real(c_double) :: Rjc, ajc, bjc
      integer(c_int) :: key_rivers, filt_2D, filt_3D

@@ -111,6 +111,7 @@ program fesomc_run
   par%mom_adv_2D = geti('mom_adv_2D', 2)
   par%mom_adv_3D = geti('mom_adv_3D', 1)
   par%vert_adv = geti('vert_adv', 1)
+  par%im_vert_visc = merge(1, 0, getl('im_vert_visc'))
   par%im_vert_visc_fic = merge(1, 0, getl('im_vert_visc_fic', .true.))
   par%fic_point = geti('fic_point', 1)
   par%wet_dry_on = merge(1, 0, getl('WET_DRY_ON'))
@@ -543,6 +544,10 @@ contains
     end if
     if (getl('key_atm') .and. geti('type_swr_body', 2) == 1 .and. .not. getl('jer_const', .true.)) then
        write(*,*) 'ERROR: option not available in this build: jer_const = F (per-node Jerlov coefficients)'
+       bad = .true.
+    end if
+    if (geti('type_swr_body', 2) /= 1 .and. geti('type_swr_body', 2) /= 2) then  ! fv_read_run.F90:799-802
+       write(*,*) 'type_swr_body should be 1 or 2'
        bad = .true.
     end if
     if (.not. (getl('SALINITY_ON') .or. getl('TEMP_ON'))) then          ! type_task 3 (fv_read_run.F90)

@@ -42,7 +42,7 @@ class Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dt", "dt_2D", "time_jd0", "Dmin", "C_d", "z0b_min", "z0s_min", "snul",
                                           "Cd_bz", "Bp_bz", "clim_relax", "relax2clim_ac")] + \
                [(n, C.c_int) for n in ("nsigma", "Mt", "tracer_adv", "muscl_call", "mom_adv_2D", "mom_adv_3D",
-                                       "vert_adv", "im_vert_visc_fic", "fic_point", "wet_dry_on", "tf_presence",
+                                       "vert_adv", "im_vert_visc", "im_vert_visc_fic", "fic_point", "wet_dry_on", "tf_presence",
                                        "restart", "cl_relax", "precision", "closure", "key_atm", "type_swr_body")] + \
                [(n, C.c_double) for n in ("Rjc", "ajc", "bjc")] + \
                [(n, C.c_int) for n in ("key_rivers", "filt_2D", "filt_3D")] + [("tau_inv_filt", C.c_double)]
@@ -205,6 +205,7 @@ def params(cfg: RunConfig, precision=FCG_FP64, closure=FCG_CLOSURE_D3END) -> Par
     p.nsigma, p.Mt = cfg.nsigma, cfg.Mt
     p.tracer_adv, p.muscl_call = cfg.i("tracer_adv", 1), int(cfg.l("MUSCL_CALL"))
     p.mom_adv_2D, p.mom_adv_3D, p.vert_adv = cfg.i("mom_adv_2D", 2), cfg.i("mom_adv_3D", 1), cfg.i("vert_adv", 1)
+    p.im_vert_visc = int(cfg.l("im_vert_visc"))
     p.im_vert_visc_fic, p.fic_point = int(cfg.l("im_vert_visc_fic", True)), cfg.i("fic_point", 1)
     p.wet_dry_on, p.tf_presence = int(cfg.l("WET_DRY_ON")), int(cfg.l("TF_presence"))
     p.restart, p.cl_relax = int(cfg.restart), int(cfg.l("cl_relax"))
@@ -224,12 +225,22 @@ class GpuError(RuntimeError):
 class Model:
     """One FESOM-C model instance resident on the GPU."""
 
-    def __init__(self, cfg: RunConfig, mesh: meshio.RawMesh, precision=FCG_FP64, closure=FCG_CLOSURE_D3END):
+    rivers = None   # river table of the host (key_rivers), see __init__
+
+    def __init__(self, cfg: RunConfig, mesh: meshio.RawMesh, precision=FCG_FP64, closure=FCG_CLOSURE_D3END,
+                 rivers=None):
+        """``rivers``: with key_rivers, the path of the run's MESHPATH/rivers.dat
+        (or a ``rivers.Rivers`` table); ``step`` then interpolates it after each
+        clock advance and hands the discharge to the step as the field
+        "runoff", as the reference's main loop calls rivers_do
+        (src/fv_main.F90:400-412).  Without a table the field "runoff" keeps
+        what the caller last set (zero at start)."""
         L = lib()
         bad = cfg.unsupported()
         if bad:
             raise GpuError(f"options outside this build's hot path: {bad}")
         self.cfg = cfg
+        self.rivers = None
         self._mc = mesh_cfg(cfg)
         xy = np.ascontiguousarray(mesh.xy, dtype=np.float64).reshape(-1)
         ind = np.ascontiguousarray(mesh.index, dtype=np.int32)
@@ -255,6 +266,11 @@ class Model:
             msg = self.last_error() if self.ctx else ""
             raise GpuError(f"fcg_create failed: {ERRORS.get(rc, rc)} {msg}")
         self.init_tracers(mesh)
+        if rivers is not None and cfg.l("key_rivers"):
+            from .rivers import Rivers
+            t0 = self.par.time_jd0
+            # rivers_ini (src/fv_main.F90:352) reads the table at the start-up clock
+            self.rivers = rivers if isinstance(rivers, Rivers) else Rivers(rivers, self.N, self.clock()[1], t0)
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -349,7 +365,17 @@ class Model:
         self._ck(getattr(lib(), name)(self.ctx), name)
 
     def step(self, n=1):
-        self._ck(lib().fcg_run(self.ctx, int(n)), "fcg_run")
+        if self.rivers is None:
+            self._ck(lib().fcg_run(self.ctx, int(n)), "fcg_run")
+            return
+        # the river table is interpolated on the host after each clock advance
+        t = self.clock()[1]
+        for _ in range(int(n)):
+            self.phase("fcg_advance_clock")
+            t = t + self.par.dt / 86400.0          # the device clock, src/fv_main.F90:400
+            self.set("runoff", self.rivers.do(t))
+            for ph in self.PHASES[1:]:
+                self.phase(ph)
 
     def sync(self):
         self._ck(lib().fcg_sync(self.ctx), "fcg_sync")

@@ -67,11 +67,13 @@ VARLIST = {
     36: ("uice", "U_n_ice", "ice velocity in x direction", "m/s", 0, 1, 0, 3),
     37: ("vice", "U_n_ice", "ice velocity in y direction", "m/s", 0, 1, 0, 3),
 }
-# fields of the surface-forcing and Richardson-number code, which this build
-# does not hold (ice and rivers are off, so their fields are zero as in
-# getvar1d, src/fv_ncoutput.F90:773-815)
-NOT_HELD = {14, 15, 16, 17, 18, 19, 30}
-ZERO = {31, 32, 33, 34, 35, 36, 37}
+# fields this build does not hold: the nodal wind stress of the bulk formulas
+# (taux_node, tauy_node) and the Richardson number of d3_end.  The surface
+# fields mslp, emp, qns, qsr are the ones the host sets for the step
+# (key_atm).  Ice is off, so its fields are zero as in getvar1d
+# (src/fv_ncoutput.F90:773-815); runoff_rivers is zero without key_rivers.
+NOT_HELD = {14, 15, 30}
+ZERO = {31, 32, 33, 34, 36, 37}
 
 
 class ClassicFile:
@@ -243,6 +245,7 @@ class Output:
         self.cartesian = bool(cfg.l("cartesian"))
         self.Dmin, self.C_d, self.dt = cfg.f("Dmin", 0.25), cfg.f("Cd", 0.0), cfg.f("dt")
         self.wet_dry = bool(cfg.l("WET_DRY_ON"))
+        self.rivers_on = bool(cfg.l("key_rivers"))
         self.depth = m["depth"]
         en = m["elem_nodes"].reshape(self.E, 4)            # 1-based, reference element order
         wcv = m["w_cv"].reshape(self.E, 4)
@@ -254,7 +257,7 @@ class Output:
             bad = [v for v in f["vars"] if v in NOT_HELD]
             if bad:
                 raise ValueError(f"{f['name']}: variables {bad} ({', '.join(VARLIST[v][0] for v in bad)}) "
-                                 "belong to code this build does not hold (surface forcing, Richardson number)")
+                                 "belong to code this build does not hold (nodal wind stress, Richardson number)")
             if f["ftype"] not in (0, 3):
                 raise ValueError("output_ini: wrong/not supported output file type")
         # element centres and depth (output_ini, src/fv_ncoutput.F90:965-981)
@@ -375,6 +378,14 @@ class Output:
             a = np.full(E, self.C_d)
         elif vid in (25, 26):
             a = g("windx" if vid == 25 else "windy")
+        elif vid in (16, 17, 18, 19):
+            a = g({16: "mslp", 17: "emp", 18: "qns", 19: "qsr"}[vid])
+        elif vid == 35:
+            # the discharge of every mouth on its node (summed where mouths
+            # share one).  The reference's serial writer assigns its per-river
+            # array to this nodal slot (src/fv_ncoutput.F90:797-798) and its
+            # MPI writer writes zeros (fv_ncoutputmpi.F90:992).
+            a = g("runoff") if self.rivers_on else np.zeros(N)
         elif vid in (27, 28, 29):
             a = g({27: "Wvel", 28: "snu", 29: "bt"}[vid]).reshape(N, K)
         elif vid in ZERO:

@@ -125,6 +125,8 @@ class RunConfig:
             bad.append("Vert_turb")
         if self.l("key_atm") and self.i("type_swr_body", 2) == 1 and not self.l("jer_const", True):
             bad.append("jer_const")   # per-node Jerlov coefficients
+        if self.i("type_swr_body", 2) not in (1, 2):   # src/fv_read_run.F90:799-802
+            bad.append("type_swr_body")
         if self.type_task != 3:
             bad.append("type_task")
         return bad

@@ -29,7 +29,7 @@ extern "C" {
 #define FCG_E_ARG (-1)        /* invalid argument / unsupported option      */
 #define FCG_E_MESH (-2)       /* mesh check failed (test_elem, edges)       */
 #define FCG_E_HIP (-3)        /* HIP runtime error                          */
-#define FCG_E_NAN (-4)        /* non-finite state detected                  */
+#define FCG_E_NAN (-4)        /* non-finite elevation or tracer (no reference STOP fired) */
 #define FCG_E_SALINITY (-5)   /* "Negative salinity ..." STOP (fv_tracer.F90:1469-1473) */
 #define FCG_E_TEMP (-6)       /* "Negative temperature ..." STOP (fv_tracer.F90:1479-1482) */
 #define FCG_E_NOGPU (-7)      /* no HIP device: the library does not fall back */
@@ -60,7 +60,10 @@ typedef struct fcg_params {
     int tracer_adv;            /* 1 upwind, 2 MUSCL, 4 Miura                 */
     int muscl_call;            /* 1: tracer_adv=2 runs solve_tracer_muscl    */
     int mom_adv_2D, mom_adv_3D, vert_adv;
-    int im_vert_visc_fic;      /* 1: momentum_vert_impl_visc_tmp             */
+    int im_vert_visc;          /* 1: momentum_vert_impl_visc (fv_viscosity.F90:1266-1371) */
+    int im_vert_visc_fic;      /* 1: momentum_vert_impl_visc_tmp; with both set the
+                                  _tmp solve follows on the first's result
+                                  (fv_3D_vel.F90:183-186)                   */
     int fic_point;             /* 0,1,2 (fv_viscosity.F90:1449-1460)         */
     int wet_dry_on;
     int tf_presence;           /* tidal open-boundary elevation              */

@@ -292,3 +292,100 @@ def test_mixed_precision_with_rivers_filters_and_forcing():
     assert np.sqrt(np.mean((a.get("SF") - b.get("SF")) ** 2)) < 1e-3
     for m in runs:
         m.close()
+
+
+@pytest.mark.parametrize("fic", ["F", "T"])
+def test_implicit_vertical_viscosity_matches_oracle(fic):
+    """im_vert_visc = T (momentum_vert_impl_visc, fv_viscosity.F90:1266-1371:
+    Thomas solve with the wind stress on the surface row and the explicit
+    bottom drag on the bottom row), alone and followed by the _tmp solve
+    (im_vert_visc_fic = T, fv_3D_vel.F90:183-186), on the stratified wet/dry
+    channel: bit for bit (1e-12) against the oracle."""
+    import re
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    for k, v in (("im_vert_visc", "T"), ("im_vert_visc_fic", fic)):
+        txt, n = re.subn(rf"^(\s*{k}\s*=\s*)\S+", rf"\g<1>{v}", txt, count=1, flags=re.M)
+        assert n == 1, k
+    g, o = _model(txt, mesh, ex)
+    g.step(8)
+    o.step(8)
+    _compare(g, o)
+    assert g.diagnostics()["status"] == 0
+    g.close()
+
+
+def test_negative_salinity_stop_is_reported():
+    """The reference STOPs when S<0 after the tracer step
+    (src/fv_tracer.F90:1466-1473).  A fresh/salt front (S = 0 west of 5 km,
+    30 east) under MUSCL undershoots below zero in the first step, in the
+    oracle and on the device alike; the device reports FCG_E_SALINITY and
+    holds the same salinity field, negative cells included.  (A negative
+    salinity set from the host would not do: densityJM's sqrt(S**3) turns the
+    state into NaN, which passes the reference's test; the device reports that
+    as FCG_E_NAN, see test_non_finite_state_is_reported.)"""
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c1")
+    txt = _with_scheme(txt, "muscl")
+    g, o = _model(txt, mesh, ex)
+    S = np.where(mesh.xy[:, 0:1] < 5000.0, 0.0, 30.0) * np.ones((1, g.L))
+    g.set_tracers(None, S)
+    o.set_tracers(None, S.reshape(-1))
+    g.step(1)
+    d = g.diagnostics()
+    assert d["status"] == -5, d
+    with pytest.raises(RuntimeError, match="Negative salinity"):
+        o.step(1)
+    sg, so = g.get("SF"), o.get("SF")
+    assert sg.min() < 0.0
+    assert np.abs(sg - so).max() <= RTOL * np.abs(so).max()
+    g.close()
+
+
+@pytest.mark.parametrize("field", ["TF", "eta_n"])
+def test_non_finite_state_is_reported(field):
+    """A NaN in the temperature or the elevation passes the reference's
+    T < -3 and S < 0 tests; the device reports FCG_E_NAN instead of status 0."""
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c1")
+    g, _ = _model(txt, mesh, ex)
+    a = g.get(field)
+    a[len(a) // 2] = np.nan
+    g.set(field, a)
+    g.step(1)
+    d = g.diagnostics()
+    assert d["status"] == -4, d
+    g.close()
+
+
+def test_python_host_applies_the_river_table(tmp_path):
+    """gpu.Model with key_rivers and a two-record rivers.dat interpolates the
+    table after each clock advance (rivers_do, src/fv_main.F90:400-412) and
+    hands the discharge to the step: bit for bit (1e-12) against the oracle
+    driven by the same table."""
+    import re
+    from fesomc_amd import gpu, meshgen
+    from fesomc_amd.rivers import Rivers
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, ex = meshgen.config("c2_small")
+    txt = re.sub(r"^(\s*key_rivers\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    cfg = RunConfig(txt)
+    jd0 = cfg.f("JD0")
+    o = Oracle(txt, mesh)
+    mouths = np.argsort(np.where(o.get("ac") >= 1.0, o.get("depth"), -np.inf))[-2:] + 1
+    table = tmp_path / "rivers.dat"
+    table.write_text(f"2\n2\n{jd0 - 1.0} {jd0 + 1.0}\n{mouths[0]} {mouths[1]}\n4.0e4 1.0e4\n8.0e4 2.0e4\n")
+    g = gpu.Model(cfg, mesh, rivers=str(table))
+    g.set_tracers(ex["T"], ex["S"])
+    o.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
+    riv = Rivers(str(table), mesh.nod2D, jd0, jd0)
+    t = jd0
+    g.step(6)
+    for _ in range(6):
+        t = t + cfg.f("dt") / 86400.0
+        o.set("runoff", riv.do(t))
+        o.step(1)
+    assert o.get("runoff").max() > 0.0
+    assert np.array_equal(g.get("runoff"), o.get("runoff"))
+    _compare(g, o, FIELDS)
+    g.close()

@@ -220,3 +220,62 @@ def test_writer_over_device_model(tmp_path):
         m.set("tauy", np.full(mesh.elem2D, ty))
         return m, lambda: m.step(1)
     _check_files(*_run_and_check(tmp_path, make), rtol=1e-12)
+
+
+FORCED_FILELIST = """!total number of files:
+1
+!=========== Field File Name ===
+forced.nc
+! snapshot
+1
+! type
+0
+! position
+0.0, 0.0, 0.0, 0.0
+! frequency
+2
+! special
+0
+! number of variables
+5
+! list
+16,17,18,19,35,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0,0
+"""
+
+
+def test_writer_serves_surface_forcing_and_river_fields(tmp_path):
+    """With key_atm and key_rivers the writer's mslp, emp, qns, qsr and
+    runoff_rivers (ids 16-19, 35) are the fields the step used -- the host's
+    surface fields and the per-node discharge -- times the node mask."""
+    import re
+    from fesomc_amd import meshgen, ncout
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, ex = _case()
+    for k in ("key_atm", "key_rivers"):
+        txt = re.sub(rf"^(\s*{k}\s*=\s*)\S+", r"\g<1>T", txt, count=1, flags=re.M)
+    (tmp_path / "outputfilelist.dat").write_text(FORCED_FILELIST)
+    o = _oracle(txt, mesh, ex)
+    x, y = mesh.xy[:, 0], mesh.xy[:, 1]
+    fields = {"qsr": 200.0 + 50.0 * np.sin(x / 3000.0), "qns": -40.0 + 10.0 * np.cos(y / 2000.0),
+              "emp": np.full(mesh.nod2D, 1e-6), "mslp": 101325.0 + 300.0 * np.sin(y / 5000.0)}
+    r = np.zeros(mesh.nod2D)
+    r[np.argsort(np.where(o.get("ac") >= 1.0, o.get("depth"), -np.inf))[-2:]] = (30.0, 7.0)
+    fields["runoff"] = r
+    for k, v in fields.items():
+        o.set(k, v)
+    out = ncout.Output(OracleSource(o), RunConfig(txt), mesh, workdir=str(tmp_path))
+    want = []
+    for n in range(1, 5):
+        o.step(1)
+        out.step()
+        mn = o.get("mask_wd_node")
+        if n % 2 == 0 or n == 1:
+            want.append({name: o.get(f) * mn for name, f in (("mslp", "mslp"), ("emp", "emp"), ("qns", "qns"),
+                                                            ("qsr", "qsr"), ("runoff_rivers", "runoff"))})
+    out.close()
+    with netcdf_file(str(tmp_path / "nc_output" / "forced.nc"), "r", mmap=False) as f:
+        assert f.variables["time"].shape[0] == len(want) == 3
+        for rec, w in enumerate(want):
+            for name, a in w.items():
+                np.testing.assert_array_equal(f.variables[name][rec], a, err_msg=name)
+        assert f.variables["runoff_rivers"][0].max() == 30.0
