@@ -38,6 +38,7 @@ struct fcg_mlp {
     void *frag = nullptr;
     int nparts = 0;   // waves per gradient launch
     int step = 0;     // Adam step count
+    int device = 0;   // the HIP device the network lives on
     MlpNorm norm{};
     hipStream_t stream = nullptr;
 };

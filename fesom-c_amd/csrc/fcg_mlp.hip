@@ -16,6 +16,7 @@
 // Adam.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -247,14 +248,40 @@ int mlp_cast(fcg_mlp *net) {
     return mlp_fail(hipGetLastError());
 }
 
-int mlp_grad(fcg_mlp *net, const float *X, const float *T, long n) {
+// gradient of the batch into g, with the per-wave partials in part
+// (kMaxParts x kMlpParams floats), on stream st
+int mlp_grad_into(fcg_mlp *net, const float *X, const float *T, long n, float *part, float *g, hipStream_t st) {
     const long ntile = (n + 15) / 16;
     const int parts = (int)std::min<long>(kMaxParts, ntile);
-    hipLaunchKernelGGL(fcgm::k_mlp_grad, dim3(parts), dim3(64), 0, net->stream, net->w,
-                       (const fcgm::bf16x4 *)net->frag, X, T, n, net->part);
-    hipLaunchKernelGGL(fcgm::k_mlp_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, net->stream,
-                       (const float *)net->part, parts, net->g);
+    hipLaunchKernelGGL(fcgm::k_mlp_grad, dim3(parts), dim3(64), 0, st, net->w, (const fcgm::bf16x4 *)net->frag, X,
+                       T, n, part);
+    hipLaunchKernelGGL(fcgm::k_mlp_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, (const float *)part,
+                       parts, g);
     return mlp_fail(hipGetLastError());
+}
+
+int mlp_grad(fcg_mlp *net, const float *X, const float *T, long n) {
+    return mlp_grad_into(net, X, T, n, net->part, net->g, net->stream);
+}
+
+// Partial sums of fcg_mlp_gradient_static: one array per GPU for the whole
+// process (6.4 MB), addressed through the device pointer looked up once per
+// device in fcg_mlp_create.  Every wave writes all kMlpParams entries of its
+// partial before the reduction reads them, so nothing needs clearing between
+// calls; calls that overlap on one GPU race on it.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kMaxParts * kMlpParams];
+float *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_lookup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return FCG_E_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return FCG_OK;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return FCG_E_HIP;
+    g_scratch_ptr[dev] = (float *)ptr;
+    return FCG_OK;
 }
 
 }  // namespace
@@ -266,6 +293,8 @@ extern "C" int fcg_mlp_create(unsigned seed, fcg_mlp **out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FCG_E_NOGPU;
     auto *net = new fcg_mlp();
     *out = net;
+    MLP_CHECK(hipGetDevice(&net->device));
+    if (int rc = scratch_lookup(net->device)) return rc;  // device address of the static partials
     MLP_CHECK(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
     const size_t pb = sizeof(float) * kMlpParams;
     MLP_CHECK(hipMalloc((void **)&net->w, pb));
@@ -366,6 +395,15 @@ extern "C" int fcg_mlp_gradient(fcg_mlp *net, const float *X, const float *T, lo
     MLP_CHECK(hipStreamSynchronize(net->stream));
     MLP_CHECK(hipMemcpy(grad, net->g, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
     return FCG_OK;
+}
+
+extern "C" int fcg_mlp_gradient_static(fcg_mlp *net, const float *X, const float *T, long n, float *grad_dev,
+                                       void *stream) {
+    if (!net || !X || !T || !grad_dev) return FCG_E_ARG;
+    if (n <= 0 || n > FCG_MLP_STATIC_MAX_SAMPLES) return FCG_E_ARG;  // batch outside 1..2^31-1
+    if (net->device < 0 || net->device >= kMaxDevices || !g_scratch_ptr[net->device]) return FCG_E_ARG;
+    hipStream_t st = stream ? (hipStream_t)stream : net->stream;
+    return mlp_grad_into(net, X, T, n, g_scratch_ptr[net->device], grad_dev, st);
 }
 
 extern "C" int fcg_mlp_train_step(fcg_mlp *net, const float *X, const float *T, long n, float lr) {

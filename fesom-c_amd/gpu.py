@@ -33,7 +33,7 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_mlp_destroy", "fcg_mlp_get_params", "fcg_mlp_set_params", "fcg_mlp_set_norm", "fcg_mlp_forward",
            "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect",
            "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member", "fcg_set_launch",
-           "fcg_get_launch", "fcg_hot_start"]
+           "fcg_get_launch", "fcg_hot_start", "fcg_mlp_gradient_static"]
 
 MLP_FEATURES, MLP_PARAMS = 5, 6288
 
@@ -141,6 +141,8 @@ def lib():
         L.fcg_mlp_forward.argtypes = [P, P, P, C.c_long]
         L.fcg_mlp_gradient.restype = C.c_int
         L.fcg_mlp_gradient.argtypes = [P, P, P, C.c_long, _FP]
+        L.fcg_mlp_gradient_static.restype = C.c_int
+        L.fcg_mlp_gradient_static.argtypes = [P, P, P, C.c_long, P, P]
         L.fcg_mlp_train_step.restype = C.c_int
         L.fcg_mlp_train_step.argtypes = [P, P, P, C.c_long, C.c_float]
         L.fcg_set_mlp.restype = C.c_int
@@ -495,6 +497,14 @@ class Mlp:
         self._ck(lib().fcg_mlp_gradient(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), n,
                                         g.ctypes.data_as(C.POINTER(C.c_float))), "gradient")
         return g
+
+    def gradient_static(self, X_ptr, T_ptr, n, G_ptr, stream=None):
+        """fcg_mlp_gradient_static: the gradient into the device buffer at
+        G_ptr (6288 floats) without any device allocation, launched on the
+        HIP stream handle ``stream`` (None: the network's own stream) and not
+        synchronised."""
+        self._ck(lib().fcg_mlp_gradient_static(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), n, C.c_void_p(G_ptr),
+                                                C.c_void_p(stream) if stream else None), "gradient_static")
 
     def train_step(self, X_ptr, T_ptr, n, lr):
         self._ck(lib().fcg_mlp_train_step(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), n, lr), "train_step")

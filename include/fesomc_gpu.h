@@ -281,6 +281,20 @@ int fcg_mlp_forward(fcg_mlp *net, const float *X_dev, float *Y_dev, long n);   /
 /* Gradient of 0.5*mean_i |f(X_i) - T_i|^2 with respect to the parameters,
  * copied to grad_host (count 6288); synchronous. */
 int fcg_mlp_gradient(fcg_mlp *net, const float *X_dev, const float *T_dev, long n, float *grad_host);
+/* The same gradient without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight training loops).  Writes the 6288 gradient entries to
+ * the caller's device buffer grad_dev, launching on `stream` (a hipStream_t;
+ * NULL = the network's own stream) without synchronising.  The per-wave
+ * partial sums live in a file-scope device array of the library (256 x 6288
+ * floats, 6.4 MB per GPU, held until the process ends); it serves any batch
+ * since each wave sums several 16-sample tiles.  n must lie in
+ * 1..FCG_MLP_STATIC_MAX_SAMPLES (FCG_E_ARG otherwise).  Calls that overlap
+ * on one GPU race on that array.  The result equals fcg_mlp_gradient's bit
+ * for bit. */
+#define FCG_MLP_STATIC_MAX_SAMPLES 2147483647L
+int fcg_mlp_gradient_static(fcg_mlp *net, const float *X_dev, const float *T_dev, long n, float *grad_dev,
+                            void *stream);
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on that loss; synchronous. */
 int fcg_mlp_train_step(fcg_mlp *net, const float *X_dev, const float *T_dev, long n, float lr);
 /* Attach a trained network to a context created with FCG_CLOSURE_MLP (the
