@@ -13,6 +13,7 @@
 // the node scalar a = max(depth+eta_n, Dmin) (src/fv_main.F90:1228-1263),
 // which gives the same bits as the reference's stored arrays.
 #include "fcg_internal.h"
+#include "fcg_logcr.h"
 #include "fcg_mlp.h"
 
 namespace fcgk {
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
     const double snu_mw = 0.5 * (SNU(K) + SNU(K - 1));
     const double yy2 = (0.4e-3 * (uds * uds * uds) * dmean * d1w) / (snu_mw * 0.76);
     const double delta_b = a * m.dsig[L - 1] + z0b_min;
-    const double lg = log(delta_b / z0b_min);
+    const double lg = fcglog::log_cr(delta_b / z0b_min);  // correctly rounded, as the host libm's
     const double bott_R = B1cka2 / (lg * lg);
     R *bc = s.mx_bA, *bn = s.mx_bB;
     for (int it = 1; it <= 200; ++it) {

@@ -108,6 +108,18 @@ typedef struct {
 } orc_t;
 
 /* ------------------------------------------------------------------ */
+/* test hooks                                                            */
+/* ------------------------------------------------------------------ */
+/* A planted mistake, for the pin tests only: they switch one on to show that
+ * each closed-form check fails when the restatement is wrong (0 = none; the
+ * ids are listed at orc_set_mutation).  g_faces, when set by
+ * orc_tracer_faces, receives the horizontal face value of every (edge,
+ * layer, segment) of the MUSCL or Miura flux. */
+static int g_mut = 0;
+static double *g_faces = NULL, *g_rhs = NULL;
+#define MUT(id) (g_mut == (id))
+
+/* ------------------------------------------------------------------ */
 /* small helpers                                                        */
 /* ------------------------------------------------------------------ */
 static void *xcalloc(size_t n, size_t sz) {
@@ -117,6 +129,75 @@ static void *xcalloc(size_t n, size_t sz) {
 }
 #define DALLOC(n) ((double *)xcalloc((size_t)(n), sizeof(double)))
 #define IALLOC(n) ((int *)xcalloc((size_t)(n), sizeof(int)))
+/* Natural logarithm rounded correctly (to the double nearest the exact
+ * value), for d3_end's bottom boundary value.  The reference takes log from
+ * the host libm; glibc's rounds correctly for all but about 1.5e-4 of the
+ * arguments d3_end meets and is 1 ulp off on the rest.  The oracle and the
+ * device both use the correctly rounded value, so they agree bit for bit;
+ * against a libm-linked run of the reference the bottom boundary value can
+ * differ by an ulp in those rare columns.  Evaluation: x = 2^k m, m in
+ * [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), as
+ * s * sum_{n<=22} s^(2n)/(2n+1) in double-double arithmetic (error-free sums
+ * and fma products), plus k ln 2 (ln 2 in two parts); the result is accurate
+ * to about 2^-104 before its final rounding.  Pinned against exact decimal
+ * logarithms in tests/test_logcr.py. */
+typedef struct { double h, l; } ddbl;
+static ddbl dd_two_sum(double a, double b) {
+    double s = a + b, bb = s - a;
+    ddbl r = {s, (a - (s - bb)) + (b - bb)};
+    return r;
+}
+static ddbl dd_quick(double a, double b) {
+    double s = a + b;
+    ddbl r = {s, b - (s - a)};
+    return r;
+}
+static ddbl dd_plus(ddbl a, ddbl b) {
+    ddbl s = dd_two_sum(a.h, b.h), t = dd_two_sum(a.l, b.l);
+    s.l += t.h;
+    s = dd_quick(s.h, s.l);
+    s.l += t.l;
+    return dd_quick(s.h, s.l);
+}
+static ddbl dd_times(ddbl a, ddbl b) {
+    double p = a.h * b.h;
+    double e = fma(a.h, b.h, -p);
+    e += a.h * b.l + a.l * b.h;
+    return dd_quick(p, e);
+}
+static double log_correctly_rounded(double x) {
+    if (!(x > 0.0) || !(x < INFINITY) || x < 2.2250738585072014e-308) return log(x);
+    int k = 0;
+    double m = frexp(x, &k);
+    if (m < 0.70710678118654752440) { m = m * 2.0; k -= 1; }
+    ddbl num = {m - 1.0, 0.0}, den = dd_two_sum(m, 1.0);
+    /* s = num/den: quotient and two corrections */
+    double q1 = num.h / den.h;
+    ddbl mq = {-q1, 0.0}, r = dd_plus(num, dd_times(mq, den));
+    double q2 = r.h / den.h;
+    ddbl mq2 = {-q2, 0.0};
+    r = dd_plus(r, dd_times(mq2, den));
+    ddbl q3 = {r.h / den.h, 0.0};
+    ddbl sq = dd_plus(dd_quick(q1, q2), q3);
+    ddbl s2 = dd_times(sq, sq), p = {0.0, 0.0};
+    for (int n = 22; n >= 0; n--) {
+        double d = (double)(2 * n + 1), c = 1.0 / d;
+        ddbl cn = {c, fma(-c, d, 1.0) / d};
+        p = dd_plus(dd_times(p, s2), cn);
+    }
+    ddbl lm = dd_times(sq, p);
+    lm.h *= 2.0;
+    lm.l *= 2.0;
+    const double l2h = 6.93147180559945286227e-01, l2l = 2.31904681384629955842e-17;
+    double kh = (double)k * l2h;
+    ddbl kl = {kh, fma((double)k, l2h, -kh)};
+    ddbl kc = {(double)k * l2l, 0.0};
+    kl = dd_plus(kl, kc);
+    ddbl res = dd_plus(kl, lm);
+    return res.h + res.l;
+}
+double orc_log_cr(double x) { return log_correctly_rounded(x); }
+
 static double dmax(double a, double b) { return a > b ? a : b; } /* Fortran max for non-NaN */
 static double dmin(double a, double b) { return a < b ? a : b; }
 
@@ -675,9 +756,12 @@ static void wad_mask(orc_t *o) {
         const int *en = o->elem_nodes + 4 * el;
         double mn = dmin(dmin(dmin(o->depth[en[0]], o->depth[en[1]]), o->depth[en[2]]), o->depth[en[3]]);
         double mx = dmax(dmax(dmax(o->etaAB[en[0]], o->etaAB[en[1]]), o->etaAB[en[2]]), o->etaAB[en[3]]);
-        o->mask_wd[el] = (mn + mx > o->Dmin) ? 1.0 : 0.0;
+        if (MUT(12)) mn = dmax(dmax(dmax(o->depth[en[0]], o->depth[en[1]]), o->depth[en[2]]), o->depth[en[3]]);
+        o->mask_wd[el] = (MUT(8) ? mn + mx >= o->Dmin : mn + mx > o->Dmin) ? 1.0 : 0.0;
     }
-    for (int n = 0; n < o->N; n++) o->mask_wd_node[n] = (o->depth[n] + o->etaAB[n] > o->Dmin) ? 1.0 : 0.0;
+    for (int n = 0; n < o->N; n++)
+        o->mask_wd_node[n] = (MUT(8) ? o->depth[n] + o->etaAB[n] >= o->Dmin : o->depth[n] + o->etaAB[n] > o->Dmin)
+                           ? 1.0 : 0.0;
 }
 
 /* adv_tracer_mask: src/fv_main.F90:1309-1335 */
@@ -825,7 +909,7 @@ static void d3_end(orc_t *o) {
                 yy1[nz + 1] = (am * yy1[nz] - fr) / a;
             }
             double delta_b = o->Jc[(size_t)node * L + (L - 1)] + o->z0b_min;
-            double lg = log(delta_b / o->z0b_min);
+            double lg = log_correctly_rounded(delta_b / o->z0b_min);
             double bott_R = B1cka2 / (lg * lg);
             btr1[K] = bott_R * (Un[K - 1] * Un[K - 1] + Vn[K - 1] * Vn[K - 1]);
             for (int nz = K; nz >= 2; nz--) btr1[nz - 1] = btr1[nz] * xx1[nz] + yy1[nz];
@@ -1054,7 +1138,7 @@ static void momentum_vert_impl_visc_tmp(orc_t *o) {
             }
             const double *w = o->w_cv + 4 * el;
             double acc = sum4w(w, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
-            double a_Cd = o->C_d_el[el] * ((1.0 - acc) * o->Cd_bz + 1.0);
+            double a_Cd = o->C_d_el[el] * ((MUT(11) ? 1.0 + acc : 1.0 - acc) * o->Cd_bz + 1.0);
             de = Je[L] / (Av[K] + Av[L]);
             double ub = o->U_n[(size_t)el * L + L - 1], vb = o->V_n[(size_t)el * L + L - 1];
             double friction = de * a_Cd * sqrt(ub * ub + vb * vb);
@@ -1222,6 +1306,57 @@ int orc_filter_only(orc_t *o, int which) {
     return 0;
 }
 
+static void solve_tracer_miura(orc_t *o, double *ttf, double *tto, const double *clim, char type);
+static void solve_tracer_muscl(orc_t *o, double *ttf, double *tto, const double *clim, char type);
+
+/* Test hook: the horizontal face values of solve_tracer_muscl (scheme 2) or
+ * solve_tracer_miura (scheme 4) for the tracer field T (N*L, column-major),
+ * with the current velocities and an AB history equal to T and a zero mean
+ * (T_aver = 0): faces[(edge*L + layer)*2 + segment], segment 0 of edge_tri(1),
+ * 1 of edge_tri(2) (left 0 on boundary edges); and, if rhs is not NULL, the
+ * advective tendency (horizontal and vertical, before any relaxation) per
+ * node and layer.  The model state is unchanged. */
+int orc_tracer_faces(orc_t *o, int scheme, const double *T, double *faces, double *rhs) {
+    if (!o || !T || !faces || (scheme != 2 && scheme != 4)) return -1;
+    size_t NL = (size_t)o->N * o->L;
+    double *ttf = DALLOC(NL), *tto = DALLOC(NL), *clim = DALLOC(NL);
+    memcpy(ttf, T, sizeof(double) * NL);
+    memcpy(tto, T, sizeof(double) * NL);
+    memcpy(clim, T, sizeof(double) * NL);
+    memset(faces, 0, sizeof(double) * 2 * (size_t)o->D * o->L);
+    double ta = o->T_aver;
+    o->T_aver = 0.0;
+    g_faces = faces;
+    g_rhs = rhs;
+    if (scheme == 2) solve_tracer_muscl(o, ttf, tto, clim, 't');
+    else { adv_tracer_mask(o); solve_tracer_miura(o, ttf, tto, clim, 't'); }
+    g_faces = NULL;
+    g_rhs = NULL;
+    o->T_aver = ta;
+    free(ttf); free(tto); free(clim);
+    return 0;
+}
+
+/* Test hook: wad_mask alone on the current etaAB (src/fv_main.F90:1269-1307). */
+int orc_wad_mask_only(orc_t *o) {
+    if (!o) return -1;
+    wad_mask(o);
+    return 0;
+}
+
+/* Planted mistakes of the pin tests (0 = none):
+ *  1 MUSCL upwind/central weights swapped (0.85 upwind)   7 OB tide without the cold-start ramp
+ *  2 MUSCL second segment dropped                         8 wad_mask with >= for >
+ *  3 MUSCL reconstruction without its 2*(T2-T1) term      9 Coriolis with the wrong sign
+ *  4 Miura departure point ahead of the face (+u*dt)     10 wind stress left out of the 2-D momentum
+ *  5 Miura's missing *dt (fv_tracer.F90:1057) restored   11 sponge drag factor (1+ac) for (1-ac)
+ *  6 OB tide phase with the wrong sign                   12 element wet/dry mask from the deepest node
+ * 13 Miura edge x-length in metres (the reference's a = 1 workaround, fv_tracer.F90:1018-1026, undone) */
+int orc_set_mutation(int id) {
+    g_mut = id;
+    return 0;
+}
+
 /* compute_vel_rhs: src/fv_3D_vel.F90:1-197 */
 static void compute_vel_rhs(orc_t *o) {
     int L = o->L;
@@ -1243,8 +1378,9 @@ static void compute_vel_rhs(orc_t *o) {
             size_t i = (size_t)el * L + z;
             o->U_rhs[i] = -(0.5 + EPS) * o->U_rhsAB[i] + Fx * o->Je[i];
             o->V_rhs[i] = -(0.5 + EPS) * o->V_rhsAB[i] + Fy * o->Je[i];
-            o->U_rhsAB[i] = o->V_n[i] * o->coriolis[el] * o->elem_area[el] * o->Je[i];
-            o->V_rhsAB[i] = -o->U_n[i] * o->coriolis[el] * o->elem_area[el] * o->Je[i];
+            double cor = MUT(9) ? -o->coriolis[el] : o->coriolis[el];
+            o->U_rhsAB[i] = o->V_n[i] * cor * o->elem_area[el] * o->Je[i];
+            o->V_rhsAB[i] = -o->U_n[i] * cor * o->elem_area[el] * o->Je[i];
         }
     }
     vel_gradients(o);
@@ -1313,8 +1449,8 @@ static void compute_ssh_rhs_elem(orc_t *o) {
             for (int j = 0; j < 12; j++)
                 o->ssh_rhs[n] = o->ssh_rhs[n] + o->amp_factor[j] * o->ampt[i * 12 + j]
                     * cos((o->time_2D - o->time_jd0 * 86400.0) * 2.0 * PI / (TIDE_PERIOD[j] * 3600.0)
-                          - (o->fazt[i * 12 + j] + o->phy_factor[j]));
-            if (!o->restart)
+                          - (MUT(6) ? -1.0 : 1.0) * (o->fazt[i * 12 + j] + o->phy_factor[j]));
+            if (!o->restart && !MUT(7))
                 if (o->time_jd - o->time_jd0 < 10.0) o->ssh_rhs[n] = o->ssh_rhs[n] * (o->time_jd - o->time_jd0) / 10.0;
         }
     }
@@ -1426,9 +1562,11 @@ static void update_2D_vel(orc_t *o) {
         double dm = dmax(o->Dmin, w[0] * (o->etaAB[en[0]] + o->depth[en[0]]) + w[1] * (o->etaAB[en[1]] + o->depth[en[1]])
                                 + w[2] * (o->etaAB[en[2]] + o->depth[en[2]]) + w[3] * (o->etaAB[en[3]] + o->depth[en[3]]));
         o->U_rhs_2D[2 * el] = o->elem_area[el] * (-dm * (gs[0] * eta[en[0]] + gs[1] * eta[en[1]] + gs[2] * eta[en[2]] + gs[3] * eta[en[3]])
-                              + o->UAB[2 * el + 1] * dm * o->coriolis[el] + o->taux[el] * d0i);
+                              + o->UAB[2 * el + 1] * dm * (MUT(9) ? -o->coriolis[el] : o->coriolis[el])
+                              + (MUT(10) ? 0.0 : o->taux[el] * d0i));
         o->U_rhs_2D[2 * el + 1] = o->elem_area[el] * (-dm * (gs[4] * eta[en[0]] + gs[5] * eta[en[1]] + gs[6] * eta[en[2]] + gs[7] * eta[en[3]])
-                                  - o->UAB[2 * el] * dm * o->coriolis[el] + o->tauy[el] * d0i);
+                                  - o->UAB[2 * el] * dm * (MUT(9) ? -o->coriolis[el] : o->coriolis[el])
+                                  + (MUT(10) ? 0.0 : o->tauy[el] * d0i));
     }
     free(eta);
     if (o->mom_adv_2D == 2) momentum_adv_upwind_2D(o);
@@ -1444,7 +1582,7 @@ static void update_2D_vel(orc_t *o) {
         double dm = dmax(o->Dmin, sum4w(w, o->eta_n[en[0]] + o->depth[en[0]], o->eta_n[en[1]] + o->depth[en[1]],
                                         o->eta_n[en[2]] + o->depth[en[2]], o->eta_n[en[3]] + o->depth[en[3]]));
         double acc = sum4w(w, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
-        double a_Cd = o->C_d_el[el] * ((1.0 - acc) * o->Cd_bz + 1.0);
+        double a_Cd = o->C_d_el[el] * ((MUT(11) ? 1.0 + acc : 1.0 - acc) * o->Cd_bz + 1.0);
         double ub = o->U_n[(size_t)el * L + L - 1], vb = o->V_n[(size_t)el * L + L - 1];
         double friction = a_Cd * sqrt(ub * ub + vb * vb);
         double ff = 1.0 / (dmax(o->Dmin, sum4w(w, o->ssh_rhs[en[0]] + o->depth[en[0]], o->ssh_rhs[en[1]] + o->depth[en[1]],
@@ -1777,6 +1915,10 @@ static void solve_tracer_miura(orc_t *o, double *ttf, double *tto, const double 
         double a = 1.0, b = 1.0;
         if (e2 >= 0) { dX2 = cd[2]; dY2 = cd[3]; Kh = 0.5 * (Kh + o->elem_area[e2]); }
         Kh = K_HOR * Kh / o->scale_area;
+        if (MUT(13)) {  /* the metric factor the reference's workaround replaces by 1 */
+            a = R_EARTH * o->elem_cos[e1];
+            if (e2 >= 0) b = R_EARTH * o->elem_cos[e2];
+        }
         for (int z = 0; z < L; z++) {
             size_t i1 = (size_t)e1 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
             double Tx, Ty;
@@ -1786,13 +1928,16 @@ static void solve_tracer_miura(orc_t *o, double *ttf, double *tto, const double 
                 Ty = 0.5 * Kh * (tty[i1] + tty[i2]);
             } else { Tx = Kh * ttx[i1]; Ty = Kh * tty[i1]; }
             double u = o->U_n_filt[i1], v = o->V_n_filt[i1], Tm;
+            double sdt = MUT(4) ? -o->dt : o->dt;          /* departure point ahead of the face */
+            double vq = MUT(5) ? v * o->dt : v;            /* the reference's missing *dt restored */
             if (v * dX1 - u * dY1 > 0) {
-                Tm = ttf[a2] + (0.5 * (-dd[0] * a + dX1 - u * o->dt) * txn[a2]
-                               + 0.5 * (-dd[1] * R_EARTH + dY1 - v) * tyn[a2]) * crt1;
+                Tm = ttf[a2] + (0.5 * (-dd[0] * a + dX1 - u * sdt) * txn[a2]
+                               + 0.5 * (-dd[1] * R_EARTH + dY1 - vq) * tyn[a2]) * crt1;
             } else {
-                Tm = ttf[a1] + (0.5 * (dd[0] * a + dX1 - u * o->dt) * txn[a1]
-                               + 0.5 * (dd[1] * R_EARTH + dY1 - v * o->dt) * tyn[a1]) * crt1;
+                Tm = ttf[a1] + (0.5 * (dd[0] * a + dX1 - u * sdt) * txn[a1]
+                               + 0.5 * (dd[1] * R_EARTH + dY1 - v * sdt) * tyn[a1]) * crt1;
             }
+            if (g_faces) g_faces[((size_t)ed * L + z) * 2] = Tm;
             double c1 = (o->Je[i1] * v * Tm - Ty) * dX1 - (o->Je[i1] * u * Tm - Tx) * dY1;
             rhs[a1] = rhs[a1] + c1;
             rhs[a2] = rhs[a2] - c1;
@@ -1803,13 +1948,15 @@ static void solve_tracer_miura(orc_t *o, double *ttf, double *tto, const double 
                 double Tx = 0.5 * Kh * (ttx[i1] + ttx[i2]);
                 double Ty = 0.5 * Kh * (tty[i1] + tty[i2]);
                 double u = o->U_n_filt[i2], v = o->V_n_filt[i2], Tm;
+                double sdt = MUT(4) ? -o->dt : o->dt;
                 if (v * dX2 - u * dY2 < 0) {
-                    Tm = ttf[a2] + (0.5 * (-dd[0] * b + dX2 - u * o->dt) * txn[a2]
-                                   + 0.5 * (-dd[1] * R_EARTH + dY2 - v * o->dt) * tyn[a2]) * crt1;
+                    Tm = ttf[a2] + (0.5 * (-dd[0] * b + dX2 - u * sdt) * txn[a2]
+                                   + 0.5 * (-dd[1] * R_EARTH + dY2 - v * sdt) * tyn[a2]) * crt1;
                 } else {
-                    Tm = ttf[a1] + (0.5 * (dd[0] * b + dX2 - u * o->dt) * txn[a1]
-                                   + 0.5 * (dd[1] * R_EARTH + dY2 - v * o->dt) * tyn[a1]) * crt1;
+                    Tm = ttf[a1] + (0.5 * (dd[0] * b + dX2 - u * sdt) * txn[a1]
+                                   + 0.5 * (dd[1] * R_EARTH + dY2 - v * sdt) * tyn[a1]) * crt1;
                 }
+                if (g_faces) g_faces[((size_t)ed * L + z) * 2 + 1] = Tm;
                 double c2 = -(o->Je[i2] * v * Tm - Ty) * dX2 + (o->Je[i2] * u * Tm - Tx) * dY2;
                 rhs[a1] = rhs[a1] + c2;
                 rhs[a2] = rhs[a2] - c2;
@@ -1864,6 +2011,7 @@ static void solve_tracer_miura(orc_t *o, double *ttf, double *tto, const double 
             rhs[i] = ((rhs[i] - tto[i] * rhs_c[i]) / o->area[n] + tto[i] * (W[nz] - W[nz + 1]) + tv[nz] - tv[nz + 1]) / o->Jc[i];
         }
     }
+    if (g_rhs) memcpy(g_rhs, rhs, sizeof(double) * NL);
     if (o->cl_relax && (type == 't' || type == 's')) tracer_relax(o, rhs, ttf, clim, TSa, 1);
     for (size_t i = 0; i < NL; i++) { tto[i] = ttf[i] + TSa; ttf[i] = ttf[i] + rhs[i] * o->dt + TSa; }
     if (o->cl_solid_boder_relax)
@@ -1914,11 +2062,13 @@ static void solve_tracer_muscl(orc_t *o, double *ttf, double *tto, const double 
             size_t i1 = (size_t)e1 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
             const double *g = gud + ((size_t)ed * L + z) * 4;
             double Tx = 0.0, Ty = 0.0;
-            double Tm2 = tto[a2] - (2.0 * (tto[a2] - tto[a1]) + dd[0] * a * g[1] + dd[1] * R_EARTH * g[3]) / 6.0;
-            double Tm1 = tto[a1] + (2.0 * (tto[a2] - tto[a1]) + dd[0] * a * g[0] + dd[1] * R_EARTH * g[2]) / 6.0;
+            double k2 = MUT(3) ? 0.0 : 2.0, wu = MUT(1) ? 0.85 : 0.15, wc = MUT(1) ? 0.15 : 0.85;
+            double Tm2 = tto[a2] - (k2 * (tto[a2] - tto[a1]) + dd[0] * a * g[1] + dd[1] * R_EARTH * g[3]) / 6.0;
+            double Tm1 = tto[a1] + (k2 * (tto[a2] - tto[a1]) + dd[0] * a * g[0] + dd[1] * R_EARTH * g[2]) / 6.0;
             double Tm;
-            if (o->V_n_filt[i1] * dX1 - o->U_n_filt[i1] * dY1 > 0) Tm = 0.15 * Tm2 + 0.85 * (Tm1 + Tm2) / 2.0;
-            else Tm = 0.15 * Tm1 + 0.85 * (Tm1 + Tm2) / 2.0;
+            if (o->V_n_filt[i1] * dX1 - o->U_n_filt[i1] * dY1 > 0) Tm = wu * Tm2 + wc * (Tm1 + Tm2) / 2.0;
+            else Tm = wu * Tm1 + wc * (Tm1 + Tm2) / 2.0;
+            if (g_faces) g_faces[((size_t)ed * L + z) * 2] = Tm;
             double c1 = (o->Je[i1] * o->V_n_filt[i1] * Tm - Ty) * dX1 - (o->Je[i1] * o->U_n_filt[i1] * Tm - Tx) * dY1;
             rhs[a1] = rhs[a1] + c1;
             rhs[a2] = rhs[a2] - c1;
@@ -1928,12 +2078,15 @@ static void solve_tracer_muscl(orc_t *o, double *ttf, double *tto, const double 
                 size_t i2 = (size_t)e2 * L + z, a1 = (size_t)n1 * L + z, a2 = (size_t)n2 * L + z;
                 const double *g = gud + ((size_t)ed * L + z) * 4;
                 double Tx = 0.0, Ty = 0.0;
-                double Tm2 = tto[a2] - (2.0 * (tto[a2] - tto[a1]) + dd[0] * b * g[1] + dd[1] * R_EARTH * g[3]) / 6.0;
-                double Tm1 = tto[a1] + (2.0 * (tto[a2] - tto[a1]) + dd[0] * b * g[0] + dd[1] * R_EARTH * g[2]) / 6.0;
+                double k2 = MUT(3) ? 0.0 : 2.0, wu = MUT(1) ? 0.85 : 0.15, wc = MUT(1) ? 0.15 : 0.85;
+                double Tm2 = tto[a2] - (k2 * (tto[a2] - tto[a1]) + dd[0] * b * g[1] + dd[1] * R_EARTH * g[3]) / 6.0;
+                double Tm1 = tto[a1] + (k2 * (tto[a2] - tto[a1]) + dd[0] * b * g[0] + dd[1] * R_EARTH * g[2]) / 6.0;
                 double Tm;
-                if (o->V_n_filt[i2] * dX2 - o->U_n_filt[i2] * dY2 < 0) Tm = 0.15 * Tm2 + 0.85 * (Tm1 + Tm2) / 2.0;
-                else Tm = 0.15 * Tm1 + 0.85 * (Tm1 + Tm2) / 2.0;
+                if (o->V_n_filt[i2] * dX2 - o->U_n_filt[i2] * dY2 < 0) Tm = wu * Tm2 + wc * (Tm1 + Tm2) / 2.0;
+                else Tm = wu * Tm1 + wc * (Tm1 + Tm2) / 2.0;
+                if (g_faces) g_faces[((size_t)ed * L + z) * 2 + 1] = Tm;
                 double c2 = -(o->Je[i2] * o->V_n[i2] * Tm - Ty) * dX2 + (o->Je[i2] * o->U_n[i2] * Tm - Tx) * dY2;
+                if (MUT(2)) c2 = 0.0;
                 rhs[a1] = rhs[a1] + c2;
                 rhs[a2] = rhs[a2] - c2;
             }
@@ -1993,6 +2146,7 @@ static void solve_tracer_muscl(orc_t *o, double *ttf, double *tto, const double 
             rhs[i] = (rhs[i] / aj - a * rhs_c[i] / aj + a * (W[nz] - W[nz + 1]) / o->Jc[i] + (tv[nz] - tv[nz + 1]) / o->Jc[i]);
         }
     }
+    if (g_rhs) memcpy(g_rhs, rhs, sizeof(double) * NL);
     if (o->cl_relax) tracer_relax(o, rhs, ttf, clim, aver, 1);
     for (size_t i = 0; i < NL; i++) { tto[i] = ttf[i]; ttf[i] = ttf[i] + rhs[i] * o->dt; }
     for (size_t i = 0; i < NL; i++) { tto[i] = tto[i] + aver; ttf[i] = ttf[i] + aver; }

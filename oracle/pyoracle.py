@@ -64,6 +64,15 @@ def lib():
         L.orc_filter_only.restype = C.c_int
         L.orc_filter_only.argtypes = [C.c_void_p, C.c_int]
         L.orc_sw_cp.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_tracer_faces.restype = C.c_int
+        L.orc_tracer_faces.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        L.orc_wad_mask_only.restype = C.c_int
+        L.orc_wad_mask_only.argtypes = [C.c_void_p]
+        L.orc_log_cr.restype = C.c_double
+        L.orc_log_cr.argtypes = [C.c_double]
+        L.orc_set_mutation.restype = C.c_int
+        L.orc_set_mutation.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -131,6 +140,23 @@ class Oracle:
         if lib().orc_filter_only(self.h, int(which)):
             raise ValueError(f"no filter {which}")
 
+    def tracer_faces(self, scheme, T, with_rhs=False):
+        """Face values of the MUSCL (scheme 2) or Miura (scheme 4) horizontal
+        flux for the tracer field T with the current velocities, as an
+        (edge, layer, segment) array, and with ``with_rhs`` the advective
+        tendency (node, layer); the state is unchanged."""
+        D, L, N = self.size["D"], self.size["L"], self.size["N"]
+        t = np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
+        out = np.zeros(2 * D * L)
+        rhs = np.zeros(N * L)
+        if lib().orc_tracer_faces(self.h, int(scheme), _dp(t), _dp(out), _dp(rhs) if with_rhs else None):
+            raise ValueError(f"no tracer scheme {scheme}")
+        return (out.reshape(D, L, 2), rhs.reshape(N, L)) if with_rhs else out.reshape(D, L, 2)
+
+    def wad_mask_only(self):
+        """wad_mask alone on the current etaAB."""
+        lib().orc_wad_mask_only(self.h)
+
     def scalar(self, name):
         return lib().orc_scalar(self.h, SCALARS[name])
 
@@ -159,3 +185,8 @@ class Oracle:
         t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
         s = None if S is None else np.ascontiguousarray(S, dtype=np.float64).reshape(-1)
         lib().orc_set_tracers(self.h, None if t is None else _dp(t), None if s is None else _dp(s))
+
+
+def set_mutation(mid):
+    """Plant mistake ``mid`` in the oracle (0 removes it); for the pin tests."""
+    lib().orc_set_mutation(int(mid))

@@ -1,0 +1,490 @@
+"""Closed-form pins of the oracle functions that no reference-held golden file
+covers: the MUSCL and Miura face values, the tidal open-boundary elevation,
+wad_mask, the Coriolis and wind-stress terms and the sponge-layer drag.
+
+Each check states what the mathematics of the reference's formula fixes for a
+special input (a linear or cubic tracer, a basin at rest, uniform flow), so a
+plausible slip in the restatement fails it.  Each is also run against the
+oracle with such a slip planted (pyoracle.set_mutation, ids listed at
+orc_set_mutation in oracle/fesomc_oracle.c) and must then fail: that is what
+shows the check pins something.
+"""
+import math
+import re
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle, set_mutation
+
+R_EARTH = 6400000.0
+PI = 3.14159265358979
+OMEGA = 2.0 * PI / 86400.0        # src/fv_var.F90:15
+M2_PERIOD_H = 12.42               # src/fv_var.F90:171
+
+
+@pytest.fixture(autouse=True)
+def _no_mutation():
+    set_mutation(0)
+    yield
+    set_mutation(0)
+
+
+def _set_key(txt, key, val):
+    txt, n = re.subn(rf"^(\s*{key}\s*=\s*)\S+", rf"\g<1>{val}", txt, count=1, flags=re.M)
+    assert n == 1, key
+    return txt
+
+
+def _mutated(mid, check):
+    """True when ``check`` fails with mistake ``mid`` planted in the oracle."""
+    set_mutation(mid)
+    try:
+        check()
+    except AssertionError:
+        return True
+    finally:
+        set_mutation(0)
+    return False
+
+
+# ---------------------------------------------------------------------------
+# MUSCL (solve_tracer_muscl, src/fv_tracer.F90:8-260)
+# ---------------------------------------------------------------------------
+def _c1_with(scheme_adv, muscl, steps=3):
+    from fesomc_amd import meshgen
+    txt, mesh, _ = meshgen.config("c1")
+    txt = _set_key(txt, "tracer_adv", scheme_adv)
+    txt = _set_key(txt, "MUSCL_CALL", "T" if muscl else "F")
+    o = Oracle(txt, mesh)
+    o.step(steps)   # tides through the open boundary: nonuniform velocities
+    return txt, mesh, o
+
+
+def _edge_midpoints(o, mesh):
+    en = o.get_int("edge_nodes").reshape(-1, 2)
+    return 0.5 * (mesh.xy[en[:, 0]] + mesh.xy[en[:, 1]]), en
+
+
+def _check_muscl_linear(o, mesh):
+    """For a tracer linear in x and y (per layer) the node-averaged gradients
+    are exact, so both one-sided reconstructions -- and any upwind/central
+    blend of them -- equal the tracer at the edge midpoint."""
+    N, L = o.size["N"], o.size["L"]
+    z = np.arange(L)
+    T = 10.0 + 1e-3 * mesh.xy[:, 0:1] - 2e-3 * mesh.xy[:, 1:2] + 0.3 * z[None, :]
+    faces = o.tracer_faces(2, T)
+    mid, _ = _edge_midpoints(o, mesh)
+    want = 10.0 + 1e-3 * mid[:, 0:1] - 2e-3 * mid[:, 1:2] + 0.3 * z[None, :]
+    et = o.get_int("edge_tri").reshape(-1, 2)
+    assert np.abs(faces[:, :, 0] - want).max() <= 1e-11 * np.abs(want).max()
+    inner = et[:, 1] >= 0
+    assert np.abs(faces[inner, :, 1] - want[inner]).max() <= 1e-11 * np.abs(want).max()
+
+
+def test_muscl_face_value_of_a_linear_tracer_is_the_edge_midpoint_value():
+    _, mesh, o = _c1_with(2, True)
+    _check_muscl_linear(o, mesh)
+    assert _mutated(3, lambda: _check_muscl_linear(o, mesh))
+
+
+def _uniform_quads(nx=21, ny=11):
+    from fesomc_amd import meshgen
+    m = meshgen.channel(nx, ny, Lx=2000.0 * (nx - 1) / 20.0, Ly=1000.0 * (ny - 1) / 10.0, split=0.0, bumps=0,
+                        depth_west=20.0, depth_east=20.0)
+    m.ob = None
+    m.index[m.index == 2] = 1
+    return m
+
+
+def _check_muscl_cubic(o, mesh, u0):
+    """Uniform eastward flow over a tracer cubic in x on a uniform rectangle
+    grid.  On an x-edge from the western node W to the eastern node E
+    (spacing h) the node gradients are central differences, and the
+    reference's face value is its third-order (kappa = 1/3) reconstruction
+    from each side, T_W + (2(T_E-T_W) + h g_W)/6 and T_E - (2(T_E-T_W) +
+    h g_E)/6, blended 15 % upwind (the western one) and 85 % central.  On
+    y-edges the tracer does not change: the face value is the node value."""
+    N, L = o.size["N"], o.size["L"]
+    x = mesh.xy[:, 0]
+    x0 = x.mean()
+    T = np.repeat((1e-9 * (x - x0) ** 3 + 5.0)[:, None], L, axis=1)
+    faces = o.tracer_faces(2, T)
+    en = o.get_int("edge_nodes").reshape(-1, 2)
+    et = o.get_int("edge_tri").reshape(-1, 2)
+    nx = mesh.meta["nx"]
+    h = mesh.xy[1, 0] - mesh.xy[0, 0]
+    f = lambda v: 1e-9 * (v - x0) ** 3 + 5.0
+    ii = np.arange(N) % nx
+    checked = 0
+    for ed in range(len(en)):
+        a, b = en[ed]
+        if mesh.xy[a, 1] != mesh.xy[b, 1]:            # y-edge
+            want = T[a, 0]
+            assert abs(faces[ed, 0, 0] - want) <= 1e-12 * abs(want)
+            continue
+        w, e = (a, b) if mesh.xy[a, 0] < mesh.xy[b, 0] else (b, a)
+        if min(ii[w], ii[e]) < 1 or max(ii[w], ii[e]) > nx - 2:
+            continue                                   # node gradients one-sided at the walls
+        xw, xe = mesh.xy[w, 0], mesh.xy[e, 0]
+        gw = (f(xw + h) - f(xw - h)) / (2 * h)
+        ge = (f(xe + h) - f(xe - h)) / (2 * h)
+        tw = f(xw) + (2 * (f(xe) - f(xw)) + h * gw) / 6.0
+        te = f(xe) - (2 * (f(xe) - f(xw)) + h * ge) / 6.0
+        want = 0.15 * tw + 0.85 * 0.5 * (tw + te)
+        segs = [0] + ([1] if et[ed, 1] >= 0 else [])
+        for s in segs:
+            assert abs(faces[ed, 0, s] - want) <= 1e-12 * abs(want), (ed, s, faces[ed, 0, s], want)
+            checked += 1
+    assert checked > 100
+
+
+def test_muscl_face_value_of_a_cubic_tracer_is_the_upwind_biased_third_order_value():
+    from fesomc_amd import meshgen
+    mesh = _uniform_quads()
+    txt = meshgen.fv_run("q", 2.0, 20.0, 4, tracer_adv=2, muscl=True, coriolis=False)
+    o = Oracle(txt, mesh)
+    E, L = o.size["E"], o.size["L"]
+    u0 = 0.2
+    for k, v in (("U_n_filt", u0), ("V_n_filt", 0.0), ("U_n", u0), ("V_n", 0.0)):
+        o.set(k, np.full(E * L, v))
+    _check_muscl_cubic(o, mesh, u0)
+    assert _mutated(1, lambda: _check_muscl_cubic(o, mesh, u0))   # 0.85 upwind
+    assert _mutated(3, lambda: _check_muscl_cubic(o, mesh, u0))
+
+
+def _check_translation(o, mesh, scheme, u0, alpha):
+    """Uniform flow u0 along x over a tracer linear in x on a uniform
+    rectangle grid of uniform depth: at interior nodes the advective
+    tendency is exactly -u0 * alpha.  Each dual face (two segments, from the
+    edge midpoint to the two element centres) carries the face value times
+    its volume flux; the faces east and west of a node differ by alpha*h in
+    value, and the north/south faces carry no flux."""
+    N, L = o.size["N"], o.size["L"]
+    T = np.repeat((5.0 + alpha * mesh.xy[:, 0])[:, None], L, axis=1)
+    _, rhs = o.tracer_faces(scheme, T, with_rhs=True)
+    nx, ny = mesh.meta["nx"], mesh.meta["ny"]
+    ii, jj = np.arange(N) % nx, np.arange(N) // nx
+    inner = (ii >= 2) & (ii <= nx - 3) & (jj >= 1) & (jj <= ny - 2)
+    want = -u0 * alpha
+    assert np.abs(rhs[inner] / want - 1.0).max() <= 1e-9, (rhs[inner].min(), rhs[inner].max(), want)
+
+
+def _translation_case(scheme):
+    from fesomc_amd import meshgen
+    mesh = _uniform_quads()
+    txt = meshgen.fv_run("q", 2.0, 20.0, 4, tracer_adv=scheme, muscl=scheme == 2, coriolis=False)
+    o = Oracle(txt, mesh)
+    E, L, N, K = o.size["E"], o.size["L"], o.size["N"], o.size["K"]
+    u0 = 0.2
+    for k, v in (("U_n_filt", u0), ("V_n_filt", 0.0), ("U_n", u0), ("V_n", 0.0)):
+        o.set(k, np.full(E * L, v))
+    o.set("Wvel", np.zeros(N * K))
+    return o, mesh, u0
+
+
+@pytest.mark.parametrize("scheme", [2, 4])
+def test_uniform_flow_advects_a_linear_tracer_at_rate_u_dot_grad(scheme):
+    """MUSCL (2; its second segment uses the unfiltered velocity,
+    fv_tracer.F90:158-159 -- equal to the filtered one here) and Miura (4;
+    its diffusive fluxes of the uniform gradient cancel)."""
+    o, mesh, u0 = _translation_case(scheme)
+    _check_translation(o, mesh, scheme, u0, 1e-3)
+    if scheme == 2:
+        assert _mutated(2, lambda: _check_translation(o, mesh, scheme, u0, 1e-3))
+
+
+# ---------------------------------------------------------------------------
+# Miura (solve_tracer_miura, src/fv_tracer.F90:930-1316)
+# ---------------------------------------------------------------------------
+def _check_miura_linear(o, mesh, txt):
+    """For a linear tracer the Miura value is the tracer at the point the
+    reference reconstructs to: from the upwind node half way along the edge
+    and half way along the segment to the element centre (the face
+    midpoint), moved back by half the displacement (u dt, v dt) of the
+    segment's element -- with the reference's two quirks: the edge's x-length
+    enters in radians, not metres (a = 1, fv_tracer.F90:1018-1026, "SH
+    WORKAROUND"), and in the first segment with flow into edge_nodes(1) the v
+    term lacks its *dt (fv_tracer.F90:1057), moving the point back by v/2."""
+    N, L = o.size["N"], o.size["L"]
+    dt = float(re.search(r"^\s*dt\s*=\s*(\S+)", txt, re.M).group(1))
+    al, be = 1e-3, -2e-3
+    T = np.repeat((12.0 + al * mesh.xy[:, 0] + be * mesh.xy[:, 1])[:, None], L, axis=1)
+    faces = o.tracer_faces(4, T)
+    en = o.get_int("edge_nodes").reshape(-1, 2)
+    et = o.get_int("edge_tri").reshape(-1, 2)
+    el = o.get_int("elem_nodes").reshape(-1, 4)
+    # element centres (triangles: 3 nodes; rectangles: 4)
+    tri = el[:, 0] == el[:, 3]
+    cx = np.where(tri, mesh.xy[el[:, :3], 0].mean(1), mesh.xy[el, 0].mean(1))
+    cy = np.where(tri, mesh.xy[el[:, :3], 1].mean(1), mesh.xy[el, 1].mean(1))
+    x1, x2 = mesh.xy[en[:, 0]], mesh.xy[en[:, 1]]
+    mid = 0.5 * (x1 + x2)
+    U, V = o.get("U_n_filt").reshape(-1, L), o.get("V_n_filt").reshape(-1, L)
+    err = 0.0
+    for s in (0, 1):
+        ok = et[:, s] >= 0
+        e = np.where(ok, et[:, s], 0)
+        u, v = U[e], V[e]
+        dX, dY = (cx[e] - mid[:, 0])[:, None], (cy[e] - mid[:, 1])[:, None]
+        # segment 0 faces n1 with normal (dY, -dX) rotated; flow into n1's volume: v dX - u dY > 0
+        q = v * dX - u * dY
+        from_n2 = q > 0 if s == 0 else q < 0
+        up = np.where(from_n2, 1, 0)                 # 1: the upwind node is edge_nodes(2)
+        sgn = np.where(from_n2, -1.0, 1.0)
+        xu = np.where(up == 1, x2[:, 0:1], x1[:, 0:1])
+        yu = np.where(up == 1, x2[:, 1:2], x1[:, 1:2])
+        ex = ((x2[:, 0] - x1[:, 0]) / R_EARTH)[:, None]   # radians (a = 1)
+        ey = (x2[:, 1] - x1[:, 1])[:, None]
+        vdt = v * dt
+        if s == 0:
+            vdt = np.where(from_n2, v, vdt)
+        px = xu + 0.5 * (sgn * ex + dX - u * dt)
+        py = yu + 0.5 * (sgn * ey + dY - vdt)
+        want = 12.0 + al * px + be * py
+        d = np.abs(faces[:, :, s] - want)[ok]
+        err = max(err, d.max())
+    assert err <= 1e-10 * 12.0, err
+
+
+def test_miura_face_value_of_a_linear_tracer_is_its_value_at_the_departure_point():
+    txt, mesh, o = _c1_with(4, False)
+    _check_miura_linear(o, mesh, txt)
+    for mid in (4, 5, 13):
+        assert _mutated(mid, lambda: _check_miura_linear(o, mesh, txt)), mid
+
+
+# ---------------------------------------------------------------------------
+# tidal open-boundary elevation (compute_ssh_rhs_elem, fv_average_dynamic.F90:186-216)
+# ---------------------------------------------------------------------------
+def _check_ob_tide(txt, mesh, amp, phase_deg, steps, ramped):
+    """After each internal step the open-boundary nodes hold
+    amp * cos(2 pi t / T_M2 - phase), t the model time of the last substep
+    since JD0 -- the clock of step n (time_jd advanced by dt/86400 per step,
+    src/fv_main.F90:400-401, in seconds) plus Mt*dt_2D -- scaled on a cold
+    start by the ten-day ramp (time_jd - JD0)/10."""
+    o = Oracle(txt, mesh)
+    dt = float(re.search(r"^\s*dt\s*=\s*(\S+)", txt, re.M).group(1))
+    dt2 = float(re.search(r"^\s*dt_2D\s*=\s*(\S+)", txt, re.M).group(1))
+    jd0 = float(re.search(r"^\s*JD0\s*=\s*(\S+)", txt, re.M).group(1))
+    Mt = int(dt / dt2)
+    ob = mesh.index == 2
+    jd = jd0
+    for n in range(1, steps + 1):
+        o.step(1)
+        jd = jd + dt / 86400.0
+        t = (jd * 86400.0 + dt2 * Mt) - jd0 * 86400.0
+        want = amp * np.cos(t * 2.0 * PI / (M2_PERIOD_H * 3600.0) - phase_deg * PI / 180.0)
+        if ramped:
+            want = want * (jd - jd0) / 10.0
+        got = o.get("eta_n")[ob]
+        assert np.abs(got - want).max() <= 1e-12 * amp, (n, got[:3], want)
+
+
+@pytest.mark.parametrize("start", ["cold", "hot"])
+def test_open_boundary_tide_is_the_m2_harmonic(start):
+    from fesomc_amd import meshgen
+    txt, mesh, _ = meshgen.config("c1")
+    mesh.ob[:, 0] = 0.8
+    mesh.ob[:, 1] = 40.0                    # m2.out phase in degrees
+    if start == "hot":
+        txt = _set_key(txt, "RESTART", "hot_start")
+    ramped = start == "cold"
+    _check_ob_tide(txt, mesh, 0.8, 40.0, 30, ramped)
+    assert _mutated(6, lambda: _check_ob_tide(txt, mesh, 0.8, 40.0, 30, ramped))
+    if ramped:
+        assert _mutated(7, lambda: _check_ob_tide(txt, mesh, 0.8, 40.0, 30, ramped))
+
+
+# ---------------------------------------------------------------------------
+# wad_mask (src/fv_main.F90:1269-1307)
+# ---------------------------------------------------------------------------
+def _check_masks(o, Dmin):
+    """mask_wd = [min depth over the element's nodes + max etaAB > Dmin],
+    mask_wd_node = [depth + etaAB > Dmin], strictly greater."""
+    d, ea = o.get("depth"), o.get("etaAB")
+    el = o.get_int("elem_nodes").reshape(-1, 4)
+    want_e = (d[el].min(1) + ea[el].max(1) > Dmin).astype(float)
+    want_n = (d + ea > Dmin).astype(float)
+    assert np.array_equal(o.get("mask_wd"), want_e)
+    assert np.array_equal(o.get("mask_wd_node"), want_n)
+
+
+def test_wet_dry_masks_are_the_threshold_of_depth_plus_elevation():
+    """On etaAB fields made to sit on the threshold at many nodes
+    (Dmin - depth, which adds back to Dmin exactly for most depths) and on a
+    random wet/dry field; then on a tidal run that floods and dries."""
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    o = Oracle(txt, mesh)
+    Dmin = 0.25
+    d = o.get("depth")
+    on = (Dmin - d) + d == Dmin
+    assert on.sum() > 100
+    rng = np.random.default_rng(5)
+    for ea in (Dmin - d, rng.uniform(-1.0, 1.0, d.size) - np.minimum(d, 2.0)):
+        o.set("etaAB", ea)
+        o.wad_mask_only()
+        _check_masks(o, Dmin)
+        for mid in (8, 12):
+            def check():
+                o.wad_mask_only()
+                _check_masks(o, Dmin)
+            if mid == 8 and ea is not None and np.array_equal(ea, Dmin - d):
+                assert _mutated(8, check)
+            if mid == 12:
+                assert _mutated(12, check)
+
+
+def flooding_case(nsigma=8):
+    """c2_small hot-started (no ten-day ramp) with a 1 m M2 tide of phase 90
+    degrees, so the open boundary rises smoothly from zero: the shoreline of
+    the intertidal east end floods within ~100 steps and dries again after
+    ~650 (up to 93 nodes and 174 elements change state)."""
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    txt = _set_key(txt, "RESTART", "hot_start")
+    if nsigma != 8:
+        txt = _set_key(txt, "nsigma", nsigma)
+        T, S = meshgen.stratified_ts(mesh, nsigma, seed=0)
+        ex = dict(ex, T=T, S=S)
+    mesh.ob[:, 1] = 90.0
+    return txt, mesh, ex
+
+
+def test_wet_dry_masks_follow_a_moving_shoreline():
+    from fesomc_amd import meshgen
+    txt, mesh, ex = flooding_case()
+    o = Oracle(txt, mesh)
+    o.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
+    m0 = o.get("mask_wd_node").copy()
+    flips = 0
+    for _ in range(5):
+        o.step(50)
+        _check_masks(o, 0.25)
+        m = o.get("mask_wd_node")
+        flips = max(flips, int((m != m0).sum()))
+    assert flips > 30
+
+
+# ---------------------------------------------------------------------------
+# Coriolis, wind stress, sponge drag: a flat closed basin
+# ---------------------------------------------------------------------------
+def _basin(coriolis, Cd):
+    from fesomc_amd import meshgen
+    m = meshgen.channel(61, 31, split=0.3, seed=4, bumps=0, depth_west=20.0, depth_east=20.0)
+    m.ob = None
+    m.index[m.index == 2] = 1
+    txt = meshgen.fv_run("basin", 2.0, 20.0, 6, tracer_adv=1, coriolis=coriolis, Cd=Cd)
+    return txt, m
+
+
+def _interior_elements(o, mesh, margin):
+    el = o.get_int("elem_nodes").reshape(-1, 4)
+    c = mesh.xy[el].mean(1)
+    Lx, Ly = mesh.xy[:, 0].max(), mesh.xy[:, 1].max()
+    return (c[:, 0] > margin) & (c[:, 0] < Lx - margin) & (c[:, 1] > margin) & (c[:, 1] < Ly - margin)
+
+
+def _uniform_flow(o, u0):
+    E, L = o.size["E"], o.size["L"]
+    for k in ("U_n", "U_n_filt"):
+        o.set(k, np.full(E * L, u0))
+    u2 = np.zeros((E, 2))
+    u2[:, 0] = u0
+    for k in ("U_n_2D", "U_n_1", "U_n_2"):
+        o.set(k, u2.reshape(-1))
+
+
+def _check_coriolis(txt, mesh, u0=0.1):
+    """Uniform eastward flow in a flat basin at latitude 54 N turns to the
+    right: after one step of dt the interior velocity has gained
+    v = -f u0 dt, f = 2 Omega sin(54 deg) (within 1 %)."""
+    o = Oracle(txt, mesh)
+    _uniform_flow(o, u0)
+    o.step(1)
+    f = 2.0 * OMEGA * math.sin(54.0 * PI / 180.0)
+    inner = _interior_elements(o, mesh, 2000.0)
+    L = o.size["L"]
+    V = o.get("V_n").reshape(-1, L)[inner]
+    v2 = o.get("U_n_2D").reshape(-1, 2)[inner, 1]
+    want = -f * u0 * 20.0
+    assert np.abs(V / want - 1.0).max() < 0.01
+    assert np.abs(v2 / want - 1.0).max() < 0.01
+
+
+def test_coriolis_turns_uniform_flow_to_the_right_at_rate_f():
+    txt, mesh = _basin(True, 0.0)
+    _check_coriolis(txt, mesh)
+    assert _mutated(9, lambda: _check_coriolis(txt, mesh))
+
+
+def _check_wind(txt, mesh, tx=0.1):
+    """A basin at rest under a uniform wind stress tau gains, in the first
+    internal step (Mt substeps of dt_2D), the momentum
+    sum_e area_e * H_e * u_e = Mt dt_2D tau / rho0 * sum_e area_e (within
+    1 %; the wall set-up it starts is second order in the step)."""
+    o = Oracle(txt, mesh)
+    E = o.size["E"]
+    o.set("taux", np.full(E, tx))
+    o.set("tauy", np.zeros(E))
+    o.step(1)
+    rho0 = o.scalar("density_0")
+    area = o.get("elem_area")
+    u2 = o.get("U_n_2D").reshape(-1, 2)[:, 0]
+    mom = float((area * 20.0 * u2).sum())
+    want = 10 * 2.0 * tx / rho0 * float(area.sum())
+    assert abs(mom / want - 1.0) < 0.01, (mom, want)
+
+
+def test_wind_stress_adds_its_impulse_to_a_basin_at_rest():
+    txt, mesh = _basin(False, 0.0)
+    _check_wind(txt, mesh)
+    assert _mutated(10, lambda: _check_wind(txt, mesh))
+
+
+def _decay(txt, mesh, ac_value, u0=0.5):
+    o = Oracle(txt, mesh)
+    o.set("ac", np.full(o.size["N"], ac_value))
+    _uniform_flow(o, u0)
+    o.step(1)
+    inner = _interior_elements(o, mesh, 2000.0)
+    return 1.0 - o.get("U_n_2D").reshape(-1, 2)[inner, 0] / u0
+
+
+def _check_sponge_drag(txt, mesh):
+    """Bottom drag in the sponge layer is C_d ((1 - ac) Cd_bz + 1)
+    (fv_average_dynamic.F90:389-399): uniform flow over a flat bottom loses,
+    in one step, a fraction proportional to that factor, so ac = 0.5 with
+    Cd_bz = 10 decays 6 times as fast as ac = 1 (within 2 %)."""
+    r = _decay(txt, mesh, 0.5) / _decay(txt, mesh, 1.0)
+    assert np.abs(r / 6.0 - 1.0).max() < 0.02, (r.min(), r.max())
+
+
+def test_sponge_layer_scales_the_bottom_drag():
+    txt, mesh = _basin(False, 0.0025)
+    _check_sponge_drag(txt, mesh)
+    assert _mutated(11, lambda: _check_sponge_drag(txt, mesh))
+
+
+def _check_no_sponge_ignores_buffer_factors(txt, mesh, ex):
+    """Without a sponge layer (ac = 1 everywhere) Cd_bz and Bp_bz enter only
+    as (1 - acc) Cd_bz and acc**Bp_bz with acc = sum w_cv*ac = 1 to round-off:
+    the run is the same for any values (to 1e-10 of each field)."""
+    outs = []
+    for cd, bp in ((10.0, 5.0), (0.0, 0.0)):
+        t = _set_key(_set_key(txt, "Cd_bz", cd), "Bp_bz", bp)
+        o = Oracle(t, mesh)
+        o.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
+        o.step(5)
+        outs.append([o.get(f) for f in ("eta_n", "U_n", "V_n", "TF")])
+    for a, b in zip(*outs):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+
+
+def test_buffer_zone_factors_act_only_inside_the_sponge_layer():
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    txt = _set_key(txt, "SL_obn", "F")
+    _check_no_sponge_ignores_buffer_factors(txt, mesh, ex)
+    assert _mutated(11, lambda: _check_no_sponge_ignores_buffer_factors(txt, mesh, ex))
