@@ -106,7 +106,13 @@ def kernel_bytes(k, N, E, D, L, K, e3=8):
         return E * (4 * i4 * 3 + 8 * d8) + d8 * D * 2 + d8 * E + e3 * 2 * LE + e3 * 4 * LE
     if k == "k_update3d":
         return mesh_e + d8 * 3 * N + e3 * 2 * LE + e3 * LE + d8 * 4 * E + e3 * 4 * LE
+    if k == "k_mlp_closure":
+        # reads Unode, Vnode, TF, SF (L x N), a, eta, depth, ac (N); writes snu, Kv (K x N), the flag (N)
+        return e3 * 4 * LN + d8 * 4 * N + e3 * 2 * KN + i4 * N
     raise KeyError(k)
+
+
+MLP_FLOP_PER_SAMPLE = 2 * (5 * 128 + 128 * 128 + 128 * 128 + 128 * 2)   # 5 -> 128 -> 128 -> 128 -> 2
 
 
 def pmc_traffic(kernel, path):

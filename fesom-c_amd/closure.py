@@ -9,10 +9,10 @@ backward, FP32 accumulation, Adam).  PyTorch only provides device buffers and
 random minibatch indices.
 
     python -c "import __graft_entry__ as g; g.load(); from fesomc_amd import closure; closure.main()" \
-        --config c2 --steps 24 --every 532 --stride 20 --iters 12000 --out closure_c2.npz
+        --config c2 --steps 24 --every 532 --stride 12 --iters 20000 --out closure_c2.npz
 
 collects 24 steps spread over one M2 cycle of the 1M-node channel (every
-532nd internal step), every 20th column.
+532nd internal step), every 12th column: about 2.0 M columns (48 M samples).
 """
 from __future__ import annotations
 
@@ -108,7 +108,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--stride", type=int, default=10)
     ap.add_argument("--every", type=int, default=1)
-    ap.add_argument("--iters", type=int, default=12000)
+    ap.add_argument("--iters", type=int, default=20000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     from .workload import build_model
@@ -126,7 +128,7 @@ def main():
     perm = torch.tensor(perm, device="cuda")
     ntr = int(0.9 * n)
     Xtr, Ttr, Xva, Tva = X[perm[:ntr]], T[perm[:ntr]], X[perm[ntr:]], T[perm[ntr:]]
-    net = fit(Xtr, Ttr, iters=args.iters, log=lambda m: print(m, flush=True))
+    net = fit(Xtr, Ttr, iters=args.iters, batch=args.batch, lr=args.lr, log=lambda m: print(m, flush=True))
     print(f"validation loss {validation_loss(net, Xva, Tva):.4e} (normalised units), "
           f"train loss {validation_loss(net, Xtr, Ttr):.4e}, {time.time() - t0:.1f} s", flush=True)
     save(net, args.out)

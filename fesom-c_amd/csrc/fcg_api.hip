@@ -62,8 +62,8 @@ __global__ void k_mask_count(const double *, int, int *);
 template <typename R>
 __global__ void k_mlp_collect(DevMesh, DevStateT<R>, double, double, double, int, long, float *, float *);
 template <typename R>
-__global__ void k_mlp_closure(DevMesh, DevStateT<R>, const float *, const fcgm::bf16x4 *, MlpNorm, double, double, double,
-                              double, int *);
+__global__ void k_mlp_closure(DevMesh, DevStateT<R>, const fcgm::bf16x8 *, const float *, MlpNorm, double, double,
+                              double, double, int *);
 __global__ void k_count_flags(const int *, int, unsigned long long *);
 
 }  // namespace fcgk
@@ -653,6 +653,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         const size_t gather = std::max(std::max(lds_muscl<double>(ctx), lds_vdiv<double>(ctx)),
                                        std::max(lds_gradn(ctx), lds_el2n(ctx)));
         ctx->lds_limit = lds_limit;
+        FCG_CHECK(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
         if (m.K > kFlatT || std::max(lds_max(m, ctx->esize), gather) > (size_t)lds_limit) {
             fcg_fail(ctx, FCG_E_ARG, "nsigma too large for the LDS column tiles of this device");
             *out = ctx;
@@ -1049,9 +1050,12 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
         const fcg_mlp *net = ctx->mlp;
-        LAUNCH_EX(ctx, k_mlp_closure<R>, nflatblk(ctx->m.N, ctx->m.K), kFlatT, 0, ctx->m, s, (const float *)net->w,
-                  (const fcgm::bf16x4 *)net->frag, net->norm, p.Dmin, ctx->density_0, p.z0b_min, p.snul,
-                  ctx->mlp_flag);
+        // persistent blocks, one per compute unit (the network's fragments fill most of a unit's LDS)
+        const int cpb = kMlpT / ctx->m.K;
+        const int groups = (ctx->m.N + cpb - 1) / cpb;
+        LAUNCH_EX(ctx, k_mlp_closure<R>, std::min(groups, ctx->n_cu), kMlpT, kMlpClosureLds, ctx->m, s,
+                  (const fcgm::bf16x8 *)net->frag, (const float *)net->bias, net->norm, p.Dmin, ctx->density_0,
+                  p.z0b_min, p.snul, ctx->mlp_flag);
         d3end(ctx, s, (const int *)ctx->mlp_flag);
         hipLaunchKernelGGL(fcgk::k_count_flags, dim3(64), dim3(kBlk), 0, ctx->stream, (const int *)ctx->mlp_flag,
                            ctx->m.N, ctx->mlp_count);
@@ -1320,6 +1324,8 @@ extern "C" int fcg_diagnostics_member(fcg_ctx *ctx, int member, fcg_diag *out) {
 extern "C" int fcg_set_mlp(fcg_ctx *ctx, fcg_mlp *net) {
     if (!ctx) return FCG_E_ARG;
     if (ctx->p.closure != FCG_CLOSURE_MLP) return fcg_fail(ctx, FCG_E_ARG, "context was created with closure = d3_end");
+    if (net && kMlpClosureLds > (size_t)ctx->lds_limit)
+        return fcg_fail(ctx, FCG_E_ARG, "the learned closure's LDS exceeds the device limit");
     ctx->mlp = net;
     return FCG_OK;
 }

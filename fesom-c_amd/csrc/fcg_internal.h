@@ -213,6 +213,7 @@ struct fcg_ctx {
     // node-gather blocks (cpb = 256/L nodes): most CSR slots of any block
     int cap_n2e = 0, cap_n2el = 0;
     int lds_limit = 0;           // bytes of LDS one block may request on this device
+    int n_cu = 1;                // compute units of the device (persistent launches)
     // learned closure
     fcg_mlp *mlp = nullptr;      // attached network (not owned)
     int *mlp_flag = nullptr;     // N: column sent back to d3_end this step

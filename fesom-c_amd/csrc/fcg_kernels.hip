@@ -57,6 +57,38 @@ __device__ double densityJM(double t, double s, double pz, double rho0) {
     return rhopot * bulk / (bulk + 0.1 * pz) - rho0;
 }
 
+// densityJM split by level, for the learned closure's features: the
+// pressure-free parts of one (T, S) -- rhopot and the bulk modulus as
+// c0 - pz*c1 + pz^2*c2 -- formed once, then the density at any pressure from
+// them (the same polynomial, summed in another order).
+struct EosLevel {
+    double rp, c0, c1, c2;
+};
+__device__ EosLevel eos_level(double t, double s) {
+    const double ss3 = sqrt(s * s * s);
+    EosLevel e;
+    e.c0 = 19092.56 + t * (209.8925 - t * (3.041638 - t * (-(double)1.852732e-3f - t * ((double)1.361629e-5f))))
+         + s * (104.4077 - t * (6.500517 - t * (.1553190 - t * (-(double)2.326469e-4f))))
+         + ss3 * (-5.587545 + t * (0.7390729 - t * ((double)1.909078e-2f)));
+    e.c1 = ((double)4.721788e-1f + t * ((double)1.028859e-2f + t * (-(double)2.512549e-4f - t * ((double)5.939910e-7f))))
+         + s * (-(double)1.571896e-2f - t * ((double)2.598241e-4f + t * (-(double)7.267926e-6f)))
+         + ss3 * (double)2.042967e-3f;
+    e.c2 = ((double)1.045941e-5f - t * ((double)5.782165e-10f - t * ((double)1.296821e-7f)))
+         + s * (-(double)2.595994e-7f + t * (-(double)1.248266e-9f + t * (-(double)3.508914e-9f)));
+    e.rp = 999.842594
+         + t * ((double)6.793952e-2f + t * (-(double)9.095290e-3f + t * ((double)1.001685e-4f
+           + t * (-(double)1.120083e-6f + t * (double)6.536332e-9f))))
+         + s * (0.824493 + t * (-(double)4.08990e-3f + t * ((double)7.64380e-5f
+           + t * (-(double)8.24670e-7f + t * (double)5.38750e-9f))))
+         + ss3 * (-(double)5.72466e-3f + t * ((double)1.02270e-4f + t * (-(double)1.65460e-6f)))
+         + (double)4.8314e-4f * s * s;
+    return e;
+}
+__device__ __forceinline__ double eos_rho(const EosLevel &e, double pz, double rho0) {
+    const double bulk = e.c0 - pz * e.c1 + pz * pz * e.c2;
+    return e.rp * bulk / (bulk + 0.1 * pz) - rho0;
+}
+
 // XCD-aware block order: the hardware deals consecutive workgroups round-robin
 // to the 8 XCDs; remap so each XCD (own L2) walks one contiguous range of the
 // locality-ordered entities instead of every 8th block.
@@ -2863,94 +2895,123 @@ __global__ __launch_bounds__(256) void k_mlp_collect(DevMesh m, DevStateT<R> s, 
     T[i * 2 + 1] = (float)log10((double)PL(s.Kv, nz - 1, n, KC));
 }
 
-// The closure step on column-aligned blocks (cpb = 256/K whole node columns,
-// one thread per (column, level 1..K) sample).  Per column, the depth
-// factors and the bottom-stress feature are formed once; per sample the five
-// features (as column_features), normalised and checked against the
-// training box; each wavefront then runs its 64 samples through the network
-// as four 16-sample MFMA tiles.  snu and Kv = 10^(de-normalised outputs),
-// snu held in [snul, 0.2] as d3_end holds it (fv_mixing.F90:282-285), go
-// straight to the column slots, for every column whose samples all lie in
-// the box; any other column is flagged and left to d3_end.
+// The closure step.  Persistent blocks of kMlpT threads, one per compute
+// unit: the network's forward fragments and biases go to LDS once per block;
+// the block then walks a contiguous range of column groups (cpb = kMlpT/K
+// whole node columns, one thread per (column, level 1..K) sample).  Per
+// column the depth factors and the bottom-stress feature are formed once;
+// per sample the five features (as column_features), normalised and checked
+// against the training box; each wavefront then runs its 64 samples through
+// the network as four 16-sample tiles on the matrix cores.  snu and Kv =
+// 10^(de-normalised outputs), snu held in [snul, 0.2] as d3_end holds it
+// (fv_mixing.F90:282-285), go straight to the column slots, for every column
+// whose samples all lie in the box; any other column is flagged and left to
+// d3_end.
 template <typename R>
-__global__ __launch_bounds__(256) void k_mlp_closure(DevMesh m, DevStateT<R> s, const float *__restrict__ w,
-                                                     const fcgm::bf16x4 *__restrict__ frag, MlpNorm nm, double Dmin,
-                                                     double rho0, double z0b_min, double snul, int *flag) {
-    __shared__ float xs[256 * kMlpF];
-    __shared__ float ys[256 * 2];
-    __shared__ double col_a[256], col_dm[256], col_f3[256];
-    __shared__ int col_ood[256];
+__global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s, const fcgm::bf16x8 *__restrict__ frag,
+                                                       const float *__restrict__ bias, MlpNorm nm, double Dmin,
+                                                       double rho0, double z0b_min, double snul, int *flag) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    fcgm::bf16x8 *fl = (fcgm::bf16x8 *)lds_;
+    float *bl = (float *)(fl + kFragFwd * 64);
+    float *xs = bl + kMlpBias;
+    float *ys = xs + kMlpT * kMlpF;
+    double *col_a = (double *)(ys + kMlpT * 2), *col_dm = col_a + kMlpT / 4, *col_f3 = col_dm + kMlpT / 4;
+    EosLevel *lv = (EosLevel *)(col_f3 + kMlpT / 4);   // per (column, level): density coefficients
+    double2 *uv = (double2 *)(lv + kMlpT);             // and the node-averaged velocity
+    int *col_ood = (int *)(uv + kMlpT);
     const int K = m.K, L = m.L, KC = K;
-    const ColFlat f = col_flat(m.N, K);
-    if (f.nc <= 0) return;
-    if ((int)threadIdx.x < f.nc) {
-        const int n = f.c0 + threadIdx.x;
-        const double a = s.a[n];
-        const double dmean = dmaxr(Dmin, m.depth[n] + s.eta_n[n]);
-        const double ub = PL(s.Unode, L - 1, n, KC), vb = PL(s.Vnode, L - 1, n, KC);
-        const double delta_b = a * m.dsig[L - 1] + z0b_min;
-        const double lg = log(delta_b / z0b_min);
-        col_a[threadIdx.x] = a;
-        col_dm[threadIdx.x] = dmean;
-        col_f3[threadIdx.x] = log10f((float)(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12));
-        // columns of the open-boundary sponge layer (ac < 1), whose damped
-        // dynamics the network is not trained for, keep d3_end
-        col_ood[threadIdx.x] = m.ac[n] < 1.0 ? 1 : 0;
-    }
-    __syncthreads();
-    const int n = f.c0 + f.c, nz = f.k + 1;
-    // a block whose columns all lie in the sponge layer skips the network
-    if (__syncthreads_and(!f.on() || col_ood[f.c] != 0)) {
-        if (f.on() && f.k == 0) flag[n] = 1;
-        return;
-    }
-    float fx[kMlpF] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (f.on()) {
-        const double a = col_a[f.c], dmean = col_dm[f.c];
-        const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
-        const double u1 = PL(s.Unode, iz - 2, n, KC), v1 = PL(s.Vnode, iz - 2, n, KC);
-        const double u2 = PL(s.Unode, iz - 1, n, KC), v2 = PL(s.Vnode, iz - 1, n, KC);
-        const double d1 = m.sigma[iz - 2] - m.sigma[iz - 1], d2 = m.sigma[iz - 1] - m.sigma[iz];
-        const double d = 0.5 * (d1 + d2);
-        const double sh2 = ((u1 - u2) * (u1 - u2) + (v1 - v2) * (v1 - v2)) / (d * d) / (dmean * dmean);
-        const double zb = m.oms[iz - 1] * a;
-        const double rho_up = densityJM(PL(s.TF, iz - 2, n, KC), PL(s.SF, iz - 2, n, KC), -zb, rho0);
-        const double rho_dn = densityJM(PL(s.TF, iz - 1, n, KC), PL(s.SF, iz - 1, n, KC), -zb, rho0);
-        const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
-        fx[0] = log10f((float)(sh2 + 1e-12));
-        fx[1] = asinhf((float)(n2 * 1e5));
-        fx[2] = log10f((float)dmean);
-        fx[3] = (float)col_f3[f.c];
-        fx[4] = (float)m.sigma[nz - 1];
-        bool ood = false;
-        for (int j = 0; j < kMlpF; ++j) {
-            fx[j] = (fx[j] - nm.x_mean[j]) / nm.x_std[j];
-            ood = ood || !(fx[j] >= nm.x_lo[j] && fx[j] <= nm.x_hi[j]);
+    const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
+    for (int i = tid; i < kFragFwd * 64; i += kMlpT) fl[i] = frag[i];
+    for (int i = tid; i < kMlpBias; i += kMlpT) bl[i] = bias[i];
+    const int cpb = kMlpT / K;
+    const int ngroups = (m.N + cpb - 1) / cpb;
+    const int per = (ngroups + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int g0 = (int)xcd_block() * per, g1 = min(ngroups, g0 + per);
+    const int c = tid / K, k = tid - (tid / K) * K;
+    for (int g = g0; g < g1; ++g) {
+        __syncthreads();  // fragments loaded / the previous group's LDS read
+        const int c0 = g * cpb, nc = min(cpb, m.N - c0);
+        const bool on = c < nc;
+        const int n = c0 + c, nz = k + 1;
+        if (tid < nc) {
+            const int nn = c0 + tid;
+            const double a = s.a[nn];
+            const double dmean = dmaxr(Dmin, m.depth[nn] + s.eta_n[nn]);
+            const double ub = PL(s.Unode, L - 1, nn, KC), vb = PL(s.Vnode, L - 1, nn, KC);
+            const double delta_b = a * m.dsig[L - 1] + z0b_min;
+            const double lg = log(delta_b / z0b_min);
+            col_a[tid] = a;
+            col_dm[tid] = dmean;
+            col_f3[tid] = log10f((float)(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12));
+            // columns of the open-boundary sponge layer (ac < 1), whose damped
+            // dynamics the network is not trained for, keep d3_end
+            col_ood[tid] = m.ac[nn] < 1.0 ? 1 : 0;
         }
-        if (ood) atomicOr(&col_ood[f.c], 1);
-    }
-    for (int j = 0; j < kMlpF; ++j) xs[threadIdx.x * kMlpF + j] = fx[j];
-    __syncthreads();
-    // the wavefront's 64 samples as four tiles
-    const int lane = threadIdx.x & 63, wbase = threadIdx.x & ~63;
-    for (int t = 0; t < 4; ++t) {
-        const fcgm::f32x4 y = fcgm::fwd_infer(frag, w, fcgm::load_bx(&xs[(wbase + t * 16) * kMlpF], lane), lane);
-        if (lane < 16) {
-            ys[(wbase + t * 16 + lane) * 2] = y[0];
-            ys[(wbase + t * 16 + lane) * 2 + 1] = y[1];
+        if (on && k < L) {  // each level's T, S and velocity once (thread tid = c*K + k)
+            lv[tid] = eos_level(PL(s.TF, k, n, KC), PL(s.SF, k, n, KC));
+            uv[tid] = make_double2(PL(s.Unode, k, n, KC), PL(s.Vnode, k, n, KC));
         }
+        __syncthreads();
+        // a group whose columns all lie in the sponge layer skips the network
+        if (__syncthreads_and(!on || col_ood[c] != 0)) {
+            if (on && k == 0) flag[n] = 1;
+            continue;
+        }
+        float fx[kMlpF] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (on) {
+            const double a = col_a[c], dmean = col_dm[c];
+            const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
+            const int up = c * K + iz - 2, dn = up + 1;                 // its two layers in LDS
+            const double2 w1 = uv[up], w2 = uv[dn];
+            const double d1 = m.sigma[iz - 2] - m.sigma[iz - 1], d2 = m.sigma[iz - 1] - m.sigma[iz];
+            const double d = 0.5 * (d1 + d2);
+            const double sh2 = ((w1.x - w2.x) * (w1.x - w2.x) + (w1.y - w2.y) * (w1.y - w2.y)) / (d * d) /
+                               (dmean * dmean);
+            const double zb = m.oms[iz - 1] * a;
+            const double rho_up = eos_rho(lv[up], -zb, rho0), rho_dn = eos_rho(lv[dn], -zb, rho0);
+            const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
+            fx[0] = log10f((float)(sh2 + 1e-12));
+            fx[1] = asinhf((float)(n2 * 1e5));
+            fx[2] = log10f((float)dmean);
+            fx[3] = (float)col_f3[c];
+            fx[4] = (float)m.sigma[nz - 1];
+            bool ood = false;
+            for (int j = 0; j < kMlpF; ++j) {
+                fx[j] = (fx[j] - nm.x_mean[j]) / nm.x_std[j];
+                ood = ood || !(fx[j] >= nm.x_lo[j] && fx[j] <= nm.x_hi[j]);
+            }
+            if (ood) atomicOr(&col_ood[c], 1);
+        }
+        for (int j = 0; j < kMlpF; ++j) xs[tid * kMlpF + j] = fx[j];
+        // the features stay within the wavefront: no block barrier, so one
+        // wavefront's matrix products overlap another's feature arithmetic
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        {  // the wavefront's 64 samples as four tiles
+            fcgm::bf16x8 x[4];
+            for (int t = 0; t < 4; ++t) x[t] = fcgm::load_bx(&xs[(wbase + 16 * t) * kMlpF], lane);
+            fcgm::f32x4 y[4];
+            fcgm::infer_tiles<4>(fl, bl, x, lane, y);
+            if (lane < 16)
+                for (int t = 0; t < 4; ++t) {
+                    ys[(wbase + 16 * t + lane) * 2] = y[t][0];
+                    ys[(wbase + 16 * t + lane) * 2 + 1] = y[t][1];
+                }
+        }
+        __syncthreads();
+        if (!on) continue;
+        const bool ood = col_ood[c] != 0;
+        if (k == 0) flag[n] = ood ? 1 : 0;
+        if (ood) continue;
+        const float ls = ys[tid * 2] * nm.y_std[0] + nm.y_mean[0];
+        const float lk = ys[tid * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
+        double sv = (double)exp10f(ls);
+        sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
+        PL(s.snu, nz - 1, n, KC) = (R)sv;
+        PL(s.Kv, nz - 1, n, KC) = (R)exp10f(lk);
     }
-    __syncthreads();
-    if (!f.on()) return;
-    const bool ood = col_ood[f.c] != 0;
-    if (f.k == 0) flag[n] = ood ? 1 : 0;
-    if (ood) return;
-    const float ls = ys[threadIdx.x * 2] * nm.y_std[0] + nm.y_mean[0];
-    const float lk = ys[threadIdx.x * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
-    double sv = (double)exp10f(ls);
-    sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
-    PL(s.snu, nz - 1, n, KC) = (R)sv;
-    PL(s.Kv, nz - 1, n, KC) = (R)exp10f(lk);
 }
 
 __global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, unsigned long long *count) {

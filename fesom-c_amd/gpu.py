@@ -35,7 +35,7 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member", "fcg_set_launch",
            "fcg_get_launch", "fcg_hot_start", "fcg_mlp_gradient_static"]
 
-MLP_FEATURES, MLP_PARAMS = 5, 6288
+MLP_FEATURES, MLP_PARAMS = 5, 39312
 
 
 class Params(C.Structure):
@@ -500,7 +500,7 @@ class Mlp:
 
     def gradient_static(self, X_ptr, T_ptr, n, G_ptr, stream=None):
         """fcg_mlp_gradient_static: the gradient into the device buffer at
-        G_ptr (6288 floats) without any device allocation, launched on the
+        G_ptr (39312 floats) without any device allocation, launched on the
         HIP stream handle ``stream`` (None: the network's own stream) and not
         synchronised."""
         self._ck(lib().fcg_mlp_gradient_static(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), n, C.c_void_p(G_ptr),

@@ -259,8 +259,8 @@ int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches);
  * (src/fv_mixing.F90:73-347, called at src/fv_main.F90:2025) when a context
  * is created with closure = FCG_CLOSURE_MLP.  The network maps five features
  * of each (column, level) -- log10 shear^2, asinh(1e5 N^2), log10 depth,
- * log10 bottom stress, sigma -- to log10 snu and log10 Kv (5 -> 64 -> 64 -> 2,
- * tanh, BF16 matrix-core products with FP32 accumulation).  Columns whose
+ * log10 bottom stress, sigma -- to log10 snu and log10 Kv (5 -> 128 -> 128 ->
+ * 128 -> 2, ReLU, BF16 matrix-core products with FP32 accumulation).  Columns whose
  * normalised features leave the box [x_lo, x_hi], and columns of the
  * open-boundary sponge layer (ac < 1), fall back to d3_end; the number of
  * such columns summed over all steps is fcg_diag.mlp_fallback.
@@ -269,8 +269,8 @@ int fcg_probe_read(fcg_ctx *ctx, double *total_ms, int *launches);
  * normalised targets, float, row-major) owned by the caller. */
 int fcg_mlp_create(unsigned seed, fcg_mlp **out);        /* Glorot init */
 void fcg_mlp_destroy(fcg_mlp *net);
-/* Flat FP32 parameters W1[64][16] b1[64] W2[64][64] b2[64] W3[16][64] b3[16]
- * (count 6288); padding entries stay zero. */
+/* Flat FP32 parameters W1[128][32] b1[128] W2[128][128] b2[128] W3[128][128]
+ * b3[128] W4[16][128] b4[16] (count 39312); padding entries stay zero. */
 int fcg_mlp_get_params(fcg_mlp *net, float *host, long count);
 int fcg_mlp_set_params(fcg_mlp *net, const float *host, long count);
 /* Feature/target normalisation x' = (x - mean)/std, y = y'*std + mean, and
@@ -279,15 +279,15 @@ int fcg_mlp_set_norm(fcg_mlp *net, const float *x_mean, const float *x_std, cons
                      const float *y_std, const float *x_lo, const float *x_hi);
 int fcg_mlp_forward(fcg_mlp *net, const float *X_dev, float *Y_dev, long n);   /* synchronous */
 /* Gradient of 0.5*mean_i |f(X_i) - T_i|^2 with respect to the parameters,
- * copied to grad_host (count 6288); synchronous. */
+ * copied to grad_host (count 39312); synchronous. */
 int fcg_mlp_gradient(fcg_mlp *net, const float *X_dev, const float *T_dev, long n, float *grad_host);
 /* The same gradient without any device allocation: no counterpart in the
  * reference; for callers that cannot allocate device memory (code under HIP
- * graph capture, tight training loops).  Writes the 6288 gradient entries to
+ * graph capture, tight training loops).  Writes the 39312 gradient entries to
  * the caller's device buffer grad_dev, launching on `stream` (a hipStream_t;
  * NULL = the network's own stream) without synchronising.  The per-wave
- * partial sums live in a file-scope device array of the library (256 x 6288
- * floats, 6.4 MB per GPU, held until the process ends); it serves any batch
+ * partial sums live in a file-scope device array of the library (128 x 39312
+ * floats, 20 MB per GPU, held until the process ends); it serves any batch
  * since each wave sums several 16-sample tiles.  n must lie in
  * 1..FCG_MLP_STATIC_MAX_SAMPLES (FCG_E_ARG otherwise).  Calls that overlap
  * on one GPU race on that array.  The result equals fcg_mlp_gradient's bit

@@ -26,7 +26,8 @@ def _torch():
 
 
 GROUPS = {"W1": (mlp_ref.W1, mlp_ref.B1), "b1": (mlp_ref.B1, mlp_ref.W2), "W2": (mlp_ref.W2, mlp_ref.B2),
-          "b2": (mlp_ref.B2, mlp_ref.W3), "W3": (mlp_ref.W3, mlp_ref.B3), "b3": (mlp_ref.B3, mlp_ref.P)}
+          "b2": (mlp_ref.B2, mlp_ref.W3), "W3": (mlp_ref.W3, mlp_ref.B3), "b3": (mlp_ref.B3, mlp_ref.W4),
+          "W4": (mlp_ref.W4, mlp_ref.B4), "b4": (mlp_ref.B4, mlp_ref.P)}
 
 
 @pytest.mark.gpu
