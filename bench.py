@@ -1,25 +1,28 @@
 """Benchmark: node-level-steps/s of the FESOM-C internal step on one MI355X.
 
-Workload (the 1M-node channel): seeded 20x10 km channel, 1415x708 = 1,001,820
-nodes, 90 % of quads split (~1.9M elements), 24 sigma levels (23 layers),
-wetting/drying on (depth -2..30 m, 15 % intertidal), stratified T/S with
-seeded smooth noise, seeded uniform wind, M2 tide at the open west boundary,
-MUSCL tracers.  One node-level-step = one 2-D node x one layer advanced by one
-internal step (including its Mt barotropic substeps).
-
-Default (--path headline): the mixed-precision step with the learned closure
-(weights fesom-c_amd/closure_c2.npz, trained on columns of the plain model by
-fesomc_amd.closure) is timed, then the plain FP64 drop-in on the same
-workload; the line reports both rates, the speed-up and the SSH/velocity/T/S
-differences between the two after the same steps.  --path fp64 / mixed time
-one path alone.
+Default workload (--workload c5): the benchmark's ensemble -- 64 seeded
+channels (20x10 km) of 50k..400k nodes each, 14.93M nodes and 28.4M elements
+in all, with their own number of sigma levels (16, 20, 24, 28 or 32; 330.7M
+node-levels), tidal amplitude (+-10 %) and wind (+-3 m/s per component);
+wetting/drying, MUSCL tracers, stratified T/S with seeded smooth noise, 3 km
+open-boundary sponge, dt 5.5 s, 10 barotropic substeps.  Members that share a
+level count share one context (one launch per stage over all of them); the
+five contexts step together.  The headline path is mixed precision with the
+learned closure in every member (weights fesom-c_amd/closure_c2.npz, trained
+on columns of the plain 1M-node channel), timed against the plain FP64
+drop-in on the same ensemble.  --workload c2 times the single 1M-node channel
+instead (1415x708 nodes, 24 levels).  One node-level-step = one 2-D node x one
+layer advanced by one internal step (including its barotropic substeps).
 
 Each timed region holds exactly --steps internal steps with all inputs
-resident in HBM, bracketed by synchronisation on both sides.  The probed
-kernel is timed with HIP events on the library's stream (fcg_probe) and
-reported against its algorithmic HBM bytes; traffic comes from the committed
-copy-calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes.  The CPU baseline is
-the FP64 oracle (scalar C, one core) on a bounded sample of the same physics.
+resident in HBM, bracketed by synchronisation on both sides.  The dominant
+kernel (--probe) is timed with HIP events on the library's streams during the
+timed region; a second kernel (--probe2) is timed the same way over two
+further steps of both runs.  The line reports the learned closure against the
+BF16 matrix-core peak and the momentum kernel against the HBM peak, the
+errors of the headline path against the FP64 run after the same steps, and
+the CPU baseline: the FP64 oracle (scalar C, one core) on the CPU
+configuration (the 5.1k-node channel).
 """
 import argparse
 import json
@@ -34,38 +37,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-
-
-def field_errors(model, ref, masks=None):
-    """RMS and max differences of SSH, velocity and T/S between two runs, over
-    all nodes/elements, or over the (node, element) masks given."""
-    errs = {}
-    for f, key, where in (("eta_n", "ssh_m", 0), ("U_n", "u_m_s", 1), ("V_n", "v_m_s", 1), ("TF", "t_degC", 0),
-                          ("SF", "s_psu", 0)):
-        a, b = model.get(f), ref.get(f)
-        d = a - b
-        if masks is not None:
-            m = masks[where]
-            d = d.reshape(m.size, -1)[m]
-        errs[key + "_rms"] = float(np.sqrt(np.mean(d ** 2)))
-        errs[key + "_max"] = float(np.max(np.abs(d)))
-    return errs
-
-
-def interior_masks(mesh, txt):
-    """(node, element) masks of the part of the mesh outside the open-boundary
-    sponge layer (ac == 1), elements in the reference's order (triangles
-    first); None when the run has no sponge layer."""
-    from fesomc_amd import gpu
-    from fesomc_amd.runconfig import RunConfig
-    ac = gpu.mesh_arrays(RunConfig(txt), mesh)["ac"]
-    if np.all(ac == 1.0):
-        return None
-    nm = ac == 1.0
-    tri = mesh.elem[:, 0] == mesh.elem[:, 3]
-    order = np.concatenate([np.where(tri)[0], np.where(~tri)[0]])
-    em = nm[mesh.elem[order] - 1].all(axis=1)
-    return nm, em
 
 
 def kernel_bytes(k, N, E, D, L, K, e3=8):
@@ -128,36 +99,115 @@ def pmc_traffic(kernel, path):
     return None
 
 
-def cpu_baseline(budget_s=15.0):
-    """FP64 oracle (scalar C, 1 core) on the same physics at 354x177 nodes."""
-    import numpy as np
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=20.0):
+    """The FP64 oracle (scalar C restatement, one core) on the CPU
+    configuration: the 5.1k-node channel, 10 sigma levels, upwind tracers,
+    M2 tide, timed over internal steps after one warm-up step."""
     import __graft_entry__ as ge
     ge.load()
     from fesomc_amd import meshgen
     from oracle.pyoracle import Oracle
-    scale = (354, 177)
-    txt, mesh, ex = meshgen.config("c2", scale=scale)
+    txt, mesh, ex = meshgen.config("c1")
     o = Oracle(txt, mesh)
-    o.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
-    wx, wy = ex["wind"]
-    tx, ty = meshgen.wind_stress(wx, wy)
-    o.set("windx", np.full(mesh.nod2D, wx))
-    o.set("windy", np.full(mesh.nod2D, wy))
-    o.set("taux", np.full(mesh.elem2D, tx))
-    o.set("tauy", np.full(mesh.elem2D, ty))
     o.step(1)
     n = 0
     t0 = time.perf_counter()
     while True:
         o.step(1)
         n += 1
-        if time.perf_counter() - t0 > budget_s or n >= 200:
+        if time.perf_counter() - t0 > budget_s or n >= 1000:
             break
     dt = time.perf_counter() - t0
     N, L = o.size["N"], o.size["L"]
     return {"value": N * L * n / dt, "unit": "node-level-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle (scalar C restatement), config-2 physics on a {scale[0]}x{scale[1]} channel "
-                      f"({N} nodes, {L} layers), {n} internal steps after 1 warm-up step, {dt:.1f} s"}
+            "host_cpu": cpu_model(), "host_nproc": os.cpu_count(),
+            "sample": f"FP64 oracle (scalar C restatement, one thread) on the CPU configuration: c1 channel "
+                      f"{N} nodes, {L} layers, upwind tracers, M2 tide; {n} internal steps after 1 warm-up step, "
+                      f"{dt:.1f} s"}
+
+
+def mlp_samples(model_parts):
+    """Samples of the learned closure per step: every level of every column
+    outside the open-boundary sponge layer (a lower bound: groups of columns
+    that touch the layer run the network on all of theirs)."""
+    tot = 0
+    for part in model_parts:
+        k = part.K
+        for v in getattr(part, "views", None) or [part.view]:
+            ac = np.ctypeslib.as_array(v.ac, shape=(v.nod2D,))
+            tot += int((ac >= 1.0).sum()) * k
+    return tot
+
+
+def ensemble_errors(a_fields, b_fields):
+    errs = {}
+    for key in a_fields:
+        d = a_fields[key] - b_fields[key]
+        errs[key + "_rms"] = float(np.sqrt(np.mean(d ** 2)))
+        errs[key + "_max"] = float(np.max(np.abs(d)))
+    return errs
+
+
+ERR_FIELDS = (("eta_n", "ssh_m"), ("U_n", "u_m_s"), ("V_n", "v_m_s"), ("TF", "t_degC"), ("SF", "s_psu"))
+
+
+MFMA_BF16_PEAK_TFS = 2500.0   # MI355X dense BF16 matrix-core peak
+
+
+def part_list(model):
+    return model.parts if hasattr(model, "parts") else [model]
+
+
+def probe_parts(model, kernel, steps):
+    """Time `kernel` with HIP events over `steps` further (untimed) steps:
+    per part, (summed ms, launches)."""
+    for p in part_list(model):
+        p.probe(kernel)
+    model.step(steps)
+    model.sync()
+    out = []
+    for p in part_list(model):
+        p.probe(None)
+        out.append(p.probe_read())
+    return out
+
+
+def roofline_hbm(kernel, parts, reads, e3, traffic=None):
+    """Algorithmic bytes per step of `kernel` over all parts against the summed
+    event time of its launches."""
+    byts = sum(kernel_bytes(kernel, p.N, p.E, p.D, p.L, p.K, e3) * n for p, (_, n) in zip(parts, reads))
+    ms = sum(t for t, _ in reads)
+    launches = sum(n for _, n in reads)
+    achieved = byts / (ms * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": ms / max(launches, 1),
+            "launches": launches, "algorithmic_bytes": byts / max(launches, 1)}
+
+
+def roofline_mlp(parts, reads):
+    """Useful matrix-core work of the learned closure (the network's FLOPs on
+    the samples outside the sponge layer) against the summed event time."""
+    flop = sum(mlp_samples([p]) * MLP_FLOP_PER_SAMPLE * n for p, (_, n) in zip(parts, reads))
+    ms = sum(t for t, _ in reads)
+    launches = sum(n for _, n in reads)
+    achieved = flop / (ms * 1e-3) / 1e12
+    return {"kernel": "k_mlp_closure", "bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFS,
+            "unit": "TFLOP/s", "frac": achieved / MFMA_BF16_PEAK_TFS, "traffic": None, "avg_ms": ms / max(launches, 1),
+            "launches": launches, "flop_per_launch": flop / max(launches, 1)}
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -165,19 +215,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--workload", choices=["c5", "c2"], default="c5",
+                    help="c5: the 64-member ensemble with the learned closure (default); c2: the 1M-node channel")
     ap.add_argument("--scale", default=None, metavar="NX,NY",
-                    help="mesh nodes along and across the channel instead of the config's (same generator)")
-    ap.add_argument("--probe", default="k_velrhs", help="kernel timed live for the roofline (the dominant one)")
-    ap.add_argument("--error-steps", default="0", metavar="N|m2",
-                    help="after the timed run, advance both the headline and the FP64 model (untimed) to N internal "
-                         "steps in total ('m2': one M2 tidal period, 44712 s) and report their differences then")
+                    help="c2 only: mesh nodes along and across the channel instead of the config's")
+    ap.add_argument("--probe", default=None, help="kernel timed during the timed region (default: the dominant "
+                                                  "one of the path: k_mlp_closure with the learned closure)")
+    ap.add_argument("--probe2", default="k_velrhs", help="kernel timed over two further steps (HBM roofline)")
     ap.add_argument("--path", choices=["headline", "mixed-mlp", "fp64", "mixed"], default="headline",
                     help="headline: mixed precision + learned closure, timed beside the plain FP64 drop-in")
     ap.add_argument("--closure-weights", default=os.path.join(ROOT, "fesom-c_amd", "closure_c2.npz"))
     ap.add_argument("--launch-table", default="auto",
                     help="block-size table from fesomc_amd.autotune (JSON); 'auto' = the stored table of this "
-                         "config and precision if there is one, 'none' = library defaults")
+                         "workload and precision if there is one, 'none' = library defaults")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
@@ -194,62 +244,107 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs one FESOM-C mesh partition on one GPU; --gpus {args.gpus} is not "
-              "supported (the multi-GPU halo-exchange design is documented, not built)", file=sys.stderr)
+        print(f"bench.py: this build runs one FESOM-C mesh partition (or one ensemble) on one GPU; --gpus "
+              f"{args.gpus} is not supported (the multi-GPU halo-exchange design is documented, not built)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import __graft_entry__ as ge
     ge.load()
-    from fesomc_amd import autotune, closure, gpu
-    from fesomc_amd.workload import build_model
+    from fesomc_amd import autotune, closure, gpu, meshgen
+    from fesomc_amd.runconfig import RunConfig
+    from fesomc_amd.workload import EnsembleSet, build_model
 
+    t_setup = time.perf_counter()
     held = {}
+    if args.workload == "c5":
+        classes = meshgen.ensemble_classes("c4")
+        banks = [gpu.build_meshes(RunConfig(c[1]), c[2]) for c in classes]
+    t_mesh = time.perf_counter() - t_setup
+    log(f"meshes ready in {t_mesh:.1f} s")
+    net = None
 
-    def run(precision, clos, probe):
-        t_setup = time.perf_counter()
+    def make(precision, clos):
+        nonlocal net
+        if clos == gpu.FCG_CLOSURE_MLP and net is None:
+            net = closure.load(args.closure_weights)
+        if args.workload == "c5":
+            return EnsembleSet(classes, precision=precision, closure=clos, banks=banks,
+                               net=net if clos == gpu.FCG_CLOSURE_MLP else None)
         scale = tuple(int(v) for v in args.scale.split(",")) if args.scale else None
-        txt, mesh, ex, model = build_model(args.config, scale=scale, precision=precision, closure=clos)
+        txt, mesh, ex, model = build_model("c2", scale=scale, precision=precision, closure=clos)
         held["mesh"], held["txt"] = mesh, txt
         if clos == gpu.FCG_CLOSURE_MLP:
-            model.attach_mlp(closure.load(args.closure_weights))
+            model.attach_mlp(net)
         table = args.launch_table
         if table == "auto":
-            table = os.path.join(ROOT, "fesom-c_amd", f"launch_{'mixed' if precision else 'fp64'}_{args.config}.json")
+            table = os.path.join(ROOT, "fesom-c_amd", f"launch_{'mixed' if precision else 'fp64'}_c2.json")
             table = table if os.path.exists(table) else "none"
         if table != "none":
             autotune.apply(model, autotune.load(table))
-        setup = time.perf_counter() - t_setup
+        return model
+
+    def run(precision, clos, probe):
+        t0s = time.perf_counter()
+        model = make(precision, clos)
+        setup = time.perf_counter() - t0s
+        log(f"precision {precision} closure {clos}: model ready in {setup:.1f} s")
         model.step(args.warmup)
         model.sync()
         torch.cuda.synchronize()
-        model.probe(probe)
+        for p in part_list(model):
+            p.probe(probe)
         t0 = time.perf_counter()
         model.step(args.steps)
         model.sync()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        model.probe(None)
-        k_ms, k_n = model.probe_read()
+        log(f"precision {precision} closure {clos}: {args.steps} steps in {t1 - t0:.3f} s")
+        reads = []
+        for p in part_list(model):
+            p.probe(None)
+            reads.append(p.probe_read())
+        reads2 = probe_parts(model, args.probe2, 2)
         diag = model.diagnostics()
         if diag["status"] != 0 or not math.isfinite(diag["energy"]):
             print(json.dumps({"error": "model state invalid", "diag": diag}), file=sys.stderr)
             sys.exit(1)
-        return model, t1 - t0, k_ms / max(k_n, 1), k_n, setup, diag
+        return model, t1 - t0, reads, reads2, setup, diag
 
     headline = args.path in ("headline", "mixed")   # paths compared against the FP64 drop-in
     prec = 0 if args.path == "fp64" else 1
     clos = gpu.FCG_CLOSURE_MLP if args.path in ("headline", "mixed-mlp") else gpu.FCG_CLOSURE_D3END
-    model, sec, avg_ms, k_n, setup_s, diag = run(prec, clos, args.probe)
-    N, E, D, L, K = model.N, model.E, model.D, model.L, model.K
-    nls = N * L * args.steps / sec
+    probe = args.probe or ("k_mlp_closure" if clos == gpu.FCG_CLOSURE_MLP else "k_velrhs")
+    model, sec, reads, reads2, setup_s, diag = run(prec, clos, probe)
+    parts = part_list(model)
+    NL = sum(p.N * p.L for p in parts)
+    N, E, D = sum(p.N for p in parts), sum(p.E for p in parts), sum(p.D for p in parts)
+    nls = NL * args.steps / sec
     e3 = 4 if prec else 8
-    kb = kernel_bytes(args.probe, N, E, D, L, K, e3)
-    achieved = kb / (avg_ms * 1e-3) / 1e9
     tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{'mixed' if prec else 'fp64'}_step_c2.md")
-    desc = {"headline": "mixed precision (FP32 3-D fluxes/tendencies; FP64 elevation/volume, wet/dry, BPG) "
-                        "+ learned closure (BF16 MFMA MLP, d3_end fallback)",
+    traffic = (lambda k: pmc_traffic(k, tfile)) if (args.workload == "c2" and not args.scale) else (lambda k: None)
+    if probe == "k_mlp_closure":
+        roof = roofline_mlp(parts, reads)
+    else:
+        roof = roofline_hbm(probe, parts, reads, e3, traffic(probe))
+    roof2 = roofline_hbm(args.probe2, parts, reads2, e3, traffic(args.probe2))
+    desc = {"headline": "mixed precision (FP32 3-D fields, fluxes and tendencies; FP64 elevation/volume, wet/dry, "
+                        "BPG) + learned closure (5-128-128-128-2 BF16 MFMA network, d3_end fallback)",
             "fp64": "plain FP64 drop-in", "mixed": "mixed precision, d3_end closure",
             "mixed-mlp": "mixed precision + learned closure (not compared)"}[args.path]
+    if args.workload == "c5":
+        cls = [{"nsigma": c[0], "members": len(c[2]), "nodes": int(sum(m.nod2D for m in c[2]))} for c in classes]
+        workload = (f"c5: 64-member channel ensemble, 50k-400k nodes per member ({N} nodes, {E} elements, {D} "
+                    f"edges), 16-32 sigma levels in {len(classes)} classes ({NL} node-levels), Mt {model.par.Mt}, "
+                    f"dt {model.par.dt} s, wet/dry, MUSCL tracers, tide amplitude +-10 %, wind +-3 m/s per member; "
+                    f"{desc}")
+        config = {"workload": workload, "members": model.members, "nodes": N, "elements": E, "node_levels": NL,
+                  "classes": cls, "Mt": model.par.Mt, "path": args.path}
+    else:
+        workload = (f"c2: channel {N} nodes / {E} elements / {D} edges, nsigma {model.K}, Mt {model.par.Mt}, "
+                    f"wet/dry, MUSCL tracers; {desc}")
+        config = {"workload": workload, "nodes": N, "elements": E, "layers": model.L, "Mt": model.par.Mt,
+                  "path": args.path}
     out = {
         "metric": "node-level-steps/s, mixed+learned vs plain FP64 GPU at <=1cm SSH RMS",
         "value": nls,
@@ -261,51 +356,40 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if not prec else "f32/f64 (3-D fluxes f32, 2-D and BPG f64), mlp bf16",
+        "dtype": "f64" if not prec else "f32/f64 (3-D fields and fluxes f32; elevation, wet/dry, BPG f64), mlp bf16",
         "data": "synthetic (seeded channel mesh generator, analytic forcing)",
-        "config": {"workload": f"{args.config}: channel {N} nodes / {E} elements / {D} edges, nsigma {K}, "
-                               f"Mt {model.par.Mt}, wet/dry, MUSCL tracers; {desc}",
-                   "nodes": N, "elements": E, "layers": L, "Mt": model.par.Mt, "path": args.path},
-        "roofline": {"kernel": args.probe, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": pmc_traffic(args.probe, tfile) if (args.config == "c2" and not args.scale) else None,
-                     "avg_ms": avg_ms, "launches": k_n, "algorithmic_bytes": kb},
-        "setup_s": setup_s,
+        "config": config,
+        "roofline": roof,
+        "roofline_hbm": roof2,
+        "setup_s": {"meshes": t_mesh, "model": setup_s},
         "energy": diag["energy"],
-        "mlp_fallback_column_steps": diag["mlp_fallback"],
     }
+    if clos == gpu.FCG_CLOSURE_MLP:
+        out["mlp_fallback_column_fraction"] = diag["mlp_fallback"] / (N * (args.warmup + args.steps + 2))
     if headline:
         # the plain FP64 drop-in on the same workload: throughput and the error
-        # of the headline path against it after the same warmup + timed steps
-        ref, sec64, _, _, _, _ = run(0, gpu.FCG_CLOSURE_D3END, "")
-        out["fp64_drop_in"] = {"value": N * L * args.steps / sec64, "ms_per_step": 1e3 * sec64 / args.steps}
+        # of the headline path against it after the same steps
+        head_fields = {key: model.get(f) for f, key in ERR_FIELDS}
+        head_mask = model.get("mask_wd")
+        model.close()
+        ref, sec64, _, _, setup64, _ = run(0, gpu.FCG_CLOSURE_D3END, "k_velrhs")
+        out["fp64_drop_in"] = {"value": NL * args.steps / sec64, "ms_per_step": 1e3 * sec64 / args.steps}
         out["speedup_vs_fp64"] = sec64 / sec
-        interior = interior_masks(held["mesh"], held["txt"])
-        out["error_vs_fp64_after_steps"] = field_errors(model, ref)
-        total = args.warmup + args.steps
-        target = (int(math.ceil(44712.0 / model.par.dt)) if args.error_steps == "m2" else int(args.error_steps))
-        if target > total:
-            # both runs advance in lock-step chunks, each chunk reported on stderr
-            t_err = time.perf_counter()
-            while total < target:
-                n = min(500, target - total)
-                model.step(n)
-                ref.step(n)
-                model.sync()
-                ref.sync()
-                total += n
-                print(f"error leg: {total}/{target} steps, {time.perf_counter() - t_err:.0f} s", file=sys.stderr,
-                      flush=True)
-            d_m, d_r = model.diagnostics(), ref.diagnostics()
-            out["error_vs_fp64_long"] = {"steps": total, "model_time_s": total * model.par.dt,
-                                         **field_errors(model, ref),
-                                         "outside_sponge": field_errors(model, ref, interior) if interior else None,
-                                         "status": [d_m["status"], d_r["status"]],
-                                         "mlp_fallback_column_steps": d_m["mlp_fallback"]}
+        ref_fields = {key: ref.get(f) for f, key in ERR_FIELDS}
+        out["error_vs_fp64_after_steps"] = {"steps": args.warmup + args.steps + 2,
+                                            **ensemble_errors(head_fields, ref_fields)}
+        out["mask_wd_flips_vs_fp64"] = int((ref.get("mask_wd") != head_mask).sum())
         ref.close()
+    else:
+        model.close()
+    if args.workload == "c5":
+        for b in banks:
+            gpu.free_meshes(b)
+    if net is not None:
+        net.close()
     if not args.no_cpu_baseline:
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline()
-    model.close()
     print(json.dumps(out))
 
 

@@ -262,7 +262,7 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
 // Tidal elevation at the open boundary for each substep of the coming
 // barotropic loop: compute_ssh_rhs_elem, src/fv_average_dynamic.F90:190-216,
 // evaluated on the host with the same libm cos as a CPU build.
-void fill_tides(fcg_ctx *c) {
+void fill_tides(fcg_ctx *c, double *dst) {
     const int nb = c->m.nobn, Mt = c->p.Mt;
     if (!nb || !c->p.tf_presence) return;
     for (int sub = 1; sub <= Mt; ++sub) {
@@ -275,7 +275,7 @@ void fill_tides(fcg_ctx *c) {
                                    - (c->fazt[(size_t)j * nb + i] + c->phy_factor[j]));
             if (!c->p.restart)
                 if (c->time_jd - c->p.time_jd0 < 10.0) v = v * (c->time_jd - c->p.time_jd0) / 10.0;
-            c->tide_pinned[(size_t)(sub - 1) * nb + i] = v;
+            dst[(size_t)(sub - 1) * nb + i] = v;
         }
     }
 }
@@ -691,7 +691,9 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     if (p.closure == FCG_CLOSURE_MLP) {
         DA(ctx->mlp_flag, N) DA(ctx->mlp_count, 1)
     }
-    FCG_CHECK(hipHostMalloc((void **)&ctx->tide_pinned, sizeof(double) * std::max(1, p.Mt * mesh->nobn)));
+    FCG_CHECK(hipHostMalloc((void **)&ctx->tide_pinned,
+                            sizeof(double) * kTideRing * (size_t)std::max(1, p.Mt * mesh->nobn)));
+    for (int b = 0; b < kTideRing; ++b) FCG_CHECK(hipEventCreateWithFlags(&ctx->tide_ev[b], hipEventDisableTiming));
     // ---- cold start: array_setup + jacobian + initial_state (src/fv_main.F90:157-168, 320-362)
     std::vector<double> eta(N, 0.0), ea(N), a1(N), a2(N);
     for (int n = 0; n < N; ++n) {
@@ -874,6 +876,8 @@ extern "C" void fcg_destroy(fcg_ctx *ctx) {
     if (ctx->bt_graph) (void)hipGraphExecDestroy(ctx->bt_graph);
     for (void *p : ctx->allocs) (void)hipFree(p);
     if (ctx->tide_pinned) (void)hipHostFree(ctx->tide_pinned);
+    for (int b = 0; b < kTideRing; ++b)
+        if (ctx->tide_ev[b]) (void)hipEventDestroy(ctx->tide_ev[b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1115,10 +1119,18 @@ template <typename R>
 static int barotropic_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     if (p.tf_presence && ctx->m.nobn) {
-        FCG_CHECK(hipStreamSynchronize(ctx->stream));  // tide_pinned is reused each step
-        fill_tides(ctx);
-        FCG_CHECK(hipMemcpyAsync(s.tide, ctx->tide_pinned, sizeof(double) * p.Mt * ctx->m.nobn,
-                                 hipMemcpyHostToDevice, ctx->stream));
+        // the step's open-boundary values go through a ring of pinned staging
+        // buffers; a buffer is refilled once the copy issued from it kTideRing
+        // steps ago has completed, so the host never drains the stream
+        const int b = ctx->tide_slot;
+        ctx->tide_slot = (b + 1) % kTideRing;
+        double *buf = ctx->tide_pinned + (size_t)b * p.Mt * ctx->m.nobn;
+        if (ctx->tide_ev_used[b]) FCG_CHECK(hipEventSynchronize(ctx->tide_ev[b]));
+        fill_tides(ctx, buf);
+        FCG_CHECK(hipMemcpyAsync(s.tide, buf, sizeof(double) * p.Mt * ctx->m.nobn, hipMemcpyHostToDevice,
+                                 ctx->stream));
+        FCG_CHECK(hipEventRecord(ctx->tide_ev[b], ctx->stream));
+        ctx->tide_ev_used[b] = true;
     }
     if (!ctx->bt_graph_ok) {
         hipGraph_t g = nullptr;

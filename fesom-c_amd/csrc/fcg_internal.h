@@ -187,6 +187,8 @@ inline DevStateT<R> retype(const DevState &s) {
 constexpr size_t kVelrhsElemBytes = 160;
 constexpr size_t velrhs_edge_bytes(size_t esz) { return esz == 8 ? 104 : 64; }
 
+constexpr int kTideRing = 4;   // pinned staging slots of the tidal open-boundary values
+
 struct fcg_ctx {
     fcg_params p{};
     bool mixed = false;    // FCG_MIXED: 3-D fields stored as float
@@ -204,8 +206,10 @@ struct fcg_ctx {
     std::vector<int> pn, pe;             // device index -> reference index (nodes, elements)
     std::vector<int> noff, eoff;         // per member: first device node / element (nmemb+1 each)
     std::vector<double> ampt, fazt, amp_factor, phy_factor; // (nobn,12), (12)
-    std::vector<double> tide_host;       // Mt*nobn staging (pinned below)
-    double *tide_pinned = nullptr;
+    double *tide_pinned = nullptr;       // kTideRing x Mt*nobn staging of the open-boundary values
+    hipEvent_t tide_ev[4] = {};          // completion of the copy issued from each ring slot
+    bool tide_ev_used[4] = {};
+    int tide_slot = 0;
     // barotropic graph
     hipGraphExec_t bt_graph = nullptr;
     bool bt_graph_ok = false;

@@ -526,35 +526,58 @@ Model.attach_mlp = model_attach_mlp
 Model.collect = model_collect
 
 
+def build_meshes(cfg: RunConfig, meshes, workers=None):
+    """The reference preprocessing of each mesh (fcg_mesh_build, which keeps no
+    shared state, so the meshes are processed on a thread pool): a list of
+    (mesh object, its fcg_mesh view); free with free_meshes."""
+    from concurrent.futures import ThreadPoolExecutor
+    workers = workers or min(16, os.cpu_count() or 1, max(1, len(meshes)))
+    if workers > 1 and len(meshes) > 1:
+        with ThreadPoolExecutor(workers) as ex:
+            parts = list(ex.map(lambda m: build_meshes(cfg, [m], workers=1), meshes))
+        return [p[0] for p in parts]
+    L = lib()
+    mc = mesh_cfg(cfg)
+    out = []
+    for mesh in meshes:
+        xy = np.ascontiguousarray(mesh.xy, dtype=np.float64).reshape(-1)
+        ind = np.ascontiguousarray(mesh.index, dtype=np.int32)
+        el = np.ascontiguousarray(mesh.elem, dtype=np.int32).reshape(-1)
+        dep = np.ascontiguousarray(mesh.depth, dtype=np.float64)
+        ob = None if mesh.ob is None else np.ascontiguousarray(mesh.ob, dtype=np.float64).reshape(-1)
+        nobn = 0 if mesh.ob is None else int(mesh.ob.shape[0])
+        h = C.c_void_p()
+        rc = L.fcg_mesh_build(C.byref(mc), mesh.nod2D, _dp(xy), _ip(ind), mesh.elem2D, _ip(el), _dp(dep), nobn,
+                              None if ob is None else _dp(ob), C.byref(h))
+        if rc:
+            raise GpuError(f"fcg_mesh_build failed: {ERRORS.get(rc, rc)}")
+        v = Mesh()
+        L.fcg_mesh_view(h, C.byref(v))
+        out.append((h, v))
+    return out
+
+
+def free_meshes(bank):
+    for h, _ in bank:
+        lib().fcg_mesh_free(h)
+
+
 class Ensemble(Model):
     """Several models with their own meshes (uneven sizes) and forcing,
     advanced together by one context (fcg_create_ensemble).  ``get``/``set``
-    address the joined arrays; ``split`` cuts them per member."""
+    address the joined arrays; ``split`` cuts them per member.  ``prebuilt``
+    (from build_meshes) reuses mesh objects the caller owns."""
 
-    def __init__(self, cfg: RunConfig, meshes, precision=FCG_FP64, closure=FCG_CLOSURE_D3END):
+    def __init__(self, cfg: RunConfig, meshes, precision=FCG_FP64, closure=FCG_CLOSURE_D3END, prebuilt=None):
         L = lib()
         bad = cfg.unsupported()
         if bad:
             raise GpuError(f"options outside this build's hot path: {bad}")
         self.cfg = cfg
         self._mc = mesh_cfg(cfg)
-        self._meshes, self.views = [], []
-        for mesh in meshes:
-            xy = np.ascontiguousarray(mesh.xy, dtype=np.float64).reshape(-1)
-            ind = np.ascontiguousarray(mesh.index, dtype=np.int32)
-            el = np.ascontiguousarray(mesh.elem, dtype=np.int32).reshape(-1)
-            dep = np.ascontiguousarray(mesh.depth, dtype=np.float64)
-            ob = None if mesh.ob is None else np.ascontiguousarray(mesh.ob, dtype=np.float64).reshape(-1)
-            nobn = 0 if mesh.ob is None else int(mesh.ob.shape[0])
-            h = C.c_void_p()
-            rc = L.fcg_mesh_build(C.byref(self._mc), mesh.nod2D, _dp(xy), _ip(ind), mesh.elem2D, _ip(el), _dp(dep),
-                                  nobn, None if ob is None else _dp(ob), C.byref(h))
-            if rc:
-                raise GpuError(f"fcg_mesh_build failed: {ERRORS.get(rc, rc)}")
-            v = Mesh()
-            L.fcg_mesh_view(h, C.byref(v))
-            self._meshes.append(h)
-            self.views.append(v)
+        bank = prebuilt if prebuilt is not None else build_meshes(cfg, meshes)
+        self._meshes = [] if prebuilt is not None else [h for h, _ in bank]
+        self.views = [v for _, v in bank]
         arr = (Mesh * len(self.views))(*self.views)
         self.nodes = [v.nod2D for v in self.views]
         self.elems = [v.elem2D for v in self.views]

@@ -265,6 +265,51 @@ def config(name, seed=0, scale=None):
     raise KeyError(name)
 
 
+# Ensemble at the benchmark's configuration: 64 channels of 50k..400k nodes,
+# each with its own number of sigma levels (one of 16, 20, 24, 28, 32),
+# tidal amplitude (+-10 %) and wind (+-3 m/s per component about a common
+# seeded wind).  "c4_mini" has the same structure with 2k..16k-node members.
+ENSEMBLE_SPECS = {"c4": (64, 50_000, 400_000), "c4_mini": (64, 2_000, 16_000)}
+NSIGMA_CLASSES = (16, 20, 24, 28, 32)
+
+
+def ensemble_classes(name="c4", members=None, seed=0):
+    """Members of an ensemble with per-member level counts, grouped by nsigma:
+    a list of (nsigma, fv_run text, [RawMesh], [extras], [member ids]) in
+    ascending nsigma.  Every member shares dt (it fits the finest member) and
+    the physics switches; members with the same nsigma can share one
+    fcg_create_ensemble context."""
+    nm, lo, hi = ENSEMBLE_SPECS[name]
+    nm = members or nm
+    rng = np.random.default_rng(seed + 4343)
+    nodes = rng.uniform(lo, hi, nm)
+    nxs = np.maximum(8, np.round(np.sqrt(2.0 * nodes))).astype(int)
+    nsig = rng.choice(NSIGMA_CLASSES, nm)
+    amps = 1.0 + rng.uniform(-0.1, 0.1, nm)
+    base = rng.uniform(0.0, 10.0, 2) - 5.0
+    winds = base[None, :] + rng.uniform(-3.0, 3.0, (nm, 2))
+    dx_min = 20000.0 / (nxs.max() - 1)
+    dt_2D = min(2.0, round(0.45 * dx_min / math.sqrt(9.8 * 34.0), 2))
+    def member(b):   # every member draws from its own seeded generators
+        nx = int(nxs[b])
+        m = channel(nx, max(8, nx // 2), split=0.9, seed=seed + 100 + b, intertidal=0.15, depth_west=30.0,
+                    depth_dry=-2.0, tide_amp=float(amps[b]))
+        T, S = stratified_ts(m, int(nsig[b]), seed=seed + 200 + b)
+        return m, dict(T=T, S=S, wind=(float(winds[b, 0]), float(winds[b, 1])), tide_amp=float(amps[b]))
+
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        built = list(ex.map(member, range(nm)))
+    out = []
+    for ns in sorted(set(int(v) for v in nsig)):
+        ids = [b for b in range(nm) if nsig[b] == ns]
+        txt = fv_run(f"{name}_s{ns}", dt_2D, 10 * dt_2D, ns, tracer_adv=2, muscl=True, wetdry=True, Dmin=0.25,
+                     sponge=SPONGE_M, ob_relax_days=OB_RELAX_DAYS)
+        out.append((ns, txt, [built[b][0] for b in ids], [built[b][1] for b in ids], ids))
+    return out
+
+
 def ensemble(name="c4", members=None, seed=0):
     """Ensemble of channels with uneven mesh sizes and perturbed forcing, for
     fcg_create_ensemble.  Returns (fv_run text, [RawMesh], [extras]) where
@@ -272,11 +317,13 @@ def ensemble(name="c4", members=None, seed=0):
     finest member), and differs in resolution, bathymetry seed, tidal
     amplitude (+-10 %), wind (+-3 m/s per component) and T/S noise.
 
-    ``c4``: 16 members, 150..400 nodes along the channel (~11k..80k nodes
+    ``c4_16``: 16 members, 150..400 nodes along the channel (~11k..80k nodes
     each), nsigma 24.  ``c4_small``: 3 members of 30..60 nodes along, nsigma 8.
+    (The configuration's ensemble, with its own level count per member, is
+    ``ensemble_classes``.)
     """
     rng = np.random.default_rng(seed + 4242)
-    if name == "c4":
+    if name == "c4_16":
         nm, lo, hi, nsig = members or 16, 150, 400, 24
     elif name == "c4_small":
         nm, lo, hi, nsig = members or 3, 30, 60, 8
