@@ -213,8 +213,8 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["c5", "c2"], default="c5",
                     help="c5: the 64-member ensemble with the learned closure (default); c2: the 1M-node channel")
     ap.add_argument("--scale", default=None, metavar="NX,NY",
@@ -228,6 +228,9 @@ def main():
     ap.add_argument("--launch-table", default="auto",
                     help="block-size table from fesomc_amd.autotune (JSON); 'auto' = the stored table of this "
                          "workload and precision if there is one, 'none' = library defaults")
+    ap.add_argument("--serial-classes", action="store_true",
+                    help="c5 only: finish each level class's step before the next starts (kernel times without "
+                         "overlap; the default lets the classes' streams overlap)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
@@ -270,7 +273,7 @@ def main():
             net = closure.load(args.closure_weights)
         if args.workload == "c5":
             return EnsembleSet(classes, precision=precision, closure=clos, banks=banks,
-                               net=net if clos == gpu.FCG_CLOSURE_MLP else None)
+                               net=net if clos == gpu.FCG_CLOSURE_MLP else None, serial=args.serial_classes)
         scale = tuple(int(v) for v in args.scale.split(",")) if args.scale else None
         txt, mesh, ex, model = build_model("c2", scale=scale, precision=precision, closure=clos)
         held["mesh"], held["txt"] = mesh, txt
@@ -339,7 +342,7 @@ def main():
                     f"dt {model.par.dt} s, wet/dry, MUSCL tracers, tide amplitude +-10 %, wind +-3 m/s per member; "
                     f"{desc}")
         config = {"workload": workload, "members": model.members, "nodes": N, "elements": E, "node_levels": NL,
-                  "classes": cls, "Mt": model.par.Mt, "path": args.path}
+                  "classes": cls, "Mt": model.par.Mt, "path": args.path, "serial_classes": args.serial_classes}
     else:
         workload = (f"c2: channel {N} nodes / {E} elements / {D} edges, nsigma {model.K}, Mt {model.par.Mt}, "
                     f"wet/dry, MUSCL tracers; {desc}")

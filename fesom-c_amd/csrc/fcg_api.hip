@@ -28,7 +28,7 @@ template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, d
 template <typename R> __global__ void k_velvisc_im(DevMesh, DevStateT<R>, double, double, double);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_node(DevMesh, DevStateT<R>, double, const double *, int, double,
-                                                const double *);
+                                                const double *, double);
 template <typename R> __global__ void k_bt_edge_ssh(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_edge_mom(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_bt_velgrad(DevMesh, DevStateT<R>);
@@ -38,7 +38,7 @@ __global__ void k_el2nodes(DevMesh, const R *, const R *, R *, R *, const double
 template <typename R> __global__ void k_bt_p1edge(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_p1elem(DevMesh, DevStateT<R>, double);
 template <typename R>
-__global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int, double);
+__global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int, double, R *);
 template <typename R> __global__ void k_jac_node(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_jac_restart(DevMesh, DevStateT<R>, double);
@@ -211,12 +211,13 @@ int upload_egeo(fcg_ctx *ctx, const std::vector<int2> &ednodes, const std::vecto
 
 // 3-D field descriptor for Fortran-layout transfer
 struct Field {
-    double *dev;       // float storage when r3 and the context is mixed precision
+    double *dev;       // float storage when rtype and the context is mixed precision
     long levels;   // levels per column (L or K), or components for 2-D vectors
     long ents;     // entities
     bool interleaved;  // device stores (comp, ent) interleaved like Fortran (2,E)
     bool nodal;        // node field (else element field)
-    bool r3;           // 3-D field held in the context's storage type
+    bool r3;           // 3-D field (columns of K slots)
+    bool rtype;        // held in the context's storage type (3-D fields and the barotropic momentum path)
 };
 
 bool find_field(fcg_ctx *c, const char *name, Field &f) {
@@ -253,7 +254,9 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
             if (!t.p) return false;  // not held in this configuration (rho_c without key_atm)
             const bool r3 = !t.il && (t.lev == L || t.lev == K) && std::strcmp(t.n, "hpressure") != 0
                             && std::strcmp(t.n, "rho_c") != 0;
-            f = Field{t.p, t.lev, t.nodal ? N : E, t.il, t.nodal, r3};
+            const bool r2 = !std::strcmp(t.n, "UAB") || !std::strcmp(t.n, "U_rhs_2D_3D")
+                            || !std::strcmp(t.n, "Bar_pr_2D") || !std::strcmp(t.n, "vel_grad");
+            f = Field{t.p, t.lev, t.nodal ? N : E, t.il, t.nodal, r3, r3 || r2};
             return true;
         }
     return false;
@@ -281,16 +284,23 @@ void fill_tides(fcg_ctx *c, double *dst) {
 }
 
 template <typename R>
-int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
+int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s0) {
     const DevMesh &m = ctx->m;
     const fcg_params &p = ctx->p;
     hipStream_t st = ctx->stream;
+    // two AB3 velocity buffers, alternating by substep; the last substep's is
+    // the state's own (UAB as the reference holds it after the loop)
+    R *const uab[2] = {s0.UAB, s0.UAB + 2 * (size_t)m.E};
     for (int sub = 1; sub <= p.Mt; ++sub) {
+        DevStateT<R> s = s0;
+        s.UAB = uab[(p.Mt - sub) & 1];
         const double *tide = (p.tf_presence && m.nobn) ? s.tide + (size_t)(sub - 1) * m.nobn : nullptr;
-        hipLaunchKernelGGL(fcgk::k_bt_elem1<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
+        // the AB3 velocity and element depth of later substeps come from the
+        // previous substep's k_bt_elem2
+        if (sub == 1) hipLaunchKernelGGL(fcgk::k_bt_elem1<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
         hipLaunchKernelGGL(fcgk::k_bt_edge_ssh<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s);
         hipLaunchKernelGGL(fcgk::k_bt_node<R>, dim3(nblk(m.N)), dim3(kBlk), 0, st, m, s, p.dt_2D, tide, p.key_atm,
-                           1.0 / ctx->density_0, p.key_rivers ? (const double *)s.runoff : nullptr);
+                           1.0 / ctx->density_0, p.key_rivers ? (const double *)s.runoff : nullptr, p.Dmin);
         hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
         if (p.mom_adv_2D == 2)
             hipLaunchKernelGGL(fcgk::k_bt_edge_mom<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, p.Dmin);
@@ -305,7 +315,7 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s) {
         }
         hipLaunchKernelGGL(fcgk::k_bt_elem2<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
                            ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0,
-                           p.filt_2D ? p.tau_inv_filt : 0.0);
+                           p.filt_2D ? p.tau_inv_filt : 0.0, sub < p.Mt ? uab[(p.Mt - sub - 1) & 1] : (R *)nullptr);
     }
     return FCG_OK;
 }
@@ -661,8 +671,8 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         }
     }
     DA(s.eta_n, N) DA(s.eta_n_1, N) DA(s.eta_n_2, N) DA(s.etaAB, N) DA(s.eta_g, N) DA(s.ssh_rhs, N)
-    DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 2 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
-    DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
+    DA(s.U2, 2 * E) DA(s.U2_1, 2 * E) DA(s.U2_2, 2 * E) DA(s.UAB, 4 * E) DA(s.aux_el, E) DA(s.vel_grad, 4 * E)
+    DA(s.U_rhs_2D_3D, 2 * E) DA(s.Bar_pr_2D, 2 * E) DA(s.adn, N) DA(s.U_filt, E) DA(s.V_filt, E) DA(s.dmean_n, E)
     DA(s.mask_wd, E) DA(s.taux, E) DA(s.tauy, E) DA(s.C_d_el, E) DA(s.windx, N) DA(s.windy, N)
     DA(s.a, N) DA(s.a_old, N) DA(s.ubot, 2 * E) DA(s.fl_ssh, D) DA(s.fl_uv, 2 * (size_t)D) DA(s.fl_p1, 4 * (size_t)D) DA(s.qsr, N) DA(s.qns, N) DA(s.emp, N) DA(s.mslp, N) DA(s.runoff, N) DA(s.node2d, 4 * (size_t)N) DA(s.tide, (size_t)std::max(1, p.Mt * mesh->nobn))
     if (p.key_atm) {
@@ -955,7 +965,7 @@ static int transfer(fcg_ctx *ctx, const char *name, double *host, long count, bo
     const size_t dcount = (size_t)slots * f.ents;
     auto dev_at = [&](long lev, long i) -> size_t { return (size_t)i * slots + lev; };
     std::vector<double> tmp(dcount, 0.0);
-    const bool f32 = f.r3 && ctx->mixed;  // device holds float: round on upload, widen on download
+    const bool f32 = f.rtype && ctx->mixed;  // device holds float: round on upload, widen on download
     std::vector<float> tmpf(f32 ? dcount : 0);
     if (to_dev) {
         if (col3 && f.levels < slots) {  // keep the unused slot of layer columns as it is on the device

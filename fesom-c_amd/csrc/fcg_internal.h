@@ -122,23 +122,31 @@ struct DevMesh {
     double *oms = nullptr;     // K: 1-sigma(k)
 };
 
-// Device state.  2-D barotropic fields, the hydrostatic pressure and the
-// reductions are FP64 in every precision mode; the 3-D fields use the storage
-// type R (double for the FP64 drop-in, float for the mixed-precision path).
+// Device state.  The elevation and volume chain of the barotropic loop (eta,
+// etaAB, depths, the ssh_rhs sums, the Dmin clamps, masks, the 2-D velocity
+// and its history), the hydrostatic pressure and the reductions are FP64 in
+// every precision mode; the 3-D fields and the barotropic momentum path (AB3
+// velocity, its gradients, the edge momentum fluxes, the 3-D-to-2-D
+// tendencies and the bottom velocity cache) use the storage type R (double
+// for the FP64 drop-in, float for the mixed-precision path).
 template <typename R>
 struct DevStateT {
-    // 2-D barotropic (FP64)
+    // 2-D barotropic
     double *eta_n = nullptr, *eta_n_1 = nullptr, *eta_n_2 = nullptr, *etaAB = nullptr, *eta_g = nullptr;
     double *ssh_rhs = nullptr;            // flux sum of the last substep (diagnostic)
     double *U2 = nullptr, *U2_1 = nullptr, *U2_2 = nullptr;   // (2,E) interleaved [2e],[2e+1]
-    double *UAB = nullptr, *aux_el = nullptr, *vel_grad = nullptr; // 2E, E, 4E
-    double *U_rhs_2D_3D = nullptr, *Bar_pr_2D = nullptr;      // 2E
+    R *UAB = nullptr;                                          // 2E
+    double *aux_el = nullptr;                                  // E
+    R *vel_grad = nullptr;                                     // 4E
+    R *U_rhs_2D_3D = nullptr, *Bar_pr_2D = nullptr;            // 2E
+    double *adn = nullptr;   // N: max(Dmin, AB3 elevation + depth) of the coming substep
     double *U_filt = nullptr, *V_filt = nullptr, *dmean_n = nullptr; // E
     double *mask_wd = nullptr, *taux = nullptr, *tauy = nullptr, *C_d_el = nullptr; // E
     double *windx = nullptr, *windy = nullptr;                 // N
     double *a = nullptr, *a_old = nullptr;                     // N: max(depth+eta_n, Dmin) now / previous jacobian
-    double *ubot = nullptr;                                    // 2E: U_n, V_n of the bottom layer (substep cache)
-    double *fl_ssh = nullptr, *fl_uv = nullptr;                // D, 2D: per-edge transport and 2-D momentum flux
+    R *ubot = nullptr;                                         // 2E: U_n, V_n of the bottom layer (substep cache)
+    double *fl_ssh = nullptr;                                  // D: per-edge transport
+    R *fl_uv = nullptr;                                        // 2D: per-edge 2-D momentum flux
     double *fl_p1 = nullptr;                                   // 4D: layer sums of the P1 3-D-to-2-D advection
     double *puls_ap = nullptr;                                 // E: a of U_puls = U_n - a*U_n_2D (first substep)
     double *qsr = nullptr, *qns = nullptr, *emp = nullptr, *mslp = nullptr;  // N: surface forcing (key_atm)

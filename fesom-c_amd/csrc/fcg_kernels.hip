@@ -18,6 +18,29 @@
 
 namespace fcgk {
 
+// (2,E)/(4,E) interleaved records of the barotropic momentum path in the
+// storage type, widened to double on load and rounded on store
+__device__ __forceinline__ double2 ld2(const double *p, size_t i) { return ((const double2 *)p)[i]; }
+__device__ __forceinline__ double2 ld2(const float *p, size_t i) {
+    const float2 v = ((const float2 *)p)[i];
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(double *p, size_t i, double a, double b) { ((double2 *)p)[i] = make_double2(a, b); }
+__device__ __forceinline__ void st2(float *p, size_t i, double a, double b) {
+    ((float2 *)p)[i] = make_float2((float)a, (float)b);
+}
+__device__ __forceinline__ double4 ld4(const double *p, size_t i) { return ((const double4 *)p)[i]; }
+__device__ __forceinline__ double4 ld4(const float *p, size_t i) {
+    const float4 v = ((const float4 *)p)[i];
+    return make_double4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st4(double *p, size_t i, double a, double b, double c, double d) {
+    ((double4 *)p)[i] = make_double4(a, b, c, d);
+}
+__device__ __forceinline__ void st4(float *p, size_t i, double a, double b, double c, double d) {
+    ((float4 *)p)[i] = make_float4((float)a, (float)b, (float)c, (float)d);
+}
+
 __device__ __forceinline__ double sum4(const double4 w, double x0, double x1, double x2, double x3) {
     double s = w.x * x0;
     s = s + w.y * x1;
@@ -316,8 +339,7 @@ __global__ __launch_bounds__(kColT) void k_bpg(DevMesh m, DevStateT<R> s) {
             su = su + tu[c * KP + k];
             sv = sv + tv[c * KP + k];
         }
-        s.Bar_pr_2D[2 * e] = su;
-        s.Bar_pr_2D[2 * e + 1] = sv;
+        st2(s.Bar_pr_2D, e, su, sv);
     }
 }
 
@@ -1224,20 +1246,24 @@ __global__ __launch_bounds__(kColT) void k_velvisc_im(DevMesh m, DevStateT<R> s,
 // ---------------------------------------------------------------------------
 constexpr double kAB0 = 1.5 + kBeta, kAB1 = 0.5 + 2.0 * kBeta;
 
+__device__ __forceinline__ double ab3(double a, double b, double c) { return kAB0 * a - kAB1 * b + kBeta * c; }
+
 template <typename R>
 __device__ __forceinline__ double etaAB3(const DevStateT<R> &s, int n) {
-    return kAB0 * s.eta_n[n] - kAB1 * s.eta_n_1[n] + kBeta * s.eta_n_2[n];
+    return ab3(s.eta_n[n], s.eta_n_1[n], s.eta_n_2[n]);
 }
 
-// AB3 velocity + element depth of compute_ssh_rhs_elem (fv_average_dynamic.F90:66-86)
+
+// AB3 velocity + element depth of compute_ssh_rhs_elem (fv_average_dynamic.F90:66-86),
+// first substep of a step; k_bt_elem2 forms both for the substeps after it.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_elem1(DevMesh m, DevStateT<R> s, double Dmin) {
     int e = TID();
     if (e >= m.E) return;
     const int4 q = m.en[e];
     const double4 w = m.wcv[e];
-    s.UAB[2 * e] = kAB0 * s.U2[2 * e] - kAB1 * s.U2_1[2 * e] + kBeta * s.U2_2[2 * e];
-    s.UAB[2 * e + 1] = kAB0 * s.U2[2 * e + 1] - kAB1 * s.U2_1[2 * e + 1] + kBeta * s.U2_2[2 * e + 1];
+    const double2 u0 = ld2(s.U2, e), u1 = ld2(s.U2_1, e), u2 = ld2(s.U2_2, e);
+    st2(s.UAB, e, ab3(u0.x, u1.x, u2.x), ab3(u0.y, u1.y, u2.y));
     const int ns[4] = {q.x, q.y, q.z, q.w};
     double ad[4];
     for (int i = 0; i < 4; ++i) {
@@ -1257,8 +1283,12 @@ __global__ __launch_bounds__(256) void k_bt_edge_ssh(DevMesh m, DevStateT<R> s) 
     if (ed >= m.D) return;
     const int2 et = m.edtri[ed];
     const double4 cd = m.ecross[ed];
-    double c1 = (s.UAB[2 * et.x + 1] * cd.x - s.UAB[2 * et.x] * cd.y) * s.aux_el[et.x];
-    if (et.y >= 0) c1 = c1 + (-s.UAB[2 * et.y + 1] * cd.z + s.UAB[2 * et.y] * cd.w) * s.aux_el[et.y];
+    const double2 ua = ld2(s.UAB, et.x);
+    double c1 = (ua.y * cd.x - ua.x * cd.y) * s.aux_el[et.x];
+    if (et.y >= 0) {
+        const double2 ub = ld2(s.UAB, et.y);
+        c1 = c1 + (-ub.y * cd.z + ub.x * cd.w) * s.aux_el[et.y];
+    }
     s.fl_ssh[ed] = c1;
 }
 
@@ -1266,10 +1296,12 @@ __global__ __launch_bounds__(256) void k_bt_edge_ssh(DevMesh m, DevStateT<R> s) 
 // the mouths (177-183; runoff null without rivers), eta update (fv_main.F90:1432-1435),
 // tidal OB value (fv_average_dynamic.F90:190-216, host-evaluated), the AM4
 // elevation of update_2D_vel (fv_average_dynamic.F90:288-296) and the time-level
-// shift (fv_main.F90:1547-1551).
+// shift (fv_main.F90:1547-1551).  Also the clamped AB3 depth of the coming
+// substep's compute_ssh_rhs_elem (its elevation levels are the shifted ones),
+// which k_bt_elem2 averages to the element.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, double dt_2D, const double *__restrict__ tide,
-                                                 int atm, double d0i, const double *__restrict__ runoff) {
+                                                 int atm, double d0i, const double *__restrict__ runoff, double Dmin) {
     int n = TID();
     if (n >= m.N) return;
     double ssh = 0.0;
@@ -1303,6 +1335,8 @@ __global__ __launch_bounds__(256) void k_bt_node(DevMesh m, DevStateT<R> s, doub
     s.eta_n[n] = en;
     // the four node values update_2D_vel gathers, as one record per node
     ((double4 *)s.node2d)[n] = make_double4(ea, eg, e0, en);
+    const double t = ab3(en, e0, e1) + m.depth[n];
+    s.adn[n] = t > Dmin ? t : Dmin;
 }
 
 // vel_gradients_2D(UAB): src/fv_gradients.F90:97-176
@@ -1318,15 +1352,17 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
     const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
     const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
     const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
-    const double u = s.UAB[2 * e], v = s.UAB[2 * e + 1];
+    const double2 uv = ld2(s.UAB, e);
+    const double u = uv.x, v = uv.y;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
     for (int j = 0; j < jend; ++j) {
         int e2 = nbs[j];
         if (e2 >= 0) {
-            g0 = g0 + gxs[j] * (s.UAB[2 * e2] - u);
-            g1 = g1 + gys[j] * (s.UAB[2 * e2] - u);
-            g2 = g2 + gxs[j] * (s.UAB[2 * e2 + 1] - v);
-            g3 = g3 + gys[j] * (s.UAB[2 * e2 + 1] - v);
+            const double2 w2 = ld2(s.UAB, e2);
+            g0 = g0 + gxs[j] * (w2.x - u);
+            g1 = g1 + gys[j] * (w2.x - u);
+            g2 = g2 + gxs[j] * (w2.y - v);
+            g3 = g3 + gys[j] * (w2.y - v);
         } else {
             double2 dd = m.edxdy[eds[j]];
             double zc0 = dd.x * m.elem_cos[e], zc1 = dd.y;
@@ -1337,10 +1373,7 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
             g3 = g3 - gys[j] * un * zc0;
         }
     }
-    s.vel_grad[4 * e] = g0;
-    s.vel_grad[4 * e + 1] = g1;
-    s.vel_grad[4 * e + 2] = g2;
-    s.vel_grad[4 * e + 3] = g3;
+    st4(s.vel_grad, e, g0, g1, g2, g3);
 }
 
 // first substep only: dmean_n (fv_main.F90:1467-1473) and compute_puls_vel
@@ -1364,8 +1397,7 @@ __global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevStateT<R> s, doub
     const double snew = sum4(w, s.eta_n[q.x] + m.depth[q.x], s.eta_n[q.y] + m.depth[q.y],
                              s.eta_n[q.z] + m.depth[q.z], s.eta_n[q.w] + m.depth[q.w]);
     s.puls_ap[e] = dm / dmaxr(Dmin, snew);
-    s.ubot[2 * e] = PL(s.U_n, m.L - 1, e, KC);
-    s.ubot[2 * e + 1] = PL(s.V_n, m.L - 1, e, KC);
+    st2(s.ubot, e, PL(s.U_n, m.L - 1, e, KC), PL(s.V_n, m.L - 1, e, KC));
 }
 
 // compute_el2nodes_3D (src/fv_utilit.F90:669-689), one thread per node and
@@ -1538,8 +1570,7 @@ __global__ __launch_bounds__(256) void k_bt_p1elem(DevMesh m, DevStateT<R> s, do
             rv = rv + tau * fv * area * jo;
         }
     }
-    s.U_rhs_2D_3D[2 * e] = ru / s.dmean_n[e];
-    s.U_rhs_2D_3D[2 * e + 1] = rv / s.dmean_n[e];
+    st2(s.U_rhs_2D_3D, e, ru / s.dmean_n[e], rv / s.dmean_n[e]);
 }
 
 // momentum_adv_upwind_2D edge flux: fv_advection.F90:589-643
@@ -1553,18 +1584,19 @@ __device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevStateT<R> &
     const double4 cd = m.ecross[ed];
     const double2 dd = m.edxdy[ed];
     const int e1 = et.x, e2 = et.y;
-    const double *g1 = s.vel_grad + 4 * e1, *g2 = s.vel_grad + 4 * e2;
-    double u1 = s.UAB[2 * e1] - g1[0] * cd.x - g1[1] * cd.y;
-    double v1 = s.UAB[2 * e1 + 1] - g1[2] * cd.x - g1[3] * cd.y;
-    double u2 = s.UAB[2 * e2] - g2[0] * cd.z - g2[1] * cd.w;
-    double v2 = s.UAB[2 * e2 + 1] - g2[2] * cd.z - g2[3] * cd.w;
+    const double4 g1 = ld4(s.vel_grad, e1), g2 = ld4(s.vel_grad, e2);
+    const double2 w1 = ld2(s.UAB, e1), w2 = ld2(s.UAB, e2);
+    double u1 = w1.x - g1.x * cd.x - g1.y * cd.y;
+    double v1 = w1.y - g1.z * cd.x - g1.w * cd.y;
+    double u2 = w2.x - g2.x * cd.z - g2.y * cd.w;
+    double v2 = w2.y - g2.z * cd.z - g2.w * cd.w;
     double un = 0.5 * kREarthD * dm * ((u1 + u2) * dd.y - (v1 * m.elem_cos[e1] + v2 * m.elem_cos[e2]) * dd.x);
     if (un >= 0.0) {
-        uu = un * s.UAB[2 * e1] * acc;
-        vv = un * s.UAB[2 * e1 + 1] * acc;
+        uu = un * w1.x * acc;
+        vv = un * w1.y * acc;
     } else {
-        uu = un * s.UAB[2 * e2] * acc;
-        vv = un * s.UAB[2 * e2 + 1] * acc;
+        uu = un * w2.x * acc;
+        vv = un * w2.y * acc;
     }
 }
 
@@ -1576,14 +1608,20 @@ __global__ __launch_bounds__(256) void k_bt_edge_mom(DevMesh m, DevStateT<R> s, 
     if (ed >= m.D || m.edtri[ed].y < 0) return;
     double uu, vv;
     edge_uu2d(m, s, ed, Dmin, uu, vv);
-    ((double2 *)s.fl_uv)[ed] = make_double2(uu, vv);
+    st2(s.fl_uv, ed, uu, vv);
 }
 
 // update_2D_vel (fv_average_dynamic.F90:228-443, 3-D branch) for one element,
 // followed by the velocity shift and U_filt accumulation (fv_main.F90:1569-1606).
+// With uab_next, also the AB3 velocity and element depth of the following
+// substep's compute_ssh_rhs_elem (k_bt_elem1's values, from the shifted
+// velocities and k_bt_node's clamped node depths); the velocity goes to the
+// other of the two AB3 buffers, since the 2-D filter reads the neighbours'
+// current one.  C_d_el = C_d is set on the first substep only: nothing
+// changes it between substeps.
 template <typename R>
 __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
-                           int wet_dry, int adv2d, int first, double tau2d) {
+                           int wet_dry, int adv2d, int first, double tau2d, R *__restrict__ uab_next) {
     const int KC = m.K;
     int e = TID();
     if (e >= m.E) return;
@@ -1603,11 +1641,12 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
         mask = (mn + mx > Dmin) ? 1.0 : 0.0;
         s.mask_wd[e] = mask;
     }
-    s.C_d_el[e] = C_d;
+    if (first) s.C_d_el[e] = C_d;
     const double rho_inv = 1.0 / (rho0);
     const double dm = dmaxr(Dmin, w.x * (ea0 + d0) + w.y * (ea1 + d1) + w.z * (ea2 + d2) + w.w * (ea3 + d3));
     const double eg0 = r0.y, eg1 = r1.y, eg2 = r2.y, eg3 = r3.y;
-    const double uab = s.UAB[2 * e], vab = s.UAB[2 * e + 1];
+    const double2 wab = ld2(s.UAB, e);
+    const double uab = wab.x, vab = wab.y;
     double ru = area * (-dm * (gx.x * eg0 + gx.y * eg1 + gx.z * eg2 + gx.w * eg3) + vab * dm * cor + s.taux[e] * rho_inv);
     double rv = area * (-dm * (gy.x * eg0 + gy.y * eg1 + gy.z * eg2 + gy.w * eg3) - uab * dm * cor + s.tauy[e] * rho_inv);
     if (adv2d) {  // edge fluxes of k_bt_edge_mom, in ascending edge id
@@ -1616,7 +1655,7 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
         for (int j = 0; j < 4; ++j) {
             int c = slots[j];
             if (c == kEmpty) break;
-            const double2 f = ((const double2 *)s.fl_uv)[c >= 0 ? c : ~c];
+            const double2 f = ld2(s.fl_uv, c >= 0 ? c : ~c);
             if (c >= 0) {
                 ru = ru + f.x;
                 rv = rv + f.y;
@@ -1633,35 +1672,40 @@ __global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, dou
         double fu = 0.0, fv = 0.0;
         for (int j = 0; j < jend; ++j) {
             if (nbs[j] >= 0) {
-                fu = fu + s.UAB[2 * nbs[j]] - uab;
-                fv = fv + s.UAB[2 * nbs[j] + 1] - vab;
+                const double2 wn = ld2(s.UAB, nbs[j]);
+                fu = fu + wn.x - uab;
+                fv = fv + wn.y - vab;
             }
         }
         const double dsum = sum4(w, d0, d1, d2, d3);
         ru = ru + tau2d * fu * area * dsum;
         rv = rv + tau2d * fv * area * dsum;
     }
-    ru = ru + s.U_rhs_2D_3D[2 * e];
-    rv = rv + s.U_rhs_2D_3D[2 * e + 1];
+    const double2 r23 = ld2(s.U_rhs_2D_3D, e);
+    ru = ru + r23.x;
+    rv = rv + r23.y;
     // eta_n_1 holds eta^n, eta_n holds eta^{n+1} (the reference's ssh_rhs)
     const double dmo = dmaxr(Dmin, sum4(w, r0.z + d0, r1.z + d1, r2.z + d2, r3.z + d3));
     const double acc = m.acc_el[e];  // sum w_cv*ac, formed at set-up in the same order
     const double a_Cd = C_d * ((1.0 - acc) * Cd_bz + 1.0);
-    const double ub = s.ubot[2 * e], vb = s.ubot[2 * e + 1];  // U_n, V_n at the bottom layer
+    const double2 bot = ld2(s.ubot, e);  // U_n, V_n at the bottom layer
+    const double ub = bot.x, vb = bot.y;
     const double friction = a_Cd * sqrt(ub * ub + vb * vb);
     const double snew = sum4(w, r0.w + d0, r1.w + d1, r2.w + d2, r3.w + d3);
     const double ffc = 1.0 / (dmaxr(Dmin, snew) + friction * dt_2D);
-    const double U0 = s.U2[2 * e], V0 = s.U2[2 * e + 1];
-    const double un = mask * ((dmo * U0 - dt_2D * friction * (ub - U0) + dt_2D * s.Bar_pr_2D[2 * e]
+    const double2 u0 = ld2(s.U2, e), h1 = ld2(s.U2_1, e), bp = ld2(s.Bar_pr_2D, e);
+    const double U0 = u0.x, V0 = u0.y;
+    const double un = mask * ((dmo * U0 - dt_2D * friction * (ub - U0) + dt_2D * bp.x
                                + dt_2D * ru / area) * ffc + 0.0);
-    const double vn = mask * ((dmo * V0 - dt_2D * friction * (vb - V0) + dt_2D * s.Bar_pr_2D[2 * e + 1]
+    const double vn = mask * ((dmo * V0 - dt_2D * friction * (vb - V0) + dt_2D * bp.y
                                + dt_2D * rv / area) * ffc + 0.0);
-    s.U2_2[2 * e] = s.U2_1[2 * e];
-    s.U2_2[2 * e + 1] = s.U2_1[2 * e + 1];
-    s.U2_1[2 * e] = U0;
-    s.U2_1[2 * e + 1] = V0;
-    s.U2[2 * e] = un;
-    s.U2[2 * e + 1] = vn;
+    st2(s.U2_2, e, h1.x, h1.y);
+    st2(s.U2_1, e, U0, V0);
+    st2(s.U2, e, un, vn);
+    if (uab_next) {
+        st2(uab_next, e, ab3(un, U0, h1.x), ab3(vn, V0, h1.y));
+        s.aux_el[e] = sum4(w, s.adn[q.x], s.adn[q.y], s.adn[q.z], s.adn[q.w]);
+    }
     const double dmf = dmaxr(Dmin, w.x * (r0.w + d0) + w.y * (r1.w + d1) + w.z * (r2.w + d2) + w.w * (r3.w + d3));
     const double uf0 = first ? 0.0 : s.U_filt[e];
     const double vf0 = first ? 0.0 : s.V_filt[e];

@@ -51,8 +51,12 @@ class EnsembleSet:
     meshgen.ensemble_classes; ``banks`` (from gpu.build_meshes, one per
     class) let several sets share the preprocessed meshes."""
 
-    def __init__(self, classes, precision=0, closure=0, banks=None, net=None):
+    def __init__(self, classes, precision=0, closure=0, banks=None, net=None, serial=False):
+        """``serial``: finish each class's step before the next class starts
+        (no overlap between the classes' streams: per-kernel times without
+        other kernels beside them)."""
         self.classes = classes
+        self.serial = serial
         self.parts = []
         for i, (ns, txt, meshes, extras, ids) in enumerate(classes):
             cfg = RunConfig(txt)
@@ -73,6 +77,8 @@ class EnsembleSet:
         for _ in range(int(n)):   # one step of every class in turn keeps every stream fed
             for e in self.parts:
                 e.step(1)
+                if self.serial:
+                    e.sync()
 
     def sync(self):
         for e in self.parts:
