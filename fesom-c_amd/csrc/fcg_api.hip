@@ -1064,10 +1064,14 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
         const fcg_mlp *net = ctx->mlp;
-        // persistent blocks, one per compute unit (the network's fragments fill most of a unit's LDS)
-        const int cpb = kMlpT / ctx->m.K;
-        const int groups = (ctx->m.N + cpb - 1) / cpb;
-        LAUNCH_EX(ctx, k_mlp_closure<R>, std::min(groups, ctx->n_cu), kMlpT, kMlpClosureLds, ctx->m, s,
+        // persistent blocks, one per compute unit (the network's fragments fill
+        // most of a unit's LDS); each wavefront takes whole columns
+        if (ctx->m.K < kMlpMinK || ctx->m.K > kMlpMaxK)
+            return fcg_fail(ctx, FCG_E_ARG, "the learned closure needs 4 <= nsigma <= 64");
+        const int cpw = 64 / ctx->m.K;
+        const int groups = (ctx->m.N + cpw - 1) / cpw;
+        const int blocks = std::min((groups + kMlpT / 64 - 1) / (kMlpT / 64), ctx->n_cu);
+        LAUNCH_EX(ctx, k_mlp_closure<R>, blocks, kMlpT, kMlpClosureLds, ctx->m, s,
                   (const fcgm::bf16x8 *)net->frag, (const float *)net->bias, net->norm, p.Dmin, ctx->density_0,
                   p.z0b_min, p.snul, ctx->mlp_flag);
         d3end(ctx, s, (const int *)ctx->mlp_flag);
@@ -1348,6 +1352,8 @@ extern "C" int fcg_set_mlp(fcg_ctx *ctx, fcg_mlp *net) {
     if (ctx->p.closure != FCG_CLOSURE_MLP) return fcg_fail(ctx, FCG_E_ARG, "context was created with closure = d3_end");
     if (net && kMlpClosureLds > (size_t)ctx->lds_limit)
         return fcg_fail(ctx, FCG_E_ARG, "the learned closure's LDS exceeds the device limit");
+    if (net && (ctx->m.K < kMlpMinK || ctx->m.K > kMlpMaxK))
+        return fcg_fail(ctx, FCG_E_ARG, "the learned closure needs 4 <= nsigma <= 64");
     ctx->mlp = net;
     return FCG_OK;
 }

@@ -18,6 +18,19 @@
 
 namespace fcgk {
 
+// Loads through pointers of a named address space.  Where one branch reads a
+// neighbour from the block's LDS columns and the other from global memory,
+// these keep the two loads apart; through plain pointers the compiler merges
+// them into one generic (flat) load behind a pointer select.
+template <typename T>
+__device__ __forceinline__ T lds_ld(const T *p) {
+    return *(const __attribute__((address_space(3))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ T glb_ld(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
 // (2,E)/(4,E) interleaved records of the barotropic momentum path in the
 // storage type, widened to double on load and rounded on store
 __device__ __forceinline__ double2 ld2(const double *p, size_t i) { return ((const double2 *)p)[i]; }
@@ -922,20 +935,20 @@ __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt
             R ux, vx, guxx, guyx, gvxx, gvyx;
             if (g.oc >= 0) {  // the other element is a column of this block
                 const int i = g.oc * L + k;
-                ux = cu[i];
-                vx = cv[i];
-                guxx = cgux[i];
-                guyx = cguy[i];
-                gvxx = cgvx[i];
-                gvyx = cgvy[i];
+                ux = lds_ld(cu + i);
+                vx = lds_ld(cv + i);
+                guxx = lds_ld(cgux + i);
+                guyx = lds_ld(cguy + i);
+                gvxx = lds_ld(cgvx + i);
+                gvyx = lds_ld(cgvy + i);
             } else {
                 const int io = g.oK + k;
-                ux = s.U_n[io];
-                vx = s.V_n[io];
-                guxx = s.gux[io];
-                guyx = s.guy[io];
-                gvxx = s.gvx[io];
-                gvyx = s.gvy[io];
+                ux = glb_ld(s.U_n + io);
+                vx = glb_ld(s.V_n + io);
+                guxx = glb_ld(s.gux + io);
+                guyx = glb_ld(s.guy + io);
+                gvxx = glb_ld(s.gvx + io);
+                gvyx = glb_ld(s.gvy + io);
             }
             const R U1 = own1 ? uo : ux, V1 = own1 ? vo : vx, U2 = own1 ? ux : uo, V2 = own1 ? vx : vo;
             const R gux1 = own1 ? guxo : guxx, guy1 = own1 ? guyo : guyx, gvx1 = own1 ? gvxo : gvxx, gvy1 = own1 ? gvyo : gvyx;
@@ -2644,20 +2657,20 @@ __global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, dou
         R Tx, Sx, gTxx, gTyx, gSxx, gSyx;
         if (col >= 0) {  // the other end is a column of this block
             const int i = col * L + k;
-            Tx = cT[i];
-            Sx = cS[i];
-            gTxx = cTx[i];
-            gTyx = cTy[i];
-            gSxx = cSx[i];
-            gSyx = cSy[i];
+            Tx = lds_ld(cT + i);
+            Sx = lds_ld(cS + i);
+            gTxx = lds_ld(cTx + i);
+            gTyx = lds_ld(cTy + i);
+            gSxx = lds_ld(cSx + i);
+            gSyx = lds_ld(cSy + i);
         } else {
             const int ao = (own1 ? g.n2K : g.n1K) + k;
-            Tx = s.abT[ao];
-            Sx = s.abS[ao];
-            gTxx = s.gTx[ao];
-            gTyx = s.gTy[ao];
-            gSxx = s.gSx[ao];
-            gSyx = s.gSy[ao];
+            Tx = glb_ld(s.abT + ao);
+            Sx = glb_ld(s.abS + ao);
+            gTxx = glb_ld(s.gTx + ao);
+            gTyx = glb_ld(s.gTy + ao);
+            gSxx = glb_ld(s.gSx + ao);
+            gSyx = glb_ld(s.gSy + ao);
         }
         const R T1 = own1 ? To : Tx, T2 = own1 ? Tx : To;
         const R S1 = own1 ? So : Sx, S2 = own1 ? Sx : So;
@@ -2940,17 +2953,37 @@ __global__ __launch_bounds__(256) void k_mlp_collect(DevMesh m, DevStateT<R> s, 
 }
 
 // The closure step.  Persistent blocks of kMlpT threads, one per compute
-// unit: the network's forward fragments and biases go to LDS once per block;
-// the block then walks a contiguous range of column groups (cpb = kMlpT/K
-// whole node columns, one thread per (column, level 1..K) sample).  Per
-// column the depth factors and the bottom-stress feature are formed once;
+// unit: the network's forward fragments and biases go to LDS once per block.
+// Each wavefront then walks its own contiguous range of column groups
+// (64/K whole node columns, one lane per (column, level 1..K) sample), with
+// no block-wide barrier, so one wavefront's matrix products overlap another's
+// feature arithmetic.  Per column the depth factors and the bottom-stress
+// feature are formed once; per level the density coefficients and velocity;
 // per sample the five features (as column_features), normalised and checked
-// against the training box; each wavefront then runs its 64 samples through
+// against the training box; the wavefront then runs its 64 samples through
 // the network as four 16-sample tiles on the matrix cores.  snu and Kv =
 // 10^(de-normalised outputs), snu held in [snul, 0.2] as d3_end holds it
 // (fv_mixing.F90:282-285), go straight to the column slots, for every column
 // whose samples all lie in the box; any other column is flagged and left to
 // d3_end.
+struct MlpWaveLds {
+    float xs[64 * kMlpF];
+    float ys[64 * 2];
+    EosLevel lv[64];
+    double2 uv[64];
+    double col_a[64 / kMlpMinK], col_dm[64 / kMlpMinK], col_f3[64 / kMlpMinK];
+    int col_ood[64 / kMlpMinK];
+};
+static_assert(sizeof(MlpWaveLds) == kMlpWaveLds, "closure LDS record");
+
+// LDS writes of this wavefront visible to its other lanes, and no code motion
+// of its memory operations across this point
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename R>
 __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s, const fcgm::bf16x8 *__restrict__ frag,
                                                        const float *__restrict__ bias, MlpNorm nm, double Dmin,
@@ -2958,99 +2991,137 @@ __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s
     extern __shared__ __align__(16) unsigned char lds_[];
     fcgm::bf16x8 *fl = (fcgm::bf16x8 *)lds_;
     float *bl = (float *)(fl + kFragFwd * 64);
-    float *xs = bl + kMlpBias;
-    float *ys = xs + kMlpT * kMlpF;
-    double *col_a = (double *)(ys + kMlpT * 2), *col_dm = col_a + kMlpT / 4, *col_f3 = col_dm + kMlpT / 4;
-    EosLevel *lv = (EosLevel *)(col_f3 + kMlpT / 4);   // per (column, level): density coefficients
-    double2 *uv = (double2 *)(lv + kMlpT);             // and the node-averaged velocity
-    int *col_ood = (int *)(uv + kMlpT);
+    double *sg = (double *)(bl + kMlpBias), *om = sg + kMlpMaxK;   // sigma and 1-sigma of the K levels
+    MlpWaveLds &w = ((MlpWaveLds *)(om + kMlpMaxK))[threadIdx.x >> 6];
     const int K = m.K, L = m.L, KC = K;
-    const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
+    const int tid = threadIdx.x, lane = tid & 63;
     for (int i = tid; i < kFragFwd * 64; i += kMlpT) fl[i] = frag[i];
     for (int i = tid; i < kMlpBias; i += kMlpT) bl[i] = bias[i];
-    const int cpb = kMlpT / K;
-    const int ngroups = (m.N + cpb - 1) / cpb;
-    const int per = (ngroups + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int g0 = (int)xcd_block() * per, g1 = min(ngroups, g0 + per);
-    const int c = tid / K, k = tid - (tid / K) * K;
+    for (int i = tid; i < K; i += kMlpT) {
+        sg[i] = m.sigma[i];
+        om[i] = m.oms[i];
+    }
+    const double dsb = m.dsig[L - 1];
+    __syncthreads();
+    const int cpw = 64 / K;  // whole columns per wavefront
+    const int ngroups = (m.N + cpw - 1) / cpw;
+    const int nwave = (int)gridDim.x * (kMlpT / 64);
+    const int per = (ngroups + nwave - 1) / nwave;
+    const int g0 = ((int)xcd_block() * (kMlpT / 64) + (tid >> 6)) * per, g1 = min(ngroups, g0 + per);
+    const int c = lane / K, k = lane - (lane / K) * K;
+    // 16-sample tiles that hold samples: 64/K columns of K levels
+    const int ntile = (cpw * K + 15) / 16;
+    // HBM operands of a group, loaded one group ahead so that their latency
+    // hides behind the previous group's matrix products
+    struct Pre {
+        R t, sa, u, v;                   // this lane's level
+        double a, dep, eta, ub, vb, ac;  // column lane (lane < columns of the group)
+    };
+    // (every lane loads, from a clamped in-range slot where it has no sample,
+    // so the loads stay unconditional and nothing waits for them before use)
+    auto fetch = [&](int g) {
+        Pre q;
+        const int c0 = g * cpw;
+        const int n = min(c0 + min(c, cpw - 1), m.N - 1), kl = min(k, L - 1);
+        q.t = PL(s.TF, kl, n, KC);
+        q.sa = PL(s.SF, kl, n, KC);
+        q.u = PL(s.Unode, kl, n, KC);
+        q.v = PL(s.Vnode, kl, n, KC);
+        const int nn = min(c0 + min(lane, cpw - 1), m.N - 1);
+        q.a = s.a[nn];
+        q.dep = m.depth[nn];
+        q.eta = s.eta_n[nn];
+        q.ub = PL(s.Unode, L - 1, nn, KC);
+        q.vb = PL(s.Vnode, L - 1, nn, KC);
+        q.ac = m.ac[nn];
+        return q;
+    };
+    Pre nxt = fetch(min(g0, ngroups - 1));
     for (int g = g0; g < g1; ++g) {
-        __syncthreads();  // fragments loaded / the previous group's LDS read
-        const int c0 = g * cpb, nc = min(cpb, m.N - c0);
+        wave_sync();  // the previous group's LDS reads are done
+        const Pre cur = nxt;
+        const int c0 = g * cpw, nc = min(cpw, m.N - c0);
         const bool on = c < nc;
         const int n = c0 + c, nz = k + 1;
-        if (tid < nc) {
-            const int nn = c0 + tid;
-            const double a = s.a[nn];
-            const double dmean = dmaxr(Dmin, m.depth[nn] + s.eta_n[nn]);
-            const double ub = PL(s.Unode, L - 1, nn, KC), vb = PL(s.Vnode, L - 1, nn, KC);
-            const double delta_b = a * m.dsig[L - 1] + z0b_min;
+        if (lane < nc) {
+            const double a = cur.a;
+            const double dmean = dmaxr(Dmin, cur.dep + cur.eta);
+            const double ub = cur.ub, vb = cur.vb;
+            const double delta_b = a * dsb + z0b_min;
             const double lg = log(delta_b / z0b_min);
-            col_a[tid] = a;
-            col_dm[tid] = dmean;
-            col_f3[tid] = log10f((float)(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12));
+            w.col_a[lane] = a;
+            w.col_dm[lane] = dmean;
+            w.col_f3[lane] = log10f((float)(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12));
             // columns of the open-boundary sponge layer (ac < 1), whose damped
             // dynamics the network is not trained for, keep d3_end
-            col_ood[tid] = m.ac[nn] < 1.0 ? 1 : 0;
+            w.col_ood[lane] = cur.ac < 1.0 ? 1 : 0;
         }
-        if (on && k < L) {  // each level's T, S and velocity once (thread tid = c*K + k)
-            lv[tid] = eos_level(PL(s.TF, k, n, KC), PL(s.SF, k, n, KC));
-            uv[tid] = make_double2(PL(s.Unode, k, n, KC), PL(s.Vnode, k, n, KC));
+        if (on && k < L) {  // each level's T, S and velocity once
+            w.lv[lane] = eos_level(cur.t, cur.sa);
+            w.uv[lane] = make_double2(cur.u, cur.v);
         }
-        __syncthreads();
+        nxt = fetch(min(g + 1, g1 - 1));
+        wave_sync();
         // a group whose columns all lie in the sponge layer skips the network
-        if (__syncthreads_and(!on || col_ood[c] != 0)) {
+        if (__all(!on || w.col_ood[c] != 0)) {
             if (on && k == 0) flag[n] = 1;
             continue;
         }
         float fx[kMlpF] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (on) {
-            const double a = col_a[c], dmean = col_dm[c];
+            const double a = w.col_a[c], dmean = w.col_dm[c];
             const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
-            const int up = c * K + iz - 2, dn = up + 1;                 // its two layers in LDS
-            const double2 w1 = uv[up], w2 = uv[dn];
-            const double d1 = m.sigma[iz - 2] - m.sigma[iz - 1], d2 = m.sigma[iz - 1] - m.sigma[iz];
+            const int up = c * K + iz - 2, dn = up + 1;                 // its two layers
+            const double2 w1 = w.uv[up], w2 = w.uv[dn];
+            const double d1 = sg[iz - 2] - sg[iz - 1], d2 = sg[iz - 1] - sg[iz];
             const double d = 0.5 * (d1 + d2);
             const double sh2 = ((w1.x - w2.x) * (w1.x - w2.x) + (w1.y - w2.y) * (w1.y - w2.y)) / (d * d) /
                                (dmean * dmean);
-            const double zb = m.oms[iz - 1] * a;
-            const double rho_up = eos_rho(lv[up], -zb, rho0), rho_dn = eos_rho(lv[dn], -zb, rho0);
+            const double zb = om[iz - 1] * a;
+            const double rho_up = eos_rho(w.lv[up], -zb, rho0), rho_dn = eos_rho(w.lv[dn], -zb, rho0);
             const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
             fx[0] = log10f((float)(sh2 + 1e-12));
             fx[1] = asinhf((float)(n2 * 1e5));
             fx[2] = log10f((float)dmean);
-            fx[3] = (float)col_f3[c];
-            fx[4] = (float)m.sigma[nz - 1];
+            fx[3] = (float)w.col_f3[c];
+            fx[4] = (float)sg[nz - 1];
             bool ood = false;
             for (int j = 0; j < kMlpF; ++j) {
                 fx[j] = (fx[j] - nm.x_mean[j]) / nm.x_std[j];
                 ood = ood || !(fx[j] >= nm.x_lo[j] && fx[j] <= nm.x_hi[j]);
             }
-            if (ood) atomicOr(&col_ood[c], 1);
+            if (ood) atomicOr(&w.col_ood[c], 1);
         }
-        for (int j = 0; j < kMlpF; ++j) xs[tid * kMlpF + j] = fx[j];
-        // the features stay within the wavefront: no block barrier, so one
-        // wavefront's matrix products overlap another's feature arithmetic
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        {  // the wavefront's 64 samples as four tiles
+        for (int j = 0; j < kMlpF; ++j) w.xs[lane * kMlpF + j] = fx[j];
+        wave_sync();
+        if (ntile == 4) {  // the wavefront's samples as 16-sample tiles
             fcgm::bf16x8 x[4];
-            for (int t = 0; t < 4; ++t) x[t] = fcgm::load_bx(&xs[(wbase + 16 * t) * kMlpF], lane);
+            for (int t = 0; t < 4; ++t) x[t] = fcgm::load_bx(&w.xs[16 * t * kMlpF], lane);
             fcgm::f32x4 y[4];
             fcgm::infer_tiles<4>(fl, bl, x, lane, y);
             if (lane < 16)
                 for (int t = 0; t < 4; ++t) {
-                    ys[(wbase + 16 * t + lane) * 2] = y[t][0];
-                    ys[(wbase + 16 * t + lane) * 2 + 1] = y[t][1];
+                    w.ys[(16 * t + lane) * 2] = y[t][0];
+                    w.ys[(16 * t + lane) * 2 + 1] = y[t][1];
+                }
+        } else {  // at most 48 samples (24 levels: two columns)
+            fcgm::bf16x8 x[3];
+            for (int t = 0; t < 3; ++t) x[t] = fcgm::load_bx(&w.xs[16 * t * kMlpF], lane);
+            fcgm::f32x4 y[3];
+            fcgm::infer_tiles<3>(fl, bl, x, lane, y);
+            if (lane < 16)
+                for (int t = 0; t < 3; ++t) {
+                    w.ys[(16 * t + lane) * 2] = y[t][0];
+                    w.ys[(16 * t + lane) * 2 + 1] = y[t][1];
                 }
         }
-        __syncthreads();
+        wave_sync();
         if (!on) continue;
-        const bool ood = col_ood[c] != 0;
+        const bool ood = w.col_ood[c] != 0;
         if (k == 0) flag[n] = ood ? 1 : 0;
         if (ood) continue;
-        const float ls = ys[tid * 2] * nm.y_std[0] + nm.y_mean[0];
-        const float lk = ys[tid * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
+        const float ls = w.ys[lane * 2] * nm.y_std[0] + nm.y_mean[0];
+        const float lk = w.ys[lane * 2 + 1] * nm.y_std[1] + nm.y_mean[1];
         double sv = (double)exp10f(ls);
         sv = sv < snul ? snul : (sv > 0.2 ? 0.2 : sv);
         PL(s.snu, nz - 1, n, KC) = (R)sv;

@@ -40,14 +40,17 @@ constexpr int kFragG4 = 76, kFragG3 = 84, kFragG2 = 116;
 constexpr int kMlpFragCount = 148;
 
 // The fused closure kernel: threads per (persistent) block and its LDS: the
-// forward fragments and biases, then per column group the features and
-// outputs of every sample, three doubles and a flag for at most kMlpT/4
-// columns (nsigma >= 4), and per (column, level) the four density
-// coefficients and the two velocity components (doubles).
+// forward fragments, biases and the sigma levels, shared by the block, then one record per
+// wavefront: the features and outputs of its 64 samples, per sample the four
+// density coefficients and the two velocity components (doubles), and three
+// doubles and a flag for each of its at most 16 columns (4 <= nsigma <= 64:
+// a wavefront holds 64/nsigma whole columns).
 constexpr int kMlpT = 512;
-constexpr size_t kMlpClosureLds = 16 * kFragFwd * 64 + sizeof(float) * (kMlpBias + kMlpT * (kMlpF + 2)) +
-                                  sizeof(double) * 3 * (kMlpT / 4) + sizeof(double) * 6 * kMlpT +
-                                  sizeof(int) * (kMlpT / 4);
+constexpr int kMlpMinK = 4, kMlpMaxK = 64;
+constexpr size_t kMlpWaveLds = sizeof(float) * 64 * (kMlpF + 2) + sizeof(double) * 6 * 64 +
+                               (sizeof(double) * 3 + sizeof(int)) * (64 / kMlpMinK);
+constexpr size_t kMlpClosureLds = 16 * kFragFwd * 64 + sizeof(float) * kMlpBias + 2 * sizeof(double) * kMlpMaxK +
+                                  (kMlpT / 64) * kMlpWaveLds;
 
 // Entries that are real weights (the rest is zero padding and stays zero).
 __host__ __device__ inline bool mlp_param_used(int p) {
@@ -96,9 +99,18 @@ __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-// floats -> BF16 (round to nearest even): the gfx950 packed conversion
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// floats -> BF16 (round to nearest even): the gfx950 packed conversion, one
+// instruction per pair (a 4-wide conversion is split into single lanes and
+// re-packed by the compiler)
+__device__ __forceinline__ unsigned pack2(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{a, b}, bf16x2v));
+}
 __device__ __forceinline__ bf16x4 pack4(f32x4 v) {
-    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16x4v));
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(bf16x4, u32x2{pack2(v[0], v[1]), pack2(v[2], v[3])});
 }
 // two D tiles -> one B slab (K order kappa), ReLU applied to the BF16 values:
 // a BF16 value is negative exactly when its sign bit is set, so a signed

@@ -187,3 +187,60 @@ def test_out_of_distribution_columns_fall_back_to_d3end(trained_net):
     for m in (ref, fb):
         m.close()
     net.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nsigma", [8, 20, 24, 40])
+def test_closure_kernel_matches_cpu_network(nsigma):
+    """k_mlp_closure against the CPU network on the device's own features.
+    Each wavefront of the kernel holds 64/nsigma whole node columns: 8, 3, 2
+    and 1 of them at these level counts, in four, four, three and three
+    16-sample tiles.  With no sponge layer and an unbounded feature box every
+    column takes the network; right after the vertical-mixing call,
+    fcg_mlp_collect returns the features the kernel saw and log10 of the
+    snu/Kv it wrote, in the same order.  The CPU forward (BF16 operands as the
+    matrix cores take them, float64 sums) must give the same outputs, after
+    snu is held in [snul, 0.2], up to BF16 rounding ties of the hidden values:
+    within 0.05 in log10 everywhere and 1e-3 at the median."""
+    import re
+    import torch
+    from fesomc_amd import gpu, meshgen
+    from fesomc_amd.runconfig import RunConfig
+    txt, mesh, ex = meshgen.config("c2_small")
+    for k, v in (("nsigma", nsigma), ("SL_obn", "F")):
+        txt, n = re.subn(rf"^(\s*{k}\s*=\s*)\S+", rf"\g<1>{v}", txt, count=1, flags=re.M)
+        assert n == 1, k
+    T0, S0 = meshgen.stratified_ts(mesh, nsigma, seed=0)
+    cfg = RunConfig(txt)
+    m = gpu.Model(cfg, mesh, precision=gpu.FCG_MIXED, closure=gpu.FCG_CLOSURE_MLP)
+    m.set_tracers(T0, S0)
+    net = gpu.Mlp(seed=3)
+    p = mlp_ref.init(seed=3)
+    net.set_params(p)
+    xm, xs = np.array([-6.0, 0.0, 1.0, -6.0, 0.5]), np.array([2.0, 3.0, 0.5, 2.0, 0.3])
+    ym, ys = np.array([-3.0, -4.0]), np.array([0.7, 0.7])
+    net.set_norm(xm, xs, ym, ys, np.full(5, -1e30), np.full(5, 1e30))
+    m.attach_mlp(net)
+    m.step(3)
+    for ph in gpu.Model.PHASES:
+        m.phase(ph)
+        if ph == "fcg_vertical_mixing":
+            break
+    cap = m.N * m.K
+    X = torch.empty((cap, gpu.MLP_FEATURES), dtype=torch.float32, device="cuda")
+    Tt = torch.empty((cap, 2), dtype=torch.float32, device="cuda")
+    n = m.collect(X.data_ptr(), Tt.data_ptr(), cap, 1)
+    assert n == cap
+    assert m.diagnostics()["mlp_fallback"] == 0
+    X, Tt = X.cpu().numpy().astype(np.float64), Tt.cpu().numpy().astype(np.float64)
+    Xn = ((X.astype(np.float32) - xm.astype(np.float32)) / xs.astype(np.float32)).astype(np.float64)
+    y, _ = mlp_ref.forward(p, Xn, bf=True)
+    ls = y[:, 0] * ys[0] + ym[0]
+    lk = y[:, 1] * ys[1] + ym[1]
+    snul = m.par.snul
+    ls = np.log10(np.clip(10.0 ** ls, snul, 0.2))
+    d = np.abs(np.concatenate([Tt[:, 0] - ls, Tt[:, 1] - lk]))
+    print(f"nsigma {nsigma}: max |dlog10| {d.max():.3e}, median {np.median(d):.3e}")
+    assert np.median(d) < 1e-3 and d.max() < 0.05
+    m.close()
+    net.close()
