@@ -126,6 +126,7 @@ inline size_t lds_max(const DevMesh &m, size_t esz) {
     size_t b = std::max(2 * tile_bytes(m, 8), tile_bytes(m, 8) + 2 * tile_bytes(m, esz));  // bpg, velvisc
     b = std::max(b, 5 * tile_bytes(m, esz));                // vdiff
     b = std::max(b, 5 * flat_bytes(m.K, esz));              // d3_end prep
+    b = std::max(b, 4 * (size_t)m.K * kD3endT * esz);       // d3_end iteration
     return std::max(b, lds_p1edge(m));
 }
 
@@ -1048,7 +1049,8 @@ static void d3end(fcg_ctx *ctx, const DevStateT<R> &s, const int *only) {
     const DevMesh &m = ctx->m;
     LAUNCH_EX(ctx, k_d3end_prep<R>, nflatblk(m.N, m.K), kFlatT, 5 * flat_bytes(m.K, sizeof(R)), m, s, p.dt, p.Dmin,
               ctx->density_0, only);
-    LAUNCH1(ctx, k_d3end<R>, m.N, m, s, p.dt, p.Dmin, p.z0b_min, only);
+    LAUNCH_EX(ctx, k_d3end<R>, (unsigned)((m.N + kD3endT - 1) / kD3endT), kD3endT,
+              4 * (size_t)m.K * kD3endT * sizeof(R), m, s, p.dt, p.Dmin, p.z0b_min, only);
     LAUNCH_EX(ctx, k_d3end_fin<R>, nflatblk(m.N, m.K), kFlatT, flat_bytes(m.K, sizeof(R)), m, s, p.Dmin, p.z0b_min,
               p.z0s_min, p.snul, only);
 }

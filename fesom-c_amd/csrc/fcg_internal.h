@@ -196,6 +196,7 @@ constexpr size_t kVelrhsElemBytes = 160;
 constexpr size_t velrhs_edge_bytes(size_t esz) { return esz == 8 ? 104 : 64; }
 
 constexpr int kTideRing = 4;   // pinned staging slots of the tidal open-boundary values
+constexpr int kD3endT = 64;    // node columns per block of the d3_end iteration (its column state in LDS)
 
 struct fcg_ctx {
     fcg_params p{};

@@ -438,9 +438,17 @@ __global__ __launch_bounds__(256) void k_d3end_prep(DevMesh m, DevStateT<R> s, d
     rows_store(s.mx_c2, tc, f.c0, f.nc, K, N, 1, K - 1, only);
 }
 
+// One thread per node column, kD3endT columns per block.  The iterated
+// column -- the btr ping-pong and the sweep's xx1/yy1 -- stays in the block's
+// LDS, slot [k*kD3endT + thread], for all iterations; the per-level operands
+// of k_d3end_prep (p, q, c2, the snu column) are read from their level planes.
 template <typename R>
-__global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double dt, double Dmin, double z0b_min,
-                                               const int *__restrict__ only) {
+__global__ __launch_bounds__(kD3endT) void k_d3end(DevMesh m, DevStateT<R> s, double dt, double Dmin, double z0b_min,
+                                                   const int *__restrict__ only) {
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int KB = m.K * kD3endT, th = threadIdx.x;
+    R *lA = (R *)lds_, *lB = lA + KB, *lxx = lB + KB, *lyy = lxx + KB;
+#define LX(arr, k) (arr)[(k) * kD3endT + th]
     int n = TID();
     if (n >= m.N) return;
     if (only && only[n] == 0) return;  // learned closure active: only the columns it sent back
@@ -461,14 +469,15 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
     const double delta_b = a * m.dsig[L - 1] + z0b_min;
     const double lg = fcglog::log_cr(delta_b / z0b_min);  // correctly rounded, as the host libm's
     const double bott_R = B1cka2 / (lg * lg);
-    R *bc = s.mx_bA, *bn = s.mx_bB;
+    for (int k = 0; k < K; ++k) LX(lA, k) = PP(s.mx_bA, k, n, N);
+    R *bc = lA, *bn = lB;
     for (int it = 1; it <= 200; ++it) {
         double xxp = 1.0, yyp = yy2;
         for (int nz = 2; nz <= K - 1; ++nz) {
             double d1 = SG(nz - 1) - SG(nz);
             double d2 = SG(nz) - SG(nz + 1);
             double d = 0.5 * (d1 + d2);
-            double btr = PP(bc, nz - 1, n, N);
+            double btr = LX(bc, nz - 1);
             double sn = SNU(nz);
             double c1 = 0.046 * (btr * btr) / sn;
             double fr = PP(s.mx_p, nz - 1, n, N) - c1 - PP(s.mx_q, nz - 1, n, N) - PP(s.mx_c2, nz - 1, n, N);
@@ -480,29 +489,29 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
             double aa = bm - am * xxp;
             double xxn = cm / aa;
             double yyn = (am * yyp - fr) / aa;
-            PP(s.mx_xx, nz, n, N) = xxn;   // xx1(nz+1)
-            PP(s.mx_yy, nz, n, N) = yyn;
+            LX(lxx, nz) = xxn;   // xx1(nz+1)
+            LX(lyy, nz) = yyn;
             xxp = xxn;
             yyp = yyn;
         }
         // back substitution from the bottom boundary value, with convergence measure
         double b1v = bott_R * (ub * ub + vb * vb);
-        PP(bn, K - 1, n, N) = b1v;
+        LX(bn, K - 1) = b1v;
         double bbb = 0.0;
         {
-            double bo = PP(bc, K - 1, n, N);
+            double bo = LX(bc, K - 1);
             double aa = bo - b1v;
             double bb = dmaxr(b1v, bo) + 1.0e-8;
             double cc = fabs(aa / bb);
             if (cc > bbb) bbb = cc;
         }
         for (int nz = K; nz >= 2; --nz) {
-            double xx = (nz == 2) ? 1.0 : PP(s.mx_xx, nz - 1, n, N);
-            double yy = (nz == 2) ? yy2 : PP(s.mx_yy, nz - 1, n, N);
+            double xx = (nz == 2) ? 1.0 : (double)LX(lxx, nz - 1);
+            double yy = (nz == 2) ? yy2 : (double)LX(lyy, nz - 1);
             if (nz == K) { xx = xxp; yy = yyp; }
             b1v = b1v * xx + yy;
-            PP(bn, nz - 2, n, N) = b1v;
-            double bo = PP(bc, nz - 2, n, N);
+            LX(bn, nz - 2) = b1v;
+            double bo = LX(bc, nz - 2);
             double aa = bo - b1v;
             double bb = dmaxr(b1v, bo) + 1.0e-8;
             double cc = fabs(aa / bb);
@@ -512,8 +521,9 @@ __global__ __launch_bounds__(256) void k_d3end(DevMesh m, DevStateT<R> s, double
         R *t = bc; bc = bn; bn = t;
     }
     // the converged column, for the per-level closure values
-    for (int k = 0; k < K; ++k) PP(s.mx_xx, k, n, N) = PP(bc, k, n, N);
+    for (int k = 0; k < K; ++k) PP(s.mx_xx, k, n, N) = LX(bc, k);
 #undef SNU
+#undef LX
 }
 
 template <typename R>
