@@ -226,8 +226,8 @@ def main():
                     help="headline: mixed precision + learned closure, timed beside the plain FP64 drop-in")
     ap.add_argument("--closure-weights", default=os.path.join(ROOT, "fesom-c_amd", "closure_c2.npz"))
     ap.add_argument("--launch-table", default="auto",
-                    help="block-size table from fesomc_amd.autotune (JSON); 'auto' = the stored table of this "
-                         "workload and precision if there is one, 'none' = library defaults")
+                    help="block-size tables from fesomc_amd.autotune (JSON, one per level count); 'auto' = the "
+                         "stored tables of this workload and precision if there are some, 'none' = library defaults")
     ap.add_argument("--serial-classes", action="store_true",
                     help="c5 only: finish each level class's step before the next starts (kernel times without "
                          "overlap; the default lets the classes' streams overlap)")
@@ -279,17 +279,20 @@ def main():
         held["mesh"], held["txt"] = mesh, txt
         if clos == gpu.FCG_CLOSURE_MLP:
             model.attach_mlp(net)
+        return model
+
+    def tuned(model, precision):
         table = args.launch_table
         if table == "auto":
-            table = os.path.join(ROOT, "fesom-c_amd", f"launch_{'mixed' if precision else 'fp64'}_c2.json")
-            table = table if os.path.exists(table) else "none"
+            table = autotune.table_path(args.workload, precision)
+            table = table if os.path.exists(table) and not args.scale else "none"
         if table != "none":
-            autotune.apply(model, autotune.load(table))
-        return model
+            autotune.apply_parts(model, autotune.load_parts(table))
 
     def run(precision, clos, probe):
         t0s = time.perf_counter()
         model = make(precision, clos)
+        tuned(model, precision)
         setup = time.perf_counter() - t0s
         log(f"precision {precision} closure {clos}: model ready in {setup:.1f} s")
         model.step(args.warmup)

@@ -100,7 +100,9 @@ inline int cpb_of(int S, int T = kFlatT) { return T / S; }
 inline unsigned nflatblk(int ents, int S, int T = kFlatT) {
     return (unsigned)((ents + cpb_of(S, T) - 1) / cpb_of(S, T));
 }
-inline size_t flat_bytes(int S, size_t esz) { return (size_t)cpb_of(S) * S * esz; }
+inline size_t flat_bytes(int S, size_t esz, int T = kFlatT) { return (size_t)cpb_of(S, T) * S * esz; }
+// node-gather block sizes (threads) the launch table may choose
+inline int gather_slot(int T) { return T == 1024 ? 2 : (T == 512 ? 1 : 0); }
 // k_velrhs: per column one element record and four edge records, for T-thread
 // blocks (wider blocks hold more columns, so more neighbours come from LDS)
 inline size_t velrhs_bytes(const DevMesh &m, size_t esz, int T = kFlatT) {
@@ -112,14 +114,15 @@ inline size_t lds_p1edge(const DevMesh &m) {
 }
 // node-gather blocks: LDS records for every CSR slot of the block's nodes
 template <typename R>
-inline size_t lds_muscl(const fcg_ctx *c) {
-    return (size_t)c->cap_n2e * sizeof(EdgeGeo<R>) + (size_t)((c->cap_n2e + 1) & ~1) * sizeof(int) +
-           6 * flat_bytes(c->m.L, sizeof(R));
+inline size_t lds_muscl(const fcg_ctx *c, int T = kFlatT) {
+    const int cap = c->cap_n2e_t[gather_slot(T)];
+    return (size_t)cap * sizeof(EdgeGeo<R>) + (size_t)((cap + 1) & ~1) * sizeof(int) +
+           6 * flat_bytes(c->m.L, sizeof(R), T);
 }
 template <typename R>
-inline size_t lds_vdiv(const fcg_ctx *c) { return (size_t)c->cap_n2e * sizeof(DivRec<R>); }
-inline size_t lds_gradn(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(GradRec); }
-inline size_t lds_el2n(const fcg_ctx *c) { return (size_t)c->cap_n2el * sizeof(PulsRec); }
+inline size_t lds_vdiv(const fcg_ctx *c, int T = kFlatT) { return (size_t)c->cap_n2e_t[gather_slot(T)] * sizeof(DivRec<R>); }
+inline size_t lds_gradn(const fcg_ctx *c, int T = kFlatT) { return (size_t)c->cap_n2el_t[gather_slot(T)] * sizeof(GradRec); }
+inline size_t lds_el2n(const fcg_ctx *c, int T = kFlatT) { return (size_t)c->cap_n2el_t[gather_slot(T)] * sizeof(PulsRec); }
 
 // largest LDS request of the tile kernels for this mesh and storage type
 inline size_t lds_max(const DevMesh &m, size_t esz) {
@@ -263,6 +266,14 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
     return false;
 }
 
+// Launch block size of a kernel (without its storage-type argument): the
+// autotuned value when one was set, else 256.
+static int block_of(const fcg_ctx *ctx, const char *kern) {
+    const std::string k(kern, std::strcspn(kern, "<"));
+    auto it = ctx->launch_block.find(k);
+    return it == ctx->launch_block.end() ? 256 : it->second;
+}
+
 // Tidal elevation at the open boundary for each substep of the coming
 // barotropic loop: compute_ssh_rhs_elem, src/fv_average_dynamic.F90:190-216,
 // evaluated on the host with the same libm cos as a CPU build.
@@ -307,7 +318,8 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s0) {
             hipLaunchKernelGGL(fcgk::k_bt_edge_mom<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, p.Dmin);
         if (sub == 1) {
             hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
-            hipLaunchKernelGGL((fcgk::k_el2nodes<R, true>), dim3(nflatblk(m.N, m.L)), dim3(kFlatT), lds_el2n(ctx), st,
+            const int be = block_of(ctx, "k_el2nodes");
+            hipLaunchKernelGGL((fcgk::k_el2nodes<R, true>), dim3(nflatblk(m.N, m.L, be)), dim3(be), lds_el2n(ctx, be), st,
                                m, (const R *)s.U_n, (const R *)s.V_n, s.Unode_p, s.Vnode_p, (const double *)s.puls_ap,
                                (const double *)s.U2);
             hipLaunchKernelGGL(fcgk::k_bt_p1edge<R>, dim3(nflatblk(m.D, m.L)), dim3(kFlatT), lds_p1edge(m), st, m, s);
@@ -325,13 +337,6 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s0) {
 
 // Launch a kernel on the context stream, bracketed by probe events when the
 // kernel's name (without its storage-type argument) is the probed one.
-// Launch block size of a kernel (without its storage-type argument): the
-// autotuned value when one was set, else 256.
-static int block_of(const fcg_ctx *ctx, const char *kern) {
-    const std::string k(kern, std::strcspn(kern, "<"));
-    auto it = ctx->launch_block.find(k);
-    return it == ctx->launch_block.end() ? 256 : it->second;
-}
 
 static bool probe_is(const std::string &probe, const char *kern) {
     if (probe.empty()) return false;
@@ -347,16 +352,19 @@ static bool probe_is(const std::string &probe, const char *kern) {
     } while (0)
 // one thread per (entity, layer), consecutive threads along a column
 #define LAUNCHLV(ctx, kern, ents, levels, ...) LAUNCH1(ctx, kern, (size_t)(ents) * (size_t)(levels), __VA_ARGS__)
-#define LAUNCH_EX(ctx, kern, grid, block, lds, ...)                                         \
+#define LAUNCH_EX(ctx, kern, grid, block, lds, ...) LAUNCH_NAMED(ctx, #kern, fcgk::kern, grid, block, lds, __VA_ARGS__)
+// the same for a kernel expression whose probe name is given (a template
+// with several arguments)
+#define LAUNCH_NAMED(ctx, name, kern, grid, block, lds, ...)                               \
     do {                                                                                   \
-        bool pr__ = probe_is((ctx)->probe_name, #kern);                                    \
+        bool pr__ = probe_is((ctx)->probe_name, name);                                     \
         hipEvent_t a__ = nullptr, b__ = nullptr;                                           \
         if (pr__) {                                                                        \
             (void)hipEventCreate(&a__);                                                    \
             (void)hipEventCreate(&b__);                                                    \
             (void)hipEventRecord(a__, (ctx)->stream);                                      \
         }                                                                                  \
-        hipLaunchKernelGGL(fcgk::kern, dim3(grid), dim3(block), (lds), (ctx)->stream, __VA_ARGS__); \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), (lds), (ctx)->stream, __VA_ARGS__);       \
         if (pr__) {                                                                        \
             (void)hipEventRecord(b__, (ctx)->stream);                                      \
             (ctx)->probe_ev.emplace_back(a__, b__);                                        \
@@ -592,12 +600,12 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         n2el_ptr[n + 1] = n2el_ptr[n] + (int)n2el[n].size();
         n2el_idx.insert(n2el_idx.end(), n2el[n].begin(), n2el[n].end());
     }
-    {  // most CSR slots in one node-gather block of cpb = 256/L consecutive nodes
-        const int cpb = std::max(1, 256 / L);
+    for (int t = 0; t < 3; ++t) {  // most CSR slots in one node-gather block of cpb = T/L consecutive nodes
+        const int cpb = std::max(1, (256 << t) / L);
         for (int c0 = 0; c0 < N; c0 += cpb) {
             const int c1 = std::min(N, c0 + cpb);
-            ctx->cap_n2e = std::max(ctx->cap_n2e, n2e_ptr[c1] - n2e_ptr[c0]);
-            ctx->cap_n2el = std::max(ctx->cap_n2el, n2el_ptr[c1] - n2el_ptr[c0]);
+            ctx->cap_n2e_t[t] = std::max(ctx->cap_n2e_t[t], n2e_ptr[c1] - n2e_ptr[c0]);
+            ctx->cap_n2el_t[t] = std::max(ctx->cap_n2el_t[t], n2el_ptr[c1] - n2el_ptr[c0]);
         }
     }
     // ---- node constants
@@ -1060,8 +1068,10 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const fcg_params &p = ctx->p;
     // z0b_min/z0s_min/snul as fv_run.dat gives them
     // compute_el2nodes_3D(U_n, V_n, Unode, Vnode), src/fv_main.F90:2006
-    hipLaunchKernelGGL((fcgk::k_el2nodes<R, false>), dim3(nflatblk(ctx->m.N, ctx->m.L)), dim3(kFlatT), lds_el2n(ctx),
-                       ctx->stream, ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode, nullptr, nullptr);
+    const int be = block_of(ctx, "k_el2nodes");
+    LAUNCH_NAMED(ctx, "k_el2nodes", (fcgk::k_el2nodes<R, false>), nflatblk(ctx->m.N, ctx->m.L, be), be,
+                 lds_el2n(ctx, be), ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode, (const double *)nullptr,
+                 (const double *)nullptr);
     if (p.closure == FCG_CLOSURE_MLP) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
@@ -1212,7 +1222,8 @@ extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
 
 template <typename R>
 static int vert_vel_sigma_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCH_EX(ctx, k_vertvel_div<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, lds_vdiv<R>(ctx), ctx->m, s);
+    const int bv = block_of(ctx, "k_vertvel_div");
+    LAUNCH_EX(ctx, k_vertvel_div<R>, nflatblk(ctx->m.N, ctx->m.L, bv), bv, lds_vdiv<R>(ctx, bv), ctx->m, s);
     LAUNCH_EX(ctx, k_vertvel_scan<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, flat_bytes(ctx->m.L, sizeof(R)), ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
@@ -1242,8 +1253,9 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCHLV(ctx, k_tr_upwind<R>, m.N, m.L, m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 2 && p.muscl_call) {
         LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
-        LAUNCH_EX(ctx, k_tr_gradn_muscl<R>, nflatblk(m.N, m.L), kFlatT, lds_gradn(ctx), m, s);
-        LAUNCH_EX(ctx, k_tr_muscl<R>, nflatblk(m.N, m.L), kFlatT, lds_muscl<R>(ctx), m, s, p.dt, p.Dmin, relax,
+        const int bg = block_of(ctx, "k_tr_gradn_muscl"), bm = block_of(ctx, "k_tr_muscl");
+        LAUNCH_EX(ctx, k_tr_gradn_muscl<R>, nflatblk(m.N, m.L, bg), bg, lds_gradn(ctx, bg), m, s);
+        LAUNCH_EX(ctx, k_tr_muscl<R>, nflatblk(m.N, m.L, bm), bm, lds_muscl<R>(ctx, bm), m, s, p.dt, p.Dmin, relax,
                   p.clim_relax, p.relax2clim_ac);
     } else if (p.tracer_adv == 4) {
         LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
@@ -1382,9 +1394,28 @@ extern "C" int fcg_mlp_collect(fcg_ctx *ctx, float *X, float *T, long cap, int s
 
 // Launch shapes for the autotuner: block size 64, 128 or 256 threads (for
 // the layer-parallel kernels: 64 entities x 1, 2 or 4 layers per block); for
-// k_velrhs 256, 512 or 1024 threads of whole columns, when its LDS fits.
+// k_velrhs and the node-gather kernels 256, 512 or 1024 threads of whole
+// columns, when their LDS fits.
 extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
     if (!ctx || !kernel) return FCG_E_ARG;
+    static const char *gather[] = {"k_tr_muscl", "k_tr_gradn_muscl", "k_vertvel_div", "k_el2nodes"};
+    for (const char *g : gather)
+        if (!std::strcmp(g, kernel)) {
+            if (!(block == 256 || block == 512 || block == 1024)) return FCG_E_ARG;
+            const size_t lds = std::max(std::max(ctx->mixed ? lds_muscl<float>(ctx, block) : lds_muscl<double>(ctx, block),
+                                                 ctx->mixed ? lds_vdiv<float>(ctx, block) : lds_vdiv<double>(ctx, block)),
+                                        std::max(lds_gradn(ctx, block), lds_el2n(ctx, block)));
+            if (lds > (size_t)ctx->lds_limit)
+                return fcg_fail(ctx, FCG_E_ARG, std::string(kernel) + ": LDS of this block size exceeds the device limit");
+            ctx->launch_block[kernel] = block;
+            if (!std::strcmp(kernel, "k_el2nodes") && ctx->bt_graph) {  // captured in the barotropic graph
+                FCG_CHECK(hipStreamSynchronize(ctx->stream));
+                FCG_CHECK(hipGraphExecDestroy(ctx->bt_graph));
+                ctx->bt_graph = nullptr;
+                ctx->bt_graph_ok = false;
+            }
+            return FCG_OK;
+        }
     if (!std::strcmp(kernel, "k_velrhs")) {
         if (!(block == 256 || block == 512 || block == 1024)) return FCG_E_ARG;
         if (velrhs_bytes(ctx->m, ctx->esize, block) > (size_t)ctx->lds_limit)
@@ -1393,8 +1424,7 @@ extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
         return FCG_OK;
     }
     if (!(block == 64 || block == 128 || block == 256)) return FCG_E_ARG;
-    static const char *tunable[] = {"k_d3end", "k_jac_node", "k_tr_ab", "k_tr_upwind", "k_tr_gradn_miura",
-                                    "k_tr_miura"};
+    static const char *tunable[] = {"k_jac_node", "k_tr_ab", "k_tr_upwind", "k_tr_gradn_miura", "k_tr_miura"};
     for (const char *t : tunable)
         if (!std::strcmp(t, kernel)) {
             ctx->launch_block[kernel] = block;

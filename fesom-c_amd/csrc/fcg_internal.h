@@ -224,7 +224,9 @@ struct fcg_ctx {
     bool bt_graph_ok = false;
     int red_blocks = 0;
     // node-gather blocks (cpb = 256/L nodes): most CSR slots of any block
-    int cap_n2e = 0, cap_n2el = 0;
+    // most CSR slots (node -> edge, node -> element) in one node-gather block
+    // of 256, 512 and 1024 threads (256/L, 512/L, 1024/L node columns)
+    int cap_n2e_t[3] = {0, 0, 0}, cap_n2el_t[3] = {0, 0, 0};
     int lds_limit = 0;           // bytes of LDS one block may request on this device
     int n_cu = 1;                // compute units of the device (persistent launches)
     // learned closure

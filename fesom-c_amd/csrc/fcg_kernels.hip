@@ -1429,7 +1429,7 @@ __global__ __launch_bounds__(256) void k_bt_puls(DevMesh m, DevStateT<R> s, doub
 // U_puls = U_n - a*U_n_2D (compute_puls_vel, fv_3D_vel.F90:286-295), formed
 // here in the storage type from U_n, a (k_bt_puls) and U_n_2D.
 template <typename R, bool P>
-__global__ __launch_bounds__(256) void k_el2nodes(DevMesh m, const R *__restrict__ Ue, const R *__restrict__ Ve,
+__global__ __launch_bounds__(1024) void k_el2nodes(DevMesh m, const R *__restrict__ Ue, const R *__restrict__ Ve,
                            R *__restrict__ Un, R *__restrict__ Vn, const double *__restrict__ ap,
                            const double *__restrict__ U2) {
     extern __shared__ __align__(16) unsigned char lds_[];
@@ -1864,7 +1864,7 @@ __global__ __launch_bounds__(256) void k_update3d(DevMesh m, DevStateT<R> s, dou
 // the divergence of the baroclinic transport (U_n_filt - U_filt_2D)*Je,
 // gathered over the node's edges in ascending id (354-437), stored in W.
 template <typename R>
-__global__ __launch_bounds__(256) void k_vertvel_div(DevMesh m, DevStateT<R> s) {
+__global__ __launch_bounds__(1024) void k_vertvel_div(DevMesh m, DevStateT<R> s) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
     const ColFlat f = col_flat(m.N, m.L);
@@ -2515,7 +2515,7 @@ __global__ __launch_bounds__(256) void k_tr_upwind(DevMesh m, DevStateT<R> s, do
 // fv_gradients.F90:263-323): area-weighted mean over the node's elements in
 // ascending element id, divided by the summed area.
 template <typename R>
-__global__ __launch_bounds__(256) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> s) {
+__global__ __launch_bounds__(1024) void k_tr_gradn_muscl(DevMesh m, DevStateT<R> s) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;
     const ColFlat f = col_flat(m.N, m.L);
@@ -2617,7 +2617,7 @@ __device__ __forceinline__ R muscl_mean(R T1, R T2, R dT, R dx, R dy, R g1x, R g
 // gradients are loaded once; per edge only the other end's.  The implicit
 // vertical diffusion follows in k_tr_vdiff.
 template <typename R>
-__global__ __launch_bounds__(256) void k_tr_muscl(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax, double clim_relax,
+__global__ __launch_bounds__(1024) void k_tr_muscl(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax, double clim_relax,
                            double relax2clim_ac) {
     extern __shared__ __align__(16) unsigned char lds_[];
     const int KC = m.K;

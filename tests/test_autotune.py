@@ -48,19 +48,25 @@ def test_block_size_does_not_change_results(precision, block):
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("block", [512, 1024])
 def test_wide_blocks_do_not_change_results(precision, block):
-    """k_velrhs with 512- or 1024-thread blocks (more columns per block, so
-    more neighbours read from LDS) gives the 256-thread results bit for bit."""
-    from fesomc_amd import gpu
+    """k_velrhs and the node-gather kernels (MUSCL tracer advection and its
+    node gradients, the vertical-velocity divergence, the node averaging,
+    also inside the captured barotropic loop) with 512- or 1024-thread blocks
+    (more columns per block, so more neighbours read from LDS) give the
+    256-thread results bit for bit."""
+    from fesomc_amd import autotune, gpu
     a, b = _model(precision), _model(precision)
-    try:
-        b.set_launch("k_velrhs", block)
-    except gpu.GpuError as err:          # LDS of the block above the device limit
-        assert "LDS" in str(err)
-        pytest.skip(str(err))
-    assert b.get_launch("k_velrhs") == block
+    a.step(1)
+    b.step(1)   # the barotropic graph is captured with the default shapes, then re-captured
+    for k in autotune.WIDE_KERNELS:
+        try:
+            b.set_launch(k, block)
+        except gpu.GpuError as err:          # LDS of the block above the device limit
+            assert "LDS" in str(err)
+            continue
+        assert b.get_launch(k) == block
     a.step(4)
     b.step(4)
-    for f in ("eta_n", "U_n", "V_n", "U_rhsAB", "TF", "SF", "Wvel"):
+    for f in ("eta_n", "U_n", "V_n", "U_rhsAB", "TF", "SF", "Wvel", "Unode", "Kv"):
         assert np.array_equal(a.get(f), b.get(f)), f
     a.close()
     b.close()
@@ -74,7 +80,9 @@ def test_tuner_picks_a_candidate_for_each_kernel():
         m.set_launch("k_velrhs", 96)
     with pytest.raises(RuntimeError):
         m.set_launch("k_bt_elem2", 64)
-    kern = ["k_tr_ab", "k_jac_node", "k_d3end", "k_velrhs"]
+    with pytest.raises(RuntimeError):
+        m.set_launch("k_tr_muscl", 128)
+    kern = ["k_tr_ab", "k_jac_node", "k_velrhs", "k_tr_muscl", "k_el2nodes"]
     table = autotune.tune(m, kernels=kern, steps=1)
     assert set(table) == set(kern)
     for k, (bs, ms) in table.items():
