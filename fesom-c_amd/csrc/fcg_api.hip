@@ -24,6 +24,11 @@ template <typename R> __global__ void k_velrhs_atm(DevMesh, DevStateT<R>, double
 template <typename R> __global__ void k_velrhs_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
 template <typename R>
 __global__ void k_velrhs_atm_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
+template <typename R> __global__ void k_velrhs_v(DevMesh, DevStateT<R>, double, double, int, int, double);
+template <typename R> __global__ void k_velrhs_v_atm(DevMesh, DevStateT<R>, double, double, int, int, double);
+template <typename R> __global__ void k_velrhs_v_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
+template <typename R>
+__global__ void k_velrhs_v_atm_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
 template <typename R> __global__ void k_velvisc(DevMesh, DevStateT<R>, double, double, double, int);
 template <typename R> __global__ void k_velvisc_im(DevMesh, DevStateT<R>, double, double, double);
 template <typename R> __global__ void k_bt_elem1(DevMesh, DevStateT<R>, double);
@@ -108,6 +113,18 @@ inline int gather_slot(int T) { return T == 1024 ? 2 : (T == 512 ? 1 : 0); }
 inline size_t velrhs_bytes(const DevMesh &m, size_t esz, int T = kFlatT) {
     return (size_t)cpb_of(m.L, T) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) +
            (size_t)cpb_of(m.L, T) * m.L * (3 * sizeof(double) + 6 * esz);
+}
+// the level-vector k_velrhs: W = 16/esz layers per lane, K/W lanes per
+// column; column arrays with a stride of K slots
+inline int velrhs_v_cpb(const DevMesh &m, size_t esz, int T) { return T / (m.K / (int)(16 / esz)); }
+inline size_t velrhs_v_bytes(const DevMesh &m, size_t esz, int T) {
+    const size_t cpb = (size_t)velrhs_v_cpb(m, esz, T);
+    return cpb * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) + cpb * m.K * (sizeof(double) + 6 * esz);
+}
+// its block: 256 threads, or 128 where three 256-thread blocks' LDS would not
+// share one compute unit (160 KB; the kernel's registers allow three)
+inline int velrhs_v_threads(const DevMesh &m, size_t esz) {
+    return velrhs_v_bytes(m, esz, 256) <= (160u << 10) / 3 ? 256 : 128;
 }
 inline size_t lds_p1edge(const DevMesh &m) {
     return 4 * flat_bytes(m.L, 8) + (size_t)cpb_of(m.L) * sizeof(P1Edge);
@@ -272,6 +289,16 @@ static int block_of(const fcg_ctx *ctx, const char *kern) {
     const std::string k(kern, std::strcspn(kern, "<"));
     auto it = ctx->launch_block.find(k);
     return it == ctx->launch_block.end() ? 256 : it->second;
+}
+
+// Levels per lane of a kernel with a level-vector variant (key "<kernel>:levels"
+// of the launch table): the tuned value when one was set, else W =
+// 16/sizeof(R) -- one 16-byte access per W levels -- when nsigma is a
+// multiple of W, else 1 (the one-level kernel).
+static int levels_of(const fcg_ctx *ctx, const char *kern, int W) {
+    if (ctx->m.K % W) return 1;
+    auto it = ctx->launch_block.find(std::string(kern) + ":levels");
+    return it == ctx->launch_block.end() ? W : it->second;
 }
 
 // Tidal elevation at the open boundary for each substep of the coming
@@ -1111,7 +1138,26 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     }
     LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec), ctx->m, s);
     const int vt = block_of(ctx, "k_velrhs");
-    if (p.filt_3D && p.key_atm)
+    constexpr int W = 16 / (int)sizeof(R);
+    if (levels_of(ctx, "k_velrhs", W) == W &&
+        velrhs_v_bytes(ctx->m, sizeof(R), velrhs_v_threads(ctx->m, sizeof(R))) <= (size_t)ctx->lds_limit) {  // W layers per lane
+        const int tv = velrhs_v_threads(ctx->m, sizeof(R)), cpb = velrhs_v_cpb(ctx->m, sizeof(R), tv);
+        const unsigned g = (unsigned)((ctx->m.E + cpb - 1) / cpb);
+        const size_t lds = velrhs_v_bytes(ctx->m, sizeof(R), tv);
+        const int hadv = p.mom_adv_3D == 1 ? 1 : 0, vadv = p.vert_adv == 1 ? 1 : 0;
+        if (p.filt_3D && p.key_atm)
+            LAUNCH_NAMED(ctx, "k_velrhs", fcgk::k_velrhs_v_atm_filt<R>, g, tv, lds, ctx->m, s, p.dt, ff, hadv, vadv,
+                         1.0 / ctx->density_0, p.tau_inv_filt);
+        else if (p.filt_3D)
+            LAUNCH_NAMED(ctx, "k_velrhs", fcgk::k_velrhs_v_filt<R>, g, tv, lds, ctx->m, s, p.dt, ff, hadv, vadv,
+                         1.0 / ctx->density_0, p.tau_inv_filt);
+        else if (p.key_atm)
+            LAUNCH_NAMED(ctx, "k_velrhs", fcgk::k_velrhs_v_atm<R>, g, tv, lds, ctx->m, s, p.dt, ff, hadv, vadv,
+                         1.0 / ctx->density_0);
+        else
+            LAUNCH_NAMED(ctx, "k_velrhs", fcgk::k_velrhs_v<R>, g, tv, lds, ctx->m, s, p.dt, ff, hadv, vadv,
+                         1.0 / ctx->density_0);
+    } else if (p.filt_3D && p.key_atm)
         LAUNCH_EX(ctx, k_velrhs_atm_filt<R>, nflatblk(ctx->m.E, ctx->m.L, vt), vt, velrhs_bytes(ctx->m, sizeof(R), vt),
                   ctx->m, s, p.dt, ff, p.mom_adv_3D == 1 ? 1 : 0, p.vert_adv == 1 ? 1 : 0, 1.0 / ctx->density_0,
                   p.tau_inv_filt);
@@ -1416,6 +1462,24 @@ extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
             }
             return FCG_OK;
         }
+    {  // levels per lane: 1, or W = 16/(storage width) when nsigma is a multiple of W
+        const char *colon = std::strchr(kernel, ':');
+        if (colon && !std::strcmp(colon, ":levels")) {
+            const std::string k(kernel, colon - kernel);
+            const int W = 16 / (int)ctx->esize;
+            static const char *vec[] = {"k_velrhs"};
+            bool known = false;
+            for (const char *v : vec) known = known || k == v;
+            if (!known) return fcg_fail(ctx, FCG_E_ARG, "kernel without a level-vector variant: " + k);
+            if (!(block == 1 || block == W)) return FCG_E_ARG;
+            if (block == W && ctx->m.K % W)
+                return fcg_fail(ctx, FCG_E_ARG, k + ": " + std::to_string(W) + " levels per lane need nsigma to be a multiple of it");
+            if (block == W && velrhs_v_bytes(ctx->m, ctx->esize, velrhs_v_threads(ctx->m, ctx->esize)) > (size_t)ctx->lds_limit)
+                return fcg_fail(ctx, FCG_E_ARG, k + ": LDS of the level-vector variant exceeds the device limit");
+            ctx->launch_block[kernel] = block;
+            return FCG_OK;
+        }
+    }
     if (!std::strcmp(kernel, "k_velrhs")) {
         if (!(block == 256 || block == 512 || block == 1024)) return FCG_E_ARG;
         if (velrhs_bytes(ctx->m, ctx->esize, block) > (size_t)ctx->lds_limit)
@@ -1434,5 +1498,9 @@ extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
 }
 
 extern "C" int fcg_get_launch(const fcg_ctx *ctx, const char *kernel) {
-    return (ctx && kernel) ? block_of(ctx, kernel) : 0;
+    if (!ctx || !kernel) return 0;
+    const char *colon = std::strchr(kernel, ':');
+    if (colon && !std::strcmp(colon, ":levels"))
+        return levels_of(ctx, std::string(kernel, colon - kernel).c_str(), 16 / (int)ctx->esize);
+    return block_of(ctx, kernel);
 }

@@ -31,6 +31,57 @@ __device__ __forceinline__ T glb_ld(const T *p) {
     return *(const __attribute__((address_space(1))) T *)p;
 }
 
+// Level vectors: W = 16/sizeof(R) consecutive levels of one column (4 in
+// FP32, 2 in FP64), moved as one 16-byte access.  The vector kernels that use
+// them need nsigma to be a multiple of W, so that every column starts on a
+// 16-byte boundary; a lane then owns levels [W*j, W*j+W) of its column.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <typename R>
+struct LV {
+    static constexpr int W = 16 / (int)sizeof(R);
+    R x[W];
+};
+template <typename R>
+__device__ __forceinline__ LV<R> ldv(const R *p) {
+    const u32x4 u = *(const u32x4 *)p;
+    LV<R> r;
+    __builtin_memcpy(r.x, &u, 16);
+    return r;
+}
+template <typename R>
+__device__ __forceinline__ LV<R> ldv_lds(const R *p) {
+    const u32x4 u = *(const __attribute__((address_space(3))) u32x4 *)p;
+    LV<R> r;
+    __builtin_memcpy(r.x, &u, 16);
+    return r;
+}
+template <typename R>
+__device__ __forceinline__ LV<R> ldv_glb(const R *p) {
+    const u32x4 u = *(const __attribute__((address_space(1))) u32x4 *)p;
+    LV<R> r;
+    __builtin_memcpy(r.x, &u, 16);
+    return r;
+}
+template <typename R>
+__device__ __forceinline__ void stv(R *p, const LV<R> &v) {
+    u32x4 u;
+    __builtin_memcpy(&u, v.x, 16);
+    *(u32x4 *)p = u;
+}
+// the first n (<= W) levels of v: a 16-byte store when n == W; the lane that
+// holds a column's unused last slot stores its layers one by one, so that
+// slot keeps what it held
+template <typename R>
+__device__ __forceinline__ void stv_n(R *p, const LV<R> &v, int n) {
+    if (n >= LV<R>::W) {
+        stv(p, v);
+    } else {
+#pragma unroll
+        for (int i = 0; i < LV<R>::W; ++i)
+            if (i < n) p[i] = v.x[i];
+    }
+}
+
 // (2,E)/(4,E) interleaved records of the barotropic momentum path in the
 // storage type, widened to double on load and rounded on store
 __device__ __forceinline__ double2 ld2(const double *p, size_t i) { return ((const double2 *)p)[i]; }
@@ -791,6 +842,50 @@ __device__ __forceinline__ void uvert_lds(const double *ws, const R *u, const R 
     fv = -vm * wv;
 }
 
+// uvert_lds with the element-averaged layer and level depths formed from the
+// column record (ElemCol::Zw / zbw: the same operands, the same bits)
+template <typename R>
+__device__ __forceinline__ void uvert_col(const DevMesh &m, const ElemCol &col, const double *ws, const R *u,
+                                          const R *v, int K, double area, int nz, double &fu, double &fv) {
+    const double wv = ws[nz - 1] * area;
+    double um = 0.0, vm = 0.0;
+    if (wv >= 0.0) {
+        if (nz == K - 1) {
+            um = 0.5 * (u[nz - 2] + u[nz - 1]);
+            vm = 0.5 * (v[nz - 2] + v[nz - 1]);
+        } else {
+            double Z1 = col.zbw(m, nz - 1);
+            double aa = Z1 - col.Zw(m, nz - 2);
+            double bb = col.Zw(m, nz - 1) - Z1;
+            double cc = col.Zw(m, nz) - Z1;
+            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+            double dg = aa * bb * (aa + bb) / d;
+            double db = -aa * cc * (aa + cc) / d;
+            double da = 1.0 - dg - db;
+            um = u[nz - 2] * da + u[nz - 1] * db + u[nz] * dg;
+            vm = v[nz - 2] * da + v[nz - 1] * db + v[nz] * dg;
+        }
+    } else {
+        if (nz == 2) {
+            um = 0.5 * (u[nz - 2] + u[nz - 1]);
+            vm = 0.5 * (v[nz - 2] + v[nz - 1]);
+        } else {
+            double Z1 = col.zbw(m, nz - 1);
+            double aa = col.Zw(m, nz - 1) - Z1;
+            double bb = Z1 - col.Zw(m, nz - 2);
+            double cc = Z1 - col.Zw(m, nz - 3);
+            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+            double dg = aa * bb * (aa + bb) / d;
+            double db = -aa * cc * (aa + cc) / d;
+            double da = 1.0 - dg - db;
+            um = u[nz - 1] * da + u[nz - 2] * db + u[nz - 3] * dg;
+            vm = v[nz - 1] * da + v[nz - 2] * db + v[nz - 3] * dg;
+        }
+    }
+    fu = -um * wv;
+    fv = -vm * wv;
+}
+
 // Per-edge constants of momentum_adv_upwind (fv_advection.F90:120-165), as
 // the flux uses them (storage type R for the geometry, FP64 node depths).
 template <typename R>
@@ -1028,6 +1123,274 @@ template <typename R>
 __global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
                                                   double d0i) {
     velrhs_body<R, false, false>(m, s, dt, ff, hadv, vadv, d0i, 0.0);
+}
+
+// compute_vel_rhs as velrhs_body, with each lane owning W consecutive layers
+// of its element column (LV<R>; nsigma a multiple of W): every column field
+// and every neighbour column is read with one 16-byte load per W layers, so
+// the kernel issues a quarter (FP32) or half (FP64) of the vector-memory
+// instructions.  The column arrays in LDS use a stride of K slots so each
+// lane's run is 16-byte aligned.  Per layer the arithmetic, and the order of
+// the edge sum, are those of velrhs_body: results are identical bit for bit.
+template <typename R, bool ATM, bool FILT>
+__device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
+                                                double d0i, double tau) {
+    constexpr int W = LV<R>::W;
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int L = m.L, K = m.K, KC = K;
+    const ColFlat f = col_flat(m.E, K / W);  // f.k: the lane's run of W slots in its column
+    ElemRec *er = (ElemRec *)lds_;                                        // [cpb]
+    EdgeRec<R> *edr = (EdgeRec<R> *)(lds_ + f.cpb * kVelrhsElemBytes);   // [cpb][4]
+    const int T = f.cpb * K;
+    double *cws = (double *)(edr + 4 * f.cpb);
+    R *cu = (R *)(cws + T), *cv = cu + T, *cgux = cv + T, *cguy = cgux + T, *cgvx = cguy + T, *cgvy = cgvx + T;
+    if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {  // element and edge records: as velrhs_body
+        const int c = threadIdx.x, e = f.c0 + c;
+        ElemRec r;
+        r.col = elem_col(m, s, e);
+        const int4 q = r.col.q;
+        const double4 gx = m.gsx[e], gy = m.gsy[e];
+        r.area = m.elem_area[e];
+        r.cor = m.coriolis[e];
+        r.mask = s.mask_wd[e];
+        double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
+        double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
+        double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
+        double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
+        if (ATM) {
+            const double4 w = r.col.w;
+            const double am = w.x * s.mslp[q.x] + w.y * s.mslp[q.y] + w.z * s.mslp[q.z] + w.w * s.mslp[q.w];
+            const double em = w.x * s.emp[q.x] + w.y * s.emp[q.y] + w.z * s.emp[q.z] + w.w * s.emp[q.w];
+            pre0 = pre0 - (kPref - am) * d0i + kG * em * d0i;
+            pre1 = pre1 - (kPref - am) * d0i + kG * em * d0i;
+            pre2 = pre2 - (kPref - am) * d0i + kG * em * d0i;
+            pre3 = pre3 - (kPref - am) * d0i + kG * em * d0i;
+        }
+        r.Fx = sum4(gx, pre0, pre1, pre2, pre3) * r.area;
+        r.Fy = sum4(gy, pre0, pre1, pre2, pre3) * r.area;
+        r.acc = sum4(r.col.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+        er[c] = r;
+        const int4 ied = m.el_ied[e];
+        for (int j = 0; j < 4; ++j) {
+            const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
+            EdgeRec<R> g;
+            g.cc = cc;
+            if (cc != kEmpty && hadv) {
+                const int ed = cc >= 0 ? cc : ~cc;
+                const int2 nd = m.ednodes[ed];
+                const int2 et = m.edtri[ed];
+                const double4 cd = m.ecross[ed];
+                const double2 dd = m.edxdy[ed];
+                const int oe = cc >= 0 ? et.x : et.y;
+                g.oK = oe * K;
+                g.oc = (oe >= f.c0 && oe < f.c0 + f.nc) ? oe - f.c0 : -1;
+                g.acc = (R)(0.5 * (m.ac[nd.x] + m.ac[nd.y]));
+                g.cx = (R)cd.x;
+                g.cy = (R)cd.y;
+                g.cz = (R)cd.z;
+                g.cw = (R)cd.w;
+                g.dx = (R)dd.x;
+                g.dy = (R)dd.y;
+                g.cos1 = (R)m.elem_cos[et.x];
+                g.cos2 = (R)m.elem_cos[et.y];
+                g.an1 = s.a[nd.x];
+                g.an2 = s.a[nd.y];
+            }
+            edr[c * 4 + j] = g;
+        }
+    }
+    __syncthreads();
+    const int c = f.c, e = f.c0 + c, k0 = f.k * W, ck0 = c * K + k0;
+    const size_t ek0 = (size_t)e * KC + k0;
+    if (f.on()) {  // the column arrays of the vertical advection and the neighbours
+        const ElemCol &col = er[c].col;
+        const int4 q = col.q;
+        const LV<R> w0 = ldv_glb(s.W + (size_t)q.x * KC + k0), w1 = ldv_glb(s.W + (size_t)q.y * KC + k0);
+        const LV<R> w2 = ldv_glb(s.W + (size_t)q.z * KC + k0), w3 = ldv_glb(s.W + (size_t)q.w * KC + k0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) cws[ck0 + i] = sum4(col.w, w0.x[i], w1.x[i], w2.x[i], w3.x[i]);
+        stv(cu + ck0, ldv_glb(s.U_n + ek0));
+        stv(cv + ck0, ldv_glb(s.V_n + ek0));
+        if (hadv) {
+            stv(cgux + ck0, ldv_glb(s.gux + ek0));
+            stv(cguy + ck0, ldv_glb(s.guy + ek0));
+            stv(cgvx + ck0, ldv_glb(s.gvx + ek0));
+            stv(cgvy + ck0, ldv_glb(s.gvy + ek0));
+        }
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int nl = min(W, L - k0);  // layers of this lane (the column's last lane: W-1)
+    const ElemRec &r = er[c];
+    const ElemCol &col = r.col;
+    const double area = r.area;
+    const double area_inv = 1.0 / area;
+    const LV<R> ua = ldv_glb(s.U_rhsAB + ek0), va = ldv_glb(s.V_rhsAB + ek0);
+    double je[W], ur[W], vr[W];
+    R abu[W], abv[W];
+    LV<R> uo = ldv_lds(cu + ck0), vo = ldv_lds(cv + ck0);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        je[i] = col.Je(m, k0 + i < L ? k0 + i : L - 1);
+        ur[i] = -(0.5 + kEps) * ua.x[i] + r.Fx * je[i];
+        vr[i] = -(0.5 + kEps) * va.x[i] + r.Fy * je[i];
+    }
+    if (FILT) {  // viscosity_filt_3D: neighbours minus own U_n, elem_neighbors order
+        const int4 q4 = m.en[e];
+        const int4 nb4 = m.elnb[e];
+        const int nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        const int jend = (q4.x == q4.w) ? 3 : 4;
+        double fu[W], fv[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i) fu[i] = fv[i] = 0.0;
+        for (int j = 0; j < jend; ++j) {
+            if (nbs[j] >= 0) {
+                const LV<R> un = ldv_glb(s.U_n + (size_t)nbs[j] * KC + k0), vn = ldv_glb(s.V_n + (size_t)nbs[j] * KC + k0);
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    fu[i] = fu[i] + (double)un.x[i] - uo.x[i];
+                    fv[i] = fv[i] + (double)vn.x[i] - vo.x[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            ur[i] = ur[i] + tau * fu[i] * area * je[i];
+            vr[i] = vr[i] + tau * fv[i] * area * je[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        abu[i] = (R)(vo.x[i] * r.cor * area * je[i]);
+        abv[i] = (R)(-uo.x[i] * r.cor * area * je[i]);
+    }
+    if (hadv) {
+        const LV<R> gxo = ldv_lds(cgux + ck0), gyo = ldv_lds(cguy + ck0);
+        const LV<R> hxo = ldv_lds(cgvx + ck0), hyo = ldv_lds(cgvy + ck0);
+        for (int j = 0; j < 4; ++j) {
+            const EdgeRec<R> &g = edr[c * 4 + j];
+            if (g.cc == kEmpty) break;
+            const bool own1 = g.cc < 0;  // the element is edge_tri(1)
+            LV<R> ux, vx, gxx, gyx, hxx, hyx;
+            if (g.oc >= 0) {  // the other element is a column of this block
+                const int i0 = g.oc * K + k0;
+                ux = ldv_lds(cu + i0);
+                vx = ldv_lds(cv + i0);
+                gxx = ldv_lds(cgux + i0);
+                gyx = ldv_lds(cguy + i0);
+                hxx = ldv_lds(cgvx + i0);
+                hyx = ldv_lds(cgvy + i0);
+            } else {
+                const size_t io = (size_t)g.oK + k0;
+                ux = ldv_glb(s.U_n + io);
+                vx = ldv_glb(s.V_n + io);
+                gxx = ldv_glb(s.gux + io);
+                gyx = ldv_glb(s.guy + io);
+                hxx = ldv_glb(s.gvx + io);
+                hyx = ldv_glb(s.gvy + io);
+            }
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const double ds = m.dsig[k0 + i < L ? k0 + i : L - 1];
+                const R U1 = own1 ? uo.x[i] : ux.x[i], V1 = own1 ? vo.x[i] : vx.x[i];
+                const R U2 = own1 ? ux.x[i] : uo.x[i], V2 = own1 ? vx.x[i] : vo.x[i];
+                const R gux1 = own1 ? gxo.x[i] : gxx.x[i], guy1 = own1 ? gyo.x[i] : gyx.x[i];
+                const R gvx1 = own1 ? hxo.x[i] : hxx.x[i], gvy1 = own1 ? hyo.x[i] : hyx.x[i];
+                const R gux2 = own1 ? gxx.x[i] : gxo.x[i], guy2 = own1 ? gyx.x[i] : gyo.x[i];
+                const R gvx2 = own1 ? hxx.x[i] : hxo.x[i], gvy2 = own1 ? hyx.x[i] : hyo.x[i];
+                const R u1 = U1 - gux1 * g.cx - guy1 * g.cy;
+                const R v1 = V1 - gvx1 * g.cx - gvy1 * g.cy;
+                const R u2 = U2 - gux2 * g.cz - guy2 * g.cw;
+                const R v2 = V2 - gvx2 * g.cz - gvy2 * g.cw;
+                const R jd = (R)(0.5 * (g.an1 * ds + g.an2 * ds));
+                const R un = R(0.5) * R(kREarthD) * jd * ((u1 + u2) * g.dy - (v1 * g.cos1 + v2 * g.cos2) * g.dx);
+                R uu, vv;
+                if (un >= R(0)) {
+                    uu = u1 * un * g.acc;
+                    vv = v1 * un * g.acc;
+                } else {
+                    uu = u2 * un * g.acc;
+                    vv = v2 * un * g.acc;
+                }
+                if (g.cc >= 0) {
+                    abu[i] = abu[i] + uu;
+                    abv[i] = abv[i] + vv;
+                } else {
+                    abu[i] = abu[i] - uu;
+                    abv[i] = abv[i] - vv;
+                }
+            }
+        }
+    }
+    if (vadv) {
+        if (r.mask != 0.0) {
+            // flux through 1-based interface nz (0 at the bottom, nz = K);
+            // the bottom flux of one layer is the top flux of the next
+            const double *ws = cws + c * K;
+            const R *uc = cu + c * K, *vc = cv + c * K;
+            auto flux = [&](int nz, double &fu, double &fv) {
+                fu = 0.0;
+                fv = 0.0;
+                if (nz == 1) {
+                    const double w1 = ws[0] * area;
+                    fu = -w1 * uc[0];
+                    fv = -w1 * vc[0];
+                } else if (nz <= K - 1) {
+                    uvert_col(m, col, ws, uc, vc, K, area, nz, fu, fv);
+                }
+            };
+            double ftu, ftv;
+            flux(k0 + 1, ftu, ftv);
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                double fbu = 0.0, fbv = 0.0;
+                if (i < nl) flux(k0 + i + 2, fbu, fbv);
+                abu[i] = abu[i] + (R)((ftu - fbu) * r.acc);
+                abv[i] = abv[i] + (R)((ftv - fbv) * r.acc);
+                ftu = fbu;
+                ftv = fbv;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; ++i) abu[i] = abv[i] = R(0);
+        }
+    }
+    const double mask = r.mask;
+    const LV<R> bu = ldv_glb(s.Bpu + ek0), bv = ldv_glb(s.Bpv + ek0);
+    LV<R> oa, ob, oc, od;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        oa.x[i] = abu[i];
+        ob.x[i] = abv[i];
+        oc.x[i] = mask * (uo.x[i] + bu.x[i] * dt + (dt * (ur[i] + abu[i] * ff) * area_inv + 0.0) / je[i]);
+        od.x[i] = mask * (vo.x[i] + bv.x[i] * dt + (dt * (vr[i] + abv[i] * ff) * area_inv + 0.0) / je[i]);
+    }
+    stv_n(s.U_rhsAB + ek0, oa, nl);
+    stv_n(s.V_rhsAB + ek0, ob, nl);
+    stv_n(s.U_rhs + ek0, oc, nl);
+    stv_n(s.V_rhs + ek0, od, nl);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_velrhs_v(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
+                                                   double d0i) {
+    velrhs_vec_body<R, false, false>(m, s, dt, ff, hadv, vadv, d0i, 0.0);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_velrhs_v_atm(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
+                                                       int vadv, double d0i) {
+    velrhs_vec_body<R, true, false>(m, s, dt, ff, hadv, vadv, d0i, 0.0);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_velrhs_v_filt(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
+                                                        int vadv, double d0i, double tau) {
+    velrhs_vec_body<R, false, true>(m, s, dt, ff, hadv, vadv, d0i, tau);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_velrhs_v_atm_filt(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
+                                                            int vadv, double d0i, double tau) {
+    velrhs_vec_body<R, true, true>(m, s, dt, ff, hadv, vadv, d0i, tau);
 }
 // the same with surface forcing (key_atm)
 template <typename R>
@@ -3268,6 +3631,10 @@ void instantiate_kernels() {
     (void)&k_tr_ab<R>;
     (void)&k_velrhs_filt<R>;
     (void)&k_velrhs_atm_filt<R>;
+    (void)&k_velrhs_v<R>;
+    (void)&k_velrhs_v_atm<R>;
+    (void)&k_velrhs_v_filt<R>;
+    (void)&k_velrhs_v_atm_filt<R>;
     (void)&k_tr_rivers<R>;
     (void)&k_tr_upwind<R>;
     (void)&k_tr_gradn_muscl<R>;

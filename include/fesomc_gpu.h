@@ -311,7 +311,10 @@ int fcg_mlp_collect(fcg_ctx *ctx, float *X_dev, float *T_dev, long cap, int stri
  * device kernel ("k_d3end", "k_tr_upwind", ...; layer-parallel kernels then
  * cover 64 entities x block/64 layers per block); 256, 512 or 1024 threads of
  * whole columns for "k_velrhs" (FCG_E_ARG when its LDS would not fit).
- * Default 256.  Results do not depend on the shape. */
+ * Default 256.  "<kernel>:levels" sets the levels each lane owns in kernels
+ * with a level-vector variant: 1, or W = 4 (FP32 storage) / 2 (FP64) for one
+ * 16-byte access per W levels, which needs nsigma to be a multiple of W (the
+ * default then).  Results do not depend on the shape. */
 int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block);
 int fcg_get_launch(const fcg_ctx *ctx, const char *kernel);
 

@@ -12,10 +12,15 @@ def test_set_launch_checks_its_arguments_without_a_gpu():
     assert L.fcg_get_launch(None, b"k_velrhs") == 0
 
 
-def _model(precision):
+def _model(precision, nsigma=None):
+    """The c2 physics on the small channel (8 levels, or nsigma)."""
     from fesomc_amd import gpu, meshgen
     from fesomc_amd.runconfig import RunConfig
     txt, mesh, ex = meshgen.config("c2_small")
+    if nsigma is not None:
+        txt = txt.replace("nsigma = 8\n", f"nsigma = {nsigma}\n")
+        assert f"nsigma = {nsigma}\n" in txt
+        ex["T"], ex["S"] = meshgen.stratified_ts(mesh, nsigma)
     m = gpu.Model(RunConfig(txt), mesh, precision=precision)
     m.set_tracers(ex["T"], ex["S"])
     wx, wy = ex["wind"]
@@ -70,6 +75,38 @@ def test_wide_blocks_do_not_change_results(precision, block):
         assert np.array_equal(a.get(f), b.get(f)), f
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nsigma", [8, 16, 24, 32])
+def test_level_vectors_do_not_change_results(precision, nsigma):
+    """The level-vector kernels (each lane owns 4 (FP32) or 2 (FP64)
+    consecutive levels and moves them with 16-byte accesses) give the
+    one-level kernels' results bit for bit, at the ensemble's level counts;
+    the column's unused last slot is left as it was."""
+    a, b = _model(precision, nsigma), _model(precision, nsigma)
+    W = 4 if precision else 2
+    vec = [k for k in ("k_velrhs",)]
+    for k in vec:
+        assert a.get_launch(k + ":levels") == W   # the default where nsigma is a multiple of W
+        b.set_launch(k + ":levels", 1)
+        assert b.get_launch(k + ":levels") == 1
+    a.step(4)
+    b.step(4)
+    for f in ("eta_n", "U_n", "V_n", "U_rhsAB", "V_rhsAB", "TF", "SF", "Wvel", "Kv", "snu"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
+def test_level_vector_option_checks_its_arguments():
+    """Levels per lane other than 1 or the 16-byte vector width are refused
+    before anything reaches a device."""
+    from fesomc_amd import gpu
+    L = gpu.lib()
+    assert L.fcg_set_launch(None, b"k_velrhs:levels", 4) != 0
+    assert L.fcg_get_launch(None, b"k_velrhs:levels") == 0
 
 
 @pytest.mark.gpu
