@@ -119,7 +119,7 @@ inline size_t velrhs_bytes(const DevMesh &m, size_t esz, int T = kFlatT) {
 inline int velrhs_v_cpb(const DevMesh &m, size_t esz, int T) { return T / (m.K / (int)(16 / esz)); }
 inline size_t velrhs_v_bytes(const DevMesh &m, size_t esz, int T) {
     const size_t cpb = (size_t)velrhs_v_cpb(m, esz, T);
-    return cpb * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) + cpb * m.K * (sizeof(double) + 6 * esz);
+    return cpb * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) + cpb * m.K * (3 * sizeof(double) + 2 * esz);
 }
 // its block: 256 threads, or 128 where three 256-thread blocks' LDS would not
 // share one compute unit (160 KB; the kernel's registers allow three)

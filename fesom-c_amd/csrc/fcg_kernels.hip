@@ -842,45 +842,33 @@ __device__ __forceinline__ void uvert_lds(const double *ws, const R *u, const R 
     fv = -vm * wv;
 }
 
-// uvert_lds with the element-averaged layer and level depths formed from the
-// column record (ElemCol::Zw / zbw: the same operands, the same bits)
+// uvert_lds as one instruction stream for both signs of W, so lanes of a
+// wavefront whose interfaces carry opposite W do not diverge: the downward
+// branch is the upward one on the mirrored operands (layers nz-1, nz-2,
+// nz-3 instead of nz-2, nz-1, nz) with the depth differences negated, and
+// x - y = -(y - x) exactly, so every value equals uvert_lds's bit for bit.
 template <typename R>
-__device__ __forceinline__ void uvert_col(const DevMesh &m, const ElemCol &col, const double *ws, const R *u,
-                                          const R *v, int K, double area, int nz, double &fu, double &fv) {
+__device__ __forceinline__ void uvert_uni(const double *ws, const R *u, const R *v, const double *Zw, const double *zbw,
+                                          int K, double area, int nz, double &fu, double &fv) {
     const double wv = ws[nz - 1] * area;
-    double um = 0.0, vm = 0.0;
-    if (wv >= 0.0) {
-        if (nz == K - 1) {
-            um = 0.5 * (u[nz - 2] + u[nz - 1]);
-            vm = 0.5 * (v[nz - 2] + v[nz - 1]);
-        } else {
-            double Z1 = col.zbw(m, nz - 1);
-            double aa = Z1 - col.Zw(m, nz - 2);
-            double bb = col.Zw(m, nz - 1) - Z1;
-            double cc = col.Zw(m, nz) - Z1;
-            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
-            double dg = aa * bb * (aa + bb) / d;
-            double db = -aa * cc * (aa + cc) / d;
-            double da = 1.0 - dg - db;
-            um = u[nz - 2] * da + u[nz - 1] * db + u[nz] * dg;
-            vm = v[nz - 2] * da + v[nz - 1] * db + v[nz] * dg;
-        }
+    const bool up = wv >= 0.0;
+    double um, vm;
+    if (up ? nz == K - 1 : nz == 2) {
+        um = 0.5 * (u[nz - 2] + u[nz - 1]);
+        vm = 0.5 * (v[nz - 2] + v[nz - 1]);
     } else {
-        if (nz == 2) {
-            um = 0.5 * (u[nz - 2] + u[nz - 1]);
-            vm = 0.5 * (v[nz - 2] + v[nz - 1]);
-        } else {
-            double Z1 = col.zbw(m, nz - 1);
-            double aa = col.Zw(m, nz - 1) - Z1;
-            double bb = Z1 - col.Zw(m, nz - 2);
-            double cc = Z1 - col.Zw(m, nz - 3);
-            double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
-            double dg = aa * bb * (aa + bb) / d;
-            double db = -aa * cc * (aa + cc) / d;
-            double da = 1.0 - dg - db;
-            um = u[nz - 1] * da + u[nz - 2] * db + u[nz - 3] * dg;
-            vm = v[nz - 1] * da + v[nz - 2] * db + v[nz - 3] * dg;
-        }
+        const int iP = up ? nz - 2 : nz - 1, iQ = up ? nz - 1 : nz - 2, iR = up ? nz : nz - 3;
+        const double sg = up ? 1.0 : -1.0;
+        const double Z1 = zbw[nz - 1];
+        const double aa = sg * (Z1 - Zw[iP]);
+        const double bb = sg * (Zw[iQ] - Z1);
+        const double cc = sg * (Zw[iR] - Z1);
+        const double d = (cc + aa) * (bb * bb - aa * aa) - (cc * cc - aa * aa) * (bb + aa);
+        const double dg = aa * bb * (aa + bb) / d;
+        const double db = -aa * cc * (aa + cc) / d;
+        const double da = 1.0 - dg - db;
+        um = u[iP] * da + u[iQ] * db + u[iR] * dg;
+        vm = v[iP] * da + v[iQ] * db + v[iR] * dg;
     }
     fu = -um * wv;
     fv = -vm * wv;
@@ -1129,9 +1117,16 @@ __global__ __launch_bounds__(1024) void k_velrhs(DevMesh m, DevStateT<R> s, doub
 // of its element column (LV<R>; nsigma a multiple of W): every column field
 // and every neighbour column is read with one 16-byte load per W layers, so
 // the kernel issues a quarter (FP32) or half (FP64) of the vector-memory
-// instructions.  The column arrays in LDS use a stride of K slots so each
-// lane's run is 16-byte aligned.  Per layer the arithmetic, and the order of
-// the edge sum, are those of velrhs_body: results are identical bit for bit.
+// instructions.  LDS holds, per column at a stride of K slots (each lane's
+// run 16-byte aligned), the vertical-advection operands (sum w_cv*W, Zw,
+// zbw, formed once per slot) and the velocity, which neighbours in the block
+// read from there; velocity gradients come through 16-byte global loads.
+// Each edge is oriented once (own/other element instead of edge_tri 1/2:
+// the sums of the flux formula commute exactly), and the vertical fluxes are
+// formed once per interface of the lane's run (the bottom flux of one layer
+// is the top flux of the next) without divergence (uvert_uni).  Per layer
+// the arithmetic and the order of the edge sum are those of velrhs_body:
+// results are identical bit for bit.
 template <typename R, bool ATM, bool FILT>
 __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
                                                 double d0i, double tau) {
@@ -1142,37 +1137,43 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
     ElemRec *er = (ElemRec *)lds_;                                        // [cpb]
     EdgeRec<R> *edr = (EdgeRec<R> *)(lds_ + f.cpb * kVelrhsElemBytes);   // [cpb][4]
     const int T = f.cpb * K;
-    double *cws = (double *)(edr + 4 * f.cpb);
-    R *cu = (R *)(cws + T), *cv = cu + T, *cgux = cv + T, *cguy = cgux + T, *cgvx = cguy + T, *cgvy = cgvx + T;
+    double *cws = (double *)(edr + 4 * f.cpb), *cZw = cws + T, *czb = cZw + T;
+    R *cu = (R *)(czb + T), *cv = cu + T;
     if (f.nc <= 0) return;
-    if ((int)threadIdx.x < f.nc) {  // element and edge records: as velrhs_body
-        const int c = threadIdx.x, e = f.c0 + c;
-        ElemRec r;
-        r.col = elem_col(m, s, e);
-        const int4 q = r.col.q;
-        const double4 gx = m.gsx[e], gy = m.gsy[e];
-        r.area = m.elem_area[e];
-        r.cor = m.coriolis[e];
-        r.mask = s.mask_wd[e];
-        double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
-        double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
-        double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
-        double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
-        if (ATM) {
-            const double4 w = r.col.w;
-            const double am = w.x * s.mslp[q.x] + w.y * s.mslp[q.y] + w.z * s.mslp[q.z] + w.w * s.mslp[q.w];
-            const double em = w.x * s.emp[q.x] + w.y * s.emp[q.y] + w.z * s.emp[q.z] + w.w * s.emp[q.w];
-            pre0 = pre0 - (kPref - am) * d0i + kG * em * d0i;
-            pre1 = pre1 - (kPref - am) * d0i + kG * em * d0i;
-            pre2 = pre2 - (kPref - am) * d0i + kG * em * d0i;
-            pre3 = pre3 - (kPref - am) * d0i + kG * em * d0i;
-        }
-        r.Fx = sum4(gx, pre0, pre1, pre2, pre3) * r.area;
-        r.Fy = sum4(gy, pre0, pre1, pre2, pre3) * r.area;
-        r.acc = sum4(r.col.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
-        er[c] = r;
-        const int4 ied = m.el_ied[e];
-        for (int j = 0; j < 4; ++j) {
+    // element and edge records, one thread per record (the element's four
+    // edge slots, then the element), while every lane stages its run of the
+    // column arrays: all of the block's loads are in flight before the one
+    // barrier
+    for (int t = threadIdx.x; t < 5 * f.nc; t += blockDim.x) {
+        if (t >= 4 * f.nc) {  // element record: as velrhs_body
+            const int c = t - 4 * f.nc, e = f.c0 + c;
+            ElemRec r;
+            r.col = elem_col(m, s, e);
+            const int4 q = r.col.q;
+            const double4 gx = m.gsx[e], gy = m.gsy[e];
+            r.area = m.elem_area[e];
+            r.cor = m.coriolis[e];
+            r.mask = s.mask_wd[e];
+            double pre0 = -kG * (m.atp1[q.x] * s.eta_n[q.x] - m.atp2[q.x] * 0.0);
+            double pre1 = -kG * (m.atp1[q.y] * s.eta_n[q.y] - m.atp2[q.y] * 0.0);
+            double pre2 = -kG * (m.atp1[q.z] * s.eta_n[q.z] - m.atp2[q.z] * 0.0);
+            double pre3 = -kG * (m.atp1[q.w] * s.eta_n[q.w] - m.atp2[q.w] * 0.0);
+            if (ATM) {
+                const double4 w = r.col.w;
+                const double am = w.x * s.mslp[q.x] + w.y * s.mslp[q.y] + w.z * s.mslp[q.z] + w.w * s.mslp[q.w];
+                const double em = w.x * s.emp[q.x] + w.y * s.emp[q.y] + w.z * s.emp[q.z] + w.w * s.emp[q.w];
+                pre0 = pre0 - (kPref - am) * d0i + kG * em * d0i;
+                pre1 = pre1 - (kPref - am) * d0i + kG * em * d0i;
+                pre2 = pre2 - (kPref - am) * d0i + kG * em * d0i;
+                pre3 = pre3 - (kPref - am) * d0i + kG * em * d0i;
+            }
+            r.Fx = sum4(gx, pre0, pre1, pre2, pre3) * r.area;
+            r.Fy = sum4(gy, pre0, pre1, pre2, pre3) * r.area;
+            r.acc = sum4(r.col.w, m.ac[q.x], m.ac[q.y], m.ac[q.z], m.ac[q.w]);
+            er[c] = r;
+        } else {  // edge slot j of element column c
+            const int c = t >> 2, j = t & 3, e = f.c0 + c;
+            const int4 ied = m.el_ied[e];
             const int cc = j == 0 ? ied.x : (j == 1 ? ied.y : (j == 2 ? ied.z : ied.w));
             EdgeRec<R> g;
             g.cc = cc;
@@ -1182,54 +1183,55 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
                 const int2 et = m.edtri[ed];
                 const double4 cd = m.ecross[ed];
                 const double2 dd = m.edxdy[ed];
-                const int oe = cc >= 0 ? et.x : et.y;
+                const bool own1 = cc < 0;   // this element is edge_tri(1)
+                const int oe = own1 ? et.y : et.x;
                 g.oK = oe * K;
                 g.oc = (oe >= f.c0 && oe < f.c0 + f.nc) ? oe - f.c0 : -1;
                 g.acc = (R)(0.5 * (m.ac[nd.x] + m.ac[nd.y]));
-                g.cx = (R)cd.x;
-                g.cy = (R)cd.y;
-                g.cz = (R)cd.z;
-                g.cw = (R)cd.w;
+                // cross distances and cosines of the own (cx, cy, cos1) and
+                // the other element (cz, cw, cos2)
+                g.cx = (R)(own1 ? cd.x : cd.z);
+                g.cy = (R)(own1 ? cd.y : cd.w);
+                g.cz = (R)(own1 ? cd.z : cd.x);
+                g.cw = (R)(own1 ? cd.w : cd.y);
                 g.dx = (R)dd.x;
                 g.dy = (R)dd.y;
-                g.cos1 = (R)m.elem_cos[et.x];
-                g.cos2 = (R)m.elem_cos[et.y];
+                g.cos1 = (R)m.elem_cos[own1 ? et.x : et.y];
+                g.cos2 = (R)m.elem_cos[own1 ? et.y : et.x];
                 g.an1 = s.a[nd.x];
                 g.an2 = s.a[nd.y];
             }
             edr[c * 4 + j] = g;
         }
     }
-    __syncthreads();
     const int c = f.c, e = f.c0 + c, k0 = f.k * W, ck0 = c * K + k0;
     const size_t ek0 = (size_t)e * KC + k0;
-    if (f.on()) {  // the column arrays of the vertical advection and the neighbours
-        const ElemCol &col = er[c].col;
+    ElemCol col;
+    if (f.on()) {  // the column arrays of the vertical advection and the velocity
+        col = elem_col(m, s, e);
         const int4 q = col.q;
         const LV<R> w0 = ldv_glb(s.W + (size_t)q.x * KC + k0), w1 = ldv_glb(s.W + (size_t)q.y * KC + k0);
         const LV<R> w2 = ldv_glb(s.W + (size_t)q.z * KC + k0), w3 = ldv_glb(s.W + (size_t)q.w * KC + k0);
 #pragma unroll
-        for (int i = 0; i < W; ++i) cws[ck0 + i] = sum4(col.w, w0.x[i], w1.x[i], w2.x[i], w3.x[i]);
+        for (int i = 0; i < W; ++i) {
+            const int k = k0 + i;
+            cws[ck0 + i] = sum4(col.w, w0.x[i], w1.x[i], w2.x[i], w3.x[i]);
+            cZw[ck0 + i] = k < L ? col.Zw(m, k) : 0.0;
+            czb[ck0 + i] = col.zbw(m, k);
+        }
         stv(cu + ck0, ldv_glb(s.U_n + ek0));
         stv(cv + ck0, ldv_glb(s.V_n + ek0));
-        if (hadv) {
-            stv(cgux + ck0, ldv_glb(s.gux + ek0));
-            stv(cguy + ck0, ldv_glb(s.guy + ek0));
-            stv(cgvx + ck0, ldv_glb(s.gvx + ek0));
-            stv(cgvy + ck0, ldv_glb(s.gvy + ek0));
-        }
     }
     __syncthreads();
     if (!f.on()) return;
     const int nl = min(W, L - k0);  // layers of this lane (the column's last lane: W-1)
     const ElemRec &r = er[c];
-    const ElemCol &col = r.col;
     const double area = r.area;
     const double area_inv = 1.0 / area;
     const LV<R> ua = ldv_glb(s.U_rhsAB + ek0), va = ldv_glb(s.V_rhsAB + ek0);
     double je[W], ur[W], vr[W];
     R abu[W], abv[W];
-    LV<R> uo = ldv_lds(cu + ck0), vo = ldv_lds(cv + ck0);
+    const LV<R> uo = ldv_lds(cu + ck0), vo = ldv_lds(cv + ck0);
 #pragma unroll
     for (int i = 0; i < W; ++i) {
         je[i] = col.Je(m, k0 + i < L ? k0 + i : L - 1);
@@ -1266,53 +1268,37 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
         abv[i] = (R)(-uo.x[i] * r.cor * area * je[i]);
     }
     if (hadv) {
-        const LV<R> gxo = ldv_lds(cgux + ck0), gyo = ldv_lds(cguy + ck0);
-        const LV<R> hxo = ldv_lds(cgvx + ck0), hyo = ldv_lds(cgvy + ck0);
+        const LV<R> gxo = ldv_glb(s.gux + ek0), gyo = ldv_glb(s.guy + ek0);
+        const LV<R> hxo = ldv_glb(s.gvx + ek0), hyo = ldv_glb(s.gvy + ek0);
         for (int j = 0; j < 4; ++j) {
             const EdgeRec<R> &g = edr[c * 4 + j];
             if (g.cc == kEmpty) break;
             const bool own1 = g.cc < 0;  // the element is edge_tri(1)
-            LV<R> ux, vx, gxx, gyx, hxx, hyx;
+            const size_t io = (size_t)g.oK + k0;
+            LV<R> ux, vx;
             if (g.oc >= 0) {  // the other element is a column of this block
-                const int i0 = g.oc * K + k0;
-                ux = ldv_lds(cu + i0);
-                vx = ldv_lds(cv + i0);
-                gxx = ldv_lds(cgux + i0);
-                gyx = ldv_lds(cguy + i0);
-                hxx = ldv_lds(cgvx + i0);
-                hyx = ldv_lds(cgvy + i0);
+                ux = ldv_lds(cu + g.oc * K + k0);
+                vx = ldv_lds(cv + g.oc * K + k0);
             } else {
-                const size_t io = (size_t)g.oK + k0;
                 ux = ldv_glb(s.U_n + io);
                 vx = ldv_glb(s.V_n + io);
-                gxx = ldv_glb(s.gux + io);
-                gyx = ldv_glb(s.guy + io);
-                hxx = ldv_glb(s.gvx + io);
-                hyx = ldv_glb(s.gvy + io);
             }
+            const LV<R> gxx = ldv_glb(s.gux + io), gyx = ldv_glb(s.guy + io);
+            const LV<R> hxx = ldv_glb(s.gvx + io), hyx = ldv_glb(s.gvy + io);
+            // g: own element's cross distances (cx, cy) and cosine (cos1),
+            // the other's (cz, cw, cos2)
 #pragma unroll
             for (int i = 0; i < W; ++i) {
                 const double ds = m.dsig[k0 + i < L ? k0 + i : L - 1];
-                const R U1 = own1 ? uo.x[i] : ux.x[i], V1 = own1 ? vo.x[i] : vx.x[i];
-                const R U2 = own1 ? ux.x[i] : uo.x[i], V2 = own1 ? vx.x[i] : vo.x[i];
-                const R gux1 = own1 ? gxo.x[i] : gxx.x[i], guy1 = own1 ? gyo.x[i] : gyx.x[i];
-                const R gvx1 = own1 ? hxo.x[i] : hxx.x[i], gvy1 = own1 ? hyo.x[i] : hyx.x[i];
-                const R gux2 = own1 ? gxx.x[i] : gxo.x[i], guy2 = own1 ? gyx.x[i] : gyo.x[i];
-                const R gvx2 = own1 ? hxx.x[i] : hxo.x[i], gvy2 = own1 ? hyx.x[i] : hyo.x[i];
-                const R u1 = U1 - gux1 * g.cx - guy1 * g.cy;
-                const R v1 = V1 - gvx1 * g.cx - gvy1 * g.cy;
-                const R u2 = U2 - gux2 * g.cz - guy2 * g.cw;
-                const R v2 = V2 - gvx2 * g.cz - gvy2 * g.cw;
+                const R uor = uo.x[i] - gxo.x[i] * g.cx - gyo.x[i] * g.cy;
+                const R vor = vo.x[i] - hxo.x[i] * g.cx - hyo.x[i] * g.cy;
+                const R uxr = ux.x[i] - gxx.x[i] * g.cz - gyx.x[i] * g.cw;
+                const R vxr = vx.x[i] - hxx.x[i] * g.cz - hyx.x[i] * g.cw;
                 const R jd = (R)(0.5 * (g.an1 * ds + g.an2 * ds));
-                const R un = R(0.5) * R(kREarthD) * jd * ((u1 + u2) * g.dy - (v1 * g.cos1 + v2 * g.cos2) * g.dx);
-                R uu, vv;
-                if (un >= R(0)) {
-                    uu = u1 * un * g.acc;
-                    vv = v1 * un * g.acc;
-                } else {
-                    uu = u2 * un * g.acc;
-                    vv = v2 * un * g.acc;
-                }
+                const R un = R(0.5) * R(kREarthD) * jd * ((uor + uxr) * g.dy - (vor * g.cos1 + vxr * g.cos2) * g.dx);
+                const bool po = (un >= R(0)) == own1;   // the upwind element is this one
+                const R uu = (po ? uor : uxr) * un * g.acc;
+                const R vv = (po ? vor : vxr) * un * g.acc;
                 if (g.cc >= 0) {
                     abu[i] = abu[i] + uu;
                     abv[i] = abv[i] + vv;
@@ -1327,7 +1313,7 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
         if (r.mask != 0.0) {
             // flux through 1-based interface nz (0 at the bottom, nz = K);
             // the bottom flux of one layer is the top flux of the next
-            const double *ws = cws + c * K;
+            const double *ws = cws + c * K, *zw = cZw + c * K, *zb = czb + c * K;
             const R *uc = cu + c * K, *vc = cv + c * K;
             auto flux = [&](int nz, double &fu, double &fv) {
                 fu = 0.0;
@@ -1337,7 +1323,7 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
                     fu = -w1 * uc[0];
                     fv = -w1 * vc[0];
                 } else if (nz <= K - 1) {
-                    uvert_col(m, col, ws, uc, vc, K, area, nz, fu, fv);
+                    uvert_uni(ws, uc, vc, zw, zb, K, area, nz, fu, fv);
                 }
             };
             double ftu, ftv;
