@@ -3331,6 +3331,7 @@ struct MlpWaveLds {
     EosLevel lv[64];
     double2 uv[64];
     double col_a[64 / kMlpMinK], col_dm[64 / kMlpMinK], col_f3[64 / kMlpMinK];
+    double col_idm2[64 / kMlpMinK], col_irdm[64 / kMlpMinK];   // 1/dmean^2, 1/(rho0*dmean)
     int col_ood[64 / kMlpMinK];
 };
 static_assert(sizeof(MlpWaveLds) == kMlpWaveLds, "closure LDS record");
@@ -3351,7 +3352,8 @@ __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s
     fcgm::bf16x8 *fl = (fcgm::bf16x8 *)lds_;
     float *bl = (float *)(fl + kFragFwd * 64);
     double *sg = (double *)(bl + kMlpBias), *om = sg + kMlpMaxK;   // sigma and 1-sigma of the K levels
-    MlpWaveLds &w = ((MlpWaveLds *)(om + kMlpMaxK))[threadIdx.x >> 6];
+    double *rd = om + kMlpMaxK, *rd2 = rd + kMlpMaxK;   // 1/d, 1/d^2 of interface iz at iz-1
+    MlpWaveLds &w = ((MlpWaveLds *)(rd2 + kMlpMaxK))[threadIdx.x >> 6];
     const int K = m.K, L = m.L, KC = K;
     const int tid = threadIdx.x, lane = tid & 63;
     for (int i = tid; i < kFragFwd * 64; i += kMlpT) fl[i] = frag[i];
@@ -3359,8 +3361,17 @@ __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s
     for (int i = tid; i < K; i += kMlpT) {
         sg[i] = m.sigma[i];
         om[i] = m.oms[i];
+        if (i >= 1 && i <= K - 2) {  // interface iz = i + 1 between layers iz-1 and iz
+            const double d1 = m.sigma[i - 1] - m.sigma[i], d2 = m.sigma[i] - m.sigma[i + 1];
+            const double d = 0.5 * (d1 + d2);
+            rd[i] = 1.0 / d;
+            rd2[i] = 1.0 / (d * d);
+        }
     }
     const double dsb = m.dsig[L - 1];
+    // normalisation by multiplication
+    float xis[kMlpF];
+    for (int j = 0; j < kMlpF; ++j) xis[j] = 1.0f / nm.x_std[j];
     __syncthreads();
     const int cpw = 64 / K;  // whole columns per wavefront
     const int ngroups = (m.N + cpw - 1) / cpw;
@@ -3410,6 +3421,8 @@ __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s
             const double lg = log(delta_b / z0b_min);
             w.col_a[lane] = a;
             w.col_dm[lane] = dmean;
+            w.col_idm2[lane] = 1.0 / (dmean * dmean);
+            w.col_irdm[lane] = 1.0 / (rho0 * dmean);
             w.col_f3[lane] = log10f((float)(16.6 * (kCka * kCka) / (lg * lg) * (ub * ub + vb * vb) + 1e-12));
             // columns of the open-boundary sponge layer (ac < 1), whose damped
             // dynamics the network is not trained for, keep d3_end
@@ -3428,17 +3441,22 @@ __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s
         }
         float fx[kMlpF] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (on) {
+            // the features of column_features, with its divisions by the
+            // layer spacing and the column depth as multiplications by
+            // reciprocals formed once per level or column, and the two
+            // interface densities' difference over one common denominator
             const double a = w.col_a[c], dmean = w.col_dm[c];
             const int iz = nz < 2 ? 2 : (nz > K - 1 ? K - 1 : nz);  // 1-based interface
             const int up = c * K + iz - 2, dn = up + 1;                 // its two layers
             const double2 w1 = w.uv[up], w2 = w.uv[dn];
-            const double d1 = sg[iz - 2] - sg[iz - 1], d2 = sg[iz - 1] - sg[iz];
-            const double d = 0.5 * (d1 + d2);
-            const double sh2 = ((w1.x - w2.x) * (w1.x - w2.x) + (w1.y - w2.y) * (w1.y - w2.y)) / (d * d) /
-                               (dmean * dmean);
-            const double zb = om[iz - 1] * a;
-            const double rho_up = eos_rho(w.lv[up], -zb, rho0), rho_dn = eos_rho(w.lv[dn], -zb, rho0);
-            const double n2 = -kG * (rho_up - rho_dn) / d / (rho0 * dmean);
+            const double sh2 = ((w1.x - w2.x) * (w1.x - w2.x) + (w1.y - w2.y) * (w1.y - w2.y)) * rd2[iz - 1] *
+                               w.col_idm2[c];
+            const double pz = -(om[iz - 1] * a);
+            const EosLevel eu = w.lv[up], ed = w.lv[dn];
+            const double bu = eu.c0 - pz * eu.c1 + pz * pz * eu.c2, bd = ed.c0 - pz * ed.c1 + pz * pz * ed.c2;
+            const double hu = bu + 0.1 * pz, hd = bd + 0.1 * pz;
+            const double drho = (eu.rp * bu * hd - ed.rp * bd * hu) / (hu * hd);   // rho_up - rho_dn
+            const double n2 = -kG * drho * rd[iz - 1] * w.col_irdm[c];
             fx[0] = log10f((float)(sh2 + 1e-12));
             fx[1] = asinhf((float)(n2 * 1e5));
             fx[2] = log10f((float)dmean);
@@ -3446,7 +3464,7 @@ __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s
             fx[4] = (float)sg[nz - 1];
             bool ood = false;
             for (int j = 0; j < kMlpF; ++j) {
-                fx[j] = (fx[j] - nm.x_mean[j]) / nm.x_std[j];
+                fx[j] = (fx[j] - nm.x_mean[j]) * xis[j];
                 ood = ood || !(fx[j] >= nm.x_lo[j] && fx[j] <= nm.x_hi[j]);
             }
             if (ood) atomicOr(&w.col_ood[c], 1);

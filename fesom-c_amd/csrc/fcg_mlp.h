@@ -48,8 +48,9 @@ constexpr int kMlpFragCount = 148;
 constexpr int kMlpT = 512;
 constexpr int kMlpMinK = 4, kMlpMaxK = 64;
 constexpr size_t kMlpWaveLds = sizeof(float) * 64 * (kMlpF + 2) + sizeof(double) * 6 * 64 +
-                               (sizeof(double) * 3 + sizeof(int)) * (64 / kMlpMinK);
-constexpr size_t kMlpClosureLds = 16 * kFragFwd * 64 + sizeof(float) * kMlpBias + 2 * sizeof(double) * kMlpMaxK +
+                               (sizeof(double) * 5 + sizeof(int)) * (64 / kMlpMinK);
+// (the block's sigma tables: sigma, 1-sigma, and per interface 1/d, 1/d^2)
+constexpr size_t kMlpClosureLds = 16 * kFragFwd * 64 + sizeof(float) * kMlpBias + 4 * sizeof(double) * kMlpMaxK +
                                   (kMlpT / 64) * kMlpWaveLds;
 
 // Entries that are real weights (the rest is zero padding and stays zero).
