@@ -56,6 +56,7 @@ template <typename R> __global__ void k_tr_ab(DevMesh, DevStateT<R>, size_t);
 template <typename R> __global__ void k_tr_upwind(DevMesh, DevStateT<R>, double, double, int, double, double);
 template <typename R> __global__ void k_tr_gradn_muscl(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_muscl(DevMesh, DevStateT<R>, double, double, int, double, double);
+template <typename R> __global__ void k_tr_muscl_v(DevMesh, DevStateT<R>, double, double, int, double, double, int);
 template <typename R> __global__ void k_tr_gradn_miura(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_tr_miura(DevMesh, DevStateT<R>, double, double, double, int, double, double);
 template <typename R> __global__ void k_tr_rivers(DevMesh, DevStateT<R>, double);
@@ -135,6 +136,20 @@ inline size_t lds_muscl(const fcg_ctx *c, int T = kFlatT) {
     const int cap = c->cap_n2e_t[gather_slot(T)];
     return (size_t)cap * sizeof(EdgeGeo<R>) + (size_t)((cap + 1) & ~1) * sizeof(int) +
            6 * flat_bytes(c->m.L, sizeof(R), T);
+}
+// level-vector MUSCL: the block's CSR edge records and orientations, then
+// (16-byte aligned) seven column arrays of K slots and the K levels' 1-sigma
+inline size_t muscl_v_cols(size_t nrec, size_t esz) {
+    const size_t geo = esz == 8 ? sizeof(EdgeGeo<double>) : sizeof(EdgeGeo<float>);
+    return (nrec * (geo + sizeof(int)) + 15) & ~(size_t)15;
+}
+inline int vec_threads(const fcg_ctx *c, size_t (*bytes)(const fcg_ctx *, int)) {
+    return bytes(c, 256) <= (160u << 10) / 3 ? 256 : 128;   // three 256-thread blocks' LDS per unit, else 128
+}
+inline size_t lds_muscl_v(const fcg_ctx *c, int T) {
+    const int W = 16 / (int)c->esize, cpb = T / (c->m.K / W);
+    return muscl_v_cols((size_t)c->cap_n2e_v[T == 256 ? 1 : 0], c->esize) + 7 * (size_t)cpb * c->m.K * c->esize +
+           (size_t)c->m.K * sizeof(double);
 }
 template <typename R>
 inline size_t lds_vdiv(const fcg_ctx *c, int T = kFlatT) { return (size_t)c->cap_n2e_t[gather_slot(T)] * sizeof(DivRec<R>); }
@@ -626,6 +641,15 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     for (int n = 0; n < N; ++n) {
         n2el_ptr[n + 1] = n2el_ptr[n] + (int)n2el[n].size();
         n2el_idx.insert(n2el_idx.end(), n2el[n].begin(), n2el[n].end());
+    }
+    for (int t = 0; t < 2; ++t) {  // level-vector node blocks: cpb = T/(K/W) nodes, W = 16/storage width
+        const int W = p.precision == FCG_MIXED ? 4 : 2;   // the storage width is set further down
+        const int cpb = std::max(1, (128 << t) / std::max(1, K / W));
+        for (int c0 = 0; c0 < N; c0 += cpb) {
+            const int c1 = std::min(N, c0 + cpb);
+            ctx->cap_n2e_v[t] = std::max(ctx->cap_n2e_v[t], n2e_ptr[c1] - n2e_ptr[c0]);
+            ctx->cap_n2el_v[t] = std::max(ctx->cap_n2el_v[t], n2el_ptr[c1] - n2el_ptr[c0]);
+        }
     }
     for (int t = 0; t < 3; ++t) {  // most CSR slots in one node-gather block of cpb = T/L consecutive nodes
         const int cpb = std::max(1, (256 << t) / L);
@@ -1301,8 +1325,17 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
         const int bg = block_of(ctx, "k_tr_gradn_muscl"), bm = block_of(ctx, "k_tr_muscl");
         LAUNCH_EX(ctx, k_tr_gradn_muscl<R>, nflatblk(m.N, m.L, bg), bg, lds_gradn(ctx, bg), m, s);
-        LAUNCH_EX(ctx, k_tr_muscl<R>, nflatblk(m.N, m.L, bm), bm, lds_muscl<R>(ctx, bm), m, s, p.dt, p.Dmin, relax,
-                  p.clim_relax, p.relax2clim_ac);
+        constexpr int W = 16 / (int)sizeof(R);
+        const int tv = vec_threads(ctx, lds_muscl_v);
+        if (levels_of(ctx, "k_tr_muscl", W) == W && lds_muscl_v(ctx, tv) <= (size_t)ctx->lds_limit) {
+            const int cpb = tv / (m.K / W);
+            LAUNCH_NAMED(ctx, "k_tr_muscl", fcgk::k_tr_muscl_v<R>, (unsigned)((m.N + cpb - 1) / cpb), tv,
+                         lds_muscl_v(ctx, tv), m, s, p.dt, p.Dmin, relax, p.clim_relax, p.relax2clim_ac,
+                         ctx->cap_n2e_v[tv == 256 ? 1 : 0]);
+        } else {
+            LAUNCH_EX(ctx, k_tr_muscl<R>, nflatblk(m.N, m.L, bm), bm, lds_muscl<R>(ctx, bm), m, s, p.dt, p.Dmin, relax,
+                      p.clim_relax, p.relax2clim_ac);
+        }
     } else if (p.tracer_adv == 4) {
         LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
         LAUNCHLV(ctx, k_tr_gradn_miura<R>, m.N, m.L, m, s);
@@ -1362,7 +1395,7 @@ static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out, 
     hipLaunchKernelGGL(fcgk::k_diag_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)s.red, nb, s.scal + 4);
     FCG_CHECK(hipGetLastError());
     double r[12];
-    int wet = 0, err = 0, nonfinite = 0;
+    int wet = 0, err = 0, nonfinite = 0, shape = 0;
     FCG_CHECK(hipMemsetAsync(s.err + 1, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(fcgk::k_mask_count, dim3(64), dim3(kBlk), 0, ctx->stream, (const double *)s.mask_wd + e0,
                        e1 - e0, s.err + 1);
@@ -1370,7 +1403,11 @@ static int diagnostics_impl(fcg_ctx *ctx, const DevStateT<R> &s, fcg_diag *out, 
     FCG_CHECK(hipMemcpyAsync(&err, s.err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipMemcpyAsync(&wet, s.err + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipMemcpyAsync(&nonfinite, s.err + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCG_CHECK(hipMemcpyAsync(&shape, s.err + 3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    // a node-gather block that found more CSR slots than its LDS was sized
+    // for skipped its work: a set-up fault of this library, never a result
+    if (shape) return fcg_fail(ctx, FCG_E_HIP, "a node-gather block exceeded its LDS capacity (launch shape)");
     std::memset(out, 0, sizeof *out);
     out->energy = (r[0] + r[2]) / r[1];
     out->volume = r[3];
@@ -1467,14 +1504,16 @@ extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
         if (colon && !std::strcmp(colon, ":levels")) {
             const std::string k(kernel, colon - kernel);
             const int W = 16 / (int)ctx->esize;
-            static const char *vec[] = {"k_velrhs"};
+            static const char *vec[] = {"k_velrhs", "k_tr_muscl"};
             bool known = false;
             for (const char *v : vec) known = known || k == v;
             if (!known) return fcg_fail(ctx, FCG_E_ARG, "kernel without a level-vector variant: " + k);
             if (!(block == 1 || block == W)) return FCG_E_ARG;
             if (block == W && ctx->m.K % W)
                 return fcg_fail(ctx, FCG_E_ARG, k + ": " + std::to_string(W) + " levels per lane need nsigma to be a multiple of it");
-            if (block == W && velrhs_v_bytes(ctx->m, ctx->esize, velrhs_v_threads(ctx->m, ctx->esize)) > (size_t)ctx->lds_limit)
+            const size_t lds = k == "k_velrhs" ? velrhs_v_bytes(ctx->m, ctx->esize, velrhs_v_threads(ctx->m, ctx->esize))
+                                               : lds_muscl_v(ctx, vec_threads(ctx, lds_muscl_v));
+            if (block == W && lds > (size_t)ctx->lds_limit)
                 return fcg_fail(ctx, FCG_E_ARG, k + ": LDS of the level-vector variant exceeds the device limit");
             ctx->launch_block[kernel] = block;
             return FCG_OK;

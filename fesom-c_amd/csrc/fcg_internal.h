@@ -227,6 +227,9 @@ struct fcg_ctx {
     // most CSR slots (node -> edge, node -> element) in one node-gather block
     // of 256, 512 and 1024 threads (256/L, 512/L, 1024/L node columns)
     int cap_n2e_t[3] = {0, 0, 0}, cap_n2el_t[3] = {0, 0, 0};
+    // the same for the level-vector node kernels: blocks of 128 and 256
+    // threads, K/W lanes per column (W = 16/storage width)
+    int cap_n2e_v[2] = {0, 0}, cap_n2el_v[2] = {0, 0};
     int lds_limit = 0;           // bytes of LDS one block may request on this device
     int n_cu = 1;                // compute units of the device (persistent launches)
     // learned closure

@@ -3108,6 +3108,264 @@ __global__ __launch_bounds__(1024) void k_tr_muscl(DevMesh m, DevStateT<R> s, do
     PL(s.SF, k, n, KC) = sdm + rs * dt + Sa;
 }
 
+// hi_vert2 on a node column held in LDS (X, Y: T and S; Wc: W, slot z-1 of
+// 1-based level z; om: 1-sigma of the levels), both signs of W in one
+// instruction stream: the downward branch is the upward one on the mirrored
+// operands (levels nz, nz-1, nz-2 instead of nz-1, nz, nz+1) with the depth
+// differences negated (x - y = -(y - x) exactly), so every value equals
+// hi_vert2's bit for bit; TmX/TmY keep their values when W is zero (or NaN).
+template <bool kChained, typename R>
+__device__ __forceinline__ void hi_vert2_col(const double *om, const R *X, const R *Y, const R *Wc, int K, int nz,
+                                             double a, double &TmX, double &TmY, double &fx, double &fy) {
+    if (nz == 1) {
+        const double w1 = Wc[0];
+        fx = -w1 * (X[0] - 0.0);
+        fy = -w1 * (Y[0] - 0.0);
+        return;
+    }
+    if (nz == K) {
+        fx = 0.0;
+        fy = 0.0;
+        return;
+    }
+    const double w = Wc[nz - 1];
+    if (w > 0.0 || w < 0.0) {
+        const bool up = w > 0.0;
+        const int iP = up ? nz - 1 : nz, iQ = up ? nz : nz - 1, iR = up ? nz + 1 : nz - 2;  // 1-based levels
+        if (up ? nz == K - 1 : nz == 2) {
+            TmX = X[iQ - 1] - 0.0;
+            TmY = Y[iQ - 1] - 0.0;
+        } else {
+            auto Zc = [&](int z) { return 0.5 * (om[z - 1] * a + om[z] * a); };
+            const double sg = up ? 1.0 : -1.0;
+            const double zb = om[nz - 1] * a;
+            const double aa = sg * (zb - Zc(iP)), bb = sg * (Zc(iQ) - zb), cc = sg * (Zc(iR) - zb);
+            double dg, db;
+            if (kChained) {
+                dg = aa * bb / (cc + aa) / (bb - cc);
+                db = -aa * cc / (bb + aa) / (bb - cc);
+            } else {
+                dg = aa * bb / ((cc + aa) * (bb - cc));
+                db = -aa * cc / ((bb + aa) * (bb - cc));
+            }
+            const double da = 1.0 - dg - db;
+            TmX = (X[iP - 1] - 0.0) * da + (X[iQ - 1] - 0.0) * db + (X[iR - 1] - 0.0) * dg;
+            TmY = (Y[iP - 1] - 0.0) * da + (Y[iQ - 1] - 0.0) * db + (Y[iR - 1] - 0.0) * dg;
+        }
+    }
+    fx = -TmX * w;
+    fy = -TmY * w;
+}
+
+// solve_tracer_muscl as k_tr_muscl, with each lane owning W consecutive
+// layers of its node column (LV<R>; nsigma a multiple of W): the block's
+// columns of AB-interpolated T and S, their node gradients and W sit in LDS
+// at a stride of K slots, neighbours outside the block and the element
+// velocities come through 16-byte loads, so the edge records, orientation
+// and addresses of an edge serve W layers.  Each edge is taken from this
+// node's side: the face value of the 15 %-upwind blend is formed from the
+// own and the other node's reconstruction (x + y = y + x and x - y =
+// x + (-y) exactly, so no operand needs reordering by edge_nodes), and the
+// QUICK vertical fluxes are formed once per interface of the lane's run.
+// Per layer the arithmetic and the order of the edge sum are k_tr_muscl's:
+// the results are identical bit for bit (a flux through an interface where
+// W is exactly zero can differ in the sign of that zero).
+// cap: the most CSR slots of any block, for which the LDS is sized (the
+// layout below follows it); a block with more sets a flag and does nothing.
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_muscl_v(DevMesh m, DevStateT<R> s, double dt, double Dmin, int cl_relax,
+                                                    double clim_relax, double relax2clim_ac, int cap) {
+    constexpr int W = LV<R>::W;
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int KC = m.K, K = m.K, L = m.L;
+    const ColFlat f = col_flat(m.N, K / W);  // f.k: the lane's run of W slots
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2e_ptr, f);
+    const int nrec = jr.y - jr.x, T = f.cpb * K;
+    if (nrec > cap) {
+        if (threadIdx.x == 0) atomicOr(s.err + 3, 1);
+        return;
+    }
+    EdgeGeo<R> *rec = (EdgeGeo<R> *)lds_;
+    int *oc = (int *)(rec + cap);  // the other end's column in this block (-1: outside), sign: node is edge_nodes(1)
+    R *cT = (R *)(lds_ + (((size_t)cap * (sizeof(EdgeGeo<R>) + sizeof(int)) + 15) & ~(size_t)15));
+    R *cS = cT + T, *cTx = cS + T, *cTy = cTx + T, *cSx = cTy + T, *cSy = cSx + T, *cW = cSy + T;
+    double *om = (double *)(cW + T);
+    {
+        const EdgeGeo<R> *eg = (const EdgeGeo<R> *)m.egeo;
+        for (int t = threadIdx.x; t < nrec; t += blockDim.x) {
+            const int cc = m.n2e[jr.x + t];
+            const EdgeGeo<R> g = eg[cc >= 0 ? cc : ~cc];
+            rec[t] = g;
+            const int on = (cc >= 0 ? g.n2K : g.n1K) / KC - f.c0;
+            const int col = (on >= 0 && on < f.nc) ? on : -1;
+            oc[t] = cc >= 0 ? col + 2 : -(col + 2);  // |oc| - 2 = column, sign = orientation
+        }
+        for (int t = threadIdx.x; t < K; t += blockDim.x) om[t] = m.oms[t];
+    }
+    const int n = f.c0 + f.c, k0 = f.k * W, ck0 = f.c * K + k0;
+    const size_t nk0 = (size_t)n * KC + k0;
+    if (f.on()) {
+        stv(cT + ck0, ldv_glb(s.abT + nk0));
+        stv(cS + ck0, ldv_glb(s.abS + nk0));
+        stv(cTx + ck0, ldv_glb(s.gTx + nk0));
+        stv(cTy + ck0, ldv_glb(s.gTy + nk0));
+        stv(cSx + ck0, ldv_glb(s.gSx + nk0));
+        stv(cSy + ck0, ldv_glb(s.gSy + nk0));
+        stv(cW + ck0, ldv_glb(s.W + nk0));
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int nl = min(W, L - k0);  // layers of this lane (the column's last lane: W-1)
+    const int mb = m.memb[n];
+    const double Ta = s.tsa[2 * mb], Sa = s.tsa[2 * mb + 1];
+    const double a = s.a[n];
+    const int p0 = m.n2e_ptr[n], p1 = m.n2e_ptr[n + 1];
+    const LV<R> To = ldv_lds(cT + ck0), So = ldv_lds(cS + ck0);
+    const LV<R> gTxo = ldv_lds(cTx + ck0), gTyo = ldv_lds(cTy + ck0), gSxo = ldv_lds(cSx + ck0), gSyo = ldv_lds(cSy + ck0);
+    // horizontal fluxes in the storage type (FP32 in the mixed-precision path)
+    R rt[W], rs[W], rc[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) rt[i] = rs[i] = rc[i] = R(0);
+    for (int j = p0 - jr.x; j < p1 - jr.x; ++j) {
+        const EdgeGeo<R> &g = rec[j];
+        const int o = oc[j];
+        const bool own1 = o > 0;  // this node is edge_nodes(1)
+        const int col = (own1 ? o : -o) - 2;
+        const R sg = own1 ? R(1) : R(-1);
+        const R cdx = g.cx, cdy = g.cy, cdz = g.cz, cdw = g.cw, ddy = g.dy, dxa = g.dxa, dxb = g.dxb;
+        LV<R> Tx, Sx, gTxx, gTyx, gSxx, gSyx;
+        if (col >= 0) {  // the other end is a column of this block
+            const int i0 = col * K + k0;
+            Tx = ldv_lds(cT + i0);
+            Sx = ldv_lds(cS + i0);
+            gTxx = ldv_lds(cTx + i0);
+            gTyx = ldv_lds(cTy + i0);
+            gSxx = ldv_lds(cSx + i0);
+            gSyx = ldv_lds(cSy + i0);
+        } else {
+            const size_t ao = (size_t)(own1 ? g.n2K : g.n1K) + k0;
+            Tx = ldv_glb(s.abT + ao);
+            Sx = ldv_glb(s.abS + ao);
+            gTxx = ldv_glb(s.gTx + ao);
+            gTyx = ldv_glb(s.gTy + ao);
+            gSxx = ldv_glb(s.gSx + ao);
+            gSyx = ldv_glb(s.gSy + ao);
+        }
+        const size_t i1 = (size_t)g.e1K + k0;
+        const LV<R> u1 = ldv_glb(s.U_nf + i1), v1 = ldv_glb(s.V_nf + i1), J1 = ldv_glb(s.Je + i1);
+        const bool seg2 = g.e2K >= 0;
+        LV<R> u2f, v2f, u2, v2, J2;
+        if (seg2) {
+            const size_t i2 = (size_t)g.e2K + k0;
+            u2f = ldv_glb(s.U_nf + i2);
+            v2f = ldv_glb(s.V_nf + i2);
+            u2 = ldv_glb(s.U_n + i2);
+            v2 = ldv_glb(s.V_n + i2);
+            J2 = ldv_glb(s.Je + i2);
+        }
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            // dT = 2 (T2 - T1) from this node's side
+            const R dTt = sg * (R(2.0) * (Tx.x[i] - To.x[i])), dTs = sg * (R(2.0) * (Sx.x[i] - So.x[i]));
+            // face value of one segment (length dx): the own and the other
+            // node's reconstructions (muscl_mean's Tm1/Tm2 by orientation),
+            // 15 % from the upwind one (from2: from edge_nodes(2))
+            auto face = [&](R To_, R Tx_, R dT, R dx, R gxo, R gyo, R gxx, R gyx, bool from2) {
+                const R Ao = div6(dT + dx * gxo + ddy * R(kREarthD) * gyo);
+                const R Ax = div6(dT + dx * gxx + ddy * R(kREarthD) * gyx);
+                const R Tmo = To_ + sg * Ao, Tmx = Tx_ - sg * Ax;
+                const R up = (from2 == own1) ? Tmx : Tmo;
+                return R(0.15) * up + R(0.85) * (Tmo + Tmx) / R(2.0);
+            };
+            const bool f2 = v1.x[i] * cdx - u1.x[i] * cdy > R(0);
+            const R mT = face(To.x[i], Tx.x[i], dTt, dxa, gTxo.x[i], gTyo.x[i], gTxx.x[i], gTyx.x[i], f2);
+            const R mS = face(So.x[i], Sx.x[i], dTs, dxa, gSxo.x[i], gSyo.x[i], gSxx.x[i], gSyx.x[i], f2);
+            const R J1i = J1.x[i], u1i = u1.x[i], v1i = v1.x[i];
+            const R cT = (J1i * v1i * mT) * cdx - (J1i * u1i * mT) * cdy;
+            const R cS = (J1i * v1i * mS) * cdx - (J1i * u1i * mS) * cdy;
+            const R cV = J1i * v1i * cdx - J1i * u1i * cdy;
+            if (own1) {
+                rt[i] = rt[i] + cT;
+                rs[i] = rs[i] + cS;
+            } else {
+                rt[i] = rt[i] - cT;
+                rs[i] = rs[i] - cS;
+            }
+            if (seg2) {  // second segment
+                const bool g2 = v2f.x[i] * cdz - u2f.x[i] * cdw < R(0);
+                const R nT = face(To.x[i], Tx.x[i], dTt, dxb, gTxo.x[i], gTyo.x[i], gTxx.x[i], gTyx.x[i], g2);
+                const R nS = face(So.x[i], Sx.x[i], dTs, dxb, gSxo.x[i], gSyo.x[i], gSxx.x[i], gSyx.x[i], g2);
+                const R J2i = J2.x[i], u2i = u2.x[i], v2i = v2.x[i];
+                const R c2T = -(J2i * v2i * nT) * cdz + (J2i * u2i * nT) * cdw;
+                const R c2S = -(J2i * v2i * nS) * cdz + (J2i * u2i * nS) * cdw;
+                const R c2V = -J2i * v2f.x[i] * cdz + J2i * u2f.x[i] * cdw;
+                if (own1) {
+                    rt[i] = rt[i] + c2T;
+                    rs[i] = rs[i] + c2S;
+                    rc[i] = rc[i] + cV;
+                    rc[i] = rc[i] + c2V;
+                } else {
+                    rt[i] = rt[i] - c2T;
+                    rs[i] = rs[i] - c2S;
+                    rc[i] = rc[i] - cV;
+                    rc[i] = rc[i] - c2V;
+                }
+            } else {
+                rc[i] = own1 ? rc[i] + cV : rc[i] - cV;
+            }
+        }
+    }
+    // QUICK vertical fluxes through the interfaces k0+1 .. k0+nl+1; the
+    // bottom flux of one layer is the top flux of the next
+    const R *Tc = cT + f.c * K, *Sc = cS + f.c * K, *Wc = cW + f.c * K;
+    double TmT = 0.0, TmS = 0.0, tvt, svt;
+    hi_vert2_col<true>(om, Tc, Sc, Wc, K, k0 + 1, a, TmT, TmS, tvt, svt);
+    const LV<R> TF0 = ldv_glb(s.TF + nk0), SF0 = ldv_glb(s.SF + nk0);
+    const bool relax = cl_relax && m.index[n] == 2;
+    const bool inflow = relax && ob_inflow(m, s, p0, p1, Dmin) > 0.0;
+    LV<R> Tcl, Scl;
+    if (relax) {
+        Tcl = ldv_glb(s.Tclim + nk0);
+        Scl = ldv_glb(s.Sclim + nk0);
+    }
+    const double arn = m.area[n];
+    LV<R> oTo, oSo, oT, oS;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const int k = k0 + i;
+        double tvb = 0.0, svb = 0.0;
+        if (i < nl) hi_vert2_col<true>(om, Tc, Sc, Wc, K, k + 2, a, TmT, TmS, tvb, svb);
+        const double tab = To.x[i], sab = So.x[i];
+        const double wdiff = Wc[k] - Wc[k < K - 1 ? k + 1 : k];
+        const double jc = a * m.dsig[k < L ? k : L - 1];
+        const double aj = arn * jc;
+        R rti = rt[i], rsi = rs[i];
+        rti = rti / aj - tab * rc[i] / aj + tab * wdiff / jc + (tvt - tvb) / jc;
+        rsi = rsi / aj - sab * rc[i] / aj + sab * wdiff / jc + (svt - svb) / jc;
+        const double tdm = TF0.x[i] - Ta, sdm = SF0.x[i] - Sa;
+        if (relax) {  // climatology relaxation on the current de-meaned tracer
+            if (inflow) {
+                rti = rti + relax2clim_ac * (Tcl.x[i] - (tdm + Ta));
+                rsi = rsi + relax2clim_ac * (Scl.x[i] - (sdm + Sa));
+            } else {
+                rti = clim_relax * (Tcl.x[i] - (tdm + Ta));
+                rsi = clim_relax * (Scl.x[i] - (sdm + Sa));
+            }
+        }
+        oTo.x[i] = tdm + Ta;
+        oSo.x[i] = sdm + Sa;
+        oT.x[i] = tdm + rti * dt + Ta;
+        oS.x[i] = sdm + rsi * dt + Sa;
+        tvt = tvb;
+        svt = svb;
+    }
+    stv_n(s.T_old + nk0, oTo, nl);
+    stv_n(s.S_old + nk0, oSo, nl);
+    stv_n(s.TF + nk0, oT, nl);
+    stv_n(s.SF + nk0, oS, nl);
+}
+
 // mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
 // H_advtr_min = 0 (fv_var.F90:49-51)
 template <typename R>
@@ -3644,6 +3902,7 @@ void instantiate_kernels() {
     (void)&k_tr_gradn_muscl<R>;
     (void)&k_tr_gradn_miura<R>;
     (void)&k_tr_muscl<R>;
+    (void)&k_tr_muscl_v<R>;
     (void)&k_tr_miura<R>;
     (void)&k_tr_vdiff<R>;
     (void)&k_tr_vdiff_atm<R>;
