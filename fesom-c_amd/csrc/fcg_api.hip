@@ -19,6 +19,11 @@ template <typename R> __global__ void k_d3end_prep(DevMesh, DevStateT<R>, double
 template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, double, double, const int *);
 template <typename R> __global__ void k_d3end_fin(DevMesh, DevStateT<R>, double, double, double, double, const int *);
 template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_velgrad3d_v(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_vertvel_div_v(DevMesh, DevStateT<R>, int);
+template <typename R> __global__ void k_tr_gradn_muscl_v(DevMesh, DevStateT<R>, int);
+template <typename R, bool P>
+__global__ void k_el2nodes_v(DevMesh, const R *, const R *, R *, R *, const double *, const double *, int, int *);
 template <typename R> __global__ void k_velrhs(DevMesh, DevStateT<R>, double, double, int, int, double);
 template <typename R> __global__ void k_velrhs_atm(DevMesh, DevStateT<R>, double, double, int, int, double);
 template <typename R> __global__ void k_velrhs_filt(DevMesh, DevStateT<R>, double, double, int, int, double, double);
@@ -150,6 +155,22 @@ inline size_t lds_muscl_v(const fcg_ctx *c, int T) {
     const int W = 16 / (int)c->esize, cpb = T / (c->m.K / W);
     return muscl_v_cols((size_t)c->cap_n2e_v[T == 256 ? 1 : 0], c->esize) + 7 * (size_t)cpb * c->m.K * c->esize +
            (size_t)c->m.K * sizeof(double);
+}
+// the other level-vector kernels run 256-thread blocks: cpb = 256/(K/W)
+// columns, LDS records for the block's CSR slots (node kernels) or columns
+inline int vec_cpb(const fcg_ctx *c) { return 256 / (c->m.K / (16 / (int)c->esize)); }
+inline size_t lds_el2n_v(const fcg_ctx *c) { return (size_t)c->cap_n2el_v[1] * sizeof(PulsRec); }
+inline size_t lds_gradn_v(const fcg_ctx *c) { return (size_t)c->cap_n2el_v[1] * sizeof(GradRec); }
+inline size_t lds_vdiv_v(const fcg_ctx *c) {
+    return (size_t)c->cap_n2e_v[1] * (c->esize == 8 ? sizeof(DivRec<double>) : sizeof(DivRec<float>));
+}
+inline size_t lds_velgrad_v(const fcg_ctx *c) { return (size_t)vec_cpb(c) * sizeof(VelGradRec); }
+// a level-vector variant runs when asked for (the default where nsigma is a
+// multiple of W) and its LDS fits
+static int levels_of(const fcg_ctx *ctx, const char *kern, int W);
+inline bool vec_on(const fcg_ctx *c, const char *kern, size_t lds) {
+    const int W = 16 / (int)c->esize;
+    return levels_of(c, kern, W) == W && lds <= (size_t)c->lds_limit;
 }
 template <typename R>
 inline size_t lds_vdiv(const fcg_ctx *c, int T = kFlatT) { return (size_t)c->cap_n2e_t[gather_slot(T)] * sizeof(DivRec<R>); }
@@ -361,9 +382,15 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s0) {
         if (sub == 1) {
             hipLaunchKernelGGL(fcgk::k_bt_puls<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.Dmin);
             const int be = block_of(ctx, "k_el2nodes");
-            hipLaunchKernelGGL((fcgk::k_el2nodes<R, true>), dim3(nflatblk(m.N, m.L, be)), dim3(be), lds_el2n(ctx, be), st,
-                               m, (const R *)s.U_n, (const R *)s.V_n, s.Unode_p, s.Vnode_p, (const double *)s.puls_ap,
-                               (const double *)s.U2);
+            if (vec_on(ctx, "k_el2nodes", lds_el2n_v(ctx)))
+                hipLaunchKernelGGL((fcgk::k_el2nodes_v<R, true>), dim3((m.N + vec_cpb(ctx) - 1) / vec_cpb(ctx)),
+                                   dim3(256), lds_el2n_v(ctx), st, m, (const R *)s.U_n, (const R *)s.V_n, s.Unode_p,
+                                   s.Vnode_p, (const double *)s.puls_ap, (const double *)s.U2, ctx->cap_n2el_v[1],
+                                   s.err + 3);
+            else
+                hipLaunchKernelGGL((fcgk::k_el2nodes<R, true>), dim3(nflatblk(m.N, m.L, be)), dim3(be), lds_el2n(ctx, be),
+                                   st, m, (const R *)s.U_n, (const R *)s.V_n, s.Unode_p, s.Vnode_p,
+                                   (const double *)s.puls_ap, (const double *)s.U2);
             hipLaunchKernelGGL(fcgk::k_bt_p1edge<R>, dim3(nflatblk(m.D, m.L)), dim3(kFlatT), lds_p1edge(m), st, m, s);
             hipLaunchKernelGGL(fcgk::k_bt_p1elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s,
                                p.filt_3D ? p.tau_inv_filt : 0.0);
@@ -1120,9 +1147,14 @@ static int vertical_mixing_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     // z0b_min/z0s_min/snul as fv_run.dat gives them
     // compute_el2nodes_3D(U_n, V_n, Unode, Vnode), src/fv_main.F90:2006
     const int be = block_of(ctx, "k_el2nodes");
-    LAUNCH_NAMED(ctx, "k_el2nodes", (fcgk::k_el2nodes<R, false>), nflatblk(ctx->m.N, ctx->m.L, be), be,
-                 lds_el2n(ctx, be), ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode, (const double *)nullptr,
-                 (const double *)nullptr);
+    if (vec_on(ctx, "k_el2nodes", lds_el2n_v(ctx)))
+        LAUNCH_NAMED(ctx, "k_el2nodes", (fcgk::k_el2nodes_v<R, false>), (ctx->m.N + vec_cpb(ctx) - 1) / vec_cpb(ctx),
+                     256, lds_el2n_v(ctx), ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode,
+                     (const double *)nullptr, (const double *)nullptr, ctx->cap_n2el_v[1], s.err + 3);
+    else
+        LAUNCH_NAMED(ctx, "k_el2nodes", (fcgk::k_el2nodes<R, false>), nflatblk(ctx->m.N, ctx->m.L, be), be,
+                     lds_el2n(ctx, be), ctx->m, (const R *)s.U_n, (const R *)s.V_n, s.Unode, s.Vnode,
+                     (const double *)nullptr, (const double *)nullptr);
     if (p.closure == FCG_CLOSURE_MLP) {
         // learned closure; columns outside its training box go back to d3_end
         if (!ctx->mlp) return fcg_fail(ctx, FCG_E_ARG, "closure = MLP but no network attached (fcg_set_mlp)");
@@ -1160,7 +1192,12 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
         ff = 1.0;
         ctx->lfirst = 0;
     }
-    LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec), ctx->m, s);
+    if (vec_on(ctx, "k_velgrad3d", lds_velgrad_v(ctx)))
+        LAUNCH_NAMED(ctx, "k_velgrad3d", fcgk::k_velgrad3d_v<R>, (ctx->m.E + vec_cpb(ctx) - 1) / vec_cpb(ctx), 256,
+                     lds_velgrad_v(ctx), ctx->m, s);
+    else
+        LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec),
+                  ctx->m, s);
     const int vt = block_of(ctx, "k_velrhs");
     constexpr int W = 16 / (int)sizeof(R);
     if (levels_of(ctx, "k_velrhs", W) == W &&
@@ -1293,7 +1330,11 @@ extern "C" int fcg_update_3d_vel(fcg_ctx *ctx) {
 template <typename R>
 static int vert_vel_sigma_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     const int bv = block_of(ctx, "k_vertvel_div");
-    LAUNCH_EX(ctx, k_vertvel_div<R>, nflatblk(ctx->m.N, ctx->m.L, bv), bv, lds_vdiv<R>(ctx, bv), ctx->m, s);
+    if (vec_on(ctx, "k_vertvel_div", lds_vdiv_v(ctx)))
+        LAUNCH_NAMED(ctx, "k_vertvel_div", fcgk::k_vertvel_div_v<R>, (ctx->m.N + vec_cpb(ctx) - 1) / vec_cpb(ctx), 256,
+                     lds_vdiv_v(ctx), ctx->m, s, ctx->cap_n2e_v[1]);
+    else
+        LAUNCH_EX(ctx, k_vertvel_div<R>, nflatblk(ctx->m.N, ctx->m.L, bv), bv, lds_vdiv<R>(ctx, bv), ctx->m, s);
     LAUNCH_EX(ctx, k_vertvel_scan<R>, nflatblk(ctx->m.N, ctx->m.L), kFlatT, flat_bytes(ctx->m.L, sizeof(R)), ctx->m, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
@@ -1324,7 +1365,11 @@ static int solve_tracers_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     } else if (p.tracer_adv == 2 && p.muscl_call) {
         LAUNCH1(ctx, k_tr_ab<R>, NL, m, s, NL);
         const int bg = block_of(ctx, "k_tr_gradn_muscl"), bm = block_of(ctx, "k_tr_muscl");
-        LAUNCH_EX(ctx, k_tr_gradn_muscl<R>, nflatblk(m.N, m.L, bg), bg, lds_gradn(ctx, bg), m, s);
+        if (vec_on(ctx, "k_tr_gradn_muscl", lds_gradn_v(ctx)))
+            LAUNCH_NAMED(ctx, "k_tr_gradn_muscl", fcgk::k_tr_gradn_muscl_v<R>, (m.N + vec_cpb(ctx) - 1) / vec_cpb(ctx),
+                         256, lds_gradn_v(ctx), m, s, ctx->cap_n2el_v[1]);
+        else
+            LAUNCH_EX(ctx, k_tr_gradn_muscl<R>, nflatblk(m.N, m.L, bg), bg, lds_gradn(ctx, bg), m, s);
         constexpr int W = 16 / (int)sizeof(R);
         const int tv = vec_threads(ctx, lds_muscl_v);
         if (levels_of(ctx, "k_tr_muscl", W) == W && lds_muscl_v(ctx, tv) <= (size_t)ctx->lds_limit) {
@@ -1504,18 +1549,29 @@ extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
         if (colon && !std::strcmp(colon, ":levels")) {
             const std::string k(kernel, colon - kernel);
             const int W = 16 / (int)ctx->esize;
-            static const char *vec[] = {"k_velrhs", "k_tr_muscl"};
+            static const char *vec[] = {"k_velrhs", "k_tr_muscl", "k_el2nodes", "k_vertvel_div", "k_tr_gradn_muscl",
+                                        "k_velgrad3d"};
             bool known = false;
             for (const char *v : vec) known = known || k == v;
             if (!known) return fcg_fail(ctx, FCG_E_ARG, "kernel without a level-vector variant: " + k);
             if (!(block == 1 || block == W)) return FCG_E_ARG;
             if (block == W && ctx->m.K % W)
                 return fcg_fail(ctx, FCG_E_ARG, k + ": " + std::to_string(W) + " levels per lane need nsigma to be a multiple of it");
-            const size_t lds = k == "k_velrhs" ? velrhs_v_bytes(ctx->m, ctx->esize, velrhs_v_threads(ctx->m, ctx->esize))
-                                               : lds_muscl_v(ctx, vec_threads(ctx, lds_muscl_v));
+            const size_t lds = k == "k_velrhs"   ? velrhs_v_bytes(ctx->m, ctx->esize, velrhs_v_threads(ctx->m, ctx->esize))
+                               : k == "k_tr_muscl" ? lds_muscl_v(ctx, vec_threads(ctx, lds_muscl_v))
+                               : k == "k_el2nodes" ? lds_el2n_v(ctx)
+                               : k == "k_vertvel_div" ? lds_vdiv_v(ctx)
+                               : k == "k_tr_gradn_muscl" ? lds_gradn_v(ctx)
+                                                          : lds_velgrad_v(ctx);
             if (block == W && lds > (size_t)ctx->lds_limit)
                 return fcg_fail(ctx, FCG_E_ARG, k + ": LDS of the level-vector variant exceeds the device limit");
             ctx->launch_block[kernel] = block;
+            if (k == "k_el2nodes" && ctx->bt_graph) {  // captured in the barotropic graph
+                FCG_CHECK(hipStreamSynchronize(ctx->stream));
+                FCG_CHECK(hipGraphExecDestroy(ctx->bt_graph));
+                ctx->bt_graph = nullptr;
+                ctx->bt_graph_ok = false;
+            }
             return FCG_OK;
         }
     }

@@ -3366,6 +3366,299 @@ __global__ __launch_bounds__(256) void k_tr_muscl_v(DevMesh m, DevStateT<R> s, d
     stv_n(s.SF + nk0, oS, nl);
 }
 
+// ---------------------------------------------------------------------------
+// Level-vector variants of the node-gather and column kernels below: each
+// lane owns W = LV<R>::W consecutive layers of its node or element column
+// (nsigma a multiple of W) and moves them with 16-byte loads and stores, so
+// a CSR slot's LDS record, address and neighbour load serve W layers.  Per
+// layer the arithmetic and the order of every sum are those of the one-level
+// kernel named in each comment: the results are identical bit for bit.  A
+// node-gather block with more CSR slots than its LDS capacity `cap` sets a
+// flag and does nothing.
+// ---------------------------------------------------------------------------
+
+// compute_el2nodes_3D / pulsation velocity as k_el2nodes
+template <typename R, bool P>
+__global__ __launch_bounds__(256) void k_el2nodes_v(DevMesh m, const R *__restrict__ Ue, const R *__restrict__ Ve,
+                                                    R *__restrict__ Un, R *__restrict__ Vn,
+                                                    const double *__restrict__ ap, const double *__restrict__ U2,
+                                                    int cap, int *err) {
+    constexpr int W = LV<R>::W;
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int KC = m.K, L = m.L;
+    const ColFlat f = col_flat(m.N, KC / W);
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2el_ptr, f);
+    if (jr.y - jr.x > cap) {
+        if (threadIdx.x == 0) atomicOr(err, 1);
+        return;
+    }
+    PulsRec *rec = (PulsRec *)lds_;
+    for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
+        PulsRec r;
+        const int e = m.n2el[jr.x + t];
+        r.elK = m.n2elK[jr.x + t];
+        r.ar = m.elem_area[e];
+        if (P) {
+            r.ap = ap[e];
+            r.u2 = U2[2 * e];
+            r.v2 = U2[2 * e + 1];
+        }
+        rec[t] = r;
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, k0 = f.k * W, nl = min(W, L - k0);
+    const int p0 = m.n2el_ptr[n] - jr.x, p1 = m.n2el_ptr[n + 1] - jr.x;
+    double sa = 0.0;
+    for (int j = p0; j < p1; ++j) sa = sa + rec[j].ar;
+    const double tvi = 1.0 / sa;
+    double su[W], sv[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) su[i] = sv[i] = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const PulsRec r = rec[j];
+        const LV<R> ue = ldv_glb(Ue + r.elK + k0), ve = ldv_glb(Ve + r.elK + k0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            R uei = ue.x[i], vei = ve.x[i];
+            if (P) {
+                const double un = uei, vn = vei;
+                uei = (R)(un - r.ap * r.u2);
+                vei = (R)(vn - r.ap * r.v2);
+            }
+            su[i] = su[i] + uei * r.ar;
+            sv[i] = sv[i] + vei * r.ar;
+        }
+    }
+    LV<R> ou, ov;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        ou.x[i] = su[i] * tvi;
+        ov.x[i] = sv[i] * tvi;
+    }
+    stv_n(Un + (size_t)n * KC + k0, ou, nl);
+    stv_n(Vn + (size_t)n * KC + k0, ov, nl);
+}
+
+// vert_vel_sigma's transport divergence as k_vertvel_div
+template <typename R>
+__global__ __launch_bounds__(256) void k_vertvel_div_v(DevMesh m, DevStateT<R> s, int cap) {
+    constexpr int W = LV<R>::W;
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int KC = m.K, L = m.L;
+    const ColFlat f = col_flat(m.N, KC / W);
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2e_ptr, f);
+    if (jr.y - jr.x > cap) {
+        if (threadIdx.x == 0) atomicOr(s.err + 3, 1);
+        return;
+    }
+    DivRec<R> *rec = (DivRec<R> *)lds_;
+    {
+        const EdgeGeo<R> *eg = (const EdgeGeo<R> *)m.egeo;
+        for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
+            const int cc = m.n2e[jr.x + t];
+            const EdgeGeo<R> g = eg[cc >= 0 ? cc : ~cc];
+            DivRec<R> r;
+            r.e1K = g.e1K;
+            r.e2K = g.e2K;
+            r.pos = cc >= 0;
+            r.cx = g.cx;
+            r.cy = g.cy;
+            r.cz = g.cz;
+            r.cw = g.cw;
+            r.vf1 = (R)s.V_filt[g.e1];
+            r.uf1 = (R)s.U_filt[g.e1];
+            r.vf2 = g.e2K >= 0 ? (R)s.V_filt[g.e2] : R(0);
+            r.uf2 = g.e2K >= 0 ? (R)s.U_filt[g.e2] : R(0);
+            rec[t] = r;
+        }
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, k0 = f.k * W, nl = min(W, L - k0);
+    const int p0 = m.n2e_ptr[n] - jr.x, p1 = m.n2e_ptr[n + 1] - jr.x;
+    LV<R> c;  // storage type: FP32 transport fluxes in the mixed-precision path
+#pragma unroll
+    for (int i = 0; i < W; ++i) c.x[i] = R(0);
+    for (int j = p0; j < p1; ++j) {
+        const DivRec<R> g = rec[j];
+        const LV<R> J1 = ldv_glb(s.Je + g.e1K + k0), V1 = ldv_glb(s.V_nf + g.e1K + k0), U1 = ldv_glb(s.U_nf + g.e1K + k0);
+        const bool two = g.e2K >= 0;
+        LV<R> J2, V2, U2;
+        if (two) {
+            J2 = ldv_glb(s.Je + g.e2K + k0);
+            V2 = ldv_glb(s.V_nf + g.e2K + k0);
+            U2 = ldv_glb(s.U_nf + g.e2K + k0);
+        }
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            R fl = J1.x[i] * ((V1.x[i] - g.vf1) * g.cx - (U1.x[i] - g.uf1) * g.cy);
+            if (two) fl = fl + J2.x[i] * (-(V2.x[i] - g.vf2) * g.cz + (U2.x[i] - g.uf2) * g.cw);
+            c.x[i] = g.pos ? c.x[i] + fl : c.x[i] - fl;
+        }
+    }
+    stv_n(s.W + (size_t)n * KC + k0, c, nl);
+}
+
+// node-averaged gradients of the AB-interpolated T and S as k_tr_gradn_muscl
+template <typename R>
+__global__ __launch_bounds__(256) void k_tr_gradn_muscl_v(DevMesh m, DevStateT<R> s, int cap) {
+    constexpr int W = LV<R>::W;
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int KC = m.K, L = m.L;
+    const ColFlat f = col_flat(m.N, KC / W);
+    if (f.nc <= 0) return;
+    const int2 jr = csr_block(m.n2el_ptr, f);
+    if (jr.y - jr.x > cap) {
+        if (threadIdx.x == 0) atomicOr(s.err + 3, 1);
+        return;
+    }
+    GradRec *rec = (GradRec *)lds_;
+    for (int t = threadIdx.x; t < jr.y - jr.x; t += blockDim.x) {
+        const int el = m.n2el[jr.x + t];
+        const int4 q = m.enK[el];
+        const double4 gx = m.gsx[el], gy = m.gsy[el];
+        GradRec r;
+        r.qK[0] = q.x;
+        r.qK[1] = q.y;
+        r.qK[2] = q.z;
+        r.qK[3] = q.w;
+        r.ea = m.elem_area[el];
+        r.gx[0] = gx.x;
+        r.gx[1] = gx.y;
+        r.gx[2] = gx.z;
+        r.gx[3] = gx.w;
+        r.gy[0] = gy.x;
+        r.gy[1] = gy.y;
+        r.gy[2] = gy.z;
+        r.gy[3] = gy.w;
+        rec[t] = r;
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int n = f.c0 + f.c, k0 = f.k * W, nl = min(W, L - k0);
+    const int p0 = m.n2el_ptr[n] - jr.x, p1 = m.n2el_ptr[n + 1] - jr.x;
+    double tvol = 0.0, tx[W], ty[W], sx[W], sy[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) tx[i] = ty[i] = sx[i] = sy[i] = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const GradRec &r = rec[j];
+        const double ea = r.ea;
+        const LV<R> a0 = ldv_glb(s.abT + r.qK[0] + k0), a1 = ldv_glb(s.abT + r.qK[1] + k0);
+        const LV<R> a2 = ldv_glb(s.abT + r.qK[2] + k0), a3 = ldv_glb(s.abT + r.qK[3] + k0);
+        const LV<R> b0 = ldv_glb(s.abS + r.qK[0] + k0), b1 = ldv_glb(s.abS + r.qK[1] + k0);
+        const LV<R> b2 = ldv_glb(s.abS + r.qK[2] + k0), b3 = ldv_glb(s.abS + r.qK[3] + k0);
+        tvol = tvol + ea;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const double t0 = a0.x[i] - 0.0, t1 = a1.x[i] - 0.0, t2 = a2.x[i] - 0.0, t3 = a3.x[i] - 0.0;
+            const double s0 = b0.x[i] - 0.0, s1 = b1.x[i] - 0.0, s2 = b2.x[i] - 0.0, s3 = b3.x[i] - 0.0;
+            tx[i] = tx[i] + sum4a(r.gx, t0, t1, t2, t3) * ea;
+            ty[i] = ty[i] + sum4a(r.gy, t0, t1, t2, t3) * ea;
+            sx[i] = sx[i] + sum4a(r.gx, s0, s1, s2, s3) * ea;
+            sy[i] = sy[i] + sum4a(r.gy, s0, s1, s2, s3) * ea;
+        }
+    }
+    LV<R> o0, o1, o2, o3;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        o0.x[i] = tx[i] / tvol;
+        o1.x[i] = ty[i] / tvol;
+        o2.x[i] = sx[i] / tvol;
+        o3.x[i] = sy[i] / tvol;
+    }
+    const size_t nk0 = (size_t)n * KC + k0;
+    stv_n(s.gTx + nk0, o0, nl);
+    stv_n(s.gTy + nk0, o1, nl);
+    stv_n(s.gSx + nk0, o2, nl);
+    stv_n(s.gSy + nk0, o3, nl);
+}
+
+// vel_gradients (3-D, free slip) as k_velgrad3d
+template <typename R>
+__global__ __launch_bounds__(256) void k_velgrad3d_v(DevMesh m, DevStateT<R> s) {
+    constexpr int W = LV<R>::W;
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int KC = m.K, L = m.L;
+    const ColFlat f = col_flat(m.E, KC / W);
+    VelGradRec *rec = (VelGradRec *)lds_;
+    if (f.nc <= 0) return;
+    if ((int)threadIdx.x < f.nc) {
+        const int e = f.c0 + threadIdx.x;
+        const int4 q = m.en[e];
+        const int4 nb = m.elnb[e];
+        const int4 ed = m.eledges[e];
+        const double4 gx = m.gvx[e], gy = m.gvy[e];
+        const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
+        const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
+        const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
+        const double ec = m.elem_cos[e];
+        VelGradRec r;
+        r.jend = (q.x == q.w) ? 3 : 4;
+        for (int j = 0; j < 4; ++j) {
+            r.gx[j] = gxs[j];
+            r.gy[j] = gys[j];
+            r.nbK[j] = nbs[j] >= 0 ? nbs[j] * KC : -1;
+            r.zc0[j] = 0.0;
+            r.zc1[j] = 0.0;
+            if (j < r.jend && nbs[j] < 0) {
+                const double2 dd = m.edxdy[eds[j]];
+                r.zc0[j] = dd.x * ec;
+                r.zc1[j] = dd.y;
+            }
+        }
+        rec[threadIdx.x] = r;
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int e = f.c0 + f.c, k0 = f.k * W, nl = min(W, L - k0);
+    const size_t ek0 = (size_t)e * KC + k0;
+    const VelGradRec &r = rec[f.c];
+    const LV<R> u = ldv_glb(s.U_n + ek0), v = ldv_glb(s.V_n + ek0);
+    double ux[W], uy[W], vx[W], vy[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) ux[i] = uy[i] = vx[i] = vy[i] = 0.0;
+    for (int j = 0; j < r.jend; ++j) {
+        if (r.nbK[j] >= 0) {
+            const LV<R> un = ldv_glb(s.U_n + r.nbK[j] + k0), vn = ldv_glb(s.V_n + r.nbK[j] + k0);
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const double du = (double)un.x[i] - (double)u.x[i];
+                const double dv = (double)vn.x[i] - (double)v.x[i];
+                ux[i] = ux[i] + r.gx[j] * du;
+                uy[i] = uy[i] + r.gy[j] * du;
+                vx[i] = vx[i] + r.gx[j] * dv;
+                vy[i] = vy[i] + r.gy[j] * dv;
+            }
+        } else {
+            const double zc0 = r.zc0[j], zc1 = r.zc1[j];
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const double uu = u.x[i], vv = v.x[i];
+                double un = -2 * (uu * zc1 - vv * zc0) / (zc0 * zc0 + zc1 * zc1);
+                ux[i] = ux[i] + r.gx[j] * un * zc1;
+                uy[i] = uy[i] + r.gy[j] * un * zc1;
+                vx[i] = vx[i] - r.gx[j] * un * zc0;
+                vy[i] = vy[i] - r.gy[j] * un * zc0;
+            }
+        }
+    }
+    LV<R> o0, o1, o2, o3;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        o0.x[i] = ux[i];
+        o1.x[i] = uy[i];
+        o2.x[i] = vx[i];
+        o3.x[i] = vy[i];
+    }
+    stv_n(s.gux + ek0, o0, nl);
+    stv_n(s.guy + ek0, o1, nl);
+    stv_n(s.gvx + ek0, o2, nl);
+    stv_n(s.gvy + ek0, o3, nl);
+}
+
 // mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
 // H_advtr_min = 0 (fv_var.F90:49-51)
 template <typename R>
@@ -3903,6 +4196,11 @@ void instantiate_kernels() {
     (void)&k_tr_gradn_miura<R>;
     (void)&k_tr_muscl<R>;
     (void)&k_tr_muscl_v<R>;
+    (void)&k_el2nodes_v<R, false>;
+    (void)&k_el2nodes_v<R, true>;
+    (void)&k_vertvel_div_v<R>;
+    (void)&k_tr_gradn_muscl_v<R>;
+    (void)&k_velgrad3d_v<R>;
     (void)&k_tr_miura<R>;
     (void)&k_tr_vdiff<R>;
     (void)&k_tr_vdiff_atm<R>;

@@ -87,15 +87,16 @@ def test_level_vectors_do_not_change_results(precision, nsigma):
     the column's unused last slot is left as it was."""
     a, b = _model(precision, nsigma), _model(precision, nsigma)
     W = 4 if precision else 2
-    vec = ["k_velrhs", "k_tr_muscl"]
+    vec = ["k_velrhs", "k_tr_muscl", "k_el2nodes", "k_vertvel_div", "k_tr_gradn_muscl", "k_velgrad3d"]
     for k in vec:
         assert a.get_launch(k + ":levels") == W   # the default where nsigma is a multiple of W
         b.set_launch(k + ":levels", 1)
         assert b.get_launch(k + ":levels") == 1
     a.step(4)
     b.step(4)
-    for f in ("eta_n", "U_n", "V_n", "U_rhsAB", "V_rhsAB", "TF", "SF", "Wvel", "Kv", "snu"):
+    for f in ("eta_n", "U_n", "V_n", "U_rhsAB", "V_rhsAB", "TF", "SF", "Wvel", "Kv", "snu", "Unode", "Vnode"):
         assert np.array_equal(a.get(f), b.get(f)), f
+    assert a.diagnostics()["status"] == 0 and b.diagnostics()["status"] == 0
     a.close()
     b.close()
 
