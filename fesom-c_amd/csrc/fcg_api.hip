@@ -20,6 +20,8 @@ template <typename R> __global__ void k_d3end(DevMesh, DevStateT<R>, double, dou
 template <typename R> __global__ void k_d3end_fin(DevMesh, DevStateT<R>, double, double, double, double, const int *);
 template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_velgrad3d_v(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_sbc_node(DevMesh, DevStateT<R>, double, double, int);
+template <typename R> __global__ void k_sbc_elem(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_vertvel_div_v(DevMesh, DevStateT<R>, int);
 template <typename R> __global__ void k_tr_gradn_muscl_v(DevMesh, DevStateT<R>, int);
 template <typename R, bool P>
@@ -1083,6 +1085,50 @@ static int transfer(fcg_ctx *ctx, const char *name, double *host, long count, bo
             for (long k = 0; k < f.levels; ++k) host[(size_t)perm[i] * f.levels + k] = tmp[dev_at(k, i)];
     }
     return FCG_OK;
+}
+
+// ---- surface boundary conditions from atmospheric fields (key_atm) -----
+extern "C" int fcg_sbc_set_coef(fcg_ctx *ctx, const double *coef_a, const double *coef_b) {
+    if (!ctx || !coef_a || !coef_b) return FCG_E_ARG;
+    if (!ctx->p.key_atm) return fcg_fail(ctx, FCG_E_ARG, "fcg_sbc_set_coef: the context runs without key_atm");
+    const int N = ctx->m.N;
+    if (!ctx->s.sbc_coef) {  // allocated with the first coefficients
+        FCG_CHECK(hipSetDevice(ctx->device));
+        int r = dalloc(ctx, &ctx->s.sbc_coef, 16 * (size_t)N);
+        if (r) return r;
+        r = dalloc(ctx, &ctx->s.tau_node, 2 * (size_t)N);
+        if (r) return r;
+    }
+    // (8, nod2D) per array in mesh node order -> device order, a then b per node
+    std::vector<double> h(16 * (size_t)N);
+    for (int i = 0; i < N; ++i) {
+        const size_t r = (size_t)ctx->pn[i];
+        for (int f = 0; f < 8; ++f) {
+            h[16 * (size_t)i + f] = coef_a[8 * r + f];
+            h[16 * (size_t)i + 8 + f] = coef_b[8 * r + f];
+        }
+    }
+    FCG_CHECK(hipMemcpyAsync(ctx->s.sbc_coef, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    return FCG_OK;
+}
+
+template <typename R>
+static int sbc_do_impl(fcg_ctx *ctx, const DevStateT<R> &s, double prec_coef, int net_flux) {
+    LAUNCH(ctx, k_sbc_node<R>, nblk(ctx->m.N), ctx->m, s, ctx->time_jd, prec_coef, net_flux);
+    LAUNCH(ctx, k_sbc_elem<R>, nblk(ctx->m.E), ctx->m, s);
+    FCG_CHECK(hipGetLastError());
+    return FCG_OK;
+}
+
+extern "C" int fcg_sbc_do(fcg_ctx *ctx, double nm_prec_coef, int nm_net_flux) {
+    if (!ctx) return FCG_E_ARG;
+    if (!ctx->p.key_atm || !ctx->s.sbc_coef)
+        return fcg_fail(ctx, FCG_E_ARG, "fcg_sbc_do: no atmospheric coefficients (key_atm and fcg_sbc_set_coef)");
+    if (!(nm_prec_coef != 0.0) || (nm_net_flux != 0 && nm_net_flux != 1))
+        return fcg_fail(ctx, FCG_E_ARG, "fcg_sbc_do: nm_prec_coef must be nonzero and nm_net_flux 0 or 1");
+    return ctx->mixed ? sbc_do_impl(ctx, retype<float>(ctx->s), nm_prec_coef, nm_net_flux)
+                      : sbc_do_impl(ctx, ctx->s, nm_prec_coef, nm_net_flux);
 }
 
 extern "C" int fcg_set_state(fcg_ctx *ctx, const char *name, const double *host, long count) {

@@ -150,6 +150,8 @@ struct DevStateT {
     double *fl_p1 = nullptr;                                   // 4D: layer sums of the P1 3-D-to-2-D advection
     double *puls_ap = nullptr;                                 // E: a of U_puls = U_n - a*U_n_2D (first substep)
     double *qsr = nullptr, *qns = nullptr, *emp = nullptr, *mslp = nullptr;  // N: surface forcing (key_atm)
+    double *sbc_coef = nullptr;    // 16N: per node the 8 fields' time coefficients a, then b (key_atm, bulk)
+    double *tau_node = nullptr;    // 2N: bulk wind stress at the nodes (key_atm, bulk)
     double *runoff = nullptr;                                  // N: river discharge [m3/s], 0 off the mouths (key_rivers)
     double *rho_c = nullptr;                                   // L*N (n*L + k): density anomaly of pressure (key_atm)
     double *node2d = nullptr;                                  // 4N: etaAB, eta_g, eta_n_1, eta_n per node (substep)

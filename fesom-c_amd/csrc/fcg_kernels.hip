@@ -3659,6 +3659,150 @@ __global__ __launch_bounds__(256) void k_velgrad3d_v(DevMesh m, DevStateT<R> s) 
     stv_n(s.gvy + ek0, o3, nl);
 }
 
+// ---------------------------------------------------------------------------
+// surface boundary conditions from atmospheric fields (key_atm with the
+// CORE bulk formulas, src/fv_sbc.F90:368-867)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sbc_sign(double a, double b) { return copysign(fabs(a), b); }  // Fortran SIGN
+
+// L&Y stability functions (src/fv_sbc.F90:616-660)
+__device__ __forceinline__ double psi_h(double zta) {
+    double X2 = sqrt(fabs(1.0 - 16.0 * zta));
+    X2 = X2 > 1.0 ? X2 : 1.0;
+    const double stabit = 0.5 + sbc_sign(0.5, zta);
+    return -5.0 * zta * stabit + (1.0 - stabit) * (2.0 * log((1.0 + X2) / 2.0));
+}
+__device__ __forceinline__ double psi_m(double zta) {
+    const double pi = 3.141592653589793;
+    double X2 = sqrt(fabs(1.0 - 16.0 * zta));
+    X2 = X2 > 1.0 ? X2 : 1.0;
+    const double X = sqrt(X2);
+    const double stabit = 0.5 + sbc_sign(0.5, zta);
+    return -5.0 * zta * stabit
+           + (1.0 - stabit) * (2.0 * log((1.0 + X) / 2.0) + log((1.0 + X2) / 2.0) - 2.0 * atan(X) + pi / 2.0);
+}
+
+// core_coeff_2z (src/fv_sbc.F90:485-614): transfer coefficients of
+// momentum, heat and evaporation for T/q at zt and wind at zu, after five
+// Monin-Obukhov iterations; the air temperature and humidity shifted to zu
+__device__ void core_coeff_2z(double zt, double zu, double sst, double T_zt, double q_sat, double q_zt, double dU,
+                              double &Cd, double &Ch, double &Ce, double &T_zu, double &q_zu) {
+    const double grav = 9.8, kappa = 0.4;
+    const double dU10 = dU > 0.5 ? dU : 0.5;
+    double dT = T_zt - sst, dq;
+    double stab = 0.5 + sbc_sign(0.5, dT);
+    double Cd_n10 = 1E-3 * (2.7 / dU10 + 0.142 + dU10 / 13.09);
+    double sqrt_Cd_n10 = sqrt(Cd_n10);
+    double Ce_n10 = 1E-3 * (34.6 * sqrt_Cd_n10);
+    double Ch_n10 = 1E-3 * sqrt_Cd_n10 * (18.0 * stab + 32.7 * (1.0 - stab));
+    Cd = Cd_n10;
+    Ce = Ce_n10;
+    Ch = Ch_n10;
+    double sqrt_Cd = sqrt(Cd);
+    T_zu = T_zt;
+    q_zu = q_zt;
+    const double lzu = log(zu / 10.0), lzt = log(zt / zu);
+    for (int it = 0; it < 5; ++it) {
+        dT = T_zu - sst;
+        dq = q_zu - q_sat;
+        const double T_vpot = T_zu * (1.0 + 0.608 * q_zu);
+        const double U_star = sqrt_Cd * dU10;
+        const double T_star = Ch / sqrt_Cd * dT;
+        const double q_star = Ce / sqrt_Cd * dq;
+        const double Lmo = (U_star * U_star) / (kappa * grav / T_vpot * (T_star * (1.0 + 0.608 * q_zu) + 0.608 * T_zu * q_star));
+        double zeta_u = zu / Lmo;
+        zeta_u = sbc_sign(fabs(zeta_u) < 10.0 ? fabs(zeta_u) : 10.0, zeta_u);
+        double zeta_t = zt / Lmo;
+        zeta_t = sbc_sign(fabs(zeta_t) < 10.0 ? fabs(zeta_t) : 10.0, zeta_t);
+        const double zpsi_hu = psi_h(zeta_u), zpsi_ht = psi_h(zeta_t), zpsi_m = psi_m(zeta_u);
+        double U_n10 = dU10 / (1.0 + sqrt_Cd_n10 / kappa * (lzu - zpsi_m));
+        U_n10 = 0.25 > U_n10 ? 0.25 : U_n10;
+        T_zu = T_zt - T_star / kappa * (lzt + zpsi_hu - zpsi_ht);
+        q_zu = q_zt - q_star / kappa * (lzt + zpsi_hu - zpsi_ht);
+        stab = 0.5 + sbc_sign(0.5, q_zu);
+        q_zu = stab * q_zu;
+        Cd_n10 = 1E-3 * (2.7 / U_n10 + 0.142 + U_n10 / 13.09);
+        sqrt_Cd_n10 = sqrt(Cd_n10);
+        Ce_n10 = 1E-3 * (34.6 * sqrt_Cd_n10);
+        stab = 0.5 + sbc_sign(0.5, zeta_u);
+        Ch_n10 = 1E-3 * sqrt_Cd_n10 * (18.0 * stab + 32.7 * (1.0 - stab));
+        double xct = 1.0 + sqrt_Cd_n10 / kappa * (lzu - zpsi_m);
+        Cd = Cd_n10 / (xct * xct);
+        sqrt_Cd = sqrt(Cd);
+        const double xlogt = lzu - zpsi_hu;
+        xct = 1.0 + Ch_n10 * xlogt / kappa / sqrt_Cd_n10;
+        Ch = Ch_n10 * sqrt_Cd / sqrt_Cd_n10 / xct;
+        xct = 1.0 + Ce_n10 * xlogt / kappa / sqrt_Cd_n10;
+        Ce = Ce_n10 * sqrt_Cd / sqrt_Cd_n10 / xct;
+    }
+}
+
+// sbc_do per node: the fields at the clock from their time coefficients
+// (data_timeinterp, src/fv_sbc.F90:843-867), then apply_atm_fluxes (368-470)
+// for task type 3 without sea ice: the wind relative to the top layer's node
+// velocity, the surface temperature of the top layer, stress at the node,
+// evaporation, sensible, latent and longwave heat, net shortwave, pressure.
+// Un-suffixed literals of the reference (air density, albedo, emissivity)
+// are default REAL (single precision).
+template <typename R>
+__global__ __launch_bounds__(256) void k_sbc_node(DevMesh m, DevStateT<R> s, double now_date, double prec_coef,
+                                                  int net_flux) {
+    const int n = TID();
+    if (n >= m.N) return;
+    const int KC = m.K;
+    const double rhoa = (double)1.3f, cpa = 1005.0, Lv = 2.5e6, Stef = 5.67e-8, albo = (double)0.1f;
+    const double emiss_wat = (double)0.97f;
+    const double4 *c = (const double4 *)(s.sbc_coef + 16 * (size_t)n);
+    const double4 a0 = c[0], a1 = c[1], b0 = c[2], b1 = c[3];
+    const double xw = now_date * a0.x + b0.x, yw = now_date * a0.y + b0.y, humi = now_date * a0.z + b0.z;
+    const double qsr_in = now_date * a0.w + b0.w, qlw = now_date * a1.x + b1.x, tair = now_date * a1.y + b1.y;
+    const double prec = now_date * a1.z + b1.z, mslp = now_date * a1.w + b1.w;
+    s.windx[n] = xw;
+    s.windy[n] = yw;
+    const double wdx = xw - (double)PL(s.Unode, 0, n, KC), wdy = yw - (double)PL(s.Vnode, 0, n, KC);
+    const double wndm = sqrt(wdx * wdx + wdy * wdy);
+    const double tsur = (double)PL(s.TF, 0, n, KC);
+    const double zst = tsur + 273.15;
+    const double q_sat = 0.98 * 640380.0 / rhoa * exp(-5107.4 / zst);
+    double Cd, Ch, Ce, t_zu, q_zu;
+    core_coeff_2z(2.0, 10.0, zst, tair, q_sat, humi, wndm, Cd, Ch, Ce, t_zu, q_zu);
+    const double rtmp = rhoa * wndm * Cd;
+    s.tau_node[2 * (size_t)n] = rtmp * wdx;
+    s.tau_node[2 * (size_t)n + 1] = rtmp * wdy;
+    double zevap = rhoa * Ce * (q_sat - q_zu) * wndm;
+    zevap = 0.0 > zevap ? 0.0 : zevap;
+    const double zqsb = rhoa * cpa * Ch * (zst - t_zu) * wndm;
+    const double zqlw = net_flux == 0 ? (qlw - emiss_wat * Stef * zst * zst * zst * zst) : qlw;
+    const double zqla = Lv * zevap;
+    double qns = zqlw - zqsb - zqla;
+    if (tsur < -2.0 && qns < 0.0) qns = 0.0;
+    s.qns[n] = qns;
+    s.emp[n] = zevap - prec / prec_coef;
+    s.qsr[n] = (1.0 - albo) * qsr_in;
+    s.mslp[n] = mslp;
+}
+
+// the node stress on the elements: sum(w_cv(1:4)*taux_node(elem2D_nodes))
+// in order (src/fv_sbc.F90:475-479)
+template <typename R>
+__global__ __launch_bounds__(256) void k_sbc_elem(DevMesh m, DevStateT<R> s) {
+    const int e = TID();
+    if (e >= m.E) return;
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e];
+    const double2 *t = (const double2 *)s.tau_node;
+    const double2 t0 = t[q.x], t1 = t[q.y], t2 = t[q.z], t3 = t[q.w];
+    double sx = w.x * t0.x, sy = w.x * t0.y;
+    sx = sx + w.y * t1.x;
+    sy = sy + w.y * t1.y;
+    sx = sx + w.z * t2.x;
+    sy = sy + w.z * t2.y;
+    sx = sx + w.w * t3.x;
+    sy = sy + w.w * t3.y;
+    s.taux[e] = sx;
+    s.tauy[e] = sy;
+}
+
 // mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
 // H_advtr_min = 0 (fv_var.F90:49-51)
 template <typename R>
@@ -4201,6 +4345,8 @@ void instantiate_kernels() {
     (void)&k_vertvel_div_v<R>;
     (void)&k_tr_gradn_muscl_v<R>;
     (void)&k_velgrad3d_v<R>;
+    (void)&k_sbc_node<R>;
+    (void)&k_sbc_elem<R>;
     (void)&k_tr_miura<R>;
     (void)&k_tr_vdiff<R>;
     (void)&k_tr_vdiff_atm<R>;

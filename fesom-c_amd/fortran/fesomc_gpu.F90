@@ -90,6 +90,19 @@ module fesomc_gpu
        type(c_ptr), value :: ctx
      end subroutine fcg_destroy
 
+     integer(c_int) function fcg_sbc_set_coef(ctx, coef_a, coef_b) bind(C, name="fcg_sbc_set_coef")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: coef_a(*), coef_b(*)
+     end function fcg_sbc_set_coef
+
+     integer(c_int) function fcg_sbc_do(ctx, nm_prec_coef, nm_net_flux) bind(C, name="fcg_sbc_do")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: nm_prec_coef
+       integer(c_int), value :: nm_net_flux
+     end function fcg_sbc_do
+
      integer(c_int) function fcg_set_state(ctx, name, host, count) bind(C, name="fcg_set_state")
        import :: c_int, c_ptr, c_char, c_double, c_long
        type(c_ptr), value :: ctx

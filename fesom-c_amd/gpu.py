@@ -33,7 +33,7 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_mlp_destroy", "fcg_mlp_get_params", "fcg_mlp_set_params", "fcg_mlp_set_norm", "fcg_mlp_forward",
            "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect",
            "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member", "fcg_set_launch",
-           "fcg_get_launch", "fcg_hot_start", "fcg_mlp_gradient_static"]
+           "fcg_get_launch", "fcg_hot_start", "fcg_mlp_gradient_static", "fcg_sbc_set_coef", "fcg_sbc_do"]
 
 MLP_FEATURES, MLP_PARAMS = 5, 39312
 
@@ -118,6 +118,10 @@ def lib():
             getattr(L, n).argtypes = [P]
         L.fcg_run.restype = C.c_int
         L.fcg_run.argtypes = [P, C.c_int]
+        L.fcg_sbc_set_coef.restype = C.c_int
+        L.fcg_sbc_set_coef.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.fcg_sbc_do.restype = C.c_int
+        L.fcg_sbc_do.argtypes = [P, C.c_double, C.c_int]
         L.fcg_diagnostics.restype = C.c_int
         L.fcg_diagnostics.argtypes = [P, C.POINTER(Diag)]
         L.fcg_last_error.restype = C.c_int
@@ -228,6 +232,7 @@ class Model:
     """One FESOM-C model instance resident on the GPU."""
 
     rivers = None   # river table of the host (key_rivers), see __init__
+    sbc = None      # atmospheric forcing of the host (key_atm bulk formulas), see attach_sbc
 
     def __init__(self, cfg: RunConfig, mesh: meshio.RawMesh, precision=FCG_FP64, closure=FCG_CLOSURE_D3END,
                  rivers=None):
@@ -366,16 +371,52 @@ class Model:
     def phase(self, name):
         self._ck(getattr(lib(), name)(self.ctx), name)
 
+    # -- surface boundary conditions from atmospheric fields (key_atm) ----
+    def sbc_set_coef(self, coef_a, coef_b):
+        """getcoeffld's coefficients (nod2D, 8) to the device (fcg_sbc_set_coef)."""
+        a = np.ascontiguousarray(coef_a, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(coef_b, dtype=np.float64).reshape(-1)
+        if a.size != 8 * self.N or b.size != 8 * self.N:
+            raise GpuError("sbc coefficients must be (nod2D, 8)")
+        self._ck(lib().fcg_sbc_set_coef(self.ctx, _dp(a), _dp(b)), "fcg_sbc_set_coef")
+
+    def sbc_do(self, nm_prec_coef=1.0, nm_net_flux=0):
+        """data_timeinterp + apply_atm_fluxes at the device clock (fcg_sbc_do)."""
+        self._ck(lib().fcg_sbc_do(self.ctx, float(nm_prec_coef), int(nm_net_flux)), "fcg_sbc_do")
+
+    def attach_sbc(self, forcing):
+        """Drive the bulk formulas from ``forcing`` (sbc.AtmForcing, or any
+        object with update(time_jd) -> (coef_a, coef_b) | None, prec_coef and
+        net_flux): as sbc_ini does at start-up (src/fv_main.F90:343), the
+        fields are formed now, and ``step`` calls sbc_do after each clock
+        advance (src/fv_main.F90:419)."""
+        if not self.cfg.l("key_atm"):
+            raise GpuError("attach_sbc: the run has key_atm = F")
+        self.sbc = forcing
+        t = self.clock()[1]
+        coef = forcing.update(t)
+        if coef is not None:
+            self.sbc_set_coef(*coef)
+        self.sbc_do(forcing.prec_coef, forcing.net_flux)
+
     def step(self, n=1):
-        if self.rivers is None:
+        if self.rivers is None and self.sbc is None:
             self._ck(lib().fcg_run(self.ctx, int(n)), "fcg_run")
             return
-        # the river table is interpolated on the host after each clock advance
+        # the river table and the atmospheric records are interpolated on the
+        # host after each clock advance, in the reference's order
+        # (src/fv_main.F90:400-439: rivers_do, sbc_do, oce_timestep)
         t = self.clock()[1]
         for _ in range(int(n)):
             self.phase("fcg_advance_clock")
             t = t + self.par.dt / 86400.0          # the device clock, src/fv_main.F90:400
-            self.set("runoff", self.rivers.do(t))
+            if self.rivers is not None:
+                self.set("runoff", self.rivers.do(t))
+            if self.sbc is not None:
+                coef = self.sbc.update(t)
+                if coef is not None:
+                    self.sbc_set_coef(*coef)
+                self.sbc_do(self.sbc.prec_coef, self.sbc.net_flux)
             for ph in self.PHASES[1:]:
                 self.phase(ph)
 

@@ -214,6 +214,26 @@ int fcg_set_clock(fcg_ctx *ctx, double time_jd, int lfirst, int n_dt);
  * The context must have been created with restart = 1 (no tidal ramp). */
 int fcg_hot_start(fcg_ctx *ctx, double time_jd);
 
+/* ---- surface boundary conditions from atmospheric fields (key_atm) -----
+ * The reference's sbc_do (src/fv_sbc.F90:1012-1046, called at
+ * src/fv_main.F90:419 after the clock advance) reads NetCDF records and
+ * interpolates them onto the nodes (getcoeffld, 665-841), then forms each
+ * field at the clock and applies the CORE bulk formulas.  The reading stays
+ * on the host; fcg_sbc_set_coef hands over getcoeffld's result: coef_a,
+ * coef_b (8, nod2D), Fortran layout, mesh node order, fields x-wind, y-wind,
+ * specific humidity, shortwave, longwave, 2 m air temperature [K],
+ * precipitation, sea-level pressure (the reference's i_xwind..i_mslp); the
+ * field at time t [days] is t*coef_a + coef_b.  Call again whenever the host
+ * reads the next records.  FCG_E_ARG without key_atm. */
+int fcg_sbc_set_coef(fcg_ctx *ctx, const double *coef_a, const double *coef_b);
+/* data_timeinterp + apply_atm_fluxes at the context's clock (src/fv_sbc.F90:
+ * 843-867, 368-482) for task type 3 without sea ice: windx/y, the element
+ * stress taux/y, qns, emp, qsr and mslp, from the wind relative to the top
+ * layer's node velocity and the top layer's temperature (core_coeff_2z, Large
+ * & Yeager 2004, 485-663).  nm_prec_coef: precipitation divisor (1 NCEP,
+ * 3600 CoastDat); nm_net_flux: 0 downward longwave, 1 net longwave. */
+int fcg_sbc_do(fcg_ctx *ctx, double nm_prec_coef, int nm_net_flux);
+
 /* ---- the hot path: one entry per call site of oce_timestep ------------
  * fcg_advance_clock:        time_jd += dt/86400; time = time_jd*86400 (src/fv_main.F90:400-401)
  * fcg_pressure:             call pressure                  (src/fv_main.F90:1996)

@@ -1351,7 +1351,9 @@ int orc_wad_mask_only(orc_t *o) {
  *  4 Miura departure point ahead of the face (+u*dt)     10 wind stress left out of the 2-D momentum
  *  5 Miura's missing *dt (fv_tracer.F90:1057) restored   11 sponge drag factor (1+ac) for (1-ac)
  *  6 OB tide phase with the wrong sign                   12 element wet/dry mask from the deepest node
- * 13 Miura edge x-length in metres (the reference's a = 1 workaround, fv_tracer.F90:1018-1026, undone) */
+ * 13 Miura edge x-length in metres (the reference's a = 1 workaround, fv_tracer.F90:1018-1026, undone)
+ * 14 psi_m with pi/4 for pi/2                            15 ocean albedo left out of the net shortwave
+ * 16 latent heat with the wrong sign in qns */
 int orc_set_mutation(int id) {
     g_mut = id;
     return 0;
@@ -2172,6 +2174,157 @@ static double swr_soloviev(double z) {
 
 /* sw_cp: src/fv_tracer.F90:1775-1910, the UNESCO 1983 heat capacity of sea
  * water (Fofonoff and Millard 1983), P in bar; literals without _WP are REAL(4) */
+/* ---- surface boundary conditions from atmospheric fields (key_atm, nm_sbc = 2)
+ * data_timeinterp (src/fv_sbc.F90:843-867), apply_atm_fluxes (368-482),
+ * core_coeff_2z (485-614) with psi_h / psi_m (616-660): the CORE bulk
+ * formulas of Large & Yeager (2004), for task type 3 (surface temperature
+ * TF(1,i), wind relative to the top layer's node velocity) without sea ice.
+ * Fortran SIGN(a, b) is copysign(|a|, b); un-suffixed literals (rhoa, albo,
+ * emiss_wat) are default REAL (single precision) as in the reference. */
+static double sbc_sign(double a, double b) { return copysign(fabs(a), b); }
+
+double orc_psi_h(double zta) {
+    double X2 = sqrt(fabs(1.0 - 16.0 * zta));
+    X2 = X2 > 1.0 ? X2 : 1.0;
+    const double stabit = 0.5 + sbc_sign(0.5, zta);
+    return -5.0 * zta * stabit + (1.0 - stabit) * (2.0 * log((1.0 + X2) / 2.0));
+}
+
+double orc_psi_m(double zta) {
+    const double pi = 3.141592653589793;
+    double X2 = sqrt(fabs(1.0 - 16.0 * zta));
+    X2 = X2 > 1.0 ? X2 : 1.0;
+    const double X = sqrt(X2);
+    const double stabit = 0.5 + sbc_sign(0.5, zta);
+    return -5.0 * zta * stabit
+           + (1.0 - stabit) * (2.0 * log((1.0 + X) / 2.0) + log((1.0 + X2) / 2.0) - 2.0 * atan(X)
+                               + (MUT(14) ? pi / 4.0 : pi / 2.0));
+}
+
+/* out: Cd, Ch, Ce, T_zu, q_zu (src/fv_sbc.F90:485-614, 5 iterations) */
+void orc_core_coeff_2z(double zt, double zu, double sst, double T_zt, double q_sat, double q_zt, double dU,
+                       double *out) {
+    const double grav = 9.8, kappa = 0.4;
+    const double dU10 = dU > 0.5 ? dU : 0.5;
+    double dT = T_zt - sst;
+    double dq = q_zt - q_sat;
+    (void)dq;
+    double stab = 0.5 + sbc_sign(0.5, dT);
+    double Cd_n10 = 1E-3 * (2.7 / dU10 + 0.142 + dU10 / 13.09);
+    double sqrt_Cd_n10 = sqrt(Cd_n10);
+    double Ce_n10 = 1E-3 * (34.6 * sqrt_Cd_n10);
+    double Ch_n10 = 1E-3 * sqrt_Cd_n10 * (18.0 * stab + 32.7 * (1.0 - stab));
+    double Cd = Cd_n10, Ce = Ce_n10, Ch = Ch_n10, sqrt_Cd = sqrt(Cd);
+    double T_zu = T_zt, q_zu = q_zt;
+    for (int it = 0; it < 5; it++) {
+        dT = T_zu - sst;
+        dq = q_zu - q_sat;
+        const double T_vpot = T_zu * (1.0 + 0.608 * q_zu);
+        const double U_star = sqrt_Cd * dU10;
+        const double T_star = Ch / sqrt_Cd * dT;
+        const double q_star = Ce / sqrt_Cd * dq;
+        const double L = (U_star * U_star) / (kappa * grav / T_vpot * (T_star * (1.0 + 0.608 * q_zu) + 0.608 * T_zu * q_star));
+        double zeta_u = zu / L;
+        zeta_u = sbc_sign(fabs(zeta_u) < 10.0 ? fabs(zeta_u) : 10.0, zeta_u);
+        double zeta_t = zt / L;
+        zeta_t = sbc_sign(fabs(zeta_t) < 10.0 ? fabs(zeta_t) : 10.0, zeta_t);
+        const double zpsi_hu = orc_psi_h(zeta_u), zpsi_ht = orc_psi_h(zeta_t), zpsi_m = orc_psi_m(zeta_u);
+        double U_n10 = dU10 / (1.0 + sqrt_Cd_n10 / kappa * (log(zu / 10.0) - zpsi_m));
+        U_n10 = 0.25 > U_n10 ? 0.25 : U_n10;
+        T_zu = T_zt - T_star / kappa * (log(zt / zu) + zpsi_hu - zpsi_ht);
+        q_zu = q_zt - q_star / kappa * (log(zt / zu) + zpsi_hu - zpsi_ht);
+        stab = 0.5 + sbc_sign(0.5, q_zu);
+        q_zu = stab * q_zu;
+        Cd_n10 = 1E-3 * (2.7 / U_n10 + 0.142 + U_n10 / 13.09);
+        sqrt_Cd_n10 = sqrt(Cd_n10);
+        Ce_n10 = 1E-3 * (34.6 * sqrt_Cd_n10);
+        stab = 0.5 + sbc_sign(0.5, zeta_u);
+        Ch_n10 = 1E-3 * sqrt_Cd_n10 * (18.0 * stab + 32.7 * (1.0 - stab));
+        double xct = 1.0 + sqrt_Cd_n10 / kappa * (log(zu / 10.0) - zpsi_m);
+        Cd = Cd_n10 / (xct * xct);
+        sqrt_Cd = sqrt(Cd);
+        const double xlogt = log(zu / 10.0) - zpsi_hu;
+        xct = 1.0 + Ch_n10 * xlogt / kappa / sqrt_Cd_n10;
+        Ch = Ch_n10 * sqrt_Cd / sqrt_Cd_n10 / xct;
+        xct = 1.0 + Ce_n10 * xlogt / kappa / sqrt_Cd_n10;
+        Ce = Ce_n10 * sqrt_Cd / sqrt_Cd_n10 / xct;
+    }
+    out[0] = Cd;
+    out[1] = Ch;
+    out[2] = Ce;
+    out[3] = T_zu;
+    out[4] = q_zu;
+}
+
+/* sbc_do's time interpolation and apply_atm_fluxes at the oracle's clock:
+ * coef_a/coef_b (8, nod2D) as getcoeffld leaves them (fields x-wind, y-wind,
+ * humidity, shortwave, longwave, air temperature, precipitation, sea-level
+ * pressure); writes windx/y, taux/y (elements), qns, emp, qsr, mslp.
+ * node_out (optional, 7 per node): Cd, Ch, Ce, T_zu, q_zu, taux_node, tauy_node. */
+int orc_apply_atm_fluxes(orc_t *o, const double *coef_a, const double *coef_b, double nm_prec_coef, int nm_net_flux,
+                         double *node_out) {
+    const int N = o->N, E = o->E, L = o->L;
+    const double rhoa = (double)1.3f, cpa = 1005.0, Lv = 2.5e6, Stef = 5.67e-8, albo = (double)0.1f;
+    const double emiss_wat = (double)0.97f;
+    double *tx = (double *)malloc(sizeof(double) * (size_t)N), *ty = (double *)malloc(sizeof(double) * (size_t)N);
+    if (!tx || !ty) {
+        free(tx);
+        free(ty);
+        return -1;
+    }
+    for (int i = 0; i < N; i++) {
+        double atm[8];
+        for (int f = 0; f < 8; f++) atm[f] = o->time_jd * coef_a[8 * (size_t)i + f] + coef_b[8 * (size_t)i + f];
+        o->windx[i] = atm[0];
+        o->windy[i] = atm[1];
+        const double wdx = atm[0] - o->Unode[(size_t)i * L], wdy = atm[1] - o->Vnode[(size_t)i * L];
+        const double wndm = sqrt(wdx * wdx + wdy * wdy);
+        const double zst = o->TF[(size_t)i * L] + 273.15;
+        const double q_sat = 0.98 * 640380.0 / rhoa * exp(-5107.4 / zst);
+        double c[5];
+        orc_core_coeff_2z(2.0, 10.0, zst, atm[5], q_sat, atm[2], wndm, c);
+        const double Cd = c[0], Ch = c[1], Ce = c[2], t_zu = c[3], q_zu = c[4];
+        const double rtmp = rhoa * wndm * Cd;
+        tx[i] = rtmp * wdx;
+        ty[i] = rtmp * wdy;
+        double zevap = rhoa * Ce * (q_sat - q_zu) * wndm;
+        zevap = 0.0 > zevap ? 0.0 : zevap;
+        const double zqsb = rhoa * cpa * Ch * (zst - t_zu) * wndm;
+        const double zqlw = nm_net_flux == 0 ? (atm[4] - emiss_wat * Stef * zst * zst * zst * zst) : atm[4];
+        const double zqla = Lv * zevap;
+        double qns = MUT(16) ? zqlw - zqsb + zqla : zqlw - zqsb - zqla;
+        if (o->TF[(size_t)i * L] < -2.0 && qns < 0.0) qns = 0.0;
+        o->qns[i] = qns;
+        o->emp[i] = zevap - atm[6] / nm_prec_coef;
+        o->qsr[i] = MUT(15) ? atm[3] : (1.0 - albo) * atm[3];
+        o->mslp[i] = atm[7];
+        if (node_out) {
+            double *q = node_out + 7 * (size_t)i;
+            q[0] = Cd;
+            q[1] = Ch;
+            q[2] = Ce;
+            q[3] = t_zu;
+            q[4] = q_zu;
+            q[5] = tx[i];
+            q[6] = ty[i];
+        }
+    }
+    for (int e = 0; e < E; e++) {  /* sum(w_cv(1:4,e)*taux_node(elem2D_nodes(:,e))) in order */
+        const int *q = o->elem_nodes + 4 * (size_t)e;
+        const double *w = o->w_cv + 4 * (size_t)e;
+        double sx = w[0] * tx[q[0]], sy = w[0] * ty[q[0]];
+        for (int j = 1; j < 4; j++) {
+            sx = sx + w[j] * tx[q[j]];
+            sy = sy + w[j] * ty[q[j]];
+        }
+        o->taux[e] = sx;
+        o->tauy[e] = sy;
+    }
+    free(tx);
+    free(ty);
+    return 0;
+}
+
 double orc_sw_cp(double S, double Ti, double P) {
     double T = Ti < 0 ? 0.0 : Ti, S0, sS;
     if (S < 0) { S0 = 0.0; sS = 0.0; } else { S0 = S; sS = S * sqrt(S); }

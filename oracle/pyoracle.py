@@ -64,6 +64,14 @@ def lib():
         L.orc_filter_only.restype = C.c_int
         L.orc_filter_only.argtypes = [C.c_void_p, C.c_int]
         L.orc_sw_cp.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_psi_h.restype = C.c_double
+        L.orc_psi_h.argtypes = [C.c_double]
+        L.orc_psi_m.restype = C.c_double
+        L.orc_psi_m.argtypes = [C.c_double]
+        L.orc_core_coeff_2z.restype = None
+        L.orc_core_coeff_2z.argtypes = [C.c_double] * 7 + [dp]
+        L.orc_apply_atm_fluxes.restype = C.c_int
+        L.orc_apply_atm_fluxes.argtypes = [P, dp, dp, C.c_double, C.c_int, dp]
         L.orc_tracer_faces.restype = C.c_int
         L.orc_tracer_faces.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
@@ -181,10 +189,42 @@ class Oracle:
         lib().orc_get_int(self.h, INTS[name], _ip(out))
         return out
 
+    def apply_atm_fluxes(self, coef_a, coef_b, nm_prec_coef=1.0, nm_net_flux=0, node_out=False):
+        """sbc_do's data_timeinterp + apply_atm_fluxes at the oracle's clock
+        (src/fv_sbc.F90:1042-1044); coef_a/coef_b (nod2D, 8).  With
+        node_out: per node (Cd, Ch, Ce, T_zu, q_zu, taux_node, tauy_node)."""
+        N = self.size["N"]
+        a = np.ascontiguousarray(coef_a, dtype=np.float64).reshape(N, 8)
+        b = np.ascontiguousarray(coef_b, dtype=np.float64).reshape(N, 8)
+        out = np.empty((N, 7)) if node_out else None
+        r = lib().orc_apply_atm_fluxes(self.h, _dp(a), _dp(b), float(nm_prec_coef), int(nm_net_flux),
+                                       None if out is None else _dp(out))
+        if r:
+            raise RuntimeError("orc_apply_atm_fluxes failed")
+        return out
+
     def set_tracers(self, T=None, S=None):
         t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
         s = None if S is None else np.ascontiguousarray(S, dtype=np.float64).reshape(-1)
         lib().orc_set_tracers(self.h, None if t is None else _dp(t), None if s is None else _dp(s))
+
+
+def psi_h(zeta):
+    """L&Y stability function for heat (src/fv_sbc.F90:616-631)."""
+    return lib().orc_psi_h(float(zeta))
+
+
+def psi_m(zeta):
+    """L&Y stability function for momentum (src/fv_sbc.F90:633-660)."""
+    return lib().orc_psi_m(float(zeta))
+
+
+def core_coeff_2z(zt, zu, sst, T_zt, q_sat, q_zt, dU):
+    """(Cd, Ch, Ce, T_zu, q_zu) of core_coeff_2z (src/fv_sbc.F90:485-614)."""
+    out = np.empty(5)
+    lib().orc_core_coeff_2z(float(zt), float(zu), float(sst), float(T_zt), float(q_sat), float(q_zt), float(dU),
+                            _dp(out))
+    return tuple(float(v) for v in out)
 
 
 def set_mutation(mid):
