@@ -22,6 +22,7 @@ template <typename R> __global__ void k_velgrad3d(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_velgrad3d_v(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_sbc_node(DevMesh, DevStateT<R>, double, double, int);
 template <typename R> __global__ void k_sbc_elem(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_obc_apply(DevMesh, DevStateT<R>, double, int, int *);
 template <typename R> __global__ void k_vertvel_div_v(DevMesh, DevStateT<R>, int);
 template <typename R> __global__ void k_tr_gradn_muscl_v(DevMesh, DevStateT<R>, int);
 template <typename R, bool P>
@@ -1129,6 +1130,70 @@ extern "C" int fcg_sbc_do(fcg_ctx *ctx, double nm_prec_coef, int nm_net_flux) {
         return fcg_fail(ctx, FCG_E_ARG, "fcg_sbc_do: nm_prec_coef must be nonzero and nm_net_flux 0 or 1");
     return ctx->mixed ? sbc_do_impl(ctx, retype<float>(ctx->s), nm_prec_coef, nm_net_flux)
                       : sbc_do_impl(ctx, ctx->s, nm_prec_coef, nm_net_flux);
+}
+
+// ---- open-boundary T/S climatology (key_obc) -----------------------------
+extern "C" int fcg_obc_set_coef(fcg_ctx *ctx, int ndepth, const double *depth, const double *coef_a,
+                                const double *coef_b) {
+    if (!ctx || ndepth < 1 || !depth || !coef_a || !coef_b) return FCG_E_ARG;
+    const int nb = ctx->m.nobn, N = ctx->m.N;
+    if (nb == 0) return fcg_fail(ctx, FCG_E_ARG, "fcg_obc_set_coef: the mesh has no open-boundary nodes");
+    for (int d = 1; d < ndepth; ++d)
+        if (!(depth[d] > depth[d - 1]))
+            return fcg_fail(ctx, FCG_E_ARG, "fcg_obc_set_coef: the depths must increase");
+    FCG_CHECK(hipSetDevice(ctx->device));
+    if (ctx->obc_ndep != ndepth) {  // (re)allocated for this profile length
+        if (ctx->s.obc_coef) {
+            FCG_CHECK(hipStreamSynchronize(ctx->stream));
+            FCG_CHECK(hipFree(ctx->s.obc_coef));
+            FCG_CHECK(hipFree(ctx->s.obc_depth));
+            ctx->allocs.erase(std::remove(ctx->allocs.begin(), ctx->allocs.end(), (void *)ctx->s.obc_coef),
+                              ctx->allocs.end());
+            ctx->allocs.erase(std::remove(ctx->allocs.begin(), ctx->allocs.end(), (void *)ctx->s.obc_depth),
+                              ctx->allocs.end());
+            ctx->s.obc_coef = nullptr;
+            ctx->s.obc_depth = nullptr;
+        }
+        int r = dalloc(ctx, &ctx->s.obc_coef, 4 * (size_t)nb * ndepth);
+        if (r) return r;
+        r = dalloc(ctx, &ctx->s.obc_depth, (size_t)ndepth);
+        if (r) return r;
+        ctx->obc_ndep = ndepth;
+    }
+    if (!ctx->s.obc_node) {
+        std::vector<int> qn(N), node(nb);
+        for (int i = 0; i < N; ++i) qn[ctx->pn[i]] = i;
+        for (int i = 0; i < nb; ++i) node[i] = qn[ctx->in_obn0[i]];
+        int r = dalloc(ctx, &ctx->s.obc_node, (size_t)nb);
+        if (r) return r;
+        FCG_CHECK(hipMemcpy(ctx->s.obc_node, node.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+    }
+    const size_t n = 2 * (size_t)nb * ndepth;
+    std::vector<double> h(2 * n);
+    std::copy(coef_a, coef_a + n, h.begin());
+    std::copy(coef_b, coef_b + n, h.begin() + n);
+    FCG_CHECK(hipMemcpyAsync(ctx->s.obc_coef, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    FCG_CHECK(hipMemcpyAsync(ctx->s.obc_depth, depth, ndepth * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    return FCG_OK;
+}
+
+template <typename R>
+static int obc_do_impl(fcg_ctx *ctx, const DevStateT<R> &s, double time_jd) {
+    const int items = ctx->m.nobn * ctx->m.L;
+    FCG_CHECK(hipMemsetAsync(s.err + 3, 0, sizeof(int), ctx->stream));
+    LAUNCH(ctx, k_obc_apply<R>, nblk(items), ctx->m, s, time_jd, ctx->obc_ndep, s.err + 3);
+    int bad = 0;
+    FCG_CHECK(hipMemcpyAsync(&bad, s.err + 3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCG_CHECK(hipStreamSynchronize(ctx->stream));
+    if (bad) return fcg_fail(ctx, FCG_E_ARG, "fcg_obc_do: a layer lies above the first depth of the climatology");
+    return FCG_OK;
+}
+
+extern "C" int fcg_obc_do(fcg_ctx *ctx, double time_jd) {
+    if (!ctx) return FCG_E_ARG;
+    if (!ctx->s.obc_coef) return fcg_fail(ctx, FCG_E_ARG, "fcg_obc_do: no climatology coefficients (fcg_obc_set_coef)");
+    return ctx->mixed ? obc_do_impl(ctx, retype<float>(ctx->s), time_jd) : obc_do_impl(ctx, ctx->s, time_jd);
 }
 
 extern "C" int fcg_set_state(fcg_ctx *ctx, const char *name, const double *host, long count) {

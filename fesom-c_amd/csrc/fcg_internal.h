@@ -152,6 +152,9 @@ struct DevStateT {
     double *qsr = nullptr, *qns = nullptr, *emp = nullptr, *mslp = nullptr;  // N: surface forcing (key_atm)
     double *sbc_coef = nullptr;    // 16N: per node the 8 fields' time coefficients a, then b (key_atm, bulk)
     double *tau_node = nullptr;    // 2N: bulk wind stress at the nodes (key_atm, bulk)
+    double *obc_coef = nullptr;    // 4*nobn*ndep: OB climatology time coefficients (a then b, (2, nobn, ndep) each)
+    double *obc_depth = nullptr;   // ndep: depths of the climatology profiles [m]
+    int *obc_node = nullptr;       // nobn: device node of each open-boundary node
     double *runoff = nullptr;                                  // N: river discharge [m3/s], 0 off the mouths (key_rivers)
     double *rho_c = nullptr;                                   // L*N (n*L + k): density anomaly of pressure (key_atm)
     double *node2d = nullptr;                                  // 4N: etaAB, eta_g, eta_n_1, eta_n per node (substep)
@@ -232,6 +235,7 @@ struct fcg_ctx {
     // the same for the level-vector node kernels: blocks of 128 and 256
     // threads, K/W lanes per column (W = 16/storage width)
     int cap_n2e_v[2] = {0, 0}, cap_n2el_v[2] = {0, 0};
+    int obc_ndep = 0;            // depths of the open-boundary climatology profiles on the device
     int lds_limit = 0;           // bytes of LDS one block may request on this device
     int n_cu = 1;                // compute units of the device (persistent launches)
     // learned closure

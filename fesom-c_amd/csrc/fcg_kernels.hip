@@ -3803,6 +3803,64 @@ __global__ __launch_bounds__(256) void k_sbc_elem(DevMesh m, DevStateT<R> s) {
     s.tauy[e] = sy;
 }
 
+// ---------------------------------------------------------------------------
+// open-boundary T/S climatology (key_obc): obc_do's data_timeinterp and
+// apply_obc / profile_interp (src/fv_obc.F90:376-419, 531-539, 577-623)
+// ---------------------------------------------------------------------------
+// binarysearch over the climatology depths (fv_obc.F90:669-714): 1-based
+// index of the depth within the REAL(4) tolerance 1e-9 of v, else of the last
+// one above it, 0 if none
+__device__ __forceinline__ int obc_search(int len, const double *arr, double v) {
+    const double d = (double)1e-9f;
+    int left = 1, right = len;
+    while (left <= right) {
+        const int middle = (left + right + 1) >> 1;   // nint((left+right)/2.0) for positive sums
+        if (fabs(arr[middle - 1] - v) <= d) return middle;
+        if (arr[middle - 1] > v) right = middle - 1;
+        else left = middle + 1;
+    }
+    return right;
+}
+
+// One thread per (open-boundary node, layer): the T and S profiles at the
+// clock, interpolated linearly to the layer's mid-depth Z = ((1-sigma_k) a +
+// (1-sigma_k+1) a)/2 (constant below the last depth), into Tclim/Sclim.  A
+// layer above the first depth (no bracketing record) sets the flag err and
+// keeps its value.
+template <typename R>
+__global__ __launch_bounds__(256) void k_obc_apply(DevMesh m, DevStateT<R> s, double now_date, int ndep, int *err) {
+    const int t = TID(), L = m.L, KC = m.K, nb = m.nobn;
+    if (t >= nb * L) return;
+    const int i = t / L, k = t - i * L;
+    const int n = s.obc_node[i];
+    const double a = s.a[n];
+    const double Z = 0.5 * (m.oms[k] * a + m.oms[k + 1] * a);
+    const int d = obc_search(ndep, s.obc_depth, Z);
+    if (d < 1) {
+        atomicOr(err, 1);
+        return;
+    }
+    int dp1;
+    double delta_d;
+    if (d < ndep) {
+        dp1 = d + 1;
+        delta_d = s.obc_depth[d] - s.obc_depth[d - 1];
+    } else {
+        dp1 = d;
+        delta_d = 1.0;
+    }
+    const size_t half = 2 * (size_t)nb * ndep;   // coefficients a, then b
+    for (int f = 0; f < 2; ++f) {
+        const size_t j1 = f + 2 * ((size_t)i + (size_t)nb * (d - 1)), j2 = f + 2 * ((size_t)i + (size_t)nb * (dp1 - 1));
+        const double data1 = now_date * s.obc_coef[j1] + s.obc_coef[half + j1];
+        const double data2 = now_date * s.obc_coef[j2] + s.obc_coef[half + j2];
+        const double cf_a = (data2 - data1) / delta_d;
+        const double cf_b = data1 - cf_a * s.obc_depth[d - 1];
+        R *clim = f == 0 ? s.Tclim : s.Sclim;
+        PL(clim, k, n, KC) = (R)(cf_a * Z + cf_b);
+    }
+}
+
 // mask_ad (adv_tracer_mask, fv_main.F90:1309-1335) with H_advtr_crit = 0.1,
 // H_advtr_min = 0 (fv_var.F90:49-51)
 template <typename R>
@@ -4347,6 +4405,7 @@ void instantiate_kernels() {
     (void)&k_velgrad3d_v<R>;
     (void)&k_sbc_node<R>;
     (void)&k_sbc_elem<R>;
+    (void)&k_obc_apply<R>;
     (void)&k_tr_miura<R>;
     (void)&k_tr_vdiff<R>;
     (void)&k_tr_vdiff_atm<R>;

@@ -103,6 +103,19 @@ module fesomc_gpu
        integer(c_int), value :: nm_net_flux
      end function fcg_sbc_do
 
+     integer(c_int) function fcg_obc_set_coef(ctx, ndepth, depth, coef_a, coef_b) bind(C, name="fcg_obc_set_coef")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ndepth
+       real(c_double), intent(in) :: depth(*), coef_a(*), coef_b(*)
+     end function fcg_obc_set_coef
+
+     integer(c_int) function fcg_obc_do(ctx, time_jd) bind(C, name="fcg_obc_do")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: time_jd
+     end function fcg_obc_do
+
      integer(c_int) function fcg_set_state(ctx, name, host, count) bind(C, name="fcg_set_state")
        import :: c_int, c_ptr, c_char, c_double, c_long
        type(c_ptr), value :: ctx

@@ -33,7 +33,8 @@ EXPORTS = ["fcg_mesh_build", "fcg_mesh_view", "fcg_mesh_free", "fcg_create", "fc
            "fcg_mlp_destroy", "fcg_mlp_get_params", "fcg_mlp_set_params", "fcg_mlp_set_norm", "fcg_mlp_forward",
            "fcg_mlp_gradient", "fcg_mlp_train_step", "fcg_set_mlp", "fcg_mlp_collect",
            "fcg_create_ensemble", "fcg_ensemble_size", "fcg_diagnostics_member", "fcg_set_launch",
-           "fcg_get_launch", "fcg_hot_start", "fcg_mlp_gradient_static", "fcg_sbc_set_coef", "fcg_sbc_do"]
+           "fcg_get_launch", "fcg_hot_start", "fcg_mlp_gradient_static", "fcg_sbc_set_coef", "fcg_sbc_do",
+           "fcg_obc_set_coef", "fcg_obc_do"]
 
 MLP_FEATURES, MLP_PARAMS = 5, 39312
 
@@ -122,6 +123,11 @@ def lib():
         L.fcg_sbc_set_coef.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.fcg_sbc_do.restype = C.c_int
         L.fcg_sbc_do.argtypes = [P, C.c_double, C.c_int]
+        L.fcg_obc_set_coef.restype = C.c_int
+        L.fcg_obc_set_coef.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        L.fcg_obc_do.restype = C.c_int
+        L.fcg_obc_do.argtypes = [P, C.c_double]
         L.fcg_diagnostics.restype = C.c_int
         L.fcg_diagnostics.argtypes = [P, C.POINTER(Diag)]
         L.fcg_last_error.restype = C.c_int
@@ -233,6 +239,7 @@ class Model:
 
     rivers = None   # river table of the host (key_rivers), see __init__
     sbc = None      # atmospheric forcing of the host (key_atm bulk formulas), see attach_sbc
+    obc = None      # open-boundary T/S climatology of the host (key_obc), see attach_obc
 
     def __init__(self, cfg: RunConfig, mesh: meshio.RawMesh, precision=FCG_FP64, closure=FCG_CLOSURE_D3END,
                  rivers=None):
@@ -399,8 +406,34 @@ class Model:
             self.sbc_set_coef(*coef)
         self.sbc_do(forcing.prec_coef, forcing.net_flux)
 
+    # -- open-boundary T/S climatology (key_obc) ---------------------------
+    def obc_set_coef(self, depth, coef_a, coef_b):
+        """getcoeffld's profile coefficients (nobn, ndepth, 2) to the device."""
+        d = np.ascontiguousarray(depth, dtype=np.float64)
+        a = np.ascontiguousarray(np.transpose(coef_a, (1, 0, 2)), dtype=np.float64).reshape(-1)   # (2, nobn, ndep) F
+        b = np.ascontiguousarray(np.transpose(coef_b, (1, 0, 2)), dtype=np.float64).reshape(-1)
+        if a.size != 2 * self.view.nobn * d.size or b.size != a.size:
+            raise GpuError("obc coefficients must be (nobn, ndepth, 2)")
+        self._ck(lib().fcg_obc_set_coef(self.ctx, d.size, _dp(d), _dp(a), _dp(b)), "fcg_obc_set_coef")
+
+    def obc_do(self, time_jd):
+        """Tclim/Sclim of the open-boundary nodes at time_jd (fcg_obc_do)."""
+        self._ck(lib().fcg_obc_do(self.ctx, float(time_jd)), "fcg_obc_do")
+
+    def attach_obc(self, clim):
+        """Drive Tclim/Sclim at the open boundary from ``clim`` (obc.ObcClimatology,
+        or any object with update(time_jd) -> (depth, coef_a, coef_b) | None and
+        eval_time(time_jd)): as obc_ini does at start-up (src/fv_main.F90:348)
+        the coefficients are taken now, and ``step`` calls obc_do after each
+        clock advance (src/fv_main.F90:431)."""
+        self.obc = clim
+        t = self.clock()[1]
+        c = clim.update(t)
+        if c is not None:
+            self.obc_set_coef(*c)
+
     def step(self, n=1):
-        if self.rivers is None and self.sbc is None:
+        if self.rivers is None and self.sbc is None and self.obc is None:
             self._ck(lib().fcg_run(self.ctx, int(n)), "fcg_run")
             return
         # the river table and the atmospheric records are interpolated on the
@@ -417,6 +450,11 @@ class Model:
                 if coef is not None:
                     self.sbc_set_coef(*coef)
                 self.sbc_do(self.sbc.prec_coef, self.sbc.net_flux)
+            if self.obc is not None:
+                c = self.obc.update(t)
+                if c is not None:
+                    self.obc_set_coef(*c)
+                self.obc_do(self.obc.eval_time(t))
             for ph in self.PHASES[1:]:
                 self.phase(ph)
 

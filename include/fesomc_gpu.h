@@ -234,6 +234,22 @@ int fcg_sbc_set_coef(fcg_ctx *ctx, const double *coef_a, const double *coef_b);
  * 3600 CoastDat); nm_net_flux: 0 downward longwave, 1 net longwave. */
 int fcg_sbc_do(fcg_ctx *ctx, double nm_prec_coef, int nm_net_flux);
 
+/* ---- open-boundary T/S climatology (key_obc) ---------------------------
+ * obc_do (src/fv_obc.F90:577-623, called at src/fv_main.F90:431 after the
+ * clock advance) reads the file of T and S profiles at the open-boundary
+ * nodes and keeps time coefficients per node and depth (getcoeffld,
+ * 418-520).  fcg_obc_set_coef hands them over: ndepth ascending depths [m,
+ * positive down], coef_a, coef_b (2, nobn, ndepth) Fortran layout (T, S;
+ * open-boundary nodes in in_obn order); call again when the host reads the
+ * next records.  fcg_obc_do forms the profiles at time_jd (the clock, or the
+ * time of the last interpolation once the reference's one_field holds,
+ * 610-616) and interpolates them linearly to every layer's mid-depth Z
+ * (constant below the last depth) into Tclim/Sclim of the open-boundary
+ * nodes (apply_obc, profile_interp, 376-419).  FCG_E_ARG for a layer above
+ * the first depth, where the reference has no record to read. */
+int fcg_obc_set_coef(fcg_ctx *ctx, int ndepth, const double *depth, const double *coef_a, const double *coef_b);
+int fcg_obc_do(fcg_ctx *ctx, double time_jd);
+
 /* ---- the hot path: one entry per call site of oce_timestep ------------
  * fcg_advance_clock:        time_jd += dt/86400; time = time_jd*86400 (src/fv_main.F90:400-401)
  * fcg_pressure:             call pressure                  (src/fv_main.F90:1996)

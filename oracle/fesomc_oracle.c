@@ -1353,7 +1353,8 @@ int orc_wad_mask_only(orc_t *o) {
  *  6 OB tide phase with the wrong sign                   12 element wet/dry mask from the deepest node
  * 13 Miura edge x-length in metres (the reference's a = 1 workaround, fv_tracer.F90:1018-1026, undone)
  * 14 psi_m with pi/4 for pi/2                            15 ocean albedo left out of the net shortwave
- * 16 latent heat with the wrong sign in qns */
+ * 16 latent heat with the wrong sign in qns              17 OB climatology profile taken at the level (zbar)
+ *                                                           instead of the layer mid-depth Z */
 int orc_set_mutation(int id) {
     g_mut = id;
     return 0;
@@ -2322,6 +2323,56 @@ int orc_apply_atm_fluxes(orc_t *o, const double *coef_a, const double *coef_b, d
     }
     free(tx);
     free(ty);
+    return 0;
+}
+
+/* ---- open-boundary T/S climatology (key_obc): obc_do's data_timeinterp and
+ * apply_obc / profile_interp (src/fv_obc.F90:531-539, 376-419, 577-623).
+ * coef_a/coef_b (2, nobn, ndep) Fortran layout (fields T, S): the profile
+ * at time t is t*coef_a + coef_b on the file depths (positive down, ascending);
+ * each layer takes the linear interpolation at its mid-depth Z (constant
+ * below the last depth).  Layers above the first depth have no bracketing
+ * record (the reference reads out of bounds there): -1. */
+static int obc_search(int len, const double *arr, double v) {   /* binarysearch, 1-based (fv_obc.F90:669-714) */
+    const double d = (double)1e-9f;
+    int left = 1, right = len;
+    while (left <= right) {
+        const int middle = (int)lround((left + right) / 2.0);
+        if (fabs(arr[middle - 1] - v) <= d) return middle;
+        if (arr[middle - 1] > v) right = middle - 1;
+        else left = middle + 1;
+    }
+    return right;
+}
+
+int orc_obc_apply(orc_t *o, int ndep, const double *depth, const double *coef_a, const double *coef_b, double now_date) {
+    const int L = o->L, K = o->K, nb = o->nobn;
+    for (int i = 0; i < nb; i++) {
+        const int n = o->in_obn[i];
+        for (int f = 0; f < 2; f++) {
+            double *clim = f == 0 ? o->Tclim : o->Sclim;
+            for (int k = 0; k < L; k++) {
+                const double Z = MUT(17) ? o->zbar[(size_t)n * K + k] : o->Z[(size_t)n * L + k];
+                const int d = obc_search(ndep, depth, Z);
+                if (d < 1) return -1;
+                int dp1;
+                double delta_d;
+                if (d < ndep) {
+                    dp1 = d + 1;
+                    delta_d = depth[d] - depth[d - 1];
+                } else {
+                    dp1 = d;
+                    delta_d = 1.0;
+                }
+                const size_t j1 = f + 2 * ((size_t)i + (size_t)nb * (d - 1)), j2 = f + 2 * ((size_t)i + (size_t)nb * (dp1 - 1));
+                const double data1 = now_date * coef_a[j1] + coef_b[j1];
+                const double data2 = now_date * coef_a[j2] + coef_b[j2];
+                const double cf_a = (data2 - data1) / delta_d;
+                const double cf_b = data1 - cf_a * depth[d - 1];
+                clim[(size_t)n * L + k] = cf_a * Z + cf_b;
+            }
+        }
+    }
     return 0;
 }
 

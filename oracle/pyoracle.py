@@ -72,6 +72,8 @@ def lib():
         L.orc_core_coeff_2z.argtypes = [C.c_double] * 7 + [dp]
         L.orc_apply_atm_fluxes.restype = C.c_int
         L.orc_apply_atm_fluxes.argtypes = [P, dp, dp, C.c_double, C.c_int, dp]
+        L.orc_obc_apply.restype = C.c_int
+        L.orc_obc_apply.argtypes = [P, C.c_int, dp, dp, dp, C.c_double]
         L.orc_tracer_faces.restype = C.c_int
         L.orc_tracer_faces.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
@@ -202,6 +204,15 @@ class Oracle:
         if r:
             raise RuntimeError("orc_apply_atm_fluxes failed")
         return out
+
+    def obc_apply(self, depth, coef_a, coef_b, time_jd):
+        """obc_do's data_timeinterp at time_jd and apply_obc on the OB nodes'
+        Tclim/Sclim (src/fv_obc.F90:376-419, 577-623); coef (nobn, ndep, 2)."""
+        d = np.ascontiguousarray(depth, dtype=np.float64)
+        a = np.ascontiguousarray(np.transpose(coef_a, (1, 0, 2)), dtype=np.float64)   # -> (ndep, nobn, 2)
+        b = np.ascontiguousarray(np.transpose(coef_b, (1, 0, 2)), dtype=np.float64)
+        if lib().orc_obc_apply(self.h, d.size, _dp(d), _dp(a), _dp(b), float(time_jd)):
+            raise ValueError("obc_apply: a layer lies above the first depth of the climatology")
 
     def set_tracers(self, T=None, S=None):
         t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
