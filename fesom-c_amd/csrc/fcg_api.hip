@@ -23,6 +23,8 @@ template <typename R> __global__ void k_velgrad3d_v(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_sbc_node(DevMesh, DevStateT<R>, double, double, int);
 template <typename R> __global__ void k_sbc_elem(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_obc_apply(DevMesh, DevStateT<R>, double, int, int *);
+template <typename R> __global__ void k_je_v(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_update3d_v(DevMesh, DevStateT<R>, double, double);
 template <typename R> __global__ void k_vertvel_div_v(DevMesh, DevStateT<R>, int);
 template <typename R> __global__ void k_tr_gradn_muscl_v(DevMesh, DevStateT<R>, int);
 template <typename R, bool P>
@@ -1395,10 +1397,19 @@ extern "C" int fcg_barotropic_timestep(fcg_ctx *ctx) {
     return ctx->mixed ? barotropic_impl(ctx, retype<float>(ctx->s)) : barotropic_impl(ctx, ctx->s);
 }
 
+// the element Jacobians (jacobian, src/fv_main.F90:1240-1252), level vectors where possible
+template <typename R>
+static void je_launch(fcg_ctx *ctx, const DevStateT<R> &s) {
+    if (vec_on(ctx, "k_je", 0))
+        LAUNCH_NAMED(ctx, "k_je", fcgk::k_je_v<R>, (ctx->m.E + vec_cpb(ctx) - 1) / vec_cpb(ctx), 256, 0, ctx->m, s);
+    else
+        LAUNCH_EX(ctx, k_je<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, 8 * cpb_of(ctx->m.L) * sizeof(double), ctx->m, s);
+}
+
 template <typename R>
 static int jacobian_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     LAUNCH1(ctx, k_jac_node<R>, ctx->m.N, ctx->m, s, ctx->p.Dmin);
-    LAUNCH_EX(ctx, k_je<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, 8 * cpb_of(ctx->m.L) * sizeof(double), ctx->m, s);
+    je_launch(ctx, s);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -1411,7 +1422,7 @@ extern "C" int fcg_jacobian(fcg_ctx *ctx) {
 template <typename R>
 static int hot_start_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     hipLaunchKernelGGL(fcgk::k_jac_restart<R>, dim3(nblk(ctx->m.N)), dim3(kBlk), 0, ctx->stream, ctx->m, s, ctx->p.Dmin);
-    LAUNCH_EX(ctx, k_je<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, 8 * cpb_of(ctx->m.L) * sizeof(double), ctx->m, s);
+    je_launch(ctx, s);
     FCG_CHECK(hipGetLastError());
     FCG_CHECK(hipStreamSynchronize(ctx->stream));
     return FCG_OK;
@@ -1426,9 +1437,14 @@ extern "C" int fcg_hot_start(fcg_ctx *ctx, double time_jd) {
 
 template <typename R>
 static int update_3d_vel_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
-    LAUNCH_EX(ctx, k_update3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT,
-              2 * flat_bytes(ctx->m.L, 8) + 14 * (size_t)cpb_of(ctx->m.L) * 8, ctx->m, s, ctx->p.Dmin,
-              1.0 / (double)ctx->p.Mt);
+    const size_t lv = (2 * (size_t)ctx->m.K + 4) * vec_cpb(ctx) * sizeof(double);
+    if (vec_on(ctx, "k_update3d", lv))
+        LAUNCH_NAMED(ctx, "k_update3d", fcgk::k_update3d_v<R>, (ctx->m.E + vec_cpb(ctx) - 1) / vec_cpb(ctx), 256, lv,
+                     ctx->m, s, ctx->p.Dmin, 1.0 / (double)ctx->p.Mt);
+    else
+        LAUNCH_EX(ctx, k_update3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT,
+                  2 * flat_bytes(ctx->m.L, 8) + 14 * (size_t)cpb_of(ctx->m.L) * 8, ctx->m, s, ctx->p.Dmin,
+                  1.0 / (double)ctx->p.Mt);
     FCG_CHECK(hipGetLastError());
     return FCG_OK;
 }
@@ -1661,7 +1677,7 @@ extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
             const std::string k(kernel, colon - kernel);
             const int W = 16 / (int)ctx->esize;
             static const char *vec[] = {"k_velrhs", "k_tr_muscl", "k_el2nodes", "k_vertvel_div", "k_tr_gradn_muscl",
-                                        "k_velgrad3d"};
+                                        "k_velgrad3d", "k_je", "k_update3d"};
             bool known = false;
             for (const char *v : vec) known = known || k == v;
             if (!known) return fcg_fail(ctx, FCG_E_ARG, "kernel without a level-vector variant: " + k);
@@ -1673,7 +1689,9 @@ extern "C" int fcg_set_launch(fcg_ctx *ctx, const char *kernel, int block) {
                                : k == "k_el2nodes" ? lds_el2n_v(ctx)
                                : k == "k_vertvel_div" ? lds_vdiv_v(ctx)
                                : k == "k_tr_gradn_muscl" ? lds_gradn_v(ctx)
-                                                          : lds_velgrad_v(ctx);
+                               : k == "k_velgrad3d" ? lds_velgrad_v(ctx)
+                               : k == "k_update3d" ? (2 * (size_t)ctx->m.K + 4) * vec_cpb(ctx) * sizeof(double)
+                                                    : 0;
             if (block == W && lds > (size_t)ctx->lds_limit)
                 return fcg_fail(ctx, FCG_E_ARG, k + ": LDS of the level-vector variant exceeds the device limit");
             ctx->launch_block[kernel] = block;

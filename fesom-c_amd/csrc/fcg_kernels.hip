@@ -3659,6 +3659,101 @@ __global__ __launch_bounds__(256) void k_velgrad3d_v(DevMesh m, DevStateT<R> s) 
     stv_n(s.gvy + ek0, o3, nl);
 }
 
+// element Jacobians Je as k_je
+template <typename R>
+__global__ __launch_bounds__(256) void k_je_v(DevMesh m, DevStateT<R> s) {
+    constexpr int W = LV<R>::W;
+    const int KC = m.K, L = m.L;
+    const ColFlat f = col_flat(m.E, KC / W);
+    if (!f.on()) return;
+    const int e = f.c0 + f.c, k0 = f.k * W, nl = min(W, L - k0);
+    const int4 q = m.en[e];
+    const double4 w = m.wcv[e];
+    const double c[4] = {w.x, w.y, w.z, w.w};
+    const double a0 = s.a[q.x], a1 = s.a[q.y], a2 = s.a[q.z], a3 = s.a[q.w];
+    LV<R> o;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const double ds = m.dsig[k0 + i < L ? k0 + i : L - 1];
+        o.x[i] = sum4a(c, a0 * ds, a1 * ds, a2 * ds, a3 * ds);
+    }
+    stv_n(s.Je + (size_t)e * KC + k0, o, nl);
+}
+
+// update_3D_vel as k_update3d: the layers' U_rhs*Je_old products go to LDS
+// (stride K), one thread per column sums them in layer order
+template <typename R>
+__global__ __launch_bounds__(256) void k_update3d_v(DevMesh m, DevStateT<R> s, double Dmin, double Mt_inv) {
+    constexpr int W = LV<R>::W;
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const int L = m.L, KC = m.K;
+    const ColFlat f = col_flat(m.E, KC / W);
+    double *pu = (double *)lds_, *pv = pu + f.cpb * KC;
+    double *sc = pv + f.cpb * KC;  // per column: U2p, V2p, uf, vf
+    if (f.nc <= 0) return;
+    const int e = f.c0 + f.c, k0 = f.k * W;
+    const size_t ek0 = (size_t)e * KC + k0;
+    double ak[W], je[W];
+    LV<R> ur, vr;
+    if (f.on()) {
+        const int4 q = m.en[e];
+        const double4 w = m.wcv[e];
+        const double c[4] = {w.x, w.y, w.z, w.w};
+        const double o0 = s.a_old[q.x], o1 = s.a_old[q.y], o2 = s.a_old[q.z], o3 = s.a_old[q.w];
+        ur = ldv_glb(s.U_rhs + ek0);
+        vr = ldv_glb(s.V_rhs + ek0);
+        const LV<R> jv = ldv_glb(s.Je + ek0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const double ds = m.dsig[k0 + i < L ? k0 + i : L - 1];
+            ak[i] = sum4a(c, o0 * ds, o1 * ds, o2 * ds, o3 * ds);
+            je[i] = jv.x[i];
+            pu[f.c * KC + k0 + i] = (double)ur.x[i] * ak[i];
+            pv[f.c * KC + k0 + i] = (double)vr.x[i] * ak[i];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < f.nc) {
+        const int ci = threadIdx.x, ec = f.c0 + ci;
+        const int4 q = m.en[ec];
+        const double4 w = m.wcv[ec];
+        const double dmi = 1.0 / dmaxr(Dmin, sum4(w, m.depth[q.x] + s.eta_n[q.x], m.depth[q.y] + s.eta_n[q.y],
+                                                  m.depth[q.z] + s.eta_n[q.z], m.depth[q.w] + s.eta_n[q.w]));
+        double su = 0.0, sv = 0.0;
+        for (int j = 0; j < L; ++j) {
+            su = su + pu[ci * KC + j];
+            sv = sv + pv[ci * KC + j];
+        }
+        const double uf = (s.U_filt[ec] * Mt_inv) * dmi;
+        const double vf = (s.V_filt[ec] * Mt_inv) * dmi;
+        s.U_filt[ec] = uf;
+        s.V_filt[ec] = vf;
+        sc[4 * ci] = su * dmi;
+        sc[4 * ci + 1] = sv * dmi;
+        sc[4 * ci + 2] = uf;
+        sc[4 * ci + 3] = vf;
+    }
+    __syncthreads();
+    if (!f.on()) return;
+    const int nl = min(W, L - k0);
+    const double U2p = sc[4 * f.c], V2p = sc[4 * f.c + 1], uf = sc[4 * f.c + 2], vf = sc[4 * f.c + 3];
+    const double u2 = s.U2[2 * e], v2 = s.U2[2 * e + 1];
+    LV<R> onf, vnf, on, vn;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const double a = ak[i] / je[i];
+        const double uri = ur.x[i], vri = vr.x[i];
+        onf.x[i] = a * uri + uf - U2p;
+        vnf.x[i] = a * vri + vf - V2p;
+        on.x[i] = a * uri + u2 - U2p;
+        vn.x[i] = a * vri + v2 - V2p;
+    }
+    stv_n(s.U_nf + ek0, onf, nl);
+    stv_n(s.V_nf + ek0, vnf, nl);
+    stv_n(s.U_n + ek0, on, nl);
+    stv_n(s.V_n + ek0, vn, nl);
+}
+
 // ---------------------------------------------------------------------------
 // surface boundary conditions from atmospheric fields (key_atm with the
 // CORE bulk formulas, src/fv_sbc.F90:368-867)
@@ -4406,6 +4501,8 @@ void instantiate_kernels() {
     (void)&k_sbc_node<R>;
     (void)&k_sbc_elem<R>;
     (void)&k_obc_apply<R>;
+    (void)&k_je_v<R>;
+    (void)&k_update3d_v<R>;
     (void)&k_tr_miura<R>;
     (void)&k_tr_vdiff<R>;
     (void)&k_tr_vdiff_atm<R>;

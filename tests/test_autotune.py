@@ -87,7 +87,8 @@ def test_level_vectors_do_not_change_results(precision, nsigma):
     the column's unused last slot is left as it was."""
     a, b = _model(precision, nsigma), _model(precision, nsigma)
     W = 4 if precision else 2
-    vec = ["k_velrhs", "k_tr_muscl", "k_el2nodes", "k_vertvel_div", "k_tr_gradn_muscl", "k_velgrad3d"]
+    vec = ["k_velrhs", "k_tr_muscl", "k_el2nodes", "k_vertvel_div", "k_tr_gradn_muscl", "k_velgrad3d", "k_je",
+           "k_update3d"]
     for k in vec:
         assert a.get_launch(k + ":levels") == W   # the default where nsigma is a multiple of W
         b.set_launch(k + ":levels", 1)
