@@ -232,6 +232,10 @@ def main():
     ap.add_argument("--serial-classes", action="store_true",
                     help="c5 only: finish each level class's step before the next starts (kernel times without "
                          "overlap; the default lets the classes' streams overlap)")
+    ap.add_argument("--error-steps", default=None, metavar="N|m2",
+                    help="c2 only: continue the headline and the FP64 run in lock step to N internal steps in "
+                         "all ('m2': one M2 period, 44 712 s) and report their differences then, over all "
+                         "nodes and outside the open-boundary sponge layer")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
@@ -331,10 +335,12 @@ def main():
     N, E, D = sum(p.N for p in parts), sum(p.E for p in parts), sum(p.D for p in parts)
     nls = NL * args.steps / sec
     e3 = 4 if prec else 8
-    tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{'mixed' if prec else 'fp64'}_step_c2.md")
-    traffic = (lambda k: pmc_traffic(k, tfile)) if (args.workload == "c2" and not args.scale) else (lambda k: None)
+    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes of this workload
+    tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{'mixed' if prec else 'fp64'}_step_{args.workload}.md")
+    traffic = (lambda k: pmc_traffic(k, tfile)) if not args.scale else (lambda k: None)
     if probe == "k_mlp_closure":
         roof = roofline_mlp(parts, reads)
+        roof["traffic"] = traffic(probe)
     else:
         roof = roofline_hbm(probe, parts, reads, e3, traffic(probe))
     roof2 = roofline_hbm(args.probe2, parts, reads2, e3, traffic(args.probe2))
@@ -381,7 +387,9 @@ def main():
         # of the headline path against it after the same steps
         head_fields = {key: model.get(f) for f, key in ERR_FIELDS}
         head_mask = model.get("mask_wd")
-        model.close()
+        long_run = args.error_steps is not None and args.workload == "c2"
+        if not long_run:
+            model.close()
         ref, sec64, _, _, setup64, _ = run(0, gpu.FCG_CLOSURE_D3END, "k_velrhs")
         out["fp64_drop_in"] = {"value": NL * args.steps / sec64, "ms_per_step": 1e3 * sec64 / args.steps}
         out["speedup_vs_fp64"] = sec64 / sec
@@ -389,6 +397,33 @@ def main():
         out["error_vs_fp64_after_steps"] = {"steps": args.warmup + args.steps + 4,
                                             **ensemble_errors(head_fields, ref_fields)}
         out["mask_wd_flips_vs_fp64"] = int((ref.get("mask_wd") != head_mask).sum())
+        if long_run:
+            done = args.warmup + args.steps + 4
+            target = (int(math.ceil(44712.0 / model.par.dt)) if args.error_steps == "m2"
+                      else int(args.error_steps))
+            while done < target:
+                k = min(500, target - done)
+                model.step(k)
+                ref.step(k)
+                done += k
+                model.sync()
+                ref.sync()
+                log(f"lock-step run: {done}/{target} internal steps")
+            ac = np.ctypeslib.as_array(model.view.ac, shape=(model.view.nod2D,))
+            inner = np.nonzero(ac >= 1.0)[0]
+            L = model.L
+            hf = {key: model.get(f) for f, key in ERR_FIELDS}
+            rf = {key: ref.get(f) for f, key in ERR_FIELDS}
+            dm, dr = model.diagnostics(), ref.diagnostics()
+            out["error_vs_fp64_long"] = {"steps": done, "seconds": done * model.par.dt,
+                                         "status": [dm["status"], dr["status"]], **ensemble_errors(hf, rf)}
+            # outside the sponge layer: node fields at inner nodes
+            sel = {"ssh_m": inner, "t_degC": (inner[:, None] * L + np.arange(L)).ravel(),
+                   "s_psu": (inner[:, None] * L + np.arange(L)).ravel()}
+            out["error_vs_fp64_long_outside_sponge"] = ensemble_errors({k: hf[k][v] for k, v in sel.items()},
+                                                                       {k: rf[k][v] for k, v in sel.items()})
+            out["mask_wd_flips_vs_fp64_long"] = int((ref.get("mask_wd") != model.get("mask_wd")).sum())
+            model.close()
         ref.close()
     else:
         model.close()
