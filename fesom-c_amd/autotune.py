@@ -6,7 +6,8 @@ block; 256, 512 or 1024 threads of whole columns for the wide-block kernels
 -- the momentum right-hand side and the node-gather kernels of the tracer
 advection, the vertical velocity and the node averaging, whose larger blocks
 hold more node or element columns and so take more neighbours from LDS, at
-the price of fewer resident blocks) are timed on the model's own mesh and
+the price of fewer resident blocks; and, for the kernels with a level-vector
+variant, 1 or 4 (FP32) / 2 (FP64) layers per lane) are timed on the model's own mesh and
 state with the library's HIP-event probe (fcg_probe), over a few internal
 steps each, and the fastest is kept (fcg_set_launch).  The search advances
 the model: run it during the warm-up of a run, or on a throw-away model of
@@ -25,8 +26,12 @@ import os
 LAYER_KERNELS = ["k_tr_upwind", "k_tr_gradn_miura", "k_tr_miura", "k_tr_ab"]
 COLUMN_KERNELS = ["k_jac_node"]
 WIDE_KERNELS = ["k_velrhs", "k_tr_muscl", "k_tr_gradn_muscl", "k_vertvel_div", "k_el2nodes"]
+# kernels with a level-vector variant: option "<kernel>:levels" = 1 or the
+# vector width (4 layers per lane in FP32, 2 in FP64)
+VECTOR_KERNELS = ["k_velrhs", "k_tr_muscl", "k_tr_gradn_muscl", "k_vertvel_div", "k_el2nodes", "k_velgrad3d",
+                  "k_je", "k_update3d"]
 # the tunable kernels the benchmark workloads launch (MUSCL tracers)
-BENCH_KERNELS = ["k_tr_ab", "k_jac_node"] + WIDE_KERNELS
+BENCH_KERNELS = ["k_tr_ab", "k_jac_node"] + WIDE_KERNELS + ["k_velgrad3d", "k_je", "k_update3d"]
 CANDIDATES = (64, 128, 256)
 WIDE_CANDIDATES = (256, 512, 1024)
 
@@ -148,27 +153,50 @@ def tune_set(model, kernels=None, candidates=CANDIDATES, steps=3, log=None, marg
         model.sync()
         return (time.perf_counter() - t0) / steps * 1e3
 
+    def settings(k):
+        """Candidate settings of kernel k, the current one first: block sizes
+        of the one-level variant and, for a kernel with a level-vector
+        variant, the vector variant (which runs whole columns in 256-thread
+        blocks, so its block option does not apply)."""
+        blocks = list(_cands(k, candidates)) if k in LAYER_KERNELS + COLUMN_KERNELS + WIDE_KERNELS else [None]
+        out = []
+        if k in VECTOR_KERNELS:
+            levels = [parts[0].get_launch(k + ":levels")]
+            levels += [v for v in (1, 2, 4) if v != levels[0]]
+            keep = {k: parts[0].get_launch(k)} if blocks[0] else {}   # the one-level variant's shape, kept
+            for lv in levels:
+                if lv > 1:
+                    out.append({k + ":levels": lv, **keep})
+                else:
+                    out += [{k + ":levels": 1, **({k: bs} if bs else {})} for bs in blocks]
+        else:
+            cur = parts[0].get_launch(k)
+            out = [{k: bs} for bs in [cur] + [c for c in blocks if c != cur]]
+        return out
+
     for k in kernels or (LAYER_KERNELS + COLUMN_KERNELS + WIDE_KERNELS):
         best = None
-        default = parts[0].get_launch(k)
-        # the current shape first; another replaces it only when it is at
+        # the current setting first; another replaces it only when it is at
         # least `margin` faster (whole-step times carry a little noise)
-        for bs in [default] + [c for c in _cands(k, candidates) if c != default]:
+        for cand in settings(k):
             try:
                 for p in parts:
-                    p.set_launch(k, bs)
-            except GpuError:
+                    for name, v in cand.items():
+                        p.set_launch(name, v)
+            except GpuError:          # a width that does not divide nsigma, or a block whose LDS does not fit
                 continue
             t = timed()
             if best is None or t < best[1] * (1.0 - margin):
-                best = (bs, t)
+                best = (cand, t)
         if best is None:
             continue
         for p in parts:
-            p.set_launch(k, best[0])
-        chosen[k] = best
+            for name, v in best[0].items():
+                p.set_launch(name, v)
+        for name, v in best[0].items():
+            chosen[name] = (v, best[1])
         if log:
-            log(f"{k}: block {best[0]} ({best[1]:.2f} ms per step)")
+            log(f"{k}: {best[0]} ({best[1]:.2f} ms per step)")
     return {str(p.K): dict(chosen) for p in parts}
 
 

@@ -5,8 +5,15 @@
 // deterministic gather here: one thread per receiving node/element walks its
 // incident edges in ascending edge id, recomputing each edge flux from the
 // same operands, so the summation order -- and in FP64 every bit -- equals
-// the reference's serial scatter (no atomics).  The translation unit is
-// compiled with -ffp-contract=off to keep the reference's rounding.
+// the reference's serial scatter (no atomics).
+//
+// The file is compiled twice, once per storage type R of the state
+// (FCG_STORAGE_DOUBLE = 1: the FP64 drop-in, with -ffp-contract=off to keep
+// the reference's rounding; = 0: the mixed-precision path, where the
+// compiler may fuse a multiplication and an addition written in one
+// expression into one FMA, which rounds once instead of twice).  Each build
+// instantiates the kernels of its own storage type; the kernels that do not
+// depend on it are built with the FP64 one.
 //
 // Vertical Jacobians are not stored: Jc = a*(sigma_k - sigma_k+1),
 // zbar = (1-sigma_k)*a, Z = mid(zbar), Jd = (Jc1+Jc2)/2 are recomputed from
@@ -2325,6 +2332,7 @@ __global__ __launch_bounds__(256) void k_minmax_partial(DevMesh m, const R *__re
 
 // T_aver, S_aver of each member: midpoints of its extrema (one block per
 // member, order-free min/max over its chunks)
+#if FCG_STORAGE_DOUBLE
 __global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb, int nmemb, double *tsa) {
     __shared__ double sm[4][256];
     const int mb = blockIdx.x;
@@ -2355,6 +2363,7 @@ __global__ __launch_bounds__(256) void k_minmax_final(const double *part, int nb
         tsa[2 * mb + 1] = 0.5 * (sm[2][0] + sm[3][0]);
     }
 }
+#endif
 
 // AB2 interpolation to n+1/2, de-meaned (fv_tracer.F90:432-444), over every
 // slot of the tracer columns (the unused last slot stays finite)
@@ -4354,12 +4363,14 @@ __global__ __launch_bounds__(kMlpT) void k_mlp_closure(DevMesh m, DevStateT<R> s
     }
 }
 
+#if FCG_STORAGE_DOUBLE
 __global__ __launch_bounds__(256) void k_count_flags(const int *flag, int n, unsigned long long *count) {
     unsigned long long c = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += flag[i] != 0;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // diagnostics: energy (fv_average_dynamic.F90:476-554), volume, tracer
@@ -4419,6 +4430,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, DevStateT<R> s,
         for (int c = 0; c < kDiag; ++c) out[kDiag * blockIdx.x + c] = sm[c][0];
 }
 
+#if FCG_STORAGE_DOUBLE
 __global__ __launch_bounds__(256) void k_diag_final(const double *part, int nb, double *res) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double v[kDiag];
@@ -4432,7 +4444,9 @@ __global__ __launch_bounds__(256) void k_diag_final(const double *part, int nb, 
     }
     for (int c = 0; c < kDiag; ++c) res[c] = v[c];
 }
+#endif
 
+#if FCG_STORAGE_DOUBLE
 __global__ __launch_bounds__(256) void k_mask_count(const double *mask, int E, int *out) {
     __shared__ int sm[256];
     int c = 0;
@@ -4445,10 +4459,11 @@ __global__ __launch_bounds__(256) void k_mask_count(const double *mask, int E, i
     }
     if (threadIdx.x == 0) atomicAdd(out, sm[0]);
 }
+#endif
 
 
-// Both storage types of every state kernel are built in this translation
-// unit; the C-ABI unit launches them through their declarations.
+// The state kernels of this build's storage type; the C-ABI unit launches
+// them through their declarations.
 template <typename R>
 void instantiate_kernels() {
     (void)&k_pressure<R>;
@@ -4510,7 +4525,10 @@ void instantiate_kernels() {
     (void)&k_mlp_collect<R>;
     (void)&k_mlp_closure<R>;
 }
+#if FCG_STORAGE_DOUBLE
 template void instantiate_kernels<double>();
+#else
 template void instantiate_kernels<float>();
+#endif
 
 }  // namespace fcgk

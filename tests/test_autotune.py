@@ -128,3 +128,23 @@ def test_tuner_picks_a_candidate_for_each_kernel():
         assert bs in autotune._cands(k, autotune.CANDIDATES) and ms > 0 and m.get_launch(k) == bs
     assert m.diagnostics()["status"] == 0
     m.close()
+
+
+@pytest.mark.gpu
+def test_whole_step_tuner_searches_levels_per_lane():
+    """tune_set over a kernel with a block option and a level-vector variant
+    and over one with only the latter: every chosen setting is a valid one and
+    is left set."""
+    from fesomc_amd import autotune
+    m = _model(1, nsigma=16)
+    W = m.get_launch("k_velrhs:levels")
+    assert W == 4
+    tables = autotune.tune_set(m, kernels=["k_velrhs", "k_je"], steps=1)
+    t = tables[str(m.K)]
+    assert t["k_velrhs:levels"][0] in (1, W) and t["k_je:levels"][0] in (1, W)
+    for name, (v, ms) in t.items():
+        assert ms > 0 and m.get_launch(name) == v
+    if t["k_velrhs:levels"][0] == 1:
+        assert t["k_velrhs"][0] in autotune.WIDE_CANDIDATES
+    assert m.diagnostics()["status"] == 0
+    m.close()
