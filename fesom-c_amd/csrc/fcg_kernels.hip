@@ -13,7 +13,7 @@
 // compiler may fuse a multiplication and an addition written in one
 // expression into one FMA, which rounds once instead of twice).  Each build
 // instantiates the kernels of its own storage type; the kernels that do not
-// depend on it are built with the FP64 one.
+// depend on it, and the learned closure's, are built with the FP64 one.
 //
 // Vertical Jacobians are not stored: Jc = a*(sigma_k - sigma_k+1),
 // zbar = (1-sigma_k)*a, Z = mid(zbar), Jd = (Jc1+Jc2)/2 are recomputed from
@@ -4462,8 +4462,8 @@ __global__ __launch_bounds__(256) void k_mask_count(const double *mask, int E, i
 #endif
 
 
-// The state kernels of this build's storage type; the C-ABI unit launches
-// them through their declarations.
+// The state kernels of this build's storage type (except the learned
+// closure's, below); the C-ABI unit launches them through their declarations.
 template <typename R>
 void instantiate_kernels() {
     (void)&k_pressure<R>;
@@ -4522,11 +4522,19 @@ void instantiate_kernels() {
     (void)&k_tr_vdiff<R>;
     (void)&k_tr_vdiff_atm<R>;
     (void)&k_diag_partial<R>;
+}
+// The learned closure's kernels of both storage types are built with the FP64
+// build's options (no contraction): with fused multiply-adds the inference
+// kernel's feature code needs more than the 256 registers a lane has.
+template <typename R>
+void instantiate_closure_kernels() {
     (void)&k_mlp_collect<R>;
     (void)&k_mlp_closure<R>;
 }
 #if FCG_STORAGE_DOUBLE
 template void instantiate_kernels<double>();
+template void instantiate_closure_kernels<double>();
+template void instantiate_closure_kernels<float>();
 #else
 template void instantiate_kernels<float>();
 #endif
