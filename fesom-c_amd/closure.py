@@ -9,7 +9,7 @@ backward, FP32 accumulation, Adam).  PyTorch only provides device buffers and
 random minibatch indices.
 
     python -c "import __graft_entry__ as g; g.load(); from fesomc_amd import closure; closure.main()" \
-        --config c2 --steps 24 --every 532 --stride 12 --iters 20000 --out closure_c2.npz
+        --config c2 --steps 24 --every 532 --stride 12 --iters 60000 --out closure_c2.npz
 
 collects 24 steps spread over one M2 cycle of the 1M-node channel (every
 532nd internal step), every 12th column: about 2.0 M columns (48 M samples).
@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--stride", type=int, default=10)
     ap.add_argument("--every", type=int, default=1)
-    ap.add_argument("--iters", type=int, default=20000)
+    ap.add_argument("--iters", type=int, default=60000)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--out", required=True)
