@@ -309,7 +309,7 @@ bool find_field(fcg_ctx *c, const char *name, Field &f) {
         {"bt", s.bt, K, true, false},          {"Wvel", s.W, K, true, false},
         {"qsr", s.qsr, 1, true, false},        {"qns", s.qns, 1, true, false},        {"emp", s.emp, 1, true, false},
         {"mslp", s.mslp, 1, true, false},      {"rho_c", s.rho_c, L, true, false},
-        {"runoff", s.runoff, 1, true, false},
+        {"runoff", s.runoff, 1, true, false},  {"tau_node", s.tau_node, 2, true, true},
     };
     for (const Ent &t : tab)
         if (!std::strcmp(t.n, name)) {

@@ -364,6 +364,8 @@ class Model:
             return 2 * E
         if name == "vel_grad":
             return 4 * E
+        if name == "tau_node":
+            return 2 * N
         if name in nodeL:
             return N * L
         if name in nodeK:

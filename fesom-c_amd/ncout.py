@@ -67,12 +67,13 @@ VARLIST = {
     36: ("uice", "U_n_ice", "ice velocity in x direction", "m/s", 0, 1, 0, 3),
     37: ("vice", "U_n_ice", "ice velocity in y direction", "m/s", 0, 1, 0, 3),
 }
-# fields this build does not hold: the nodal wind stress of the bulk formulas
-# (taux_node, tauy_node) and the Richardson number of d3_end.  The surface
-# fields mslp, emp, qns, qsr are the ones the host sets for the step
-# (key_atm).  Ice is off, so its fields are zero as in getvar1d
-# (src/fv_ncoutput.F90:773-815); runoff_rivers is zero without key_rivers.
-NOT_HELD = {14, 15, 30}
+# fields this build does not hold: the Richardson number of d3_end.  The
+# surface fields mslp, emp, qns, qsr are the ones the host sets for the step
+# or the bulk formulas form (key_atm); the nodal wind stress (taux_node,
+# tauy_node) is held where the bulk formulas run.  Ice is off, so its fields
+# are zero as in getvar1d (src/fv_ncoutput.F90:773-815); runoff_rivers is
+# zero without key_rivers.
+NOT_HELD = {30}
 ZERO = {31, 32, 33, 34, 36, 37}
 
 
@@ -257,7 +258,7 @@ class Output:
             bad = [v for v in f["vars"] if v in NOT_HELD]
             if bad:
                 raise ValueError(f"{f['name']}: variables {bad} ({', '.join(VARLIST[v][0] for v in bad)}) "
-                                 "belong to code this build does not hold (nodal wind stress, Richardson number)")
+                                 "belong to code this build does not hold (Richardson number)")
             if f["ftype"] not in (0, 3):
                 raise ValueError("output_ini: wrong/not supported output file type")
         # element centres and depth (output_ini, src/fv_ncoutput.F90:965-981)
@@ -378,6 +379,12 @@ class Output:
             a = np.full(E, self.C_d)
         elif vid in (25, 26):
             a = g("windx" if vid == 25 else "windy")
+        elif vid in (14, 15):  # the bulk formulas' nodal stress (src/fv_sbc.F90:436-437)
+            try:
+                a = g("tau_node").reshape(N, 2)[:, vid - 14].copy()
+            except Exception as err:
+                raise ValueError(f"{VARLIST[vid][0]} is formed by the bulk formulas (sbc), which this run "
+                                 "does not apply") from err
         elif vid in (16, 17, 18, 19):
             a = g({16: "mslp", 17: "emp", 18: "qns", 19: "qsr"}[vid])
         elif vid == 35:

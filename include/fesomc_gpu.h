@@ -197,7 +197,9 @@ int fcg_ensemble_size(const fcg_ctx *ctx);
  * U_n_2, U_n, V_n, U_rhsAB, V_rhsAB, Wvel, bt, snu, Kv, TF, SF, T_old,
  * S_old, Tclim, Sclim, taux, tauy, windx, windy, mask_wd, C_d_el, Je,
  * Bar_pru_3D, Bar_prv_3D, Bar_pr_2D, hpressure, U_n_filt, V_n_filt,
- * U_filt_2D, V_filt_2D, U_rhs, V_rhs, ssh_rhs, etaAB, dmean_n; and, for
+ * U_filt_2D, V_filt_2D, U_rhs, V_rhs, ssh_rhs, etaAB, dmean_n, qsr, qns,
+ * emp, mslp, runoff; tau_node (2, nod2D: taux_node, tauy_node of the bulk
+ * formulas, held once fcg_sbc_set_coef has been called); and, for
  * fcg_get_state only, Jc, Jc_old, Z, zbar (formed from the column depth and
  * sigma as the jacobian forms them; the device does not store them).
  * count is the number of doubles the Fortran array holds; the library

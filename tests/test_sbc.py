@@ -390,9 +390,12 @@ def test_model_reads_forcing_files_and_steps_like_the_oracle(tmp_path):
     assert f.out_of_bounds == 0
     g.attach_sbc(f)
     coef = f._coefficients(t_start)          # the oracle's copy of the same coefficients
-    o.apply_atm_fluxes(*coef, nm_prec_coef=f.prec_coef, nm_net_flux=f.net_flux)
+    node = o.apply_atm_fluxes(*coef, nm_prec_coef=f.prec_coef, nm_net_flux=f.net_flux, node_out=True)
     for name in ("windx", "taux", "qns", "emp", "qsr", "mslp"):
         assert np.allclose(g.get(name), o.get(name), rtol=1e-12, atol=1e-12 * np.abs(o.get(name)).max()), name
+    # the nodal stress (taux_node, tauy_node), which the NetCDF output serves as ids 14 and 15
+    tn = g.get("tau_node").reshape(N, 2)
+    assert np.allclose(tn, node[:, 5:7], rtol=1e-12, atol=1e-12 * np.abs(node[:, 5:7]).max())
     for _ in range(4):
         g.step(1)
         o.phase(0)
@@ -403,4 +406,20 @@ def test_model_reads_forcing_files_and_steps_like_the_oracle(tmp_path):
         gv, ov = g.get(name), o.get(name)
         assert np.allclose(gv, ov, rtol=1e-9, atol=1e-9 * np.abs(ov).max()), name
     assert g.diagnostics()["status"] == 0
+    # NetCDF output of the nodal stress (ids 14, 15): the device's field times the node mask
+    from scipy.io import netcdf_file
+    from fesomc_amd import ncout
+    (tmp_path / "outputfilelist.dat").write_text(
+        "!total number of files:\n1\n!=========== Field File Name ===\nstress.nc\n! snapshot\n1\n! type\n0\n"
+        "! position\n0.0, 0.0, 0.0, 0.0\n! frequency\n1\n! special\n0\n! number of variables\n2\n! list\n"
+        "14,15" + ",0" * 28 + "\n")
+    out = ncout.Output(g, cfg, mesh, workdir=str(tmp_path))
+    g.step(1)
+    out.step()
+    out.close()
+    tn = g.get("tau_node").reshape(N, 2)
+    mn = np.where(out.depth + g.get("etaAB") > out.Dmin, 1.0, 0.0)
+    with netcdf_file(str(tmp_path / "nc_output" / "stress.nc"), "r", mmap=False) as nc:
+        np.testing.assert_array_equal(nc.variables["taux_node"][0], tn[:, 0] * mn)
+        np.testing.assert_array_equal(nc.variables["tauy_node"][0], tn[:, 1] * mn)
     g.close()
