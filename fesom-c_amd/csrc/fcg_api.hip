@@ -742,6 +742,18 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         std::vector<int> n2elK(n2el_idx.size());
         for (size_t j = 0; j < n2el_idx.size(); ++j) n2elK[j] = n2el_idx[j] * K;
         UP(m.enK, enK) UP(m.n2elK, n2elK)
+        if (ctx->mixed) {  // the barotropic momentum path's geometry in FP32
+            auto f4 = [](const std::vector<double4> &v) {
+                std::vector<float4> o(v.size());
+                for (size_t i = 0; i < v.size(); ++i)
+                    o[i] = make_float4((float)v[i].x, (float)v[i].y, (float)v[i].z, (float)v[i].w);
+                return o;
+            };
+            std::vector<float2> dxdy_f(edxdy.size());
+            for (size_t i = 0; i < edxdy.size(); ++i) dxdy_f[i] = make_float2((float)edxdy[i].x, (float)edxdy[i].y);
+            UP(m.gsx_f, f4(gsx)) UP(m.gsy_f, f4(gsy)) UP(m.gvx_f, f4(gvx)) UP(m.gvy_f, f4(gvy))
+            UP(m.ecross_f, f4(ecross)) UP(m.edxdy_f, dxdy_f)
+        }
         rc = ctx->mixed ? upload_egeo<float>(ctx, ednodes, edtri, ecross, edxdy, v_ec)
                         : upload_egeo<double>(ctx, ednodes, edtri, ecross, edxdy, v_ec);
         if (rc) {

@@ -103,6 +103,11 @@ struct DevMesh {
     int2 *edtri = nullptr;     // 0-based, -1 none
     double4 *ecross = nullptr; // edge_cross_dxdy
     double2 *edxdy = nullptr;  // edge_dxdy
+    // FP32 copies of the geometry of the barotropic momentum path (gradients of
+    // the AB3 velocity and of the elevation, the 2-D momentum edge fluxes),
+    // read by the mixed-precision kernels instead of the FP64 arrays above
+    float4 *gsx_f = nullptr, *gsy_f = nullptr, *gvx_f = nullptr, *gvy_f = nullptr, *ecross_f = nullptr;
+    float2 *edxdy_f = nullptr;
     // node constants
     double *depth = nullptr, *area = nullptr, *ac = nullptr, *atp1 = nullptr, *atp2 = nullptr;
     int *index = nullptr, *obidx = nullptr;   // obidx: position in the OB list or -1

@@ -1632,6 +1632,27 @@ __device__ __forceinline__ double etaAB3(const DevStateT<R> &s, int n) {
     return ab3(s.eta_n[n], s.eta_n_1[n], s.eta_n_2[n]);
 }
 
+// Geometry of the 2-D momentum path: the FP64 arrays for the drop-in, their
+// FP32 copies in the mixed-precision path (half the bytes per element/edge).
+template <typename R>
+__device__ __forceinline__ double4 mom_geo4(const double4 *d, const float4 *f, int i) {
+    if constexpr (sizeof(R) == sizeof(float)) {
+        const float4 v = f[i];
+        return make_double4(v.x, v.y, v.z, v.w);
+    } else {
+        return d[i];
+    }
+}
+template <typename R>
+__device__ __forceinline__ double2 mom_geo2(const double2 *d, const float2 *f, int i) {
+    if constexpr (sizeof(R) == sizeof(float)) {
+        const float2 v = f[i];
+        return make_double2(v.x, v.y);
+    } else {
+        return d[i];
+    }
+}
+
 
 // AB3 velocity + element depth of compute_ssh_rhs_elem (fv_average_dynamic.F90:66-86),
 // first substep of a step; k_bt_elem2 forms both for the substeps after it.
@@ -1727,7 +1748,7 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
     const int jend = (q.x == q.w) ? 3 : 4;
     const int4 nb = m.elnb[e];
     const int4 ed = m.eledges[e];
-    const double4 gx = m.gvx[e], gy = m.gvy[e];
+    const double4 gx = mom_geo4<R>(m.gvx, m.gvx_f, e), gy = mom_geo4<R>(m.gvy, m.gvy_f, e);
     const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
     const int eds[4] = {ed.x, ed.y, ed.z, ed.w};
     const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
@@ -1743,7 +1764,7 @@ __global__ __launch_bounds__(256) void k_bt_velgrad(DevMesh m, DevStateT<R> s) {
             g2 = g2 + gxs[j] * (w2.y - v);
             g3 = g3 + gys[j] * (w2.y - v);
         } else {
-            double2 dd = m.edxdy[eds[j]];
+            double2 dd = mom_geo2<R>(m.edxdy, m.edxdy_f, eds[j]);
             double zc0 = dd.x * m.elem_cos[e], zc1 = dd.y;
             double un = -2.0 * (u * zc1 - v * zc0) / (zc0 * zc0 + zc1 * zc1);
             g0 = g0 + gxs[j] * un * zc1;
@@ -1960,8 +1981,8 @@ __device__ __forceinline__ void edge_uu2d(const DevMesh &m, const DevStateT<R> &
     const int2 et = m.edtri[ed];
     const double dm = dmaxr(Dmin, 0.5 * ((s.etaAB[nd.x] + m.depth[nd.x]) + (s.etaAB[nd.y] + m.depth[nd.y])));
     const double acc = (m.ac[nd.x] + m.ac[nd.y]) * 0.5;
-    const double4 cd = m.ecross[ed];
-    const double2 dd = m.edxdy[ed];
+    const double4 cd = mom_geo4<R>(m.ecross, m.ecross_f, ed);
+    const double2 dd = mom_geo2<R>(m.edxdy, m.edxdy_f, ed);
     const int e1 = et.x, e2 = et.y;
     const double4 g1 = ld4(s.vel_grad, e1), g2 = ld4(s.vel_grad, e2);
     const double2 w1 = ld2(s.UAB, e1), w2 = ld2(s.UAB, e2);
@@ -1999,14 +2020,14 @@ __global__ __launch_bounds__(256) void k_bt_edge_mom(DevMesh m, DevStateT<R> s, 
 // current one.  C_d_el = C_d is set on the first substep only: nothing
 // changes it between substeps.
 template <typename R>
-__global__ __launch_bounds__(256) void k_bt_elem2(DevMesh m, DevStateT<R> s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
+__global__ __launch_bounds__(256, 5) void k_bt_elem2(DevMesh m, DevStateT<R> s, double dt_2D, double Dmin, double rho0, double C_d, double Cd_bz,
                            int wet_dry, int adv2d, int first, double tau2d, R *__restrict__ uab_next) {
     const int KC = m.K;
     int e = TID();
     if (e >= m.E) return;
     const int E = m.E, L = m.L;
     const int4 q = m.en[e];
-    const double4 w = m.wcv[e], gx = m.gsx[e], gy = m.gsy[e];
+    const double4 w = m.wcv[e], gx = mom_geo4<R>(m.gsx, m.gsx_f, e), gy = mom_geo4<R>(m.gsy, m.gsy_f, e);
     const double area = m.elem_area[e], cor = m.coriolis[e];
     const double d0 = m.depth[q.x], d1 = m.depth[q.y], d2 = m.depth[q.z], d3 = m.depth[q.w];
     // per node: etaAB, eta_g, eta^n (eta_n_1), eta^{n+1} (eta_n), written by k_bt_node this substep
