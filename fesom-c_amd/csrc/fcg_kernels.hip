@@ -2034,12 +2034,14 @@ __global__ __launch_bounds__(256, 5) void k_bt_elem2(DevMesh m, DevStateT<R> s, 
     const double4 r0 = ((const double4 *)s.node2d)[q.x], r1 = ((const double4 *)s.node2d)[q.y];
     const double4 r2 = ((const double4 *)s.node2d)[q.z], r3 = ((const double4 *)s.node2d)[q.w];
     const double ea0 = r0.x, ea1 = r1.x, ea2 = r2.x, ea3 = r3.x;
-    double mask = s.mask_wd[e];
+    double mask;
     if (wet_dry) {  // wad_mask: fv_main.F90:1284-1294
         double mn = fmin(fmin(fmin(d0, d1), d2), d3);
         double mx = fmax(fmax(fmax(ea0, ea1), ea2), ea3);
         mask = (mn + mx > Dmin) ? 1.0 : 0.0;
         s.mask_wd[e] = mask;
+    } else {
+        mask = s.mask_wd[e];
     }
     if (first) s.C_d_el[e] = C_d;
     const double rho_inv = 1.0 / (rho0);
