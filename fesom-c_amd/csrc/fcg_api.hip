@@ -54,6 +54,9 @@ template <typename R> __global__ void k_bt_p1edge(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_p1elem(DevMesh, DevStateT<R>, double);
 template <typename R>
 __global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double, double, int, int, int, double, R *);
+template <typename R> __global__ void k_filt2x_3d_c(DevMesh, DevStateT<R>, double);
+template <typename R> __global__ void k_filt2x_3d_uv(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_bt_filt2x_c(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_jac_node(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_jac_restart(DevMesh, DevStateT<R>, double);
@@ -400,6 +403,9 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s0) {
             hipLaunchKernelGGL(fcgk::k_bt_p1elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s,
                                p.filt_3D ? p.tau_inv_filt : 0.0);
         }
+        if (p.filt_bi_2D)  // src/fv_average_dynamic.F90:347 (tau_inv: fv_viscosity.F90:695)
+            hipLaunchKernelGGL(fcgk::k_bt_filt2x_c<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s,
+                               p.dt_2D * p.tau_c / 3600.0 / 24.0);
         hipLaunchKernelGGL(fcgk::k_bt_elem2<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s, p.dt_2D, p.Dmin,
                            ctx->density_0, p.C_d, p.Cd_bz, p.wet_dry_on, p.mom_adv_2D == 2 ? 1 : 0, sub == 1 ? 1 : 0,
                            p.filt_2D ? p.tau_inv_filt : 0.0, sub < p.Mt ? uab[(p.Mt - sub - 1) & 1] : (R *)nullptr);
@@ -600,7 +606,20 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     // ---- element constants
     std::vector<int4> en(E), nb(E), eled(E), ied(E);
     std::vector<double4> wcv(E), gsx(E), gsy(E), gvx(E), gvy(E);
-    std::vector<double> accB(E), acc_el(E), v_ea(E), v_ec(E), v_co(E);
+    std::vector<double> accB(E), acc_el(E), v_ea(E), v_ec(E), v_co(E), kbi;
+    const bool filt_bi = p.filt_bi_2D || p.filt_bi_3D;
+    std::vector<double> sarea(nmemb, mesh->scale_area);
+    if (filt_bi && node_member) {  // scale_area of each member: sum(elem_area)/elem2D (fv_ini.F90:81)
+        std::vector<double> sum(nmemb, 0.0);
+        std::vector<long> cnt(nmemb, 0);
+        for (int e = 0; e < E; ++e) {
+            const int mb = node_member[mesh->elem2D_nodes[4 * e] - 1];
+            sum[mb] = sum[mb] + mesh->elem_area[e];
+            cnt[mb]++;
+        }
+        for (int b = 0; b < nmemb; ++b) sarea[b] = cnt[b] ? sum[b] / (double)cnt[b] : 0.0;
+    }
+    if (filt_bi) kbi.resize(E);
     for (int i = 0; i < E; ++i) {
         const int e = pe[i];
         const int *q = mesh->elem2D_nodes + 4 * e;
@@ -625,6 +644,8 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         accB[i] = std::pow(acc, p.Bp_bz) * 1.0;  // mask_bpr = 1 (Mask_Bar_pr = .false., src/fv_var.F90:57)
         ied[i] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
         v_ea[i] = mesh->elem_area[e];
+        if (filt_bi)  // the factor of K_bi (fv_viscosity.F90:599, 706)
+            kbi[i] = std::sqrt(sarea[node_member ? node_member[q[0] - 1] : 0] / mesh->elem_area[e]);
         v_ec[i] = mesh->elem_cos[e];
         v_co[i] = mesh->coriolis[e];
     }
@@ -723,6 +744,9 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     UP(m.en, en) UP(m.elnb, nb) UP(m.eledges, eled) UP(m.el_ied, ied) UP(m.wcv, wcv) UP(m.gsx, gsx) UP(m.gsy, gsy)
     UP(m.gvx, gvx) UP(m.gvy, gvy) UP(m.accB, accB) UP(m.acc_el, acc_el)
     UP(m.elem_area, v_ea) UP(m.elem_cos, v_ec) UP(m.coriolis, v_co)
+    if (filt_bi) {
+        UP(m.kbi, kbi)
+    }
     UP(m.ednodes, ednodes) UP(m.edtri, edtri) UP(m.ecross, ecross) UP(m.edxdy, edxdy)
     UP(m.depth, depth) UP(m.area, area) UP(m.ac, ac) UP(m.atp1, atp1) UP(m.atp2, atp2) UP(m.index, index)
     UP(m.obidx, obidx) UP(m.n2e_ptr, n2e_ptr) UP(m.n2e, n2e_idx) UP(m.n2el_ptr, n2el_ptr) UP(m.n2el, n2el_idx)
@@ -785,6 +809,12 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     DA3(s.U_n, EL) DA3(s.V_n, EL) DA3(s.U_rhs, EL) DA3(s.V_rhs, EL) DA3(s.U_rhsAB, EL) DA3(s.V_rhsAB, EL)
     DA3(s.U_nf, EL) DA3(s.V_nf, EL) DA3(s.gux, EL) DA3(s.guy, EL) DA3(s.gvx, EL) DA3(s.gvy, EL)
     DA3(s.Bpu, EL) DA3(s.Bpv, EL) DA3(s.Je, EL) DA(s.puls_ap, E)
+    if (p.filt_bi_3D) {
+        DA3(s.fbu, EL) DA3(s.fbv, EL) DA3(s.uvu, EL) DA3(s.uvv, EL)
+    }
+    if (p.filt_bi_2D) {
+        DA(s.fb2c, 2 * E)
+    }
     DA3(s.vxx, EK) DA3(s.vyu, EK) DA3(s.vyv, EK)
     DA3(s.TF, NL) DA3(s.SF, NL) DA3(s.T_old, NL) DA3(s.S_old, NL) DA3(s.Tclim, NL) DA3(s.Sclim, NL)
     DA3(s.abT, NL) DA3(s.abS, NL) DA3(s.Unode_p, NL) DA3(s.Vnode_p, NL) DA3(s.Unode, NL) DA3(s.Vnode, NL) DA3(s.tcp, NL) DA3(s.ttp, NL) DA3(s.tsp, NL)
@@ -1323,6 +1353,11 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     else
         LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec),
                   ctx->m, s);
+    if (p.filt_bi_3D) {  // src/fv_3D_vel.F90:112 (tau_inv: fv_viscosity.F90:586); UV_rhs enters k_velrhs
+        const unsigned g = (unsigned)(((size_t)ctx->m.E * ctx->m.L + kFlatT - 1) / kFlatT);  // dense (element, layer)
+        LAUNCH_EX(ctx, k_filt2x_3d_c<R>, g, kFlatT, 0, ctx->m, s, p.dt * p.tau_c / 3600.0 / 24);
+        LAUNCH_EX(ctx, k_filt2x_3d_uv<R>, g, kFlatT, 0, ctx->m, s);
+    }
     const int vt = block_of(ctx, "k_velrhs");
     constexpr int W = 16 / (int)sizeof(R);
     if (levels_of(ctx, "k_velrhs", W) == W &&

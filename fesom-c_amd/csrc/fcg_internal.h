@@ -97,6 +97,7 @@ struct DevMesh {
     double4 *gvx = nullptr, *gvy = nullptr;   // gradient_vec(1:4), (5:8)
     double *elem_area = nullptr, *elem_cos = nullptr, *coriolis = nullptr;
     double *accB = nullptr;    // (sum w_cv*ac)**Bp_bz * mask_bpr, host-evaluated with libm pow
+    double *kbi = nullptr;     // sqrt(scale_area/elem_area), scale_area of the element's member (filt_bi)
     double *acc_el = nullptr;  // sum w_cv*ac
     // edge constants
     int2 *ednodes = nullptr;   // 0-based
@@ -169,6 +170,11 @@ struct DevStateT {
     R *U_rhsAB = nullptr, *V_rhsAB = nullptr, *U_nf = nullptr, *V_nf = nullptr; // L*E
     R *gux = nullptr, *guy = nullptr, *gvx = nullptr, *gvy = nullptr;         // L*E
     R *Bpu = nullptr, *Bpv = nullptr, *Je = nullptr;                          // L*E
+    // twice-applied filters (filt_bi): first-pass differences U_c, V_c and the
+    // 3-D tendency UV_rhs (L*E, null when filt_bi_3D is off); the 2-D first
+    // pass (2E, null when filt_bi_2D is off)
+    R *fbu = nullptr, *fbv = nullptr, *uvu = nullptr, *uvv = nullptr;
+    double *fb2c = nullptr;
 
     R *vxx = nullptr, *vyu = nullptr, *vyv = nullptr;                         // K*E Thomas scratch
     R *TF = nullptr, *SF = nullptr, *T_old = nullptr, *S_old = nullptr;       // L*N

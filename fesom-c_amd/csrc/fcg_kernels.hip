@@ -1110,8 +1110,10 @@ __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt
     const double mask = r.mask;
     PL(s.U_rhsAB, k, e, KC) = abu;
     PL(s.V_rhsAB, k, e, KC) = abv;
-    PL(s.U_rhs, k, e, KC) = mask * (u + PL(s.Bpu, k, e, KC) * dt + (dt * (ur + abu * ff) * area_inv + 0.0) / je);
-    PL(s.V_rhs, k, e, KC) = mask * (v + PL(s.Bpv, k, e, KC) * dt + (dt * (vr + abv * ff) * area_inv + 0.0) / je);
+    // UV_rhs: the filt_bi_3D tendency (zero when that filter is off)
+    const double uvx = s.uvu ? (double)PL(s.uvu, k, e, KC) : 0.0, uvy = s.uvu ? (double)PL(s.uvv, k, e, KC) : 0.0;
+    PL(s.U_rhs, k, e, KC) = mask * (u + PL(s.Bpu, k, e, KC) * dt + (dt * (ur + abu * ff) * area_inv + uvx) / je);
+    PL(s.V_rhs, k, e, KC) = mask * (v + PL(s.Bpv, k, e, KC) * dt + (dt * (vr + abv * ff) * area_inv + uvy) / je);
 }
 
 template <typename R>
@@ -1356,8 +1358,11 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
     for (int i = 0; i < W; ++i) {
         oa.x[i] = abu[i];
         ob.x[i] = abv[i];
-        oc.x[i] = mask * (uo.x[i] + bu.x[i] * dt + (dt * (ur[i] + abu[i] * ff) * area_inv + 0.0) / je[i]);
-        od.x[i] = mask * (vo.x[i] + bv.x[i] * dt + (dt * (vr[i] + abv[i] * ff) * area_inv + 0.0) / je[i]);
+        // UV_rhs: the filt_bi_3D tendency (zero when that filter is off)
+        const double uvx = (s.uvu && i < nl) ? (double)s.uvu[ek0 + i] : 0.0;
+        const double uvy = (s.uvu && i < nl) ? (double)s.uvv[ek0 + i] : 0.0;
+        oc.x[i] = mask * (uo.x[i] + bu.x[i] * dt + (dt * (ur[i] + abu[i] * ff) * area_inv + uvx) / je[i]);
+        od.x[i] = mask * (vo.x[i] + bv.x[i] * dt + (dt * (vr[i] + abv[i] * ff) * area_inv + uvy) / je[i]);
     }
     stv_n(s.U_rhsAB + ek0, oa, nl);
     stv_n(s.V_rhsAB + ek0, ob, nl);
@@ -2097,10 +2102,27 @@ __global__ __launch_bounds__(256, 5) void k_bt_elem2(DevMesh m, DevStateT<R> s, 
     const double ffc = 1.0 / (dmaxr(Dmin, snew) + friction * dt_2D);
     const double2 u0 = ld2(s.U2, e), h1 = ld2(s.U2_1, e), bp = ld2(s.Bar_pr_2D, e);
     const double U0 = u0.x, V0 = u0.y;
+    double uv2x = 0.0, uv2y = 0.0;
+    if (s.fb2c) {  // UV2_rhs of viscosity_filt2x_2D, second pass (fv_viscosity.F90:735-758)
+        const int4 nb4 = m.elnb[e];
+        const int nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        const double2 c0 = ld2(s.fb2c, e);
+        for (int j = 0; j < 4; ++j) {
+            if (nbs[j] < 0 || (j == 3 && nbs[3] == nbs[0])) continue;
+            const double2 cn = ld2(s.fb2c, nbs[j]);
+            if (j == 0) {
+                uv2x = -c0.x + cn.x;
+                uv2y = -c0.y + cn.y;
+            } else {
+                uv2x = uv2x - c0.x + cn.x;
+                uv2y = uv2y - c0.y + cn.y;
+            }
+        }
+    }
     const double un = mask * ((dmo * U0 - dt_2D * friction * (ub - U0) + dt_2D * bp.x
-                               + dt_2D * ru / area) * ffc + 0.0);
+                               + dt_2D * ru / area) * ffc + uv2x);
     const double vn = mask * ((dmo * V0 - dt_2D * friction * (vb - V0) + dt_2D * bp.y
-                               + dt_2D * rv / area) * ffc + 0.0);
+                               + dt_2D * rv / area) * ffc + uv2y);
     st2(s.U2_2, e, h1.x, h1.y);
     st2(s.U2_1, e, U0, V0);
     st2(s.U2, e, un, vn);
@@ -2113,6 +2135,100 @@ __global__ __launch_bounds__(256, 5) void k_bt_elem2(DevMesh m, DevStateT<R> s, 
     const double vf0 = first ? 0.0 : s.V_filt[e];
     s.U_filt[e] = uf0 + un * dmf;
     s.V_filt[e] = vf0 + vn * dmf;
+}
+
+// ---------------------------------------------------------------------------
+// Twice-applied neighbour-difference filters (filt_bi_2D, filt_bi_3D):
+// viscosity_filt2x_2D / _3D, src/fv_viscosity.F90:570-767.  The first pass
+// forms U_c = K_bi * sum over the neighbours of (own - neighbour), K_bi =
+// tau_inv*sqrt(scale_area/elem_area); the second pass sums (neighbour's U_c -
+// own U_c) into the tendency UV_rhs / UV2_rhs that the velocity updates add.
+// Neighbours are taken as the routines take them: slots 1-3 when present,
+// slot 4 when present and not a repeat of slot 1 (a triangle's slot 4); each
+// pass forms its first term apart and then accumulates, as they do.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool f2x_take(const int (&nb)[4], int j) {
+    return nb[j] >= 0 && (j < 3 || nb[3] != nb[0]);
+}
+
+// first pass of the 3-D filter on U_n*Je (fv_viscosity.F90:593-630), one
+// thread per element layer
+template <typename R>
+__global__ __launch_bounds__(256) void k_filt2x_3d_c(DevMesh m, DevStateT<R> s, double tau_inv) {
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    const int K = m.K;
+    const int4 nb4 = m.elnb[e];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const size_t i = (size_t)e * K + k;
+    const double po = (double)s.U_n[i] * (double)s.Je[i], qo = (double)s.V_n[i] * (double)s.Je[i];
+    double cu = 0.0, cv = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        if (!f2x_take(nb, j)) continue;
+        const size_t q = (size_t)nb[j] * K + k;
+        const double pn = (double)s.U_n[q] * (double)s.Je[q], qn = (double)s.V_n[q] * (double)s.Je[q];
+        if (j == 0) {
+            cu = po - pn;
+            cv = qo - qn;
+        } else {
+            cu = cu + po - pn;
+            cv = cv + qo - qn;
+        }
+    }
+    const double K_bi = tau_inv * m.kbi[e];
+    s.fbu[i] = (R)(cu * K_bi);
+    s.fbv[i] = (R)(cv * K_bi);
+}
+
+// second pass of the 3-D filter into UV_rhs (fv_viscosity.F90:638-660)
+template <typename R>
+__global__ __launch_bounds__(256) void k_filt2x_3d_uv(DevMesh m, DevStateT<R> s) {
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    const int K = m.K;
+    const int4 nb4 = m.elnb[e];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const size_t i = (size_t)e * K + k;
+    const double co = (double)s.fbu[i], do_ = (double)s.fbv[i];
+    double u = 0.0, v = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        if (!f2x_take(nb, j)) continue;
+        const size_t q = (size_t)nb[j] * K + k;
+        if (j == 0) {
+            u = -co + (double)s.fbu[q];
+            v = -do_ + (double)s.fbv[q];
+        } else {
+            u = u - co + (double)s.fbu[q];
+            v = v - do_ + (double)s.fbv[q];
+        }
+    }
+    s.uvu[i] = (R)u;
+    s.uvv[i] = (R)v;
+}
+
+// first pass of the 2-D filter on U_n_2D (fv_viscosity.F90:700-727); the
+// second pass runs inside k_bt_elem2, which adds UV2_rhs in the update
+template <typename R>
+__global__ __launch_bounds__(256) void k_bt_filt2x_c(DevMesh m, DevStateT<R> s, double tau_inv) {
+    const int e = TID();
+    if (e >= m.E) return;
+    const int4 nb4 = m.elnb[e];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const double2 uo = ld2(s.U2, e);
+    double cu = 0.0, cv = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        if (!f2x_take(nb, j)) continue;
+        const double2 un = ld2(s.U2, nb[j]);
+        if (j == 0) {
+            cu = uo.x - un.x;
+            cv = uo.y - un.y;
+        } else {
+            cu = cu + uo.x - un.x;
+            cv = cv + uo.y - un.y;
+        }
+    }
+    const double K_bi = tau_inv * m.kbi[e];
+    st2(s.fb2c, e, cu * K_bi, cv * K_bi);
 }
 
 // ---------------------------------------------------------------------------
@@ -4511,6 +4627,9 @@ void instantiate_kernels() {
     (void)&k_bt_p1edge<R>;
     (void)&k_bt_p1elem<R>;
     (void)&k_bt_elem2<R>;
+    (void)&k_filt2x_3d_c<R>;
+    (void)&k_filt2x_3d_uv<R>;
+    (void)&k_bt_filt2x_c<R>;
     (void)&k_jac_node<R>;
     (void)&k_jac_restart<R>;
     (void)&k_je<R>;

@@ -28,6 +28,8 @@ module fesomc_gpu
      real(c_double) :: Rjc, ajc, bjc
      integer(c_int) :: key_rivers, filt_2D, filt_3D
      real(c_double) :: tau_inv_filt
+     integer(c_int) :: filt_bi_2D, filt_bi_3D
+     real(c_double) :: tau_c
   end type fcg_params
 
   ! the o_MESH arrays after the reference's preprocessing (pointers into host memory)

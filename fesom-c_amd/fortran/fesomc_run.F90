@@ -129,6 +129,9 @@ program fesomc_run
   par%filt_2D = merge(1, 0, getl('filt_2D'))
   par%filt_3D = merge(1, 0, getl('filt_3D'))
   par%tau_inv_filt = getf('tau_inv_filt', 0.0d0)
+  par%filt_bi_2D = merge(1, 0, getl('filt_bi_2D'))
+  par%filt_bi_3D = merge(1, 0, getl('filt_bi_3D'))
+  par%tau_c = getf('tau_c', 0.4d0)
 
   ! ---- read_mesh_ser (src/fv_main.F90:787-945) -----------------------------
   meshdir = trim(rundir) // '/' // trim(gets('MESHPATH', './MESH/'))
@@ -527,8 +530,8 @@ contains
   ! Options whose code this build does not hold stop the run before any set-up
   ! (the same list as the Python host's RunConfig.unsupported).
   subroutine refuse_unsupported()
-    character(len=20), parameter :: opts(7) = [character(len=20) :: 'filt_bi_2D', 'bih_2D', &
-         'filt_bi_3D', 'bih_3D', 'mixed_BP_grad_OB', 'T_potential', 'comp_sediment']
+    character(len=20), parameter :: opts(5) = [character(len=20) :: 'bih_2D', 'bih_3D', &
+         'mixed_BP_grad_OB', 'T_potential', 'comp_sediment']
     integer :: j
     logical :: bad
     bad = .false.

@@ -46,7 +46,8 @@ class Params(C.Structure):
                                        "vert_adv", "im_vert_visc", "im_vert_visc_fic", "fic_point", "wet_dry_on", "tf_presence",
                                        "restart", "cl_relax", "precision", "closure", "key_atm", "type_swr_body")] + \
                [(n, C.c_double) for n in ("Rjc", "ajc", "bjc")] + \
-               [(n, C.c_int) for n in ("key_rivers", "filt_2D", "filt_3D")] + [("tau_inv_filt", C.c_double)]
+               [(n, C.c_int) for n in ("key_rivers", "filt_2D", "filt_3D")] + [("tau_inv_filt", C.c_double)] + \
+               [(n, C.c_int) for n in ("filt_bi_2D", "filt_bi_3D")] + [("tau_c", C.c_double)]
 
 
 _IP = C.POINTER(C.c_int)
@@ -227,6 +228,8 @@ def params(cfg: RunConfig, precision=FCG_FP64, closure=FCG_CLOSURE_D3END) -> Par
     p.key_rivers = int(cfg.l("key_rivers"))
     p.filt_2D, p.filt_3D = int(cfg.l("filt_2D")), int(cfg.l("filt_3D"))
     p.tau_inv_filt = cfg.f("tau_inv_filt", 0.0)
+    p.filt_bi_2D, p.filt_bi_3D = int(cfg.l("filt_bi_2D")), int(cfg.l("filt_bi_3D"))
+    p.tau_c = cfg.f("tau_c", 0.4)
     return p
 
 

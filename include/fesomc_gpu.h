@@ -87,9 +87,17 @@ typedef struct fcg_params {
     /* neighbour-difference filters (fv_viscosity.F90:769-939): filt_2D on
      * the barotropic velocity, filt_3D on the 3-D velocity and its
      * pulsation part in the barotropic forcing, both at the rate
-     * tau_inv_filt [1/s]; the biharmonic and filt_bi options are not built */
+     * tau_inv_filt [1/s]; the biharmonic options (bih_2D, bih_3D) are not
+     * built */
     int filt_2D, filt_3D;
     double tau_inv_filt;
+    /* the twice-applied filters (viscosity_filt2x_2D/_3D, fv_viscosity.F90:
+     * 570-767; switched at fv_average_dynamic.F90:347 and fv_3D_vel.F90:112):
+     * filt_bi_2D on U_n_2D into UV2_rhs of update_2D_vel, filt_bi_3D on
+     * U_n*Je into UV_rhs of compute_vel_rhs, at the rate tau_c [1/day]
+     * (fv_read_run.F90:336-341) scaled by sqrt(scale_area/elem_area) */
+    int filt_bi_2D, filt_bi_3D;
+    double tau_c;
 } fcg_params;
 
 /*

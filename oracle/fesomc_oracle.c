@@ -53,6 +53,7 @@ typedef struct {
     double T_const, S_const, lat_cartesian, cyclic_length, clim_relax, relax2clim_ac;
     double Cd_bz, Bp_bz, pow_sig, lev_bot, lev_sur, SL_radius;
     double tau_inv_filt; /* relaxation rate of the neighbour-difference filters [1/s] */
+    double tau_c;        /* rate of the twice-applied filters filt_bi_2D/3D [1/day] */
     int type_swr_body, jer_const;   /* shortwave absorption profile (key_atm) */
     double Rjc, ajc, bjc;
     int nsteps, irep, cartesian, coriolis_tf, mom_adv_2D, mom_adv_3D, tracer_adv, vert_adv;
@@ -1285,9 +1286,115 @@ static void viscosity_filt_3D_to_2D(orc_t *o) {
     }
 }
 
+/* The twice-applied neighbour-difference filters (src/fv_viscosity.F90:570-767).
+ * First pass: U_c = K_bi * sum over the neighbours of (own - neighbour), with
+ * K_bi = tau_inv*sqrt(scale_area/elem_area); second pass: the tendency is the
+ * sum over the neighbours of (neighbour's U_c - own U_c).  The neighbour rule is
+ * the routines' own: slots 1-3 when present, slot 4 when present and not equal
+ * to slot 1 (a triangle repeats slot 1 there).  Both passes keep the reference's
+ * first-term-then-accumulate structure. */
+static int f2x_take(const int *nb, int j) {
+    if (nb[j] < 0) return 0;
+    return j < 3 || nb[3] != nb[0];
+}
+
+/* viscosity_filt2x_3D: src/fv_viscosity.F90:570-670, on U_n*Je into UV_rhs */
+static void viscosity_filt2x_3D(orc_t *o) {
+    const int L = o->L, E = o->E;
+    const double tau_inv = o->dt * o->tau_c / 3600.0 / 24;
+    double *Uc = DALLOC((size_t)E * L), *Vc = DALLOC((size_t)E * L);
+    for (int el = 0; el < E; el++) {
+        const double K_bi = tau_inv * sqrt(o->scale_area / o->elem_area[el]);
+        const int *nb = o->elem_nb + 4 * el;
+        for (int z = 0; z < L; z++) {
+            const size_t i = (size_t)el * L + z;
+            double u = 0.0, v = 0.0;
+            for (int j = 0; j < 4; j++) {
+                if (!f2x_take(nb, j)) continue;
+                const size_t q = (size_t)nb[j] * L + z;
+                if (j == 0) {
+                    u = o->U_n[i] * o->Je[i] - o->U_n[q] * o->Je[q];
+                    v = o->V_n[i] * o->Je[i] - o->V_n[q] * o->Je[q];
+                } else {
+                    u = u + o->U_n[i] * o->Je[i] - o->U_n[q] * o->Je[q];
+                    v = v + o->V_n[i] * o->Je[i] - o->V_n[q] * o->Je[q];
+                }
+            }
+            Uc[i] = u * (MUT(18) ? tau_inv : K_bi);
+            Vc[i] = v * (MUT(18) ? tau_inv : K_bi);
+        }
+    }
+    for (int el = 0; el < E; el++) {
+        const int *nb = o->elem_nb + 4 * el;
+        for (int z = 0; z < L; z++) {
+            const size_t i = (size_t)el * L + z;
+            double u = 0.0, v = 0.0;
+            for (int j = 0; j < 4; j++) {
+                if (!f2x_take(nb, j)) continue;
+                const size_t q = (size_t)nb[j] * L + z;
+                if (j == 0) {
+                    u = -Uc[i] + Uc[q];
+                    v = -Vc[i] + Vc[q];
+                } else if (MUT(19)) {
+                    u = u + Uc[i] - Uc[q];
+                    v = v + Vc[i] - Vc[q];
+                } else {
+                    u = u - Uc[i] + Uc[q];
+                    v = v - Vc[i] + Vc[q];
+                }
+            }
+            o->UV_rhs[2 * i] = u;
+            o->UV_rhs[2 * i + 1] = v;
+        }
+    }
+    free(Uc); free(Vc);
+}
+
+/* viscosity_filt2x_2D: src/fv_viscosity.F90:672-767, on U_n_2D into UV2_rhs */
+static void viscosity_filt2x_2D(orc_t *o) {
+    const int E = o->E;
+    const double tau_inv = (MUT(20) ? o->dt : o->dt_2D) * o->tau_c / 3600.0 / 24.0;
+    double *Uc = DALLOC((size_t)E), *Vc = DALLOC((size_t)E);
+    for (int el = 0; el < E; el++) {
+        const double K_bi = tau_inv * sqrt(o->scale_area / o->elem_area[el]);
+        const int *nb = o->elem_nb + 4 * el;
+        double u = 0.0, v = 0.0;
+        for (int j = 0; j < 4; j++) {
+            if (!f2x_take(nb, j)) continue;
+            if (j == 0) {
+                u = o->U_n_2D[2 * el] - o->U_n_2D[2 * nb[j]];
+                v = o->U_n_2D[2 * el + 1] - o->U_n_2D[2 * nb[j] + 1];
+            } else {
+                u = u + o->U_n_2D[2 * el] - o->U_n_2D[2 * nb[j]];
+                v = v + o->U_n_2D[2 * el + 1] - o->U_n_2D[2 * nb[j] + 1];
+            }
+        }
+        Uc[el] = u * K_bi;
+        Vc[el] = v * K_bi;
+    }
+    for (int el = 0; el < E; el++) {
+        const int *nb = o->elem_nb + 4 * el;
+        double u = 0.0, v = 0.0;
+        for (int j = 0; j < 4; j++) {
+            if (!f2x_take(nb, j)) continue;
+            if (j == 0) {
+                u = -Uc[el] + Uc[nb[j]];
+                v = -Vc[el] + Vc[nb[j]];
+            } else {
+                u = u - Uc[el] + Uc[nb[j]];
+                v = v - Vc[el] + Vc[nb[j]];
+            }
+        }
+        o->UV2_rhs[2 * el] = u;
+        o->UV2_rhs[2 * el + 1] = v;
+    }
+    free(Uc); free(Vc);
+}
+
 /* Test hook: one filter alone on the current state, its target zeroed first
  * (2: filt_2D into U_rhs_2D, 3: filt_3D into U_rhs/V_rhs, 4: filt_3D_to_2D
- * into U_rhs_2D_3D). */
+ * into U_rhs_2D_3D, 5: filt_bi_2D into UV2_rhs, 6: filt_bi_3D into UV_rhs;
+ * the last two assign their whole target). */
 int orc_filter_only(orc_t *o, int which) {
     if (!o) return -1;
     if (which == 2) {
@@ -1300,6 +1407,10 @@ int orc_filter_only(orc_t *o, int which) {
     } else if (which == 4) {
         memset(o->U_rhs_2D_3D, 0, sizeof(double) * 2 * (size_t)o->E);
         viscosity_filt_3D_to_2D(o);
+    } else if (which == 5) {
+        viscosity_filt2x_2D(o);
+    } else if (which == 6) {
+        viscosity_filt2x_3D(o);
     } else {
         return -1;
     }
@@ -1354,7 +1465,10 @@ int orc_wad_mask_only(orc_t *o) {
  * 13 Miura edge x-length in metres (the reference's a = 1 workaround, fv_tracer.F90:1018-1026, undone)
  * 14 psi_m with pi/4 for pi/2                            15 ocean albedo left out of the net shortwave
  * 16 latent heat with the wrong sign in qns              17 OB climatology profile taken at the level (zbar)
- *                                                           instead of the layer mid-depth Z */
+ *                                                           instead of the layer mid-depth Z
+ * 18 filt_bi_3D without the sqrt(scale_area/elem_area) factor of K_bi
+ * 19 filt_bi_3D second pass accumulating own minus neighbour
+ * 20 filt_bi_2D rate from the internal step dt instead of dt_2D */
 int orc_set_mutation(int id) {
     g_mut = id;
     return 0;
@@ -1389,6 +1503,7 @@ static void compute_vel_rhs(orc_t *o) {
     vel_gradients(o);
     if (o->mom_adv_3D == 1) momentum_adv_upwind(o);
     if (o->filt_3D) viscosity_filt_3D(o);   /* src/fv_3D_vel.F90:111 */
+    if (o->filt_bi_3D) viscosity_filt2x_3D(o);   /* src/fv_3D_vel.F90:112 */
     if (o->vert_adv == 1) momentum_vert_adv_upwind(o);
     double ff = (1.5 + EPS);
     if (o->lfirst) { ff = 1.0; o->lfirst = 0; }
@@ -1574,6 +1689,7 @@ static void update_2D_vel(orc_t *o) {
     free(eta);
     if (o->mom_adv_2D == 2) momentum_adv_upwind_2D(o);
     if (o->filt_2D) viscosity_filt_2D(o);   /* src/fv_average_dynamic.F90:346 */
+    if (o->filt_bi_2D) viscosity_filt2x_2D(o);   /* src/fv_average_dynamic.F90:347 */
     if (o->type_task > 2)
         for (int el = 0; el < o->E; el++) {
             o->U_rhs_2D[2 * el] = o->U_rhs_2D[2 * el] + o->U_rhs_2D_3D[2 * el];
@@ -2666,6 +2782,7 @@ static void parse_config(orc_t *o, const char *t) {
     o->SL_obn = cfg_l(t, "SL_obn", 0) && !cfg_l(t, "SL_obn_user_def", 0); /* src/fv_main.F90:297 */
     o->SL_radius = cfg_f(t, "SL_radius", 0.0);
     o->tau_inv_filt = cfg_f(t, "tau_inv_filt", 0.0);
+    o->tau_c = cfg_f(t, "tau_c", 0.4);
     o->type_swr_body = cfg_i(t, "type_swr_body", 2);
     o->jer_const = cfg_l(t, "jer_const", 1);
     o->Rjc = cfg_f(t, "R", 0.58); o->ajc = cfg_f(t, "a", 0.35); o->bjc = cfg_f(t, "b", 23.0);
@@ -2696,7 +2813,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     orc_t *o = (orc_t *)xcalloc(1, sizeof(orc_t));
     parse_config(o, cfg);
     if (o->type_task != 3) { o->status = -10; snprintf(o->errmsg, sizeof o->errmsg, "only type_task=3"); return o; }
-    if (o->filt_bi_2D || o->bih_2D || o->filt_bi_3D || o->bih_3D || o->mixed_BP_grad_OB
+    if (o->bih_2D || o->bih_3D || o->mixed_BP_grad_OB
         || (o->key_atm && o->type_swr_body == 1 && !o->jer_const) || o->T_potential || o->ver_mix != 3
         || (o->SL_obn && !(o->SL_radius > 0.0))) {
         o->status = -14;
@@ -2905,6 +3022,7 @@ static double *fld(orc_t *o, const char *name, long *cnt) {
     F("eta_n", o->eta_n, N) F("eta_n_1", o->eta_n_1, N) F("eta_n_2", o->eta_n_2, N) F("etaAB", o->etaAB, N)
     F("ssh_rhs", o->ssh_rhs, N) F("U_n_2D", o->U_n_2D, 2 * E) F("U_n_1", o->U_n_1, 2 * E) F("U_n_2", o->U_n_2, 2 * E)
     F("UAB", o->UAB, 2 * E) F("U_rhs_2D", o->U_rhs_2D, 2 * E) F("U_rhs_2D_3D", o->U_rhs_2D_3D, 2 * E)
+    F("UV2_rhs", o->UV2_rhs, 2 * E) F("UV_rhs", o->UV_rhs, 2 * E * L)
     F("taux", o->taux, E) F("tauy", o->tauy, E) F("windx", o->windx, N) F("windy", o->windy, N)
     F("Jc", o->Jc, N * L) F("Jc_old", o->Jc_old, N * L) F("Je", o->Je, E * L) F("Jd", o->Jd, D * L)
     F("Z", o->Z, N * L) F("zbar", o->zbar, N * K) F("hpressure", o->hpressure, N * K)

@@ -146,7 +146,8 @@ class Oracle:
 
     def filter_only(self, which):
         """One neighbour-difference filter alone on the current state into its
-        zeroed target (2: filt_2D, 3: filt_3D, 4: filt_3D_to_2D)."""
+        zeroed target (2: filt_2D, 3: filt_3D, 4: filt_3D_to_2D; 5: filt_bi_2D
+        into UV2_rhs, 6: filt_bi_3D into UV_rhs)."""
         if lib().orc_filter_only(self.h, int(which)):
             raise ValueError(f"no filter {which}")
 

@@ -250,17 +250,52 @@ def test_filters_match_oracle():
     assert np.abs(o0.get("U_n") - o.get("U_n")).max() > 1e-6 * np.abs(o.get("U_n")).max()
 
 
+def test_twice_applied_filters_match_oracle():
+    """filt_bi_2D and filt_bi_3D (viscosity_filt2x_2D/_3D, fv_viscosity.F90:
+    570-767) on the stratified wet/dry channel at tau_c = 20/day: the 2-D
+    filter's first pass before the barotropic element update of every
+    substep and its second pass inside it (UV2_rhs), the 3-D one as two
+    element-layer passes whose UV_rhs enters the momentum update: bit for bit
+    against the oracle, and visibly different from the unfiltered run."""
+    import re
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    for k, v in (("filt_bi_2D", "T"), ("filt_bi_3D", "T"), ("tau_c", "20.0")):
+        txt, n = re.subn(rf"^(\s*{k}\s*=\s*)\S+", rf"\g<1>{v}", txt, count=1, flags=re.M)
+        assert n == 1, k
+    g, o = _model(txt, mesh, ex)
+    print("filt_bi: c2_small, 5 steps", flush=True)
+    g.step(5)
+    o.step(5)
+    _compare(g, o, FIELDS)
+    assert g.diagnostics()["status"] == 0
+    g.close()
+    assert np.abs(o.get("UV_rhs")).max() > 0 and np.abs(o.get("UV2_rhs")).max() > 0
+    plain, _, _ = meshgen.config("c2_small")
+    o0 = Oracle(plain, mesh)
+    o0.set_tracers(ex["T"].reshape(-1), ex["S"].reshape(-1))
+    wx, wy = ex["wind"]
+    tx, ty = meshgen.wind_stress(wx, wy)
+    for k, v in (("windx", np.full(mesh.nod2D, wx)), ("windy", np.full(mesh.nod2D, wy)),
+                 ("taux", np.full(mesh.elem2D, tx)), ("tauy", np.full(mesh.elem2D, ty))):
+        o0.set(k, v)
+    o0.step(5)
+    assert np.abs(o0.get("U_n") - o.get("U_n")).max() > 1e-6 * np.abs(o.get("U_n")).max()
+    assert np.abs(o0.get("U_n_2D") - o.get("U_n_2D")).max() > 1e-6 * np.abs(o.get("U_n_2D")).max()
+
+
 def test_mixed_precision_with_rivers_filters_and_forcing():
     """The FP32-storage variants of the river, filter and forcing kernels
     (k_tr_rivers, k_velrhs_atm_filt, the filter terms of k_bt_elem2 and
-    k_bt_p1elem) against the FP64 device run on the same case: after 20
+    k_bt_p1elem, the twice-applied filters' passes) against the FP64 device
+    run on the same case: after 20
     steps the elevation agrees to 1e-4 m RMS and T, S to 1e-3 RMS."""
     import re
     from fesomc_amd import gpu, meshgen
     from fesomc_amd.runconfig import RunConfig
     txt, mesh, ex = meshgen.config("c2_small")
     for k, v in (("filt_2D", "T"), ("filt_3D", "T"), ("tau_inv_filt", "1.0e-3"), ("key_rivers", "T"),
-                 ("key_atm", "T")):
+                 ("key_atm", "T"), ("filt_bi_2D", "T"), ("filt_bi_3D", "T"), ("tau_c", "20.0")):
         txt, n = re.subn(rf"^(\s*{k}\s*=\s*)\S+", rf"\g<1>{v}", txt, count=1, flags=re.M)
         assert n == 1, k
     x, y = mesh.xy[:, 0], mesh.xy[:, 1]

@@ -488,3 +488,80 @@ def test_buffer_zone_factors_act_only_inside_the_sponge_layer():
     txt = _set_key(txt, "SL_obn", "F")
     _check_no_sponge_ignores_buffer_factors(txt, mesh, ex)
     assert _mutated(11, lambda: _check_no_sponge_ignores_buffer_factors(txt, mesh, ex))
+
+
+# ---------------------------------------------------------------------------
+# Twice-applied filters (viscosity_filt2x_2D/_3D, src/fv_viscosity.F90:570-767)
+# ---------------------------------------------------------------------------
+def _filt_bi_case():
+    from fesomc_amd import meshgen
+    txt, mesh, ex = meshgen.config("c2_small")
+    for k in ("filt_bi_2D", "filt_bi_3D"):
+        txt = _set_key(txt, k, "T")
+    txt = _set_key(txt, "tau_c", "0.7")
+    return txt, mesh, ex
+
+
+def _edge_laplacian(o):
+    """Graph Laplacian of the elements from the internal edges (edge_tri),
+    an assembly independent of the elem_neighbors slots the filters walk:
+    (B u)_e = sum over the elements sharing an edge with e of (u_e - u_nb)."""
+    import scipy.sparse as sp
+    E = o.size["E"]
+    et = o.get_int("edge_tri").reshape(-1, 2)
+    et = et[(et >= 0).all(1)]
+    pairs = {tuple(sorted(p)) for p in et.tolist()}
+    assert len(pairs) == len(et), "two elements share more than one edge"
+    a, b = et[:, 0], et[:, 1]
+    A = sp.coo_matrix((np.ones(2 * len(a)), (np.r_[a, b], np.r_[b, a])), shape=(E, E)).tocsr()
+    return sp.diags(np.asarray(A.sum(1)).ravel()) - A
+
+
+def _check_filt_bi(o, txt):
+    """The tendency equals -B K B u, K = tau_c*dt/86400*sqrt(scale_area/area)
+    (dt_2D in 2-D, dt in 3-D on u = U_n*Je per layer), B the element graph
+    Laplacian: the composition of two neighbour differences, each the
+    negative of a discrete Laplacian, so it is symmetric and dissipates
+    (u . tendency <= 0).  A uniform field gives nothing."""
+    from fesomc_amd.runconfig import RunConfig
+    cfg = RunConfig(txt)
+    E, L = o.size["E"], o.size["L"]
+    area = o.get("elem_area")
+    sa = area.sum() / E
+    assert abs(sa - o.scalar("scale_area")) <= 1e-12 * sa
+    B = _edge_laplacian(o)
+    rng = np.random.default_rng(11)
+    # 2-D on U_n_2D
+    k2 = cfg.f("dt_2D") * 0.7 / 86400.0 * np.sqrt(sa / area)
+    u = rng.standard_normal(2 * E)
+    o.set("U_n_2D", u)
+    o.filter_only(5)
+    r = o.get("UV2_rhs").reshape(E, 2)
+    for c in range(2):
+        want = -(B @ (k2 * (B @ u[c::2])))
+        np.testing.assert_allclose(r[:, c], want, rtol=0, atol=1e-12 * np.abs(want).max())
+        assert np.dot(u[c::2], r[:, c]) < 0
+    o.set("U_n_2D", np.tile([0.4, -0.1], E))
+    o.filter_only(5)
+    assert np.abs(o.get("UV2_rhs")).max() < 1e-14
+    # 3-D on U_n*Je, layer by layer
+    k3 = cfg.f("dt") * 0.7 / 86400.0 * np.sqrt(sa / area)
+    U = rng.standard_normal(E * L)
+    V = rng.standard_normal(E * L)
+    o.set("U_n", U)
+    o.set("V_n", V)
+    je = o.get("Je").reshape(E, L)
+    o.filter_only(6)
+    r = o.get("UV_rhs").reshape(E, L, 2)
+    for c, F in enumerate((U, V)):
+        p = F.reshape(E, L) * je
+        want = -(B @ (k3[:, None] * (B @ p)))
+        np.testing.assert_allclose(r[:, :, c], want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_twice_applied_filters_are_minus_a_scaled_squared_laplacian():
+    txt, mesh, _ = _filt_bi_case()
+    o = Oracle(txt, mesh)
+    _check_filt_bi(o, txt)
+    for mid in (18, 19, 20):   # sqrt factor dropped; second-pass sign; dt for dt_2D
+        assert _mutated(mid, lambda: _check_filt_bi(o, txt)), mid
