@@ -57,6 +57,14 @@ __global__ void k_bt_elem2(DevMesh, DevStateT<R>, double, double, double, double
 template <typename R> __global__ void k_filt2x_3d_c(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_filt2x_3d_uv(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_bt_filt2x_c(DevMesh, DevStateT<R>, double);
+template <typename R>
+__global__ void k_bh3_edge(DevMesh, DevStateT<R>, const R *, const R *, const R *, const R *, const R *, const R *, int,
+                           int);
+template <typename R> __global__ void k_bh3_elem(DevMesh, DevStateT<R>, R *, R *, int);
+template <typename R> __global__ void k_bh3_grad(DevMesh, DevStateT<R>, const R *, const R *, int);
+template <typename R> __global__ void k_bh_puls(DevMesh, DevStateT<R>);
+template <typename R> __global__ void k_bh2_edge(DevMesh, DevStateT<R>, const R *, const R *);
+template <typename R> __global__ void k_bh2_elem(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_jac_node(DevMesh, DevStateT<R>, double);
 template <typename R> __global__ void k_je(DevMesh, DevStateT<R>);
 template <typename R> __global__ void k_jac_restart(DevMesh, DevStateT<R>, double);
@@ -400,8 +408,38 @@ int enqueue_substeps(fcg_ctx *ctx, const DevStateT<R> &s0) {
                                    st, m, (const R *)s.U_n, (const R *)s.V_n, s.Unode_p, s.Vnode_p,
                                    (const double *)s.puls_ap, (const double *)s.U2);
             hipLaunchKernelGGL(fcgk::k_bt_p1edge<R>, dim3(nflatblk(m.D, m.L)), dim3(kFlatT), lds_p1edge(m), st, m, s);
+            if (p.bih_3D) {  // biharmonic_viscosity_3D_to_2D (src/fv_main.F90:1501, fv_viscosity.F90:964-1220);
+                             // k_bt_p1elem adds the column sum
+                const unsigned gd = (unsigned)(((size_t)m.D * m.L + kFlatT - 1) / kFlatT);
+                const unsigned ge = (unsigned)(((size_t)m.E * m.L + kFlatT - 1) / kFlatT);
+                hipLaunchKernelGGL(fcgk::k_bh_puls<R>, dim3(ge), dim3(kFlatT), 0, st, m, s);
+                hipLaunchKernelGGL(fcgk::k_bh3_grad<R>, dim3(ge), dim3(kFlatT), 0, st, m, s, (const R *)s.bhpu,
+                                   (const R *)s.bhpv, 1);
+                hipLaunchKernelGGL(fcgk::k_bh3_edge<R>, dim3(gd), dim3(kFlatT), 0, st, m, s, (const R *)s.bhpu,
+                                   (const R *)s.bhpv, (const R *)s.bhgux, (const R *)s.bhguy, (const R *)s.bhgvx,
+                                   (const R *)s.bhgvy, 1, 0);
+                hipLaunchKernelGGL(fcgk::k_bh3_elem<R>, dim3(ge), dim3(kFlatT), 0, st, m, s, s.bhcu, s.bhcv, 2);
+                hipLaunchKernelGGL(fcgk::k_bh3_grad<R>, dim3(ge), dim3(kFlatT), 0, st, m, s, (const R *)s.bhcu,
+                                   (const R *)s.bhcv, 0);
+                hipLaunchKernelGGL(fcgk::k_bh3_edge<R>, dim3(gd), dim3(kFlatT), 0, st, m, s, (const R *)s.bhcu,
+                                   (const R *)s.bhcv, (const R *)s.bhgux, (const R *)s.bhguy, (const R *)s.bhgvx,
+                                   (const R *)s.bhgvy, 0, 0);
+                hipLaunchKernelGGL(fcgk::k_bh3_elem<R>, dim3(ge), dim3(kFlatT), 0, st, m, s, s.bhub, s.bhvb, 0);
+            }
             hipLaunchKernelGGL(fcgk::k_bt_p1elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s,
                                p.filt_3D ? p.tau_inv_filt : 0.0);
+        }
+        if (p.bih_2D) {  // biharmonic_2D (src/fv_average_dynamic.F90:348, fv_viscosity.F90:338-568): first
+                         // pass on UAB, Ul*Ah*acc, its gradients; k_bt_elem2 adds the second pass
+            hipLaunchKernelGGL(fcgk::k_bh2_edge<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, (const R *)s.UAB,
+                               (const R *)s.vel_grad);
+            hipLaunchKernelGGL(fcgk::k_bh2_elem<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s);
+            DevStateT<R> sg = s;  // vel_gradients_2D(Ul)
+            sg.UAB = s.bh_ul;
+            sg.vel_grad = s.bh_g2;
+            hipLaunchKernelGGL(fcgk::k_bt_velgrad<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, sg);
+            hipLaunchKernelGGL(fcgk::k_bh2_edge<R>, dim3(nblk(m.D)), dim3(kBlk), 0, st, m, s, (const R *)s.bh_ul,
+                               (const R *)s.bh_g2);
         }
         if (p.filt_bi_2D)  // src/fv_average_dynamic.F90:347 (tau_inv: fv_viscosity.F90:695)
             hipLaunchKernelGGL(fcgk::k_bt_filt2x_c<R>, dim3(nblk(m.E)), dim3(kBlk), 0, st, m, s,
@@ -607,9 +645,10 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     std::vector<int4> en(E), nb(E), eled(E), ied(E);
     std::vector<double4> wcv(E), gsx(E), gsy(E), gvx(E), gvy(E);
     std::vector<double> accB(E), acc_el(E), v_ea(E), v_ec(E), v_co(E), kbi;
-    const bool filt_bi = p.filt_bi_2D || p.filt_bi_3D;
-    std::vector<double> sarea(nmemb, mesh->scale_area);
-    if (filt_bi && node_member) {  // scale_area of each member: sum(elem_area)/elem2D (fv_ini.F90:81)
+    const bool filt_bi = p.filt_bi_2D || p.filt_bi_3D, bih = p.bih_2D || p.bih_3D;
+    std::vector<double> sarea(nmemb, mesh->scale_area), bh_ah, bh_ah_acc;
+    std::vector<int4> el_bed;
+    if ((filt_bi || bih) && node_member) {  // scale_area of each member: sum(elem_area)/elem2D (fv_ini.F90:81)
         std::vector<double> sum(nmemb, 0.0);
         std::vector<long> cnt(nmemb, 0);
         for (int e = 0; e < E; ++e) {
@@ -620,6 +659,11 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         for (int b = 0; b < nmemb; ++b) sarea[b] = cnt[b] ? sum[b] / (double)cnt[b] : 0.0;
     }
     if (filt_bi) kbi.resize(E);
+    if (bih) {
+        bh_ah.resize(E);
+        bh_ah_acc.resize(E);
+        el_bed.resize(E);
+    }
     for (int i = 0; i < E; ++i) {
         const int e = pe[i];
         const int *q = mesh->elem2D_nodes + 4 * e;
@@ -644,8 +688,23 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         accB[i] = std::pow(acc, p.Bp_bz) * 1.0;  // mask_bpr = 1 (Mask_Bar_pr = .false., src/fv_var.F90:57)
         ied[i] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
         v_ea[i] = mesh->elem_area[e];
+        const double sa = sarea[node_member ? node_member[q[0] - 1] : 0];
         if (filt_bi)  // the factor of K_bi (fv_viscosity.F90:599, 706)
-            kbi[i] = std::sqrt(sarea[node_member ? node_member[q[0] - 1] : 0] / mesh->elem_area[e]);
+            kbi[i] = std::sqrt(sa / mesh->elem_area[e]);
+        if (bih) {  // Ah of biharmonic_viscosity/_2D and of the 3-D-to-2-D form (fv_viscosity.F90:232, 465, 1088-1089)
+            bh_ah[i] = -p.Abh0 * std::sqrt(mesh->elem_area[e] / sa) / sa;
+            bh_ah_acc[i] = -acc * p.Abh0 * std::sqrt(mesh->elem_area[e] / sa) / sa;
+            // boundary edges in ascending reference id: after all internal edges
+            // in the serial loops (ed = edge2D_in+1..edge2D)
+            int b[4], nbnd = 0;
+            const int nq = (d[0] == d[3]) ? 3 : 4;
+            for (int j = 0; j < nq; ++j)
+                if (mesh->edge_tri[2 * (d[j] - 1) + 1] <= 0) b[nbnd++] = d[j];
+            std::sort(b, b + nbnd);
+            int bs[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+            for (int j = 0; j < nbnd; ++j) bs[j] = QD(b[j]);
+            el_bed[i] = make_int4(bs[0], bs[1], bs[2], bs[3]);
+        }
         v_ec[i] = mesh->elem_cos[e];
         v_co[i] = mesh->coriolis[e];
     }
@@ -653,6 +712,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     std::vector<int2> ednodes(D), edtri(D);
     std::vector<double4> ecross(D);
     std::vector<double2> edxdy(D);
+    std::vector<BhEdge> bhe(bih ? D : 0);
     for (int i = 0; i < D; ++i) {
         const int d = pd[i];
         ednodes[i] = make_int2(QN(mesh->edge_nodes[2 * d]), QN(mesh->edge_nodes[2 * d + 1]));
@@ -660,6 +720,26 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
         const double *c = mesh->edge_cross_dxdy + 4 * d;
         ecross[i] = make_double4(c[0], c[1], c[2], c[3]);
         edxdy[i] = make_double2(mesh->edge_dxdy[2 * d], mesh->edge_dxdy[2 * d + 1]);
+        if (bih) {  // biharmonic edge geometry (fv_viscosity.F90:148-156, 208-210, 381-392)
+            BhEdge &g = bhe[i];
+            const int t1 = mesh->edge_tri[2 * d] - 1, t2 = mesh->edge_tri[2 * d + 1] - 1;
+            const double dx = mesh->edge_dxdy[2 * d], dy = mesh->edge_dxdy[2 * d + 1];
+            g = BhEdge{};
+            g.ye = dy * kREarthD;
+            g.acc = (mesh->ac[mesh->edge_nodes[2 * d] - 1] + mesh->ac[mesh->edge_nodes[2 * d + 1] - 1]) / 2.0;
+            if (t2 >= 0) {
+                g.xe = dx * kREarthD * 0.5 * (mesh->elem_cos[t1] + mesh->elem_cos[t2]);
+                g.tx = -c[0] + c[2];
+                g.ty = -c[1] + c[3];
+                g.tl = std::sqrt(g.tx * g.tx + g.ty * g.ty);
+                g.tx = g.tx / g.tl;
+                g.ty = g.ty / g.tl;
+            } else {
+                g.xb = dx * kREarthD * mesh->elem_cos[t1];
+                g.xe = dx * kREarthD;
+                g.xbd = g.xe * mesh->elem_cos[t1];
+            }
+        }
     }
     std::vector<std::vector<int>> n2e(N);
     std::vector<int> icnt(E, 0);
@@ -747,6 +827,9 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     if (filt_bi) {
         UP(m.kbi, kbi)
     }
+    if (bih) {
+        UP(m.bh_ah, bh_ah) UP(m.bh_ah_acc, bh_ah_acc) UP(m.el_bed, el_bed) UP(m.bhe, bhe)
+    }
     UP(m.ednodes, ednodes) UP(m.edtri, edtri) UP(m.ecross, ecross) UP(m.edxdy, edxdy)
     UP(m.depth, depth) UP(m.area, area) UP(m.ac, ac) UP(m.atp1, atp1) UP(m.atp2, atp2) UP(m.index, index)
     UP(m.obidx, obidx) UP(m.n2e_ptr, n2e_ptr) UP(m.n2e, n2e_idx) UP(m.n2el_ptr, n2el_ptr) UP(m.n2el, n2el_idx)
@@ -814,6 +897,15 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     }
     if (p.filt_bi_2D) {
         DA(s.fb2c, 2 * E)
+    }
+    if (p.bih_3D) {
+        const size_t DK = (size_t)D * K;
+        DA3(s.bhfu, DK) DA3(s.bhfv, DK) DA3(s.bhcu, EL) DA3(s.bhcv, EL)
+        DA3(s.bhgux, EL) DA3(s.bhguy, EL) DA3(s.bhgvx, EL) DA3(s.bhgvy, EL)
+        DA3(s.bhpu, EL) DA3(s.bhpv, EL) DA3(s.bhub, EL) DA3(s.bhvb, EL)
+    }
+    if (p.bih_2D) {
+        DA(s.bh_fl2, 2 * (size_t)D) DA3(s.bh_ul, 2 * (size_t)E) DA3(s.bh_g2, 4 * (size_t)E)
     }
     DA3(s.vxx, EK) DA3(s.vyu, EK) DA3(s.vyv, EK)
     DA3(s.TF, NL) DA3(s.SF, NL) DA3(s.T_old, NL) DA3(s.S_old, NL) DA3(s.Tclim, NL) DA3(s.Sclim, NL)
@@ -1353,6 +1445,16 @@ static int compute_vel_rhs_impl(fcg_ctx *ctx, const DevStateT<R> &s) {
     else
         LAUNCH_EX(ctx, k_velgrad3d<R>, nflatblk(ctx->m.E, ctx->m.L), kFlatT, cpb_of(ctx->m.L) * sizeof(VelGradRec),
                   ctx->m, s);
+    if (p.bih_3D) {  // src/fv_3D_vel.F90:113, the first pass, Ah and the gradients of U_c; k_velrhs
+                     // adds the second pass's edge fluxes after the filters (fv_viscosity.F90:104-336)
+        const DevMesh &m = ctx->m;
+        const unsigned gd = (unsigned)(((size_t)m.D * m.L + kFlatT - 1) / kFlatT);
+        const unsigned ge = (unsigned)(((size_t)m.E * m.L + kFlatT - 1) / kFlatT);
+        LAUNCH_EX(ctx, k_bh3_edge<R>, gd, kFlatT, 0, m, s, s.U_n, s.V_n, s.gux, s.guy, s.gvx, s.gvy, 1, 0);
+        LAUNCH_EX(ctx, k_bh3_elem<R>, ge, kFlatT, 0, m, s, s.bhcu, s.bhcv, 1);
+        LAUNCH_EX(ctx, k_bh3_grad<R>, ge, kFlatT, 0, m, s, s.bhcu, s.bhcv, 0);
+        LAUNCH_EX(ctx, k_bh3_edge<R>, gd, kFlatT, 0, m, s, s.bhcu, s.bhcv, s.bhgux, s.bhguy, s.bhgvx, s.bhgvy, 1, 1);
+    }
     if (p.filt_bi_3D) {  // src/fv_3D_vel.F90:112 (tau_inv: fv_viscosity.F90:586); UV_rhs enters k_velrhs
         const unsigned g = (unsigned)(((size_t)ctx->m.E * ctx->m.L + kFlatT - 1) / kFlatT);  // dense (element, layer)
         LAUNCH_EX(ctx, k_filt2x_3d_c<R>, g, kFlatT, 0, ctx->m, s, p.dt * p.tau_c / 3600.0 / 24);

@@ -84,6 +84,15 @@ struct P1Edge {
     double xe, ye, an1, an2;
 };
 
+// Geometry of one edge for the biharmonic operators (fv_viscosity.F90:148-156,
+// 208-210, 381-392): internal edges: the normal (ye, -xe) with the mean cosine
+// of the two elements and the unit centre-to-centre direction (tx, ty) over
+// its length tl; boundary edges: xb = dx*R*cos of the element (3-D loops) and
+// xe = dx*R, xbd = xe*cos (2-D loop); acc = (ac1+ac2)/2.
+struct BhEdge {
+    double xe, ye, tx, ty, tl, acc, xb, xbd;
+};
+
 // Device mirror of the mesh and state of one model instance.
 struct DevMesh {
     int N = 0, E = 0, D = 0, Din = 0, L = 0, K = 0, nobn = 0;
@@ -98,6 +107,13 @@ struct DevMesh {
     double *elem_area = nullptr, *elem_cos = nullptr, *coriolis = nullptr;
     double *accB = nullptr;    // (sum w_cv*ac)**Bp_bz * mask_bpr, host-evaluated with libm pow
     double *kbi = nullptr;     // sqrt(scale_area/elem_area), scale_area of the element's member (filt_bi)
+    // biharmonic options (bih_2D, bih_3D): per element the boundary edges in
+    // ascending reference id (kEmpty after the last) and the viscosity factors
+    // Ah = -Abh0*sqrt(elem_area/scale_area)/scale_area and its 3-D-to-2-D form
+    // with the sponge weight in front; per edge the corrected-Laplacian geometry
+    int4 *el_bed = nullptr;
+    double *bh_ah = nullptr, *bh_ah_acc = nullptr;
+    struct BhEdge *bhe = nullptr;
     double *acc_el = nullptr;  // sum w_cv*ac
     // edge constants
     int2 *ednodes = nullptr;   // 0-based
@@ -175,6 +191,15 @@ struct DevStateT {
     // pass (2E, null when filt_bi_2D is off)
     R *fbu = nullptr, *fbv = nullptr, *uvu = nullptr, *uvv = nullptr;
     double *fb2c = nullptr;
+    // biharmonic options: per (edge, layer) fluxes; per (element, layer) the
+    // first-pass field U_c, its gradients and the pulsation field (then the
+    // 3-D-to-2-D result); 2-D: per edge fluxes, the first-pass field Ul and its
+    // gradients (null when the option is off)
+    R *bhfu = nullptr, *bhfv = nullptr, *bhcu = nullptr, *bhcv = nullptr;
+    R *bhgux = nullptr, *bhguy = nullptr, *bhgvx = nullptr, *bhgvy = nullptr;
+    R *bhpu = nullptr, *bhpv = nullptr, *bhub = nullptr, *bhvb = nullptr;
+    double *bh_fl2 = nullptr;
+    R *bh_ul = nullptr, *bh_g2 = nullptr;
 
     R *vxx = nullptr, *vyu = nullptr, *vyv = nullptr;                         // K*E Thomas scratch
     R *TF = nullptr, *SF = nullptr, *T_old = nullptr, *S_old = nullptr;       // L*N

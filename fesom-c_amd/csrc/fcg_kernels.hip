@@ -901,6 +901,76 @@ static_assert(sizeof(ElemRec) <= kVelrhsElemBytes && alignof(ElemRec) <= 16 + 16
 static_assert(sizeof(EdgeRec<double>) == velrhs_edge_bytes(8) && sizeof(EdgeRec<float>) == velrhs_edge_bytes(4),
               "edge record size");
 
+// Biharmonic viscosity helpers (the kernels are with the barotropic filters below).
+// corrected normal flux across an internal edge (fv_viscosity.F90:158-169):
+// the mean gradient with its centre-line component replaced by the difference
+// quotient, dotted with the edge normal
+__device__ __forceinline__ double bh_flux(const BhEdge &g, double gx1, double gx2, double gy1, double gy2, double f1,
+                                          double f2) {
+    double g1 = 0.5 * (gx1 + gx2), g2 = 0.5 * (gy1 + gy2);
+    const double tt = g1 * g.tx + g2 * g.ty - (f2 - f1) / g.tl;
+    g1 = g1 - g.tx * tt;
+    g2 = g2 - g.ty * tt;
+    return g1 * g.ye - g2 * g.xe;
+}
+
+// the element's sum of a 3-D pass's edge fluxes at layer k (internal edges
+// +/- by side, then boundary edges +)
+template <typename R>
+__device__ __forceinline__ void bh_gather(const DevMesh &m, const DevStateT<R> &s, int e, int k, double &u, double &v) {
+    const int K = m.K;
+    const int4 ied = m.el_ied[e];
+    const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+    for (int j = 0; j < 4; ++j) {
+        const int c = slots[j];
+        if (c == kEmpty) break;
+        const size_t q = (size_t)(c >= 0 ? c : ~c) * K + k;
+        if (c >= 0) {  // element is edge_tri(2)
+            u = u - (double)s.bhfu[q];
+            v = v - (double)s.bhfv[q];
+        } else {
+            u = u + (double)s.bhfu[q];
+            v = v + (double)s.bhfv[q];
+        }
+    }
+    const int4 bed = m.el_bed[e];
+    const int bs[4] = {bed.x, bed.y, bed.z, bed.w};
+    for (int j = 0; j < 4; ++j) {
+        if (bs[j] == kEmpty) break;
+        const size_t q = (size_t)bs[j] * K + k;
+        u = u + (double)s.bhfu[q];
+        v = v + (double)s.bhfv[q];
+    }
+}
+
+// the element's sum of the 2-D pass's edge fluxes, in ascending edge id
+template <typename R>
+__device__ __forceinline__ void bh2_gather(const DevMesh &m, const DevStateT<R> &s, int e, double &u, double &v) {
+    const double2 *fl = (const double2 *)s.bh_fl2;
+    const int4 ied = m.el_ied[e];
+    const int slots[4] = {ied.x, ied.y, ied.z, ied.w};
+    for (int j = 0; j < 4; ++j) {
+        const int c = slots[j];
+        if (c == kEmpty) break;
+        const double2 f = fl[c >= 0 ? c : ~c];
+        if (c >= 0) {
+            u = u - f.x;
+            v = v - f.y;
+        } else {
+            u = u + f.x;
+            v = v + f.y;
+        }
+    }
+    const int4 bed = m.el_bed[e];
+    const int bs[4] = {bed.x, bed.y, bed.z, bed.w};
+    for (int j = 0; j < 4; ++j) {
+        if (bs[j] == kEmpty) break;
+        const double2 f = fl[bs[j]];
+        u = u + f.x;
+        v = v + f.y;
+    }
+}
+
 // compute_vel_rhs (src/fv_3D_vel.F90:1-197), explicit part, on column-aligned
 // blocks (cpb = 256/L element columns, one thread per element and layer):
 // the element's and its internal edges' constants are read once per column
@@ -1021,6 +1091,7 @@ __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt
         ur = ur + tau * fu * area * je;
         vr = vr + tau * fv * area * je;
     }
+    if (s.bhfu) bh_gather(m, s, e, k, ur, vr);  // biharmonic_viscosity, second pass (fv_viscosity.F90:262-333)
     R abu = (R)(v * r.cor * area * je);
     R abv = (R)(-u * r.cor * area * je);
     if (hadv) {
@@ -1270,6 +1341,11 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
             ur[i] = ur[i] + tau * fu[i] * area * je[i];
             vr[i] = vr[i] + tau * fv[i] * area * je[i];
         }
+    }
+    if (s.bhfu) {  // biharmonic_viscosity, second pass (fv_viscosity.F90:262-333)
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+            if (k0 + i < L) bh_gather(m, s, e, k0 + i, ur[i], vr[i]);
     }
 #pragma unroll
     for (int i = 0; i < W; ++i) {
@@ -1975,6 +2051,13 @@ __global__ __launch_bounds__(256) void k_bt_p1elem(DevMesh m, DevStateT<R> s, do
             rv = rv + tau * fv * area * jo;
         }
     }
+    if (s.bhub) {  // biharmonic_viscosity_3D_to_2D: the column sum times Je (fv_viscosity.F90:1200-1206)
+        for (int k = 0; k < m.L; ++k) {
+            const double jo = (double)PL(s.Je, k, e, KC);
+            ru = ru + (double)PL(s.bhub, k, e, KC) * jo;
+            rv = rv + (double)PL(s.bhvb, k, e, KC) * jo;
+        }
+    }
     st2(s.U_rhs_2D_3D, e, ru / s.dmean_n[e], rv / s.dmean_n[e]);
 }
 
@@ -2088,6 +2171,7 @@ __global__ __launch_bounds__(256, 5) void k_bt_elem2(DevMesh m, DevStateT<R> s, 
         ru = ru + tau2d * fu * area * dsum;
         rv = rv + tau2d * fv * area * dsum;
     }
+    if (s.bh_fl2) bh2_gather(m, s, e, ru, rv);  // biharmonic_2D, second pass (fv_viscosity.F90:456-541)
     const double2 r23 = ld2(s.U_rhs_2D_3D, e);
     ru = ru + r23.x;
     rv = rv + r23.y;
@@ -2229,6 +2313,174 @@ __global__ __launch_bounds__(256) void k_bt_filt2x_c(DevMesh m, DevStateT<R> s, 
     }
     const double K_bi = tau_inv * m.kbi[e];
     st2(s.fb2c, e, cu * K_bi, cv * K_bi);
+}
+
+// ---------------------------------------------------------------------------
+// Biharmonic viscosity (bih_2D, bih_3D): biharmonic_2D, biharmonic_viscosity
+// and biharmonic_viscosity_3D_to_2D, src/fv_viscosity.F90:104-568, 964-1220:
+// a corrected edge Laplacian applied twice, with the gradients of the first
+// result in between.  Edge kernels form each edge's flux once; element kernels
+// gather them in ascending reference edge id (the internal edges, el_ied,
+// then the boundary edges, el_bed), which is the order of the serial loops'
+// scatters, so FP64 sums keep their bits.  free_slip = .true.
+// ---------------------------------------------------------------------------
+
+
+// one pass over (edge, layer) of the 3-D operator: the fluxes of (F, G) with
+// the gradients (gux, guy, gvx, gvy) into bhfu/bhfv, times Jd (with_jd) and
+// the edge's sponge weight (with_acc); boundary edges take the element's own
+// gradient (fv_viscosity.F90:148-184, 208-218, 262-333, 998-1100, 1130-1196)
+template <typename R>
+__global__ __launch_bounds__(256) void k_bh3_edge(DevMesh m, DevStateT<R> s, const R *__restrict__ F,
+                                                   const R *__restrict__ G, const R *__restrict__ gux,
+                                                   const R *__restrict__ guy, const R *__restrict__ gvx,
+                                                   const R *__restrict__ gvy, int with_jd, int with_acc) {
+    int ed, k;
+    if (!ent_layer(m.D, m.L, ed, k)) return;
+    const int K = m.K;
+    const int2 et = m.edtri[ed];
+    const BhEdge g = m.bhe[ed];
+    const size_t i1 = (size_t)et.x * K + k;
+    double fu, fv;
+    if (et.y >= 0) {
+        const size_t i2 = (size_t)et.y * K + k;
+        fu = bh_flux(g, gux[i1], gux[i2], guy[i1], guy[i2], F[i1], F[i2]);
+        fv = bh_flux(g, gvx[i1], gvx[i2], gvy[i1], gvy[i2], G[i1], G[i2]);
+    } else {
+        fu = (double)gux[i1] * g.ye - (double)guy[i1] * g.xb;
+        fv = (double)gvx[i1] * g.ye - (double)gvy[i1] * g.xb;
+    }
+    if (with_jd) {
+        const int2 nd = m.ednodes[ed];
+        const double ds = m.dsig[k];
+        const double jd = 0.5 * (s.a[nd.x] * ds + s.a[nd.y] * ds);
+        fu = fu * jd;
+        fv = fv * jd;
+    }
+    if (with_acc) {
+        fu = fu * g.acc;
+        fv = fv * g.acc;
+    }
+    s.bhfu[(size_t)ed * K + k] = (R)fu;
+    s.bhfv[(size_t)ed * K + k] = (R)fv;
+}
+
+
+// element sums of a pass into (oU, oV): mode 1 times Ah (biharmonic_viscosity,
+// fv_viscosity.F90:224-235), mode 2 times (Ah_acc - Visc) with the Smagorinsky
+// Visc = 0 (1084-1095; only the sediment module sets it), mode 0 as summed
+template <typename R>
+__global__ __launch_bounds__(256) void k_bh3_elem(DevMesh m, DevStateT<R> s, R *__restrict__ oU, R *__restrict__ oV,
+                                                   int mode) {
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    double u = 0.0, v = 0.0;
+    bh_gather(m, s, e, k, u, v);
+    if (mode == 1) {
+        u = u * m.bh_ah[e];
+        v = v * m.bh_ah[e];
+    } else if (mode == 2) {
+        u = u * (m.bh_ah_acc[e] - 0.0);
+        v = v * (m.bh_ah_acc[e] - 0.0);
+    }
+    const size_t i = (size_t)e * m.K + k;
+    oU[i] = (R)u;
+    oV[i] = (R)v;
+}
+
+// vel_lapl_gradients (fv_gradients.F90:327-402, mirror value -U at the
+// boundary; puls = 0) or vel_gradients_puls (404-496, free-slip reflection of
+// the pulsation velocity; puls = 1) of (Uc, Vc) into bhgux..bhgvy
+template <typename R>
+__global__ __launch_bounds__(256) void k_bh3_grad(DevMesh m, DevStateT<R> s, const R *__restrict__ Uc,
+                                                   const R *__restrict__ Vc, int puls) {
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    const int K = m.K;
+    const int4 q = m.en[e];
+    const int jend = (q.x == q.w) ? 3 : 4;
+    const int4 nb4 = m.elnb[e], ed4 = m.eledges[e];
+    const int nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w}, eds[4] = {ed4.x, ed4.y, ed4.z, ed4.w};
+    const double4 gx = m.gvx[e], gy = m.gvy[e];
+    const double gxs[4] = {gx.x, gx.y, gx.z, gx.w}, gys[4] = {gy.x, gy.y, gy.z, gy.w};
+    const size_t i = (size_t)e * K + k;
+    const double U = (double)Uc[i], V = (double)Vc[i];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int j = 0; j < jend; ++j) {
+        double u, v;
+        if (nbs[j] >= 0) {
+            u = (double)Uc[(size_t)nbs[j] * K + k] - U;
+            v = (double)Vc[(size_t)nbs[j] * K + k] - V;
+        } else if (puls) {
+            const double2 dd = m.edxdy[eds[j]];
+            const double zc0 = dd.x * m.elem_cos[e], zc1 = dd.y;
+            const double un = -2.0 * (U * zc1 - V * zc0) / (zc0 * zc0 + zc1 * zc1);
+            u = (U + un * zc1) - U;
+            v = (V - un * zc0) - V;
+        } else {
+            u = -2.0 * U;
+            v = -2.0 * V;
+        }
+        a0 = a0 + gxs[j] * u;
+        a1 = a1 + gys[j] * u;
+        a2 = a2 + gxs[j] * v;
+        a3 = a3 + gys[j] * v;
+    }
+    s.bhgux[i] = (R)a0;
+    s.bhguy[i] = (R)a1;
+    s.bhgvx[i] = (R)a2;
+    s.bhgvy[i] = (R)a3;
+}
+
+// the pulsation velocity U_puls = U_n - a*U_n_2D of compute_puls_vel
+// (fv_3D_vel.F90:285-299) into bhpu/bhpv, formed as k_bt_p1elem forms it
+template <typename R>
+__global__ __launch_bounds__(256) void k_bh_puls(DevMesh m, DevStateT<R> s) {
+    int e, k;
+    if (!ent_layer(m.E, m.L, e, k)) return;
+    const size_t i = (size_t)e * m.K + k;
+    const double ao = s.puls_ap[e];
+    s.bhpu[i] = (R)((double)s.U_n[i] - ao * s.U2[2 * e]);
+    s.bhpv[i] = (R)((double)s.V_n[i] - ao * s.U2[2 * e + 1]);
+}
+
+// one pass of the 2-D operator (fv_viscosity.F90:381-432, 456-541): fluxes of
+// Uin (2E) with the gradients grad (4E) into bh_fl2; boundary edges with the
+// free-slip projection, whose v line takes the already projected u
+template <typename R>
+__global__ __launch_bounds__(256) void k_bh2_edge(DevMesh m, DevStateT<R> s, const R *__restrict__ Uin,
+                                                   const R *__restrict__ grad) {
+    const int ed = TID();
+    if (ed >= m.D) return;
+    const int2 et = m.edtri[ed];
+    const BhEdge g = m.bhe[ed];
+    const double4 a = ld4(grad, et.x);
+    double u1, v1;
+    if (et.y >= 0) {
+        const double4 b = ld4(grad, et.y);
+        const double2 w1 = ld2(Uin, et.x), w2 = ld2(Uin, et.y);
+        u1 = bh_flux(g, a.x, b.x, a.y, b.y, w1.x, w2.x);
+        v1 = bh_flux(g, a.z, b.z, a.w, b.w, w1.y, w2.y);
+    } else {
+        u1 = a.x * g.ye - a.y * g.xbd;
+        v1 = a.z * g.ye - a.w * g.xbd;
+        u1 = g.ye * (u1 * g.ye - v1 * g.xe) / (g.xe * g.xe + g.ye * g.ye);
+        v1 = -g.xe * (u1 * g.ye - v1 * g.xe) / (g.xe * g.xe + g.ye * g.ye);
+    }
+    ((double2 *)s.bh_fl2)[ed] = make_double2(u1, v1);
+}
+
+
+// first 2-D pass summed per element and scaled: Ul = Ul*Ah*acc
+// (fv_viscosity.F90:438-451)
+template <typename R>
+__global__ __launch_bounds__(256) void k_bh2_elem(DevMesh m, DevStateT<R> s) {
+    const int e = TID();
+    if (e >= m.E) return;
+    double u = 0.0, v = 0.0;
+    bh2_gather(m, s, e, u, v);
+    const double ah = m.bh_ah[e], acc = m.acc_el[e];
+    st2(s.bh_ul, e, u * (ah) * acc, v * (ah) * acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -4630,6 +4882,12 @@ void instantiate_kernels() {
     (void)&k_filt2x_3d_c<R>;
     (void)&k_filt2x_3d_uv<R>;
     (void)&k_bt_filt2x_c<R>;
+    (void)&k_bh3_edge<R>;
+    (void)&k_bh3_elem<R>;
+    (void)&k_bh3_grad<R>;
+    (void)&k_bh_puls<R>;
+    (void)&k_bh2_edge<R>;
+    (void)&k_bh2_elem<R>;
     (void)&k_jac_node<R>;
     (void)&k_jac_restart<R>;
     (void)&k_je<R>;

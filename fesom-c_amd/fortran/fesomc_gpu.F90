@@ -30,6 +30,8 @@ module fesomc_gpu
      real(c_double) :: tau_inv_filt
      integer(c_int) :: filt_bi_2D, filt_bi_3D
      real(c_double) :: tau_c
+     integer(c_int) :: bih_2D, bih_3D
+     real(c_double) :: Abh0
   end type fcg_params
 
   ! the o_MESH arrays after the reference's preprocessing (pointers into host memory)

@@ -132,6 +132,9 @@ program fesomc_run
   par%filt_bi_2D = merge(1, 0, getl('filt_bi_2D'))
   par%filt_bi_3D = merge(1, 0, getl('filt_bi_3D'))
   par%tau_c = getf('tau_c', 0.4d0)
+  par%bih_2D = merge(1, 0, getl('bih_2D'))
+  par%bih_3D = merge(1, 0, getl('bih_3D'))
+  par%Abh0 = getf('Abh0', 0.0d0)
 
   ! ---- read_mesh_ser (src/fv_main.F90:787-945) -----------------------------
   meshdir = trim(rundir) // '/' // trim(gets('MESHPATH', './MESH/'))
@@ -530,8 +533,8 @@ contains
   ! Options whose code this build does not hold stop the run before any set-up
   ! (the same list as the Python host's RunConfig.unsupported).
   subroutine refuse_unsupported()
-    character(len=20), parameter :: opts(5) = [character(len=20) :: 'bih_2D', 'bih_3D', &
-         'mixed_BP_grad_OB', 'T_potential', 'comp_sediment']
+    character(len=20), parameter :: opts(3) = [character(len=20) :: 'mixed_BP_grad_OB', 'T_potential', &
+         'comp_sediment']
     integer :: j
     logical :: bad
     bad = .false.

@@ -47,7 +47,8 @@ class Params(C.Structure):
                                        "restart", "cl_relax", "precision", "closure", "key_atm", "type_swr_body")] + \
                [(n, C.c_double) for n in ("Rjc", "ajc", "bjc")] + \
                [(n, C.c_int) for n in ("key_rivers", "filt_2D", "filt_3D")] + [("tau_inv_filt", C.c_double)] + \
-               [(n, C.c_int) for n in ("filt_bi_2D", "filt_bi_3D")] + [("tau_c", C.c_double)]
+               [(n, C.c_int) for n in ("filt_bi_2D", "filt_bi_3D")] + [("tau_c", C.c_double)] + \
+               [(n, C.c_int) for n in ("bih_2D", "bih_3D")] + [("Abh0", C.c_double)]
 
 
 _IP = C.POINTER(C.c_int)
@@ -230,6 +231,8 @@ def params(cfg: RunConfig, precision=FCG_FP64, closure=FCG_CLOSURE_D3END) -> Par
     p.tau_inv_filt = cfg.f("tau_inv_filt", 0.0)
     p.filt_bi_2D, p.filt_bi_3D = int(cfg.l("filt_bi_2D")), int(cfg.l("filt_bi_3D"))
     p.tau_c = cfg.f("tau_c", 0.4)
+    p.bih_2D, p.bih_3D = int(cfg.l("bih_2D")), int(cfg.l("bih_3D"))
+    p.Abh0 = cfg.f("Abh0", 0.0)
     return p
 
 

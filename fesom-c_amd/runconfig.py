@@ -118,7 +118,7 @@ class RunConfig:
 
     def unsupported(self):
         """Options outside the hot path this build implements (empty if none)."""
-        bad = [k for k in ("bih_2D", "bih_3D", "mixed_BP_grad_OB", "T_potential",
+        bad = [k for k in ("mixed_BP_grad_OB", "T_potential",
                            "comp_sediment") if self.l(k)]
         if self.i("Vert_turb", 3) != 3:
             bad.append("Vert_turb")

@@ -87,8 +87,7 @@ typedef struct fcg_params {
     /* neighbour-difference filters (fv_viscosity.F90:769-939): filt_2D on
      * the barotropic velocity, filt_3D on the 3-D velocity and its
      * pulsation part in the barotropic forcing, both at the rate
-     * tau_inv_filt [1/s]; the biharmonic options (bih_2D, bih_3D) are not
-     * built */
+     * tau_inv_filt [1/s] */
     int filt_2D, filt_3D;
     double tau_inv_filt;
     /* the twice-applied filters (viscosity_filt2x_2D/_3D, fv_viscosity.F90:
@@ -98,6 +97,13 @@ typedef struct fcg_params {
      * (fv_read_run.F90:336-341) scaled by sqrt(scale_area/elem_area) */
     int filt_bi_2D, filt_bi_3D;
     double tau_c;
+    /* biharmonic viscosity (fv_viscosity.F90:104-568, 964-1220): bih_2D on
+     * UAB into U_rhs_2D of update_2D_vel (fv_average_dynamic.F90:348),
+     * bih_3D on U_n into U_rhs of compute_vel_rhs (fv_3D_vel.F90:113) and on
+     * the pulsation velocity into the barotropic forcing of the first substep
+     * (fv_main.F90:1501), with the coefficient Abh0 (fv_read_run.F90:320-325) */
+    int bih_2D, bih_3D;
+    double Abh0;
 } fcg_params;
 
 /*

@@ -54,6 +54,7 @@ typedef struct {
     double Cd_bz, Bp_bz, pow_sig, lev_bot, lev_sur, SL_radius;
     double tau_inv_filt; /* relaxation rate of the neighbour-difference filters [1/s] */
     double tau_c;        /* rate of the twice-applied filters filt_bi_2D/3D [1/day] */
+    double Abh0;         /* biharmonic viscosity coefficient (bih_2D, bih_3D) */
     int type_swr_body, jer_const;   /* shortwave absorption profile (key_atm) */
     double Rjc, ajc, bjc;
     int nsteps, irep, cartesian, coriolis_tf, mom_adv_2D, mom_adv_3D, tracer_adv, vert_adv;
@@ -1286,6 +1287,228 @@ static void viscosity_filt_3D_to_2D(orc_t *o) {
     }
 }
 
+/* Biharmonic viscosity (src/fv_viscosity.F90:104-568, 964-1220): the
+ * Blazek-corrected edge Laplacian applied twice, bh = -L Abh L u.  Per
+ * internal edge the mean of the two elements' gradients has its component
+ * along the centre-to-centre direction (tx, ty) replaced by the difference
+ * quotient, and the flux is that gradient dotted with the edge normal
+ * (ye, -xe); boundary edges take the element's own gradient.  Internal edges
+ * are visited in ascending id, then the boundary edges, as the serial loops
+ * do.  free_slip is .true. (src/fv_var.F90:243). */
+typedef struct { double xe, ye, tx, ty, tl; } bh_edge_t;
+
+static bh_edge_t bh_edge(const orc_t *o, int ed) {
+    bh_edge_t g;
+    const int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+    const double *cd = o->edge_cross_dxdy + 4 * ed;
+    g.xe = o->edge_dxdy[2 * ed] * R_EARTH * 0.5 * (o->elem_cos[e1] + o->elem_cos[e2]);
+    g.ye = o->edge_dxdy[2 * ed + 1] * R_EARTH;
+    g.tx = -cd[0] + cd[2];
+    g.ty = -cd[1] + cd[3];
+    g.tl = sqrt(g.tx * g.tx + g.ty * g.ty);
+    g.tx = g.tx / g.tl;
+    g.ty = g.ty / g.tl;
+    return g;
+}
+
+/* corrected normal flux of one component at an internal edge */
+static double bh_flux(const bh_edge_t *g, double gx1, double gx2, double gy1, double gy2, double f1, double f2) {
+    double g1 = 0.5 * (gx1 + gx2), g2 = 0.5 * (gy1 + gy2);
+    const double tt = MUT(22) ? 0.0 : g1 * g->tx + g2 * g->ty - (f2 - f1) / g->tl;
+    g1 = g1 - g->tx * tt;
+    g2 = g2 - g->ty * tt;
+    return g1 * g->ye - g2 * g->xe;
+}
+
+/* vel_lapl_gradients: src/fv_gradients.F90:327-402 (mirror value -U at the
+ * boundary, no free-slip projection), into vel_grad_ux/uy/vx/vy */
+static void vel_lapl_gradients(orc_t *o, const double *Uc, const double *Vc) {
+    const int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        const int jend = (o->elem_nodes[4 * el] == o->elem_nodes[4 * el + 3]) ? 3 : 4;
+        const double *gv = o->gradient_vec + 8 * el;
+        for (int z = 0; z < L; z++) {
+            const size_t i = (size_t)el * L + z;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            for (int j = 0; j < jend; j++) {
+                const int nb = o->elem_nb[4 * el + j];
+                double u, v;
+                if (nb >= 0) {
+                    u = Uc[(size_t)nb * L + z] - Uc[i];
+                    v = Vc[(size_t)nb * L + z] - Vc[i];
+                } else {
+                    u = -2.0 * Uc[i];
+                    v = -2.0 * Vc[i];
+                }
+                a0 = a0 + gv[j] * u;
+                a1 = a1 + gv[j + 4] * u;
+                a2 = a2 + gv[j] * v;
+                a3 = a3 + gv[j + 4] * v;
+            }
+            o->vel_grad_ux[i] = a0; o->vel_grad_uy[i] = a1; o->vel_grad_vx[i] = a2; o->vel_grad_vy[i] = a3;
+        }
+    }
+}
+
+/* vel_gradients_puls: src/fv_gradients.F90:404-496 (free slip), into vel_grad_* */
+static void vel_gradients_puls(orc_t *o) {
+    const int L = o->L;
+    for (int el = 0; el < o->E; el++) {
+        const int jend = (o->elem_nodes[4 * el] == o->elem_nodes[4 * el + 3]) ? 3 : 4;
+        const double *gv = o->gradient_vec + 8 * el;
+        for (int z = 0; z < L; z++) {
+            const size_t i = (size_t)el * L + z;
+            const double U = o->U_puls[i], V = o->V_puls[i];
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            for (int j = 0; j < jend; j++) {
+                const int nb = o->elem_nb[4 * el + j];
+                double u, v;
+                if (nb >= 0) {
+                    u = o->U_puls[(size_t)nb * L + z] - U;
+                    v = o->V_puls[(size_t)nb * L + z] - V;
+                } else {
+                    const int ed = o->elem_edges[4 * el + j];
+                    const double zc0 = o->edge_dxdy[2 * ed] * o->elem_cos[el], zc1 = o->edge_dxdy[2 * ed + 1];
+                    const double un = -2.0 * (U * zc1 - V * zc0) / (zc0 * zc0 + zc1 * zc1);
+                    u = (U + un * zc1) - U;
+                    v = (V - un * zc0) - V;
+                }
+                a0 = a0 + gv[j] * u;
+                a1 = a1 + gv[j + 4] * u;
+                a2 = a2 + gv[j] * v;
+                a3 = a3 + gv[j + 4] * v;
+            }
+            o->vel_grad_ux[i] = a0; o->vel_grad_uy[i] = a1; o->vel_grad_vx[i] = a2; o->vel_grad_vy[i] = a3;
+        }
+    }
+}
+
+/* One corrected Laplacian of the layer fields (F, G) with the current
+ * vel_grad_* into (outF, outG), which it adds to: internal edges +/- flux*w,
+ * boundary edges + flux*w, w = Jd (with_jd), times (ac1+ac2)/2 (with_acc). */
+static void bh_lapl_3d(orc_t *o, const double *F, const double *G, double *outF, double *outG, int with_jd, int with_acc) {
+    const int L = o->L;
+    for (int ed = 0; ed < o->D_in; ed++) {
+        const int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const bh_edge_t g = bh_edge(o, ed);
+        const double acc = (o->ac[o->edge_nodes[2 * ed]] + o->ac[o->edge_nodes[2 * ed + 1]]) / 2.0;
+        for (int z = 0; z < L; z++) {
+            const size_t i1 = (size_t)e1 * L + z, i2 = (size_t)e2 * L + z;
+            const double u1 = bh_flux(&g, o->vel_grad_ux[i1], o->vel_grad_ux[i2], o->vel_grad_uy[i1], o->vel_grad_uy[i2], F[i1], F[i2]);
+            const double v1 = bh_flux(&g, o->vel_grad_vx[i1], o->vel_grad_vx[i2], o->vel_grad_vy[i1], o->vel_grad_vy[i2], G[i1], G[i2]);
+            double fu = u1, fv = v1;
+            if (with_jd) { fu = fu * o->Jd[(size_t)ed * L + z]; fv = fv * o->Jd[(size_t)ed * L + z]; }
+            if (with_acc) { fu = fu * acc; fv = fv * acc; }
+            outF[i1] = outF[i1] + fu; outG[i1] = outG[i1] + fv;
+            outF[i2] = outF[i2] - fu; outG[i2] = outG[i2] - fv;
+        }
+    }
+    for (int ed = o->D_in; ed < o->D; ed++) {
+        const int e1 = o->edge_tri[2 * ed];
+        const double xe = o->edge_dxdy[2 * ed] * R_EARTH * o->elem_cos[e1], ye = o->edge_dxdy[2 * ed + 1] * R_EARTH;
+        const double acc = (o->ac[o->edge_nodes[2 * ed]] + o->ac[o->edge_nodes[2 * ed + 1]]) / 2.0;
+        for (int z = 0; z < L; z++) {
+            const size_t i1 = (size_t)e1 * L + z;
+            double fu = o->vel_grad_ux[i1] * ye - o->vel_grad_uy[i1] * xe;
+            double fv = o->vel_grad_vx[i1] * ye - o->vel_grad_vy[i1] * xe;
+            if (with_jd) { fu = fu * o->Jd[(size_t)ed * L + z]; fv = fv * o->Jd[(size_t)ed * L + z]; }
+            if (with_acc) { fu = fu * acc; fv = fv * acc; }
+            outF[i1] = outF[i1] + fu; outG[i1] = outG[i1] + fv;
+        }
+    }
+}
+
+/* biharmonic_viscosity: src/fv_viscosity.F90:104-336, into U_rhs, V_rhs */
+static void biharmonic_viscosity(orc_t *o) {
+    const int L = o->L, E = o->E;
+    double *Uc = DALLOC((size_t)E * L), *Vc = DALLOC((size_t)E * L);
+    bh_lapl_3d(o, o->U_n, o->V_n, Uc, Vc, 1, 0);
+    for (int el = 0; el < E; el++) {
+        const double Ah = -o->Abh0 * sqrt(o->elem_area[el] / o->scale_area) / o->scale_area;
+        for (int z = 0; z < L; z++) {
+            Uc[(size_t)el * L + z] = Uc[(size_t)el * L + z] * (MUT(21) ? -Ah : Ah);
+            Vc[(size_t)el * L + z] = Vc[(size_t)el * L + z] * (MUT(21) ? -Ah : Ah);
+        }
+    }
+    vel_lapl_gradients(o, Uc, Vc);
+    bh_lapl_3d(o, Uc, Vc, o->U_rhs, o->V_rhs, 1, 1);
+    free(Uc); free(Vc);
+}
+
+/* biharmonic_viscosity_3D_to_2D: src/fv_viscosity.F90:964-1220, on the
+ * pulsation velocity; the column sum of the result times Je into U_rhs_2D_3D.
+ * Visc (Smagorinsky, set only by the sediment module) is zero. */
+static void biharmonic_viscosity_3D_to_2D(orc_t *o) {
+    const int L = o->L, E = o->E;
+    double *Uc = DALLOC((size_t)E * L), *Vc = DALLOC((size_t)E * L);
+    double *Ub = DALLOC((size_t)E * L), *Vb = DALLOC((size_t)E * L);
+    vel_gradients_puls(o);
+    bh_lapl_3d(o, o->U_puls, o->V_puls, Uc, Vc, 1, 0);
+    for (int el = 0; el < E; el++) {
+        const double acc = sum4w(o->w_cv + 4 * el, o->ac[o->elem_nodes[4 * el]], o->ac[o->elem_nodes[4 * el + 1]],
+                                 o->ac[o->elem_nodes[4 * el + 2]], o->ac[o->elem_nodes[4 * el + 3]]);
+        const double Ah = -acc * o->Abh0 * sqrt(o->elem_area[el] / o->scale_area) / o->scale_area;
+        for (int z = 0; z < L; z++) {
+            Uc[(size_t)el * L + z] = Uc[(size_t)el * L + z] * (Ah - 0.0);
+            Vc[(size_t)el * L + z] = Vc[(size_t)el * L + z] * (Ah - 0.0);
+        }
+    }
+    vel_lapl_gradients(o, Uc, Vc);
+    bh_lapl_3d(o, Uc, Vc, Ub, Vb, 0, 0);
+    for (int el = 0; el < E; el++)
+        for (int z = 0; z < L; z++) {
+            const size_t i = (size_t)el * L + z;
+            o->U_rhs_2D_3D[2 * el] = o->U_rhs_2D_3D[2 * el] + Ub[i] * o->Je[i];
+            o->U_rhs_2D_3D[2 * el + 1] = o->U_rhs_2D_3D[2 * el + 1] + Vb[i] * o->Je[i];
+        }
+    free(Uc); free(Vc); free(Ub); free(Vb);
+}
+
+/* One corrected Laplacian of the 2-D field Uin (2E) with the current
+ * vel_grad into out (2E), which it adds to; boundary edges with the
+ * free-slip projection (the v line uses the already projected u, as the
+ * reference writes it). */
+static void bh_lapl_2d(orc_t *o, const double *Uin, double *out) {
+    for (int ed = 0; ed < o->D_in; ed++) {
+        const int e1 = o->edge_tri[2 * ed], e2 = o->edge_tri[2 * ed + 1];
+        const bh_edge_t g = bh_edge(o, ed);
+        const double *a = o->vel_grad + 4 * e1, *b = o->vel_grad + 4 * e2;
+        const double u1 = bh_flux(&g, a[0], b[0], a[1], b[1], Uin[2 * e1], Uin[2 * e2]);
+        const double v1 = bh_flux(&g, a[2], b[2], a[3], b[3], Uin[2 * e1 + 1], Uin[2 * e2 + 1]);
+        out[2 * e1] = out[2 * e1] + u1; out[2 * e1 + 1] = out[2 * e1 + 1] + v1;
+        out[2 * e2] = out[2 * e2] - u1; out[2 * e2 + 1] = out[2 * e2 + 1] - v1;
+    }
+    for (int ed = o->D_in; ed < o->D; ed++) {
+        const int e1 = o->edge_tri[2 * ed];
+        const double xe = o->edge_dxdy[2 * ed] * R_EARTH, xed = xe * o->elem_cos[e1];
+        const double ye = o->edge_dxdy[2 * ed + 1] * R_EARTH;
+        const double *a = o->vel_grad + 4 * e1;
+        double u1 = a[0] * ye - a[1] * xed, v1 = a[2] * ye - a[3] * xed;
+        u1 = ye * (u1 * ye - v1 * xe) / (xe * xe + ye * ye);
+        v1 = -xe * (u1 * ye - v1 * xe) / (xe * xe + ye * ye);
+        out[2 * e1] = out[2 * e1] + u1; out[2 * e1 + 1] = out[2 * e1 + 1] + v1;
+    }
+}
+
+/* biharmonic_2D: src/fv_viscosity.F90:338-568, on UAB (vel_grad holds its
+ * gradients, update_2D_vel's vel_gradients_2D) into U_rhs_2D.  h_viscosity_2D
+ * (called there) sets only Visc2D, which nothing on the path reads. */
+static void biharmonic_2D(orc_t *o) {
+    const int E = o->E;
+    double *Ul = DALLOC(2 * (size_t)E);
+    bh_lapl_2d(o, o->UAB, Ul);
+    for (int el = 0; el < E; el++) {
+        const int *en = o->elem_nodes + 4 * el;
+        const double acc = sum4w(o->w_cv + 4 * el, o->ac[en[0]], o->ac[en[1]], o->ac[en[2]], o->ac[en[3]]);
+        const double Ah = -o->Abh0 * sqrt(o->elem_area[el] / o->scale_area) / o->scale_area;
+        Ul[2 * el] = Ul[2 * el] * (Ah) * acc;
+        Ul[2 * el + 1] = Ul[2 * el + 1] * (Ah) * acc;
+    }
+    vel_gradients_2D(o, Ul);
+    bh_lapl_2d(o, Ul, o->U_rhs_2D);
+    free(Ul);
+}
+
 /* The twice-applied neighbour-difference filters (src/fv_viscosity.F90:570-767).
  * First pass: U_c = K_bi * sum over the neighbours of (own - neighbour), with
  * K_bi = tau_inv*sqrt(scale_area/elem_area); second pass: the tendency is the
@@ -1393,8 +1616,10 @@ static void viscosity_filt2x_2D(orc_t *o) {
 
 /* Test hook: one filter alone on the current state, its target zeroed first
  * (2: filt_2D into U_rhs_2D, 3: filt_3D into U_rhs/V_rhs, 4: filt_3D_to_2D
- * into U_rhs_2D_3D, 5: filt_bi_2D into UV2_rhs, 6: filt_bi_3D into UV_rhs;
- * the last two assign their whole target). */
+ * into U_rhs_2D_3D, 5: filt_bi_2D into UV2_rhs, 6: filt_bi_3D into UV_rhs,
+ * which assign their whole target; 7: bih_2D on UAB into U_rhs_2D, 8: bih_3D
+ * on U_n into U_rhs/V_rhs, 9: the 3-D-to-2-D biharmonic on U_puls into
+ * U_rhs_2D_3D, each with the gradients its caller has formed). */
 int orc_filter_only(orc_t *o, int which) {
     if (!o) return -1;
     if (which == 2) {
@@ -1411,6 +1636,18 @@ int orc_filter_only(orc_t *o, int which) {
         viscosity_filt2x_2D(o);
     } else if (which == 6) {
         viscosity_filt2x_3D(o);
+    } else if (which == 7) {
+        memset(o->U_rhs_2D, 0, sizeof(double) * 2 * (size_t)o->E);
+        vel_gradients_2D(o, o->UAB);
+        biharmonic_2D(o);
+    } else if (which == 8) {
+        memset(o->U_rhs, 0, sizeof(double) * (size_t)o->E * o->L);
+        memset(o->V_rhs, 0, sizeof(double) * (size_t)o->E * o->L);
+        vel_gradients(o);
+        biharmonic_viscosity(o);
+    } else if (which == 9) {
+        memset(o->U_rhs_2D_3D, 0, sizeof(double) * 2 * (size_t)o->E);
+        biharmonic_viscosity_3D_to_2D(o);
     } else {
         return -1;
     }
@@ -1468,7 +1705,9 @@ int orc_wad_mask_only(orc_t *o) {
  *                                                           instead of the layer mid-depth Z
  * 18 filt_bi_3D without the sqrt(scale_area/elem_area) factor of K_bi
  * 19 filt_bi_3D second pass accumulating own minus neighbour
- * 20 filt_bi_2D rate from the internal step dt instead of dt_2D */
+ * 20 filt_bi_2D rate from the internal step dt instead of dt_2D
+ * 21 bih_3D with the sign of Ah flipped (anti-diffusion)
+ * 22 biharmonic edge correction along the normal instead of the centre line */
 int orc_set_mutation(int id) {
     g_mut = id;
     return 0;
@@ -1504,6 +1743,7 @@ static void compute_vel_rhs(orc_t *o) {
     if (o->mom_adv_3D == 1) momentum_adv_upwind(o);
     if (o->filt_3D) viscosity_filt_3D(o);   /* src/fv_3D_vel.F90:111 */
     if (o->filt_bi_3D) viscosity_filt2x_3D(o);   /* src/fv_3D_vel.F90:112 */
+    if (o->bih_3D) biharmonic_viscosity(o);   /* src/fv_3D_vel.F90:113 */
     if (o->vert_adv == 1) momentum_vert_adv_upwind(o);
     double ff = (1.5 + EPS);
     if (o->lfirst) { ff = 1.0; o->lfirst = 0; }
@@ -1690,6 +1930,7 @@ static void update_2D_vel(orc_t *o) {
     if (o->mom_adv_2D == 2) momentum_adv_upwind_2D(o);
     if (o->filt_2D) viscosity_filt_2D(o);   /* src/fv_average_dynamic.F90:346 */
     if (o->filt_bi_2D) viscosity_filt2x_2D(o);   /* src/fv_average_dynamic.F90:347 */
+    if (o->bih_2D) biharmonic_2D(o);   /* src/fv_average_dynamic.F90:348 */
     if (o->type_task > 2)
         for (int el = 0; el < o->E; el++) {
             o->U_rhs_2D[2 * el] = o->U_rhs_2D[2 * el] + o->U_rhs_2D_3D[2 * el];
@@ -1739,6 +1980,7 @@ static void timestep_AB_2D(orc_t *o, int step) {
         el2nodes_3D(o, o->U_puls, o->V_puls, o->Unode_p, o->Vnode_p);
         momentum_adv_P1_3D_to_2D(o);
         if (o->filt_3D) viscosity_filt_3D_to_2D(o);   /* src/fv_main.F90:1499 */
+        if (o->bih_3D) biharmonic_viscosity_3D_to_2D(o);   /* src/fv_main.F90:1501 */
         for (int el = 0; el < o->E; el++) {
             o->U_rhs_2D_3D[2 * el] = o->U_rhs_2D_3D[2 * el] / o->dmean_n[el];
             o->U_rhs_2D_3D[2 * el + 1] = o->U_rhs_2D_3D[2 * el + 1] / o->dmean_n[el];
@@ -2783,6 +3025,7 @@ static void parse_config(orc_t *o, const char *t) {
     o->SL_radius = cfg_f(t, "SL_radius", 0.0);
     o->tau_inv_filt = cfg_f(t, "tau_inv_filt", 0.0);
     o->tau_c = cfg_f(t, "tau_c", 0.4);
+    o->Abh0 = cfg_f(t, "Abh0", 0.0);
     o->type_swr_body = cfg_i(t, "type_swr_body", 2);
     o->jer_const = cfg_l(t, "jer_const", 1);
     o->Rjc = cfg_f(t, "R", 0.58); o->ajc = cfg_f(t, "a", 0.35); o->bjc = cfg_f(t, "b", 23.0);
@@ -2813,7 +3056,7 @@ orc_t *orc_create(const char *cfg, int N, const double *xy, const int *ind, int 
     orc_t *o = (orc_t *)xcalloc(1, sizeof(orc_t));
     parse_config(o, cfg);
     if (o->type_task != 3) { o->status = -10; snprintf(o->errmsg, sizeof o->errmsg, "only type_task=3"); return o; }
-    if (o->bih_2D || o->bih_3D || o->mixed_BP_grad_OB
+    if (o->mixed_BP_grad_OB
         || (o->key_atm && o->type_swr_body == 1 && !o->jer_const) || o->T_potential || o->ver_mix != 3
         || (o->SL_obn && !(o->SL_radius > 0.0))) {
         o->status = -14;

@@ -160,12 +160,14 @@ def test_driver_reads_and_applies_the_river_table(tmp_path):
 
 
 def test_driver_refuses_options_it_does_not_hold(tmp_path):
-    """A biharmonic viscosity in fv_run.dat stops the driver before any set-up,
-    naming the option (no GPU: the check precedes the mesh)."""
+    """The tidal potential (T_potential, off the hot path) in fv_run.dat stops
+    the driver before any set-up, naming the option (no GPU: the check
+    precedes the mesh)."""
     d = _le_rundir(tmp_path)
     import re
-    txt, n = re.subn(r"^(\s*bih_2D\s*=\s*)\S+", r"\g<1>T", (d / "fv_run.dat").read_text(), count=1, flags=re.M)
+    txt, n = re.subn(r"^(\s*T_potential\s*=\s*)\S+", r"\g<1>T", (d / "fv_run.dat").read_text(), count=1,
+                     flags=re.M)
     assert n == 1
     (d / "fv_run.dat").write_text(txt)
     r = _run([d, 1, "--mesh-only"], d)
-    assert r.returncode == 2 and "bih_2D" in r.stdout
+    assert r.returncode == 2 and "T_potential" in r.stdout
