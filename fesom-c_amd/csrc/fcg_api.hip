@@ -516,8 +516,9 @@ extern "C" const char *fcg_version(void) { return "fesomc_gpu 0.1 gfx950 fp64+gr
 
 // node_member: ensemble member of each (union) node, members numbered 0..nmemb-1
 // and contiguous in the union numbering; nullptr for a single model.
+// member_sa: scale_area of each member (fv_ini.F90:81), nullptr for a single model.
 static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *node_member, int nmemb,
-                       fcg_ctx **out) {
+                       fcg_ctx **out, const double *member_sa = nullptr) {
     if (!mesh || !par || !out || nmemb < 1) return FCG_E_ARG;
     *out = nullptr;
     int ndev = 0;
@@ -648,16 +649,7 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     const bool filt_bi = p.filt_bi_2D || p.filt_bi_3D, bih = p.bih_2D || p.bih_3D;
     std::vector<double> sarea(nmemb, mesh->scale_area), bh_ah, bh_ah_acc;
     std::vector<int4> el_bed;
-    if ((filt_bi || bih) && node_member) {  // scale_area of each member: sum(elem_area)/elem2D (fv_ini.F90:81)
-        std::vector<double> sum(nmemb, 0.0);
-        std::vector<long> cnt(nmemb, 0);
-        for (int e = 0; e < E; ++e) {
-            const int mb = node_member[mesh->elem2D_nodes[4 * e] - 1];
-            sum[mb] = sum[mb] + mesh->elem_area[e];
-            cnt[mb]++;
-        }
-        for (int b = 0; b < nmemb; ++b) sarea[b] = cnt[b] ? sum[b] / (double)cnt[b] : 0.0;
-    }
+    if (member_sa) sarea.assign(member_sa, member_sa + nmemb);
     if (filt_bi) kbi.resize(E);
     if (bih) {
         bh_ah.resize(E);
@@ -829,6 +821,9 @@ static int create_impl(const fcg_mesh *mesh, const fcg_params *par, const int *n
     }
     if (bih) {
         UP(m.bh_ah, bh_ah) UP(m.bh_ah_acc, bh_ah_acc) UP(m.el_bed, el_bed) UP(m.bhe, bhe)
+    }
+    if (member_sa) {
+        UP(m.memb_sa, sarea)
     }
     UP(m.ednodes, ednodes) UP(m.edtri, edtri) UP(m.ecross, ecross) UP(m.edxdy, edxdy)
     UP(m.depth, depth) UP(m.area, area) UP(m.ac, ac) UP(m.atp1, atp1) UP(m.atp2, atp2) UP(m.index, index)
@@ -1101,7 +1096,9 @@ extern "C" int fcg_create_ensemble(const fcg_mesh *members, int nmemb, const fcg
     u.ampt = ampt.data();
     u.fazt = fazt.data();
     u.coord_nod2D = m0.coord_nod2D ? coord.data() : nullptr;
-    return create_impl(&u, par, node_member.data(), nmemb, out);
+    std::vector<double> msa(nmemb);
+    for (int b = 0; b < nmemb; ++b) msa[b] = members[b].scale_area;
+    return create_impl(&u, par, node_member.data(), nmemb, out, msa.data());
 }
 
 extern "C" int fcg_ensemble_size(const fcg_ctx *ctx) { return ctx ? ctx->m.nmemb : 0; }

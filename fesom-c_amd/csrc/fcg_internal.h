@@ -138,6 +138,7 @@ struct DevMesh {
     int nmemb = 1;
     int *memb = nullptr;       // N: member of each node
     int *memb_noff = nullptr;  // nmemb+1: first node of each member
+    double *memb_sa = nullptr; // nmemb: scale_area of each member (null for a single model)
     // vertical grid
     double *sigma = nullptr;   // K
     double *dsig = nullptr;    // L: sigma(k)-sigma(k+1)

@@ -4394,7 +4394,7 @@ __global__ __launch_bounds__(256) void k_tr_miura(DevMesh m, DevStateT<R> s, dou
         const double crt1 = 0.5 * (mask_ad(m, s, nd.x, Dmin) + mask_ad(m, s, nd.y, Dmin));
         double Kh = m.elem_area[e1];
         if (e2 >= 0) Kh = 0.5 * (Kh + m.elem_area[e2]);
-        Kh = 0.010 * Kh / scale_area;
+        Kh = 0.010 * Kh / (m.memb_sa ? m.memb_sa[m.memb[n]] : scale_area);  // the node's member's scale_area
         const double T1 = PL(s.tcur, k, nd.x, KC), T2 = PL(s.tcur, k, nd.y, KC);
         const double tx1 = PL(s.gTx, k, nd.x, KC), tx2 = PL(s.gTx, k, nd.y, KC);
         const double ty1 = PL(s.gTy, k, nd.x, KC), ty2 = PL(s.gTy, k, nd.y, KC);

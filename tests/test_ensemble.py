@@ -68,13 +68,23 @@ def test_ensemble_mixed_precision_close_to_fp64():
         e.close()
 
 
-def test_ensemble_members_match_their_oracle_runs():
+@pytest.mark.parametrize("opts", [(), (("tracer_adv", "4"), ("filt_bi_2D", "T"), ("filt_bi_3D", "T"),
+                                         ("tau_c", "20.0"), ("bih_2D", "T"), ("bih_3D", "T"), ("Abh0", "1.0e5"))],
+                         ids=["muscl", "miura-filt_bi-bih"])
+def test_ensemble_members_match_their_oracle_runs(opts):
     """Every member of the joined ensemble equals the FP64 oracle run of that
-    member alone (its own mesh, tide amplitude, wind and T/S), bit for bit."""
+    member alone (its own mesh, tide amplitude, wind and T/S), bit for bit;
+    also with the options that read each member's own scale_area (the Miura
+    horizontal diffusion, the twice-applied filters, the biharmonic factors)."""
+    import re
     from oracle.pyoracle import Oracle
     from fesomc_amd import gpu, meshgen
     from fesomc_amd.runconfig import RunConfig
     txt, meshes, extras = meshgen.ensemble("c4_small")
+    for k, v in opts:
+        txt, n = re.subn(rf"^(\s*{k}\s*=\s*)\S+", rf"\g<1>{v}", txt, count=1, flags=re.M)
+        assert n == 1, k
+    print("ensemble vs oracle:", opts, flush=True)
     ens = gpu.Ensemble(RunConfig(txt), meshes)
     _apply_forcing(ens, [m.nod2D for m in meshes], [m.elem2D for m in meshes], extras)
     ens.step(6)
