@@ -134,14 +134,14 @@ inline int gather_slot(int T) { return T == 1024 ? 2 : (T == 512 ? 1 : 0); }
 // blocks (wider blocks hold more columns, so more neighbours come from LDS)
 inline size_t velrhs_bytes(const DevMesh &m, size_t esz, int T = kFlatT) {
     return (size_t)cpb_of(m.L, T) * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) +
-           (size_t)cpb_of(m.L, T) * m.L * (3 * sizeof(double) + 6 * esz);
+           (size_t)cpb_of(m.L, T) * m.L * 9 * esz;
 }
 // the level-vector k_velrhs: W = 16/esz layers per lane, K/W lanes per
 // column; column arrays with a stride of K slots
 inline int velrhs_v_cpb(const DevMesh &m, size_t esz, int T) { return T / (m.K / (int)(16 / esz)); }
 inline size_t velrhs_v_bytes(const DevMesh &m, size_t esz, int T) {
     const size_t cpb = (size_t)velrhs_v_cpb(m, esz, T);
-    return cpb * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) + cpb * m.K * (3 * sizeof(double) + 2 * esz);
+    return cpb * (kVelrhsElemBytes + 4 * velrhs_edge_bytes(esz)) + cpb * m.K * 5 * esz;
 }
 // its block: 256 threads, or 128 where three 256-thread blocks' LDS would not
 // share one compute unit (160 KB; the kernel's registers allow three)

@@ -807,8 +807,8 @@ __device__ __forceinline__ void uvert(const DevMesh &m, const DevStateT<R> &s, c
 // uvert on the column arrays a block of compute_vel_rhs shares in LDS: ws =
 // sum w_cv*W at each level, u/v the element's velocity, Zw/zbw the
 // element-averaged layer and level depths (the same operands uvert forms).
-template <typename R>
-__device__ __forceinline__ void uvert_lds(const double *ws, const R *u, const R *v, const double *Zw, const double *zbw,
+template <typename R, typename C>
+__device__ __forceinline__ void uvert_lds(const C *ws, const R *u, const R *v, const C *Zw, const C *zbw,
                                           int K, double area, int nz, double &fu, double &fv) {
     const double wv = ws[nz - 1] * area;
     double um = 0.0, vm = 0.0;
@@ -854,8 +854,8 @@ __device__ __forceinline__ void uvert_lds(const double *ws, const R *u, const R 
 // branch is the upward one on the mirrored operands (layers nz-1, nz-2,
 // nz-3 instead of nz-2, nz-1, nz) with the depth differences negated, and
 // x - y = -(y - x) exactly, so every value equals uvert_lds's bit for bit.
-template <typename R>
-__device__ __forceinline__ void uvert_uni(const double *ws, const R *u, const R *v, const double *Zw, const double *zbw,
+template <typename R, typename C>
+__device__ __forceinline__ void uvert_uni(const C *ws, const R *u, const R *v, const C *Zw, const C *zbw,
                                           int K, double area, int nz, double &fu, double &fv) {
     const double wv = ws[nz - 1] * area;
     const bool up = wv >= 0.0;
@@ -991,8 +991,10 @@ __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt
     // per column and layer: sum w_cv*W, Zw, zbw, U_n, V_n, and the velocity
     // gradients (read by the block's neighbours across internal edges)
     const int T = f.cpb * L;
-    double *cws = (double *)(edr + 4 * f.cpb), *cZw = cws + T, *czb = cZw + T;
-    R *cu = (R *)(czb + T), *cv = cu + T, *cgux = cv + T, *cguy = cgux + T, *cgvx = cguy + T, *cgvy = cgvx + T;
+    // the vertical advection's column arrays in the storage type (as the
+    // level-vector kernel keeps them, so both variants round alike)
+    R *cws = (R *)(edr + 4 * f.cpb), *cZw = cws + T, *czb = cZw + T;
+    R *cu = czb + T, *cv = cu + T, *cgux = cv + T, *cguy = cgux + T, *cgvx = cguy + T, *cgvy = cgvx + T;
     if (f.nc <= 0) return;
     if ((int)threadIdx.x < f.nc) {
         const int c = threadIdx.x, e = f.c0 + c;
@@ -1054,9 +1056,9 @@ __device__ __forceinline__ void velrhs_body(DevMesh m, DevStateT<R> s, double dt
     if (f.on()) {  // the column arrays of the vertical advection
         const ElemCol &col = er[c].col;
         const int4 q = col.q;
-        cws[c * L + k] = sum4(col.w, PL(s.W, k, q.x, KC), PL(s.W, k, q.y, KC), PL(s.W, k, q.z, KC), PL(s.W, k, q.w, KC));
-        cZw[c * L + k] = col.Zw(m, k);
-        czb[c * L + k] = col.zbw(m, k);
+        cws[c * L + k] = (R)sum4(col.w, PL(s.W, k, q.x, KC), PL(s.W, k, q.y, KC), PL(s.W, k, q.z, KC), PL(s.W, k, q.w, KC));
+        cZw[c * L + k] = (R)col.Zw(m, k);
+        czb[c * L + k] = (R)col.zbw(m, k);
         cu[c * L + k] = PL(s.U_n, k, e, KC);
         cv[c * L + k] = PL(s.V_n, k, e, KC);
         if (hadv) {
@@ -1217,8 +1219,11 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
     ElemRec *er = (ElemRec *)lds_;                                        // [cpb]
     EdgeRec<R> *edr = (EdgeRec<R> *)(lds_ + f.cpb * kVelrhsElemBytes);   // [cpb][4]
     const int T = f.cpb * K;
-    double *cws = (double *)(edr + 4 * f.cpb), *cZw = cws + T, *czb = cZw + T;
-    R *cu = (R *)(czb + T), *cv = cu + T;
+    // the vertical advection's column arrays in the storage type: FP64 in the
+    // drop-in; FP32 in the mixed path, which keeps the block's LDS within a
+    // quarter of the compute unit's
+    R *cws = (R *)(edr + 4 * f.cpb), *cZw = cws + T, *czb = cZw + T;
+    R *cu = czb + T, *cv = cu + T;
     if (f.nc <= 0) return;
     // element and edge records, one thread per record (the element's four
     // edge slots, then the element), while every lane stages its run of the
@@ -1295,9 +1300,9 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
 #pragma unroll
         for (int i = 0; i < W; ++i) {
             const int k = k0 + i;
-            cws[ck0 + i] = sum4(col.w, w0.x[i], w1.x[i], w2.x[i], w3.x[i]);
-            cZw[ck0 + i] = k < L ? col.Zw(m, k) : 0.0;
-            czb[ck0 + i] = col.zbw(m, k);
+            cws[ck0 + i] = (R)sum4(col.w, w0.x[i], w1.x[i], w2.x[i], w3.x[i]);
+            cZw[ck0 + i] = (R)(k < L ? col.Zw(m, k) : 0.0);
+            czb[ck0 + i] = (R)col.zbw(m, k);
         }
         stv(cu + ck0, ldv_glb(s.U_n + ek0));
         stv(cv + ck0, ldv_glb(s.V_n + ek0));
@@ -1398,7 +1403,7 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
         if (r.mask != 0.0) {
             // flux through 1-based interface nz (0 at the bottom, nz = K);
             // the bottom flux of one layer is the top flux of the next
-            const double *ws = cws + c * K, *zw = cZw + c * K, *zb = czb + c * K;
+            const R *ws = cws + c * K, *zw = cZw + c * K, *zb = czb + c * K;
             const R *uc = cu + c * K, *vc = cv + c * K;
             auto flux = [&](int nz, double &fu, double &fv) {
                 fu = 0.0;
@@ -1447,12 +1452,12 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_velrhs_v(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_velrhs_v(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv, int vadv,
                                                    double d0i) {
     velrhs_vec_body<R, false, false>(m, s, dt, ff, hadv, vadv, d0i, 0.0);
 }
 template <typename R>
-__global__ __launch_bounds__(256) void k_velrhs_v_atm(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_velrhs_v_atm(DevMesh m, DevStateT<R> s, double dt, double ff, int hadv,
                                                        int vadv, double d0i) {
     velrhs_vec_body<R, true, false>(m, s, dt, ff, hadv, vadv, d0i, 0.0);
 }
