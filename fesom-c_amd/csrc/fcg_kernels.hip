@@ -1434,16 +1434,20 @@ __device__ __forceinline__ void velrhs_vec_body(DevMesh m, DevStateT<R> s, doubl
     }
     const double mask = r.mask;
     const LV<R> bu = ldv_glb(s.Bpu + ek0), bv = ldv_glb(s.Bpv + ek0);
+    // the layer thicknesses and the own velocity again (from the LDS record and
+    // columns), rather than held in registers through the flux loops
+    const LV<R> ue = ldv_lds(cu + ck0), ve = ldv_lds(cv + ck0);
     LV<R> oa, ob, oc, od;
 #pragma unroll
     for (int i = 0; i < W; ++i) {
         oa.x[i] = abu[i];
         ob.x[i] = abv[i];
+        const double jei = r.col.Je(m, k0 + i < L ? k0 + i : L - 1);
         // UV_rhs: the filt_bi_3D tendency (zero when that filter is off)
         const double uvx = (s.uvu && i < nl) ? (double)s.uvu[ek0 + i] : 0.0;
         const double uvy = (s.uvu && i < nl) ? (double)s.uvv[ek0 + i] : 0.0;
-        oc.x[i] = mask * (uo.x[i] + bu.x[i] * dt + (dt * (ur[i] + abu[i] * ff) * area_inv + uvx) / je[i]);
-        od.x[i] = mask * (vo.x[i] + bv.x[i] * dt + (dt * (vr[i] + abv[i] * ff) * area_inv + uvy) / je[i]);
+        oc.x[i] = mask * (ue.x[i] + bu.x[i] * dt + (dt * (ur[i] + abu[i] * ff) * area_inv + uvx) / jei);
+        od.x[i] = mask * (ve.x[i] + bv.x[i] * dt + (dt * (vr[i] + abv[i] * ff) * area_inv + uvy) / jei);
     }
     stv_n(s.U_rhsAB + ek0, oa, nl);
     stv_n(s.V_rhsAB + ek0, ob, nl);
