@@ -3499,8 +3499,14 @@ __global__ __launch_bounds__(1024) void k_tr_muscl(DevMesh m, DevStateT<R> s, do
     const double wdiff = PL(s.W, k, n, KC) - PL(s.W, k + 1, n, KC);
     const double jc = a * m.dsig[k];
     const double aj = m.area[n] * jc;
-    rt = rt / aj - tab * rc / aj + tab * wdiff / jc + (tvt - tvb) / jc;
-    rs = rs / aj - sab * rc / aj + sab * wdiff / jc + (svt - svb) / jc;
+    if constexpr (sizeof(R) == 4) {  // mixed path: two reciprocals instead of eight FP64 divisions
+        const double iaj = 1.0 / aj, ijc = 1.0 / jc;
+        rt = rt * iaj - tab * rc * iaj + tab * wdiff * ijc + (tvt - tvb) * ijc;
+        rs = rs * iaj - sab * rc * iaj + sab * wdiff * ijc + (svt - svb) * ijc;
+    } else {
+        rt = rt / aj - tab * rc / aj + tab * wdiff / jc + (tvt - tvb) / jc;
+        rs = rs / aj - sab * rc / aj + sab * wdiff / jc + (svt - svb) / jc;
+    }
     const double tdm = PL(s.TF, k, n, KC) - Ta, sdm = PL(s.SF, k, n, KC) - Sa;
     if (cl_relax && m.index[n] == 2) {  // climatology relaxation on the current de-meaned tracer
         if (ob_inflow(m, s, p0, p1, Dmin) > 0.0) {
@@ -3750,8 +3756,14 @@ __global__ __launch_bounds__(256) void k_tr_muscl_v(DevMesh m, DevStateT<R> s, d
         const double jc = a * m.dsig[k < L ? k : L - 1];
         const double aj = arn * jc;
         R rti = rt[i], rsi = rs[i];
-        rti = rti / aj - tab * rc[i] / aj + tab * wdiff / jc + (tvt - tvb) / jc;
-        rsi = rsi / aj - sab * rc[i] / aj + sab * wdiff / jc + (svt - svb) / jc;
+        if constexpr (sizeof(R) == 4) {  // as the one-level kernel: two reciprocals in the mixed path
+            const double iaj = 1.0 / aj, ijc = 1.0 / jc;
+            rti = rti * iaj - tab * rc[i] * iaj + tab * wdiff * ijc + (tvt - tvb) * ijc;
+            rsi = rsi * iaj - sab * rc[i] * iaj + sab * wdiff * ijc + (svt - svb) * ijc;
+        } else {
+            rti = rti / aj - tab * rc[i] / aj + tab * wdiff / jc + (tvt - tvb) / jc;
+            rsi = rsi / aj - sab * rc[i] / aj + sab * wdiff / jc + (svt - svb) / jc;
+        }
         const double tdm = TF0.x[i] - Ta, sdm = SF0.x[i] - Sa;
         if (relax) {  // climatology relaxation on the current de-meaned tracer
             if (inflow) {
